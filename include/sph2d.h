@@ -1,0 +1,186 @@
+/*
+ * sph2d.h -- C-ABI of the MI355X WCSPH time step (drop-in for the reference's
+ * hot-path subroutines of mini-walker/SPH-2D, Taylor_Green_Vortex/).
+ *
+ * Conventions
+ *  - Every entry point returns int: SPH2D_OK (0) or an SPH2D_ERR_* code;
+ *    sph2d_last_error() gives the message of the calling thread's last failure.
+ *  - Host arrays use the reference's Fortran layout: 1-D arrays of length n,
+ *    2-D arrays (ld,2) column-major (x block, then y block), REAL(8) = double,
+ *    INTEGER = int32. The library never keeps host pointers after a call.
+ *  - A context owns device (HBM) copies of the reference's global particle_*
+ *    arrays and of the subdomain_* arrays; state stays on the GPU between
+ *    calls and the host synchronises only in sph2d_download_* and sph2d_sample.
+ *  - One host thread per context; all work is queued on the context's stream.
+ *  - The scalar entry points keep the reference's by-reference argument lists,
+ *    so a Fortran bind(C) interface without VALUE can call them unchanged.
+ */
+#ifndef SPH2D_H
+#define SPH2D_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SPH2D_OK = 0,
+  SPH2D_ERR_ARGUMENT = 1,        /* bad parameter or size */
+  SPH2D_ERR_CAPACITY = 2,        /* particle, ghost or cell capacity exceeded */
+  SPH2D_ERR_HIP = 3,             /* HIP runtime failure */
+  SPH2D_ERR_STATE = 4,           /* call out of order (e.g. no state uploaded) */
+  SPH2D_ERR_UNSUPPORTED = 5,     /* option not built */
+  SPH2D_ERR_NONFINITE = 6        /* non-finite value in the state */
+};
+
+/* Solver variants (Solver_Normalized_*_Riemann_Delta_SPS of the reference). */
+enum {
+  SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN = 0, /* Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90 (default, single_step_compute.f90:137) */
+  SPH2D_SOLVER_LINEAR_RIEMANN = 1           /* Solver_Normalized_Linear_Riemann_Delta_SPS.f90 */
+};
+
+/* Integrators of time_iteration.f90 (leapforg_or_eular, information_module.f90:303). */
+enum { SPH2D_INTEGRATOR_PC = 2, SPH2D_INTEGRATOR_RK = 3 };
+
+/* Arithmetic forms. */
+enum { SPH2D_FP64 = 0, SPH2D_FP32_MIXED = 1 };
+
+/* Re-use of the previous step's rates (time_iteration.f90:542-555, 942-957):
+ * REFERENCE reads them by the new step's local index, as the reference does;
+ * BY_PARTICLE carries them with the particle (gid). */
+enum { SPH2D_RATE_REUSE_REFERENCE = 0, SPH2D_RATE_REUSE_BY_PARTICLE = 1 };
+
+/* Runtime form of information_module.f90's compile-time parameters. */
+typedef struct {
+  int32_t struct_size;       /* sizeof(sph2d_params), checked */
+  int32_t nx, ny;            /* interior_particle_number_x/_y (:75-76): lattice for sph2d_init_tgv */
+  double box_x, box_y;       /* periodic box [0,box_x]x[0,box_y] (:54-57, :94-103) */
+  double rho0;               /* water_rho_0 (:31) */
+  double c0sq;               /* square_c_0 (:145) */
+  double nu;                 /* water_kinematic_viscosity (:37); REAL(4) literal in the reference */
+  double U;                  /* U_Module (:52) */
+  double dx;                 /* interior_dx (:79) */
+  double h;                  /* smooth_length = smooth_length_factor*dx (:135) */
+  double dt;                 /* dt (:309) */
+  int32_t integrator;        /* SPH2D_INTEGRATOR_* */
+  int32_t solver;            /* SPH2D_SOLVER_* */
+  int32_t precision;         /* SPH2D_FP64 or SPH2D_FP32_MIXED */
+  int32_t rate_reuse;        /* SPH2D_RATE_REUSE_* */
+  int32_t shift_on;          /* ShiftParticle_Switch (:349) */
+  int32_t mls_on;            /* RefreshDensityByMLS_Switch (:324) */
+  int32_t mls_every;         /* MLS_time_step (:332) */
+  int32_t mls_order;         /* RefreshDensity_MLS_Order (:330): 1 or 6 */
+  int32_t sample_every;      /* sampling_time_step (:336) */
+  int32_t periodic;          /* DistributePeriodBoundaryParticleOrNot (:92) */
+  double alpha, beta, fai;   /* Monaghan artificial viscosity (:210-212) */
+  double delta, delta_alpha; /* delta-SPH (:217-218) */
+  double epsilon_xsph;       /* XSPH epsilon (:207) */
+  double shift_beta, shift_R;/* shift_belta, shift_R (:350-351) */
+  double max_shift;          /* max_shift = 0.25*dx (:353) */
+  double mls_alpha;          /* slighted_modified_alpha (:359) */
+} sph2d_params;
+
+typedef struct sph2d_ctx sph2d_ctx;
+
+/* Counters of the last step (sph2d_get_diagnostics). */
+typedef struct {
+  int32_t n_fluid;           /* particle_ture_number */
+  int32_t n_ghost;           /* Period_Boundary_Particle_Number */
+  int32_t n_local;           /* total_particle_number_in_subdomain */
+  int32_t n_ill;             /* particles typed 120 */
+  int32_t n_surface;         /* subdomain_surface_particle_number after the geometric test */
+  int32_t n_singular;        /* singular dyadic tensors (matrix_inversion_less_three check_error=0) */
+  int32_t n_mls_fallback;    /* MLS points that fell back to Shepard */
+  int32_t max_cell_count;    /* largest background-cell population */
+  int64_t n_pairs;           /* pair_number of the last force evaluation (counted on request) */
+  int32_t rebin_needed;      /* a particle moved > h/2 since binning (stencil no longer covers 3h) */
+  int32_t error_flags;       /* device-side capacity flags */
+} sph2d_diag;
+
+/* --- library --------------------------------------------------------------- */
+const char* sph2d_last_error(void);
+/* Reference defaults of Taylor_Green_Vortex/information_module.f90 (100x100 TGV). */
+int sph2d_default_params(sph2d_params* p);
+
+/* --- context (no reference counterpart: owns the HBM-resident arrays) ------ */
+int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx** out);
+int sph2d_ctx_destroy(sph2d_ctx* ctx);
+
+/* Distribute_Initial_Particle.f90:204-249: TGV lattice, rho from inverse Tait,
+ * mass = dx*dy*rho, c = c0, e = e_0, type 2. Computed on the GPU. */
+int sph2d_init_tgv(sph2d_ctx* ctx);
+
+/* Upload the fluid particles in global (reference) order: pos/vel are (ld,2). */
+int sph2d_upload_state(sph2d_ctx* ctx, int32_t n, const double* pos, const double* vel,
+                       const double* rho, const double* mass, const double* energy,
+                       const double* h, const int32_t* type, int32_t ld);
+/* Download global arrays (particle_*; ghosts excluded): pos/vel (ld,2). Any may be NULL. */
+int sph2d_download_state(sph2d_ctx* ctx, int32_t n, double* pos, double* vel, double* rho,
+                         double* press, double* c, double* mass, int32_t* type, int32_t ld);
+
+/* --- scalar and array entry points with the reference's argument lists ----- */
+/* compute_kernel(dim,distance,position_difference,average_smooth_length,w,temp_dwdx)
+ * -- Taylor_Green_Vortex/compute_kernel.f90:21-27. dim must be 2. */
+int sph2d_compute_kernel(const int32_t* dim, const double* distance, const double* position_difference,
+                         const double* average_smooth_length, double* w, double* dwdx);
+/* Array form over n pairs: dx is (n,2) column-major, dwdx (n,2). */
+int sph2d_compute_kernel_n(int32_t n, const double* distance, const double* dx, const double* h,
+                           double* w, double* dwdx);
+/* status_equation(rho,energy,type,press,c) -- Taylor_Green_Vortex/status_equation.f90:19-24. */
+int sph2d_status_equation(const double* rho, const double* energy, const int32_t* type,
+                          double* press, double* c);
+/* Array form over n particles (EOS per single_step_compute.f90:62-73 skips type 120). */
+int sph2d_status_equation_n(const sph2d_params* p, int32_t n, const double* rho, const double* energy,
+                            const int32_t* type, double* press, double* c);
+
+/* --- one entry per replaced routine (context form) ------------------------- */
+/* Distribute_Period_Boundary_Particle.f90:56-206: wrap into the box and append ghosts. */
+int sph2d_distribute_period_boundary(sph2d_ctx* ctx);
+/* Devide_block_and_send_tasks.f90 (one rank): binning into 4h cells, local order
+ * (cell column, cell row, gid) and the global -> subdomain copy; the analytic TGV
+ * fields at step start (:88-116) when i_time_step is a sampling step. */
+int sph2d_devide_block(sph2d_ctx* ctx, int32_t i_time_step);
+/* single_step_compute.f90:62-171: EOS, pair search, free surface, solver, rate assembly. */
+int sph2d_single_step_compute(sph2d_ctx* ctx, int32_t i_time_step);
+/* Reinitialize_particle_density.f90:66-182 (MLS order 6 or 1). */
+int sph2d_reinitialize_particle_density(sph2d_ctx* ctx, int32_t i_time_step);
+/* Shift_particle_position.f90:56-136 with the last evaluation's pair data. */
+int sph2d_shift_particle_position(sph2d_ctx* ctx, int32_t i_time_step);
+/* Refresh_domain.f90:136-300 (one rank): subdomain -> global arrays. */
+int sph2d_refresh_domain(sph2d_ctx* ctx, int32_t i_time_step);
+/* One full step of time_iteration.f90 (PC :509-657 or RK-variant :779-1049),
+ * including ghosts, ordering, reinitialisation, shift and refresh. */
+int sph2d_time_step(sph2d_ctx* ctx, int32_t i_time_step);
+/* Steps first..first+nsteps-1 with no host synchronisation. */
+int sph2d_run(sph2d_ctx* ctx, int32_t first_step, int32_t nsteps);
+/* Sampling_subroutine.f90:340-461: out[0]=KE/KE0, [1]=max|v|/U, then (Linf,L1,L2)
+ * for u, v, p/(rho0 U^2), |v| at [2..4], [5..7], [8..10], [11..13]. */
+int sph2d_sample(sph2d_ctx* ctx, double* out14);
+
+/* --- parity / inspection -------------------------------------------------- */
+/* Download a named subdomain (local-order) array of the last evaluation:
+ * f64: x y u v rho mass p c volume fra_v sum_dwdx_x sum_dwdx_y tau_xx tau_xy tau_yy
+ *      grad_rho_x grad_rho_y drhodt ax ay shift_x shift_y normal_x normal_y
+ *      internal_x internal_y visc_x visc_y art_internal_x art_internal_y sps_x sps_y
+ *      lap_x lap_y (the last 10 only after sph2d_set_debug(ctx,1))
+ * i32: type name(gid, 0-based) surface near_surface */
+int sph2d_download_local(sph2d_ctx* ctx, const char* name, void* out, int32_t n);
+/* Pair set of the current state (find_particle_pair.f90:64-123): unordered pairs
+ * as 0-based gids (gid_lo < gid_hi), sorted ascending by (lo,hi). */
+int sph2d_download_pairs(sph2d_ctx* ctx, int64_t capacity, int32_t* gid_lo, int32_t* gid_hi,
+                         int64_t* n_pairs);
+/* 1: also write the acceleration components of the solver (for parity tests). */
+int sph2d_set_debug(sph2d_ctx* ctx, int32_t on);
+int sph2d_get_diagnostics(sph2d_ctx* ctx, sph2d_diag* out);
+/* Average duration (ms, HIP events on the context's stream) of `reps` back-to-back
+ * launches of one part of the last evaluation: pass 0 = force pass (Riemann,
+ * delta-SPH, SPS, shift sums), 1 = neighbour-sum pass A, 2 = the whole evaluation.
+ * No reference counterpart: measurement hook for the roofline report. */
+int sph2d_time_pass(sph2d_ctx* ctx, int32_t pass, int32_t reps, double* avg_ms);
+/* Device stream of the context (hipStream_t) for callers that time or order work. */
+int sph2d_get_stream(sph2d_ctx* ctx, void** stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1255 @@
+/*
+ * sph2d_oracle.c -- CPU restatement of the reference WCSPH step (TEST INFRASTRUCTURE).
+ *
+ * The checker for the HIP path, never the product. It follows the Fortran
+ * reference (Taylor_Green_Vortex/ unless stated) line by line: the same
+ * global/subdomain arrays, the same local ordering, the same half pair list
+ * and scatter order, the same REAL(4) literals and integer-power chains
+ * (flang 22 expands x**n with a constant n into x*x*...*x left to right;
+ * that chain is checked against the compiled reference in tests/).
+ * Build: gcc -O2 -ffp-contract=off (no FMA contraction, like the reference build).
+ */
+#include "sph2d_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+/* information_module.f90:29  PI (REAL(8) parameter) */
+#define PI_MODULE 3.14159265358979324
+/* compute_kernel.f90:32  PI=3.14159265358 is a REAL(4) literal stored in REAL(8) */
+#define PI_KERNEL ((double)3.14159265358f)
+
+typedef struct { double x, y; } v2;
+
+struct orc_ctx {
+  orc_params P;
+  /* derived constants (information_module.f90) */
+  double dx, dy, h, chain_d, chain_ox, chain_oy, e0, max_shift;
+  int chain_nx, chain_ny;
+  int n_fluid;                 /* particle_ture_number */
+  int n_ghost;                 /* Period_Boundary_Particle_Number */
+  int cap_global, cap_local;
+  /* global arrays: particle_* (Public_variable_module.f90:33-49) */
+  double *gx, *gy, *gu, *gv, *gmass, *grho, *gh, *gc, *gp, *ge;
+  int *gtype;
+  double *ana_u, *ana_v, *ana_p, *ana_vm;        /* Analytical_particle_* */
+  double ke0;
+  /* subdomain arrays (local order) */
+  int n_local;
+  int *name;                    /* Full_subdomain_particle_name (gid) */
+  double *x, *y, *u, *v, *mass, *rho, *hh, *c, *p, *e, *vol;
+  int *type;
+  /* background grid of the subdomain */
+  int sub_nx, sub_ny, sub_ncell, sub_start;
+  int *cell_count, *cell_list;  /* subdomain_number, subdomain_particle_in_chain (cap 300) */
+  int *near_n, *near_name;      /* subdomain_chain_near_mesh_number/_name (9 per cell) */
+  int *chain_no;                /* subdomain_particle_chain_number */
+  int *cand_n, *cand;           /* subdomain_effect_particle_number / _particle (cap 300) */
+  /* pairs */
+  int64_t npair, cap_pair;
+  int *pi, *pj;
+  double *pw, *pdwx, *pdwy, *pdx, *pdy, *pvx, *pvy, *prd, *pc_, *prho_, *ph_;
+  double *ndix, *ndiy, *ndjx, *ndjy;            /* Normalized_dwdx_i/_j */
+  /* free surface */
+  double *fra_v, *sum_w, *sdwx, *sdwy, *T, *Tinv, *nrmx, *nrmy;
+  int *surf, *near;
+  int n_singular;
+  /* solver */
+  double *Sxx, *Sxy, *Syy, *txx, *txy, *tyy;
+  double *drhodt, *dedt, *ax, *ay;
+  double *ia_x, *ia_y, *av_x, *av_y, *ai_x, *ai_y, *sps_x, *sps_y, *lap_x, *lap_y, *ext_x, *ext_y;
+  double *avg_x, *avg_y, *corr, *grx, *gry;
+  /* shift */
+  double *shx, *shy;
+  /* time integration (local index arrays, persist across steps) */
+  double *b_rho, *b_e, *b_x, *b_y, *b_u, *b_v;
+  double *t_u, *t_v, *t_drho, *t_de, *t_ax, *t_ay;              /* PC temp_* */
+  double *k_u[3], *k_v[3], *k_drho[3], *k_de[3], *k_ax[3], *k_ay[3]; /* RK K_1..K_3 */
+  double *mls_rho;
+};
+
+/* ------------------------------------------------------------------ */
+/* small helpers                                                       */
+/* ------------------------------------------------------------------ */
+static void* xcalloc(size_t n, size_t s) {
+  void* p = calloc(n ? n : 1, s);
+  if (!p) { fprintf(stderr, "orc: out of memory\n"); abort(); }
+  return p;
+}
+static double ipow7(double x) { return x * x * x * x * x * x * x; }
+static double ipow3(double x) { return x * x * x; }
+static double ipow4(double x) { return x * x * x * x; }
+
+void orc_default_params(orc_params* p) {
+  memset(p, 0, sizeof(*p));
+  p->nx = 100; p->ny = 100;
+  p->box_x = 1.0; p->box_y = 1.0;
+  p->rho0 = 1000.0;
+  p->c0sq = 100.0;
+  p->nu = (double)1.0E-2f;
+  p->U = 1.0;
+  p->h_factor = 1.20;
+  p->dt = 0.0001;
+  p->integrator = 3;
+  p->shift_on = 1;
+  p->mls_on = 1;
+  p->mls_every = 20;
+  p->mls_order = 6;
+  p->solver = ORC_SOLVER_MODIFIED_LINEAR_RIEMANN;
+  p->alpha = 0.0; p->beta = 2.0; p->fai = 0.1;
+  p->delta = 0.1; p->delta_alpha = 0.02;
+  p->epsilon_xsph = 0.0;
+  p->shift_beta = 0.5; p->shift_R = 0.2;
+  p->mls_alpha = (double)0.0001f;
+}
+
+/* ------------------------------------------------------------------ */
+/* compute_kernel.f90:21-133 (model: improved Gaussian, support 3h)   */
+/* ------------------------------------------------------------------ */
+void orc_compute_kernel(int32_t dim, double distance, const double* pd, double hbar,
+                        double* w, double* dwdx) {
+  double q = distance / hbar;
+  /* factor=1.0/(PI**(dim/2.0)*(h**dim)*(1-10*exp(-9.0)))  -- last factor is REAL(4) */
+  float c4 = 1.0f - 10.0f * expf(-9.0f);
+  double hd = (dim == 2) ? hbar * hbar : (dim == 1 ? hbar : hbar * hbar * hbar);
+  double pid = (dim == 2) ? PI_KERNEL : pow(PI_KERNEL, (double)((float)dim / 2.0f));
+  double factor = 1.0 / (pid * hd * (double)c4);
+  for (int i = 0; i < dim; ++i) dwdx[i] = 0.0;
+  *w = 0.0;
+  if (distance <= 3.0 * hbar) {
+    double eq = exp(-(q * q));
+    double e9 = (double)expf(-9.0f);
+    *w = (eq - e9) * factor;
+    for (int i = 0; i < dim; ++i)
+      dwdx[i] = (eq * (-2.0 * q)) * pd[i] / (distance * hbar) * factor;
+  }
+}
+
+/* status_equation.f90:19-82 (water branch, Tait gamma=7) */
+void orc_status_equation(const orc_ctx* c, double rho, double e, int32_t type,
+                         double* p, double* cs) {
+  (void)e;
+  if (abs(type) == 1) {            /* air branch: c only, p untouched (:46-50) */
+    float gamma = 1.4f;
+    *cs = sqrt(((double)gamma - 1.0) * rho);
+    return;
+  }
+  double b = c->P.c0sq * c->P.rho0 / 7.0;
+  double r = rho / c->P.rho0;
+  *p = b * (ipow7(r) - 1.0);
+  *cs = sqrt(c->P.c0sq) * ipow3(r);
+}
+
+/* function_module.f90:157-249  matrix_inversion_less_three (column-major n x n).
+ * Returns check_error (1 = ok). On det==0 only the diagonal of res is set to 1. */
+int32_t orc_matrix_inversion_less_three(int32_t n, const double* A, double* res) {
+#define M(a, i, j) a[(i) + (j) * n]
+  double inv[9] = {0};
+  double det = 0.0;
+  for (int k = 0; k < n; ++k) M(res, k, k) = 1.0;
+  if (n == 2) {
+    double a1 = M(A, 0, 0), b1 = M(A, 0, 1), a2 = M(A, 1, 0), b2 = M(A, 1, 1);
+    det = (a1 * b2 - b1 * a2);
+    if (det != 0) {
+      inv[0] = b2 / det; inv[2] = -b1 / det; inv[1] = -a2 / det; inv[3] = a1 / det;
+    }
+  } else if (n == 3) {
+    double a1 = M(A,0,0), b1 = M(A,0,1), c1 = M(A,0,2);
+    double a2 = M(A,1,0), b2 = M(A,1,1), c2 = M(A,1,2);
+    double a3 = M(A,2,0), b3 = M(A,2,1), c3 = M(A,2,2);
+    det = (a1 * (b2 * c3 - c2 * b3) - a2 * (b1 * c3 - c1 * b3) + a3 * (b1 * c2 - c1 * b2));
+    if (det != 0) {
+      inv[0 + 0*3] = (b2*c3 - c2*b3) / det; inv[0 + 1*3] = (b3*c1 - c3*b1) / det;
+      inv[0 + 2*3] = (b1*c2 - c1*b2) / det; inv[1 + 0*3] = (c2*a3 - a2*c3) / det;
+      inv[1 + 1*3] = (c3*a1 - a3*c1) / det; inv[1 + 2*3] = (c1*a2 - a1*c2) / det;
+      inv[2 + 0*3] = (a2*b3 - b2*a3) / det; inv[2 + 1*3] = (a3*b1 - b3*a1) / det;
+      inv[2 + 2*3] = (a1*b2 - b1*a2) / det;
+    }
+  } else {
+    return 0;
+  }
+  if (det != 0) { memcpy(res, inv, sizeof(double) * n * n); return 1; }
+  return 0;
+#undef M
+}
+
+/* function_module.f90:568-639  BRINV: in-place Gauss-Jordan inverse with full
+ * pivoting (column-major). Returns L (1 = ok, 0 = singular: D+1.0==1.0). */
+int32_t orc_brinv(double* A, int32_t N) {
+#define M(i, j) A[(i) + (j) * N]
+  int IS[16], JS[16];
+  double T, D;
+  for (int K = 0; K < N; ++K) {
+    D = 0.0;
+    for (int I = K; I < N; ++I)
+      for (int J = K; J < N; ++J)
+        if (fabs(M(I, J)) > D) { D = fabs(M(I, J)); IS[K] = I; JS[K] = J; }
+    if (D + 1.0 == 1.0) return 0;
+    for (int J = 0; J < N; ++J) { T = M(K, J); M(K, J) = M(IS[K], J); M(IS[K], J) = T; }
+    for (int I = 0; I < N; ++I) { T = M(I, K); M(I, K) = M(I, JS[K]); M(I, JS[K]) = T; }
+    M(K, K) = 1.0 / M(K, K);
+    for (int J = 0; J < N; ++J) if (J != K) M(K, J) = M(K, J) * M(K, K);
+    for (int I = 0; I < N; ++I)
+      if (I != K)
+        for (int J = 0; J < N; ++J)
+          if (J != K) M(I, J) = M(I, J) - M(I, K) * M(K, J);
+    for (int I = 0; I < N; ++I) if (I != K) M(I, K) = -M(I, K) * M(K, K);
+  }
+  for (int K = N - 1; K >= 0; --K) {
+    for (int J = 0; J < N; ++J) { T = M(K, J); M(K, J) = M(JS[K], J); M(JS[K], J) = T; }
+    for (int I = 0; I < N; ++I) { T = M(I, K); M(I, K) = M(I, IS[K]); M(I, IS[K]) = T; }
+  }
+  return 1;
+#undef M
+}
+
+/* ------------------------------------------------------------------ */
+/* context                                                             */
+/* ------------------------------------------------------------------ */
+static void alloc_local(orc_ctx* c, int cap) {
+  c->cap_local = cap;
+#define A(f) c->f = xcalloc(cap, sizeof(double))
+#define I(f) c->f = xcalloc(cap, sizeof(int))
+  I(name); A(x); A(y); A(u); A(v); A(mass); A(rho); A(hh); A(c); A(p); A(e); A(vol); I(type);
+  I(chain_no); I(cand_n); c->cand = xcalloc((size_t)cap * 300, sizeof(int));
+  A(fra_v); A(sum_w); A(sdwx); A(sdwy); c->T = xcalloc((size_t)cap * 4, sizeof(double));
+  c->Tinv = xcalloc((size_t)cap * 4, sizeof(double)); A(nrmx); A(nrmy); I(surf); I(near);
+  A(Sxx); A(Sxy); A(Syy); A(txx); A(txy); A(tyy);
+  A(drhodt); A(dedt); A(ax); A(ay);
+  A(ia_x); A(ia_y); A(av_x); A(av_y); A(ai_x); A(ai_y); A(sps_x); A(sps_y);
+  A(lap_x); A(lap_y); A(ext_x); A(ext_y); A(avg_x); A(avg_y); A(corr); A(grx); A(gry);
+  A(shx); A(shy);
+  A(b_rho); A(b_e); A(b_x); A(b_y); A(b_u); A(b_v);
+  A(t_u); A(t_v); A(t_drho); A(t_de); A(t_ax); A(t_ay);
+  for (int k = 0; k < 3; ++k) { A(k_u[k]); A(k_v[k]); A(k_drho[k]); A(k_de[k]); A(k_ax[k]); A(k_ay[k]); }
+  A(mls_rho);
+#undef A
+#undef I
+}
+
+orc_ctx* orc_create(const orc_params* p) {
+  orc_ctx* c = xcalloc(1, sizeof(orc_ctx));
+  c->P = *p;
+  /* information_module.f90:77-81,133-168 */
+  c->dx = p->box_x / p->nx;
+  c->dy = p->box_y / p->ny;
+  c->h = p->h_factor * c->dx;
+  c->chain_d = (3 + 1) * c->h;
+  c->chain_ox = -(5 * c->chain_d);
+  c->chain_oy = -(5 * c->chain_d);
+  c->chain_nx = (int)(p->box_x / c->chain_d + 2 * 5);
+  c->chain_ny = (int)(p->box_y / c->chain_d + 2 * 5);
+  c->e0 = p->c0sq / (7.0 - 1.0);
+  c->max_shift = 0.25 * c->dx;
+  int n = p->nx * p->ny;
+  c->cap_global = (int)(n * 2.5) + 64;      /* n_total = 2.5 * interior number (:124-125) */
+  c->n_fluid = 0;
+#define A(f) c->f = xcalloc(c->cap_global, sizeof(double))
+  A(gx); A(gy); A(gu); A(gv); A(gmass); A(grho); A(gh); A(gc); A(gp); A(ge);
+  A(ana_u); A(ana_v); A(ana_p); A(ana_vm);
+#undef A
+  c->gtype = xcalloc(c->cap_global, sizeof(int));
+  alloc_local(c, c->cap_global);
+  return c;
+}
+
+void orc_destroy(orc_ctx* c) {
+  if (!c) return;
+  void* ptrs[] = {c->gx, c->gy, c->gu, c->gv, c->gmass, c->grho, c->gh, c->gc, c->gp, c->ge,
+                  c->ana_u, c->ana_v, c->ana_p, c->ana_vm, c->gtype,
+                  c->name, c->x, c->y, c->u, c->v, c->mass, c->rho, c->hh, c->c, c->p, c->e,
+                  c->vol, c->type, c->chain_no, c->cand_n, c->cand, c->fra_v, c->sum_w, c->sdwx,
+                  c->sdwy, c->T, c->Tinv, c->nrmx, c->nrmy, c->surf, c->near, c->Sxx, c->Sxy,
+                  c->Syy, c->txx, c->txy, c->tyy, c->drhodt, c->dedt, c->ax, c->ay, c->ia_x,
+                  c->ia_y, c->av_x, c->av_y, c->ai_x, c->ai_y, c->sps_x, c->sps_y, c->lap_x,
+                  c->lap_y, c->ext_x, c->ext_y, c->avg_x, c->avg_y, c->corr, c->grx, c->gry,
+                  c->shx, c->shy, c->b_rho, c->b_e, c->b_x, c->b_y, c->b_u, c->b_v, c->t_u,
+                  c->t_v, c->t_drho, c->t_de, c->t_ax, c->t_ay, c->mls_rho,
+                  c->cell_count, c->cell_list, c->near_n, c->near_name,
+                  c->pi, c->pj, c->pw, c->pdwx, c->pdwy, c->pdx, c->pdy, c->pvx, c->pvy, c->prd,
+                  c->pc_, c->prho_, c->ph_, c->ndix, c->ndiy, c->ndjx, c->ndjy};
+  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
+  for (int k = 0; k < 3; ++k) {
+    free(c->k_u[k]); free(c->k_v[k]); free(c->k_drho[k]); free(c->k_de[k]);
+    free(c->k_ax[k]); free(c->k_ay[k]);
+  }
+  free(c);
+}
+
+/* ------------------------------------------------------------------ */
+/* Distribute_Initial_Particle.f90:204-249  (TGV lattice)             */
+/* ------------------------------------------------------------------ */
+void orc_init_tgv(orc_ctx* c) {
+  const orc_params* P = &c->P;
+  int k = 0;
+  for (int i = 1; i <= P->ny; ++i) {
+    for (int j = 1; j <= P->nx; ++j) {
+      double x = (j - 1) * c->dx + c->dx / 2.0;
+      double y = (i - 1) * c->dy + c->dy / 2.0;
+      c->gx[k] = x; c->gy[k] = y;
+      c->gu[k] = P->U * sin(2 * PI_MODULE * x) * cos(2 * PI_MODULE * y);
+      c->gv[k] = -P->U * cos(2 * PI_MODULE * x) * sin(2 * PI_MODULE * y);
+      double pr = 0.25 * P->rho0 * (P->U * P->U) * (cos(4 * PI_MODULE * x) + cos(4 * PI_MODULE * y));
+      c->gp[k] = pr;
+      /* exponent 0.142857142 is a REAL(4) literal */
+      c->grho[k] = P->rho0 * pow(1 + 7 * pr / (P->c0sq * P->rho0), (double)0.142857142f);
+      ++k;
+    }
+  }
+  c->n_fluid = k;
+  double vol0 = c->dx * c->dy;
+  for (int i = 0; i < c->n_fluid; ++i) {
+    c->gmass[i] = vol0 * c->grho[i];
+    c->gtype[i] = 2;
+    c->gh[i] = c->h;
+    c->gc[i] = sqrt(P->c0sq);
+    c->ge[i] = c->e0;
+  }
+  /* Initialize_SPH_Calculation.f90:490-496 */
+  double ke = 0.0;
+  for (int i = 0; i < c->n_fluid; ++i)
+    ke = ke + 0.5 * c->gmass[i] * (c->gu[i] * c->gu[i] + c->gv[i] * c->gv[i]);
+  c->ke0 = ke;
+  c->n_ghost = 0;
+}
+
+int32_t orc_set_fluid_state(orc_ctx* c, int32_t n, const double* x, const double* y,
+                            const double* u, const double* v, const double* rho,
+                            const double* mass, const int32_t* type) {
+  if (n + 8 > c->cap_global / 2) return -1;
+  for (int i = 0; i < n; ++i) {
+    c->gx[i] = x[i]; c->gy[i] = y[i]; c->gu[i] = u[i]; c->gv[i] = v[i];
+    c->grho[i] = rho[i]; c->gmass[i] = mass[i]; c->gtype[i] = type ? type[i] : 2;
+    c->gh[i] = c->h; c->gc[i] = sqrt(c->P.c0sq); c->ge[i] = c->e0; c->gp[i] = 0.0;
+  }
+  c->n_fluid = n;
+  c->n_ghost = 0;
+  double ke = 0.0;
+  for (int i = 0; i < n; ++i) ke = ke + 0.5 * c->gmass[i] * (c->gu[i] * c->gu[i] + c->gv[i] * c->gv[i]);
+  c->ke0 = ke;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Distribute_Period_Boundary_Particle.f90:56-206                     */
+/* ------------------------------------------------------------------ */
+static void copy_ghost(orc_ctx* c, int j, int i, double x, double y) {
+  c->gx[j] = x; c->gy[j] = y;
+  c->gu[j] = c->gu[i]; c->gv[j] = c->gv[i];
+  c->grho[j] = c->grho[i]; c->gmass[j] = c->gmass[i];
+  c->gtype[j] = -101;
+  c->gh[j] = c->gh[i]; c->gp[j] = c->gp[i]; c->gc[j] = c->gc[i]; c->ge[j] = c->ge[i];
+}
+
+void orc_period_boundary(orc_ctx* c) {
+  const double xmin = 0.0, xmax = c->P.box_x, lx = c->P.box_x;
+  const double ymin = 0.0, ymax = c->P.box_y, ly = c->P.box_y;
+  const double r3h = 3 * c->h;   /* kernel_scale*smooth_length */
+  int N = c->n_fluid;
+  for (int i = 0; i < N; ++i) {                       /* :56-90 wrap */
+    if (c->gtype[i] == 120) continue;
+    if (c->gx[i] > xmax) c->gx[i] = c->gx[i] - lx;
+    else if (c->gx[i] < xmin) c->gx[i] = c->gx[i] + lx;
+    if (c->gy[i] > ymax) c->gy[i] = c->gy[i] - ly;
+    else if (c->gy[i] < ymin) c->gy[i] = c->gy[i] + ly;
+  }
+  int ng = 0;
+  for (int i = 0; i < N; ++i) {                       /* :98-148 x-direction ghosts */
+    if (c->gtype[i] == 120) continue;
+    if (c->gx[i] <= (xmin + r3h) && c->gx[i] >= xmin) {
+      ++ng; copy_ghost(c, N + ng - 1, i, c->gx[i] + lx, c->gy[i]);
+    }
+    if (c->gx[i] >= (xmax - r3h) && c->gx[i] <= xmax) {
+      ++ng; copy_ghost(c, N + ng - 1, i, c->gx[i] - lx, c->gy[i]);
+    }
+  }
+  int nx_g = ng;
+  for (int i = 0; i < N + nx_g; ++i) {                /* :153-206 y-direction (and corners) */
+    if (c->gtype[i] == 120) continue;
+    if (c->gy[i] <= (ymin + r3h) && c->gy[i] >= ymin) {
+      ++ng; copy_ghost(c, N + ng - 1, i, c->gx[i], c->gy[i] + ly);
+    }
+    if (c->gy[i] >= (ymax - r3h) && c->gy[i] <= ymax) {
+      ++ng; copy_ghost(c, N + ng - 1, i, c->gx[i], c->gy[i] - ly);
+    }
+  }
+  c->n_ghost = ng;
+}
+
+/* ------------------------------------------------------------------ */
+/* Devide_block_and_send_tasks.f90 on one rank:                       */
+/*  analytic fields (:88-116), global binning (:123-162), local order */
+/*  column-major by cell then ascending gid (:488-633), copy (:666-)  */
+/* ------------------------------------------------------------------ */
+void orc_devide_block(orc_ctx* c, int32_t i_time_step) {
+  const orc_params* P = &c->P;
+  int N = c->n_fluid;
+  int NG = N + c->n_ghost;
+  for (int i = 0; i < N; ++i) {                       /* :75-83 */
+    if (c->gtype[i] == 120) continue;
+    if (c->gy[i] < 0.0) c->gtype[i] = 120;
+  }
+  double t = i_time_step * P->dt / (P->box_x / P->U);  /* Nondimensionalized_real_time */
+  double Re = P->U * P->box_x / P->nu;
+  for (int i = 0; i < N; ++i) {                       /* :88-116 */
+    double x = c->gx[i], y = c->gy[i];
+    c->ana_u[i] = P->U * sin(2 * PI_MODULE * x) * cos(2 * PI_MODULE * y) *
+                  exp(-8 * (PI_MODULE * PI_MODULE) / Re * t);
+    c->ana_v[i] = -P->U * cos(2 * PI_MODULE * x) * sin(2 * PI_MODULE * y) *
+                  exp(-8 * (PI_MODULE * PI_MODULE) / Re * t);
+    c->ana_p[i] = 0.25 * P->rho0 * (P->U * P->U) *
+                  (cos(4 * PI_MODULE * x) + cos(4 * PI_MODULE * y)) *
+                  exp(-16 * (PI_MODULE * PI_MODULE) / Re * t);
+    c->ana_vm[i] = sqrt(c->ana_u[i] * c->ana_u[i] + c->ana_v[i] * c->ana_v[i]);
+  }
+  /* :123-162 binning of all global particles into 4h cells, row-major id */
+  int nx = c->chain_nx, ny = c->chain_ny, ncell = nx * ny;
+  int* cnt = xcalloc(ncell + 1, sizeof(int));
+  int* cellof = xcalloc(NG, sizeof(int));
+  for (int i = 0; i < NG; ++i) {
+    cellof[i] = -1;
+    if (c->gtype[i] == 120 || c->gtype[i] == 0) continue;
+    int xn = (int)floor((c->gx[i] - c->chain_ox) / c->chain_d + 1);
+    int yn = (int)floor((c->gy[i] - c->chain_oy) / c->chain_d + 1);
+    int k = (yn - 1) * nx + xn;
+    if (k <= ncell && k >= 1 && xn >= 1 && xn <= nx) {
+      cellof[i] = k; cnt[k]++;
+    } else {
+      c->gtype[i] = 120;
+    }
+  }
+  /* single rank: start/end column = first/last non-empty column (:281-300) */
+  int* colcnt = xcalloc(nx + 2, sizeof(int));
+  for (int i = 0; i < NG; ++i)
+    if (cellof[i] > 0) colcnt[(cellof[i] - 1) % nx + 1]++;
+  int start = 1, end = nx;
+  for (start = 1; start <= nx; ++start) if (colcnt[start] != 0) break;
+  for (end = nx; end >= 1; --end) if (colcnt[end] != 0) break;
+  /* stable bucket: row-major cell k holds gids in ascending order */
+  int* off = xcalloc(ncell + 2, sizeof(int));
+  for (int k = 1; k <= ncell; ++k) off[k + 1] = off[k] + cnt[k];
+  int* members = xcalloc(NG + 1, sizeof(int));
+  int* fill = xcalloc(ncell + 2, sizeof(int));
+  for (int i = 0; i < NG; ++i)
+    if (cellof[i] > 0) { int k = cellof[i]; members[off[k] + fill[k]++] = i; }
+  /* local order: column k (x) ascending, row j (y) ascending, members ascending gid */
+  int n = 0;
+  for (int col = start; col <= end; ++col)
+    for (int row = 1; row <= ny; ++row) {
+      int k = col + (row - 1) * nx;
+      for (int m = 0; m < cnt[k]; ++m) c->name[n++] = members[off[k] + m];
+    }
+  c->n_local = n;
+  for (int k = 0; k < n; ++k) {                       /* :666-765 copy global -> subdomain */
+    int i = c->name[k];
+    c->x[k] = c->gx[i]; c->y[k] = c->gy[i];
+    c->u[k] = c->gu[i]; c->v[k] = c->gv[i];
+    c->mass[k] = c->gmass[i]; c->type[k] = c->gtype[i];
+    c->rho[k] = c->grho[i]; c->hh[k] = c->gh[i]; c->c[k] = c->gc[i];
+    c->p[k] = c->gp[i]; c->e[k] = c->ge[i];
+  }
+  /* subdomain background grid (:304-360 + Distribute_Backgroud_Grid.f90:72-200) */
+  c->sub_nx = (end - start + 1) + 2;
+  c->sub_ny = ny;
+  c->sub_ncell = c->sub_nx * c->sub_ny;
+  free(c->cell_count); free(c->cell_list); free(c->near_n); free(c->near_name);
+  c->cell_count = xcalloc(c->sub_ncell + 1, sizeof(int));
+  c->cell_list = xcalloc((size_t)(c->sub_ncell + 1) * 300, sizeof(int));
+  c->near_n = xcalloc(c->sub_ncell + 1, sizeof(int));
+  c->near_name = xcalloc((size_t)(c->sub_ncell + 1) * 9, sizeof(int));
+  int gx_n = c->sub_nx, gy_n = c->sub_ny;
+  for (int i = 1; i <= c->sub_ncell; ++i) {
+    int mx = (i - 1) % gx_n + 1, my = (i - 1) / gx_n + 1;
+    int* nm = &c->near_name[i * 9];
+    int* nn = &c->near_n[i];
+    if (mx > 1 && mx < gx_n && my > 1 && my < gy_n) {
+      *nn = 9; nm[0] = i; nm[1] = i - 1; nm[2] = i + 1; nm[3] = i - 1 + gx_n; nm[4] = i + gx_n;
+      nm[5] = i + 1 + gx_n; nm[6] = i - 1 - gx_n; nm[7] = i - gx_n; nm[8] = i + 1 - gx_n;
+    } else if (mx == 1 && my > 1 && my < gy_n) {
+      *nn = 6; nm[0] = i; nm[1] = i + 1; nm[2] = i + gx_n; nm[3] = i + 1 + gx_n; nm[4] = i - gx_n; nm[5] = i + 1 - gx_n;
+    } else if (mx == gx_n && my > 1 && my < gy_n) {
+      *nn = 6; nm[0] = i; nm[1] = i - 1; nm[2] = i + gx_n; nm[3] = i - 1 + gx_n; nm[4] = i - gx_n; nm[5] = i - 1 - gx_n;
+    } else if (my == gy_n && mx > 1 && mx < gx_n) {
+      *nn = 6; nm[0] = i; nm[1] = i - 1; nm[2] = i + 1; nm[3] = i - 1 - gx_n; nm[4] = i - gx_n; nm[5] = i + 1 - gx_n;
+    } else if (my == 1 && mx > 1 && mx < gx_n) {
+      *nn = 6; nm[0] = i; nm[1] = i - 1; nm[2] = i + 1; nm[3] = i - 1 + gx_n; nm[4] = i + gx_n; nm[5] = i + 1 + gx_n;
+    } else if (mx == 1 && my == gy_n) {
+      *nn = 4; nm[0] = i; nm[1] = i + 1; nm[2] = i - gx_n; nm[3] = i + 1 - gx_n;
+    } else if (mx == gx_n && my == gy_n) {
+      *nn = 4; nm[0] = i; nm[1] = i - 1; nm[2] = i - gx_n; nm[3] = i - 1 - gx_n;
+    } else if (mx == 1 && my == 1) {
+      *nn = 4; nm[0] = i; nm[1] = i + 1; nm[2] = i + gx_n; nm[3] = i + 1 + gx_n;
+    } else if (mx == gx_n && my == 1) {
+      *nn = 4; nm[0] = i; nm[1] = i - 1; nm[2] = i + gx_n; nm[3] = i - 1 + gx_n;
+    } else {
+      *nn = 0;
+    }
+  }
+  /* the subdomain grid origin (:330-333) */
+  c->sub_start = start;
+  free(cnt); free(cellof); free(colcnt); free(off); free(members); free(fill);
+}
+
+/* ------------------------------------------------------------------ */
+/* Assign_boundary_particles.f90:75-152,303-321                       */
+/* ------------------------------------------------------------------ */
+static void assign_boundary_particles(orc_ctx* c) {
+  int n = c->n_local;
+  double o_sub = c->chain_ox + (c->sub_start - 1) * c->chain_d;
+  memset(c->cell_count, 0, sizeof(int) * (c->sub_ncell + 1));
+  memset(c->chain_no, 0, sizeof(int) * n);
+  for (int i = 0; i < n; ++i) {
+    if (c->type[i] == 120 || c->type[i] == 0) continue;
+    int xn = (int)floor((c->x[i] - o_sub) / c->chain_d + 1) + 1;
+    int yn = (int)floor((c->y[i] - c->chain_oy) / c->chain_d + 1);
+    int k = (yn - 1) * c->sub_nx + xn;
+    if (k <= c->sub_ncell && k >= 1) {
+      c->chain_no[i] = k;
+      c->cell_count[k]++;
+      if (c->cell_count[k] <= 300) c->cell_list[(size_t)k * 300 + c->cell_count[k] - 1] = i;
+    } else {
+      c->type[i] = 120;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    int mesh = c->chain_no[i];
+    int nn = c->near_n[mesh];
+    int cnt = 0;
+    for (int k = 0; k < nn; ++k) {
+      int nm = c->near_name[mesh * 9 + k];
+      int pm = c->cell_count[nm] < 300 ? c->cell_count[nm] : 300;
+      for (int v = 0; v < pm; ++v) {
+        int j = c->cell_list[(size_t)nm * 300 + v];
+        ++cnt;
+        if (cnt <= 300) c->cand[(size_t)i * 300 + cnt - 1] = j;
+      }
+    }
+    c->cand_n[i] = cnt < 300 ? cnt : 300;
+  }
+  for (int k = 0; k < n; ++k) c->vol[k] = c->mass[k] / c->rho[k];
+}
+
+/* ------------------------------------------------------------------ */
+/* find_particle_pair.f90:61-125                                       */
+/* ------------------------------------------------------------------ */
+static void grow_pairs(orc_ctx* c, int64_t need) {
+  if (need <= c->cap_pair) return;
+  int64_t cap = need * 3 / 2 + 1024;
+#define R(f, t) c->f = realloc(c->f, cap * sizeof(t)); if (!c->f) abort();
+  R(pi, int); R(pj, int); R(pw, double); R(pdwx, double); R(pdwy, double);
+  R(pdx, double); R(pdy, double); R(pvx, double); R(pvy, double); R(prd, double);
+  R(pc_, double); R(prho_, double); R(ph_, double);
+  R(ndix, double); R(ndiy, double); R(ndjx, double); R(ndjy, double);
+#undef R
+  c->cap_pair = cap;
+}
+
+static void find_particle_pair(orc_ctx* c) {
+  int n = c->n_local;
+  c->npair = 0;
+  grow_pairs(c, (int64_t)n * 40 + 16);
+  const double rmin = (double)1.0E-7f;
+  for (int i = 0; i < n - 1; ++i) {
+    for (int L = 0; L < c->cand_n[i]; ++L) {
+      int j = c->cand[(size_t)i * 300 + L];
+      if (j <= i || j >= n) continue;
+      if (c->type[i] != 2 && c->type[j] != 2) continue;
+      double pd[2] = {c->x[i] - c->x[j], c->y[i] - c->y[j]};
+      double r = sqrt(pd[0] * pd[0] + pd[1] * pd[1]);
+      double hb = (c->hh[i] + c->hh[j]) / 2.0;
+      if (r <= (3 * hb) && r >= rmin) {
+        grow_pairs(c, c->npair + 1);
+        int64_t k = c->npair++;
+        double w, dw[2];
+        orc_compute_kernel(2, r, pd, hb, &w, dw);
+        c->pi[k] = i; c->pj[k] = j; c->pw[k] = w;
+        c->pdwx[k] = dw[0]; c->pdwy[k] = dw[1];
+        c->pdx[k] = pd[0]; c->pdy[k] = pd[1];
+        c->pvx[k] = c->u[i] - c->u[j]; c->pvy[k] = c->v[i] - c->v[j];
+        c->prd[k] = c->rho[i] - c->rho[j];
+        c->pc_[k] = (c->c[i] + c->c[j]) / 2.0;
+        c->prho_[k] = (c->rho[i] + c->rho[j]) / 2.0;
+        c->ph_[k] = hb;
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Define_freesurface_and_Normailized_derivates.f90:65-440            */
+/* ------------------------------------------------------------------ */
+static void define_freesurface(orc_ctx* c) {
+  int n = c->n_local;
+  const double sv = PI_MODULE * ((3 * c->h) * (3 * c->h));     /* :65 support_volume */
+  memset(c->fra_v, 0, sizeof(double) * n);
+  memset(c->sum_w, 0, sizeof(double) * n);
+  for (int64_t k = 0; k < c->npair; ++k) {                       /* :79-93 */
+    int i = c->pi[k], j = c->pj[k];
+    c->fra_v[i] = c->fra_v[i] + c->vol[j] / sv;
+    c->fra_v[j] = c->fra_v[j] + c->vol[i] / sv;
+    if (c->type[i] == c->type[j]) {
+      c->sum_w[i] = c->sum_w[i] + c->pw[k] * c->vol[j];
+      c->sum_w[j] = c->sum_w[j] + c->pw[k] * c->vol[i];
+    }
+  }
+  int* slist = xcalloc(n + 1, sizeof(int));
+  int ns = 0;
+  memset(c->surf, 0, sizeof(int) * n);
+  for (int i = 0; i < n; ++i) {                                   /* :119-131 */
+    if (c->type[i] != 2) continue;
+    if (c->fra_v[i] <= (double)0.65f) { slist[ns++] = i; c->surf[i] = 1; }
+  }
+  memset(c->T, 0, sizeof(double) * 4 * n);
+  memset(c->sdwx, 0, sizeof(double) * n);
+  memset(c->sdwy, 0, sizeof(double) * n);
+  for (int64_t k = 0; k < c->npair; ++k) {                       /* :141-159 */
+    int i = c->pi[k], j = c->pj[k];
+    double pd[2] = {-c->pdx[k], -c->pdy[k]};
+    double dw[2] = {c->pdwx[k], c->pdwy[k]};
+    double* sd_i[2] = {&c->sdwx[i], &c->sdwy[i]};
+    double* sd_j[2] = {&c->sdwx[j], &c->sdwy[j]};
+    for (int L = 0; L < 2; ++L) {
+      for (int o = 0; o < 2; ++o) {
+        c->T[i * 4 + L + 2 * o] = c->T[i * 4 + L + 2 * o] + pd[L] * dw[o] * c->vol[j];
+        c->T[j * 4 + L + 2 * o] = c->T[j * 4 + L + 2 * o] + pd[L] * dw[o] * c->vol[i];
+      }
+      *sd_i[L] = *sd_i[L] + dw[L] * c->vol[j];
+      *sd_j[L] = *sd_j[L] - dw[L] * c->vol[i];
+    }
+  }
+  c->n_singular = 0;
+  for (int i = 0; i < n; ++i) {                                   /* :168-188 */
+    double* Ti = &c->Tinv[i * 4];
+    Ti[0] = 1.0; Ti[1] = 0.0; Ti[2] = 0.0; Ti[3] = 1.0;
+    if (c->type[i] != 2) continue;
+    double A[4] = {c->T[i * 4 + 0], c->T[i * 4 + 1], c->T[i * 4 + 2], c->T[i * 4 + 3]};
+    double Ainv[4] = {1.0, 0.0, 0.0, 1.0};
+    int ok = orc_matrix_inversion_less_three(2, A, Ainv);
+    if (!ok) c->n_singular++;
+    /* check_error==1 -> the caller replaces the inverse by the identity (:180-186);
+     * check_error==0 keeps diag=1 and the previous (identity) off-diagonals. */
+    Ti[0] = 1.0; Ti[1] = 0.0; Ti[2] = 0.0; Ti[3] = 1.0;
+  }
+  for (int L = 0; L < ns; ++L) {                                  /* :295-376 */
+    int i = slist[L];
+    double* Ti = &c->Tinv[i * 4];
+    double r1 = c->sdwx[i], r2 = c->sdwy[i];
+    double n1 = Ti[0] * r1 + Ti[2] * r2;
+    double n2 = Ti[1] * r1 + Ti[3] * r2;
+    double len = 0.0;
+    len = len + n1 * n1;
+    len = len + n2 * n2;
+    len = sqrt(len);
+    if (isnan(len)) { c->surf[i] = 0; continue; }
+    if (len != 0.0) { n1 = -n1 / len; n2 = -n2 / len; }
+    else { c->surf[i] = 0; continue; }
+    c->nrmx[i] = n1; c->nrmy[i] = n2;
+    double vx = c->x[i] + c->h * n1, vy = c->y[i] + c->h * n2;
+    for (int k = 0; k < c->cand_n[i]; ++k) {
+      int j = c->cand[(size_t)i * 300 + k];
+      if (c->type[j] != 2) continue;
+      double pdx = c->x[j] - c->x[i], pdy = c->y[j] - c->y[i];
+      double JI = sqrt(pdx * pdx + pdy * pdy);
+      double Nd = n1 * pdx + n2 * pdy;
+      double qx = c->x[j] - vx, qy = c->y[j] - vy;
+      double JT = sqrt(qx * qx + qy * qy);
+      double lim = (double)1.414f * c->h;
+      if (JI >= lim && JT < c->h) {
+        c->surf[i] = 0;
+      } else if (JI < lim && acos(Nd / JI) <= (PI_MODULE / 4.0)) {
+        c->surf[i] = 0;
+      }
+    }
+  }
+  free(slist);
+  memset(c->near, 0, sizeof(int) * n);
+  for (int64_t k = 0; k < c->npair; ++k) {                       /* :418-440 */
+    int i = c->pi[k], j = c->pj[k];
+    if (c->surf[i] == 1) c->near[j] = 1;
+    if (c->surf[j] == 1) c->near[i] = 1;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90:126-635    */
+/* (and Solver_Normalized_Linear_Riemann_Delta_SPS.f90:362-371)       */
+/* ------------------------------------------------------------------ */
+static void solver_riemann_delta_sps(orc_ctx* c) {
+  const orc_params* P = &c->P;
+  int n = c->n_local;
+  for (int64_t k = 0; k < c->npair; ++k) {                       /* :126-139 */
+    int i = c->pi[k], j = c->pj[k];
+    double* Ti = &c->Tinv[i * 4];
+    double* Tj = &c->Tinv[j * 4];
+    double d1 = c->pdwx[k], d2 = c->pdwy[k];
+    c->ndix[k] = Ti[0] * d1 + Ti[2] * d2;
+    c->ndiy[k] = Ti[1] * d1 + Ti[3] * d2;
+    c->ndjx[k] = Tj[0] * d1 + Tj[2] * d2;
+    c->ndjy[k] = Tj[1] * d1 + Tj[3] * d2;
+  }
+  memset(c->Sxx, 0, sizeof(double) * n); memset(c->Sxy, 0, sizeof(double) * n);
+  memset(c->Syy, 0, sizeof(double) * n);
+  memset(c->txx, 0, sizeof(double) * n); memset(c->txy, 0, sizeof(double) * n);
+  memset(c->tyy, 0, sizeof(double) * n);
+  for (int64_t k = 0; k < c->npair; ++k) {                       /* :144-182 */
+    int i = c->pi[k], j = c->pj[k];
+    double vd1 = -c->pvx[k], vd2 = -c->pvy[k];
+    c->Sxx[i] = c->Sxx[i] + vd1 * c->ndix[k] * c->vol[j];
+    c->Sxy[i] = c->Sxy[i] + (vd1 * c->ndiy[k] + vd2 * c->ndix[k]) * c->vol[j];
+    c->Syy[i] = c->Syy[i] + vd2 * c->ndiy[k] * c->vol[j];
+    c->Sxx[j] = c->Sxx[j] + vd1 * c->ndjx[k] * c->vol[i];
+    c->Sxy[j] = c->Sxy[j] + (vd1 * c->ndjy[k] + vd2 * c->ndjx[k]) * c->vol[i];
+    c->Syy[j] = c->Syy[j] + vd2 * c->ndjy[k] * c->vol[i];
+  }
+  const double dL = (double)(sqrtf(2.0f) / 2.0f) * c->dx;        /* :266 */
+  const double c012 = (double)0.12f;
+  const double c23 = (double)(2.0f / 3.0f);
+  const double c23b = (double)((2.0f / 3.0f) * 0.0066f);
+  for (int i = 0; i < n; ++i) {                                   /* :256-296 */
+    if (c->type[i] != 2) continue;
+    double master = c->Sxx[i] * c->Sxx[i] + c->Syy[i] * c->Syy[i] + 0.0;
+    double last = c->Sxy[i] * c->Sxy[i] + 0.0 + 0.0;
+    double abs2 = 2 * master + last;
+    double t0 = c012 * dL;
+    double nut = (t0 * t0) * sqrt(abs2);
+    double kk = c23 * nut * (c->Sxx[i] + c->Syy[i]);
+    double blin = c23b * (dL * dL) * abs2;
+    c->txx[i] = (2 * nut * c->Sxx[i] - kk - blin) * c->rho[i];
+    c->txy[i] = (nut * c->Sxy[i]) * c->rho[i];
+    c->tyy[i] = (2 * nut * c->Syy[i] - kk - blin) * c->rho[i];
+  }
+  const double sqc0 = sqrt(P->c0sq);
+  for (int64_t k = 0; k < c->npair; ++k) {                       /* :300-502 */
+    int i = c->pi[k], j = c->pj[k];
+    double PL = c->p[i], PR = c->p[j];
+    double Pb = (PL + PR) / 2.0;
+    double Iv[2] = {c->u[i], c->v[i]}, Jv[2] = {c->u[j], c->v[j]};
+    double vb[2] = {(Iv[0] + Jv[0]) / 2.0, (Iv[1] + Jv[1]) / 2.0};
+    double xij[2] = {c->pdx[k], c->pdy[k]};
+    double dist = sqrt(xij[0] * xij[0] + xij[1] * xij[1]);
+    double ev[2] = {-(xij[0] / dist), -(xij[1] / dist)};
+    double UL = Iv[0] * ev[0] + Iv[1] * ev[1];
+    double UR = Jv[0] * ev[0] + Jv[1] * ev[1];
+    double Ub = (UL + UR) / 2.0;
+    double cb = c->pc_[k], rb = c->prho_[k];
+    double Us, vs[2], Ps;
+    if (P->solver == ORC_SOLVER_LINEAR_RIEMANN) {
+      double zi = c->rho[i] * c->c[i], zj = c->rho[j] * c->c[j];
+      Us = zi * UR + zj * UL + PL - PR;
+      Us = Us / (zi + zj);
+      vs[0] = Us * ev[0]; vs[1] = Us * ev[1];
+      Ps = zi * PR + zj * PL + c->rho[i] * c->c[i] * c->rho[j] * c->c[j] * (UL - UR);
+      Ps = Ps / (zi + zj);
+    } else {
+      Us = Ub + 0.5 * (PL - PR) / (rb * sqc0);
+      vs[0] = (Us - Ub) * ev[0] + vb[0];
+      vs[1] = (Us - Ub) * ev[1] + vb[1];
+      double d = UL - UR;
+      double bR = 3.0 * (d > 0.0 ? d : 0.0);
+      bR = bR < cb ? bR : cb;
+      Ps = Pb + 0.5 * bR * rb * (UL - UR);
+    }
+    double di[2] = {c->ndix[k], c->ndiy[k]}, dj[2] = {c->ndjx[k], c->ndjy[k]};
+    double Ivs[2], Jvs[2], IPs, JPs;
+    if (c->near[i] == 1) { Ivs[0] = vb[0]; Ivs[1] = vb[1]; IPs = Pb; }
+    else { Ivs[0] = vs[0]; Ivs[1] = vs[1]; IPs = Ps; }
+    if (c->near[j] == 1) { Jvs[0] = vb[0]; Jvs[1] = vb[1]; JPs = Pb; }
+    else { Jvs[0] = vs[0]; Jvs[1] = vs[1]; JPs = Ps; }
+    double ti = (Iv[0] - Ivs[0]) * di[0] + (Iv[1] - Ivs[1]) * di[1];
+    double tj = -((Jv[0] - Jvs[0]) * dj[0] + (Jv[1] - Jvs[1]) * dj[1]);
+    c->drhodt[i] = c->drhodt[i] + 2 * c->rho[i] * c->vol[j] * ti;
+    c->drhodt[j] = c->drhodt[j] + 2 * c->rho[j] * c->vol[i] * tj;
+    double rij = 1.0 / (c->rho[i] * c->rho[j]);
+    double* iax[2] = {&c->ia_x[i], &c->ia_y[i]};
+    double* jax[2] = {&c->ia_x[j], &c->ia_y[j]};
+    for (int L = 0; L < 2; ++L) {
+      double tpi = -2 * IPs * di[L] * rij;
+      double tpj = -2 * JPs * dj[L] * rij;
+      *iax[L] = *iax[L] + c->mass[j] * tpi;
+      *jax[L] = *jax[L] - c->mass[i] * tpj;
+    }
+    double hb = c->ph_[k];
+    double vij[2] = {c->pvx[k], c->pvy[k]};
+    double r2 = xij[0] * xij[0] + xij[1] * xij[1];
+    double vx = vij[0] * xij[0] + vij[1] * xij[1];
+    if (vx < 0.0) {                                              /* :392-417 */
+      double fhb = P->fai * hb;
+      double phi = hb * vx / (r2 + fhb * fhb);
+      double Pi = (-(P->alpha * cb * phi) + P->beta * (phi * phi)) / rb;
+      double* iav[2] = {&c->av_x[i], &c->av_y[i]};
+      double* jav[2] = {&c->av_x[j], &c->av_y[j]};
+      for (int L = 0; L < 2; ++L) {
+        double tvi = -Pi * di[L];
+        double tvj = -Pi * dj[L];
+        *iav[L] = *iav[L] + c->mass[j] * tvi;
+        *jav[L] = *jav[L] - c->mass[i] * tvj;
+      }
+    }
+    double Ri = xij[0] * di[0] + xij[1] * di[1];                  /* :432-436 */
+    c->lap_x[i] = c->lap_x[i] + c->vol[j] * 2 * P->nu * Ri / r2 * vij[0];
+    c->lap_y[i] = c->lap_y[i] + c->vol[j] * 2 * P->nu * Ri / r2 * vij[1];
+    double Rj = xij[0] * dj[0] + xij[1] * dj[1];
+    c->lap_x[j] = c->lap_x[j] - c->vol[i] * 2 * P->nu * Rj / r2 * vij[0];
+    c->lap_y[j] = c->lap_y[j] - c->vol[i] * 2 * P->nu * Rj / r2 * vij[1];
+    double tmp = P->delta_alpha * hb * cb * vx / r2;              /* :442-446 */
+    double* iai[2] = {&c->ai_x[i], &c->ai_y[i]};
+    double* jai[2] = {&c->ai_x[j], &c->ai_y[j]};
+    for (int L = 0; L < 2; ++L) {
+      *iai[L] = *iai[L] + tmp * di[L] * P->rho0 / c->rho[i] * c->vol[j];
+      *jai[L] = *jai[L] - tmp * dj[L] * P->rho0 / c->rho[j] * c->vol[i];
+    }
+    /* :450-492 SPS with raw dwdx and the GLOBAL particle_mass at LOCAL index */
+    double d1 = c->pdwx[k], d2 = c->pdwy[k];
+    double tp = 0.0;
+    tp = tp + (c->txx[i] + c->txx[j]) * d1;
+    tp = tp + (c->txy[i] + c->txy[j]) * d2;
+    tp = tp * rij;
+    c->sps_x[i] = c->sps_x[i] + c->gmass[j] * tp;
+    c->sps_x[j] = c->sps_x[j] - c->gmass[i] * tp;
+    tp = tp + (c->tyy[i] + c->tyy[j]) * d2 + (c->txy[i] + c->txy[j]) * d1;
+    tp = tp * rij;
+    c->sps_y[i] = c->sps_y[i] + c->gmass[j] * tp;
+    c->sps_y[j] = c->sps_y[j] - c->gmass[i] * tp;
+    if (P->solver == ORC_SOLVER_MODIFIED_LINEAR_RIEMANN) {       /* :495-500 XSPH */
+      double rs = c->rho[i] + c->rho[j];
+      c->avg_x[i] = c->avg_x[i] - P->epsilon_xsph * 2 * c->mass[j] * vij[0] / rs * c->pw[k];
+      c->avg_y[i] = c->avg_y[i] - P->epsilon_xsph * 2 * c->mass[j] * vij[1] / rs * c->pw[k];
+      c->avg_x[j] = c->avg_x[j] + P->epsilon_xsph * 2 * c->mass[i] * vij[0] / rs * c->pw[k];
+      c->avg_y[j] = c->avg_y[j] + P->epsilon_xsph * 2 * c->mass[i] * vij[1] / rs * c->pw[k];
+    }
+  }
+  memset(c->grx, 0, sizeof(double) * n); memset(c->gry, 0, sizeof(double) * n);
+  for (int64_t k = 0; k < c->npair; ++k) {                       /* :508-526 */
+    int i = c->pi[k], j = c->pj[k];
+    double rd = -c->prd[k];
+    double* Ti = &c->Tinv[i * 4];
+    double* Tj = &c->Tinv[j * 4];
+    double d1 = c->pdwx[k], d2 = c->pdwy[k];
+    double mi1 = Ti[0] * d1 + Ti[2] * d2, mi2 = Ti[1] * d1 + Ti[3] * d2;
+    double mj1 = Tj[0] * d1 + Tj[2] * d2, mj2 = Tj[1] * d1 + Tj[3] * d2;
+    c->grx[i] = c->grx[i] + rd * c->vol[j] * mi1;
+    c->gry[i] = c->gry[i] + rd * c->vol[j] * mi2;
+    c->grx[j] = c->grx[j] + rd * c->vol[i] * mj1;
+    c->gry[j] = c->gry[j] + rd * c->vol[i] * mj2;
+  }
+  for (int64_t k = 0; k < c->npair; ++k) {                       /* :552-583 */
+    int i = c->pi[k], j = c->pj[k];
+    double rd = -c->prd[k];
+    double hb = c->ph_[k], cb = c->pc_[k];
+    double pd[2] = {-c->pdx[k], -c->pdy[k]};
+    double r2 = pd[0] * pd[0] + pd[1] * pd[1];
+    double sg[2] = {c->grx[i] + c->grx[j], c->gry[i] + c->gry[j]};
+    double psi = rd - 0.5 * (sg[0] * pd[0] + sg[1] * pd[1]);
+    double ti = (pd[0] * c->ndix[k] + pd[1] * c->ndiy[k]) / r2;
+    double tj = (pd[0] * c->ndjx[k] + pd[1] * c->ndjy[k]) / r2;
+    c->corr[i] = c->corr[i] + 2 * P->delta * hb * cb * psi * ti * c->vol[j];
+    c->corr[j] = c->corr[j] - 2 * P->delta * hb * cb * psi * tj * c->vol[i];
+  }
+  for (int i = 0; i < n; ++i) {                                   /* :627-635 */
+    if (c->type[i] != 2) continue;
+    c->drhodt[i] = c->drhodt[i] + c->corr[i];
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* single_step_compute.f90:62-171                                      */
+/* ------------------------------------------------------------------ */
+void orc_single_step_compute(orc_ctx* c, int32_t i_time_step) {
+  (void)i_time_step;
+  int n = c->n_local;
+  for (int i = 0; i < n; ++i) {                                   /* :62-73 EOS */
+    if (c->type[i] == 120) continue;
+    orc_status_equation(c, c->rho[i], c->e[i], c->type[i], &c->p[i], &c->c[i]);
+  }
+  memset(c->vol, 0, sizeof(double) * n);
+  assign_boundary_particles(c);                                  /* :80 */
+  find_particle_pair(c);                                         /* :93 */
+  define_freesurface(c);                                         /* :108 */
+  size_t nb = sizeof(double) * n;                                /* :113-126 */
+  memset(c->drhodt, 0, nb); memset(c->dedt, 0, nb);
+  memset(c->ia_x, 0, nb); memset(c->ia_y, 0, nb);
+  memset(c->ai_x, 0, nb); memset(c->ai_y, 0, nb);
+  memset(c->av_x, 0, nb); memset(c->av_y, 0, nb);
+  memset(c->ext_x, 0, nb); memset(c->ext_y, 0, nb);
+  memset(c->avg_x, 0, nb); memset(c->avg_y, 0, nb);
+  memset(c->corr, 0, nb);
+  memset(c->lap_x, 0, nb); memset(c->lap_y, 0, nb);
+  memset(c->sps_x, 0, nb); memset(c->sps_y, 0, nb);
+  solver_riemann_delta_sps(c);                                   /* :137 */
+  memset(c->ax, 0, nb); memset(c->ay, 0, nb);                    /* :152-171 */
+  for (int i = 0; i < n; ++i) {
+    if (c->type[i] != 2) continue;
+    c->ax[i] = c->ia_x[i] + c->ext_x[i] + c->av_x[i] + c->ai_x[i] + c->sps_x[i] + c->lap_x[i];
+    c->ay[i] = c->ia_y[i] + c->ext_y[i] + c->av_y[i] + c->ai_y[i] + c->sps_y[i] + c->lap_y[i];
+    c->dedt[i] = c->dedt[i] + 0.0 + 0.0;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Reinitialize_particle_density.f90:66-182 +                          */
+/* MLS_Interpolation_In_Subdomain_Domain.f90:68-297 (orders 1 and 6)   */
+/* ------------------------------------------------------------------ */
+static void mls6_fai(const orc_ctx* c, double xi, double yi, int ne, const int* nm,
+                     const double* wk, double* fai) {
+  const double al = c->P.mls_alpha;
+  const double h = c->h;
+  /* P (6 x (ne+6)), B = P*W, A = B*P^T with k ascending */
+  int nc = ne + 6;
+  double* Pm = xcalloc((size_t)6 * nc, sizeof(double));
+  double* Bm = xcalloc((size_t)6 * nc, sizeof(double));
+  for (int k = 0; k < ne; ++k) {
+    int j = nm[k];
+    double a = (c->x[j] - xi) / h, b = (c->y[j] - yi) / h;
+    Pm[0 + 6 * k] = 1;
+    Pm[1 + 6 * k] = a;
+    Pm[2 + 6 * k] = b;
+    Pm[3 + 6 * k] = a * a;
+    Pm[4 + 6 * k] = (c->x[j] - xi) / h * (c->y[j] - yi) / h;
+    Pm[5 + 6 * k] = b * b;
+  }
+  for (int L = 0; L < 6; ++L) Pm[L + 6 * (ne + L)] = al;
+  for (int k = 0; k < nc; ++k) {
+    double wkk = k < ne ? wk[k] : al;
+    for (int m = 0; m < 6; ++m) Bm[m + 6 * k] = Pm[m + 6 * k] * wkk;
+  }
+  double A[36] = {0}, Ai[36];
+  for (int l = 0; l < 6; ++l)
+    for (int m = 0; m < 6; ++m) {
+      double s = 0.0;
+      for (int k = 0; k < nc; ++k) s = s + Bm[m + 6 * k] * Pm[l + 6 * k];
+      A[m + 6 * l] = s;
+    }
+  memcpy(Ai, A, sizeof(A));
+  int ok = orc_brinv(Ai, 6);
+  if (ok) {
+    for (int k = 0; k < ne; ++k) {
+      double s = 0.0;
+      for (int l = 0; l < 6; ++l) s = s + Ai[0 + 6 * l] * Bm[l + 6 * k];
+      fai[k] = s;
+    }
+  } else {
+    double sw = 0.0;
+    for (int k = 0; k < ne; ++k) sw = sw + wk[k];
+    for (int k = 0; k < ne; ++k) fai[k] = wk[k] / sw;
+  }
+  if (ne <= 8) {
+    double sw = 0.0;
+    for (int k = 0; k < ne; ++k) sw = sw + wk[k];
+    for (int k = 0; k < ne; ++k) fai[k] = wk[k] / sw;
+  }
+  free(Pm); free(Bm);
+}
+
+void orc_reinitialize_density(orc_ctx* c) {
+  int n = c->n_local;
+  memcpy(c->mls_rho, c->rho, sizeof(double) * n);
+  memset(c->rho, 0, sizeof(double) * n);
+  if (c->P.mls_order == 1) {
+    for (int64_t k = 0; k < c->npair; ++k) {
+      int i = c->pi[k], j = c->pj[k];
+      if (c->type[i] == c->type[j]) {
+        if (c->type[i] == 2) c->rho[i] = c->rho[i] + c->mass[j] * c->pw[k] / c->sum_w[i];
+        if (c->type[j] == 2) c->rho[j] = c->rho[j] + c->mass[i] * c->pw[k] / c->sum_w[j];
+      }
+    }
+    return;
+  }
+  int nm[300]; double wk[300], fai[300];
+  for (int i = 0; i < n; ++i) {
+    if (c->type[i] != 2) continue;
+    c->rho[i] = 0.0;
+    int ne = 0;
+    for (int L = 0; L < c->cand_n[i]; ++L) {
+      int j = c->cand[(size_t)i * 300 + L];
+      double pd[2] = {c->x[i] - c->x[j], c->y[i] - c->y[j]};
+      double r = sqrt(pd[0] * pd[0] + pd[1] * pd[1]);
+      double hb = (c->hh[i] + c->hh[j]) / 2.0;
+      if (r <= (3 * hb) && r >= (double)1.0e-7f) {
+        double w, dw[2];
+        orc_compute_kernel(2, r, pd, hb, &w, dw);
+        nm[ne] = j; wk[ne] = w; ++ne;
+      }
+    }
+    mls6_fai(c, c->x[i], c->y[i], ne, nm, wk, fai);
+    for (int L = 0; L < ne; ++L) c->rho[i] = c->rho[i] + c->mls_rho[nm[L]] * fai[L];
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Shift_particle_position.f90:56-136 (pairs of the last evaluation)   */
+/* ------------------------------------------------------------------ */
+void orc_shift(orc_ctx* c) {
+  const orc_params* P = &c->P;
+  int n = c->n_local;
+  double pd0[2] = {0.0, 0.0}, wdx, dwd[2];
+  orc_compute_kernel(2, c->dx, pd0, c->h, &wdx, dwd);
+  memset(c->shx, 0, sizeof(double) * n); memset(c->shy, 0, sizeof(double) * n);
+  for (int64_t k = 0; k < c->npair; ++k) {
+    int i = c->pi[k], j = c->pj[k];
+    double f = P->shift_beta * (c->h * c->h) * (1 + P->shift_R * ipow4(c->pw[k] / wdx));
+    c->shx[i] = c->shx[i] - f * c->ndix[k] * c->vol[j];
+    c->shx[j] = c->shx[j] + f * c->ndjx[k] * c->vol[i];
+    c->shy[i] = c->shy[i] - f * c->ndiy[k] * c->vol[j];
+    c->shy[j] = c->shy[j] + f * c->ndjy[k] * c->vol[i];
+  }
+  for (int i = 0; i < n; ++i) {
+    if (c->type[i] != 2) continue;
+    if (c->near[i] == 1) continue;
+    double a = sqrt(c->shx[i] * c->shx[i] + c->shy[i] * c->shy[i]);
+    if (a > c->max_shift) {
+      c->shx[i] = c->shx[i] / a * c->max_shift;
+      c->shy[i] = c->shy[i] / a * c->max_shift;
+    }
+    c->x[i] = c->x[i] + c->shx[i];
+    c->y[i] = c->y[i] + c->shy[i];
+    c->rho[i] = c->rho[i] + c->grx[i] * c->shx[i];
+    c->rho[i] = c->rho[i] + c->gry[i] * c->shy[i];
+    c->p[i] = (P->c0sq * P->rho0 / 7.0) * (ipow7(c->rho[i] / P->rho0) - 1.0);
+  }
+}
+
+/* Refresh_domain.f90:136-300 on one rank: global arrays zeroed, then filled. */
+void orc_refresh(orc_ctx* c) {
+  int NG = c->n_fluid + c->n_ghost;
+  size_t nb = sizeof(double) * NG;
+  memset(c->gx, 0, nb); memset(c->gy, 0, nb); memset(c->gu, 0, nb); memset(c->gv, 0, nb);
+  memset(c->gp, 0, nb); memset(c->gmass, 0, nb); memset(c->grho, 0, nb); memset(c->gh, 0, nb);
+  memset(c->gc, 0, nb); memset(c->ge, 0, nb); memset(c->gtype, 0, sizeof(int) * NG);
+  for (int k = 0; k < c->n_local; ++k) {
+    int g = c->name[k];
+    c->gx[g] = c->x[k]; c->gy[g] = c->y[k]; c->gu[g] = c->u[k]; c->gv[g] = c->v[k];
+    c->gp[g] = c->p[k]; c->gmass[g] = c->mass[k]; c->grho[g] = c->rho[k];
+    c->gh[g] = c->hh[k]; c->gc[g] = c->c[k]; c->ge[g] = c->e[k]; c->gtype[g] = c->type[k];
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* time_iteration.f90: PC (:509-657) and RK-variant (:779-1049) steps  */
+/* ------------------------------------------------------------------ */
+static void save_before(orc_ctx* c) {
+  size_t nb = sizeof(double) * c->n_local;
+  memcpy(c->b_rho, c->rho, nb); memcpy(c->b_u, c->u, nb); memcpy(c->b_v, c->v, nb);
+  memcpy(c->b_x, c->x, nb); memcpy(c->b_y, c->y, nb); memcpy(c->b_e, c->e, nb);
+}
+
+/* stage with rates r at fraction f of dt: the RK-variant writes drhodt*dt/2 etc. */
+static void stage_half(orc_ctx* c) {
+  double dt = c->P.dt;
+  for (int i = 0; i < c->n_local; ++i) {
+    if (c->type[i] != 2) continue;
+    c->rho[i] = c->b_rho[i] + c->drhodt[i] * dt / 2;
+    c->e[i] = c->b_e[i] + c->dedt[i] * dt / 2;
+    c->u[i] = c->b_u[i] + 0.5 * dt * c->ax[i] + c->avg_x[i];
+    c->x[i] = c->b_x[i] + 0.5 * dt * c->u[i];
+    c->v[i] = c->b_v[i] + 0.5 * dt * c->ay[i] + c->avg_y[i];
+    c->y[i] = c->b_y[i] + 0.5 * dt * c->v[i];
+  }
+}
+static void stage_full(orc_ctx* c) {
+  double dt = c->P.dt;
+  for (int i = 0; i < c->n_local; ++i) {
+    if (c->type[i] != 2) continue;
+    c->rho[i] = c->b_rho[i] + c->drhodt[i] * dt;
+    c->e[i] = c->b_e[i] + c->dedt[i] * dt;
+    c->u[i] = c->b_u[i] + dt * c->ax[i] + c->avg_x[i];
+    c->x[i] = c->b_x[i] + dt * c->u[i];
+    c->v[i] = c->b_v[i] + dt * c->ay[i] + c->avg_y[i];
+    c->y[i] = c->b_y[i] + dt * c->v[i];
+  }
+}
+static void save_k(orc_ctx* c, int s) {
+  size_t nb = sizeof(double) * c->n_local;
+  memcpy(c->k_u[s], c->u, nb); memcpy(c->k_v[s], c->v, nb);
+  memcpy(c->k_drho[s], c->drhodt, nb); memcpy(c->k_de[s], c->dedt, nb);
+  memcpy(c->k_ax[s], c->ax, nb); memcpy(c->k_ay[s], c->ay, nb);
+}
+static void rk_final(orc_ctx* c) {
+  double dt = c->P.dt;
+  for (int i = 0; i < c->n_local; ++i) {
+    if (c->type[i] != 2) continue;
+    c->rho[i] = c->b_rho[i] + (c->k_drho[0][i] + 2 * c->k_drho[1][i] + 2 * c->k_drho[2][i] + c->drhodt[i]) * dt / 6.0;
+    c->e[i] = c->b_e[i] + (c->k_de[0][i] + 2 * c->k_de[1][i] + 2 * c->k_de[2][i] + c->dedt[i]) * dt / 6.0;
+    c->u[i] = c->b_u[i] + (c->k_ax[0][i] + 2 * c->k_ax[1][i] + 2 * c->k_ax[2][i] + c->ax[i]) * dt / 6.0;
+    c->x[i] = c->b_x[i] + (c->k_u[0][i] + 2 * c->k_u[1][i] + 2 * c->k_u[2][i] + c->u[i]) * dt / 6.0 + 0.5 * dt * dt * c->ax[i];
+    c->v[i] = c->b_v[i] + (c->k_ay[0][i] + 2 * c->k_ay[1][i] + 2 * c->k_ay[2][i] + c->ay[i]) * dt / 6.0;
+    c->y[i] = c->b_y[i] + (c->k_v[0][i] + 2 * c->k_v[1][i] + 2 * c->k_v[2][i] + c->v[i]) * dt / 6.0 + 0.5 * dt * dt * c->ay[i];
+  }
+}
+static void pc_predict(orc_ctx* c) { stage_full(c); }
+static void pc_save(orc_ctx* c) {
+  size_t nb = sizeof(double) * c->n_local;
+  memcpy(c->t_u, c->u, nb); memcpy(c->t_v, c->v, nb); memcpy(c->t_drho, c->drhodt, nb);
+  memcpy(c->t_de, c->dedt, nb); memcpy(c->t_ax, c->ax, nb); memcpy(c->t_ay, c->ay, nb);
+}
+static void pc_correct(orc_ctx* c) {
+  double dt = c->P.dt;
+  for (int i = 0; i < c->n_local; ++i) {
+    if (c->type[i] != 2) continue;
+    c->rho[i] = c->b_rho[i] + 0.5 * (c->drhodt[i] + c->t_drho[i]) * dt;
+    c->e[i] = c->b_e[i] + 0.5 * (c->dedt[i] + c->t_de[i]) * dt;
+    c->u[i] = c->b_u[i] + 0.5 * dt * (c->ax[i] + c->t_ax[i]) + c->avg_x[i];
+    c->x[i] = c->b_x[i] + 0.5 * dt * (c->u[i] + c->t_u[i]) + 0.5 * dt * dt * c->ax[i];
+    c->v[i] = c->b_v[i] + 0.5 * dt * (c->ay[i] + c->t_ay[i]) + c->avg_y[i];
+    c->y[i] = c->b_y[i] + 0.5 * dt * (c->v[i] + c->t_v[i]) + 0.5 * dt * dt * c->ay[i];
+  }
+}
+
+/* Carry the previous step's rates with the particle (by gid) instead of by local
+ * index -- a deliberate deviation from time_iteration.f90:542-555,942-957. */
+static void remap_rates(orc_ctx* c, const int* old_name, int old_n) {
+  int NG = c->n_fluid + c->n_ghost + 1;
+  int cap = c->cap_global;
+  int* inv = xcalloc(cap, sizeof(int));
+  for (int i = 0; i < cap; ++i) inv[i] = -1;
+  for (int k = 0; k < old_n; ++k) inv[old_name[k]] = k;
+  double *d = xcalloc(c->n_local, 8), *ax = xcalloc(c->n_local, 8), *ay = xcalloc(c->n_local, 8);
+  for (int k = 0; k < c->n_local; ++k) {
+    int g = c->name[k];
+    int o = (g < cap) ? inv[g] : -1;
+    if (o >= 0 && g < c->n_fluid) { d[k] = c->drhodt[o]; ax[k] = c->ax[o]; ay[k] = c->ay[o]; }
+  }
+  memcpy(c->drhodt, d, 8 * c->n_local); memcpy(c->ax, ax, 8 * c->n_local); memcpy(c->ay, ay, 8 * c->n_local);
+  free(d); free(ax); free(ay); free(inv); (void)NG;
+}
+
+void orc_time_step(orc_ctx* c, int32_t it) {
+  int* old_name = NULL; int old_n = c->n_local;
+  if (c->P.rate_reuse == 1 && it > 1) {
+    old_name = xcalloc(old_n + 1, sizeof(int));
+    memcpy(old_name, c->name, sizeof(int) * old_n);
+  }
+  orc_period_boundary(c);
+  orc_devide_block(c, it);
+  if (old_name) { remap_rates(c, old_name, old_n); free(old_name); }
+  if (c->P.integrator == 2) {                                    /* :509-657 */
+    save_before(c);
+    if (it == 1) orc_single_step_compute(c, it);
+    pc_predict(c);
+    pc_save(c);
+    orc_single_step_compute(c, it);
+    pc_correct(c);
+  } else {                                                       /* :779-1049 */
+    save_before(c);
+    if (it == 1) orc_single_step_compute(c, it);
+    stage_half(c); save_k(c, 0);
+    orc_single_step_compute(c, it);
+    stage_half(c); save_k(c, 1);
+    orc_single_step_compute(c, it);
+    stage_full(c); save_k(c, 2);
+    orc_single_step_compute(c, it);
+    rk_final(c);
+  }
+  if (c->P.mls_on && (it % c->P.mls_every) == 0) orc_reinitialize_density(c);
+  if (c->P.shift_on) orc_shift(c);
+  orc_refresh(c);
+}
+
+/* ------------------------------------------------------------------ */
+/* Sampling_subroutine.f90:340-461                                     */
+/* ------------------------------------------------------------------ */
+void orc_sample(const orc_ctx* c, double* out) {
+  const orc_params* P = &c->P;
+  int N = c->n_fluid;
+  double ke = 0.0, vmax = 0.0;
+  double *eu = xcalloc(N, 8), *ev = xcalloc(N, 8), *ep = xcalloc(N, 8), *em = xcalloc(N, 8);
+  double p0 = P->rho0 * P->U * P->U;
+  for (int i = 0; i < N; ++i) {
+    double vv = c->gu[i] * c->gu[i] + c->gv[i] * c->gv[i];
+    ke = ke + 0.5 * c->gmass[i] * vv;
+    double vm = sqrt(vv);
+    if (vm >= vmax) vmax = vm;
+    eu[i] = (c->gu[i] - c->ana_u[i]) / P->U;
+    ev[i] = (c->gv[i] - c->ana_v[i]) / P->U;
+    ep[i] = (c->gp[i] - c->ana_p[i]) / p0;
+    em[i] = (vm - c->ana_vm[i]) / P->U;
+  }
+  out[0] = ke / c->ke0;
+  out[1] = vmax / P->U;
+  double* arrs[4] = {eu, ev, ep, em};
+  for (int a = 0; a < 4; ++a) {
+    double n1 = 0.0, n2 = 0.0, ninf = 0.0;
+    for (int i = 0; i < N; ++i) {
+      double e = fabs(arrs[a][i]);
+      if (e >= ninf) ninf = e;
+      n1 = n1 + e;
+      n2 = n2 + e * e;
+    }
+    out[2 + 3 * a] = ninf;
+    out[3 + 3 * a] = n1 / N;
+    out[4 + 3 * a] = sqrt(n2 / N);
+  }
+  free(eu); free(ev); free(ep); free(em);
+}
+
+double orc_initial_ke(const orc_ctx* c) { return c->ke0; }
+int32_t orc_n_fluid(const orc_ctx* c) { return c->n_fluid; }
+int32_t orc_n_global(const orc_ctx* c) { return c->n_fluid + c->n_ghost; }
+int32_t orc_n_local(const orc_ctx* c) { return c->n_local; }
+int64_t orc_pair_number(const orc_ctx* c) { return c->npair; }
+
+/* ------------------------------------------------------------------ */
+/* named array access                                                  */
+/* ------------------------------------------------------------------ */
+typedef struct { const char* name; size_t off; int kind; } fld;  /* kind 0 local, 1 global, 2 pair, 3 local*4 */
+#define FL(nm, f, k) {nm, offsetof(struct orc_ctx, f), k}
+#include <stddef.h>
+static const fld F64[] = {
+  FL("x", x, 0), FL("y", y, 0), FL("u", u, 0), FL("v", v, 0), FL("mass", mass, 0),
+  FL("rho", rho, 0), FL("h", hh, 0), FL("c", c, 0), FL("p", p, 0), FL("e", e, 0),
+  FL("volume", vol, 0), FL("fra_v", fra_v, 0), FL("sum_w", sum_w, 0),
+  FL("sum_dwdx_x", sdwx, 0), FL("sum_dwdx_y", sdwy, 0), FL("normal_x", nrmx, 0),
+  FL("normal_y", nrmy, 0), FL("S_xx", Sxx, 0), FL("S_xy", Sxy, 0), FL("S_yy", Syy, 0),
+  FL("tau_xx", txx, 0), FL("tau_xy", txy, 0), FL("tau_yy", tyy, 0),
+  FL("drhodt", drhodt, 0), FL("dedt", dedt, 0), FL("ax", ax, 0), FL("ay", ay, 0),
+  FL("internal_x", ia_x, 0), FL("internal_y", ia_y, 0), FL("visc_x", av_x, 0),
+  FL("visc_y", av_y, 0), FL("art_internal_x", ai_x, 0), FL("art_internal_y", ai_y, 0),
+  FL("sps_x", sps_x, 0), FL("sps_y", sps_y, 0), FL("lap_x", lap_x, 0), FL("lap_y", lap_y, 0),
+  FL("avg_vel_x", avg_x, 0), FL("avg_vel_y", avg_y, 0), FL("rho_correct", corr, 0),
+  FL("grad_rho_x", grx, 0), FL("grad_rho_y", gry, 0), FL("shift_x", shx, 0),
+  FL("shift_y", shy, 0),
+  FL("T", T, 3), FL("T_inv", Tinv, 3),
+  FL("g_x", gx, 1), FL("g_y", gy, 1), FL("g_u", gu, 1), FL("g_v", gv, 1),
+  FL("g_mass", gmass, 1), FL("g_rho", grho, 1), FL("g_h", gh, 1), FL("g_c", gc, 1),
+  FL("g_p", gp, 1), FL("g_e", ge, 1),
+  FL("ana_u", ana_u, 4), FL("ana_v", ana_v, 4), FL("ana_p", ana_p, 4), FL("ana_vm", ana_vm, 4),
+  FL("pair_w", pw, 2), FL("pair_dwdx_x", pdwx, 2), FL("pair_dwdx_y", pdwy, 2),
+};
+static const fld I32[] = {
+  FL("type", type, 0), FL("name", name, 0), FL("surface", surf, 0), FL("near_surface", near, 0),
+  FL("candidate_n", cand_n, 0), FL("g_type", gtype, 1), FL("pair_i", pi, 2), FL("pair_j", pj, 2),
+};
+static int64_t fld_count(const orc_ctx* c, int kind) {
+  switch (kind) {
+    case 0: return c->n_local;
+    case 1: return c->n_fluid + c->n_ghost;
+    case 2: return c->npair;
+    case 3: return (int64_t)c->n_local * 4;
+    case 4: return c->n_fluid;
+  }
+  return 0;
+}
+double* orc_mut_f64(orc_ctx* c, const char* name, int64_t* n) {
+  for (size_t k = 0; k < sizeof(F64) / sizeof(F64[0]); ++k)
+    if (!strcmp(F64[k].name, name)) {
+      if (n) *n = fld_count(c, F64[k].kind);
+      return *(double**)((char*)c + F64[k].off);
+    }
+  if (n) *n = -1;
+  return NULL;
+}
+int32_t* orc_mut_i32(orc_ctx* c, const char* name, int64_t* n) {
+  for (size_t k = 0; k < sizeof(I32) / sizeof(I32[0]); ++k)
+    if (!strcmp(I32[k].name, name)) {
+      if (n) *n = fld_count(c, I32[k].kind);
+      return *(int32_t**)((char*)c + I32[k].off);
+    }
+  if (n) *n = -1;
+  return NULL;
+}
+const double* orc_get_f64(const orc_ctx* c, const char* name, int64_t* n) {
+  return orc_mut_f64((orc_ctx*)c, name, n);
+}
+const int32_t* orc_get_i32(const orc_ctx* c, const char* name, int64_t* n) {
+  return orc_mut_i32((orc_ctx*)c, name, n);
+}

@@ -1,0 +1,97 @@
+/*
+ * sph2d_oracle.h -- CPU restatement of the reference WCSPH step (TEST INFRASTRUCTURE).
+ *
+ * This library is the checker, never the product: only tests/, the smoke
+ * check and the CPU-baseline leg of bench.py may load it. It restates, in
+ * plain C and in the reference's own order of operations, the Fortran path
+ *   Taylor_Green_Vortex/time_iteration.f90 -> single_step_compute.f90 -> ...
+ * with every reference quirk kept (identity renormalisation, SPS mass index,
+ * stale-index rate re-use, frozen ghosts, REAL(4) literals).
+ * Arrays are 0-based here; "gid" is the reference's 1-based global index - 1.
+ */
+#ifndef SPH2D_ORACLE_H
+#define SPH2D_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Runtime form of the compile-time parameters of
+ * Taylor_Green_Vortex/information_module.f90 (defaults in orc_default_params). */
+typedef struct {
+  int32_t nx, ny;            /* interior_particle_number_x/_y              (:75-76)  */
+  double box_x, box_y;       /* domain_size_x/_y == boundary_size_x/_y     (:54-57)  */
+  double rho0;               /* water_rho_0                                (:31)     */
+  double c0sq;               /* square_c_0                                 (:145)    */
+  double nu;                 /* water_kinematic_viscosity (REAL(4) 1.0E-2) (:37)     */
+  double U;                  /* U_Module                                   (:52)     */
+  double h_factor;           /* smooth_length_factor                       (:134)    */
+  double dt;                 /* dt                                         (:309)    */
+  int32_t integrator;        /* leapforg_or_eular: 2 = PC, 3 = RK-variant  (:303)    */
+  int32_t shift_on;          /* ShiftParticle_Switch                       (:349)    */
+  int32_t mls_on;            /* RefreshDensityByMLS_Switch                 (:324)    */
+  int32_t mls_every;         /* MLS_time_step                              (:332)    */
+  int32_t mls_order;         /* RefreshDensity_MLS_Order (1 or 6)          (:330)    */
+  int32_t solver;            /* 0 = Modified linear Riemann (default), see ORC_SOLVER_* */
+  double alpha, beta, fai;   /* Monaghan artificial viscosity              (:210-212) */
+  double delta, delta_alpha; /* delta-SPH                                  (:217-218) */
+  double epsilon_xsph;       /* XSPH epsilon                               (:207)    */
+  double shift_beta, shift_R;/* shift_belta, shift_R                       (:350-351) */
+  double mls_alpha;          /* slighted_modified_alpha (REAL(4) 0.0001)   (:359)    */
+  int32_t rate_reuse;        /* 0 = reference (stale local index), 1 = carried by gid  */
+} orc_params;
+
+enum { ORC_SOLVER_MODIFIED_LINEAR_RIEMANN = 0,
+       ORC_SOLVER_LINEAR_RIEMANN = 1 };
+
+typedef struct orc_ctx orc_ctx;
+
+void orc_default_params(orc_params* p);
+orc_ctx* orc_create(const orc_params* p);
+void orc_destroy(orc_ctx* c);
+
+/* Scalar restatements (reference argument order). */
+void orc_compute_kernel(int32_t dim, double r, const double* dx, double h, double* w, double* dwdx);
+void orc_status_equation(const orc_ctx* c, double rho, double e, int32_t type, double* p, double* cs);
+int32_t orc_brinv(double* a_colmajor, int32_t n);
+int32_t orc_matrix_inversion_less_three(int32_t n, const double* a_colmajor, double* res_colmajor);
+
+/* Initial lattice, Distribute_Initial_Particle.f90:204-249. */
+void orc_init_tgv(orc_ctx* c);
+/* Overwrite the fluid state (global order, n fluid particles). */
+int32_t orc_set_fluid_state(orc_ctx* c, int32_t n, const double* x, const double* y,
+                            const double* u, const double* v, const double* rho,
+                            const double* mass, const int32_t* type);
+
+/* Pieces of one time step. */
+void orc_period_boundary(orc_ctx* c);                       /* Distribute_Period_Boundary_Particle */
+void orc_devide_block(orc_ctx* c, int32_t i_time_step);     /* Devide_block_and_send_tasks (1 rank) */
+void orc_single_step_compute(orc_ctx* c, int32_t i_time_step);
+void orc_time_step(orc_ctx* c, int32_t i_time_step);        /* one full step of time_iteration */
+void orc_reinitialize_density(orc_ctx* c);                  /* Reinitialize_particle_density */
+void orc_shift(orc_ctx* c);                                 /* Shift_particle_position */
+void orc_refresh(orc_ctx* c);                               /* Refresh_domain (1 rank) */
+
+/* Sampling_subroutine.f90:340-461. out[0]=KE, [1]=max|v|, [2..4]=u Linf,L1,L2,
+ * [5..7]=v, [8..10]=p/(rho0 U^2), [11..13]=|v|. */
+void orc_sample(const orc_ctx* c, double* out);
+double orc_initial_ke(const orc_ctx* c);
+
+/* Sizes. */
+int32_t orc_n_fluid(const orc_ctx* c);
+int32_t orc_n_global(const orc_ctx* c);      /* fluid + periodic ghosts */
+int32_t orc_n_local(const orc_ctx* c);       /* subdomain particles */
+int64_t orc_pair_number(const orc_ctx* c);
+
+/* Named array access: returns a pointer into the context (valid until the next call
+ * that reallocates), and the element count in *n. Names are the reference's. */
+const double* orc_get_f64(const orc_ctx* c, const char* name, int64_t* n);
+const int32_t* orc_get_i32(const orc_ctx* c, const char* name, int64_t* n);
+double* orc_mut_f64(orc_ctx* c, const char* name, int64_t* n);
+int32_t* orc_mut_i32(orc_ctx* c, const char* name, int64_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,731 @@
+// sph2d_capi.hip -- C-ABI (include/sph2d.h): context, host driver of the step
+// (the restated time_iteration.f90), uploads/downloads, diagnostics.
+#include "sph2d_internal.h"
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace sph2d;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_OR_FAIL(x)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(SPH2D_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+constexpr double kPiModule = 3.14159265358979324;  // information_module.f90:29
+
+}  // namespace
+
+struct sph2d_ctx {
+  sph2d_params prm{};
+  DevParams P{};
+  Buffers B{};
+  hipStream_t stream = nullptr;
+  int cap_fluid = 0, cap_ghost = 0, cap_global = 0, cap_local = 0, n_cells = 0;
+  int n_fluid = 0;
+  double ke0 = 0.0;
+  bool have_state = false;
+  bool debug = false;
+  bool evaluated = false;
+  bool mls_pending = false;
+  double* d_out = nullptr;          // 16 doubles for reductions
+  std::vector<void*> allocs;
+
+  template <class T>
+  int alloc(T** p, size_t n) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) return fail(SPH2D_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemset(q, 0, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) return fail(SPH2D_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return SPH2D_OK;
+  }
+};
+
+namespace {
+
+// host-side kernel value (compute_kernel.f90:123-129) for W(dx, h) of the shift (:56-68)
+double host_w(double r, double h) {
+  const float c4 = 1.0f - 10.0f * expf(-9.0f);
+  const double factor = 1.0 / ((double)3.14159265358f * (h * h) * (double)c4);
+  const double q = r / h;
+  if (r <= 3.0 * h) return (std::exp(-(q * q)) - (double)expf(-9.0f)) * factor;
+  return 0.0;
+}
+
+void make_devparams(const sph2d_params& p, DevParams& P) {
+  P = DevParams{};
+  P.box_x = p.box_x; P.box_y = p.box_y;
+  P.rho0 = p.rho0; P.c0sq = p.c0sq; P.sqrt_c0sq = std::sqrt(p.c0sq);
+  P.eos_b = p.c0sq * p.rho0 / 7.0;
+  P.nu = p.nu; P.U = p.U;
+  P.dx = p.dx; P.h = p.h; P.dt = p.dt;
+  P.three_h = 3 * p.h;
+  const float c4 = 1.0f - 10.0f * expf(-9.0f);
+  P.kfactor = 1.0 / ((double)3.14159265358f * (p.h * p.h) * (double)c4);
+  P.e9 = (double)expf(-9.0f);
+  P.r_min = (double)1.0E-7f;
+  P.support_volume = kPiModule * ((3 * p.h) * (3 * p.h));
+  P.fra_limit = (double)0.65f;
+  P.chain_d = (3 + 1) * p.h;
+  P.chain_ox = -(5 * P.chain_d);
+  P.chain_oy = -(5 * P.chain_d);
+  P.chain_nx = (int)(p.box_x / P.chain_d + 2 * 5);
+  P.chain_ny = (int)(p.box_y / P.chain_d + 2 * 5);
+  P.alpha = p.alpha; P.beta = p.beta; P.fai = p.fai;
+  P.delta = p.delta; P.delta_alpha = p.delta_alpha; P.epsilon_xsph = p.epsilon_xsph;
+  P.shift_beta = p.shift_beta; P.shift_R = p.shift_R; P.max_shift = p.max_shift;
+  P.w_dx = host_w(p.dx, p.h);
+  P.h2 = p.h * p.h;
+  P.sps_dl = (double)(sqrtf(2.0f) / 2.0f) * p.dx;
+  P.sps_c012 = (double)0.12f;
+  P.sps_c23 = (double)(2.0f / 3.0f);
+  {
+    volatile float a = 2.0f / 3.0f, b = 0.0066f;
+    P.sps_c23b = (double)(float)(a * b);
+  }
+  P.mls_alpha = p.mls_alpha;
+  P.surf_lim = (double)1.414f * p.h;
+  P.solver = p.solver;
+  P.mls_order = p.mls_order;
+  P.write_debug = 0;
+}
+
+int check_params(const sph2d_params* p) {
+  if (!p) return fail(SPH2D_ERR_ARGUMENT, "params is NULL");
+  if (p->struct_size != (int32_t)sizeof(sph2d_params))
+    return fail(SPH2D_ERR_ARGUMENT, "params.struct_size mismatch (header/library version)");
+  if (!(p->h > 0) || !(p->dx > 0) || !(p->box_x > 0) || !(p->box_y > 0) || !(p->dt > 0))
+    return fail(SPH2D_ERR_ARGUMENT, "h, dx, box and dt must be positive");
+  if (p->integrator != SPH2D_INTEGRATOR_PC && p->integrator != SPH2D_INTEGRATOR_RK)
+    return fail(SPH2D_ERR_UNSUPPORTED, "integrator must be 2 (predictor-corrector) or 3 (RK-variant)");
+  if (p->solver != SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && p->solver != SPH2D_SOLVER_LINEAR_RIEMANN)
+    return fail(SPH2D_ERR_UNSUPPORTED, "solver variant not built");
+  if (p->precision != SPH2D_FP64)
+    return fail(SPH2D_ERR_UNSUPPORTED, "only the FP64 form is built in this library");
+  if (p->rate_reuse != SPH2D_RATE_REUSE_REFERENCE)
+    return fail(SPH2D_ERR_UNSUPPORTED, "only the reference's rate re-use is built");
+  if (p->mls_on && p->mls_order != 6)
+    return fail(SPH2D_ERR_UNSUPPORTED, "only MLS order 6 (the reference default) is built");
+  if (!p->periodic) return fail(SPH2D_ERR_UNSUPPORTED, "only the periodic box is built");
+  if (p->mls_every <= 0 || p->sample_every <= 0)
+    return fail(SPH2D_ERR_ARGUMENT, "mls_every and sample_every must be positive");
+  return SPH2D_OK;
+}
+
+int grid_n(int n) { return (n + kBlock - 1) / kBlock; }
+
+int read_counters(sph2d_ctx* c, int* host) {
+  HIP_OR_FAIL(hipMemcpyAsync(host, c->B.counters, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  if (host[C_ERR] & E_GHOST_CAP) return fail(SPH2D_ERR_CAPACITY, "periodic ghost capacity exceeded");
+  return SPH2D_OK;
+}
+
+int compute_ke0(sph2d_ctx* c) {
+  launch_sample(c->stream, c->P, c->B, c->cap_fluid, c->d_out);
+  double out[16];
+  HIP_OR_FAIL(hipMemcpyAsync(out, c->d_out, sizeof(double) * 14, hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  c->ke0 = out[0];
+  return SPH2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sph2d_last_error(void) { return g_err.c_str(); }
+
+int sph2d_default_params(sph2d_params* p) {
+  if (!p) return fail(SPH2D_ERR_ARGUMENT, "params is NULL");
+  std::memset(p, 0, sizeof(*p));
+  p->struct_size = sizeof(sph2d_params);
+  p->nx = 100; p->ny = 100;
+  p->box_x = 1.0; p->box_y = 1.0;
+  p->rho0 = 1000.0;
+  p->c0sq = 100.0;
+  p->nu = (double)1.0E-2f;
+  p->U = 1.0;
+  p->dx = 1.0 / 100;
+  p->h = 1.20 * p->dx;
+  p->dt = 0.0001;
+  p->integrator = SPH2D_INTEGRATOR_RK;
+  p->solver = SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN;
+  p->precision = SPH2D_FP64;
+  p->rate_reuse = SPH2D_RATE_REUSE_REFERENCE;
+  p->shift_on = 1;
+  p->mls_on = 1;
+  p->mls_every = 20;
+  p->mls_order = 6;
+  p->sample_every = 100;
+  p->periodic = 1;
+  p->alpha = 0.0; p->beta = 2.0; p->fai = 0.1;
+  p->delta = 0.1; p->delta_alpha = 0.02;
+  p->epsilon_xsph = 0.0;
+  p->shift_beta = 0.5; p->shift_R = 0.2;
+  p->max_shift = 0.25 * p->dx;
+  p->mls_alpha = (double)0.0001f;
+  return SPH2D_OK;
+}
+
+int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx** out) {
+  if (!out) return fail(SPH2D_ERR_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (n_fluid_capacity <= 0) return fail(SPH2D_ERR_ARGUMENT, "capacity must be positive");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(SPH2D_ERR_HIP, "no HIP device: the sph2d library runs on the GPU only");
+  sph2d_ctx* c = new sph2d_ctx();
+  c->prm = *p;
+  make_devparams(*p, c->P);
+  // periodic ghosts: 2*(m+1) layers along every edge, m = 3h/dx (plus corners), x3 margin
+  const int m = (int)std::ceil(3 * p->h / p->dx) + 1;
+  const long ghost_bound = 2L * m * (long)(p->nx + p->ny + 4 * m);
+  c->cap_fluid = n_fluid_capacity;
+  c->cap_ghost = (int)std::max(16384L, 3 * ghost_bound);
+  c->cap_global = c->cap_fluid + c->cap_ghost;
+  c->cap_local = c->cap_global;
+  c->n_cells = c->P.chain_nx * c->P.chain_ny;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete c; return fail(SPH2D_ERR_HIP, "hipStreamCreate failed"); }
+  Buffers& B = c->B;
+  const size_t G = c->cap_global, L = c->cap_local, F = c->cap_fluid, H = c->cap_ghost;
+  rc = 0;
+  rc |= c->alloc(&B.g_pos, G); rc |= c->alloc(&B.g_vel, G);
+  rc |= c->alloc(&B.g_rho, G); rc |= c->alloc(&B.g_mass, G); rc |= c->alloc(&B.g_p, G);
+  rc |= c->alloc(&B.g_c, G); rc |= c->alloc(&B.g_e, G);
+  rc |= c->alloc(&B.g_type, G); rc |= c->alloc(&B.g_key, G); rc |= c->alloc(&B.g_slot, G);
+  rc |= c->alloc(&B.ana, 4 * F);
+  rc |= c->alloc(&B.cnt_x, F); rc |= c->alloc(&B.cnt_yf, F); rc |= c->alloc(&B.off_x, F);
+  rc |= c->alloc(&B.off_yf, F); rc |= c->alloc(&B.cnt_yx, H); rc |= c->alloc(&B.off_yx, H);
+  rc |= c->alloc(&B.src_x, H);
+  rc |= c->alloc(&B.cell_count, c->n_cells); rc |= c->alloc(&B.cell_start, c->n_cells + 1);
+  rc |= c->alloc(&B.cell_members, G);
+  rc |= c->alloc(&B.name, L); rc |= c->alloc(&B.l_type, L); rc |= c->alloc(&B.l_cell, L);
+  rc |= c->alloc(&B.l_pos, L); rc |= c->alloc(&B.l_vel, L); rc |= c->alloc(&B.l_rv, L);
+  rc |= c->alloc(&B.l_pc, L); rc |= c->alloc(&B.l_pos_bin, L);
+  rc |= c->alloc(&B.l_mass, L); rc |= c->alloc(&B.l_e, L);
+  rc |= c->alloc(&B.fra_v, L); rc |= c->alloc(&B.sdw, L); rc |= c->alloc(&B.grad, L);
+  rc |= c->alloc(&B.nrm, L); rc |= c->alloc(&B.shift, L); rc |= c->alloc(&B.avg, L);
+  rc |= c->alloc(&B.tau, 3 * L); rc |= c->alloc(&B.surf, L); rc |= c->alloc(&B.near_, L);
+  rc |= c->alloc(&B.drhodt, L); rc |= c->alloc(&B.acc, L);
+  rc |= c->alloc(&B.b_rho, L); rc |= c->alloc(&B.t_drho, L); rc |= c->alloc(&B.k_drho, 3 * L);
+  rc |= c->alloc(&B.b_pos, L); rc |= c->alloc(&B.b_vel, L); rc |= c->alloc(&B.t_vel, L);
+  rc |= c->alloc(&B.t_acc, L); rc |= c->alloc(&B.k_vel, 3 * L); rc |= c->alloc(&B.k_acc, 3 * L);
+  rc |= c->alloc(&B.mls_rho, L);
+  rc |= c->alloc(&B.scan_blocks, (std::max<size_t>(G, (size_t)c->n_cells + 1) + 1023) / 1024 + 16);
+  rc |= c->alloc(&B.counters, C_COUNT);
+  rc |= c->alloc(&B.red, (size_t)grid_n(c->cap_fluid) * 14 + 16);
+  rc |= c->alloc(&B.pair_count, L); rc |= c->alloc(&B.pair_off, L);
+  rc |= c->alloc(&c->d_out, 16);
+  B.dbg = nullptr; B.pair_lo = nullptr; B.pair_hi = nullptr;
+  B.kstride = (int)L;
+  if (rc) {
+    std::string msg = g_err;
+    sph2d_ctx_destroy(c);
+    return fail(SPH2D_ERR_HIP, "allocation failed: " + msg);
+  }
+  *out = c;
+  return SPH2D_OK;
+}
+
+int sph2d_ctx_destroy(sph2d_ctx* c) {
+  if (!c) return SPH2D_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return SPH2D_OK;
+}
+
+static int set_nfluid(sph2d_ctx* c, int n) {
+  int zeros[C_COUNT] = {0};
+  zeros[C_NFLUID] = n;
+  HIP_OR_FAIL(hipMemcpyAsync(c->B.counters, zeros, sizeof(zeros), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  c->n_fluid = n;
+  return SPH2D_OK;
+}
+
+int sph2d_init_tgv(sph2d_ctx* c) {
+  if (!c) return fail(SPH2D_ERR_ARGUMENT, "ctx is NULL");
+  const int n = c->prm.nx * c->prm.ny;
+  if (n > c->cap_fluid) return fail(SPH2D_ERR_CAPACITY, "nx*ny exceeds the context capacity");
+  int rc = set_nfluid(c, n);
+  if (rc) return rc;
+  launch_init_tgv(c->stream, c->P, c->B, c->prm.nx, c->prm.ny);
+  HIP_OR_FAIL(hipGetLastError());
+  rc = compute_ke0(c);
+  if (rc) return rc;
+  c->have_state = true;
+  c->evaluated = false;
+  return SPH2D_OK;
+}
+
+int sph2d_upload_state(sph2d_ctx* c, int32_t n, const double* pos, const double* vel,
+                       const double* rho, const double* mass, const double* energy,
+                       const double* h, const int32_t* type, int32_t ld) {
+  if (!c || !pos || !vel || !rho || !mass) return fail(SPH2D_ERR_ARGUMENT, "NULL argument");
+  if (n <= 0 || n > c->cap_fluid) return fail(SPH2D_ERR_CAPACITY, "n outside (0, capacity]");
+  if (ld < n) return fail(SPH2D_ERR_ARGUMENT, "ld < n");
+  std::vector<double2> p2(n), v2(n);
+  std::vector<double> e(n), pr(n, 0.0), cs(n, std::sqrt(c->prm.c0sq));
+  std::vector<int> t(n, 2);
+  for (int i = 0; i < n; ++i) {
+    p2[i] = make_double2(pos[i], pos[ld + i]);
+    v2[i] = make_double2(vel[i], vel[ld + i]);
+    e[i] = energy ? energy[i] : c->prm.c0sq / (7.0 - 1.0);
+    if (h && h[i] != c->prm.h)
+      return fail(SPH2D_ERR_UNSUPPORTED, "per-particle smoothing lengths must all equal params.h");
+    if (type) {
+      t[i] = type[i];
+      if (t[i] != 2 && t[i] != 120)
+        return fail(SPH2D_ERR_UNSUPPORTED, "only fluid (2) and ill (120) particle types are built");
+    }
+    if (!std::isfinite(pos[i]) || !std::isfinite(pos[ld + i]) || !std::isfinite(rho[i]) || !(rho[i] > 0))
+      return fail(SPH2D_ERR_NONFINITE, "non-finite or non-positive state at particle " + std::to_string(i));
+  }
+  Buffers& B = c->B;
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_pos, p2.data(), n * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_vel, v2.data(), n * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_rho, rho, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_mass, mass, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_e, e.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_p, pr.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_c, cs.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_type, t.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  int rc = set_nfluid(c, n);
+  if (rc) return rc;
+  rc = compute_ke0(c);
+  if (rc) return rc;
+  c->have_state = true;
+  c->evaluated = false;
+  return SPH2D_OK;
+}
+
+int sph2d_download_state(sph2d_ctx* c, int32_t n, double* pos, double* vel, double* rho,
+                         double* press, double* cs, double* mass, int32_t* type, int32_t ld) {
+  if (!c) return fail(SPH2D_ERR_ARGUMENT, "ctx is NULL");
+  if (n != c->n_fluid) return fail(SPH2D_ERR_ARGUMENT, "n must equal the number of fluid particles");
+  if (ld < n) return fail(SPH2D_ERR_ARGUMENT, "ld < n");
+  Buffers& B = c->B;
+  std::vector<double2> tmp(n);
+  if (pos) {
+    HIP_OR_FAIL(hipMemcpyAsync(tmp.data(), B.g_pos, n * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) { pos[i] = tmp[i].x; pos[ld + i] = tmp[i].y; }
+  }
+  if (vel) {
+    HIP_OR_FAIL(hipMemcpyAsync(tmp.data(), B.g_vel, n * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) { vel[i] = tmp[i].x; vel[ld + i] = tmp[i].y; }
+  }
+  if (rho) HIP_OR_FAIL(hipMemcpyAsync(rho, B.g_rho, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (press) HIP_OR_FAIL(hipMemcpyAsync(press, B.g_p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (cs) HIP_OR_FAIL(hipMemcpyAsync(cs, B.g_c, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (mass) HIP_OR_FAIL(hipMemcpyAsync(mass, B.g_mass, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (type) HIP_OR_FAIL(hipMemcpyAsync(type, B.g_type, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  return SPH2D_OK;
+}
+
+// ---- scalar and array forms ---------------------------------------------------
+static hipStream_t scalar_stream() {
+  static hipStream_t s = nullptr;
+  if (!s) (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  return s;
+}
+
+int sph2d_compute_kernel_n(int32_t n, const double* r, const double* dx, const double* h,
+                           double* w, double* dwdx) {
+  if (n <= 0 || !r || !dx || !h || !w || !dwdx) return fail(SPH2D_ERR_ARGUMENT, "bad argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(SPH2D_ERR_HIP, "no HIP device: the sph2d library runs on the GPU only");
+  hipStream_t s = scalar_stream();
+  double *dr, *ddx, *dh, *dw, *ddw;
+  HIP_OR_FAIL(hipMalloc(&dr, n * sizeof(double)));
+  HIP_OR_FAIL(hipMalloc(&ddx, 2 * n * sizeof(double)));
+  HIP_OR_FAIL(hipMalloc(&dh, n * sizeof(double)));
+  HIP_OR_FAIL(hipMalloc(&dw, n * sizeof(double)));
+  HIP_OR_FAIL(hipMalloc(&ddw, 2 * n * sizeof(double)));
+  HIP_OR_FAIL(hipMemcpyAsync(dr, r, n * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_OR_FAIL(hipMemcpyAsync(ddx, dx, 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_OR_FAIL(hipMemcpyAsync(dh, h, n * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_kernel_n(s, n, dr, ddx, dh, dw, ddw, (double)expf(-9.0f));
+  HIP_OR_FAIL(hipGetLastError());
+  HIP_OR_FAIL(hipMemcpyAsync(w, dw, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_OR_FAIL(hipMemcpyAsync(dwdx, ddw, 2 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_OR_FAIL(hipStreamSynchronize(s));
+  (void)hipFree(dr); (void)hipFree(ddx); (void)hipFree(dh); (void)hipFree(dw); (void)hipFree(ddw);
+  return SPH2D_OK;
+}
+
+int sph2d_compute_kernel(const int32_t* dim, const double* distance, const double* position_difference,
+                         const double* average_smooth_length, double* w, double* dwdx) {
+  if (!dim || *dim != 2) return fail(SPH2D_ERR_UNSUPPORTED, "only dim = 2 is built");
+  return sph2d_compute_kernel_n(1, distance, position_difference, average_smooth_length, w, dwdx);
+}
+
+int sph2d_status_equation_n(const sph2d_params* p, int32_t n, const double* rho, const double* energy,
+                            const int32_t* type, double* press, double* cs) {
+  (void)energy;
+  if (n <= 0 || !rho || !type || !press || !cs) return fail(SPH2D_ERR_ARGUMENT, "bad argument");
+  sph2d_params d;
+  if (!p) { sph2d_default_params(&d); p = &d; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(SPH2D_ERR_HIP, "no HIP device: the sph2d library runs on the GPU only");
+  hipStream_t s = scalar_stream();
+  double *dr, *dp, *dc;
+  int* dt;
+  HIP_OR_FAIL(hipMalloc(&dr, n * sizeof(double)));
+  HIP_OR_FAIL(hipMalloc(&dp, n * sizeof(double)));
+  HIP_OR_FAIL(hipMalloc(&dc, n * sizeof(double)));
+  HIP_OR_FAIL(hipMalloc(&dt, n * sizeof(int)));
+  HIP_OR_FAIL(hipMemcpyAsync(dr, rho, n * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_OR_FAIL(hipMemcpyAsync(dt, type, n * sizeof(int), hipMemcpyHostToDevice, s));
+  HIP_OR_FAIL(hipMemcpyAsync(dp, press, n * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_OR_FAIL(hipMemcpyAsync(dc, cs, n * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_eos_n(s, n, p->rho0, p->c0sq, dr, dt, dp, dc);
+  HIP_OR_FAIL(hipGetLastError());
+  HIP_OR_FAIL(hipMemcpyAsync(press, dp, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_OR_FAIL(hipMemcpyAsync(cs, dc, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_OR_FAIL(hipStreamSynchronize(s));
+  (void)hipFree(dr); (void)hipFree(dp); (void)hipFree(dc); (void)hipFree(dt);
+  return SPH2D_OK;
+}
+
+int sph2d_status_equation(const double* rho, const double* energy, const int32_t* type,
+                          double* press, double* cs) {
+  return sph2d_status_equation_n(nullptr, 1, rho, energy, type, press, cs);
+}
+
+// ---- step pieces --------------------------------------------------------------
+static int need_state(sph2d_ctx* c) {
+  if (!c) return fail(SPH2D_ERR_ARGUMENT, "ctx is NULL");
+  if (!c->have_state) return fail(SPH2D_ERR_STATE, "no particle state: call sph2d_init_tgv or sph2d_upload_state");
+  return SPH2D_OK;
+}
+
+int sph2d_distribute_period_boundary(sph2d_ctx* c) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  launch_period_boundary(c->stream, c->P, c->B, c->cap_fluid, c->cap_ghost);
+  HIP_OR_FAIL(hipGetLastError());
+  return SPH2D_OK;
+}
+
+int sph2d_devide_block(sph2d_ctx* c, int32_t it) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  const bool ana = (it % c->prm.sample_every) == 0;
+  const double t = it * c->prm.dt / (c->prm.box_x / c->prm.U);   // Nondimensionalized_real_time
+  launch_fill_i32(c->stream, c->B.counters + C_NILL, 1, 0);
+  launch_fill_i32(c->stream, c->B.counters + C_MAXCELL, 1, 0);
+  launch_devide_block(c->stream, c->P, c->B, c->cap_global, c->n_cells, ana, t);
+  HIP_OR_FAIL(hipGetLastError());
+  return SPH2D_OK;
+}
+
+static void do_eval(sph2d_ctx* c, bool with_eos) {
+  if (with_eos) launch_stage(c->stream, c->P, c->B, c->cap_local, ST_EOS);
+  c->P.write_debug = c->debug ? 1 : 0;
+  launch_eval(c->stream, c->P, c->B, c->cap_local);
+  c->evaluated = true;
+}
+
+int sph2d_single_step_compute(sph2d_ctx* c, int32_t it) {
+  (void)it;
+  int rc = need_state(c);
+  if (rc) return rc;
+  do_eval(c, true);
+  HIP_OR_FAIL(hipGetLastError());
+  return SPH2D_OK;
+}
+
+int sph2d_reinitialize_particle_density(sph2d_ctx* c, int32_t it) {
+  (void)it;
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (!c->evaluated) return fail(SPH2D_ERR_STATE, "no force evaluation in this step yet");
+  launch_fill_i32(c->stream, c->B.counters + C_MLSFB, 1, 0);
+  launch_mls(c->stream, c->P, c->B, c->cap_local);
+  launch_shift_apply_mode(c->stream, c->P, c->B, c->cap_local, 2);   // commit rho
+  HIP_OR_FAIL(hipGetLastError());
+  return SPH2D_OK;
+}
+
+int sph2d_shift_particle_position(sph2d_ctx* c, int32_t it) {
+  (void)it;
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (!c->evaluated) return fail(SPH2D_ERR_STATE, "no force evaluation in this step yet");
+  launch_shift_apply_mode(c->stream, c->P, c->B, c->cap_local, 0);
+  HIP_OR_FAIL(hipGetLastError());
+  return SPH2D_OK;
+}
+
+int sph2d_refresh_domain(sph2d_ctx* c, int32_t it) {
+  (void)it;
+  int rc = need_state(c);
+  if (rc) return rc;
+  launch_refresh(c->stream, c->P, c->B, c->cap_local);
+  HIP_OR_FAIL(hipGetLastError());
+  return SPH2D_OK;
+}
+
+// time_iteration.f90 PC (:509-657) / RK-variant (:779-1049), then :1064-1078.
+static int enqueue_step(sph2d_ctx* c, int it) {
+  int rc = sph2d_distribute_period_boundary(c);
+  if (rc) return rc;
+  rc = sph2d_devide_block(c, it);
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  if (c->prm.integrator == SPH2D_INTEGRATOR_PC) {
+    if (it == 1) do_eval(c, false);
+    launch_stage(s, c->P, c->B, c->cap_local, ST_PC_PREDICT);
+    do_eval(c, false);
+    launch_stage(s, c->P, c->B, c->cap_local, ST_PC_CORRECT);
+  } else {
+    if (it == 1) do_eval(c, false);
+    launch_stage(s, c->P, c->B, c->cap_local, ST_RK_HALF0);
+    do_eval(c, false);
+    launch_stage(s, c->P, c->B, c->cap_local, ST_RK_HALF1);
+    do_eval(c, false);
+    launch_stage(s, c->P, c->B, c->cap_local, ST_RK_FULL2);
+    do_eval(c, false);
+    launch_stage(s, c->P, c->B, c->cap_local, ST_RK_FINAL);
+  }
+  const bool mls = c->prm.mls_on && (it % c->prm.mls_every) == 0;
+  if (mls) {
+    launch_fill_i32(s, c->B.counters + C_MLSFB, 1, 0);
+    launch_mls(s, c->P, c->B, c->cap_local);
+  }
+  if (c->prm.shift_on) launch_shift_apply_mode(s, c->P, c->B, c->cap_local, mls ? 1 : 0);
+  else if (mls) launch_shift_apply_mode(s, c->P, c->B, c->cap_local, 2);
+  launch_refresh(s, c->P, c->B, c->cap_local);
+  HIP_OR_FAIL(hipGetLastError());
+  return SPH2D_OK;
+}
+
+int sph2d_time_step(sph2d_ctx* c, int32_t it) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (it < 1) return fail(SPH2D_ERR_ARGUMENT, "i_time_step starts at 1");
+  return enqueue_step(c, it);
+}
+
+int sph2d_run(sph2d_ctx* c, int32_t first, int32_t nsteps) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (first < 1 || nsteps < 0) return fail(SPH2D_ERR_ARGUMENT, "bad step range");
+  for (int k = 0; k < nsteps; ++k) {
+    rc = enqueue_step(c, first + k);
+    if (rc) return rc;
+  }
+  return SPH2D_OK;
+}
+
+int sph2d_sample(sph2d_ctx* c, double* out14) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (!out14) return fail(SPH2D_ERR_ARGUMENT, "out is NULL");
+  launch_sample(c->stream, c->P, c->B, c->cap_fluid, c->d_out);
+  HIP_OR_FAIL(hipGetLastError());
+  double v[16];
+  HIP_OR_FAIL(hipMemcpyAsync(v, c->d_out, sizeof(double) * 14, hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  int cnt[C_COUNT];
+  rc = read_counters(c, cnt);
+  if (rc) return rc;
+  const double N = (double)c->n_fluid;
+  out14[0] = v[0] / c->ke0;
+  out14[1] = v[1] / c->prm.U;
+  for (int a = 0; a < 4; ++a) {
+    out14[2 + 3 * a] = v[2 + 3 * a];
+    out14[3 + 3 * a] = v[3 + 3 * a] / N;
+    out14[4 + 3 * a] = std::sqrt(v[4 + 3 * a] / N);
+  }
+  return SPH2D_OK;
+}
+
+// ---- inspection -------------------------------------------------------------------
+int sph2d_set_debug(sph2d_ctx* c, int32_t on) {
+  if (!c) return fail(SPH2D_ERR_ARGUMENT, "ctx is NULL");
+  if (on && !c->B.dbg) {
+    int rc = c->alloc(&c->B.dbg, (size_t)10 * c->cap_local);
+    if (rc) return rc;
+  }
+  c->debug = on != 0;
+  return SPH2D_OK;
+}
+
+int sph2d_download_local(sph2d_ctx* c, const char* name, void* out, int32_t n) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (!name || !out) return fail(SPH2D_ERR_ARGUMENT, "NULL argument");
+  int cnt[C_COUNT];
+  rc = read_counters(c, cnt);
+  if (rc) return rc;
+  const int nl = cnt[C_NLOCAL];
+  if (n != nl) return fail(SPH2D_ERR_ARGUMENT, "n must equal the local particle count " + std::to_string(nl));
+  Buffers& B = c->B;
+  const std::string s(name);
+  auto d2 = [&](double2* src, int comp) -> int {
+    std::vector<double2> t(nl);
+    HIP_OR_FAIL(hipMemcpyAsync(t.data(), src, nl * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+    double* o = static_cast<double*>(out);
+    for (int i = 0; i < nl; ++i) o[i] = comp ? t[i].y : t[i].x;
+    return SPH2D_OK;
+  };
+  auto d1 = [&](const double* src, int stride, int comp) -> int {
+    std::vector<double> t((size_t)nl * stride);
+    HIP_OR_FAIL(hipMemcpyAsync(t.data(), src, (size_t)nl * stride * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+    double* o = static_cast<double*>(out);
+    for (int i = 0; i < nl; ++i) o[i] = t[(size_t)i * stride + comp];
+    return SPH2D_OK;
+  };
+  auto i1 = [&](const int* src) -> int {
+    HIP_OR_FAIL(hipMemcpyAsync(out, src, nl * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+    return SPH2D_OK;
+  };
+  if (s == "x") return d2(B.l_pos, 0);
+  if (s == "y") return d2(B.l_pos, 1);
+  if (s == "u") return d2(B.l_vel, 0);
+  if (s == "v") return d2(B.l_vel, 1);
+  if (s == "rho") return d2(B.l_rv, 0);
+  if (s == "volume") return d2(B.l_rv, 1);
+  if (s == "p") return d2(B.l_pc, 0);
+  if (s == "c") return d2(B.l_pc, 1);
+  if (s == "mass") return d1(B.l_mass, 1, 0);
+  if (s == "fra_v") return d1(B.fra_v, 1, 0);
+  if (s == "sum_dwdx_x") return d2(B.sdw, 0);
+  if (s == "sum_dwdx_y") return d2(B.sdw, 1);
+  if (s == "normal_x") return d2(B.nrm, 0);
+  if (s == "normal_y") return d2(B.nrm, 1);
+  if (s == "tau_xx") return d1(B.tau, 3, 0);
+  if (s == "tau_xy") return d1(B.tau, 3, 1);
+  if (s == "tau_yy") return d1(B.tau, 3, 2);
+  if (s == "grad_rho_x") return d2(B.grad, 0);
+  if (s == "grad_rho_y") return d2(B.grad, 1);
+  if (s == "drhodt") return d1(B.drhodt, 1, 0);
+  if (s == "ax") return d2(B.acc, 0);
+  if (s == "ay") return d2(B.acc, 1);
+  if (s == "shift_x") return d2(B.shift, 0);
+  if (s == "shift_y") return d2(B.shift, 1);
+  if (s == "type") return i1(B.l_type);
+  if (s == "name") return i1(B.name);
+  if (s == "surface") return i1(B.surf);
+  if (s == "near_surface") return i1(B.near_);
+  static const char* dbg_names[10] = {"internal_x", "internal_y", "visc_x", "visc_y", "art_internal_x",
+                                      "art_internal_y", "sps_x", "sps_y", "lap_x", "lap_y"};
+  for (int k = 0; k < 10; ++k)
+    if (s == dbg_names[k]) {
+      if (!B.dbg || !c->debug) return fail(SPH2D_ERR_STATE, "acceleration components need sph2d_set_debug(ctx,1) before the evaluation");
+      return d1(B.dbg, 10, k);
+    }
+  return fail(SPH2D_ERR_ARGUMENT, "unknown array name '" + s + "'");
+}
+
+int sph2d_download_pairs(sph2d_ctx* c, int64_t capacity, int32_t* lo, int32_t* hi, int64_t* n_pairs) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (!n_pairs) return fail(SPH2D_ERR_ARGUMENT, "n_pairs is NULL");
+  launch_count_pairs(c->stream, c->P, c->B, c->cap_local);
+  int cnt[C_COUNT];
+  rc = read_counters(c, cnt);
+  if (rc) return rc;
+  const int64_t np = (int64_t)(unsigned)cnt[C_NPAIR_LO];
+  *n_pairs = np;
+  if (!lo || !hi || capacity <= 0) return SPH2D_OK;
+  if (capacity < np) return fail(SPH2D_ERR_CAPACITY, "pair capacity too small");
+  Buffers& B = c->B;
+  int *dlo = nullptr, *dhi = nullptr;
+  HIP_OR_FAIL(hipMalloc(&dlo, std::max<int64_t>(np, 1) * sizeof(int)));
+  HIP_OR_FAIL(hipMalloc(&dhi, std::max<int64_t>(np, 1) * sizeof(int)));
+  B.pair_lo = dlo; B.pair_hi = dhi;
+  launch_pairs(c->stream, c->P, B, c->cap_local, np, nullptr);
+  std::vector<int> a(np), b(np);
+  HIP_OR_FAIL(hipMemcpyAsync(a.data(), dlo, np * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(b.data(), dhi, np * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  (void)hipFree(dlo); (void)hipFree(dhi);
+  B.pair_lo = nullptr; B.pair_hi = nullptr;
+  std::vector<int64_t> key(np);
+  for (int64_t k = 0; k < np; ++k) key[k] = ((int64_t)a[k] << 32) | (uint32_t)b[k];
+  std::sort(key.begin(), key.end());
+  for (int64_t k = 0; k < np; ++k) { lo[k] = (int32_t)(key[k] >> 32); hi[k] = (int32_t)(key[k] & 0xffffffff); }
+  return SPH2D_OK;
+}
+
+int sph2d_get_diagnostics(sph2d_ctx* c, sph2d_diag* d) {
+  if (!c || !d) return fail(SPH2D_ERR_ARGUMENT, "NULL argument");
+  int cnt[C_COUNT];
+  int rc = read_counters(c, cnt);
+  if (rc) return rc;
+  std::memset(d, 0, sizeof(*d));
+  d->n_fluid = cnt[C_NFLUID];
+  d->n_ghost = cnt[C_NGHOST];
+  d->n_local = cnt[C_NLOCAL];
+  d->n_ill = cnt[C_NILL];
+  d->n_surface = cnt[C_NSURF];
+  d->n_singular = cnt[C_NSING];
+  d->n_mls_fallback = cnt[C_MLSFB];
+  d->max_cell_count = cnt[C_MAXCELL];
+  d->rebin_needed = cnt[C_REBIN];
+  d->error_flags = cnt[C_ERR];
+  d->n_pairs = -1;
+  return SPH2D_OK;
+}
+
+int sph2d_time_pass(sph2d_ctx* c, int32_t pass, int32_t reps, double* avg_ms) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (!avg_ms || reps <= 0 || pass < 0 || pass > 2) return fail(SPH2D_ERR_ARGUMENT, "bad argument");
+  if (!c->evaluated) return fail(SPH2D_ERR_STATE, "no force evaluation yet");
+  hipEvent_t e0, e1;
+  HIP_OR_FAIL(hipEventCreate(&e0));
+  HIP_OR_FAIL(hipEventCreate(&e1));
+  c->P.write_debug = 0;
+  HIP_OR_FAIL(hipEventRecord(e0, c->stream));
+  for (int k = 0; k < reps; ++k) launch_eval_pass(c->stream, c->P, c->B, c->cap_local, pass);
+  HIP_OR_FAIL(hipEventRecord(e1, c->stream));
+  HIP_OR_FAIL(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_OR_FAIL(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_ms = (double)ms / reps;
+  return SPH2D_OK;
+}
+
+int sph2d_get_stream(sph2d_ctx* c, void** stream) {
+  if (!c || !stream) return fail(SPH2D_ERR_ARGUMENT, "NULL argument");
+  *stream = (void*)c->stream;
+  return SPH2D_OK;
+}
+
+}  // extern "C"

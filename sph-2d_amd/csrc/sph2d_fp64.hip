@@ -1,0 +1,1141 @@
+// sph2d_fp64.hip -- plain FP64 form of the WCSPH step (gfx950).
+//
+// Gather form: every particle sums over its own neighbours, so no pair list is
+// stored and no float atomics are used; each sum runs in a fixed order
+// (neighbour columns cx-1, cx, cx+1, rows ascending, particles ascending) and
+// the results are deterministic. Each pair term is evaluated in the reference's
+// pair orientation (a = lower local index, b = higher), so a term equals the
+// Fortran term up to the last bit of exp(); only the summation order differs.
+// Built with -ffp-contract=off (no FMA contraction), like the reference build.
+#include "sph2d_internal.h"
+#include "sph2d_device.h"
+#include <cstdio>
+
+namespace sph2d {
+
+static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
+
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { \
+  fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); } } while (0)
+
+// ---------------------------------------------------------------------------
+// utilities: fill, device-wide exclusive scan of int32 (3 launches, fixed grid)
+// ---------------------------------------------------------------------------
+__global__ void k_fill_i32(int* p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void launch_fill_i32(hipStream_t s, int* p, int n, int v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(kBlock), 0, s, p, n, v);
+}
+
+constexpr int kScanItems = 4;                    // items per thread
+constexpr int kScanTile = kBlock * kScanItems;   // 1024 items per block
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int& total) {
+  // Hillis-Steele over 256 threads in LDS.
+  const int t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    int add = t >= off ? lds[t - off] : 0;
+    __syncthreads();
+    lds[t] += add;
+    __syncthreads();
+  }
+  total = lds[kBlock - 1];
+  int incl = lds[t];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void k_scan_reduce(const int* __restrict__ in, int n, int* __restrict__ blocks) {
+  __shared__ int lds[kBlock];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) s += (base + k < n) ? in[base + k] : 0;
+  int total;
+  block_exclusive_scan(s, lds, total);
+  if (threadIdx.x == 0) blocks[blockIdx.x] = total;
+}
+
+__global__ void k_scan_top(int* blocks, int nb, int* total_out, int* total_out2) {
+  __shared__ int lds[kBlock];
+  int carry = 0;
+  for (int base = 0; base < nb; base += kBlock) {
+    int i = base + threadIdx.x;
+    int v = i < nb ? blocks[i] : 0;
+    int total;
+    int ex = block_exclusive_scan(v, lds, total);
+    if (i < nb) blocks[i] = ex + carry;
+    carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (total_out) *total_out = carry;
+    if (total_out2) *total_out2 = carry;
+  }
+}
+
+__global__ void k_scan_apply(const int* __restrict__ in, int n, const int* __restrict__ blocks,
+                             int* __restrict__ out) {
+  __shared__ int lds[kBlock];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) { v[k] = (base + k < n) ? in[base + k] : 0; s += v[k]; }
+  int total;
+  int ex = block_exclusive_scan(s, lds, total) + blocks[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) out[base + k] = ex;
+    ex += v[k];
+  }
+}
+
+// out[i] = sum_{k<i} in[k] for i < n; the grand total goes to *total (and *total2).
+static void exclusive_scan(hipStream_t s, const int* in, int n, int* out, int* blocks,
+                           int* total, int* total2 = nullptr) {
+  if (n <= 0) n = 1;
+  int nb = (n + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, in, n, blocks);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, s, blocks, nb, total, total2);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, s, in, n, blocks, out);
+}
+
+// ---------------------------------------------------------------------------
+// Distribute_Initial_Particle.f90:204-249 (TGV lattice)
+// ---------------------------------------------------------------------------
+__global__ void k_init_tgv(DevParams P, Buffers B, int nx, int ny, double pi, double expo,
+                           double vol0) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nx * ny) return;
+  int i = k / nx + 1, j = k - (k / nx) * nx + 1;   // row i (y), column j (x), both 1-based
+  double x = (j - 1) * P.dx + P.dx / 2.0;
+  double y = (i - 1) * (P.box_y / ny) + (P.box_y / ny) / 2.0;
+  B.g_pos[k] = make_double2(x, y);
+  double u = P.U * sin(2 * pi * x) * cos(2 * pi * y);
+  double v = -P.U * cos(2 * pi * x) * sin(2 * pi * y);
+  B.g_vel[k] = make_double2(u, v);
+  double pr = 0.25 * P.rho0 * (P.U * P.U) * (cos(4 * pi * x) + cos(4 * pi * y));
+  double rho = P.rho0 * pow(1 + 7 * pr / (P.c0sq * P.rho0), expo);
+  B.g_p[k] = pr;
+  B.g_rho[k] = rho;
+  B.g_mass[k] = vol0 * rho;
+  B.g_type[k] = 2;
+  B.g_c[k] = P.sqrt_c0sq;
+  B.g_e[k] = P.c0sq / (7.0 - 1.0);
+}
+
+void launch_init_tgv(hipStream_t s, const DevParams& P, Buffers& B, int nx, int ny) {
+  const double pi = 3.14159265358979324;            // information_module.f90:29
+  const double expo = (double)0.142857142f;         // REAL(4) literal, Distribute_Initial_Particle.f90:222
+  const double vol0 = P.dx * (P.box_y / ny);
+  hipLaunchKernelGGL(k_init_tgv, dim3(grid_for(nx * ny)), dim3(kBlock), 0, s, P, B, nx, ny, pi, expo, vol0);
+}
+
+// ---------------------------------------------------------------------------
+// Distribute_Period_Boundary_Particle.f90:56-206
+// ---------------------------------------------------------------------------
+__global__ void k_wrap_flags(DevParams P, Buffers B, int cap_fluid) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cap_fluid) return;
+  const int N = B.counters[C_NFLUID];
+  if (i >= N || B.g_type[i] == 120) { B.cnt_x[i] = 0; B.cnt_yf[i] = 0; return; }
+  double2 p = B.g_pos[i];
+  if (p.x > P.box_x) p.x = p.x - P.box_x;
+  else if (p.x < 0.0) p.x = p.x + P.box_x;
+  if (p.y > P.box_y) p.y = p.y - P.box_y;
+  else if (p.y < 0.0) p.y = p.y + P.box_y;
+  B.g_pos[i] = p;
+  const double r3h = P.three_h;
+  int l = (p.x <= (0.0 + r3h) && p.x >= 0.0);
+  int r = (p.x >= (P.box_x - r3h) && p.x <= P.box_x);
+  int yl = (p.y <= (0.0 + r3h) && p.y >= 0.0);
+  int yr = (p.y >= (P.box_y - r3h) && p.y <= P.box_y);
+  B.cnt_x[i] = l + r;
+  B.cnt_yf[i] = yl + yr;
+}
+
+__device__ __forceinline__ void copy_particle(Buffers& B, int dst, int src, double2 pos) {
+  B.g_pos[dst] = pos;
+  B.g_vel[dst] = B.g_vel[src];
+  B.g_rho[dst] = B.g_rho[src];
+  B.g_mass[dst] = B.g_mass[src];
+  B.g_type[dst] = -101;
+  B.g_p[dst] = B.g_p[src];
+  B.g_c[dst] = B.g_c[src];
+  B.g_e[dst] = B.g_e[src];
+}
+
+// x-direction ghosts (:98-148); records each x-ghost's own y-ghost count.
+__global__ void k_ghost_x(DevParams P, Buffers B, int cap_fluid, int cap_ghost) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cap_fluid) return;
+  const int N = B.counters[C_NFLUID];
+  if (i >= N || B.cnt_x[i] == 0) return;
+  const double2 p = B.g_pos[i];
+  const double r3h = P.three_h;
+  int k = B.off_x[i];
+  if (p.x <= (0.0 + r3h) && p.x >= 0.0) {
+    if (k < cap_ghost) {
+      copy_particle(B, N + k, i, make_double2(p.x + P.box_x, p.y));
+      B.cnt_yx[k] = B.cnt_yf[i];
+      B.src_x[k] = i;
+    } else {
+      atomicOr(&B.counters[C_ERR], E_GHOST_CAP);
+    }
+    ++k;
+  }
+  if (p.x >= (P.box_x - r3h) && p.x <= P.box_x) {
+    if (k < cap_ghost) {
+      copy_particle(B, N + k, i, make_double2(p.x - P.box_x, p.y));
+      B.cnt_yx[k] = B.cnt_yf[i];
+      B.src_x[k] = i;
+    } else {
+      atomicOr(&B.counters[C_ERR], E_GHOST_CAP);
+    }
+  }
+}
+
+// y-direction ghosts of fluid particles (:153-206, first part of the loop)
+__global__ void k_ghost_yf(DevParams P, Buffers B, int cap_fluid, int cap_ghost) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cap_fluid) return;
+  const int N = B.counters[C_NFLUID];
+  if (i >= N || B.cnt_yf[i] == 0) return;
+  const int nxg = B.counters[C_NXG];
+  const double2 p = B.g_pos[i];
+  const double r3h = P.three_h;
+  int k = nxg + B.off_yf[i];
+  if (p.y <= (0.0 + r3h) && p.y >= 0.0) {
+    if (k < cap_ghost) copy_particle(B, N + k, i, make_double2(p.x, p.y + P.box_y));
+    else atomicOr(&B.counters[C_ERR], E_GHOST_CAP);
+    ++k;
+  }
+  if (p.y >= (P.box_y - r3h) && p.y <= P.box_y) {
+    if (k < cap_ghost) copy_particle(B, N + k, i, make_double2(p.x, p.y - P.box_y));
+    else atomicOr(&B.counters[C_ERR], E_GHOST_CAP);
+  }
+}
+
+// y-direction ghosts of the x-ghosts (corners), in x-ghost order; sets the ghost count.
+__global__ void k_ghost_yx(DevParams P, Buffers B, int cap_ghost) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = B.counters[C_NFLUID];
+  const int nxg = B.counters[C_NXG], nyf = B.counters[C_NYF], nyx = B.counters[C_NYX];
+  if (g == 0) {
+    int ng = nxg + nyf + nyx;
+    if (ng > cap_ghost) { atomicOr(&B.counters[C_ERR], E_GHOST_CAP); ng = cap_ghost; }
+    B.counters[C_NGHOST] = ng;
+  }
+  if (g >= cap_ghost || g >= nxg || B.cnt_yx[g] == 0) return;
+  const int src = N + g;
+  const double2 p = B.g_pos[src];
+  const double r3h = P.three_h;
+  int k = nxg + nyf + B.off_yx[g];
+  if (p.y <= (0.0 + r3h) && p.y >= 0.0) {
+    if (k < cap_ghost) copy_particle(B, N + k, src, make_double2(p.x, p.y + P.box_y));
+    else atomicOr(&B.counters[C_ERR], E_GHOST_CAP);
+    ++k;
+  }
+  if (p.y >= (P.box_y - r3h) && p.y <= P.box_y) {
+    if (k < cap_ghost) copy_particle(B, N + k, src, make_double2(p.x, p.y - P.box_y));
+    else atomicOr(&B.counters[C_ERR], E_GHOST_CAP);
+  }
+}
+
+void launch_period_boundary(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int cap_ghost) {
+  hipLaunchKernelGGL(k_wrap_flags, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, s, P, B, cap_fluid);
+  exclusive_scan(s, B.cnt_x, cap_fluid, B.off_x, B.scan_blocks, B.counters + C_NXG);
+  exclusive_scan(s, B.cnt_yf, cap_fluid, B.off_yf, B.scan_blocks, B.counters + C_NYF);
+  launch_fill_i32(s, B.cnt_yx, cap_ghost, 0);
+  hipLaunchKernelGGL(k_ghost_x, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, s, P, B, cap_fluid, cap_ghost);
+  exclusive_scan(s, B.cnt_yx, cap_ghost, B.off_yx, B.scan_blocks, B.counters + C_NYX);
+  hipLaunchKernelGGL(k_ghost_yf, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, s, P, B, cap_fluid, cap_ghost);
+  hipLaunchKernelGGL(k_ghost_yx, dim3(grid_for(cap_ghost)), dim3(kBlock), 0, s, P, B, cap_ghost);
+}
+
+// ---------------------------------------------------------------------------
+// Devide_block_and_send_tasks.f90 on one rank
+// ---------------------------------------------------------------------------
+__global__ void k_analytic(DevParams P, Buffers B, int cap_fluid, double pi, double t, double Re) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cap_fluid || i >= B.counters[C_NFLUID]) return;
+  const double2 p = B.g_pos[i];
+  double ua = P.U * sin(2 * pi * p.x) * cos(2 * pi * p.y) * exp(-8 * (pi * pi) / Re * t);
+  double va = -P.U * cos(2 * pi * p.x) * sin(2 * pi * p.y) * exp(-8 * (pi * pi) / Re * t);
+  double pa = 0.25 * P.rho0 * (P.U * P.U) * (cos(4 * pi * p.x) + cos(4 * pi * p.y)) *
+              exp(-16 * (pi * pi) / Re * t);
+  B.ana[4 * i + 0] = ua;
+  B.ana[4 * i + 1] = va;
+  B.ana[4 * i + 2] = pa;
+  B.ana[4 * i + 3] = sqrt(ua * ua + va * va);
+}
+
+// :75-83 (y<0 -> ill) and :123-162 binning; key = column-major cell index.
+__global__ void k_cell_key(DevParams P, Buffers B, int cap_global) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= cap_global) return;
+  const int N = B.counters[C_NFLUID];
+  const int ng = N + B.counters[C_NGHOST];
+  if (g >= ng) return;
+  int t = B.g_type[g];
+  const double2 p = B.g_pos[g];
+  if (g < N && t != 120 && p.y < 0.0) { t = 120; B.g_type[g] = 120; }
+  if (t == 120 || t == 0) { B.g_key[g] = -1; return; }
+  const int xn = (int)floor((p.x - P.chain_ox) / P.chain_d + 1);
+  const int yn = (int)floor((p.y - P.chain_oy) / P.chain_d + 1);
+  if (xn < 1 || xn > P.chain_nx || yn < 1 || yn > P.chain_ny) {
+    B.g_type[g] = 120;
+    B.g_key[g] = -1;
+    atomicAdd(&B.counters[C_NILL], 1);
+    return;
+  }
+  const int key = (xn - 1) * P.chain_ny + (yn - 1);
+  B.g_key[g] = key;
+  B.g_slot[g] = atomicAdd(&B.cell_count[key], 1);
+}
+
+__global__ void k_cell_scatter(Buffers B, int cap_global) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= cap_global) return;
+  const int ng = B.counters[C_NFLUID] + B.counters[C_NGHOST];
+  if (g >= ng) return;
+  const int key = B.g_key[g];
+  if (key < 0) return;
+  B.cell_members[B.cell_start[key] + B.g_slot[g]] = g;
+}
+
+// Rank inside the cell by gid gives the reference's within-cell order; then the
+// global -> subdomain copy (:666-765) with EOS and volume (single_step_compute.f90:62-73,
+// Assign_boundary_particles.f90:303-321).
+__global__ void k_cell_rank_gather(DevParams P, Buffers B, int cap_global) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= cap_global) return;
+  const int ng = B.counters[C_NFLUID] + B.counters[C_NGHOST];
+  if (g >= ng) return;
+  const int key = B.g_key[g];
+  if (key < 0) return;
+  const int s0 = B.cell_start[key], s1 = B.cell_start[key + 1];
+  int rank = 0;
+  for (int m = s0; m < s1; ++m) rank += (B.cell_members[m] < g);
+  atomicMax(&B.counters[C_MAXCELL], s1 - s0);
+  const int l = s0 + rank;
+  B.name[l] = g;
+  B.l_cell[l] = key;
+  const double2 pos = B.g_pos[g];
+  B.l_pos[l] = pos;
+  B.l_pos_bin[l] = pos;
+  B.l_vel[l] = B.g_vel[g];
+  const double rho = B.g_rho[g];
+  const double m = B.g_mass[g];
+  const int t = B.g_type[g];
+  B.l_mass[l] = m;
+  B.l_type[l] = t;
+  B.l_e[l] = B.g_e[g];
+  double p = B.g_p[g], c = B.g_c[g];
+  if (t != 120) tait(P, rho, p, c);
+  B.l_pc[l] = make_double2(p, c);
+  B.l_rv[l] = make_double2(rho, m / rho);
+}
+
+void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_global, int n_cells,
+                         bool analytic, double t_nondim) {
+  const double pi = 3.14159265358979324;
+  if (analytic) {
+    const double Re = P.U * P.box_x / P.nu;        // information_module.f90:60
+    hipLaunchKernelGGL(k_analytic, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, P, B, cap_global, pi,
+                       t_nondim, Re);
+  }
+  launch_fill_i32(s, B.cell_count, n_cells, 0);
+  hipLaunchKernelGGL(k_cell_key, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, P, B, cap_global);
+  exclusive_scan(s, B.cell_count, n_cells, B.cell_start, B.scan_blocks, B.counters + C_NLOCAL,
+                 B.cell_start + n_cells);
+  hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, B, cap_global);
+  hipLaunchKernelGGL(k_cell_rank_gather, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, P, B, cap_global);
+}
+
+// ---------------------------------------------------------------------------
+// single_step_compute: pass A (volume fraction, sum dW, SPS strain -> tau, grad rho)
+// Define_freesurface_and_Normailized_derivates.f90:79-131,141-159 and
+// Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90:144-296,508-526
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  const double2 ps = B.l_pos[s];
+  const double2 vs = B.l_vel[s];
+  const double2 rvs = B.l_rv[s];
+  const int ts = B.l_type[s];
+  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
+  const double cut2 = P.three_h * P.three_h * (1.0 + 1e-10);
+  double fra = 0.0, sdx = 0.0, sdy = 0.0, Sxx = 0.0, Sxy = 0.0, Syy = 0.0, grx = 0.0, gry = 0.0;
+#pragma unroll 1
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll 1
+    for (int o = R.lo[d]; o < R.hi[d]; ++o) {
+      if (o == s) continue;
+      const double2 po = B.l_pos[o];
+      const bool sa = s < o;                       // self is the pair's i (lower local index)
+      const double pdx = sa ? ps.x - po.x : po.x - ps.x;   // position_difference = x_a - x_b
+      const double pdy = sa ? ps.y - po.y : po.y - ps.y;
+      const double r2 = pdx * pdx + pdy * pdy;
+      if (r2 > cut2) continue;
+      const double r = sqrt(r2);
+      if (!(r <= P.three_h && r >= P.r_min)) continue;
+      const int to = B.l_type[o];
+      if (ts != 2 && to != 2) continue;
+      double w, dwx, dwy;
+      kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
+      const double2 rvo = B.l_rv[o];
+      const double2 vo = B.l_vel[o];
+      const double Vo = rvo.y;
+      fra = fra + Vo / P.support_volume;
+      if (sa) { sdx = sdx + dwx * Vo; sdy = sdy + dwy * Vo; }
+      else { sdx = sdx - dwx * Vo; sdy = sdy - dwy * Vo; }
+      // velocity_difference = -(v_a - v_b); identity renormalisation: nd = dwdx
+      const double vdx = sa ? -(vs.x - vo.x) : -(vo.x - vs.x);
+      const double vdy = sa ? -(vs.y - vo.y) : -(vo.y - vs.y);
+      Sxx = Sxx + vdx * dwx * Vo;
+      Sxy = Sxy + (vdx * dwy + vdy * dwx) * Vo;
+      Syy = Syy + vdy * dwy * Vo;
+      const double rd = sa ? -(rvs.x - rvo.x) : -(rvo.x - rvs.x);   // -(rho_a - rho_b)
+      grx = grx + rd * Vo * dwx;
+      gry = gry + rd * Vo * dwy;
+    }
+  }
+  B.fra_v[s] = fra;
+  B.sdw[s] = make_double2(sdx, sdy);
+  B.grad[s] = make_double2(grx, gry);
+  double txx = 0.0, txy = 0.0, tyy = 0.0;
+  if (ts == 2) {                                   // Solver...SPS.f90:256-296
+    const double master = Sxx * Sxx + Syy * Syy + 0.0;
+    const double last = Sxy * Sxy + 0.0 + 0.0;
+    const double abs2 = 2 * master + last;
+    const double t0 = P.sps_c012 * P.sps_dl;
+    const double nut = (t0 * t0) * sqrt(abs2);
+    const double kk = P.sps_c23 * nut * (Sxx + Syy);
+    const double blin = P.sps_c23b * (P.sps_dl * P.sps_dl) * abs2;
+    txx = (2 * nut * Sxx - kk - blin) * rvs.x;
+    txy = (nut * Sxy) * rvs.x;
+    tyy = (2 * nut * Syy - kk - blin) * rvs.x;
+  }
+  B.tau[3 * s + 0] = txx;
+  B.tau[3 * s + 1] = txy;
+  B.tau[3 * s + 2] = tyy;
+  const int sf = (ts == 2 && fra <= P.fra_limit) ? 1 : 0;
+  B.surf[s] = sf;
+  B.near_[s] = 0;
+  if (sf) atomicAdd(&B.counters[C_NSURF], 1);
+}
+
+// Surface normal and geometric test (Define_freesurface...f90:295-376), surface particles only.
+__global__ void __launch_bounds__(kBlock) k_eval_b1(DevParams P, Buffers B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  if (B.surf[s] != 1) { B.nrm[s] = make_double2(0.0, 0.0); return; }
+  const double2 sd = B.sdw[s];
+  // identity inverse: n = matmul(I, sum_dwdx)
+  const double n1 = 1.0 * sd.x + 0.0 * sd.y;
+  const double n2 = 0.0 * sd.x + 1.0 * sd.y;
+  double len = 0.0;
+  len = len + n1 * n1;
+  len = len + n2 * n2;
+  len = sqrt(len);
+  bool keep = true;
+  double nx = 0.0, ny = 0.0;
+  if (isnan(len) || len == 0.0) {
+    keep = false;
+  } else {
+    nx = -n1 / len;
+    ny = -n2 / len;
+    const double2 ps = B.l_pos[s];
+    const double vx = ps.x + P.h * nx, vy = ps.y + P.h * ny;
+    const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
+    const double pi = 3.14159265358979324;
+    for (int d = 0; d < 3 && keep; ++d)
+      for (int o = R.lo[d]; o < R.hi[d]; ++o) {
+        if (B.l_type[o] != 2) continue;
+        const double2 po = B.l_pos[o];
+        const double pdx = po.x - ps.x, pdy = po.y - ps.y;
+        const double JI = sqrt(pdx * pdx + pdy * pdy);
+        const double Nd = nx * pdx + ny * pdy;
+        const double qx = po.x - vx, qy = po.y - vy;
+        const double JT = sqrt(qx * qx + qy * qy);
+        if (JI >= P.surf_lim && JT < P.h) { keep = false; break; }
+        if (JI < P.surf_lim && acos(Nd / JI) <= (pi / 4.0)) { keep = false; break; }
+      }
+  }
+  B.nrm[s] = make_double2(nx, ny);
+  if (!keep) {
+    B.surf[s] = 0;
+    atomicSub(&B.counters[C_NSURF], 1);
+  }
+}
+
+// Near-surface flag (Define_freesurface...f90:418-440): any pair partner is a surface particle.
+__global__ void __launch_bounds__(kBlock) k_eval_b2(DevParams P, Buffers B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  if (B.counters[C_NSURF] == 0) return;            // near_ was zeroed by pass A
+  const double2 ps = B.l_pos[s];
+  const int ts = B.l_type[s];
+  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
+  int nr = 0;
+  for (int d = 0; d < 3 && !nr; ++d)
+    for (int o = R.lo[d]; o < R.hi[d]; ++o) {
+      if (o == s || B.surf[o] != 1) continue;
+      const double2 po = B.l_pos[o];
+      const double pdx = ps.x - po.x, pdy = ps.y - po.y;
+      const double r = sqrt(pdx * pdx + pdy * pdy);
+      if (!(r <= P.three_h && r >= P.r_min)) continue;
+      if (ts != 2 && B.l_type[o] != 2) continue;
+      nr = 1;
+      break;
+    }
+  B.near_[s] = nr;
+}
+
+// ---------------------------------------------------------------------------
+// pass C: Riemann continuity/momentum, artificial viscosity, laplacian,
+// delta-SPH internal, SPS, XSPH, delta-SPH density correction, shift sums.
+// Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90:300-635,
+// Solver_Normalized_Linear_Riemann_Delta_SPS.f90:362-371,
+// single_step_compute.f90:152-171, Shift_particle_position.f90:78-103.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_eval_c(DevParams P, Buffers B, int do_shift) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  const double2 ps = B.l_pos[s];
+  const double2 vs = B.l_vel[s];
+  const double2 rvs = B.l_rv[s];
+  const double2 pcs = B.l_pc[s];
+  const int ts = B.l_type[s];
+  const double ms = B.l_mass[s];
+  const double gms = B.g_mass[s];                  // particle_mass(i) with the LOCAL index (quirk)
+  const double txs = B.tau[3 * s], tys = B.tau[3 * s + 1], tzs = B.tau[3 * s + 2];
+  const double2 grs = B.grad[s];
+  const int nes = B.near_[s];
+  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
+  const double cut2 = P.three_h * P.three_h * (1.0 + 1e-10);
+  double drho = 0.0, corr = 0.0;
+  double iax = 0.0, iay = 0.0, avx = 0.0, avy = 0.0, aix = 0.0, aiy = 0.0;
+  double spx = 0.0, spy = 0.0, lpx = 0.0, lpy = 0.0, agx = 0.0, agy = 0.0, shx = 0.0, shy = 0.0;
+#pragma unroll 1
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll 1
+    for (int o = R.lo[d]; o < R.hi[d]; ++o) {
+      if (o == s) continue;
+      const double2 po = B.l_pos[o];
+      const bool sa = s < o;
+      const double pdx = sa ? ps.x - po.x : po.x - ps.x;   // x_ij = x_a - x_b
+      const double pdy = sa ? ps.y - po.y : po.y - ps.y;
+      const double r2 = pdx * pdx + pdy * pdy;
+      if (r2 > cut2) continue;
+      const double r = sqrt(r2);
+      if (!(r <= P.three_h && r >= P.r_min)) continue;
+      const int to = B.l_type[o];
+      if (ts != 2 && to != 2) continue;
+      double w, dwx, dwy;
+      kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
+      const double2 vo = B.l_vel[o];
+      const double2 rvo = B.l_rv[o];
+      const double2 pco = B.l_pc[o];
+      // pair quantities in reference orientation a = min(s,o), b = max(s,o)
+      const double2 va = sa ? vs : vo, vb = sa ? vo : vs;
+      const double ra = sa ? rvs.x : rvo.x, rb = sa ? rvo.x : rvs.x;
+      const double Va = sa ? rvs.y : rvo.y, Vb = sa ? rvo.y : rvs.y;
+      const double pa = sa ? pcs.x : pco.x, pb = sa ? pco.x : pcs.x;
+      const double ca = sa ? pcs.y : pco.y, cb = sa ? pco.y : pcs.y;
+      const double Vother = sa ? Vb : Va;
+      const double hb = P.h;
+      const double Pbar = (pa + pb) / 2.0;
+      const double vbx = (va.x + vb.x) / 2.0, vby = (va.y + vb.y) / 2.0;
+      const double ex = -(pdx / r), ey = -(pdy / r);
+      const double UL = va.x * ex + va.y * ey;
+      const double UR = vb.x * ex + vb.y * ey;
+      const double Ub = (UL + UR) / 2.0;
+      const double cbar = (ca + cb) / 2.0;
+      const double rbar = (ra + rb) / 2.0;
+      double vsx, vsy, Ps;
+      if (P.solver == SPH2D_SOLVER_LINEAR_RIEMANN) {
+        const double zi = ra * ca, zj = rb * cb;
+        double Us = zi * UR + zj * UL + pa - pb;
+        Us = Us / (zi + zj);
+        vsx = Us * ex; vsy = Us * ey;
+        Ps = zi * pb + zj * pa + ra * ca * rb * cb * (UL - UR);
+        Ps = Ps / (zi + zj);
+      } else {
+        const double Us = Ub + 0.5 * (pa - pb) / (rbar * P.sqrt_c0sq);
+        vsx = (Us - Ub) * ex + vbx;
+        vsy = (Us - Ub) * ey + vby;
+        const double dU = UL - UR;
+        double bR = 3.0 * (dU > 0.0 ? dU : 0.0);
+        bR = bR < cbar ? bR : cbar;
+        Ps = Pbar + 0.5 * bR * rbar * (UL - UR);
+      }
+      const int nea = sa ? nes : B.near_[o];
+      const int neb = sa ? B.near_[o] : nes;
+      const double rij = 1.0 / (ra * rb);
+      if (sa) {
+        const double Ivx = nea ? vbx : vsx, Ivy = nea ? vby : vsy, IP = nea ? Pbar : Ps;
+        const double ti = (va.x - Ivx) * dwx + (va.y - Ivy) * dwy;
+        drho = drho + 2 * ra * Vb * ti;
+        const double mb = B.l_mass[o];
+        iax = iax + mb * (-2 * IP * dwx * rij);
+        iay = iay + mb * (-2 * IP * dwy * rij);
+      } else {
+        const double Jvx = neb ? vbx : vsx, Jvy = neb ? vby : vsy, JP = neb ? Pbar : Ps;
+        const double tj = -((vb.x - Jvx) * dwx + (vb.y - Jvy) * dwy);
+        drho = drho + 2 * rb * Va * tj;
+        const double ma = B.l_mass[o];
+        iax = iax - ma * (-2 * JP * dwx * rij);
+        iay = iay - ma * (-2 * JP * dwy * rij);
+      }
+      const double vijx = va.x - vb.x, vijy = va.y - vb.y;
+      const double vx = vijx * pdx + vijy * pdy;
+      if (vx < 0.0) {
+        const double fh = P.fai * hb;
+        const double phi = hb * vx / (r2 + fh * fh);
+        const double Pi = (-(P.alpha * cbar * phi) + P.beta * (phi * phi)) / rbar;
+        const double mo = B.l_mass[o];
+        if (sa) { avx = avx + mo * (-Pi * dwx); avy = avy + mo * (-Pi * dwy); }
+        else { avx = avx - mo * (-Pi * dwx); avy = avy - mo * (-Pi * dwy); }
+      }
+      const double Rd = pdx * dwx + pdy * dwy;
+      const double lf = Vother * 2 * P.nu * Rd / r2;
+      if (sa) { lpx = lpx + lf * vijx; lpy = lpy + lf * vijy; }
+      else { lpx = lpx - lf * vijx; lpy = lpy - lf * vijy; }
+      const double tmp = P.delta_alpha * hb * cbar * vx / r2;
+      if (sa) {
+        aix = aix + tmp * dwx * P.rho0 / ra * Vb;
+        aiy = aiy + tmp * dwy * P.rho0 / ra * Vb;
+      } else {
+        aix = aix - tmp * dwx * P.rho0 / rb * Va;
+        aiy = aiy - tmp * dwy * P.rho0 / rb * Va;
+      }
+      {  // SPS: raw dwdx, tau summed over both particles, temp_p carried from x to y
+        const double txo = B.tau[3 * o], tyo = B.tau[3 * o + 1], tzo = B.tau[3 * o + 2];
+        const double gmo = B.g_mass[o];
+        double tp = 0.0;
+        tp = tp + (txs + txo) * dwx;
+        tp = tp + (tys + tyo) * dwy;
+        tp = tp * rij;
+        if (sa) spx = spx + gmo * tp; else spx = spx - gmo * tp;
+        tp = tp + (tzs + tzo) * dwy + (tys + tyo) * dwx;
+        tp = tp * rij;
+        if (sa) spy = spy + gmo * tp; else spy = spy - gmo * tp;
+      }
+      if (P.epsilon_xsph != 0.0 && P.solver == SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN) {
+        const double mo = B.l_mass[o];
+        const double rs = ra + rb;
+        if (sa) {
+          agx = agx - P.epsilon_xsph * 2 * mo * vijx / rs * w;
+          agy = agy - P.epsilon_xsph * 2 * mo * vijy / rs * w;
+        } else {
+          agx = agx + P.epsilon_xsph * 2 * mo * vijx / rs * w;
+          agy = agy + P.epsilon_xsph * 2 * mo * vijy / rs * w;
+        }
+      }
+      {  // delta-SPH density diffusion (Solver...:552-583)
+        const double2 gro = B.grad[o];
+        const double rd = -(ra - rb);
+        const double qx = -pdx, qy = -pdy;
+        const double q2 = qx * qx + qy * qy;
+        const double sgx = sa ? grs.x + gro.x : gro.x + grs.x;
+        const double sgy = sa ? grs.y + gro.y : gro.y + grs.y;
+        const double psi = rd - 0.5 * (sgx * qx + sgy * qy);
+        const double tt = (qx * dwx + qy * dwy) / q2;
+        const double cc = 2 * P.delta * hb * cbar * psi * tt * Vother;
+        corr = sa ? corr + cc : corr - cc;
+      }
+      if (do_shift) {  // Shift_particle_position.f90:78-103
+        const double wr = w / P.w_dx;
+        const double f = P.shift_beta * P.h2 * (1 + P.shift_R * (wr * wr * wr * wr));
+        if (sa) { shx = shx - f * dwx * Vother; shy = shy - f * dwy * Vother; }
+        else { shx = shx + f * dwx * Vother; shy = shy + f * dwy * Vother; }
+      }
+    }
+  }
+  (void)ms; (void)gms;
+  if (ts == 2) drho = drho + corr;
+  B.drhodt[s] = drho;
+  double ax = 0.0, ay = 0.0;
+  if (ts == 2) {
+    ax = iax + 0.0 + avx + aix + spx + lpx;
+    ay = iay + 0.0 + avy + aiy + spy + lpy;
+  }
+  B.acc[s] = make_double2(ax, ay);
+  if (P.epsilon_xsph != 0.0) B.avg[s] = make_double2(agx, agy);
+  if (do_shift) B.shift[s] = make_double2(shx, shy);
+  if (P.write_debug) {
+    double* d = B.dbg + 10 * s;
+    d[0] = iax; d[1] = iay; d[2] = avx; d[3] = avy; d[4] = aix; d[5] = aiy;
+    d[6] = spx; d[7] = spy; d[8] = lpx; d[9] = lpy;
+  }
+}
+
+void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
+  const int g = grid_for(cap_local);
+  launch_fill_i32(s, B.counters + C_NSURF, 1, 0);
+  hipLaunchKernelGGL(k_eval_a, dim3(g), dim3(kBlock), 0, s, P, B);
+  hipLaunchKernelGGL(k_eval_b1, dim3(g), dim3(kBlock), 0, s, P, B);
+  hipLaunchKernelGGL(k_eval_b2, dim3(g), dim3(kBlock), 0, s, P, B);
+  hipLaunchKernelGGL(k_eval_c, dim3(g), dim3(kBlock), 0, s, P, B, 1);
+}
+
+void launch_eval_pass(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int pass) {
+  const int g = grid_for(cap_local);
+  if (pass == 0) hipLaunchKernelGGL(k_eval_c, dim3(g), dim3(kBlock), 0, s, P, B, 1);
+  else if (pass == 1) hipLaunchKernelGGL(k_eval_a, dim3(g), dim3(kBlock), 0, s, P, B);
+  else launch_eval(s, P, B, cap_local);
+}
+
+// ---------------------------------------------------------------------------
+// Stage updates of time_iteration.f90 (fluid particles only; whole-array saves)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void eos_volume(const DevParams& P, Buffers& B, int s, double rho) {
+  double p, c;
+  tait(P, rho, p, c);
+  B.l_pc[s] = make_double2(p, c);
+  B.l_rv[s] = make_double2(rho, B.l_mass[s] / rho);
+}
+
+__global__ void k_stage(DevParams P, Buffers B, int kind) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  const double dt = P.dt;
+  const int t = B.l_type[s];
+  const double drho = B.drhodt[s];
+  const double2 a = B.acc[s];
+  const double2 avg = (P.epsilon_xsph != 0.0) ? B.avg[s] : make_double2(0.0, 0.0);
+  double rho = B.l_rv[s].x;
+  double2 v = B.l_vel[s];
+  double2 x = B.l_pos[s];
+  if (kind == ST_SAVE_BEFORE) {
+    B.b_rho[s] = rho; B.b_vel[s] = v; B.b_pos[s] = x;
+    return;
+  }
+  if (kind == ST_EOS) {
+    if (t != 120) eos_volume(P, B, s, rho);
+    return;
+  }
+  if (kind == ST_PC_PREDICT || kind == ST_RK_HALF0) {      // before_* = state (whole arrays)
+    B.b_rho[s] = rho; B.b_vel[s] = v; B.b_pos[s] = x;
+  }
+  const double brho = B.b_rho[s];
+  const double2 bv = B.b_vel[s], bx = B.b_pos[s];
+  if (kind == ST_PC_PREDICT) {                              // :553-566
+    if (t == 2) {
+      rho = brho + drho * dt;
+      v.x = bv.x + dt * a.x + avg.x;
+      x.x = bx.x + dt * v.x;
+      v.y = bv.y + dt * a.y + avg.y;
+      x.y = bx.y + dt * v.y;
+      B.l_vel[s] = v; B.l_pos[s] = x;
+      eos_volume(P, B, s, rho);
+    }
+    B.t_vel[s] = v; B.t_drho[s] = drho; B.t_acc[s] = a;    // :573-576
+  } else if (kind == ST_PC_CORRECT) {                       // :590-599
+    if (t == 2) {
+      const double td = B.t_drho[s];
+      const double2 ta = B.t_acc[s], tv = B.t_vel[s];
+      rho = brho + 0.5 * (drho + td) * dt;
+      v.x = bv.x + 0.5 * dt * (a.x + ta.x) + avg.x;
+      x.x = bx.x + 0.5 * dt * (v.x + tv.x) + 0.5 * dt * dt * a.x;
+      v.y = bv.y + 0.5 * dt * (a.y + ta.y) + avg.y;
+      x.y = bx.y + 0.5 * dt * (v.y + tv.y) + 0.5 * dt * dt * a.y;
+      B.l_vel[s] = v; B.l_pos[s] = x;
+      B.l_rv[s] = make_double2(rho, B.l_rv[s].y);           // p, c, volume keep the last evaluation's values
+    }
+  } else if (kind == ST_RK_HALF0 || kind == ST_RK_HALF1 || kind == ST_RK_FULL2) {
+    const bool half = kind != ST_RK_FULL2;
+    if (t == 2) {
+      if (half) {
+        rho = brho + drho * dt / 2;
+        v.x = bv.x + 0.5 * dt * a.x + avg.x;
+        x.x = bx.x + 0.5 * dt * v.x;
+        v.y = bv.y + 0.5 * dt * a.y + avg.y;
+        x.y = bx.y + 0.5 * dt * v.y;
+      } else {
+        rho = brho + drho * dt;
+        v.x = bv.x + dt * a.x + avg.x;
+        x.x = bx.x + dt * v.x;
+        v.y = bv.y + dt * a.y + avg.y;
+        x.y = bx.y + dt * v.y;
+      }
+      B.l_vel[s] = v; B.l_pos[s] = x;
+      eos_volume(P, B, s, rho);
+    }
+    const int k = kind == ST_RK_HALF0 ? 0 : (kind == ST_RK_HALF1 ? 1 : 2);
+    const size_t off = (size_t)k * (size_t)B.kstride;
+    B.k_vel[off + s] = v; B.k_drho[off + s] = drho; B.k_acc[off + s] = a;
+  } else if (kind == ST_RK_FINAL) {                         // :918-928
+    if (t == 2) {
+      const size_t st = (size_t)B.kstride;
+      const double k1 = B.k_drho[s], k2 = B.k_drho[st + s], k3 = B.k_drho[2 * st + s];
+      const double2 a1 = B.k_acc[s], a2 = B.k_acc[st + s], a3 = B.k_acc[2 * st + s];
+      const double2 v1 = B.k_vel[s], v2 = B.k_vel[st + s], v3 = B.k_vel[2 * st + s];
+      rho = brho + (k1 + 2 * k2 + 2 * k3 + drho) * dt / 6.0;
+      v.x = bv.x + (a1.x + 2 * a2.x + 2 * a3.x + a.x) * dt / 6.0;
+      x.x = bx.x + (v1.x + 2 * v2.x + 2 * v3.x + v.x) * dt / 6.0 + 0.5 * dt * dt * a.x;
+      v.y = bv.y + (a1.y + 2 * a2.y + 2 * a3.y + a.y) * dt / 6.0;
+      x.y = bx.y + (v1.y + 2 * v2.y + 2 * v3.y + v.y) * dt / 6.0 + 0.5 * dt * dt * a.y;
+      B.l_vel[s] = v; B.l_pos[s] = x;
+      B.l_rv[s] = make_double2(rho, B.l_rv[s].y);
+    }
+  }
+  // a particle that moved more than h/2 since binning leaves the 3x3 stencil's guarantee
+  if (t == 2) {
+    const double2 xb = B.l_pos_bin[s];
+    const double ddx = x.x - xb.x, ddy = x.y - xb.y;
+    if (ddx * ddx + ddy * ddy > 0.25 * P.h * P.h) atomicOr(&B.counters[C_REBIN], 1);
+  }
+}
+
+void launch_stage(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int kind) {
+  hipLaunchKernelGGL(k_stage, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B, kind);
+}
+
+// ---------------------------------------------------------------------------
+// Reinitialize_particle_density.f90:66-182 + MLS_Interpolation_In_Subdomain_Domain.f90
+// order 6: A = sum_k w_k p_k p_k^T + alpha^3 I, BRINV (full pivoting),
+// rho_i = sum_k rho_k Phi_k with Phi_k = (A^-1 B)(1,k) = w_k (A^-1 p_k)_1.
+// ---------------------------------------------------------------------------
+__device__ int brinv6(double* A) {   // function_module.f90:568-639, column-major 6x6
+  constexpr int N = 6;
+  int IS[N], JS[N];
+#define M(i, j) A[(i) + (j) * N]
+  for (int K = 0; K < N; ++K) {
+    double D = 0.0;
+    IS[K] = K; JS[K] = K;
+    for (int I = K; I < N; ++I)
+      for (int J = K; J < N; ++J)
+        if (fabs(M(I, J)) > D) { D = fabs(M(I, J)); IS[K] = I; JS[K] = J; }
+    if (D + 1.0 == 1.0) return 0;
+    for (int J = 0; J < N; ++J) { double T = M(K, J); M(K, J) = M(IS[K], J); M(IS[K], J) = T; }
+    for (int I = 0; I < N; ++I) { double T = M(I, K); M(I, K) = M(I, JS[K]); M(I, JS[K]) = T; }
+    M(K, K) = 1.0 / M(K, K);
+    for (int J = 0; J < N; ++J) if (J != K) M(K, J) = M(K, J) * M(K, K);
+    for (int I = 0; I < N; ++I)
+      if (I != K)
+        for (int J = 0; J < N; ++J)
+          if (J != K) M(I, J) = M(I, J) - M(I, K) * M(K, J);
+    for (int I = 0; I < N; ++I) if (I != K) M(I, K) = -M(I, K) * M(K, K);
+  }
+  for (int K = N - 1; K >= 0; --K) {
+    for (int J = 0; J < N; ++J) { double T = M(K, J); M(K, J) = M(JS[K], J); M(JS[K], J) = T; }
+    for (int I = 0; I < N; ++I) { double T = M(I, K); M(I, K) = M(I, IS[K]); M(I, IS[K]) = T; }
+  }
+#undef M
+  return 1;
+}
+
+__global__ void __launch_bounds__(kBlock) k_mls(DevParams P, Buffers B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  const int ts = B.l_type[s];
+  if (ts != 2) { B.mls_rho[s] = 0.0; return; }     // :69 rho=0 for every non-fluid particle
+  const double2 ps = B.l_pos[s];
+  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
+  const double cut2 = P.three_h * P.three_h * (1.0 + 1e-10);
+  double A[36];
+#pragma unroll
+  for (int k = 0; k < 36; ++k) A[k] = 0.0;
+  double bvec[6] = {0, 0, 0, 0, 0, 0};
+  double sw = 0.0, srw = 0.0;
+  int ne = 0;
+  for (int d = 0; d < 3; ++d)
+    for (int o = R.lo[d]; o < R.hi[d]; ++o) {
+      const double2 po = B.l_pos[o];
+      const double pdx = ps.x - po.x, pdy = ps.y - po.y;
+      const double r2 = pdx * pdx + pdy * pdy;
+      if (r2 > cut2) continue;
+      const double r = sqrt(r2);
+      if (!(r <= P.three_h && r >= P.r_min)) continue;
+      double w, dwx, dwy;
+      kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
+      const double a = (po.x - ps.x) / P.h, b = (po.y - ps.y) / P.h;
+      const double pv[6] = {1.0, a, b, a * a, (po.x - ps.x) / P.h * (po.y - ps.y) / P.h, b * b};
+      const double rho_o = B.l_rv[o].x;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        const double bm = pv[m] * w;
+#pragma unroll
+        for (int l = 0; l < 6; ++l) A[m + 6 * l] = A[m + 6 * l] + bm * pv[l];
+        bvec[m] = bvec[m] + rho_o * bm;
+      }
+      sw = sw + w;
+      srw = srw + rho_o * w;
+      ++ne;
+    }
+  const double al3 = (P.mls_alpha * P.mls_alpha) * P.mls_alpha;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) A[m + 6 * m] = A[m + 6 * m] + al3;
+  double rho;
+  int ok = (ne > 8) ? brinv6(A) : 0;
+  if (ok) {
+    rho = 0.0;
+#pragma unroll
+    for (int l = 0; l < 6; ++l) rho = rho + A[0 + 6 * l] * bvec[l];
+  } else {
+    rho = srw / sw;
+    atomicAdd(&B.counters[C_MLSFB], 1);
+  }
+  B.mls_rho[s] = rho;
+}
+
+void launch_mls(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
+  hipLaunchKernelGGL(k_mls, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B);
+}
+
+// ---------------------------------------------------------------------------
+// Shift_particle_position.f90:108-136 (after an optional MLS reinitialisation:
+// mode 1 = take rho from mls_rho first, 2 = only commit mls_rho, 0 = shift only)
+// ---------------------------------------------------------------------------
+__global__ void k_shift_apply(DevParams P, Buffers B, int mode) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  double rho = B.l_rv[s].x;
+  if (mode >= 1) {
+    rho = B.mls_rho[s];
+    B.l_rv[s] = make_double2(rho, B.l_rv[s].y);
+  }
+  if (mode == 2) return;
+  if (B.l_type[s] != 2 || B.near_[s] == 1) return;
+  double2 sh = B.shift[s];
+  const double amp = sqrt(sh.x * sh.x + sh.y * sh.y);
+  if (amp > P.max_shift) {
+    sh.x = sh.x / amp * P.max_shift;
+    sh.y = sh.y / amp * P.max_shift;
+  }
+  double2 x = B.l_pos[s];
+  x.x = x.x + sh.x;
+  x.y = x.y + sh.y;
+  B.l_pos[s] = x;
+  const double2 gr = B.grad[s];
+  rho = rho + gr.x * sh.x;
+  rho = rho + gr.y * sh.y;
+  B.l_rv[s] = make_double2(rho, B.l_rv[s].y);
+  B.l_pc[s] = make_double2(tait_p(P, rho), B.l_pc[s].y);
+}
+
+void launch_shift_apply(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
+  (void)P;
+  hipLaunchKernelGGL(k_shift_apply, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B, 0);
+}
+
+void launch_shift_apply_mode(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int mode) {
+  hipLaunchKernelGGL(k_shift_apply, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B, mode);
+}
+
+// Refresh_domain.f90:136-300 on one rank.
+__global__ void k_refresh(Buffers B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  const int g = B.name[s];
+  B.g_pos[g] = B.l_pos[s];
+  B.g_vel[g] = B.l_vel[s];
+  const double2 rv = B.l_rv[s];
+  const double2 pc = B.l_pc[s];
+  B.g_rho[g] = rv.x;
+  B.g_p[g] = pc.x;
+  B.g_c[g] = pc.y;
+  B.g_mass[g] = B.l_mass[s];
+  B.g_type[g] = B.l_type[s];
+  B.g_e[g] = B.l_e[s];
+}
+
+void launch_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
+  (void)P;
+  hipLaunchKernelGGL(k_refresh, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, B);
+}
+
+// ---------------------------------------------------------------------------
+// Sampling_subroutine.f90:340-461: block partials in fixed order, then one block.
+// partial layout per block: [ke, vmax, (max, sum, sum2) x 4]
+// ---------------------------------------------------------------------------
+constexpr int kRed = 14;
+
+__global__ void k_sample_partial(DevParams P, Buffers B, int cap_fluid) {
+  __shared__ double lds[kBlock][kRed];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[kRed];
+#pragma unroll
+  for (int k = 0; k < kRed; ++k) v[k] = 0.0;
+  if (i < cap_fluid && i < B.counters[C_NFLUID]) {
+    const double2 u = B.g_vel[i];
+    const double vv = u.x * u.x + u.y * u.y;
+    v[0] = 0.5 * B.g_mass[i] * vv;
+    const double vm = sqrt(vv);
+    v[1] = vm;
+    const double p0 = P.rho0 * P.U * P.U;
+    const double e[4] = {(u.x - B.ana[4 * i]) / P.U, (u.y - B.ana[4 * i + 1]) / P.U,
+                         (B.g_p[i] - B.ana[4 * i + 2]) / p0, (vm - B.ana[4 * i + 3]) / P.U};
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const double ea = fabs(e[a]);
+      v[2 + 3 * a] = ea; v[3 + 3 * a] = ea; v[4 + 3 * a] = ea * ea;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kRed; ++k) lds[threadIdx.x][k] = v[k];
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+#pragma unroll
+      for (int k = 0; k < kRed; ++k) {
+        const bool mx = (k == 1) || (k >= 2 && (k - 2) % 3 == 0);
+        const double a = lds[threadIdx.x][k], b = lds[threadIdx.x + off][k];
+        lds[threadIdx.x][k] = mx ? (a > b ? a : b) : a + b;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kRed; ++k) B.red[blockIdx.x * kRed + k] = lds[0][k];
+}
+
+__global__ void k_sample_final(Buffers B, int nb, double* out) {
+  if (threadIdx.x != 0) return;
+  double v[kRed];
+  for (int k = 0; k < kRed; ++k) v[k] = 0.0;
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < kRed; ++k) {
+      const bool mx = (k == 1) || (k >= 2 && (k - 2) % 3 == 0);
+      const double x = B.red[b * kRed + k];
+      v[k] = mx ? (v[k] > x ? v[k] : x) : v[k] + x;
+    }
+  for (int k = 0; k < kRed; ++k) out[k] = v[k];
+}
+
+void launch_sample(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, double* dev_out14) {
+  const int nb = grid_for(cap_fluid);
+  hipLaunchKernelGGL(k_sample_partial, dim3(nb), dim3(kBlock), 0, s, P, B, cap_fluid);
+  hipLaunchKernelGGL(k_sample_final, dim3(1), dim3(64), 0, s, B, nb, dev_out14);
+}
+
+// ---------------------------------------------------------------------------
+// Pair set export (find_particle_pair.f90:64-123): pairs with a < b (local).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool is_pair(const DevParams& P, const Buffers& B, int s, int o,
+                                        double2 ps, int ts) {
+  const double2 po = B.l_pos[o];
+  const double pdx = ps.x - po.x, pdy = ps.y - po.y;
+  const double r = sqrt(pdx * pdx + pdy * pdy);
+  if (!(r <= P.three_h && r >= P.r_min)) return false;
+  return ts == 2 || B.l_type[o] == 2;
+}
+
+__global__ void k_pair_count(DevParams P, Buffers B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  const double2 ps = B.l_pos[s];
+  const int ts = B.l_type[s];
+  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
+  int cnt = 0;
+  for (int d = 0; d < 3; ++d)
+    for (int o = R.lo[d]; o < R.hi[d]; ++o)
+      if (o > s && is_pair(P, B, s, o, ps, ts)) ++cnt;
+  B.pair_count[s] = cnt;
+}
+
+__global__ void k_pair_write(DevParams P, Buffers B, int64_t capacity) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  const double2 ps = B.l_pos[s];
+  const int ts = B.l_type[s];
+  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
+  int64_t k = B.pair_off[s];
+  const int gs = B.name[s];
+  for (int d = 0; d < 3; ++d)
+    for (int o = R.lo[d]; o < R.hi[d]; ++o)
+      if (o > s && is_pair(P, B, s, o, ps, ts)) {
+        if (k < capacity) {
+          const int go = B.name[o];
+          B.pair_lo[k] = gs < go ? gs : go;
+          B.pair_hi[k] = gs < go ? go : gs;
+        }
+        ++k;
+      }
+}
+
+void launch_count_pairs(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
+  hipLaunchKernelGGL(k_pair_count, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B);
+  launch_fill_i32(s, B.counters + C_NPAIR_LO, 1, 0);
+  exclusive_scan(s, B.pair_count, cap_local, B.pair_off, B.scan_blocks, B.counters + C_NPAIR_LO);
+}
+
+void launch_pairs(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int64_t capacity,
+                  int64_t* n_pairs) {
+  (void)n_pairs;
+  hipLaunchKernelGGL(k_pair_write, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B, capacity);
+}
+
+// ---------------------------------------------------------------------------
+// array forms of compute_kernel / status_equation
+// ---------------------------------------------------------------------------
+__global__ void k_kernel_n(int n, const double* r, const double* dx, const double* h, double* w,
+                           double* dwdx, double e9, float c4, double pik) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double hb = h[i], dist = r[i];
+  const double q = dist / hb;
+  const double factor = 1.0 / (pik * (hb * hb) * (double)c4);
+  double wi = 0.0, d0 = 0.0, d1 = 0.0;
+  if (dist <= 3.0 * hb) {
+    const double eq = exp(-(q * q));
+    wi = (eq - e9) * factor;
+    const double t = eq * (-2.0 * q);
+    d0 = t * dx[i] / (dist * hb) * factor;
+    d1 = t * dx[n + i] / (dist * hb) * factor;
+  }
+  w[i] = wi;
+  dwdx[i] = d0;
+  dwdx[n + i] = d1;
+}
+
+void launch_kernel_n(hipStream_t s, int n, const double* r, const double* dx, const double* h,
+                     double* w, double* dwdx, double e9) {
+  const float c4 = 1.0f - 10.0f * expf(-9.0f);
+  const double pik = (double)3.14159265358f;
+  hipLaunchKernelGGL(k_kernel_n, dim3(grid_for(n)), dim3(kBlock), 0, s, n, r, dx, h, w, dwdx, e9, c4, pik);
+}
+
+__global__ void k_eos_n(int n, double rho0, double c0sq, const double* rho, const int* type,
+                        double* p, double* c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int t = type[i];
+  if (t == 120) return;
+  if (t == 1 || t == -1) {                          // air branch: c only (status_equation.f90:46-50)
+    c[i] = sqrt(((double)1.4f - 1.0) * rho[i]);
+    return;
+  }
+  const double b = c0sq * rho0 / 7.0;
+  const double x = rho[i] / rho0;
+  p[i] = b * (x * x * x * x * x * x * x - 1.0);
+  c[i] = sqrt(c0sq) * (x * x * x);
+}
+
+void launch_eos_n(hipStream_t s, int n, double rho0, double c0sq, const double* rho,
+                  const int* type, double* p, double* c) {
+  hipLaunchKernelGGL(k_eos_n, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rho0, c0sq, rho, type, p, c);
+}
+
+}  // namespace sph2d

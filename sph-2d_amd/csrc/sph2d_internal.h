@@ -1,0 +1,118 @@
+// sph2d_internal.h -- device-side layout of one context and the kernel launchers.
+//
+// Layout in HBM (one context = one simulation):
+//   global order (gid = reference global index - 1): fluid 0..N-1, then the
+//   periodic ghosts regenerated every step (Distribute_Period_Boundary_Particle)
+//   local order (the reference's subdomain order on one rank): 4h cells taken
+//   column by column (x), row by row (y), particles of a cell by ascending gid.
+// Positions and velocities are double2 (x,y) so a neighbour costs one 16-byte
+// load; (rho, volume) and (p, c) are packed the same way.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "../../include/sph2d.h"
+
+namespace sph2d {
+
+constexpr int kBlock = 256;
+
+// Constants of one run, passed by value to every kernel.
+struct DevParams {
+  double box_x, box_y;
+  double rho0, c0sq, sqrt_c0sq, eos_b;   // eos_b = c0sq*rho0/7.0 (status_equation.f90:61)
+  double nu, U;
+  double dx, h, dt;
+  double three_h;                        // kernel_scale*smooth_length (3*h)
+  double kfactor;                        // compute_kernel.f90:123 factor for h
+  double e9;                             // REAL(4) exp(-9.0) widened
+  double r_min;                          // REAL(4) 1.0E-7 widened (find_particle_pair.f90:87)
+  double support_volume;                 // PI*(3h)**2 (Define_freesurface...f90:65)
+  double fra_limit;                      // REAL(4) 0.65 widened
+  double chain_d, chain_ox, chain_oy;    // 4h cells, origin -5 cells
+  int chain_nx, chain_ny;
+  double alpha, beta, fai, delta, delta_alpha, epsilon_xsph;
+  double shift_beta, shift_R, max_shift, w_dx, h2;
+  double sps_dl, sps_c012, sps_c23, sps_c23b;  // REAL(4) constants of the SPS closure
+  double mls_alpha;
+  double surf_lim;                       // REAL(4) 1.414 * h
+  int solver, write_debug;
+  int mls_order;
+};
+
+// Device counters (one int array, indices below).
+enum Counter {
+  C_NFLUID = 0, C_NGHOST, C_NLOCAL, C_NXG, C_NYF, C_NYX, C_NSURF, C_NILL, C_ERR,
+  C_MAXCELL, C_REBIN, C_MLSFB, C_NSING, C_NPAIR_LO, C_NPAIR_HI, C_COUNT = 16
+};
+enum ErrFlag { E_GHOST_CAP = 1, E_LOCAL_CAP = 2, E_CELL = 4 };
+
+struct Buffers {
+  // global order
+  double2 *g_pos, *g_vel;
+  double *g_rho, *g_mass, *g_p, *g_c, *g_e;
+  int *g_type, *g_key, *g_slot;
+  double *ana;                            // 4 per fluid particle: u, v, p, |v|
+  int *cnt_x, *cnt_yf, *off_x, *off_yf, *cnt_yx, *off_yx, *src_x;
+  // cells (column-major key over the global grid)
+  int *cell_count, *cell_start, *cell_members;
+  // local order
+  int *name, *l_type, *l_cell;
+  double2 *l_pos, *l_vel, *l_rv, *l_pc, *l_pos_bin;
+  double *l_mass, *l_e;
+  // one force evaluation
+  double *fra_v;
+  double2 *sdw, *grad, *nrm, *shift, *avg;
+  double *tau;                            // 3 per particle: xx, xy, yy
+  int *surf, *near_;
+  double *drhodt;
+  double2 *acc;
+  double *dbg;                            // 10 per particle (debug components)
+  // integrator (local index, persist across steps)
+  double *b_rho, *t_drho, *k_drho;        // k_drho: 3 stages
+  double2 *b_pos, *b_vel, *t_vel, *t_acc, *k_vel, *k_acc;
+  double *mls_rho;
+  // scratch
+  int *scan_blocks;
+  int *counters;
+  double *red;                            // reduction partials
+  int *pair_count, *pair_off;
+  int *pair_lo, *pair_hi;
+  int kstride;                            // stride of the 3 RK stage slabs (= local capacity)
+};
+
+// Launchers (sph2d_fp64.hip). All run on `s` and take sizes from device counters.
+void launch_init_tgv(hipStream_t s, const DevParams& P, Buffers& B, int nx, int ny);
+void launch_period_boundary(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int cap_ghost);
+void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_global, int n_cells,
+                         bool analytic, double t_nondim);
+void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+// pass 0: force pass only, 1: pass A only, 2: the whole evaluation
+void launch_eval_pass(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int pass);
+void launch_stage(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int kind);
+void launch_mls(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+void launch_shift_apply(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+void launch_shift_apply_mode(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int mode);
+void launch_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+void launch_sample(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, double* host_out14);
+void launch_pairs(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int64_t capacity,
+                  int64_t* n_pairs);
+void launch_kernel_n(hipStream_t s, int n, const double* r, const double* dx, const double* h,
+                     double* w, double* dwdx, double e9);
+void launch_eos_n(hipStream_t s, int n, double rho0, double c0sq, const double* rho,
+                  const int* type, double* p, double* c);
+void launch_fill_i32(hipStream_t s, int* p, int n, int v);
+void launch_count_pairs(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+
+// Stage kinds for launch_stage.
+enum StageKind {
+  ST_SAVE_BEFORE = 0,   // before_* = state
+  ST_PC_PREDICT,        // time_iteration.f90:553-566 (+ save temp_*)
+  ST_PC_CORRECT,        // :590-599
+  ST_RK_HALF0,          // :825-836 (+ save K_1)
+  ST_RK_HALF1,          // :859-870 (+ save K_2)
+  ST_RK_FULL2,          // :889-900 (+ save K_3)
+  ST_RK_FINAL,          // :918-928
+  ST_EOS                // status_equation + volume for all local particles
+};
+
+}  // namespace sph2d

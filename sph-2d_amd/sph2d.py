@@ -1,0 +1,321 @@
+"""Python host mirror of the reference's hot-path interface, over the C-ABI.
+
+The reference (mini-walker/SPH-2D, Taylor_Green_Vortex/) drives its WCSPH step
+from Fortran: time_iteration -> Distribute_Period_Boundary_Particle ->
+Devide_block_and_send_tasks -> single_step_compute -> stage updates ->
+Reinitialize_particle_density -> Shift_particle_position -> Refresh_domain.
+`Simulation` exposes the same routines, with the same names and step
+numbering, on a context whose arrays live in the GPU's HBM
+(libsph2d.so, include/sph2d.h). There is no CPU path: if the HIP library is
+missing or no GPU is present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsph2d.so")
+
+SOLVER_MODIFIED_LINEAR_RIEMANN = 0
+SOLVER_LINEAR_RIEMANN = 1
+INTEGRATOR_PC = 2
+INTEGRATOR_RK = 3
+FP64 = 0
+FP32_MIXED = 1
+
+ERRORS = {1: "argument", 2: "capacity", 3: "hip", 4: "state", 5: "unsupported", 6: "nonfinite"}
+
+
+class SphError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sph2d error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    """sph2d_params (include/sph2d.h); defaults = Taylor_Green_Vortex/information_module.f90."""
+    _fields_ = [
+        ("struct_size", ctypes.c_int32),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("box_x", ctypes.c_double), ("box_y", ctypes.c_double),
+        ("rho0", ctypes.c_double), ("c0sq", ctypes.c_double),
+        ("nu", ctypes.c_double), ("U", ctypes.c_double),
+        ("dx", ctypes.c_double), ("h", ctypes.c_double), ("dt", ctypes.c_double),
+        ("integrator", ctypes.c_int32), ("solver", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("rate_reuse", ctypes.c_int32),
+        ("shift_on", ctypes.c_int32), ("mls_on", ctypes.c_int32),
+        ("mls_every", ctypes.c_int32), ("mls_order", ctypes.c_int32),
+        ("sample_every", ctypes.c_int32), ("periodic", ctypes.c_int32),
+        ("alpha", ctypes.c_double), ("beta", ctypes.c_double), ("fai", ctypes.c_double),
+        ("delta", ctypes.c_double), ("delta_alpha", ctypes.c_double),
+        ("epsilon_xsph", ctypes.c_double),
+        ("shift_beta", ctypes.c_double), ("shift_R", ctypes.c_double),
+        ("max_shift", ctypes.c_double), ("mls_alpha", ctypes.c_double),
+    ]
+
+
+class Diag(ctypes.Structure):
+    _fields_ = [
+        ("n_fluid", ctypes.c_int32), ("n_ghost", ctypes.c_int32), ("n_local", ctypes.c_int32),
+        ("n_ill", ctypes.c_int32), ("n_surface", ctypes.c_int32), ("n_singular", ctypes.c_int32),
+        ("n_mls_fallback", ctypes.c_int32), ("max_cell_count", ctypes.c_int32),
+        ("n_pairs", ctypes.c_int64), ("rebin_needed", ctypes.c_int32), ("error_flags", ctypes.c_int32),
+    ]
+
+
+# every symbol include/sph2d.h declares (checked by tests on CPU)
+EXPORTS = (
+    "sph2d_last_error", "sph2d_default_params", "sph2d_ctx_create", "sph2d_ctx_destroy",
+    "sph2d_init_tgv", "sph2d_upload_state", "sph2d_download_state", "sph2d_compute_kernel",
+    "sph2d_compute_kernel_n", "sph2d_status_equation", "sph2d_status_equation_n",
+    "sph2d_distribute_period_boundary", "sph2d_devide_block", "sph2d_single_step_compute",
+    "sph2d_reinitialize_particle_density", "sph2d_shift_particle_position",
+    "sph2d_refresh_domain", "sph2d_time_step", "sph2d_run", "sph2d_sample",
+    "sph2d_download_local", "sph2d_download_pairs", "sph2d_set_debug",
+    "sph2d_get_diagnostics", "sph2d_get_stream", "sph2d_time_pass",
+)
+
+_lib = None
+
+
+def lib():
+    """Load libsph2d.so (never a fallback: raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        vp, i32, i64, dp, ip = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, P(ctypes.c_double), P(ctypes.c_int32)
+        L.sph2d_last_error.restype = ctypes.c_char_p
+        sig = {
+            "sph2d_default_params": [P(Params)],
+            "sph2d_ctx_create": [P(Params), i32, P(vp)],
+            "sph2d_ctx_destroy": [vp],
+            "sph2d_init_tgv": [vp],
+            "sph2d_upload_state": [vp, i32, dp, dp, dp, dp, dp, dp, ip, i32],
+            "sph2d_download_state": [vp, i32, dp, dp, dp, dp, dp, dp, ip, i32],
+            "sph2d_compute_kernel": [ip, dp, dp, dp, dp, dp],
+            "sph2d_compute_kernel_n": [i32, dp, dp, dp, dp, dp],
+            "sph2d_status_equation": [dp, dp, ip, dp, dp],
+            "sph2d_status_equation_n": [P(Params), i32, dp, dp, ip, dp, dp],
+            "sph2d_distribute_period_boundary": [vp],
+            "sph2d_devide_block": [vp, i32],
+            "sph2d_single_step_compute": [vp, i32],
+            "sph2d_reinitialize_particle_density": [vp, i32],
+            "sph2d_shift_particle_position": [vp, i32],
+            "sph2d_refresh_domain": [vp, i32],
+            "sph2d_time_step": [vp, i32],
+            "sph2d_run": [vp, i32, i32],
+            "sph2d_sample": [vp, dp],
+            "sph2d_download_local": [vp, ctypes.c_char_p, vp, i32],
+            "sph2d_download_pairs": [vp, i64, ip, ip, P(i64)],
+            "sph2d_set_debug": [vp, i32],
+            "sph2d_get_diagnostics": [vp, P(Diag)],
+            "sph2d_get_stream": [vp, P(vp)],
+            "sph2d_time_pass": [vp, i32, i32, dp],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise SphError(rc, lib().sph2d_last_error().decode())
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    _check(lib().sph2d_default_params(ctypes.byref(p)))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def tgv_params(n_side: int, **kw) -> Params:
+    """Taylor-Green vortex on an n_side^2 lattice of the unit box; dt keeps the
+    reference's c0*dt/h ratio (dt = 1e-4 * 100/n_side) unless given."""
+    dx = 1.0 / n_side
+    p = default_params(nx=n_side, ny=n_side, dx=dx, h=1.20 * dx, max_shift=0.25 * dx,
+                       dt=1e-4 * (100.0 / n_side))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+LOCAL_F64 = ("x", "y", "u", "v", "rho", "mass", "p", "c", "volume", "fra_v", "sum_dwdx_x",
+             "sum_dwdx_y", "tau_xx", "tau_xy", "tau_yy", "grad_rho_x", "grad_rho_y", "drhodt",
+             "ax", "ay", "shift_x", "shift_y", "normal_x", "normal_y", "internal_x", "internal_y",
+             "visc_x", "visc_y", "art_internal_x", "art_internal_y", "sps_x", "sps_y", "lap_x",
+             "lap_y")
+LOCAL_I32 = ("type", "name", "surface", "near_surface")
+
+
+class Simulation:
+    """One HBM-resident simulation context (sph2d_ctx)."""
+
+    def __init__(self, params: Params, capacity: Optional[int] = None):
+        self.params = params
+        cap = capacity if capacity is not None else params.nx * params.ny
+        h = ctypes.c_void_p()
+        _check(lib().sph2d_ctx_create(ctypes.byref(params), int(cap), ctypes.byref(h)))
+        self._c = h
+        self.n_fluid = 0
+
+    def close(self):
+        if getattr(self, "_c", None):
+            lib().sph2d_ctx_destroy(self._c)
+            self._c = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- state ------------------------------------------------------------------
+    def init_tgv(self):
+        _check(lib().sph2d_init_tgv(self._c))
+        self.n_fluid = self.params.nx * self.params.ny
+
+    def upload_state(self, x, y, u, v, rho, mass, energy=None, type_=None):
+        n = len(x)
+        pos = np.ascontiguousarray(np.concatenate([x, y]), dtype=np.float64)
+        vel = np.ascontiguousarray(np.concatenate([u, v]), dtype=np.float64)
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        e = None if energy is None else np.ascontiguousarray(energy, dtype=np.float64)
+        t = None if type_ is None else np.ascontiguousarray(type_, dtype=np.int32)
+        _check(lib().sph2d_upload_state(self._c, n, _dp(pos), _dp(vel), _dp(rho), _dp(mass),
+                                        _dp(e) if e is not None else None, None,
+                                        _ip(t) if t is not None else None, n))
+        self.n_fluid = n
+
+    def download_state(self) -> Dict[str, np.ndarray]:
+        n = self.n_fluid
+        pos, vel = np.zeros(2 * n), np.zeros(2 * n)
+        rho, p, c, m = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+        t = np.zeros(n, dtype=np.int32)
+        _check(lib().sph2d_download_state(self._c, n, _dp(pos), _dp(vel), _dp(rho), _dp(p), _dp(c),
+                                          _dp(m), _ip(t), n))
+        return {"x": pos[:n], "y": pos[n:], "u": vel[:n], "v": vel[n:], "rho": rho, "p": p,
+                "c": c, "mass": m, "type": t}
+
+    # --- the reference's routines -------------------------------------------------
+    def distribute_period_boundary(self):
+        _check(lib().sph2d_distribute_period_boundary(self._c))
+
+    def devide_block(self, i_time_step: int):
+        _check(lib().sph2d_devide_block(self._c, i_time_step))
+
+    def single_step_compute(self, i_time_step: int = 1):
+        _check(lib().sph2d_single_step_compute(self._c, i_time_step))
+
+    def reinitialize_particle_density(self, i_time_step: int):
+        _check(lib().sph2d_reinitialize_particle_density(self._c, i_time_step))
+
+    def shift_particle_position(self, i_time_step: int):
+        _check(lib().sph2d_shift_particle_position(self._c, i_time_step))
+
+    def refresh_domain(self, i_time_step: int):
+        _check(lib().sph2d_refresh_domain(self._c, i_time_step))
+
+    def time_step(self, i_time_step: int):
+        _check(lib().sph2d_time_step(self._c, i_time_step))
+
+    def run(self, first_step: int, nsteps: int):
+        _check(lib().sph2d_run(self._c, first_step, nsteps))
+
+    def sample(self) -> np.ndarray:
+        out = np.zeros(14)
+        _check(lib().sph2d_sample(self._c, _dp(out)))
+        return out
+
+    # --- inspection -------------------------------------------------------------------
+    def set_debug(self, on: bool = True):
+        _check(lib().sph2d_set_debug(self._c, 1 if on else 0))
+
+    def diagnostics(self) -> Diag:
+        d = Diag()
+        _check(lib().sph2d_get_diagnostics(self._c, ctypes.byref(d)))
+        return d
+
+    def n_local(self) -> int:
+        return self.diagnostics().n_local
+
+    def local(self, name: str) -> np.ndarray:
+        n = self.n_local()
+        if name in LOCAL_I32:
+            out = np.zeros(n, dtype=np.int32)
+        elif name in LOCAL_F64:
+            out = np.zeros(n, dtype=np.float64)
+        else:
+            raise KeyError(name)
+        _check(lib().sph2d_download_local(self._c, name.encode(), out.ctypes.data_as(ctypes.c_void_p), n))
+        return out
+
+    def pairs(self) -> np.ndarray:
+        """Sorted keys lo*2^32+hi of the unordered pair set (0-based gids)."""
+        n = ctypes.c_int64()
+        _check(lib().sph2d_download_pairs(self._c, 0, None, None, ctypes.byref(n)))
+        lo = np.zeros(max(n.value, 1), dtype=np.int32)
+        hi = np.zeros(max(n.value, 1), dtype=np.int32)
+        _check(lib().sph2d_download_pairs(self._c, n.value, _ip(lo), _ip(hi), ctypes.byref(n)))
+        return lo[: n.value].astype(np.int64) * (1 << 32) + hi[: n.value]
+
+    def time_pass(self, pass_id: int, reps: int) -> float:
+        """Average ms of one part of the evaluation (0 force pass, 1 pass A, 2 all)."""
+        ms = ctypes.c_double()
+        _check(lib().sph2d_time_pass(self._c, pass_id, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        _check(lib().sph2d_get_stream(self._c, ctypes.byref(s)))
+        return s.value or 0
+
+
+def compute_kernel(r, dx, h):
+    """Array form of compute_kernel.f90 on the GPU: r (n,), dx (n,2), h (n,)."""
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    n = len(r)
+    d = np.ascontiguousarray(np.asarray(dx, dtype=np.float64).T.reshape(-1))  # (n,2) column-major
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    w = np.zeros(n)
+    dw = np.zeros(2 * n)
+    _check(lib().sph2d_compute_kernel_n(n, _dp(r), _dp(d), _dp(h), _dp(w), _dp(dw)))
+    return w, np.stack([dw[:n], dw[n:]], axis=1)
+
+
+def status_equation(rho, type_=None, params: Optional[Params] = None):
+    """Array form of status_equation.f90 on the GPU: returns (p, c)."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    n = len(rho)
+    t = np.full(n, 2, dtype=np.int32) if type_ is None else np.ascontiguousarray(type_, dtype=np.int32)
+    p, c = np.zeros(n), np.zeros(n)
+    e = np.zeros(n)
+    _check(lib().sph2d_status_equation_n(ctypes.byref(params) if params is not None else None, n,
+                                         _dp(rho), _dp(e), _ip(t), _dp(p), _dp(c)))
+    return p, c

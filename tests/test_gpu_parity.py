@@ -1,0 +1,267 @@
+"""GPU parity of the HIP step (through the C-ABI) against the oracle.
+
+Bars: integer/index work (ghost generation, local order, pair set, flags) is
+bit-exact; FP64 fields agree with the oracle within 1e-12 relative to the
+field's largest magnitude (the gather sums run in another order than the
+reference's scatter sums, and exp() differs in the last bit); the Tait EOS,
+which has no sum and no exp, is bit-exact.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import sph2d
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tgv_reference.npz")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(g, r):
+    scale = np.max(np.abs(r))
+    if scale == 0:
+        return float(np.max(np.abs(g)))
+    return float(np.max(np.abs(g - r)) / scale)
+
+
+def pair(n, integrator=2, state=None, **kw):
+    """GPU simulation and oracle on the same TGV lattice (optionally a perturbed state)."""
+    prm = sph2d.tgv_params(n, integrator=integrator, **kw)
+    sim = sph2d.Simulation(prm)
+    o = Oracle(nx=n, ny=n, integrator=integrator, dt=prm.dt, **{k: v for k, v in kw.items() if k in ("shift_on", "mls_on", "solver")})
+    o.init_tgv()
+    if state is None:
+        sim.init_tgv()
+    else:
+        sim.upload_state(*state)
+        o.set_fluid_state(*state)
+    return sim, o
+
+
+def perturbed_tgv(n, amp=0.25, seed=0):
+    o = Oracle(nx=n, ny=n)
+    o.init_tgv()
+    N = n * n
+    rng = np.random.default_rng(seed)
+    dx = 1.0 / n
+    x = o.f64("g_x")[:N] + rng.uniform(-amp, amp, N) * dx
+    y = o.f64("g_y")[:N] + rng.uniform(-amp, amp, N) * dx
+    u = o.f64("g_u")[:N] + rng.normal(0, 0.05, N)
+    v = o.f64("g_v")[:N] + rng.normal(0, 0.05, N)
+    rho = o.f64("g_rho")[:N] * (1 + rng.normal(0, 1e-3, N))
+    m = o.f64("g_mass")[:N]
+    return x, y, u, v, rho, m
+
+
+def evaluate_both(sim, o, it=1):
+    sim.set_debug(True)
+    sim.distribute_period_boundary()
+    sim.devide_block(it)
+    sim.single_step_compute(it)
+    o.period_boundary()
+    o.devide_block(it)
+    o.single_step_compute(it)
+
+
+def test_array_kernel_and_eos():
+    say("test_array_kernel_and_eos")
+    rng = np.random.default_rng(2)
+    h = np.full(5000, 0.012)
+    r = rng.uniform(1e-6, 0.037, 5000)
+    th = rng.uniform(0, 2 * np.pi, 5000)
+    dx = np.stack([r * np.cos(th), r * np.sin(th)], axis=1)
+    r = np.sqrt(dx[:, 0] ** 2 + dx[:, 1] ** 2)
+    w, dw = sph2d.compute_kernel(r, dx, h)
+    from oracle import pyoracle as po
+    for k in range(0, 5000, 37):
+        w0, d0 = po.compute_kernel(r[k], dx[k], h[k])
+        assert abs(w[k] - w0) <= 4e-16 * max(abs(w0), 1e-300) + 1e-300
+        assert np.all(np.abs(dw[k] - d0) <= 4e-16 * np.abs(d0) + 1e-300)
+    rho = 1000 * (1 + rng.uniform(-0.01, 0.01, 4000))
+    p, c = sph2d.status_equation(rho)
+    o = Oracle()
+    import ctypes
+    for k in range(0, 4000, 13):
+        pp, cc = ctypes.c_double(), ctypes.c_double()
+        po.lib().orc_status_equation(o._c, rho[k], 0.0, 2, ctypes.byref(pp), ctypes.byref(cc))
+        assert p[k] == pp.value and c[k] == cc.value
+
+
+def test_initial_condition_and_ghosts():
+    say("test_initial_condition_and_ghosts")
+    sim, o = pair(100)
+    g = sim.download_state()
+    assert rel_err(g["x"], o.f64("g_x")[:10000]) == 0.0
+    for f, of in (("u", "g_u"), ("v", "g_v"), ("rho", "g_rho"), ("mass", "g_mass")):
+        assert rel_err(g[f], o.f64(of)[:10000]) < 1e-14, f
+    sim.distribute_period_boundary()
+    sim.devide_block(1)
+    o.period_boundary()
+    o.devide_block(1)
+    d = sim.diagnostics()
+    assert d.n_ghost == o.n_global - o.n_fluid == 1664
+    assert d.n_local == o.n_local
+    assert np.array_equal(sim.local("name"), o.i32("name"))
+    assert np.array_equal(sim.local("type"), o.i32("type"))
+
+
+@pytest.mark.parametrize("n,perturb", [(100, False), (64, True), (37, True)])
+def test_pair_set_bit_exact(n, perturb):
+    say(f"test_pair_set_bit_exact n={n} perturb={perturb}")
+    st = perturbed_tgv(n) if perturb else None
+    sim, o = pair(n, state=st)
+    evaluate_both(sim, o)
+    assert np.array_equal(sim.pairs(), o.pairs_gid())
+
+
+FIELDS = ("p", "c", "volume", "fra_v", "sum_dwdx_x", "sum_dwdx_y", "tau_xx", "tau_xy", "tau_yy",
+          "grad_rho_x", "grad_rho_y", "drhodt", "ax", "ay", "internal_x", "internal_y", "visc_x",
+          "visc_y", "art_internal_x", "art_internal_y", "sps_x", "sps_y", "lap_x", "lap_y")
+
+
+@pytest.mark.parametrize("n,perturb,solver", [(100, False, 0), (64, True, 0), (64, True, 1)])
+def test_single_step_compute_parity(n, perturb, solver):
+    say(f"test_single_step_compute_parity n={n} perturb={perturb} solver={solver}")
+    st = perturbed_tgv(n, seed=n) if perturb else None
+    sim, o = pair(n, state=st, solver=solver)
+    evaluate_both(sim, o)
+    assert np.array_equal(sim.local("name"), o.i32("name"))
+    for f in ("surface", "near_surface"):
+        assert np.array_equal(sim.local(f), o.i32(f)), f
+    oname = {"volume": "volume", "sum_dwdx_x": "sum_dwdx_x"}
+    worst = {}
+    for f in FIELDS:
+        g, r = sim.local(f), o.f64(oname.get(f, f))
+        worst[f] = rel_err(g, r)
+    say(worst)
+    for f, e in worst.items():
+        assert e < TOL, (f, e)
+    # EOS has no sum: bit-exact
+    assert np.array_equal(sim.local("p"), o.f64("p"))
+
+
+def test_shift_sums_parity():
+    say("test_shift_sums_parity")
+    sim, o = pair(64, state=perturbed_tgv(64, seed=5))
+    evaluate_both(sim, o)
+    o.shift()  # fills shift_x/y from the same pairs
+    for f in ("shift_x", "shift_y"):
+        assert rel_err(sim.local(f), o.f64(f)) < TOL, f
+
+
+@pytest.mark.parametrize("integrator,steps", [(2, 25), (3, 21)])
+def test_multi_step_parity(integrator, steps):
+    """Whole steps (ghosts, ordering, stale-index re-use, stages, MLS every 20, shift,
+    refresh); differences grow from the last bit, so the bar is 1e-10 after 20+ steps."""
+    say(f"test_multi_step_parity integrator={integrator} steps={steps}")
+    n = 48
+    sim, o = pair(n, integrator=integrator)
+    for it in range(1, steps + 1):
+        sim.time_step(it)
+        o.time_step(it)
+    g = sim.download_state()
+    N = n * n
+    for f, of in (("x", "g_x"), ("y", "g_y"), ("u", "g_u"), ("v", "g_v"), ("rho", "g_rho"), ("p", "g_p")):
+        r = o.f64(of)[:N]
+        if f in ("x", "y"):
+            e = float(np.max(np.abs(g[f] - r)))
+            assert e < 1e-12, (f, e)
+        else:
+            e = rel_err(g[f], r)
+            assert e < 1e-10, (f, e)
+    assert np.allclose(sim.sample()[:2], o.sample()[:2], rtol=1e-11)
+
+
+def test_free_surface_flags_bit_exact():
+    """A fluid patch with an open boundary: surface, geometric un-flagging and
+    near-surface flags must match the oracle exactly."""
+    say("test_free_surface_flags_bit_exact")
+    n = 60
+    o0 = Oracle(nx=n, ny=n)
+    o0.init_tgv()
+    N = n * n
+    x, y = o0.f64("g_x")[:N], o0.f64("g_y")[:N]
+    keep = ((x - 0.5) ** 2 + (y - 0.5) ** 2) < 0.3 ** 2
+    rng = np.random.default_rng(9)
+    xs = x[keep] + rng.uniform(-0.1, 0.1, keep.sum()) / n
+    ys = y[keep] + rng.uniform(-0.1, 0.1, keep.sum()) / n
+    u = o0.f64("g_u")[:N][keep]
+    v = o0.f64("g_v")[:N][keep]
+    rho = o0.f64("g_rho")[:N][keep]
+    m = o0.f64("g_mass")[:N][keep]
+    st = (xs, ys, u, v, rho, m)
+    prm = sph2d.tgv_params(n, integrator=2)
+    sim = sph2d.Simulation(prm, capacity=N)
+    sim.upload_state(*st)
+    o = Oracle(nx=n, ny=n, integrator=2)
+    o.set_fluid_state(*st)
+    evaluate_both(sim, o)
+    s_o = o.i32("surface")
+    assert s_o.sum() > 50
+    assert np.array_equal(sim.local("surface"), s_o)
+    assert np.array_equal(sim.local("near_surface"), o.i32("near_surface"))
+    for f in ("drhodt", "ax", "ay", "fra_v"):
+        assert rel_err(sim.local(f), o.f64(f)) < TOL, f
+
+
+def test_tgv_100_to_t05_matches_reference_run():
+    """The default case (100^2, RK-variant, dt=1e-4) to t=0.5: KE/KE0 and the L2
+    norms against the reference's shipped 5-rank run (Kinetic_energy.dat,
+    Norm_*.dat) and the analytic decay exp(-16 pi^2 nu t)."""
+    say("test_tgv_100_to_t05_matches_reference_run")
+    gold = np.load(GOLD)
+    prm = sph2d.tgv_params(100, integrator=3, dt=1e-4)
+    sim = sph2d.Simulation(prm)
+    sim.init_tgv()
+    rows = []
+    for block in range(5):
+        sim.run(1 + block * 1000, 1000)
+        s = sim.sample()
+        rows.append(s)
+        say(f"t={0.1 * (block + 1):.1f} KE={s[0]:.8f} L2u={s[4]:.6f} L2p={s[10]:.6f}")
+    for k, s in enumerate(rows):
+        t = 0.1 * (k + 1)
+        i = int(round(t / 0.01)) - 1
+        assert gold["ke"][i, 0] == pytest.approx(t)
+        ke_ref = gold["ke"][i, 1]
+        assert abs(s[0] - ke_ref) / ke_ref < 2e-3, (t, s[0], ke_ref)
+        ke_an = np.exp(-16 * np.pi ** 2 * prm.nu * t)
+        assert abs(s[0] - ke_an) / ke_an < 0.06
+        assert s[4] == pytest.approx(gold["norm_u"][i, 3], rel=0.15)
+        assert s[7] == pytest.approx(gold["norm_v"][i, 3], rel=0.15)
+        assert s[10] == pytest.approx(gold["norm_p"][i, 3], rel=0.15)
+    d = sim.diagnostics()
+    assert d.rebin_needed == 0 and d.error_flags == 0 and d.n_surface == 0
+
+
+def test_full_size_2048_pair_count_and_step():
+    """BASELINE's 4.2M-particle case: ghost count, local count and the exact
+    lattice pair count (18 N + G/2) at full size, then one PC step stays finite."""
+    say("test_full_size_2048_pair_count_and_step")
+    from test_oracle_physics import lattice_pair_count
+    n = 2048
+    prm = sph2d.tgv_params(n, integrator=2)
+    sim = sph2d.Simulation(prm)
+    sim.init_tgv()
+    sim.distribute_period_boundary()
+    sim.devide_block(1)
+    sim.single_step_compute(1)
+    d = sim.diagnostics()
+    assert d.n_ghost == 32832 and d.n_local == n * n + 32832
+    import ctypes
+    cnt = ctypes.c_int64()
+    rc = sph2d.lib().sph2d_download_pairs(sim._c, 0, None, None, ctypes.byref(cnt))
+    assert rc == 0
+    assert cnt.value == lattice_pair_count(n)
+    sim.time_step(1)
+    s = sim.sample()
+    assert np.all(np.isfinite(s)) and abs(s[0] - 1.0) < 1e-3
