@@ -59,7 +59,7 @@ def lib():
         L.orc_create.restype = ctypes.c_void_p
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         for name in ("orc_init_tgv", "orc_period_boundary", "orc_reinitialize_density",
-                     "orc_shift", "orc_refresh"):
+                     "orc_shift", "orc_shift_sums", "orc_refresh"):
             getattr(L, name).argtypes = [ctypes.c_void_p]
         for name in ("orc_devide_block", "orc_single_step_compute", "orc_time_step"):
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_int32]
@@ -151,6 +151,9 @@ class Oracle:
 
     def shift(self):
         lib().orc_shift(self._c)
+
+    def shift_sums(self):
+        lib().orc_shift_sums(self._c)
 
     def refresh(self):
         lib().orc_refresh(self._c)

@@ -979,7 +979,7 @@ void orc_reinitialize_density(orc_ctx* c) {
 /* ------------------------------------------------------------------ */
 /* Shift_particle_position.f90:56-136 (pairs of the last evaluation)   */
 /* ------------------------------------------------------------------ */
-void orc_shift(orc_ctx* c) {
+void orc_shift_sums(orc_ctx* c) {
   const orc_params* P = &c->P;
   int n = c->n_local;
   double pd0[2] = {0.0, 0.0}, wdx, dwd[2];
@@ -993,6 +993,12 @@ void orc_shift(orc_ctx* c) {
     c->shy[i] = c->shy[i] - f * c->ndiy[k] * c->vol[j];
     c->shy[j] = c->shy[j] + f * c->ndjy[k] * c->vol[i];
   }
+}
+
+void orc_shift(orc_ctx* c) {
+  const orc_params* P = &c->P;
+  int n = c->n_local;
+  orc_shift_sums(c);
   for (int i = 0; i < n; ++i) {
     if (c->type[i] != 2) continue;
     if (c->near[i] == 1) continue;

@@ -71,6 +71,7 @@ void orc_single_step_compute(orc_ctx* c, int32_t i_time_step);
 void orc_time_step(orc_ctx* c, int32_t i_time_step);        /* one full step of time_iteration */
 void orc_reinitialize_density(orc_ctx* c);                  /* Reinitialize_particle_density */
 void orc_shift(orc_ctx* c);                                 /* Shift_particle_position */
+void orc_shift_sums(orc_ctx* c);                            /* its pair sums only (:78-103) */
 void orc_refresh(orc_ctx* c);                               /* Refresh_domain (1 rank) */
 
 /* Sampling_subroutine.f90:340-461. out[0]=KE, [1]=max|v|, [2..4]=u Linf,L1,L2,

@@ -102,6 +102,12 @@ void make_devparams(const sph2d_params& p, DevParams& P) {
   }
   P.mls_alpha = p.mls_alpha;
   P.surf_lim = (double)1.414f * p.h;
+  P.inv_h = 1.0 / p.h;
+  P.inv_c0 = 1.0 / std::sqrt(p.c0sq);
+  P.inv_sv = 1.0 / P.support_volume;
+  P.inv_wdx = 1.0 / P.w_dx;
+  // any pair with r <= 3h has r^2 below this bound, so the exact test r <= 3h stays decisive
+  P.cut2 = P.three_h * P.three_h * (1.0 + 1e-10);
   P.solver = p.solver;
   P.mls_order = p.mls_order;
   P.write_debug = 0;

@@ -1,0 +1,309 @@
+// sph2d_eval64.hip -- one force evaluation of single_step_compute.f90 in FP64.
+//
+// Gather form: every particle sums over its own neighbours, so no pair list is
+// stored and no float atomics are used. Each sum runs in a fixed order
+// (neighbour columns cx-1, cx, cx+1, rows ascending, particles ascending), so the
+// results are deterministic run to run.
+//
+// Pair terms are written for the summing particle s and its neighbour o with
+// P = x_s - x_o and D = dW(P). The reference evaluates every pair once for
+// i < j and scatters to both sides; its j-side formulas with x_ij = -P and
+// dwdx = -D reduce to the i-side formulas written here (the two sign flips
+// cancel in every term, and the Riemann star state v*, P* is the same vector
+// from either side), so each term equals the reference's up to rounding.
+// Built with -ffp-contract=off; reciprocals replace repeated divisions.
+#include "sph2d_internal.h"
+#include "sph2d_device.h"
+
+namespace sph2d {
+
+static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
+
+// Kernel gradient and value from the neighbour offset (compute_kernel.f90:123-129).
+struct KernelTerm {
+  double w, dwx, dwy;
+};
+
+__device__ __forceinline__ KernelTerm kernel_term(const DevParams& P, double dx, double dy, double r,
+                                                  double inv_r) {
+  KernelTerm k;
+  const double q = r * P.inv_h;
+  const double eq = exp(-(q * q));
+  k.w = (eq - P.e9) * P.kfactor;
+  const double t = eq * (-2.0 * q) * (inv_r * P.inv_h) * P.kfactor;
+  k.dwx = t * dx;
+  k.dwy = t * dy;
+  return k;
+}
+
+// ---------------------------------------------------------------------------
+// pass A: volume fraction, sum dW, SPS strain -> tau, grad rho
+// Define_freesurface_and_Normailized_derivates.f90:79-131,141-159 and
+// Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90:144-296,508-526
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  const double2 ps = B.l_pos[s];
+  const double2 vs = B.l_vel[s];
+  const double rhos = B.l_rv[s].x;
+  const int ts = B.l_type[s];
+  double fra = 0.0, sdx = 0.0, sdy = 0.0, Sxx = 0.0, Sxy = 0.0, Syy = 0.0, grx = 0.0, gry = 0.0;
+  for_each_neighbour(P, B.cell_start, B.l_pos, s, ps, B.l_cell[s], [&](int o) {
+    if (ts != 2 && B.l_type[o] != 2) return;     // no pair between two non-fluid particles
+    const double2 po = B.l_pos[o];
+    const double dx = ps.x - po.x, dy = ps.y - po.y;
+    const double r = sqrt(dx * dx + dy * dy);
+    const KernelTerm K = kernel_term(P, dx, dy, r, 1.0 / r);
+    const double2 rvo = B.l_rv[o];
+    const double2 vo = B.l_vel[o];
+    const double Vo = rvo.y;
+    fra = fra + Vo * P.inv_sv;
+    sdx = sdx + K.dwx * Vo;
+    sdy = sdy + K.dwy * Vo;
+    const double vdx = vo.x - vs.x, vdy = vo.y - vs.y;   // -(v_i - v_j); identity renormalisation
+    Sxx = Sxx + vdx * K.dwx * Vo;
+    Sxy = Sxy + (vdx * K.dwy + vdy * K.dwx) * Vo;
+    Syy = Syy + vdy * K.dwy * Vo;
+    const double rd = (rvo.x - rhos) * Vo;
+    grx = grx + rd * K.dwx;
+    gry = gry + rd * K.dwy;
+  });
+  B.fra_v[s] = fra;
+  B.sdw[s] = make_double2(sdx, sdy);
+  B.grad[s] = make_double2(grx, gry);
+  double txx = 0.0, txy = 0.0, tyy = 0.0;
+  if (ts == 2) {                                   // Solver...SPS.f90:256-296
+    const double master = Sxx * Sxx + Syy * Syy + 0.0;
+    const double last = Sxy * Sxy + 0.0 + 0.0;
+    const double abs2 = 2 * master + last;
+    const double t0 = P.sps_c012 * P.sps_dl;
+    const double nut = (t0 * t0) * sqrt(abs2);
+    const double kk = P.sps_c23 * nut * (Sxx + Syy);
+    const double blin = P.sps_c23b * (P.sps_dl * P.sps_dl) * abs2;
+    txx = (2 * nut * Sxx - kk - blin) * rhos;
+    txy = (nut * Sxy) * rhos;
+    tyy = (2 * nut * Syy - kk - blin) * rhos;
+  }
+  B.tau[3 * s + 0] = txx;
+  B.tau[3 * s + 1] = txy;
+  B.tau[3 * s + 2] = tyy;
+  const int sf = (ts == 2 && fra <= P.fra_limit) ? 1 : 0;
+  B.surf[s] = sf;
+  B.near_[s] = 0;
+  if (sf) atomicAdd(&B.counters[C_NSURF], 1);
+}
+
+// Surface normal and geometric test (Define_freesurface...f90:295-376), surface particles only.
+__global__ void __launch_bounds__(kBlock) k_eval_b1(DevParams P, Buffers B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  if (B.surf[s] != 1) { B.nrm[s] = make_double2(0.0, 0.0); return; }
+  const double2 sd = B.sdw[s];
+  const double n1 = 1.0 * sd.x + 0.0 * sd.y;     // identity inverse: matmul(I, sum_dwdx)
+  const double n2 = 0.0 * sd.x + 1.0 * sd.y;
+  double len = 0.0;
+  len = len + n1 * n1;
+  len = len + n2 * n2;
+  len = sqrt(len);
+  bool keep = true;
+  double nx = 0.0, ny = 0.0;
+  if (isnan(len) || len == 0.0) {
+    keep = false;
+  } else {
+    nx = -n1 / len;
+    ny = -n2 / len;
+    const double2 ps = B.l_pos[s];
+    const double vx = ps.x + P.h * nx, vy = ps.y + P.h * ny;
+    const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
+    const double pi = 3.14159265358979324;
+    for (int d = 0; d < 3 && keep; ++d)
+      for (int o = R.lo[d]; o < R.hi[d]; ++o) {
+        if (B.l_type[o] != 2) continue;
+        const double2 po = B.l_pos[o];
+        const double pdx = po.x - ps.x, pdy = po.y - ps.y;
+        const double JI = sqrt(pdx * pdx + pdy * pdy);
+        const double Nd = nx * pdx + ny * pdy;
+        const double qx = po.x - vx, qy = po.y - vy;
+        const double JT = sqrt(qx * qx + qy * qy);
+        if (JI >= P.surf_lim && JT < P.h) { keep = false; break; }
+        if (JI < P.surf_lim && acos(Nd / JI) <= (pi / 4.0)) { keep = false; break; }
+      }
+  }
+  B.nrm[s] = make_double2(nx, ny);
+  if (!keep) {
+    B.surf[s] = 0;
+    atomicSub(&B.counters[C_NSURF], 1);
+  }
+}
+
+// Near-surface flag (Define_freesurface...f90:418-440): any pair partner is a surface particle.
+__global__ void __launch_bounds__(kBlock) k_eval_b2(DevParams P, Buffers B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  if (B.counters[C_NSURF] == 0) return;            // near_ was zeroed by pass A
+  const int ts = B.l_type[s];
+  int nr = 0;
+  for_each_neighbour(P, B.cell_start, B.l_pos, s, B.l_pos[s], B.l_cell[s], [&](int o) {
+    if (B.surf[o] == 1 && (ts == 2 || B.l_type[o] == 2)) nr = 1;
+  });
+  B.near_[s] = nr;
+}
+
+// ---------------------------------------------------------------------------
+// pass C: Riemann continuity/momentum, artificial viscosity, laplacian,
+// delta-SPH internal, SPS, XSPH, delta-SPH density correction, shift sums.
+// Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90:300-635,
+// Solver_Normalized_Linear_Riemann_Delta_SPS.f90:362-371,
+// single_step_compute.f90:152-171, Shift_particle_position.f90:78-103.
+// DEBUG keeps the five acceleration components apart and writes them.
+// ---------------------------------------------------------------------------
+template <bool DEBUG>
+__global__ void __launch_bounds__(kBlock) k_eval_c(DevParams P, Buffers B, int do_shift) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  const double2 ps = B.l_pos[s];
+  const double2 vs = B.l_vel[s];
+  const double rhos = B.l_rv[s].x;
+  const double2 pcs = B.l_pc[s];
+  const int ts = B.l_type[s];
+  const double txs = B.tau[3 * s], tys = B.tau[3 * s + 1], tzs = B.tau[3 * s + 2];
+  const double2 grs = B.grad[s];
+  const int nes = B.near_[s];
+  const double inv_rhos = 1.0 / rhos;
+  const bool linear = P.solver == SPH2D_SOLVER_LINEAR_RIEMANN;
+  double drho = 0.0, corr = 0.0, ax = 0.0, ay = 0.0, shx = 0.0, shy = 0.0, agx = 0.0, agy = 0.0;
+  double iax = 0.0, iay = 0.0, avx = 0.0, avy = 0.0, aix = 0.0, aiy = 0.0;
+  double spx = 0.0, spy = 0.0, lpx = 0.0, lpy = 0.0;
+  for_each_neighbour(P, B.cell_start, B.l_pos, s, ps, B.l_cell[s], [&](int o) {
+    if (ts != 2 && B.l_type[o] != 2) return;
+    const double2 po = B.l_pos[o];
+    const double dx = ps.x - po.x, dy = ps.y - po.y;      // x_ij with i = s
+    const double r2 = dx * dx + dy * dy;
+    const double r = sqrt(r2);
+    const double inv_r = 1.0 / r;
+    const double inv_r2 = inv_r * inv_r;
+    const KernelTerm K = kernel_term(P, dx, dy, r, inv_r);
+    const double2 vo = B.l_vel[o];
+    const double2 rvo = B.l_rv[o];
+    const double2 pco = B.l_pc[o];
+    const double rhoo = rvo.x, Vo = rvo.y;
+    // Riemann problem along e = -x_ij/r (Solver...:306-356)
+    const double ex = -(dx * inv_r), ey = -(dy * inv_r);
+    const double UL = vs.x * ex + vs.y * ey;
+    const double UR = vo.x * ex + vo.y * ey;
+    const double dU = UL - UR;
+    const double Pbar = (pcs.x + pco.x) * 0.5;
+    const double vbx = (vs.x + vo.x) * 0.5, vby = (vs.y + vo.y) * 0.5;
+    const double cbar = (pcs.y + pco.y) * 0.5;
+    const double rbar = (rhos + rhoo) * 0.5;
+    const double inv_rbar = 1.0 / rbar;
+    double vsx, vsy, Ps;
+    if (linear) {
+      const double zs = rhos * pcs.y, zo = rhoo * pco.y;
+      const double inv_z = 1.0 / (zs + zo);
+      const double Us = (zs * UR + zo * UL + pcs.x - pco.x) * inv_z;
+      vsx = Us * ex;
+      vsy = Us * ey;
+      Ps = (zs * pco.x + zo * pcs.x + zs * zo * dU) * inv_z;
+    } else {
+      const double Ub = (UL + UR) * 0.5;
+      const double Us = Ub + 0.5 * (pcs.x - pco.x) * inv_rbar * P.inv_c0;
+      vsx = (Us - Ub) * ex + vbx;
+      vsy = (Us - Ub) * ey + vby;
+      double bR = 3.0 * (dU > 0.0 ? dU : 0.0);
+      bR = bR < cbar ? bR : cbar;
+      Ps = Pbar + 0.5 * bR * rbar * dU;
+    }
+    if (nes) { vsx = vbx; vsy = vby; Ps = Pbar; }  // near the free surface: averages
+    const double ti = (vs.x - vsx) * K.dwx + (vs.y - vsy) * K.dwy;
+    drho = drho + 2 * rhos * Vo * ti;                        // :360-369
+    const double rij = inv_rhos / rhoo;                      // 1/(rho_i rho_j)
+    const double mo = B.l_mass[o];
+    const double fp = mo * (-2 * Ps * rij);                  // :374-388
+    double tix = fp * K.dwx, tiy = fp * K.dwy;
+    const double vijx = vs.x - vo.x, vijy = vs.y - vo.y;
+    const double vx = vijx * dx + vijy * dy;
+    double tvx = 0.0, tvy = 0.0;
+    if (vx < 0.0) {                                          // :392-417
+      const double fh = P.fai * P.h;
+      const double phi = P.h * vx / (r2 + fh * fh);
+      const double Pi = (-(P.alpha * cbar * phi) + P.beta * (phi * phi)) * inv_rbar;
+      tvx = mo * (-Pi * K.dwx);
+      tvy = mo * (-Pi * K.dwy);
+    }
+    const double Rd = dx * K.dwx + dy * K.dwy;
+    const double lf = Vo * 2 * P.nu * Rd * inv_r2;           // :432-436
+    const double tlx = lf * vijx, tly = lf * vijy;
+    const double tmp = P.delta_alpha * P.h * cbar * vx * inv_r2 * P.rho0 * inv_rhos * Vo;   // :442-446
+    const double tax = tmp * K.dwx, tay = tmp * K.dwy;
+    // SPS (:450-492): raw dwdx, particle_mass at the LOCAL index, temp_p carried from x to y
+    const double gmo = B.g_mass[o];
+    const double txo = B.tau[3 * o], tyo = B.tau[3 * o + 1], tzo = B.tau[3 * o + 2];
+    double tp = (txs + txo) * K.dwx + (tys + tyo) * K.dwy;
+    tp = tp * rij;
+    const double tsx = gmo * tp;
+    tp = tp + (tzs + tzo) * K.dwy + (tys + tyo) * K.dwx;
+    tp = tp * rij;
+    const double tsy = gmo * tp;
+    if (DEBUG) {
+      iax += tix; iay += tiy; avx += tvx; avy += tvy; aix += tax; aiy += tay;
+      spx += tsx; spy += tsy; lpx += tlx; lpy += tly;
+    } else {
+      ax = ax + ((((tix + tvx) + tax) + tsx) + tlx);
+      ay = ay + ((((tiy + tvy) + tay) + tsy) + tly);
+    }
+    if (P.epsilon_xsph != 0.0 && !linear) {                  // :495-500
+      const double f = P.epsilon_xsph * 2 * mo / (rhos + rhoo) * K.w;
+      agx = agx - f * vijx;
+      agy = agy - f * vijy;
+    }
+    {  // delta-SPH density diffusion (:552-583) with x_j - x_i = -x_ij
+      const double2 gro = B.grad[o];
+      const double psi = (rhoo - rhos) + 0.5 * ((grs.x + gro.x) * dx + (grs.y + gro.y) * dy);
+      corr = corr - 2 * P.delta * P.h * cbar * psi * Rd * inv_r2 * Vo;
+    }
+    if (do_shift) {                                           // Shift_particle_position.f90:78-103
+      const double wr = K.w * P.inv_wdx;
+      const double f = P.shift_beta * P.h2 * (1 + P.shift_R * (wr * wr * wr * wr)) * Vo;
+      shx = shx - f * K.dwx;
+      shy = shy - f * K.dwy;
+    }
+  });
+  if (ts == 2) drho = drho + corr;
+  B.drhodt[s] = drho;
+  if (DEBUG) {
+    ax = iax + 0.0 + avx + aix + spx + lpx;
+    ay = iay + 0.0 + avy + aiy + spy + lpy;
+    double* d = B.dbg + 10 * s;
+    d[0] = iax; d[1] = iay; d[2] = avx; d[3] = avy; d[4] = aix; d[5] = aiy;
+    d[6] = spx; d[7] = spy; d[8] = lpx; d[9] = lpy;
+  }
+  if (ts != 2) { ax = 0.0; ay = 0.0; }
+  B.acc[s] = make_double2(ax, ay);
+  if (P.epsilon_xsph != 0.0) B.avg[s] = make_double2(agx, agy);
+  if (do_shift) B.shift[s] = make_double2(shx, shy);
+}
+
+void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
+  const int g = grid_for(cap_local);
+  launch_fill_i32(s, B.counters + C_NSURF, 1, 0);
+  hipLaunchKernelGGL(k_eval_a, dim3(g), dim3(kBlock), 0, s, P, B);
+  hipLaunchKernelGGL(k_eval_b1, dim3(g), dim3(kBlock), 0, s, P, B);
+  hipLaunchKernelGGL(k_eval_b2, dim3(g), dim3(kBlock), 0, s, P, B);
+  if (P.write_debug) hipLaunchKernelGGL(k_eval_c<true>, dim3(g), dim3(kBlock), 0, s, P, B, 1);
+  else hipLaunchKernelGGL(k_eval_c<false>, dim3(g), dim3(kBlock), 0, s, P, B, 1);
+}
+
+void launch_eval_pass(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int pass) {
+  const int g = grid_for(cap_local);
+  if (pass == 0) hipLaunchKernelGGL(k_eval_c<false>, dim3(g), dim3(kBlock), 0, s, P, B, 1);
+  else if (pass == 1) hipLaunchKernelGGL(k_eval_a, dim3(g), dim3(kBlock), 0, s, P, B);
+  else launch_eval(s, P, B, cap_local);
+}
+
+}  // namespace sph2d

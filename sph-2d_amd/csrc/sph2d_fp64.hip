@@ -361,346 +361,6 @@ void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_
 }
 
 // ---------------------------------------------------------------------------
-// single_step_compute: pass A (volume fraction, sum dW, SPS strain -> tau, grad rho)
-// Define_freesurface_and_Normailized_derivates.f90:79-131,141-159 and
-// Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90:144-296,508-526
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = B.counters[C_NLOCAL];
-  if (s >= n) return;
-  const double2 ps = B.l_pos[s];
-  const double2 vs = B.l_vel[s];
-  const double2 rvs = B.l_rv[s];
-  const int ts = B.l_type[s];
-  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
-  const double cut2 = P.three_h * P.three_h * (1.0 + 1e-10);
-  double fra = 0.0, sdx = 0.0, sdy = 0.0, Sxx = 0.0, Sxy = 0.0, Syy = 0.0, grx = 0.0, gry = 0.0;
-#pragma unroll 1
-  for (int d = 0; d < 3; ++d) {
-#pragma unroll 1
-    for (int o = R.lo[d]; o < R.hi[d]; ++o) {
-      if (o == s) continue;
-      const double2 po = B.l_pos[o];
-      const bool sa = s < o;                       // self is the pair's i (lower local index)
-      const double pdx = sa ? ps.x - po.x : po.x - ps.x;   // position_difference = x_a - x_b
-      const double pdy = sa ? ps.y - po.y : po.y - ps.y;
-      const double r2 = pdx * pdx + pdy * pdy;
-      if (r2 > cut2) continue;
-      const double r = sqrt(r2);
-      if (!(r <= P.three_h && r >= P.r_min)) continue;
-      const int to = B.l_type[o];
-      if (ts != 2 && to != 2) continue;
-      double w, dwx, dwy;
-      kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
-      const double2 rvo = B.l_rv[o];
-      const double2 vo = B.l_vel[o];
-      const double Vo = rvo.y;
-      fra = fra + Vo / P.support_volume;
-      if (sa) { sdx = sdx + dwx * Vo; sdy = sdy + dwy * Vo; }
-      else { sdx = sdx - dwx * Vo; sdy = sdy - dwy * Vo; }
-      // velocity_difference = -(v_a - v_b); identity renormalisation: nd = dwdx
-      const double vdx = sa ? -(vs.x - vo.x) : -(vo.x - vs.x);
-      const double vdy = sa ? -(vs.y - vo.y) : -(vo.y - vs.y);
-      Sxx = Sxx + vdx * dwx * Vo;
-      Sxy = Sxy + (vdx * dwy + vdy * dwx) * Vo;
-      Syy = Syy + vdy * dwy * Vo;
-      const double rd = sa ? -(rvs.x - rvo.x) : -(rvo.x - rvs.x);   // -(rho_a - rho_b)
-      grx = grx + rd * Vo * dwx;
-      gry = gry + rd * Vo * dwy;
-    }
-  }
-  B.fra_v[s] = fra;
-  B.sdw[s] = make_double2(sdx, sdy);
-  B.grad[s] = make_double2(grx, gry);
-  double txx = 0.0, txy = 0.0, tyy = 0.0;
-  if (ts == 2) {                                   // Solver...SPS.f90:256-296
-    const double master = Sxx * Sxx + Syy * Syy + 0.0;
-    const double last = Sxy * Sxy + 0.0 + 0.0;
-    const double abs2 = 2 * master + last;
-    const double t0 = P.sps_c012 * P.sps_dl;
-    const double nut = (t0 * t0) * sqrt(abs2);
-    const double kk = P.sps_c23 * nut * (Sxx + Syy);
-    const double blin = P.sps_c23b * (P.sps_dl * P.sps_dl) * abs2;
-    txx = (2 * nut * Sxx - kk - blin) * rvs.x;
-    txy = (nut * Sxy) * rvs.x;
-    tyy = (2 * nut * Syy - kk - blin) * rvs.x;
-  }
-  B.tau[3 * s + 0] = txx;
-  B.tau[3 * s + 1] = txy;
-  B.tau[3 * s + 2] = tyy;
-  const int sf = (ts == 2 && fra <= P.fra_limit) ? 1 : 0;
-  B.surf[s] = sf;
-  B.near_[s] = 0;
-  if (sf) atomicAdd(&B.counters[C_NSURF], 1);
-}
-
-// Surface normal and geometric test (Define_freesurface...f90:295-376), surface particles only.
-__global__ void __launch_bounds__(kBlock) k_eval_b1(DevParams P, Buffers B) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = B.counters[C_NLOCAL];
-  if (s >= n) return;
-  if (B.surf[s] != 1) { B.nrm[s] = make_double2(0.0, 0.0); return; }
-  const double2 sd = B.sdw[s];
-  // identity inverse: n = matmul(I, sum_dwdx)
-  const double n1 = 1.0 * sd.x + 0.0 * sd.y;
-  const double n2 = 0.0 * sd.x + 1.0 * sd.y;
-  double len = 0.0;
-  len = len + n1 * n1;
-  len = len + n2 * n2;
-  len = sqrt(len);
-  bool keep = true;
-  double nx = 0.0, ny = 0.0;
-  if (isnan(len) || len == 0.0) {
-    keep = false;
-  } else {
-    nx = -n1 / len;
-    ny = -n2 / len;
-    const double2 ps = B.l_pos[s];
-    const double vx = ps.x + P.h * nx, vy = ps.y + P.h * ny;
-    const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
-    const double pi = 3.14159265358979324;
-    for (int d = 0; d < 3 && keep; ++d)
-      for (int o = R.lo[d]; o < R.hi[d]; ++o) {
-        if (B.l_type[o] != 2) continue;
-        const double2 po = B.l_pos[o];
-        const double pdx = po.x - ps.x, pdy = po.y - ps.y;
-        const double JI = sqrt(pdx * pdx + pdy * pdy);
-        const double Nd = nx * pdx + ny * pdy;
-        const double qx = po.x - vx, qy = po.y - vy;
-        const double JT = sqrt(qx * qx + qy * qy);
-        if (JI >= P.surf_lim && JT < P.h) { keep = false; break; }
-        if (JI < P.surf_lim && acos(Nd / JI) <= (pi / 4.0)) { keep = false; break; }
-      }
-  }
-  B.nrm[s] = make_double2(nx, ny);
-  if (!keep) {
-    B.surf[s] = 0;
-    atomicSub(&B.counters[C_NSURF], 1);
-  }
-}
-
-// Near-surface flag (Define_freesurface...f90:418-440): any pair partner is a surface particle.
-__global__ void __launch_bounds__(kBlock) k_eval_b2(DevParams P, Buffers B) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = B.counters[C_NLOCAL];
-  if (s >= n) return;
-  if (B.counters[C_NSURF] == 0) return;            // near_ was zeroed by pass A
-  const double2 ps = B.l_pos[s];
-  const int ts = B.l_type[s];
-  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
-  int nr = 0;
-  for (int d = 0; d < 3 && !nr; ++d)
-    for (int o = R.lo[d]; o < R.hi[d]; ++o) {
-      if (o == s || B.surf[o] != 1) continue;
-      const double2 po = B.l_pos[o];
-      const double pdx = ps.x - po.x, pdy = ps.y - po.y;
-      const double r = sqrt(pdx * pdx + pdy * pdy);
-      if (!(r <= P.three_h && r >= P.r_min)) continue;
-      if (ts != 2 && B.l_type[o] != 2) continue;
-      nr = 1;
-      break;
-    }
-  B.near_[s] = nr;
-}
-
-// ---------------------------------------------------------------------------
-// pass C: Riemann continuity/momentum, artificial viscosity, laplacian,
-// delta-SPH internal, SPS, XSPH, delta-SPH density correction, shift sums.
-// Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90:300-635,
-// Solver_Normalized_Linear_Riemann_Delta_SPS.f90:362-371,
-// single_step_compute.f90:152-171, Shift_particle_position.f90:78-103.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_eval_c(DevParams P, Buffers B, int do_shift) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = B.counters[C_NLOCAL];
-  if (s >= n) return;
-  const double2 ps = B.l_pos[s];
-  const double2 vs = B.l_vel[s];
-  const double2 rvs = B.l_rv[s];
-  const double2 pcs = B.l_pc[s];
-  const int ts = B.l_type[s];
-  const double ms = B.l_mass[s];
-  const double gms = B.g_mass[s];                  // particle_mass(i) with the LOCAL index (quirk)
-  const double txs = B.tau[3 * s], tys = B.tau[3 * s + 1], tzs = B.tau[3 * s + 2];
-  const double2 grs = B.grad[s];
-  const int nes = B.near_[s];
-  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
-  const double cut2 = P.three_h * P.three_h * (1.0 + 1e-10);
-  double drho = 0.0, corr = 0.0;
-  double iax = 0.0, iay = 0.0, avx = 0.0, avy = 0.0, aix = 0.0, aiy = 0.0;
-  double spx = 0.0, spy = 0.0, lpx = 0.0, lpy = 0.0, agx = 0.0, agy = 0.0, shx = 0.0, shy = 0.0;
-#pragma unroll 1
-  for (int d = 0; d < 3; ++d) {
-#pragma unroll 1
-    for (int o = R.lo[d]; o < R.hi[d]; ++o) {
-      if (o == s) continue;
-      const double2 po = B.l_pos[o];
-      const bool sa = s < o;
-      const double pdx = sa ? ps.x - po.x : po.x - ps.x;   // x_ij = x_a - x_b
-      const double pdy = sa ? ps.y - po.y : po.y - ps.y;
-      const double r2 = pdx * pdx + pdy * pdy;
-      if (r2 > cut2) continue;
-      const double r = sqrt(r2);
-      if (!(r <= P.three_h && r >= P.r_min)) continue;
-      const int to = B.l_type[o];
-      if (ts != 2 && to != 2) continue;
-      double w, dwx, dwy;
-      kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
-      const double2 vo = B.l_vel[o];
-      const double2 rvo = B.l_rv[o];
-      const double2 pco = B.l_pc[o];
-      // pair quantities in reference orientation a = min(s,o), b = max(s,o)
-      const double2 va = sa ? vs : vo, vb = sa ? vo : vs;
-      const double ra = sa ? rvs.x : rvo.x, rb = sa ? rvo.x : rvs.x;
-      const double Va = sa ? rvs.y : rvo.y, Vb = sa ? rvo.y : rvs.y;
-      const double pa = sa ? pcs.x : pco.x, pb = sa ? pco.x : pcs.x;
-      const double ca = sa ? pcs.y : pco.y, cb = sa ? pco.y : pcs.y;
-      const double Vother = sa ? Vb : Va;
-      const double hb = P.h;
-      const double Pbar = (pa + pb) / 2.0;
-      const double vbx = (va.x + vb.x) / 2.0, vby = (va.y + vb.y) / 2.0;
-      const double ex = -(pdx / r), ey = -(pdy / r);
-      const double UL = va.x * ex + va.y * ey;
-      const double UR = vb.x * ex + vb.y * ey;
-      const double Ub = (UL + UR) / 2.0;
-      const double cbar = (ca + cb) / 2.0;
-      const double rbar = (ra + rb) / 2.0;
-      double vsx, vsy, Ps;
-      if (P.solver == SPH2D_SOLVER_LINEAR_RIEMANN) {
-        const double zi = ra * ca, zj = rb * cb;
-        double Us = zi * UR + zj * UL + pa - pb;
-        Us = Us / (zi + zj);
-        vsx = Us * ex; vsy = Us * ey;
-        Ps = zi * pb + zj * pa + ra * ca * rb * cb * (UL - UR);
-        Ps = Ps / (zi + zj);
-      } else {
-        const double Us = Ub + 0.5 * (pa - pb) / (rbar * P.sqrt_c0sq);
-        vsx = (Us - Ub) * ex + vbx;
-        vsy = (Us - Ub) * ey + vby;
-        const double dU = UL - UR;
-        double bR = 3.0 * (dU > 0.0 ? dU : 0.0);
-        bR = bR < cbar ? bR : cbar;
-        Ps = Pbar + 0.5 * bR * rbar * (UL - UR);
-      }
-      const int nea = sa ? nes : B.near_[o];
-      const int neb = sa ? B.near_[o] : nes;
-      const double rij = 1.0 / (ra * rb);
-      if (sa) {
-        const double Ivx = nea ? vbx : vsx, Ivy = nea ? vby : vsy, IP = nea ? Pbar : Ps;
-        const double ti = (va.x - Ivx) * dwx + (va.y - Ivy) * dwy;
-        drho = drho + 2 * ra * Vb * ti;
-        const double mb = B.l_mass[o];
-        iax = iax + mb * (-2 * IP * dwx * rij);
-        iay = iay + mb * (-2 * IP * dwy * rij);
-      } else {
-        const double Jvx = neb ? vbx : vsx, Jvy = neb ? vby : vsy, JP = neb ? Pbar : Ps;
-        const double tj = -((vb.x - Jvx) * dwx + (vb.y - Jvy) * dwy);
-        drho = drho + 2 * rb * Va * tj;
-        const double ma = B.l_mass[o];
-        iax = iax - ma * (-2 * JP * dwx * rij);
-        iay = iay - ma * (-2 * JP * dwy * rij);
-      }
-      const double vijx = va.x - vb.x, vijy = va.y - vb.y;
-      const double vx = vijx * pdx + vijy * pdy;
-      if (vx < 0.0) {
-        const double fh = P.fai * hb;
-        const double phi = hb * vx / (r2 + fh * fh);
-        const double Pi = (-(P.alpha * cbar * phi) + P.beta * (phi * phi)) / rbar;
-        const double mo = B.l_mass[o];
-        if (sa) { avx = avx + mo * (-Pi * dwx); avy = avy + mo * (-Pi * dwy); }
-        else { avx = avx - mo * (-Pi * dwx); avy = avy - mo * (-Pi * dwy); }
-      }
-      const double Rd = pdx * dwx + pdy * dwy;
-      const double lf = Vother * 2 * P.nu * Rd / r2;
-      if (sa) { lpx = lpx + lf * vijx; lpy = lpy + lf * vijy; }
-      else { lpx = lpx - lf * vijx; lpy = lpy - lf * vijy; }
-      const double tmp = P.delta_alpha * hb * cbar * vx / r2;
-      if (sa) {
-        aix = aix + tmp * dwx * P.rho0 / ra * Vb;
-        aiy = aiy + tmp * dwy * P.rho0 / ra * Vb;
-      } else {
-        aix = aix - tmp * dwx * P.rho0 / rb * Va;
-        aiy = aiy - tmp * dwy * P.rho0 / rb * Va;
-      }
-      {  // SPS: raw dwdx, tau summed over both particles, temp_p carried from x to y
-        const double txo = B.tau[3 * o], tyo = B.tau[3 * o + 1], tzo = B.tau[3 * o + 2];
-        const double gmo = B.g_mass[o];
-        double tp = 0.0;
-        tp = tp + (txs + txo) * dwx;
-        tp = tp + (tys + tyo) * dwy;
-        tp = tp * rij;
-        if (sa) spx = spx + gmo * tp; else spx = spx - gmo * tp;
-        tp = tp + (tzs + tzo) * dwy + (tys + tyo) * dwx;
-        tp = tp * rij;
-        if (sa) spy = spy + gmo * tp; else spy = spy - gmo * tp;
-      }
-      if (P.epsilon_xsph != 0.0 && P.solver == SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN) {
-        const double mo = B.l_mass[o];
-        const double rs = ra + rb;
-        if (sa) {
-          agx = agx - P.epsilon_xsph * 2 * mo * vijx / rs * w;
-          agy = agy - P.epsilon_xsph * 2 * mo * vijy / rs * w;
-        } else {
-          agx = agx + P.epsilon_xsph * 2 * mo * vijx / rs * w;
-          agy = agy + P.epsilon_xsph * 2 * mo * vijy / rs * w;
-        }
-      }
-      {  // delta-SPH density diffusion (Solver...:552-583)
-        const double2 gro = B.grad[o];
-        const double rd = -(ra - rb);
-        const double qx = -pdx, qy = -pdy;
-        const double q2 = qx * qx + qy * qy;
-        const double sgx = sa ? grs.x + gro.x : gro.x + grs.x;
-        const double sgy = sa ? grs.y + gro.y : gro.y + grs.y;
-        const double psi = rd - 0.5 * (sgx * qx + sgy * qy);
-        const double tt = (qx * dwx + qy * dwy) / q2;
-        const double cc = 2 * P.delta * hb * cbar * psi * tt * Vother;
-        corr = sa ? corr + cc : corr - cc;
-      }
-      if (do_shift) {  // Shift_particle_position.f90:78-103
-        const double wr = w / P.w_dx;
-        const double f = P.shift_beta * P.h2 * (1 + P.shift_R * (wr * wr * wr * wr));
-        if (sa) { shx = shx - f * dwx * Vother; shy = shy - f * dwy * Vother; }
-        else { shx = shx + f * dwx * Vother; shy = shy + f * dwy * Vother; }
-      }
-    }
-  }
-  (void)ms; (void)gms;
-  if (ts == 2) drho = drho + corr;
-  B.drhodt[s] = drho;
-  double ax = 0.0, ay = 0.0;
-  if (ts == 2) {
-    ax = iax + 0.0 + avx + aix + spx + lpx;
-    ay = iay + 0.0 + avy + aiy + spy + lpy;
-  }
-  B.acc[s] = make_double2(ax, ay);
-  if (P.epsilon_xsph != 0.0) B.avg[s] = make_double2(agx, agy);
-  if (do_shift) B.shift[s] = make_double2(shx, shy);
-  if (P.write_debug) {
-    double* d = B.dbg + 10 * s;
-    d[0] = iax; d[1] = iay; d[2] = avx; d[3] = avy; d[4] = aix; d[5] = aiy;
-    d[6] = spx; d[7] = spy; d[8] = lpx; d[9] = lpy;
-  }
-}
-
-void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
-  const int g = grid_for(cap_local);
-  launch_fill_i32(s, B.counters + C_NSURF, 1, 0);
-  hipLaunchKernelGGL(k_eval_a, dim3(g), dim3(kBlock), 0, s, P, B);
-  hipLaunchKernelGGL(k_eval_b1, dim3(g), dim3(kBlock), 0, s, P, B);
-  hipLaunchKernelGGL(k_eval_b2, dim3(g), dim3(kBlock), 0, s, P, B);
-  hipLaunchKernelGGL(k_eval_c, dim3(g), dim3(kBlock), 0, s, P, B, 1);
-}
-
-void launch_eval_pass(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int pass) {
-  const int g = grid_for(cap_local);
-  if (pass == 0) hipLaunchKernelGGL(k_eval_c, dim3(g), dim3(kBlock), 0, s, P, B, 1);
-  else if (pass == 1) hipLaunchKernelGGL(k_eval_a, dim3(g), dim3(kBlock), 0, s, P, B);
-  else launch_eval(s, P, B, cap_local);
-}
-
-// ---------------------------------------------------------------------------
 // Stage updates of time_iteration.f90 (fluid particles only; whole-array saves)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void eos_volume(const DevParams& P, Buffers& B, int s, double rho) {

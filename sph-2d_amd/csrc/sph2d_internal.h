@@ -35,6 +35,7 @@ struct DevParams {
   double sps_dl, sps_c012, sps_c23, sps_c23b;  // REAL(4) constants of the SPS closure
   double mls_alpha;
   double surf_lim;                       // REAL(4) 1.414 * h
+  double inv_h, inv_c0, inv_sv, inv_wdx, cut2;  // reciprocals and the squared-distance pre-test
   int solver, write_debug;
   int mls_order;
 };

@@ -1,8 +1,8 @@
 """GPU parity of the HIP step (through the C-ABI) against the oracle.
 
 Bars: integer/index work (ghost generation, local order, pair set, flags) is
-bit-exact; FP64 fields agree with the oracle within 1e-12 relative to the
-field's largest magnitude (the gather sums run in another order than the
+bit-exact; FP64 fields agree with the oracle within 1e-12 relative to the largest
+magnitude of the field (of its vector or tensor: components share one scale; the gather sums run in another order than the
 reference's scatter sums, and exp() differs in the last bit); the Tait EOS,
 which has no sum and no exp, is bit-exact.
 """
@@ -137,11 +137,20 @@ def test_single_step_compute_parity(n, perturb, solver):
     assert np.array_equal(sim.local("name"), o.i32("name"))
     for f in ("surface", "near_surface"):
         assert np.array_equal(sim.local(f), o.i32(f)), f
-    oname = {"volume": "volume", "sum_dwdx_x": "sum_dwdx_x"}
-    worst = {}
+    # components of one vector/tensor share the scale of the whole vector/tensor
+    # (S_xy of the TGV lattice is zero analytically: alone it is only rounding noise)
+    groups = {}
     for f in FIELDS:
-        g, r = sim.local(f), o.f64(oname.get(f, f))
-        worst[f] = rel_err(g, r)
+        key = f.rsplit("_", 1)[0] if f.endswith(("_x", "_y", "_xx", "_xy", "_yy")) else f
+        if f in ("ax", "ay"):
+            key = "a"
+        groups.setdefault(key, []).append(f)
+    worst = {}
+    for key, fs in groups.items():
+        ref = {f: o.f64(f) for f in fs}
+        scale = max(np.max(np.abs(r)) for r in ref.values())
+        for f in fs:
+            worst[f] = float(np.max(np.abs(sim.local(f) - ref[f])) / scale) if scale > 0 else 0.0
     say(worst)
     for f, e in worst.items():
         assert e < TOL, (f, e)
@@ -153,7 +162,7 @@ def test_shift_sums_parity():
     say("test_shift_sums_parity")
     sim, o = pair(64, state=perturbed_tgv(64, seed=5))
     evaluate_both(sim, o)
-    o.shift()  # fills shift_x/y from the same pairs
+    o.shift_sums()  # Shift_particle_position.f90:78-103 sums over the same pairs, unclamped
     for f in ("shift_x", "shift_y"):
         assert rel_err(sim.local(f), o.f64(f)) < TOL, f
 
