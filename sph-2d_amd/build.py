@@ -38,6 +38,27 @@ def _run(cmd):
     return p
 
 
+FC = os.environ.get("FC", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN = os.path.join(HERE, "fortran")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The ISO_C_BINDING interface module and the Fortran example driver, linked
+    against libsph2d.so (skipped when flang is absent)."""
+    if not os.path.exists(FC):
+        return ""
+    bdir = os.path.join(HERE, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    exe = os.path.join(FORTRAN, "tgv_driver")
+    cmd = [FC, "-O2", "-J", bdir, "-o", exe,
+           os.path.join(FORTRAN, "sph2d_interface.f90"), os.path.join(FORTRAN, "tgv_driver.f90"),
+           "-L" + HERE, "-lsph2d", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd)
+    return exe
+
+
 def build(verbose: bool = False) -> str:
     objs = []
     bdir = os.path.join(HERE, "_build")
@@ -53,6 +74,7 @@ def build(verbose: bool = False) -> str:
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     _run(cmd)
     os.replace(tmp, OUT)
+    build_fortran(verbose)
     return OUT
 
 

@@ -1,0 +1,138 @@
+! sph2d_interface.f90 -- ISO_C_BINDING interface to libsph2d.so (include/sph2d.h).
+!
+! A Fortran driver of mini-walker/SPH-2D `use`s this module and calls the GPU
+! routines with the reference's own argument lists: compute_kernel and
+! status_equation take their arguments by reference exactly as
+! Taylor_Green_Vortex/compute_kernel.f90:21-27 and status_equation.f90:19-24
+! declare them, so existing call sites only change the routine name.
+module sph2d_interface
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: SPH2D_INTEGRATOR_PC = 2, SPH2D_INTEGRATOR_RK = 3
+
+  type, bind(C) :: sph2d_params
+    integer(c_int32_t) :: struct_size
+    integer(c_int32_t) :: nx, ny
+    real(c_double) :: box_x, box_y, rho0, c0sq, nu, U, dx, h, dt
+    integer(c_int32_t) :: integrator, solver, precision, rate_reuse
+    integer(c_int32_t) :: shift_on, mls_on, mls_every, mls_order, sample_every, periodic
+    real(c_double) :: alpha, beta, fai, delta, delta_alpha, epsilon_xsph
+    real(c_double) :: shift_beta, shift_R, max_shift, mls_alpha
+  end type sph2d_params
+
+  interface
+    function sph2d_default_params(p) bind(C, name="sph2d_default_params") result(rc)
+      import :: c_int, sph2d_params
+      type(sph2d_params), intent(out) :: p
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_ctx_create(p, capacity, ctx) bind(C, name="sph2d_ctx_create") result(rc)
+      import :: c_int, c_int32_t, c_ptr, sph2d_params
+      type(sph2d_params), intent(in) :: p
+      integer(c_int32_t), value :: capacity
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_ctx_destroy(ctx) bind(C, name="sph2d_ctx_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_init_tgv(ctx) bind(C, name="sph2d_init_tgv") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    ! particle_position(n_total,dim) etc. pass straight through: (ld,2) column-major
+    function sph2d_upload_state(ctx, n, pos, vel, rho, mass, energy, h, ptype, ld) &
+        bind(C, name="sph2d_upload_state") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: n, ld
+      real(c_double), intent(in) :: pos(ld, 2), vel(ld, 2), rho(ld), mass(ld), energy(ld), h(ld)
+      integer(c_int32_t), intent(in) :: ptype(ld)
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_download_state(ctx, n, pos, vel, rho, press, c, mass, ptype, ld) &
+        bind(C, name="sph2d_download_state") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: n, ld
+      real(c_double), intent(out) :: pos(ld, 2), vel(ld, 2), rho(ld), press(ld), c(ld), mass(ld)
+      integer(c_int32_t), intent(out) :: ptype(ld)
+      integer(c_int) :: rc
+    end function
+
+    ! compute_kernel(dim, distance, position_difference, average_smooth_length, w, temp_dwdx)
+    function sph2d_compute_kernel(dim, distance, position_difference, average_smooth_length, w, &
+        temp_dwdx) bind(C, name="sph2d_compute_kernel") result(rc)
+      import :: c_int, c_int32_t, c_double
+      integer(c_int32_t), intent(in) :: dim
+      real(c_double), intent(in) :: distance, position_difference(2), average_smooth_length
+      real(c_double), intent(out) :: w, temp_dwdx(2)
+      integer(c_int) :: rc
+    end function
+
+    ! status_equation(particle_rho, particle_energy, particle_type, particle_press, particle_c)
+    function sph2d_status_equation(particle_rho, particle_energy, particle_type, particle_press, &
+        particle_c) bind(C, name="sph2d_status_equation") result(rc)
+      import :: c_int, c_int32_t, c_double
+      real(c_double), intent(in) :: particle_rho, particle_energy
+      integer(c_int32_t), intent(in) :: particle_type
+      real(c_double), intent(out) :: particle_press, particle_c
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_distribute_period_boundary(ctx) bind(C, name="sph2d_distribute_period_boundary") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_devide_block(ctx, i_time_step) bind(C, name="sph2d_devide_block") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: i_time_step
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_single_step_compute(ctx, i_time_step) bind(C, name="sph2d_single_step_compute") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: i_time_step
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_time_step(ctx, i_time_step) bind(C, name="sph2d_time_step") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: i_time_step
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_run(ctx, first_step, nsteps) bind(C, name="sph2d_run") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: first_step, nsteps
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_sample(ctx, out14) bind(C, name="sph2d_sample") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out14(14)
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_last_error() bind(C, name="sph2d_last_error") result(msg)
+      import :: c_ptr
+      type(c_ptr) :: msg
+    end function
+  end interface
+end module sph2d_interface

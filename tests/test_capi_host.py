@@ -94,3 +94,14 @@ def test_tgv_params_scale_with_lattice():
     assert p.nx == p.ny == 2048
     assert p.dx == 1.0 / 2048 and p.h == 1.2 * p.dx and p.max_shift == 0.25 * p.dx
     assert p.dt == pytest.approx(1e-4 * 100 / 2048)
+
+
+def test_fortran_driver_links_and_fails_loudly_without_gpu():
+    exe = os.path.join(os.path.dirname(sph2d.LIB_PATH), "fortran", "tgv_driver")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built (flang absent)")
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    out = subprocess.run([exe, "16", "1"], capture_output=True, text=True, timeout=60)
+    assert out.returncode != 0
+    assert "runs on the GPU only" in out.stdout

@@ -274,3 +274,33 @@ def test_full_size_2048_pair_count_and_step():
     sim.time_step(1)
     s = sim.sample()
     assert np.all(np.isfinite(s)) and abs(s[0] - 1.0) < 1e-3
+
+
+def test_fortran_driver_matches_python_path():
+    """The ISO_C_BINDING caller (sph-2d_amd/fortran/tgv_driver) gets the same
+    scalar results and the same trajectory as the Python mirror."""
+    say("test_fortran_driver_matches_python_path")
+    import subprocess
+    exe = os.path.join(os.path.dirname(sph2d.LIB_PATH), "fortran", "tgv_driver")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built (flang absent)")
+    out = subprocess.run([exe, "40", "200"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = out.stdout.split("\n")
+    assert lines[-2].strip() == "ok"
+    kern = [float(v) for v in lines[0].split()[1:]]
+    eos = [float(v) for v in lines[1].split()[1:]]
+    pd = np.array([[0.006, 0.008]])
+    w, dw = sph2d.compute_kernel(np.sqrt((pd ** 2).sum(1)), pd, [0.012])
+    assert kern[0] == pytest.approx(w[0], rel=1e-15) and kern[1:] == pytest.approx(list(dw[0]), rel=1e-15)
+    p, c = sph2d.status_equation([1001.0])
+    assert eos == pytest.approx([p[0], c[0]], rel=1e-15)
+    prm = sph2d.tgv_params(40, integrator=3)
+    sim = sph2d.Simulation(prm)
+    sim.init_tgv()
+    sim.run(1, 200)
+    s = sim.sample()
+    f_step, f_ke, f_l2 = lines[3].split()
+    assert int(f_step) == 200
+    assert float(f_ke) == pytest.approx(s[0], rel=1e-14)
+    assert float(f_l2) == pytest.approx(s[4], rel=1e-12)
