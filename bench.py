@@ -1,6 +1,10 @@
 """Throughput of the WCSPH step on one MI355X: Taylor-Green vortex on a 2048^2
 lattice (4,194,304 fluid particles + 32,832 periodic ghosts), predictor-corrector
-(time_iteration.f90, leapforg_or_eular=2), plain FP64 form.
+(time_iteration.f90, leapforg_or_eular=2). Both forms run on the same workload:
+the plain FP64 form and the FP32-mixed form (FP32 pair sums with Kahan
+accumulation, FP64 positions, integration and EOS); `value` is the mixed form's
+particle-steps/s, with the FP64 figure, the speedup and the KE error between the
+two forms in `config`.
 
 One step = one pass of time_iteration.f90's loop body: periodic ghosts, cell-key
 sort + reorder, stage updates with one force evaluation (two on step 1), MLS
@@ -39,17 +43,20 @@ def hip_events():
     return hip
 
 
-def force_pass_bytes_per_particle() -> int:
-    """Algorithmic HBM bytes per subdomain particle of one force-pass launch (k_eval_c):
+def force_pass_bytes_per_particle(form: int) -> int:
+    """Algorithmic HBM bytes per subdomain particle of one force-pass launch:
     every array the pass reads for its own particle once, every output once; the
     neighbours' data are the same arrays (served from L2/L1), not counted again.
-    reads:  pos 16, vel 16, (rho,V) 16, (p,c) 16, type 4, cell 4, mass 8,
-            global mass at local index 8, tau 24, grad rho 16, near flag 4 = 132
-    writes: drho/dt 8, acceleration 16, shift 16 = 40."""
-    return 132 + 40
+    FP64 (k_eval_c) reads: pos 16, vel 16, (rho,V) 16, (p,c) 16, type 4, cell 4,
+      near 4, tau 24, grad rho 16, mass 8, particle_mass at local index 8 = 132
+    FP32-mixed (k_eval_c32) reads: pos 16 (FP64), vel 8, (drho,V) 8, (p,c) 8,
+      (mass, particle_mass at local index) 8, tau 16, grad rho 8, type 4, cell 4,
+      near 4 = 84
+    both write: drho/dt 8, acceleration 16, shift 16 = 40."""
+    return (132 if form == 64 else 84) + 40
 
 
-def cpu_baseline(n_side: int = 512, steps: int = 4) -> dict:
+def cpu_baseline(n_side: int = 512, steps: int = 6) -> dict:
     """The C oracle (oracle/liboracle.so, the reference's algorithm restated,
     scatter over a half pair list as in find_particle_pair.f90), one core, on a
     bounded sample: n_side^2 TGV, predictor-corrector, `steps` steps."""
@@ -71,12 +78,12 @@ def cpu_baseline(n_side: int = 512, steps: int = 4) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n-side", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-side", type=int, default=512)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=6)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every GPU box here has one, and the "
@@ -87,86 +94,105 @@ def main() -> int:
 
     import sph2d
     n = args.n_side
-    prm = sph2d.tgv_params(n, integrator=sph2d.INTEGRATOR_PC)
-    sim = sph2d.Simulation(prm)
-    sim.init_tgv()
-    print(f"# setup {n}^2 done", file=sys.stderr, flush=True)
-    if args.warmup > 0:
-        sim.run(1, args.warmup)
-    d = sim.diagnostics()                         # synchronises
-    first = 1 + args.warmup
     hip = hip_events()
-    stream = ctypes.c_void_p(sim.stream())
-    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
-    hip.hipEventCreate(ctypes.byref(e0))
-    hip.hipEventCreate(ctypes.byref(e1))
-    hip.hipEventRecord(e0, stream)
-    sim.run(first, args.steps)
-    hip.hipEventRecord(e1, stream)
-    hip.hipEventSynchronize(e1)
-    ms = ctypes.c_float()
-    hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
-    total_s = ms.value / 1000.0
-    s = sim.sample()
-    d = sim.diagnostics()
-    if d.error_flags or d.rebin_needed:
-        print(f"# warning: error_flags={d.error_flags} rebin={d.rebin_needed}", file=sys.stderr)
-    print(f"# timed {args.steps} steps in {total_s:.3f} s; KE/KE0={s[0]:.9f}", file=sys.stderr, flush=True)
 
-    # roofline of the dominant kernel: the force pass, timed alone with HIP events
-    reps = 20
-    force_ms = sim.time_pass(0, reps)
-    passa_ms = sim.time_pass(1, reps)
-    eval_ms = sim.time_pass(2, reps)
-    n_local = d.n_local
-    bytes_launch = force_pass_bytes_per_particle() * n_local
-    achieved = bytes_launch / (force_ms * 1e-3) / 1e9
-    # pair count of the state (for pair interactions/s)
-    cnt = ctypes.c_int64()
-    sph2d._check(sph2d.lib().sph2d_download_pairs(sim._c, 0, None, None, ctypes.byref(cnt)))
-    pairs = cnt.value
-    steps_per_s = args.steps / total_s
-    value = n * n * steps_per_s
-    evals_per_step = 1
+    def run_form(precision: int) -> dict:
+        prm = sph2d.tgv_params(n, integrator=sph2d.INTEGRATOR_PC, precision=precision)
+        sim = sph2d.Simulation(prm)
+        sim.init_tgv()
+        if args.warmup > 0:
+            sim.run(1, args.warmup)
+        sim.diagnostics()                          # synchronises
+        stream = ctypes.c_void_p(sim.stream())
+        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        hip.hipEventCreate(ctypes.byref(e0))
+        hip.hipEventCreate(ctypes.byref(e1))
+        hip.hipEventRecord(e0, stream)
+        sim.run(1 + args.warmup, args.steps)
+        hip.hipEventRecord(e1, stream)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        total_s = ms.value / 1000.0
+        smp = sim.sample()
+        d = sim.diagnostics()
+        if d.error_flags or d.rebin_needed:
+            print(f"# warning: error_flags={d.error_flags} rebin={d.rebin_needed}", file=sys.stderr)
+        reps = 20
+        force_ms = sim.time_pass(0, reps)
+        passa_ms = sim.time_pass(1, reps)
+        eval_ms = sim.time_pass(2, reps)
+        cnt = ctypes.c_int64()
+        sph2d._check(sph2d.lib().sph2d_download_pairs(sim._c, 0, None, None, ctypes.byref(cnt)))
+        out = {"total_s": total_s, "sample": smp, "n_local": d.n_local, "n_ghost": d.n_ghost,
+               "force_ms": force_ms, "passa_ms": passa_ms, "eval_ms": eval_ms, "pairs": cnt.value,
+               "dt": prm.dt}
+        sim.close()
+        print(f"# form {precision}: {args.steps} steps in {total_s:.3f} s, KE/KE0={smp[0]:.10f}",
+              file=sys.stderr, flush=True)
+        return out
+
+    r64 = run_form(sph2d.FP64)
+    r32 = run_form(sph2d.FP32_MIXED)
+    N = n * n
+    sps64 = N * args.steps / r64["total_s"]
+    sps32 = N * args.steps / r32["total_s"]
+    ke_err = abs(r32["sample"][0] - r64["sample"][0]) / r64["sample"][0]
+    t_end = (args.warmup + args.steps) * r64["dt"]
+    bytes64 = force_pass_bytes_per_particle(64) * r64["n_local"]
+    bytes32 = force_pass_bytes_per_particle(32) * r32["n_local"]
+    ach32 = bytes32 / (r32["force_ms"] * 1e-3) / 1e9
+    ach64 = bytes64 / (r64["force_ms"] * 1e-3) / 1e9
+    steps_per_s = args.steps / r32["total_s"]
     out = {
         "metric": "particle-steps/s, 4.2M-particle TGV, FP32-mixed vs FP64 at stated KE error",
-        "value": value,
+        "value": sps32,
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1000.0 * total_s / args.steps,
+        "ms_per_step": 1000.0 * r32["total_s"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32-mixed",
         "data": "synthetic (Taylor-Green lattice initial condition generated on the GPU)",
         "config": {
-            "workload": f"Taylor-Green vortex {n}x{n} ({n * n} fluid + {d.n_ghost} ghost particles), "
+            "workload": f"Taylor-Green vortex {n}x{n} ({N} fluid + {r32['n_ghost']} ghost particles), "
                         f"predictor-corrector (1 force evaluation per step), modified linear Riemann "
-                        f"+ delta-SPH + SPS, shifting, MLS every 20 steps, plain FP64 form",
-            "n_side": n, "dt": prm.dt, "form": "fp64",
-            "ke_over_ke0_at_end": s[0],
-            "pairs": pairs,
-            "pair_interactions_per_s": pairs * evals_per_step * steps_per_s,
-            "force_pass_ms": force_ms, "pass_a_ms": passa_ms, "evaluation_ms": eval_ms,
+                        f"+ delta-SPH + SPS, shifting, MLS every 20 steps; value = FP32-mixed form",
+            "n_side": n, "dt": r32["dt"],
+            "fp64_particle_steps_per_s": sps64,
+            "fp64_ms_per_step": 1000.0 * r64["total_s"] / args.steps,
+            "mixed_over_fp64_speedup": sps32 / sps64,
+            "ke_rel_err_mixed_vs_fp64": ke_err,
+            "ke_err_at_t": t_end,
+            "ke_over_ke0_fp64": r64["sample"][0], "ke_over_ke0_mixed": r32["sample"][0],
+            "pairs": r32["pairs"],
+            "pair_interactions_per_s": r32["pairs"] * steps_per_s,
+            "force_pass_ms": {"fp64": r64["force_ms"], "mixed": r32["force_ms"]},
+            "pass_a_ms": {"fp64": r64["passa_ms"], "mixed": r32["passa_ms"]},
+            "evaluation_ms": {"fp64": r64["eval_ms"], "mixed": r32["eval_ms"]},
+            "fp64_force_pass_roofline": {"achieved": ach64, "unit": "GB/s", "frac": ach64 / HBM_PEAK_GBS,
+                                         "bytes_per_particle": force_pass_bytes_per_particle(64)},
         },
         "roofline": {
-            "kernel": "k_eval_c (force pass)",
+            "kernel": "k_eval_c32 (FP32-mixed force pass)",
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": ach32,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": ach32 / HBM_PEAK_GBS,
             "traffic": None,
-            "bytes_per_particle": force_pass_bytes_per_particle(),
-            "units_per_launch": n_local,
+            "bytes_per_particle": force_pass_bytes_per_particle(32),
+            "units_per_launch": r32["n_local"],
         },
     }
     if rank == 0 and not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_side, args.cpu_steps)
-    sim.close()
     print(json.dumps(out), flush=True)
     return 0
 

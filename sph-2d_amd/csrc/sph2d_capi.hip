@@ -111,6 +111,7 @@ void make_devparams(const sph2d_params& p, DevParams& P) {
   P.solver = p.solver;
   P.mls_order = p.mls_order;
   P.write_debug = 0;
+  P.mixed = p.precision == SPH2D_FP32_MIXED ? 1 : 0;
 }
 
 int check_params(const sph2d_params* p) {
@@ -123,8 +124,8 @@ int check_params(const sph2d_params* p) {
     return fail(SPH2D_ERR_UNSUPPORTED, "integrator must be 2 (predictor-corrector) or 3 (RK-variant)");
   if (p->solver != SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && p->solver != SPH2D_SOLVER_LINEAR_RIEMANN)
     return fail(SPH2D_ERR_UNSUPPORTED, "solver variant not built");
-  if (p->precision != SPH2D_FP64)
-    return fail(SPH2D_ERR_UNSUPPORTED, "only the FP64 form is built in this library");
+  if (p->precision != SPH2D_FP64 && p->precision != SPH2D_FP32_MIXED)
+    return fail(SPH2D_ERR_UNSUPPORTED, "precision must be SPH2D_FP64 or SPH2D_FP32_MIXED");
   if (p->rate_reuse != SPH2D_RATE_REUSE_REFERENCE)
     return fail(SPH2D_ERR_UNSUPPORTED, "only the reference's rate re-use is built");
   if (p->mls_on && p->mls_order != 6)
@@ -243,6 +244,10 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   rc |= c->alloc(&B.red, (size_t)grid_n(c->cap_fluid) * 14 + 16);
   rc |= c->alloc(&B.pair_count, L); rc |= c->alloc(&B.pair_off, L);
   rc |= c->alloc(&c->d_out, 16);
+  if (c->P.mixed) {
+    rc |= c->alloc(&B.m_vel, L); rc |= c->alloc(&B.m_rv, L); rc |= c->alloc(&B.m_pc, L);
+    rc |= c->alloc(&B.m_mass, L); rc |= c->alloc(&B.m_grad, L); rc |= c->alloc(&B.m_tau, L);
+  }
   B.dbg = nullptr; B.pair_lo = nullptr; B.pair_hi = nullptr;
   B.kstride = (int)L;
   if (rc) {
@@ -650,7 +655,8 @@ int sph2d_download_local(sph2d_ctx* c, const char* name, void* out, int32_t n) {
                                       "art_internal_y", "sps_x", "sps_y", "lap_x", "lap_y"};
   for (int k = 0; k < 10; ++k)
     if (s == dbg_names[k]) {
-      if (!B.dbg || !c->debug) return fail(SPH2D_ERR_STATE, "acceleration components need sph2d_set_debug(ctx,1) before the evaluation");
+      if (!B.dbg || !c->debug || c->P.mixed)
+        return fail(SPH2D_ERR_STATE, "acceleration components need the FP64 form and sph2d_set_debug(ctx,1) before the evaluation");
       return d1(B.dbg, 10, k);
     }
   return fail(SPH2D_ERR_ARGUMENT, "unknown array name '" + s + "'");

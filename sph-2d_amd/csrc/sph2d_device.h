@@ -28,6 +28,14 @@ __device__ __forceinline__ void tait(const DevParams& P, double rho, double& p, 
   c = P.sqrt_c0sq * (x * x * x);
 }
 
+// FP32 mirrors of the state one neighbour reads in the mixed form.
+__device__ __forceinline__ void write_mirror(const DevParams& P, const Buffers& B, int s, double2 vel,
+                                             double rho, double vol, double p, double c) {
+  B.m_vel[s] = make_float2((float)vel.x, (float)vel.y);
+  B.m_rv[s] = make_float2((float)(rho - P.rho0), (float)vol);
+  B.m_pc[s] = make_float2((float)p, (float)c);
+}
+
 __device__ __forceinline__ double tait_p(const DevParams& P, double rho) {
   const double x = rho / P.rho0;
   return P.eos_b * (x * x * x * x * x * x * x - 1.0);

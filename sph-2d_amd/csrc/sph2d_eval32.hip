@@ -1,0 +1,253 @@
+// sph2d_eval32.hip -- the FP32-mixed force evaluation.
+//
+// Same passes, order and pair terms as sph2d_eval64.hip, with the pair sums in
+// FP32 and Kahan-compensated accumulation. What stays FP64: particle positions
+// (the exact pair test 1e-7 <= r <= 3h and the offset x_s - x_o, rounded to
+// FP32 only after the subtraction), the integrator state and stage updates, the
+// Tait EOS and the per-particle SPS closure. Density enters the pair sums as
+// delta rho = rho - rho0 (FP32), so rho_o - rho_s carries no cancellation.
+// Neighbour data come from the FP32 mirrors (80 bytes per neighbour instead of
+// 128 in FP64).
+#include "sph2d_internal.h"
+#include "sph2d_device.h"
+
+namespace sph2d {
+
+static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
+
+// Kahan-compensated FP32 accumulator.
+struct KSum {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
+struct DevParams32 {
+  float inv_h, e9, kfactor, inv_sv, rho0, inv_c0, h, fai_h2, alpha, beta, two_nu;
+  float delta_alpha_h, two_delta_h, shift_beta_h2, shift_R, inv_wdx, epsilon2;
+};
+
+static DevParams32 make32(const DevParams& P) {
+  DevParams32 Q;
+  Q.inv_h = (float)P.inv_h;
+  Q.e9 = (float)P.e9;
+  Q.kfactor = (float)P.kfactor;
+  Q.inv_sv = (float)P.inv_sv;
+  Q.rho0 = (float)P.rho0;
+  Q.inv_c0 = (float)P.inv_c0;
+  Q.h = (float)P.h;
+  Q.fai_h2 = (float)((P.fai * P.h) * (P.fai * P.h));
+  Q.alpha = (float)P.alpha;
+  Q.beta = (float)P.beta;
+  Q.two_nu = (float)(2 * P.nu);
+  Q.delta_alpha_h = (float)(P.delta_alpha * P.h);
+  Q.two_delta_h = (float)(2 * P.delta * P.h);
+  Q.shift_beta_h2 = (float)(P.shift_beta * P.h2);
+  Q.shift_R = (float)P.shift_R;
+  Q.inv_wdx = (float)P.inv_wdx;
+  Q.epsilon2 = (float)(P.epsilon_xsph * 2);
+  return Q;
+}
+
+struct KernelTerm32 {
+  float w, dwx, dwy;
+};
+
+__device__ __forceinline__ KernelTerm32 kernel_term32(const DevParams32& Q, float dx, float dy, float r,
+                                                      float inv_r) {
+  KernelTerm32 k;
+  const float q = r * Q.inv_h;
+  const float eq = __expf(-(q * q));
+  k.w = (eq - Q.e9) * Q.kfactor;
+  const float t = eq * (-2.0f * q) * (inv_r * Q.inv_h) * Q.kfactor;
+  k.dwx = t * dx;
+  k.dwy = t * dy;
+  return k;
+}
+
+// pass A (mixed): see k_eval_a in sph2d_eval64.hip for the reference lines.
+__global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q, Buffers B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  const double2 ps = B.l_pos[s];
+  const float2 vs = B.m_vel[s];
+  const float drs = B.m_rv[s].x;
+  const int ts = B.l_type[s];
+  KSum fra, sdx, sdy, Sxx, Sxy, Syy, grx, gry;
+  for_each_neighbour(P, B.cell_start, B.l_pos, s, ps, B.l_cell[s], [&](int o) {
+    if (ts != 2 && B.l_type[o] != 2) return;
+    const double2 po = B.l_pos[o];
+    const float dx = (float)(ps.x - po.x), dy = (float)(ps.y - po.y);
+    const float r2 = dx * dx + dy * dy;
+    const float inv_r = rsqrtf(r2);
+    const float r = r2 * inv_r;
+    const KernelTerm32 K = kernel_term32(Q, dx, dy, r, inv_r);
+    const float2 rvo = B.m_rv[o];
+    const float2 vo = B.m_vel[o];
+    const float Vo = rvo.y;
+    fra.add(Vo * Q.inv_sv);
+    sdx.add(K.dwx * Vo);
+    sdy.add(K.dwy * Vo);
+    const float vdx = vo.x - vs.x, vdy = vo.y - vs.y;
+    Sxx.add(vdx * K.dwx * Vo);
+    Sxy.add((vdx * K.dwy + vdy * K.dwx) * Vo);
+    Syy.add(vdy * K.dwy * Vo);
+    const float rd = (rvo.x - drs) * Vo;
+    grx.add(rd * K.dwx);
+    gry.add(rd * K.dwy);
+  });
+  const double f = (double)fra.s;
+  B.fra_v[s] = f;
+  B.sdw[s] = make_double2((double)sdx.s, (double)sdy.s);
+  B.grad[s] = make_double2((double)grx.s, (double)gry.s);
+  B.m_grad[s] = make_float2(grx.s, gry.s);
+  double txx = 0.0, txy = 0.0, tyy = 0.0;
+  if (ts == 2) {                                   // SPS closure in FP64 (per particle)
+    const double sxx = Sxx.s, sxy = Sxy.s, syy = Syy.s;
+    const double rhos = (double)drs + P.rho0;
+    const double master = sxx * sxx + syy * syy + 0.0;
+    const double last = sxy * sxy + 0.0 + 0.0;
+    const double abs2 = 2 * master + last;
+    const double t0 = P.sps_c012 * P.sps_dl;
+    const double nut = (t0 * t0) * sqrt(abs2);
+    const double kk = P.sps_c23 * nut * (sxx + syy);
+    const double blin = P.sps_c23b * (P.sps_dl * P.sps_dl) * abs2;
+    txx = (2 * nut * sxx - kk - blin) * rhos;
+    txy = (nut * sxy) * rhos;
+    tyy = (2 * nut * syy - kk - blin) * rhos;
+  }
+  B.m_tau[s] = make_float4((float)txx, (float)txy, (float)tyy, 0.f);
+  B.tau[3 * s + 0] = txx;
+  B.tau[3 * s + 1] = txy;
+  B.tau[3 * s + 2] = tyy;
+  const int sf = (ts == 2 && f <= P.fra_limit) ? 1 : 0;
+  B.surf[s] = sf;
+  B.near_[s] = 0;
+  if (sf) atomicAdd(&B.counters[C_NSURF], 1);
+}
+
+// pass C (mixed): see k_eval_c in sph2d_eval64.hip for the reference lines.
+__global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q, Buffers B, int do_shift) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  const double2 ps = B.l_pos[s];
+  const float2 vs = B.m_vel[s];
+  const float2 rvs = B.m_rv[s];
+  const float2 pcs = B.m_pc[s];
+  const int ts = B.l_type[s];
+  const float4 taus = B.m_tau[s];
+  const float2 grs = B.m_grad[s];
+  const int nes = B.near_[s];
+  const float rhos = Q.rho0 + rvs.x;
+  const float inv_rhos = 1.0f / rhos;
+  const bool linear = P.solver == SPH2D_SOLVER_LINEAR_RIEMANN;
+  KSum drho, corr, ax, ay, shx, shy, agx, agy;
+  for_each_neighbour(P, B.cell_start, B.l_pos, s, ps, B.l_cell[s], [&](int o) {
+    if (ts != 2 && B.l_type[o] != 2) return;
+    const double2 po = B.l_pos[o];
+    const float dx = (float)(ps.x - po.x), dy = (float)(ps.y - po.y);
+    const float r2 = dx * dx + dy * dy;
+    const float inv_r = rsqrtf(r2);
+    const float r = r2 * inv_r;
+    const float inv_r2 = inv_r * inv_r;
+    const KernelTerm32 K = kernel_term32(Q, dx, dy, r, inv_r);
+    const float2 vo = B.m_vel[o];
+    const float2 rvo = B.m_rv[o];
+    const float2 pco = B.m_pc[o];
+    const float rhoo = Q.rho0 + rvo.x, Vo = rvo.y;
+    const float ex = -(dx * inv_r), ey = -(dy * inv_r);
+    const float UL = vs.x * ex + vs.y * ey;
+    const float UR = vo.x * ex + vo.y * ey;
+    const float dU = UL - UR;
+    const float Pbar = (pcs.x + pco.x) * 0.5f;
+    const float vbx = (vs.x + vo.x) * 0.5f, vby = (vs.y + vo.y) * 0.5f;
+    const float cbar = (pcs.y + pco.y) * 0.5f;
+    const float rbar = Q.rho0 + (rvs.x + rvo.x) * 0.5f;
+    const float inv_rbar = 1.0f / rbar;
+    float vsx, vsy, Ps;
+    if (linear) {
+      const float zs = rhos * pcs.y, zo = rhoo * pco.y;
+      const float inv_z = 1.0f / (zs + zo);
+      const float Us = (zs * UR + zo * UL + (pcs.x - pco.x)) * inv_z;
+      vsx = Us * ex;
+      vsy = Us * ey;
+      Ps = (zs * pco.x + zo * pcs.x + zs * zo * dU) * inv_z;
+    } else {
+      const float Ub = (UL + UR) * 0.5f;
+      const float dUs = 0.5f * (pcs.x - pco.x) * inv_rbar * Q.inv_c0;   // U* - U_bar
+      vsx = dUs * ex + vbx;
+      vsy = dUs * ey + vby;
+      float bR = 3.0f * (dU > 0.0f ? dU : 0.0f);
+      bR = bR < cbar ? bR : cbar;
+      Ps = Pbar + 0.5f * bR * rbar * dU;
+      (void)Ub;
+    }
+    if (nes) { vsx = vbx; vsy = vby; Ps = Pbar; }
+    const float ti = (vs.x - vsx) * K.dwx + (vs.y - vsy) * K.dwy;
+    drho.add(2.0f * rhos * Vo * ti);
+    const float rij = inv_rhos / rhoo;
+    const float2 mo = B.m_mass[o];
+    const float fp = mo.x * (-2.0f * Ps * rij);
+    float tx = fp * K.dwx, ty = fp * K.dwy;
+    const float vijx = vs.x - vo.x, vijy = vs.y - vo.y;
+    const float vx = vijx * dx + vijy * dy;
+    if (vx < 0.0f) {
+      const float phi = Q.h * vx / (r2 + Q.fai_h2);
+      const float Pi = (-(Q.alpha * cbar * phi) + Q.beta * (phi * phi)) * inv_rbar;
+      tx += mo.x * (-Pi * K.dwx);
+      ty += mo.x * (-Pi * K.dwy);
+    }
+    const float Rd = dx * K.dwx + dy * K.dwy;
+    const float tmp = Q.delta_alpha_h * cbar * vx * inv_r2 * Q.rho0 * inv_rhos * Vo;
+    tx += tmp * K.dwx;
+    ty += tmp * K.dwy;
+    const float4 tauo = B.m_tau[o];
+    float tp = (taus.x + tauo.x) * K.dwx + (taus.y + tauo.y) * K.dwy;
+    tp = tp * rij;
+    tx += mo.y * tp;
+    tp = tp + (taus.z + tauo.z) * K.dwy + (taus.y + tauo.y) * K.dwx;
+    tp = tp * rij;
+    ty += mo.y * tp;
+    const float lf = Vo * Q.two_nu * Rd * inv_r2;
+    tx += lf * vijx;
+    ty += lf * vijy;
+    ax.add(tx);
+    ay.add(ty);
+    if (Q.epsilon2 != 0.0f && !linear) {
+      const float f = Q.epsilon2 * mo.x / (rhos + rhoo) * K.w;
+      agx.add(-f * vijx);
+      agy.add(-f * vijy);
+    }
+    const float2 gro = B.m_grad[o];
+    const float psi = (rvo.x - rvs.x) + 0.5f * ((grs.x + gro.x) * dx + (grs.y + gro.y) * dy);
+    corr.add(-(Q.two_delta_h * cbar * psi * Rd * inv_r2 * Vo));
+    if (do_shift) {
+      const float wr = K.w * Q.inv_wdx;
+      const float f = Q.shift_beta_h2 * (1.0f + Q.shift_R * (wr * wr * wr * wr)) * Vo;
+      shx.add(-f * K.dwx);
+      shy.add(-f * K.dwy);
+    }
+  });
+  double dr = (double)drho.s;
+  if (ts == 2) dr = dr + (double)corr.s;
+  B.drhodt[s] = dr;
+  B.acc[s] = ts == 2 ? make_double2((double)ax.s, (double)ay.s) : make_double2(0.0, 0.0);
+  if (P.epsilon_xsph != 0.0) B.avg[s] = make_double2((double)agx.s, (double)agy.s);
+  if (do_shift) B.shift[s] = make_double2((double)shx.s, (double)shy.s);
+}
+
+void launch_eval32_a(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
+  hipLaunchKernelGGL(k_eval_a32, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, make32(P), B);
+}
+
+void launch_eval32_c(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
+  hipLaunchKernelGGL(k_eval_c32, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, make32(P), B, 1);
+}
+
+}  // namespace sph2d

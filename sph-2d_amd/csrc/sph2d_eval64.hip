@@ -292,18 +292,26 @@ __global__ void __launch_bounds__(kBlock) k_eval_c(DevParams P, Buffers B, int d
 void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
   const int g = grid_for(cap_local);
   launch_fill_i32(s, B.counters + C_NSURF, 1, 0);
-  hipLaunchKernelGGL(k_eval_a, dim3(g), dim3(kBlock), 0, s, P, B);
+  if (P.mixed) launch_eval32_a(s, P, B, cap_local);
+  else hipLaunchKernelGGL(k_eval_a, dim3(g), dim3(kBlock), 0, s, P, B);
   hipLaunchKernelGGL(k_eval_b1, dim3(g), dim3(kBlock), 0, s, P, B);
   hipLaunchKernelGGL(k_eval_b2, dim3(g), dim3(kBlock), 0, s, P, B);
-  if (P.write_debug) hipLaunchKernelGGL(k_eval_c<true>, dim3(g), dim3(kBlock), 0, s, P, B, 1);
+  if (P.mixed) launch_eval32_c(s, P, B, cap_local);
+  else if (P.write_debug) hipLaunchKernelGGL(k_eval_c<true>, dim3(g), dim3(kBlock), 0, s, P, B, 1);
   else hipLaunchKernelGGL(k_eval_c<false>, dim3(g), dim3(kBlock), 0, s, P, B, 1);
 }
 
 void launch_eval_pass(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int pass) {
   const int g = grid_for(cap_local);
-  if (pass == 0) hipLaunchKernelGGL(k_eval_c<false>, dim3(g), dim3(kBlock), 0, s, P, B, 1);
-  else if (pass == 1) hipLaunchKernelGGL(k_eval_a, dim3(g), dim3(kBlock), 0, s, P, B);
-  else launch_eval(s, P, B, cap_local);
+  if (pass == 0) {
+    if (P.mixed) launch_eval32_c(s, P, B, cap_local);
+    else hipLaunchKernelGGL(k_eval_c<false>, dim3(g), dim3(kBlock), 0, s, P, B, 1);
+  } else if (pass == 1) {
+    if (P.mixed) launch_eval32_a(s, P, B, cap_local);
+    else hipLaunchKernelGGL(k_eval_a, dim3(g), dim3(kBlock), 0, s, P, B);
+  } else {
+    launch_eval(s, P, B, cap_local);
+  }
 }
 
 }  // namespace sph2d

@@ -342,6 +342,10 @@ __global__ void k_cell_rank_gather(DevParams P, Buffers B, int cap_global) {
   if (t != 120) tait(P, rho, p, c);
   B.l_pc[l] = make_double2(p, c);
   B.l_rv[l] = make_double2(rho, m / rho);
+  if (P.mixed) {
+    write_mirror(P, B, l, B.g_vel[g], rho, m / rho, p, c);
+    B.m_mass[l] = make_float2((float)m, (float)B.g_mass[l]);   // particle_mass(j) with j local
+  }
 }
 
 void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_global, int n_cells,
@@ -366,8 +370,10 @@ void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_
 __device__ __forceinline__ void eos_volume(const DevParams& P, Buffers& B, int s, double rho) {
   double p, c;
   tait(P, rho, p, c);
+  const double vol = B.l_mass[s] / rho;
   B.l_pc[s] = make_double2(p, c);
-  B.l_rv[s] = make_double2(rho, B.l_mass[s] / rho);
+  B.l_rv[s] = make_double2(rho, vol);
+  if (P.mixed) write_mirror(P, B, s, B.l_vel[s], rho, vol, p, c);
 }
 
 __global__ void k_stage(DevParams P, Buffers B, int kind) {

@@ -38,6 +38,7 @@ struct DevParams {
   double inv_h, inv_c0, inv_sv, inv_wdx, cut2;  // reciprocals and the squared-distance pre-test
   int solver, write_debug;
   int mls_order;
+  int mixed;                             // 1: FP32-mixed evaluation (mirrors are written)
 };
 
 // Device counters (one int array, indices below).
@@ -78,6 +79,11 @@ struct Buffers {
   double *red;                            // reduction partials
   int *pair_count, *pair_off;
   int *pair_lo, *pair_hi;
+  // FP32 mirrors of the neighbour data for the mixed form (local order):
+  // velocity, (rho - rho0, volume), (p, c), (mass, particle_mass at the local index),
+  // tau (xx, xy, yy, 0), grad rho
+  float2 *m_vel, *m_rv, *m_pc, *m_mass, *m_grad;
+  float4 *m_tau;
   int kstride;                            // stride of the 3 RK stage slabs (= local capacity)
 };
 
@@ -87,6 +93,9 @@ void launch_period_boundary(hipStream_t s, const DevParams& P, Buffers& B, int c
 void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_global, int n_cells,
                          bool analytic, double t_nondim);
 void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+// FP32-mixed passes A and C (sph2d_eval32.hip)
+void launch_eval32_a(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+void launch_eval32_c(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
 // pass 0: force pass only, 1: pass A only, 2: the whole evaluation
 void launch_eval_pass(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int pass);
 void launch_stage(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int kind);

@@ -304,3 +304,53 @@ def test_fortran_driver_matches_python_path():
     assert int(f_step) == 200
     assert float(f_ke) == pytest.approx(s[0], rel=1e-14)
     assert float(f_l2) == pytest.approx(s[4], rel=1e-12)
+
+
+# --- FP32-mixed form -------------------------------------------------------------------
+MIXED_TOL = 1e-4   # FP32 pair sums (Kahan): error per field relative to the field's vector norm
+
+
+@pytest.mark.parametrize("n,perturb", [(100, False), (64, True)])
+def test_mixed_single_step_vs_fp64_oracle(n, perturb):
+    """FP32-mixed evaluation against the FP64 oracle: every output's error, stated."""
+    say(f"test_mixed_single_step_vs_fp64_oracle n={n} perturb={perturb}")
+    st = perturbed_tgv(n, seed=n + 1) if perturb else None
+    sim, o = pair(n, state=st, precision=sph2d.FP32_MIXED)
+    evaluate_both(sim, o)
+    assert np.array_equal(sim.local("name"), o.i32("name"))
+    assert np.array_equal(sim.local("surface"), o.i32("surface"))
+    groups = {"fra_v": ["fra_v"], "sum_dwdx": ["sum_dwdx_x", "sum_dwdx_y"],
+              "tau": ["tau_xx", "tau_xy", "tau_yy"], "grad_rho": ["grad_rho_x", "grad_rho_y"],
+              "drhodt": ["drhodt"], "a": ["ax", "ay"]}
+    worst = {}
+    for key, fs in groups.items():
+        ref = {f: o.f64(f) for f in fs}
+        scale = max(np.max(np.abs(r)) for r in ref.values())
+        for f in fs:
+            worst[f] = float(np.max(np.abs(sim.local(f) - ref[f])) / scale)
+    say("mixed-form errors vs FP64 oracle:", worst)
+    for f, e in worst.items():
+        assert e < MIXED_TOL, (f, e)
+    sim.shift_particle_position(1)
+
+
+def test_mixed_ke_error_vs_fp64_at_t05():
+    """North-star check at 256^2: KE/KE0 of the FP32-mixed form within 1e-4 (relative)
+    of the FP64 form at t = 0.5 (12,800 predictor-corrector steps)."""
+    say("test_mixed_ke_error_vs_fp64_at_t05")
+    n = 256
+    steps = int(round(0.5 / (1e-4 * 100.0 / n)))
+    ke = {}
+    for prec in (sph2d.FP64, sph2d.FP32_MIXED):
+        prm = sph2d.tgv_params(n, integrator=2, precision=prec)
+        sim = sph2d.Simulation(prm)
+        sim.init_tgv()
+        sim.run(1, steps)
+        s = sim.sample()
+        ke[prec] = s
+        sim.close()
+    err = abs(ke[1][0] - ke[0][0]) / ke[0][0]
+    say(f"256^2 t=0.5 KE fp64={ke[0][0]:.10f} mixed={ke[1][0]:.10f} rel err={err:.3e}; "
+        f"L2u fp64={ke[0][4]:.6e} mixed={ke[1][4]:.6e}")
+    assert err < 1e-4
+    assert abs(ke[0][0] - np.exp(-16 * np.pi ** 2 * 0.01 * 0.5)) < 0.05
