@@ -84,6 +84,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-side", type=int, default=512)
     ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--forms", choices=("both", "fp64", "mixed"), default="both")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB: a plain copy kernel of "
+                         "known size that scales rocprofv3's FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every GPU box here has one, and the "
@@ -134,8 +138,28 @@ def main() -> int:
               file=sys.stderr, flush=True)
         return out
 
-    r64 = run_form(sph2d.FP64)
-    r32 = run_form(sph2d.FP32_MIXED)
+    if args.copy_calibration_mb > 0:
+        hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        hip.hipFree.argtypes = [ctypes.c_void_p]
+        nbytes = args.copy_calibration_mb << 20
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        hip.hipMalloc(ctypes.byref(a), nbytes)
+        hip.hipMalloc(ctypes.byref(b), nbytes)
+        hip.hipMemcpy(b, a, nbytes, 3)           # hipMemcpyDeviceToDevice
+        hip.hipDeviceSynchronize()
+        hip.hipFree(a)
+        hip.hipFree(b)
+        print(f"# copy calibration: {nbytes} bytes device to device", file=sys.stderr, flush=True)
+
+    r64 = run_form(sph2d.FP64) if args.forms in ("both", "fp64") else None
+    r32 = run_form(sph2d.FP32_MIXED) if args.forms in ("both", "mixed") else None
+    if r64 is None or r32 is None:
+        r = r64 or r32
+        print(json.dumps({"partial_run": args.forms, "ms_per_step": 1000.0 * r["total_s"] / args.steps,
+                          "force_pass_ms": r["force_ms"], "pass_a_ms": r["passa_ms"],
+                          "evaluation_ms": r["eval_ms"]}), flush=True)
+        return 0
     N = n * n
     sps64 = N * args.steps / r64["total_s"]
     sps32 = N * args.steps / r32["total_s"]
