@@ -108,6 +108,14 @@ void make_devparams(const sph2d_params& p, DevParams& P) {
   P.inv_wdx = 1.0 / P.w_dx;
   // any pair with r <= 3h has r^2 below this bound, so the exact test r <= 3h stays decisive
   P.cut2 = P.three_h * P.three_h * (1.0 + 1e-10);
+  {
+    // FP32 coordinates are within 2^-24*|x| of the FP64 ones; positions stay within
+    // the box widened by the ghost band, so each FP32 offset is within 4 ulp of that span
+    const double span = std::max(p.box_x, p.box_y) + 8 * p.h;
+    const double eps = 4.0 * span * std::ldexp(1.0, -24);
+    const double bound = (P.three_h + 2 * eps) * (P.three_h + 2 * eps) * (1.0 + 1e-6);
+    P.cut2_32 = std::nextafter((float)bound, 1e30f);
+  }
   P.solver = p.solver;
   P.mls_order = p.mls_order;
   P.write_debug = 0;
@@ -229,7 +237,7 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   rc |= c->alloc(&B.cell_members, G);
   rc |= c->alloc(&B.name, L); rc |= c->alloc(&B.l_type, L); rc |= c->alloc(&B.l_cell, L);
   rc |= c->alloc(&B.l_pos, L); rc |= c->alloc(&B.l_vel, L); rc |= c->alloc(&B.l_rv, L);
-  rc |= c->alloc(&B.l_pc, L); rc |= c->alloc(&B.l_pos_bin, L);
+  rc |= c->alloc(&B.l_pc, L); rc |= c->alloc(&B.l_pos_bin, L); rc |= c->alloc(&B.l_pos32, L);
   rc |= c->alloc(&B.l_mass, L); rc |= c->alloc(&B.l_e, L);
   rc |= c->alloc(&B.fra_v, L); rc |= c->alloc(&B.sdw, L); rc |= c->alloc(&B.grad, L);
   rc |= c->alloc(&B.nrm, L); rc |= c->alloc(&B.shift, L); rc |= c->alloc(&B.avg, L);

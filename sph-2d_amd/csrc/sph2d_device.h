@@ -72,15 +72,18 @@ namespace sph2d {
 
 // Neighbour loop of the gather kernels. The 3 column ranges of the 3x3 cell
 // stencil form one candidate list (about 207 candidates, 36 of them inside 3h).
-// Phase 1 tests every candidate with the exact criterion of
-// find_particle_pair.f90:87 (1e-7 <= r <= 3h, r = sqrt(dx*dx+dy*dy)) and keeps
-// a 256-bit mask; phase 2 calls f(o) for the accepted candidates only, in
+// Phase 1 screens every candidate with FP32 positions against a bound that no
+// pair within 3h can exceed, then applies the exact FP64 criterion of
+// find_particle_pair.f90:87 (1e-7 <= r <= 3h, r = sqrt(dx*dx+dy*dy)) to the
+// survivors and keeps a 256-bit mask; phase 2 calls f(o) for the accepted candidates only, in
 // ascending candidate order, so the lanes of a wave stay converged on the
 // expensive pair body. Lists longer than 256 run in several chunks.
 template <class F>
 __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int* __restrict__ cell_start,
-                                                   const double2* __restrict__ pos, int s, double2 ps,
+                                                   const double2* __restrict__ pos,
+                                                   const float2* __restrict__ pos32, int s, double2 ps,
                                                    int cell, F&& f) {
+  const float2 ps32 = pos32[s];
   const CellRanges R = cell_ranges(P, cell_start, cell);
   const int n0 = R.hi[0] - R.lo[0], n1 = R.hi[1] - R.lo[1], n2 = R.hi[2] - R.lo[2];
   const int n01 = n0 + n1, total = n01 + n2;
@@ -92,12 +95,13 @@ __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int
     for (int k = 0; k < cn; ++k) {
       const int kk = c0 + k;
       const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
-      const double2 po = pos[o];
-      const double dx = ps.x - po.x, dy = ps.y - po.y;
-      const double r2 = dx * dx + dy * dy;
-      bool ok = (r2 <= P.cut2) && (o != s);
-      if (ok) {
-        const double r = sqrt(r2);
+      const float2 q = pos32[o];
+      const float fx = ps32.x - q.x, fy = ps32.y - q.y;
+      bool ok = (fx * fx + fy * fy <= P.cut2_32) && (o != s);
+      if (ok) {                                   // exact FP64 test of find_particle_pair.f90:87
+        const double2 po = pos[o];
+        const double dx = ps.x - po.x, dy = ps.y - po.y;
+        const double r = sqrt(dx * dx + dy * dy);
         ok = (r <= P.three_h) && (r >= P.r_min);
       }
       if (ok) {

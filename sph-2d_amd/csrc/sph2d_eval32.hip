@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
   const float drs = B.m_rv[s].x;
   const int ts = B.l_type[s];
   KSum fra, sdx, sdy, Sxx, Sxy, Syy, grx, gry;
-  for_each_neighbour(P, B.cell_start, B.l_pos, s, ps, B.l_cell[s], [&](int o) {
+  for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o) {
     if (ts != 2 && B.l_type[o] != 2) return;
     const double2 po = B.l_pos[o];
     const float dx = (float)(ps.x - po.x), dy = (float)(ps.y - po.y);
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
   const float inv_rhos = 1.0f / rhos;
   const bool linear = P.solver == SPH2D_SOLVER_LINEAR_RIEMANN;
   KSum drho, corr, ax, ay, shx, shy, agx, agy;
-  for_each_neighbour(P, B.cell_start, B.l_pos, s, ps, B.l_cell[s], [&](int o) {
+  for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o) {
     if (ts != 2 && B.l_type[o] != 2) return;
     const double2 po = B.l_pos[o];
     const float dx = (float)(ps.x - po.x), dy = (float)(ps.y - po.y);

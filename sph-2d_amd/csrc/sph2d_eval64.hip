@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
   const double rhos = B.l_rv[s].x;
   const int ts = B.l_type[s];
   double fra = 0.0, sdx = 0.0, sdy = 0.0, Sxx = 0.0, Sxy = 0.0, Syy = 0.0, grx = 0.0, gry = 0.0;
-  for_each_neighbour(P, B.cell_start, B.l_pos, s, ps, B.l_cell[s], [&](int o) {
+  for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o) {
     if (ts != 2 && B.l_type[o] != 2) return;     // no pair between two non-fluid particles
     const double2 po = B.l_pos[o];
     const double dx = ps.x - po.x, dy = ps.y - po.y;
@@ -147,7 +147,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_b2(DevParams P, Buffers B) {
   if (B.counters[C_NSURF] == 0) return;            // near_ was zeroed by pass A
   const int ts = B.l_type[s];
   int nr = 0;
-  for_each_neighbour(P, B.cell_start, B.l_pos, s, B.l_pos[s], B.l_cell[s], [&](int o) {
+  for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, B.l_pos[s], B.l_cell[s], [&](int o) {
     if (B.surf[o] == 1 && (ts == 2 || B.l_type[o] == 2)) nr = 1;
   });
   B.near_[s] = nr;
@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_c(DevParams P, Buffers B, int d
   double drho = 0.0, corr = 0.0, ax = 0.0, ay = 0.0, shx = 0.0, shy = 0.0, agx = 0.0, agy = 0.0;
   double iax = 0.0, iay = 0.0, avx = 0.0, avy = 0.0, aix = 0.0, aiy = 0.0;
   double spx = 0.0, spy = 0.0, lpx = 0.0, lpy = 0.0;
-  for_each_neighbour(P, B.cell_start, B.l_pos, s, ps, B.l_cell[s], [&](int o) {
+  for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o) {
     if (ts != 2 && B.l_type[o] != 2) return;
     const double2 po = B.l_pos[o];
     const double dx = ps.x - po.x, dy = ps.y - po.y;      // x_ij with i = s

@@ -331,6 +331,7 @@ __global__ void k_cell_rank_gather(DevParams P, Buffers B, int cap_global) {
   const double2 pos = B.g_pos[g];
   B.l_pos[l] = pos;
   B.l_pos_bin[l] = pos;
+  B.l_pos32[l] = make_float2((float)pos.x, (float)pos.y);
   B.l_vel[l] = B.g_vel[g];
   const double rho = B.g_rho[g];
   const double m = B.g_mass[g];
@@ -409,6 +410,7 @@ __global__ void k_stage(DevParams P, Buffers B, int kind) {
       v.y = bv.y + dt * a.y + avg.y;
       x.y = bx.y + dt * v.y;
       B.l_vel[s] = v; B.l_pos[s] = x;
+      B.l_pos32[s] = make_float2((float)x.x, (float)x.y);
       eos_volume(P, B, s, rho);
     }
     B.t_vel[s] = v; B.t_drho[s] = drho; B.t_acc[s] = a;    // :573-576
@@ -441,6 +443,7 @@ __global__ void k_stage(DevParams P, Buffers B, int kind) {
         x.y = bx.y + dt * v.y;
       }
       B.l_vel[s] = v; B.l_pos[s] = x;
+      B.l_pos32[s] = make_float2((float)x.x, (float)x.y);
       eos_volume(P, B, s, rho);
     }
     const int k = kind == ST_RK_HALF0 ? 0 : (kind == ST_RK_HALF1 ? 1 : 2);
@@ -475,38 +478,14 @@ void launch_stage(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, 
 
 // ---------------------------------------------------------------------------
 // Reinitialize_particle_density.f90:66-182 + MLS_Interpolation_In_Subdomain_Domain.f90
-// order 6: A = sum_k w_k p_k p_k^T + alpha^3 I, BRINV (full pivoting),
-// rho_i = sum_k rho_k Phi_k with Phi_k = (A^-1 B)(1,k) = w_k (A^-1 p_k)_1.
+// (order 6): A = sum_k w_k p_k p_k^T + alpha^3 I with p_k = (1, dx, dy, dx^2, dx dy, dy^2)/h
+// powers, rho_i = sum_k rho_k Phi_k with Phi_k = w_k (A^-1 p_k)_1 = (A^-1 e_1) . (w_k p_k).
+// The reference inverts A by full-pivot Gauss-Jordan (BRINV, function_module.f90:568-639)
+// and keeps row 1; A is symmetric positive definite here (w_k >= 0 inside the support,
+// alpha^3 = 1e-12 on the diagonal), so y = A^-1 e_1 comes from a Cholesky solve in
+// registers: same row, different rounding (relative ~ cond(A) * eps). Fewer than 9
+// neighbours fall back to the Shepard average, as the reference does (:284-295).
 // ---------------------------------------------------------------------------
-__device__ int brinv6(double* A) {   // function_module.f90:568-639, column-major 6x6
-  constexpr int N = 6;
-  int IS[N], JS[N];
-#define M(i, j) A[(i) + (j) * N]
-  for (int K = 0; K < N; ++K) {
-    double D = 0.0;
-    IS[K] = K; JS[K] = K;
-    for (int I = K; I < N; ++I)
-      for (int J = K; J < N; ++J)
-        if (fabs(M(I, J)) > D) { D = fabs(M(I, J)); IS[K] = I; JS[K] = J; }
-    if (D + 1.0 == 1.0) return 0;
-    for (int J = 0; J < N; ++J) { double T = M(K, J); M(K, J) = M(IS[K], J); M(IS[K], J) = T; }
-    for (int I = 0; I < N; ++I) { double T = M(I, K); M(I, K) = M(I, JS[K]); M(I, JS[K]) = T; }
-    M(K, K) = 1.0 / M(K, K);
-    for (int J = 0; J < N; ++J) if (J != K) M(K, J) = M(K, J) * M(K, K);
-    for (int I = 0; I < N; ++I)
-      if (I != K)
-        for (int J = 0; J < N; ++J)
-          if (J != K) M(I, J) = M(I, J) - M(I, K) * M(K, J);
-    for (int I = 0; I < N; ++I) if (I != K) M(I, K) = -M(I, K) * M(K, K);
-  }
-  for (int K = N - 1; K >= 0; --K) {
-    for (int J = 0; J < N; ++J) { double T = M(K, J); M(K, J) = M(JS[K], J); M(JS[K], J) = T; }
-    for (int I = 0; I < N; ++I) { double T = M(I, K); M(I, K) = M(I, IS[K]); M(I, IS[K]) = T; }
-  }
-#undef M
-  return 1;
-}
-
 __global__ void __launch_bounds__(kBlock) k_mls(DevParams P, Buffers B) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
@@ -514,48 +493,79 @@ __global__ void __launch_bounds__(kBlock) k_mls(DevParams P, Buffers B) {
   const int ts = B.l_type[s];
   if (ts != 2) { B.mls_rho[s] = 0.0; return; }     // :69 rho=0 for every non-fluid particle
   const double2 ps = B.l_pos[s];
-  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
-  const double cut2 = P.three_h * P.three_h * (1.0 + 1e-10);
-  double A[36];
+  // upper triangle of A, row-major over (m <= l): a00 a01 .. a05 a11 .. a55
+  double A[21];
 #pragma unroll
-  for (int k = 0; k < 36; ++k) A[k] = 0.0;
-  double bvec[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < 21; ++k) A[k] = 0.0;
+  double bv[6] = {0, 0, 0, 0, 0, 0};
   double sw = 0.0, srw = 0.0;
   int ne = 0;
-  for (int d = 0; d < 3; ++d)
-    for (int o = R.lo[d]; o < R.hi[d]; ++o) {
-      const double2 po = B.l_pos[o];
-      const double pdx = ps.x - po.x, pdy = ps.y - po.y;
-      const double r2 = pdx * pdx + pdy * pdy;
-      if (r2 > cut2) continue;
-      const double r = sqrt(r2);
-      if (!(r <= P.three_h && r >= P.r_min)) continue;
-      double w, dwx, dwy;
-      kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
-      const double a = (po.x - ps.x) / P.h, b = (po.y - ps.y) / P.h;
-      const double pv[6] = {1.0, a, b, a * a, (po.x - ps.x) / P.h * (po.y - ps.y) / P.h, b * b};
-      const double rho_o = B.l_rv[o].x;
+  for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o) {
+    const double2 po = B.l_pos[o];
+    const double pdx = ps.x - po.x, pdy = ps.y - po.y;
+    const double r = sqrt(pdx * pdx + pdy * pdy);
+    double w, dwx, dwy;
+    kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
+    const double a = (po.x - ps.x) * P.inv_h, b = (po.y - ps.y) * P.inv_h;
+    const double pv[6] = {1.0, a, b, a * a, a * b, b * b};
+    const double rho_o = B.l_rv[o].x;
+    int k = 0;
 #pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        const double bm = pv[m] * w;
+    for (int m = 0; m < 6; ++m) {
+      const double bm = pv[m] * w;
 #pragma unroll
-        for (int l = 0; l < 6; ++l) A[m + 6 * l] = A[m + 6 * l] + bm * pv[l];
-        bvec[m] = bvec[m] + rho_o * bm;
-      }
-      sw = sw + w;
-      srw = srw + rho_o * w;
-      ++ne;
+      for (int l = m; l < 6; ++l) { A[k] = A[k] + bm * pv[l]; ++k; }
+      bv[m] = bv[m] + rho_o * bm;
     }
+    sw = sw + w;
+    srw = srw + rho_o * w;
+    ++ne;
+  });
   const double al3 = (P.mls_alpha * P.mls_alpha) * P.mls_alpha;
-#pragma unroll
-  for (int m = 0; m < 6; ++m) A[m + 6 * m] = A[m + 6 * m] + al3;
-  double rho;
-  int ok = (ne > 8) ? brinv6(A) : 0;
+  double rho = 0.0;
+  bool ok = ne > 8;
   if (ok) {
-    rho = 0.0;
+    // Cholesky A = L L^T (L lower, row-major packed), then L z = e1, L^T y = z.
+    double L[21];
+    const int diag[6] = {0, 6, 11, 15, 18, 20};
+#define AU(m, l) A[diag[(m)] + (l) - (m)]          // upper-triangle access, m <= l
+#define LL(i, j) L[(i) * ((i) + 1) / 2 + (j)]      // lower-triangle access, j <= i
 #pragma unroll
-    for (int l = 0; l < 6; ++l) rho = rho + A[0 + 6 * l] * bvec[l];
-  } else {
+    for (int i = 0; i < 6; ++i) {
+#pragma unroll
+      for (int j = 0; j <= i; ++j) {
+        double v = AU(j, i) + (i == j ? al3 : 0.0);
+#pragma unroll
+        for (int k = 0; k < j; ++k) v = v - LL(i, k) * LL(j, k);
+        if (i == j) {
+          ok = ok && (v > 0.0);
+          LL(i, i) = sqrt(v > 0.0 ? v : 1.0);
+        } else {
+          LL(i, j) = v / LL(j, j);
+        }
+      }
+    }
+    double z[6], y[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double v = (i == 0) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < i; ++k) v = v - LL(i, k) * z[k];
+      z[i] = v / LL(i, i);
+    }
+#pragma unroll
+    for (int i = 5; i >= 0; --i) {
+      double v = z[i];
+#pragma unroll
+      for (int k = i + 1; k < 6; ++k) v = v - LL(k, i) * y[k];
+      y[i] = v / LL(i, i);
+    }
+#undef AU
+#undef LL
+#pragma unroll
+    for (int l = 0; l < 6; ++l) rho = rho + y[l] * bv[l];
+  }
+  if (!ok) {
     rho = srw / sw;
     atomicAdd(&B.counters[C_MLSFB], 1);
   }
@@ -675,23 +685,41 @@ __global__ void k_sample_partial(DevParams P, Buffers B, int cap_fluid) {
     for (int k = 0; k < kRed; ++k) B.red[blockIdx.x * kRed + k] = lds[0][k];
 }
 
+// One block: thread t folds partial blocks t, t+256, ... in order, then a fixed tree.
 __global__ void k_sample_final(Buffers B, int nb, double* out) {
-  if (threadIdx.x != 0) return;
+  __shared__ double lds[kBlock][kRed];
   double v[kRed];
+#pragma unroll
   for (int k = 0; k < kRed; ++k) v[k] = 0.0;
-  for (int b = 0; b < nb; ++b)
+  for (int b = threadIdx.x; b < nb; b += kBlock)
+#pragma unroll
     for (int k = 0; k < kRed; ++k) {
       const bool mx = (k == 1) || (k >= 2 && (k - 2) % 3 == 0);
       const double x = B.red[b * kRed + k];
       v[k] = mx ? (v[k] > x ? v[k] : x) : v[k] + x;
     }
-  for (int k = 0; k < kRed; ++k) out[k] = v[k];
+#pragma unroll
+  for (int k = 0; k < kRed; ++k) lds[threadIdx.x][k] = v[k];
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+#pragma unroll
+      for (int k = 0; k < kRed; ++k) {
+        const bool mx = (k == 1) || (k >= 2 && (k - 2) % 3 == 0);
+        const double a = lds[threadIdx.x][k], c = lds[threadIdx.x + off][k];
+        lds[threadIdx.x][k] = mx ? (a > c ? a : c) : a + c;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kRed; ++k) out[k] = lds[0][k];
 }
 
 void launch_sample(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, double* dev_out14) {
   const int nb = grid_for(cap_fluid);
   hipLaunchKernelGGL(k_sample_partial, dim3(nb), dim3(kBlock), 0, s, P, B, cap_fluid);
-  hipLaunchKernelGGL(k_sample_final, dim3(1), dim3(64), 0, s, B, nb, dev_out14);
+  hipLaunchKernelGGL(k_sample_final, dim3(1), dim3(kBlock), 0, s, B, nb, dev_out14);
 }
 
 // ---------------------------------------------------------------------------

@@ -36,6 +36,7 @@ struct DevParams {
   double mls_alpha;
   double surf_lim;                       // REAL(4) 1.414 * h
   double inv_h, inv_c0, inv_sv, inv_wdx, cut2;  // reciprocals and the squared-distance pre-test
+  float cut2_32;                         // FP32 pre-test bound: never below (3h + FP32 position error)^2
   int solver, write_debug;
   int mls_order;
   int mixed;                             // 1: FP32-mixed evaluation (mirrors are written)
@@ -60,6 +61,7 @@ struct Buffers {
   // local order
   int *name, *l_type, *l_cell;
   double2 *l_pos, *l_vel, *l_rv, *l_pc, *l_pos_bin;
+  float2 *l_pos32;                        // FP32 copy of l_pos for the neighbour pre-test
   double *l_mass, *l_e;
   // one force evaluation
   double *fra_v;
