@@ -73,10 +73,10 @@ namespace sph2d {
 // Neighbour loop of the gather kernels. The 3 column ranges of the 3x3 cell
 // stencil form one candidate list (about 207 candidates, 36 of them inside 3h).
 // Phase 1 screens every candidate with FP32 positions against a bound that no
-// pair within 3h can exceed, then applies the exact FP64 criterion of
-// find_particle_pair.f90:87 (1e-7 <= r <= 3h, r = sqrt(dx*dx+dy*dy)) to the
-// survivors and keeps a 256-bit mask; phase 2 calls f(o) for the accepted candidates only, in
-// ascending candidate order, so the lanes of a wave stay converged on the
+// pair within 3h can exceed and keeps a 256-bit mask of the survivors (about 40
+// of 207); phase 2 applies the exact FP64 criterion of find_particle_pair.f90:87
+// (1e-7 <= r <= 3h, r = sqrt(dx*dx+dy*dy)) to them and calls f(o) for the
+// accepted ones, in ascending candidate order, so the lanes of a wave stay converged on the
 // expensive pair body. Lists longer than 256 run in several chunks.
 template <class F>
 __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int* __restrict__ cell_start,
@@ -97,13 +97,7 @@ __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int
       const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
       const float2 q = pos32[o];
       const float fx = ps32.x - q.x, fy = ps32.y - q.y;
-      bool ok = (fx * fx + fy * fy <= P.cut2_32) && (o != s);
-      if (ok) {                                   // exact FP64 test of find_particle_pair.f90:87
-        const double2 po = pos[o];
-        const double dx = ps.x - po.x, dy = ps.y - po.y;
-        const double r = sqrt(dx * dx + dy * dy);
-        ok = (r <= P.three_h) && (r >= P.r_min);
-      }
+      const bool ok = (fx * fx + fy * fy <= P.cut2_32) && (o != s);
       if (ok) {
         const uint64_t bit = 1ull << (k & 63);
         if (k < 64) m0 |= bit;
@@ -121,7 +115,10 @@ __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int
       if (w == 0) m0 ^= bit; else if (w == 1) m1 ^= bit; else if (w == 2) m2 ^= bit; else m3 ^= bit;
       const int kk = c0 + w * 64 + b;
       const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
-      f(o);
+      const double2 po = pos[o];                  // exact FP64 test of find_particle_pair.f90:87
+      const double dx = ps.x - po.x, dy = ps.y - po.y;
+      const double r = sqrt(dx * dx + dy * dy);
+      if ((r <= P.three_h) && (r >= P.r_min)) f(o);
     }
   }
 }
