@@ -116,6 +116,18 @@ void make_devparams(const sph2d_params& p, DevParams& P) {
     const double bound = (P.three_h + 2 * eps) * (P.three_h + 2 * eps) * (1.0 + 1e-6);
     P.cut2_32 = std::nextafter((float)bound, 1e30f);
   }
+  {
+    // The reference keeps a pair when r = sqrt(dx*dx+dy*dy) satisfies 1e-7 <= r <= 3h.
+    // sqrt is correctly rounded and monotone, so these are bounds on r2 itself.
+    double x = P.three_h * P.three_h;
+    while (std::sqrt(x) > P.three_h) x = std::nextafter(x, 0.0);
+    while (std::sqrt(std::nextafter(x, 1e300)) <= P.three_h) x = std::nextafter(x, 1e300);
+    P.r2_max = x;
+    double y = P.r_min * P.r_min;
+    while (std::sqrt(y) < P.r_min) y = std::nextafter(y, 1e300);
+    while (std::sqrt(std::nextafter(y, 0.0)) >= P.r_min) y = std::nextafter(y, 0.0);
+    P.r2_min = y;
+  }
   P.solver = p.solver;
   P.mls_order = p.mls_order;
   P.write_debug = 0;

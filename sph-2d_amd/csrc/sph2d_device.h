@@ -78,6 +78,13 @@ namespace sph2d {
 // (1e-7 <= r <= 3h, r = sqrt(dx*dx+dy*dy)) to them and calls f(o) for the
 // accepted ones, in ascending candidate order, so the lanes of a wave stay converged on the
 // expensive pair body. Lists longer than 256 run in several chunks.
+// find_particle_pair.f90:87 with r2 = dx*dx+dy*dy rounded as the reference rounds it
+// (no contraction, whatever the translation unit's flags).
+__device__ __forceinline__ bool is_pair_r2(const DevParams& P, double dx, double dy) {
+  const double r2 = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+  return (r2 <= P.r2_max) && (r2 >= P.r2_min);
+}
+
 template <class F>
 __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int* __restrict__ cell_start,
                                                    const double2* __restrict__ pos,
@@ -90,21 +97,21 @@ __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int
   const int l0 = R.lo[0], l1 = R.lo[1] - n0, l2 = R.lo[2] - n01;
   for (int c0 = 0; c0 < total; c0 += 256) {
     uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-    const int cn = min(256, total - c0);
-#pragma unroll 4
-    for (int k = 0; k < cn; ++k) {
-      const int kk = c0 + k;
-      const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
-      const float2 q = pos32[o];
-      const float fx = ps32.x - q.x, fy = ps32.y - q.y;
-      const bool ok = (fx * fx + fy * fy <= P.cut2_32) && (o != s);
-      if (ok) {
-        const uint64_t bit = 1ull << (k & 63);
-        if (k < 64) m0 |= bit;
-        else if (k < 128) m1 |= bit;
-        else if (k < 192) m2 |= bit;
-        else m3 |= bit;
+#pragma unroll 1
+    for (int w = 0; w < 4; ++w) {
+      const int base = c0 + 64 * w;
+      const int cn = min(64, total - base);
+      uint64_t mw = 0;
+#pragma unroll 2
+      for (int k = 0; k < cn; ++k) {
+        const int kk = base + k;
+        const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
+        const float2 q = pos32[o];
+        const float fx = ps32.x - q.x, fy = ps32.y - q.y;
+        const bool ok = (fx * fx + fy * fy <= P.cut2_32) && (o != s);
+        mw |= (uint64_t)ok << k;
       }
+      if (w == 0) m0 = mw; else if (w == 1) m1 = mw; else if (w == 2) m2 = mw; else m3 = mw;
     }
 #pragma unroll 1
     while ((m0 | m1 | m2 | m3) != 0) {
@@ -115,10 +122,8 @@ __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int
       if (w == 0) m0 ^= bit; else if (w == 1) m1 ^= bit; else if (w == 2) m2 ^= bit; else m3 ^= bit;
       const int kk = c0 + w * 64 + b;
       const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
-      const double2 po = pos[o];                  // exact FP64 test of find_particle_pair.f90:87
-      const double dx = ps.x - po.x, dy = ps.y - po.y;
-      const double r = sqrt(dx * dx + dy * dy);
-      if ((r <= P.three_h) && (r >= P.r_min)) f(o);
+      const double2 po = pos[o];
+      if (is_pair_r2(P, ps.x - po.x, ps.y - po.y)) f(o);
     }
   }
 }

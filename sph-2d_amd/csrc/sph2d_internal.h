@@ -37,6 +37,7 @@ struct DevParams {
   double surf_lim;                       // REAL(4) 1.414 * h
   double inv_h, inv_c0, inv_sv, inv_wdx, cut2;  // reciprocals and the squared-distance pre-test
   float cut2_32;                         // FP32 pre-test bound: never below (3h + FP32 position error)^2
+  double r2_max, r2_min;                 // fl(sqrt(r2)) <= 3h  <=>  r2 <= r2_max; >= 1e-7 <=> r2 >= r2_min
   int solver, write_debug;
   int mls_order;
   int mixed;                             // 1: FP32-mixed evaluation (mirrors are written)
