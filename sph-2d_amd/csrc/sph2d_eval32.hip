@@ -169,11 +169,11 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
     const float vbx = (vs.x + vo.x) * 0.5f, vby = (vs.y + vo.y) * 0.5f;
     const float cbar = (pcs.y + pco.y) * 0.5f;
     const float rbar = Q.rho0 + (rvs.x + rvo.x) * 0.5f;
-    const float inv_rbar = 1.0f / rbar;
+    const float inv_rbar = __builtin_amdgcn_rcpf(rbar);    // v_rcp_f32, 1 ulp
     float vsx, vsy, Ps;
     if (linear) {
       const float zs = rhos * pcs.y, zo = rhoo * pco.y;
-      const float inv_z = 1.0f / (zs + zo);
+      const float inv_z = __builtin_amdgcn_rcpf(zs + zo);
       const float Us = (zs * UR + zo * UL + (pcs.x - pco.x)) * inv_z;
       vsx = Us * ex;
       vsy = Us * ey;
@@ -191,14 +191,14 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
     if (nes) { vsx = vbx; vsy = vby; Ps = Pbar; }
     const float ti = (vs.x - vsx) * K.dwx + (vs.y - vsy) * K.dwy;
     drho.add(2.0f * rhos * Vo * ti);
-    const float rij = inv_rhos / rhoo;
+    const float rij = inv_rhos * __builtin_amdgcn_rcpf(rhoo);
     const float2 mo = B.m_mass[o];
     const float fp = mo.x * (-2.0f * Ps * rij);
     float tx = fp * K.dwx, ty = fp * K.dwy;
     const float vijx = vs.x - vo.x, vijy = vs.y - vo.y;
     const float vx = vijx * dx + vijy * dy;
     if (vx < 0.0f) {
-      const float phi = Q.h * vx / (r2 + Q.fai_h2);
+      const float phi = Q.h * vx * __builtin_amdgcn_rcpf(r2 + Q.fai_h2);
       const float Pi = (-(Q.alpha * cbar * phi) + Q.beta * (phi * phi)) * inv_rbar;
       tx += mo.x * (-Pi * K.dwx);
       ty += mo.x * (-Pi * K.dwy);

@@ -314,25 +314,20 @@ __global__ void k_cell_scatter(Buffers B, int cap_global) {
 // Rank inside the cell by gid gives the reference's within-cell order; then the
 // global -> subdomain copy (:666-765) with EOS and volume (single_step_compute.f90:62-73,
 // Assign_boundary_particles.f90:303-321).
-__global__ void k_cell_rank_gather(DevParams P, Buffers B, int cap_global) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= cap_global) return;
-  const int ng = B.counters[C_NFLUID] + B.counters[C_NGHOST];
-  if (g >= ng) return;
-  const int key = B.g_key[g];
-  if (key < 0) return;
-  const int s0 = B.cell_start[key], s1 = B.cell_start[key + 1];
-  int rank = 0;
-  for (int m = s0; m < s1; ++m) rank += (B.cell_members[m] < g);
-  atomicMax(&B.counters[C_MAXCELL], s1 - s0);
-  const int l = s0 + rank;
+// One wavefront per cell: each lane holds one member gid, its rank among the cell's
+// members is the count of smaller gids (broadcast lane by lane), which gives the
+// reference's within-cell order; then the global -> subdomain copy (:666-765) with
+// EOS and volume (single_step_compute.f90:62-73, Assign_boundary_particles.f90:303-321)
+// into the cell's contiguous slice of the local arrays.
+__device__ __forceinline__ void gather_one(const DevParams& P, Buffers& B, int g, int l, int key) {
   B.name[l] = g;
   B.l_cell[l] = key;
   const double2 pos = B.g_pos[g];
   B.l_pos[l] = pos;
   B.l_pos_bin[l] = pos;
   B.l_pos32[l] = make_float2((float)pos.x, (float)pos.y);
-  B.l_vel[l] = B.g_vel[g];
+  const double2 vel = B.g_vel[g];
+  B.l_vel[l] = vel;
   const double rho = B.g_rho[g];
   const double m = B.g_mass[g];
   const int t = B.g_type[g];
@@ -344,8 +339,29 @@ __global__ void k_cell_rank_gather(DevParams P, Buffers B, int cap_global) {
   B.l_pc[l] = make_double2(p, c);
   B.l_rv[l] = make_double2(rho, m / rho);
   if (P.mixed) {
-    write_mirror(P, B, l, B.g_vel[g], rho, m / rho, p, c);
+    write_mirror(P, B, l, vel, rho, m / rho, p, c);
     B.m_mass[l] = make_float2((float)m, (float)B.g_mass[l]);   // particle_mass(j) with j local
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cell_rank_gather(DevParams P, Buffers B, int n_cells) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int cell = blockIdx.x * (kBlock / 64) + wave;
+  if (cell >= n_cells) return;
+  const int s0 = B.cell_start[cell];
+  const int cnt = B.cell_start[cell + 1] - s0;
+  if (cnt == 0) return;
+  if (lane == 0) atomicMax(&B.counters[C_MAXCELL], cnt);
+  for (int base = 0; base < cnt; base += 64) {
+    const int idx = base + lane;
+    const int my = idx < cnt ? B.cell_members[s0 + idx] : 0x7fffffff;
+    int rank = 0;
+    for (int b2 = 0; b2 < cnt; b2 += 64) {
+      const int v = (b2 + lane < cnt) ? B.cell_members[s0 + b2 + lane] : 0x7fffffff;
+      const int m = min(64, cnt - b2);
+      for (int k = 0; k < m; ++k) rank += (__builtin_amdgcn_readlane(v, k) < my);
+    }
+    if (idx < cnt) gather_one(P, B, my, s0 + rank, cell);
   }
 }
 
@@ -362,7 +378,8 @@ void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_
   exclusive_scan(s, B.cell_count, n_cells, B.cell_start, B.scan_blocks, B.counters + C_NLOCAL,
                  B.cell_start + n_cells);
   hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, B, cap_global);
-  hipLaunchKernelGGL(k_cell_rank_gather, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, P, B, cap_global);
+  hipLaunchKernelGGL(k_cell_rank_gather, dim3((n_cells + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, s, P, B,
+                     n_cells);
 }
 
 // ---------------------------------------------------------------------------
