@@ -472,7 +472,6 @@ int sph2d_devide_block(sph2d_ctx* c, int32_t it) {
   const bool ana = (it % c->prm.sample_every) == 0;
   const double t = it * c->prm.dt / (c->prm.box_x / c->prm.U);   // Nondimensionalized_real_time
   launch_fill_i32(c->stream, c->B.counters + C_NILL, 1, 0);
-  launch_fill_i32(c->stream, c->B.counters + C_MAXCELL, 1, 0);
   launch_devide_block(c->stream, c->P, c->B, c->cap_global, c->n_cells, ana, t);
   HIP_OR_FAIL(hipGetLastError());
   return SPH2D_OK;
@@ -715,6 +714,7 @@ int sph2d_download_pairs(sph2d_ctx* c, int64_t capacity, int32_t* lo, int32_t* h
 
 int sph2d_get_diagnostics(sph2d_ctx* c, sph2d_diag* d) {
   if (!c || !d) return fail(SPH2D_ERR_ARGUMENT, "NULL argument");
+  if (c->have_state) launch_max_cell(c->stream, c->B, c->n_cells);
   int cnt[C_COUNT];
   int rc = read_counters(c, cnt);
   if (rc) return rc;

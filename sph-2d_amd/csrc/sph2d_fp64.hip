@@ -351,7 +351,6 @@ __global__ void __launch_bounds__(kBlock) k_cell_rank_gather(DevParams P, Buffer
   const int s0 = B.cell_start[cell];
   const int cnt = B.cell_start[cell + 1] - s0;
   if (cnt == 0) return;
-  if (lane == 0) atomicMax(&B.counters[C_MAXCELL], cnt);
   for (int base = 0; base < cnt; base += 64) {
     const int idx = base + lane;
     const int my = idx < cnt ? B.cell_members[s0 + idx] : 0x7fffffff;
@@ -363,6 +362,24 @@ __global__ void __launch_bounds__(kBlock) k_cell_rank_gather(DevParams P, Buffer
     }
     if (idx < cnt) gather_one(P, B, my, s0 + rank, cell);
   }
+}
+
+// Largest cell population (diagnostic, on demand): one block folds cell_start.
+__global__ void k_max_cell(Buffers B, int n_cells) {
+  __shared__ int lds[kBlock];
+  int m = 0;
+  for (int c = threadIdx.x; c < n_cells; c += kBlock) m = max(m, B.cell_start[c + 1] - B.cell_start[c]);
+  lds[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) lds[threadIdx.x] = max(lds[threadIdx.x], lds[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) B.counters[C_MAXCELL] = lds[0];
+}
+
+void launch_max_cell(hipStream_t s, Buffers& B, int n_cells) {
+  hipLaunchKernelGGL(k_max_cell, dim3(1), dim3(kBlock), 0, s, B, n_cells);
 }
 
 void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_global, int n_cells,

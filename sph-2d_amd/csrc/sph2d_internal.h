@@ -114,6 +114,7 @@ void launch_kernel_n(hipStream_t s, int n, const double* r, const double* dx, co
 void launch_eos_n(hipStream_t s, int n, double rho0, double c0sq, const double* rho,
                   const int* type, double* p, double* c);
 void launch_fill_i32(hipStream_t s, int* p, int n, int v);
+void launch_max_cell(hipStream_t s, Buffers& B, int n_cells);
 void launch_count_pairs(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
 
 // Stage kinds for launch_stage.
