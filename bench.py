@@ -90,9 +90,8 @@ def main() -> int:
                          "known size that scales rocprofv3's FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: every GPU box here has one, and the "
-                                   "multi-GPU form (spatial slabs with a halo exchange per evaluation) "
-                                   "is designed but not built"}))
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU form (spatial slabs "
+                                   "with a halo exchange per evaluation) is designed but not built"}))
         return 2
     rank = int(os.environ.get("RANK", "0"))
 

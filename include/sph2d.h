@@ -36,7 +36,9 @@ enum {
 /* Solver variants (Solver_Normalized_*_Riemann_Delta_SPS of the reference). */
 enum {
   SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN = 0, /* Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90 (default, single_step_compute.f90:137) */
-  SPH2D_SOLVER_LINEAR_RIEMANN = 1           /* Solver_Normalized_Linear_Riemann_Delta_SPS.f90 */
+  SPH2D_SOLVER_LINEAR_RIEMANN = 1,          /* Solver_Normalized_Linear_Riemann_Delta_SPS.f90 */
+  SPH2D_SOLVER_HLLC = 2,                    /* Solver_Normalized_HLLC_Linear_Riemann_Delta_SPS.f90 (single_step_compute.f90:143) */
+  SPH2D_SOLVER_MUSCL = 3                    /* Solver_Normalized_MUSCL_Linear_Riemann_Delta_SPS.f90 (single_step_compute.f90:146) */
 };
 
 /* Integrators of time_iteration.f90 (leapforg_or_eular, information_module.f90:303). */
@@ -163,6 +165,7 @@ int sph2d_sample(sph2d_ctx* ctx, double* out14);
  *      grad_rho_x grad_rho_y drhodt ax ay shift_x shift_y normal_x normal_y
  *      internal_x internal_y visc_x visc_y art_internal_x art_internal_y sps_x sps_y
  *      lap_x lap_y (the last 10 only after sph2d_set_debug(ctx,1))
+ *      grad_p_x grad_p_y grad_u_x grad_u_y grad_v_x grad_v_y (SPH2D_SOLVER_MUSCL only)
  * i32: type name(gid, 0-based) surface near_surface */
 int sph2d_download_local(sph2d_ctx* ctx, const char* name, void* out, int32_t n);
 /* Pair set of the current state (find_particle_pair.f90:64-123): unordered pairs

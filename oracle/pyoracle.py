@@ -14,6 +14,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libsph2d_ref.so")
 
+# solver variants (sph2d_oracle.h ORC_SOLVER_*)
+ORC_SOLVER_MODIFIED_LINEAR_RIEMANN = 0
+ORC_SOLVER_LINEAR_RIEMANN = 1
+ORC_SOLVER_HLLC = 2
+ORC_SOLVER_MUSCL = 3
+
 
 class OrcParams(ctypes.Structure):
     _fields_ = [
@@ -86,6 +92,9 @@ def lib():
         L.orc_matrix_inversion_less_three.restype = ctypes.c_int32
         L.orc_set_fluid_state.argtypes = [ctypes.c_void_p, ctypes.c_int32] + [P(ctypes.c_double)] * 6 + [P(ctypes.c_int32)]
         L.orc_set_fluid_state.restype = ctypes.c_int32
+        D = ctypes.c_double
+        L.orc_muscl_limiter.argtypes = [D, D, D, P(D), P(D)]
+        L.orc_hllc_star.argtypes = [P(D), D, D, D, P(D), D, D, D, P(D), D, P(D), P(D)]
         _lib = L
     return _lib
 
@@ -236,6 +245,23 @@ def matrix_inversion_less_three(a: np.ndarray):
     ok = lib().orc_matrix_inversion_less_three(m.shape[0], m.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                                res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return ok, res
+
+
+def muscl_limiter(delphi_i: float, delphi_j: float, d_phi_ij: float):
+    li, lj = ctypes.c_double(), ctypes.c_double()
+    lib().orc_muscl_limiter(delphi_i, delphi_j, d_phi_ij, ctypes.byref(li), ctypes.byref(lj))
+    return li.value, lj.value
+
+
+def hllc_star(v_left, p_left, rho_left, c_left, v_right, p_right, rho_right, c_right, e, x_over_t):
+    vl = np.ascontiguousarray(v_left, dtype=np.float64)
+    vr = np.ascontiguousarray(v_right, dtype=np.float64)
+    ev = np.ascontiguousarray(e, dtype=np.float64)
+    vs = np.zeros(2)
+    ps = ctypes.c_double()
+    lib().orc_hllc_star(_dp(vl), p_left, rho_left, c_left, _dp(vr), p_right, rho_right, c_right,
+                        _dp(ev), x_over_t, _dp(vs), ctypes.byref(ps))
+    return vs, ps.value
 
 
 # compiled reference pieces (oracle/_ref) --------------------------------------

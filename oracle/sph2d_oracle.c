@@ -60,6 +60,7 @@ struct orc_ctx {
   double *drhodt, *dedt, *ax, *ay;
   double *ia_x, *ia_y, *av_x, *av_y, *ai_x, *ai_y, *sps_x, *sps_y, *lap_x, *lap_y, *ext_x, *ext_y;
   double *avg_x, *avg_y, *corr, *grx, *gry;
+  double *gpx, *gpy, *gux, *guy, *gvx, *gvy;    /* MUSCL: grad_pressure, grad_velocity_u/_v */
   /* shift */
   double *shx, *shy;
   /* time integration (local index arrays, persist across steps) */
@@ -219,6 +220,7 @@ static void alloc_local(orc_ctx* c, int cap) {
   A(drhodt); A(dedt); A(ax); A(ay);
   A(ia_x); A(ia_y); A(av_x); A(av_y); A(ai_x); A(ai_y); A(sps_x); A(sps_y);
   A(lap_x); A(lap_y); A(ext_x); A(ext_y); A(avg_x); A(avg_y); A(corr); A(grx); A(gry);
+  A(gpx); A(gpy); A(gux); A(guy); A(gvx); A(gvy);
   A(shx); A(shy);
   A(b_rho); A(b_e); A(b_x); A(b_y); A(b_u); A(b_v);
   A(t_u); A(t_v); A(t_drho); A(t_de); A(t_ax); A(t_ay);
@@ -264,7 +266,7 @@ void orc_destroy(orc_ctx* c) {
                   c->Syy, c->txx, c->txy, c->tyy, c->drhodt, c->dedt, c->ax, c->ay, c->ia_x,
                   c->ia_y, c->av_x, c->av_y, c->ai_x, c->ai_y, c->sps_x, c->sps_y, c->lap_x,
                   c->lap_y, c->ext_x, c->ext_y, c->avg_x, c->avg_y, c->corr, c->grx, c->gry,
-                  c->shx, c->shy, c->b_rho, c->b_e, c->b_x, c->b_y, c->b_u, c->b_v, c->t_u,
+                  c->gpx, c->gpy, c->gux, c->guy, c->gvx, c->gvy, c->shx, c->shy, c->b_rho, c->b_e, c->b_x, c->b_y, c->b_u, c->b_v, c->t_u,
                   c->t_v, c->t_drho, c->t_de, c->t_ax, c->t_ay, c->mls_rho,
                   c->cell_count, c->cell_list, c->near_n, c->near_name,
                   c->pi, c->pj, c->pw, c->pdwx, c->pdwy, c->pdx, c->pdy, c->pvx, c->pvy, c->prd,
@@ -672,6 +674,94 @@ static void define_freesurface(orc_ctx* c) {
 }
 
 /* ------------------------------------------------------------------ */
+/* function_module.f90:34-80: limited MUSCL slopes (Toro, beta_lim=1.3) */
+/* ------------------------------------------------------------------ */
+void orc_muscl_limiter(double delphi_i, double delphi_j, double d_phi_ij,
+                       double* del_phi_lim_i, double* del_phi_lim_j) {
+  const double beta_lim = 1.3;                                    /* real(8) 1.3d0 */
+  const double diff_factor = 0.5;
+  double lim, fi, fj, a, b;
+  if (delphi_j > 0.0) {                                           /* :47-61 */
+    a = beta_lim * delphi_i < delphi_j ? beta_lim * delphi_i : delphi_j;
+    b = delphi_i < beta_lim * delphi_j ? delphi_i : beta_lim * delphi_j;
+    lim = 0.0;
+    if (a > lim) lim = a;
+    if (b > lim) lim = b;
+    if (lim > 0.5 * d_phi_ij && fabs(lim) > 0.0) {
+      if (delphi_i > delphi_j) { fi = diff_factor; fj = 0.0; }
+      else { fi = 0.0; fj = diff_factor; }
+      *del_phi_lim_i = d_phi_ij * fi;
+      *del_phi_lim_j = d_phi_ij * fj;
+    } else {
+      *del_phi_lim_i = lim;
+      *del_phi_lim_j = lim;
+    }
+  } else {                                                        /* :62-76 */
+    a = beta_lim * delphi_i > delphi_j ? beta_lim * delphi_i : delphi_j;
+    b = delphi_i > beta_lim * delphi_j ? delphi_i : beta_lim * delphi_j;
+    lim = 0.0;
+    if (a < lim) lim = a;
+    if (b < lim) lim = b;
+    if (lim < 0.5 * d_phi_ij && fabs(lim) > 0.0) {
+      if (delphi_i < delphi_j) { fi = diff_factor; fj = 0.0; }
+      else { fi = 0.0; fj = diff_factor; }
+      *del_phi_lim_i = d_phi_ij * fi;
+      *del_phi_lim_j = d_phi_ij * fj;
+    } else {
+      *del_phi_lim_i = lim;
+      *del_phi_lim_j = lim;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Solver_Normalized_HLLC_Linear_Riemann_Delta_SPS.f90:362-490:        */
+/* rotate to the pair frame, acoustic U*, P*, wave speeds S_L, S_R     */
+/* (Q = 1 unless P* exceeds that side's pressure), sample the fan at   */
+/* x/t, rotate back. Rho_asterisk (:387) is computed there but unused. */
+/* ------------------------------------------------------------------ */
+void orc_hllc_star(const double* vl, double PL, double rhoL, double cL,
+                   const double* vr, double PR, double rhoR, double cR,
+                   const double* e, double x_over_t, double* vstar, double* pstar) {
+  const double sn = e[1], cs = e[0];
+  const double L1 = vl[0] * cs + vl[1] * sn, L2 = -vl[0] * sn + vl[1] * cs;
+  const double R1 = vr[0] * cs + vr[1] * sn, R2 = -vr[0] * sn + vr[1] * cs;
+  const double Us = (L1 * cL * rhoL + R1 * cR * rhoR + PL - PR) / (rhoL * cL + rhoR * cR);
+  const double Ps = (PL * cL * rhoL + PR * cR * rhoR - cL * rhoL * cR * rhoR * (R1 - L1)) /
+                    (rhoL * cL + rhoR * cR);
+  const double ex7 = (double)0.142857142f;
+  const double RsL = rhoL * pow(Ps / PL, ex7);
+  const double RsR = rhoR * pow(Ps / PR, ex7);
+  double QL = 1.0, QR = 1.0, t;
+  if (!(Ps <= PL)) {
+    t = ((rhoL / RsL) * (PL - Ps)) / ((rhoL - RsL) * (cL * cL));
+    QL = t >= 0.0 ? sqrt(t) : 1.0;
+  }
+  const double SL = L1 - cL * QL;
+  if (!(Ps <= PR)) {
+    t = ((rhoR / RsR) * (PR - Ps)) / ((rhoR - RsR) * (cR * cR));
+    QR = t >= 0.0 ? sqrt(t) : 1.0;
+  }
+  const double SR = R1 + cR * QR;
+  const double S = Us;
+  double H1, H2;
+  if (SL > x_over_t) {
+    H1 = L1; H2 = L2;
+  } else if (S >= x_over_t && x_over_t >= SL) {
+    const double q = rhoL * ((SL - L1) / (SL - S));
+    H1 = (S * q) / q; H2 = (L2 * q) / q;
+  } else if (SR >= x_over_t && x_over_t >= S) {
+    const double q = rhoR * ((SR - R1) / (SR - S));
+    H1 = (S * q) / q; H2 = (R2 * q) / q;
+  } else {
+    H1 = R1; H2 = R2;
+  }
+  vstar[0] = H1 * cs - H2 * sn;
+  vstar[1] = H1 * sn + H2 * cs;
+  *pstar = Ps;
+}
+
+/* ------------------------------------------------------------------ */
 /* Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90:126-635    */
 /* (and Solver_Normalized_Linear_Riemann_Delta_SPS.f90:362-371)       */
 /* ------------------------------------------------------------------ */
@@ -719,7 +809,31 @@ static void solver_riemann_delta_sps(orc_ctx* c) {
     c->txy[i] = (nut * c->Sxy[i]) * c->rho[i];
     c->tyy[i] = (2 * nut * c->Syy[i] - kk - blin) * c->rho[i];
   }
+  if (P->solver == ORC_SOLVER_MUSCL) {
+    /* MUSCL_..._SPS.f90:145-185: grad p, grad u, grad v with the renormalised
+     * dwdx (identity inverse), summed like grad rho */
+    memset(c->gpx, 0, sizeof(double) * n); memset(c->gpy, 0, sizeof(double) * n);
+    memset(c->gux, 0, sizeof(double) * n); memset(c->guy, 0, sizeof(double) * n);
+    memset(c->gvx, 0, sizeof(double) * n); memset(c->gvy, 0, sizeof(double) * n);
+    for (int64_t k = 0; k < c->npair; ++k) {
+      int i = c->pi[k], j = c->pj[k];
+      double* Ti = &c->Tinv[i * 4];
+      double* Tj = &c->Tinv[j * 4];
+      double d1 = c->pdwx[k], d2 = c->pdwy[k];
+      double mi1 = Ti[0] * d1 + Ti[2] * d2, mi2 = Ti[1] * d1 + Ti[3] * d2;
+      double mj1 = Tj[0] * d1 + Tj[2] * d2, mj2 = Tj[1] * d1 + Tj[3] * d2;
+      double pd = c->p[j] - c->p[i];
+      double ud = -c->pvx[k], vd = -c->pvy[k];
+      c->gpx[i] = c->gpx[i] + pd * c->vol[j] * mi1; c->gpy[i] = c->gpy[i] + pd * c->vol[j] * mi2;
+      c->gpx[j] = c->gpx[j] + pd * c->vol[i] * mj1; c->gpy[j] = c->gpy[j] + pd * c->vol[i] * mj2;
+      c->gux[i] = c->gux[i] + ud * c->vol[j] * mi1; c->guy[i] = c->guy[i] + ud * c->vol[j] * mi2;
+      c->gux[j] = c->gux[j] + ud * c->vol[i] * mj1; c->guy[j] = c->guy[j] + ud * c->vol[i] * mj2;
+      c->gvx[i] = c->gvx[i] + vd * c->vol[j] * mi1; c->gvy[i] = c->gvy[i] + vd * c->vol[j] * mi2;
+      c->gvx[j] = c->gvx[j] + vd * c->vol[i] * mj1; c->gvy[j] = c->gvy[j] + vd * c->vol[i] * mj2;
+    }
+  }
   const double sqc0 = sqrt(P->c0sq);
+  const double x_over_t = c->dx / P->dt;                         /* HLLC :444 interior_dx/dt */
   for (int64_t k = 0; k < c->npair; ++k) {                       /* :300-502 */
     int i = c->pi[k], j = c->pj[k];
     double PL = c->p[i], PR = c->p[j];
@@ -729,12 +843,33 @@ static void solver_riemann_delta_sps(orc_ctx* c) {
     double xij[2] = {c->pdx[k], c->pdy[k]};
     double dist = sqrt(xij[0] * xij[0] + xij[1] * xij[1]);
     double ev[2] = {-(xij[0] / dist), -(xij[1] / dist)};
+    if (P->solver == ORC_SOLVER_MUSCL) {                         /* MUSCL :464-504 */
+      double xr[2] = {-xij[0], -xij[1]};                         /* x_j - x_i */
+      double li, lj, dpi, dpj;
+      dpi = 0.5 * (c->gpx[i] * xr[0] + c->gpy[i] * xr[1]);
+      dpj = -0.5 * (c->gpx[j] * xr[0] + c->gpy[j] * xr[1]);
+      orc_muscl_limiter(dpi, dpj, c->p[j] - c->p[i], &li, &lj);
+      PL = c->p[i] + li;
+      PR = c->p[j] - lj;
+      dpi = 0.5 * (c->gux[i] * xr[0] + c->guy[i] * xr[1]);
+      dpj = -0.5 * (c->gux[j] * xr[0] + c->guy[j] * xr[1]);
+      orc_muscl_limiter(dpi, dpj, -c->pvx[k], &li, &lj);
+      Iv[0] = c->u[i] + li;
+      Jv[0] = c->u[j] - lj;
+      dpi = 0.5 * (c->gvx[i] * xr[0] + c->gvy[i] * xr[1]);
+      dpj = -0.5 * (c->gvx[j] * xr[0] + c->gvy[j] * xr[1]);
+      orc_muscl_limiter(dpi, dpj, -c->pvy[k], &li, &lj);
+      Iv[1] = c->v[i] + li;
+      Jv[1] = c->v[j] - lj;
+    }
     double UL = Iv[0] * ev[0] + Iv[1] * ev[1];
     double UR = Jv[0] * ev[0] + Jv[1] * ev[1];
     double Ub = (UL + UR) / 2.0;
     double cb = c->pc_[k], rb = c->prho_[k];
     double Us, vs[2], Ps;
-    if (P->solver == ORC_SOLVER_LINEAR_RIEMANN) {
+    if (P->solver == ORC_SOLVER_HLLC) {
+      orc_hllc_star(Iv, PL, c->rho[i], c->c[i], Jv, PR, c->rho[j], c->c[j], ev, x_over_t, vs, &Ps);
+    } else if (P->solver == ORC_SOLVER_LINEAR_RIEMANN) {
       double zi = c->rho[i] * c->c[i], zj = c->rho[j] * c->c[j];
       Us = zi * UR + zj * UL + PL - PR;
       Us = Us / (zi + zj);
@@ -1214,6 +1349,8 @@ static const fld F64[] = {
   FL("avg_vel_x", avg_x, 0), FL("avg_vel_y", avg_y, 0), FL("rho_correct", corr, 0),
   FL("grad_rho_x", grx, 0), FL("grad_rho_y", gry, 0), FL("shift_x", shx, 0),
   FL("shift_y", shy, 0),
+  FL("grad_p_x", gpx, 0), FL("grad_p_y", gpy, 0), FL("grad_u_x", gux, 0), FL("grad_u_y", guy, 0),
+  FL("grad_v_x", gvx, 0), FL("grad_v_y", gvy, 0),
   FL("T", T, 3), FL("T_inv", Tinv, 3),
   FL("g_x", gx, 1), FL("g_y", gy, 1), FL("g_u", gu, 1), FL("g_v", gv, 1),
   FL("g_mass", gmass, 1), FL("g_rho", grho, 1), FL("g_h", gh, 1), FL("g_c", gc, 1),

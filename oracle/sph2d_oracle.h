@@ -42,8 +42,10 @@ typedef struct {
   int32_t rate_reuse;        /* 0 = reference (stale local index), 1 = carried by gid  */
 } orc_params;
 
-enum { ORC_SOLVER_MODIFIED_LINEAR_RIEMANN = 0,
-       ORC_SOLVER_LINEAR_RIEMANN = 1 };
+enum { ORC_SOLVER_MODIFIED_LINEAR_RIEMANN = 0,   /* Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90 */
+       ORC_SOLVER_LINEAR_RIEMANN = 1,            /* Solver_Normalized_Linear_Riemann_Delta_SPS.f90 */
+       ORC_SOLVER_HLLC = 2,                      /* Solver_Normalized_HLLC_Linear_Riemann_Delta_SPS.f90 */
+       ORC_SOLVER_MUSCL = 3 };                   /* Solver_Normalized_MUSCL_Linear_Riemann_Delta_SPS.f90 */
 
 typedef struct orc_ctx orc_ctx;
 
@@ -56,6 +58,15 @@ void orc_compute_kernel(int32_t dim, double r, const double* dx, double h, doubl
 void orc_status_equation(const orc_ctx* c, double rho, double e, int32_t type, double* p, double* cs);
 int32_t orc_brinv(double* a_colmajor, int32_t n);
 int32_t orc_matrix_inversion_less_three(int32_t n, const double* a_colmajor, double* res_colmajor);
+/* function_module.f90:34-80 (MUSCL_Limiter). */
+void orc_muscl_limiter(double delphi_i, double delphi_j, double d_phi_ij,
+                       double* del_phi_lim_i, double* del_phi_lim_j);
+/* HLLC star state of one pair in the reference's pair order (i = left, j = right),
+ * e = unit vector from i to j, x_over_t = interior_dx/dt
+ * (Solver_Normalized_HLLC_Linear_Riemann_Delta_SPS.f90:362-490). */
+void orc_hllc_star(const double* v_left, double p_left, double rho_left, double c_left,
+                   const double* v_right, double p_right, double rho_right, double c_right,
+                   const double* e, double x_over_t, double* v_star, double* p_star);
 
 /* Initial lattice, Distribute_Initial_Particle.f90:204-249. */
 void orc_init_tgv(orc_ctx* c);

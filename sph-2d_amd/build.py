@@ -1,8 +1,9 @@
 """Builds libsph2d.so (gfx950) in-tree with hipcc.
 
-Two translation units: the FP64 step (no FMA contraction, for parity with the
-reference build) and the C-ABI/host driver. Packed FP32 instructions are
-disabled on every HIP line.
+Four translation units: the FP64 step and the FP64 force evaluation (no FMA
+contraction, for parity with the reference build), the FP32-mixed force
+evaluation (contraction allowed) and the C-ABI/host driver. Packed FP32
+instructions are disabled on every HIP line.
 """
 from __future__ import annotations
 
@@ -29,7 +30,7 @@ UNITS = [
     ("sph2d_capi.hip", ["-ffp-contract=off"]),
 ]
 
-HEADERS = ["sph2d_internal.h", "sph2d_device.h"]
+HEADERS = ["sph2d_internal.h", "sph2d_device.h", "sph2d_riemann.h"]
 
 
 def _run(cmd):

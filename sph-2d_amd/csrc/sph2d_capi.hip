@@ -128,6 +128,7 @@ void make_devparams(const sph2d_params& p, DevParams& P) {
     while (std::sqrt(std::nextafter(y, 0.0)) >= P.r_min) y = std::nextafter(y, 0.0);
     P.r2_min = y;
   }
+  P.x_over_t = p.dx / p.dt;            // HLLC: interior_dx/dt
   P.solver = p.solver;
   P.mls_order = p.mls_order;
   P.write_debug = 0;
@@ -142,8 +143,8 @@ int check_params(const sph2d_params* p) {
     return fail(SPH2D_ERR_ARGUMENT, "h, dx, box and dt must be positive");
   if (p->integrator != SPH2D_INTEGRATOR_PC && p->integrator != SPH2D_INTEGRATOR_RK)
     return fail(SPH2D_ERR_UNSUPPORTED, "integrator must be 2 (predictor-corrector) or 3 (RK-variant)");
-  if (p->solver != SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && p->solver != SPH2D_SOLVER_LINEAR_RIEMANN)
-    return fail(SPH2D_ERR_UNSUPPORTED, "solver variant not built");
+  if (p->solver < SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN || p->solver > SPH2D_SOLVER_MUSCL)
+    return fail(SPH2D_ERR_UNSUPPORTED, "solver must be one of SPH2D_SOLVER_*");
   if (p->precision != SPH2D_FP64 && p->precision != SPH2D_FP32_MIXED)
     return fail(SPH2D_ERR_UNSUPPORTED, "precision must be SPH2D_FP64 or SPH2D_FP32_MIXED");
   if (p->rate_reuse != SPH2D_RATE_REUSE_REFERENCE)
@@ -267,6 +268,11 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   if (c->P.mixed) {
     rc |= c->alloc(&B.m_vel, L); rc |= c->alloc(&B.m_rv, L); rc |= c->alloc(&B.m_pc, L);
     rc |= c->alloc(&B.m_mass, L); rc |= c->alloc(&B.m_grad, L); rc |= c->alloc(&B.m_tau, L);
+  }
+  B.mg = nullptr; B.m_mg = nullptr;
+  if (p->solver == SPH2D_SOLVER_MUSCL) {
+    rc |= c->alloc(&B.mg, 3 * L);
+    if (c->P.mixed) rc |= c->alloc(&B.m_mg, 3 * L);
   }
   B.dbg = nullptr; B.pair_lo = nullptr; B.pair_hi = nullptr;
   B.kstride = (int)L;
@@ -661,6 +667,17 @@ int sph2d_download_local(sph2d_ctx* c, const char* name, void* out, int32_t n) {
   if (s == "tau_yy") return d1(B.tau, 3, 2);
   if (s == "grad_rho_x") return d2(B.grad, 0);
   if (s == "grad_rho_y") return d2(B.grad, 1);
+  static const char* mg_names[6] = {"grad_p_x", "grad_p_y", "grad_u_x", "grad_u_y", "grad_v_x", "grad_v_y"};
+  for (int k = 0; k < 6; ++k)
+    if (s == mg_names[k]) {
+      if (!B.mg) return fail(SPH2D_ERR_STATE, "pressure and velocity gradients exist for the MUSCL solver only");
+      std::vector<double2> t((size_t)nl * 3);
+      HIP_OR_FAIL(hipMemcpyAsync(t.data(), B.mg, (size_t)nl * 3 * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+      HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+      double* o = static_cast<double*>(out);
+      for (int i = 0; i < nl; ++i) o[i] = (k & 1) ? t[(size_t)i * 3 + k / 2].y : t[(size_t)i * 3 + k / 2].x;
+      return SPH2D_OK;
+    }
   if (s == "drhodt") return d1(B.drhodt, 1, 0);
   if (s == "ax") return d2(B.acc, 0);
   if (s == "ay") return d2(B.acc, 1);

@@ -10,6 +10,7 @@
 // 128 in FP64).
 #include "sph2d_internal.h"
 #include "sph2d_device.h"
+#include "sph2d_riemann.h"
 
 namespace sph2d {
 
@@ -28,7 +29,7 @@ struct KSum {
 
 struct DevParams32 {
   float inv_h, e9, kfactor, inv_sv, rho0, inv_c0, h, fai_h2, alpha, beta, two_nu;
-  float delta_alpha_h, two_delta_h, shift_beta_h2, shift_R, inv_wdx, epsilon2;
+  float delta_alpha_h, two_delta_h, shift_beta_h2, shift_R, inv_wdx, epsilon2, x_over_t;
 };
 
 static DevParams32 make32(const DevParams& P) {
@@ -50,6 +51,7 @@ static DevParams32 make32(const DevParams& P) {
   Q.shift_R = (float)P.shift_R;
   Q.inv_wdx = (float)P.inv_wdx;
   Q.epsilon2 = (float)(P.epsilon_xsph * 2);
+  Q.x_over_t = (float)P.x_over_t;
   return Q;
 }
 
@@ -70,6 +72,7 @@ __device__ __forceinline__ KernelTerm32 kernel_term32(const DevParams32& Q, floa
 }
 
 // pass A (mixed): see k_eval_a in sph2d_eval64.hip for the reference lines.
+template <bool MUSCL>
 __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q, Buffers B) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
@@ -77,8 +80,10 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
   const double2 ps = B.l_pos[s];
   const float2 vs = B.m_vel[s];
   const float drs = B.m_rv[s].x;
+  const float pres = B.m_pc[s].x;
   const int ts = B.l_type[s];
   KSum fra, sdx, sdy, Sxx, Sxy, Syy, grx, gry;
+  KSum gpx, gpy, gux, guy, gvx, gvy;
   for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o) {
     if (ts != 2 && B.l_type[o] != 2) return;
     const double2 po = B.l_pos[o];
@@ -100,12 +105,30 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
     const float rd = (rvo.x - drs) * Vo;
     grx.add(rd * K.dwx);
     gry.add(rd * K.dwy);
+    if (MUSCL) {
+      const float pd = (B.m_pc[o].x - pres) * Vo;
+      gpx.add(pd * K.dwx);
+      gpy.add(pd * K.dwy);
+      const float ud = vdx * Vo, vd = vdy * Vo;
+      gux.add(ud * K.dwx);
+      guy.add(ud * K.dwy);
+      gvx.add(vd * K.dwx);
+      gvy.add(vd * K.dwy);
+    }
   });
   const double f = (double)fra.s;
   B.fra_v[s] = f;
   B.sdw[s] = make_double2((double)sdx.s, (double)sdy.s);
   B.grad[s] = make_double2((double)grx.s, (double)gry.s);
   B.m_grad[s] = make_float2(grx.s, gry.s);
+  if (MUSCL) {
+    B.m_mg[3 * s + 0] = make_float2(gpx.s, gpy.s);
+    B.m_mg[3 * s + 1] = make_float2(gux.s, guy.s);
+    B.m_mg[3 * s + 2] = make_float2(gvx.s, gvy.s);
+    B.mg[3 * s + 0] = make_double2((double)gpx.s, (double)gpy.s);
+    B.mg[3 * s + 1] = make_double2((double)gux.s, (double)guy.s);
+    B.mg[3 * s + 2] = make_double2((double)gvx.s, (double)gvy.s);
+  }
   double txx = 0.0, txy = 0.0, tyy = 0.0;
   if (ts == 2) {                                   // SPS closure in FP64 (per particle)
     const double sxx = Sxx.s, sxy = Sxy.s, syy = Syy.s;
@@ -132,6 +155,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
 }
 
 // pass C (mixed): see k_eval_c in sph2d_eval64.hip for the reference lines.
+template <int SOLVER>
 __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q, Buffers B, int do_shift) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
@@ -146,7 +170,9 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
   const int nes = B.near_[s];
   const float rhos = Q.rho0 + rvs.x;
   const float inv_rhos = 1.0f / rhos;
-  const bool linear = P.solver == SPH2D_SOLVER_LINEAR_RIEMANN;
+  constexpr bool kMuscl = SOLVER == SPH2D_SOLVER_MUSCL;
+  float2 gps = make_float2(0.f, 0.f), gus = gps, gvs = gps;
+  if (kMuscl) { gps = B.m_mg[3 * s]; gus = B.m_mg[3 * s + 1]; gvs = B.m_mg[3 * s + 2]; }
   KSum drho, corr, ax, ay, shx, shy, agx, agy;
   for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o) {
     if (ts != 2 && B.l_type[o] != 2) return;
@@ -171,7 +197,45 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
     const float rbar = Q.rho0 + (rvs.x + rvo.x) * 0.5f;
     const float inv_rbar = __builtin_amdgcn_rcpf(rbar);    // v_rcp_f32, 1 ulp
     float vsx, vsy, Ps;
-    if (linear) {
+    float2 vown = vs;
+    if (SOLVER == SPH2D_SOLVER_HLLC || SOLVER == SPH2D_SOLVER_MUSCL) {
+      const bool left = s < o;                          // pair_i = lower local index
+      const float epx = left ? ex : -ex, epy = left ? ey : -ey;
+      const float2 vi = left ? vs : vo, vj = left ? vo : vs;
+      const float2 pci = left ? pcs : pco, pcj = left ? pco : pcs;
+      if (SOLVER == SPH2D_SOLVER_HLLC) {
+        const float rhoi = left ? rhos : rhoo, rhoj = left ? rhoo : rhos;
+        hllc_star<float>(vi.x, vi.y, pci.x, rhoi, pci.y, vj.x, vj.y, pcj.x, rhoj, pcj.y, epx, epy, Q.x_over_t,
+                         vsx, vsy, Ps);
+      } else {
+        const float2* mgo = B.m_mg + 3 * o;
+        const float2 gpo = mgo[0], guo = mgo[1], gvo = mgo[2];
+        const float2 gpi = left ? gps : gpo, gpj = left ? gpo : gps;
+        const float2 gui = left ? gus : guo, guj = left ? guo : gus;
+        const float2 gvi = left ? gvs : gvo, gvj = left ? gvo : gvs;
+        const float xrx = left ? -dx : dx, xry = left ? -dy : dy;
+        float li, lj;
+        muscl_limiter<float>(0.5f * (gpi.x * xrx + gpi.y * xry), -(0.5f * (gpj.x * xrx + gpj.y * xry)),
+                             pcj.x - pci.x, li, lj);
+        const float PL = pci.x + li, PR = pcj.x - lj;
+        muscl_limiter<float>(0.5f * (gui.x * xrx + gui.y * xry), -(0.5f * (guj.x * xrx + guj.y * xry)),
+                             vj.x - vi.x, li, lj);
+        const float Iu = vi.x + li, Ju = vj.x - lj;
+        muscl_limiter<float>(0.5f * (gvi.x * xrx + gvi.y * xry), -(0.5f * (gvj.x * xrx + gvj.y * xry)),
+                             vj.y - vi.y, li, lj);
+        const float Iv = vi.y + li, Jv = vj.y - lj;
+        const float UL = Iu * epx + Iv * epy;
+        const float UR = Ju * epx + Jv * epy;
+        const float dUs = 0.5f * (PL - PR) * inv_rbar * Q.inv_c0;   // U* - U_bar
+        vsx = dUs * epx + vbx;
+        vsy = dUs * epy + vby;
+        const float dUp = UL - UR;
+        float bR = 3.0f * (dUp > 0.0f ? dUp : 0.0f);
+        bR = bR < cbar ? bR : cbar;
+        Ps = Pbar + 0.5f * bR * rbar * dUp;
+        vown = left ? make_float2(Iu, Iv) : make_float2(Ju, Jv);
+      }
+    } else if (SOLVER == SPH2D_SOLVER_LINEAR_RIEMANN) {
       const float zs = rhos * pcs.y, zo = rhoo * pco.y;
       const float inv_z = __builtin_amdgcn_rcpf(zs + zo);
       const float Us = (zs * UR + zo * UL + (pcs.x - pco.x)) * inv_z;
@@ -189,7 +253,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
       (void)Ub;
     }
     if (nes) { vsx = vbx; vsy = vby; Ps = Pbar; }
-    const float ti = (vs.x - vsx) * K.dwx + (vs.y - vsy) * K.dwy;
+    const float ti = (vown.x - vsx) * K.dwx + (vown.y - vsy) * K.dwy;
     drho.add(2.0f * rhos * Vo * ti);
     const float rij = inv_rhos * __builtin_amdgcn_rcpf(rhoo);
     const float2 mo = B.m_mass[o];
@@ -219,7 +283,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
     ty += lf * vijy;
     ax.add(tx);
     ay.add(ty);
-    if (Q.epsilon2 != 0.0f && !linear) {
+    if (SOLVER == SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && Q.epsilon2 != 0.0f) {
       const float f = Q.epsilon2 * mo.x / (rhos + rhoo) * K.w;
       agx.add(-f * vijx);
       agy.add(-f * vijy);
@@ -243,11 +307,29 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
 }
 
 void launch_eval32_a(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
-  hipLaunchKernelGGL(k_eval_a32, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, make32(P), B);
+  const int g = grid_for(cap_local);
+  if (P.solver == SPH2D_SOLVER_MUSCL)
+    hipLaunchKernelGGL(k_eval_a32<true>, dim3(g), dim3(kBlock), 0, s, P, make32(P), B);
+  else
+    hipLaunchKernelGGL(k_eval_a32<false>, dim3(g), dim3(kBlock), 0, s, P, make32(P), B);
 }
 
 void launch_eval32_c(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
-  hipLaunchKernelGGL(k_eval_c32, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, make32(P), B, 1);
+  const int g = grid_for(cap_local);
+  const DevParams32 Q = make32(P);
+  switch (P.solver) {
+    case SPH2D_SOLVER_LINEAR_RIEMANN:
+      hipLaunchKernelGGL(k_eval_c32<SPH2D_SOLVER_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, s, P, Q, B, 1);
+      break;
+    case SPH2D_SOLVER_HLLC:
+      hipLaunchKernelGGL(k_eval_c32<SPH2D_SOLVER_HLLC>, dim3(g), dim3(kBlock), 0, s, P, Q, B, 1);
+      break;
+    case SPH2D_SOLVER_MUSCL:
+      hipLaunchKernelGGL(k_eval_c32<SPH2D_SOLVER_MUSCL>, dim3(g), dim3(kBlock), 0, s, P, Q, B, 1);
+      break;
+    default:
+      hipLaunchKernelGGL(k_eval_c32<SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, s, P, Q, B, 1);
+  }
 }
 
 }  // namespace sph2d

@@ -14,6 +14,7 @@
 // Built with -ffp-contract=off; reciprocals replace repeated divisions.
 #include "sph2d_internal.h"
 #include "sph2d_device.h"
+#include "sph2d_riemann.h"
 
 namespace sph2d {
 
@@ -39,8 +40,11 @@ __device__ __forceinline__ KernelTerm kernel_term(const DevParams& P, double dx,
 // ---------------------------------------------------------------------------
 // pass A: volume fraction, sum dW, SPS strain -> tau, grad rho
 // Define_freesurface_and_Normailized_derivates.f90:79-131,141-159 and
-// Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90:144-296,508-526
+// Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90:144-296,508-526.
+// MUSCL also sums grad p, grad u, grad v the same way
+// (Solver_Normalized_MUSCL_Linear_Riemann_Delta_SPS.f90:145-185).
 // ---------------------------------------------------------------------------
+template <bool MUSCL>
 __global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
@@ -48,8 +52,10 @@ __global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
   const double2 ps = B.l_pos[s];
   const double2 vs = B.l_vel[s];
   const double rhos = B.l_rv[s].x;
+  const double pres = B.l_pc[s].x;
   const int ts = B.l_type[s];
   double fra = 0.0, sdx = 0.0, sdy = 0.0, Sxx = 0.0, Sxy = 0.0, Syy = 0.0, grx = 0.0, gry = 0.0;
+  double gpx = 0.0, gpy = 0.0, gux = 0.0, guy = 0.0, gvx = 0.0, gvy = 0.0;
   for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o) {
     if (ts != 2 && B.l_type[o] != 2) return;     // no pair between two non-fluid particles
     const double2 po = B.l_pos[o];
@@ -69,10 +75,25 @@ __global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
     const double rd = (rvo.x - rhos) * Vo;
     grx = grx + rd * K.dwx;
     gry = gry + rd * K.dwy;
+    if (MUSCL) {
+      const double pd = (B.l_pc[o].x - pres) * Vo;
+      gpx = gpx + pd * K.dwx;
+      gpy = gpy + pd * K.dwy;
+      const double ud = vdx * Vo, vd = vdy * Vo;
+      gux = gux + ud * K.dwx;
+      guy = guy + ud * K.dwy;
+      gvx = gvx + vd * K.dwx;
+      gvy = gvy + vd * K.dwy;
+    }
   });
   B.fra_v[s] = fra;
   B.sdw[s] = make_double2(sdx, sdy);
   B.grad[s] = make_double2(grx, gry);
+  if (MUSCL) {
+    B.mg[3 * s + 0] = make_double2(gpx, gpy);
+    B.mg[3 * s + 1] = make_double2(gux, guy);
+    B.mg[3 * s + 2] = make_double2(gvx, gvy);
+  }
   double txx = 0.0, txy = 0.0, tyy = 0.0;
   if (ts == 2) {                                   // Solver...SPS.f90:256-296
     const double master = Sxx * Sxx + Syy * Syy + 0.0;
@@ -158,11 +179,14 @@ __global__ void __launch_bounds__(kBlock) k_eval_b2(DevParams P, Buffers B) {
 // delta-SPH internal, SPS, XSPH, delta-SPH density correction, shift sums.
 // Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90:300-635,
 // Solver_Normalized_Linear_Riemann_Delta_SPS.f90:362-371,
+// Solver_Normalized_HLLC_Linear_Riemann_Delta_SPS.f90:362-483,
+// Solver_Normalized_MUSCL_Linear_Riemann_Delta_SPS.f90:464-518,
 // single_step_compute.f90:152-171, Shift_particle_position.f90:78-103.
 // DEBUG keeps the five acceleration components apart and writes them.
+// SOLVER is one of SPH2D_SOLVER_*: each variant is its own kernel.
 // ---------------------------------------------------------------------------
-template <bool DEBUG>
-__global__ void __launch_bounds__(kBlock) k_eval_c(DevParams P, Buffers B, int do_shift) {
+template <bool DEBUG, int SOLVER>
+__device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int do_shift) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
   if (s >= n) return;
@@ -175,7 +199,9 @@ __global__ void __launch_bounds__(kBlock) k_eval_c(DevParams P, Buffers B, int d
   const double2 grs = B.grad[s];
   const int nes = B.near_[s];
   const double inv_rhos = 1.0 / rhos;
-  const bool linear = P.solver == SPH2D_SOLVER_LINEAR_RIEMANN;
+  constexpr bool kMuscl = SOLVER == SPH2D_SOLVER_MUSCL;
+  double2 gps = make_double2(0.0, 0.0), gus = gps, gvs = gps;
+  if (kMuscl) { gps = B.mg[3 * s]; gus = B.mg[3 * s + 1]; gvs = B.mg[3 * s + 2]; }
   double drho = 0.0, corr = 0.0, ax = 0.0, ay = 0.0, shx = 0.0, shy = 0.0, agx = 0.0, agy = 0.0;
   double iax = 0.0, iay = 0.0, avx = 0.0, avy = 0.0, aix = 0.0, aiy = 0.0;
   double spx = 0.0, spy = 0.0, lpx = 0.0, lpy = 0.0;
@@ -203,7 +229,47 @@ __global__ void __launch_bounds__(kBlock) k_eval_c(DevParams P, Buffers B, int d
     const double rbar = (rhos + rhoo) * 0.5;
     const double inv_rbar = 1.0 / rbar;
     double vsx, vsy, Ps;
-    if (linear) {
+    double2 vown = vs;                                    // MUSCL: own reconstructed velocity
+    if (SOLVER == SPH2D_SOLVER_HLLC || SOLVER == SPH2D_SOLVER_MUSCL) {
+      // the reference's pair order: pair_i = lower local index, e from pair_i to pair_j
+      const bool left = s < o;
+      const double epx = left ? ex : -ex, epy = left ? ey : -ey;
+      const double2 vi = left ? vs : vo, vj = left ? vo : vs;
+      const double2 pci = left ? pcs : pco, pcj = left ? pco : pcs;
+      if (SOLVER == SPH2D_SOLVER_HLLC) {
+        const double rhoi = left ? rhos : rhoo, rhoj = left ? rhoo : rhos;
+        hllc_star<double>(vi.x, vi.y, pci.x, rhoi, pci.y, vj.x, vj.y, pcj.x, rhoj, pcj.y, epx, epy,
+                          P.x_over_t, vsx, vsy, Ps);
+      } else {
+        const double2* mgo = B.mg + 3 * o;
+        const double2 gpo = mgo[0], guo = mgo[1], gvo = mgo[2];
+        const double2 gpi = left ? gps : gpo, gpj = left ? gpo : gps;
+        const double2 gui = left ? gus : guo, guj = left ? guo : gus;
+        const double2 gvi = left ? gvs : gvo, gvj = left ? gvo : gvs;
+        const double xrx = left ? -dx : dx, xry = left ? -dy : dy;   // x_j - x_i of the pair
+        double li, lj;
+        muscl_limiter<double>(0.5 * (gpi.x * xrx + gpi.y * xry), -(0.5 * (gpj.x * xrx + gpj.y * xry)),
+                              pcj.x - pci.x, li, lj);
+        const double PL = pci.x + li, PR = pcj.x - lj;
+        muscl_limiter<double>(0.5 * (gui.x * xrx + gui.y * xry), -(0.5 * (guj.x * xrx + guj.y * xry)),
+                              vj.x - vi.x, li, lj);
+        const double Iu = vi.x + li, Ju = vj.x - lj;
+        muscl_limiter<double>(0.5 * (gvi.x * xrx + gvi.y * xry), -(0.5 * (gvj.x * xrx + gvj.y * xry)),
+                              vj.y - vi.y, li, lj);
+        const double Iv = vi.y + li, Jv = vj.y - lj;
+        const double UL = Iu * epx + Iv * epy;
+        const double UR = Ju * epx + Jv * epy;
+        const double Ub = (UL + UR) * 0.5;
+        const double Us = Ub + 0.5 * (PL - PR) * inv_rbar * P.inv_c0;
+        vsx = (Us - Ub) * epx + vbx;
+        vsy = (Us - Ub) * epy + vby;
+        const double dUp = UL - UR;
+        double bR = 3.0 * (dUp > 0.0 ? dUp : 0.0);
+        bR = bR < cbar ? bR : cbar;
+        Ps = Pbar + 0.5 * bR * rbar * dUp;
+        vown = left ? make_double2(Iu, Iv) : make_double2(Ju, Jv);
+      }
+    } else if (SOLVER == SPH2D_SOLVER_LINEAR_RIEMANN) {
       const double zs = rhos * pcs.y, zo = rhoo * pco.y;
       const double inv_z = 1.0 / (zs + zo);
       const double Us = (zs * UR + zo * UL + pcs.x - pco.x) * inv_z;
@@ -220,7 +286,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_c(DevParams P, Buffers B, int d
       Ps = Pbar + 0.5 * bR * rbar * dU;
     }
     if (nes) { vsx = vbx; vsy = vby; Ps = Pbar; }  // near the free surface: averages
-    const double ti = (vs.x - vsx) * K.dwx + (vs.y - vsy) * K.dwy;
+    const double ti = (vown.x - vsx) * K.dwx + (vown.y - vsy) * K.dwy;
     drho = drho + 2 * rhos * Vo * ti;                        // :360-369
     const double rij = inv_rhos / rhoo;                      // 1/(rho_i rho_j)
     const double mo = B.l_mass[o];
@@ -257,7 +323,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_c(DevParams P, Buffers B, int d
       ax = ax + ((((tix + tvx) + tax) + tsx) + tlx);
       ay = ay + ((((tiy + tvy) + tay) + tsy) + tly);
     }
-    if (P.epsilon_xsph != 0.0 && !linear) {                  // :495-500
+    if (SOLVER == SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && P.epsilon_xsph != 0.0) {   // :495-500
       const double f = P.epsilon_xsph * 2 * mo / (rhos + rhoo) * K.w;
       agx = agx - f * vijx;
       agy = agy - f * vijy;
@@ -289,26 +355,69 @@ __global__ void __launch_bounds__(kBlock) k_eval_c(DevParams P, Buffers B, int d
   if (do_shift) B.shift[s] = make_double2(shx, shy);
 }
 
+// The two linear-Riemann bodies fit 168 VGPRs (3 waves per SIMD) without
+// spilling when asked to; HLLC, MUSCL and the debug form keep the compiler's
+// own allocation (asking them for 3 waves spills to scratch).
+template <int SOLVER>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_eval_c(DevParams P, Buffers B, int do_shift) {
+  eval_c<false, SOLVER>(P, B, do_shift);
+}
+
+template <bool DEBUG, int SOLVER>
+__global__ void __launch_bounds__(kBlock) k_eval_c_wide(DevParams P, Buffers B, int do_shift) {
+  eval_c<DEBUG, SOLVER>(P, B, do_shift);
+}
+
+template <bool DEBUG>
+static void launch_c64(hipStream_t s, const DevParams& P, Buffers& B, int g) {
+  switch (P.solver) {
+    case SPH2D_SOLVER_LINEAR_RIEMANN:
+      if (DEBUG)
+        hipLaunchKernelGGL((k_eval_c_wide<true, SPH2D_SOLVER_LINEAR_RIEMANN>), dim3(g), dim3(kBlock), 0, s, P, B, 1);
+      else
+        hipLaunchKernelGGL(k_eval_c<SPH2D_SOLVER_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, s, P, B, 1);
+      break;
+    case SPH2D_SOLVER_HLLC:
+      hipLaunchKernelGGL((k_eval_c_wide<DEBUG, SPH2D_SOLVER_HLLC>), dim3(g), dim3(kBlock), 0, s, P, B, 1);
+      break;
+    case SPH2D_SOLVER_MUSCL:
+      hipLaunchKernelGGL((k_eval_c_wide<DEBUG, SPH2D_SOLVER_MUSCL>), dim3(g), dim3(kBlock), 0, s, P, B, 1);
+      break;
+    default:
+      if (DEBUG)
+        hipLaunchKernelGGL((k_eval_c_wide<true, SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN>), dim3(g), dim3(kBlock), 0,
+                           s, P, B, 1);
+      else
+        hipLaunchKernelGGL(k_eval_c<SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, s, P, B, 1);
+  }
+}
+
+static void launch_a64(hipStream_t s, const DevParams& P, Buffers& B, int g) {
+  if (P.solver == SPH2D_SOLVER_MUSCL) hipLaunchKernelGGL(k_eval_a<true>, dim3(g), dim3(kBlock), 0, s, P, B);
+  else hipLaunchKernelGGL(k_eval_a<false>, dim3(g), dim3(kBlock), 0, s, P, B);
+}
+
 void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
   const int g = grid_for(cap_local);
   launch_fill_i32(s, B.counters + C_NSURF, 1, 0);
   if (P.mixed) launch_eval32_a(s, P, B, cap_local);
-  else hipLaunchKernelGGL(k_eval_a, dim3(g), dim3(kBlock), 0, s, P, B);
+  else launch_a64(s, P, B, g);
   hipLaunchKernelGGL(k_eval_b1, dim3(g), dim3(kBlock), 0, s, P, B);
   hipLaunchKernelGGL(k_eval_b2, dim3(g), dim3(kBlock), 0, s, P, B);
   if (P.mixed) launch_eval32_c(s, P, B, cap_local);
-  else if (P.write_debug) hipLaunchKernelGGL(k_eval_c<true>, dim3(g), dim3(kBlock), 0, s, P, B, 1);
-  else hipLaunchKernelGGL(k_eval_c<false>, dim3(g), dim3(kBlock), 0, s, P, B, 1);
+  else if (P.write_debug) launch_c64<true>(s, P, B, g);
+  else launch_c64<false>(s, P, B, g);
 }
 
 void launch_eval_pass(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int pass) {
   const int g = grid_for(cap_local);
   if (pass == 0) {
     if (P.mixed) launch_eval32_c(s, P, B, cap_local);
-    else hipLaunchKernelGGL(k_eval_c<false>, dim3(g), dim3(kBlock), 0, s, P, B, 1);
+    else launch_c64<false>(s, P, B, g);
   } else if (pass == 1) {
     if (P.mixed) launch_eval32_a(s, P, B, cap_local);
-    else hipLaunchKernelGGL(k_eval_a, dim3(g), dim3(kBlock), 0, s, P, B);
+    else launch_a64(s, P, B, g);
   } else {
     launch_eval(s, P, B, cap_local);
   }

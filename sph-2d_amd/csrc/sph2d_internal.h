@@ -38,6 +38,7 @@ struct DevParams {
   double inv_h, inv_c0, inv_sv, inv_wdx, cut2;  // reciprocals and the squared-distance pre-test
   float cut2_32;                         // FP32 pre-test bound: never below (3h + FP32 position error)^2
   double r2_max, r2_min;                 // fl(sqrt(r2)) <= 3h  <=>  r2 <= r2_max; >= 1e-7 <=> r2 >= r2_min
+  double x_over_t;                       // interior_dx/dt, the HLLC fan sample point
   int solver, write_debug;
   int mls_order;
   int mixed;                             // 1: FP32-mixed evaluation (mirrors are written)
@@ -68,6 +69,7 @@ struct Buffers {
   double *fra_v;
   double2 *sdw, *grad, *nrm, *shift, *avg;
   double *tau;                            // 3 per particle: xx, xy, yy
+  double2 *mg;                            // MUSCL only, 3 per particle: grad p, grad u, grad v
   int *surf, *near_;
   double *drhodt;
   double2 *acc;
@@ -87,6 +89,8 @@ struct Buffers {
   // tau (xx, xy, yy, 0), grad rho
   float2 *m_vel, *m_rv, *m_pc, *m_mass, *m_grad;
   float4 *m_tau;
+  float2 *m_mg;                           // MUSCL only: FP32 copy of mg
+
   int kstride;                            // stride of the 3 RK stage slabs (= local capacity)
 };
 

@@ -22,6 +22,8 @@ LIB_PATH = os.path.join(HERE, "libsph2d.so")
 
 SOLVER_MODIFIED_LINEAR_RIEMANN = 0
 SOLVER_LINEAR_RIEMANN = 1
+SOLVER_HLLC = 2
+SOLVER_MUSCL = 3
 INTEGRATOR_PC = 2
 INTEGRATOR_RK = 3
 FP64 = 0
@@ -163,7 +165,7 @@ LOCAL_F64 = ("x", "y", "u", "v", "rho", "mass", "p", "c", "volume", "fra_v", "su
              "sum_dwdx_y", "tau_xx", "tau_xy", "tau_yy", "grad_rho_x", "grad_rho_y", "drhodt",
              "ax", "ay", "shift_x", "shift_y", "normal_x", "normal_y", "internal_x", "internal_y",
              "visc_x", "visc_y", "art_internal_x", "art_internal_y", "sps_x", "sps_y", "lap_x",
-             "lap_y")
+             "lap_y", "grad_p_x", "grad_p_y", "grad_u_x", "grad_u_y", "grad_v_x", "grad_v_y")
 LOCAL_I32 = ("type", "name", "surface", "near_surface")
 
 

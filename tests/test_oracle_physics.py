@@ -127,9 +127,14 @@ def test_lattice_pair_count_and_free_surface(gold):
 
 
 # --- one evaluation: invariants ------------------------------------------------------------
-def test_uniform_state_gives_zero_rates():
+ALL_SOLVERS = (po.ORC_SOLVER_MODIFIED_LINEAR_RIEMANN, po.ORC_SOLVER_LINEAR_RIEMANN, po.ORC_SOLVER_HLLC,
+               po.ORC_SOLVER_MUSCL)
+
+
+@pytest.mark.parametrize("solver", ALL_SOLVERS)
+def test_uniform_state_gives_zero_rates(solver):
     n = 40
-    o = tgv(n)
+    o = tgv(n, solver=solver)
     N = n * n
     rho = np.full(N, 1000.0)
     o.set_fluid_state(o.f64("g_x")[:N], o.f64("g_y")[:N], np.zeros(N), np.zeros(N), rho, rho / (n * n))
@@ -138,13 +143,27 @@ def test_uniform_state_gives_zero_rates():
         assert np.max(np.abs(o.f64(f))) < 1e-9, f
 
 
-def test_galilean_invariance():
+@pytest.mark.parametrize("solver", (po.ORC_SOLVER_MODIFIED_LINEAR_RIEMANN, po.ORC_SOLVER_LINEAR_RIEMANN,
+                                    po.ORC_SOLVER_MUSCL))
+def test_galilean_invariance(solver):
+    """A uniform velocity added to every particle leaves the rates unchanged.
+    HLLC samples its fan at the fixed lab-frame x/t = dx/dt, so it is not Galilean
+    invariant and is not listed. MUSCL runs on a noisy state: its limiter
+    (function_module.f90:47-61) jumps from (0, 0) to (0, d/2) as one one-sided
+    slope crosses zero, and the exact TGV lattice has exactly-zero slopes on its
+    extremal lines, where the rounding of the shifted velocities picks the side."""
     n = 40
-    a = evaluate(tgv(n))
-    b = tgv(n)
     N = n * n
-    b.set_fluid_state(b.f64("g_x")[:N], b.f64("g_y")[:N], b.f64("g_u")[:N] + 0.3, b.f64("g_v")[:N] - 0.2,
-                      b.f64("g_rho")[:N], b.f64("g_mass")[:N])
+    a = tgv(n, solver=solver)
+    rng = np.random.default_rng(2)
+    noise = 0.02 if solver == po.ORC_SOLVER_MUSCL else 0.0
+    u = a.f64("g_u")[:N] + noise * rng.normal(size=N)
+    v = a.f64("g_v")[:N] + noise * rng.normal(size=N)
+    x, y, rho, m = a.f64("g_x")[:N].copy(), a.f64("g_y")[:N].copy(), a.f64("g_rho")[:N].copy(), a.f64("g_mass")[:N].copy()
+    a.set_fluid_state(x, y, u, v, rho, m)
+    evaluate(a)
+    b = tgv(n, solver=solver)
+    b.set_fluid_state(x, y, u + 0.3, v - 0.2, rho, m)
     evaluate(b)
     for f in ("drhodt", "ax", "ay"):
         ra, rb = a.f64(f), b.f64(f)
@@ -185,10 +204,12 @@ def test_rotation_equivariance():
         assert np.max(np.abs(fb(f) - sgn * fa(g))) < 1e-10 * (np.max(np.abs(fa(g))) + 1e-300), f
 
 
-def test_pairwise_antisymmetric_terms_conserve_momentum():
+@pytest.mark.parametrize("solver", ALL_SOLVERS)
+def test_pairwise_antisymmetric_terms_conserve_momentum(solver):
     """Internal (Riemann pressure), artificial viscosity and delta-SPH internal terms
-    are pairwise antisymmetric: sum_k m_k a_k = 0 over all subdomain particles."""
-    o = evaluate(tgv(40))
+    are pairwise antisymmetric: sum_k m_k a_k = 0 over all subdomain particles. For
+    every solver variant the pair's star pressure is one number shared by both sides."""
+    o = evaluate(tgv(40, solver=solver))
     m = o.f64("mass")
     for f in ("internal", "visc", "art_internal"):
         for c in ("x", "y"):
@@ -197,9 +218,12 @@ def test_pairwise_antisymmetric_terms_conserve_momentum():
             assert abs(s) < 1e-11 * np.sum(np.abs(m * a)) + 1e-300, (f, c, s)
 
 
-def test_tgv_initial_rates_match_navier_stokes():
-    """At t=0, a = -grad p / rho + nu lap u = -(pi sin 4 pi x)... to O(h^2)."""
-    o = evaluate(tgv(100))
+@pytest.mark.parametrize("solver", (po.ORC_SOLVER_MODIFIED_LINEAR_RIEMANN, po.ORC_SOLVER_MUSCL))
+def test_tgv_initial_rates_match_navier_stokes(solver):
+    """At t=0, a = -grad p / rho + nu lap u = -(pi sin 4 pi x)... to O(h^2).
+    (The linear and HLLC variants add an acoustic dissipation z (U_L - U_R) of the
+    size of the pressure term on this lattice, so they are not listed.)"""
+    o = evaluate(tgv(100, solver=solver))
     t = o.i32("type")
     m = t == 2
     x, u, v = o.f64("x")[m], o.f64("u")[m], o.f64("v")[m]

@@ -2,7 +2,7 @@
 compiled from the reference sources by oracle/build_ref.sh (flang, no FMA):
 compute_kernel.f90, status_equation.f90, function_module.f90 (BRINV,
 matrix_inversion_less_three) and information_module.f90's constants.
-Skipped where oracle/_ref is not built (the reference is not on the GPU box)."""
+Skipped where oracle/_ref is not built (where the reference sources are absent)."""
 import ctypes
 
 import numpy as np
