@@ -135,6 +135,22 @@ int sph2d_status_equation(const double* rho, const double* energy, const int32_t
 int sph2d_status_equation_n(const sph2d_params* p, int32_t n, const double* rho, const double* energy,
                             const int32_t* type, double* press, double* c);
 
+/* Reorder_Particle_Index -- Moving_Square_Or_Cylinder_Inside_a_Rectangular_Box/
+ * Reorder_Particle_Index.f90:54-287 (argument-less there: module arrays), on DEVICE
+ * arrays of n_total entries. Fluid particles (type 2) among the first n_fluid that lie
+ * outside [0,box_x]x[0,box_y] (lower bound the REAL(4) -1.0E-10) become ill (120); then
+ * the particles are stably partitioned into fluid, wavemaker (130), fixed ghost (-3),
+ * inlet (-201), outlet (-202), every other type dropped. x, y, type and every listed
+ * column (distinct device arrays; positions and velocities as two columns each) are
+ * permuted in place; entries past the kept total are left as they were.
+ * counts[6] = (fluid, wavemaker, fixed ghost, inlet, outlet, total kept); perm (device,
+ * optional) receives the source index of every kept slot. Runs on `stream` (a
+ * hipStream_t; NULL = the null stream) and synchronises it before returning. */
+int sph2d_reorder_particle_index(int32_t n_total, int32_t n_fluid, double box_x, double box_y,
+                                 double* x, double* y, int32_t* type, int32_t n_f64, double* const* f64_cols,
+                                 int32_t n_i32, int32_t* const* i32_cols, int32_t* perm, int32_t* counts,
+                                 void* stream);
+
 /* --- one entry per replaced routine (context form) ------------------------- */
 /* Distribute_Period_Boundary_Particle.f90:56-206: wrap into the box and append ghosts. */
 int sph2d_distribute_period_boundary(sph2d_ctx* ctx);

@@ -94,6 +94,10 @@ def lib():
         L.orc_set_fluid_state.restype = ctypes.c_int32
         D = ctypes.c_double
         L.orc_muscl_limiter.argtypes = [D, D, D, P(D), P(D)]
+        VP = ctypes.c_void_p
+        L.orc_reorder_particle_index.argtypes = [ctypes.c_int32, ctypes.c_int32, D, D, P(D), P(D),
+                                                 P(ctypes.c_int32), ctypes.c_int32, P(VP), ctypes.c_int32,
+                                                 P(VP), P(ctypes.c_int32), P(ctypes.c_int32)]
         L.orc_hllc_star.argtypes = [P(D), D, D, D, P(D), D, D, D, P(D), D, P(D), P(D)]
         _lib = L
     return _lib
@@ -251,6 +255,26 @@ def muscl_limiter(delphi_i: float, delphi_j: float, d_phi_ij: float):
     li, lj = ctypes.c_double(), ctypes.c_double()
     lib().orc_muscl_limiter(delphi_i, delphi_j, d_phi_ij, ctypes.byref(li), ctypes.byref(lj))
     return li.value, lj.value
+
+
+def reorder_particle_index(n_fluid, box_x, box_y, x, y, type_, f64_cols=(), i32_cols=()):
+    """Reorder_Particle_Index (MS) on copies of host arrays; returns
+    (x, y, type, f64_cols, i32_cols, perm, counts) cut to the kept particles."""
+    x = np.array(x, dtype=np.float64)
+    y = np.array(y, dtype=np.float64)
+    t = np.array(type_, dtype=np.int32)
+    n = len(x)
+    fc = [np.array(c, dtype=np.float64) for c in f64_cols]
+    ic = [np.array(c, dtype=np.int32) for c in i32_cols]
+    fptr = (ctypes.c_void_p * max(len(fc), 1))(*[c.ctypes.data for c in fc])
+    iptr = (ctypes.c_void_p * max(len(ic), 1))(*[c.ctypes.data for c in ic])
+    perm = np.full(n, -1, dtype=np.int32)
+    counts = np.zeros(6, dtype=np.int32)
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    lib().orc_reorder_particle_index(n, n_fluid, box_x, box_y, _dp(x), _dp(y), ip(t), len(fc), fptr, len(ic),
+                                     iptr, ip(perm), ip(counts))
+    k = int(counts[5])
+    return x[:k], y[:k], t[:k], [c[:k] for c in fc], [c[:k] for c in ic], perm[:k], counts
 
 
 def hllc_star(v_left, p_left, rho_left, c_left, v_right, p_right, rho_right, c_right, e, x_over_t):

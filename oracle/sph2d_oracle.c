@@ -1396,3 +1396,50 @@ const double* orc_get_f64(const orc_ctx* c, const char* name, int64_t* n) {
 const int32_t* orc_get_i32(const orc_ctx* c, const char* name, int64_t* n) {
   return orc_mut_i32((orc_ctx*)c, name, n);
 }
+
+/* ------------------------------------------------------------------ */
+/* Moving_Square_Or_Cylinder_Inside_a_Rectangular_Box/                 */
+/* Reorder_Particle_Index.f90:54-287                                   */
+/* ------------------------------------------------------------------ */
+void orc_reorder_particle_index(int32_t n_total, int32_t n_fluid, double box_x, double box_y,
+                                double* x, double* y, int32_t* type, int32_t n_f64, double** f64_cols,
+                                int32_t n_i32, int32_t** i32_cols, int32_t* perm, int32_t* counts) {
+  const double lo = (double)-1.0E-10f;
+  for (int i = 0; i < n_fluid; ++i) {                             /* :56-64 */
+    if (type[i] == 2) {
+      if (x[i] <= lo || x[i] > box_x || y[i] <= lo || y[i] > box_y) type[i] = 120;
+    }
+  }
+  int* lists[5];                                                  /* :84-115 */
+  int nl[5] = {0, 0, 0, 0, 0};
+  for (int c = 0; c < 5; ++c) lists[c] = xcalloc(n_total > 0 ? n_total : 1, sizeof(int));
+  for (int i = 0; i < n_total; ++i) {
+    int c = -1;
+    if (type[i] == 2) c = 0;                 /* Water_Particle_Label */
+    else if (type[i] == -3) c = 2;           /* Fix_Ghost_Particle_Label */
+    else if (type[i] == -201) c = 3;         /* Inlet_Particle_Label */
+    else if (type[i] == -202) c = 4;         /* Outlet_Particle_Label */
+    else if (type[i] == 130) c = 1;          /* Wave_Maker_Particle_Label */
+    if (c >= 0) lists[c][nl[c]++] = i;
+  }
+  int k = 0;                                                      /* :157-287 */
+  for (int c = 0; c < 5; ++c)
+    for (int L = 0; L < nl[c]; ++L) perm[k++] = lists[c][L];
+  double* tf = xcalloc(k > 0 ? k : 1, sizeof(double));
+  int* ti = xcalloc(k > 0 ? k : 1, sizeof(int));
+  double* fc[2] = {x, y};
+  for (int col = 0; col < 2 + n_f64; ++col) {
+    double* a = col < 2 ? fc[col] : f64_cols[col - 2];
+    for (int m = 0; m < k; ++m) tf[m] = a[perm[m]];
+    memcpy(a, tf, sizeof(double) * (size_t)k);
+  }
+  for (int col = 0; col < 1 + n_i32; ++col) {
+    int32_t* a = col == 0 ? type : i32_cols[col - 1];
+    for (int m = 0; m < k; ++m) ti[m] = a[perm[m]];
+    memcpy(a, ti, sizeof(int) * (size_t)k);
+  }
+  for (int c = 0; c < 5; ++c) { counts[c] = nl[c]; free(lists[c]); }
+  counts[5] = k;
+  free(tf);
+  free(ti);
+}

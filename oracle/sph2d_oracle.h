@@ -68,6 +68,16 @@ void orc_hllc_star(const double* v_left, double p_left, double rho_left, double 
                    const double* v_right, double p_right, double rho_right, double c_right,
                    const double* e, double x_over_t, double* v_star, double* p_star);
 
+/* Reorder_Particle_Index, Moving_Square_Or_Cylinder_Inside_a_Rectangular_Box/
+ * Reorder_Particle_Index.f90:54-287, on host arrays of n_total entries: ill marking of
+ * fluid outside the box among the first n_fluid, the per-type index lists in index
+ * order, the copy back as fluid, wavemaker, fixed ghost, inlet, outlet. Columns are
+ * permuted in place (entries past the kept total unchanged); perm[k] = source index;
+ * counts[6] = (fluid, wavemaker, fixed ghost, inlet, outlet, total kept). */
+void orc_reorder_particle_index(int32_t n_total, int32_t n_fluid, double box_x, double box_y,
+                                double* x, double* y, int32_t* type, int32_t n_f64, double** f64_cols,
+                                int32_t n_i32, int32_t** i32_cols, int32_t* perm, int32_t* counts);
+
 /* Initial lattice, Distribute_Initial_Particle.f90:204-249. */
 void orc_init_tgv(orc_ctx* c);
 /* Overwrite the fluid state (global order, n fluid particles). */

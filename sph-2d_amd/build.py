@@ -27,6 +27,7 @@ UNITS = [
     ("sph2d_fp64.hip", ["-ffp-contract=off"]),
     ("sph2d_eval64.hip", ["-ffp-contract=off"]),
     ("sph2d_eval32.hip", ["-ffp-contract=fast"]),
+    ("sph2d_reorder.hip", ["-ffp-contract=off"]),
     ("sph2d_capi.hip", ["-ffp-contract=off"]),
 ]
 

@@ -175,6 +175,33 @@ int compute_ke0(sph2d_ctx* c) {
   return SPH2D_OK;
 }
 
+// True when [p, p + bytes) lies inside one device allocation of this process.
+bool device_range(const void* p, size_t bytes) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) { (void)hipGetLastError(); return false; }
+  if (a.type != hipMemoryTypeDevice) return false;
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, const_cast<void*>(p)) != hipSuccess) { (void)hipGetLastError(); return false; }
+  const char* b = static_cast<const char*>(base);
+  const char* q = static_cast<const char*>(p);
+  return q >= b && q + bytes <= b + size;
+}
+
+struct DeviceScratch {
+  std::vector<void*> ptrs;
+  ~DeviceScratch() { for (void* p : ptrs) (void)hipFree(p); }
+  template <typename T>
+  hipError_t get(T** p, size_t n) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    if (e == hipSuccess) ptrs.push_back(q);
+    *p = static_cast<T*>(q);
+    return e;
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -455,6 +482,78 @@ int sph2d_status_equation_n(const sph2d_params* p, int32_t n, const double* rho,
 int sph2d_status_equation(const double* rho, const double* energy, const int32_t* type,
                           double* press, double* cs) {
   return sph2d_status_equation_n(nullptr, 1, rho, energy, type, press, cs);
+}
+
+// ---- Reorder_Particle_Index (device arrays) -------------------------------------
+int sph2d_reorder_particle_index(int32_t n_total, int32_t n_fluid, double box_x, double box_y, double* x,
+                                 double* y, int32_t* type, int32_t n_f64, double* const* f64_cols, int32_t n_i32,
+                                 int32_t* const* i32_cols, int32_t* perm, int32_t* counts, void* stream) {
+  if (n_total <= 0 || n_fluid < 0 || n_fluid > n_total || n_f64 < 0 || n_i32 < 0 || !counts)
+    return fail(SPH2D_ERR_ARGUMENT, "bad sizes or NULL counts");
+  if ((n_f64 > 0 && !f64_cols) || (n_i32 > 0 && !i32_cols)) return fail(SPH2D_ERR_ARGUMENT, "NULL column list");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(SPH2D_ERR_HIP, "no HIP device: the sph2d library runs on the GPU only");
+  const size_t nd = (size_t)n_total * sizeof(double), ni = (size_t)n_total * sizeof(int32_t);
+  if (!device_range(x, nd) || !device_range(y, nd) || !device_range(type, ni))
+    return fail(SPH2D_ERR_ARGUMENT, "x, y and type must be device arrays of n_total entries");
+  for (int k = 0; k < n_f64; ++k)
+    if (!device_range(f64_cols[k], nd))
+      return fail(SPH2D_ERR_ARGUMENT, "f64 column " + std::to_string(k) + " is not a device array of n_total entries");
+  for (int k = 0; k < n_i32; ++k)
+    if (!device_range(i32_cols[k], ni))
+      return fail(SPH2D_ERR_ARGUMENT, "i32 column " + std::to_string(k) + " is not a device array of n_total entries");
+  if (perm && !device_range(perm, ni)) return fail(SPH2D_ERR_ARGUMENT, "perm must be a device array of n_total entries");
+  {
+    std::vector<const void*> all = {x, y, type};
+    for (int k = 0; k < n_f64; ++k) all.push_back(f64_cols[k]);
+    for (int k = 0; k < n_i32; ++k) all.push_back(i32_cols[k]);
+    if (perm) all.push_back(perm);
+    std::sort(all.begin(), all.end());
+    if (std::adjacent_find(all.begin(), all.end()) != all.end())
+      return fail(SPH2D_ERR_ARGUMENT, "x, y, type, perm and the columns must be distinct arrays");
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int G = (n_total + kBlock - 1) / kBlock;
+  DeviceScratch sc;
+  int *blk, *off, *sblk, *misc, *dperm;
+  double* tmp;
+  HIP_OR_FAIL(sc.get(&blk, 5 * (size_t)G));
+  HIP_OR_FAIL(sc.get(&off, 5 * (size_t)G));
+  HIP_OR_FAIL(sc.get(&sblk, (size_t)scan_block_count(5 * G) + 1));
+  HIP_OR_FAIL(sc.get(&misc, 8));
+  HIP_OR_FAIL(sc.get(&tmp, (size_t)n_total));
+  if (perm) dperm = perm;
+  else HIP_OR_FAIL(sc.get(&dperm, (size_t)n_total));
+  int* total = misc;
+  int* dcounts = misc + 1;
+  launch_reorder_partition(s, n_total, n_fluid, type, x, y, box_x, box_y, blk, off, sblk, total, dcounts, dperm);
+  HIP_OR_FAIL(hipGetLastError());
+  int hc[6];
+  HIP_OR_FAIL(hipMemcpyAsync(hc, dcounts, sizeof(hc), hipMemcpyDeviceToHost, s));
+  HIP_OR_FAIL(hipStreamSynchronize(s));
+  const size_t kept = (size_t)hc[5];
+  auto permute_f64 = [&](double* col) -> int {
+    launch_reorder_gather_f64(s, n_total, dperm, total, col, tmp);
+    HIP_OR_FAIL(hipMemcpyAsync(col, tmp, kept * sizeof(double), hipMemcpyDeviceToDevice, s));
+    return SPH2D_OK;
+  };
+  auto permute_i32 = [&](int32_t* col) -> int {
+    int* t = reinterpret_cast<int*>(tmp);
+    launch_reorder_gather_i32(s, n_total, dperm, total, col, t);
+    HIP_OR_FAIL(hipMemcpyAsync(col, t, kept * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    return SPH2D_OK;
+  };
+  int rc = permute_f64(x);
+  if (!rc) rc = permute_f64(y);
+  if (!rc) rc = permute_i32(type);
+  for (int k = 0; k < n_f64 && !rc; ++k) rc = permute_f64(f64_cols[k]);
+  for (int k = 0; k < n_i32 && !rc; ++k) rc = permute_i32(i32_cols[k]);
+  if (rc) return rc;
+  HIP_OR_FAIL(hipGetLastError());
+  HIP_OR_FAIL(hipStreamSynchronize(s));
+  for (int k = 0; k < 6; ++k) counts[k] = hc[k];
+  return SPH2D_OK;
 }
 
 // ---- step pieces --------------------------------------------------------------

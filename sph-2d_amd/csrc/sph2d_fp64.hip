@@ -107,6 +107,12 @@ static void exclusive_scan(hipStream_t s, const int* in, int n, int* out, int* b
   hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, s, in, n, blocks, out);
 }
 
+void launch_exclusive_scan(hipStream_t s, const int* in, int n, int* out, int* blocks, int* total) {
+  exclusive_scan(s, in, n, out, blocks, total);
+}
+
+int scan_block_count(int n) { return (n <= 0 ? 1 : (n + kScanTile - 1) / kScanTile); }
+
 // ---------------------------------------------------------------------------
 // Distribute_Initial_Particle.f90:204-249 (TGV lattice)
 // ---------------------------------------------------------------------------

@@ -118,7 +118,19 @@ void launch_kernel_n(hipStream_t s, int n, const double* r, const double* dx, co
 void launch_eos_n(hipStream_t s, int n, double rho0, double c0sq, const double* rho,
                   const int* type, double* p, double* c);
 void launch_fill_i32(hipStream_t s, int* p, int n, int v);
+// out[i] = sum_{k<i} in[k]; the total goes to *total. `blocks` holds scan_block_count(n) ints.
+void launch_exclusive_scan(hipStream_t s, const int* in, int n, int* out, int* blocks, int* total);
+int scan_block_count(int n);
 void launch_max_cell(hipStream_t s, Buffers& B, int n_cells);
+// Reorder_Particle_Index (sph2d_reorder.hip): blk and off hold 5*G ints (G = ceil(n/256)),
+// scan_blocks scan_block_count(5*G), total 1, counts 6, perm n_total.
+void launch_reorder_partition(hipStream_t s, int n_total, int n_fluid, int* type, const double* x,
+                              const double* y, double box_x, double box_y, int* blk, int* off,
+                              int* scan_blocks, int* total, int* counts, int* perm);
+void launch_reorder_gather_f64(hipStream_t s, int n_total, const int* perm, const int* total, const double* src,
+                               double* dst);
+void launch_reorder_gather_i32(hipStream_t s, int n_total, const int* perm, const int* total, const int* src,
+                               int* dst);
 void launch_count_pairs(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
 
 // Stage kinds for launch_stage.

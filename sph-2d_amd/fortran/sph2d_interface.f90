@@ -10,6 +10,8 @@ module sph2d_interface
   implicit none
 
   integer(c_int), parameter :: SPH2D_INTEGRATOR_PC = 2, SPH2D_INTEGRATOR_RK = 3
+  integer(c_int), parameter :: SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN = 0, SPH2D_SOLVER_LINEAR_RIEMANN = 1, &
+                               SPH2D_SOLVER_HLLC = 2, SPH2D_SOLVER_MUSCL = 3
 
   type, bind(C) :: sph2d_params
     integer(c_int32_t) :: struct_size
@@ -127,6 +129,21 @@ module sph2d_interface
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: out14(14)
+      integer(c_int) :: rc
+    end function
+
+    ! Reorder_Particle_Index (MS) on device arrays: x, y, ptype and the columns are
+    ! c_ptr device addresses (e.g. from a GPU-resident driver); f64_cols/i32_cols are
+    ! arrays of such addresses; counts(6) comes back on the host.
+    function sph2d_reorder_particle_index(n_total, n_fluid, box_x, box_y, x, y, ptype, n_f64, f64_cols, &
+                                          n_i32, i32_cols, perm, counts, stream) &
+        bind(C, name="sph2d_reorder_particle_index") result(rc)
+      import :: c_int, c_int32_t, c_double, c_ptr
+      integer(c_int32_t), value :: n_total, n_fluid, n_f64, n_i32
+      real(c_double), value :: box_x, box_y
+      type(c_ptr), value :: x, y, ptype, perm, stream
+      type(c_ptr), intent(in) :: f64_cols(*), i32_cols(*)
+      integer(c_int32_t), intent(out) :: counts(6)
       integer(c_int) :: rc
     end function
 

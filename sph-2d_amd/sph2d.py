@@ -79,6 +79,7 @@ EXPORTS = (
     "sph2d_refresh_domain", "sph2d_time_step", "sph2d_run", "sph2d_sample",
     "sph2d_download_local", "sph2d_download_pairs", "sph2d_set_debug",
     "sph2d_get_diagnostics", "sph2d_get_stream", "sph2d_time_pass",
+    "sph2d_reorder_particle_index",
 )
 
 _lib = None
@@ -120,6 +121,8 @@ def lib():
             "sph2d_get_diagnostics": [vp, P(Diag)],
             "sph2d_get_stream": [vp, P(vp)],
             "sph2d_time_pass": [vp, i32, i32, dp],
+            "sph2d_reorder_particle_index": [i32, i32, ctypes.c_double, ctypes.c_double, vp, vp, vp, i32,
+                                             P(vp), i32, P(vp), vp, ip, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -321,3 +324,73 @@ def status_equation(rho, type_=None, params: Optional[Params] = None):
     _check(lib().sph2d_status_equation_n(ctypes.byref(params) if params is not None else None, n,
                                          _dp(rho), _dp(e), _ip(t), _dp(p), _dp(c)))
     return p, c
+
+
+# --- device buffers for the device-array entry points (HIP runtime via ctypes) ----------
+_hip = None
+
+
+def _hiprt():
+    global _hip
+    if _hip is None:
+        h = ctypes.CDLL("libamdhip64.so")
+        h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        h.hipFree.argtypes = [ctypes.c_void_p]
+        h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _hip = h
+    return _hip
+
+
+class DeviceArray:
+    """A device copy of a 1-D numpy array (hipMalloc); .get() copies it back."""
+
+    def __init__(self, a: np.ndarray):
+        self.host = np.ascontiguousarray(a)
+        self.ptr = ctypes.c_void_p()
+        h = _hiprt()
+        if h.hipMalloc(ctypes.byref(self.ptr), max(self.host.nbytes, 1)) != 0:
+            raise SphError(3, "hipMalloc failed")
+        if h.hipMemcpy(self.ptr, self.host.ctypes.data, self.host.nbytes, 1) != 0:   # host to device
+            raise SphError(3, "hipMemcpy failed")
+
+    def get(self) -> np.ndarray:
+        out = np.empty_like(self.host)
+        if _hiprt().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) != 0:        # device to host
+            raise SphError(3, "hipMemcpy failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            _hiprt().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def reorder_particle_index(n_fluid, box_x, box_y, x, y, type_, f64_cols=(), i32_cols=()):
+    """Reorder_Particle_Index (MS/Reorder_Particle_Index.f90:54-287) on the GPU for host
+    arrays: uploads, calls sph2d_reorder_particle_index, downloads. Returns
+    (x, y, type, f64_cols, i32_cols, perm, counts) with counts = (fluid, wavemaker,
+    fixed ghost, inlet, outlet, total kept); arrays hold the kept particles only."""
+    n = len(x)
+    dx = DeviceArray(np.asarray(x, dtype=np.float64))
+    dy = DeviceArray(np.asarray(y, dtype=np.float64))
+    dt = DeviceArray(np.asarray(type_, dtype=np.int32))
+    df = [DeviceArray(np.asarray(c, dtype=np.float64)) for c in f64_cols]
+    di = [DeviceArray(np.asarray(c, dtype=np.int32)) for c in i32_cols]
+    dp = DeviceArray(np.full(n, -1, dtype=np.int32))
+    fptrs = (ctypes.c_void_p * max(len(df), 1))(*[d.ptr.value for d in df])
+    iptrs = (ctypes.c_void_p * max(len(di), 1))(*[d.ptr.value for d in di])
+    counts = np.zeros(6, dtype=np.int32)
+    _check(lib().sph2d_reorder_particle_index(n, n_fluid, box_x, box_y, dx.ptr, dy.ptr, dt.ptr, len(df), fptrs,
+                                              len(di), iptrs, dp.ptr, _ip(counts), None))
+    k = int(counts[5])
+    out = (dx.get()[:k], dy.get()[:k], dt.get()[:k], [d.get()[:k] for d in df], [d.get()[:k] for d in di],
+           dp.get()[:k], counts)
+    for d in [dx, dy, dt, dp] + df + di:
+        d.free()
+    return out
