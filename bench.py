@@ -13,8 +13,9 @@ The state is resident in HBM when the timed region starts; timing uses HIP
 events on the context's stream. Prints ONE JSON line.
 
 Also reported: the force pass's roofline (algorithmic bytes per launch over its
-HIP-event-timed duration) and the CPU baseline (the C oracle, one core, on a
-bounded sample of the same workload).
+duration, from HIP events recorded around the pass inside real steps on the
+context's stream) and the CPU baseline (the C oracle, one core, on a bounded
+sample of the same workload).
 """
 from __future__ import annotations
 
@@ -48,15 +49,15 @@ def force_pass_bytes_per_particle(form: int) -> int:
     every array the pass reads for its own particle once, every output once; the
     neighbours' data are the same arrays (served from L2/L1), not counted again.
     FP64 (k_eval_c) reads: pos 16, vel 16, (rho,V) 16, (p,c) 16, type 4, cell 4,
-      near 4, tau 24, grad rho 16, mass 8, particle_mass at local index 8 = 132
-    FP32-mixed (k_eval_c32) reads: pos 16 (FP64), vel 8, (drho,V) 8, (p,c) 8,
-      (mass, particle_mass at local index) 8, tau 16, grad rho 8, type 4, cell 4,
-      near 4 = 84
+      near 4, tau 24, grad rho 16, mass 8, particle_mass at local index 8, the
+      recorded neighbour mask 32 = 164
+    FP32-mixed (k_eval_c32) reads: the three float4 records 48, type 4, cell 4,
+      near 4, the neighbour mask 32 = 92
     both write: drho/dt 8, acceleration 16, shift 16 = 40."""
-    return (132 if form == 64 else 84) + 40
+    return (164 if form == 64 else 92) + 40
 
 
-def cpu_baseline(n_side: int = 512, steps: int = 6) -> dict:
+def cpu_baseline(n_side: int = 512, steps: int = 20) -> dict:
     """The C oracle (oracle/liboracle.so, the reference's algorithm restated,
     scatter over a half pair list as in find_particle_pair.f90), one core, on a
     bounded sample: n_side^2 TGV, predictor-corrector, `steps` steps."""
@@ -83,8 +84,10 @@ def main() -> int:
     ap.add_argument("--n-side", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-side", type=int, default=512)
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--forms", choices=("both", "fp64", "mixed"), default="both")
+    ap.add_argument("--timing-steps", type=int, default=20,
+                    help="steps after the timed run that record the per-pass HIP events")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB: a plain copy kernel of "
                          "known size that scales rocprofv3's FETCH_SIZE/WRITE_SIZE counters")
@@ -123,14 +126,17 @@ def main() -> int:
         d = sim.diagnostics()
         if d.error_flags or d.rebin_needed:
             print(f"# warning: error_flags={d.error_flags} rebin={d.rebin_needed}", file=sys.stderr)
-        reps = 20
-        force_ms = sim.time_pass(0, reps)
-        passa_ms = sim.time_pass(1, reps)
-        eval_ms = sim.time_pass(2, reps)
+        # pass durations inside real steps (HIP events around the passes), in a short
+        # run after the timed one so the events do not perturb it
+        sim.set_pass_timing(True)
+        sim.run(1 + args.warmup + args.steps, max(1, args.timing_steps))
+        force_ms, passa_ms, eval_ms, n_eval = sim.pass_timing()
+        sim.set_pass_timing(False)
         cnt = ctypes.c_int64()
         sph2d._check(sph2d.lib().sph2d_download_pairs(sim._c, 0, None, None, ctypes.byref(cnt)))
         out = {"total_s": total_s, "sample": smp, "n_local": d.n_local, "n_ghost": d.n_ghost,
-               "force_ms": force_ms, "passa_ms": passa_ms, "eval_ms": eval_ms, "pairs": cnt.value,
+               "force_ms": force_ms, "passa_ms": passa_ms, "eval_ms": eval_ms, "n_eval_timed": n_eval,
+               "pairs": cnt.value,
                "dt": prm.dt}
         sim.close()
         print(f"# form {precision}: {args.steps} steps in {total_s:.3f} s, KE/KE0={smp[0]:.10f}",
@@ -211,6 +217,8 @@ def main() -> int:
             "traffic": None,
             "bytes_per_particle": force_pass_bytes_per_particle(32),
             "units_per_launch": r32["n_local"],
+            "duration_ms": r32["force_ms"],
+            "launches_timed": r32["n_eval_timed"],
         },
     }
     if rank == 0 and not args.no_cpu_baseline:

@@ -182,7 +182,9 @@ int sph2d_sample(sph2d_ctx* ctx, double* out14);
  *      internal_x internal_y visc_x visc_y art_internal_x art_internal_y sps_x sps_y
  *      lap_x lap_y (the last 10 only after sph2d_set_debug(ctx,1))
  *      grad_p_x grad_p_y grad_u_x grad_u_y grad_v_x grad_v_y (SPH2D_SOLVER_MUSCL only)
- * i32: type name(gid, 0-based) surface near_surface */
+ * i32: type name(gid, 0-based) surface near_surface
+ *      neighbour_count (particles within 3h, from the last evaluation's recorded
+ *      acceptance masks; lists of at most 256 candidates) */
 int sph2d_download_local(sph2d_ctx* ctx, const char* name, void* out, int32_t n);
 /* Pair set of the current state (find_particle_pair.f90:64-123): unordered pairs
  * as 0-based gids (gid_lo < gid_hi), sorted ascending by (lo,hi). */
@@ -195,6 +197,13 @@ int sph2d_get_diagnostics(sph2d_ctx* ctx, sph2d_diag* out);
  * launches of one part of the last evaluation: pass 0 = force pass (Riemann,
  * delta-SPH, SPS, shift sums), 1 = neighbour-sum pass A, 2 = the whole evaluation.
  * No reference counterpart: measurement hook for the roofline report. */
+/* In-step pass timing (measurement hook, no reference counterpart): while on,
+ * every force evaluation records HIP events around pass A and the force pass;
+ * sph2d_get_pass_timing synchronises and returns the average durations (ms) over
+ * the evaluations since the last call: out3 = (force pass, pass A, pass A start to
+ * force pass end), and their number. */
+int sph2d_set_pass_timing(sph2d_ctx* ctx, int32_t on);
+int sph2d_get_pass_timing(sph2d_ctx* ctx, double* out3, int32_t* n_evaluations);
 int sph2d_time_pass(sph2d_ctx* ctx, int32_t pass, int32_t reps, double* avg_ms);
 /* Device stream of the context (hipStream_t) for callers that time or order work. */
 int sph2d_get_stream(sph2d_ctx* ctx, void** stream);

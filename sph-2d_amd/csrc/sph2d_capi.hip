@@ -44,6 +44,10 @@ struct sph2d_ctx {
   bool mls_pending = false;
   double* d_out = nullptr;          // 16 doubles for reductions
   std::vector<void*> allocs;
+  // in-step pass timing: 4 events per evaluation (pass A start/end, force pass start/end)
+  bool pass_timing = false;
+  std::vector<hipEvent_t> events;
+  size_t events_used = 0;
 
   template <class T>
   int alloc(T** p, size_t n) {
@@ -274,14 +278,15 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   rc |= c->alloc(&B.off_yf, F); rc |= c->alloc(&B.cnt_yx, H); rc |= c->alloc(&B.off_yx, H);
   rc |= c->alloc(&B.src_x, H);
   rc |= c->alloc(&B.cell_count, c->n_cells); rc |= c->alloc(&B.cell_start, c->n_cells + 1);
-  rc |= c->alloc(&B.cell_members, G);
+  rc |= c->alloc(&B.cell_members, G); rc |= c->alloc(&B.l_of_g, G);
   rc |= c->alloc(&B.name, L); rc |= c->alloc(&B.l_type, L); rc |= c->alloc(&B.l_cell, L);
   rc |= c->alloc(&B.l_pos, L); rc |= c->alloc(&B.l_vel, L); rc |= c->alloc(&B.l_rv, L);
   rc |= c->alloc(&B.l_pc, L); rc |= c->alloc(&B.l_pos_bin, L); rc |= c->alloc(&B.l_pos32, L);
-  rc |= c->alloc(&B.l_mass, L); rc |= c->alloc(&B.l_e, L);
+  rc |= c->alloc(&B.l_mass, L);
   rc |= c->alloc(&B.fra_v, L); rc |= c->alloc(&B.sdw, L); rc |= c->alloc(&B.grad, L);
   rc |= c->alloc(&B.nrm, L); rc |= c->alloc(&B.shift, L); rc |= c->alloc(&B.avg, L);
   rc |= c->alloc(&B.tau, 3 * L); rc |= c->alloc(&B.surf, L); rc |= c->alloc(&B.near_, L);
+  rc |= c->alloc(&B.nb_mask, 4 * L);
   rc |= c->alloc(&B.drhodt, L); rc |= c->alloc(&B.acc, L);
   rc |= c->alloc(&B.b_rho, L); rc |= c->alloc(&B.t_drho, L); rc |= c->alloc(&B.k_drho, 3 * L);
   rc |= c->alloc(&B.b_pos, L); rc |= c->alloc(&B.b_vel, L); rc |= c->alloc(&B.t_vel, L);
@@ -293,8 +298,7 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   rc |= c->alloc(&B.pair_count, L); rc |= c->alloc(&B.pair_off, L);
   rc |= c->alloc(&c->d_out, 16);
   if (c->P.mixed) {
-    rc |= c->alloc(&B.m_vel, L); rc |= c->alloc(&B.m_rv, L); rc |= c->alloc(&B.m_pc, L);
-    rc |= c->alloc(&B.m_mass, L); rc |= c->alloc(&B.m_grad, L); rc |= c->alloc(&B.m_tau, L);
+    rc |= c->alloc(&B.m_r0, L); rc |= c->alloc(&B.m_r1, L); rc |= c->alloc(&B.m_r2, L);
   }
   B.mg = nullptr; B.m_mg = nullptr;
   if (p->solver == SPH2D_SOLVER_MUSCL) {
@@ -315,6 +319,7 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
 int sph2d_ctx_destroy(sph2d_ctx* c) {
   if (!c) return SPH2D_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -585,7 +590,19 @@ int sph2d_devide_block(sph2d_ctx* c, int32_t it) {
 static void do_eval(sph2d_ctx* c, bool with_eos) {
   if (with_eos) launch_stage(c->stream, c->P, c->B, c->cap_local, ST_EOS);
   c->P.write_debug = c->debug ? 1 : 0;
-  launch_eval(c->stream, c->P, c->B, c->cap_local);
+  hipEvent_t* ev = nullptr;
+  if (c->pass_timing) {
+    while (c->events.size() < c->events_used + 4) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) break;
+      c->events.push_back(e);
+    }
+    if (c->events.size() >= c->events_used + 4) {
+      ev = &c->events[c->events_used];
+      c->events_used += 4;
+    }
+  }
+  launch_eval(c->stream, c->P, c->B, c->cap_local, ev);
   c->evaluated = true;
 }
 
@@ -786,6 +803,20 @@ int sph2d_download_local(sph2d_ctx* c, const char* name, void* out, int32_t n) {
   if (s == "name") return i1(B.name);
   if (s == "surface") return i1(B.surf);
   if (s == "near_surface") return i1(B.near_);
+  if (s == "neighbour_count") {                 // set bits of the last pass A's recorded masks
+    std::vector<uint64_t> t((size_t)nl * 4);
+    for (int w = 0; w < 4; ++w)
+      HIP_OR_FAIL(hipMemcpyAsync(t.data() + (size_t)w * nl, B.nb_mask + (size_t)w * B.kstride, nl * sizeof(uint64_t),
+                                 hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+    int* o = static_cast<int*>(out);
+    for (int i = 0; i < nl; ++i) {
+      int k = 0;
+      for (int w = 0; w < 4; ++w) k += __builtin_popcountll(t[(size_t)w * nl + i]);
+      o[i] = k;
+    }
+    return SPH2D_OK;
+  }
   static const char* dbg_names[10] = {"internal_x", "internal_y", "visc_x", "visc_y", "art_internal_x",
                                       "art_internal_y", "sps_x", "sps_y", "lap_x", "lap_y"};
   for (int k = 0; k < 10; ++k)
@@ -867,6 +898,36 @@ int sph2d_time_pass(sph2d_ctx* c, int32_t pass, int32_t reps, double* avg_ms) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *avg_ms = (double)ms / reps;
+  return SPH2D_OK;
+}
+
+int sph2d_set_pass_timing(sph2d_ctx* c, int32_t on) {
+  if (!c) return fail(SPH2D_ERR_ARGUMENT, "ctx is NULL");
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  c->pass_timing = on != 0;
+  c->events_used = 0;
+  return SPH2D_OK;
+}
+
+int sph2d_get_pass_timing(sph2d_ctx* c, double* out3, int32_t* n_evaluations) {
+  if (!c || !out3 || !n_evaluations) return fail(SPH2D_ERR_ARGUMENT, "NULL argument");
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  const int n = (int)(c->events_used / 4);
+  double fa = 0.0, fc = 0.0, fe = 0.0;
+  for (int k = 0; k < n; ++k) {
+    float a = 0.f, b = 0.f, e = 0.f;
+    HIP_OR_FAIL(hipEventElapsedTime(&a, c->events[4 * k], c->events[4 * k + 1]));
+    HIP_OR_FAIL(hipEventElapsedTime(&b, c->events[4 * k + 2], c->events[4 * k + 3]));
+    HIP_OR_FAIL(hipEventElapsedTime(&e, c->events[4 * k], c->events[4 * k + 3]));
+    fa += a;
+    fc += b;
+    fe += e;
+  }
+  *n_evaluations = n;
+  out3[0] = n ? fc / n : 0.0;
+  out3[1] = n ? fa / n : 0.0;
+  out3[2] = n ? fe / n : 0.0;
+  c->events_used = 0;
   return SPH2D_OK;
 }
 

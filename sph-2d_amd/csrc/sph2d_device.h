@@ -28,12 +28,18 @@ __device__ __forceinline__ void tait(const DevParams& P, double rho, double& p, 
   c = P.sqrt_c0sq * (x * x * x);
 }
 
-// FP32 mirrors of the state one neighbour reads in the mixed form.
-__device__ __forceinline__ void write_mirror(const DevParams& P, const Buffers& B, int s, double2 vel,
-                                             double rho, double vol, double p, double c) {
-  B.m_vel[s] = make_float2((float)vel.x, (float)vel.y);
-  B.m_rv[s] = make_float2((float)(rho - P.rho0), (float)vol);
-  B.m_pc[s] = make_float2((float)p, (float)c);
+// FP32 record r0 of the mixed form: (u, v, rho - rho0, m), m negated if not fluid.
+__device__ __forceinline__ void write_r0(const DevParams& P, const Buffers& B, int s, double2 vel, double rho,
+                                         double m, int type) {
+  const float mf = (float)m;
+  B.m_r0[s] = make_float4((float)vel.x, (float)vel.y, (float)(rho - P.rho0), type == 2 ? mf : -mf);
+}
+
+// Offset of a position from the lower-left corner of its binned cell (the cell of
+// Devide_block's binning: x_cell = chain_origin_x + cx * chain_dx).
+__device__ __forceinline__ float2 cell_offset(const DevParams& P, double2 x, int cell) {
+  const int cx = cell / P.chain_ny, cy = cell - cx * P.chain_ny;
+  return make_float2((float)(x.x - (P.chain_ox + cx * P.chain_d)), (float)(x.y - (P.chain_oy + cy * P.chain_d)));
 }
 
 __device__ __forceinline__ double tait_p(const DevParams& P, double rho) {
@@ -75,9 +81,18 @@ namespace sph2d {
 // Phase 1 screens every candidate with FP32 positions against a bound that no
 // pair within 3h can exceed and keeps a 256-bit mask of the survivors (about 40
 // of 207); phase 2 applies the exact FP64 criterion of find_particle_pair.f90:87
-// (1e-7 <= r <= 3h, r = sqrt(dx*dx+dy*dy)) to them and calls f(o) for the
-// accepted ones, in ascending candidate order, so the lanes of a wave stay converged on the
-// expensive pair body. Lists longer than 256 run in several chunks.
+// (1e-7 <= r <= 3h, r = sqrt(dx*dx+dy*dy)) to them and calls f(o, d) (d: the
+// candidate's stencil column, 0..2 for cell columns cx-1..cx+1) for the
+// accepted ones, in ascending candidate order, so the lanes of a wave stay
+// converged on the pair body. Lists longer than 256 run in chunks.
+//
+// Modes: kNbScan screens and tests; kNbRecord also stores the exact acceptance
+// mask of a list of at most 256 candidates (4 words per particle, word w of
+// particle s at masks[w * mstride + s]); kNbReplay loads that mask instead of
+// screening and testing, so later passes of the same evaluation (same positions,
+// same cell table) visit the same neighbours in the same order without reading
+// positions for the test. Bits at or beyond the list length are ignored, and a
+// list longer than 256 is screened and tested again.
 // find_particle_pair.f90:87 with r2 = dx*dx+dy*dy rounded as the reference rounds it
 // (no contraction, whatever the translation unit's flags).
 __device__ __forceinline__ bool is_pair_r2(const DevParams& P, double dx, double dy) {
@@ -85,34 +100,58 @@ __device__ __forceinline__ bool is_pair_r2(const DevParams& P, double dx, double
   return (r2 <= P.r2_max) && (r2 >= P.r2_min);
 }
 
-template <class F>
+enum { kNbScan = 0, kNbRecord = 1, kNbReplay = 2 };
+
+template <int MODE = kNbScan, class F>
 __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int* __restrict__ cell_start,
                                                    const double2* __restrict__ pos,
                                                    const float2* __restrict__ pos32, int s, double2 ps,
-                                                   int cell, F&& f) {
-  const float2 ps32 = pos32[s];
+                                                   int cell, F&& f, uint64_t* __restrict__ masks = nullptr,
+                                                   int mstride = 0) {
   const CellRanges R = cell_ranges(P, cell_start, cell);
   const int n0 = R.hi[0] - R.lo[0], n1 = R.hi[1] - R.lo[1], n2 = R.hi[2] - R.lo[2];
   const int n01 = n0 + n1, total = n01 + n2;
   const int l0 = R.lo[0], l1 = R.lo[1] - n0, l2 = R.lo[2] - n01;
+  const bool stored = total <= 256;
+  const bool replay = MODE == kNbReplay && stored;
+  float2 ps32 = make_float2(0.f, 0.f);
+  if (!replay) ps32 = pos32[s];
   for (int c0 = 0; c0 < total; c0 += 256) {
     uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-#pragma unroll 1
-    for (int w = 0; w < 4; ++w) {
-      const int base = c0 + 64 * w;
-      const int cn = min(64, total - base);
-      uint64_t mw = 0;
-#pragma unroll 2
-      for (int k = 0; k < cn; ++k) {
-        const int kk = base + k;
-        const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
-        const float2 q = pos32[o];
-        const float fx = ps32.x - q.x, fy = ps32.y - q.y;
-        const bool ok = (fx * fx + fy * fy <= P.cut2_32) && (o != s);
-        mw |= (uint64_t)ok << k;
+    if (replay) {
+      // bit k of word w is candidate 64 w + k; keep bits below total only
+      m0 = masks[s];
+      m1 = masks[mstride + s];
+      m2 = masks[2 * mstride + s];
+      m3 = masks[3 * mstride + s];
+      if (total < 256) {
+        const int t = total & 63;
+        const uint64_t part = t ? ((1ull << t) - 1ull) : 0ull;
+        const int wl = total >> 6;                 // the word holding the last partial bits
+        if (wl == 0) { m0 &= part; m1 = 0; m2 = 0; m3 = 0; }
+        else if (wl == 1) { m1 &= part; m2 = 0; m3 = 0; }
+        else if (wl == 2) { m2 &= part; m3 = 0; }
+        else { m3 &= part; }
       }
-      if (w == 0) m0 = mw; else if (w == 1) m1 = mw; else if (w == 2) m2 = mw; else m3 = mw;
+    } else {
+#pragma unroll 1
+      for (int w = 0; w < 4; ++w) {
+        const int base = c0 + 64 * w;
+        const int cn = min(64, total - base);
+        uint64_t mw = 0;
+#pragma unroll 2
+        for (int k = 0; k < cn; ++k) {
+          const int kk = base + k;
+          const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
+          const float2 q = pos32[o];
+          const float fx = ps32.x - q.x, fy = ps32.y - q.y;
+          const bool ok = (fx * fx + fy * fy <= P.cut2_32) && (o != s);
+          mw |= (uint64_t)ok << k;
+        }
+        if (w == 0) m0 = mw; else if (w == 1) m1 = mw; else if (w == 2) m2 = mw; else m3 = mw;
+      }
     }
+    uint64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;        // kNbRecord: the exactly accepted bits
 #pragma unroll 1
     while ((m0 | m1 | m2 | m3) != 0) {
       const int w = m0 ? 0 : (m1 ? 1 : (m2 ? 2 : 3));
@@ -121,9 +160,25 @@ __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int
       const uint64_t bit = 1ull << b;
       if (w == 0) m0 ^= bit; else if (w == 1) m1 ^= bit; else if (w == 2) m2 ^= bit; else m3 ^= bit;
       const int kk = c0 + w * 64 + b;
-      const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
-      const double2 po = pos[o];
-      if (is_pair_r2(P, ps.x - po.x, ps.y - po.y)) f(o);
+      const int d = kk < n0 ? 0 : (kk < n01 ? 1 : 2);     // stencil column of the candidate
+      const int o = d == 0 ? l0 + kk : (d == 1 ? l1 + kk : l2 + kk);
+      if (replay) {
+        f(o, d);                                  // the recorded mask is exact
+      } else {
+        const double2 po = pos[o];
+        if (is_pair_r2(P, ps.x - po.x, ps.y - po.y)) {
+          if (MODE == kNbRecord) {
+            if (w == 0) a0 |= bit; else if (w == 1) a1 |= bit; else if (w == 2) a2 |= bit; else a3 |= bit;
+          }
+          f(o, d);
+        }
+      }
+    }
+    if (MODE == kNbRecord && stored) {
+      masks[s] = a0;
+      masks[mstride + s] = a1;
+      masks[2 * mstride + s] = a2;
+      masks[3 * mstride + s] = a3;
     }
   }
 }

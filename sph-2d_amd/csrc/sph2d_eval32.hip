@@ -2,12 +2,15 @@
 //
 // Same passes, order and pair terms as sph2d_eval64.hip, with the pair sums in
 // FP32 and Kahan-compensated accumulation. What stays FP64: particle positions
-// (the exact pair test 1e-7 <= r <= 3h and the offset x_s - x_o, rounded to
-// FP32 only after the subtraction), the integrator state and stage updates, the
-// Tait EOS and the per-particle SPS closure. Density enters the pair sums as
-// delta rho = rho - rho0 (FP32), so rho_o - rho_s carries no cancellation.
-// Neighbour data come from the FP32 mirrors (80 bytes per neighbour instead of
-// 128 in FP64).
+// (the exact pair test 1e-7 <= r <= 3h in pass A, whose acceptance mask the
+// force pass replays), the integrator state and stage updates and the
+// per-particle SPS closure. Density enters the pair sums as delta rho =
+// rho - rho0 (FP32), so rho_o - rho_s carries no cancellation; p and c of a
+// neighbour follow from its delta rho by the Tait law written as polynomials in
+// delta rho / rho0 (no cancellation either). In the force pass the offset
+// x_s - x_o comes from FP32 offsets to each particle's cell corner plus the
+// whole-cell difference, which keeps it to ~1e-7 h; a neighbour costs 48 bytes
+// (three float4 records) there, 16 in pass A.
 #include "sph2d_internal.h"
 #include "sph2d_device.h"
 #include "sph2d_riemann.h"
@@ -30,6 +33,7 @@ struct KSum {
 struct DevParams32 {
   float inv_h, e9, kfactor, inv_sv, rho0, inv_c0, h, fai_h2, alpha, beta, two_nu;
   float delta_alpha_h, two_delta_h, shift_beta_h2, shift_R, inv_wdx, epsilon2, x_over_t;
+  float inv_rho0, eos_b, c0, cell_d;
 };
 
 static DevParams32 make32(const DevParams& P) {
@@ -52,7 +56,23 @@ static DevParams32 make32(const DevParams& P) {
   Q.inv_wdx = (float)P.inv_wdx;
   Q.epsilon2 = (float)(P.epsilon_xsph * 2);
   Q.x_over_t = (float)P.x_over_t;
+  Q.inv_rho0 = (float)(1.0 / P.rho0);
+  Q.eos_b = (float)P.eos_b;
+  Q.c0 = (float)P.sqrt_c0sq;
+  Q.cell_d = (float)P.chain_d;
   return Q;
+}
+
+// status_equation.f90:61-63 in terms of x = (rho - rho0)/rho0:
+// p = B((1+x)^7 - 1) = B x (7 + 21x + 35x^2 + 35x^3 + 21x^4 + 7x^5 + x^6), c = c0 (1+x)^3.
+__device__ __forceinline__ float tait_p32(const DevParams32& Q, float drho) {
+  const float x = drho * Q.inv_rho0;
+  return Q.eos_b * (x * (7.f + x * (21.f + x * (35.f + x * (35.f + x * (21.f + x * (7.f + x)))))));
+}
+
+__device__ __forceinline__ float tait_c32(const DevParams32& Q, float drho) {
+  const float x = drho * Q.inv_rho0;
+  return Q.c0 * (1.f + x * (3.f + x * (3.f + x)));
 }
 
 struct KernelTerm32 {
@@ -78,22 +98,24 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
   const int n = B.counters[C_NLOCAL];
   if (s >= n) return;
   const double2 ps = B.l_pos[s];
-  const float2 vs = B.m_vel[s];
-  const float drs = B.m_rv[s].x;
-  const float pres = B.m_pc[s].x;
+  const float4 r0s = B.m_r0[s];
+  const float2 vs = make_float2(r0s.x, r0s.y);
+  const float drs = r0s.z;
+  const float pres = MUSCL ? tait_p32(Q, drs) : 0.f;
   const int ts = B.l_type[s];
   KSum fra, sdx, sdy, Sxx, Sxy, Syy, grx, gry;
   KSum gpx, gpy, gux, guy, gvx, gvy;
-  for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o) {
-    if (ts != 2 && B.l_type[o] != 2) return;
+  for_each_neighbour<kNbRecord>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d) {
+    const float4 r0o = B.m_r0[o];
+    if (ts != 2 && r0o.w < 0.f) return;          // no pair between two non-fluid particles
     const double2 po = B.l_pos[o];
     const float dx = (float)(ps.x - po.x), dy = (float)(ps.y - po.y);
     const float r2 = dx * dx + dy * dy;
     const float inv_r = rsqrtf(r2);
     const float r = r2 * inv_r;
     const KernelTerm32 K = kernel_term32(Q, dx, dy, r, inv_r);
-    const float2 rvo = B.m_rv[o];
-    const float2 vo = B.m_vel[o];
+    const float2 rvo = make_float2(r0o.z, fabsf(r0o.w) * __builtin_amdgcn_rcpf(Q.rho0 + r0o.z));
+    const float2 vo = make_float2(r0o.x, r0o.y);
     const float Vo = rvo.y;
     fra.add(Vo * Q.inv_sv);
     sdx.add(K.dwx * Vo);
@@ -106,7 +128,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
     grx.add(rd * K.dwx);
     gry.add(rd * K.dwy);
     if (MUSCL) {
-      const float pd = (B.m_pc[o].x - pres) * Vo;
+      const float pd = (tait_p32(Q, rvo.x) - pres) * Vo;
       gpx.add(pd * K.dwx);
       gpy.add(pd * K.dwy);
       const float ud = vdx * Vo, vd = vdy * Vo;
@@ -115,12 +137,15 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
       gvx.add(vd * K.dwx);
       gvy.add(vd * K.dwy);
     }
-  });
+  }, B.nb_mask, B.kstride);
   const double f = (double)fra.s;
   B.fra_v[s] = f;
   B.sdw[s] = make_double2((double)sdx.s, (double)sdy.s);
   B.grad[s] = make_double2((double)grx.s, (double)gry.s);
-  B.m_grad[s] = make_float2(grx.s, gry.s);
+  {
+    const float4 r2s = B.m_r2[s];                  // keep the position offsets
+    B.m_r2[s] = make_float4(grx.s, gry.s, r2s.z, r2s.w);
+  }
   if (MUSCL) {
     B.m_mg[3 * s + 0] = make_float2(gpx.s, gpy.s);
     B.m_mg[3 * s + 1] = make_float2(gux.s, guy.s);
@@ -144,7 +169,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
     txy = (nut * sxy) * rhos;
     tyy = (2 * nut * syy - kk - blin) * rhos;
   }
-  B.m_tau[s] = make_float4((float)txx, (float)txy, (float)tyy, 0.f);
+  B.m_r1[s] = make_float4((float)txx, (float)txy, (float)tyy, B.m_r1[s].w);
   B.tau[3 * s + 0] = txx;
   B.tau[3 * s + 1] = txy;
   B.tau[3 * s + 2] = tyy;
@@ -161,32 +186,57 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
   const int n = B.counters[C_NLOCAL];
   if (s >= n) return;
   const double2 ps = B.l_pos[s];
-  const float2 vs = B.m_vel[s];
-  const float2 rvs = B.m_rv[s];
-  const float2 pcs = B.m_pc[s];
+  const float4 r0s = B.m_r0[s];
+  const float4 taus = B.m_r1[s];
+  const float4 r2s = B.m_r2[s];
+  const float2 vs = make_float2(r0s.x, r0s.y);
+  const float2 rvs = make_float2(r0s.z, 0.f);
+  const float2 pcs = make_float2(tait_p32(Q, r0s.z), tait_c32(Q, r0s.z));
   const int ts = B.l_type[s];
-  const float4 taus = B.m_tau[s];
-  const float2 grs = B.m_grad[s];
+  const float2 grs = make_float2(r2s.x, r2s.y);
   const int nes = B.near_[s];
   const float rhos = Q.rho0 + rvs.x;
   const float inv_rhos = 1.0f / rhos;
+  // Cell rows of the stencil columns: a neighbour's row (hence its cell corner
+  // relative to ours) follows from where its local index falls in its column.
+  const int cell_s = B.l_cell[s];
+  const int ny = P.chain_ny;
+  const int cxs = cell_s / ny, cys = cell_s - cxs * ny;
+  const int row0 = cys > 0 ? cys - 1 : 0;
+  const int row1 = cys + 1 < ny ? cys + 1 : ny - 1;
+  int rb1[3], rb2[3];                              // first local index of rows row0+1, row0+2
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int col = cxs + c - 1;
+    const bool ok = col >= 0 && col < P.chain_nx;
+    rb1[c] = ok && row0 + 1 <= row1 ? B.cell_start[col * ny + row0 + 1] : 0x7fffffff;
+    rb2[c] = ok && row0 + 2 <= row1 ? B.cell_start[col * ny + row0 + 2] : 0x7fffffff;
+  }
   constexpr bool kMuscl = SOLVER == SPH2D_SOLVER_MUSCL;
   float2 gps = make_float2(0.f, 0.f), gus = gps, gvs = gps;
   if (kMuscl) { gps = B.m_mg[3 * s]; gus = B.m_mg[3 * s + 1]; gvs = B.m_mg[3 * s + 2]; }
   KSum drho, corr, ax, ay, shx, shy, agx, agy;
-  for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o) {
-    if (ts != 2 && B.l_type[o] != 2) return;
-    const double2 po = B.l_pos[o];
-    const float dx = (float)(ps.x - po.x), dy = (float)(ps.y - po.y);
+  for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d) {
+    const float4 r0o = B.m_r0[o];
+    if (ts != 2 && r0o.w < 0.f) return;          // no pair between two non-fluid particles
+    const float4 r1o = B.m_r1[o];
+    const float4 r2o = B.m_r2[o];
+    const int b1 = d == 0 ? rb1[0] : (d == 1 ? rb1[1] : rb1[2]);
+    const int b2 = d == 0 ? rb2[0] : (d == 1 ? rb2[1] : rb2[2]);
+    const int dcy = cys - (row0 + (o >= b1) + (o >= b2));   // cy_s - cy_o
+    const float dx = (r2s.z - r2o.z) + (float)(1 - d) * Q.cell_d;
+    const float dy = (r2s.w - r2o.w) + (float)dcy * Q.cell_d;
     const float r2 = dx * dx + dy * dy;
     const float inv_r = rsqrtf(r2);
     const float r = r2 * inv_r;
     const float inv_r2 = inv_r * inv_r;
     const KernelTerm32 K = kernel_term32(Q, dx, dy, r, inv_r);
-    const float2 vo = B.m_vel[o];
-    const float2 rvo = B.m_rv[o];
-    const float2 pco = B.m_pc[o];
-    const float rhoo = Q.rho0 + rvo.x, Vo = rvo.y;
+    const float2 vo = make_float2(r0o.x, r0o.y);
+    const float mo_x = fabsf(r0o.w);
+    const float rhoo = Q.rho0 + r0o.z;
+    const float2 rvo = make_float2(r0o.z, mo_x * __builtin_amdgcn_rcpf(rhoo));
+    const float2 pco = make_float2(tait_p32(Q, r0o.z), tait_c32(Q, r0o.z));
+    const float Vo = rvo.y;
     const float ex = -(dx * inv_r), ey = -(dy * inv_r);
     const float UL = vs.x * ex + vs.y * ey;
     const float UR = vo.x * ex + vo.y * ey;
@@ -256,7 +306,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
     const float ti = (vown.x - vsx) * K.dwx + (vown.y - vsy) * K.dwy;
     drho.add(2.0f * rhos * Vo * ti);
     const float rij = inv_rhos * __builtin_amdgcn_rcpf(rhoo);
-    const float2 mo = B.m_mass[o];
+    const float2 mo = make_float2(mo_x, r1o.w);
     const float fp = mo.x * (-2.0f * Ps * rij);
     float tx = fp * K.dwx, ty = fp * K.dwy;
     const float vijx = vs.x - vo.x, vijy = vs.y - vo.y;
@@ -271,7 +321,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
     const float tmp = Q.delta_alpha_h * cbar * vx * inv_r2 * Q.rho0 * inv_rhos * Vo;
     tx += tmp * K.dwx;
     ty += tmp * K.dwy;
-    const float4 tauo = B.m_tau[o];
+    const float4 tauo = r1o;
     float tp = (taus.x + tauo.x) * K.dwx + (taus.y + tauo.y) * K.dwy;
     tp = tp * rij;
     tx += mo.y * tp;
@@ -288,7 +338,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
       agx.add(-f * vijx);
       agy.add(-f * vijy);
     }
-    const float2 gro = B.m_grad[o];
+    const float2 gro = make_float2(r2o.x, r2o.y);
     const float psi = (rvo.x - rvs.x) + 0.5f * ((grs.x + gro.x) * dx + (grs.y + gro.y) * dy);
     corr.add(-(Q.two_delta_h * cbar * psi * Rd * inv_r2 * Vo));
     if (do_shift) {
@@ -297,7 +347,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
       shx.add(-f * K.dwx);
       shy.add(-f * K.dwy);
     }
-  });
+  }, B.nb_mask, B.kstride);
   double dr = (double)drho.s;
   if (ts == 2) dr = dr + (double)corr.s;
   B.drhodt[s] = dr;

@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
   const int ts = B.l_type[s];
   double fra = 0.0, sdx = 0.0, sdy = 0.0, Sxx = 0.0, Sxy = 0.0, Syy = 0.0, grx = 0.0, gry = 0.0;
   double gpx = 0.0, gpy = 0.0, gux = 0.0, guy = 0.0, gvx = 0.0, gvy = 0.0;
-  for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o) {
+  for_each_neighbour<kNbRecord>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d) {
     if (ts != 2 && B.l_type[o] != 2) return;     // no pair between two non-fluid particles
     const double2 po = B.l_pos[o];
     const double dx = ps.x - po.x, dy = ps.y - po.y;
@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
       gvx = gvx + vd * K.dwx;
       gvy = gvy + vd * K.dwy;
     }
-  });
+  }, B.nb_mask, B.kstride);
   B.fra_v[s] = fra;
   B.sdw[s] = make_double2(sdx, sdy);
   B.grad[s] = make_double2(grx, gry);
@@ -168,9 +168,9 @@ __global__ void __launch_bounds__(kBlock) k_eval_b2(DevParams P, Buffers B) {
   if (B.counters[C_NSURF] == 0) return;            // near_ was zeroed by pass A
   const int ts = B.l_type[s];
   int nr = 0;
-  for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, B.l_pos[s], B.l_cell[s], [&](int o) {
+  for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, B.l_pos[s], B.l_cell[s], [&](int o, int d) {
     if (B.surf[o] == 1 && (ts == 2 || B.l_type[o] == 2)) nr = 1;
-  });
+  }, B.nb_mask, B.kstride);
   B.near_[s] = nr;
 }
 
@@ -205,7 +205,7 @@ __device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int
   double drho = 0.0, corr = 0.0, ax = 0.0, ay = 0.0, shx = 0.0, shy = 0.0, agx = 0.0, agy = 0.0;
   double iax = 0.0, iay = 0.0, avx = 0.0, avy = 0.0, aix = 0.0, aiy = 0.0;
   double spx = 0.0, spy = 0.0, lpx = 0.0, lpy = 0.0;
-  for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o) {
+  for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d) {
     if (ts != 2 && B.l_type[o] != 2) return;
     const double2 po = B.l_pos[o];
     const double dx = ps.x - po.x, dy = ps.y - po.y;      // x_ij with i = s
@@ -339,7 +339,7 @@ __device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int
       shx = shx - f * K.dwx;
       shy = shy - f * K.dwy;
     }
-  });
+  }, B.nb_mask, B.kstride);
   if (ts == 2) drho = drho + corr;
   B.drhodt[s] = drho;
   if (DEBUG) {
@@ -355,17 +355,10 @@ __device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int
   if (do_shift) B.shift[s] = make_double2(shx, shy);
 }
 
-// The two linear-Riemann bodies fit 168 VGPRs (3 waves per SIMD) without
-// spilling when asked to; HLLC, MUSCL and the debug form keep the compiler's
-// own allocation (asking them for 3 waves spills to scratch).
-template <int SOLVER>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
-k_eval_c(DevParams P, Buffers B, int do_shift) {
-  eval_c<false, SOLVER>(P, B, do_shift);
-}
-
+// The compiler's own register allocation: asking for 3 waves per SIMD (168
+// VGPRs) spills the force pass to scratch and costs more than it gains.
 template <bool DEBUG, int SOLVER>
-__global__ void __launch_bounds__(kBlock) k_eval_c_wide(DevParams P, Buffers B, int do_shift) {
+__global__ void __launch_bounds__(kBlock) k_eval_c(DevParams P, Buffers B, int do_shift) {
   eval_c<DEBUG, SOLVER>(P, B, do_shift);
 }
 
@@ -373,23 +366,17 @@ template <bool DEBUG>
 static void launch_c64(hipStream_t s, const DevParams& P, Buffers& B, int g) {
   switch (P.solver) {
     case SPH2D_SOLVER_LINEAR_RIEMANN:
-      if (DEBUG)
-        hipLaunchKernelGGL((k_eval_c_wide<true, SPH2D_SOLVER_LINEAR_RIEMANN>), dim3(g), dim3(kBlock), 0, s, P, B, 1);
-      else
-        hipLaunchKernelGGL(k_eval_c<SPH2D_SOLVER_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, s, P, B, 1);
+      hipLaunchKernelGGL((k_eval_c<DEBUG, SPH2D_SOLVER_LINEAR_RIEMANN>), dim3(g), dim3(kBlock), 0, s, P, B, 1);
       break;
     case SPH2D_SOLVER_HLLC:
-      hipLaunchKernelGGL((k_eval_c_wide<DEBUG, SPH2D_SOLVER_HLLC>), dim3(g), dim3(kBlock), 0, s, P, B, 1);
+      hipLaunchKernelGGL((k_eval_c<DEBUG, SPH2D_SOLVER_HLLC>), dim3(g), dim3(kBlock), 0, s, P, B, 1);
       break;
     case SPH2D_SOLVER_MUSCL:
-      hipLaunchKernelGGL((k_eval_c_wide<DEBUG, SPH2D_SOLVER_MUSCL>), dim3(g), dim3(kBlock), 0, s, P, B, 1);
+      hipLaunchKernelGGL((k_eval_c<DEBUG, SPH2D_SOLVER_MUSCL>), dim3(g), dim3(kBlock), 0, s, P, B, 1);
       break;
     default:
-      if (DEBUG)
-        hipLaunchKernelGGL((k_eval_c_wide<true, SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN>), dim3(g), dim3(kBlock), 0,
-                           s, P, B, 1);
-      else
-        hipLaunchKernelGGL(k_eval_c<SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, s, P, B, 1);
+      hipLaunchKernelGGL((k_eval_c<DEBUG, SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN>), dim3(g), dim3(kBlock), 0, s,
+                         P, B, 1);
   }
 }
 
@@ -398,16 +385,20 @@ static void launch_a64(hipStream_t s, const DevParams& P, Buffers& B, int g) {
   else hipLaunchKernelGGL(k_eval_a<false>, dim3(g), dim3(kBlock), 0, s, P, B);
 }
 
-void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
+void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, hipEvent_t* ev) {
   const int g = grid_for(cap_local);
   launch_fill_i32(s, B.counters + C_NSURF, 1, 0);
+  if (ev) (void)hipEventRecord(ev[0], s);
   if (P.mixed) launch_eval32_a(s, P, B, cap_local);
   else launch_a64(s, P, B, g);
+  if (ev) (void)hipEventRecord(ev[1], s);
   hipLaunchKernelGGL(k_eval_b1, dim3(g), dim3(kBlock), 0, s, P, B);
   hipLaunchKernelGGL(k_eval_b2, dim3(g), dim3(kBlock), 0, s, P, B);
+  if (ev) (void)hipEventRecord(ev[2], s);
   if (P.mixed) launch_eval32_c(s, P, B, cap_local);
   else if (P.write_debug) launch_c64<true>(s, P, B, g);
   else launch_c64<false>(s, P, B, g);
+  if (ev) (void)hipEventRecord(ev[3], s);
 }
 
 void launch_eval_pass(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int pass) {

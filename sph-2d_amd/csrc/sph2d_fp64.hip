@@ -293,18 +293,36 @@ __global__ void k_cell_key(DevParams P, Buffers B, int cap_global) {
   int t = B.g_type[g];
   const double2 p = B.g_pos[g];
   if (g < N && t != 120 && p.y < 0.0) { t = 120; B.g_type[g] = 120; }
-  if (t == 120 || t == 0) { B.g_key[g] = -1; return; }
-  const int xn = (int)floor((p.x - P.chain_ox) / P.chain_d + 1);
-  const int yn = (int)floor((p.y - P.chain_oy) / P.chain_d + 1);
-  if (xn < 1 || xn > P.chain_nx || yn < 1 || yn > P.chain_ny) {
-    B.g_type[g] = 120;
-    B.g_key[g] = -1;
-    atomicAdd(&B.counters[C_NILL], 1);
-    return;
+  int key = -1;
+  if (t != 120 && t != 0) {
+    const int xn = (int)floor((p.x - P.chain_ox) / P.chain_d + 1);
+    const int yn = (int)floor((p.y - P.chain_oy) / P.chain_d + 1);
+    if (xn < 1 || xn > P.chain_nx || yn < 1 || yn > P.chain_ny) {
+      B.g_type[g] = 120;
+      atomicAdd(&B.counters[C_NILL], 1);
+    } else {
+      key = (xn - 1) * P.chain_ny + (yn - 1);
+    }
   }
-  const int key = (xn - 1) * P.chain_ny + (yn - 1);
   B.g_key[g] = key;
-  B.g_slot[g] = atomicAdd(&B.cell_count[key], 1);
+  B.l_of_g[g] = -1;                                // set again by k_cell_rank for kept particles
+  // One atomic per distinct key of the wave (neighbouring gids share cells): the
+  // lowest lane holding a key adds the number of lanes holding it; the others take
+  // their slots after it. Slots only need to be distinct: the within-cell order
+  // comes from the rank by gid in k_cell_rank.
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint64_t todo = __ballot(key >= 0);
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const int lk = __builtin_amdgcn_readlane(key, leader);
+    const uint64_t same = __ballot(key == lk);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&B.cell_count[lk], __popcll(same));
+    base = __builtin_amdgcn_readlane(base, leader);
+    if (key == lk) B.g_slot[g] = base + __popcll(same & below);
+    todo &= ~same;
+  }
 }
 
 __global__ void k_cell_scatter(Buffers B, int cap_global) {
@@ -326,8 +344,6 @@ __global__ void k_cell_scatter(Buffers B, int cap_global) {
 // EOS and volume (single_step_compute.f90:62-73, Assign_boundary_particles.f90:303-321)
 // into the cell's contiguous slice of the local arrays.
 __device__ __forceinline__ void gather_one(const DevParams& P, Buffers& B, int g, int l, int key) {
-  B.name[l] = g;
-  B.l_cell[l] = key;
   const double2 pos = B.g_pos[g];
   B.l_pos[l] = pos;
   B.l_pos_bin[l] = pos;
@@ -339,18 +355,22 @@ __device__ __forceinline__ void gather_one(const DevParams& P, Buffers& B, int g
   const int t = B.g_type[g];
   B.l_mass[l] = m;
   B.l_type[l] = t;
-  B.l_e[l] = B.g_e[g];
   double p = B.g_p[g], c = B.g_c[g];
   if (t != 120) tait(P, rho, p, c);
   B.l_pc[l] = make_double2(p, c);
   B.l_rv[l] = make_double2(rho, m / rho);
   if (P.mixed) {
-    write_mirror(P, B, l, vel, rho, m / rho, p, c);
-    B.m_mass[l] = make_float2((float)m, (float)B.g_mass[l]);   // particle_mass(j) with j local
+    write_r0(P, B, l, vel, rho, m, t);
+    B.m_r1[l] = make_float4(0.f, 0.f, 0.f, (float)B.g_mass[l]);   // particle_mass(j) with j local
+    const float2 off = cell_offset(P, pos, key);
+    B.m_r2[l] = make_float4(0.f, 0.f, off.x, off.y);
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_cell_rank_gather(DevParams P, Buffers B, int n_cells) {
+// One wavefront per cell ranks the members by gid and writes the local order
+// (name, cell, and the inverse map l_of_g); the state copy runs afterwards with
+// one thread per local slot, so no lanes idle on the wide copy.
+__global__ void __launch_bounds__(kBlock) k_cell_rank(Buffers B, int n_cells) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int cell = blockIdx.x * (kBlock / 64) + wave;
   if (cell >= n_cells) return;
@@ -366,8 +386,19 @@ __global__ void __launch_bounds__(kBlock) k_cell_rank_gather(DevParams P, Buffer
       const int m = min(64, cnt - b2);
       for (int k = 0; k < m; ++k) rank += (__builtin_amdgcn_readlane(v, k) < my);
     }
-    if (idx < cnt) gather_one(P, B, my, s0 + rank, cell);
+    if (idx < cnt) {
+      const int l = s0 + rank;
+      B.name[l] = my;
+      B.l_cell[l] = cell;
+      B.l_of_g[my] = l;
+    }
   }
+}
+
+__global__ void __launch_bounds__(kBlock) k_local_gather(DevParams P, Buffers B) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= B.counters[C_NLOCAL]) return;
+  gather_one(P, B, B.name[l], l, B.l_cell[l]);
 }
 
 // Largest cell population (diagnostic, on demand): one block folds cell_start.
@@ -401,8 +432,9 @@ void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_
   exclusive_scan(s, B.cell_count, n_cells, B.cell_start, B.scan_blocks, B.counters + C_NLOCAL,
                  B.cell_start + n_cells);
   hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, B, cap_global);
-  hipLaunchKernelGGL(k_cell_rank_gather, dim3((n_cells + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, s, P, B,
+  hipLaunchKernelGGL(k_cell_rank, dim3((n_cells + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, s, B,
                      n_cells);
+  hipLaunchKernelGGL(k_local_gather, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, P, B);
 }
 
 // ---------------------------------------------------------------------------
@@ -414,7 +446,18 @@ __device__ __forceinline__ void eos_volume(const DevParams& P, Buffers& B, int s
   const double vol = B.l_mass[s] / rho;
   B.l_pc[s] = make_double2(p, c);
   B.l_rv[s] = make_double2(rho, vol);
-  if (P.mixed) write_mirror(P, B, s, B.l_vel[s], rho, vol, p, c);
+  if (P.mixed) write_r0(P, B, s, B.l_vel[s], rho, B.l_mass[s], B.l_type[s]);
+}
+
+// Positions moved: the FP32 copy for the neighbour screen and, in the mixed
+// form, the offsets to the binned cell's corner (grad rho is rewritten by the
+// next pass A).
+__device__ __forceinline__ void write_pos_mirrors(const DevParams& P, Buffers& B, int s, double2 x) {
+  B.l_pos32[s] = make_float2((float)x.x, (float)x.y);
+  if (P.mixed) {
+    const float2 off = cell_offset(P, x, B.l_cell[s]);
+    B.m_r2[s] = make_float4(0.f, 0.f, off.x, off.y);
+  }
 }
 
 __global__ void k_stage(DevParams P, Buffers B, int kind) {
@@ -434,7 +477,10 @@ __global__ void k_stage(DevParams P, Buffers B, int kind) {
     return;
   }
   if (kind == ST_EOS) {
-    if (t != 120) eos_volume(P, B, s, rho);
+    if (t != 120) {
+      eos_volume(P, B, s, rho);
+      write_pos_mirrors(P, B, s, x);
+    }
     return;
   }
   if (kind == ST_PC_PREDICT || kind == ST_RK_HALF0) {      // before_* = state (whole arrays)
@@ -450,7 +496,7 @@ __global__ void k_stage(DevParams P, Buffers B, int kind) {
       v.y = bv.y + dt * a.y + avg.y;
       x.y = bx.y + dt * v.y;
       B.l_vel[s] = v; B.l_pos[s] = x;
-      B.l_pos32[s] = make_float2((float)x.x, (float)x.y);
+      write_pos_mirrors(P, B, s, x);
       eos_volume(P, B, s, rho);
     }
     B.t_vel[s] = v; B.t_drho[s] = drho; B.t_acc[s] = a;    // :573-576
@@ -483,7 +529,7 @@ __global__ void k_stage(DevParams P, Buffers B, int kind) {
         x.y = bx.y + dt * v.y;
       }
       B.l_vel[s] = v; B.l_pos[s] = x;
-      B.l_pos32[s] = make_float2((float)x.x, (float)x.y);
+      write_pos_mirrors(P, B, s, x);
       eos_volume(P, B, s, rho);
     }
     const int k = kind == ST_RK_HALF0 ? 0 : (kind == ST_RK_HALF1 ? 1 : 2);
@@ -540,7 +586,7 @@ __global__ void __launch_bounds__(kBlock) k_mls(DevParams P, Buffers B) {
   double bv[6] = {0, 0, 0, 0, 0, 0};
   double sw = 0.0, srw = 0.0;
   int ne = 0;
-  for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o) {
+  for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d) {
     const double2 po = B.l_pos[o];
     const double pdx = ps.x - po.x, pdy = ps.y - po.y;
     const double r = sqrt(pdx * pdx + pdy * pdy);
@@ -658,11 +704,15 @@ void launch_shift_apply_mode(hipStream_t s, const DevParams& P, Buffers& B, int 
 }
 
 // Refresh_domain.f90:136-300 on one rank.
+// Gather form: one thread per fluid gid reads its local slot (l_of_g), so the
+// global arrays are written in order. Ghost gids are not written back: the
+// next step regenerates them before anything reads them. Energy stays in the
+// global arrays only: it is never updated (its rate is zeroed and never set).
 __global__ void k_refresh(Buffers B) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = B.counters[C_NLOCAL];
-  if (s >= n) return;
-  const int g = B.name[s];
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= B.counters[C_NFLUID]) return;
+  const int s = B.l_of_g[g];
+  if (s < 0) return;
   B.g_pos[g] = B.l_pos[s];
   B.g_vel[g] = B.l_vel[s];
   const double2 rv = B.l_rv[s];
@@ -670,9 +720,7 @@ __global__ void k_refresh(Buffers B) {
   B.g_rho[g] = rv.x;
   B.g_p[g] = pc.x;
   B.g_c[g] = pc.y;
-  B.g_mass[g] = B.l_mass[s];
-  B.g_type[g] = B.l_type[s];
-  B.g_e[g] = B.l_e[s];
+  // mass, type and energy do not change inside a step (dedt = 0: single_step_compute.f90:113-126)
 }
 
 void launch_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {

@@ -36,7 +36,7 @@ struct DevParams {
   double mls_alpha;
   double surf_lim;                       // REAL(4) 1.414 * h
   double inv_h, inv_c0, inv_sv, inv_wdx, cut2;  // reciprocals and the squared-distance pre-test
-  float cut2_32;                         // FP32 pre-test bound: never below (3h + FP32 position error)^2
+  float cut2_32;                         // FP32 screen: above this r^2 a pair is certainly outside 3h
   double r2_max, r2_min;                 // fl(sqrt(r2)) <= 3h  <=>  r2 <= r2_max; >= 1e-7 <=> r2 >= r2_min
   double x_over_t;                       // interior_dx/dt, the HLLC fan sample point
   int solver, write_debug;
@@ -62,15 +62,18 @@ struct Buffers {
   int *cell_count, *cell_start, *cell_members;
   // local order
   int *name, *l_type, *l_cell;
+  int *l_of_g;                            // local index of each gid (-1: not in the subdomain)
   double2 *l_pos, *l_vel, *l_rv, *l_pc, *l_pos_bin;
   float2 *l_pos32;                        // FP32 copy of l_pos for the neighbour pre-test
-  double *l_mass, *l_e;
+  double *l_mass;
   // one force evaluation
   double *fra_v;
   double2 *sdw, *grad, *nrm, *shift, *avg;
   double *tau;                            // 3 per particle: xx, xy, yy
   double2 *mg;                            // MUSCL only, 3 per particle: grad p, grad u, grad v
   int *surf, *near_;
+  // accepted-neighbour masks of the last pass A: 4 words per particle, word w at [w * kstride + s]
+  uint64_t* nb_mask;
   double *drhodt;
   double2 *acc;
   double *dbg;                            // 10 per particle (debug components)
@@ -84,11 +87,13 @@ struct Buffers {
   double *red;                            // reduction partials
   int *pair_count, *pair_off;
   int *pair_lo, *pair_hi;
-  // FP32 mirrors of the neighbour data for the mixed form (local order):
-  // velocity, (rho - rho0, volume), (p, c), (mass, particle_mass at the local index),
-  // tau (xx, xy, yy, 0), grad rho
-  float2 *m_vel, *m_rv, *m_pc, *m_mass, *m_grad;
-  float4 *m_tau;
+  // FP32 neighbour records of the mixed form (local order), 48 bytes per particle:
+  //   m_r0 = (u, v, rho - rho0, m), m negated for particles that are not fluid
+  //   m_r1 = (tau_xx, tau_xy, tau_yy, particle_mass at the local index)
+  //   m_r2 = (grad rho x, grad rho y, x - x_cell, y - y_cell): offsets from the
+  //          corner of the particle's binned cell, so neighbour offsets keep
+  //          FP32 precision relative to h
+  float4 *m_r0, *m_r1, *m_r2;
   float2 *m_mg;                           // MUSCL only: FP32 copy of mg
 
   int kstride;                            // stride of the 3 RK stage slabs (= local capacity)
@@ -99,7 +104,8 @@ void launch_init_tgv(hipStream_t s, const DevParams& P, Buffers& B, int nx, int 
 void launch_period_boundary(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int cap_ghost);
 void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_global, int n_cells,
                          bool analytic, double t_nondim);
-void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+// ev (optional, 4 events): recorded before and after pass A and before and after the force pass.
+void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, hipEvent_t* ev = nullptr);
 // FP32-mixed passes A and C (sph2d_eval32.hip)
 void launch_eval32_a(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
 void launch_eval32_c(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);

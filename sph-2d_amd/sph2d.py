@@ -79,7 +79,7 @@ EXPORTS = (
     "sph2d_refresh_domain", "sph2d_time_step", "sph2d_run", "sph2d_sample",
     "sph2d_download_local", "sph2d_download_pairs", "sph2d_set_debug",
     "sph2d_get_diagnostics", "sph2d_get_stream", "sph2d_time_pass",
-    "sph2d_reorder_particle_index",
+    "sph2d_reorder_particle_index", "sph2d_set_pass_timing", "sph2d_get_pass_timing",
 )
 
 _lib = None
@@ -121,6 +121,8 @@ def lib():
             "sph2d_get_diagnostics": [vp, P(Diag)],
             "sph2d_get_stream": [vp, P(vp)],
             "sph2d_time_pass": [vp, i32, i32, dp],
+            "sph2d_set_pass_timing": [vp, i32],
+            "sph2d_get_pass_timing": [vp, dp, ip],
             "sph2d_reorder_particle_index": [i32, i32, ctypes.c_double, ctypes.c_double, vp, vp, vp, i32,
                                              P(vp), i32, P(vp), vp, ip, vp],
         }
@@ -169,7 +171,7 @@ LOCAL_F64 = ("x", "y", "u", "v", "rho", "mass", "p", "c", "volume", "fra_v", "su
              "ax", "ay", "shift_x", "shift_y", "normal_x", "normal_y", "internal_x", "internal_y",
              "visc_x", "visc_y", "art_internal_x", "art_internal_y", "sps_x", "sps_y", "lap_x",
              "lap_y", "grad_p_x", "grad_p_y", "grad_u_x", "grad_u_y", "grad_v_x", "grad_v_y")
-LOCAL_I32 = ("type", "name", "surface", "near_surface")
+LOCAL_I32 = ("type", "name", "surface", "near_surface", "neighbour_count")
 
 
 class Simulation:
@@ -295,6 +297,17 @@ class Simulation:
         ms = ctypes.c_double()
         _check(lib().sph2d_time_pass(self._c, pass_id, reps, ctypes.byref(ms)))
         return ms.value
+
+    def set_pass_timing(self, on: bool) -> None:
+        """Record HIP events around pass A and the force pass of every evaluation."""
+        _check(lib().sph2d_set_pass_timing(self._c, 1 if on else 0))
+
+    def pass_timing(self):
+        """Average in-step ms since the last call: (force pass, pass A, evaluation), count."""
+        out = np.zeros(3)
+        n = ctypes.c_int32()
+        _check(lib().sph2d_get_pass_timing(self._c, _dp(out), ctypes.byref(n)))
+        return float(out[0]), float(out[1]), float(out[2]), n.value
 
     def stream(self) -> int:
         s = ctypes.c_void_p()

@@ -386,3 +386,38 @@ def test_mixed_ke_error_vs_fp64_at_t05():
         f"L2u fp64={ke[0][4]:.6e} mixed={ke[1][4]:.6e}")
     assert err < 1e-4
     assert abs(ke[0][0] - np.exp(-16 * np.pi ** 2 * 0.01 * 0.5)) < 0.05
+
+
+def test_neighbour_masks_exact_at_the_support_boundary():
+    """The FP32 neighbour screen decides in FP64 whatever lies within its error of
+    r = 3h or r = 1e-7: pairs placed at 3h (1 +- a few ulp) and a near-coincident
+    pair must be counted exactly as find_particle_pair.f90:87's rule counts them."""
+    say("test_neighbour_masks_exact_at_the_support_boundary")
+    n = 48
+    x, y, u, v, rho, m = perturbed_tgv(n, amp=0.2, seed=21)
+    three_h = 3 * sph2d.tgv_params(n).h
+    rng = np.random.default_rng(5)
+    moved = []
+    for k, rel in enumerate((-1e-7, -4e-16, 0.0, 4e-16, 1e-7, -2e-16, 2e-16, 3e-7)):
+        a, b = 100 + 97 * k, 1500 + 53 * k
+        th = rng.uniform(0, 2 * np.pi)
+        x[b] = x[a] + three_h * (1 + rel) * np.cos(th)
+        y[b] = y[a] + three_h * (1 + rel) * np.sin(th)
+        moved.append((a, b))
+    x[2000], y[2000] = x[1999] + 5e-8, y[1999]            # closer than r_min = 1e-7
+    x[2100], y[2100] = x[2099] + 2e-7, y[2099]            # just beyond r_min
+    sim, o = pair(n, state=(x, y, u, v, rho, m))
+    evaluate_both(sim, o)
+    cnt = sim.local("neighbour_count")
+    lx, ly = sim.local("x"), sim.local("y")
+    rmin = float(np.float32(1.0e-7))
+    ref = np.zeros(len(lx), dtype=np.int64)
+    for i in range(len(lx)):
+        dx = lx[i] - lx
+        dy = ly[i] - ly
+        r = np.sqrt(dx * dx + dy * dy)
+        ok = (r <= three_h) & (r >= rmin)
+        ok[i] = False
+        ref[i] = int(np.sum(ok))
+    bad = np.nonzero(cnt != ref)[0]
+    assert len(bad) == 0, (bad[:10], cnt[bad[:10]], ref[bad[:10]])
