@@ -81,8 +81,9 @@ namespace sph2d {
 // Phase 1 screens every candidate with FP32 positions against a bound that no
 // pair within 3h can exceed and keeps a 256-bit mask of the survivors (about 40
 // of 207); phase 2 applies the exact FP64 criterion of find_particle_pair.f90:87
-// (1e-7 <= r <= 3h, r = sqrt(dx*dx+dy*dy)) to them and calls f(o, d) (d: the
-// candidate's stencil column, 0..2 for cell columns cx-1..cx+1) for the
+// (1e-7 <= r <= 3h, r = sqrt(dx*dx+dy*dy)) to them and calls f(o, d, li) (d:
+// the candidate's stencil column, 0..2 for cell columns cx-1..cx+1; li: its slot
+// in a staged block's LDS copy, -1 when not staged) for the
 // accepted ones, in ascending candidate order, so the lanes of a wave stay
 // converged on the pair body. Lists longer than 256 run in chunks.
 //
@@ -102,18 +103,131 @@ __device__ __forceinline__ bool is_pair_r2(const DevParams& P, double dx, double
 
 enum { kNbScan = 0, kNbRecord = 1, kNbReplay = 2 };
 
+// Per-block staging of candidate positions in LDS (pass A). The particles of a
+// block are consecutive in local order; when they all sit in one cell column,
+// the union of their candidate lists is three contiguous local ranges (the
+// stencil columns over rows first-1 .. last+1), which the block copies to LDS
+// once -- FP32 and FP64 positions -- and every thread's own ranges are
+// sub-ranges of them. A block that straddles two columns, or whose union
+// exceeds kStageCap, reads global memory as before.
+constexpr int kStageCap = 1024;
+
+struct NbStage {
+  int use;
+  int lo[3];      // first staged local index of each stencil column
+  int base[3];    // its LDS slot
+  int total;
+};
+
+// The block's candidate union (st.use = 0 when it cannot be staged within cap);
+// ends with a barrier.
+__device__ __forceinline__ void plan_stage(const DevParams& P, const int* __restrict__ cell_start,
+                                           const int* __restrict__ l_cell, int n, int cap, NbStage& st) {
+  if (threadIdx.x == 0) {
+    int use = 0, t = 0;
+    int lo[3] = {0, 0, 0}, cnt[3] = {0, 0, 0};
+    const int b0 = blockIdx.x * blockDim.x;
+    if (b0 < n) {
+      const int b1 = min(b0 + (int)blockDim.x, n) - 1;
+      const int ny = P.chain_ny;
+      const int c0 = l_cell[b0], c1 = l_cell[b1];
+      const int cx = c0 / ny;
+      if (c1 / ny == cx) {
+        const int rlo = max(c0 - cx * ny - 1, 0), rhi = min(c1 - cx * ny + 1, ny - 1);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const int col = cx + d - 1;
+          if (col >= 0 && col < P.chain_nx) {
+            lo[d] = cell_start[col * ny + rlo];
+            cnt[d] = cell_start[col * ny + rhi + 1] - lo[d];
+          }
+          t += cnt[d];
+        }
+        use = t <= cap;
+      }
+    }
+    st.use = use;
+    st.total = use ? t : 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) st.lo[d] = lo[d];
+    st.base[0] = 0;
+    st.base[1] = cnt[0];
+    st.base[2] = cnt[0] + cnt[1];
+  }
+  __syncthreads();
+}
+
+// Local index of LDS slot i of a planned stage.
+__device__ __forceinline__ int stage_source(const NbStage& st, int i) {
+  const int d = i < st.base[1] ? 0 : (i < st.base[2] ? 1 : 2);
+  return st.lo[d] + (i - st.base[d]);
+}
+
+__device__ __forceinline__ void stage_candidates(const DevParams& P, const int* __restrict__ cell_start,
+                                                 const int* __restrict__ l_cell, const float2* __restrict__ pos32,
+                                                 const double2* __restrict__ pos, int n, NbStage& st,
+                                                 float2* lds32, double2* lds64) {
+  plan_stage(P, cell_start, l_cell, n, kStageCap, st);
+  const int total = st.total;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    const int o = stage_source(st, i);
+    lds32[i] = pos32[o];
+    lds64[i] = pos[o];
+  }
+  __syncthreads();
+}
+
+// Exact acceptance mask of candidates c0 .. c0+255 of a candidate list (screen,
+// then the exact test on the survivors); for lists longer than the recorded 256.
+struct Mask256 {
+  uint64_t w0, w1, w2, w3;
+};
+
+__device__ __forceinline__ Mask256 exact_chunk_mask(const DevParams& P, const double2* __restrict__ pos,
+                                                    const float2* __restrict__ pos32, int s, double2 ps,
+                                                    int n0, int n01, int l0, int l1, int l2, int total, int c0) {
+  const float2 ps32 = pos32[s];
+  uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+#pragma unroll 1
+  for (int w = 0; w < 4; ++w) {
+    const int base = c0 + 64 * w;
+    const int cn = min(64, total - base);
+    uint64_t mw = 0;
+    for (int k = 0; k < cn; ++k) {
+      const int kk = base + k;
+      const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
+      const float2 q = pos32[o];
+      const float fx = ps32.x - q.x, fy = ps32.y - q.y;
+      bool ok = (fx * fx + fy * fy <= P.cut2_32) && (o != s);
+      if (ok) {
+        const double2 po = pos[o];
+        ok = is_pair_r2(P, ps.x - po.x, ps.y - po.y);
+      }
+      mw |= (uint64_t)ok << k;
+    }
+    if (w == 0) m0 = mw; else if (w == 1) m1 = mw; else if (w == 2) m2 = mw; else m3 = mw;
+  }
+  return Mask256{m0, m1, m2, m3};
+}
+
 template <int MODE = kNbScan, class F>
 __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int* __restrict__ cell_start,
                                                    const double2* __restrict__ pos,
                                                    const float2* __restrict__ pos32, int s, double2 ps,
                                                    int cell, F&& f, uint64_t* __restrict__ masks = nullptr,
-                                                   int mstride = 0) {
+                                                   int mstride = 0, const NbStage* st = nullptr,
+                                                   const float2* lds32 = nullptr, const double2* lds64 = nullptr) {
   const CellRanges R = cell_ranges(P, cell_start, cell);
   const int n0 = R.hi[0] - R.lo[0], n1 = R.hi[1] - R.lo[1], n2 = R.hi[2] - R.lo[2];
   const int n01 = n0 + n1, total = n01 + n2;
   const int l0 = R.lo[0], l1 = R.lo[1] - n0, l2 = R.lo[2] - n01;
   const bool stored = total <= 256;
   const bool replay = MODE == kNbReplay && stored;
+  const bool staged = st != nullptr && st->use;     // uniform over the block
+  // LDS slot of candidate kk: kk + s0/s1/s2 by stencil column
+  const int s0 = staged ? l0 - st->lo[0] + st->base[0] : 0;
+  const int s1 = staged ? l1 - st->lo[1] + st->base[1] : 0;
+  const int s2 = staged ? l2 - st->lo[2] + st->base[2] : 0;
   float2 ps32 = make_float2(0.f, 0.f);
   if (!replay) ps32 = pos32[s];
   for (int c0 = 0; c0 < total; c0 += 256) {
@@ -139,14 +253,27 @@ __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int
         const int base = c0 + 64 * w;
         const int cn = min(64, total - base);
         uint64_t mw = 0;
+        if (staged) {
 #pragma unroll 2
-        for (int k = 0; k < cn; ++k) {
-          const int kk = base + k;
-          const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
-          const float2 q = pos32[o];
-          const float fx = ps32.x - q.x, fy = ps32.y - q.y;
-          const bool ok = (fx * fx + fy * fy <= P.cut2_32) && (o != s);
-          mw |= (uint64_t)ok << k;
+          for (int k = 0; k < cn; ++k) {
+            const int kk = base + k;
+            const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
+            const int li = kk < n0 ? s0 + kk : (kk < n01 ? s1 + kk : s2 + kk);
+            const float2 q = lds32[li];
+            const float fx = ps32.x - q.x, fy = ps32.y - q.y;
+            const bool ok = (fx * fx + fy * fy <= P.cut2_32) && (o != s);
+            mw |= (uint64_t)ok << k;
+          }
+        } else {
+#pragma unroll 2
+          for (int k = 0; k < cn; ++k) {
+            const int kk = base + k;
+            const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
+            const float2 q = pos32[o];
+            const float fx = ps32.x - q.x, fy = ps32.y - q.y;
+            const bool ok = (fx * fx + fy * fy <= P.cut2_32) && (o != s);
+            mw |= (uint64_t)ok << k;
+          }
         }
         if (w == 0) m0 = mw; else if (w == 1) m1 = mw; else if (w == 2) m2 = mw; else m3 = mw;
       }
@@ -162,17 +289,16 @@ __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int
       const int kk = c0 + w * 64 + b;
       const int d = kk < n0 ? 0 : (kk < n01 ? 1 : 2);     // stencil column of the candidate
       const int o = d == 0 ? l0 + kk : (d == 1 ? l1 + kk : l2 + kk);
-      if (replay) {
-        f(o, d);                                  // the recorded mask is exact
-      } else {
-        const double2 po = pos[o];
-        if (is_pair_r2(P, ps.x - po.x, ps.y - po.y)) {
-          if (MODE == kNbRecord) {
-            if (w == 0) a0 |= bit; else if (w == 1) a1 |= bit; else if (w == 2) a2 |= bit; else a3 |= bit;
-          }
-          f(o, d);
+      const int li = staged ? (d == 0 ? s0 + kk : (d == 1 ? s1 + kk : s2 + kk)) : -1;   // LDS slot
+      bool accept = true;                         // a replayed mask is exact
+      if (!replay) {
+        const double2 po = staged ? lds64[li] : pos[o];
+        accept = is_pair_r2(P, ps.x - po.x, ps.y - po.y);
+        if (MODE == kNbRecord && accept) {
+          if (w == 0) a0 |= bit; else if (w == 1) a1 |= bit; else if (w == 2) a2 |= bit; else a3 |= bit;
         }
       }
+      if (accept) f(o, d, li);                    // one call site: the body inlines once
     }
     if (MODE == kNbRecord && stored) {
       masks[s] = a0;

@@ -94,8 +94,12 @@ __device__ __forceinline__ KernelTerm32 kernel_term32(const DevParams32& Q, floa
 // pass A (mixed): see k_eval_a in sph2d_eval64.hip for the reference lines.
 template <bool MUSCL>
 __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q, Buffers B) {
+  __shared__ NbStage st;
+  __shared__ float2 lds32[kStageCap];
+  __shared__ double2 lds64[kStageCap];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
+  stage_candidates(P, B.cell_start, B.l_cell, B.l_pos32, B.l_pos, n, st, lds32, lds64);
   if (s >= n) return;
   const double2 ps = B.l_pos[s];
   const float4 r0s = B.m_r0[s];
@@ -105,7 +109,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
   const int ts = B.l_type[s];
   KSum fra, sdx, sdy, Sxx, Sxy, Syy, grx, gry;
   KSum gpx, gpy, gux, guy, gvx, gvy;
-  for_each_neighbour<kNbRecord>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d) {
+  for_each_neighbour<kNbRecord>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) __attribute__((always_inline)) {
     const float4 r0o = B.m_r0[o];
     if (ts != 2 && r0o.w < 0.f) return;          // no pair between two non-fluid particles
     const double2 po = B.l_pos[o];
@@ -137,7 +141,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
       gvx.add(vd * K.dwx);
       gvy.add(vd * K.dwy);
     }
-  }, B.nb_mask, B.kstride);
+  }, B.nb_mask, B.kstride, &st, lds32, lds64);
   const double f = (double)fra.s;
   B.fra_v[s] = f;
   B.sdw[s] = make_double2((double)sdx.s, (double)sdy.s);
@@ -180,10 +184,25 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
 }
 
 // pass C (mixed): see k_eval_c in sph2d_eval64.hip for the reference lines.
+// A block whose candidate union fits kRecCap copies the neighbour records to
+// LDS once and reads them there (fewer L2 reads); the others read global memory.
+constexpr int kRecCap = 1088;
+
 template <int SOLVER>
 __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q, Buffers B, int do_shift) {
+  __shared__ NbStage st;
+  __shared__ float4 rec0[kRecCap], rec1[kRecCap], rec2[kRecCap];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
+  plan_stage(P, B.cell_start, B.l_cell, n, kRecCap, st);
+  for (int i = threadIdx.x; i < st.total; i += blockDim.x) {
+    const int o = stage_source(st, i);
+    rec0[i] = B.m_r0[o];
+    rec1[i] = B.m_r1[o];
+    rec2[i] = B.m_r2[o];
+  }
+  __syncthreads();
+  const bool staged = st.use;                      // uniform over the block
   if (s >= n) return;
   const double2 ps = B.l_pos[s];
   const float4 r0s = B.m_r0[s];
@@ -204,25 +223,76 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
   const int cxs = cell_s / ny, cys = cell_s - cxs * ny;
   const int row0 = cys > 0 ? cys - 1 : 0;
   const int row1 = cys + 1 < ny ? cys + 1 : ny - 1;
-  int rb1[3], rb2[3];                              // first local index of rows row0+1, row0+2
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const int col = cxs + c - 1;
-    const bool ok = col >= 0 && col < P.chain_nx;
-    rb1[c] = ok && row0 + 1 <= row1 ? B.cell_start[col * ny + row0 + 1] : 0x7fffffff;
-    rb2[c] = ok && row0 + 2 <= row1 ? B.cell_start[col * ny + row0 + 2] : 0x7fffffff;
-  }
+  // first local index of rows row0+1 and row0+2 in stencil columns 0, 1, 2 (none: INT_MAX)
+  const bool c0ok = cxs >= 1, c2ok = cxs + 1 < P.chain_nx;
+  const bool r1ok = row0 + 1 <= row1, r2ok = row0 + 2 <= row1;
+  const int* const cs = B.cell_start;
+  const int rb10 = c0ok && r1ok ? cs[(cxs - 1) * ny + row0 + 1] : 0x7fffffff;
+  const int rb11 = r1ok ? cs[cxs * ny + row0 + 1] : 0x7fffffff;
+  const int rb12 = c2ok && r1ok ? cs[(cxs + 1) * ny + row0 + 1] : 0x7fffffff;
+  const int rb20 = c0ok && r2ok ? cs[(cxs - 1) * ny + row0 + 2] : 0x7fffffff;
+  const int rb21 = r2ok ? cs[cxs * ny + row0 + 2] : 0x7fffffff;
+  const int rb22 = c2ok && r2ok ? cs[(cxs + 1) * ny + row0 + 2] : 0x7fffffff;
   constexpr bool kMuscl = SOLVER == SPH2D_SOLVER_MUSCL;
   float2 gps = make_float2(0.f, 0.f), gus = gps, gvs = gps;
   if (kMuscl) { gps = B.m_mg[3 * s]; gus = B.m_mg[3 * s + 1]; gvs = B.m_mg[3 * s + 2]; }
   KSum drho, corr, ax, ay, shx, shy, agx, agy;
-  for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d) {
-    const float4 r0o = B.m_r0[o];
-    if (ts != 2 && r0o.w < 0.f) return;          // no pair between two non-fluid particles
-    const float4 r1o = B.m_r1[o];
-    const float4 r2o = B.m_r2[o];
-    const int b1 = d == 0 ? rb1[0] : (d == 1 ? rb1[1] : rb1[2]);
-    const int b2 = d == 0 ? rb2[0] : (d == 1 ? rb2[1] : rb2[2]);
+  // The neighbours recorded by pass A (for_each_neighbour<kNbReplay>, written out:
+  // a closure over this kernel's locals would be kept in scratch memory).
+  const CellRanges R = cell_ranges(P, B.cell_start, cell_s);
+  const int n0 = R.hi[0] - R.lo[0], n01 = n0 + R.hi[1] - R.lo[1], total = n01 + R.hi[2] - R.lo[2];
+  const int l0 = R.lo[0], l1 = R.lo[1] - n0, l2 = R.lo[2] - n01;
+  const int so0 = staged ? l0 - st.lo[0] + st.base[0] : 0;
+  const int so1 = staged ? l1 - st.lo[1] + st.base[1] : 0;
+  const int so2 = staged ? l2 - st.lo[2] + st.base[2] : 0;
+  for (int c0 = 0; c0 < total; c0 += 256) {
+  uint64_t m0, m1, m2, m3;
+  if (total <= 256) {
+    const uint64_t* msk = B.nb_mask;
+    const int ms = B.kstride;
+    m0 = msk[s];
+    m1 = msk[ms + s];
+    m2 = msk[2 * ms + s];
+    m3 = msk[3 * ms + s];
+    if (total < 256) {                             // keep bits below total only
+      const int t = total & 63;
+      const uint64_t part = t ? ((1ull << t) - 1ull) : 0ull;
+      const int wl = total >> 6;
+      if (wl == 0) { m0 &= part; m1 = 0; m2 = 0; m3 = 0; }
+      else if (wl == 1) { m1 &= part; m2 = 0; m3 = 0; }
+      else if (wl == 2) { m2 &= part; m3 = 0; }
+      else { m3 &= part; }
+    }
+  } else {
+    const Mask256 mk = exact_chunk_mask(P, B.l_pos, B.l_pos32, s, ps, n0, n01, l0, l1, l2, total, c0);
+    m0 = mk.w0; m1 = mk.w1; m2 = mk.w2; m3 = mk.w3;
+  }
+  while ((m0 | m1 | m2 | m3) != 0) {
+    const int w = m0 ? 0 : (m1 ? 1 : (m2 ? 2 : 3));
+    const uint64_t mm = m0 ? m0 : (m1 ? m1 : (m2 ? m2 : m3));
+    const int b = __builtin_ctzll(mm);
+    const uint64_t bit = 1ull << b;
+    if (w == 0) m0 ^= bit; else if (w == 1) m1 ^= bit; else if (w == 2) m2 ^= bit; else m3 ^= bit;
+    const int kk = c0 + w * 64 + b;
+    const int d = kk < n0 ? 0 : (kk < n01 ? 1 : 2);
+    const int o = d == 0 ? l0 + kk : (d == 1 ? l1 + kk : l2 + kk);
+    const int li = d == 0 ? so0 + kk : (d == 1 ? so1 + kk : so2 + kk);
+    float4 r0o, r1o, r2o;
+    if (staged) {
+      r0o = rec0[li];
+      r1o = rec1[li];
+      r2o = rec2[li];
+    } else {
+      r0o = B.m_r0[o];
+      r1o = B.m_r1[o];
+      r2o = B.m_r2[o];
+    }
+    if (ts != 2 && r0o.w < 0.f) continue;        // no pair between two non-fluid particles
+    // bitwise blends, not selects: a select of captured values can become a
+    // select of their addresses, which keeps the closure in scratch memory
+    const int e0 = -(d == 0), e1 = -(d == 1), e2 = -(d == 2);
+    const int b1 = (rb10 & e0) | (rb11 & e1) | (rb12 & e2);
+    const int b2 = (rb20 & e0) | (rb21 & e1) | (rb22 & e2);
     const int dcy = cys - (row0 + (o >= b1) + (o >= b2));   // cy_s - cy_o
     const float dx = (r2s.z - r2o.z) + (float)(1 - d) * Q.cell_d;
     const float dy = (r2s.w - r2o.w) + (float)dcy * Q.cell_d;
@@ -347,7 +417,8 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
       shx.add(-f * K.dwx);
       shy.add(-f * K.dwy);
     }
-  }, B.nb_mask, B.kstride);
+  }
+  }
   double dr = (double)drho.s;
   if (ts == 2) dr = dr + (double)corr.s;
   B.drhodt[s] = dr;

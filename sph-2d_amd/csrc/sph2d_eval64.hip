@@ -46,8 +46,12 @@ __device__ __forceinline__ KernelTerm kernel_term(const DevParams& P, double dx,
 // ---------------------------------------------------------------------------
 template <bool MUSCL>
 __global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
+  __shared__ NbStage st;
+  __shared__ float2 lds32[kStageCap];
+  __shared__ double2 lds64[kStageCap];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
+  stage_candidates(P, B.cell_start, B.l_cell, B.l_pos32, B.l_pos, n, st, lds32, lds64);
   if (s >= n) return;
   const double2 ps = B.l_pos[s];
   const double2 vs = B.l_vel[s];
@@ -56,7 +60,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
   const int ts = B.l_type[s];
   double fra = 0.0, sdx = 0.0, sdy = 0.0, Sxx = 0.0, Sxy = 0.0, Syy = 0.0, grx = 0.0, gry = 0.0;
   double gpx = 0.0, gpy = 0.0, gux = 0.0, guy = 0.0, gvx = 0.0, gvy = 0.0;
-  for_each_neighbour<kNbRecord>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d) {
+  for_each_neighbour<kNbRecord>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) __attribute__((always_inline)) {
     if (ts != 2 && B.l_type[o] != 2) return;     // no pair between two non-fluid particles
     const double2 po = B.l_pos[o];
     const double dx = ps.x - po.x, dy = ps.y - po.y;
@@ -85,7 +89,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
       gvx = gvx + vd * K.dwx;
       gvy = gvy + vd * K.dwy;
     }
-  }, B.nb_mask, B.kstride);
+  }, B.nb_mask, B.kstride, &st, lds32, lds64);
   B.fra_v[s] = fra;
   B.sdw[s] = make_double2(sdx, sdy);
   B.grad[s] = make_double2(grx, gry);
@@ -168,7 +172,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_b2(DevParams P, Buffers B) {
   if (B.counters[C_NSURF] == 0) return;            // near_ was zeroed by pass A
   const int ts = B.l_type[s];
   int nr = 0;
-  for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, B.l_pos[s], B.l_cell[s], [&](int o, int d) {
+  for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, B.l_pos[s], B.l_cell[s], [&](int o, int d, int li) __attribute__((always_inline)) {
     if (B.surf[o] == 1 && (ts == 2 || B.l_type[o] == 2)) nr = 1;
   }, B.nb_mask, B.kstride);
   B.near_[s] = nr;
@@ -205,7 +209,7 @@ __device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int
   double drho = 0.0, corr = 0.0, ax = 0.0, ay = 0.0, shx = 0.0, shy = 0.0, agx = 0.0, agy = 0.0;
   double iax = 0.0, iay = 0.0, avx = 0.0, avy = 0.0, aix = 0.0, aiy = 0.0;
   double spx = 0.0, spy = 0.0, lpx = 0.0, lpy = 0.0;
-  for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d) {
+  for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) __attribute__((always_inline)) {
     if (ts != 2 && B.l_type[o] != 2) return;
     const double2 po = B.l_pos[o];
     const double dx = ps.x - po.x, dy = ps.y - po.y;      // x_ij with i = s

@@ -586,7 +586,7 @@ __global__ void __launch_bounds__(kBlock) k_mls(DevParams P, Buffers B) {
   double bv[6] = {0, 0, 0, 0, 0, 0};
   double sw = 0.0, srw = 0.0;
   int ne = 0;
-  for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d) {
+  for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) __attribute__((always_inline)) {
     const double2 po = B.l_pos[o];
     const double pdx = ps.x - po.x, pdy = ps.y - po.y;
     const double r = sqrt(pdx * pdx + pdy * pdy);
