@@ -281,7 +281,7 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   rc |= c->alloc(&B.cell_members, G); rc |= c->alloc(&B.l_of_g, G);
   rc |= c->alloc(&B.name, L); rc |= c->alloc(&B.l_type, L); rc |= c->alloc(&B.l_cell, L);
   rc |= c->alloc(&B.l_pos, L); rc |= c->alloc(&B.l_vel, L); rc |= c->alloc(&B.l_rv, L);
-  rc |= c->alloc(&B.l_pc, L); rc |= c->alloc(&B.l_pos_bin, L); rc |= c->alloc(&B.l_pos32, L);
+  rc |= c->alloc(&B.l_pc, L); rc |= c->alloc(&B.l_pos32, L);
   rc |= c->alloc(&B.l_mass, L);
   rc |= c->alloc(&B.fra_v, L); rc |= c->alloc(&B.sdw, L); rc |= c->alloc(&B.grad, L);
   rc |= c->alloc(&B.nrm, L); rc |= c->alloc(&B.shift, L); rc |= c->alloc(&B.avg, L);

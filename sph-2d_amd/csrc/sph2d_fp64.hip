@@ -62,21 +62,25 @@ __global__ void k_scan_reduce(const int* __restrict__ in, int n, int* __restrict
   if (threadIdx.x == 0) blocks[blockIdx.x] = total;
 }
 
+// One block: each thread scans a contiguous run of the block totals serially,
+// then one block-wide scan of the run totals gives the runs' offsets.
 __global__ void k_scan_top(int* blocks, int nb, int* total_out, int* total_out2) {
   __shared__ int lds[kBlock];
-  int carry = 0;
-  for (int base = 0; base < nb; base += kBlock) {
-    int i = base + threadIdx.x;
-    int v = i < nb ? blocks[i] : 0;
-    int total;
-    int ex = block_exclusive_scan(v, lds, total);
-    if (i < nb) blocks[i] = ex + carry;
-    carry += total;
-    __syncthreads();
+  const int per = (nb + kBlock - 1) / kBlock;
+  const int b0 = threadIdx.x * per;
+  const int b1 = min(b0 + per, nb);
+  int run = 0;
+  for (int i = b0; i < b1; ++i) run += blocks[i];
+  int total;
+  int ex = block_exclusive_scan(run, lds, total);
+  for (int i = b0; i < b1; ++i) {
+    const int v = blocks[i];
+    blocks[i] = ex;
+    ex += v;
   }
   if (threadIdx.x == 0) {
-    if (total_out) *total_out = carry;
-    if (total_out2) *total_out2 = carry;
+    if (total_out) *total_out = total;
+    if (total_out2) *total_out2 = total;
   }
 }
 
@@ -346,7 +350,6 @@ __global__ void k_cell_scatter(Buffers B, int cap_global) {
 __device__ __forceinline__ void gather_one(const DevParams& P, Buffers& B, int g, int l, int key) {
   const double2 pos = B.g_pos[g];
   B.l_pos[l] = pos;
-  B.l_pos_bin[l] = pos;
   B.l_pos32[l] = make_float2((float)pos.x, (float)pos.y);
   const double2 vel = B.g_vel[g];
   B.l_vel[l] = vel;
@@ -550,11 +553,15 @@ __global__ void k_stage(DevParams P, Buffers B, int kind) {
       B.l_rv[s] = make_double2(rho, B.l_rv[s].y);
     }
   }
-  // a particle that moved more than h/2 since binning leaves the 3x3 stencil's guarantee
+  // The 3x3 stencil of 4h cells holds every pair within 3h while each particle
+  // stays within h/2 of its binned cell (3h + h/2 + h/2 = 4h); flag a breach.
   if (t == 2) {
-    const double2 xb = B.l_pos_bin[s];
-    const double ddx = x.x - xb.x, ddy = x.y - xb.y;
-    if (ddx * ddx + ddy * ddy > 0.25 * P.h * P.h) atomicOr(&B.counters[C_REBIN], 1);
+    const int cell = B.l_cell[s];
+    const int cx = cell / P.chain_ny, cy = cell - cx * P.chain_ny;
+    const double x0 = P.chain_ox + cx * P.chain_d, y0 = P.chain_oy + cy * P.chain_d;
+    const double m = 0.5 * P.h;
+    if (x.x < x0 - m || x.x > x0 + P.chain_d + m || x.y < y0 - m || x.y > y0 + P.chain_d + m)
+      atomicOr(&B.counters[C_REBIN], 1);
   }
 }
 

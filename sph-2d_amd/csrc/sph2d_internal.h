@@ -63,7 +63,7 @@ struct Buffers {
   // local order
   int *name, *l_type, *l_cell;
   int *l_of_g;                            // local index of each gid (-1: not in the subdomain)
-  double2 *l_pos, *l_vel, *l_rv, *l_pc, *l_pos_bin;
+  double2 *l_pos, *l_vel, *l_rv, *l_pc;
   float2 *l_pos32;                        // FP32 copy of l_pos for the neighbour pre-test
   double *l_mass;
   // one force evaluation
