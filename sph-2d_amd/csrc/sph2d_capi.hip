@@ -673,9 +673,9 @@ static int enqueue_step(sph2d_ctx* c, int it) {
     launch_fill_i32(s, c->B.counters + C_MLSFB, 1, 0);
     launch_mls(s, c->P, c->B, c->cap_local);
   }
-  if (c->prm.shift_on) launch_shift_apply_mode(s, c->P, c->B, c->cap_local, mls ? 1 : 0);
-  else if (mls) launch_shift_apply_mode(s, c->P, c->B, c->cap_local, 2);
-  launch_refresh(s, c->P, c->B, c->cap_local);
+  // Shift_particle_position (+ the MLS density) and Refresh_domain in one pass
+  const int mode = mls ? (c->prm.shift_on ? 1 : 2) : 0;
+  launch_shift_refresh(s, c->P, c->B, c->cap_local, mode, c->prm.shift_on);
   HIP_OR_FAIL(hipGetLastError());
   return SPH2D_OK;
 }

@@ -735,6 +735,46 @@ void launch_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local
   hipLaunchKernelGGL(k_refresh, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, B);
 }
 
+// The end of a step in one pass: MLS density commit (mode >= 1), particle
+// shifting (mode <= 1; Shift_particle_position.f90:105-136) and the refresh,
+// computed for each fluid gid from its local slot. Same arithmetic as
+// k_shift_apply followed by k_refresh; the local arrays are not updated (the
+// next step rebuilds them from the global ones).
+__global__ void k_shift_refresh(DevParams P, Buffers B, int mode, int shift_on) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= B.counters[C_NFLUID]) return;
+  const int s = B.l_of_g[g];
+  if (s < 0) return;
+  double2 x = B.l_pos[s];
+  const double2 rv = B.l_rv[s];
+  double2 pc = B.l_pc[s];
+  double rho = rv.x;
+  if (mode >= 1) rho = B.mls_rho[s];
+  if (shift_on && mode <= 1 && B.l_type[s] == 2 && B.near_[s] != 1) {
+    double2 sh = B.shift[s];
+    const double amp = sqrt(sh.x * sh.x + sh.y * sh.y);
+    if (amp > P.max_shift) {
+      sh.x = sh.x / amp * P.max_shift;
+      sh.y = sh.y / amp * P.max_shift;
+    }
+    x.x = x.x + sh.x;
+    x.y = x.y + sh.y;
+    const double2 gr = B.grad[s];
+    rho = rho + gr.x * sh.x;
+    rho = rho + gr.y * sh.y;
+    pc.x = tait_p(P, rho);
+  }
+  B.g_pos[g] = x;
+  B.g_vel[g] = B.l_vel[s];
+  B.g_rho[g] = rho;
+  B.g_p[g] = pc.x;
+  B.g_c[g] = pc.y;
+}
+
+void launch_shift_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int mode, int shift_on) {
+  hipLaunchKernelGGL(k_shift_refresh, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B, mode, shift_on);
+}
+
 // ---------------------------------------------------------------------------
 // Sampling_subroutine.f90:340-461: block partials in fixed order, then one block.
 // partial layout per block: [ke, vmax, (max, sum, sum2) x 4]

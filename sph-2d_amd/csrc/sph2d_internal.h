@@ -116,6 +116,8 @@ void launch_mls(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
 void launch_shift_apply(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
 void launch_shift_apply_mode(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int mode);
 void launch_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+// MLS commit (mode >= 1), shifting (mode <= 1, when shift_on) and refresh in one pass
+void launch_shift_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int mode, int shift_on);
 void launch_sample(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, double* host_out14);
 void launch_pairs(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int64_t capacity,
                   int64_t* n_pairs);
