@@ -289,44 +289,45 @@ __global__ void k_analytic(DevParams P, Buffers B, int cap_fluid, double pi, dou
 
 // :75-83 (y<0 -> ill) and :123-162 binning; key = column-major cell index.
 __global__ void k_cell_key(DevParams P, Buffers B, int cap_global) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= cap_global) return;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const int N = B.counters[C_NFLUID];
-  const int ng = N + B.counters[C_NGHOST];
-  if (g >= ng) return;
-  int t = B.g_type[g];
-  const double2 p = B.g_pos[g];
-  if (g < N && t != 120 && p.y < 0.0) { t = 120; B.g_type[g] = 120; }
+  const int ng = min(N + B.counters[C_NGHOST], cap_global);
+  const bool valid = g < ng;                       // every lane stays for the wave operations below
   int key = -1;
-  if (t != 120 && t != 0) {
-    const int xn = (int)floor((p.x - P.chain_ox) / P.chain_d + 1);
-    const int yn = (int)floor((p.y - P.chain_oy) / P.chain_d + 1);
-    if (xn < 1 || xn > P.chain_nx || yn < 1 || yn > P.chain_ny) {
-      B.g_type[g] = 120;
-      atomicAdd(&B.counters[C_NILL], 1);
-    } else {
-      key = (xn - 1) * P.chain_ny + (yn - 1);
+  if (valid) {
+    int t = B.g_type[g];
+    const double2 p = B.g_pos[g];
+    if (g < N && t != 120 && p.y < 0.0) { t = 120; B.g_type[g] = 120; }
+    if (t != 120 && t != 0) {
+      const int xn = (int)floor((p.x - P.chain_ox) / P.chain_d + 1);
+      const int yn = (int)floor((p.y - P.chain_oy) / P.chain_d + 1);
+      if (xn < 1 || xn > P.chain_nx || yn < 1 || yn > P.chain_ny) {
+        B.g_type[g] = 120;
+        atomicAdd(&B.counters[C_NILL], 1);
+      } else {
+        key = (xn - 1) * P.chain_ny + (yn - 1);
+      }
     }
+    B.g_key[g] = key;
+    B.l_of_g[g] = -1;                              // set again by k_cell_rank for kept particles
   }
-  B.g_key[g] = key;
-  B.l_of_g[g] = -1;                                // set again by k_cell_rank for kept particles
-  // One atomic per distinct key of the wave (neighbouring gids share cells): the
-  // lowest lane holding a key adds the number of lanes holding it; the others take
-  // their slots after it. Slots only need to be distinct: the within-cell order
-  // comes from the rank by gid in k_cell_rank.
+  // Slots: runs of adjacent lanes with the same key (neighbouring gids share
+  // cells) reserve their slots with one atomic from the run's first lane, all
+  // runs of the wave at once. Slots only need to be distinct: the within-cell
+  // order comes from the rank by gid in k_cell_rank.
   const int lane = threadIdx.x & 63;
-  const uint64_t below = (1ull << lane) - 1ull;
-  uint64_t todo = __ballot(key >= 0);
-  while (todo) {
-    const int leader = __builtin_ctzll(todo);
-    const int lk = __builtin_amdgcn_readlane(key, leader);
-    const uint64_t same = __ballot(key == lk);
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&B.cell_count[lk], __popcll(same));
-    base = __builtin_amdgcn_readlane(base, leader);
-    if (key == lk) B.g_slot[g] = base + __popcll(same & below);
-    todo &= ~same;
-  }
+  const int prev = __shfl_up(key, 1);
+  const bool cont = key >= 0 && lane > 0 && prev == key;
+  const bool lead = key >= 0 && !cont;
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const uint64_t breaks = __ballot(!cont) & ~upto;  // run starts (or unkeyed lanes) above this lane
+  const uint64_t leads = __ballot(lead) & upto;     // run starts at or below this lane
+  const int next = breaks ? __builtin_ctzll(breaks) : 64;
+  const int my_lead = leads ? 63 - __builtin_clzll(leads) : lane;
+  int base = 0;
+  if (lead) base = atomicAdd(&B.cell_count[key], next - lane);
+  base = __shfl(base, my_lead);
+  if (key >= 0) B.g_slot[g] = base + (lane - my_lead);
 }
 
 __global__ void k_cell_scatter(Buffers B, int cap_global) {
@@ -358,8 +359,9 @@ __device__ __forceinline__ void gather_one(const DevParams& P, Buffers& B, int g
   const int t = B.g_type[g];
   B.l_mass[l] = m;
   B.l_type[l] = t;
-  double p = B.g_p[g], c = B.g_c[g];
+  double p, c;
   if (t != 120) tait(P, rho, p, c);
+  else { p = B.g_p[g]; c = B.g_c[g]; }
   B.l_pc[l] = make_double2(p, c);
   B.l_rv[l] = make_double2(rho, m / rho);
   if (P.mixed) {
