@@ -289,7 +289,7 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   rc |= c->alloc(&B.nb_mask, 4 * L);
   rc |= c->alloc(&B.drhodt, L); rc |= c->alloc(&B.acc, L);
   rc |= c->alloc(&B.b_rho, L); rc |= c->alloc(&B.t_drho, L); rc |= c->alloc(&B.k_drho, 3 * L);
-  rc |= c->alloc(&B.b_pos, L); rc |= c->alloc(&B.b_vel, L); rc |= c->alloc(&B.t_vel, L);
+  rc |= c->alloc(&B.b_pos, L); rc |= c->alloc(&B.b_vel, L);
   rc |= c->alloc(&B.t_acc, L); rc |= c->alloc(&B.k_vel, 3 * L); rc |= c->alloc(&B.k_acc, 3 * L);
   rc |= c->alloc(&B.mls_rho, L);
   rc |= c->alloc(&B.scan_blocks, (std::max<size_t>(G, (size_t)c->n_cells + 1) + 1023) / 1024 + 16);
@@ -576,16 +576,18 @@ int sph2d_distribute_period_boundary(sph2d_ctx* c) {
   return SPH2D_OK;
 }
 
-int sph2d_devide_block(sph2d_ctx* c, int32_t it) {
+static int devide_block(sph2d_ctx* c, int32_t it, bool predict) {
   int rc = need_state(c);
   if (rc) return rc;
   const bool ana = (it % c->prm.sample_every) == 0;
   const double t = it * c->prm.dt / (c->prm.box_x / c->prm.U);   // Nondimensionalized_real_time
   launch_fill_i32(c->stream, c->B.counters + C_NILL, 1, 0);
-  launch_devide_block(c->stream, c->P, c->B, c->cap_global, c->n_cells, ana, t);
+  launch_devide_block(c->stream, c->P, c->B, c->cap_global, c->n_cells, ana, t, predict);
   HIP_OR_FAIL(hipGetLastError());
   return SPH2D_OK;
 }
+
+int sph2d_devide_block(sph2d_ctx* c, int32_t it) { return devide_block(c, it, false); }
 
 static void do_eval(sph2d_ctx* c, bool with_eos) {
   if (with_eos) launch_stage(c->stream, c->P, c->B, c->cap_local, ST_EOS);
@@ -650,13 +652,24 @@ int sph2d_refresh_domain(sph2d_ctx* c, int32_t it) {
 static int enqueue_step(sph2d_ctx* c, int it) {
   int rc = sph2d_distribute_period_boundary(c);
   if (rc) return rc;
-  rc = sph2d_devide_block(c, it);
+  const bool pc = c->prm.integrator == SPH2D_INTEGRATOR_PC;
+  const bool mls = c->prm.mls_on && (it % c->prm.mls_every) == 0;
+  // After step 1 the predictor needs no evaluation first: it runs inside the local copy.
+  rc = devide_block(c, it, pc && it > 1);
   if (rc) return rc;
   hipStream_t s = c->stream;
-  if (c->prm.integrator == SPH2D_INTEGRATOR_PC) {
-    if (it == 1) do_eval(c, false);
-    launch_stage(s, c->P, c->B, c->cap_local, ST_PC_PREDICT);
+  if (pc) {
+    if (it == 1) {
+      do_eval(c, false);
+      launch_stage(s, c->P, c->B, c->cap_local, ST_PC_PREDICT);
+    }
     do_eval(c, false);
+    if (!mls) {
+      // corrector, Shift_particle_position and Refresh_domain in one pass
+      launch_correct_refresh(s, c->P, c->B, c->cap_local, c->prm.shift_on);
+      HIP_OR_FAIL(hipGetLastError());
+      return SPH2D_OK;
+    }
     launch_stage(s, c->P, c->B, c->cap_local, ST_PC_CORRECT);
   } else {
     if (it == 1) do_eval(c, false);
@@ -668,7 +681,6 @@ static int enqueue_step(sph2d_ctx* c, int it) {
     do_eval(c, false);
     launch_stage(s, c->P, c->B, c->cap_local, ST_RK_FINAL);
   }
-  const bool mls = c->prm.mls_on && (it % c->prm.mls_every) == 0;
   if (mls) {
     launch_fill_i32(s, c->B.counters + C_MLSFB, 1, 0);
     launch_mls(s, c->P, c->B, c->cap_local);

@@ -424,8 +424,10 @@ void launch_max_cell(hipStream_t s, Buffers& B, int n_cells) {
   hipLaunchKernelGGL(k_max_cell, dim3(1), dim3(kBlock), 0, s, B, n_cells);
 }
 
+__global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers B);
+
 void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_global, int n_cells,
-                         bool analytic, double t_nondim) {
+                         bool analytic, double t_nondim, bool predict) {
   const double pi = 3.14159265358979324;
   if (analytic) {
     const double Re = P.U * P.box_x / P.nu;        // information_module.f90:60
@@ -439,7 +441,8 @@ void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_
   hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, B, cap_global);
   hipLaunchKernelGGL(k_cell_rank, dim3((n_cells + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, s, B,
                      n_cells);
-  hipLaunchKernelGGL(k_local_gather, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, P, B);
+  if (predict) hipLaunchKernelGGL(k_gather_predict, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, P, B);
+  else hipLaunchKernelGGL(k_local_gather, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, P, B);
 }
 
 // ---------------------------------------------------------------------------
@@ -463,6 +466,38 @@ __device__ __forceinline__ void write_pos_mirrors(const DevParams& P, Buffers& B
     const float2 off = cell_offset(P, x, B.l_cell[s]);
     B.m_r2[s] = make_float4(0.f, 0.f, off.x, off.y);
   }
+}
+
+// time_iteration.f90:553-566: the predictor from the step-start state (bx, bv, brho)
+// and the rates read by the current local index.
+__device__ __forceinline__ void pc_predict(double dt, double drho, double2 a, double2 avg, double brho, double2 bv,
+                                           double2 bx, double& rho, double2& v, double2& x) {
+  rho = brho + drho * dt;
+  v.x = bv.x + dt * a.x + avg.x;
+  x.x = bx.x + dt * v.x;
+  v.y = bv.y + dt * a.y + avg.y;
+  x.y = bx.y + dt * v.y;
+}
+
+// :590-599: the corrector; (td, ta, tv) are the predictor's rates and velocity.
+__device__ __forceinline__ void pc_correct(double dt, double drho, double2 a, double2 avg, double td, double2 ta,
+                                           double2 tv, double brho, double2 bv, double2 bx, double& rho, double2& v,
+                                           double2& x) {
+  rho = brho + 0.5 * (drho + td) * dt;
+  v.x = bv.x + 0.5 * dt * (a.x + ta.x) + avg.x;
+  x.x = bx.x + 0.5 * dt * (v.x + tv.x) + 0.5 * dt * dt * a.x;
+  v.y = bv.y + 0.5 * dt * (a.y + ta.y) + avg.y;
+  x.y = bx.y + 0.5 * dt * (v.y + tv.y) + 0.5 * dt * dt * a.y;
+}
+
+// The 3x3 stencil of 4h cells holds every pair within 3h while each particle
+// stays within h/2 of its binned cell (3h + h/2 + h/2 = 4h); flag a breach.
+__device__ __forceinline__ void rebin_check(const DevParams& P, Buffers& B, int cell, double2 x) {
+  const int cx = cell / P.chain_ny, cy = cell - cx * P.chain_ny;
+  const double x0 = P.chain_ox + cx * P.chain_d, y0 = P.chain_oy + cy * P.chain_d;
+  const double m = 0.5 * P.h;
+  if (x.x < x0 - m || x.x > x0 + P.chain_d + m || x.y < y0 - m || x.y > y0 + P.chain_d + m)
+    atomicOr(&B.counters[C_REBIN], 1);
 }
 
 __global__ void k_stage(DevParams P, Buffers B, int kind) {
@@ -495,25 +530,15 @@ __global__ void k_stage(DevParams P, Buffers B, int kind) {
   const double2 bv = B.b_vel[s], bx = B.b_pos[s];
   if (kind == ST_PC_PREDICT) {                              // :553-566
     if (t == 2) {
-      rho = brho + drho * dt;
-      v.x = bv.x + dt * a.x + avg.x;
-      x.x = bx.x + dt * v.x;
-      v.y = bv.y + dt * a.y + avg.y;
-      x.y = bx.y + dt * v.y;
+      pc_predict(dt, drho, a, avg, brho, bv, bx, rho, v, x);
       B.l_vel[s] = v; B.l_pos[s] = x;
       write_pos_mirrors(P, B, s, x);
       eos_volume(P, B, s, rho);
     }
-    B.t_vel[s] = v; B.t_drho[s] = drho; B.t_acc[s] = a;    // :573-576
+    B.t_drho[s] = drho; B.t_acc[s] = a;                     // :573-576 (temp velocity = l_vel)
   } else if (kind == ST_PC_CORRECT) {                       // :590-599
     if (t == 2) {
-      const double td = B.t_drho[s];
-      const double2 ta = B.t_acc[s], tv = B.t_vel[s];
-      rho = brho + 0.5 * (drho + td) * dt;
-      v.x = bv.x + 0.5 * dt * (a.x + ta.x) + avg.x;
-      x.x = bx.x + 0.5 * dt * (v.x + tv.x) + 0.5 * dt * dt * a.x;
-      v.y = bv.y + 0.5 * dt * (a.y + ta.y) + avg.y;
-      x.y = bx.y + 0.5 * dt * (v.y + tv.y) + 0.5 * dt * dt * a.y;
+      pc_correct(dt, drho, a, avg, B.t_drho[s], B.t_acc[s], v, brho, bv, bx, rho, v, x);
       B.l_vel[s] = v; B.l_pos[s] = x;
       B.l_rv[s] = make_double2(rho, B.l_rv[s].y);           // p, c, volume keep the last evaluation's values
     }
@@ -555,20 +580,53 @@ __global__ void k_stage(DevParams P, Buffers B, int kind) {
       B.l_rv[s] = make_double2(rho, B.l_rv[s].y);
     }
   }
-  // The 3x3 stencil of 4h cells holds every pair within 3h while each particle
-  // stays within h/2 of its binned cell (3h + h/2 + h/2 = 4h); flag a breach.
-  if (t == 2) {
-    const int cell = B.l_cell[s];
-    const int cx = cell / P.chain_ny, cy = cell - cx * P.chain_ny;
-    const double x0 = P.chain_ox + cx * P.chain_d, y0 = P.chain_oy + cy * P.chain_d;
-    const double m = 0.5 * P.h;
-    if (x.x < x0 - m || x.x > x0 + P.chain_d + m || x.y < y0 - m || x.y > y0 + P.chain_d + m)
-      atomicOr(&B.counters[C_REBIN], 1);
-  }
+  if (t == 2) rebin_check(P, B, B.l_cell[s], x);
 }
 
 void launch_stage(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int kind) {
   hipLaunchKernelGGL(k_stage, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B, kind);
+}
+
+// The global -> subdomain copy and the predictor in one pass (steps after the
+// first, whose predictor uses the previous step's rates): the gathered state is
+// the step-start state (before_*), the local arrays receive the predicted state.
+// Same arithmetic as k_local_gather followed by k_stage(ST_PC_PREDICT).
+__global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers B) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= B.counters[C_NLOCAL]) return;
+  const int g = B.name[l], key = B.l_cell[l];
+  const double2 pos = B.g_pos[g];
+  const double2 vel = B.g_vel[g];
+  const double rho0 = B.g_rho[g];
+  const double m = B.g_mass[g];
+  const int t = B.g_type[g];
+  B.b_rho[l] = rho0; B.b_vel[l] = vel; B.b_pos[l] = pos;
+  const double drho = B.drhodt[l];
+  const double2 a = B.acc[l];
+  B.t_drho[l] = drho; B.t_acc[l] = a;
+  double rho = rho0;
+  double2 v = vel, x = pos;
+  if (t == 2) {
+    const double2 avg = (P.epsilon_xsph != 0.0) ? B.avg[l] : make_double2(0.0, 0.0);
+    pc_predict(P.dt, drho, a, avg, rho0, vel, pos, rho, v, x);
+    rebin_check(P, B, key, x);
+  }
+  B.l_pos[l] = x;
+  B.l_pos32[l] = make_float2((float)x.x, (float)x.y);
+  B.l_vel[l] = v;
+  B.l_mass[l] = m;
+  B.l_type[l] = t;
+  double p, c;
+  if (t != 120) tait(P, rho, p, c);
+  else { p = B.g_p[g]; c = B.g_c[g]; }
+  B.l_pc[l] = make_double2(p, c);
+  B.l_rv[l] = make_double2(rho, m / rho);
+  if (P.mixed) {
+    write_r0(P, B, l, v, rho, m, t);
+    B.m_r1[l] = make_float4(0.f, 0.f, 0.f, (float)B.g_mass[l]);   // particle_mass(j) with j local
+    const float2 off = cell_offset(P, x, key);
+    B.m_r2[l] = make_float4(0.f, 0.f, off.x, off.y);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -742,6 +800,25 @@ void launch_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local
 // computed for each fluid gid from its local slot. Same arithmetic as
 // k_shift_apply followed by k_refresh; the local arrays are not updated (the
 // next step rebuilds them from the global ones).
+// Shift_particle_position.f90:108-136 for local slot s of a fluid particle that is
+// not near the surface: the capped shift moves x, corrects rho by grad(rho).shift
+// and p by the EOS.
+__device__ __forceinline__ void shift_one(const DevParams& P, const Buffers& B, int s, double2& x, double& rho,
+                                          double& p) {
+  double2 sh = B.shift[s];
+  const double amp = sqrt(sh.x * sh.x + sh.y * sh.y);
+  if (amp > P.max_shift) {
+    sh.x = sh.x / amp * P.max_shift;
+    sh.y = sh.y / amp * P.max_shift;
+  }
+  x.x = x.x + sh.x;
+  x.y = x.y + sh.y;
+  const double2 gr = B.grad[s];
+  rho = rho + gr.x * sh.x;
+  rho = rho + gr.y * sh.y;
+  p = tait_p(P, rho);
+}
+
 __global__ void k_shift_refresh(DevParams P, Buffers B, int mode, int shift_on) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= B.counters[C_NFLUID]) return;
@@ -752,20 +829,7 @@ __global__ void k_shift_refresh(DevParams P, Buffers B, int mode, int shift_on) 
   double2 pc = B.l_pc[s];
   double rho = rv.x;
   if (mode >= 1) rho = B.mls_rho[s];
-  if (shift_on && mode <= 1 && B.l_type[s] == 2 && B.near_[s] != 1) {
-    double2 sh = B.shift[s];
-    const double amp = sqrt(sh.x * sh.x + sh.y * sh.y);
-    if (amp > P.max_shift) {
-      sh.x = sh.x / amp * P.max_shift;
-      sh.y = sh.y / amp * P.max_shift;
-    }
-    x.x = x.x + sh.x;
-    x.y = x.y + sh.y;
-    const double2 gr = B.grad[s];
-    rho = rho + gr.x * sh.x;
-    rho = rho + gr.y * sh.y;
-    pc.x = tait_p(P, rho);
-  }
+  if (shift_on && mode <= 1 && B.l_type[s] == 2 && B.near_[s] != 1) shift_one(P, B, s, x, rho, pc.x);
   B.g_pos[g] = x;
   B.g_vel[g] = B.l_vel[s];
   B.g_rho[g] = rho;
@@ -775,6 +839,41 @@ __global__ void k_shift_refresh(DevParams P, Buffers B, int mode, int shift_on) 
 
 void launch_shift_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int mode, int shift_on) {
   hipLaunchKernelGGL(k_shift_refresh, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B, mode, shift_on);
+}
+
+// The end of a predictor-corrector step without MLS in one pass: the corrector
+// (time_iteration.f90:590-599), shifting and the refresh, for each fluid gid from
+// its local slot. Same arithmetic as k_stage(ST_PC_CORRECT) followed by
+// k_shift_refresh(mode 0); the local arrays keep the predicted state. The
+// predictor's velocity is still in l_vel (the evaluation does not write it).
+__global__ void k_correct_refresh(DevParams P, Buffers B, int shift_on) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= B.counters[C_NFLUID]) return;
+  const int s = B.l_of_g[g];
+  if (s < 0) return;
+  const int t = B.l_type[s];
+  double2 x, v = B.l_vel[s];
+  double rho;
+  double2 pc = B.l_pc[s];                    // p, c of the last evaluation
+  if (t == 2) {
+    const double2 avg = (P.epsilon_xsph != 0.0) ? B.avg[s] : make_double2(0.0, 0.0);
+    pc_correct(P.dt, B.drhodt[s], B.acc[s], avg, B.t_drho[s], B.t_acc[s], v, B.b_rho[s], B.b_vel[s],
+               B.b_pos[s], rho, v, x);
+    rebin_check(P, B, B.l_cell[s], x);
+    if (shift_on && B.near_[s] != 1) shift_one(P, B, s, x, rho, pc.x);
+  } else {
+    x = B.l_pos[s];
+    rho = B.l_rv[s].x;
+  }
+  B.g_pos[g] = x;
+  B.g_vel[g] = v;
+  B.g_rho[g] = rho;
+  B.g_p[g] = pc.x;
+  B.g_c[g] = pc.y;
+}
+
+void launch_correct_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int shift_on) {
+  hipLaunchKernelGGL(k_correct_refresh, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B, shift_on);
 }
 
 // ---------------------------------------------------------------------------

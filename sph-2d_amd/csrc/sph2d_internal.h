@@ -79,7 +79,7 @@ struct Buffers {
   double *dbg;                            // 10 per particle (debug components)
   // integrator (local index, persist across steps)
   double *b_rho, *t_drho, *k_drho;        // k_drho: 3 stages
-  double2 *b_pos, *b_vel, *t_vel, *t_acc, *k_vel, *k_acc;
+  double2 *b_pos, *b_vel, *t_acc, *k_vel, *k_acc;
   double *mls_rho;
   // scratch
   int *scan_blocks;
@@ -102,8 +102,9 @@ struct Buffers {
 // Launchers (sph2d_fp64.hip). All run on `s` and take sizes from device counters.
 void launch_init_tgv(hipStream_t s, const DevParams& P, Buffers& B, int nx, int ny);
 void launch_period_boundary(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int cap_ghost);
+// predict: the local copy also applies the predictor-corrector's predictor (k_gather_predict)
 void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_global, int n_cells,
-                         bool analytic, double t_nondim);
+                         bool analytic, double t_nondim, bool predict = false);
 // ev (optional, 4 events): recorded before and after pass A and before and after the force pass.
 void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, hipEvent_t* ev = nullptr);
 // FP32-mixed passes A and C (sph2d_eval32.hip)
@@ -118,6 +119,8 @@ void launch_shift_apply_mode(hipStream_t s, const DevParams& P, Buffers& B, int 
 void launch_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
 // MLS commit (mode >= 1), shifting (mode <= 1, when shift_on) and refresh in one pass
 void launch_shift_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int mode, int shift_on);
+// predictor-corrector corrector, shifting (when shift_on) and refresh in one pass (no MLS this step)
+void launch_correct_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int shift_on);
 void launch_sample(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, double* host_out14);
 void launch_pairs(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int64_t capacity,
                   int64_t* n_pairs);
