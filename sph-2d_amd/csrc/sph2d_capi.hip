@@ -54,8 +54,10 @@ struct sph2d_ctx {
     void* q = nullptr;
     hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
     if (e != hipSuccess) return fail(SPH2D_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-    e = hipMemset(q, 0, std::max<size_t>(n, 1) * sizeof(T));
-    if (e != hipSuccess) return fail(SPH2D_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    // Zeroed on the context's stream: a null-stream hipMemset is not ordered with the
+    // non-blocking stream's kernels and could land after them.
+    e = hipMemsetAsync(q, 0, std::max<size_t>(n, 1) * sizeof(T), stream);
+    if (e != hipSuccess) return fail(SPH2D_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
     allocs.push_back(q);
     *p = static_cast<T*>(q);
     return SPH2D_OK;
@@ -326,10 +328,12 @@ int sph2d_ctx_destroy(sph2d_ctx* c) {
   return SPH2D_OK;
 }
 
+// The counters are set by kernels, not by a host copy: nothing host-side must be
+// kept alive, and every later kernel on the stream sees them in order.
 static int set_nfluid(sph2d_ctx* c, int n) {
-  int zeros[C_COUNT] = {0};
-  zeros[C_NFLUID] = n;
-  HIP_OR_FAIL(hipMemcpyAsync(c->B.counters, zeros, sizeof(zeros), hipMemcpyHostToDevice, c->stream));
+  launch_fill_i32(c->stream, c->B.counters, C_COUNT, 0);
+  launch_fill_i32(c->stream, c->B.counters + C_NFLUID, 1, n);
+  HIP_OR_FAIL(hipGetLastError());
   HIP_OR_FAIL(hipStreamSynchronize(c->stream));
   c->n_fluid = n;
   return SPH2D_OK;
