@@ -276,8 +276,8 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   rc |= c->alloc(&B.g_c, G); rc |= c->alloc(&B.g_e, G);
   rc |= c->alloc(&B.g_type, G); rc |= c->alloc(&B.g_key, G); rc |= c->alloc(&B.g_slot, G);
   rc |= c->alloc(&B.ana, 4 * F);
-  rc |= c->alloc(&B.cnt_x, F); rc |= c->alloc(&B.cnt_yf, F); rc |= c->alloc(&B.off_x, F);
-  rc |= c->alloc(&B.off_yf, F); rc |= c->alloc(&B.cnt_yx, H); rc |= c->alloc(&B.off_yx, H);
+  rc |= c->alloc(&B.gblk, 2 * ((F + 255) / 256) + 2);
+  rc |= c->alloc(&B.cnt_yx, H); rc |= c->alloc(&B.off_yx, H);
   rc |= c->alloc(&B.src_x, H);
   rc |= c->alloc(&B.cell_count, c->n_cells); rc |= c->alloc(&B.cell_start, c->n_cells + 1);
   rc |= c->alloc(&B.cell_members, G); rc |= c->alloc(&B.l_of_g, G);
@@ -580,7 +580,7 @@ int sph2d_distribute_period_boundary(sph2d_ctx* c) {
   return SPH2D_OK;
 }
 
-static int devide_block(sph2d_ctx* c, int32_t it, bool predict) {
+static int devide_block(sph2d_ctx* c, int32_t it, int predict) {
   int rc = need_state(c);
   if (rc) return rc;
   const bool ana = (it % c->prm.sample_every) == 0;
@@ -591,7 +591,7 @@ static int devide_block(sph2d_ctx* c, int32_t it, bool predict) {
   return SPH2D_OK;
 }
 
-int sph2d_devide_block(sph2d_ctx* c, int32_t it) { return devide_block(c, it, false); }
+int sph2d_devide_block(sph2d_ctx* c, int32_t it) { return devide_block(c, it, 0); }
 
 static void do_eval(sph2d_ctx* c, bool with_eos) {
   if (with_eos) launch_stage(c->stream, c->P, c->B, c->cap_local, ST_EOS);
@@ -658,8 +658,9 @@ static int enqueue_step(sph2d_ctx* c, int it) {
   if (rc) return rc;
   const bool pc = c->prm.integrator == SPH2D_INTEGRATOR_PC;
   const bool mls = c->prm.mls_on && (it % c->prm.mls_every) == 0;
-  // After step 1 the predictor needs no evaluation first: it runs inside the local copy.
-  rc = devide_block(c, it, pc && it > 1);
+  // After step 1 the predictor needs no evaluation first: it runs inside the local copy
+  // (which keeps before_* only for an MLS step, whose corrector is k_stage's).
+  rc = devide_block(c, it, pc && it > 1 ? (mls ? 2 : 1) : 0);
   if (rc) return rc;
   hipStream_t s = c->stream;
   if (pc) {

@@ -151,24 +151,46 @@ void launch_init_tgv(hipStream_t s, const DevParams& P, Buffers& B, int nx, int 
 // ---------------------------------------------------------------------------
 // Distribute_Period_Boundary_Particle.f90:56-206
 // ---------------------------------------------------------------------------
-__global__ void k_wrap_flags(DevParams P, Buffers B, int cap_fluid) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= cap_fluid) return;
-  const int N = B.counters[C_NFLUID];
-  if (i >= N || B.g_type[i] == 120) { B.cnt_x[i] = 0; B.cnt_yf[i] = 0; return; }
-  double2 p = B.g_pos[i];
-  if (p.x > P.box_x) p.x = p.x - P.box_x;
-  else if (p.x < 0.0) p.x = p.x + P.box_x;
-  if (p.y > P.box_y) p.y = p.y - P.box_y;
-  else if (p.y < 0.0) p.y = p.y + P.box_y;
-  B.g_pos[i] = p;
+// The flags of one wrapped fluid position: x-ghost to the right of the box (l),
+// to the left (r), y-ghosts above (yl) and below (yr) (:98-148, :153-206).
+__device__ __forceinline__ void ghost_flags(const DevParams& P, double2 p, int& l, int& r, int& yl, int& yr) {
   const double r3h = P.three_h;
-  int l = (p.x <= (0.0 + r3h) && p.x >= 0.0);
-  int r = (p.x >= (P.box_x - r3h) && p.x <= P.box_x);
-  int yl = (p.y <= (0.0 + r3h) && p.y >= 0.0);
-  int yr = (p.y >= (P.box_y - r3h) && p.y <= P.box_y);
-  B.cnt_x[i] = l + r;
-  B.cnt_yf[i] = yl + yr;
+  l = (p.x <= (0.0 + r3h) && p.x >= 0.0);
+  r = (p.x >= (P.box_x - r3h) && p.x <= P.box_x);
+  yl = (p.y <= (0.0 + r3h) && p.y >= 0.0);
+  yr = (p.y >= (P.box_y - r3h) && p.y <= P.box_y);
+}
+
+// :56-90 wrap (written back only where it moves a particle), then per block of
+// kBlock gids the number of x-ghosts (gblk[b]) and of y-ghosts of fluid
+// particles (gblk[nb + b]).
+__global__ void __launch_bounds__(kBlock) k_wrap_count(DevParams P, Buffers B, int cap_fluid) {
+  __shared__ int sx[kBlock / 64], sy[kBlock / 64];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = B.counters[C_NFLUID];
+  int l = 0, r = 0, yl = 0, yr = 0;
+  if (i < N && i < cap_fluid && B.g_type[i] != 120) {
+    const double2 p0 = B.g_pos[i];
+    double2 p = p0;
+    if (p.x > P.box_x) p.x = p.x - P.box_x;
+    else if (p.x < 0.0) p.x = p.x + P.box_x;
+    if (p.y > P.box_y) p.y = p.y - P.box_y;
+    else if (p.y < 0.0) p.y = p.y + P.box_y;
+    if (p.x != p0.x || p.y != p0.y) B.g_pos[i] = p;
+    ghost_flags(P, p, l, r, yl, yr);
+  }
+  const int w = threadIdx.x >> 6;
+  const int cx = __popcll(__ballot(l)) + __popcll(__ballot(r));
+  const int cy = __popcll(__ballot(yl)) + __popcll(__ballot(yr));
+  if ((threadIdx.x & 63) == 0) { sx[w] = cx; sy[w] = cy; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tx = 0, ty = 0;
+#pragma unroll
+    for (int k = 0; k < kBlock / 64; ++k) { tx += sx[k]; ty += sy[k]; }
+    B.gblk[blockIdx.x] = tx;
+    B.gblk[gridDim.x + blockIdx.x] = ty;
+  }
 }
 
 __device__ __forceinline__ void copy_particle(Buffers& B, int dst, int src, double2 pos) {
@@ -182,52 +204,62 @@ __device__ __forceinline__ void copy_particle(Buffers& B, int dst, int src, doub
   B.g_e[dst] = B.g_e[src];
 }
 
-// x-direction ghosts (:98-148); records each x-ghost's own y-ghost count.
-__global__ void k_ghost_x(DevParams P, Buffers B, int cap_fluid, int cap_ghost) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= cap_fluid) return;
+// The x-ghosts (:98-148, left copy before right copy for each gid) and the
+// y-ghosts of fluid particles (:153-206, first part of the loop), each in gid
+// order: a block's slots start at its scanned count (gblk) and each lane's
+// after the ghosts of the lanes before it. Blocks without ghosts leave at once.
+// Records each x-ghost's own y-ghost count for the corner pass.
+__global__ void __launch_bounds__(kBlock) k_ghost_emit(DevParams P, Buffers B, int cap_fluid, int cap_ghost) {
+  __shared__ int sx[kBlock / 64], sy[kBlock / 64];
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int nxg = B.counters[C_NXG], nyf = B.counters[C_NYF];
+  const int ox = B.gblk[b], oy = B.gblk[nb + b];
+  const int ex = (b + 1 < nb ? B.gblk[b + 1] : nxg) - ox;
+  const int ey = (b + 1 < nb ? B.gblk[nb + b + 1] : nyf) - oy;
+  if (ex == 0 && ey == 0) return;                  // uniform over the block
+  const int i = b * kBlock + threadIdx.x;
   const int N = B.counters[C_NFLUID];
-  if (i >= N || B.cnt_x[i] == 0) return;
-  const double2 p = B.g_pos[i];
-  const double r3h = P.three_h;
-  int k = B.off_x[i];
-  if (p.x <= (0.0 + r3h) && p.x >= 0.0) {
+  int l = 0, r = 0, yl = 0, yr = 0;
+  double2 p = make_double2(0.0, 0.0);
+  if (i < N && i < cap_fluid && B.g_type[i] != 120) {
+    p = B.g_pos[i];
+    ghost_flags(P, p, l, r, yl, yr);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const uint64_t bl = __ballot(l), br = __ballot(r), byl = __ballot(yl), byr = __ballot(yr);
+  int px = __popcll(bl & lt) + __popcll(br & lt);
+  int py = __popcll(byl & lt) + __popcll(byr & lt);
+  if (lane == 0) { sx[w] = __popcll(bl) + __popcll(br); sy[w] = __popcll(byl) + __popcll(byr); }
+  __syncthreads();
+  for (int k = 0; k < w; ++k) { px += sx[k]; py += sy[k]; }
+  int k = ox + px;
+  if (l) {
     if (k < cap_ghost) {
       copy_particle(B, N + k, i, make_double2(p.x + P.box_x, p.y));
-      B.cnt_yx[k] = B.cnt_yf[i];
+      B.cnt_yx[k] = yl + yr;
       B.src_x[k] = i;
     } else {
       atomicOr(&B.counters[C_ERR], E_GHOST_CAP);
     }
     ++k;
   }
-  if (p.x >= (P.box_x - r3h) && p.x <= P.box_x) {
+  if (r) {
     if (k < cap_ghost) {
       copy_particle(B, N + k, i, make_double2(p.x - P.box_x, p.y));
-      B.cnt_yx[k] = B.cnt_yf[i];
+      B.cnt_yx[k] = yl + yr;
       B.src_x[k] = i;
     } else {
       atomicOr(&B.counters[C_ERR], E_GHOST_CAP);
     }
   }
-}
-
-// y-direction ghosts of fluid particles (:153-206, first part of the loop)
-__global__ void k_ghost_yf(DevParams P, Buffers B, int cap_fluid, int cap_ghost) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= cap_fluid) return;
-  const int N = B.counters[C_NFLUID];
-  if (i >= N || B.cnt_yf[i] == 0) return;
-  const int nxg = B.counters[C_NXG];
-  const double2 p = B.g_pos[i];
-  const double r3h = P.three_h;
-  int k = nxg + B.off_yf[i];
-  if (p.y <= (0.0 + r3h) && p.y >= 0.0) {
+  k = nxg + oy + py;
+  if (yl) {
     if (k < cap_ghost) copy_particle(B, N + k, i, make_double2(p.x, p.y + P.box_y));
     else atomicOr(&B.counters[C_ERR], E_GHOST_CAP);
     ++k;
   }
-  if (p.y >= (P.box_y - r3h) && p.y <= P.box_y) {
+  if (yr) {
     if (k < cap_ghost) copy_particle(B, N + k, i, make_double2(p.x, p.y - P.box_y));
     else atomicOr(&B.counters[C_ERR], E_GHOST_CAP);
   }
@@ -260,13 +292,13 @@ __global__ void k_ghost_yx(DevParams P, Buffers B, int cap_ghost) {
 }
 
 void launch_period_boundary(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int cap_ghost) {
-  hipLaunchKernelGGL(k_wrap_flags, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, s, P, B, cap_fluid);
-  exclusive_scan(s, B.cnt_x, cap_fluid, B.off_x, B.scan_blocks, B.counters + C_NXG);
-  exclusive_scan(s, B.cnt_yf, cap_fluid, B.off_yf, B.scan_blocks, B.counters + C_NYF);
+  const int nb = grid_for(cap_fluid);              // gblk holds 2 * nb ints
+  hipLaunchKernelGGL(k_wrap_count, dim3(nb), dim3(kBlock), 0, s, P, B, cap_fluid);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, s, B.gblk, nb, B.counters + C_NXG, nullptr);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, s, B.gblk + nb, nb, B.counters + C_NYF, nullptr);
   launch_fill_i32(s, B.cnt_yx, cap_ghost, 0);
-  hipLaunchKernelGGL(k_ghost_x, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, s, P, B, cap_fluid, cap_ghost);
+  hipLaunchKernelGGL(k_ghost_emit, dim3(nb), dim3(kBlock), 0, s, P, B, cap_fluid, cap_ghost);
   exclusive_scan(s, B.cnt_yx, cap_ghost, B.off_yx, B.scan_blocks, B.counters + C_NYX);
-  hipLaunchKernelGGL(k_ghost_yf, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, s, P, B, cap_fluid, cap_ghost);
   hipLaunchKernelGGL(k_ghost_yx, dim3(grid_for(cap_ghost)), dim3(kBlock), 0, s, P, B, cap_ghost);
 }
 
@@ -424,10 +456,10 @@ void launch_max_cell(hipStream_t s, Buffers& B, int n_cells) {
   hipLaunchKernelGGL(k_max_cell, dim3(1), dim3(kBlock), 0, s, B, n_cells);
 }
 
-__global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers B);
+__global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers B, int save_before);
 
 void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_global, int n_cells,
-                         bool analytic, double t_nondim, bool predict) {
+                         bool analytic, double t_nondim, int predict) {
   const double pi = 3.14159265358979324;
   if (analytic) {
     const double Re = P.U * P.box_x / P.nu;        // information_module.f90:60
@@ -441,7 +473,8 @@ void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_
   hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, B, cap_global);
   hipLaunchKernelGGL(k_cell_rank, dim3((n_cells + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, s, B,
                      n_cells);
-  if (predict) hipLaunchKernelGGL(k_gather_predict, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, P, B);
+  if (predict) hipLaunchKernelGGL(k_gather_predict, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, P, B,
+                                  (int)(predict == 2));
   else hipLaunchKernelGGL(k_local_gather, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, P, B);
 }
 
@@ -589,9 +622,11 @@ void launch_stage(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, 
 
 // The global -> subdomain copy and the predictor in one pass (steps after the
 // first, whose predictor uses the previous step's rates): the gathered state is
-// the step-start state (before_*), the local arrays receive the predicted state.
+// the step-start state (kept as before_* when save_before: k_stage's corrector
+// reads it; k_correct_refresh reads the global arrays instead), the local arrays
+// receive the predicted state.
 // Same arithmetic as k_local_gather followed by k_stage(ST_PC_PREDICT).
-__global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers B) {
+__global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers B, int save_before) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= B.counters[C_NLOCAL]) return;
   const int g = B.name[l], key = B.l_cell[l];
@@ -600,7 +635,7 @@ __global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers 
   const double rho0 = B.g_rho[g];
   const double m = B.g_mass[g];
   const int t = B.g_type[g];
-  B.b_rho[l] = rho0; B.b_vel[l] = vel; B.b_pos[l] = pos;
+  if (save_before) { B.b_rho[l] = rho0; B.b_vel[l] = vel; B.b_pos[l] = pos; }
   const double drho = B.drhodt[l];
   const double2 a = B.acc[l];
   B.t_drho[l] = drho; B.t_acc[l] = a;
@@ -845,7 +880,8 @@ void launch_shift_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap
 // (time_iteration.f90:590-599), shifting and the refresh, for each fluid gid from
 // its local slot. Same arithmetic as k_stage(ST_PC_CORRECT) followed by
 // k_shift_refresh(mode 0); the local arrays keep the predicted state. The
-// predictor's velocity is still in l_vel (the evaluation does not write it).
+// predictor's velocity is still in l_vel (the evaluation does not write it), and
+// the global arrays of fluid gids still hold the step-start state.
 __global__ void k_correct_refresh(DevParams P, Buffers B, int shift_on) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= B.counters[C_NFLUID]) return;
@@ -857,8 +893,9 @@ __global__ void k_correct_refresh(DevParams P, Buffers B, int shift_on) {
   double2 pc = B.l_pc[s];                    // p, c of the last evaluation
   if (t == 2) {
     const double2 avg = (P.epsilon_xsph != 0.0) ? B.avg[s] : make_double2(0.0, 0.0);
-    pc_correct(P.dt, B.drhodt[s], B.acc[s], avg, B.t_drho[s], B.t_acc[s], v, B.b_rho[s], B.b_vel[s],
-               B.b_pos[s], rho, v, x);
+    // the step-start state is still this gid's global state (before_* of the gather)
+    pc_correct(P.dt, B.drhodt[s], B.acc[s], avg, B.t_drho[s], B.t_acc[s], v, B.g_rho[g], B.g_vel[g],
+               B.g_pos[g], rho, v, x);
     rebin_check(P, B, B.l_cell[s], x);
     if (shift_on && B.near_[s] != 1) shift_one(P, B, s, x, rho, pc.x);
   } else {

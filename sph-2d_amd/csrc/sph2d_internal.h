@@ -57,7 +57,7 @@ struct Buffers {
   double *g_rho, *g_mass, *g_p, *g_c, *g_e;
   int *g_type, *g_key, *g_slot;
   double *ana;                            // 4 per fluid particle: u, v, p, |v|
-  int *cnt_x, *cnt_yf, *off_x, *off_yf, *cnt_yx, *off_yx, *src_x;
+  int *gblk, *cnt_yx, *off_yx, *src_x;   // gblk: per block of gids, x-ghost then y-ghost counts
   // cells (column-major key over the global grid)
   int *cell_count, *cell_start, *cell_members;
   // local order
@@ -102,9 +102,10 @@ struct Buffers {
 // Launchers (sph2d_fp64.hip). All run on `s` and take sizes from device counters.
 void launch_init_tgv(hipStream_t s, const DevParams& P, Buffers& B, int nx, int ny);
 void launch_period_boundary(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int cap_ghost);
-// predict: the local copy also applies the predictor-corrector's predictor (k_gather_predict)
+// predict 1: the local copy also applies the predictor-corrector's predictor (k_gather_predict);
+// 2: and keeps the step-start state in before_* for k_stage's corrector
 void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_global, int n_cells,
-                         bool analytic, double t_nondim, bool predict = false);
+                         bool analytic, double t_nondim, int predict = 0);
 // ev (optional, 4 events): recorded before and after pass A and before and after the force pass.
 void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, hipEvent_t* ev = nullptr);
 // FP32-mixed passes A and C (sph2d_eval32.hip)
