@@ -210,6 +210,19 @@ __device__ __forceinline__ Mask256 exact_chunk_mask(const DevParams& P, const do
   return Mask256{m0, m1, m2, m3};
 }
 
+// (w3:w2:w1:w0) |= (hi:lo) << sh for sh in [0, 128]; bits beyond 256 are dropped.
+__device__ __forceinline__ void or_shl128(uint64_t& w0, uint64_t& w1, uint64_t& w2, uint64_t& w3, uint64_t lo,
+                                          uint64_t hi, int sh) {
+  const int q = sh >> 6, r = sh & 63;
+  const uint64_t a = lo << r;
+  const uint64_t b = r ? ((hi << r) | (lo >> (64 - r))) : hi;
+  const uint64_t c = r ? (hi >> (64 - r)) : 0ull;
+  w0 |= q == 0 ? a : 0ull;
+  w1 |= q == 1 ? a : (q == 0 ? b : 0ull);
+  w2 |= q == 2 ? a : (q == 1 ? b : (q == 0 ? c : 0ull));
+  w3 |= q == 2 ? b : (q == 1 ? c : 0ull);
+}
+
 template <int MODE = kNbScan, class F>
 __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int* __restrict__ cell_start,
                                                    const double2* __restrict__ pos,
@@ -246,6 +259,36 @@ __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int
         else if (wl == 1) { m1 &= part; m2 = 0; m3 = 0; }
         else if (wl == 2) { m2 &= part; m3 = 0; }
         else { m3 &= part; }
+      }
+    } else if (staged && total <= 256 && n0 <= 128 && n1 <= 128 && n2 <= 128) {
+      // Column by column from LDS: a column's slots are contiguous and the loop
+      // index is the same in every lane, so a candidate costs one LDS read, the
+      // distance test and one bit insert; the self pair is cleared afterwards.
+#pragma unroll 1
+      for (int d = 0; d < 3; ++d) {
+        const int nd = d == 0 ? n0 : (d == 1 ? n1 : n2);
+        const float2* q = lds32 + (d == 0 ? s0 : (d == 1 ? s1 + n0 : s2 + n01));
+        uint64_t lo = 0, hi = 0;
+#pragma unroll 1
+        for (int j = 0; j < 4; ++j) {
+          uint32_t acc = 0;
+          const int kend = min(nd - 32 * j, 32);
+#pragma unroll 4
+          for (int k = 0; k < kend; ++k) {
+            const float2 v = q[32 * j + k];
+            const float fx = ps32.x - v.x, fy = ps32.y - v.y;
+            acc |= (uint32_t)(fx * fx + fy * fy <= P.cut2_32) << k;
+          }
+          if (j == 0) lo |= acc;
+          else if (j == 1) lo |= (uint64_t)acc << 32;
+          else if (j == 2) hi |= acc;
+          else hi |= (uint64_t)acc << 32;
+        }
+        if (d == 1) {
+          const int ks = s - R.lo[1];                 // the particle itself
+          if (ks < 64) lo &= ~(1ull << ks); else hi &= ~(1ull << (ks - 64));
+        }
+        or_shl128(m0, m1, m2, m3, lo, hi, d == 0 ? 0 : (d == 1 ? n0 : n01));
       }
     } else {
 #pragma unroll 1
