@@ -47,6 +47,16 @@ __device__ __forceinline__ double tait_p(const DevParams& P, double rho) {
   return P.eos_b * (x * x * x * x * x * x * x - 1.0);
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD b % 8),
+// each XCD with its own L2. The logical block of workgroup b gives every XCD one
+// contiguous run of blocks, so blocks whose neighbour columns overlap share an L2.
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb >> 3, r = nb & 7;
+  const int x = b & 7, k = b >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 // Neighbour cells of a particle: 3 column ranges of the column-major cell table.
 struct CellRanges {
   int lo[3], hi[3];
@@ -122,11 +132,11 @@ struct NbStage {
 // The block's candidate union (st.use = 0 when it cannot be staged within cap);
 // ends with a barrier.
 __device__ __forceinline__ void plan_stage(const DevParams& P, const int* __restrict__ cell_start,
-                                           const int* __restrict__ l_cell, int n, int cap, NbStage& st) {
+                                           const int* __restrict__ l_cell, int n, int cap, NbStage& st, int blk) {
   if (threadIdx.x == 0) {
     int use = 0, t = 0;
     int lo[3] = {0, 0, 0}, cnt[3] = {0, 0, 0};
-    const int b0 = blockIdx.x * blockDim.x;
+    const int b0 = blk * blockDim.x;
     if (b0 < n) {
       const int b1 = min(b0 + (int)blockDim.x, n) - 1;
       const int ny = P.chain_ny;
@@ -166,8 +176,8 @@ __device__ __forceinline__ int stage_source(const NbStage& st, int i) {
 __device__ __forceinline__ void stage_candidates(const DevParams& P, const int* __restrict__ cell_start,
                                                  const int* __restrict__ l_cell, const float2* __restrict__ pos32,
                                                  const double2* __restrict__ pos, int n, NbStage& st,
-                                                 float2* lds32, double2* lds64) {
-  plan_stage(P, cell_start, l_cell, n, kStageCap, st);
+                                                 float2* lds32, double2* lds64, int blk) {
+  plan_stage(P, cell_start, l_cell, n, kStageCap, st, blk);
   const int total = st.total;
   for (int i = threadIdx.x; i < total; i += blockDim.x) {
     const int o = stage_source(st, i);

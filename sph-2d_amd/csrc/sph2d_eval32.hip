@@ -97,9 +97,9 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
   __shared__ NbStage st;
   __shared__ float2 lds32[kStageCap];
   __shared__ double2 lds64[kStageCap];
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
-  stage_candidates(P, B.cell_start, B.l_cell, B.l_pos32, B.l_pos, n, st, lds32, lds64);
+  stage_candidates(P, B.cell_start, B.l_cell, B.l_pos32, B.l_pos, n, st, lds32, lds64, xcd_block());
   if (s >= n) return;
   const double2 ps = B.l_pos[s];
   const float4 r0s = B.m_r0[s];
@@ -192,9 +192,10 @@ template <int SOLVER>
 __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q, Buffers B, int do_shift) {
   __shared__ NbStage st;
   __shared__ float4 rec0[kRecCap], rec1[kRecCap], rec2[kRecCap];
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int blk = xcd_block();
+  const int s = blk * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
-  plan_stage(P, B.cell_start, B.l_cell, n, kRecCap, st);
+  plan_stage(P, B.cell_start, B.l_cell, n, kRecCap, st, blk);
   for (int i = threadIdx.x; i < st.total; i += blockDim.x) {
     const int o = stage_source(st, i);
     rec0[i] = B.m_r0[o];

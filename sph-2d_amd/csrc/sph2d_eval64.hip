@@ -49,9 +49,9 @@ __global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
   __shared__ NbStage st;
   __shared__ float2 lds32[kStageCap];
   __shared__ double2 lds64[kStageCap];
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
-  stage_candidates(P, B.cell_start, B.l_cell, B.l_pos32, B.l_pos, n, st, lds32, lds64);
+  stage_candidates(P, B.cell_start, B.l_cell, B.l_pos32, B.l_pos, n, st, lds32, lds64, xcd_block());
   if (s >= n) return;
   const double2 ps = B.l_pos[s];
   const double2 vs = B.l_vel[s];
@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
 
 // Surface normal and geometric test (Define_freesurface...f90:295-376), surface particles only.
 __global__ void __launch_bounds__(kBlock) k_eval_b1(DevParams P, Buffers B) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
   if (s >= n) return;
   if (B.surf[s] != 1) { B.nrm[s] = make_double2(0.0, 0.0); return; }
@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_b1(DevParams P, Buffers B) {
 
 // Near-surface flag (Define_freesurface...f90:418-440): any pair partner is a surface particle.
 __global__ void __launch_bounds__(kBlock) k_eval_b2(DevParams P, Buffers B) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
   if (s >= n) return;
   if (B.counters[C_NSURF] == 0) return;            // near_ was zeroed by pass A
@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_b2(DevParams P, Buffers B) {
 // ---------------------------------------------------------------------------
 template <bool DEBUG, int SOLVER>
 __device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int do_shift) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
   if (s >= n) return;
   const double2 ps = B.l_pos[s];

@@ -675,7 +675,7 @@ __global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers 
 // neighbours fall back to the Shepard average, as the reference does (:284-295).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_mls(DevParams P, Buffers B) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
   if (s >= n) return;
   const int ts = B.l_type[s];
