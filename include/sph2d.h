@@ -175,6 +175,69 @@ int sph2d_run(sph2d_ctx* ctx, int32_t first_step, int32_t nsteps);
  * for u, v, p/(rho0 U^2), |v| at [2..4], [5..7], [8..10], [11..13]. */
 int sph2d_sample(sph2d_ctx* ctx, double* out14);
 
+/* --- output and restart (the reference's on-disk formats) ------------------ */
+/* Normalisation of the output routines (information_module.f90:56,150-152):
+ * domain_size_y, velocity_0 = U_Module, pressure_0 = rho0*U^2, time_0 = Lx/U. */
+typedef struct {
+  double domain_size_y, velocity_0, pressure_0, time_0;
+} sph2d_scales;
+int sph2d_get_scales(sph2d_ctx* ctx, sph2d_scales* out);
+
+/* Vorticity (Refresh_domain.f90:320-351: sum over the last evaluation's pairs of
+ * V_j (v_i - v_j) x Normalized_dwdx) and the step-start analytic fields
+ * (Devide_block_and_send_tasks.f90:88-116: u, v, p, |v| as columns of the (ld,4)
+ * array ana), in global order, of the last sampling step (i_time_step a multiple of
+ * sample_every: both are computed on those steps only); *step receives that step.
+ * Any output may be NULL. */
+int sph2d_download_output_fields(sph2d_ctx* ctx, int32_t n, double* vort_z, double* ana, int32_t ld,
+                                 int32_t* step);
+
+/* Output_for_tecplot.f90:64-291 for the current state: a ZONE appended to the
+ * video file (video_result.dat, time_iteration.f90:147-151), one appended to the
+ * free-surface file (Free_Surface.dat, :137-140) and the #!TDV101 binary file; a
+ * NULL path skips that file. i_time_step must be the last sampling step. */
+int sph2d_output_for_tecplot(sph2d_ctx* ctx, int32_t i_time_step, const char* video_path,
+                             const char* surface_path, const char* binary_path);
+/* Output_for_paraview.f90:188-263 for the current state (VTK XML, ASCII). */
+int sph2d_output_for_paraview(sph2d_ctx* ctx, int32_t i_time_step, const char* path);
+
+/* Host-array writers (no device; Fortran layout, ld >= n). Text follows the
+ * reference's edit descriptors exactly; files named "appended" get the
+ * reference's header lines when they are new or empty. */
+/* Fw.d ('F') or Ew.dE3 ('E') of one value into out (cap > w). */
+int sph2d_format_fortran(double value, char kind, int32_t w, int32_t d, char* out, int32_t cap);
+/* Parallel_Taylor_Green_Vortices_2D_Release.f90:185-199 (Final_Doamin_Results.dat). */
+int sph2d_write_final_results(const char* path, int32_t n, const double* pos, const double* vel,
+                              const double* rho, const double* press, int32_t ld);
+/* Output_for_tecplot.f90:70-137 (appended ZONE; ana as in sph2d_download_output_fields). */
+int sph2d_write_tecplot_zone(const char* path, double current_time, int32_t n, const double* pos,
+                             const double* vel, const double* press, const double* vort_z, const double* ana,
+                             int32_t ld, const sph2d_scales* scales);
+/* Output_for_tecplot.f90:141-158 (appended ZONE of the surface particles given). */
+int sph2d_write_surface_zone(const char* path, double current_time, int32_t n, const double* pos,
+                             const double* normal, int32_t ld, const sph2d_scales* scales);
+/* Output_for_tecplot.f90:162-291 (#!TDV101, point-packed doubles X Y VX VY P U). */
+int sph2d_write_tecplot_binary(const char* path, double time, int32_t n, const double* pos, const double* vel,
+                               const double* press, int32_t ld, const sph2d_scales* scales);
+/* Output_for_paraview.f90:188-263. */
+int sph2d_write_paraview(const char* path, int32_t n, const double* pos, const double* vel, const double* press,
+                         int32_t ld, const sph2d_scales* scales);
+/* Restart record: time_iteration.f90:143-144,495-498 (appended ZONE, 9F20.8,3I12);
+ * wavemaker_layer may be NULL (zeros). */
+int sph2d_write_latest_results_zone(const char* path, int32_t i_time_step, int32_t n, const double* pos,
+                                    const double* vel, const double* press, const double* rho,
+                                    const double* mass, const double* energy, const double* c,
+                                    const int32_t* initial_type, const int32_t* wavemaker_layer, int32_t ld);
+/* Distribute_Latest_Particle.f90:60-215: reads zone `zone` (1-based; 0 = the last),
+ * stepping back while the zone holds a NaN. Outputs may be NULL; *n_out receives the
+ * particle count and *step_out the step the zone was saved at. */
+int sph2d_read_latest_results(const char* path, int32_t zone, int32_t capacity, int32_t* n_out, double* pos,
+                              double* vel, double* press, double* rho, double* mass, double* energy, double* c,
+                              int32_t* initial_type, int32_t* wavemaker_layer, int32_t* step_out, int32_t ld);
+/* Sampling_subroutine.f90:360-461 (format 20F20.8): appends one line to each of the
+ * six history files in dir from sph2d_sample's 14 values. */
+int sph2d_write_sampling(const char* dir, double nondim_time, const double* out14);
+
 /* --- parity / inspection -------------------------------------------------- */
 /* Download a named subdomain (local-order) array of the last evaluation:
  * f64: x y u v rho mass p c volume fra_v sum_dwdx_x sum_dwdx_y tau_xx tau_xy tau_yy

@@ -63,6 +63,8 @@ struct orc_ctx {
   double *gpx, *gpy, *gux, *guy, *gvx, *gvy;    /* MUSCL: grad_pressure, grad_velocity_u/_v */
   /* shift */
   double *shx, *shy;
+  /* Refresh_domain's vorticity (z), local order */
+  double *vort;
   /* time integration (local index arrays, persist across steps) */
   double *b_rho, *b_e, *b_x, *b_y, *b_u, *b_v;
   double *t_u, *t_v, *t_drho, *t_de, *t_ax, *t_ay;              /* PC temp_* */
@@ -221,7 +223,7 @@ static void alloc_local(orc_ctx* c, int cap) {
   A(ia_x); A(ia_y); A(av_x); A(av_y); A(ai_x); A(ai_y); A(sps_x); A(sps_y);
   A(lap_x); A(lap_y); A(ext_x); A(ext_y); A(avg_x); A(avg_y); A(corr); A(grx); A(gry);
   A(gpx); A(gpy); A(gux); A(guy); A(gvx); A(gvy);
-  A(shx); A(shy);
+  A(shx); A(shy); A(vort);
   A(b_rho); A(b_e); A(b_x); A(b_y); A(b_u); A(b_v);
   A(t_u); A(t_v); A(t_drho); A(t_de); A(t_ax); A(t_ay);
   for (int k = 0; k < 3; ++k) { A(k_u[k]); A(k_v[k]); A(k_drho[k]); A(k_de[k]); A(k_ax[k]); A(k_ay[k]); }
@@ -266,7 +268,7 @@ void orc_destroy(orc_ctx* c) {
                   c->Syy, c->txx, c->txy, c->tyy, c->drhodt, c->dedt, c->ax, c->ay, c->ia_x,
                   c->ia_y, c->av_x, c->av_y, c->ai_x, c->ai_y, c->sps_x, c->sps_y, c->lap_x,
                   c->lap_y, c->ext_x, c->ext_y, c->avg_x, c->avg_y, c->corr, c->grx, c->gry,
-                  c->gpx, c->gpy, c->gux, c->guy, c->gvx, c->gvy, c->shx, c->shy, c->b_rho, c->b_e, c->b_x, c->b_y, c->b_u, c->b_v, c->t_u,
+                  c->gpx, c->gpy, c->gux, c->guy, c->gvx, c->gvy, c->shx, c->shy, c->vort, c->b_rho, c->b_e, c->b_x, c->b_y, c->b_u, c->b_v, c->t_u,
                   c->t_v, c->t_drho, c->t_de, c->t_ax, c->t_ay, c->mls_rho,
                   c->cell_count, c->cell_list, c->near_n, c->near_name,
                   c->pi, c->pj, c->pw, c->pdwx, c->pdwy, c->pdx, c->pdy, c->pvx, c->pvy, c->prd,
@@ -1150,8 +1152,22 @@ void orc_shift(orc_ctx* c) {
   }
 }
 
+/* Refresh_domain.f90:320-335: vorticity from the last evaluation's pair cache,
+ * vort_i += V_j (v_i - v_j) x Normalized_dwdx_i, vort_j += V_i (v_i - v_j) x
+ * Normalized_dwdx_j; Cross_Product(...)(3) = X1*Y2 - X2*Y1 (function_module.f90:150). */
+static void refresh_vorticity(orc_ctx* c) {
+  memset(c->vort, 0, sizeof(double) * c->n_local);
+  for (int64_t k = 0; k < c->npair; ++k) {
+    int i = c->pi[k], j = c->pj[k];
+    double vx = c->pvx[k], vy = c->pvy[k];
+    c->vort[i] = c->vort[i] + c->vol[j] * (vx * c->ndiy[k] - vy * c->ndix[k]);
+    c->vort[j] = c->vort[j] + c->vol[i] * (vx * c->ndjy[k] - vy * c->ndjx[k]);
+  }
+}
+
 /* Refresh_domain.f90:136-300 on one rank: global arrays zeroed, then filled. */
 void orc_refresh(orc_ctx* c) {
+  refresh_vorticity(c);
   int NG = c->n_fluid + c->n_ghost;
   size_t nb = sizeof(double) * NG;
   memset(c->gx, 0, nb); memset(c->gy, 0, nb); memset(c->gu, 0, nb); memset(c->gv, 0, nb);
@@ -1348,7 +1364,7 @@ static const fld F64[] = {
   FL("sps_x", sps_x, 0), FL("sps_y", sps_y, 0), FL("lap_x", lap_x, 0), FL("lap_y", lap_y, 0),
   FL("avg_vel_x", avg_x, 0), FL("avg_vel_y", avg_y, 0), FL("rho_correct", corr, 0),
   FL("grad_rho_x", grx, 0), FL("grad_rho_y", gry, 0), FL("shift_x", shx, 0),
-  FL("shift_y", shy, 0),
+  FL("shift_y", shy, 0), FL("vorticity", vort, 0),
   FL("grad_p_x", gpx, 0), FL("grad_p_y", gpy, 0), FL("grad_u_x", gux, 0), FL("grad_u_y", guy, 0),
   FL("grad_v_x", gvx, 0), FL("grad_v_y", gvy, 0),
   FL("T", T, 3), FL("T_inv", Tinv, 3),

@@ -1,8 +1,9 @@
 """Builds libsph2d.so (gfx950) in-tree with hipcc.
 
-Four translation units: the FP64 step and the FP64 force evaluation (no FMA
+Translation units: the FP64 step and the FP64 force evaluation (no FMA
 contraction, for parity with the reference build), the FP32-mixed force
-evaluation (contraction allowed) and the C-ABI/host driver. Packed FP32
+evaluation (contraction allowed), the C-ABI/host driver and the host-side
+writers of the reference's file formats. Packed FP32
 instructions are disabled on every HIP line.
 """
 from __future__ import annotations
@@ -29,6 +30,7 @@ UNITS = [
     ("sph2d_eval32.hip", ["-ffp-contract=fast"]),
     ("sph2d_reorder.hip", ["-ffp-contract=off"]),
     ("sph2d_capi.hip", ["-ffp-contract=off"]),
+    ("sph2d_output.hip", ["-ffp-contract=off"]),
 ]
 
 HEADERS = ["sph2d_internal.h", "sph2d_device.h", "sph2d_riemann.h"]

@@ -19,6 +19,13 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+}  // namespace
+
+// the message of sph2d_last_error for the other translation units
+int sph2d::set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+namespace {
+
 #define HIP_OR_FAIL(x)                                                                      \
   do {                                                                                      \
     hipError_t e_ = (x);                                                                    \
@@ -45,6 +52,7 @@ struct sph2d_ctx {
   double* d_out = nullptr;          // 16 doubles for reductions
   std::vector<void*> allocs;
   // in-step pass timing: 4 events per evaluation (pass A start/end, force pass start/end)
+  int ana_step = 0, vort_step = 0;   // last steps whose analytic fields / vorticity are held
   bool pass_timing = false;
   std::vector<hipEvent_t> events;
   size_t events_used = 0;
@@ -275,7 +283,7 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   rc |= c->alloc(&B.g_rho, G); rc |= c->alloc(&B.g_mass, G); rc |= c->alloc(&B.g_p, G);
   rc |= c->alloc(&B.g_c, G); rc |= c->alloc(&B.g_e, G);
   rc |= c->alloc(&B.g_type, G); rc |= c->alloc(&B.g_key, G); rc |= c->alloc(&B.g_slot, G);
-  rc |= c->alloc(&B.ana, 4 * F);
+  rc |= c->alloc(&B.ana, 4 * F); rc |= c->alloc(&B.g_vort, F);
   rc |= c->alloc(&B.gblk, 2 * ((F + 255) / 256) + 2);
   rc |= c->alloc(&B.cnt_yx, H); rc |= c->alloc(&B.off_yx, H);
   rc |= c->alloc(&B.src_x, H);
@@ -587,13 +595,14 @@ static int devide_block(sph2d_ctx* c, int32_t it, int predict) {
   const double t = it * c->prm.dt / (c->prm.box_x / c->prm.U);   // Nondimensionalized_real_time
   launch_fill_i32(c->stream, c->B.counters + C_NILL, 1, 0);
   launch_devide_block(c->stream, c->P, c->B, c->cap_global, c->n_cells, ana, t, predict);
+  if (ana) c->ana_step = it;
   HIP_OR_FAIL(hipGetLastError());
   return SPH2D_OK;
 }
 
 int sph2d_devide_block(sph2d_ctx* c, int32_t it) { return devide_block(c, it, 0); }
 
-static void do_eval(sph2d_ctx* c, bool with_eos) {
+static void do_eval(sph2d_ctx* c, bool with_eos, int it) {
   if (with_eos) launch_stage(c->stream, c->P, c->B, c->cap_local, ST_EOS);
   c->P.write_debug = c->debug ? 1 : 0;
   hipEvent_t* ev = nullptr;
@@ -610,13 +619,18 @@ static void do_eval(sph2d_ctx* c, bool with_eos) {
   }
   launch_eval(c->stream, c->P, c->B, c->cap_local, ev);
   c->evaluated = true;
+  // Refresh_domain's vorticity uses the step's last evaluation; it is kept on
+  // sampling steps (each evaluation of such a step overwrites it).
+  if (it > 0 && it % c->prm.sample_every == 0) {
+    launch_vorticity(c->stream, c->P, c->B, c->cap_local);
+    c->vort_step = it;
+  }
 }
 
 int sph2d_single_step_compute(sph2d_ctx* c, int32_t it) {
-  (void)it;
   int rc = need_state(c);
   if (rc) return rc;
-  do_eval(c, true);
+  do_eval(c, true, it);
   HIP_OR_FAIL(hipGetLastError());
   return SPH2D_OK;
 }
@@ -665,10 +679,10 @@ static int enqueue_step(sph2d_ctx* c, int it) {
   hipStream_t s = c->stream;
   if (pc) {
     if (it == 1) {
-      do_eval(c, false);
+      do_eval(c, false, it);
       launch_stage(s, c->P, c->B, c->cap_local, ST_PC_PREDICT);
     }
-    do_eval(c, false);
+    do_eval(c, false, it);
     if (!mls) {
       // corrector, Shift_particle_position and Refresh_domain in one pass
       launch_correct_refresh(s, c->P, c->B, c->cap_local, c->prm.shift_on);
@@ -677,13 +691,13 @@ static int enqueue_step(sph2d_ctx* c, int it) {
     }
     launch_stage(s, c->P, c->B, c->cap_local, ST_PC_CORRECT);
   } else {
-    if (it == 1) do_eval(c, false);
+    if (it == 1) do_eval(c, false, it);
     launch_stage(s, c->P, c->B, c->cap_local, ST_RK_HALF0);
-    do_eval(c, false);
+    do_eval(c, false, it);
     launch_stage(s, c->P, c->B, c->cap_local, ST_RK_HALF1);
-    do_eval(c, false);
+    do_eval(c, false, it);
     launch_stage(s, c->P, c->B, c->cap_local, ST_RK_FULL2);
-    do_eval(c, false);
+    do_eval(c, false, it);
     launch_stage(s, c->P, c->B, c->cap_local, ST_RK_FINAL);
   }
   if (mls) {
@@ -736,6 +750,116 @@ int sph2d_sample(sph2d_ctx* c, double* out14) {
     out14[4 + 3 * a] = std::sqrt(v[4 + 3 * a] / N);
   }
   return SPH2D_OK;
+}
+
+// ---- output -----------------------------------------------------------------------
+int sph2d_get_scales(sph2d_ctx* c, sph2d_scales* out) {
+  if (!c || !out) return fail(SPH2D_ERR_ARGUMENT, "NULL argument");
+  out->domain_size_y = c->prm.box_y;
+  out->velocity_0 = c->prm.U;
+  out->pressure_0 = c->prm.rho0 * c->prm.U * c->prm.U;
+  out->time_0 = c->prm.box_x / c->prm.U;
+  return SPH2D_OK;
+}
+
+int sph2d_download_output_fields(sph2d_ctx* c, int32_t n, double* vort_z, double* ana, int32_t ld, int32_t* step) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (n != c->n_fluid) return fail(SPH2D_ERR_ARGUMENT, "n must equal the number of fluid particles");
+  if (ld < n) return fail(SPH2D_ERR_ARGUMENT, "ld < n");
+  if (c->vort_step == 0 || c->vort_step != c->ana_step)
+    return fail(SPH2D_ERR_STATE, "no sampling step (i_time_step a multiple of sample_every) has run yet");
+  if (vort_z) HIP_OR_FAIL(hipMemcpyAsync(vort_z, c->B.g_vort, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> a;
+  if (ana) {
+    a.resize((size_t)4 * n);
+    HIP_OR_FAIL(hipMemcpyAsync(a.data(), c->B.ana, a.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  if (ana)
+    for (int k = 0; k < 4; ++k)
+      for (int i = 0; i < n; ++i) ana[(size_t)k * ld + i] = a[(size_t)4 * i + k];
+  if (step) *step = c->vort_step;
+  return SPH2D_OK;
+}
+
+// Surface particles of the last evaluation in gid order (Calculation_For_PostProceeding.f90:96-102
+// builds surface_particle_name this way) with their positions and normals.
+static int surface_particles(sph2d_ctx* c, std::vector<double>& pos, std::vector<double>& nrm) {
+  int cnt[C_COUNT];
+  int rc = read_counters(c, cnt);
+  if (rc) return rc;
+  const int nl = cnt[C_NLOCAL];
+  std::vector<int> surf(nl), name(nl);
+  std::vector<double2> lp(nl), ln(nl);
+  HIP_OR_FAIL(hipMemcpyAsync(surf.data(), c->B.surf, nl * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(name.data(), c->B.name, nl * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(ln.data(), c->B.nrm, nl * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  std::vector<std::pair<int, int>> sel;              // (gid, local slot)
+  for (int l = 0; l < nl; ++l)
+    if (surf[l] == 1 && name[l] < c->n_fluid) sel.emplace_back(name[l], l);
+  std::sort(sel.begin(), sel.end());
+  const int m = (int)sel.size();
+  std::vector<double2> gp(c->n_fluid);
+  HIP_OR_FAIL(hipMemcpyAsync(gp.data(), c->B.g_pos, c->n_fluid * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  pos.assign((size_t)2 * m, 0.0);
+  nrm.assign((size_t)2 * m, 0.0);
+  for (int k = 0; k < m; ++k) {
+    pos[k] = gp[sel[k].first].x; pos[m + k] = gp[sel[k].first].y;
+    nrm[k] = ln[sel[k].second].x; nrm[m + k] = ln[sel[k].second].y;
+  }
+  return SPH2D_OK;
+}
+
+int sph2d_output_for_tecplot(sph2d_ctx* c, int32_t it, const char* video_path, const char* surface_path,
+                             const char* binary_path) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (it != c->vort_step || it != c->ana_step)
+    return fail(SPH2D_ERR_STATE, "i_time_step must be the last step run and a multiple of sample_every");
+  const int n = c->n_fluid;
+  sph2d_scales sc;
+  sph2d_get_scales(c, &sc);
+  std::vector<double> pos((size_t)2 * n), vel((size_t)2 * n), press(n);
+  rc = sph2d_download_state(c, n, pos.data(), vel.data(), nullptr, press.data(), nullptr, nullptr, nullptr, n);
+  if (rc) return rc;
+  const double t = it * c->prm.dt;                   // (i_time_step - standing_time_step) * dt
+  if (video_path) {
+    std::vector<double> vort(n), ana((size_t)4 * n);
+    rc = sph2d_download_output_fields(c, n, vort.data(), ana.data(), n, nullptr);
+    if (rc) return rc;
+    rc = sph2d_write_tecplot_zone(video_path, t, n, pos.data(), vel.data(), press.data(), vort.data(), ana.data(), n,
+                                  &sc);
+    if (rc) return rc;
+  }
+  if (surface_path) {
+    std::vector<double> sp, sn;
+    rc = surface_particles(c, sp, sn);
+    if (rc) return rc;
+    const int m = (int)(sp.size() / 2);
+    rc = sph2d_write_surface_zone(surface_path, t, m, sp.data(), sn.data(), m, &sc);
+    if (rc) return rc;
+  }
+  if (binary_path) {
+    rc = sph2d_write_tecplot_binary(binary_path, t, n, pos.data(), vel.data(), press.data(), n, &sc);
+    if (rc) return rc;
+  }
+  return SPH2D_OK;
+}
+
+int sph2d_output_for_paraview(sph2d_ctx* c, int32_t it, const char* path) {
+  (void)it;                                          // the caller names the file by the step's time
+  int rc = need_state(c);
+  if (rc) return rc;
+  const int n = c->n_fluid;
+  sph2d_scales sc;
+  sph2d_get_scales(c, &sc);
+  std::vector<double> pos((size_t)2 * n), vel((size_t)2 * n), press(n);
+  rc = sph2d_download_state(c, n, pos.data(), vel.data(), nullptr, press.data(), nullptr, nullptr, nullptr, n);
+  if (rc) return rc;
+  return sph2d_write_paraview(path, n, pos.data(), vel.data(), press.data(), n, &sc);
 }
 
 // ---- inspection -------------------------------------------------------------------

@@ -405,6 +405,38 @@ void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, h
   if (ev) (void)hipEventRecord(ev[3], s);
 }
 
+// Refresh_domain.f90:320-335: the vorticity of the last evaluation's pairs,
+// sum_j V_j (v_i - v_j) x Normalized_dwdx with the identity renormalisation; the
+// pair's j-side term V_i (v_i - v_j) x dwdx_ij equals the same gather term seen
+// from j. Written for fluid particles, in global order (Refresh's reduce).
+__global__ void __launch_bounds__(kBlock) k_vorticity(DevParams P, Buffers B) {
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  const int g = B.name[s];
+  if (g >= B.counters[C_NFLUID]) return;           // ghosts are not refreshed
+  const int ts = B.l_type[s];
+  const double2 ps = B.l_pos[s];
+  const double2 vs = B.l_vel[s];
+  double wz = 0.0;
+  for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) __attribute__((always_inline)) {
+    if (ts != 2 && B.l_type[o] != 2) return;       // find_particle_pair.f90:76
+    const double2 po = B.l_pos[o];
+    const double pdx = ps.x - po.x, pdy = ps.y - po.y;
+    const double r = sqrt(pdx * pdx + pdy * pdy);
+    double w, dwx, dwy;
+    kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
+    const double2 vo = B.l_vel[o];
+    const double vx = vs.x - vo.x, vy = vs.y - vo.y;
+    wz = wz + B.l_rv[o].y * (vx * dwy - vy * dwx);   // Cross_Product(...)(3), function_module.f90:150
+  }, B.nb_mask, B.kstride);
+  B.g_vort[g] = wz;
+}
+
+void launch_vorticity(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
+  hipLaunchKernelGGL(k_vorticity, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B);
+}
+
 void launch_eval_pass(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int pass) {
   const int g = grid_for(cap_local);
   if (pass == 0) {

@@ -10,6 +10,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <string>
 #include "../../include/sph2d.h"
 
 namespace sph2d {
@@ -57,6 +58,7 @@ struct Buffers {
   double *g_rho, *g_mass, *g_p, *g_c, *g_e;
   int *g_type, *g_key, *g_slot;
   double *ana;                            // 4 per fluid particle: u, v, p, |v|
+  double *g_vort;                         // vorticity (z) per fluid gid, on sampling steps
   int *gblk, *cnt_yx, *off_yx, *src_x;   // gblk: per block of gids, x-ghost then y-ghost counts
   // cells (column-major key over the global grid)
   int *cell_count, *cell_start, *cell_members;
@@ -99,6 +101,9 @@ struct Buffers {
   int kstride;                            // stride of the 3 RK stage slabs (= local capacity)
 };
 
+// Records the message sph2d_last_error returns and passes the code through.
+int set_error(int code, const std::string& msg);
+
 // Launchers (sph2d_fp64.hip). All run on `s` and take sizes from device counters.
 void launch_init_tgv(hipStream_t s, const DevParams& P, Buffers& B, int nx, int ny);
 void launch_period_boundary(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int cap_ghost);
@@ -115,6 +120,8 @@ void launch_eval32_c(hipStream_t s, const DevParams& P, Buffers& B, int cap_loca
 void launch_eval_pass(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int pass);
 void launch_stage(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int kind);
 void launch_mls(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+// Refresh_domain.f90:320-351 from the last evaluation (sph2d_eval64.hip)
+void launch_vorticity(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
 void launch_shift_apply(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
 void launch_shift_apply_mode(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int mode);
 void launch_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);

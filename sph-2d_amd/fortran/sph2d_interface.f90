@@ -147,6 +147,75 @@ module sph2d_interface
       integer(c_int) :: rc
     end function
 
+    ! Output and restart (the reference's file formats). Paths are C strings:
+    ! pass trim(name)//c_null_char.
+    function sph2d_download_output_fields(ctx, n, vort_z, ana, ld, step) &
+        bind(C, name="sph2d_download_output_fields") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: n, ld
+      real(c_double), intent(out) :: vort_z(ld), ana(ld, 4)
+      integer(c_int32_t), intent(out) :: step
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_output_for_tecplot(ctx, i_time_step, video_path, surface_path, binary_path) &
+        bind(C, name="sph2d_output_for_tecplot") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: i_time_step
+      character(kind=c_char), intent(in) :: video_path(*), surface_path(*), binary_path(*)
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_output_for_paraview(ctx, i_time_step, path) bind(C, name="sph2d_output_for_paraview") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: i_time_step
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_write_final_results(path, n, pos, vel, rho, press, ld) &
+        bind(C, name="sph2d_write_final_results") result(rc)
+      import :: c_int, c_int32_t, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: n, ld
+      real(c_double), intent(in) :: pos(ld, 2), vel(ld, 2), rho(ld), press(ld)
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_write_latest_results_zone(path, i_time_step, n, pos, vel, press, rho, mass, energy, c, &
+                                             initial_type, wavemaker_layer, ld) &
+        bind(C, name="sph2d_write_latest_results_zone") result(rc)
+      import :: c_int, c_int32_t, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: i_time_step, n, ld
+      real(c_double), intent(in) :: pos(ld, 2), vel(ld, 2), press(ld), rho(ld), mass(ld), energy(ld), c(ld)
+      integer(c_int32_t), intent(in) :: initial_type(ld), wavemaker_layer(ld)
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_read_latest_results(path, zone, capacity, n_out, pos, vel, press, rho, mass, energy, c, &
+                                       initial_type, wavemaker_layer, step_out, ld) &
+        bind(C, name="sph2d_read_latest_results") result(rc)
+      import :: c_int, c_int32_t, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: zone, capacity, ld
+      integer(c_int32_t), intent(out) :: n_out, step_out
+      real(c_double), intent(out) :: pos(ld, 2), vel(ld, 2), press(ld), rho(ld), mass(ld), energy(ld), c(ld)
+      integer(c_int32_t), intent(out) :: initial_type(ld), wavemaker_layer(ld)
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_write_sampling(dir, nondim_time, out14) bind(C, name="sph2d_write_sampling") result(rc)
+      import :: c_int, c_double, c_char
+      character(kind=c_char), intent(in) :: dir(*)
+      real(c_double), value :: nondim_time
+      real(c_double), intent(in) :: out14(14)
+      integer(c_int) :: rc
+    end function
+
     function sph2d_last_error() bind(C, name="sph2d_last_error") result(msg)
       import :: c_ptr
       type(c_ptr) :: msg

@@ -60,6 +60,12 @@ class Params(ctypes.Structure):
     ]
 
 
+class Scales(ctypes.Structure):
+    """sph2d_scales: the output routines' normalisation (information_module.f90:56,150-152)."""
+    _fields_ = [("domain_size_y", ctypes.c_double), ("velocity_0", ctypes.c_double),
+                ("pressure_0", ctypes.c_double), ("time_0", ctypes.c_double)]
+
+
 class Diag(ctypes.Structure):
     _fields_ = [
         ("n_fluid", ctypes.c_int32), ("n_ghost", ctypes.c_int32), ("n_local", ctypes.c_int32),
@@ -80,6 +86,11 @@ EXPORTS = (
     "sph2d_download_local", "sph2d_download_pairs", "sph2d_set_debug",
     "sph2d_get_diagnostics", "sph2d_get_stream", "sph2d_time_pass",
     "sph2d_reorder_particle_index", "sph2d_set_pass_timing", "sph2d_get_pass_timing",
+    "sph2d_get_scales", "sph2d_download_output_fields", "sph2d_output_for_tecplot",
+    "sph2d_output_for_paraview", "sph2d_format_fortran", "sph2d_write_final_results",
+    "sph2d_write_tecplot_zone", "sph2d_write_surface_zone", "sph2d_write_tecplot_binary",
+    "sph2d_write_paraview", "sph2d_write_latest_results_zone", "sph2d_read_latest_results",
+    "sph2d_write_sampling",
 )
 
 _lib = None
@@ -94,6 +105,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         P = ctypes.POINTER
         vp, i32, i64, dp, ip = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, P(ctypes.c_double), P(ctypes.c_int32)
+        cs = ctypes.c_char_p
         L.sph2d_last_error.restype = ctypes.c_char_p
         sig = {
             "sph2d_default_params": [P(Params)],
@@ -125,6 +137,19 @@ def lib():
             "sph2d_get_pass_timing": [vp, dp, ip],
             "sph2d_reorder_particle_index": [i32, i32, ctypes.c_double, ctypes.c_double, vp, vp, vp, i32,
                                              P(vp), i32, P(vp), vp, ip, vp],
+            "sph2d_get_scales": [vp, P(Scales)],
+            "sph2d_download_output_fields": [vp, i32, dp, dp, i32, ip],
+            "sph2d_output_for_tecplot": [vp, i32, cs, cs, cs],
+            "sph2d_output_for_paraview": [vp, i32, cs],
+            "sph2d_format_fortran": [ctypes.c_double, ctypes.c_char, i32, i32, ctypes.c_char_p, i32],
+            "sph2d_write_final_results": [cs, i32, dp, dp, dp, dp, i32],
+            "sph2d_write_tecplot_zone": [cs, ctypes.c_double, i32, dp, dp, dp, dp, dp, i32, P(Scales)],
+            "sph2d_write_surface_zone": [cs, ctypes.c_double, i32, dp, dp, i32, P(Scales)],
+            "sph2d_write_tecplot_binary": [cs, ctypes.c_double, i32, dp, dp, dp, i32, P(Scales)],
+            "sph2d_write_paraview": [cs, i32, dp, dp, dp, i32, P(Scales)],
+            "sph2d_write_latest_results_zone": [cs, i32, i32, dp, dp, dp, dp, dp, dp, dp, ip, ip, i32],
+            "sph2d_read_latest_results": [cs, i32, i32, ip, dp, dp, dp, dp, dp, dp, dp, ip, ip, ip, i32],
+            "sph2d_write_sampling": [cs, ctypes.c_double, dp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -259,6 +284,43 @@ class Simulation:
         out = np.zeros(14)
         _check(lib().sph2d_sample(self._c, _dp(out)))
         return out
+
+    # --- output (the reference's files) --------------------------------------------
+    def scales(self) -> Scales:
+        sc = Scales()
+        _check(lib().sph2d_get_scales(self._c, ctypes.byref(sc)))
+        return sc
+
+    def output_fields(self):
+        """(vorticity_z, analytic (n,4): u, v, p, |v|, step) of the last sampling step,
+        global order (Refresh_domain.f90:320-351, Devide_block_and_send_tasks.f90:88-116)."""
+        n = self.n_fluid
+        w, a = np.zeros(n), np.zeros(4 * n)
+        step = ctypes.c_int32()
+        _check(lib().sph2d_download_output_fields(self._c, n, _dp(w), _dp(a), n, ctypes.byref(step)))
+        return w, a.reshape(4, n).T.copy(), step.value
+
+    def output_for_tecplot(self, i_time_step: int, video_path=None, surface_path=None, binary_path=None):
+        """Output_for_tecplot.f90: ZONEs appended to video_path / surface_path and the
+        #!TDV101 file binary_path (None skips a file)."""
+        enc = lambda p: None if p is None else os.fspath(p).encode()
+        _check(lib().sph2d_output_for_tecplot(self._c, i_time_step, enc(video_path), enc(surface_path),
+                                              enc(binary_path)))
+
+    def output_for_paraview(self, i_time_step: int, path):
+        _check(lib().sph2d_output_for_paraview(self._c, i_time_step, os.fspath(path).encode()))
+
+    def output_final_results(self, path):
+        """Final_Doamin_Results.dat (Parallel_Taylor_Green_Vortices_2D_Release.f90:185-199)."""
+        st = self.download_state()
+        write_final_results(path, st["x"], st["y"], st["u"], st["v"], st["rho"], st["p"])
+
+    def output_latest_results(self, i_time_step: int, path):
+        """The restart ZONE of Latest_Time_Results.dat (time_iteration.f90:495-498)."""
+        st = self.download_state()
+        n = self.n_fluid
+        write_latest_results_zone(path, i_time_step, st["x"], st["y"], st["u"], st["v"], st["p"], st["rho"],
+                                  st["mass"], np.zeros(n), st["c"], st["type"])
 
     # --- inspection -------------------------------------------------------------------
     def set_debug(self, on: bool = True):
@@ -407,3 +469,90 @@ def reorder_particle_index(n_fluid, box_x, box_y, x, y, type_, f64_cols=(), i32_
     for d in [dx, dy, dt, dp] + df + di:
         d.free()
     return out
+
+
+# --- the reference's file formats, from host arrays (no GPU needed) ---------------------
+def _col2(a, b):
+    return np.ascontiguousarray(np.concatenate([np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)]))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def format_fortran(value: float, kind: str, w: int, d: int) -> str:
+    """One value under the Fortran edit descriptor Fw.d (kind 'F') or Ew.dE3 ('E')."""
+    buf = ctypes.create_string_buffer(w + 1)
+    _check(lib().sph2d_format_fortran(float(value), kind.encode(), w, d, buf, w + 1))
+    return buf.value.decode()
+
+
+def time_name(t: float, w: int, d: int) -> str:
+    """trim(adjustl(char_time)) of write(char_time,'(fW.D)') t: the reference's file and zone names."""
+    return format_fortran(t, "F", w, d).strip()
+
+
+def write_final_results(path, x, y, u, v, rho, p):
+    n = len(x)
+    _check(lib().sph2d_write_final_results(os.fspath(path).encode(), n, _dp(_col2(x, y)), _dp(_col2(u, v)),
+                                           _dp(_f64(rho)), _dp(_f64(p)), n))
+
+
+def write_tecplot_zone(path, current_time, x, y, u, v, p, vort_z, ana, scales: Scales):
+    """ana: (n,4) analytic u, v, p, |v|."""
+    n = len(x)
+    a = np.ascontiguousarray(np.asarray(ana, dtype=np.float64).T.reshape(-1))
+    _check(lib().sph2d_write_tecplot_zone(os.fspath(path).encode(), current_time, n, _dp(_col2(x, y)),
+                                          _dp(_col2(u, v)), _dp(_f64(p)), _dp(_f64(vort_z)), _dp(a), n,
+                                          ctypes.byref(scales)))
+
+
+def write_surface_zone(path, current_time, x, y, nx, ny, scales: Scales):
+    n = len(x)
+    _check(lib().sph2d_write_surface_zone(os.fspath(path).encode(), current_time, n, _dp(_col2(x, y)),
+                                          _dp(_col2(nx, ny)), n, ctypes.byref(scales)))
+
+
+def write_tecplot_binary(path, time, x, y, u, v, p, scales: Scales):
+    n = len(x)
+    _check(lib().sph2d_write_tecplot_binary(os.fspath(path).encode(), time, n, _dp(_col2(x, y)), _dp(_col2(u, v)),
+                                            _dp(_f64(p)), n, ctypes.byref(scales)))
+
+
+def write_paraview(path, x, y, u, v, p, scales: Scales):
+    n = len(x)
+    _check(lib().sph2d_write_paraview(os.fspath(path).encode(), n, _dp(_col2(x, y)), _dp(_col2(u, v)),
+                                      _dp(_f64(p)), n, ctypes.byref(scales)))
+
+
+def write_latest_results_zone(path, i_time_step, x, y, u, v, p, rho, mass, energy, c, initial_type,
+                              wavemaker_layer=None):
+    n = len(x)
+    t = np.ascontiguousarray(initial_type, dtype=np.int32)
+    wl = None if wavemaker_layer is None else np.ascontiguousarray(wavemaker_layer, dtype=np.int32)
+    _check(lib().sph2d_write_latest_results_zone(os.fspath(path).encode(), i_time_step, n, _dp(_col2(x, y)),
+                                                 _dp(_col2(u, v)), _dp(_f64(p)), _dp(_f64(rho)), _dp(_f64(mass)),
+                                                 _dp(_f64(energy)), _dp(_f64(c)), _ip(t),
+                                                 _ip(wl) if wl is not None else None, n))
+
+
+def read_latest_results(path, zone: int = 0, capacity: int = 1 << 24) -> Dict[str, np.ndarray]:
+    """Distribute_Latest_Particle.f90: the restart zone (0 = last), stepping back past NaN zones."""
+    cap = int(capacity)
+    n = ctypes.c_int32()
+    step = ctypes.c_int32()
+    pos, vel = np.zeros(2 * cap), np.zeros(2 * cap)
+    p, rho, m, e, c = (np.zeros(cap) for _ in range(5))
+    t, wl = np.zeros(cap, dtype=np.int32), np.zeros(cap, dtype=np.int32)
+    _check(lib().sph2d_read_latest_results(os.fspath(path).encode(), zone, cap, ctypes.byref(n), _dp(pos),
+                                           _dp(vel), _dp(p), _dp(rho), _dp(m), _dp(e), _dp(c), _ip(t), _ip(wl),
+                                           ctypes.byref(step), cap))
+    k = n.value
+    return {"x": pos[:k], "y": pos[cap:cap + k], "u": vel[:k], "v": vel[cap:cap + k], "p": p[:k], "rho": rho[:k],
+            "mass": m[:k], "energy": e[:k], "c": c[:k], "type": t[:k], "wavemaker_layer": wl[:k], "step": step.value}
+
+
+def write_sampling(dir_path, nondim_time: float, out14):
+    """One line appended to each of Sampling_subroutine.f90's six history files."""
+    o = _f64(out14)
+    _check(lib().sph2d_write_sampling(os.fspath(dir_path).encode(), nondim_time, _dp(o)))

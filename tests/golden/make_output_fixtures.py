@@ -1,0 +1,32 @@
+"""Copies excerpts of the reference's shipped output files, verbatim, into
+tests/golden/output_formats/ (run in the development container, where
+/root/reference exists). They pin the text layout of the writers in
+sph2d_output.hip: header lines, ZONE lines and the F20.10 / F20.8 rows.
+
+Sources (mini-walker/SPH-2D, Taylor_Green_Vortex/, the shipped 100x100 run):
+  Final_Doamin_Results.dat  header + the first 200 particle rows (8F20.10 of 6 values)
+  Free_Surface.dat          header + the first 4 ZONE lines (all zones are empty)
+  Kinetic_energy.dat        first 40 rows (format 310 = 20F20.8)
+  Norm_Pressure.dat         first 40 rows
+Only data lines are copied; no reference source is.
+"""
+import os
+
+REF = os.environ.get("SPH2D_REFERENCE", "/root/reference") + "/Taylor_Green_Vortex"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "output_formats")
+EXCERPTS = {"Final_Doamin_Results.dat": 203, "Free_Surface.dat": 6, "Kinetic_energy.dat": 40,
+            "Norm_Pressure.dat": 40}
+
+
+def main():
+    os.makedirs(OUT, exist_ok=True)
+    for name, nlines in EXCERPTS.items():
+        with open(os.path.join(REF, name), "rb") as f:
+            lines = f.read().split(b"\n")[:nlines]
+        with open(os.path.join(OUT, name + ".head"), "wb") as f:
+            f.write(b"\n".join(lines) + b"\n")
+        print(name, len(lines), "lines")
+
+
+if __name__ == "__main__":
+    main()

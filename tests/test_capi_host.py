@@ -51,7 +51,8 @@ def _c_layout(struct, fields):
     return vals[0], vals[1:]
 
 
-@pytest.mark.parametrize("py,cname", [(sph2d.Params, "sph2d_params"), (sph2d.Diag, "sph2d_diag")])
+@pytest.mark.parametrize("py,cname", [(sph2d.Params, "sph2d_params"), (sph2d.Diag, "sph2d_diag"),
+                                     (sph2d.Scales, "sph2d_scales")])
 def test_struct_layout_matches_header(py, cname):
     names = [f[0] for f in py._fields_]
     size, offs = _c_layout(cname, names)

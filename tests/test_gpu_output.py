@@ -1,0 +1,154 @@
+"""GPU output path: Refresh_domain's vorticity and the step-start analytic fields
+against the oracle, and the context-level writers (Output_for_tecplot.f90,
+Output_for_paraview.f90, the restart record) against the downloaded state."""
+import os
+import struct
+import sys
+
+import numpy as np
+import pytest
+
+import sph2d
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run_pair(n, steps, sample_every, precision=sph2d.FP64, integrator=2):
+    prm = sph2d.tgv_params(n, integrator=integrator, sample_every=sample_every, precision=precision)
+    sim = sph2d.Simulation(prm)
+    sim.init_tgv()
+    o = Oracle(nx=n, ny=n, integrator=integrator, dt=prm.dt)
+    o.init_tgv()
+    for it in range(1, steps + 1):
+        sim.time_step(it)
+        o.time_step(it)
+    return sim, o
+
+
+def oracle_global(o, name, n):
+    """Local-order oracle field in global (gid) order, fluid particles only."""
+    loc = o.f64(name)
+    g = o.i32("name")
+    out = np.full(n, np.nan)
+    sel = g < n
+    out[g[sel]] = loc[sel]
+    return out
+
+
+@pytest.mark.parametrize("integrator,steps", [(2, 5), (2, 20), (3, 5)])
+def test_vorticity_and_analytic_fields_match_the_oracle(integrator, steps):
+    """Step 20 is an MLS step (its corrector is k_stage's); RK keeps the third
+    evaluation's pairs. FP64 bar: 1e-12 of the field's largest magnitude."""
+    say(f"test_vorticity_and_analytic_fields_match_the_oracle integrator={integrator} steps={steps}")
+    n = 48
+    sim, o = run_pair(n, steps, 5, integrator=integrator)
+    w, ana, step = sim.output_fields()
+    assert step == steps
+    ref = oracle_global(o, "vorticity", n * n)
+    err = np.max(np.abs(w - ref)) / np.max(np.abs(ref))
+    say(f"vorticity rel err {err:.3e}")
+    assert err < 1e-12
+    for k, f in enumerate(("ana_u", "ana_v", "ana_p", "ana_vm")):
+        r = o.f64(f)[: n * n]
+        assert np.max(np.abs(ana[:, k] - r)) <= 1e-14 * np.max(np.abs(r)), f
+    sim.close()
+
+
+def test_vorticity_mixed_form_within_its_tolerance():
+    say("test_vorticity_mixed_form_within_its_tolerance")
+    n = 48
+    sim, o = run_pair(n, 10, 10, precision=sph2d.FP32_MIXED)
+    w, _, _ = sim.output_fields()
+    ref = oracle_global(o, "vorticity", n * n)
+    err = np.max(np.abs(w - ref)) / np.max(np.abs(ref))
+    say(f"mixed vorticity rel err {err:.3e}")
+    assert err < 1e-4
+    sim.close()
+
+
+def test_output_fields_need_a_sampling_step():
+    say("test_output_fields_need_a_sampling_step")
+    prm = sph2d.tgv_params(32, sample_every=10)
+    sim = sph2d.Simulation(prm)
+    sim.init_tgv()
+    sim.run(1, 3)
+    with pytest.raises(sph2d.SphError) as e:
+        sim.output_fields()
+    assert e.value.code == 4
+    sim.run(4, 7)
+    with pytest.raises(sph2d.SphError):
+        sim.output_for_tecplot(9, video_path=os.devnull)        # 10 was the last step
+    sim.close()
+
+
+def test_tecplot_and_paraview_files_of_a_context(tmp_path):
+    say("test_tecplot_and_paraview_files_of_a_context")
+    n = 40
+    prm = sph2d.tgv_params(n, sample_every=10)
+    sim = sph2d.Simulation(prm)
+    sim.init_tgv()
+    sim.run(1, 10)
+    video, surf, binf = tmp_path / "video_result.dat", tmp_path / "Free_Surface.dat", tmp_path / "b.dat"
+    sim.output_for_tecplot(10, video, surf, binf)
+    sim.output_for_paraview(10, tmp_path / "flow.vtu")
+    st = sim.download_state()
+    w, ana, _ = sim.output_fields()
+    sc = sim.scales()
+    assert (sc.domain_size_y, sc.velocity_0, sc.pressure_0, sc.time_0) == (1.0, 1.0, 1000.0, 1.0)
+    ref = tmp_path / "ref.dat"
+    sph2d.write_tecplot_zone(ref, 10 * prm.dt, st["x"], st["y"], st["u"], st["v"], st["p"], w, ana, sc)
+    assert video.read_bytes() == ref.read_bytes()
+    lines = video.read_text().split("\n")
+    assert lines[2] == ' ZONE T=" %s ", I=%12d  F=POINT' % (sph2d.time_name(10 * prm.dt, 12, 5), n * n)
+    row0 = [float(lines[3][20 * k:20 * k + 20]) for k in range(11)]
+    assert abs(row0[6] - w[0]) < 1e-8 and abs(row0[0] - st["x"][0]) < 1e-8
+    assert surf.read_text().split("\n")[2].endswith("I=           0  F=POINT")   # no surface in the TGV
+    b = binf.read_bytes()
+    assert b[:8] == b"#!TDV101"
+    data = np.frombuffer(b[-48 * n * n:], dtype="<f8").reshape(n * n, 6)
+    assert np.array_equal(data[:, 0], st["x"]) and np.array_equal(data[:, 4], st["p"] / 1000.0)
+    vtu = (tmp_path / "flow.vtu").read_text()
+    assert 'NumberOfPoints="%12d "' % (n * n) in vtu
+    sim.close()
+
+
+def test_restart_from_the_latest_results_file(tmp_path):
+    """Write the restart zone at step 10, read it back (Distribute_Latest_Particle),
+    continue in a new context: the state is the F20.8-rounded one, and the run
+    stays within the restart's perturbation of the uninterrupted run (the restarted
+    first predictor has no previous rates, as in the reference)."""
+    say("test_restart_from_the_latest_results_file")
+    n = 40
+    prm = sph2d.tgv_params(n, sample_every=10)
+    a = sph2d.Simulation(prm)
+    a.init_tgv()
+    a.run(1, 10)
+    path = tmp_path / "Latest_Time_Results.dat"
+    a.output_latest_results(10, path)
+    st = a.download_state()
+    a.run(11, 10)
+    ke_a = a.sample()[0]
+    r = sph2d.read_latest_results(path)
+    assert r["step"] == 10 and len(r["x"]) == n * n
+    assert np.max(np.abs(r["x"] - st["x"])) <= 5e-9 and np.max(np.abs(r["rho"] - st["rho"])) <= 5e-9
+    b = sph2d.Simulation(prm)
+    b.upload_state(r["x"], r["y"], r["u"], r["v"], r["rho"], r["mass"], r["energy"], r["type"])
+    b.run(11, 10)
+    s = b.sample()
+    say(f"KE/KE0 uninterrupted {ke_a:.10f} restarted {s[0]:.10f}")
+    assert np.all(np.isfinite(s))
+    # KE0 of the restarted context is its own step-11 state: compare the kinetic energies
+    sa, sb = a.download_state(), b.download_state()
+    ea = np.sum(sa["mass"] * (sa["u"] ** 2 + sa["v"] ** 2))
+    eb = np.sum(sb["mass"] * (sb["u"] ** 2 + sb["v"] ** 2))
+    # the restarted first step's corrector adds only half an acceleration (zero
+    # previous rates): ~0.5 dt |a| / U ~ 4e-4 in v, ~1e-3 in the kinetic energy
+    assert abs(eb - ea) / ea < 5e-3
+    a.close()
+    b.close()

@@ -276,3 +276,24 @@ def test_ke_trajectory_matches_reference_run(gold):
         assert s[k] == pytest.approx(gold[name][0, 1], rel=0.05)       # Linf
         assert s[k + 2] == pytest.approx(gold[name][0, 3], rel=0.05)   # L2
     assert s[10] == pytest.approx(gold["norm_p"][0, 3], rel=0.05)
+
+
+# --- Refresh_domain's vorticity: the curl of the TGV field, second order --------------
+@pytest.mark.parametrize("n", [50])
+def test_vorticity_is_the_curl_of_the_tgv_field(n):
+    """Refresh_domain.f90:320-335 on the t=0 lattice: sum_j V_j (v_i - v_j) x grad W
+    estimates dv/dx - du/dy = 4 pi U sin(2 pi x) sin(2 pi y) for u = U sin cos,
+    v = -U cos sin. The error halves twice when dx halves (second order), which a
+    wrong sign, a swapped component or a missing volume would not do."""
+    errs = []
+    for m in (n, 2 * n):
+        o = evaluate(tgv(m))
+        o.refresh()
+        w = o.f64("vorticity")
+        fl = o.i32("type") == 2
+        x, y = o.f64("x")[fl], o.f64("y")[fl]
+        exact = 4 * np.pi * np.sin(2 * np.pi * x) * np.sin(2 * np.pi * y)
+        errs.append(np.max(np.abs(w[fl] - exact)) / (4 * np.pi))
+        o.close()
+    assert errs[1] < 0.01
+    assert 3.0 < errs[0] / errs[1] < 5.0, errs
