@@ -201,6 +201,22 @@ int sph2d_output_for_tecplot(sph2d_ctx* ctx, int32_t i_time_step, const char* vi
 /* Output_for_paraview.f90:188-263 for the current state (VTK XML, ASCII). */
 int sph2d_output_for_paraview(sph2d_ctx* ctx, int32_t i_time_step, const char* path);
 
+/* Post-processing grid (Distribute_Grid_For_Postproceeding.f90, information_module.f90:
+ * 176-188): element_number = int(box / dx) per direction, element_dx = box_x /
+ * element_number_x; node k = j + (i-1)*(element_number_x+1) (1-based) sits at
+ * ((j-1) element_dx, (i-1) element_dx). */
+int sph2d_grid_size(sph2d_ctx* ctx, int32_t* element_number_x, int32_t* element_number_y, double* element_dx);
+/* Calculation_For_PostProceeding.f90:96-300 after the last step (which must be a
+ * sampling step): order-6 MLS interpolation of velocity (ld,2), pressure and
+ * vorticity onto the nodes, the level-set value Fai, and the analytic u, v, p, |v|
+ * at the nodes as columns of ana (ld,4). Any output may be NULL. */
+int sph2d_calculation_for_postproceeding(sph2d_ctx* ctx, int32_t i_time_step, int32_t n_nodes, double* vel,
+                                         double* press, double* vort_z, double* fai, double* ana, int32_t ld);
+/* Output_for_Grid_PostProceding.f90: a ZONE appended to grid_path
+ * (Post_Proceeding_Grid.dat) and the #!TDV101 grid file binary_path (NULL skips). */
+int sph2d_output_for_grid_postproceding(sph2d_ctx* ctx, int32_t i_time_step, const char* grid_path,
+                                        const char* binary_path);
+
 /* Host-array writers (no device; Fortran layout, ld >= n). Text follows the
  * reference's edit descriptors exactly; files named "appended" get the
  * reference's header lines when they are new or empty. */
@@ -234,6 +250,18 @@ int sph2d_write_latest_results_zone(const char* path, int32_t i_time_step, int32
 int sph2d_read_latest_results(const char* path, int32_t zone, int32_t capacity, int32_t* n_out, double* pos,
                               double* vel, double* press, double* rho, double* mass, double* energy, double* c,
                               int32_t* initial_type, int32_t* wavemaker_layer, int32_t* step_out, int32_t ld);
+/* Distribute_Grid_For_Postproceeding.f90:60-95 (either path may be NULL). */
+int sph2d_write_initial_grid(const char* grid_path, const char* node_path, int32_t element_number_x,
+                             int32_t element_number_y, double element_dx);
+/* Output_for_Grid_PostProceding.f90:110-138 (appended ZONE + connectivity). */
+int sph2d_write_grid_zone(const char* path, double current_time, int32_t element_number_x,
+                          int32_t element_number_y, double element_dx, const double* vel, const double* press,
+                          const double* vort_z, const double* fai, const double* ana, int32_t ld,
+                          const sph2d_scales* scales);
+/* Output_for_Grid_PostProceding.f90:140-291 (#!TDV101, X Y VX VY P U Fai). */
+int sph2d_write_grid_binary(const char* path, double time, int32_t element_number_x, int32_t element_number_y,
+                            double element_dx, const double* vel, const double* press, const double* fai,
+                            int32_t ld, const sph2d_scales* scales);
 /* Sampling_subroutine.f90:360-461 (format 20F20.8): appends one line to each of the
  * six history files in dir from sph2d_sample's 14 values. */
 int sph2d_write_sampling(const char* dir, double nondim_time, const double* out14);

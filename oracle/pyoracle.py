@@ -70,6 +70,8 @@ def lib():
         for name in ("orc_devide_block", "orc_single_step_compute", "orc_time_step"):
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_int32]
         L.orc_sample.argtypes = [ctypes.c_void_p, P(ctypes.c_double)]
+        L.orc_grid_postprocess.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_double, P(ctypes.c_double)]
         L.orc_initial_ke.argtypes = [ctypes.c_void_p]
         L.orc_initial_ke.restype = ctypes.c_double
         for name in ("orc_n_fluid", "orc_n_global", "orc_n_local"):
@@ -171,6 +173,14 @@ class Oracle:
     def refresh(self):
         lib().orc_refresh(self._c)
 
+    def grid_postprocess(self, it: int, nex: int, ney: int, edx: float) -> np.ndarray:
+        """Calculation_For_PostProceeding on the element nodes: (n_nodes, 9) columns
+        u, v, p, vorticity_z, Fai, analytic u, v, p, |v|."""
+        nn = (nex + 1) * (ney + 1)
+        out = np.zeros(9 * nn)
+        lib().orc_grid_postprocess(self._c, it, nex, ney, edx, _dp(out))
+        return out.reshape(9, nn).T.copy()
+
     def sample(self) -> np.ndarray:
         out = np.zeros(14)
         lib().orc_sample(self._c, _dp(out))
@@ -214,6 +224,12 @@ class Oracle:
     def set_f64(self, name: str, values) -> None:
         n = ctypes.c_int64()
         ptr = lib().orc_mut_f64(self._c, name.encode(), ctypes.byref(n))
+        arr = np.ctypeslib.as_array(ptr, shape=(n.value,))
+        arr[:] = values
+
+    def set_i32(self, name: str, values) -> None:
+        n = ctypes.c_int64()
+        ptr = lib().orc_mut_i32(self._c, name.encode(), ctypes.byref(n))
         arr = np.ctypeslib.as_array(ptr, shape=(n.value,))
         arr[:] = values
 

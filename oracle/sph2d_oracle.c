@@ -63,8 +63,12 @@ struct orc_ctx {
   double *gpx, *gpy, *gux, *guy, *gvx, *gvy;    /* MUSCL: grad_pressure, grad_velocity_u/_v */
   /* shift */
   double *shx, *shy;
-  /* Refresh_domain's vorticity (z), local order */
-  double *vort;
+  /* Refresh_domain's vorticity (z), local order; global vorticity, surface flags and
+   * normals after the refresh */
+  double *vort, *gvort, *gnx, *gny;
+  int *gsurf;
+  /* particle_in_chain of the step start: row-major cell k holds gchain_members[off[k] ..] */
+  int *gchain_off, *gchain_members, gchain_ncell;
   /* time integration (local index arrays, persist across steps) */
   double *b_rho, *b_e, *b_x, *b_y, *b_u, *b_v;
   double *t_u, *t_v, *t_drho, *t_de, *t_ax, *t_ay;              /* PC temp_* */
@@ -251,9 +255,10 @@ orc_ctx* orc_create(const orc_params* p) {
   c->n_fluid = 0;
 #define A(f) c->f = xcalloc(c->cap_global, sizeof(double))
   A(gx); A(gy); A(gu); A(gv); A(gmass); A(grho); A(gh); A(gc); A(gp); A(ge);
-  A(ana_u); A(ana_v); A(ana_p); A(ana_vm);
+  A(ana_u); A(ana_v); A(ana_p); A(ana_vm); A(gvort); A(gnx); A(gny);
 #undef A
   c->gtype = xcalloc(c->cap_global, sizeof(int));
+  c->gsurf = xcalloc(c->cap_global, sizeof(int));
   alloc_local(c, c->cap_global);
   return c;
 }
@@ -261,7 +266,8 @@ orc_ctx* orc_create(const orc_params* p) {
 void orc_destroy(orc_ctx* c) {
   if (!c) return;
   void* ptrs[] = {c->gx, c->gy, c->gu, c->gv, c->gmass, c->grho, c->gh, c->gc, c->gp, c->ge,
-                  c->ana_u, c->ana_v, c->ana_p, c->ana_vm, c->gtype,
+                  c->ana_u, c->ana_v, c->ana_p, c->ana_vm, c->gtype, c->gvort, c->gnx, c->gny, c->gsurf,
+                  c->gchain_off, c->gchain_members,
                   c->name, c->x, c->y, c->u, c->v, c->mass, c->rho, c->hh, c->c, c->p, c->e,
                   c->vol, c->type, c->chain_no, c->cand_n, c->cand, c->fra_v, c->sum_w, c->sdwx,
                   c->sdwy, c->T, c->Tinv, c->nrmx, c->nrmy, c->surf, c->near, c->Sxx, c->Sxy,
@@ -385,6 +391,43 @@ void orc_period_boundary(orc_ctx* c) {
 /* Devide_block_and_send_tasks.f90 on one rank:                       */
 /*  analytic fields (:88-116), global binning (:123-162), local order */
 /*  column-major by cell then ascending gid (:488-633), copy (:666-)  */
+/* Distribute_Backgroud_Grid.f90:102-200: the near cells of row-major cell i (1-based)
+ * of a gx_n x gy_n grid, in the order self, L, R, UL, U, UR, DL, D, DR (edges 6,
+ * corners 4); returns their number. */
+static int stencil_names(int i, int gx_n, int gy_n, int* nm) {
+  int mx = (i - 1) % gx_n + 1, my = (i - 1) / gx_n + 1;
+  if (mx > 1 && mx < gx_n && my > 1 && my < gy_n) {
+    nm[0] = i; nm[1] = i - 1; nm[2] = i + 1; nm[3] = i - 1 + gx_n; nm[4] = i + gx_n;
+    nm[5] = i + 1 + gx_n; nm[6] = i - 1 - gx_n; nm[7] = i - gx_n; nm[8] = i + 1 - gx_n;
+    return 9;
+  } else if (mx == 1 && my > 1 && my < gy_n) {
+    nm[0] = i; nm[1] = i + 1; nm[2] = i + gx_n; nm[3] = i + 1 + gx_n; nm[4] = i - gx_n; nm[5] = i + 1 - gx_n;
+    return 6;
+  } else if (mx == gx_n && my > 1 && my < gy_n) {
+    nm[0] = i; nm[1] = i - 1; nm[2] = i + gx_n; nm[3] = i - 1 + gx_n; nm[4] = i - gx_n; nm[5] = i - 1 - gx_n;
+    return 6;
+  } else if (my == gy_n && mx > 1 && mx < gx_n) {
+    nm[0] = i; nm[1] = i - 1; nm[2] = i + 1; nm[3] = i - 1 - gx_n; nm[4] = i - gx_n; nm[5] = i + 1 - gx_n;
+    return 6;
+  } else if (my == 1 && mx > 1 && mx < gx_n) {
+    nm[0] = i; nm[1] = i - 1; nm[2] = i + 1; nm[3] = i - 1 + gx_n; nm[4] = i + gx_n; nm[5] = i + 1 + gx_n;
+    return 6;
+  } else if (mx == 1 && my == gy_n) {
+    nm[0] = i; nm[1] = i + 1; nm[2] = i - gx_n; nm[3] = i + 1 - gx_n;
+    return 4;
+  } else if (mx == gx_n && my == gy_n) {
+    nm[0] = i; nm[1] = i - 1; nm[2] = i - gx_n; nm[3] = i - 1 - gx_n;
+    return 4;
+  } else if (mx == 1 && my == 1) {
+    nm[0] = i; nm[1] = i + 1; nm[2] = i + gx_n; nm[3] = i + 1 + gx_n;
+    return 4;
+  } else if (mx == gx_n && my == 1) {
+    nm[0] = i; nm[1] = i - 1; nm[2] = i + gx_n; nm[3] = i - 1 + gx_n;
+    return 4;
+  }
+  return 0;
+}
+
 /* ------------------------------------------------------------------ */
 void orc_devide_block(orc_ctx* c, int32_t i_time_step) {
   const orc_params* P = &c->P;
@@ -462,37 +505,13 @@ void orc_devide_block(orc_ctx* c, int32_t i_time_step) {
   c->cell_list = xcalloc((size_t)(c->sub_ncell + 1) * 300, sizeof(int));
   c->near_n = xcalloc(c->sub_ncell + 1, sizeof(int));
   c->near_name = xcalloc((size_t)(c->sub_ncell + 1) * 9, sizeof(int));
-  int gx_n = c->sub_nx, gy_n = c->sub_ny;
-  for (int i = 1; i <= c->sub_ncell; ++i) {
-    int mx = (i - 1) % gx_n + 1, my = (i - 1) / gx_n + 1;
-    int* nm = &c->near_name[i * 9];
-    int* nn = &c->near_n[i];
-    if (mx > 1 && mx < gx_n && my > 1 && my < gy_n) {
-      *nn = 9; nm[0] = i; nm[1] = i - 1; nm[2] = i + 1; nm[3] = i - 1 + gx_n; nm[4] = i + gx_n;
-      nm[5] = i + 1 + gx_n; nm[6] = i - 1 - gx_n; nm[7] = i - gx_n; nm[8] = i + 1 - gx_n;
-    } else if (mx == 1 && my > 1 && my < gy_n) {
-      *nn = 6; nm[0] = i; nm[1] = i + 1; nm[2] = i + gx_n; nm[3] = i + 1 + gx_n; nm[4] = i - gx_n; nm[5] = i + 1 - gx_n;
-    } else if (mx == gx_n && my > 1 && my < gy_n) {
-      *nn = 6; nm[0] = i; nm[1] = i - 1; nm[2] = i + gx_n; nm[3] = i - 1 + gx_n; nm[4] = i - gx_n; nm[5] = i - 1 - gx_n;
-    } else if (my == gy_n && mx > 1 && mx < gx_n) {
-      *nn = 6; nm[0] = i; nm[1] = i - 1; nm[2] = i + 1; nm[3] = i - 1 - gx_n; nm[4] = i - gx_n; nm[5] = i + 1 - gx_n;
-    } else if (my == 1 && mx > 1 && mx < gx_n) {
-      *nn = 6; nm[0] = i; nm[1] = i - 1; nm[2] = i + 1; nm[3] = i - 1 + gx_n; nm[4] = i + gx_n; nm[5] = i + 1 + gx_n;
-    } else if (mx == 1 && my == gy_n) {
-      *nn = 4; nm[0] = i; nm[1] = i + 1; nm[2] = i - gx_n; nm[3] = i + 1 - gx_n;
-    } else if (mx == gx_n && my == gy_n) {
-      *nn = 4; nm[0] = i; nm[1] = i - 1; nm[2] = i - gx_n; nm[3] = i - 1 - gx_n;
-    } else if (mx == 1 && my == 1) {
-      *nn = 4; nm[0] = i; nm[1] = i + 1; nm[2] = i + gx_n; nm[3] = i + 1 + gx_n;
-    } else if (mx == gx_n && my == 1) {
-      *nn = 4; nm[0] = i; nm[1] = i - 1; nm[2] = i + gx_n; nm[3] = i - 1 + gx_n;
-    } else {
-      *nn = 0;
-    }
-  }
+  for (int i = 1; i <= c->sub_ncell; ++i) c->near_n[i] = stencil_names(i, c->sub_nx, c->sub_ny, &c->near_name[i * 9]);
   /* the subdomain grid origin (:330-333) */
   c->sub_start = start;
-  free(cnt); free(cellof); free(colcnt); free(off); free(members); free(fill);
+  /* the global binning (particle_in_chain, :144-148) stays for the post-processing */
+  free(c->gchain_off); free(c->gchain_members);
+  c->gchain_off = off; c->gchain_members = members; c->gchain_ncell = ncell;
+  free(cnt); free(cellof); free(colcnt); free(fill);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1168,6 +1187,15 @@ static void refresh_vorticity(orc_ctx* c) {
 /* Refresh_domain.f90:136-300 on one rank: global arrays zeroed, then filled. */
 void orc_refresh(orc_ctx* c) {
   refresh_vorticity(c);
+  {
+    int NG0 = c->n_fluid + c->n_ghost;
+    memset(c->gvort, 0, sizeof(double) * NG0); memset(c->gnx, 0, sizeof(double) * NG0);
+    memset(c->gny, 0, sizeof(double) * NG0); memset(c->gsurf, 0, sizeof(int) * NG0);
+    for (int k = 0; k < c->n_local; ++k) {          /* :300-351 particle_vorticity, Normal_vector */
+      int g = c->name[k];
+      c->gvort[g] = c->vort[k]; c->gnx[g] = c->nrmx[k]; c->gny[g] = c->nrmy[k]; c->gsurf[g] = c->surf[k];
+    }
+  }
   int NG = c->n_fluid + c->n_ghost;
   size_t nb = sizeof(double) * NG;
   memset(c->gx, 0, nb); memset(c->gy, 0, nb); memset(c->gu, 0, nb); memset(c->gv, 0, nb);
@@ -1364,7 +1392,7 @@ static const fld F64[] = {
   FL("sps_x", sps_x, 0), FL("sps_y", sps_y, 0), FL("lap_x", lap_x, 0), FL("lap_y", lap_y, 0),
   FL("avg_vel_x", avg_x, 0), FL("avg_vel_y", avg_y, 0), FL("rho_correct", corr, 0),
   FL("grad_rho_x", grx, 0), FL("grad_rho_y", gry, 0), FL("shift_x", shx, 0),
-  FL("shift_y", shy, 0), FL("vorticity", vort, 0),
+  FL("shift_y", shy, 0), FL("vorticity", vort, 0), FL("g_vorticity", gvort, 1), FL("g_normal_x", gnx, 1), FL("g_normal_y", gny, 1),
   FL("grad_p_x", gpx, 0), FL("grad_p_y", gpy, 0), FL("grad_u_x", gux, 0), FL("grad_u_y", guy, 0),
   FL("grad_v_x", gvx, 0), FL("grad_v_y", gvy, 0),
   FL("T", T, 3), FL("T_inv", Tinv, 3),
@@ -1376,7 +1404,7 @@ static const fld F64[] = {
 };
 static const fld I32[] = {
   FL("type", type, 0), FL("name", name, 0), FL("surface", surf, 0), FL("near_surface", near, 0),
-  FL("candidate_n", cand_n, 0), FL("g_type", gtype, 1), FL("pair_i", pi, 2), FL("pair_j", pj, 2),
+  FL("candidate_n", cand_n, 0), FL("g_type", gtype, 1), FL("g_surface", gsurf, 1), FL("pair_i", pi, 2), FL("pair_j", pj, 2),
 };
 static int64_t fld_count(const orc_ctx* c, int kind) {
   switch (kind) {
@@ -1458,4 +1486,126 @@ void orc_reorder_particle_index(int32_t n_total, int32_t n_fluid, double box_x, 
   counts[5] = k;
   free(tf);
   free(ti);
+}
+
+
+/* ------------------------------------------------------------------ */
+/* Calculation_For_PostProceeding.f90:96-300 (one rank) on the element nodes of  */
+/* Distribute_Grid_For_Postproceeding.f90:30-39 (node k = j + (i-1) nnx at       */
+/* ((j-1) edx, (i-1) edx)), with MLS_Interpolation_In_Gobal_Domain.f90 order 6.  */
+/* Uses the step-start binning (particle_in_chain) and the refreshed global       */
+/* arrays, as the reference does after Refresh_domain. out: 9 blocks of n_nodes:  */
+/* u, v, p, vorticity_z, Fai, analytic u, v, p, |v| (Devide_block...:104-115).    */
+/* ------------------------------------------------------------------ */
+void orc_grid_postprocess(orc_ctx* c, int32_t i_time_step, int32_t nex, int32_t ney, double edx, double* out) {
+  const orc_params* P = &c->P;
+  const int nnx = nex + 1, nny = ney + 1, nn = nnx * nny, N = c->n_fluid;
+  const double h = c->h, pi = PI_MODULE;
+  const double t = i_time_step * P->dt / (P->box_x / P->U);
+  const double Re = P->U * P->box_x / P->nu;
+  const double al = P->mls_alpha;
+  memset(out, 0, sizeof(double) * 9 * (size_t)nn);
+  /* :96-102 surface particles in gid order */
+  int ns = 0;
+  int* sname = xcalloc(N + 1, sizeof(int));
+  for (int i = 0; i < N; ++i) if (c->gsurf[i] == 1) sname[ns++] = i;
+  int* eff = xcalloc(300, sizeof(int));
+  int* tn = xcalloc(300, sizeof(int));
+  double* wk = xcalloc(300, sizeof(double));
+  double* pm = xcalloc(6 * 306, sizeof(double));       /* P(m,k), column k */
+  double* fai = xcalloc(300, sizeof(double));
+  for (int k = 0; k < nn; ++k) {
+    const int j0 = k % nnx, i0 = k / nnx;
+    const double ex = j0 * edx, ey = i0 * edx;          /* both coordinates use element_dx */
+    out[5 * (size_t)nn + k] = P->U * sin(2 * pi * ex) * cos(2 * pi * ey) * exp(-8 * (pi * pi) / Re * t);
+    out[6 * (size_t)nn + k] = -P->U * cos(2 * pi * ex) * sin(2 * pi * ey) * exp(-8 * (pi * pi) / Re * t);
+    out[7 * (size_t)nn + k] = 0.25 * P->rho0 * (P->U * P->U) * (cos(4 * pi * ex) + cos(4 * pi * ey)) *
+                              exp(-16 * (pi * pi) / Re * t);
+    out[8 * (size_t)nn + k] = sqrt(out[5 * (size_t)nn + k] * out[5 * (size_t)nn + k] +
+                                   out[6 * (size_t)nn + k] * out[6 * (size_t)nn + k]);
+    /* :125-170 level set from the nearest surface particle */
+    double f = 1.0;
+    if (ns > 0) {
+      double best = 1000000000.0, dfx = 0.0, dfy = 0.0;
+      int idx = -1;
+      for (int q = 0; q < ns; ++q) {
+        int jj = sname[q];
+        double pdx = ex - c->gx[jj], pdy = ey - c->gy[jj];
+        double d = sqrt(pdx * pdx + pdy * pdy);
+        if (d <= best) { best = d; idx = jj; dfx = -pdx; dfy = -pdy; }
+      }
+      double det = dfx * c->gnx[idx] + dfy * c->gny[idx];
+      if (det <= -2 * h) f = -1.0;
+      else if (det >= 2 * h) f = 1.0;
+      else f = det / (2 * h);
+    }
+    out[4 * (size_t)nn + k] = f;
+    if (f <= -0.9) continue;
+    /* :184-215 candidates: the 9-cell stencil of the node's cell, fluid particles only */
+    int xn = (int)floor((ex - c->chain_ox) / c->chain_d + 1);
+    int yn = (int)floor((ey - c->chain_oy) / c->chain_d + 1);
+    int mesh = (yn - 1) * c->chain_nx + xn;
+    int nm[9];
+    int nnear = stencil_names(mesh, c->chain_nx, c->chain_ny, nm);
+    int ne = 0;
+    for (int q = 0; q < nnear; ++q) {
+      int cell = nm[q];
+      for (int m = c->gchain_off[cell]; m < c->gchain_off[cell + 1]; ++m) {
+        int jj = c->gchain_members[m];
+        if (jj < N && ne < 300) eff[ne++] = jj;
+      }
+    }
+    /* :219-250 the particles in the support, with W */
+    int nt = 0;
+    for (int q = 0; q < ne; ++q) {
+      int jj = eff[q];
+      double pd[2] = {ex - c->gx[jj], ey - c->gy[jj]};
+      double d = sqrt(pd[0] * pd[0] + pd[1] * pd[1]);
+      if (d <= 3.0 * h && d >= (double)1.0e-7f) {
+        double w, dw[2];
+        orc_compute_kernel(2, d, pd, h, &w, dw);
+        tn[nt] = jj; wk[nt] = w; ++nt;
+      }
+    }
+    if (nt < 3) continue;
+    /* MLS_Interpolation_In_Gobal_Domain.f90:110-250, order 6 */
+    for (int q = 0; q < nt; ++q) {
+      int jj = tn[q];
+      double a = (c->gx[jj] - ex) / h, b = (c->gy[jj] - ey) / h;
+      double* col = &pm[6 * q];
+      col[0] = 1; col[1] = a; col[2] = b; col[3] = a * a; col[4] = a * b; col[5] = b * b;
+    }
+    double A[36];                                       /* column-major 6x6 */
+    for (int m = 0; m < 6; ++m)
+      for (int l = 0; l < 6; ++l) {
+        double v = 0.0;
+        for (int q = 0; q < nt; ++q) v = v + (pm[6 * q + m] * wk[q]) * pm[6 * q + l];
+        for (int L = 0; L < 6; ++L) v = v + ((m == L ? al : 0.0) * al) * (l == L ? al : 0.0);
+        A[m + 6 * l] = v;
+      }
+    int ok = orc_brinv(A, 6);
+    double sw = 0.0;
+    for (int q = 0; q < nt; ++q) sw = sw + wk[q];
+    for (int q = 0; q < nt; ++q) {
+      if (ok) {
+        double v = 0.0;
+        for (int m = 0; m < 6; ++m) v = v + A[0 + 6 * m] * (pm[6 * q + m] * wk[q]);
+        fai[q] = v;
+      } else {
+        fai[q] = wk[q] / sw;
+      }
+      if (nt <= 10) fai[q] = wk[q] / sw;
+    }
+    double su = 0.0, sv = 0.0, sp = 0.0, so = 0.0;
+    for (int q = 0; q < nt; ++q) {
+      int jj = tn[q];
+      sp = sp + c->gp[jj] * fai[q];
+      su = su + c->gu[jj] * fai[q];
+      sv = sv + c->gv[jj] * fai[q];
+      so = so + c->gvort[jj] * fai[q];
+    }
+    out[0 * (size_t)nn + k] = su; out[1 * (size_t)nn + k] = sv;
+    out[2 * (size_t)nn + k] = sp; out[3 * (size_t)nn + k] = so;
+  }
+  free(sname); free(eff); free(tn); free(wk); free(pm); free(fai);
 }

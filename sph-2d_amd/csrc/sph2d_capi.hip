@@ -53,6 +53,12 @@ struct sph2d_ctx {
   std::vector<void*> allocs;
   // in-step pass timing: 4 events per evaluation (pass A start/end, force pass start/end)
   int ana_step = 0, vort_step = 0;   // last steps whose analytic fields / vorticity are held
+  // post-processing grid (allocated on first use)
+  double* grid_out = nullptr;
+  size_t grid_cap = 0;
+  int* surf_gid = nullptr;
+  double2* surf_nrm = nullptr;
+  size_t surf_cap = 0;
   bool pass_timing = false;
   std::vector<hipEvent_t> events;
   size_t events_used = 0;
@@ -785,7 +791,8 @@ int sph2d_download_output_fields(sph2d_ctx* c, int32_t n, double* vort_z, double
 
 // Surface particles of the last evaluation in gid order (Calculation_For_PostProceeding.f90:96-102
 // builds surface_particle_name this way) with their positions and normals.
-static int surface_particles(sph2d_ctx* c, std::vector<double>& pos, std::vector<double>& nrm) {
+static int surface_particles(sph2d_ctx* c, std::vector<double>& pos, std::vector<double>& nrm,
+                             std::vector<int>* gids = nullptr) {
   int cnt[C_COUNT];
   int rc = read_counters(c, cnt);
   if (rc) return rc;
@@ -806,6 +813,10 @@ static int surface_particles(sph2d_ctx* c, std::vector<double>& pos, std::vector
   HIP_OR_FAIL(hipStreamSynchronize(c->stream));
   pos.assign((size_t)2 * m, 0.0);
   nrm.assign((size_t)2 * m, 0.0);
+  if (gids) {
+    gids->resize(m);
+    for (int k = 0; k < m; ++k) (*gids)[k] = sel[k].first;
+  }
   for (int k = 0; k < m; ++k) {
     pos[k] = gp[sel[k].first].x; pos[m + k] = gp[sel[k].first].y;
     nrm[k] = ln[sel[k].second].x; nrm[m + k] = ln[sel[k].second].y;
@@ -844,6 +855,102 @@ int sph2d_output_for_tecplot(sph2d_ctx* c, int32_t it, const char* video_path, c
   }
   if (binary_path) {
     rc = sph2d_write_tecplot_binary(binary_path, t, n, pos.data(), vel.data(), press.data(), n, &sc);
+    if (rc) return rc;
+  }
+  return SPH2D_OK;
+}
+
+int sph2d_grid_size(sph2d_ctx* c, int32_t* element_number_x, int32_t* element_number_y, double* element_dx) {
+  if (!c || !element_number_x || !element_number_y || !element_dx) return fail(SPH2D_ERR_ARGUMENT, "NULL argument");
+  // information_module.f90:179-183 with Factor_Grid_Point = 1 and the box as boundary_size
+  const int nex = (int)(c->prm.box_x / (1.0 * c->prm.dx));
+  const int ney = (int)(c->prm.box_y / (1.0 * c->prm.dx));
+  if (nex < 1 || ney < 1) return fail(SPH2D_ERR_ARGUMENT, "box smaller than one particle spacing");
+  *element_number_x = nex;
+  *element_number_y = ney;
+  *element_dx = c->prm.box_x / nex;
+  return SPH2D_OK;
+}
+
+int sph2d_calculation_for_postproceeding(sph2d_ctx* c, int32_t it, int32_t n_nodes, double* vel, double* press,
+                                         double* vort_z, double* fai, double* ana, int32_t ld) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  int32_t nex, ney;
+  double edx;
+  rc = sph2d_grid_size(c, &nex, &ney, &edx);
+  if (rc) return rc;
+  const int nnx = nex + 1, nny = ney + 1, nn = nnx * nny;
+  if (n_nodes != nn) return fail(SPH2D_ERR_ARGUMENT, "n_nodes must be (element_number_x+1)*(element_number_y+1) = " +
+                                                        std::to_string(nn));
+  if (ld < nn) return fail(SPH2D_ERR_ARGUMENT, "ld < n_nodes");
+  if (it != c->vort_step || it != c->ana_step)
+    return fail(SPH2D_ERR_STATE, "i_time_step must be the last step run and a multiple of sample_every");
+  std::vector<double> sp, sn;
+  std::vector<int> sg;                               // surface_particle_name (gid order)
+  rc = surface_particles(c, sp, sn, &sg);
+  if (rc) return rc;
+  const int ns = (int)sg.size();
+  if ((size_t)ns > c->surf_cap) {
+    rc = c->alloc(&c->surf_gid, ns);
+    if (!rc) rc = c->alloc(&c->surf_nrm, ns);
+    if (rc) return rc;
+    c->surf_cap = ns;
+  }
+  if (ns > 0) {
+    std::vector<double2> nr(ns);
+    for (int q = 0; q < ns; ++q) nr[q] = make_double2(sn[q], sn[ns + q]);
+    HIP_OR_FAIL(hipMemcpyAsync(c->surf_gid, sg.data(), ns * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(c->surf_nrm, nr.data(), ns * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  }
+  if ((size_t)9 * nn > c->grid_cap) {
+    rc = c->alloc(&c->grid_out, (size_t)9 * nn);
+    if (rc) return rc;
+    c->grid_cap = (size_t)9 * nn;
+  }
+  GridArgs G;
+  G.nnx = nnx; G.nny = nny; G.n_nodes = nn; G.n_surf = ns;
+  G.edx = edx;
+  G.t_nondim = it * c->prm.dt / (c->prm.box_x / c->prm.U);
+  G.Re = c->prm.U * c->prm.box_x / c->prm.nu;
+  G.surf_gid = c->surf_gid; G.surf_nrm = c->surf_nrm; G.out = c->grid_out;
+  launch_grid_mls(c->stream, c->P, c->B, G);
+  HIP_OR_FAIL(hipGetLastError());
+  std::vector<double> o((size_t)9 * nn);
+  HIP_OR_FAIL(hipMemcpyAsync(o.data(), c->grid_out, o.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  const size_t N = nn, L = ld;
+  for (size_t k = 0; k < N; ++k) {
+    if (vel) { vel[k] = o[k]; vel[L + k] = o[N + k]; }
+    if (press) press[k] = o[2 * N + k];
+    if (vort_z) vort_z[k] = o[3 * N + k];
+    if (fai) fai[k] = o[4 * N + k];
+    if (ana)
+      for (int q = 0; q < 4; ++q) ana[q * L + k] = o[(5 + q) * N + k];
+  }
+  return SPH2D_OK;
+}
+
+int sph2d_output_for_grid_postproceding(sph2d_ctx* c, int32_t it, const char* grid_path, const char* binary_path) {
+  int32_t nex, ney;
+  double edx;
+  int rc = sph2d_grid_size(c, &nex, &ney, &edx);
+  if (rc) return rc;
+  const int nn = (nex + 1) * (ney + 1);
+  std::vector<double> vel((size_t)2 * nn), press(nn), vort(nn), fai(nn), ana((size_t)4 * nn);
+  rc = sph2d_calculation_for_postproceeding(c, it, nn, vel.data(), press.data(), vort.data(), fai.data(), ana.data(),
+                                            nn);
+  if (rc) return rc;
+  sph2d_scales sc;
+  sph2d_get_scales(c, &sc);
+  const double t = it * c->prm.dt;
+  if (grid_path) {
+    rc = sph2d_write_grid_zone(grid_path, t, nex, ney, edx, vel.data(), press.data(), vort.data(), fai.data(),
+                               ana.data(), nn, &sc);
+    if (rc) return rc;
+  }
+  if (binary_path) {
+    rc = sph2d_write_grid_binary(binary_path, t, nex, ney, edx, vel.data(), press.data(), fai.data(), nn, &sc);
     if (rc) return rc;
   }
   return SPH2D_OK;

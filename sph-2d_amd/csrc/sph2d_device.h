@@ -57,6 +57,53 @@ __device__ __forceinline__ int xcd_block() {
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 
+// Row 1 of the inverse of the order-6 MLS moment matrix, y = (A + al3 I)^-1 e_1, with
+// A symmetric positive semi-definite and given by its upper triangle, row-major over
+// (m <= l): a00 a01 .. a05 a11 .. a55. The reference inverts with the full-pivot
+// Gauss-Jordan BRINV (function_module.f90:568-639) and keeps row 1; a Cholesky
+// solve in registers gives the same row with another rounding (relative ~ cond * eps).
+// Returns false when a pivot is not positive.
+__device__ __forceinline__ bool mls6_first_row(const double* A, double al3, double* y) {
+  double L[21];
+  bool ok = true;
+  const int diag[6] = {0, 6, 11, 15, 18, 20};
+#define AU(m, l) A[diag[(m)] + (l) - (m)]          // upper-triangle access, m <= l
+#define LL(i, j) L[(i) * ((i) + 1) / 2 + (j)]      // lower-triangle access, j <= i
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      double v = AU(j, i) + (i == j ? al3 : 0.0);
+#pragma unroll
+      for (int k = 0; k < j; ++k) v = v - LL(i, k) * LL(j, k);
+      if (i == j) {
+        ok = ok && (v > 0.0);
+        LL(i, i) = sqrt(v > 0.0 ? v : 1.0);
+      } else {
+        LL(i, j) = v / LL(j, j);
+      }
+    }
+  }
+  double z[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double v = (i == 0) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < i; ++k) v = v - LL(i, k) * z[k];
+    z[i] = v / LL(i, i);
+  }
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {
+    double v = z[i];
+#pragma unroll
+    for (int k = i + 1; k < 6; ++k) v = v - LL(k, i) * y[k];
+    y[i] = v / LL(i, i);
+  }
+#undef AU
+#undef LL
+  return ok;
+}
+
 // Neighbour cells of a particle: 3 column ranges of the column-major cell table.
 struct CellRanges {
   int lo[3], hi[3];

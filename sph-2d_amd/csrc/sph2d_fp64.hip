@@ -713,43 +713,8 @@ __global__ void __launch_bounds__(kBlock) k_mls(DevParams P, Buffers B) {
   double rho = 0.0;
   bool ok = ne > 8;
   if (ok) {
-    // Cholesky A = L L^T (L lower, row-major packed), then L z = e1, L^T y = z.
-    double L[21];
-    const int diag[6] = {0, 6, 11, 15, 18, 20};
-#define AU(m, l) A[diag[(m)] + (l) - (m)]          // upper-triangle access, m <= l
-#define LL(i, j) L[(i) * ((i) + 1) / 2 + (j)]      // lower-triangle access, j <= i
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-#pragma unroll
-      for (int j = 0; j <= i; ++j) {
-        double v = AU(j, i) + (i == j ? al3 : 0.0);
-#pragma unroll
-        for (int k = 0; k < j; ++k) v = v - LL(i, k) * LL(j, k);
-        if (i == j) {
-          ok = ok && (v > 0.0);
-          LL(i, i) = sqrt(v > 0.0 ? v : 1.0);
-        } else {
-          LL(i, j) = v / LL(j, j);
-        }
-      }
-    }
-    double z[6], y[6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      double v = (i == 0) ? 1.0 : 0.0;
-#pragma unroll
-      for (int k = 0; k < i; ++k) v = v - LL(i, k) * z[k];
-      z[i] = v / LL(i, i);
-    }
-#pragma unroll
-    for (int i = 5; i >= 0; --i) {
-      double v = z[i];
-#pragma unroll
-      for (int k = i + 1; k < 6; ++k) v = v - LL(k, i) * y[k];
-      y[i] = v / LL(i, i);
-    }
-#undef AU
-#undef LL
+    double y[6];
+    ok = mls6_first_row(A, al3, y);
 #pragma unroll
     for (int l = 0; l < 6; ++l) rho = rho + y[l] * bv[l];
   }
@@ -762,6 +727,126 @@ __global__ void __launch_bounds__(kBlock) k_mls(DevParams P, Buffers B) {
 
 void launch_mls(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
   hipLaunchKernelGGL(k_mls, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B);
+}
+
+// ---------------------------------------------------------------------------
+// Calculation_For_PostProceeding.f90:96-300 on the element nodes of
+// Distribute_Grid_For_Postproceeding.f90:30-39 (node k = j + (i-1) nnx at
+// ((j-1) edx, (i-1) edx): both coordinates use element_dx, as there), after the
+// step's refresh. Candidates are the fluid particles (gid < N) of the node's 3x3
+// cells in the step-start binning, as the reference's particle_in_chain (each
+// particle is within h/2 of its binned cell, so the stencil holds every particle
+// within 3h); positions, velocity, pressure and vorticity are the refreshed global
+// ones. MLS_Interpolation_In_Gobal_Domain.f90, order 6: Fai_k = w_k (A^-1 p_k)_1,
+// Shepard weights w_k / sum w when 10 or fewer particles are in the support or A
+// has no Cholesky factor; at least 3 are needed. Level set from the nearest surface
+// particle (:125-170). out: 9 blocks of n_nodes (u, v, p, vorticity_z, Fai,
+// analytic u, v, p, |v| of Devide_block_and_send_tasks.f90:104-115).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_grid_mls(DevParams P, Buffers B, GridArgs G) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= G.n_nodes) return;
+  const size_t nn = (size_t)G.n_nodes;
+  const int N = B.counters[C_NFLUID];
+  const double ex = (k % G.nnx) * G.edx, ey = (k / G.nnx) * G.edx;
+  const double pi = 3.14159265358979324;
+  const double au = P.U * sin(2 * pi * ex) * cos(2 * pi * ey) * exp(-8 * (pi * pi) / G.Re * G.t_nondim);
+  const double av = -P.U * cos(2 * pi * ex) * sin(2 * pi * ey) * exp(-8 * (pi * pi) / G.Re * G.t_nondim);
+  G.out[5 * nn + k] = au;
+  G.out[6 * nn + k] = av;
+  G.out[7 * nn + k] = 0.25 * P.rho0 * (P.U * P.U) * (cos(4 * pi * ex) + cos(4 * pi * ey)) *
+                      exp(-16 * (pi * pi) / G.Re * G.t_nondim);
+  G.out[8 * nn + k] = sqrt(au * au + av * av);
+  double f = 1.0;
+  if (G.n_surf > 0) {
+    double best = 1000000000.0, dfx = 0.0, dfy = 0.0;
+    int idx = 0;
+    for (int q = 0; q < G.n_surf; ++q) {
+      const double2 pj = B.g_pos[G.surf_gid[q]];
+      const double pdx = ex - pj.x, pdy = ey - pj.y;
+      const double d = sqrt(pdx * pdx + pdy * pdy);
+      if (d <= best) { best = d; idx = q; dfx = -pdx; dfy = -pdy; }
+    }
+    const double2 nr = G.surf_nrm[idx];
+    const double det = dfx * nr.x + dfy * nr.y;
+    if (det <= -2 * P.h) f = -1.0;
+    else if (det >= 2 * P.h) f = 1.0;
+    else f = det / (2 * P.h);
+  }
+  G.out[4 * nn + k] = f;
+  double su = 0.0, sv = 0.0, sp = 0.0, so = 0.0;
+  if (f > -0.9) {
+    const int xn = (int)floor((ex - P.chain_ox) / P.chain_d + 1);
+    const int yn = (int)floor((ey - P.chain_oy) / P.chain_d + 1);
+    const CellRanges R = cell_ranges(P, B.cell_start, (xn - 1) * P.chain_ny + (yn - 1));
+    const double rmin = (double)1.0e-7f, rmax = P.three_h;
+    double A[21];
+#pragma unroll
+    for (int q = 0; q < 21; ++q) A[q] = 0.0;
+    double sw = 0.0;
+    int nt = 0;
+    for (int d = 0; d < 3; ++d)
+      for (int o = R.lo[d]; o < R.hi[d]; ++o) {
+        const int g = B.name[o];
+        if (g >= N) continue;
+        const double2 pj = B.g_pos[g];
+        const double pdx = ex - pj.x, pdy = ey - pj.y;
+        const double r = sqrt(pdx * pdx + pdy * pdy);
+        if (!(r <= rmax && r >= rmin)) continue;
+        double w, dwx, dwy;
+        kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
+        const double a = (pj.x - ex) / P.h, b = (pj.y - ey) / P.h;
+        const double pv[6] = {1.0, a, b, a * a, a * b, b * b};
+        int q = 0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          const double bm = pv[m] * w;
+#pragma unroll
+          for (int l = m; l < 6; ++l) { A[q] = A[q] + bm * pv[l]; ++q; }
+        }
+        sw = sw + w;
+        ++nt;
+      }
+    if (nt >= 3) {
+      double y[6] = {0, 0, 0, 0, 0, 0};
+      bool mls = nt > 10;
+      if (mls) mls = mls6_first_row(A, (P.mls_alpha * P.mls_alpha) * P.mls_alpha, y);
+      for (int d = 0; d < 3; ++d)
+        for (int o = R.lo[d]; o < R.hi[d]; ++o) {
+          const int g = B.name[o];
+          if (g >= N) continue;
+          const double2 pj = B.g_pos[g];
+          const double pdx = ex - pj.x, pdy = ey - pj.y;
+          const double r = sqrt(pdx * pdx + pdy * pdy);
+          if (!(r <= rmax && r >= rmin)) continue;
+          double w, dwx, dwy;
+          kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
+          double fai;
+          if (mls) {
+            const double a = (pj.x - ex) / P.h, b = (pj.y - ey) / P.h;
+            const double pv[6] = {1.0, a, b, a * a, a * b, b * b};
+            fai = 0.0;
+#pragma unroll
+            for (int m = 0; m < 6; ++m) fai = fai + y[m] * (pv[m] * w);
+          } else {
+            fai = w / sw;
+          }
+          const double2 vj = B.g_vel[g];
+          sp = sp + B.g_p[g] * fai;
+          su = su + vj.x * fai;
+          sv = sv + vj.y * fai;
+          so = so + B.g_vort[g] * fai;
+        }
+    }
+  }
+  G.out[0 * nn + k] = su;
+  G.out[1 * nn + k] = sv;
+  G.out[2 * nn + k] = sp;
+  G.out[3 * nn + k] = so;
+}
+
+void launch_grid_mls(hipStream_t s, const DevParams& P, Buffers& B, const GridArgs& G) {
+  hipLaunchKernelGGL(k_grid_mls, dim3(grid_for(G.n_nodes)), dim3(kBlock), 0, s, P, B, G);
 }
 
 // ---------------------------------------------------------------------------

@@ -120,6 +120,15 @@ void launch_eval32_c(hipStream_t s, const DevParams& P, Buffers& B, int cap_loca
 void launch_eval_pass(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int pass);
 void launch_stage(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int kind);
 void launch_mls(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+// Calculation_For_PostProceeding.f90 on the element nodes (sph2d_fp64.hip)
+struct GridArgs {
+  int nnx, nny, n_nodes, n_surf;
+  double edx, t_nondim, Re;
+  const int* surf_gid;                    // surface particles (gid order) and their normals
+  const double2* surf_nrm;
+  double* out;                            // 9 blocks of n_nodes
+};
+void launch_grid_mls(hipStream_t s, const DevParams& P, Buffers& B, const GridArgs& G);
 // Refresh_domain.f90:320-351 from the last evaluation (sph2d_eval64.hip)
 void launch_vorticity(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
 void launch_shift_apply(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);

@@ -76,6 +76,35 @@ std::string ld_int(int64_t v) {
   return buf;
 }
 
+// A list-directed record sequence: items are appended to the current record and a
+// character item that would pass column 80 starts a new record with a blank (the
+// shipped Initial_Post_Proceeding_Grid.dat breaks its ZONE line this way).
+struct LdRecords {
+  std::string cur = " ";
+  std::string out;
+  void item(const std::string& s) {
+    if (cur.size() > 1 && cur.size() + s.size() > 80) {
+      out += cur + "\n";
+      cur = " ";
+    }
+    cur += s;
+  }
+  std::string done() { return out + cur + "\n"; }
+};
+
+// The quadrilaterals of Distribute_Grid_For_Postproceeding.f90:47-58 (format 8I8).
+void grid_elements(FILE* f, int nex, int ney) {
+  const int nnx = nex + 1;
+  char b[64];
+  for (int i = 1; i <= ney; ++i)
+    for (int j = 1; j <= nex; ++j) {
+      const int k = j + (i - 1) * nex;
+      const int L = k + (i - 1);
+      std::snprintf(b, sizeof b, "%8d%8d%8d%8d\n", L, L + 1, L + nnx + 1, L + nnx);
+      std::fputs(b, f);
+    }
+}
+
 bool file_is_empty(const char* path) {
   FILE* f = std::fopen(path, "rb");
   if (!f) return true;
@@ -396,6 +425,125 @@ int sph2d_write_sampling(const char* dir, double nondim_time, const double* out1
     std::fprintf(F.f, "%s\n", line.c_str());
   }
   return SPH2D_OK;
+}
+
+// Distribute_Grid_For_Postproceeding.f90:60-95: Initial_Post_Proceeding_Grid.dat and
+// Initial_Post_Proceeding_Node.dat of the (nex+1) x (ney+1) element nodes.
+int sph2d_write_initial_grid(const char* grid_path, const char* node_path, int32_t nex, int32_t ney,
+                             double element_dx) {
+  if ((!grid_path && !node_path) || nex < 1 || ney < 1) return out_fail(SPH2D_ERR_ARGUMENT, "bad argument");
+  const int nnx = nex + 1, nny = ney + 1, nn = nnx * nny;
+  auto nodes = [&](FILE* f) {
+    std::string line;
+    for (int i = 0; i < nny; ++i)
+      for (int j = 0; j < nnx; ++j) {
+        line = fmt_f(j * element_dx, 20, 10) + fmt_f(i * element_dx, 20, 10);
+        std::fprintf(f, "%s\n", line.c_str());
+      }
+  };
+  if (grid_path) {
+    File F(grid_path, "w");
+    if (!F.f) return open_err(grid_path);
+    std::fprintf(F.f, " TITLE='mesh'\n VARIABLES= 'X' 'Y'\n");
+    LdRecords z;
+    z.item("ZONE T= '0' ,N=");
+    z.item(ld_int(nn));
+    z.item(",E=");
+    z.item(ld_int((int64_t)nex * ney));
+    z.item(",DATAPACKING=POINT,ZONETYPE=FEQUADRILATERAL ");
+    std::fputs(z.done().c_str(), F.f);
+    nodes(F.f);
+    grid_elements(F.f, nex, ney);
+  }
+  if (node_path) {
+    File F(node_path, "w");
+    if (!F.f) return open_err(node_path);
+    std::fprintf(F.f, " TITLE='Node'\n VARIABLES= 'X' 'Y'\n");
+    std::fprintf(F.f, " ZONE I=%s F=POINT\n", ld_int(nn).c_str());
+    nodes(F.f);
+  }
+  return SPH2D_OK;
+}
+
+// Output_for_Grid_PostProceding.f90:110-138: one ZONE of Post_Proceeding_Grid.dat
+// (header of time_iteration.f90:153-157 when the file is new), 20F20.8 rows and
+// the element connectivity.
+int sph2d_write_grid_zone(const char* path, double current_time, int32_t nex, int32_t ney, double element_dx,
+                          const double* vel, const double* press, const double* vort_z, const double* fai,
+                          const double* ana, int32_t ld, const sph2d_scales* sc) {
+  const int nnx = nex + 1, nny = ney + 1, nn = nnx * nny;
+  if (!path || nex < 1 || ney < 1 || !vel || !press || !vort_z || !fai || !ana || ld < nn || !ok_scales(sc))
+    return out_fail(SPH2D_ERR_ARGUMENT, "bad argument");
+  const bool fresh = file_is_empty(path);
+  File F(path, "a");
+  if (!F.f) return open_err(path);
+  if (fresh) {
+    std::fprintf(F.f, " TITLE='mesh'\n");
+    std::fprintf(F.f, " VARIABLES= 'X' 'Y' 'VX' 'VY' 'P' 'U' 'Vor' 'EVX' 'EVY' 'EP' 'EU' 'Fai'\n");
+  }
+  LdRecords z;
+  z.item("ZONE T=\" " + time_string(current_time, 12, 5) + " \",N=");
+  z.item(ld_int(nn));
+  z.item(",E=");
+  z.item(ld_int((int64_t)nex * ney));
+  z.item(",DATAPACKING=POINT,ZONETYPE=FEQUADRILATERAL ");
+  std::fputs(z.done().c_str(), F.f);
+  const double L = sc->domain_size_y, U = sc->velocity_0, P0 = sc->pressure_0, T0 = sc->time_0;
+  std::string line;
+  for (int k = 0; k < nn; ++k) {
+    const double ex = (k % nnx) * element_dx, ey = (k / nnx) * element_dx;
+    const double u = vel[k], v = vel[ld + k];
+    const double vm = std::sqrt(u * u + v * v);
+    const double eu = std::log10(std::fabs((u - ana[k]) / U));
+    const double ev = std::log10(std::fabs((v - ana[ld + k]) / U));
+    const double ep = std::log10(std::fabs((press[k] - ana[2 * (size_t)ld + k]) / P0));
+    const double em = std::log10(std::fabs((vm - ana[3 * (size_t)ld + k]) / U));
+    line.clear();
+    for (double x : {ex / L, ey / L, u / U, v / U, press[k] / P0, vm / U, vort_z[k] * T0, eu, ev, ep, em, fai[k]})
+      line += fmt_f(x, 20, 8);
+    std::fprintf(F.f, "%s\n", line.c_str());
+  }
+  grid_elements(F.f, nex, ney);
+  return SPH2D_OK;
+}
+
+// Output_for_Grid_PostProceding.f90:140-291: the #!TDV101 file of the grid
+// (ordered zone nnx x nny, point-packed doubles X Y VX VY P U Fai).
+int sph2d_write_grid_binary(const char* path, double time, int32_t nex, int32_t ney, double element_dx,
+                            const double* vel, const double* press, const double* fai, int32_t ld,
+                            const sph2d_scales* sc) {
+  const int nnx = nex + 1, nny = ney + 1, nn = nnx * nny;
+  if (!path || nex < 1 || ney < 1 || !vel || !press || !fai || ld < nn || !ok_scales(sc))
+    return out_fail(SPH2D_ERR_ARGUMENT, "bad argument");
+  File F(path, "wb");
+  if (!F.f) return open_err(path);
+  FILE* f = F.f;
+  const std::string t = time_string(time, 8, 4);
+  std::fwrite("#!TDV101", 1, 8, f);
+  bin_i32(f, 1);
+  bin_string(f, "TITLE='" + t + "'");
+  bin_i32(f, 7);
+  for (const char* name : {"X", "Y", "VX", "VY", "P", "U", "Fai"}) bin_string(f, name);
+  bin_f32(f, 299.0f);
+  bin_string(f, "ZONE " + t);
+  bin_i32(f, -1); bin_i32(f, 0); bin_i32(f, 1); bin_i32(f, 0); bin_i32(f, 0);
+  bin_i32(f, nnx); bin_i32(f, nny); bin_i32(f, 1);
+  bin_i32(f, 0);
+  bin_f32(f, 357.0f);
+  bin_f32(f, 299.0f);
+  for (int k = 0; k < 7; ++k) bin_i32(f, 2);
+  bin_i32(f, 0);
+  bin_i32(f, -1);
+  const double L = sc->domain_size_y, U = sc->velocity_0, P0 = sc->pressure_0;
+  double row[7];
+  for (int k = 0; k < nn; ++k) {
+    const double u = vel[k], v = vel[ld + k];
+    row[0] = ((k % nnx) * element_dx) / L; row[1] = ((k / nnx) * element_dx) / L;
+    row[2] = u / U; row[3] = v / U; row[4] = press[k] / P0; row[5] = std::sqrt(u * u + v * v) / U;
+    row[6] = fai[k];
+    std::fwrite(row, 8, 7, f);
+  }
+  return std::ferror(f) ? out_fail(SPH2D_ERR_ARGUMENT, std::string("write failed: ") + path) : SPH2D_OK;
 }
 
 }  // extern "C"

@@ -90,7 +90,9 @@ EXPORTS = (
     "sph2d_output_for_paraview", "sph2d_format_fortran", "sph2d_write_final_results",
     "sph2d_write_tecplot_zone", "sph2d_write_surface_zone", "sph2d_write_tecplot_binary",
     "sph2d_write_paraview", "sph2d_write_latest_results_zone", "sph2d_read_latest_results",
-    "sph2d_write_sampling",
+    "sph2d_write_sampling", "sph2d_grid_size", "sph2d_calculation_for_postproceeding",
+    "sph2d_output_for_grid_postproceding", "sph2d_write_initial_grid", "sph2d_write_grid_zone",
+    "sph2d_write_grid_binary",
 )
 
 _lib = None
@@ -150,6 +152,14 @@ def lib():
             "sph2d_write_latest_results_zone": [cs, i32, i32, dp, dp, dp, dp, dp, dp, dp, ip, ip, i32],
             "sph2d_read_latest_results": [cs, i32, i32, ip, dp, dp, dp, dp, dp, dp, dp, ip, ip, ip, i32],
             "sph2d_write_sampling": [cs, ctypes.c_double, dp],
+            "sph2d_grid_size": [vp, ip, ip, dp],
+            "sph2d_calculation_for_postproceeding": [vp, i32, i32, dp, dp, dp, dp, dp, i32],
+            "sph2d_output_for_grid_postproceding": [vp, i32, cs, cs],
+            "sph2d_write_initial_grid": [cs, cs, i32, i32, ctypes.c_double],
+            "sph2d_write_grid_zone": [cs, ctypes.c_double, i32, i32, ctypes.c_double, dp, dp, dp, dp, dp, i32,
+                                      P(Scales)],
+            "sph2d_write_grid_binary": [cs, ctypes.c_double, i32, i32, ctypes.c_double, dp, dp, dp, i32,
+                                        P(Scales)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -309,6 +319,27 @@ class Simulation:
 
     def output_for_paraview(self, i_time_step: int, path):
         _check(lib().sph2d_output_for_paraview(self._c, i_time_step, os.fspath(path).encode()))
+
+    def grid_size(self):
+        """(element_number_x, element_number_y, element_dx) of the post-processing grid."""
+        nx, ny, dx = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        _check(lib().sph2d_grid_size(self._c, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(dx)))
+        return nx.value, ny.value, dx.value
+
+    def calculation_for_postproceeding(self, i_time_step: int) -> Dict[str, np.ndarray]:
+        """Calculation_For_PostProceeding.f90 on the grid nodes: u, v, p, vorticity, Fai
+        and the analytic u, v, p, |v| (node order of Distribute_Grid_For_Postproceeding)."""
+        nex, ney, _ = self.grid_size()
+        nn = (nex + 1) * (ney + 1)
+        vel, p, w, f, a = np.zeros(2 * nn), np.zeros(nn), np.zeros(nn), np.zeros(nn), np.zeros(4 * nn)
+        _check(lib().sph2d_calculation_for_postproceeding(self._c, i_time_step, nn, _dp(vel), _dp(p), _dp(w), _dp(f),
+                                                         _dp(a), nn))
+        return {"u": vel[:nn], "v": vel[nn:], "p": p, "vorticity": w, "fai": f, "ana_u": a[:nn],
+                "ana_v": a[nn:2 * nn], "ana_p": a[2 * nn:3 * nn], "ana_vm": a[3 * nn:]}
+
+    def output_for_grid_postproceding(self, i_time_step: int, grid_path=None, binary_path=None):
+        enc = lambda p: None if p is None else os.fspath(p).encode()
+        _check(lib().sph2d_output_for_grid_postproceding(self._c, i_time_step, enc(grid_path), enc(binary_path)))
 
     def output_final_results(self, path):
         """Final_Doamin_Results.dat (Parallel_Taylor_Green_Vortices_2D_Release.f90:185-199)."""
@@ -556,3 +587,24 @@ def write_sampling(dir_path, nondim_time: float, out14):
     """One line appended to each of Sampling_subroutine.f90's six history files."""
     o = _f64(out14)
     _check(lib().sph2d_write_sampling(os.fspath(dir_path).encode(), nondim_time, _dp(o)))
+
+
+def write_initial_grid(grid_path, node_path, nex: int, ney: int, element_dx: float):
+    """Initial_Post_Proceeding_Grid.dat / _Node.dat (Distribute_Grid_For_Postproceeding.f90:60-95)."""
+    enc = lambda p: None if p is None else os.fspath(p).encode()
+    _check(lib().sph2d_write_initial_grid(enc(grid_path), enc(node_path), nex, ney, element_dx))
+
+
+def write_grid_zone(path, current_time, nex, ney, element_dx, u, v, p, vort_z, fai, ana, scales: Scales):
+    """ana: (n_nodes,4) analytic u, v, p, |v|."""
+    nn = (nex + 1) * (ney + 1)
+    a = np.ascontiguousarray(np.asarray(ana, dtype=np.float64).T.reshape(-1))
+    _check(lib().sph2d_write_grid_zone(os.fspath(path).encode(), current_time, nex, ney, element_dx,
+                                       _dp(_col2(u, v)), _dp(_f64(p)), _dp(_f64(vort_z)), _dp(_f64(fai)), _dp(a), nn,
+                                       ctypes.byref(scales)))
+
+
+def write_grid_binary(path, time, nex, ney, element_dx, u, v, p, fai, scales: Scales):
+    nn = (nex + 1) * (ney + 1)
+    _check(lib().sph2d_write_grid_binary(os.fspath(path).encode(), time, nex, ney, element_dx, _dp(_col2(u, v)),
+                                         _dp(_f64(p)), _dp(_f64(fai)), nn, ctypes.byref(scales)))

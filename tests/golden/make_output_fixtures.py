@@ -8,8 +8,12 @@ Sources (mini-walker/SPH-2D, Taylor_Green_Vortex/, the shipped 100x100 run):
   Free_Surface.dat          header + the first 4 ZONE lines (all zones are empty)
   Kinetic_energy.dat        first 40 rows (format 310 = 20F20.8)
   Norm_Pressure.dat         first 40 rows
+  Initial_Post_Proceeding_Grid.dat / _Node.dat   the first 8 lines, the line count and
+                            the SHA-256 of the whole file (10201 nodes, 10000 quads)
 Only data lines are copied; no reference source is.
 """
+import hashlib
+import json
 import os
 
 REF = os.environ.get("SPH2D_REFERENCE", "/root/reference") + "/Taylor_Green_Vortex"
@@ -26,6 +30,15 @@ def main():
         with open(os.path.join(OUT, name + ".head"), "wb") as f:
             f.write(b"\n".join(lines) + b"\n")
         print(name, len(lines), "lines")
+    digest = {}
+    for name in ("Initial_Post_Proceeding_Grid.dat", "Initial_Post_Proceeding_Node.dat"):
+        with open(os.path.join(REF, name), "rb") as f:
+            data = f.read()
+        digest[name] = {"sha256": hashlib.sha256(data).hexdigest(), "lines": data.count(b"\n"),
+                        "head": [ln.decode() for ln in data.split(b"\n")[:8]]}
+    with open(os.path.join(OUT, "initial_grid.json"), "w") as f:
+        json.dump(digest, f, indent=1)
+    print("initial grid digests written")
 
 
 if __name__ == "__main__":

@@ -152,3 +152,122 @@ def test_restart_from_the_latest_results_file(tmp_path):
     assert abs(eb - ea) / ea < 5e-3
     a.close()
     b.close()
+
+
+# --- Calculation_For_PostProceeding on the GPU ---------------------------------------------
+GRID_TOL = 1e-10     # order-6 MLS: Cholesky vs the reference's Gauss-Jordan and another sum order
+
+
+def _grid_cmp(g, ref, name):
+    scale = np.max(np.abs(ref))
+    err = np.max(np.abs(g - ref)) / scale
+    say(f"grid {name}: rel err {err:.3e}")
+    assert err < GRID_TOL, name
+
+
+def test_grid_postprocess_matches_the_oracle():
+    say("test_grid_postprocess_matches_the_oracle")
+    n = 48
+    sim, o = run_pair(n, 10, 10)
+    nex, ney, edx = sim.grid_size()
+    assert (nex, ney) == (n, n) and edx == 1.0 / n
+    g = sim.calculation_for_postproceeding(10)
+    ref = o.grid_postprocess(10, nex, ney, edx)
+    for col, name in enumerate(("u", "v", "p", "vorticity")):
+        _grid_cmp(g[name], ref[:, col], name)
+    assert np.all(g["fai"] == 1.0) and np.all(ref[:, 4] == 1.0)
+    for col, name in ((5, "ana_u"), (6, "ana_v"), (7, "ana_p"), (8, "ana_vm")):
+        assert np.max(np.abs(g[name] - ref[:, col])) <= 1e-14 * np.max(np.abs(ref[:, col])), name
+    sim.close()
+
+
+def test_grid_mls_is_exact_for_quadratic_velocity():
+    """Per-routine path on an uploaded state whose velocity is a quadratic of position:
+    interior nodes get the polynomial's value (order-6 MLS reproduces quadratics)."""
+    say("test_grid_mls_is_exact_for_quadratic_velocity")
+    n = 40
+    o0 = Oracle(nx=n, ny=n)
+    o0.init_tgv()
+    N = n * n
+    rng = np.random.default_rng(12)
+    x = o0.f64("g_x")[:N] + rng.uniform(-0.2, 0.2, N) / n
+    y = o0.f64("g_y")[:N] + rng.uniform(-0.2, 0.2, N) / n
+    x, y = np.mod(x, 1.0), np.mod(y, 1.0)
+    quad = lambda X, Y, c: c[0] + c[1] * X + c[2] * Y + c[3] * X * X + c[4] * X * Y + c[5] * Y * Y
+    cu, cv = (0.3, 1.1, -0.7, 0.9, -1.3, 0.5), (-0.2, 0.4, 0.8, -0.6, 0.7, 1.2)
+    prm = sph2d.tgv_params(n, sample_every=1)
+    sim = sph2d.Simulation(prm)
+    sim.upload_state(x, y, quad(x, y, cu), quad(x, y, cv), o0.f64("g_rho")[:N], o0.f64("g_mass")[:N])
+    sim.distribute_period_boundary()
+    sim.devide_block(1)
+    sim.single_step_compute(1)
+    sim.refresh_domain(1)
+    g = sim.calculation_for_postproceeding(1)
+    ex = np.tile(np.arange(n + 1) / n, n + 1)
+    ey = np.repeat(np.arange(n + 1) / n, n + 1)
+    inner = (ex > 0.05) & (ex < 0.95) & (ey > 0.05) & (ey < 0.95)
+    for name, c in (("u", cu), ("v", cv)):
+        err = np.abs(g[name] - quad(ex, ey, c))[inner].max()
+        say(f"quadratic {name}: max err {err:.3e}")
+        assert err < 1e-11
+    sim.close()
+
+
+def test_grid_level_set_on_a_free_surface_patch():
+    """A fluid disc with an open boundary: surface particles and normals from the
+    evaluation give the level set Fai and the nodes left empty outside; the MLS
+    values inside match the oracle."""
+    say("test_grid_level_set_on_a_free_surface_patch")
+    n = 60
+    o0 = Oracle(nx=n, ny=n)
+    o0.init_tgv()
+    N = n * n
+    x, y = o0.f64("g_x")[:N], o0.f64("g_y")[:N]
+    keep = ((x - 0.5) ** 2 + (y - 0.5) ** 2) < 0.3 ** 2
+    rng = np.random.default_rng(9)
+    st = (x[keep] + rng.uniform(-0.1, 0.1, keep.sum()) / n, y[keep] + rng.uniform(-0.1, 0.1, keep.sum()) / n,
+          o0.f64("g_u")[:N][keep], o0.f64("g_v")[:N][keep], o0.f64("g_rho")[:N][keep], o0.f64("g_mass")[:N][keep])
+    prm = sph2d.tgv_params(n, sample_every=1)
+    sim = sph2d.Simulation(prm, capacity=N)
+    sim.upload_state(*st)
+    o = Oracle(nx=n, ny=n, dt=prm.dt)
+    o.set_fluid_state(*st)
+    sim.distribute_period_boundary()
+    sim.devide_block(1)
+    sim.single_step_compute(1)
+    sim.refresh_domain(1)
+    o.period_boundary()
+    o.devide_block(1)
+    o.single_step_compute(1)
+    o.refresh()
+    nex, ney, edx = sim.grid_size()
+    g = sim.calculation_for_postproceeding(1)
+    ref = o.grid_postprocess(1, nex, ney, edx)
+    say(f"surface particles {int(o.i32('g_surface').sum())}, empty nodes {int((ref[:, 4] <= -0.9).sum())}")
+    assert (ref[:, 4] <= -0.9).sum() > 100 and (ref[:, 4] == 1.0).sum() > 100
+    assert np.max(np.abs(g["fai"] - ref[:, 4])) < 1e-12
+    for col, name in enumerate(("u", "v", "p")):
+        _grid_cmp(g[name], ref[:, col], name)
+    sim.close()
+
+
+def test_grid_output_files_of_a_context(tmp_path):
+    say("test_grid_output_files_of_a_context")
+    n = 32
+    prm = sph2d.tgv_params(n, sample_every=10)
+    sim = sph2d.Simulation(prm)
+    sim.init_tgv()
+    sim.run(1, 10)
+    grid, binf = tmp_path / "Post_Proceeding_Grid.dat", tmp_path / "grid.dat"
+    sim.output_for_grid_postproceding(10, grid, binf)
+    g = sim.calculation_for_postproceeding(10)
+    nex, ney, edx = sim.grid_size()
+    ref = tmp_path / "ref.dat"
+    ana = np.stack([g["ana_u"], g["ana_v"], g["ana_p"], g["ana_vm"]], axis=1)
+    sph2d.write_grid_zone(ref, 10 * prm.dt, nex, ney, edx, g["u"], g["v"], g["p"], g["vorticity"], g["fai"], ana,
+                          sim.scales())
+    assert grid.read_bytes() == ref.read_bytes()
+    nn = (nex + 1) * (ney + 1)
+    vals = np.frombuffer(binf.read_bytes()[-56 * nn:], dtype="<f8").reshape(nn, 7)
+    assert np.array_equal(vals[:, 2], g["u"]) and np.array_equal(vals[:, 6], g["fai"])
+    sim.close()

@@ -297,3 +297,72 @@ def test_vorticity_is_the_curl_of_the_tgv_field(n):
         o.close()
     assert errs[1] < 0.01
     assert 3.0 < errs[0] / errs[1] < 5.0, errs
+
+
+# --- Calculation_For_PostProceeding: order-6 MLS onto the grid nodes -------------------
+def _post(n):
+    o = evaluate(tgv(n))
+    o.refresh()
+    return o
+
+
+def test_grid_mls_reproduces_quadratic_fields():
+    """MLS of order 6 (MLS_Interpolation_In_Gobal_Domain.f90) is exact for quadratics:
+    every interior node returns the polynomial of its position for u, v, p and the
+    vorticity, whatever the particle disorder; a wrong basis row, a transposed
+    moment matrix or a wrong node position fails."""
+    n = 40
+    o = _post(n)
+    N = n * n
+    x, y = o.f64("g_x")[:N], o.f64("g_y")[:N]
+    quad = lambda X, Y, c: c[0] + c[1] * X + c[2] * Y + c[3] * X * X + c[4] * X * Y + c[5] * Y * Y
+    cs = {"g_u": (0.3, 1.1, -0.7, 0.9, -1.3, 0.5), "g_v": (-0.2, 0.4, 0.8, -0.6, 0.7, 1.2),
+          "g_p": (5.0, -3.0, 2.0, 7.0, 1.0, -4.0), "g_vorticity": (1.0, 0.0, 0.0, -2.0, 3.0, 0.25)}
+    for name, c in cs.items():
+        a = o.f64(name)
+        a[:N] = quad(x, y, c)
+        o.set_f64(name, a)
+    out = o.grid_postprocess(1, n, n, 1.0 / n)
+    ex = np.tile(np.arange(n + 1) / n, n + 1)
+    ey = np.repeat(np.arange(n + 1) / n, n + 1)
+    inner = (ex > 0) & (ex < 1) & (ey > 0) & (ey < 1)
+    for col, name in enumerate(("g_u", "g_v", "g_p", "g_vorticity")):
+        err = np.abs(out[:, col] - quad(ex, ey, cs[name]))
+        scale = np.max(np.abs(quad(ex, ey, cs[name])))
+        assert err[inner].max() < 1e-12 * scale, name
+        assert err.max() < 1e-9 * scale, name         # one-sided support on the box edges
+    assert np.all(out[:, 4] == 1.0)                    # no surface particle: Fai = 1
+    t = 1 * o.params.dt
+    assert np.allclose(out[:, 5], np.sin(2 * np.pi * ex) * np.cos(2 * np.pi * ey) *
+                       np.exp(-8 * np.pi ** 2 * 0.009999999776482582 * t), atol=1e-15)
+    o.close()
+
+
+def test_grid_level_set_from_one_surface_particle():
+    """Calculation_For_PostProceeding.f90:125-170 with one surface particle at
+    (0.5, 0.5) and outward normal (0, 1): every node's nearest surface particle is
+    that one and the test value is (particle - node) . n (the code negates
+    node - particle), so Fai = clamp((0.5 - y) / 2h, -1, 1): +1 inside the fluid
+    below, and nodes with Fai <= -0.9 above are left empty."""
+    n = 40
+    o = _post(n)
+    N = n * n
+    x, y = o.f64("g_x")[:N], o.f64("g_y")[:N]
+    k = int(np.argmin((x - 0.5) ** 2 + (y - 0.5) ** 2))
+    for name, val in (("g_x", 0.5), ("g_y", 0.5), ("g_normal_x", 0.0), ("g_normal_y", 1.0)):
+        a = o.f64(name)
+        a[k] = val
+        o.set_f64(name, a)
+    s = o.i32("g_surface")
+    s[:] = 0
+    s[k] = 1
+    o.set_i32("g_surface", s)
+    out = o.grid_postprocess(1, n, n, 1.0 / n)
+    ey = np.repeat(np.arange(n + 1) / n, n + 1)
+    h = 1.2 / n
+    expect = np.clip((0.5 - ey) / (2 * h), -1.0, 1.0)
+    np.testing.assert_allclose(out[:, 4], expect, rtol=0, atol=1e-12)
+    empty = out[:, 4] <= -0.9
+    assert empty.sum() > 0 and np.all(out[empty, :4] == 0.0)
+    assert np.all(out[~empty, 2] != 0.0)
+    o.close()

@@ -221,3 +221,58 @@ def test_restart_zone_round_trip_and_nan_step_back(tmp_path):
     with pytest.raises(sph2d.SphError) as e:
         sph2d.read_latest_results(out, zone=1, capacity=3)
     assert e.value.code == 2
+
+
+# --- post-processing grid files ----------------------------------------------------------
+def test_initial_grid_files_equal_the_shipped_ones(tmp_path):
+    """Distribute_Grid_For_Postproceeding.f90 at the shipped case (100 x 100 elements,
+    element_dx = 1/100): both files byte for byte (SHA-256 of the reference's files),
+    which pins the node order, the F20.10 node rows, the quadrilateral numbering
+    (8I8) and the list-directed ZONE line broken at column 80."""
+    import hashlib
+    import json
+    gold = json.load(open(os.path.join(GOLD, "initial_grid.json")))
+    g, nd = tmp_path / "Initial_Post_Proceeding_Grid.dat", tmp_path / "Initial_Post_Proceeding_Node.dat"
+    sph2d.write_initial_grid(g, nd, 100, 100, 1.0 / 100)
+    for path, name in ((g, "Initial_Post_Proceeding_Grid.dat"), (nd, "Initial_Post_Proceeding_Node.dat")):
+        data = path.read_bytes()
+        assert [ln.decode() for ln in data.split(b"\n")[:8]] == gold[name]["head"], name
+        assert data.count(b"\n") == gold[name]["lines"], name
+        assert hashlib.sha256(data).hexdigest() == gold[name]["sha256"], name
+
+
+def test_grid_zone_and_binary(tmp_path):
+    nex, ney, edx = 3, 2, 0.25
+    nn = (nex + 1) * (ney + 1)
+    rng = np.random.default_rng(6)
+    u, v, p, w = (rng.standard_normal(nn) for _ in range(4))
+    fai = np.linspace(-1, 1, nn)
+    ana = rng.standard_normal((nn, 4))
+    sc = sph2d.Scales(1.0, 1.0, 1000.0, 1.0)
+    out = tmp_path / "Post_Proceeding_Grid.dat"
+    sph2d.write_grid_zone(out, 0.05, nex, ney, edx, u, v, p, w, fai, ana, sc)
+    lines = out.read_text().split("\n")
+    assert lines[0] == " TITLE='mesh'"
+    assert lines[1] == " VARIABLES= 'X' 'Y' 'VX' 'VY' 'P' 'U' 'Vor' 'EVX' 'EVY' 'EP' 'EU' 'Fai'"
+    assert lines[2] == ' ZONE T=" 0.05000 ",N=          12 ,E=           6 '
+    assert lines[3] == " ,DATAPACKING=POINT,ZONETYPE=FEQUADRILATERAL "
+    row = lines[4 + 5]                                       # node 6: (1*edx, 1*edx)
+    cols = [float(row[20 * k:20 * k + 20]) for k in range(12)]
+    assert cols[:2] == [0.25, 0.25] and abs(cols[11] - fai[5]) < 5e-9
+    assert lines[4 + nn] == "%8d%8d%8d%8d" % (1, 2, 6, 5)    # first quadrilateral
+    assert lines[4 + nn + 3] == "%8d%8d%8d%8d" % (5, 6, 10, 9)     # first of the second element row
+    b = tmp_path / "g.dat"
+    sph2d.write_grid_binary(b, 0.05, nex, ney, edx, u, v, p, fai, sc)
+    data = b.read_bytes()
+    o = 8 + 4
+    title, o = _read_str(data, o)
+    nv, o = _read_i32(data, o)
+    names = []
+    for _ in range(nv):
+        s, o = _read_str(data, o)
+        names.append(s)
+    assert title == "TITLE='0.0500'" and names == ["X", "Y", "VX", "VY", "P", "U", "Fai"]
+    zone, o = _read_str(data, o + 4)
+    assert struct.unpack_from("<9i", data, o) == (-1, 0, 1, 0, 0, nex + 1, ney + 1, 1, 0)
+    vals = np.frombuffer(data[-56 * nn:], dtype="<f8").reshape(nn, 7)
+    assert np.array_equal(vals[:, 6], fai) and np.array_equal(vals[:, 2], u)
