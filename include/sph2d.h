@@ -267,7 +267,10 @@ int sph2d_write_grid_binary(const char* path, double time, int32_t element_numbe
 int sph2d_write_sampling(const char* dir, double nondim_time, const double* out14);
 
 /* --- parity / inspection -------------------------------------------------- */
-/* Download a named subdomain (local-order) array of the last evaluation:
+/* Download a named subdomain (local-order) array of the last evaluation (in the
+ * FP32-mixed form after sph2d_time_step/sph2d_run, u v rho mass p c volume are
+ * current on sampling and MLS steps only: the other steps' evaluations read the
+ * FP32 records instead):
  * f64: x y u v rho mass p c volume fra_v sum_dwdx_x sum_dwdx_y tau_xx tau_xy tau_yy
  *      grad_rho_x grad_rho_y drhodt ax ay shift_x shift_y normal_x normal_y
  *      internal_x internal_y visc_x visc_y art_internal_x art_internal_y sps_x sps_y

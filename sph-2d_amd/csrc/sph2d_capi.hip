@@ -679,8 +679,12 @@ static int enqueue_step(sph2d_ctx* c, int it) {
   const bool pc = c->prm.integrator == SPH2D_INTEGRATOR_PC;
   const bool mls = c->prm.mls_on && (it % c->prm.mls_every) == 0;
   // After step 1 the predictor needs no evaluation first: it runs inside the local copy
-  // (which keeps before_* only for an MLS step, whose corrector is k_stage's).
-  rc = devide_block(c, it, pc && it > 1 ? (mls ? 2 : 1) : 0);
+  // (which keeps before_* only for an MLS step, whose corrector is k_stage's). In the
+  // mixed form the FP64 local v, p, c, rho, volume are read only by MLS, the vorticity
+  // of a sampling step and XSPH; other steps skip writing them.
+  const bool sampling = it % c->prm.sample_every == 0;
+  const bool lean = c->P.mixed && !mls && !sampling && c->prm.epsilon_xsph == 0.0;
+  rc = devide_block(c, it, pc && it > 1 ? (mls ? 2 : (lean ? 3 : 1)) : 0);
   if (rc) return rc;
   hipStream_t s = c->stream;
   if (pc) {

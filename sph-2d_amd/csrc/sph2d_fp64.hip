@@ -456,7 +456,7 @@ void launch_max_cell(hipStream_t s, Buffers& B, int n_cells) {
   hipLaunchKernelGGL(k_max_cell, dim3(1), dim3(kBlock), 0, s, B, n_cells);
 }
 
-__global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers B, int save_before);
+__global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers B, int save_before, int lean);
 
 void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_global, int n_cells,
                          bool analytic, double t_nondim, int predict) {
@@ -474,7 +474,7 @@ void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_
   hipLaunchKernelGGL(k_cell_rank, dim3((n_cells + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, s, B,
                      n_cells);
   if (predict) hipLaunchKernelGGL(k_gather_predict, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, P, B,
-                                  (int)(predict == 2));
+                                  (int)(predict == 2), (int)(predict == 3));
   else hipLaunchKernelGGL(k_local_gather, dim3(grid_for(cap_global)), dim3(kBlock), 0, s, P, B);
 }
 
@@ -626,7 +626,7 @@ void launch_stage(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, 
 // reads it; k_correct_refresh reads the global arrays instead), the local arrays
 // receive the predicted state.
 // Same arithmetic as k_local_gather followed by k_stage(ST_PC_PREDICT).
-__global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers B, int save_before) {
+__global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers B, int save_before, int lean) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= B.counters[C_NLOCAL]) return;
   const int g = B.name[l], key = B.l_cell[l];
@@ -648,14 +648,16 @@ __global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers 
   }
   B.l_pos[l] = x;
   B.l_pos32[l] = make_float2((float)x.x, (float)x.y);
-  B.l_vel[l] = v;
-  B.l_mass[l] = m;
   B.l_type[l] = t;
-  double p, c;
-  if (t != 120) tait(P, rho, p, c);
-  else { p = B.g_p[g]; c = B.g_c[g]; }
-  B.l_pc[l] = make_double2(p, c);
-  B.l_rv[l] = make_double2(rho, m / rho);
+  if (!lean) {                  // the FP32-mixed passes read the records below instead
+    B.l_vel[l] = v;
+    B.l_mass[l] = m;
+    double p, c;
+    if (t != 120) tait(P, rho, p, c);
+    else { p = B.g_p[g]; c = B.g_c[g]; }
+    B.l_pc[l] = make_double2(p, c);
+    B.l_rv[l] = make_double2(rho, m / rho);
+  }
   if (P.mixed) {
     write_r0(P, B, l, v, rho, m, t);
     B.m_r1[l] = make_float4(0.f, 0.f, 0.f, (float)B.g_mass[l]);   // particle_mass(j) with j local
@@ -973,19 +975,33 @@ __global__ void k_correct_refresh(DevParams P, Buffers B, int shift_on) {
   const int s = B.l_of_g[g];
   if (s < 0) return;
   const int t = B.l_type[s];
-  double2 x, v = B.l_vel[s];
+  // The step-start state is still this gid's global state (before_* of the gather).
+  const double2 bx = B.g_pos[g], bv = B.g_vel[g];
+  const double brho = B.g_rho[g];
+  // The predicted state (velocity, and p, c of the evaluation) again, bit for bit as
+  // the gather computed it; XSPH's average velocity of the previous evaluation is gone
+  // by now, so with epsilon != 0 it is read back instead.
+  double2 v, x;
   double rho;
-  double2 pc = B.l_pc[s];                    // p, c of the last evaluation
+  double2 pc;
+  if (P.epsilon_xsph == 0.0) {
+    double rp = brho;
+    double2 xp = bx;
+    v = bv;
+    if (t == 2) pc_predict(P.dt, B.t_drho[s], B.t_acc[s], make_double2(0.0, 0.0), brho, bv, bx, rp, v, xp);
+    tait(P, rp, pc.x, pc.y);
+  } else {
+    v = B.l_vel[s];
+    pc = B.l_pc[s];
+  }
   if (t == 2) {
     const double2 avg = (P.epsilon_xsph != 0.0) ? B.avg[s] : make_double2(0.0, 0.0);
-    // the step-start state is still this gid's global state (before_* of the gather)
-    pc_correct(P.dt, B.drhodt[s], B.acc[s], avg, B.t_drho[s], B.t_acc[s], v, B.g_rho[g], B.g_vel[g],
-               B.g_pos[g], rho, v, x);
+    pc_correct(P.dt, B.drhodt[s], B.acc[s], avg, B.t_drho[s], B.t_acc[s], v, brho, bv, bx, rho, v, x);
     rebin_check(P, B, B.l_cell[s], x);
     if (shift_on && B.near_[s] != 1) shift_one(P, B, s, x, rho, pc.x);
   } else {
-    x = B.l_pos[s];
-    rho = B.l_rv[s].x;
+    x = bx;                                  // the stages move fluid particles only
+    rho = brho;
   }
   B.g_pos[g] = x;
   B.g_vel[g] = v;

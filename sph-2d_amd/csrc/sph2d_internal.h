@@ -108,7 +108,8 @@ int set_error(int code, const std::string& msg);
 void launch_init_tgv(hipStream_t s, const DevParams& P, Buffers& B, int nx, int ny);
 void launch_period_boundary(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int cap_ghost);
 // predict 1: the local copy also applies the predictor-corrector's predictor (k_gather_predict);
-// 2: and keeps the step-start state in before_* for k_stage's corrector
+// 2: and keeps the step-start state in before_* for k_stage's corrector; 3: as 1 without
+// the FP64 local v, p, c, rho, volume, mass (FP32-mixed steps whose later kernels do not read them)
 void launch_devide_block(hipStream_t s, const DevParams& P, Buffers& B, int cap_global, int n_cells,
                          bool analytic, double t_nondim, int predict = 0);
 // ev (optional, 4 events): recorded before and after pass A and before and after the force pass.
