@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
     const double2 po = B.l_pos[o];
     const float dx = (float)(ps.x - po.x), dy = (float)(ps.y - po.y);
     const float r2 = dx * dx + dy * dy;
-    const float inv_r = rsqrtf(r2);
+    const float inv_r = __builtin_amdgcn_rsqf(r2);   // r2 >= 1e-14: no denormal scaling
     const float r = r2 * inv_r;
     const KernelTerm32 K = kernel_term32(Q, dx, dy, r, inv_r);
     const float2 rvo = make_float2(r0o.z, fabsf(r0o.w) * __builtin_amdgcn_rcpf(Q.rho0 + r0o.z));
@@ -183,15 +183,29 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
   if (sf) atomicAdd(&B.counters[C_NSURF], 1);
 }
 
+// Buffer resource over a float4 array (raw addressing, byte offsets < 2^31;
+// word 3 = 0x00020000, the CDNA data-format setting for raw dword loads).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const float4* p) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, 0x7fffffff, 0x00020000);
+}
+
+__device__ __forceinline__ float4 buf_f4(__amdgpu_buffer_rsrc_t r, int idx) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  const u4 w = __builtin_amdgcn_raw_buffer_load_b128(r, idx * 16, 0, 0);
+  return make_float4(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z), __uint_as_float(w.w));
+}
+
 // pass C (mixed): see k_eval_c in sph2d_eval64.hip for the reference lines.
-// A block whose candidate union fits kRecCap copies the neighbour records to
-// LDS once and reads them there (fewer L2 reads); the others read global memory.
+// A block whose candidate union fits kRecCap copies the neighbour records r0 and r2
+// to LDS once and reads them there; r1 (tau, read only by the SPS term) comes from
+// L2, which keeps the block at 34 KB of LDS (4 blocks per CU instead of 3). Blocks
+// whose union does not fit read every record from global memory.
 constexpr int kRecCap = 1088;
 
 template <int SOLVER>
 __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q, Buffers B, int do_shift) {
   __shared__ NbStage st;
-  __shared__ float4 rec0[kRecCap], rec1[kRecCap], rec2[kRecCap];
+  __shared__ float4 rec0[kRecCap], rec2[kRecCap];
   const int blk = xcd_block();
   const int s = blk * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
@@ -199,7 +213,6 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
   for (int i = threadIdx.x; i < st.total; i += blockDim.x) {
     const int o = stage_source(st, i);
     rec0[i] = B.m_r0[o];
-    rec1[i] = B.m_r1[o];
     rec2[i] = B.m_r2[o];
   }
   __syncthreads();
@@ -238,6 +251,10 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
   float2 gps = make_float2(0.f, 0.f), gus = gps, gvs = gps;
   if (kMuscl) { gps = B.m_mg[3 * s]; gus = B.m_mg[3 * s + 1]; gvs = B.m_mg[3 * s + 2]; }
   KSum drho, corr, ax, ay, shx, shy, agx, agy;
+  // Buffer-resource views of the records for blocks without an LDS copy: buffer and
+  // LDS loads are different instructions, never merged into one flat load of a
+  // selected address (flat loads of LDS data are slower than ds_read)
+  const __amdgpu_buffer_rsrc_t br0 = rec_rsrc(B.m_r0), br1 = rec_rsrc(B.m_r1), br2 = rec_rsrc(B.m_r2);
   // The neighbours recorded by pass A (for_each_neighbour<kNbReplay>, written out:
   // a closure over this kernel's locals would be kept in scratch memory).
   const CellRanges R = cell_ranges(P, B.cell_start, cell_s);
@@ -279,14 +296,13 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
     const int o = d == 0 ? l0 + kk : (d == 1 ? l1 + kk : l2 + kk);
     const int li = d == 0 ? so0 + kk : (d == 1 ? so1 + kk : so2 + kk);
     float4 r0o, r1o, r2o;
+    r1o = buf_f4(br1, o);
     if (staged) {
       r0o = rec0[li];
-      r1o = rec1[li];
       r2o = rec2[li];
     } else {
-      r0o = B.m_r0[o];
-      r1o = B.m_r1[o];
-      r2o = B.m_r2[o];
+      r0o = buf_f4(br0, o);
+      r2o = buf_f4(br2, o);
     }
     if (ts != 2 && r0o.w < 0.f) continue;        // no pair between two non-fluid particles
     // bitwise blends, not selects: a select of captured values can become a
@@ -298,7 +314,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
     const float dx = (r2s.z - r2o.z) + (float)(1 - d) * Q.cell_d;
     const float dy = (r2s.w - r2o.w) + (float)dcy * Q.cell_d;
     const float r2 = dx * dx + dy * dy;
-    const float inv_r = rsqrtf(r2);
+    const float inv_r = __builtin_amdgcn_rsqf(r2);   // r2 >= 1e-14: no denormal scaling
     const float r = r2 * inv_r;
     const float inv_r2 = inv_r * inv_r;
     const KernelTerm32 K = kernel_term32(Q, dx, dy, r, inv_r);
