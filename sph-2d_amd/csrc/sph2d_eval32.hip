@@ -218,7 +218,6 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
   __syncthreads();
   const bool staged = st.use;                      // uniform over the block
   if (s >= n) return;
-  const double2 ps = B.l_pos[s];
   const float4 r0s = B.m_r0[s];
   const float4 taus = B.m_r1[s];
   const float4 r2s = B.m_r2[s];
@@ -282,7 +281,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
       else { m3 &= part; }
     }
   } else {
-    const Mask256 mk = exact_chunk_mask(P, B.l_pos, B.l_pos32, s, ps, n0, n01, l0, l1, l2, total, c0);
+    const Mask256 mk = exact_chunk_mask(P, B.l_pos, B.l_pos32, s, B.l_pos[s], n0, n01, l0, l1, l2, total, c0);
     m0 = mk.w0; m1 = mk.w1; m2 = mk.w2; m3 = mk.w3;
   }
   while ((m0 | m1 | m2 | m3) != 0) {

@@ -151,6 +151,35 @@ void launch_init_tgv(hipStream_t s, const DevParams& P, Buffers& B, int nx, int 
 // ---------------------------------------------------------------------------
 // Distribute_Period_Boundary_Particle.f90:56-206
 // ---------------------------------------------------------------------------
+// Exclusive scan in place of two arrays of nb ints that follow each other (block 0
+// scans a[0..nb), block 1 a[nb..2nb)); totals to *t0 and *t1. Each of the 1024
+// threads first sums a contiguous run (independent loads), then one block scan of
+// the run totals gives the runs' offsets.
+__global__ void __launch_bounds__(1024) k_scan_top2(int* a, int nb, int* t0, int* t1) {
+  __shared__ int lds[1024];
+  int* arr = a + (size_t)blockIdx.x * nb;
+  const int t = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int b0 = min(t * per, nb), b1 = min(b0 + per, nb);
+  int run = 0;
+  for (int i = b0; i < b1; ++i) run += arr[i];
+  lds[t] = run;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int add = t >= off ? lds[t - off] : 0;
+    __syncthreads();
+    lds[t] += add;
+    __syncthreads();
+  }
+  int ex = lds[t] - run;
+  for (int i = b0; i < b1; ++i) {
+    const int v = arr[i];
+    arr[i] = ex;
+    ex += v;
+  }
+  if (t == 1023) *(blockIdx.x ? t1 : t0) = lds[1023];
+}
+
 // The flags of one wrapped fluid position: x-ghost to the right of the box (l),
 // to the left (r), y-ghosts above (yl) and below (yr) (:98-148, :153-206).
 __device__ __forceinline__ void ghost_flags(const DevParams& P, double2 p, int& l, int& r, int& yl, int& yr) {
@@ -294,8 +323,7 @@ __global__ void k_ghost_yx(DevParams P, Buffers B, int cap_ghost) {
 void launch_period_boundary(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int cap_ghost) {
   const int nb = grid_for(cap_fluid);              // gblk holds 2 * nb ints
   hipLaunchKernelGGL(k_wrap_count, dim3(nb), dim3(kBlock), 0, s, P, B, cap_fluid);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, s, B.gblk, nb, B.counters + C_NXG, nullptr);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, s, B.gblk + nb, nb, B.counters + C_NYF, nullptr);
+  hipLaunchKernelGGL(k_scan_top2, dim3(2), dim3(1024), 0, s, B.gblk, nb, B.counters + C_NXG, B.counters + C_NYF);
   launch_fill_i32(s, B.cnt_yx, cap_ghost, 0);
   hipLaunchKernelGGL(k_ghost_emit, dim3(nb), dim3(kBlock), 0, s, P, B, cap_fluid, cap_ghost);
   exclusive_scan(s, B.cnt_yx, cap_ghost, B.off_yx, B.scan_blocks, B.counters + C_NYX);
