@@ -47,6 +47,24 @@ __device__ __forceinline__ double tait_p(const DevParams& P, double rho) {
   return P.eos_b * (x * x * x * x * x * x * x - 1.0);
 }
 
+// Buffer resource over a float4 array (raw addressing, byte offsets < 2^31;
+// word 3 = 0x00020000, the CDNA data-format setting for raw dword loads).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const float4* p) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, 0x7fffffff, 0x00020000);
+}
+
+__device__ __forceinline__ float4 buf_f4(__amdgpu_buffer_rsrc_t r, int idx) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  const u4 w = __builtin_amdgcn_raw_buffer_load_b128(r, idx * 16, 0, 0);
+  return make_float4(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z), __uint_as_float(w.w));
+}
+
+__device__ __forceinline__ double2 buf_d2(__amdgpu_buffer_rsrc_t r, int idx) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  const u4 w = __builtin_amdgcn_raw_buffer_load_b128(r, idx * 16, 0, 0);
+  return make_double2(__hiloint2double((int)w.y, (int)w.x), __hiloint2double((int)w.w, (int)w.z));
+}
+
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD b % 8),
 // each XCD with its own L2. The logical block of workgroup b gives every XCD one
 // contiguous run of blocks, so blocks whose neighbour columns overlap share an L2.

@@ -109,10 +109,13 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
   const int ts = B.l_type[s];
   KSum fra, sdx, sdy, Sxx, Sxy, Syy, grx, gry;
   KSum gpx, gpy, gux, guy, gvx, gvy;
+  const __amdgpu_buffer_rsrc_t br0 = rec_rsrc(B.m_r0), bpos = rec_rsrc((const float4*)B.l_pos);
   for_each_neighbour<kNbRecord>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) __attribute__((always_inline)) {
-    const float4 r0o = B.m_r0[o];
+    const float4 r0o = buf_f4(br0, o);
     if (ts != 2 && r0o.w < 0.f) return;          // no pair between two non-fluid particles
-    const double2 po = B.l_pos[o];
+    double2 po;
+    if (li >= 0) po = lds64[li];                  // staged: the exact test's copy
+    else po = buf_d2(bpos, o);
     const float dx = (float)(ps.x - po.x), dy = (float)(ps.y - po.y);
     const float r2 = dx * dx + dy * dy;
     const float inv_r = __builtin_amdgcn_rsqf(r2);   // r2 >= 1e-14: no denormal scaling
@@ -181,18 +184,6 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
   B.surf[s] = sf;
   B.near_[s] = 0;
   if (sf) atomicAdd(&B.counters[C_NSURF], 1);
-}
-
-// Buffer resource over a float4 array (raw addressing, byte offsets < 2^31;
-// word 3 = 0x00020000, the CDNA data-format setting for raw dword loads).
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const float4* p) {
-  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, 0x7fffffff, 0x00020000);
-}
-
-__device__ __forceinline__ float4 buf_f4(__amdgpu_buffer_rsrc_t r, int idx) {
-  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-  const u4 w = __builtin_amdgcn_raw_buffer_load_b128(r, idx * 16, 0, 0);
-  return make_float4(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z), __uint_as_float(w.w));
 }
 
 // pass C (mixed): see k_eval_c in sph2d_eval64.hip for the reference lines.
