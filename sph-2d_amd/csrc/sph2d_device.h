@@ -318,6 +318,7 @@ __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int
   const int s2 = staged ? l2 - st->lo[2] + st->base[2] : 0;
   float2 ps32 = make_float2(0.f, 0.f);
   if (!replay) ps32 = pos32[s];
+  const __amdgpu_buffer_rsrc_t rpos = rec_rsrc(reinterpret_cast<const float4*>(pos));
   for (int c0 = 0; c0 < total; c0 += 256) {
     uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
     if (replay) {
@@ -410,7 +411,9 @@ __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int
       const int li = staged ? (d == 0 ? s0 + kk : (d == 1 ? s1 + kk : s2 + kk)) : -1;   // LDS slot
       bool accept = true;                         // a replayed mask is exact
       if (!replay) {
-        const double2 po = staged ? lds64[li] : pos[o];
+        double2 po;
+        if (staged) po = lds64[li];
+        else po = buf_d2(rpos, o);                 // a buffer load: never merged with the LDS one
         accept = is_pair_r2(P, ps.x - po.x, ps.y - po.y);
         if (MODE == kNbRecord && accept) {
           if (w == 0) a0 |= bit; else if (w == 1) a1 |= bit; else if (w == 2) a2 |= bit; else a3 |= bit;
