@@ -1030,11 +1030,21 @@ int sph2d_download_local(sph2d_ctx* c, const char* name, void* out, int32_t n) {
   if (s == "sum_dwdx_y") return d2(B.sdw, 1);
   if (s == "normal_x") return d2(B.nrm, 0);
   if (s == "normal_y") return d2(B.nrm, 1);
-  if (s == "tau_xx") return d1(B.tau, 3, 0);
-  if (s == "tau_xy") return d1(B.tau, 3, 1);
-  if (s == "tau_yy") return d1(B.tau, 3, 2);
-  if (s == "grad_rho_x") return d2(B.grad, 0);
-  if (s == "grad_rho_y") return d2(B.grad, 1);
+  // the FP32-mixed pass A keeps tau and grad(rho) in its records only
+  auto f4 = [&](const float4* src, int comp) -> int {
+    std::vector<float4> t(nl);
+    HIP_OR_FAIL(hipMemcpyAsync(t.data(), src, nl * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+    double* o = static_cast<double*>(out);
+    for (int i = 0; i < nl; ++i) o[i] = comp == 0 ? t[i].x : (comp == 1 ? t[i].y : t[i].z);
+    return SPH2D_OK;
+  };
+  const bool mx = c->P.mixed != 0;
+  if (s == "tau_xx") return mx ? f4(B.m_r1, 0) : d1(B.tau, 3, 0);
+  if (s == "tau_xy") return mx ? f4(B.m_r1, 1) : d1(B.tau, 3, 1);
+  if (s == "tau_yy") return mx ? f4(B.m_r1, 2) : d1(B.tau, 3, 2);
+  if (s == "grad_rho_x") return mx ? f4(B.m_r2, 0) : d2(B.grad, 0);
+  if (s == "grad_rho_y") return mx ? f4(B.m_r2, 1) : d2(B.grad, 1);
   static const char* mg_names[6] = {"grad_p_x", "grad_p_y", "grad_u_x", "grad_u_y", "grad_v_x", "grad_v_y"};
   for (int k = 0; k < 6; ++k)
     if (s == mg_names[k]) {

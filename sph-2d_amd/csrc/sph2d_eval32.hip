@@ -148,7 +148,6 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
   const double f = (double)fra.s;
   B.fra_v[s] = f;
   B.sdw[s] = make_double2((double)sdx.s, (double)sdy.s);
-  B.grad[s] = make_double2((double)grx.s, (double)gry.s);
   {
     const float4 r2s = B.m_r2[s];                  // keep the position offsets
     B.m_r2[s] = make_float4(grx.s, gry.s, r2s.z, r2s.w);
@@ -177,9 +176,6 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
     tyy = (2 * nut * syy - kk - blin) * rhos;
   }
   B.m_r1[s] = make_float4((float)txx, (float)txy, (float)tyy, B.m_r1[s].w);
-  B.tau[3 * s + 0] = txx;
-  B.tau[3 * s + 1] = txy;
-  B.tau[3 * s + 2] = tyy;
   const int sf = (ts == 2 && f <= P.fra_limit) ? 1 : 0;
   B.surf[s] = sf;
   B.near_[s] = 0;

@@ -162,7 +162,13 @@ __global__ void __launch_bounds__(1024) k_scan_top2(int* a, int nb, int* t0, int
   const int per = (nb + 1023) / 1024;
   const int b0 = min(t * per, nb), b1 = min(b0 + per, nb);
   int run = 0;
-  for (int i = b0; i < b1; ++i) run += arr[i];
+  for (int c = b0; c < b1; c += 16) {            // 16 loads in flight at a time
+    int v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = c + k < b1 ? arr[c + k] : 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) run += v[k];
+  }
   lds[t] = run;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
@@ -172,10 +178,15 @@ __global__ void __launch_bounds__(1024) k_scan_top2(int* a, int nb, int* t0, int
     __syncthreads();
   }
   int ex = lds[t] - run;
-  for (int i = b0; i < b1; ++i) {
-    const int v = arr[i];
-    arr[i] = ex;
-    ex += v;
+  for (int c = b0; c < b1; c += 16) {
+    int v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = c + k < b1 ? arr[c + k] : 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (c + k < b1) arr[c + k] = ex;
+      ex += v[k];
+    }
   }
   if (t == 1023) *(blockIdx.x ? t1 : t0) = lds[1023];
 }
@@ -879,6 +890,15 @@ void launch_grid_mls(hipStream_t s, const DevParams& P, Buffers& B, const GridAr
   hipLaunchKernelGGL(k_grid_mls, dim3(grid_for(G.n_nodes)), dim3(kBlock), 0, s, P, B, G);
 }
 
+// The density gradient of pass A: FP64 array, or the FP32-mixed record.
+__device__ __forceinline__ double2 grad_rho(const DevParams& P, const Buffers& B, int s) {
+  if (P.mixed) {
+    const float4 r2 = B.m_r2[s];
+    return make_double2((double)r2.x, (double)r2.y);
+  }
+  return B.grad[s];
+}
+
 // ---------------------------------------------------------------------------
 // Shift_particle_position.f90:108-136 (after an optional MLS reinitialisation:
 // mode 1 = take rho from mls_rho first, 2 = only commit mls_rho, 0 = shift only)
@@ -904,7 +924,7 @@ __global__ void k_shift_apply(DevParams P, Buffers B, int mode) {
   x.x = x.x + sh.x;
   x.y = x.y + sh.y;
   B.l_pos[s] = x;
-  const double2 gr = B.grad[s];
+  const double2 gr = grad_rho(P, B, s);
   rho = rho + gr.x * sh.x;
   rho = rho + gr.y * sh.y;
   B.l_rv[s] = make_double2(rho, B.l_rv[s].y);
@@ -963,7 +983,7 @@ __device__ __forceinline__ void shift_one(const DevParams& P, const Buffers& B, 
   }
   x.x = x.x + sh.x;
   x.y = x.y + sh.y;
-  const double2 gr = B.grad[s];
+  const double2 gr = grad_rho(P, B, s);
   rho = rho + gr.x * sh.x;
   rho = rho + gr.y * sh.y;
   p = tait_p(P, rho);
