@@ -57,6 +57,15 @@ def force_pass_bytes_per_particle(form: int) -> int:
     return (164 if form == 64 else 92) + 40
 
 
+# HBM bytes per launch of the FP32-mixed force pass (k_eval_c32) at 2048^2, measured by
+# rocprofv3 --pmc FETCH_SIZE and then WRITE_SIZE on this build (summary and commands in
+# profiles/tgv2048_force_pass_hbm_traffic.md), each counter scaled by what it reads for a
+# 256 MiB device-to-device copy (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00): 392,315 KiB
+# fetched x 2.00 + 165,123 KiB written = 570.8 MB for 4,227,136 subdomain particles.
+FORCE_PASS_TRAFFIC = {"bytes_per_particle": 135.0, "n_side": 2048,
+                      "source": "profiles/tgv2048_force_pass_hbm_traffic.md"}
+
+
 def cpu_baseline(n_side: int = 512, steps: int = 20) -> dict:
     """The C oracle (oracle/liboracle.so, the reference's algorithm restated,
     scatter over a half pair list as in find_particle_pair.f90), one core, on a
@@ -214,7 +223,9 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": ach32 / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": (FORCE_PASS_TRAFFIC["bytes_per_particle"] * r32["n_local"]
+                        if n == FORCE_PASS_TRAFFIC["n_side"] else None),
+            "traffic_source": FORCE_PASS_TRAFFIC["source"],
             "bytes_per_particle": force_pass_bytes_per_particle(32),
             "units_per_launch": r32["n_local"],
             "duration_ms": r32["force_ms"],
