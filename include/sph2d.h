@@ -217,11 +217,53 @@ int sph2d_calculation_for_postproceeding(sph2d_ctx* ctx, int32_t i_time_step, in
 int sph2d_output_for_grid_postproceding(sph2d_ctx* ctx, int32_t i_time_step, const char* grid_path,
                                         const char* binary_path);
 
+/* Sampling_subroutine.f90:21-177 (Sampling) at n points pos (ld,2) after the last
+ * step (a multiple of sample_every): over the fluid particles within 3h of each
+ * point, the average pressure, the order-6 MLS pressure (less g rho_j (y - y_j) Fai_j
+ * when gravity != 0, Gravity_Switch 'on') and the MLS velocity vel (ld,2); zeros
+ * for a point with 3 or fewer particles in its support. Any output may be NULL. */
+int sph2d_sampling(sph2d_ctx* ctx, int32_t i_time_step, int32_t n, const double* pos, int32_t ld, double gravity,
+                   double* average_press, double* mls_press, double* vel);
+/* Sampling_subroutine.f90:246-279 after any step: the wave front (largest x of the
+ * fluid particles, from 0) and the elevation at n_probes wave probes at x = probe_x
+ * (largest y of the fluid particles within 0.55 dx, from 0), unscaled. */
+int sph2d_wave_probes(sph2d_ctx* ctx, int32_t n_probes, const double* probe_x, double* wave_front,
+                      double* elevation);
+/* Sampling_subroutine.f90:466-512: for each of n_lines lines from first(2,k) to
+ * last(2,k) (n_points points), sph2d_sampling at the points and the analytic field,
+ * written to dir/<k>th_<t>_SamplingLine_Results.dat. */
+int sph2d_output_sampling_lines(sph2d_ctx* ctx, int32_t i_time_step, const char* dir, int32_t n_lines,
+                                const double* first, const double* last, int32_t n_points);
+
 /* Host-array writers (no device; Fortran layout, ld >= n). Text follows the
  * reference's edit descriptors exactly; files named "appended" get the
  * reference's header lines when they are new or empty. */
 /* Fw.d ('F') or Ew.dE3 ('E') of one value into out (cap > w). */
 int sph2d_format_fortran(double value, char kind, int32_t w, int32_t d, char* out, int32_t cap);
+/* One REAL(8) list-directed item as ifort writes it (23 characters; cap > 23). */
+int sph2d_format_list_directed(double value, char* out, int32_t cap);
+/* write(unit,*) of n REAL(8) values appended to path (Wave_front_results.dat and
+ * the wave gauge files, Sampling_subroutine.f90:283,287). */
+int sph2d_write_list_directed(const char* path, int32_t n, const double* values);
+/* write(unit,'(20F20.8)') of n values appended to path (the pressure probe files,
+ * Sampling_subroutine.f90:318-322). */
+int sph2d_write_f20_8(const char* path, int32_t n, const double* values);
+/* Sampling_subroutine.f90:300-322: Kalman filter of n measurements; last_x, last_p
+ * carry the state between samplings (set on current_time_number == 1). */
+int sph2d_kalman_filter(int32_t n, int32_t current_time_number, const double* measured, double* last_x,
+                        double* last_p, double* filtered);
+/* Initialize_SPH_Calculation.f90:385-398: n_points points from first(2) to last(2),
+ * columns x, y of out (ld,2). */
+int sph2d_sampling_line_points(int32_t n_points, const double* first, const double* last, double* out, int32_t ld);
+/* Initialize_SPH_Calculation.f90:405-436: dir/<line>th_SamplingLine_Position.dat. */
+int sph2d_write_sampling_line_positions(const char* dir, int32_t line, int32_t n_points, const double* pos,
+                                        int32_t ld);
+/* Sampling_subroutine.f90:490-512: dir/<line>th_<t_nondim, F7.4>_SamplingLine_Results.dat
+ * (20f12.6: x y, analytic u v p, MLS u v p, average p, normalised by sc). */
+int sph2d_write_sampling_line_results(const char* dir, int32_t line, double t_nondim, int32_t n_points,
+                                      const double* pos, const double* ana_vel, const double* ana_press,
+                                      const double* vel, const double* mls_press, const double* average_press,
+                                      int32_t ld, const sph2d_scales* sc);
 /* Parallel_Taylor_Green_Vortices_2D_Release.f90:185-199 (Final_Doamin_Results.dat). */
 int sph2d_write_final_results(const char* path, int32_t n, const double* pos, const double* vel,
                               const double* rho, const double* press, int32_t ld);

@@ -72,6 +72,9 @@ def lib():
         L.orc_sample.argtypes = [ctypes.c_void_p, P(ctypes.c_double)]
         L.orc_grid_postprocess.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                            ctypes.c_double, P(ctypes.c_double)]
+        L.orc_sampling_points.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_double), P(ctypes.c_double),
+                                          ctypes.c_double, P(ctypes.c_double)]
+        L.orc_wave_probes.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_double), P(ctypes.c_double)]
         L.orc_initial_ke.argtypes = [ctypes.c_void_p]
         L.orc_initial_ke.restype = ctypes.c_double
         for name in ("orc_n_fluid", "orc_n_global", "orc_n_local"):
@@ -180,6 +183,23 @@ class Oracle:
         out = np.zeros(9 * nn)
         lib().orc_grid_postprocess(self._c, it, nex, ney, edx, _dp(out))
         return out.reshape(9, nn).T.copy()
+
+    def sampling_points(self, x, y, gravity: float = 0.0) -> np.ndarray:
+        """Sampling (Sampling_subroutine.f90:21-177) at the points: (n, 4) columns
+        average p, MLS p, MLS u, MLS v."""
+        px = np.ascontiguousarray(x, dtype=np.float64)
+        py = np.ascontiguousarray(y, dtype=np.float64)
+        n = len(px)
+        out = np.zeros(4 * max(n, 1))
+        lib().orc_sampling_points(self._c, n, _dp(px), _dp(py), gravity, _dp(out))
+        return out[:4 * n].reshape(4, n).T.copy()
+
+    def wave_probes(self, probe_x=()):
+        """(wave front, elevations) of Sampling_subroutine.f90:246-279, unscaled."""
+        px = np.ascontiguousarray(probe_x, dtype=np.float64).reshape(-1)
+        out = np.zeros(1 + len(px))
+        lib().orc_wave_probes(self._c, len(px), _dp(px) if len(px) else None, _dp(out))
+        return out[0], out[1:]
 
     def sample(self) -> np.ndarray:
         out = np.zeros(14)

@@ -1490,6 +1490,77 @@ void orc_reorder_particle_index(int32_t n_total, int32_t n_fluid, double box_x, 
 
 
 /* ------------------------------------------------------------------ */
+/* The support of a point in the global chain (Calculation_For_PostProceeding.f90 */
+/* :184-250, Sampling_subroutine.f90:55-111): fluid particles (gid < N) of the     */
+/* 9-cell stencil of the point's cell, in stencil then chain order, at distance    */
+/* within [1e-7 (REAL(4)), 3h]; tn = their gids, wk = W. Returns the count.        */
+/* ------------------------------------------------------------------ */
+static int point_support(orc_ctx* c, double ex, double ey, int N, int* eff, int* tn, double* wk) {
+  const double h = c->h;
+  int xn = (int)floor((ex - c->chain_ox) / c->chain_d + 1);
+  int yn = (int)floor((ey - c->chain_oy) / c->chain_d + 1);
+  if (xn < 1 || xn > c->chain_nx || yn < 1 || yn > c->chain_ny) return 0;
+  int mesh = (yn - 1) * c->chain_nx + xn;
+  int nm[9];
+  int nnear = stencil_names(mesh, c->chain_nx, c->chain_ny, nm);
+  int ne = 0;
+  for (int q = 0; q < nnear; ++q) {
+    int cell = nm[q];
+    for (int m = c->gchain_off[cell]; m < c->gchain_off[cell + 1]; ++m) {
+      int jj = c->gchain_members[m];
+      if (jj < N && ne < 300) eff[ne++] = jj;
+    }
+  }
+  int nt = 0;
+  for (int q = 0; q < ne; ++q) {
+    int jj = eff[q];
+    double pd[2] = {ex - c->gx[jj], ey - c->gy[jj]};
+    double d = sqrt(pd[0] * pd[0] + pd[1] * pd[1]);
+    if (d <= 3.0 * h && d >= (double)1.0e-7f) {
+      double w, dw[2];
+      orc_compute_kernel(2, d, pd, h, &w, dw);
+      tn[nt] = jj; wk[nt] = w; ++nt;
+    }
+  }
+  return nt;
+}
+
+/* MLS_Interpolation_In_Gobal_Domain.f90:110-250, order 6, at (ex, ey): Fai of */
+/* the nt support particles (Shepard W / sum W with 10 or fewer, or when the   */
+/* moment matrix has no inverse). pm: 6 x nt scratch.                          */
+static void point_mls_fai(orc_ctx* c, double ex, double ey, int nt, const int* tn, const double* wk, double* pm,
+                          double* fai) {
+  const double h = c->h, al = c->P.mls_alpha;
+  for (int q = 0; q < nt; ++q) {
+    int jj = tn[q];
+    double a = (c->gx[jj] - ex) / h, b = (c->gy[jj] - ey) / h;
+    double* col = &pm[6 * q];
+    col[0] = 1; col[1] = a; col[2] = b; col[3] = a * a; col[4] = a * b; col[5] = b * b;
+  }
+  double A[36];                                       /* column-major 6x6 */
+  for (int m = 0; m < 6; ++m)
+    for (int l = 0; l < 6; ++l) {
+      double v = 0.0;
+      for (int q = 0; q < nt; ++q) v = v + (pm[6 * q + m] * wk[q]) * pm[6 * q + l];
+      for (int L = 0; L < 6; ++L) v = v + ((m == L ? al : 0.0) * al) * (l == L ? al : 0.0);
+      A[m + 6 * l] = v;
+    }
+  int ok = orc_brinv(A, 6);
+  double sw = 0.0;
+  for (int q = 0; q < nt; ++q) sw = sw + wk[q];
+  for (int q = 0; q < nt; ++q) {
+    if (ok) {
+      double v = 0.0;
+      for (int m = 0; m < 6; ++m) v = v + A[0 + 6 * m] * (pm[6 * q + m] * wk[q]);
+      fai[q] = v;
+    } else {
+      fai[q] = wk[q] / sw;
+    }
+    if (nt <= 10) fai[q] = wk[q] / sw;
+  }
+}
+
+/* ------------------------------------------------------------------ */
 /* Calculation_For_PostProceeding.f90:96-300 (one rank) on the element nodes of  */
 /* Distribute_Grid_For_Postproceeding.f90:30-39 (node k = j + (i-1) nnx at       */
 /* ((j-1) edx, (i-1) edx)), with MLS_Interpolation_In_Gobal_Domain.f90 order 6.  */
@@ -1503,7 +1574,6 @@ void orc_grid_postprocess(orc_ctx* c, int32_t i_time_step, int32_t nex, int32_t 
   const double h = c->h, pi = PI_MODULE;
   const double t = i_time_step * P->dt / (P->box_x / P->U);
   const double Re = P->U * P->box_x / P->nu;
-  const double al = P->mls_alpha;
   memset(out, 0, sizeof(double) * 9 * (size_t)nn);
   /* :96-102 surface particles in gid order */
   int ns = 0;
@@ -1541,61 +1611,9 @@ void orc_grid_postprocess(orc_ctx* c, int32_t i_time_step, int32_t nex, int32_t 
     }
     out[4 * (size_t)nn + k] = f;
     if (f <= -0.9) continue;
-    /* :184-215 candidates: the 9-cell stencil of the node's cell, fluid particles only */
-    int xn = (int)floor((ex - c->chain_ox) / c->chain_d + 1);
-    int yn = (int)floor((ey - c->chain_oy) / c->chain_d + 1);
-    int mesh = (yn - 1) * c->chain_nx + xn;
-    int nm[9];
-    int nnear = stencil_names(mesh, c->chain_nx, c->chain_ny, nm);
-    int ne = 0;
-    for (int q = 0; q < nnear; ++q) {
-      int cell = nm[q];
-      for (int m = c->gchain_off[cell]; m < c->gchain_off[cell + 1]; ++m) {
-        int jj = c->gchain_members[m];
-        if (jj < N && ne < 300) eff[ne++] = jj;
-      }
-    }
-    /* :219-250 the particles in the support, with W */
-    int nt = 0;
-    for (int q = 0; q < ne; ++q) {
-      int jj = eff[q];
-      double pd[2] = {ex - c->gx[jj], ey - c->gy[jj]};
-      double d = sqrt(pd[0] * pd[0] + pd[1] * pd[1]);
-      if (d <= 3.0 * h && d >= (double)1.0e-7f) {
-        double w, dw[2];
-        orc_compute_kernel(2, d, pd, h, &w, dw);
-        tn[nt] = jj; wk[nt] = w; ++nt;
-      }
-    }
+    int nt = point_support(c, ex, ey, N, eff, tn, wk);
     if (nt < 3) continue;
-    /* MLS_Interpolation_In_Gobal_Domain.f90:110-250, order 6 */
-    for (int q = 0; q < nt; ++q) {
-      int jj = tn[q];
-      double a = (c->gx[jj] - ex) / h, b = (c->gy[jj] - ey) / h;
-      double* col = &pm[6 * q];
-      col[0] = 1; col[1] = a; col[2] = b; col[3] = a * a; col[4] = a * b; col[5] = b * b;
-    }
-    double A[36];                                       /* column-major 6x6 */
-    for (int m = 0; m < 6; ++m)
-      for (int l = 0; l < 6; ++l) {
-        double v = 0.0;
-        for (int q = 0; q < nt; ++q) v = v + (pm[6 * q + m] * wk[q]) * pm[6 * q + l];
-        for (int L = 0; L < 6; ++L) v = v + ((m == L ? al : 0.0) * al) * (l == L ? al : 0.0);
-        A[m + 6 * l] = v;
-      }
-    int ok = orc_brinv(A, 6);
-    double sw = 0.0;
-    for (int q = 0; q < nt; ++q) sw = sw + wk[q];
-    for (int q = 0; q < nt; ++q) {
-      if (ok) {
-        double v = 0.0;
-        for (int m = 0; m < 6; ++m) v = v + A[0 + 6 * m] * (pm[6 * q + m] * wk[q]);
-        fai[q] = v;
-      } else {
-        fai[q] = wk[q] / sw;
-      }
-      if (nt <= 10) fai[q] = wk[q] / sw;
-    }
+    point_mls_fai(c, ex, ey, nt, tn, wk, pm, fai);
     double su = 0.0, sv = 0.0, sp = 0.0, so = 0.0;
     for (int q = 0; q < nt; ++q) {
       int jj = tn[q];
@@ -1608,4 +1626,58 @@ void orc_grid_postprocess(orc_ctx* c, int32_t i_time_step, int32_t nex, int32_t 
     out[2 * (size_t)nn + k] = sp; out[3 * (size_t)nn + k] = so;
   }
   free(sname); free(eff); free(tn); free(wk); free(pm); free(fai);
+}
+
+/* ------------------------------------------------------------------ */
+/* Sampling_subroutine.f90:21-177 (Sampling) at n points (px, py) of the       */
+/* state after Refresh_domain: with more than 3 particles in the support, the   */
+/* average pressure over the support, the order-6 MLS pressure (:160-164, minus */
+/* g rho_j (y_s - y_j) Fai_j when gravity != 0) and the MLS velocity; zeros     */
+/* otherwise. out: 4 blocks of n (average p, MLS p, u, v).                      */
+/* ------------------------------------------------------------------ */
+void orc_sampling_points(orc_ctx* c, int32_t n, const double* px, const double* py, double gravity, double* out) {
+  const int N = c->n_fluid;
+  int* eff = xcalloc(300, sizeof(int));
+  int* tn = xcalloc(300, sizeof(int));
+  double* wk = xcalloc(300, sizeof(double));
+  double* pm = xcalloc(6 * 306, sizeof(double));
+  double* fai = xcalloc(300, sizeof(double));
+  memset(out, 0, sizeof(double) * 4 * (size_t)n);
+  for (int k = 0; k < n; ++k) {
+    const double sx = px[k], sy = py[k];
+    int nt = point_support(c, sx, sy, N, eff, tn, wk);
+    if (nt <= 3) continue;
+    double avg = 0.0, mp = 0.0, su = 0.0, sv = 0.0;
+    for (int q = 0; q < nt; ++q) avg = avg + c->gp[tn[q]];
+    point_mls_fai(c, sx, sy, nt, tn, wk, pm, fai);
+    for (int q = 0; q < nt; ++q) {
+      int jj = tn[q];
+      if (gravity != 0.0) mp = mp + c->gp[jj] * fai[q] - gravity * c->grho[jj] * (sy - c->gy[jj]) * fai[q];
+      else mp = mp + c->gp[jj] * fai[q];
+      su = su + c->gu[jj] * fai[q];
+      sv = sv + c->gv[jj] * fai[q];
+    }
+    out[k] = avg / nt;
+    out[(size_t)n + k] = mp;
+    out[2 * (size_t)n + k] = su;
+    out[3 * (size_t)n + k] = sv;
+  }
+  free(eff); free(tn); free(wk); free(pm); free(fai);
+}
+
+/* Sampling_subroutine.f90:246-279: the wave front (largest x of the fluid        */
+/* particles, from 0) and each probe's elevation (largest y, from 0, of the fluid */
+/* particles within 0.55 (REAL(4)) dx of the probe's x). out: 1 + n_probes.        */
+void orc_wave_probes(orc_ctx* c, int32_t n_probes, const double* probe_x, double* out) {
+  const int N = c->n_fluid;
+  double front = 0.0;
+  for (int j = 0; j < n_probes; ++j) out[1 + j] = 0.0;
+  for (int i = 0; i < N; ++i) {
+    for (int j = 0; j < n_probes; ++j) {
+      double d = fabs(probe_x[j] - c->gx[i]);
+      if (d <= (double)0.55f * c->dx && c->gy[i] > out[1 + j]) out[1 + j] = c->gy[i];
+    }
+    if (c->gx[i] >= front) front = c->gx[i];
+  }
+  out[0] = front;
 }

@@ -97,6 +97,8 @@ void orc_refresh(orc_ctx* c);                               /* Refresh_domain (1
 /* Calculation_For_PostProceeding on the (nex+1)x(ney+1) element nodes, spacing edx;
  * out = 9 blocks of n_nodes: u, v, p, vorticity_z, Fai, analytic u, v, p, |v|. */
 void orc_grid_postprocess(orc_ctx* c, int32_t i_time_step, int32_t nex, int32_t ney, double edx, double* out);
+void orc_sampling_points(orc_ctx* c, int32_t n, const double* px, const double* py, double gravity, double* out);
+void orc_wave_probes(orc_ctx* c, int32_t n_probes, const double* probe_x, double* out);
 
 /* Sampling_subroutine.f90:340-461. out[0]=KE, [1]=max|v|, [2..4]=u Linf,L1,L2,
  * [5..7]=v, [8..10]=p/(rho0 U^2), [11..13]=|v|. */

@@ -59,6 +59,9 @@ struct sph2d_ctx {
   int* surf_gid = nullptr;
   double2* surf_nrm = nullptr;
   size_t surf_cap = 0;
+  // sampling points and wave probes (allocated on first use, in doubles)
+  double* probe_buf = nullptr;
+  size_t probe_cap = 0;
   bool pass_timing = false;
   std::vector<hipEvent_t> events;
   size_t events_used = 0;
@@ -955,6 +958,111 @@ int sph2d_output_for_grid_postproceding(sph2d_ctx* c, int32_t it, const char* gr
   }
   if (binary_path) {
     rc = sph2d_write_grid_binary(binary_path, t, nex, ney, edx, vel.data(), press.data(), fai.data(), nn, &sc);
+    if (rc) return rc;
+  }
+  return SPH2D_OK;
+}
+
+namespace {
+int probe_space(sph2d_ctx* c, size_t n) {
+  if (n > c->probe_cap) {
+    const int rc = c->alloc(&c->probe_buf, n);
+    if (rc) return rc;
+    c->probe_cap = n;
+  }
+  return SPH2D_OK;
+}
+}  // namespace
+
+int sph2d_sampling(sph2d_ctx* c, int32_t it, int32_t n, const double* pos, int32_t ld, double gravity,
+                   double* average_press, double* mls_press, double* vel) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && !pos)) return fail(SPH2D_ERR_ARGUMENT, "n < 0 or pos is NULL");
+  if (ld < n) return fail(SPH2D_ERR_ARGUMENT, "ld < n");
+  if (it != c->ana_step || it != c->vort_step)
+    return fail(SPH2D_ERR_STATE, "i_time_step must be the last step run and a multiple of sample_every");
+  if (n == 0) return SPH2D_OK;
+  const double lo_x = c->P.chain_ox, lo_y = c->P.chain_oy;
+  const double hi_x = lo_x + c->P.chain_nx * c->P.chain_d, hi_y = lo_y + c->P.chain_ny * c->P.chain_d;
+  std::vector<double> h((size_t)2 * n);
+  for (int k = 0; k < n; ++k) {
+    const double x = pos[k], y = pos[(size_t)ld + k];
+    if (!(x >= lo_x && x < hi_x && y >= lo_y && y < hi_y))
+      return fail(SPH2D_ERR_ARGUMENT, "sampling point " + std::to_string(k + 1) + " lies outside the background grid");
+    h[2 * k] = x;
+    h[2 * k + 1] = y;
+  }
+  rc = probe_space(c, (size_t)6 * n);
+  if (rc) return rc;
+  HIP_OR_FAIL(hipMemcpyAsync(c->probe_buf, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  ProbeArgs Q;
+  Q.n = n;
+  Q.gravity = gravity;
+  Q.pos = reinterpret_cast<const double2*>(c->probe_buf);
+  Q.out = c->probe_buf + (size_t)2 * n;
+  launch_probe_mls(c->stream, c->P, c->B, Q);
+  HIP_OR_FAIL(hipGetLastError());
+  std::vector<double> o((size_t)4 * n);
+  HIP_OR_FAIL(hipMemcpyAsync(o.data(), Q.out, o.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < n; ++k) {
+    if (average_press) average_press[k] = o[k];
+    if (mls_press) mls_press[k] = o[(size_t)n + k];
+    if (vel) { vel[k] = o[(size_t)2 * n + k]; vel[(size_t)ld + k] = o[(size_t)3 * n + k]; }
+  }
+  return SPH2D_OK;
+}
+
+int sph2d_wave_probes(sph2d_ctx* c, int32_t n_probes, const double* probe_x, double* wave_front, double* elevation) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (n_probes < 0 || (n_probes > 0 && (!probe_x || !elevation)))
+    return fail(SPH2D_ERR_ARGUMENT, "n_probes < 0 or a NULL array");
+  rc = probe_space(c, (size_t)2 * n_probes + 1);
+  if (rc) return rc;
+  // probe x, then the 1 + n_probes maxima as bit patterns (zero = +0.0)
+  std::vector<double> h((size_t)2 * n_probes + 1, 0.0);
+  for (int j = 0; j < n_probes; ++j) h[j] = probe_x[j];
+  HIP_OR_FAIL(hipMemcpyAsync(c->probe_buf, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  auto* bits = reinterpret_cast<unsigned long long*>(c->probe_buf + n_probes);
+  launch_wave_probes(c->stream, c->P, c->B, c->cap_fluid, c->probe_buf, n_probes, bits);
+  HIP_OR_FAIL(hipGetLastError());
+  std::vector<double> o((size_t)n_probes + 1);
+  HIP_OR_FAIL(hipMemcpyAsync(o.data(), bits, o.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  if (wave_front) *wave_front = o[0];
+  for (int j = 0; j < n_probes; ++j) elevation[j] = o[1 + j];
+  return SPH2D_OK;
+}
+
+int sph2d_output_sampling_lines(sph2d_ctx* c, int32_t it, const char* dir, int32_t n_lines, const double* first,
+                                const double* last, int32_t n_points) {
+  if (!c || !dir || n_lines < 0 || (n_lines > 0 && (!first || !last)) || n_points < 2)
+    return fail(SPH2D_ERR_ARGUMENT, "bad argument (n_points must be at least 2)");
+  sph2d_scales sc;
+  int rc = sph2d_get_scales(c, &sc);
+  if (rc) return rc;
+  const double tn = it * c->prm.dt / (c->prm.box_x / c->prm.U);   // Nondimensionalized_real_time
+  const double Re = c->prm.U * c->prm.box_x / c->prm.nu;
+  const size_t n = n_points;
+  std::vector<double> pos(2 * n), avg(n), mls(n), vel(2 * n), ana_v(2 * n), ana_p(n);
+  for (int k = 0; k < n_lines; ++k) {
+    rc = sph2d_sampling_line_points(n_points, first + 2 * k, last + 2 * k, pos.data(), n_points);
+    if (rc) return rc;
+    rc = sph2d_sampling(c, it, n_points, pos.data(), n_points, 0.0, avg.data(), mls.data(), vel.data());
+    if (rc) return rc;
+    // Sampling_subroutine.f90:481-485
+    const double pi = kPiModule, U = c->prm.U;
+    for (size_t i = 0; i < n; ++i) {
+      const double x = pos[i], y = pos[n + i];
+      ana_v[i] = U * std::sin(2 * pi * x) * std::cos(2 * pi * y) * std::exp(-8 * (pi * pi) / Re * tn);
+      ana_v[n + i] = -U * std::cos(2 * pi * x) * std::sin(2 * pi * y) * std::exp(-8 * (pi * pi) / Re * tn);
+      ana_p[i] = 0.25 * c->prm.rho0 * (U * U) * (std::cos(4 * pi * x) + std::cos(4 * pi * y)) *
+                 std::exp(-16 * (pi * pi) / Re * tn);
+    }
+    rc = sph2d_write_sampling_line_results(dir, k + 1, tn, n_points, pos.data(), ana_v.data(), ana_p.data(),
+                                           vel.data(), mls.data(), avg.data(), n_points, &sc);
     if (rc) return rc;
   }
   return SPH2D_OK;

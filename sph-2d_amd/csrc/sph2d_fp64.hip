@@ -770,6 +770,75 @@ void launch_mls(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
   hipLaunchKernelGGL(k_mls, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B);
 }
 
+// The support of a point for the global-domain MLS (Calculation_For_PostProceeding.f90
+// :184-250, Sampling_subroutine.f90:60-95): the particles with gid < n_max (fluid,
+// no periodic ghosts) of the point's 3x3 cells in the step-start binning whose
+// refreshed position lies within [1e-7 (REAL(4)), 3h]; f(gid, position, W).
+template <class F>
+__device__ __forceinline__ void for_each_support(const DevParams& P, const Buffers& B, double ex, double ey,
+                                                 int n_max, F&& f) {
+  const int xn = (int)floor((ex - P.chain_ox) / P.chain_d + 1);
+  const int yn = (int)floor((ey - P.chain_oy) / P.chain_d + 1);
+  if (xn < 1 || xn > P.chain_nx || yn < 1 || yn > P.chain_ny) return;
+  const CellRanges R = cell_ranges(P, B.cell_start, (xn - 1) * P.chain_ny + (yn - 1));
+  const double rmin = (double)1.0e-7f, rmax = P.three_h;
+  for (int d = 0; d < 3; ++d)
+    for (int o = R.lo[d]; o < R.hi[d]; ++o) {
+      const int g = B.name[o];
+      if (g >= n_max) continue;
+      const double2 pj = B.g_pos[g];
+      const double pdx = ex - pj.x, pdy = ey - pj.y;
+      const double r = sqrt(pdx * pdx + pdy * pdy);
+      if (!(r <= rmax && r >= rmin)) continue;
+      double w, dwx, dwy;
+      kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
+      f(g, pj, w);
+    }
+}
+
+// MLS_Interpolation_In_Gobal_Domain.f90, order 6, at (ex, ey): y = row 1 of the
+// moment inverse (mls = true) or Shepard weights (10 or fewer particles in the
+// support, or no Cholesky factor); sw = sum of W. Returns the support count.
+__device__ __forceinline__ int point_mls_weights(const DevParams& P, const Buffers& B, double ex, double ey,
+                                                 int n_max, double* y, double& sw, bool& mls) {
+  double A[21];
+#pragma unroll
+  for (int q = 0; q < 21; ++q) A[q] = 0.0;
+  sw = 0.0;
+  int nt = 0;
+  for_each_support(P, B, ex, ey, n_max, [&](int g, double2 pj, double w) {
+    (void)g;
+    const double a = (pj.x - ex) / P.h, b = (pj.y - ey) / P.h;
+    const double pv[6] = {1.0, a, b, a * a, a * b, b * b};
+    int q = 0;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      const double bm = pv[m] * w;
+#pragma unroll
+      for (int l = m; l < 6; ++l) { A[q] = A[q] + bm * pv[l]; ++q; }
+    }
+    sw = sw + w;
+    ++nt;
+  });
+#pragma unroll
+  for (int m = 0; m < 6; ++m) y[m] = 0.0;
+  mls = nt > 10;
+  if (mls) mls = mls6_first_row(A, (P.mls_alpha * P.mls_alpha) * P.mls_alpha, y);
+  return nt;
+}
+
+// Fai of one support particle: w (A^-1 p)_1 = sum_m y_m (p_m w), or w / sum W.
+__device__ __forceinline__ double point_mls_fai(const DevParams& P, const double* y, double sw, bool mls, double ex,
+                                                double ey, double2 pj, double w) {
+  if (!mls) return w / sw;
+  const double a = (pj.x - ex) / P.h, b = (pj.y - ey) / P.h;
+  const double pv[6] = {1.0, a, b, a * a, a * b, b * b};
+  double fai = 0.0;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) fai = fai + y[m] * (pv[m] * w);
+  return fai;
+}
+
 // ---------------------------------------------------------------------------
 // Calculation_For_PostProceeding.f90:96-300 on the element nodes of
 // Distribute_Grid_For_Postproceeding.f90:30-39 (node k = j + (i-1) nnx at
@@ -817,68 +886,18 @@ __global__ void __launch_bounds__(kBlock) k_grid_mls(DevParams P, Buffers B, Gri
   G.out[4 * nn + k] = f;
   double su = 0.0, sv = 0.0, sp = 0.0, so = 0.0;
   if (f > -0.9) {
-    const int xn = (int)floor((ex - P.chain_ox) / P.chain_d + 1);
-    const int yn = (int)floor((ey - P.chain_oy) / P.chain_d + 1);
-    const CellRanges R = cell_ranges(P, B.cell_start, (xn - 1) * P.chain_ny + (yn - 1));
-    const double rmin = (double)1.0e-7f, rmax = P.three_h;
-    double A[21];
-#pragma unroll
-    for (int q = 0; q < 21; ++q) A[q] = 0.0;
-    double sw = 0.0;
-    int nt = 0;
-    for (int d = 0; d < 3; ++d)
-      for (int o = R.lo[d]; o < R.hi[d]; ++o) {
-        const int g = B.name[o];
-        if (g >= N) continue;
-        const double2 pj = B.g_pos[g];
-        const double pdx = ex - pj.x, pdy = ey - pj.y;
-        const double r = sqrt(pdx * pdx + pdy * pdy);
-        if (!(r <= rmax && r >= rmin)) continue;
-        double w, dwx, dwy;
-        kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
-        const double a = (pj.x - ex) / P.h, b = (pj.y - ey) / P.h;
-        const double pv[6] = {1.0, a, b, a * a, a * b, b * b};
-        int q = 0;
-#pragma unroll
-        for (int m = 0; m < 6; ++m) {
-          const double bm = pv[m] * w;
-#pragma unroll
-          for (int l = m; l < 6; ++l) { A[q] = A[q] + bm * pv[l]; ++q; }
-        }
-        sw = sw + w;
-        ++nt;
-      }
-    if (nt >= 3) {
-      double y[6] = {0, 0, 0, 0, 0, 0};
-      bool mls = nt > 10;
-      if (mls) mls = mls6_first_row(A, (P.mls_alpha * P.mls_alpha) * P.mls_alpha, y);
-      for (int d = 0; d < 3; ++d)
-        for (int o = R.lo[d]; o < R.hi[d]; ++o) {
-          const int g = B.name[o];
-          if (g >= N) continue;
-          const double2 pj = B.g_pos[g];
-          const double pdx = ex - pj.x, pdy = ey - pj.y;
-          const double r = sqrt(pdx * pdx + pdy * pdy);
-          if (!(r <= rmax && r >= rmin)) continue;
-          double w, dwx, dwy;
-          kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
-          double fai;
-          if (mls) {
-            const double a = (pj.x - ex) / P.h, b = (pj.y - ey) / P.h;
-            const double pv[6] = {1.0, a, b, a * a, a * b, b * b};
-            fai = 0.0;
-#pragma unroll
-            for (int m = 0; m < 6; ++m) fai = fai + y[m] * (pv[m] * w);
-          } else {
-            fai = w / sw;
-          }
-          const double2 vj = B.g_vel[g];
-          sp = sp + B.g_p[g] * fai;
-          su = su + vj.x * fai;
-          sv = sv + vj.y * fai;
-          so = so + B.g_vort[g] * fai;
-        }
-    }
+    double y[6], sw;
+    bool mls;
+    const int nt = point_mls_weights(P, B, ex, ey, N, y, sw, mls);
+    if (nt >= 3)
+      for_each_support(P, B, ex, ey, N, [&](int g, double2 pj, double w) {
+        const double fai = point_mls_fai(P, y, sw, mls, ex, ey, pj, w);
+        const double2 vj = B.g_vel[g];
+        sp = sp + B.g_p[g] * fai;
+        su = su + vj.x * fai;
+        sv = sv + vj.y * fai;
+        so = so + B.g_vort[g] * fai;
+      });
   }
   G.out[0 * nn + k] = su;
   G.out[1 * nn + k] = sv;
@@ -888,6 +907,85 @@ __global__ void __launch_bounds__(kBlock) k_grid_mls(DevParams P, Buffers B, Gri
 
 void launch_grid_mls(hipStream_t s, const DevParams& P, Buffers& B, const GridArgs& G) {
   hipLaunchKernelGGL(k_grid_mls, dim3(grid_for(G.n_nodes)), dim3(kBlock), 0, s, P, B, G);
+}
+
+// Sampling_subroutine.f90:21-177 (subroutine Sampling) at n sampling points:
+// candidates are the fluid particles (gid < N); with more than 3 in the support,
+// the average pressure over the support, the order-6 MLS pressure (minus
+// g rho_j (y_s - y_j) Fai_j when gravity is on) and the MLS velocity; zeros
+// otherwise. out: 4 blocks of n (average p, MLS p, u, v).
+__global__ void __launch_bounds__(kBlock) k_probe_mls(DevParams P, Buffers B, ProbeArgs Q) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= Q.n) return;
+  const int N = B.counters[C_NFLUID];
+  const double2 xs = Q.pos[k];
+  double y[6], sw;
+  bool mls;
+  const int nt = point_mls_weights(P, B, xs.x, xs.y, N, y, sw, mls);
+  double avg = 0.0, mp = 0.0, su = 0.0, sv = 0.0;
+  if (nt > 3) {
+    for_each_support(P, B, xs.x, xs.y, N, [&](int g, double2 pj, double w) {
+      const double fai = point_mls_fai(P, y, sw, mls, xs.x, xs.y, pj, w);
+      const double pj_p = B.g_p[g];
+      if (Q.gravity != 0.0) mp = mp + pj_p * fai - Q.gravity * B.g_rho[g] * (xs.y - pj.y) * fai;
+      else mp = mp + pj_p * fai;
+      const double2 vj = B.g_vel[g];
+      su = su + vj.x * fai;
+      sv = sv + vj.y * fai;
+      avg = avg + pj_p;
+    });
+    avg = avg / nt;
+  }
+  const size_t n = (size_t)Q.n;
+  Q.out[k] = avg;
+  Q.out[n + k] = mp;
+  Q.out[2 * n + k] = su;
+  Q.out[3 * n + k] = sv;
+}
+
+void launch_probe_mls(hipStream_t s, const DevParams& P, Buffers& B, const ProbeArgs& Q) {
+  hipLaunchKernelGGL(k_probe_mls, dim3(grid_for(Q.n)), dim3(kBlock), 0, s, P, B, Q);
+}
+
+// Sampling_subroutine.f90:209-222: over the fluid particles, the wave front (largest
+// x, from 0) and, for each wave probe, the largest y (from 0) of the particles within
+// 0.55 (REAL(4)) dx of the probe's x. Maxima of non-negative doubles are taken on their bit
+// patterns with integer atomics: exact and order-independent. out_bits[0] = front,
+// out_bits[1 + j] = probe j (zeroed by the caller).
+__global__ void __launch_bounds__(kBlock) k_wave_probes(DevParams P, Buffers B, const double* probe_x, int n_probes,
+                                                        unsigned long long* out_bits) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = B.counters[C_NFLUID];
+  double front = 0.0;
+  double2 x = make_double2(0.0, 0.0);
+  const bool ok = g < N;
+  if (ok) {
+    x = B.g_pos[g];
+    if (x.x >= front) front = x.x;
+  }
+  // a wave-wide maximum first, then one atomic per wave
+  unsigned long long fb = __double_as_longlong(front);
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(fb, off, 64);
+    fb = o > fb ? o : fb;
+  }
+  if ((threadIdx.x & 63) == 0 && fb != 0ull) atomicMax(&out_bits[0], fb);
+  for (int j = 0; j < n_probes; ++j) {
+    double e = 0.0;
+    if (ok && fabs(probe_x[j] - x.x) <= (double)0.55f * P.dx && x.y > e) e = x.y;
+    unsigned long long eb = __double_as_longlong(e);
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(eb, off, 64);
+      eb = o > eb ? o : eb;
+    }
+    if ((threadIdx.x & 63) == 0 && eb != 0ull) atomicMax(&out_bits[1 + j], eb);
+  }
+}
+
+void launch_wave_probes(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, const double* probe_x,
+                        int n_probes, unsigned long long* out_bits) {
+  hipLaunchKernelGGL(k_wave_probes, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, s, P, B, probe_x, n_probes,
+                     out_bits);
 }
 
 // The density gradient of pass A: FP64 array, or the FP32-mixed record.

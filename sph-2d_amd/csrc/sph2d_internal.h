@@ -130,6 +130,16 @@ struct GridArgs {
   double* out;                            // 9 blocks of n_nodes
 };
 void launch_grid_mls(hipStream_t s, const DevParams& P, Buffers& B, const GridArgs& G);
+// Sampling_subroutine.f90 at sampling points (sph2d_fp64.hip)
+struct ProbeArgs {
+  int n;
+  double gravity;                         // g when Gravity_Switch is on, else 0
+  const double2* pos;                     // n sampling positions
+  double* out;                            // 4 blocks of n: average p, MLS p, u, v
+};
+void launch_probe_mls(hipStream_t s, const DevParams& P, Buffers& B, const ProbeArgs& Q);
+void launch_wave_probes(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, const double* probe_x,
+                        int n_probes, unsigned long long* out_bits);
 // Refresh_domain.f90:320-351 from the last evaluation (sph2d_eval64.hip)
 void launch_vorticity(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
 void launch_shift_apply(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);

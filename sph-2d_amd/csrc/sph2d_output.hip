@@ -30,7 +30,7 @@ std::string fmt_f(double v, int w, int d) {
   } else if (std::isinf(v)) {
     std::snprintf(buf, sizeof buf, "%*s", w, v > 0 ? "Infinity" : "-Infinity");
   } else {
-    std::snprintf(buf, sizeof buf, "%*.*f", w, d, v);
+    std::snprintf(buf, sizeof buf, "%#*.*f", w, d, v);   // # keeps the point when d = 0
   }
   std::string s(buf);
   if ((int)s.size() > w) s.assign(w, '*');
@@ -67,6 +67,35 @@ std::string time_string(double t, int w, int d) {
   std::string s = fmt_f(t, w, d);
   const size_t a = s.find_first_not_of(' ');
   return a == std::string::npos ? std::string() : s.substr(a);
+}
+
+// A list-directed REAL(8) item as ifort writes it, 23 characters: the G23.15E3 choice
+// of form on the value rounded to 15 significant digits -- F18.(15-k) and 5 blanks
+// when 0.1 <= |x| < 1e15 (k digits before the point), else the E form with scale
+// 1P (one digit before the point, 15 after, 3-digit exponent), right-justified.
+// Zero takes the E form (0.000000000000000E+000).
+std::string ld_real8(double v) {
+  char buf[96];
+  if (std::isnan(v) || std::isinf(v)) return fmt_f(v, 23, 0);
+  const double a = std::fabs(v);
+  if (a != 0.0) {
+    std::snprintf(buf, sizeof buf, "%.14e", a);                 // the 15-digit rounding
+    const int e10 = std::atoi(std::strchr(buf, 'e') + 1);
+    if (e10 >= -1 && e10 < 15) {
+      const int k = e10 + 1;
+      std::snprintf(buf, sizeof buf, "%#18.*f     ", 15 - k, v);
+      return buf;
+    }
+  }
+  std::snprintf(buf, sizeof buf, "%.15e", a);                   // D.DDDDDDDDDDDDDDDe+XX
+  char* e = std::strchr(buf, 'e');
+  const int ex = std::atoi(e + 1);
+  *e = 0;
+  char item[64], out[64];
+  std::snprintf(item, sizeof item, "%s%sE%c%03d", std::signbit(v) && v != 0.0 ? "-" : "", buf, ex < 0 ? '-' : '+',
+                std::abs(ex));
+  std::snprintf(out, sizeof out, "%23s", item);
+  return out;
 }
 
 // List-directed integer item followed by the next character item.
@@ -423,6 +452,136 @@ int sph2d_write_sampling(const char* dir, double nondim_time, const double* out1
     std::string line = fmt_f(nondim_time, 20, 8);
     for (int k = 0; k < it.count; ++k) line += fmt_f(out14[it.first + k], 20, 8);
     std::fprintf(F.f, "%s\n", line.c_str());
+  }
+  return SPH2D_OK;
+}
+
+int sph2d_format_list_directed(double value, char* out, int32_t cap) {
+  if (!out) return out_fail(SPH2D_ERR_ARGUMENT, "out is NULL");
+  const std::string s = ld_real8(value);
+  if ((int)s.size() >= cap) return out_fail(SPH2D_ERR_ARGUMENT, "cap too small");
+  std::memcpy(out, s.c_str(), s.size() + 1);
+  return SPH2D_OK;
+}
+
+// write(unit,*) of n REAL(8) values appended to path (Sampling_subroutine.f90:283,
+// :287: the wave gauge files and Wave_front_results.dat): a leading blank, the items
+// separated by one blank, a new record (again with a leading blank) when the next
+// item would pass column 80.
+int sph2d_write_list_directed(const char* path, int32_t n, const double* values) {
+  if (!path || n < 0 || (n > 0 && !values)) return out_fail(SPH2D_ERR_ARGUMENT, "bad argument");
+  std::string rec = " ", all;
+  for (int k = 0; k < n; ++k) {
+    const std::string it = ld_real8(values[k]);
+    if (rec.size() > 1 && rec.size() + 1 + it.size() > 80) {
+      all += rec + "\n";
+      rec = " ";
+    }
+    if (rec.size() > 1) rec += " ";
+    rec += it;
+  }
+  all += rec + "\n";
+  File F(path, "a");
+  if (!F.f) return open_err(path);
+  std::fputs(all.c_str(), F.f);
+  return SPH2D_OK;
+}
+
+// write(unit,310) with 310 format(20F20.8) (Sampling_subroutine.f90:318-322, the
+// pressure probe files): n values appended as records of at most 20.
+int sph2d_write_f20_8(const char* path, int32_t n, const double* values) {
+  if (!path || n < 0 || (n > 0 && !values)) return out_fail(SPH2D_ERR_ARGUMENT, "bad argument");
+  std::string all, rec;
+  for (int k = 0; k < n; ++k) {
+    rec += fmt_f(values[k], 20, 8);
+    if ((k + 1) % 20 == 0 || k + 1 == n) {
+      all += rec + "\n";
+      rec.clear();
+    }
+  }
+  if (n == 0) all = "\n";
+  File F(path, "a");
+  if (!F.f) return open_err(path);
+  std::fputs(all.c_str(), F.f);
+  return SPH2D_OK;
+}
+
+// Sampling_subroutine.f90:300-322: the scalar Kalman filter of each sampling point's
+// MLS pressure (Q = 1.0e-6, R = 1.0e-5 as REAL(4) literals, x0 = 1). On the first
+// sampling (current_time_number = i_time_step / sampling_time_step = 1) the output
+// is the measurement and the state is (x0, measurement); later the gain
+// K = P / (P + R) updates x and P. last_x, last_p hold the state between calls.
+int sph2d_kalman_filter(int32_t n, int32_t current_time_number, const double* measured, double* last_x,
+                        double* last_p, double* filtered) {
+  if (n < 0 || (n > 0 && (!measured || !last_x || !last_p || !filtered)))
+    return out_fail(SPH2D_ERR_ARGUMENT, "bad argument");
+  const double Q = (double)1.0e-6f, R = (double)1.0e-5f, x0 = 1.0;
+  for (int m = 0; m < n; ++m) {
+    if (current_time_number == 1) {
+      filtered[m] = measured[m];
+      last_x[m] = x0;
+      last_p[m] = measured[m];
+    } else {
+      const double K = last_p[m] / (last_p[m] + R);
+      filtered[m] = last_x[m] + K * (measured[m] - last_x[m]);
+      last_x[m] = filtered[m];
+      last_p[m] = last_p[m] - K * last_p[m] + Q;
+    }
+  }
+  return SPH2D_OK;
+}
+
+// Initialize_SPH_Calculation.f90:385-398: n_points evenly spaced points from first
+// to last (first + (i-1) (last - first) / (n_points - 1)), as columns x, y of out (ld,2).
+int sph2d_sampling_line_points(int32_t n_points, const double* first, const double* last, double* out, int32_t ld) {
+  if (n_points < 2 || !first || !last || !out || ld < n_points) return out_fail(SPH2D_ERR_ARGUMENT, "bad argument");
+  for (int j = 0; j < 2; ++j) {
+    const double step = (last[j] - first[j]) / (n_points - 1);
+    for (int i = 0; i < n_points; ++i) out[(size_t)j * ld + i] = first[j] + i * step;
+  }
+  return SPH2D_OK;
+}
+
+// Initialize_SPH_Calculation.f90:405-436: dir/<line>th_SamplingLine_Position.dat.
+int sph2d_write_sampling_line_positions(const char* dir, int32_t line, int32_t n_points, const double* pos,
+                                        int32_t ld) {
+  if (!dir || !pos || n_points < 0 || ld < n_points) return out_fail(SPH2D_ERR_ARGUMENT, "bad argument");
+  const std::string path = std::string(dir) + "/" + std::to_string(line) + "th_SamplingLine_Position.dat";
+  File F(path.c_str(), "w");
+  if (!F.f) return open_err(path.c_str());
+  std::fputs(" TITLE='DISTRIBUTION'\n VARIABLES= 'X' 'Y'\n", F.f);
+  for (int i = 0; i < n_points; ++i)
+    std::fprintf(F.f, "%s%s\n", fmt_f(pos[i], 12, 6).c_str(), fmt_f(pos[(size_t)ld + i], 12, 6).c_str());
+  return SPH2D_OK;
+}
+
+// Sampling_subroutine.f90:490-512: dir/<line>th_<t, F7.4>_SamplingLine_Results.dat,
+// format 20f12.6: x, y, analytic u/U, v/U, p/p0, MLS u/U, v/U, MLS p/p0, average p/p0.
+int sph2d_write_sampling_line_results(const char* dir, int32_t line, double t_nondim, int32_t n_points,
+                                      const double* pos, const double* ana_vel, const double* ana_press,
+                                      const double* vel, const double* mls_press, const double* average_press,
+                                      int32_t ld, const sph2d_scales* sc) {
+  if (!dir || !pos || !ana_vel || !ana_press || !vel || !mls_press || !average_press || n_points < 0 ||
+      ld < n_points || !ok_scales(sc))
+    return out_fail(SPH2D_ERR_ARGUMENT, "bad argument");
+  const std::string path = std::string(dir) + "/" + std::to_string(line) + "th_" + time_string(t_nondim, 7, 4) +
+                           "_SamplingLine_Results.dat";
+  File F(path.c_str(), "w");
+  if (!F.f) return open_err(path.c_str());
+  const size_t L = ld;
+  for (int i = 0; i < n_points; ++i) {
+    const double v[9] = {pos[i],
+                         pos[L + i],
+                         ana_vel[i] / sc->velocity_0,
+                         ana_vel[L + i] / sc->velocity_0,
+                         ana_press[i] / sc->pressure_0,
+                         vel[i] / sc->velocity_0,
+                         vel[L + i] / sc->velocity_0,
+                         mls_press[i] / sc->pressure_0,
+                         average_press[i] / sc->pressure_0};
+    std::string rec;
+    for (double x : v) rec += fmt_f(x, 12, 6);
+    std::fprintf(F.f, "%s\n", rec.c_str());
   }
   return SPH2D_OK;
 }

@@ -216,6 +216,58 @@ module sph2d_interface
       integer(c_int) :: rc
     end function
 
+    ! Sampling_subroutine.f90:21-177 at n points (pos(ld,2)), after a sampling step
+    function sph2d_sampling(ctx, i_time_step, n, pos, ld, gravity, average_press, mls_press, vel) &
+        bind(C, name="sph2d_sampling") result(rc)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: i_time_step, n, ld
+      real(c_double), intent(in) :: pos(ld, 2)
+      real(c_double), value :: gravity
+      real(c_double), intent(out) :: average_press(n), mls_press(n), vel(ld, 2)
+      integer(c_int) :: rc
+    end function
+
+    ! Sampling_subroutine.f90:246-279: wave front and wave gauge elevations
+    function sph2d_wave_probes(ctx, n_probes, probe_x, wave_front, elevation) &
+        bind(C, name="sph2d_wave_probes") result(rc)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: n_probes
+      real(c_double), intent(in) :: probe_x(*)
+      real(c_double), intent(out) :: wave_front, elevation(*)
+      integer(c_int) :: rc
+    end function
+
+    ! Sampling_subroutine.f90:300-322
+    function sph2d_kalman_filter(n, current_time_number, measured, last_x, last_p, filtered) &
+        bind(C, name="sph2d_kalman_filter") result(rc)
+      import :: c_int, c_int32_t, c_double
+      integer(c_int32_t), value :: n, current_time_number
+      real(c_double), intent(in) :: measured(n)
+      real(c_double), intent(inout) :: last_x(n), last_p(n)
+      real(c_double), intent(out) :: filtered(n)
+      integer(c_int) :: rc
+    end function
+
+    ! write(unit,*) of n REAL(8) values, appended (Wave_front_results.dat, wave gauges)
+    function sph2d_write_list_directed(path, n, values) bind(C, name="sph2d_write_list_directed") result(rc)
+      import :: c_int, c_int32_t, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: n
+      real(c_double), intent(in) :: values(n)
+      integer(c_int) :: rc
+    end function
+
+    ! write(unit,'(20F20.8)') of n values, appended (pressure probe files)
+    function sph2d_write_f20_8(path, n, values) bind(C, name="sph2d_write_f20_8") result(rc)
+      import :: c_int, c_int32_t, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: n
+      real(c_double), intent(in) :: values(n)
+      integer(c_int) :: rc
+    end function
+
     function sph2d_last_error() bind(C, name="sph2d_last_error") result(msg)
       import :: c_ptr
       type(c_ptr) :: msg

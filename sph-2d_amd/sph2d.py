@@ -92,7 +92,9 @@ EXPORTS = (
     "sph2d_write_paraview", "sph2d_write_latest_results_zone", "sph2d_read_latest_results",
     "sph2d_write_sampling", "sph2d_grid_size", "sph2d_calculation_for_postproceeding",
     "sph2d_output_for_grid_postproceding", "sph2d_write_initial_grid", "sph2d_write_grid_zone",
-    "sph2d_write_grid_binary",
+    "sph2d_write_grid_binary", "sph2d_sampling", "sph2d_wave_probes", "sph2d_output_sampling_lines",
+    "sph2d_format_list_directed", "sph2d_write_list_directed", "sph2d_write_f20_8", "sph2d_kalman_filter",
+    "sph2d_sampling_line_points", "sph2d_write_sampling_line_positions", "sph2d_write_sampling_line_results",
 )
 
 _lib = None
@@ -160,6 +162,17 @@ def lib():
                                       P(Scales)],
             "sph2d_write_grid_binary": [cs, ctypes.c_double, i32, i32, ctypes.c_double, dp, dp, dp, i32,
                                         P(Scales)],
+            "sph2d_sampling": [vp, i32, i32, dp, i32, ctypes.c_double, dp, dp, dp],
+            "sph2d_wave_probes": [vp, i32, dp, dp, dp],
+            "sph2d_output_sampling_lines": [vp, i32, cs, i32, dp, dp, i32],
+            "sph2d_format_list_directed": [ctypes.c_double, ctypes.c_char_p, i32],
+            "sph2d_write_list_directed": [cs, i32, dp],
+            "sph2d_write_f20_8": [cs, i32, dp],
+            "sph2d_kalman_filter": [i32, i32, dp, dp, dp, dp],
+            "sph2d_sampling_line_points": [i32, dp, dp, dp, i32],
+            "sph2d_write_sampling_line_positions": [cs, i32, i32, dp, i32],
+            "sph2d_write_sampling_line_results": [cs, i32, ctypes.c_double, i32, dp, dp, dp, dp, dp, dp, i32,
+                                                  P(Scales)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -340,6 +353,30 @@ class Simulation:
     def output_for_grid_postproceding(self, i_time_step: int, grid_path=None, binary_path=None):
         enc = lambda p: None if p is None else os.fspath(p).encode()
         _check(lib().sph2d_output_for_grid_postproceding(self._c, i_time_step, enc(grid_path), enc(binary_path)))
+
+    def sampling(self, i_time_step: int, x, y, gravity: float = 0.0) -> Dict[str, np.ndarray]:
+        """Sampling_subroutine.f90:21-177 at the points (x, y): average and MLS pressure, MLS u, v."""
+        n = len(x)
+        pos = _col2(x, y)
+        avg, mp, vel = np.zeros(n), np.zeros(n), np.zeros(2 * n)
+        _check(lib().sph2d_sampling(self._c, i_time_step, n, _dp(pos), n, gravity, _dp(avg), _dp(mp), _dp(vel)))
+        return {"average_p": avg, "mls_p": mp, "u": vel[:n], "v": vel[n:]}
+
+    def wave_probes(self, probe_x=()):
+        """Sampling_subroutine.f90:246-279: (wave front, elevation at each probe x), unscaled."""
+        px = _f64(np.asarray(probe_x, dtype=np.float64).reshape(-1))
+        el = np.zeros(max(len(px), 1))
+        front = ctypes.c_double()
+        _check(lib().sph2d_wave_probes(self._c, len(px), _dp(px) if len(px) else None, ctypes.byref(front),
+                                       _dp(el)))
+        return front.value, el[:len(px)]
+
+    def output_sampling_lines(self, i_time_step: int, dir_path, first, last, n_points: int = 41):
+        """Sampling_subroutine.f90:466-512: <k>th_<t>_SamplingLine_Results.dat for each line."""
+        f = _f64(np.asarray(first, dtype=np.float64).reshape(-1))
+        l = _f64(np.asarray(last, dtype=np.float64).reshape(-1))
+        _check(lib().sph2d_output_sampling_lines(self._c, i_time_step, os.fspath(dir_path).encode(), len(f) // 2,
+                                                 _dp(f), _dp(l), n_points))
 
     def output_final_results(self, path):
         """Final_Doamin_Results.dat (Parallel_Taylor_Green_Vortices_2D_Release.f90:185-199)."""
@@ -608,3 +645,83 @@ def write_grid_binary(path, time, nex, ney, element_dx, u, v, p, fai, scales: Sc
     nn = (nex + 1) * (ney + 1)
     _check(lib().sph2d_write_grid_binary(os.fspath(path).encode(), time, nex, ney, element_dx, _dp(_col2(u, v)),
                                          _dp(_f64(p)), _dp(_f64(fai)), nn, ctypes.byref(scales)))
+
+
+def format_list_directed(value: float) -> str:
+    """One REAL(8) item of write(unit,*) as ifort writes it (23 characters)."""
+    buf = ctypes.create_string_buffer(32)
+    _check(lib().sph2d_format_list_directed(float(value), buf, 32))
+    return buf.value.decode()
+
+
+def write_list_directed(path, values):
+    v = _f64(np.asarray(values, dtype=np.float64).reshape(-1))
+    _check(lib().sph2d_write_list_directed(os.fspath(path).encode(), len(v), _dp(v)))
+
+
+def write_f20_8(path, values):
+    v = _f64(np.asarray(values, dtype=np.float64).reshape(-1))
+    _check(lib().sph2d_write_f20_8(os.fspath(path).encode(), len(v), _dp(v)))
+
+
+def kalman_filter(current_time_number: int, measured, last_x, last_p) -> np.ndarray:
+    """Sampling_subroutine.f90:300-322; last_x and last_p (float64 arrays) are updated in place."""
+    m = _f64(measured)
+    for a in (last_x, last_p):
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous and len(a) == len(m)):
+            raise ValueError("last_x / last_p must be contiguous float64 arrays of len(measured)")
+    out = np.zeros(len(m))
+    _check(lib().sph2d_kalman_filter(len(m), current_time_number, _dp(m), _dp(last_x), _dp(last_p), _dp(out)))
+    return out
+
+
+def sampling_line_points(first, last, n_points: int = 41):
+    """Initialize_SPH_Calculation.f90:385-398: (x, y) of n_points points from first to last."""
+    out = np.zeros(2 * n_points)
+    _check(lib().sph2d_sampling_line_points(n_points, _dp(_f64(first)), _dp(_f64(last)), _dp(out), n_points))
+    return out[:n_points], out[n_points:]
+
+
+def write_sampling_line_positions(dir_path, line: int, x, y):
+    n = len(x)
+    _check(lib().sph2d_write_sampling_line_positions(os.fspath(dir_path).encode(), line, n, _dp(_col2(x, y)), n))
+
+
+class SamplingProbes:
+    """Sampling_subroutine.f90:232-323 for a probe configuration: each call at a
+    sampling step appends the wave-front line, one line per wave gauge and, per
+    pressure probe, its MLS, average and Kalman-filtered pressure lines, in the
+    reference's files under dir_path (Wave_front_results.dat in wave_front_dir).
+    The Kalman state is held here between calls."""
+
+    def __init__(self, sim: "Simulation", dir_path, points=(), probe_x=(), gravity: float = 0.0,
+                 sampling_time_step: int = 100, wave_front_dir=None):
+        self.sim, self.dir = sim, os.fspath(dir_path)
+        self.front_dir = os.fspath(wave_front_dir) if wave_front_dir is not None else "."
+        pts = np.asarray(points, dtype=np.float64).reshape(-1, 2)
+        self.px, self.py = _f64(pts[:, 0]), _f64(pts[:, 1])
+        self.probe_x = _f64(np.asarray(probe_x, dtype=np.float64).reshape(-1))
+        self.gravity, self.every = gravity, sampling_time_step
+        self.last_x, self.last_p = np.zeros(len(self.px)), np.zeros(len(self.px))
+        os.makedirs(os.path.join(self.dir, "Press"), exist_ok=True)
+        os.makedirs(os.path.join(self.dir, "WaveHeight"), exist_ok=True)
+
+    def __call__(self, i_time_step: int):
+        sc = self.sim.scales()
+        t = i_time_step * self.sim.params.dt            # current_time (no standing time)
+        front, elev = self.sim.wave_probes(self.probe_x)
+        dx = self.sim.params.dx
+        for j, e in enumerate(elev):
+            write_list_directed(os.path.join(self.dir, "WaveHeight", f"wave_gauge_{j + 1}_result.dat"),
+                                [t * sc.time_0, (e + dx / 2.0) / sc.domain_size_y])
+        write_list_directed(os.path.join(self.front_dir, "Wave_front_results.dat"),
+                            [t * sc.time_0, front / sc.domain_size_y])
+        if len(self.px) == 0:
+            return
+        r = self.sim.sampling(i_time_step, self.px, self.py, self.gravity)
+        kal = kalman_filter(i_time_step // self.every, r["mls_p"], self.last_x, self.last_p)
+        for m in range(len(self.px)):
+            base = os.path.join(self.dir, "Press", f"Probe_{m + 1}_")
+            write_f20_8(base + "Normalized_Pressure.dat", [t * sc.time_0, r["mls_p"][m] / sc.pressure_0])
+            write_f20_8(base + "Average_Pressure.dat", [t * sc.time_0, r["average_p"][m] / sc.pressure_0])
+            write_f20_8(base + "Kalman_Pressure.dat", [t * sc.time_0, kal[m] / sc.pressure_0])

@@ -8,6 +8,7 @@ Sources (mini-walker/SPH-2D, Taylor_Green_Vortex/, the shipped 100x100 run):
   Free_Surface.dat          header + the first 4 ZONE lines (all zones are empty)
   Kinetic_energy.dat        first 40 rows (format 310 = 20F20.8)
   Norm_Pressure.dat         first 40 rows
+  Wave_front_results.dat    all 500 rows (ifort list-directed REAL(8): E form below 0.1)
   Initial_Post_Proceeding_Grid.dat / _Node.dat   the first 8 lines, the line count and
                             the SHA-256 of the whole file (10201 nodes, 10000 quads)
 Only data lines are copied; no reference source is.
@@ -19,7 +20,7 @@ import os
 REF = os.environ.get("SPH2D_REFERENCE", "/root/reference") + "/Taylor_Green_Vortex"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "output_formats")
 EXCERPTS = {"Final_Doamin_Results.dat": 203, "Free_Surface.dat": 6, "Kinetic_energy.dat": 40,
-            "Norm_Pressure.dat": 40}
+            "Norm_Pressure.dat": 40, "Wave_front_results.dat": 500}
 
 
 def main():

@@ -271,3 +271,121 @@ def test_grid_output_files_of_a_context(tmp_path):
     vals = np.frombuffer(binf.read_bytes()[-56 * nn:], dtype="<f8").reshape(nn, 7)
     assert np.array_equal(vals[:, 2], g["u"]) and np.array_equal(vals[:, 6], g["fai"])
     sim.close()
+
+
+# --- Sampling_subroutine on the GPU -----------------------------------------------------------
+def test_sampling_points_match_the_oracle():
+    say("test_sampling_points_match_the_oracle")
+    n = 48
+    sim, o = run_pair(n, 10, 10)
+    N = n * n
+    rng = np.random.default_rng(21)
+    gx, gy = o.f64("g_x")[:N], o.f64("g_y")[:N]
+    sx = np.concatenate([rng.uniform(0.0, 1.0, 300), gx[[3, 400, 2000]]])
+    sy = np.concatenate([rng.uniform(0.0, 1.0, 300), gy[[3, 400, 2000]]])
+    g = sim.sampling(10, sx, sy)
+    ref = o.sampling_points(sx, sy)
+    assert np.max(np.abs(g["average_p"] - ref[:, 0])) <= 1e-13 * np.max(np.abs(ref[:, 0]))
+    for col, name in ((1, "mls_p"), (2, "u"), (3, "v")):
+        _grid_cmp(g[name], ref[:, col], name)
+    with pytest.raises(sph2d.SphError):
+        sim.sampling(10, [5.0], [0.5])                  # outside the background grid
+    with pytest.raises(sph2d.SphError):
+        sim.sampling(9, sx, sy)                          # not the last sampling step
+    sim.close()
+
+
+def test_sampling_and_wave_probes_on_a_fluid_disc():
+    """Probes around the rim of a fluid disc: zeros with 3 or fewer particles in the
+    support, Shepard values with 4 to 10, MLS inside, as the oracle; the wave front
+    and the gauge elevations exactly as the oracle."""
+    say("test_sampling_and_wave_probes_on_a_fluid_disc")
+    n = 40
+    o0 = Oracle(nx=n, ny=n)
+    o0.init_tgv()
+    N = n * n
+    x, y = o0.f64("g_x")[:N], o0.f64("g_y")[:N]
+    keep = ((x - 0.5) ** 2 + (y - 0.5) ** 2) < 0.3 ** 2
+    st = [a[keep] for a in (x, y, o0.f64("g_u")[:N], o0.f64("g_v")[:N], o0.f64("g_rho")[:N],
+                            o0.f64("g_mass")[:N])]
+    prm = sph2d.tgv_params(n, sample_every=1)
+    sim = sph2d.Simulation(prm, capacity=N)
+    sim.upload_state(*st)
+    o = Oracle(nx=n, ny=n, dt=prm.dt)
+    o.set_fluid_state(*st)
+    for s in (sim,):
+        s.distribute_period_boundary()
+        s.devide_block(1)
+        s.single_step_compute(1)
+        s.refresh_domain(1)
+    o.period_boundary()
+    o.devide_block(1)
+    o.single_step_compute(1)
+    o.refresh()
+    ang = np.linspace(0, 2 * np.pi, 200, endpoint=False)
+    rad = np.concatenate([[0.0, 0.1, 0.2], np.linspace(0.27, 0.42, 10)])
+    sx = (0.5 + np.outer(rad, np.cos(ang))).ravel()
+    sy = (0.5 + np.outer(rad, np.sin(ang))).ravel()
+    g = sim.sampling(1, sx, sy, gravity=9.81)
+    ref = o.sampling_points(sx, sy, 9.81)
+    zero = np.all(ref == 0.0, axis=1)
+    say(f"probes {len(sx)}, empty {int(zero.sum())}")
+    assert zero.sum() > 100 and (~zero).sum() > 1000
+    for col, name in enumerate(("average_p", "mls_p", "u", "v")):
+        assert np.all(g[name][zero] == 0.0), name
+    assert np.max(np.abs(g["average_p"] - ref[:, 0])) <= 1e-13 * np.max(np.abs(ref[:, 0]))
+    for col, name in ((1, "mls_p"), (2, "u"), (3, "v")):
+        _grid_cmp(g[name], ref[:, col], name)
+    px = np.array([0.2, 0.21, 0.5, 0.79, 0.95, -0.5])
+    front, el = sim.wave_probes(px)
+    rf, rel = o.wave_probes(px)
+    say(f"front {front!r} oracle {rf!r}")
+    assert front == rf and np.array_equal(el, rel)
+    assert el[4] == 0.0 and el[5] == 0.0 and el[2] > 0.75
+    sim.close()
+    o.close()
+
+
+def test_sampling_files_of_a_context(tmp_path):
+    """The sampling-line result files (Sampling_subroutine.f90:466-512) and the probe
+    files written by sph2d.SamplingProbes: values as the oracle's at the same points,
+    to the 6 (20f12.6) and 8 (20F20.8) decimals written."""
+    say("test_sampling_files_of_a_context")
+    n = 40
+    sim, o = run_pair(n, 10, 10)
+    first = np.array([[0.0, 0.25], [0.0, 0.5], [0.0, 0.75]])
+    last = np.array([[1.0, 0.25], [1.0, 0.5], [1.0, 0.75]])
+    sim.output_sampling_lines(10, tmp_path, first, last, 41)
+    sc = sim.scales()
+    tn = 10 * sim.params.dt / (1.0 / sc.velocity_0)
+    for k in range(3):
+        name = tmp_path / f"{k + 1}th_{sph2d.time_name(tn, 7, 4)}_SamplingLine_Results.dat"
+        rows = np.loadtxt(name)
+        assert rows.shape == (41, 9)
+        x, y = sph2d.sampling_line_points(first[k], last[k], 41)
+        ref = o.sampling_points(x, y)
+        np.testing.assert_allclose(rows[:, 0], x, atol=6e-7)
+        np.testing.assert_allclose(rows[:, 1], y, atol=6e-7)
+        np.testing.assert_allclose(rows[:, 5], ref[:, 2] / sc.velocity_0, atol=1.1e-6)
+        np.testing.assert_allclose(rows[:, 6], ref[:, 3] / sc.velocity_0, atol=1.1e-6)
+        np.testing.assert_allclose(rows[:, 7], ref[:, 1] / sc.pressure_0, atol=1.1e-6)
+        np.testing.assert_allclose(rows[:, 8], ref[:, 0] / sc.pressure_0, atol=1.1e-6)
+        np.testing.assert_allclose(rows[:, 2], np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) *
+                                   np.exp(-8 * np.pi ** 2 * sim.params.nu / sim.params.U * tn), atol=1.1e-6)
+    pts = [(0.3, 0.3), (0.55, 0.8)]
+    probes = sph2d.SamplingProbes(sim, tmp_path / "Sampling", points=pts, probe_x=[0.5],
+                                  sampling_time_step=10, wave_front_dir=tmp_path)
+    probes(10)
+    ref = o.sampling_points([0.3, 0.55], [0.3, 0.8])
+    front, _ = o.wave_probes([0.5])
+    t = 10 * sim.params.dt
+    line = (tmp_path / "Wave_front_results.dat").read_text()
+    assert line == " " + sph2d.format_list_directed(t * sc.time_0) + " " + \
+        sph2d.format_list_directed(front / sc.domain_size_y) + "\n"
+    for m in range(2):
+        v = np.loadtxt(tmp_path / "Sampling" / "Press" / f"Probe_{m + 1}_Normalized_Pressure.dat")
+        np.testing.assert_allclose(v, [t * sc.time_0, ref[m, 1] / sc.pressure_0], atol=1.1e-8)
+        kal = np.loadtxt(tmp_path / "Sampling" / "Press" / f"Probe_{m + 1}_Kalman_Pressure.dat")
+        assert np.array_equal(kal, v)                    # the first sampling passes the value through
+    assert (tmp_path / "Sampling" / "WaveHeight" / "wave_gauge_1_result.dat").exists()
+    sim.close()

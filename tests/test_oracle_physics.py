@@ -366,3 +366,132 @@ def test_grid_level_set_from_one_surface_particle():
     assert empty.sum() > 0 and np.all(out[empty, :4] == 0.0)
     assert np.all(out[~empty, 2] != 0.0)
     o.close()
+
+
+# --- Sampling_subroutine: pressure probes, wave probes ----------------------------------
+def _support(o, N, sx, sy):
+    """Brute force over every fluid particle: 1e-7 (REAL(4)) <= r <= 3h."""
+    x, y = o.f64("g_x")[:N], o.f64("g_y")[:N]
+    r = np.sqrt((x - sx) ** 2 + (y - sy) ** 2)
+    h = o.params.h_factor * (o.params.box_x / o.params.nx)
+    return (r <= 3.0 * h) & (r >= float(np.float32(1e-7)))
+
+
+def test_sampling_points_reproduce_quadratics_and_average_the_support():
+    """Sampling (Sampling_subroutine.f90:21-177): the order-6 MLS pressure and velocity
+    are exact for quadratic fields at any interior point (also at a particle, which
+    the 1e-7 bound leaves out of its own support); the average pressure is the plain
+    mean over every fluid particle within 3h, found here by brute force."""
+    n = 40
+    o = _post(n)
+    N = n * n
+    x, y = o.f64("g_x")[:N], o.f64("g_y")[:N]
+    quad = lambda X, Y, c: c[0] + c[1] * X + c[2] * Y + c[3] * X * X + c[4] * X * Y + c[5] * Y * Y
+    cs = {"g_u": (0.3, 1.1, -0.7, 0.9, -1.3, 0.5), "g_v": (-0.2, 0.4, 0.8, -0.6, 0.7, 1.2),
+          "g_p": (5.0, -3.0, 2.0, 7.0, 1.0, -4.0)}
+    for name, c in cs.items():
+        a = o.f64(name)
+        a[:N] = quad(x, y, c)
+        o.set_f64(name, a)
+    rng = np.random.default_rng(4)
+    sx = np.concatenate([rng.uniform(0.1, 0.9, 60), x[[7, 500, 1203]]])
+    sy = np.concatenate([rng.uniform(0.1, 0.9, 60), y[[7, 500, 1203]]])
+    out = o.sampling_points(sx, sy)
+    for col, name in ((1, "g_p"), (2, "g_u"), (3, "g_v")):
+        ref = quad(sx, sy, cs[name])
+        assert np.max(np.abs(out[:, col] - ref)) < 1e-12 * np.max(np.abs(ref)), name
+    p = o.f64("g_p")[:N]
+    for k in range(len(sx)):
+        s = _support(o, N, sx[k], sy[k])
+        assert s.sum() > 10
+        assert abs(out[k, 0] - p[s].mean()) <= 1e-13 * np.abs(p).max(), k
+    o.close()
+
+
+def test_sampling_gravity_term_is_the_hydrostatic_correction():
+    """With gravity on, each particle's pressure is carried to the probe's height,
+    p_j - g rho_j (y_s - y_j): for a hydrostatic column p_j = g rho_j (H - y_j) with
+    an arbitrary (non-smooth) density, the probe value is g (H - y_s) times the MLS
+    interpolation of rho itself; a flipped sign or a dropped rho_j fails."""
+    n = 40
+    o = _post(n)
+    N = n * n
+    y = o.f64("g_y")[:N]
+    rng = np.random.default_rng(8)
+    rho = 1000.0 * (1.0 + 0.05 * rng.standard_normal(N))
+    g, H = 9.81, 1.3
+    sx, sy = rng.uniform(0.15, 0.85, 20), rng.uniform(0.15, 0.85, 20)
+    a = o.f64("g_rho")
+    a[:N] = rho
+    o.set_f64("g_rho", a)
+    a = o.f64("g_p")
+    a[:N] = rho                                        # MLS of rho with gravity off
+    o.set_f64("g_p", a)
+    mls_rho = o.sampling_points(sx, sy, 0.0)[:, 1]
+    a[:N] = g * rho * (H - y)
+    o.set_f64("g_p", a)
+    with_g = o.sampling_points(sx, sy, g)[:, 1]
+    np.testing.assert_allclose(with_g, g * (H - sy) * mls_rho, rtol=1e-12)
+    without = o.sampling_points(sx, sy, 0.0)[:, 1]
+    assert np.max(np.abs(without - with_g)) > 1e-4 * np.max(np.abs(with_g))
+    o.close()
+
+
+def test_sampling_sparse_support_zero_and_shepard():
+    """On a fluid disc, probes outside or at its rim: 3 or fewer particles in the
+    support give zeros; 4 to 10 give the average and the Shepard value
+    sum W p / sum W (MLS_Interpolation_In_Gobal_Domain.f90 below 11 particles)."""
+    n = 40
+    o0 = tgv(n)
+    N = n * n
+    x, y = o0.f64("g_x")[:N], o0.f64("g_y")[:N]
+    keep = ((x - 0.5) ** 2 + (y - 0.5) ** 2) < 0.3 ** 2
+    st = [a[keep] for a in (x, y, o0.f64("g_u")[:N], o0.f64("g_v")[:N], o0.f64("g_rho")[:N], o0.f64("g_mass")[:N])]
+    o0.close()
+    o = po.Oracle(nx=n, ny=n)
+    o.set_fluid_state(*st)
+    evaluate(o)
+    o.refresh()
+    M = int(keep.sum())
+    ang = np.linspace(0, 2 * np.pi, 400, endpoint=False)
+    rad = np.linspace(0.29, 0.36, 8)
+    sx = (0.5 + np.outer(rad, np.cos(ang))).ravel()
+    sy = (0.5 + np.outer(rad, np.sin(ang))).ravel()
+    out = o.sampling_points(sx, sy)
+    gx, gy, gp = o.f64("g_x")[:M], o.f64("g_y")[:M], o.f64("g_p")[:M]
+    h = o.params.h_factor * (o.params.box_x / o.params.nx)
+    seen = {"zero": 0, "shepard": 0}
+    for k in range(len(sx)):
+        s = _support(o, M, sx[k], sy[k])
+        c = int(s.sum())
+        if c <= 3:
+            assert np.all(out[k] == 0.0), k
+            seen["zero"] += 1
+        elif c <= 10:
+            r = np.sqrt((sx[k] - gx[s]) ** 2 + (sy[k] - gy[s]) ** 2)
+            w = np.array([po.compute_kernel(ri, np.array([sx[k] - xi, sy[k] - yi]), h)[0]
+                          for ri, xi, yi in zip(r, gx[s], gy[s])])
+            assert abs(out[k, 1] - np.sum(w * gp[s]) / np.sum(w)) <= 1e-12 * np.abs(gp).max(), k
+            assert abs(out[k, 0] - gp[s].mean()) <= 1e-13 * np.abs(gp).max(), k
+            seen["shepard"] += 1
+    assert seen["zero"] > 20 and seen["shepard"] > 20, seen
+    o.close()
+
+
+def test_wave_front_and_wave_probes_by_brute_force():
+    """Sampling_subroutine.f90:246-279: the front is the largest fluid x (from 0) and
+    a probe's elevation the largest y of the particles within 0.55 (REAL(4)) dx of its
+    x (from 0); a probe left of every particle reads 0."""
+    n = 40
+    o = _post(n)
+    N = n * n
+    x, y = o.f64("g_x")[:N], o.f64("g_y")[:N]
+    px = np.array([x[10], 0.5, x[77] + 0.3 / n, -1.0])
+    front, el = o.wave_probes(px)
+    assert front == max(0.0, x.max())
+    lim = float(np.float32(0.55)) * (o.params.box_x / o.params.nx)
+    for j, p in enumerate(px):
+        sel = np.abs(p - x) <= lim
+        assert el[j] == (max(0.0, y[sel].max()) if sel.any() else 0.0), j
+    assert el[3] == 0.0
+    o.close()

@@ -276,3 +276,86 @@ def test_grid_zone_and_binary(tmp_path):
     assert struct.unpack_from("<9i", data, o) == (-1, 0, 1, 0, 0, nex + 1, ney + 1, 1, 0)
     vals = np.frombuffer(data[-56 * nn:], dtype="<f8").reshape(nn, 7)
     assert np.array_equal(vals[:, 6], fai) and np.array_equal(vals[:, 2], u)
+
+
+# --- Sampling_subroutine.f90 files -----------------------------------------------------------
+def test_wave_front_file_is_reproduced_byte_for_byte(tmp_path):
+    """Wave_front_results.dat (write(127,*), ifort list-directed REAL(8)): all 500
+    shipped records, E form for values below 0.1 and F form with 15 significant
+    digits above, rewritten from their own parsed values."""
+    lines = gold_lines("Wave_front_results.dat")
+    assert len(lines) == 500
+    out = tmp_path / "Wave_front_results.dat"
+    for ln in lines:
+        sph2d.write_list_directed(out, [float(t) for t in ln.split()])
+    assert out.read_bytes().split(b"\n")[:-1] == lines
+
+
+@pytest.mark.parametrize("v,text", [
+    (0.01, "  1.000000000000000E-002"),
+    (0.995331222756290, "  0.995331222756290     "),
+    (4.99, "   4.99000000000000     "),
+    (-0.5, " -0.500000000000000     "),
+    (0.09999999999999999, "  0.100000000000000     "),     # 0.1 after rounding to 15 digits: F form
+    (0.0999999999999, "  9.999999999990000E-002"),
+    (123456.789, "   123456.789000000     "),
+    (999999999999999.0, "   999999999999999.     "),
+    (1e15, "  1.000000000000000E+015"),
+    (-2.5e-300, " -2.500000000000000E-300"),
+])
+def test_list_directed_real8_forms(v, text):
+    """The G-editing choice on the value rounded to 15 significant digits, the F
+    form's 5 trailing blanks and the 1P E form's 16 digits and 3-digit exponent
+    (the two forms the shipped file shows, and their boundaries)."""
+    item = sph2d.format_list_directed(v)
+    assert len(item) == 23
+    assert " " + item == text
+
+
+def test_list_directed_records_wrap_at_column_80(tmp_path):
+    """Three items fit one 80-column record (1 + 3 x 23 + 2 blanks = 72); a fourth
+    starts a new record with its own leading blank."""
+    out = tmp_path / "ld.dat"
+    sph2d.write_list_directed(out, [0.5, 0.25, 0.125, 2.0])
+    recs = out.read_text().split("\n")[:-1]
+    assert len(recs) == 2 and len(recs[0]) == 72 and recs[1] == "   2.00000000000000     "
+
+
+def test_pressure_probe_lines_use_format_310(tmp_path):
+    """The pressure probe files take format 310 (20F20.8) as Kinetic_energy.dat does:
+    a shipped row rewritten through the probe writer comes back unchanged."""
+    row = gold_lines("Kinetic_energy.dat")[3]
+    out = tmp_path / "Probe_1_Kalman_Pressure.dat"
+    sph2d.write_f20_8(out, [float(t) for t in row.split()])
+    assert out.read_bytes() == row + b"\n"
+
+
+def test_kalman_filter_first_sample_and_half_gain():
+    """Sampling_subroutine.f90:300-322: the first sampling passes the measurement and
+    sets the state to (x0 = 1, P = measurement); with P equal to R (1.0e-5 as REAL(4))
+    the next gain is exactly 1/2, so the output is the midpoint of x0 and the new
+    measurement and P becomes R/2 + Q (Q = 1.0e-6 as REAL(4))."""
+    R, Q = float(np.float32(1.0e-5)), float(np.float32(1.0e-6))
+    lx, lp = np.zeros(2), np.zeros(2)
+    first = sph2d.kalman_filter(1, [R, 0.3], lx, lp)
+    assert list(first) == [R, 0.3] and list(lx) == [1.0, 1.0] and list(lp) == [R, 0.3]
+    second = sph2d.kalman_filter(2, [3.0, 0.3], lx, lp)
+    assert second[0] == 2.0 and lx[0] == 2.0 and lp[0] == R - 0.5 * R + Q
+    assert 0.3 < second[1] < 1.0                       # between the state and the measurement
+    for k in range(3, 400):                            # a constant measurement is approached
+        last = sph2d.kalman_filter(k, [3.0, 0.3], lx, lp)
+    assert abs(last[1] - 0.3) < 1e-6 and abs(last[0] - 3.0) < 1e-6
+
+
+def test_sampling_line_points_and_position_file(tmp_path):
+    """Initialize_SPH_Calculation.f90:362-436: 41 evenly spaced points per line from
+    the first to the last point (both hit exactly), written under a two-line header
+    as 6f12.6 rows."""
+    x, y = sph2d.sampling_line_points([0.0, 0.25], [1.0, 0.25], 41)
+    assert x[0] == 0.0 and x[-1] == 1.0 and np.all(y == 0.25)
+    assert np.max(np.abs(np.diff(x) - 0.025)) < 1e-15
+    sph2d.write_sampling_line_positions(tmp_path, 2, x, y)
+    lines = (tmp_path / "2th_SamplingLine_Position.dat").read_text().split("\n")
+    assert lines[0] == " TITLE='DISTRIBUTION'" and lines[1] == " VARIABLES= 'X' 'Y'"
+    assert lines[2] == "    0.000000    0.250000" and lines[3] == "    0.025000    0.250000"
+    assert lines[42] == "    1.000000    0.250000" and lines[43] == ""
