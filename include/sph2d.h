@@ -44,6 +44,13 @@ enum {
 /* Integrators of time_iteration.f90 (leapforg_or_eular, information_module.f90:303). */
 enum { SPH2D_INTEGRATOR_PC = 2, SPH2D_INTEGRATOR_RK = 3 };
 
+/* End damping zone of single_step_compute.f90:174-260 on the fluid particles with
+ * box_x - x <= damping_length: acceleration and XSPH velocity scaled by
+ * 1 - 0.5 A (1 - cos(pi (x + L - Lx) / L)) (COSINE), by
+ * (exp(((Lx - x) / L)^3.5) - 1) / (e - 1) (y / depth) (OPENFOAM) or by
+ * 1 - 6 ((x + L - Lx) / L)^2 ((depth - y) / depth) (DUALSPH). */
+enum { SPH2D_DAMPING_OFF = 0, SPH2D_DAMPING_COSINE = 1, SPH2D_DAMPING_OPENFOAM = 2, SPH2D_DAMPING_DUALSPH = 3 };
+
 /* Arithmetic forms. */
 enum { SPH2D_FP64 = 0, SPH2D_FP32_MIXED = 1 };
 
@@ -80,6 +87,11 @@ typedef struct {
   double shift_beta, shift_R;/* shift_belta, shift_R (:350-351) */
   double max_shift;          /* max_shift = 0.25*dx (:353) */
   double mls_alpha;          /* slighted_modified_alpha (:359) */
+  int32_t damping;           /* end damping zone, SPH2D_DAMPING_* (EndDampingZoneType :286-288 with
+                                EndDampingZone_Switch 'on' and make_wave_or_not 1) */
+  double damping_length;     /* EndZoneDampingLengh (:291) */
+  double damping_amplitude;  /* EndZoneDampingAmplitude (:294), Cosin_Type only; REAL(4) 0.3 */
+  double water_depth;        /* water_depth (:194) */
 } sph2d_params;
 
 typedef struct sph2d_ctx sph2d_ctx;
@@ -234,6 +246,31 @@ int sph2d_wave_probes(sph2d_ctx* ctx, int32_t n_probes, const double* probe_x, d
  * written to dir/<k>th_<t>_SamplingLine_Results.dat. */
 int sph2d_output_sampling_lines(sph2d_ctx* ctx, int32_t i_time_step, const char* dir, int32_t n_lines,
                                 const double* first, const double* last, int32_t n_points);
+
+/* --- piston wavemaker (Wavemaker_Subroutine.f90, PistonType) ------------------- */
+enum { SPH2D_WAVE_SOLITARY = 0, SPH2D_WAVE_STOKES2 = 1, SPH2D_WAVE_AIRY = 2 };
+typedef struct {
+  int32_t wave_type;         /* SPH2D_WAVE_* (wave_type, information_module.f90:276-280) */
+  int32_t soft_steps;        /* wave_maker_soft_time (:306) */
+  int32_t standing_steps;    /* standing_time_step */
+  double dt, g;              /* dt (:309), g */
+  double water_depth;        /* water_depth (:194) */
+  double wave_height;        /* wave_height (:211) */
+  double wave_number;        /* Wave_number (:202), Stokes_2nd and Airy */
+  double wave_frequency;     /* Wave_Frequency (:201), rad/s */
+  double phase;              /* Wave_initial_phase_angle (:208) */
+} sph2d_wavemaker;
+/* Wavemaker_Subroutine.f90:66-138 and wavemaker_plate_motion.f90 (host only): the
+ * paddle position, velocity and acceleration at step i_time_step. Solitary: Goring's
+ * paddle (fixed-point iteration to 1e-4 from x = 0); Stokes_2nd and Airy: the piston
+ * laws with stroke H/m1, m1 = 2 sinh^2(kd)/(sinh(kd)cosh(kd) + kd), scaled by the
+ * soft-start coefficient, which is 0 before step soft_steps + standing_steps and 1
+ * from then on (:69-73 divide integers). */
+int sph2d_wavemaker_motion(const sph2d_wavemaker* w, int32_t i_time_step, double* position, double* velocity,
+                           double* acceleration);
+/* Wavemaker_Subroutine.f90:196-215 on the device state: fluid particles behind the
+ * paddle are mirrored in front of it; those within dx of it take its velocity. */
+int sph2d_wavemaker_pull_back(sph2d_ctx* ctx, double position, double velocity);
 
 /* Host-array writers (no device; Fortran layout, ld >= n). Text follows the
  * reference's edit descriptors exactly; files named "appended" get the

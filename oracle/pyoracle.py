@@ -37,6 +37,8 @@ class OrcParams(ctypes.Structure):
         ("shift_beta", ctypes.c_double), ("shift_R", ctypes.c_double),
         ("mls_alpha", ctypes.c_double),
         ("rate_reuse", ctypes.c_int32),
+        ("damping", ctypes.c_int32), ("damping_length", ctypes.c_double),
+        ("damping_amplitude", ctypes.c_double), ("water_depth", ctypes.c_double),
     ]
 
 
@@ -75,6 +77,7 @@ def lib():
         L.orc_sampling_points.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_double), P(ctypes.c_double),
                                           ctypes.c_double, P(ctypes.c_double)]
         L.orc_wave_probes.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_double), P(ctypes.c_double)]
+        L.orc_wavemaker_pull_back.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         L.orc_initial_ke.argtypes = [ctypes.c_void_p]
         L.orc_initial_ke.restype = ctypes.c_double
         for name in ("orc_n_fluid", "orc_n_global", "orc_n_local"):
@@ -200,6 +203,9 @@ class Oracle:
         out = np.zeros(1 + len(px))
         lib().orc_wave_probes(self._c, len(px), _dp(px) if len(px) else None, _dp(out))
         return out[0], out[1:]
+
+    def wavemaker_pull_back(self, xp: float, up: float):
+        lib().orc_wavemaker_pull_back(self._c, xp, up)
 
     def sample(self) -> np.ndarray:
         out = np.zeros(14)

@@ -109,6 +109,10 @@ void orc_default_params(orc_params* p) {
   p->epsilon_xsph = 0.0;
   p->shift_beta = 0.5; p->shift_R = 0.2;
   p->mls_alpha = (double)0.0001f;
+  p->damping = 0;                                  /* make_wave_or_not = 0 in the TGV case */
+  p->damping_length = 3.0;
+  p->damping_amplitude = (double)0.3f;
+  p->water_depth = 1.0;                            /* water_depth = domain_size_y */
 }
 
 /* ------------------------------------------------------------------ */
@@ -1011,6 +1015,35 @@ static void solver_riemann_delta_sps(orc_ctx* c) {
 /* ------------------------------------------------------------------ */
 /* single_step_compute.f90:62-171                                      */
 /* ------------------------------------------------------------------ */
+/* single_step_compute.f90:174-260: fluid particles with Lx - x <= L and x <= Lx  */
+/* get their acceleration, XSPH average velocity and subdomain velocity scaled:   */
+/* by 1 - 0.5 A (1 - cos(PI (x + L - Lx) / L)) (Cosin_Type, :181-201), by          */
+/* (exp(((Lx - x) / L)**3.5) - 1) / (exp(1.0) - 1) (y / depth) (OpenFoam_Type,     */
+/* exp(1.0) a REAL(4) intrinsic, :203-228) or by                                  */
+/* 1 - 6 ((x + L - Lx) / L)**2 ((depth - y) / depth) (DualSPH_Type, :230-255).     */
+static void end_damping_zone(orc_ctx* c) {
+  const orc_params* P = &c->P;
+  const double Lx = P->box_x, L = P->damping_length, d = P->water_depth;
+  for (int i = 0; i < c->n_local; ++i) {
+    if (c->type[i] != 2) continue;
+    const double x = c->x[i], y = c->y[i];
+    if (!((Lx - x) <= L && x <= Lx)) continue;
+    double f;
+    if (P->damping == 1) {
+      const double coef = 0.5 * P->damping_amplitude * (1 - cos(PI_MODULE * (x + L - Lx) / L));
+      f = 1 - coef;
+    } else if (P->damping == 2) {
+      f = (exp(pow((Lx - x) / L, 3.5)) - 1.0) / ((double)expf(1.0f) - 1.0) * (y / d);
+    } else {
+      const double q = (x + L - Lx) / L;
+      f = 1 - 6 * (q * q) * ((d - y) / d);
+    }
+    c->ax[i] = c->ax[i] * f; c->ay[i] = c->ay[i] * f;
+    c->avg_x[i] = c->avg_x[i] * f; c->avg_y[i] = c->avg_y[i] * f;
+    c->u[i] = c->u[i] * f; c->v[i] = c->v[i] * f;
+  }
+}
+
 void orc_single_step_compute(orc_ctx* c, int32_t i_time_step) {
   (void)i_time_step;
   int n = c->n_local;
@@ -1040,6 +1073,7 @@ void orc_single_step_compute(orc_ctx* c, int32_t i_time_step) {
     c->ay[i] = c->ia_y[i] + c->ext_y[i] + c->av_y[i] + c->ai_y[i] + c->sps_y[i] + c->lap_y[i];
     c->dedt[i] = c->dedt[i] + 0.0 + 0.0;
   }
+  if (c->P.damping != 0) end_damping_zone(c);                     /* :174-260 */
 }
 
 /* ------------------------------------------------------------------ */
@@ -1680,4 +1714,14 @@ void orc_wave_probes(orc_ctx* c, int32_t n_probes, const double* probe_x, double
     if (c->gx[i] >= front) front = c->gx[i];
   }
   out[0] = front;
+}
+
+/* Wavemaker_Subroutine.f90:196-215 on the global arrays: fluid particles (not typed */
+/* 120) with x < xp are mirrored to 2 xp - x; then those with x - xp < dx take u = up. */
+void orc_wavemaker_pull_back(orc_ctx* c, double xp, double up) {
+  for (int i = 0; i < c->n_fluid; ++i) {
+    if (c->gtype[i] == 120) continue;
+    if (c->gx[i] < xp) c->gx[i] = 2 * xp - c->gx[i];
+    if ((c->gx[i] - xp) < c->dx) c->gu[i] = up;
+  }
 }

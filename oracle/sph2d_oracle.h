@@ -40,6 +40,12 @@ typedef struct {
   double shift_beta, shift_R;/* shift_belta, shift_R                       (:350-351) */
   double mls_alpha;          /* slighted_modified_alpha (REAL(4) 0.0001)   (:359)    */
   int32_t rate_reuse;        /* 0 = reference (stale local index), 1 = carried by gid  */
+  /* end damping zone (single_step_compute.f90:174-260; EndDampingZone_Switch 'on' and */
+  /* make_wave_or_not 1): 0 off, 1 Cosin_Type, 2 OpenFoam_Type, 3 DualSPH_Type          */
+  int32_t damping;
+  double damping_length;     /* EndZoneDampingLengh                        (:291)    */
+  double damping_amplitude;  /* EndZoneDampingAmplitude (Cosin_Type)       (:294)    */
+  double water_depth;        /* water_depth                                (:194)    */
 } orc_params;
 
 enum { ORC_SOLVER_MODIFIED_LINEAR_RIEMANN = 0,   /* Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90 */
@@ -99,6 +105,7 @@ void orc_refresh(orc_ctx* c);                               /* Refresh_domain (1
 void orc_grid_postprocess(orc_ctx* c, int32_t i_time_step, int32_t nex, int32_t ney, double edx, double* out);
 void orc_sampling_points(orc_ctx* c, int32_t n, const double* px, const double* py, double gravity, double* out);
 void orc_wave_probes(orc_ctx* c, int32_t n_probes, const double* probe_x, double* out);
+void orc_wavemaker_pull_back(orc_ctx* c, double xp, double up);
 
 /* Sampling_subroutine.f90:340-461. out[0]=KE, [1]=max|v|, [2..4]=u Linf,L1,L2,
  * [5..7]=v, [8..10]=p/(rho0 U^2), [11..13]=|v|. */

@@ -124,6 +124,11 @@ void make_devparams(const sph2d_params& p, DevParams& P) {
     P.sps_c23b = (double)(float)(a * b);
   }
   P.mls_alpha = p.mls_alpha;
+  P.damping = p.damping;
+  P.damp_len = p.damping_length;
+  P.damp_amp = p.damping_amplitude;
+  P.water_depth = p.water_depth;
+  P.e32 = (double)expf(1.0f);
   P.surf_lim = (double)1.414f * p.h;
   P.inv_h = 1.0 / p.h;
   P.inv_c0 = 1.0 / std::sqrt(p.c0sq);
@@ -177,6 +182,10 @@ int check_params(const sph2d_params* p) {
   if (!p->periodic) return fail(SPH2D_ERR_UNSUPPORTED, "only the periodic box is built");
   if (p->mls_every <= 0 || p->sample_every <= 0)
     return fail(SPH2D_ERR_ARGUMENT, "mls_every and sample_every must be positive");
+  if (p->damping < SPH2D_DAMPING_OFF || p->damping > SPH2D_DAMPING_DUALSPH)
+    return fail(SPH2D_ERR_ARGUMENT, "damping must be one of SPH2D_DAMPING_*");
+  if (p->damping != SPH2D_DAMPING_OFF && (!(p->damping_length > 0) || !(p->water_depth > 0)))
+    return fail(SPH2D_ERR_ARGUMENT, "damping_length and water_depth must be positive");
   return SPH2D_OK;
 }
 
@@ -260,6 +269,10 @@ int sph2d_default_params(sph2d_params* p) {
   p->shift_beta = 0.5; p->shift_R = 0.2;
   p->max_shift = 0.25 * p->dx;
   p->mls_alpha = (double)0.0001f;
+  p->damping = SPH2D_DAMPING_OFF;                  // make_wave_or_not = 0 in the TGV case
+  p->damping_length = 3.0;
+  p->damping_amplitude = (double)0.3f;
+  p->water_depth = p->box_y;                       // water_depth = domain_size_y
   return SPH2D_OK;
 }
 
@@ -627,6 +640,7 @@ static void do_eval(sph2d_ctx* c, bool with_eos, int it) {
     }
   }
   launch_eval(c->stream, c->P, c->B, c->cap_local, ev);
+  if (c->P.damping) launch_end_damping(c->stream, c->P, c->B, c->cap_local);
   c->evaluated = true;
   // Refresh_domain's vorticity uses the step's last evaluation; it is kept on
   // sampling steps (each evaluation of such a step overwrites it).
@@ -1068,6 +1082,17 @@ int sph2d_output_sampling_lines(sph2d_ctx* c, int32_t it, const char* dir, int32
   return SPH2D_OK;
 }
 
+// ---- piston wavemaker -------------------------------------------------------------
+int sph2d_wavemaker_pull_back(sph2d_ctx* c, double position, double velocity) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (!std::isfinite(position) || !std::isfinite(velocity))
+    return fail(SPH2D_ERR_ARGUMENT, "paddle position and velocity must be finite");
+  launch_wavemaker_pull_back(c->stream, c->P, c->B, c->cap_fluid, position, velocity);
+  HIP_OR_FAIL(hipGetLastError());
+  return SPH2D_OK;
+}
+
 int sph2d_output_for_paraview(sph2d_ctx* c, int32_t it, const char* path) {
   (void)it;                                          // the caller names the file by the step's time
   int rc = need_state(c);
@@ -1308,3 +1333,74 @@ int sph2d_get_stream(sph2d_ctx* c, void** stream) {
 }
 
 }  // extern "C"
+
+// Wavemaker_Subroutine.f90:66-138 (PistonType) and wavemaker_plate_motion.f90 (Goring's
+// solitary-wave paddle). Host arithmetic in double; the reference's REAL(4) literals
+// (EZ = 0.002, CERR = 0.0001) keep their single-precision values.
+int sph2d_wavemaker_motion(const sph2d_wavemaker* w, int32_t it, double* position, double* velocity,
+                           double* acceleration) {
+  if (!w || !position || !velocity || !acceleration) return fail(SPH2D_ERR_ARGUMENT, "NULL argument");
+  if (w->wave_type < SPH2D_WAVE_SOLITARY || w->wave_type > SPH2D_WAVE_AIRY)
+    return fail(SPH2D_ERR_ARGUMENT, "wave_type must be one of SPH2D_WAVE_*");
+  if (!(w->dt > 0) || !(w->water_depth > 0) || w->soft_steps <= 0)
+    return fail(SPH2D_ERR_ARGUMENT, "dt, water_depth and soft_steps must be positive");
+  const double pi = kPiModule;
+  // :69-73. (i - standing) / soft is an integer division, so the coefficient is
+  // 0.5 (sin(-pi/2) + 1) = 0 before the soft time ends and 1 from then on.
+  double soft = 1.0;
+  if (it <= w->soft_steps + w->standing_steps) {
+    const int q = (it - w->standing_steps) / w->soft_steps;
+    soft = 0.5 * (std::sin((double)(-0.5f + (float)q) * pi) + 1);
+  }
+  double x = 0.0, u = 0.0, a = 0.0;
+  if (it > w->standing_steps) {
+    const double t = w->dt * (it - w->standing_steps);
+    const double d = w->water_depth, H = w->wave_height;
+    if (w->wave_type == SPH2D_WAVE_SOLITARY) {
+      // wavemaker_plate_motion.f90:22-58, called with WMX = WMV = 0 (:75-77)
+      const double ez = (double)0.002f, g = w->g;
+      const double cl = std::log((1. / std::pow(ez, 0.5)) * (1 + std::sqrt(1. - ez)));
+      const double ck = std::sqrt(3. * H / (4. * d));
+      const double c = std::sqrt(g * (d + H));
+      const double clamda = cl / ck * d;
+      const double cerr = (double)0.0001f;
+      double xp = 0.0 + 0.0 * w->dt, xt, err;
+      int iter = 0;
+      do {
+        xt = ck / d * (c * t - xp - clamda);
+        const double xpi = H / ck * (std::tanh(xt) + std::tanh(ck * clamda / d));
+        err = std::fabs(xpi - xp);
+        xp = xpi;
+        if (++iter > 100000) return fail(SPH2D_ERR_NONFINITE, "solitary paddle iteration did not converge");
+      } while (err > cerr);
+      const double ch = std::cosh(xt), sh = std::sinh(xt);
+      const double up = H * c / (d * (std::pow(ch, 2.) + H / d));
+      double vp = 2 * H * c * ch * sh * ck * c;
+      const double den = d * (std::pow(ch, 2.) + H / d);
+      vp = vp / (den * den);
+      x = xp; u = up; a = vp;
+    } else {
+      // :93-135 (Stokes_2nd and Airy, DualSPHysics' long-crested piston laws)
+      const double k = w->wave_number, om = w->wave_frequency;
+      const double sh = std::sinh(k * d), ch = std::cosh(k * d);
+      const double m1 = 2 * (sh * sh) / (sh * ch + k * d);
+      const double th = om * t + w->phase;
+      const double S = H / m1;
+      x = S / 2.0 * std::sin(th);
+      u = S / 2.0 * om * std::cos(th);
+      a = -S / 2.0 * (om * om) * std::sin(th);
+      if (w->wave_type == SPH2D_WAVE_STOKES2) {
+        const double s3 = std::sinh(d * k);
+        const double b = (H * H / (32 * d)) * (3 * std::cosh(d * k) / (s3 * s3 * s3)) - 2.0 / m1;
+        x = x + b * std::sin(2 * th);
+        u = u + b * 2 * om * std::cos(2 * th);
+        a = a - b * 4 * (om * om) * std::sin(2 * th);
+      }
+      x = soft * x; u = soft * u; a = soft * a;
+    }
+  }
+  *position = x;
+  *velocity = u;
+  *acceleration = a;
+  return SPH2D_OK;
+}

@@ -982,6 +982,61 @@ __global__ void __launch_bounds__(kBlock) k_wave_probes(DevParams P, Buffers B, 
   }
 }
 
+// single_step_compute.f90:174-260, the end damping zone, after the rate assembly of an
+// evaluation: fluid particles with box_x - x <= L and x <= box_x (local positions of
+// the evaluation) get their acceleration and XSPH velocity scaled by the zone's factor.
+// The reference also scales subdomain_particle_velocity there; every stage update
+// overwrites that velocity before it is read again (time_iteration.f90:559-565,
+// :590-594 and the RK-variant stages), so only the rates are scaled here.
+__global__ void __launch_bounds__(kBlock) k_end_damping(DevParams P, Buffers B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.counters[C_NLOCAL] || B.l_type[s] != 2) return;
+  const double2 x = B.l_pos[s];
+  const double Lx = P.box_x, L = P.damp_len, d = P.water_depth;
+  if (!((Lx - x.x) <= L && x.x <= Lx)) return;
+  double f;
+  if (P.damping == 1) {                                   // Cosin_Type (:181-201)
+    const double pi = 3.14159265358979324;
+    f = 1 - 0.5 * P.damp_amp * (1 - cos(pi * (x.x + L - Lx) / L));
+  } else if (P.damping == 2) {                            // OpenFoam_Type (:203-228)
+    f = (exp(pow((Lx - x.x) / L, 3.5)) - 1.0) / (P.e32 - 1.0) * (x.y / d);
+  } else {                                                // DualSPH_Type (:230-255)
+    const double q = (x.x + L - Lx) / L;
+    f = 1 - 6 * (q * q) * ((d - x.y) / d);
+  }
+  const double2 a = B.acc[s];
+  B.acc[s] = make_double2(a.x * f, a.y * f);
+  if (P.epsilon_xsph != 0.0) {
+    const double2 v = B.avg[s];
+    B.avg[s] = make_double2(v.x * f, v.y * f);
+  }
+}
+
+// Wavemaker_Subroutine.f90:196-215: each fluid particle (not typed 120) behind the
+// paddle (x < x_p) is mirrored to x = 2 x_p - x; one within dx in front of it
+// (x - x_p < dx, after the mirroring) takes the paddle's horizontal velocity.
+__global__ void __launch_bounds__(kBlock) k_wavemaker_pull_back(DevParams P, Buffers B, double xp, double up) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= B.counters[C_NFLUID] || B.g_type[g] == 120) return;
+  double2 x = B.g_pos[g];
+  if (x.x < xp) {
+    x.x = 2 * xp - x.x;
+    B.g_pos[g] = x;
+  }
+  if ((x.x - xp) < P.dx) {
+    const double2 v = B.g_vel[g];
+    B.g_vel[g] = make_double2(up, v.y);
+  }
+}
+
+void launch_wavemaker_pull_back(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, double xp, double up) {
+  hipLaunchKernelGGL(k_wavemaker_pull_back, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, s, P, B, xp, up);
+}
+
+void launch_end_damping(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
+  hipLaunchKernelGGL(k_end_damping, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B);
+}
+
 void launch_wave_probes(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, const double* probe_x,
                         int n_probes, unsigned long long* out_bits) {
   hipLaunchKernelGGL(k_wave_probes, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, s, P, B, probe_x, n_probes,

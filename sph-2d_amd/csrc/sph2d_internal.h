@@ -32,6 +32,8 @@ struct DevParams {
   double chain_d, chain_ox, chain_oy;    // 4h cells, origin -5 cells
   int chain_nx, chain_ny;
   double alpha, beta, fai, delta, delta_alpha, epsilon_xsph;
+  int damping;                           // SPH2D_DAMPING_*
+  double damp_len, damp_amp, water_depth, e32;   // e32 = exp(1.0) in REAL(4)
   double shift_beta, shift_R, max_shift, w_dx, h2;
   double sps_dl, sps_c012, sps_c23, sps_c23b;  // REAL(4) constants of the SPS closure
   double mls_alpha;
@@ -130,6 +132,8 @@ struct GridArgs {
   double* out;                            // 9 blocks of n_nodes
 };
 void launch_grid_mls(hipStream_t s, const DevParams& P, Buffers& B, const GridArgs& G);
+void launch_end_damping(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+void launch_wavemaker_pull_back(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, double xp, double up);
 // Sampling_subroutine.f90 at sampling points (sph2d_fp64.hip)
 struct ProbeArgs {
   int n;

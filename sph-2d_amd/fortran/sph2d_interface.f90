@@ -21,6 +21,8 @@ module sph2d_interface
     integer(c_int32_t) :: shift_on, mls_on, mls_every, mls_order, sample_every, periodic
     real(c_double) :: alpha, beta, fai, delta, delta_alpha, epsilon_xsph
     real(c_double) :: shift_beta, shift_R, max_shift, mls_alpha
+    integer(c_int32_t) :: damping
+    real(c_double) :: damping_length, damping_amplitude, water_depth
   end type sph2d_params
 
   interface

@@ -28,6 +28,10 @@ INTEGRATOR_PC = 2
 INTEGRATOR_RK = 3
 FP64 = 0
 FP32_MIXED = 1
+DAMPING_OFF = 0
+DAMPING_COSINE = 1
+DAMPING_OPENFOAM = 2
+DAMPING_DUALSPH = 3
 
 ERRORS = {1: "argument", 2: "capacity", 3: "hip", 4: "state", 5: "unsupported", 6: "nonfinite"}
 
@@ -57,7 +61,30 @@ class Params(ctypes.Structure):
         ("epsilon_xsph", ctypes.c_double),
         ("shift_beta", ctypes.c_double), ("shift_R", ctypes.c_double),
         ("max_shift", ctypes.c_double), ("mls_alpha", ctypes.c_double),
+        ("damping", ctypes.c_int32), ("damping_length", ctypes.c_double),
+        ("damping_amplitude", ctypes.c_double), ("water_depth", ctypes.c_double),
     ]
+
+
+WAVE_SOLITARY = 0
+WAVE_STOKES2 = 1
+WAVE_AIRY = 2
+
+
+class Wavemaker(ctypes.Structure):
+    """sph2d_wavemaker: the piston wavemaker's inputs (Wavemaker_Subroutine.f90:66-138)."""
+    _fields_ = [("wave_type", ctypes.c_int32), ("soft_steps", ctypes.c_int32), ("standing_steps", ctypes.c_int32),
+                ("dt", ctypes.c_double), ("g", ctypes.c_double), ("water_depth", ctypes.c_double),
+                ("wave_height", ctypes.c_double), ("wave_number", ctypes.c_double),
+                ("wave_frequency", ctypes.c_double), ("phase", ctypes.c_double)]
+
+
+def wavemaker_motion(w: Wavemaker, i_time_step: int):
+    """(position, velocity, acceleration) of the paddle at step i_time_step."""
+    x, u, a = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    _check(lib().sph2d_wavemaker_motion(ctypes.byref(w), i_time_step, ctypes.byref(x), ctypes.byref(u),
+                                        ctypes.byref(a)))
+    return x.value, u.value, a.value
 
 
 class Scales(ctypes.Structure):
@@ -95,6 +122,7 @@ EXPORTS = (
     "sph2d_write_grid_binary", "sph2d_sampling", "sph2d_wave_probes", "sph2d_output_sampling_lines",
     "sph2d_format_list_directed", "sph2d_write_list_directed", "sph2d_write_f20_8", "sph2d_kalman_filter",
     "sph2d_sampling_line_points", "sph2d_write_sampling_line_positions", "sph2d_write_sampling_line_results",
+    "sph2d_wavemaker_motion", "sph2d_wavemaker_pull_back",
 )
 
 _lib = None
@@ -173,6 +201,8 @@ def lib():
             "sph2d_write_sampling_line_positions": [cs, i32, i32, dp, i32],
             "sph2d_write_sampling_line_results": [cs, i32, ctypes.c_double, i32, dp, dp, dp, dp, dp, dp, i32,
                                                   P(Scales)],
+            "sph2d_wavemaker_motion": [P(Wavemaker), i32, dp, dp, dp],
+            "sph2d_wavemaker_pull_back": [vp, ctypes.c_double, ctypes.c_double],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -377,6 +407,10 @@ class Simulation:
         l = _f64(np.asarray(last, dtype=np.float64).reshape(-1))
         _check(lib().sph2d_output_sampling_lines(self._c, i_time_step, os.fspath(dir_path).encode(), len(f) // 2,
                                                  _dp(f), _dp(l), n_points))
+
+    def wavemaker_pull_back(self, position: float, velocity: float):
+        """Wavemaker_Subroutine.f90:196-215: mirror fluid particles behind the paddle."""
+        _check(lib().sph2d_wavemaker_pull_back(self._c, position, velocity))
 
     def output_final_results(self, path):
         """Final_Doamin_Results.dat (Parallel_Taylor_Green_Vortices_2D_Release.f90:185-199)."""

@@ -389,3 +389,23 @@ def test_sampling_files_of_a_context(tmp_path):
         assert np.array_equal(kal, v)                    # the first sampling passes the value through
     assert (tmp_path / "Sampling" / "WaveHeight" / "wave_gauge_1_result.dat").exists()
     sim.close()
+
+
+def test_wavemaker_pull_back_matches_the_oracle():
+    say("test_wavemaker_pull_back_matches_the_oracle")
+    n = 40
+    prm = sph2d.tgv_params(n)
+    sim = sph2d.Simulation(prm)
+    sim.init_tgv()
+    g0 = sim.download_state()
+    o = Oracle(nx=n, ny=n, dt=prm.dt)
+    o.set_fluid_state(g0["x"], g0["y"], g0["u"], g0["v"], g0["rho"], g0["mass"])
+    xp, up = 0.3 + 0.37 / n, 0.25
+    sim.wavemaker_pull_back(xp, up)
+    o.wavemaker_pull_back(xp, up)
+    g = sim.download_state()
+    N = n * n
+    for f, of in (("x", "g_x"), ("y", "g_y"), ("u", "g_u"), ("v", "g_v")):
+        assert np.array_equal(g[f], o.f64(of)[:N]), f
+    assert np.all(g["x"] >= xp)
+    sim.close()

@@ -33,11 +33,14 @@ def rel_err(g, r):
     return float(np.max(np.abs(g - r)) / scale)
 
 
+ORACLE_KW = ("shift_on", "mls_on", "solver", "damping", "damping_length", "damping_amplitude", "water_depth")
+
+
 def pair(n, integrator=2, state=None, **kw):
     """GPU simulation and oracle on the same TGV lattice (optionally a perturbed state)."""
     prm = sph2d.tgv_params(n, integrator=integrator, **kw)
     sim = sph2d.Simulation(prm)
-    o = Oracle(nx=n, ny=n, integrator=integrator, dt=prm.dt, **{k: v for k, v in kw.items() if k in ("shift_on", "mls_on", "solver")})
+    o = Oracle(nx=n, ny=n, integrator=integrator, dt=prm.dt, **{k: v for k, v in kw.items() if k in ORACLE_KW})
     o.init_tgv()
     if state is None:
         sim.init_tgv()
@@ -421,3 +424,36 @@ def test_neighbour_masks_exact_at_the_support_boundary():
         ref[i] = int(np.sum(ok))
     bad = np.nonzero(cnt != ref)[0]
     assert len(bad) == 0, (bad[:10], cnt[bad[:10]], ref[bad[:10]])
+
+
+@pytest.mark.parametrize("damping", [sph2d.DAMPING_COSINE, sph2d.DAMPING_OPENFOAM, sph2d.DAMPING_DUALSPH])
+def test_end_damping_zone_parity(damping):
+    """single_step_compute.f90:174-260 on the last 0.3 of the box: one evaluation's
+    rates, then 21 predictor-corrector steps (an MLS step included) against the
+    oracle, which also scales the subdomain velocity as the reference does; the
+    zone changes the flow near x = 1 by far more than the bar."""
+    say(f"test_end_damping_zone_parity damping={damping}")
+    n = 48
+    kw = dict(damping=damping, damping_length=0.3, damping_amplitude=float(np.float32(0.3)), water_depth=1.0)
+    sim, o = pair(n, integrator=2, **kw)
+    evaluate_both(sim, o)
+    for f in ("ax", "ay", "drhodt"):
+        assert rel_err(sim.local(f), o.f64(f)) < TOL, f
+    sim.close()
+    sim, o = pair(n, integrator=2, **kw)
+    free = Oracle(nx=n, ny=n, integrator=2, dt=sim.params.dt)
+    free.init_tgv()
+    for it in range(1, 22):
+        sim.time_step(it)
+        o.time_step(it)
+        free.time_step(it)
+    g = sim.download_state()
+    N = n * n
+    for f, of in (("x", "g_x"), ("y", "g_y"), ("u", "g_u"), ("v", "g_v"), ("rho", "g_rho"), ("p", "g_p")):
+        r = o.f64(of)[:N]
+        e = float(np.max(np.abs(g[f] - r))) if f in ("x", "y") else rel_err(g[f], r)
+        assert e < (1e-12 if f in ("x", "y") else 1e-10), (f, e)
+    effect = rel_err(o.f64("g_u")[:N], free.f64("g_u")[:N])
+    say(f"damping effect on u: {effect:.3e}")
+    assert effect > 1e-4
+    sim.close()

@@ -495,3 +495,83 @@ def test_wave_front_and_wave_probes_by_brute_force():
         assert el[j] == (max(0.0, y[sel].max()) if sel.any() else 0.0), j
     assert el[3] == 0.0
     o.close()
+
+
+# --- end damping zone (single_step_compute.f90:174-260) ------------------------------------
+def _damped_ratio(damping, L, n=40):
+    """Per-particle ratio of the damped to the undamped acceleration of one evaluation
+    (the zone acts after the rate assembly, so the ratio is the zone factor)."""
+    out = []
+    for d in (0, damping):
+        o = evaluate(tgv(n, damping=d, damping_length=L, water_depth=1.0))
+        out.append((o.f64("x").copy(), o.f64("y").copy(), o.f64("ax").copy(), o.f64("ay").copy(),
+                    o.i32("type").copy()))
+        o.close()
+    (x, y, a0, b0, t), (_, _, a1, b1, _) = out
+    fl = (t == 2) & (np.abs(a0) > 1e-3)
+    return x[fl], y[fl], a1[fl] / a0[fl], a0[fl], a1[fl]
+
+
+def test_end_damping_leaves_the_outside_alone_and_cosine_special_values():
+    """Outside the zone the rates are untouched (bitwise). Cosin_Type: the factor is
+    a function of x alone, 1 at the zone start, 1 - A/2 at its midpoint (cos(pi/2))
+    and decreasing towards the wall; A is the REAL(4) 0.3."""
+    n = 40
+    xm = (34 - 0.5) / n                          # a lattice column, taken as the midpoint
+    L = 2 * (1.0 - xm)
+    x, y, r, a0, a1 = _damped_ratio(1, L, n)
+    inside = (1.0 - x) <= L
+    assert np.array_equal(a0[~inside], a1[~inside])
+    A = float(np.float32(0.3))
+    mid = np.abs(x - xm) < 1e-12
+    assert mid.sum() > 10 and np.allclose(r[mid], 1 - A / 2, rtol=0, atol=1e-14)
+    xs = np.unique(np.round(x[inside], 12))
+    rs = np.array([r[np.abs(x - v) < 1e-9].mean() for v in xs])
+    assert np.all(np.diff(rs) < 0) and rs.max() <= 1.0 and rs.min() >= 1 - A
+    for v in xs:                                 # no dependence on y
+        rv = r[np.abs(x - v) < 1e-9]
+        assert np.ptp(rv) < 1e-13
+
+
+def test_end_damping_openfoam_and_dualsph_shapes():
+    """OpenFoam_Type: factor / (y/depth) depends on x alone, vanishes at the wall and
+    tends to (e - 1)/(e_REAL(4) - 1) ~ 1 at the zone start. DualSPH_Type: the factor
+    is 1 - 6 q^2 (depth - y)/depth with q = (x + L - Lx)/L, so
+    sqrt((1 - factor) / (6 (depth - y)/depth)) is linear in x with slope 1/L."""
+    L = 0.3
+    x, y, r, _, _ = _damped_ratio(2, L)
+    inside = (1.0 - x) <= L
+    g = r[inside] / y[inside]
+    xi = x[inside]
+    for v in np.unique(np.round(xi, 12)):
+        gv = g[np.abs(xi - v) < 1e-9]
+        assert np.ptp(gv) < 1e-12 * max(1.0, np.abs(gv).max())
+    far = np.argmin(xi)
+    assert 0.0 < g.min() and g.max() <= (np.e - 1) / (float(np.exp(np.float32(1))) - 1) + 1e-12
+    assert abs(g[far] - g.max()) < 1e-14
+    x, y, r, _, _ = _damped_ratio(3, L)
+    inside = (1.0 - x) <= L
+    q = np.sqrt((1 - r[inside]) / (6 * (1.0 - y[inside])))
+    fit = np.polyfit(x[inside], q, 1)
+    assert abs(fit[0] - 1 / L) < 1e-9 / L
+    assert np.max(np.abs(np.polyval(fit, x[inside]) - q)) < 1e-9
+
+
+def test_wavemaker_pull_back_mirrors_behind_the_paddle():
+    """Wavemaker_Subroutine.f90:196-215: particles behind the paddle land at their
+    mirror image 2 xp - x, every particle ends in front of it, and exactly those
+    within dx of it (after mirroring) carry the paddle's velocity."""
+    n = 40
+    o = tgv(n)
+    N = n * n
+    x0, u0 = o.f64("g_x")[:N].copy(), o.f64("g_u")[:N].copy()
+    xp, up = 0.3 + 0.37 / n, 0.25
+    o.wavemaker_pull_back(xp, up)
+    x1, u1 = o.f64("g_x")[:N], o.f64("g_u")[:N]
+    behind = x0 < xp
+    assert behind.sum() > 100
+    assert np.all(x1[behind] == 2 * xp - x0[behind]) and np.all(x1[~behind] == x0[~behind])
+    assert np.all(x1 >= xp)
+    near = (x1 - xp) < 1.0 / n
+    assert np.all(u1[near] == up) and np.all(u1[~near] == u0[~near]) and near.sum() >= 2 * n
+    o.close()
