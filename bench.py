@@ -85,6 +85,63 @@ def cpu_baseline(n_side: int = 512, steps: int = 20) -> dict:
                       f"timed ({dt:.1f} s); oracle/liboracle.so single-threaded"}
 
 
+def batched_leg(sph2d, instances: int, steps: int, warmup: int) -> dict:
+    """Many independent periodic TGV instances of uneven size and Reynolds number
+    (n_side = round(64 * 8^U), Re = 50 * 20^U', U, U' uniform, seed 3), FP32-mixed
+    predictor-corrector: one sph2d_batch run (each replayable step one hipGraph
+    launch) against the same instances run one after another, each alone."""
+    import numpy as np
+    rng = np.random.default_rng(3)
+    sides = [int(round(64 * 8 ** u)) for u in rng.uniform(0, 1, instances)]
+    res = [50 * 20 ** u for u in rng.uniform(0, 1, instances)]
+    total = sum(n * n for n in sides)
+
+    def make():
+        sims = []
+        for n, re in zip(sides, res):
+            sim = sph2d.Simulation(sph2d.tgv_params(n, integrator=sph2d.INTEGRATOR_PC, precision=sph2d.FP32_MIXED,
+                                                    nu=1.0 / re))
+            sim.init_tgv()
+            sims.append(sim)
+        return sims
+
+    hip = hip_events()
+    sims = make()
+    batch = sph2d.Batch(sims)
+    batch.run(1, warmup)
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    batch.run(1 + warmup, steps)
+    hip.hipDeviceSynchronize()
+    t_batch = time.perf_counter() - t0
+    nodes, replays = batch.info()
+    ke = [s.sample()[0] for s in sims]
+    batch.close()
+    for s in sims:
+        s.close()
+    sims = make()
+    for s in sims:
+        s.run(1, warmup)
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for s in sims:
+        s.run(1 + warmup, steps)
+        hip.hipDeviceSynchronize()
+    t_seq = time.perf_counter() - t0
+    same = all(abs(s.sample()[0] - k) == 0.0 for s, k in zip(sims, ke))
+    for s in sims:
+        s.close()
+    print(f"# batched: {instances} instances, {total} particles, {steps} steps: graph {t_batch:.3f} s, "
+          f"one after another {t_seq:.3f} s", file=sys.stderr, flush=True)
+    return {"instances": instances, "particles": total, "n_side_min": min(sides), "n_side_max": max(sides),
+            "re_min": min(res), "re_max": max(res), "steps": steps, "form": "FP32-mixed, predictor-corrector",
+            "particle_steps_per_s": total * steps / t_batch, "ms_per_step": 1000.0 * t_batch / steps,
+            "sequential_particle_steps_per_s": total * steps / t_seq,
+            "speedup_vs_sequential": t_seq / t_batch, "graph_nodes": nodes, "graph_replays": replays,
+            "ke_identical_to_sequential": same,
+            "timing": "host wall clock with device synchronisation, launches included"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +157,9 @@ def main() -> int:
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB: a plain copy kernel of "
                          "known size that scales rocprofv3's FETCH_SIZE/WRITE_SIZE counters")
+    ap.add_argument("--batch-instances", type=int, default=64,
+                    help="batched leg: independent TGV instances advanced by sph2d_batch (0 skips it)")
+    ap.add_argument("--batch-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU form (spatial slabs "
@@ -232,6 +292,11 @@ def main() -> int:
             "launches_timed": r32["n_eval_timed"],
         },
     }
+    if args.batch_instances > 0:
+        try:
+            out["batched"] = batched_leg(sph2d, args.batch_instances, args.batch_steps, args.warmup)
+        except Exception as e:                    # the headline line is printed regardless
+            out["batched"] = {"error": repr(e)}
     if rank == 0 and not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_side, args.cpu_steps)

@@ -247,6 +247,21 @@ int sph2d_wave_probes(sph2d_ctx* ctx, int32_t n_probes, const double* probe_x, d
 int sph2d_output_sampling_lines(sph2d_ctx* ctx, int32_t i_time_step, const char* dir, int32_t n_lines,
                                 const double* first, const double* last, int32_t n_points);
 
+/* --- batched form (no reference counterpart) ------------------------------------- */
+/* Independent contexts (uneven sizes, any Re) advanced together. Steps after step 1
+ * that are neither MLS nor sampling steps replay one hipGraph holding every
+ * instance's step as a parallel branch; the other steps are enqueued per context.
+ * Each instance's results are bit-identical to running it alone. The contexts must
+ * share integrator, mls_on, mls_every and sample_every, outlive the batch and not
+ * be used by other calls while sph2d_batch_run runs; on return, later calls on a
+ * context are ordered after the batch's work. */
+typedef struct sph2d_batch sph2d_batch;
+int sph2d_batch_create(int32_t n, sph2d_ctx* const* ctxs, sph2d_batch** out);
+int sph2d_batch_run(sph2d_batch* batch, int32_t first_step, int32_t nsteps);
+/* Nodes of the captured step graph (0 before the first replayable step) and replays so far. */
+int sph2d_batch_info(sph2d_batch* batch, int32_t* graph_nodes, int32_t* graph_replays);
+int sph2d_batch_destroy(sph2d_batch* batch);
+
 /* --- piston wavemaker (Wavemaker_Subroutine.f90, PistonType) ------------------- */
 enum { SPH2D_WAVE_SOLITARY = 0, SPH2D_WAVE_STOKES2 = 1, SPH2D_WAVE_AIRY = 2 };
 typedef struct {

@@ -1404,3 +1404,156 @@ int sph2d_wavemaker_motion(const sph2d_wavemaker* w, int32_t it, double* positio
   *acceleration = a;
   return SPH2D_OK;
 }
+
+// ---- batched form: independent contexts advanced together ------------------------------
+// Steps whose kernel sequence and arguments repeat (after step 1, neither an MLS nor a
+// sampling step) are replayed from one hipGraph that holds every instance's step as a
+// parallel branch; step 1, MLS and sampling steps are enqueued per context. The
+// instances never exchange data, so each one's results are those of running it alone.
+struct sph2d_batch {
+  std::vector<sph2d_ctx*> ctx;
+  hipStream_t stream = nullptr;               // graph launches
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join, sync;          // per context: capture joins, eager/graph ordering
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  size_t nodes = 0;
+  int replays = 0;
+  bool on_batch = false;                       // the latest work of the batch is on `stream`
+};
+
+namespace {
+bool replayable(const sph2d_ctx* c, int it) {
+  if (it <= 1 || c->debug || c->pass_timing) return false;
+  if (c->prm.mls_on && it % c->prm.mls_every == 0) return false;
+  return it % c->prm.sample_every != 0;
+}
+
+// work enqueued on the context streams completes before what follows on the batch stream
+int batch_join(sph2d_batch* b) {
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    HIP_OR_FAIL(hipEventRecord(b->sync[i], b->ctx[i]->stream));
+    HIP_OR_FAIL(hipStreamWaitEvent(b->stream, b->sync[i], 0));
+  }
+  return SPH2D_OK;
+}
+
+// work on the batch stream completes before what follows on each context stream
+int batch_release(sph2d_batch* b) {
+  HIP_OR_FAIL(hipEventRecord(b->fork, b->stream));
+  for (sph2d_ctx* c : b->ctx) HIP_OR_FAIL(hipStreamWaitEvent(c->stream, b->fork, 0));
+  return SPH2D_OK;
+}
+
+int batch_capture(sph2d_batch* b, int it) {
+  HIP_OR_FAIL(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+  int rc = SPH2D_OK;
+  hipError_t e = hipEventRecord(b->fork, b->stream);
+  for (size_t i = 0; i < b->ctx.size() && e == hipSuccess && rc == SPH2D_OK; ++i) {
+    sph2d_ctx* c = b->ctx[i];
+    e = hipStreamWaitEvent(c->stream, b->fork, 0);
+    if (e == hipSuccess) rc = enqueue_step(c, it);
+    if (e == hipSuccess && rc == SPH2D_OK) e = hipEventRecord(b->join[i], c->stream);
+    if (e == hipSuccess && rc == SPH2D_OK) e = hipStreamWaitEvent(b->stream, b->join[i], 0);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+  if (rc != SPH2D_OK || e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (rc != SPH2D_OK) return rc;
+    return fail(SPH2D_ERR_HIP, std::string("batch capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  b->graph = g;
+  HIP_OR_FAIL(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  HIP_OR_FAIL(hipGraphGetNodes(b->graph, nullptr, &b->nodes));
+  return SPH2D_OK;
+}
+}  // namespace
+
+int sph2d_batch_create(int32_t n, sph2d_ctx* const* ctxs, sph2d_batch** out) {
+  if (!out || n <= 0 || !ctxs) return fail(SPH2D_ERR_ARGUMENT, "n must be positive, ctxs and out non-NULL");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    const sph2d_ctx* c = ctxs[i];
+    if (!c) return fail(SPH2D_ERR_ARGUMENT, "a context is NULL");
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == c) return fail(SPH2D_ERR_ARGUMENT, "a context appears twice");
+    const sph2d_params &a = c->prm, &z = ctxs[0]->prm;
+    if (a.integrator != z.integrator || a.mls_on != z.mls_on || a.mls_every != z.mls_every ||
+        a.sample_every != z.sample_every)
+      return fail(SPH2D_ERR_ARGUMENT, "batched contexts must share integrator, mls_on, mls_every and sample_every");
+  }
+  auto* b = new sph2d_batch;
+  b->ctx.assign(ctxs, ctxs + n);
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  b->join.assign(n, nullptr);
+  b->sync.assign(n, nullptr);
+  for (int i = 0; i < n && e == hipSuccess; ++i) {
+    e = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->sync[i], hipEventDisableTiming);
+  }
+  if (e != hipSuccess) {
+    sph2d_batch_destroy(b);
+    return fail(SPH2D_ERR_HIP, std::string("batch create: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  return SPH2D_OK;
+}
+
+int sph2d_batch_run(sph2d_batch* b, int32_t first, int32_t nsteps) {
+  if (!b) return fail(SPH2D_ERR_ARGUMENT, "batch is NULL");
+  if (first < 1 || nsteps < 0) return fail(SPH2D_ERR_ARGUMENT, "bad step range");
+  for (sph2d_ctx* c : b->ctx) {
+    const int rc = need_state(c);
+    if (rc) return rc;
+  }
+  int rc = SPH2D_OK;
+  for (int k = 0; k < nsteps && rc == SPH2D_OK; ++k) {
+    const int it = first + k;
+    bool graph = true;
+    for (const sph2d_ctx* c : b->ctx) graph = graph && replayable(c, it);
+    if (graph) {
+      if (!b->exec) rc = batch_capture(b, it);
+      if (rc == SPH2D_OK && !b->on_batch) rc = batch_join(b);
+      if (rc == SPH2D_OK) {
+        const hipError_t e = hipGraphLaunch(b->exec, b->stream);
+        if (e != hipSuccess) rc = fail(SPH2D_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+        b->on_batch = true;
+        ++b->replays;
+      }
+    } else {
+      if (b->on_batch) {
+        rc = batch_release(b);
+        b->on_batch = false;
+      }
+      for (size_t i = 0; i < b->ctx.size() && rc == SPH2D_OK; ++i) rc = enqueue_step(b->ctx[i], it);
+    }
+  }
+  if (b->on_batch) {
+    const int r2 = batch_release(b);
+    b->on_batch = false;
+    if (rc == SPH2D_OK) rc = r2;
+  }
+  return rc;
+}
+
+int sph2d_batch_info(sph2d_batch* b, int32_t* graph_nodes, int32_t* graph_replays) {
+  if (!b) return fail(SPH2D_ERR_ARGUMENT, "batch is NULL");
+  if (graph_nodes) *graph_nodes = (int32_t)b->nodes;
+  if (graph_replays) *graph_replays = b->replays;
+  return SPH2D_OK;
+}
+
+int sph2d_batch_destroy(sph2d_batch* b) {
+  if (!b) return SPH2D_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (hipEvent_t e : b->join) if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : b->sync) if (e) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return SPH2D_OK;
+}

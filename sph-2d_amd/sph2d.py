@@ -122,7 +122,8 @@ EXPORTS = (
     "sph2d_write_grid_binary", "sph2d_sampling", "sph2d_wave_probes", "sph2d_output_sampling_lines",
     "sph2d_format_list_directed", "sph2d_write_list_directed", "sph2d_write_f20_8", "sph2d_kalman_filter",
     "sph2d_sampling_line_points", "sph2d_write_sampling_line_positions", "sph2d_write_sampling_line_results",
-    "sph2d_wavemaker_motion", "sph2d_wavemaker_pull_back",
+    "sph2d_wavemaker_motion", "sph2d_wavemaker_pull_back", "sph2d_batch_create", "sph2d_batch_run",
+    "sph2d_batch_info", "sph2d_batch_destroy",
 )
 
 _lib = None
@@ -203,6 +204,10 @@ def lib():
                                                   P(Scales)],
             "sph2d_wavemaker_motion": [P(Wavemaker), i32, dp, dp, dp],
             "sph2d_wavemaker_pull_back": [vp, ctypes.c_double, ctypes.c_double],
+            "sph2d_batch_create": [i32, P(vp), P(vp)],
+            "sph2d_batch_run": [vp, i32, i32],
+            "sph2d_batch_info": [vp, ip, ip],
+            "sph2d_batch_destroy": [vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -759,3 +764,34 @@ class SamplingProbes:
             write_f20_8(base + "Normalized_Pressure.dat", [t * sc.time_0, r["mls_p"][m] / sc.pressure_0])
             write_f20_8(base + "Average_Pressure.dat", [t * sc.time_0, r["average_p"][m] / sc.pressure_0])
             write_f20_8(base + "Kalman_Pressure.dat", [t * sc.time_0, kal[m] / sc.pressure_0])
+
+
+class Batch:
+    """Independent simulations advanced together (sph2d_batch): replayable steps run
+    as one hipGraph launch holding every instance's step."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        arr = (ctypes.c_void_p * len(self.sims))(*[s._c for s in self.sims])
+        h = ctypes.c_void_p()
+        _check(lib().sph2d_batch_create(len(self.sims), arr, ctypes.byref(h)))
+        self._b = h
+
+    def run(self, first_step: int, nsteps: int):
+        _check(lib().sph2d_batch_run(self._b, first_step, nsteps))
+
+    def info(self):
+        nodes, reps = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().sph2d_batch_info(self._b, ctypes.byref(nodes), ctypes.byref(reps)))
+        return nodes.value, reps.value
+
+    def close(self):
+        if self._b:
+            lib().sph2d_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
