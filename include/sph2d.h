@@ -249,8 +249,10 @@ int sph2d_output_sampling_lines(sph2d_ctx* ctx, int32_t i_time_step, const char*
 
 /* --- batched form (no reference counterpart) ------------------------------------- */
 /* Independent contexts (uneven sizes, any Re) advanced together. Steps after step 1
- * that are neither MLS nor sampling steps replay one hipGraph holding every
- * instance's step as a parallel branch; the other steps are enqueued per context.
+ * that are neither MLS nor sampling steps replay one hipGraph: when the contexts
+ * share the predictor-corrector, precision form, solver and shifting switch (no
+ * XSPH, no damping) it holds one launch per kernel covering every instance, else
+ * every instance's step as a parallel branch; the other steps are enqueued per context.
  * Each instance's results are bit-identical to running it alone. The contexts must
  * share integrator, mls_on, mls_every and sample_every, outlive the batch and not
  * be used by other calls while sph2d_batch_run runs; on return, later calls on a

@@ -1420,6 +1420,10 @@ struct sph2d_batch {
   size_t nodes = 0;
   int replays = 0;
   bool on_batch = false;                       // the latest work of the batch is on `stream`
+  // one launch per kernel over all instances (when their forms allow it)
+  bool fused = false;
+  BatchDev dev;
+  std::vector<void*> dev_allocs;
 };
 
 namespace {
@@ -1445,11 +1449,78 @@ int batch_release(sph2d_batch* b) {
   return SPH2D_OK;
 }
 
+// The instances can share one launch per kernel when they run the same kernels with
+// the same template arguments: predictor-corrector, one form, one solver, one
+// shifting switch, no XSPH, no end damping.
+bool fusable(const std::vector<sph2d_ctx*>& v) {
+  const sph2d_params& z = v[0]->prm;
+  for (const sph2d_ctx* c : v) {
+    const sph2d_params& a = c->prm;
+    if (a.integrator != SPH2D_INTEGRATOR_PC || a.precision != z.precision || a.solver != z.solver ||
+        a.shift_on != z.shift_on || a.epsilon_xsph != 0.0 || a.damping != SPH2D_DAMPING_OFF)
+      return false;
+  }
+  return true;
+}
+
+int batch_build_dev(sph2d_batch* b) {
+  const int n = (int)b->ctx.size();
+  std::vector<Inst> inst(n);
+  for (int i = 0; i < n; ++i) {
+    const sph2d_ctx* c = b->ctx[i];
+    inst[i].P = c->P;
+    inst[i].P.write_debug = 0;
+    inst[i].B = c->B;
+    inst[i].cap_fluid = c->cap_fluid;
+    inst[i].cap_ghost = c->cap_ghost;
+    inst[i].cap_global = c->cap_global;
+    inst[i].cap_local = c->cap_local;
+    inst[i].n_cells = c->n_cells;
+  }
+  std::vector<int> starts((size_t)BC_COUNT * (n + 1));
+  for (int cls = 0; cls < BC_COUNT; ++cls) {
+    int acc = 0;
+    for (int i = 0; i < n; ++i) {
+      starts[(size_t)cls * (n + 1) + i] = acc;
+      acc += batch_blocks(cls, inst[i]);
+    }
+    starts[(size_t)cls * (n + 1) + n] = acc;
+    b->dev.total[cls] = acc;
+  }
+  std::vector<unsigned char> q((size_t)n * q32_bytes());
+  for (int i = 0; i < n; ++i) q32_fill(inst[i].P, q.data() + (size_t)i * q32_bytes());
+  void *di = nullptr, *ds = nullptr, *dq = nullptr;
+  HIP_OR_FAIL(hipMalloc(&di, sizeof(Inst) * n));
+  b->dev_allocs.push_back(di);
+  HIP_OR_FAIL(hipMalloc(&ds, sizeof(int) * starts.size()));
+  b->dev_allocs.push_back(ds);
+  HIP_OR_FAIL(hipMalloc(&dq, q.size()));
+  b->dev_allocs.push_back(dq);
+  HIP_OR_FAIL(hipMemcpy(di, inst.data(), sizeof(Inst) * n, hipMemcpyHostToDevice));
+  HIP_OR_FAIL(hipMemcpy(ds, starts.data(), sizeof(int) * starts.size(), hipMemcpyHostToDevice));
+  HIP_OR_FAIL(hipMemcpy(dq, q.data(), q.size(), hipMemcpyHostToDevice));
+  b->dev.inst = static_cast<const Inst*>(di);
+  b->dev.n = n;
+  for (int cls = 0; cls < BC_COUNT; ++cls) b->dev.start[cls] = static_cast<const int*>(ds) + (size_t)cls * (n + 1);
+  b->dev.q32 = dq;
+  return SPH2D_OK;
+}
+
 int batch_capture(sph2d_batch* b, int it) {
   HIP_OR_FAIL(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
   int rc = SPH2D_OK;
-  hipError_t e = hipEventRecord(b->fork, b->stream);
-  for (size_t i = 0; i < b->ctx.size() && e == hipSuccess && rc == SPH2D_OK; ++i) {
+  hipError_t e = hipSuccess;
+  if (b->fused) {
+    // the step of every instance, one launch per kernel (it > 1, not MLS, not sampling)
+    const sph2d_params& z = b->ctx[0]->prm;
+    const int mixed = z.precision == SPH2D_FP32_MIXED;
+    launch_batch_step(b->stream, b->dev, mixed, mixed, z.shift_on, z.solver);
+    e = hipGetLastError();
+    for (sph2d_ctx* c : b->ctx) c->evaluated = true;
+  } else {
+    e = hipEventRecord(b->fork, b->stream);
+  }
+  for (size_t i = 0; !b->fused && i < b->ctx.size() && e == hipSuccess && rc == SPH2D_OK; ++i) {
     sph2d_ctx* c = b->ctx[i];
     e = hipStreamWaitEvent(c->stream, b->fork, 0);
     if (e == hipSuccess) rc = enqueue_step(c, it);
@@ -1497,6 +1568,14 @@ int sph2d_batch_create(int32_t n, sph2d_ctx* const* ctxs, sph2d_batch** out) {
     sph2d_batch_destroy(b);
     return fail(SPH2D_ERR_HIP, std::string("batch create: ") + hipGetErrorString(e));
   }
+  b->fused = fusable(b->ctx);
+  if (b->fused) {
+    const int rc = batch_build_dev(b);
+    if (rc) {
+      sph2d_batch_destroy(b);
+      return rc;
+    }
+  }
   *out = b;
   return SPH2D_OK;
 }
@@ -1540,6 +1619,8 @@ int sph2d_batch_run(sph2d_batch* b, int32_t first, int32_t nsteps) {
 
 int sph2d_batch_info(sph2d_batch* b, int32_t* graph_nodes, int32_t* graph_replays) {
   if (!b) return fail(SPH2D_ERR_ARGUMENT, "batch is NULL");
+  // (graph_nodes: one per kernel launch of a step in the fused form, one per launch
+  // per instance otherwise)
   if (graph_nodes) *graph_nodes = (int32_t)b->nodes;
   if (graph_replays) *graph_replays = b->replays;
   return SPH2D_OK;
@@ -1554,6 +1635,7 @@ int sph2d_batch_destroy(sph2d_batch* b) {
   for (hipEvent_t e : b->sync) if (e) (void)hipEventDestroy(e);
   if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  for (void* p : b->dev_allocs) (void)hipFree(p);
   delete b;
   return SPH2D_OK;
 }

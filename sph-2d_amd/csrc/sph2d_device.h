@@ -68,6 +68,20 @@ __device__ __forceinline__ double2 buf_d2(__amdgpu_buffer_rsrc_t r, int idx) {
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD b % 8),
 // each XCD with its own L2. The logical block of workgroup b gives every XCD one
 // contiguous run of blocks, so blocks whose neighbour columns overlap share an L2.
+// The instance of batched block b and its block index within the instance.
+struct BatchWhere {
+  int i, blk, nblk;
+};
+__device__ __forceinline__ BatchWhere batch_where(const BatchArgs& A, int b) {
+  int lo = 0, hi = A.n - 1;
+  while (lo < hi) {                                  // last instance whose first block <= b
+    const int mid = (lo + hi + 1) >> 1;
+    if (A.start[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return BatchWhere{lo, b - A.start[lo], A.start[lo + 1] - A.start[lo]};
+}
+
 __device__ __forceinline__ int xcd_block() {
   const int nb = gridDim.x, b = blockIdx.x;
   const int q = nb >> 3, r = nb & 7;

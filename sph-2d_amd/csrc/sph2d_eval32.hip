@@ -11,6 +11,7 @@
 // x_s - x_o comes from FP32 offsets to each particle's cell corner plus the
 // whole-cell difference, which keeps it to ~1e-7 h; a neighbour costs 48 bytes
 // (three float4 records) there, 16 in pass A.
+#include <cstring>
 #include "sph2d_internal.h"
 #include "sph2d_device.h"
 #include "sph2d_riemann.h"
@@ -93,13 +94,13 @@ __device__ __forceinline__ KernelTerm32 kernel_term32(const DevParams32& Q, floa
 
 // pass A (mixed): see k_eval_a in sph2d_eval64.hip for the reference lines.
 template <bool MUSCL>
-__global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q, Buffers B) {
+__device__ __forceinline__ void k_eval_a32_body(const DevParams& P, const DevParams32& Q, Buffers& B, int blk_, int nblk_) {
   __shared__ NbStage st;
   __shared__ float2 lds32[kStageCap];
   __shared__ double2 lds64[kStageCap];
-  const int s = xcd_block() * blockDim.x + threadIdx.x;
+  const int s = blk_ * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
-  stage_candidates(P, B.cell_start, B.l_cell, B.l_pos32, B.l_pos, n, st, lds32, lds64, xcd_block());
+  stage_candidates(P, B.cell_start, B.l_cell, B.l_pos32, B.l_pos, n, st, lds32, lds64, blk_);
   if (s >= n) return;
   const double2 ps = B.l_pos[s];
   const float4 r0s = B.m_r0[s];
@@ -182,6 +183,11 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
   if (sf) atomicAdd(&B.counters[C_NSURF], 1);
 }
 
+template <bool MUSCL>
+__global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q, Buffers B) {
+  k_eval_a32_body<MUSCL>(P, Q, B, xcd_block(), gridDim.x);
+}
+
 // pass C (mixed): see k_eval_c in sph2d_eval64.hip for the reference lines.
 // A block whose candidate union fits kRecCap copies the neighbour records r0 and r2
 // to LDS once and reads them there; r1 (tau, read only by the SPS term) comes from
@@ -190,10 +196,10 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
 constexpr int kRecCap = 1088;
 
 template <int SOLVER>
-__global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q, Buffers B, int do_shift) {
+__device__ __forceinline__ void k_eval_c32_body(const DevParams& P, const DevParams32& Q, Buffers& B, int do_shift, int blk_, int nblk_) {
   __shared__ NbStage st;
   __shared__ float4 rec0[kRecCap], rec2[kRecCap];
-  const int blk = xcd_block();
+  const int blk = blk_;
   const int s = blk * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
   plan_stage(P, B.cell_start, B.l_cell, n, kRecCap, st, blk);
@@ -430,6 +436,11 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
   if (do_shift) B.shift[s] = make_double2((double)shx.s, (double)shy.s);
 }
 
+template <int SOLVER>
+__global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q, Buffers B, int do_shift) {
+  k_eval_c32_body<SOLVER>(P, Q, B, do_shift, xcd_block(), gridDim.x);
+}
+
 void launch_eval32_a(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
   const int g = grid_for(cap_local);
   if (P.solver == SPH2D_SOLVER_MUSCL)
@@ -453,6 +464,55 @@ void launch_eval32_c(hipStream_t s, const DevParams& P, Buffers& B, int cap_loca
       break;
     default:
       hipLaunchKernelGGL(k_eval_c32<SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, s, P, Q, B, 1);
+  }
+}
+
+size_t q32_bytes() { return sizeof(DevParams32); }
+void q32_fill(const DevParams& P, void* out) {
+  const DevParams32 Q = make32(P);
+  std::memcpy(out, &Q, sizeof Q);
+}
+
+// Batched mixed passes A and C (see launch_batch_step).
+template <bool MUSCL>
+__global__ void __launch_bounds__(kBlock) kb_eval_a32(BatchArgs A, const DevParams32* Qs) {
+  const BatchWhere w = batch_where(A, xcd_block());
+  const DevParams P = A.inst[w.i].P;
+  const DevParams32 Q = Qs[w.i];
+  Buffers B = A.inst[w.i].B;
+  k_eval_a32_body<MUSCL>(P, Q, B, w.blk, w.nblk);
+}
+
+template <int SOLVER>
+__global__ void __launch_bounds__(kBlock) kb_eval_c32(BatchArgs A, const DevParams32* Qs) {
+  const BatchWhere w = batch_where(A, xcd_block());
+  const DevParams P = A.inst[w.i].P;
+  const DevParams32 Q = Qs[w.i];
+  Buffers B = A.inst[w.i].B;
+  k_eval_c32_body<SOLVER>(P, Q, B, 1, w.blk, w.nblk);
+}
+
+void launch_batch_eval32(hipStream_t s, const BatchDev& D, int pass, int solver) {
+  const int g = D.total[BC_LOCAL];
+  const BatchArgs A{D.inst, D.start[BC_LOCAL], D.n};
+  const auto* Qs = static_cast<const DevParams32*>(D.q32);
+  if (pass == 0) {
+    if (solver == SPH2D_SOLVER_MUSCL) hipLaunchKernelGGL(kb_eval_a32<true>, dim3(g), dim3(kBlock), 0, s, A, Qs);
+    else hipLaunchKernelGGL(kb_eval_a32<false>, dim3(g), dim3(kBlock), 0, s, A, Qs);
+    return;
+  }
+  switch (solver) {
+    case SPH2D_SOLVER_LINEAR_RIEMANN:
+      hipLaunchKernelGGL(kb_eval_c32<SPH2D_SOLVER_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, s, A, Qs);
+      break;
+    case SPH2D_SOLVER_HLLC:
+      hipLaunchKernelGGL(kb_eval_c32<SPH2D_SOLVER_HLLC>, dim3(g), dim3(kBlock), 0, s, A, Qs);
+      break;
+    case SPH2D_SOLVER_MUSCL:
+      hipLaunchKernelGGL(kb_eval_c32<SPH2D_SOLVER_MUSCL>, dim3(g), dim3(kBlock), 0, s, A, Qs);
+      break;
+    default:
+      hipLaunchKernelGGL(kb_eval_c32<SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, s, A, Qs);
   }
 }
 

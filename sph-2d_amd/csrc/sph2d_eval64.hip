@@ -45,13 +45,13 @@ __device__ __forceinline__ KernelTerm kernel_term(const DevParams& P, double dx,
 // (Solver_Normalized_MUSCL_Linear_Riemann_Delta_SPS.f90:145-185).
 // ---------------------------------------------------------------------------
 template <bool MUSCL>
-__global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
+__device__ __forceinline__ void k_eval_a_body(const DevParams& P, Buffers& B, int blk_, int nblk_) {
   __shared__ NbStage st;
   __shared__ float2 lds32[kStageCap];
   __shared__ double2 lds64[kStageCap];
-  const int s = xcd_block() * blockDim.x + threadIdx.x;
+  const int s = blk_ * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
-  stage_candidates(P, B.cell_start, B.l_cell, B.l_pos32, B.l_pos, n, st, lds32, lds64, xcd_block());
+  stage_candidates(P, B.cell_start, B.l_cell, B.l_pos32, B.l_pos, n, st, lds32, lds64, blk_);
   if (s >= n) return;
   const double2 ps = B.l_pos[s];
   const double2 vs = B.l_vel[s];
@@ -120,9 +120,14 @@ __global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
   if (sf) atomicAdd(&B.counters[C_NSURF], 1);
 }
 
+template <bool MUSCL>
+__global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
+  k_eval_a_body<MUSCL>(P, B, xcd_block(), gridDim.x);
+}
+
 // Surface normal and geometric test (Define_freesurface...f90:295-376), surface particles only.
-__global__ void __launch_bounds__(kBlock) k_eval_b1(DevParams P, Buffers B) {
-  const int s = xcd_block() * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_eval_b1_body(const DevParams& P, Buffers& B, int blk_, int nblk_) {
+  const int s = blk_ * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
   if (s >= n) return;
   if (B.surf[s] != 1) { B.nrm[s] = make_double2(0.0, 0.0); return; }
@@ -164,9 +169,13 @@ __global__ void __launch_bounds__(kBlock) k_eval_b1(DevParams P, Buffers B) {
   }
 }
 
+__global__ void __launch_bounds__(kBlock) k_eval_b1(DevParams P, Buffers B) {
+  k_eval_b1_body(P, B, xcd_block(), gridDim.x);
+}
+
 // Near-surface flag (Define_freesurface...f90:418-440): any pair partner is a surface particle.
-__global__ void __launch_bounds__(kBlock) k_eval_b2(DevParams P, Buffers B) {
-  const int s = xcd_block() * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_eval_b2_body(const DevParams& P, Buffers& B, int blk_, int nblk_) {
+  const int s = blk_ * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
   if (s >= n) return;
   if (B.counters[C_NSURF] == 0) return;            // near_ was zeroed by pass A
@@ -176,6 +185,10 @@ __global__ void __launch_bounds__(kBlock) k_eval_b2(DevParams P, Buffers B) {
     if (B.surf[o] == 1 && (ts == 2 || B.l_type[o] == 2)) nr = 1;
   }, B.nb_mask, B.kstride);
   B.near_[s] = nr;
+}
+
+__global__ void __launch_bounds__(kBlock) k_eval_b2(DevParams P, Buffers B) {
+  k_eval_b2_body(P, B, xcd_block(), gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -190,8 +203,8 @@ __global__ void __launch_bounds__(kBlock) k_eval_b2(DevParams P, Buffers B) {
 // SOLVER is one of SPH2D_SOLVER_*: each variant is its own kernel.
 // ---------------------------------------------------------------------------
 template <bool DEBUG, int SOLVER>
-__device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int do_shift) {
-  const int s = xcd_block() * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int do_shift, int blk_) {
+  const int s = blk_ * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
   if (s >= n) return;
   const double2 ps = B.l_pos[s];
@@ -363,7 +376,7 @@ __device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int
 // VGPRs) spills the force pass to scratch and costs more than it gains.
 template <bool DEBUG, int SOLVER>
 __global__ void __launch_bounds__(kBlock) k_eval_c(DevParams P, Buffers B, int do_shift) {
-  eval_c<DEBUG, SOLVER>(P, B, do_shift);
+  eval_c<DEBUG, SOLVER>(P, B, do_shift, xcd_block());
 }
 
 template <bool DEBUG>
@@ -447,6 +460,66 @@ void launch_eval_pass(hipStream_t s, const DevParams& P, Buffers& B, int cap_loc
     else launch_a64(s, P, B, g);
   } else {
     launch_eval(s, P, B, cap_local);
+  }
+}
+
+// Batched evaluation passes (see launch_batch_step): XCD-aware block order over
+// the whole launch, then the instance of the block.
+template <bool MUSCL>
+__global__ void __launch_bounds__(kBlock) kb_eval_a(BatchArgs A) {
+  const BatchWhere w = batch_where(A, xcd_block());
+  const DevParams P = A.inst[w.i].P;
+  Buffers B = A.inst[w.i].B;
+  k_eval_a_body<MUSCL>(P, B, w.blk, w.nblk);
+}
+
+__global__ void __launch_bounds__(kBlock) kb_eval_b1(BatchArgs A) {
+  const BatchWhere w = batch_where(A, xcd_block());
+  const DevParams P = A.inst[w.i].P;
+  Buffers B = A.inst[w.i].B;
+  k_eval_b1_body(P, B, w.blk, w.nblk);
+}
+
+__global__ void __launch_bounds__(kBlock) kb_eval_b2(BatchArgs A) {
+  const BatchWhere w = batch_where(A, xcd_block());
+  const DevParams P = A.inst[w.i].P;
+  Buffers B = A.inst[w.i].B;
+  k_eval_b2_body(P, B, w.blk, w.nblk);
+}
+
+template <int SOLVER>
+__global__ void __launch_bounds__(kBlock) kb_eval_c(BatchArgs A) {
+  const BatchWhere w = batch_where(A, xcd_block());
+  const DevParams P = A.inst[w.i].P;
+  const Buffers B = A.inst[w.i].B;
+  eval_c<false, SOLVER>(P, B, 1, w.blk);
+}
+
+void launch_batch_eval(hipStream_t s, const BatchDev& D, int mixed, int solver) {
+  const int g = D.total[BC_LOCAL];
+  if (g <= 0) return;
+  const BatchArgs A{D.inst, D.start[BC_LOCAL], D.n};
+  if (mixed) launch_batch_eval32(s, D, 0, solver);
+  else if (solver == SPH2D_SOLVER_MUSCL) hipLaunchKernelGGL(kb_eval_a<true>, dim3(g), dim3(kBlock), 0, s, A);
+  else hipLaunchKernelGGL(kb_eval_a<false>, dim3(g), dim3(kBlock), 0, s, A);
+  hipLaunchKernelGGL(kb_eval_b1, dim3(g), dim3(kBlock), 0, s, A);
+  hipLaunchKernelGGL(kb_eval_b2, dim3(g), dim3(kBlock), 0, s, A);
+  if (mixed) {
+    launch_batch_eval32(s, D, 1, solver);
+    return;
+  }
+  switch (solver) {
+    case SPH2D_SOLVER_LINEAR_RIEMANN:
+      hipLaunchKernelGGL(kb_eval_c<SPH2D_SOLVER_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, s, A);
+      break;
+    case SPH2D_SOLVER_HLLC:
+      hipLaunchKernelGGL(kb_eval_c<SPH2D_SOLVER_HLLC>, dim3(g), dim3(kBlock), 0, s, A);
+      break;
+    case SPH2D_SOLVER_MUSCL:
+      hipLaunchKernelGGL(kb_eval_c<SPH2D_SOLVER_MUSCL>, dim3(g), dim3(kBlock), 0, s, A);
+      break;
+    default:
+      hipLaunchKernelGGL(kb_eval_c<SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, s, A);
   }
 }
 

@@ -9,6 +9,7 @@
 // Built with -ffp-contract=off (no FMA contraction), like the reference build.
 #include "sph2d_internal.h"
 #include "sph2d_device.h"
+#include <algorithm>
 #include <cstdio>
 
 namespace sph2d {
@@ -51,20 +52,24 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int& total)
   return incl - v;
 }
 
-__global__ void k_scan_reduce(const int* __restrict__ in, int n, int* __restrict__ blocks) {
+__device__ __forceinline__ void k_scan_reduce_body(const int* __restrict__ in, int n, int* __restrict__ blocks, int blk_, int nblk_) {
   __shared__ int lds[kBlock];
-  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  const int base = blk_ * kScanTile + threadIdx.x * kScanItems;
   int s = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) s += (base + k < n) ? in[base + k] : 0;
   int total;
   block_exclusive_scan(s, lds, total);
-  if (threadIdx.x == 0) blocks[blockIdx.x] = total;
+  if (threadIdx.x == 0) blocks[blk_] = total;
+}
+
+__global__ void k_scan_reduce(const int* __restrict__ in, int n, int* __restrict__ blocks) {
+  k_scan_reduce_body(in, n, blocks, blockIdx.x, gridDim.x);
 }
 
 // One block: each thread scans a contiguous run of the block totals serially,
 // then one block-wide scan of the run totals gives the runs' offsets.
-__global__ void k_scan_top(int* blocks, int nb, int* total_out, int* total_out2) {
+__device__ __forceinline__ void k_scan_top_body(int* blocks, int nb, int* total_out, int* total_out2, int blk_, int nblk_) {
   __shared__ int lds[kBlock];
   const int per = (nb + kBlock - 1) / kBlock;
   const int b0 = threadIdx.x * per;
@@ -84,21 +89,29 @@ __global__ void k_scan_top(int* blocks, int nb, int* total_out, int* total_out2)
   }
 }
 
-__global__ void k_scan_apply(const int* __restrict__ in, int n, const int* __restrict__ blocks,
-                             int* __restrict__ out) {
+__global__ void k_scan_top(int* blocks, int nb, int* total_out, int* total_out2) {
+  k_scan_top_body(blocks, nb, total_out, total_out2, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void k_scan_apply_body(const int* __restrict__ in, int n, const int* __restrict__ blocks, int* __restrict__ out, int blk_, int nblk_) {
   __shared__ int lds[kBlock];
-  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  const int base = blk_ * kScanTile + threadIdx.x * kScanItems;
   int v[kScanItems];
   int s = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) { v[k] = (base + k < n) ? in[base + k] : 0; s += v[k]; }
   int total;
-  int ex = block_exclusive_scan(s, lds, total) + blocks[blockIdx.x];
+  int ex = block_exclusive_scan(s, lds, total) + blocks[blk_];
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     if (base + k < n) out[base + k] = ex;
     ex += v[k];
   }
+}
+
+__global__ void k_scan_apply(const int* __restrict__ in, int n, const int* __restrict__ blocks,
+                             int* __restrict__ out) {
+  k_scan_apply_body(in, n, blocks, out, blockIdx.x, gridDim.x);
 }
 
 // out[i] = sum_{k<i} in[k] for i < n; the grand total goes to *total (and *total2).
@@ -155,9 +168,9 @@ void launch_init_tgv(hipStream_t s, const DevParams& P, Buffers& B, int nx, int 
 // scans a[0..nb), block 1 a[nb..2nb)); totals to *t0 and *t1. Each of the 1024
 // threads first sums a contiguous run (independent loads), then one block scan of
 // the run totals gives the runs' offsets.
-__global__ void __launch_bounds__(1024) k_scan_top2(int* a, int nb, int* t0, int* t1) {
+__device__ __forceinline__ void k_scan_top2_body(int* a, int nb, int* t0, int* t1, int blk_, int nblk_) {
   __shared__ int lds[1024];
-  int* arr = a + (size_t)blockIdx.x * nb;
+  int* arr = a + (size_t)blk_ * nb;
   const int t = threadIdx.x;
   const int per = (nb + 1023) / 1024;
   const int b0 = min(t * per, nb), b1 = min(b0 + per, nb);
@@ -188,7 +201,11 @@ __global__ void __launch_bounds__(1024) k_scan_top2(int* a, int nb, int* t0, int
       ex += v[k];
     }
   }
-  if (t == 1023) *(blockIdx.x ? t1 : t0) = lds[1023];
+  if (t == 1023) *(blk_ ? t1 : t0) = lds[1023];
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top2(int* a, int nb, int* t0, int* t1) {
+  k_scan_top2_body(a, nb, t0, t1, blockIdx.x, gridDim.x);
 }
 
 // The flags of one wrapped fluid position: x-ghost to the right of the box (l),
@@ -204,9 +221,9 @@ __device__ __forceinline__ void ghost_flags(const DevParams& P, double2 p, int& 
 // :56-90 wrap (written back only where it moves a particle), then per block of
 // kBlock gids the number of x-ghosts (gblk[b]) and of y-ghosts of fluid
 // particles (gblk[nb + b]).
-__global__ void __launch_bounds__(kBlock) k_wrap_count(DevParams P, Buffers B, int cap_fluid) {
+__device__ __forceinline__ void k_wrap_count_body(const DevParams& P, Buffers& B, int cap_fluid, int blk_, int nblk_) {
   __shared__ int sx[kBlock / 64], sy[kBlock / 64];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blk_ * blockDim.x + threadIdx.x;
   const int N = B.counters[C_NFLUID];
   int l = 0, r = 0, yl = 0, yr = 0;
   if (i < N && i < cap_fluid && B.g_type[i] != 120) {
@@ -228,9 +245,13 @@ __global__ void __launch_bounds__(kBlock) k_wrap_count(DevParams P, Buffers B, i
     int tx = 0, ty = 0;
 #pragma unroll
     for (int k = 0; k < kBlock / 64; ++k) { tx += sx[k]; ty += sy[k]; }
-    B.gblk[blockIdx.x] = tx;
-    B.gblk[gridDim.x + blockIdx.x] = ty;
+    B.gblk[blk_] = tx;
+    B.gblk[nblk_ + blk_] = ty;
   }
+}
+
+__global__ void __launch_bounds__(kBlock) k_wrap_count(DevParams P, Buffers B, int cap_fluid) {
+  k_wrap_count_body(P, B, cap_fluid, blockIdx.x, gridDim.x);
 }
 
 __device__ __forceinline__ void copy_particle(Buffers& B, int dst, int src, double2 pos) {
@@ -249,9 +270,9 @@ __device__ __forceinline__ void copy_particle(Buffers& B, int dst, int src, doub
 // order: a block's slots start at its scanned count (gblk) and each lane's
 // after the ghosts of the lanes before it. Blocks without ghosts leave at once.
 // Records each x-ghost's own y-ghost count for the corner pass.
-__global__ void __launch_bounds__(kBlock) k_ghost_emit(DevParams P, Buffers B, int cap_fluid, int cap_ghost) {
+__device__ __forceinline__ void k_ghost_emit_body(const DevParams& P, Buffers& B, int cap_fluid, int cap_ghost, int blk_, int nblk_) {
   __shared__ int sx[kBlock / 64], sy[kBlock / 64];
-  const int b = blockIdx.x, nb = gridDim.x;
+  const int b = blk_, nb = nblk_;
   const int nxg = B.counters[C_NXG], nyf = B.counters[C_NYF];
   const int ox = B.gblk[b], oy = B.gblk[nb + b];
   const int ex = (b + 1 < nb ? B.gblk[b + 1] : nxg) - ox;
@@ -305,9 +326,13 @@ __global__ void __launch_bounds__(kBlock) k_ghost_emit(DevParams P, Buffers B, i
   }
 }
 
+__global__ void __launch_bounds__(kBlock) k_ghost_emit(DevParams P, Buffers B, int cap_fluid, int cap_ghost) {
+  k_ghost_emit_body(P, B, cap_fluid, cap_ghost, blockIdx.x, gridDim.x);
+}
+
 // y-direction ghosts of the x-ghosts (corners), in x-ghost order; sets the ghost count.
-__global__ void k_ghost_yx(DevParams P, Buffers B, int cap_ghost) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_ghost_yx_body(const DevParams& P, Buffers& B, int cap_ghost, int blk_, int nblk_) {
+  int g = blk_ * blockDim.x + threadIdx.x;
   const int N = B.counters[C_NFLUID];
   const int nxg = B.counters[C_NXG], nyf = B.counters[C_NYF], nyx = B.counters[C_NYX];
   if (g == 0) {
@@ -329,6 +354,10 @@ __global__ void k_ghost_yx(DevParams P, Buffers B, int cap_ghost) {
     if (k < cap_ghost) copy_particle(B, N + k, src, make_double2(p.x, p.y - P.box_y));
     else atomicOr(&B.counters[C_ERR], E_GHOST_CAP);
   }
+}
+
+__global__ void k_ghost_yx(DevParams P, Buffers B, int cap_ghost) {
+  k_ghost_yx_body(P, B, cap_ghost, blockIdx.x, gridDim.x);
 }
 
 void launch_period_boundary(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int cap_ghost) {
@@ -359,8 +388,8 @@ __global__ void k_analytic(DevParams P, Buffers B, int cap_fluid, double pi, dou
 }
 
 // :75-83 (y<0 -> ill) and :123-162 binning; key = column-major cell index.
-__global__ void k_cell_key(DevParams P, Buffers B, int cap_global) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_cell_key_body(const DevParams& P, Buffers& B, int cap_global, int blk_, int nblk_) {
+  const int g = blk_ * blockDim.x + threadIdx.x;
   const int N = B.counters[C_NFLUID];
   const int ng = min(N + B.counters[C_NGHOST], cap_global);
   const bool valid = g < ng;                       // every lane stays for the wave operations below
@@ -401,14 +430,22 @@ __global__ void k_cell_key(DevParams P, Buffers B, int cap_global) {
   if (key >= 0) B.g_slot[g] = base + (lane - my_lead);
 }
 
-__global__ void k_cell_scatter(Buffers B, int cap_global) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_cell_key(DevParams P, Buffers B, int cap_global) {
+  k_cell_key_body(P, B, cap_global, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void k_cell_scatter_body(Buffers& B, int cap_global, int blk_, int nblk_) {
+  int g = blk_ * blockDim.x + threadIdx.x;
   if (g >= cap_global) return;
   const int ng = B.counters[C_NFLUID] + B.counters[C_NGHOST];
   if (g >= ng) return;
   const int key = B.g_key[g];
   if (key < 0) return;
   B.cell_members[B.cell_start[key] + B.g_slot[g]] = g;
+}
+
+__global__ void k_cell_scatter(Buffers B, int cap_global) {
+  k_cell_scatter_body(B, cap_global, blockIdx.x, gridDim.x);
 }
 
 // Rank inside the cell by gid gives the reference's within-cell order; then the
@@ -446,9 +483,9 @@ __device__ __forceinline__ void gather_one(const DevParams& P, Buffers& B, int g
 // One wavefront per cell ranks the members by gid and writes the local order
 // (name, cell, and the inverse map l_of_g); the state copy runs afterwards with
 // one thread per local slot, so no lanes idle on the wide copy.
-__global__ void __launch_bounds__(kBlock) k_cell_rank(Buffers B, int n_cells) {
+__device__ __forceinline__ void k_cell_rank_body(Buffers& B, int n_cells, int blk_, int nblk_) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int cell = blockIdx.x * (kBlock / 64) + wave;
+  const int cell = blk_ * (kBlock / 64) + wave;
   if (cell >= n_cells) return;
   const int s0 = B.cell_start[cell];
   const int cnt = B.cell_start[cell + 1] - s0;
@@ -469,6 +506,10 @@ __global__ void __launch_bounds__(kBlock) k_cell_rank(Buffers B, int n_cells) {
       B.l_of_g[my] = l;
     }
   }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cell_rank(Buffers B, int n_cells) {
+  k_cell_rank_body(B, n_cells, blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_local_gather(DevParams P, Buffers B) {
@@ -665,8 +706,8 @@ void launch_stage(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, 
 // reads it; k_correct_refresh reads the global arrays instead), the local arrays
 // receive the predicted state.
 // Same arithmetic as k_local_gather followed by k_stage(ST_PC_PREDICT).
-__global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers B, int save_before, int lean) {
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_gather_predict_body(const DevParams& P, Buffers& B, int save_before, int lean, int blk_, int nblk_) {
+  const int l = blk_ * blockDim.x + threadIdx.x;
   if (l >= B.counters[C_NLOCAL]) return;
   const int g = B.name[l], key = B.l_cell[l];
   const double2 pos = B.g_pos[g];
@@ -703,6 +744,10 @@ __global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers 
     const float2 off = cell_offset(P, x, key);
     B.m_r2[l] = make_float4(0.f, 0.f, off.x, off.y);
   }
+}
+
+__global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers B, int save_before, int lean) {
+  k_gather_predict_body(P, B, save_before, lean, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1170,8 +1215,8 @@ void launch_shift_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap
 // k_shift_refresh(mode 0); the local arrays keep the predicted state. The
 // predictor's velocity is still in l_vel (the evaluation does not write it), and
 // the global arrays of fluid gids still hold the step-start state.
-__global__ void k_correct_refresh(DevParams P, Buffers B, int shift_on) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_correct_refresh_body(const DevParams& P, Buffers& B, int shift_on, int blk_, int nblk_) {
+  const int g = blk_ * blockDim.x + threadIdx.x;
   if (g >= B.counters[C_NFLUID]) return;
   const int s = B.l_of_g[g];
   if (s < 0) return;
@@ -1209,6 +1254,10 @@ __global__ void k_correct_refresh(DevParams P, Buffers B, int shift_on) {
   B.g_rho[g] = rho;
   B.g_p[g] = pc.x;
   B.g_c[g] = pc.y;
+}
+
+__global__ void k_correct_refresh(DevParams P, Buffers B, int shift_on) {
+  k_correct_refresh_body(P, B, shift_on, blockIdx.x, gridDim.x);
 }
 
 void launch_correct_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int shift_on) {
@@ -1405,6 +1454,163 @@ __global__ void k_eos_n(int n, double rho0, double c0sq, const double* rho, cons
 void launch_eos_n(hipStream_t s, int n, double rho0, double c0sq, const double* rho,
                   const int* type, double* p, double* c) {
   hipLaunchKernelGGL(k_eos_n, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rho0, c0sq, rho, type, p, c);
+}
+
+// ---------------------------------------------------------------------------
+// Batched forms of the step's kernels: block b of a launch belongs to the instance
+// found by batch_where; each runs the same body with that instance's parameters.
+// ---------------------------------------------------------------------------
+int batch_blocks(int cls, const Inst& I) {
+  switch (cls) {
+    case BC_FLUID: return grid_for(I.cap_fluid);
+    case BC_TWO: return 2;
+    case BC_ONE: return 1;
+    case BC_FILL: return grid_for(std::max(std::max(I.cap_ghost, I.n_cells), 1));
+    case BC_SCAN_G: return scan_block_count(I.cap_ghost);
+    case BC_GHOST: return grid_for(I.cap_ghost);
+    case BC_GLOBAL: return grid_for(I.cap_global);
+    case BC_SCAN_C: return scan_block_count(I.n_cells);
+    case BC_RANK: return (I.n_cells + kBlock / 64 - 1) / (kBlock / 64);
+    case BC_LOCAL: return grid_for(I.cap_local);
+    default: return 0;
+  }
+}
+
+// the zeroing of one step: corner-ghost counts, cell populations, ill and surface counters
+__global__ void kb_step_fill(BatchArgs A) {
+  const BatchWhere w = batch_where(A, blockIdx.x);
+  const Inst& I = A.inst[w.i];
+  const int k = w.blk * blockDim.x + threadIdx.x;
+  if (k < I.cap_ghost) I.B.cnt_yx[k] = 0;
+  if (k < I.n_cells) I.B.cell_count[k] = 0;
+  if (k == 0) {
+    I.B.counters[C_NILL] = 0;
+    I.B.counters[C_NSURF] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) kb_wrap_count(BatchArgs A) {
+  const BatchWhere w = batch_where(A, blockIdx.x);
+  const DevParams P = A.inst[w.i].P;
+  Buffers B = A.inst[w.i].B;
+  k_wrap_count_body(P, B, A.inst[w.i].cap_fluid, w.blk, w.nblk);
+}
+
+__global__ void __launch_bounds__(1024) kb_scan_top2(BatchArgs A) {
+  const BatchWhere w = batch_where(A, blockIdx.x);
+  const Inst& I = A.inst[w.i];
+  k_scan_top2_body(I.B.gblk, (I.cap_fluid + kBlock - 1) / kBlock, I.B.counters + C_NXG, I.B.counters + C_NYF, w.blk, w.nblk);
+}
+
+__global__ void __launch_bounds__(kBlock) kb_ghost_emit(BatchArgs A) {
+  const BatchWhere w = batch_where(A, blockIdx.x);
+  const DevParams P = A.inst[w.i].P;
+  Buffers B = A.inst[w.i].B;
+  k_ghost_emit_body(P, B, A.inst[w.i].cap_fluid, A.inst[w.i].cap_ghost, w.blk, w.nblk);
+}
+
+// which = 0: corner-ghost counts (cnt_yx -> off_yx, total C_NYX); 1: cell populations
+// (cell_count -> cell_start, total C_NLOCAL and cell_start[n_cells]), as in the single path.
+struct ScanArgs {
+  const int* in;
+  int n;
+  int* out;
+  int* blocks;
+  int* total;
+  int* total2;
+};
+__device__ __forceinline__ ScanArgs scan_args(const Inst& I, int which) {
+  if (which == 0) return ScanArgs{I.B.cnt_yx, max(I.cap_ghost, 1), I.B.off_yx, I.B.scan_blocks, I.B.counters + C_NYX,
+                                  nullptr};
+  return ScanArgs{I.B.cell_count, max(I.n_cells, 1), I.B.cell_start, I.B.scan_blocks, I.B.counters + C_NLOCAL,
+                  I.B.cell_start + I.n_cells};
+}
+
+__global__ void kb_scan_reduce(BatchArgs A, int which) {
+  const BatchWhere w = batch_where(A, blockIdx.x);
+  const ScanArgs a = scan_args(A.inst[w.i], which);
+  k_scan_reduce_body(a.in, a.n, a.blocks, w.blk, w.nblk);
+}
+
+__global__ void kb_scan_top(BatchArgs A, int which) {
+  const BatchWhere w = batch_where(A, blockIdx.x);
+  const ScanArgs a = scan_args(A.inst[w.i], which);
+  k_scan_top_body(a.blocks, (a.n + kScanTile - 1) / kScanTile, a.total, a.total2, 0, 1);
+}
+
+__global__ void kb_scan_apply(BatchArgs A, int which) {
+  const BatchWhere w = batch_where(A, blockIdx.x);
+  const ScanArgs a = scan_args(A.inst[w.i], which);
+  k_scan_apply_body(a.in, a.n, a.blocks, a.out, w.blk, w.nblk);
+}
+
+__global__ void kb_ghost_yx(BatchArgs A) {
+  const BatchWhere w = batch_where(A, blockIdx.x);
+  const DevParams P = A.inst[w.i].P;
+  Buffers B = A.inst[w.i].B;
+  k_ghost_yx_body(P, B, A.inst[w.i].cap_ghost, w.blk, w.nblk);
+}
+
+__global__ void kb_cell_key(BatchArgs A) {
+  const BatchWhere w = batch_where(A, blockIdx.x);
+  const DevParams P = A.inst[w.i].P;
+  Buffers B = A.inst[w.i].B;
+  k_cell_key_body(P, B, A.inst[w.i].cap_global, w.blk, w.nblk);
+}
+
+__global__ void kb_cell_scatter(BatchArgs A) {
+  const BatchWhere w = batch_where(A, blockIdx.x);
+  Buffers B = A.inst[w.i].B;
+  k_cell_scatter_body(B, A.inst[w.i].cap_global, w.blk, w.nblk);
+}
+
+__global__ void __launch_bounds__(kBlock) kb_cell_rank(BatchArgs A) {
+  const BatchWhere w = batch_where(A, blockIdx.x);
+  Buffers B = A.inst[w.i].B;
+  k_cell_rank_body(B, A.inst[w.i].n_cells, w.blk, w.nblk);
+}
+
+__global__ void __launch_bounds__(kBlock) kb_gather_predict(BatchArgs A, int lean) {
+  const BatchWhere w = batch_where(A, blockIdx.x);
+  const DevParams P = A.inst[w.i].P;
+  Buffers B = A.inst[w.i].B;
+  k_gather_predict_body(P, B, 0, lean, w.blk, w.nblk);
+}
+
+__global__ void kb_correct_refresh(BatchArgs A, int shift_on) {
+  const BatchWhere w = batch_where(A, blockIdx.x);
+  const DevParams P = A.inst[w.i].P;
+  Buffers B = A.inst[w.i].B;
+  k_correct_refresh_body(P, B, shift_on, w.blk, w.nblk);
+}
+
+static BatchArgs bargs(const BatchDev& D, int cls) { return BatchArgs{D.inst, D.start[cls], D.n}; }
+
+void launch_batch_step(hipStream_t s, const BatchDev& D, int mixed, int lean, int shift_on, int solver) {
+  auto go = [&](int cls, auto kernel, int threads, auto... args) {
+    if (D.total[cls] > 0)
+      hipLaunchKernelGGL(kernel, dim3(D.total[cls]), dim3(threads), 0, s, bargs(D, cls), args...);
+  };
+  go(BC_FILL, kb_step_fill, kBlock);
+  // Distribute_Period_Boundary_Particle
+  go(BC_FLUID, kb_wrap_count, kBlock);
+  go(BC_TWO, kb_scan_top2, 1024);
+  go(BC_FLUID, kb_ghost_emit, kBlock);
+  go(BC_SCAN_G, kb_scan_reduce, kBlock, 0);
+  go(BC_ONE, kb_scan_top, kBlock, 0);
+  go(BC_SCAN_G, kb_scan_apply, kBlock, 0);
+  go(BC_GHOST, kb_ghost_yx, kBlock);
+  // Devide_block_and_send_tasks with the predictor
+  go(BC_GLOBAL, kb_cell_key, kBlock);
+  go(BC_SCAN_C, kb_scan_reduce, kBlock, 1);
+  go(BC_ONE, kb_scan_top, kBlock, 1);
+  go(BC_SCAN_C, kb_scan_apply, kBlock, 1);
+  go(BC_GLOBAL, kb_cell_scatter, kBlock);
+  go(BC_RANK, kb_cell_rank, kBlock);
+  go(BC_GLOBAL, kb_gather_predict, kBlock, lean);
+  // single_step_compute, then the corrector with shifting and the refresh
+  launch_batch_eval(s, D, mixed, solver);
+  go(BC_LOCAL, kb_correct_refresh, kBlock, shift_on);
 }
 
 }  // namespace sph2d

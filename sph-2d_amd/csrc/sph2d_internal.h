@@ -133,6 +133,47 @@ struct GridArgs {
 };
 void launch_grid_mls(hipStream_t s, const DevParams& P, Buffers& B, const GridArgs& G);
 void launch_end_damping(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+
+// ---- batched launches: one launch covers the blocks of many independent instances ----
+struct Inst {
+  DevParams P;
+  Buffers B;
+  int cap_fluid, cap_ghost, cap_global, cap_local, n_cells;
+};
+// Block-count classes of the step's launches (blocks of one instance in each).
+enum BatchClass {
+  BC_FLUID,      // grid_for(cap_fluid): wrap/count, ghost emit
+  BC_TWO,        // 2 (of 1024 threads): the two ghost-count scans
+  BC_ONE,        // 1: scan tops
+  BC_FILL,       // grid_for(max(cap_ghost, n_cells)): the step's zeroing
+  BC_SCAN_G,     // scan tiles over cap_ghost
+  BC_GHOST,      // grid_for(cap_ghost): corner ghosts
+  BC_GLOBAL,     // grid_for(cap_global): keys, scatter, local copy
+  BC_SCAN_C,     // scan tiles over n_cells
+  BC_RANK,       // cells / (kBlock / 64): rank in cell
+  BC_LOCAL,      // grid_for(cap_local): evaluation passes, corrector
+  BC_COUNT
+};
+struct BatchArgs {
+  const Inst* inst;
+  const int* start;                       // n + 1 first blocks of the instances (this class)
+  int n;
+};
+struct BatchDev {
+  const Inst* inst = nullptr;
+  int n = 0;
+  const int* start[BC_COUNT] = {};
+  int total[BC_COUNT] = {};
+  const void* q32 = nullptr;              // n FP32 parameter sets (mixed form)
+};
+int batch_blocks(int cls, const Inst& I);
+size_t q32_bytes();
+void q32_fill(const DevParams& P, void* out);
+// One predictor-corrector step after step 1 that is neither an MLS nor a sampling
+// step, for every instance (same form, solver and shifting switch; no damping, no XSPH).
+void launch_batch_step(hipStream_t s, const BatchDev& D, int mixed, int lean, int shift_on, int solver);
+void launch_batch_eval(hipStream_t s, const BatchDev& D, int mixed, int solver);
+void launch_batch_eval32(hipStream_t s, const BatchDev& D, int pass, int solver);
 void launch_wavemaker_pull_back(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, double xp, double up);
 // Sampling_subroutine.f90 at sampling points (sph2d_fp64.hip)
 struct ProbeArgs {

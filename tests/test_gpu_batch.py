@@ -50,7 +50,7 @@ def test_batch_is_bit_identical_to_each_instance_alone(precision, integrator):
                 assert np.array_equal(ga[f], gb[f]), (done, k, f)
         say(f"step {done}: identical; graph nodes/replays {batch.info()}")
     nodes, replays = batch.info()
-    assert nodes >= 3 * 8 and replays == 23      # steps 2..25 except the MLS step 20
+    assert nodes >= 15 and replays == 23          # steps 2..25 except the MLS step 20
     ke = [s.sample()[0] for s in bat]
     assert all(0.9 < k < 1.0 for k in ke)
     batch.close()
