@@ -1415,8 +1415,8 @@ struct sph2d_batch {
   hipStream_t stream = nullptr;               // graph launches
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> join, sync;          // per context: capture joins, eager/graph ordering
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr, graph_mls = nullptr;
+  hipGraphExec_t exec = nullptr, exec_mls = nullptr;   // exec_mls: MLS steps, fused form only
   size_t nodes = 0;
   int replays = 0;
   bool on_batch = false;                       // the latest work of the batch is on `stream`
@@ -1427,10 +1427,10 @@ struct sph2d_batch {
 };
 
 namespace {
-bool replayable(const sph2d_ctx* c, int it) {
-  if (it <= 1 || c->debug || c->pass_timing) return false;
-  if (c->prm.mls_on && it % c->prm.mls_every == 0) return false;
-  return it % c->prm.sample_every != 0;
+// 0: enqueued per context; 1: a repeating step; 2: an MLS step (replayed in the fused form)
+int step_kind(const sph2d_ctx* c, int it) {
+  if (it <= 1 || c->debug || c->pass_timing || it % c->prm.sample_every == 0) return 0;
+  return (c->prm.mls_on && it % c->prm.mls_every == 0) ? 2 : 1;
 }
 
 // work enqueued on the context streams completes before what follows on the batch stream
@@ -1506,15 +1506,15 @@ int batch_build_dev(sph2d_batch* b) {
   return SPH2D_OK;
 }
 
-int batch_capture(sph2d_batch* b, int it) {
+int batch_capture(sph2d_batch* b, int it, bool mls) {
   HIP_OR_FAIL(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
   int rc = SPH2D_OK;
   hipError_t e = hipSuccess;
   if (b->fused) {
-    // the step of every instance, one launch per kernel (it > 1, not MLS, not sampling)
+    // the step of every instance, one launch per kernel (it > 1, not a sampling step)
     const sph2d_params& z = b->ctx[0]->prm;
     const int mixed = z.precision == SPH2D_FP32_MIXED;
-    launch_batch_step(b->stream, b->dev, mixed, mixed, z.shift_on, z.solver);
+    launch_batch_step(b->stream, b->dev, mixed, mls ? 1 : 0, z.shift_on, z.solver);
     e = hipGetLastError();
     for (sph2d_ctx* c : b->ctx) c->evaluated = true;
   } else {
@@ -1533,6 +1533,11 @@ int batch_capture(sph2d_batch* b, int it) {
     if (g) (void)hipGraphDestroy(g);
     if (rc != SPH2D_OK) return rc;
     return fail(SPH2D_ERR_HIP, std::string("batch capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  if (mls) {
+    b->graph_mls = g;
+    HIP_OR_FAIL(hipGraphInstantiate(&b->exec_mls, b->graph_mls, nullptr, nullptr, 0));
+    return SPH2D_OK;
   }
   b->graph = g;
   HIP_OR_FAIL(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -1590,13 +1595,15 @@ int sph2d_batch_run(sph2d_batch* b, int32_t first, int32_t nsteps) {
   int rc = SPH2D_OK;
   for (int k = 0; k < nsteps && rc == SPH2D_OK; ++k) {
     const int it = first + k;
-    bool graph = true;
-    for (const sph2d_ctx* c : b->ctx) graph = graph && replayable(c, it);
-    if (graph) {
-      if (!b->exec) rc = batch_capture(b, it);
+    int kind = step_kind(b->ctx[0], it);
+    for (const sph2d_ctx* c : b->ctx) kind = std::min(kind, step_kind(c, it));
+    if (kind == 2 && !b->fused) kind = 0;
+    if (kind != 0) {
+      hipGraphExec_t& ex = kind == 2 ? b->exec_mls : b->exec;
+      if (!ex) rc = batch_capture(b, it, kind == 2);
       if (rc == SPH2D_OK && !b->on_batch) rc = batch_join(b);
       if (rc == SPH2D_OK) {
-        const hipError_t e = hipGraphLaunch(b->exec, b->stream);
+        const hipError_t e = hipGraphLaunch(ex, b->stream);
         if (e != hipSuccess) rc = fail(SPH2D_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
         b->on_batch = true;
         ++b->replays;
@@ -1631,6 +1638,8 @@ int sph2d_batch_destroy(sph2d_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->exec_mls) (void)hipGraphExecDestroy(b->exec_mls);
+  if (b->graph_mls) (void)hipGraphDestroy(b->graph_mls);
   for (hipEvent_t e : b->join) if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : b->sync) if (e) (void)hipEventDestroy(e);
   if (b->fork) (void)hipEventDestroy(b->fork);

@@ -613,8 +613,8 @@ __device__ __forceinline__ void rebin_check(const DevParams& P, Buffers& B, int 
     atomicOr(&B.counters[C_REBIN], 1);
 }
 
-__global__ void k_stage(DevParams P, Buffers B, int kind) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_stage_body(const DevParams& P, Buffers& B, int kind, int blk_, int nblk_) {
+  const int s = blk_ * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
   if (s >= n) return;
   const double dt = P.dt;
@@ -696,6 +696,10 @@ __global__ void k_stage(DevParams P, Buffers B, int kind) {
   if (t == 2) rebin_check(P, B, B.l_cell[s], x);
 }
 
+__global__ void k_stage(DevParams P, Buffers B, int kind) {
+  k_stage_body(P, B, kind, blockIdx.x, gridDim.x);
+}
+
 void launch_stage(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int kind) {
   hipLaunchKernelGGL(k_stage, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B, kind);
 }
@@ -760,8 +764,8 @@ __global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers 
 // registers: same row, different rounding (relative ~ cond(A) * eps). Fewer than 9
 // neighbours fall back to the Shepard average, as the reference does (:284-295).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_mls(DevParams P, Buffers B) {
-  const int s = xcd_block() * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_mls_body(const DevParams& P, Buffers& B, int blk_, int nblk_) {
+  const int s = blk_ * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
   if (s >= n) return;
   const int ts = B.l_type[s];
@@ -809,6 +813,10 @@ __global__ void __launch_bounds__(kBlock) k_mls(DevParams P, Buffers B) {
     atomicAdd(&B.counters[C_MLSFB], 1);
   }
   B.mls_rho[s] = rho;
+}
+
+__global__ void __launch_bounds__(kBlock) k_mls(DevParams P, Buffers B) {
+  k_mls_body(P, B, xcd_block(), gridDim.x);
 }
 
 void launch_mls(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
@@ -1187,8 +1195,8 @@ __device__ __forceinline__ void shift_one(const DevParams& P, const Buffers& B, 
   p = tait_p(P, rho);
 }
 
-__global__ void k_shift_refresh(DevParams P, Buffers B, int mode, int shift_on) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_shift_refresh_body(const DevParams& P, Buffers& B, int mode, int shift_on, int blk_, int nblk_) {
+  const int g = blk_ * blockDim.x + threadIdx.x;
   if (g >= B.counters[C_NFLUID]) return;
   const int s = B.l_of_g[g];
   if (s < 0) return;
@@ -1203,6 +1211,10 @@ __global__ void k_shift_refresh(DevParams P, Buffers B, int mode, int shift_on) 
   B.g_rho[g] = rho;
   B.g_p[g] = pc.x;
   B.g_c[g] = pc.y;
+}
+
+__global__ void k_shift_refresh(DevParams P, Buffers B, int mode, int shift_on) {
+  k_shift_refresh_body(P, B, mode, shift_on, blockIdx.x, gridDim.x);
 }
 
 void launch_shift_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int mode, int shift_on) {
@@ -1486,6 +1498,7 @@ __global__ void kb_step_fill(BatchArgs A) {
   if (k == 0) {
     I.B.counters[C_NILL] = 0;
     I.B.counters[C_NSURF] = 0;
+    I.B.counters[C_MLSFB] = 0;
   }
 }
 
@@ -1570,11 +1583,32 @@ __global__ void __launch_bounds__(kBlock) kb_cell_rank(BatchArgs A) {
   k_cell_rank_body(B, A.inst[w.i].n_cells, w.blk, w.nblk);
 }
 
-__global__ void __launch_bounds__(kBlock) kb_gather_predict(BatchArgs A, int lean) {
+__global__ void __launch_bounds__(kBlock) kb_gather_predict(BatchArgs A, int save_before, int lean) {
   const BatchWhere w = batch_where(A, blockIdx.x);
   const DevParams P = A.inst[w.i].P;
   Buffers B = A.inst[w.i].B;
-  k_gather_predict_body(P, B, 0, lean, w.blk, w.nblk);
+  k_gather_predict_body(P, B, save_before, lean, w.blk, w.nblk);
+}
+
+__global__ void kb_stage(BatchArgs A, int kind) {
+  const BatchWhere w = batch_where(A, blockIdx.x);
+  const DevParams P = A.inst[w.i].P;
+  Buffers B = A.inst[w.i].B;
+  k_stage_body(P, B, kind, w.blk, w.nblk);
+}
+
+__global__ void __launch_bounds__(kBlock) kb_mls(BatchArgs A) {
+  const BatchWhere w = batch_where(A, xcd_block());
+  const DevParams P = A.inst[w.i].P;
+  Buffers B = A.inst[w.i].B;
+  k_mls_body(P, B, w.blk, w.nblk);
+}
+
+__global__ void kb_shift_refresh(BatchArgs A, int mode, int shift_on) {
+  const BatchWhere w = batch_where(A, blockIdx.x);
+  const DevParams P = A.inst[w.i].P;
+  Buffers B = A.inst[w.i].B;
+  k_shift_refresh_body(P, B, mode, shift_on, w.blk, w.nblk);
 }
 
 __global__ void kb_correct_refresh(BatchArgs A, int shift_on) {
@@ -1586,7 +1620,8 @@ __global__ void kb_correct_refresh(BatchArgs A, int shift_on) {
 
 static BatchArgs bargs(const BatchDev& D, int cls) { return BatchArgs{D.inst, D.start[cls], D.n}; }
 
-void launch_batch_step(hipStream_t s, const BatchDev& D, int mixed, int lean, int shift_on, int solver) {
+void launch_batch_step(hipStream_t s, const BatchDev& D, int mixed, int mls, int shift_on, int solver) {
+  const int lean = mixed && !mls;
   auto go = [&](int cls, auto kernel, int threads, auto... args) {
     if (D.total[cls] > 0)
       hipLaunchKernelGGL(kernel, dim3(D.total[cls]), dim3(threads), 0, s, bargs(D, cls), args...);
@@ -1607,10 +1642,18 @@ void launch_batch_step(hipStream_t s, const BatchDev& D, int mixed, int lean, in
   go(BC_SCAN_C, kb_scan_apply, kBlock, 1);
   go(BC_GLOBAL, kb_cell_scatter, kBlock);
   go(BC_RANK, kb_cell_rank, kBlock);
-  go(BC_GLOBAL, kb_gather_predict, kBlock, lean);
+  go(BC_GLOBAL, kb_gather_predict, kBlock, (int)mls, lean);
   // single_step_compute, then the corrector with shifting and the refresh
   launch_batch_eval(s, D, mixed, solver);
-  go(BC_LOCAL, kb_correct_refresh, kBlock, shift_on);
+  if (!mls) {
+    go(BC_LOCAL, kb_correct_refresh, kBlock, shift_on);
+    return;
+  }
+  // an MLS step: corrector, Reinitialize_particle_density, then shifting (with the
+  // MLS density) and the refresh
+  go(BC_LOCAL, kb_stage, kBlock, (int)ST_PC_CORRECT);
+  go(BC_LOCAL, kb_mls, kBlock);
+  go(BC_LOCAL, kb_shift_refresh, kBlock, shift_on ? 1 : 2, shift_on);
 }
 
 }  // namespace sph2d

@@ -169,9 +169,10 @@ struct BatchDev {
 int batch_blocks(int cls, const Inst& I);
 size_t q32_bytes();
 void q32_fill(const DevParams& P, void* out);
-// One predictor-corrector step after step 1 that is neither an MLS nor a sampling
-// step, for every instance (same form, solver and shifting switch; no damping, no XSPH).
-void launch_batch_step(hipStream_t s, const BatchDev& D, int mixed, int lean, int shift_on, int solver);
+// One predictor-corrector step after step 1 that is not a sampling step, for every
+// instance (same form, solver and shifting switch; no damping, no XSPH); mls: the
+// step reinitialises the density (every instance has the same MLS schedule).
+void launch_batch_step(hipStream_t s, const BatchDev& D, int mixed, int mls, int shift_on, int solver);
 void launch_batch_eval(hipStream_t s, const BatchDev& D, int mixed, int solver);
 void launch_batch_eval32(hipStream_t s, const BatchDev& D, int pass, int solver);
 void launch_wavemaker_pull_back(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, double xp, double up);

@@ -1,7 +1,9 @@
 """The batched form (sph2d_batch): independent contexts advanced together, the
-replayable steps as one hipGraph launch. Each instance must come out bit-identical
-to the same instance run alone, after graph replays separated by eager steps (step
-1, the MLS step 20) and by host synchronisations."""
+repeating and MLS steps as hipGraph launches (one launch per kernel over all
+instances for the predictor-corrector, one branch per instance for the
+RK-variant). Each instance must come out bit-identical to the same instance run
+alone, after graph replays separated by eager steps (step 1; the MLS step 20 for
+the RK-variant) and by host synchronisations."""
 import sys
 
 import numpy as np
@@ -50,7 +52,8 @@ def test_batch_is_bit_identical_to_each_instance_alone(precision, integrator):
                 assert np.array_equal(ga[f], gb[f]), (done, k, f)
         say(f"step {done}: identical; graph nodes/replays {batch.info()}")
     nodes, replays = batch.info()
-    assert nodes >= 15 and replays == 23          # steps 2..25 except the MLS step 20
+    # steps 2..25; the MLS step 20 is replayed too when the launches are fused (PC)
+    assert nodes >= 15 and replays == (24 if integrator == sph2d.INTEGRATOR_PC else 23)
     ke = [s.sample()[0] for s in bat]
     assert all(0.9 < k < 1.0 for k in ke)
     batch.close()
