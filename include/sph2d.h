@@ -248,11 +248,12 @@ int sph2d_output_sampling_lines(sph2d_ctx* ctx, int32_t i_time_step, const char*
                                 const double* first, const double* last, int32_t n_points);
 
 /* --- batched form (no reference counterpart) ------------------------------------- */
-/* Independent contexts (uneven sizes, any Re) advanced together. Steps after step 1
- * that are neither MLS nor sampling steps replay one hipGraph: when the contexts
+/* Independent contexts (uneven sizes, any Re) advanced together. When the contexts
  * share the predictor-corrector, precision form, solver and shifting switch (no
- * XSPH, no damping) it holds one launch per kernel covering every instance, else
- * every instance's step as a parallel branch; the other steps are enqueued per context.
+ * XSPH, no damping), every step after step 1 that is not a sampling step replays a
+ * hipGraph of one launch per kernel covering every instance (one graph for the
+ * repeating steps, one for MLS steps); otherwise the repeating steps replay a graph
+ * with every instance's step as a parallel branch. Other steps are enqueued per context.
  * Each instance's results are bit-identical to running it alone. The contexts must
  * share integrator, mls_on, mls_every and sample_every, outlive the batch and not
  * be used by other calls while sph2d_batch_run runs; on return, later calls on a
