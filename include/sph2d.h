@@ -92,6 +92,8 @@ typedef struct {
   double damping_length;     /* EndZoneDampingLengh (:291) */
   double damping_amplitude;  /* EndZoneDampingAmplitude (:294), Cosin_Type only; REAL(4) 0.3 */
   double water_depth;        /* water_depth (:194) */
+  int32_t walls;             /* fix_ghost_layer (:90) layers of fixed ghost particles around the box
+                                (DistributeFixedBoundaryParticleOrNot = 1, :86); needs periodic = 0 */
 } sph2d_params;
 
 typedef struct sph2d_ctx sph2d_ctx;

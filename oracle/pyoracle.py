@@ -39,6 +39,7 @@ class OrcParams(ctypes.Structure):
         ("rate_reuse", ctypes.c_int32),
         ("damping", ctypes.c_int32), ("damping_length", ctypes.c_double),
         ("damping_amplitude", ctypes.c_double), ("water_depth", ctypes.c_double),
+        ("walls", ctypes.c_int32),
     ]
 
 
@@ -77,6 +78,8 @@ def lib():
         L.orc_sampling_points.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_double), P(ctypes.c_double),
                                           ctypes.c_double, P(ctypes.c_double)]
         L.orc_wave_probes.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_double), P(ctypes.c_double)]
+        L.orc_wall_positions.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double)]
+        L.orc_wall_positions.restype = ctypes.c_int32
         L.orc_wavemaker_pull_back.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         L.orc_initial_ke.argtypes = [ctypes.c_void_p]
         L.orc_initial_ke.restype = ctypes.c_double
@@ -203,6 +206,13 @@ class Oracle:
         out = np.zeros(1 + len(px))
         lib().orc_wave_probes(self._c, len(px), _dp(px) if len(px) else None, _dp(out))
         return out[0], out[1:]
+
+    def wall_positions(self):
+        """(x, y) of the fixed ghost walls (Distribute_Initial_Particle.f90:317-495)."""
+        n = lib().orc_wall_positions(self._c, None, None)
+        x, y = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        lib().orc_wall_positions(self._c, _dp(x), _dp(y))
+        return x[:n], y[:n]
 
     def wavemaker_pull_back(self, xp: float, up: float):
         lib().orc_wavemaker_pull_back(self._c, xp, up)

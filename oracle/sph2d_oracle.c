@@ -28,7 +28,8 @@ struct orc_ctx {
   double dx, dy, h, chain_d, chain_ox, chain_oy, e0, max_shift;
   int chain_nx, chain_ny;
   int n_fluid;                 /* particle_ture_number */
-  int n_ghost;                 /* Period_Boundary_Particle_Number */
+  int n_ghost;                 /* Period_Boundary_Particle_Number, or the fixed ghosts */
+  int *gbtype;                 /* Boundary_particle_type of fixed ghosts (-6, -7, -8)  */
   int cap_global, cap_local;
   /* global arrays: particle_* (Public_variable_module.f90:33-49) */
   double *gx, *gy, *gu, *gv, *gmass, *grho, *gh, *gc, *gp, *ge;
@@ -113,6 +114,7 @@ void orc_default_params(orc_params* p) {
   p->damping_length = 3.0;
   p->damping_amplitude = (double)0.3f;
   p->water_depth = 1.0;                            /* water_depth = domain_size_y */
+  p->walls = 0;                                    /* periodic box (TGV default)     */
 }
 
 /* ------------------------------------------------------------------ */
@@ -262,6 +264,7 @@ orc_ctx* orc_create(const orc_params* p) {
   A(ana_u); A(ana_v); A(ana_p); A(ana_vm); A(gvort); A(gnx); A(gny);
 #undef A
   c->gtype = xcalloc(c->cap_global, sizeof(int));
+  c->gbtype = xcalloc(c->cap_global, sizeof(int));
   c->gsurf = xcalloc(c->cap_global, sizeof(int));
   alloc_local(c, c->cap_global);
   return c;
@@ -270,7 +273,7 @@ orc_ctx* orc_create(const orc_params* p) {
 void orc_destroy(orc_ctx* c) {
   if (!c) return;
   void* ptrs[] = {c->gx, c->gy, c->gu, c->gv, c->gmass, c->grho, c->gh, c->gc, c->gp, c->ge,
-                  c->ana_u, c->ana_v, c->ana_p, c->ana_vm, c->gtype, c->gvort, c->gnx, c->gny, c->gsurf,
+                  c->ana_u, c->ana_v, c->ana_p, c->ana_vm, c->gtype, c->gbtype, c->gvort, c->gnx, c->gny, c->gsurf,
                   c->gchain_off, c->gchain_members,
                   c->name, c->x, c->y, c->u, c->v, c->mass, c->rho, c->hh, c->c, c->p, c->e,
                   c->vol, c->type, c->chain_no, c->cand_n, c->cand, c->fra_v, c->sum_w, c->sdwx,
@@ -289,6 +292,79 @@ void orc_destroy(orc_ctx* c) {
     free(c->k_ax[k]); free(c->k_ay[k]);
   }
   free(c);
+}
+
+/* ------------------------------------------------------------------ */
+/* Distribute_Initial_Particle.f90:317-495: fix_ghost_layer layers of fixed ghost */
+/* particles around the box (spacing fix_ghost_dx = dx, fix_ghost_dy = dy): per    */
+/* layer a top run (right to left), a bottom run (left to right), a right run      */
+/* (upwards) and a left run (downwards), each without its end point; the first     */
+/* layer's point count k comes from walking until within dx/2 of the run's end,    */
+/* and grows by 2 per layer. Distribute_Fixed_Boundary_Particle.f90:52-104 types   */
+/* them: -6 top and bottom rows, -7 corners, -8 left and right columns.            */
+/* Appended after the fluid as type -3: v = 0, rho = rho0, p = 0, c = c0, e = e0,  */
+/* mass = water_particle_mass_0, which the TGV branch never sets (0).              */
+/* ------------------------------------------------------------------ */
+static int wall_run(double x0, double y0, double x1, double y1, double sx, double sy, double step, double gdx,
+                    int* k, int first, double* x, double* y, int at) {
+  if (first) {
+    int j = 0;
+    for (;;) {
+      ++j;
+      double px = x0 + sx * ((j - 1) * step), py = y0 + sy * ((j - 1) * step);
+      if (sqrt((x1 - px) * (x1 - px) + (y1 - py) * (y1 - py)) <= gdx / 2.0) break;
+    }
+    *k = j;
+  }
+  double dist = sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0)) / (*k - 1);
+  int n = 0;
+  for (int j = 1; j <= *k - 1; ++j) {
+    if (x) { x[at + n] = x0 + sx * ((j - 1) * dist); y[at + n] = y0 + sy * ((j - 1) * dist); }
+    ++n;
+  }
+  *k = *k + 2;
+  return n;
+}
+
+static int wall_positions(double bx, double by, double gdx, double gdy, int layers, double* x, double* y) {
+  int n = 0, k = 0;
+  for (int i = 1; i <= layers; ++i)                                  /* Top: :323-362 */
+    n += wall_run(bx + (i - 1) * gdy + gdy / 2.0, by + (i - 1) * gdy + gdy / 2.0,
+                  -(i - 1) * gdy - gdy / 2.0, by + (i - 1) * gdy + gdy / 2.0, -1, 0, gdx, gdx, &k, i == 1, x, y, n);
+  for (int i = 1; i <= layers; ++i)                                  /* Bottom: :366-403 */
+    n += wall_run(-(i - 1) * gdy - gdy / 2.0, -(i - 1) * gdy - gdy / 2.0,
+                  bx + (i - 1) * gdy + gdy / 2.0, -(i - 1) * gdy - gdy / 2.0, 1, 0, gdx, gdx, &k, i == 1, x, y, n);
+  for (int i = 1; i <= layers; ++i)                                  /* Right: :406-442 */
+    n += wall_run(bx + (i - 1) * gdy + gdy / 2.0, -(i - 1) * gdy - gdy / 2.0,
+                  bx + (i - 1) * gdy + gdy / 2.0, by + (i - 1) * gdy + gdy / 2.0, 0, 1, gdx, gdx, &k, i == 1, x, y, n);
+  for (int i = 1; i <= layers; ++i)                                  /* Left: :445-480 */
+    n += wall_run(-(i - 1) * gdy - gdy / 2.0, by + (i - 1) * gdy + gdy / 2.0,
+                  -(i - 1) * gdy - gdy / 2.0, -(i - 1) * gdy - gdy / 2.0, 0, -1, gdy, gdx, &k, i == 1, x, y, n);
+  return n;
+}
+
+static void append_walls(orc_ctx* c) {
+  const orc_params* P = &c->P;
+  int N = c->n_fluid;
+  int nw = wall_positions(P->box_x, P->box_y, c->dx, c->dy, P->walls, NULL, NULL);
+  if (N + nw > c->cap_global) { fprintf(stderr, "orc: wall capacity\n"); abort(); }
+  wall_positions(P->box_x, P->box_y, c->dx, c->dy, P->walls, c->gx + N, c->gy + N);
+  for (int q = 0; q < nw; ++q) {
+    int m = N + q;
+    double px = c->gx[m], py = c->gy[m];
+    int bt;
+    if (py <= 0.0) bt = (px < 0.0 || px > P->box_x) ? -7 : -6;
+    else if (py >= P->box_y) bt = (px < 0.0 || px > P->box_x) ? -7 : -6;
+    else bt = -8;
+    c->gbtype[m] = bt;
+    c->gu[m] = 0.0; c->gv[m] = 0.0; c->grho[m] = P->rho0; c->gmass[m] = 0.0;
+    c->gtype[m] = -3; c->gh[m] = c->h; c->gp[m] = 0.0; c->gc[m] = sqrt(P->c0sq); c->ge[m] = c->e0;
+  }
+  c->n_ghost = nw;
+}
+
+int32_t orc_wall_positions(const orc_ctx* c, double* x, double* y) {
+  return wall_positions(c->P.box_x, c->P.box_y, c->dx, c->dy, c->P.walls, x, y);
 }
 
 /* ------------------------------------------------------------------ */
@@ -326,6 +402,7 @@ void orc_init_tgv(orc_ctx* c) {
     ke = ke + 0.5 * c->gmass[i] * (c->gu[i] * c->gu[i] + c->gv[i] * c->gv[i]);
   c->ke0 = ke;
   c->n_ghost = 0;
+  if (P->walls > 0) append_walls(c);
 }
 
 int32_t orc_set_fluid_state(orc_ctx* c, int32_t n, const double* x, const double* y,
@@ -342,6 +419,7 @@ int32_t orc_set_fluid_state(orc_ctx* c, int32_t n, const double* x, const double
   double ke = 0.0;
   for (int i = 0; i < n; ++i) ke = ke + 0.5 * c->gmass[i] * (c->gu[i] * c->gu[i] + c->gv[i] * c->gv[i]);
   c->ke0 = ke;
+  if (c->P.walls > 0) append_walls(c);
   return 0;
 }
 
@@ -357,6 +435,7 @@ static void copy_ghost(orc_ctx* c, int j, int i, double x, double y) {
 }
 
 void orc_period_boundary(orc_ctx* c) {
+  if (c->P.walls > 0) return;                         /* fixed walls: no periodic ghosts */
   const double xmin = 0.0, xmax = c->P.box_x, lx = c->P.box_x;
   const double ymin = 0.0, ymax = c->P.box_y, ly = c->P.box_y;
   const double r3h = 3 * c->h;   /* kernel_scale*smooth_length */
@@ -521,6 +600,64 @@ void orc_devide_block(orc_ctx* c, int32_t i_time_step) {
 /* ------------------------------------------------------------------ */
 /* Assign_boundary_particles.f90:75-152,303-321                       */
 /* ------------------------------------------------------------------ */
+/* Assign_boundary_particles.f90:157-299: each fixed ghost takes the pressure and   */
+/* velocity of its non-ghost candidates within 3 hbar, hbar = 1.2 (h_i + h_j)/2    */
+/* (1.2 a REAL(4) literal), by MLS of order 1 (MLS_Interpolation_In_Subdomain_     */
+/* Domain.f90 with MLS_order_Boundary = 1: Fai_k = (1/sum W) W_k, with 3 or more   */
+/* particles, else zero); the velocity is mirrored by the boundary type (-6: v,    */
+/* -7: u and v, -8: u); then rho from the inverse Tait law (REAL(4) exponent),     */
+/* mass = rho V0 and c = c0 (rho/rho0)^3, or the rest state when p is 0.           */
+static void assign_walls(orc_ctx* c) {
+  const orc_params* P = &c->P;
+  const double vol0 = c->dx * c->dy;
+  int* tn = xcalloc(300, sizeof(int));
+  double* wk = xcalloc(300, sizeof(double));
+  for (int i = 0; i < c->n_local; ++i) {
+    if (c->type[i] != -3) continue;
+    double u = 0.0, v = 0.0, pr = 0.0;
+    int nt = 0;
+    for (int L = 0; L < c->cand_n[i]; ++L) {
+      int j = c->cand[(size_t)i * 300 + L];
+      if (c->type[j] == -3) continue;
+      double pd[2] = {c->x[i] - c->x[j], c->y[i] - c->y[j]};
+      double d = sqrt(pd[0] * pd[0] + pd[1] * pd[1]);
+      double hb = (double)1.2f * (c->hh[i] + c->hh[j]) / 2.0;
+      if (d <= 3 * hb && d >= (double)1.0E-7f) {
+        double w, dw[2];
+        orc_compute_kernel(2, d, pd, hb, &w, dw);
+        tn[nt] = j; wk[nt] = w; ++nt;
+      }
+    }
+    if (nt >= 3) {
+      double a = 0.0;
+      for (int k = 0; k < nt; ++k) a = a + wk[k] * 1.0;
+      double ainv = 1.0 / a;
+      for (int k = 0; k < nt; ++k) {
+        double fai = 1.0 * (ainv * wk[k]);
+        int j = tn[k];
+        pr = pr + c->p[j] * fai;
+        u = u + c->u[j] * fai;
+        v = v + c->v[j] * fai;
+      }
+    }
+    int bt = c->gbtype[c->name[i]];
+    if (bt == -6) v = -v;
+    else if (bt == -7) { u = -u; v = -v; }
+    else if (bt == -8) u = -u;
+    if (pr != 0.0) {
+      c->e[i] = c->e0;
+      c->rho[i] = P->rho0 * pow(1 + 7 * pr / (P->c0sq * P->rho0), (double)0.142857142f);
+      c->mass[i] = c->rho[i] * vol0;
+      c->c[i] = sqrt(P->c0sq) * ipow3(c->rho[i] / P->rho0);
+    } else {
+      u = 0.0; v = 0.0; pr = 0.0;
+      c->e[i] = c->e0; c->rho[i] = P->rho0; c->mass[i] = 0.0; c->c[i] = sqrt(P->c0sq);
+    }
+    c->u[i] = u; c->v[i] = v; c->p[i] = pr;
+  }
+  free(tn); free(wk);
+}
+
 static void assign_boundary_particles(orc_ctx* c) {
   int n = c->n_local;
   double o_sub = c->chain_ox + (c->sub_start - 1) * c->chain_d;
@@ -554,6 +691,7 @@ static void assign_boundary_particles(orc_ctx* c) {
     }
     c->cand_n[i] = cnt < 300 ? cnt : 300;
   }
+  if (c->P.walls > 0) assign_walls(c);                          /* :157-299 */
   for (int k = 0; k < n; ++k) c->vol[k] = c->mass[k] / c->rho[k];
 }
 
@@ -1438,7 +1576,7 @@ static const fld F64[] = {
 };
 static const fld I32[] = {
   FL("type", type, 0), FL("name", name, 0), FL("surface", surf, 0), FL("near_surface", near, 0),
-  FL("candidate_n", cand_n, 0), FL("g_type", gtype, 1), FL("g_surface", gsurf, 1), FL("pair_i", pi, 2), FL("pair_j", pj, 2),
+  FL("candidate_n", cand_n, 0), FL("g_type", gtype, 1), FL("g_btype", gbtype, 1), FL("g_surface", gsurf, 1), FL("pair_i", pi, 2), FL("pair_j", pj, 2),
 };
 static int64_t fld_count(const orc_ctx* c, int kind) {
   switch (kind) {

@@ -46,6 +46,10 @@ typedef struct {
   double damping_length;     /* EndZoneDampingLengh                        (:291)    */
   double damping_amplitude;  /* EndZoneDampingAmplitude (Cosin_Type)       (:294)    */
   double water_depth;        /* water_depth                                (:194)    */
+  /* fixed ghost walls around [0,box_x]x[0,box_y] instead of the periodic ghosts:   */
+  /* fix_ghost_layer layers (:86-93 DistributeFixedBoundaryParticleOrNot = 1 and     */
+  /* DistributePeriodBoundaryParticleOrNot = 0); 0 = periodic box                   */
+  int32_t walls;
 } orc_params;
 
 enum { ORC_SOLVER_MODIFIED_LINEAR_RIEMANN = 0,   /* Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90 */
@@ -92,6 +96,7 @@ int32_t orc_set_fluid_state(orc_ctx* c, int32_t n, const double* x, const double
                             const double* mass, const int32_t* type);
 
 /* Pieces of one time step. */
+int32_t orc_wall_positions(const orc_ctx* c, double* x, double* y);   /* fixed ghost positions (NULL: count) */
 void orc_period_boundary(orc_ctx* c);                       /* Distribute_Period_Boundary_Particle */
 void orc_devide_block(orc_ctx* c, int32_t i_time_step);     /* Devide_block_and_send_tasks (1 rank) */
 void orc_single_step_compute(orc_ctx* c, int32_t i_time_step);

@@ -124,6 +124,10 @@ void make_devparams(const sph2d_params& p, DevParams& P) {
     P.sps_c23b = (double)(float)(a * b);
   }
   P.mls_alpha = p.mls_alpha;
+  P.walls = p.walls;
+  P.wall_h = (double)1.2f * (p.h + p.h) / 2.0;     // Assign_boundary_particles.f90:182
+  P.wall_kfactor = 1.0 / ((double)3.14159265358f * (P.wall_h * P.wall_h) * (double)c4);
+  P.vol0 = p.dx * (p.box_y / p.ny);                // intial_volume_0 = interior_dx*interior_dy
   P.damping = p.damping;
   P.damp_len = p.damping_length;
   P.damp_amp = p.damping_amplitude;
@@ -179,7 +183,10 @@ int check_params(const sph2d_params* p) {
     return fail(SPH2D_ERR_UNSUPPORTED, "only the reference's rate re-use is built");
   if (p->mls_on && p->mls_order != 6)
     return fail(SPH2D_ERR_UNSUPPORTED, "only MLS order 6 (the reference default) is built");
-  if (!p->periodic) return fail(SPH2D_ERR_UNSUPPORTED, "only the periodic box is built");
+  if (p->walls < 0 || p->walls > 64) return fail(SPH2D_ERR_ARGUMENT, "walls must be in [0, 64]");
+  if (p->periodic && p->walls) return fail(SPH2D_ERR_ARGUMENT, "fixed walls need periodic = 0");
+  if (!p->periodic && !p->walls)
+    return fail(SPH2D_ERR_UNSUPPORTED, "the box is periodic or has fixed ghost walls (walls > 0)");
   if (p->mls_every <= 0 || p->sample_every <= 0)
     return fail(SPH2D_ERR_ARGUMENT, "mls_every and sample_every must be positive");
   if (p->damping < SPH2D_DAMPING_OFF || p->damping > SPH2D_DAMPING_DUALSPH)
@@ -273,6 +280,7 @@ int sph2d_default_params(sph2d_params* p) {
   p->damping_length = 3.0;
   p->damping_amplitude = (double)0.3f;
   p->water_depth = p->box_y;                       // water_depth = domain_size_y
+  p->walls = 0;                                    // DistributeFixedBoundaryParticleOrNot = 0
   return SPH2D_OK;
 }
 
@@ -293,6 +301,10 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   const long ghost_bound = 2L * m * (long)(p->nx + p->ny + 4 * m);
   c->cap_fluid = n_fluid_capacity;
   c->cap_ghost = (int)std::max(16384L, 3 * ghost_bound);
+  if (p->walls) {
+    const int nw = wall_layout(p->box_x, p->box_y, p->dx, p->box_y / p->ny, p->walls, nullptr, nullptr, nullptr);
+    c->cap_ghost = std::max(c->cap_ghost, nw + 64);
+  }
   c->cap_global = c->cap_fluid + c->cap_ghost;
   c->cap_local = c->cap_global;
   c->n_cells = c->P.chain_nx * c->P.chain_ny;
@@ -306,6 +318,7 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   rc |= c->alloc(&B.g_c, G); rc |= c->alloc(&B.g_e, G);
   rc |= c->alloc(&B.g_type, G); rc |= c->alloc(&B.g_key, G); rc |= c->alloc(&B.g_slot, G);
   rc |= c->alloc(&B.ana, 4 * F); rc |= c->alloc(&B.g_vort, F);
+  rc |= c->alloc(&B.g_btype, G);
   rc |= c->alloc(&B.gblk, 2 * ((F + 255) / 256) + 2);
   rc |= c->alloc(&B.cnt_yx, H); rc |= c->alloc(&B.off_yx, H);
   rc |= c->alloc(&B.src_x, H);
@@ -369,6 +382,36 @@ static int set_nfluid(sph2d_ctx* c, int n) {
   return SPH2D_OK;
 }
 
+// Distribute_Initial_Particle.f90:483-495: the fixed ghost layers after the n fluid
+// particles (type -3, at rest, rho0, p = 0, c0, e0; mass water_particle_mass_0, which
+// the TGV branch leaves 0), counted as the ghosts of every step.
+static int upload_walls(sph2d_ctx* c, int n) {
+  const sph2d_params& p = c->prm;
+  const double dy = p.box_y / p.ny;
+  const int nw = wall_layout(p.box_x, p.box_y, p.dx, dy, p.walls, nullptr, nullptr, nullptr);
+  if (nw > c->cap_ghost) return fail(SPH2D_ERR_CAPACITY, "wall layers exceed the ghost capacity");
+  std::vector<double> x(nw), y(nw), zero(nw, 0.0), rho(nw, p.rho0), cs(nw, std::sqrt(p.c0sq)),
+      e(nw, p.c0sq / (7.0 - 1.0));
+  std::vector<int> bt(nw), t(nw, -3);
+  wall_layout(p.box_x, p.box_y, p.dx, dy, p.walls, x.data(), y.data(), bt.data());
+  std::vector<double2> pos(nw), vel(nw, make_double2(0.0, 0.0));
+  for (int q = 0; q < nw; ++q) pos[q] = make_double2(x[q], y[q]);
+  Buffers& B = c->B;
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_pos + n, pos.data(), nw * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_vel + n, vel.data(), nw * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_rho + n, rho.data(), nw * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_mass + n, zero.data(), nw * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_p + n, zero.data(), nw * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_c + n, cs.data(), nw * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_e + n, e.data(), nw * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_type + n, t.data(), nw * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_btype + n, bt.data(), nw * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  launch_fill_i32(c->stream, B.counters + C_NGHOST, 1, nw);
+  HIP_OR_FAIL(hipGetLastError());
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  return SPH2D_OK;
+}
+
 int sph2d_init_tgv(sph2d_ctx* c) {
   if (!c) return fail(SPH2D_ERR_ARGUMENT, "ctx is NULL");
   const int n = c->prm.nx * c->prm.ny;
@@ -377,6 +420,10 @@ int sph2d_init_tgv(sph2d_ctx* c) {
   if (rc) return rc;
   launch_init_tgv(c->stream, c->P, c->B, c->prm.nx, c->prm.ny);
   HIP_OR_FAIL(hipGetLastError());
+  if (c->prm.walls) {
+    rc = upload_walls(c, n);
+    if (rc) return rc;
+  }
   rc = compute_ke0(c);
   if (rc) return rc;
   c->have_state = true;
@@ -418,6 +465,11 @@ int sph2d_upload_state(sph2d_ctx* c, int32_t n, const double* pos, const double*
   HIP_OR_FAIL(hipMemcpyAsync(B.g_type, t.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
   int rc = set_nfluid(c, n);
   if (rc) return rc;
+  if (c->prm.walls) {
+    if (n + c->cap_ghost > c->cap_global) return fail(SPH2D_ERR_CAPACITY, "no room for the wall layers");
+    rc = upload_walls(c, n);
+    if (rc) return rc;
+  }
   rc = compute_ke0(c);
   if (rc) return rc;
   c->have_state = true;
@@ -605,6 +657,7 @@ static int need_state(sph2d_ctx* c) {
 int sph2d_distribute_period_boundary(sph2d_ctx* c) {
   int rc = need_state(c);
   if (rc) return rc;
+  if (c->prm.walls) return SPH2D_OK;                 // fixed walls: no periodic ghosts
   launch_period_boundary(c->stream, c->P, c->B, c->cap_fluid, c->cap_ghost);
   HIP_OR_FAIL(hipGetLastError());
   return SPH2D_OK;
@@ -639,6 +692,7 @@ static void do_eval(sph2d_ctx* c, bool with_eos, int it) {
       c->events_used += 4;
     }
   }
+  if (c->P.walls) launch_assign_walls(c->stream, c->P, c->B, c->cap_local);   // single_step_compute.f90:80
   launch_eval(c->stream, c->P, c->B, c->cap_local, ev);
   if (c->P.damping) launch_end_damping(c->stream, c->P, c->B, c->cap_local);
   c->evaluated = true;
@@ -685,6 +739,7 @@ int sph2d_refresh_domain(sph2d_ctx* c, int32_t it) {
   int rc = need_state(c);
   if (rc) return rc;
   launch_refresh(c->stream, c->P, c->B, c->cap_local);
+  if (c->P.walls) launch_wall_refresh(c->stream, c->P, c->B, c->cap_local);
   HIP_OR_FAIL(hipGetLastError());
   return SPH2D_OK;
 }
@@ -700,7 +755,7 @@ static int enqueue_step(sph2d_ctx* c, int it) {
   // mixed form the FP64 local v, p, c, rho, volume are read only by MLS, the vorticity
   // of a sampling step and XSPH; other steps skip writing them.
   const bool sampling = it % c->prm.sample_every == 0;
-  const bool lean = c->P.mixed && !mls && !sampling && c->prm.epsilon_xsph == 0.0;
+  const bool lean = c->P.mixed && !mls && !sampling && c->prm.epsilon_xsph == 0.0 && !c->prm.walls;
   rc = devide_block(c, it, pc && it > 1 ? (mls ? 2 : (lean ? 3 : 1)) : 0);
   if (rc) return rc;
   hipStream_t s = c->stream;
@@ -713,6 +768,7 @@ static int enqueue_step(sph2d_ctx* c, int it) {
     if (!mls) {
       // corrector, Shift_particle_position and Refresh_domain in one pass
       launch_correct_refresh(s, c->P, c->B, c->cap_local, c->prm.shift_on);
+      if (c->P.walls) launch_wall_refresh(s, c->P, c->B, c->cap_local);
       HIP_OR_FAIL(hipGetLastError());
       return SPH2D_OK;
     }
@@ -734,6 +790,7 @@ static int enqueue_step(sph2d_ctx* c, int it) {
   // Shift_particle_position (+ the MLS density) and Refresh_domain in one pass
   const int mode = mls ? (c->prm.shift_on ? 1 : 2) : 0;
   launch_shift_refresh(s, c->P, c->B, c->cap_local, mode, c->prm.shift_on);
+  if (c->P.walls) launch_wall_refresh(s, c->P, c->B, c->cap_local);
   HIP_OR_FAIL(hipGetLastError());
   return SPH2D_OK;
 }
@@ -1457,7 +1514,7 @@ bool fusable(const std::vector<sph2d_ctx*>& v) {
   for (const sph2d_ctx* c : v) {
     const sph2d_params& a = c->prm;
     if (a.integrator != SPH2D_INTEGRATOR_PC || a.precision != z.precision || a.solver != z.solver ||
-        a.shift_on != z.shift_on || a.epsilon_xsph != 0.0 || a.damping != SPH2D_DAMPING_OFF)
+        a.shift_on != z.shift_on || a.epsilon_xsph != 0.0 || a.damping != SPH2D_DAMPING_OFF || a.walls)
       return false;
   }
   return true;

@@ -10,6 +10,7 @@
 #include "sph2d_internal.h"
 #include "sph2d_device.h"
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 
 namespace sph2d {
@@ -1084,6 +1085,146 @@ __global__ void __launch_bounds__(kBlock) k_wavemaker_pull_back(DevParams P, Buf
 
 void launch_wavemaker_pull_back(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, double xp, double up) {
   hipLaunchKernelGGL(k_wavemaker_pull_back, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, s, P, B, xp, up);
+}
+
+// Assign_boundary_particles.f90:157-299 before an evaluation's passes: each fixed
+// ghost takes the pressure and velocity of its non-ghost candidates (the 3x3 cells)
+// within 3 hbar, hbar = 1.2 (h_i + h_j) / 2 with 1.2 a REAL(4) literal, by the
+// order-1 MLS of MLS_Interpolation_In_Subdomain_Domain.f90 (Fai = (1 / sum W) W,
+// with 3 or more particles, else nothing); the velocity is mirrored by its boundary
+// type (-6: v, -7: u and v, -8: u); then rho from the inverse Tait law (REAL(4)
+// exponent), mass = rho V0 and c = c0 (rho/rho0)^3, or the rest state when p is 0.
+__device__ __forceinline__ double wall_w(const DevParams& P, double r) {
+  const double q = r / P.wall_h;
+  return (exp(-(q * q)) - P.e9) * P.wall_kfactor;
+}
+
+__global__ void __launch_bounds__(kBlock) k_assign_walls(DevParams P, Buffers B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.counters[C_NLOCAL] || B.l_type[s] != -3) return;
+  const double2 xs = B.l_pos[s];
+  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
+  const double rmax = 3 * P.wall_h, rmin = (double)1.0E-7f;
+  double a = 0.0;
+  int nt = 0;
+  for (int d = 0; d < 3; ++d)
+    for (int o = R.lo[d]; o < R.hi[d]; ++o) {
+      if (B.l_type[o] == -3) continue;
+      const double2 xo = B.l_pos[o];
+      const double px = xs.x - xo.x, py = xs.y - xo.y;
+      const double r = sqrt(px * px + py * py);
+      if (r <= rmax && r >= rmin) {
+        a = a + wall_w(P, r) * 1.0;
+        ++nt;
+      }
+    }
+  double u = 0.0, v = 0.0, pr = 0.0;
+  if (nt >= 3) {
+    const double ainv = 1.0 / a;
+    for (int d = 0; d < 3; ++d)
+      for (int o = R.lo[d]; o < R.hi[d]; ++o) {
+        if (B.l_type[o] == -3) continue;
+        const double2 xo = B.l_pos[o];
+        const double px = xs.x - xo.x, py = xs.y - xo.y;
+        const double r = sqrt(px * px + py * py);
+        if (!(r <= rmax && r >= rmin)) continue;
+        const double fai = 1.0 * (ainv * wall_w(P, r));
+        const double2 vo = B.l_vel[o];
+        pr = pr + B.l_pc[o].x * fai;
+        u = u + vo.x * fai;
+        v = v + vo.y * fai;
+      }
+  }
+  const int bt = B.g_btype[B.name[s]];
+  if (bt == -6) v = -v;
+  else if (bt == -7) { u = -u; v = -v; }
+  else if (bt == -8) u = -u;
+  double rho, mass, c;
+  if (pr != 0.0) {
+    rho = P.rho0 * pow(1 + 7 * pr / (P.c0sq * P.rho0), (double)0.142857142f);
+    mass = rho * P.vol0;
+    const double x = rho / P.rho0;
+    c = P.sqrt_c0sq * (x * x * x);
+  } else {
+    u = 0.0; v = 0.0; pr = 0.0;
+    rho = P.rho0; mass = 0.0; c = P.sqrt_c0sq;
+  }
+  B.l_vel[s] = make_double2(u, v);
+  B.l_pc[s] = make_double2(pr, c);
+  B.l_rv[s] = make_double2(rho, mass / rho);
+  B.l_mass[s] = mass;
+  if (P.mixed) write_r0(P, B, s, make_double2(u, v), rho, mass, -3);
+}
+
+void launch_assign_walls(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
+  hipLaunchKernelGGL(k_assign_walls, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B);
+}
+
+// Refresh_domain.f90:136-300 for the fixed ghosts: their assigned state goes back to
+// the global arrays (the next gather reads it; particle_mass at a local index reads
+// it in the SPS term).
+__global__ void k_wall_refresh(DevParams P, Buffers B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.counters[C_NLOCAL] || B.l_type[s] != -3) return;
+  const int g = B.name[s];
+  B.g_vel[g] = B.l_vel[s];
+  const double2 pc = B.l_pc[s];
+  B.g_p[g] = pc.x;
+  B.g_c[g] = pc.y;
+  B.g_rho[g] = B.l_rv[s].x;
+  B.g_mass[g] = B.l_mass[s];
+}
+
+void launch_wall_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
+  hipLaunchKernelGGL(k_wall_refresh, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B);
+}
+
+static int wall_run(double x0, double y0, double x1, double y1, double sx, double sy, double step, double gdx,
+                    int& k, bool first, double* x, double* y, int at) {
+  if (first) {
+    int j = 0;
+    for (;;) {
+      ++j;
+      const double px = x0 + sx * ((j - 1) * step), py = y0 + sy * ((j - 1) * step);
+      if (std::sqrt((x1 - px) * (x1 - px) + (y1 - py) * (y1 - py)) <= gdx / 2.0) break;
+    }
+    k = j;
+  }
+  const double dist = std::sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0)) / (k - 1);
+  int n = 0;
+  for (int j = 1; j <= k - 1; ++j, ++n)
+    if (x) {
+      x[at + n] = x0 + sx * ((j - 1) * dist);
+      y[at + n] = y0 + sy * ((j - 1) * dist);
+    }
+  k = k + 2;
+  return n;
+}
+
+// Layer i's runs: top from (Lx + o, Ly + o) leftwards, bottom from (-o, -o) rightwards,
+// right from (Lx + o, -o) upwards, left from (-o, Ly + o) downwards, o = (i - 1) dy + dy/2,
+// each without its end point; the first layer's count walks until within dx/2 of the end.
+int wall_layout(double bx, double by, double gdx, double gdy, int layers, double* x, double* y, int* btype) {
+  int n = 0, k = 0;
+  for (int i = 1; i <= layers; ++i)
+    n += wall_run(bx + (i - 1) * gdy + gdy / 2.0, by + (i - 1) * gdy + gdy / 2.0, -(i - 1) * gdy - gdy / 2.0,
+                  by + (i - 1) * gdy + gdy / 2.0, -1, 0, gdx, gdx, k, i == 1, x, y, n);
+  for (int i = 1; i <= layers; ++i)
+    n += wall_run(-(i - 1) * gdy - gdy / 2.0, -(i - 1) * gdy - gdy / 2.0, bx + (i - 1) * gdy + gdy / 2.0,
+                  -(i - 1) * gdy - gdy / 2.0, 1, 0, gdx, gdx, k, i == 1, x, y, n);
+  for (int i = 1; i <= layers; ++i)
+    n += wall_run(bx + (i - 1) * gdy + gdy / 2.0, -(i - 1) * gdy - gdy / 2.0, bx + (i - 1) * gdy + gdy / 2.0,
+                  by + (i - 1) * gdy + gdy / 2.0, 0, 1, gdx, gdx, k, i == 1, x, y, n);
+  for (int i = 1; i <= layers; ++i)
+    n += wall_run(-(i - 1) * gdy - gdy / 2.0, by + (i - 1) * gdy + gdy / 2.0, -(i - 1) * gdy - gdy / 2.0,
+                  -(i - 1) * gdy - gdy / 2.0, 0, -1, gdy, gdx, k, i == 1, x, y, n);
+  if (x && btype)
+    for (int q = 0; q < n; ++q) {
+      const bool corner = x[q] < 0.0 || x[q] > bx;
+      if (y[q] <= 0.0 || y[q] >= by) btype[q] = corner ? -7 : -6;
+      else btype[q] = -8;
+    }
+  return n;
 }
 
 void launch_end_damping(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {

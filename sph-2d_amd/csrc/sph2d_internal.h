@@ -32,6 +32,8 @@ struct DevParams {
   double chain_d, chain_ox, chain_oy;    // 4h cells, origin -5 cells
   int chain_nx, chain_ny;
   double alpha, beta, fai, delta, delta_alpha, epsilon_xsph;
+  int walls;                             // fixed ghost layers (0: periodic box)
+  double wall_h, wall_kfactor, vol0;     // hbar = 1.2 (h + h) / 2 and its kernel factor; dx dy
   int damping;                           // SPH2D_DAMPING_*
   double damp_len, damp_amp, water_depth, e32;   // e32 = exp(1.0) in REAL(4)
   double shift_beta, shift_R, max_shift, w_dx, h2;
@@ -61,6 +63,7 @@ struct Buffers {
   int *g_type, *g_key, *g_slot;
   double *ana;                            // 4 per fluid particle: u, v, p, |v|
   double *g_vort;                         // vorticity (z) per fluid gid, on sampling steps
+  int *g_btype;                           // boundary type of each fixed ghost (-6, -7, -8), by gid
   int *gblk, *cnt_yx, *off_yx, *src_x;   // gblk: per block of gids, x-ghost then y-ghost counts
   // cells (column-major key over the global grid)
   int *cell_count, *cell_start, *cell_members;
@@ -133,6 +136,11 @@ struct GridArgs {
 };
 void launch_grid_mls(hipStream_t s, const DevParams& P, Buffers& B, const GridArgs& G);
 void launch_end_damping(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+void launch_assign_walls(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+void launch_wall_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+// Distribute_Initial_Particle.f90:317-495 and Distribute_Fixed_Boundary_Particle.f90:52-104:
+// positions and boundary types of the fixed ghost layers (x, y, type may be null: count only)
+int wall_layout(double box_x, double box_y, double dx, double dy, int layers, double* x, double* y, int* btype);
 
 // ---- batched launches: one launch covers the blocks of many independent instances ----
 struct Inst {

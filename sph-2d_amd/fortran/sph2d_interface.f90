@@ -23,6 +23,7 @@ module sph2d_interface
     real(c_double) :: shift_beta, shift_R, max_shift, mls_alpha
     integer(c_int32_t) :: damping
     real(c_double) :: damping_length, damping_amplitude, water_depth
+    integer(c_int32_t) :: walls
   end type sph2d_params
 
   interface

@@ -33,7 +33,7 @@ def rel_err(g, r):
     return float(np.max(np.abs(g - r)) / scale)
 
 
-ORACLE_KW = ("shift_on", "mls_on", "solver", "damping", "damping_length", "damping_amplitude", "water_depth")
+ORACLE_KW = ("shift_on", "mls_on", "solver", "damping", "damping_length", "damping_amplitude", "water_depth", "walls")
 
 
 def pair(n, integrator=2, state=None, **kw):
@@ -456,4 +456,48 @@ def test_end_damping_zone_parity(damping):
     effect = rel_err(o.f64("g_u")[:N], free.f64("g_u")[:N])
     say(f"damping effect on u: {effect:.3e}")
     assert effect > 1e-4
+    sim.close()
+
+
+def test_fixed_walls_parity():
+    """Fixed ghost walls instead of the periodic box (Distribute_Initial_Particle.f90:
+    317-495, Assign_boundary_particles.f90:157-299): one evaluation's wall state and
+    rates, then 21 predictor-corrector steps (an MLS step included) against the oracle."""
+    say("test_fixed_walls_parity")
+    n = 40
+    sim, o = pair(n, integrator=2, walls=4, periodic=0)
+    evaluate_both(sim, o)
+    t = o.i32("type")
+    assert np.array_equal(sim.local("type"), t) and (t == -3).sum() == 704
+    for f in ("p", "u", "v", "rho", "mass"):
+        assert rel_err(sim.local(f), o.f64(f)) < TOL, f
+    for f in ("drhodt", "ax", "ay", "fra_v"):
+        assert rel_err(sim.local(f), o.f64(f)) < TOL, f
+    sim.close()
+    sim, o = pair(n, integrator=2, walls=4, periodic=0)
+    for it in range(1, 22):
+        sim.time_step(it)
+        o.time_step(it)
+    g = sim.download_state()
+    N = n * n
+    for f, of in (("x", "g_x"), ("y", "g_y"), ("u", "g_u"), ("v", "g_v"), ("rho", "g_rho"), ("p", "g_p")):
+        r = o.f64(of)[:N]
+        e = float(np.max(np.abs(g[f] - r))) if f in ("x", "y") else rel_err(g[f], r)
+        say(f, e)
+        assert e < (1e-12 if f in ("x", "y") else 1e-10), (f, e)
+    assert np.allclose(sim.sample()[:2], o.sample()[:2], rtol=1e-11)
+    sim.close()
+
+
+def test_fixed_walls_mixed_form():
+    """The FP32-mixed form with fixed walls: rates of one evaluation against the FP64
+    oracle within the mixed form's bar (1e-4 of the field norm)."""
+    say("test_fixed_walls_mixed_form")
+    n = 40
+    sim, o = pair(n, integrator=2, walls=4, periodic=0, precision=sph2d.FP32_MIXED)
+    evaluate_both(sim, o)
+    for f in ("drhodt", "ax", "ay"):
+        e = rel_err(sim.local(f), o.f64(f))
+        say(f, e)
+        assert e < 1e-4, (f, e)
     sim.close()

@@ -575,3 +575,65 @@ def test_wavemaker_pull_back_mirrors_behind_the_paddle():
     near = (x1 - xp) < 1.0 / n
     assert np.all(u1[near] == up) and np.all(u1[~near] == u0[~near]) and near.sum() >= 2 * n
     o.close()
+
+
+# --- fixed ghost walls (DistributeFixedBoundaryParticleOrNot = 1) ---------------------------
+def test_fixed_wall_layers_closed_form():
+    """Distribute_Initial_Particle.f90:317-495 with Distribute_Fixed_Boundary_Particle.f90:
+    layer i of an n x n box holds 4 (n + 2i - 1) particles on the ring at distance
+    (i - 1/2) dx outside the box, spaced dx; per layer 2n are typed -6 (top and bottom
+    rows), 2n -8 (side columns) and 4 (2i - 1) -7 (corners)."""
+    n, L = 40, 4
+    o = po.Oracle(nx=n, ny=n, walls=L)
+    o.init_tgv()
+    x, y = o.wall_positions()
+    assert len(x) == sum(4 * (n + 2 * i - 1) for i in range(1, L + 1))
+    dx = 1.0 / n
+    ring = np.maximum(np.maximum(-x, x - 1), np.maximum(-y, y - 1)) / dx + 0.5   # layer index
+    assert np.allclose(ring, np.round(ring), atol=1e-9) and ring.min() > 0.99 and ring.max() < L + 0.01
+    P = np.stack([x, y], 1)
+    d = np.sqrt(((P[:, None, :] - P[None, :, :]) ** 2).sum(-1)) + np.eye(len(x))
+    assert abs(d.min() - dx) < 1e-12
+    N = n * n
+    bt = o.i32("g_btype")[N:N + len(x)]
+    assert (bt == -6).sum() == 2 * n * L and (bt == -8).sum() == 2 * n * L
+    assert (bt == -7).sum() == sum(4 * (2 * i - 1) for i in range(1, L + 1))
+    assert np.all(o.i32("g_type")[N:N + len(x)] == -3)
+    o.close()
+
+
+def test_fixed_walls_take_the_fluid_state_mirrored():
+    """Assign_boundary_particles.f90:157-299 on a uniform fluid (one pressure, one
+    velocity): every wall particle with 3 or more fluid particles within 3 hbar takes
+    that pressure (the order-1 MLS weights sum to 1), rho from the inverse Tait law,
+    mass rho V0, and the velocity mirrored by its type (-6: v, -7: u and v, -8: u);
+    the others keep the rest state (p = 0, v = 0, mass 0)."""
+    n, L = 30, 4
+    o0 = po.Oracle(nx=n, ny=n)
+    o0.init_tgv()
+    N = n * n
+    x, y = o0.f64("g_x")[:N].copy(), o0.f64("g_y")[:N].copy()
+    o0.close()
+    rho0 = 1000.0
+    rho = np.full(N, rho0 * 1.002)
+    u0, v0 = 0.3, -0.7
+    o = po.Oracle(nx=n, ny=n, walls=L)
+    o.set_fluid_state(x, y, np.full(N, u0), np.full(N, v0), rho, rho / n ** 2)
+    evaluate(o)
+    t, p, u, v = o.i32("type"), o.f64("p"), o.f64("u"), o.f64("v")
+    pf = p[t == 2][0]
+    assert np.all(p[t == 2] == pf) and pf > 0
+    w = t == -3
+    bt = o.i32("g_btype")[o.i32("name")[w]]
+    pw, uw, vw, rw, mw = p[w], u[w], v[w], o.f64("rho")[w], o.f64("mass")[w]
+    on = pw != 0.0
+    assert on.sum() > 0.9 * w.sum()
+    np.testing.assert_allclose(pw[on], pf, rtol=1e-12)
+    np.testing.assert_allclose(rw[on], rho0 * 1.002, rtol=1e-7)
+    np.testing.assert_allclose(mw[on], rw[on] / n ** 2, rtol=1e-14)
+    su = np.where(bt == -6, 1.0, -1.0)
+    sv = np.where(bt == -8, 1.0, -1.0)
+    np.testing.assert_allclose(uw[on], su[on] * u0, rtol=1e-12)
+    np.testing.assert_allclose(vw[on], sv[on] * v0, rtol=1e-12)
+    assert np.all(uw[~on] == 0.0) and np.all(vw[~on] == 0.0) and np.all(mw[~on] == 0.0)
+    o.close()
