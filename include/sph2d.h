@@ -94,6 +94,8 @@ typedef struct {
   double water_depth;        /* water_depth (:194) */
   int32_t walls;             /* fix_ghost_layer (:90) layers of fixed ghost particles around the box
                                 (DistributeFixedBoundaryParticleOrNot = 1, :86); needs periodic = 0 */
+  double gravity;            /* g (:32) when Gravity_Switch (:33) is 'on', else 0: -g on the fluid's
+                                y acceleration, hydrostatic term in the wall pressure */
 } sph2d_params;
 
 typedef struct sph2d_ctx sph2d_ctx;

@@ -39,7 +39,7 @@ class OrcParams(ctypes.Structure):
         ("rate_reuse", ctypes.c_int32),
         ("damping", ctypes.c_int32), ("damping_length", ctypes.c_double),
         ("damping_amplitude", ctypes.c_double), ("water_depth", ctypes.c_double),
-        ("walls", ctypes.c_int32),
+        ("walls", ctypes.c_int32), ("gravity", ctypes.c_double),
     ]
 
 

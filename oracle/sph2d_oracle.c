@@ -115,6 +115,7 @@ void orc_default_params(orc_params* p) {
   p->damping_amplitude = (double)0.3f;
   p->water_depth = 1.0;                            /* water_depth = domain_size_y */
   p->walls = 0;                                    /* periodic box (TGV default)     */
+  p->gravity = 0.0;                                /* Gravity_Switch = 'off'         */
 }
 
 /* ------------------------------------------------------------------ */
@@ -602,6 +603,7 @@ void orc_devide_block(orc_ctx* c, int32_t i_time_step) {
 /* ------------------------------------------------------------------ */
 /* Assign_boundary_particles.f90:157-299: each fixed ghost takes the pressure and   */
 /* velocity of its non-ghost candidates within 3 hbar, hbar = 1.2 (h_i + h_j)/2    */
+/* (with gravity, each p_j carried to the wall's height: p_j - g rho_j (y_i - y_j)) */
 /* (1.2 a REAL(4) literal), by MLS of order 1 (MLS_Interpolation_In_Subdomain_     */
 /* Domain.f90 with MLS_order_Boundary = 1: Fai_k = (1/sum W) W_k, with 3 or more   */
 /* particles, else zero); the velocity is mirrored by the boundary type (-6: v,    */
@@ -635,7 +637,8 @@ static void assign_walls(orc_ctx* c) {
       for (int k = 0; k < nt; ++k) {
         double fai = 1.0 * (ainv * wk[k]);
         int j = tn[k];
-        pr = pr + c->p[j] * fai;
+        if (P->gravity != 0.0) pr = pr + c->p[j] * fai - P->gravity * c->rho[j] * (c->y[i] - c->y[j]) * fai;
+        else pr = pr + c->p[j] * fai;
         u = u + c->u[j] * fai;
         v = v + c->v[j] * fai;
       }
@@ -1207,6 +1210,7 @@ void orc_single_step_compute(orc_ctx* c, int32_t i_time_step) {
   memset(c->ax, 0, nb); memset(c->ay, 0, nb);                    /* :152-171 */
   for (int i = 0; i < n; ++i) {
     if (c->type[i] != 2) continue;
+    if (c->P.gravity != 0.0) c->ext_y[i] = c->ext_y[i] - c->P.gravity;   /* :161-163 */
     c->ax[i] = c->ia_x[i] + c->ext_x[i] + c->av_x[i] + c->ai_x[i] + c->sps_x[i] + c->lap_x[i];
     c->ay[i] = c->ia_y[i] + c->ext_y[i] + c->av_y[i] + c->ai_y[i] + c->sps_y[i] + c->lap_y[i];
     c->dedt[i] = c->dedt[i] + 0.0 + 0.0;

@@ -50,6 +50,7 @@ typedef struct {
   /* fix_ghost_layer layers (:86-93 DistributeFixedBoundaryParticleOrNot = 1 and     */
   /* DistributePeriodBoundaryParticleOrNot = 0); 0 = periodic box                   */
   int32_t walls;
+  double gravity;            /* g (9.81) when Gravity_Switch is 'on', else 0  (:32-33) */
 } orc_params;
 
 enum { ORC_SOLVER_MODIFIED_LINEAR_RIEMANN = 0,   /* Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90 */

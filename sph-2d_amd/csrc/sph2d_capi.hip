@@ -125,6 +125,7 @@ void make_devparams(const sph2d_params& p, DevParams& P) {
   }
   P.mls_alpha = p.mls_alpha;
   P.walls = p.walls;
+  P.g = p.gravity;
   P.wall_h = (double)1.2f * (p.h + p.h) / 2.0;     // Assign_boundary_particles.f90:182
   P.wall_kfactor = 1.0 / ((double)3.14159265358f * (P.wall_h * P.wall_h) * (double)c4);
   P.vol0 = p.dx * (p.box_y / p.ny);                // intial_volume_0 = interior_dx*interior_dy
@@ -184,6 +185,7 @@ int check_params(const sph2d_params* p) {
   if (p->mls_on && p->mls_order != 6)
     return fail(SPH2D_ERR_UNSUPPORTED, "only MLS order 6 (the reference default) is built");
   if (p->walls < 0 || p->walls > 64) return fail(SPH2D_ERR_ARGUMENT, "walls must be in [0, 64]");
+  if (!std::isfinite(p->gravity) || p->gravity < 0) return fail(SPH2D_ERR_ARGUMENT, "gravity must be >= 0");
   if (p->periodic && p->walls) return fail(SPH2D_ERR_ARGUMENT, "fixed walls need periodic = 0");
   if (!p->periodic && !p->walls)
     return fail(SPH2D_ERR_UNSUPPORTED, "the box is periodic or has fixed ghost walls (walls > 0)");
@@ -281,6 +283,7 @@ int sph2d_default_params(sph2d_params* p) {
   p->damping_amplitude = (double)0.3f;
   p->water_depth = p->box_y;                       // water_depth = domain_size_y
   p->walls = 0;                                    // DistributeFixedBoundaryParticleOrNot = 0
+  p->gravity = 0.0;                                // Gravity_Switch = 'off'
   return SPH2D_OK;
 }
 

@@ -431,7 +431,7 @@ __device__ __forceinline__ void k_eval_c32_body(const DevParams& P, const DevPar
   double dr = (double)drho.s;
   if (ts == 2) dr = dr + (double)corr.s;
   B.drhodt[s] = dr;
-  B.acc[s] = ts == 2 ? make_double2((double)ax.s, (double)ay.s) : make_double2(0.0, 0.0);
+  B.acc[s] = ts == 2 ? make_double2((double)ax.s, (double)ay.s - P.g) : make_double2(0.0, 0.0);
   if (P.epsilon_xsph != 0.0) B.avg[s] = make_double2((double)agx.s, (double)agy.s);
   if (do_shift) B.shift[s] = make_double2((double)shx.s, (double)shy.s);
 }

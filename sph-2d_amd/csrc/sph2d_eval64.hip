@@ -361,11 +361,12 @@ __device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int
   B.drhodt[s] = drho;
   if (DEBUG) {
     ax = iax + 0.0 + avx + aix + spx + lpx;
-    ay = iay + 0.0 + avy + aiy + spy + lpy;
+    ay = iay + (0.0 - P.g) + avy + aiy + spy + lpy;        // single_step_compute.f90:161-165
     double* d = B.dbg + 10 * s;
     d[0] = iax; d[1] = iay; d[2] = avx; d[3] = avy; d[4] = aix; d[5] = aiy;
     d[6] = spx; d[7] = spy; d[8] = lpx; d[9] = lpy;
   }
+  else ay = ay - P.g;                                       // external acceleration (gravity, or 0)
   if (ts != 2) { ax = 0.0; ay = 0.0; }
   B.acc[s] = make_double2(ax, ay);
   if (P.epsilon_xsph != 0.0) B.avg[s] = make_double2(agx, agy);

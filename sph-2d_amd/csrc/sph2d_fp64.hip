@@ -1091,7 +1091,8 @@ void launch_wavemaker_pull_back(hipStream_t s, const DevParams& P, Buffers& B, i
 // ghost takes the pressure and velocity of its non-ghost candidates (the 3x3 cells)
 // within 3 hbar, hbar = 1.2 (h_i + h_j) / 2 with 1.2 a REAL(4) literal, by the
 // order-1 MLS of MLS_Interpolation_In_Subdomain_Domain.f90 (Fai = (1 / sum W) W,
-// with 3 or more particles, else nothing); the velocity is mirrored by its boundary
+// with 3 or more particles, else nothing; with gravity each p_j is carried to the
+// wall's height, p_j - g rho_j (y_i - y_j)); the velocity is mirrored by its boundary
 // type (-6: v, -7: u and v, -8: u); then rho from the inverse Tait law (REAL(4)
 // exponent), mass = rho V0 and c = c0 (rho/rho0)^3, or the rest state when p is 0.
 __device__ __forceinline__ double wall_w(const DevParams& P, double r) {
@@ -1130,7 +1131,8 @@ __global__ void __launch_bounds__(kBlock) k_assign_walls(DevParams P, Buffers B)
         if (!(r <= rmax && r >= rmin)) continue;
         const double fai = 1.0 * (ainv * wall_w(P, r));
         const double2 vo = B.l_vel[o];
-        pr = pr + B.l_pc[o].x * fai;
+        if (P.g != 0.0) pr = pr + B.l_pc[o].x * fai - P.g * B.l_rv[o].x * (xs.y - xo.y) * fai;
+        else pr = pr + B.l_pc[o].x * fai;
         u = u + vo.x * fai;
         v = v + vo.y * fai;
       }

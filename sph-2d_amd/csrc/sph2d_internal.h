@@ -33,6 +33,7 @@ struct DevParams {
   int chain_nx, chain_ny;
   double alpha, beta, fai, delta, delta_alpha, epsilon_xsph;
   int walls;                             // fixed ghost layers (0: periodic box)
+  double g;                              // gravity (0: off)
   double wall_h, wall_kfactor, vol0;     // hbar = 1.2 (h + h) / 2 and its kernel factor; dx dy
   int damping;                           // SPH2D_DAMPING_*
   double damp_len, damp_amp, water_depth, e32;   // e32 = exp(1.0) in REAL(4)

@@ -24,6 +24,7 @@ module sph2d_interface
     integer(c_int32_t) :: damping
     real(c_double) :: damping_length, damping_amplitude, water_depth
     integer(c_int32_t) :: walls
+    real(c_double) :: gravity
   end type sph2d_params
 
   interface

@@ -63,7 +63,7 @@ class Params(ctypes.Structure):
         ("max_shift", ctypes.c_double), ("mls_alpha", ctypes.c_double),
         ("damping", ctypes.c_int32), ("damping_length", ctypes.c_double),
         ("damping_amplitude", ctypes.c_double), ("water_depth", ctypes.c_double),
-        ("walls", ctypes.c_int32),
+        ("walls", ctypes.c_int32), ("gravity", ctypes.c_double),
     ]
 
 

@@ -501,3 +501,28 @@ def test_fixed_walls_mixed_form():
         say(f, e)
         assert e < 1e-4, (f, e)
     sim.close()
+
+
+def test_hydrostatic_tank_with_gravity_parity():
+    """Gravity on the fluid and the hydrostatic term of the wall pressure: a tank at
+    rest in the walled box, 21 predictor-corrector steps against the oracle."""
+    say("test_hydrostatic_tank_with_gravity_parity")
+    from test_oracle_physics import hydrostatic_tank
+    n = 30
+    st = hydrostatic_tank(n)
+    M = len(st[0])
+    prm = sph2d.tgv_params(n, integrator=2, walls=4, periodic=0, gravity=9.81, dt=1e-4 * 100 / n * 0.5)
+    sim = sph2d.Simulation(prm, capacity=n * n)
+    sim.upload_state(*st)
+    o = Oracle(nx=n, ny=n, integrator=2, walls=4, gravity=9.81, dt=prm.dt)
+    o.set_fluid_state(*st)
+    for it in range(1, 22):
+        sim.time_step(it)
+        o.time_step(it)
+    g = sim.download_state()
+    for f, of in (("x", "g_x"), ("y", "g_y"), ("u", "g_u"), ("v", "g_v"), ("rho", "g_rho"), ("p", "g_p")):
+        r = o.f64(of)[:M]
+        e = float(np.max(np.abs(g[f] - r))) if f in ("x", "y") else rel_err(g[f], r)
+        say(f, e)
+        assert e < (1e-12 if f in ("x", "y") else 1e-10), (f, e)
+    sim.close()

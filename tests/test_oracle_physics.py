@@ -637,3 +637,45 @@ def test_fixed_walls_take_the_fluid_state_mirrored():
     np.testing.assert_allclose(vw[on], sv[on] * v0, rtol=1e-12)
     assert np.all(uw[~on] == 0.0) and np.all(vw[~on] == 0.0) and np.all(mw[~on] == 0.0)
     o.close()
+
+
+def hydrostatic_tank(n=30, H=0.5, g=9.81, rho0=1000.0, c0sq=100.0):
+    """Fluid at rest below y = H in the walled unit box, p = rho0 g (H - y), rho from
+    the inverse Tait law (REAL(4) exponent, Distribute_Initial_Particle.f90:222)."""
+    o0 = po.Oracle(nx=n, ny=n)
+    o0.init_tgv()
+    N = n * n
+    x, y = o0.f64("g_x")[:N].copy(), o0.f64("g_y")[:N].copy()
+    o0.close()
+    keep = y < H
+    x, y = x[keep], y[keep]
+    p = rho0 * g * (H - y)
+    rho = rho0 * (1 + 7 * p / (c0sq * rho0)) ** float(np.float32(0.142857142))
+    M = len(x)
+    return x, y, np.zeros(M), np.zeros(M), rho, rho / n ** 2
+
+
+def test_hydrostatic_tank_stays_at_rest():
+    """Gravity (single_step_compute.f90:161-163) with fixed walls and the hydrostatic
+    wall pressure (Assign_boundary_particles.f90:225-230): a tank at rest stays at
+    rest (speeds below 1% of sqrt(gH) after 100 steps) and keeps p = rho0 g (H - y)
+    near the bottom within 5%; without the gravity term the column starts to fall."""
+    n, H, g = 30, 0.5, 9.81
+    st = hydrostatic_tank(n, H, g)
+    M = len(st[0])
+    dt = 1e-4 * 100 / n * 0.5
+    for grav in (g, 0.0):
+        o = po.Oracle(nx=n, ny=n, walls=4, gravity=grav, integrator=2, dt=dt)
+        o.set_fluid_state(*st)
+        for it in range(1, 101):
+            o.time_step(it)
+        u, v, p, y = o.f64("g_u")[:M], o.f64("g_v")[:M], o.f64("g_p")[:M], o.f64("g_y")[:M]
+        vmax = np.sqrt(u ** 2 + v ** 2).max() / np.sqrt(g * H)
+        band = y < 0.1
+        ratio = p[band].mean() / (1000.0 * g * (H - y[band].mean()))
+        if grav:
+            assert vmax < 0.01, vmax
+            assert abs(ratio - 1) < 0.05, ratio
+        else:
+            assert vmax > 0.05, vmax                     # the pressure gradient pushes the column up
+        o.close()
