@@ -526,3 +526,43 @@ def test_hydrostatic_tank_with_gravity_parity():
         say(f, e)
         assert e < (1e-12 if f in ("x", "y") else 1e-10), (f, e)
     sim.close()
+
+
+def test_dam_break_with_walls_gravity_damping_and_probes():
+    """The pieces of a wave tank together: a water column collapsing in the walled
+    box under gravity, with the cosine end damping over the last 0.2 and the wave
+    gauges and front, 30 predictor-corrector steps against the oracle."""
+    say("test_dam_break_with_walls_gravity_damping_and_probes")
+    n, H, g = 40, 0.5, 9.81
+    o0 = Oracle(nx=n, ny=n)
+    o0.init_tgv()
+    N = n * n
+    x, y = o0.f64("g_x")[:N].copy(), o0.f64("g_y")[:N].copy()
+    o0.close()
+    keep = (y < H) & (x < 0.4)
+    x, y = x[keep], y[keep]
+    M = len(x)
+    p = 1000.0 * g * (H - y)
+    rho = 1000.0 * (1 + 7 * p / (100.0 * 1000.0)) ** float(np.float32(0.142857142))
+    st = (x, y, np.zeros(M), np.zeros(M), rho, rho / n ** 2)
+    kw = dict(integrator=2, walls=4, gravity=g, damping=sph2d.DAMPING_COSINE, damping_length=0.2, water_depth=1.0)
+    prm = sph2d.tgv_params(n, periodic=0, dt=1e-4 * 100 / n * 0.5, **kw)
+    sim = sph2d.Simulation(prm, capacity=N)
+    sim.upload_state(*st)
+    o = Oracle(nx=n, ny=n, dt=prm.dt, **kw)
+    o.set_fluid_state(*st)
+    for it in range(1, 31):
+        sim.time_step(it)
+        o.time_step(it)
+    gs = sim.download_state()
+    for f, of in (("x", "g_x"), ("y", "g_y"), ("u", "g_u"), ("v", "g_v"), ("rho", "g_rho"), ("p", "g_p")):
+        r = o.f64(of)[:M]
+        e = float(np.max(np.abs(gs[f] - r))) if f in ("x", "y") else rel_err(gs[f], r)
+        say(f, e)
+        assert e < (1e-12 if f in ("x", "y") else 1e-10), (f, e)
+    front, el = sim.wave_probes([0.1, 0.3, 0.7])
+    rf, rel = o.wave_probes([0.1, 0.3, 0.7])
+    say(front, el, rf, rel)
+    assert abs(front - rf) < 1e-12 and np.max(np.abs(el - rel)) < 1e-12
+    assert el[2] == 0.0 and el[0] > 0.45
+    sim.close()
