@@ -64,6 +64,10 @@ def force_pass_bytes_per_particle(form: int) -> int:
 # fetched x 2.00 + 165,123 KiB written = 570.8 MB for 4,227,136 subdomain particles.
 FORCE_PASS_TRAFFIC = {"bytes_per_particle": 135.0, "n_side": 2048,
                       "source": "profiles/tgv2048_force_pass_hbm_traffic.md"}
+# FLOPs per subdomain particle of one force-pass launch (SQ_INSTS_VALU_FLOPS_* x 64 lanes)
+FORCE_PASS_VALU = {"fp32_flop_per_particle": 7299.1, "fp64_flop_per_particle": 9006.3,
+                   "source": "profiles/tgv2048_force_pass_valu.md"}
+FP32_VECTOR_TFLOPS = 157.3
 
 
 def cpu_baseline(n_side: int = 512, steps: int = 20) -> dict:
@@ -290,6 +294,20 @@ def main() -> int:
             "units_per_launch": r32["n_local"],
             "duration_ms": r32["force_ms"],
             "launches_timed": r32["n_eval_timed"],
+        },
+        # the same kernel against the vector-ALU roofline it is bound by: FLOPs per particle
+        # from the counter profile, duration measured live (the HBM object above)
+        "valu_roofline": {
+            "kernel": "k_eval_c32 (FP32-mixed force pass)",
+            "bound": "valu",
+            "achieved": FORCE_PASS_VALU["fp32_flop_per_particle"] * r32["n_local"] / (r32["force_ms"] * 1e-3) / 1e12,
+            "peak": FP32_VECTOR_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": FORCE_PASS_VALU["fp32_flop_per_particle"] * r32["n_local"] / (r32["force_ms"] * 1e-3) / 1e12
+                    / FP32_VECTOR_TFLOPS,
+            "fp64_form": FORCE_PASS_VALU["fp64_flop_per_particle"] * r64["n_local"] / (r64["force_ms"] * 1e-3)
+                         / 1e12 / FP64_VECTOR_TFLOPS,
+            "source": FORCE_PASS_VALU["source"],
         },
     }
     if args.batch_instances > 0:
