@@ -96,6 +96,8 @@ typedef struct {
                                 (DistributeFixedBoundaryParticleOrNot = 1, :86); needs periodic = 0 */
   double gravity;            /* g (:32) when Gravity_Switch (:33) is 'on', else 0: -g on the fluid's
                                 y acceleration, hydrostatic term in the wall pressure */
+  int32_t wavemaker_layers;  /* wave_maker_layer (:214) columns of piston wavemaker particles left of
+                                x = 0 (fix_wave_maker_or_not, make_wave_or_not = 1); needs walls */
 } sph2d_params;
 
 typedef struct sph2d_ctx sph2d_ctx;
@@ -293,6 +295,10 @@ int sph2d_wavemaker_motion(const sph2d_wavemaker* w, int32_t i_time_step, double
 /* Wavemaker_Subroutine.f90:196-215 on the device state: fluid particles behind the
  * paddle are mirrored in front of it; those within dx of it take its velocity. */
 int sph2d_wavemaker_pull_back(sph2d_ctx* ctx, double position, double velocity);
+/* Wavemaker_Subroutine.f90:140-215 on the device state, before a step: the water
+ * height next to the paddle, the wavemaker particles placed behind the paddle at
+ * its velocity with the hydrostatic density below that height, then the pull-back. */
+int sph2d_wavemaker_update(sph2d_ctx* ctx, double position, double velocity);
 
 /* Host-array writers (no device; Fortran layout, ld >= n). Text follows the
  * reference's edit descriptors exactly; files named "appended" get the

@@ -40,6 +40,7 @@ class OrcParams(ctypes.Structure):
         ("damping", ctypes.c_int32), ("damping_length", ctypes.c_double),
         ("damping_amplitude", ctypes.c_double), ("water_depth", ctypes.c_double),
         ("walls", ctypes.c_int32), ("gravity", ctypes.c_double),
+        ("wavemaker_layers", ctypes.c_int32),
     ]
 
 
@@ -81,6 +82,7 @@ def lib():
         L.orc_wall_positions.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double)]
         L.orc_wall_positions.restype = ctypes.c_int32
         L.orc_wavemaker_pull_back.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.orc_wavemaker_update.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         L.orc_initial_ke.argtypes = [ctypes.c_void_p]
         L.orc_initial_ke.restype = ctypes.c_double
         for name in ("orc_n_fluid", "orc_n_global", "orc_n_local"):
@@ -216,6 +218,9 @@ class Oracle:
 
     def wavemaker_pull_back(self, xp: float, up: float):
         lib().orc_wavemaker_pull_back(self._c, xp, up)
+
+    def wavemaker_update(self, xp: float, up: float):
+        lib().orc_wavemaker_update(self._c, xp, up)
 
     def sample(self) -> np.ndarray:
         out = np.zeros(14)

@@ -29,6 +29,7 @@ struct orc_ctx {
   int chain_nx, chain_ny;
   int n_fluid;                 /* particle_ture_number */
   int n_ghost;                 /* Period_Boundary_Particle_Number, or the fixed ghosts */
+  int n_wavemaker;             /* wave_maker_particle_number (first of the ghosts)     */
   int *gbtype;                 /* Boundary_particle_type of fixed ghosts (-6, -7, -8)  */
   int cap_global, cap_local;
   /* global arrays: particle_* (Public_variable_module.f90:33-49) */
@@ -116,6 +117,7 @@ void orc_default_params(orc_params* p) {
   p->water_depth = 1.0;                            /* water_depth = domain_size_y */
   p->walls = 0;                                    /* periodic box (TGV default)     */
   p->gravity = 0.0;                                /* Gravity_Switch = 'off'         */
+  p->wavemaker_layers = 0;                         /* make_wave_or_not = 0           */
 }
 
 /* ------------------------------------------------------------------ */
@@ -344,9 +346,75 @@ static int wall_positions(double bx, double by, double gdx, double gdy, int laye
   return n;
 }
 
+/* Distribute_Initial_Particle.f90:254-311 (PistonType): for rows i (y = (i-1/2) dy  */
+/* <= boundary_size_y) and columns j = 1..wave_maker_layer, a particle at           */
+/* x = -(j-1) dx - dx/2 with rho = rho0 (-6.0 g (y - Ly)/c0^2 + 1.0)**(1.0/6.0)     */
+/* (REAL(4) literals; g = 9.81 whatever Gravity_Switch says), at rest, type 130,   */
+/* mass V0 rho(i) with i the wavemaker's own index into the fluid's densities      */
+/* (:300-310). Appended right after the fluid; the layer goes to gbtype.           */
+#define WAVEMAKER_G 9.81
+static void append_wavemaker(orc_ctx* c) {
+  const orc_params* P = &c->P;
+  const int N = c->n_fluid;
+  const double vol0 = c->dx * c->dy;
+  int w = 0;
+  for (int i = 1; i <= P->ny + 200; ++i)
+    for (int j = 1; j <= P->wavemaker_layers; ++j) {
+      double px = -(j - 1) * c->dx - c->dx / 2.0;
+      double py = (i - 1) * c->dy + c->dy / 2.0;
+      if (py <= P->box_y) {
+        int k = N + w;
+        c->gx[k] = px; c->gy[k] = py;
+        c->grho[k] = P->rho0 * pow(-6.0 * WAVEMAKER_G * (py - P->box_y) / P->c0sq + 1.0, (double)(1.0f / 6.0f));
+        c->gp[k] = P->c0sq * P->rho0 * (pow(c->grho[k] / P->rho0, 7.0) - 1.0) / 7.0;
+        c->gbtype[k] = j;
+        ++w;
+      }
+    }
+  for (int q = 0; q < w; ++q) {
+    int k = N + q;
+    c->gu[k] = 0.0; c->gv[k] = 0.0;
+    c->gmass[k] = vol0 * c->grho[q];
+    c->gtype[k] = 130; c->gh[k] = c->h; c->gc[k] = sqrt(P->c0sq); c->ge[k] = c->e0;
+  }
+  c->n_wavemaker = w;
+  c->n_ghost = w;
+}
+
+/* Wavemaker_Subroutine.f90:140-193 on the global arrays: the water height next to */
+/* the paddle (largest fluid y within kernel_scale dx of x_p, from 0, plus dx/2),  */
+/* then each wavemaker particle at x_p - (layer-1) dx - dx/2 with the paddle's     */
+/* velocity and, below that height, rho0 (-6.0 g (y - height)/c0^2 + 1.0)**(1/6),  */
+/* p from the Tait law with exponent 7.0, mass rho V0 (else rho0, p 0, mass 0);    */
+/* then :196-215, the fluid pull-back.                                              */
+void orc_wavemaker_update(orc_ctx* c, double xp, double up) {
+  const orc_params* P = &c->P;
+  const int N = c->n_fluid;
+  const double vol0 = c->dx * c->dy;
+  double left = 0.0;
+  for (int k = 0; k < N; ++k) {
+    double d = fabs(xp - c->gx[k]);
+    if (d <= 3 * c->dx && c->gy[k] > left) left = c->gy[k];
+  }
+  left = left + 0.5 * c->dx;
+  for (int q = 0; q < c->n_wavemaker; ++q) {
+    int i = N + q;
+    c->gx[i] = xp - (c->gbtype[i] - 1) * c->dx - c->dx / 2.0;
+    c->gu[i] = up;
+    if (c->gy[i] > left) {
+      c->grho[i] = P->rho0; c->gp[i] = 0.0; c->gmass[i] = 0.0;
+    } else {
+      c->grho[i] = P->rho0 * pow(-6.0 * WAVEMAKER_G * (c->gy[i] - left) / P->c0sq + 1.0, (double)(1.0f / 6.0f));
+      c->gp[i] = P->c0sq * P->rho0 * (pow(c->grho[i] / P->rho0, 7.0) - 1.0) / 7.0;
+      c->gmass[i] = c->grho[i] * vol0;
+    }
+  }
+  orc_wavemaker_pull_back(c, xp, up);
+}
+
 static void append_walls(orc_ctx* c) {
   const orc_params* P = &c->P;
-  int N = c->n_fluid;
+  int N = c->n_fluid + c->n_wavemaker;
   int nw = wall_positions(P->box_x, P->box_y, c->dx, c->dy, P->walls, NULL, NULL);
   if (N + nw > c->cap_global) { fprintf(stderr, "orc: wall capacity\n"); abort(); }
   wall_positions(P->box_x, P->box_y, c->dx, c->dy, P->walls, c->gx + N, c->gy + N);
@@ -361,7 +429,7 @@ static void append_walls(orc_ctx* c) {
     c->gu[m] = 0.0; c->gv[m] = 0.0; c->grho[m] = P->rho0; c->gmass[m] = 0.0;
     c->gtype[m] = -3; c->gh[m] = c->h; c->gp[m] = 0.0; c->gc[m] = sqrt(P->c0sq); c->ge[m] = c->e0;
   }
-  c->n_ghost = nw;
+  c->n_ghost = c->n_wavemaker + nw;
 }
 
 int32_t orc_wall_positions(const orc_ctx* c, double* x, double* y) {
@@ -403,6 +471,8 @@ void orc_init_tgv(orc_ctx* c) {
     ke = ke + 0.5 * c->gmass[i] * (c->gu[i] * c->gu[i] + c->gv[i] * c->gv[i]);
   c->ke0 = ke;
   c->n_ghost = 0;
+  c->n_wavemaker = 0;
+  if (P->wavemaker_layers > 0) append_wavemaker(c);
   if (P->walls > 0) append_walls(c);
 }
 
@@ -420,6 +490,8 @@ int32_t orc_set_fluid_state(orc_ctx* c, int32_t n, const double* x, const double
   double ke = 0.0;
   for (int i = 0; i < n; ++i) ke = ke + 0.5 * c->gmass[i] * (c->gu[i] * c->gu[i] + c->gv[i] * c->gv[i]);
   c->ke0 = ke;
+  c->n_wavemaker = 0;
+  if (c->P.wavemaker_layers > 0) append_wavemaker(c);
   if (c->P.walls > 0) append_walls(c);
   return 0;
 }

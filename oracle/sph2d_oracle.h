@@ -51,6 +51,9 @@ typedef struct {
   /* DistributePeriodBoundaryParticleOrNot = 0); 0 = periodic box                   */
   int32_t walls;
   double gravity;            /* g (9.81) when Gravity_Switch is 'on', else 0  (:32-33) */
+  /* piston wavemaker particles (fix_wave_maker_or_not, wave_maker_layer :213-214,  */
+  /* PistonType): columns of type-130 particles left of x = 0, before the walls    */
+  int32_t wavemaker_layers;
 } orc_params;
 
 enum { ORC_SOLVER_MODIFIED_LINEAR_RIEMANN = 0,   /* Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90 */
@@ -112,6 +115,7 @@ void orc_grid_postprocess(orc_ctx* c, int32_t i_time_step, int32_t nex, int32_t 
 void orc_sampling_points(orc_ctx* c, int32_t n, const double* px, const double* py, double gravity, double* out);
 void orc_wave_probes(orc_ctx* c, int32_t n_probes, const double* probe_x, double* out);
 void orc_wavemaker_pull_back(orc_ctx* c, double xp, double up);
+void orc_wavemaker_update(orc_ctx* c, double xp, double up);
 
 /* Sampling_subroutine.f90:340-461. out[0]=KE, [1]=max|v|, [2..4]=u Linf,L1,L2,
  * [5..7]=v, [8..10]=p/(rho0 U^2), [11..13]=|v|. */

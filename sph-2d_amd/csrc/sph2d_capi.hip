@@ -44,6 +44,7 @@ struct sph2d_ctx {
   hipStream_t stream = nullptr;
   int cap_fluid = 0, cap_ghost = 0, cap_global = 0, cap_local = 0, n_cells = 0;
   int n_fluid = 0;
+  int n_wm = 0;                     // wavemaker particles (after the fluid)
   double ke0 = 0.0;
   bool have_state = false;
   bool debug = false;
@@ -125,6 +126,7 @@ void make_devparams(const sph2d_params& p, DevParams& P) {
   }
   P.mls_alpha = p.mls_alpha;
   P.walls = p.walls;
+  P.wm_layers = p.wavemaker_layers;
   P.g = p.gravity;
   P.wall_h = (double)1.2f * (p.h + p.h) / 2.0;     // Assign_boundary_particles.f90:182
   P.wall_kfactor = 1.0 / ((double)3.14159265358f * (P.wall_h * P.wall_h) * (double)c4);
@@ -186,6 +188,9 @@ int check_params(const sph2d_params* p) {
     return fail(SPH2D_ERR_UNSUPPORTED, "only MLS order 6 (the reference default) is built");
   if (p->walls < 0 || p->walls > 64) return fail(SPH2D_ERR_ARGUMENT, "walls must be in [0, 64]");
   if (!std::isfinite(p->gravity) || p->gravity < 0) return fail(SPH2D_ERR_ARGUMENT, "gravity must be >= 0");
+  if (p->wavemaker_layers < 0 || p->wavemaker_layers > 64)
+    return fail(SPH2D_ERR_ARGUMENT, "wavemaker_layers must be in [0, 64]");
+  if (p->wavemaker_layers && !p->walls) return fail(SPH2D_ERR_ARGUMENT, "wavemaker particles need fixed walls");
   if (p->periodic && p->walls) return fail(SPH2D_ERR_ARGUMENT, "fixed walls need periodic = 0");
   if (!p->periodic && !p->walls)
     return fail(SPH2D_ERR_UNSUPPORTED, "the box is periodic or has fixed ghost walls (walls > 0)");
@@ -284,8 +289,11 @@ int sph2d_default_params(sph2d_params* p) {
   p->water_depth = p->box_y;                       // water_depth = domain_size_y
   p->walls = 0;                                    // DistributeFixedBoundaryParticleOrNot = 0
   p->gravity = 0.0;                                // Gravity_Switch = 'off'
+  p->wavemaker_layers = 0;                         // make_wave_or_not = 0
   return SPH2D_OK;
 }
+
+int wavemaker_count(const sph2d_params& p);
 
 int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx** out) {
   if (!out) return fail(SPH2D_ERR_ARGUMENT, "out is NULL");
@@ -306,7 +314,7 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   c->cap_ghost = (int)std::max(16384L, 3 * ghost_bound);
   if (p->walls) {
     const int nw = wall_layout(p->box_x, p->box_y, p->dx, p->box_y / p->ny, p->walls, nullptr, nullptr, nullptr);
-    c->cap_ghost = std::max(c->cap_ghost, nw + 64);
+    c->cap_ghost = std::max(c->cap_ghost, wavemaker_count(*p) + nw + 64);
   }
   c->cap_global = c->cap_fluid + c->cap_ghost;
   c->cap_local = c->cap_global;
@@ -385,21 +393,79 @@ static int set_nfluid(sph2d_ctx* c, int n) {
   return SPH2D_OK;
 }
 
+// Distribute_Initial_Particle.f90:254-311 (PistonType): wave_maker_layer columns at
+// x = -(j-1) dx - dx/2 for the rows y = (i-1/2) dy <= boundary_size_y, rows outer,
+// with rho = rho0 (-6.0 g (y - Ly)/c0^2 + 1.0)**(1.0/6.0) (REAL(4) literals, g = 9.81),
+// p from the Tait law with exponent 7.0, at rest, type 130, mass V0 rho(q) with q the
+// particle's own index into the fluid's densities (:300-310); the layer goes to btype.
+static void wavemaker_layout(const sph2d_params& p, const double* fluid_rho, std::vector<double2>& pos,
+                             std::vector<double>& rho, std::vector<double>& pr, std::vector<double>& mass,
+                             std::vector<int>& layer) {
+  const double dy = p.box_y / p.ny;
+  for (int i = 1; i <= p.ny + 200; ++i)
+    for (int j = 1; j <= p.wavemaker_layers; ++j) {
+      const double x = -(j - 1) * p.dx - p.dx / 2.0, y = (i - 1) * dy + dy / 2.0;
+      if (!(y <= p.box_y)) continue;
+      const double r = p.rho0 * std::pow(-6.0 * 9.81 * (y - p.box_y) / p.c0sq + 1.0, (double)(1.0f / 6.0f));
+      pos.push_back(make_double2(x, y));
+      rho.push_back(r);
+      pr.push_back(p.c0sq * p.rho0 * (std::pow(r / p.rho0, 7.0) - 1.0) / 7.0);
+      layer.push_back(j);
+    }
+  for (size_t q = 0; q < pos.size(); ++q) mass.push_back(p.dx * dy * fluid_rho[q]);
+}
+
+// wave_maker_particle_number of the layout below
+int wavemaker_count(const sph2d_params& p) {
+  if (p.wavemaker_layers <= 0) return 0;
+  const double dy = p.box_y / p.ny;
+  int rows = 0;
+  for (int i = 1; i <= p.ny + 200; ++i) rows += ((i - 1) * dy + dy / 2.0) <= p.box_y;
+  return rows * p.wavemaker_layers;
+}
+
 // Distribute_Initial_Particle.f90:483-495: the fixed ghost layers after the n fluid
-// particles (type -3, at rest, rho0, p = 0, c0, e0; mass water_particle_mass_0, which
-// the TGV branch leaves 0), counted as the ghosts of every step.
-static int upload_walls(sph2d_ctx* c, int n) {
+// particles and the wavemaker particles (type -3, at rest, rho0, p = 0, c0, e0; mass
+// water_particle_mass_0, which the TGV branch leaves 0); both counted as the ghosts
+// of every step. fluid_rho: the host densities of the fluid (NULL: read back).
+static int upload_walls(sph2d_ctx* c, int n, const double* fluid_rho) {
   const sph2d_params& p = c->prm;
   const double dy = p.box_y / p.ny;
+  Buffers& B = c->B;
+  const int nwm = wavemaker_count(p);
+  if (nwm > 0) {
+    std::vector<double> frho(nwm);
+    if (nwm > n) return fail(SPH2D_ERR_ARGUMENT, "fewer fluid particles than wavemaker particles");
+    if (fluid_rho) std::copy(fluid_rho, fluid_rho + nwm, frho.begin());
+    else {
+      HIP_OR_FAIL(hipMemcpyAsync(frho.data(), B.g_rho, nwm * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+    }
+    std::vector<double2> pos, vel(nwm, make_double2(0.0, 0.0));
+    std::vector<double> rho, pr, mass, cs(nwm, std::sqrt(p.c0sq)), e(nwm, p.c0sq / (7.0 - 1.0));
+    std::vector<int> layer, t(nwm, 130);
+    wavemaker_layout(p, frho.data(), pos, rho, pr, mass, layer);
+    HIP_OR_FAIL(hipMemcpyAsync(B.g_pos + n, pos.data(), nwm * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(B.g_vel + n, vel.data(), nwm * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(B.g_rho + n, rho.data(), nwm * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(B.g_mass + n, mass.data(), nwm * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(B.g_p + n, pr.data(), nwm * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(B.g_c + n, cs.data(), nwm * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(B.g_e + n, e.data(), nwm * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(B.g_type + n, t.data(), nwm * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(B.g_btype + n, layer.data(), nwm * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  }
+  c->n_wm = nwm;
+  n += nwm;
   const int nw = wall_layout(p.box_x, p.box_y, p.dx, dy, p.walls, nullptr, nullptr, nullptr);
-  if (nw > c->cap_ghost) return fail(SPH2D_ERR_CAPACITY, "wall layers exceed the ghost capacity");
+  if (nwm + nw > c->cap_ghost) return fail(SPH2D_ERR_CAPACITY, "wall layers exceed the ghost capacity");
   std::vector<double> x(nw), y(nw), zero(nw, 0.0), rho(nw, p.rho0), cs(nw, std::sqrt(p.c0sq)),
       e(nw, p.c0sq / (7.0 - 1.0));
   std::vector<int> bt(nw), t(nw, -3);
   wall_layout(p.box_x, p.box_y, p.dx, dy, p.walls, x.data(), y.data(), bt.data());
   std::vector<double2> pos(nw), vel(nw, make_double2(0.0, 0.0));
   for (int q = 0; q < nw; ++q) pos[q] = make_double2(x[q], y[q]);
-  Buffers& B = c->B;
   HIP_OR_FAIL(hipMemcpyAsync(B.g_pos + n, pos.data(), nw * sizeof(double2), hipMemcpyHostToDevice, c->stream));
   HIP_OR_FAIL(hipMemcpyAsync(B.g_vel + n, vel.data(), nw * sizeof(double2), hipMemcpyHostToDevice, c->stream));
   HIP_OR_FAIL(hipMemcpyAsync(B.g_rho + n, rho.data(), nw * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -409,7 +475,7 @@ static int upload_walls(sph2d_ctx* c, int n) {
   HIP_OR_FAIL(hipMemcpyAsync(B.g_e + n, e.data(), nw * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_OR_FAIL(hipMemcpyAsync(B.g_type + n, t.data(), nw * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIP_OR_FAIL(hipMemcpyAsync(B.g_btype + n, bt.data(), nw * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  launch_fill_i32(c->stream, B.counters + C_NGHOST, 1, nw);
+  launch_fill_i32(c->stream, B.counters + C_NGHOST, 1, nwm + nw);
   HIP_OR_FAIL(hipGetLastError());
   HIP_OR_FAIL(hipStreamSynchronize(c->stream));
   return SPH2D_OK;
@@ -424,7 +490,7 @@ int sph2d_init_tgv(sph2d_ctx* c) {
   launch_init_tgv(c->stream, c->P, c->B, c->prm.nx, c->prm.ny);
   HIP_OR_FAIL(hipGetLastError());
   if (c->prm.walls) {
-    rc = upload_walls(c, n);
+    rc = upload_walls(c, n, nullptr);
     if (rc) return rc;
   }
   rc = compute_ke0(c);
@@ -470,7 +536,7 @@ int sph2d_upload_state(sph2d_ctx* c, int32_t n, const double* pos, const double*
   if (rc) return rc;
   if (c->prm.walls) {
     if (n + c->cap_ghost > c->cap_global) return fail(SPH2D_ERR_CAPACITY, "no room for the wall layers");
-    rc = upload_walls(c, n);
+    rc = upload_walls(c, n, rho);
     if (rc) return rc;
   }
   rc = compute_ke0(c);
@@ -1143,6 +1209,22 @@ int sph2d_output_sampling_lines(sph2d_ctx* c, int32_t it, const char* dir, int32
 }
 
 // ---- piston wavemaker -------------------------------------------------------------
+int sph2d_wavemaker_update(sph2d_ctx* c, double position, double velocity) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (!c->prm.wavemaker_layers) return fail(SPH2D_ERR_STATE, "the context has no wavemaker particles");
+  if (!std::isfinite(position) || !std::isfinite(velocity))
+    return fail(SPH2D_ERR_ARGUMENT, "paddle position and velocity must be finite");
+  rc = probe_space(c, 1);
+  if (rc) return rc;
+  auto* bits = reinterpret_cast<unsigned long long*>(c->probe_buf);
+  launch_fill_i32(c->stream, reinterpret_cast<int*>(bits), 2, 0);
+  launch_wavemaker_place(c->stream, c->P, c->B, c->cap_fluid, c->n_wm, position, velocity, bits);
+  launch_wavemaker_pull_back(c->stream, c->P, c->B, c->cap_fluid, position, velocity);
+  HIP_OR_FAIL(hipGetLastError());
+  return SPH2D_OK;
+}
+
 int sph2d_wavemaker_pull_back(sph2d_ctx* c, double position, double velocity) {
   int rc = need_state(c);
   if (rc) return rc;

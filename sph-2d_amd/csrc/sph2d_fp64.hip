@@ -1087,6 +1087,56 @@ void launch_wavemaker_pull_back(hipStream_t s, const DevParams& P, Buffers& B, i
   hipLaunchKernelGGL(k_wavemaker_pull_back, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, s, P, B, xp, up);
 }
 
+// Wavemaker_Subroutine.f90:140-164: the largest fluid y within kernel_scale dx of
+// the paddle (from 0), as a bit-pattern maximum (exact, order-independent).
+__global__ void __launch_bounds__(kBlock) k_wavemaker_height(DevParams P, Buffers B, double xp,
+                                                             unsigned long long* bits) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0;
+  if (g < B.counters[C_NFLUID]) {
+    const double2 x = B.g_pos[g];
+    if (fabs(xp - x.x) <= 3 * P.dx && x.y > e) e = x.y;
+  }
+  unsigned long long eb = __double_as_longlong(e);
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(eb, off, 64);
+    eb = o > eb ? o : eb;
+  }
+  if ((threadIdx.x & 63) == 0 && eb != 0ull) atomicMax(bits, eb);
+}
+
+// :171-193: wavemaker particle q (gid N + q, layer in g_btype) at x_p - (layer-1) dx
+// - dx/2 with the paddle's velocity; below the height (+ dx/2): rho0 (-6.0 g (y - H)
+// / c0^2 + 1.0)**(1.0/6.0) with REAL(4) literals and g = 9.81 (information_module.f90:32),
+// p = c0^2 rho0 ((rho/rho0)**7.0 - 1)/7, mass rho V0; above it rho0, p 0, mass 0.
+__global__ void k_wavemaker_place(DevParams P, Buffers B, int n_wm, double xp, double up,
+                                  const unsigned long long* bits) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_wm) return;
+  const int i = B.counters[C_NFLUID] + q;
+  const double left = __longlong_as_double(*bits) + 0.5 * P.dx;
+  double2 x = B.g_pos[i];
+  x.x = xp - (B.g_btype[i] - 1) * P.dx - P.dx / 2.0;
+  B.g_pos[i] = x;
+  const double2 v = B.g_vel[i];
+  B.g_vel[i] = make_double2(up, v.y);
+  if (x.y > left) {
+    B.g_rho[i] = P.rho0; B.g_p[i] = 0.0; B.g_mass[i] = 0.0;
+  } else {
+    const double rho = P.rho0 * pow(-6.0 * 9.81 * (x.y - left) / P.c0sq + 1.0, (double)(1.0f / 6.0f));
+    B.g_rho[i] = rho;
+    B.g_p[i] = P.c0sq * P.rho0 * (pow(rho / P.rho0, 7.0) - 1.0) / 7.0;
+    B.g_mass[i] = rho * P.vol0;
+  }
+}
+
+void launch_wavemaker_place(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int n_wm, double xp,
+                            double up, unsigned long long* height_bits) {
+  hipLaunchKernelGGL(k_wavemaker_height, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, s, P, B, xp, height_bits);
+  if (n_wm > 0)
+    hipLaunchKernelGGL(k_wavemaker_place, dim3(grid_for(n_wm)), dim3(kBlock), 0, s, P, B, n_wm, xp, up, height_bits);
+}
+
 // Assign_boundary_particles.f90:157-299 before an evaluation's passes: each fixed
 // ghost takes the pressure and velocity of its non-ghost candidates (the 3x3 cells)
 // within 3 hbar, hbar = 1.2 (h_i + h_j) / 2 with 1.2 a REAL(4) literal, by the

@@ -33,6 +33,7 @@ struct DevParams {
   int chain_nx, chain_ny;
   double alpha, beta, fai, delta, delta_alpha, epsilon_xsph;
   int walls;                             // fixed ghost layers (0: periodic box)
+  int wm_layers;                         // wavemaker particle columns (0: none)
   double g;                              // gravity (0: off)
   double wall_h, wall_kfactor, vol0;     // hbar = 1.2 (h + h) / 2 and its kernel factor; dx dy
   int damping;                           // SPH2D_DAMPING_*
@@ -185,6 +186,10 @@ void launch_batch_step(hipStream_t s, const BatchDev& D, int mixed, int mls, int
 void launch_batch_eval(hipStream_t s, const BatchDev& D, int mixed, int solver);
 void launch_batch_eval32(hipStream_t s, const BatchDev& D, int pass, int solver);
 void launch_wavemaker_pull_back(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, double xp, double up);
+// Wavemaker_Subroutine.f90:140-193: water height next to the paddle (bit pattern at
+// height_bits, zeroed by the caller), then the n_wm wavemaker particles after the fluid
+void launch_wavemaker_place(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int n_wm, double xp,
+                            double up, unsigned long long* height_bits);
 // Sampling_subroutine.f90 at sampling points (sph2d_fp64.hip)
 struct ProbeArgs {
   int n;

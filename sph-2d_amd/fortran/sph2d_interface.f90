@@ -25,6 +25,7 @@ module sph2d_interface
     real(c_double) :: damping_length, damping_amplitude, water_depth
     integer(c_int32_t) :: walls
     real(c_double) :: gravity
+    integer(c_int32_t) :: wavemaker_layers
   end type sph2d_params
 
   interface
@@ -269,6 +270,14 @@ module sph2d_interface
       character(kind=c_char), intent(in) :: path(*)
       integer(c_int32_t), value :: n
       real(c_double), intent(in) :: values(n)
+      integer(c_int) :: rc
+    end function
+
+    ! Wavemaker_Subroutine.f90:140-215 (paddle position and velocity from sph2d_wavemaker_motion)
+    function sph2d_wavemaker_update(ctx, position, velocity) bind(C, name="sph2d_wavemaker_update") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: position, velocity
       integer(c_int) :: rc
     end function
 

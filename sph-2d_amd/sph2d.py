@@ -64,6 +64,7 @@ class Params(ctypes.Structure):
         ("damping", ctypes.c_int32), ("damping_length", ctypes.c_double),
         ("damping_amplitude", ctypes.c_double), ("water_depth", ctypes.c_double),
         ("walls", ctypes.c_int32), ("gravity", ctypes.c_double),
+        ("wavemaker_layers", ctypes.c_int32),
     ]
 
 
@@ -123,7 +124,7 @@ EXPORTS = (
     "sph2d_write_grid_binary", "sph2d_sampling", "sph2d_wave_probes", "sph2d_output_sampling_lines",
     "sph2d_format_list_directed", "sph2d_write_list_directed", "sph2d_write_f20_8", "sph2d_kalman_filter",
     "sph2d_sampling_line_points", "sph2d_write_sampling_line_positions", "sph2d_write_sampling_line_results",
-    "sph2d_wavemaker_motion", "sph2d_wavemaker_pull_back", "sph2d_batch_create", "sph2d_batch_run",
+    "sph2d_wavemaker_motion", "sph2d_wavemaker_pull_back", "sph2d_wavemaker_update", "sph2d_batch_create", "sph2d_batch_run",
     "sph2d_batch_info", "sph2d_batch_destroy",
 )
 
@@ -205,6 +206,7 @@ def lib():
                                                   P(Scales)],
             "sph2d_wavemaker_motion": [P(Wavemaker), i32, dp, dp, dp],
             "sph2d_wavemaker_pull_back": [vp, ctypes.c_double, ctypes.c_double],
+            "sph2d_wavemaker_update": [vp, ctypes.c_double, ctypes.c_double],
             "sph2d_batch_create": [i32, P(vp), P(vp)],
             "sph2d_batch_run": [vp, i32, i32],
             "sph2d_batch_info": [vp, ip, ip],
@@ -413,6 +415,10 @@ class Simulation:
         l = _f64(np.asarray(last, dtype=np.float64).reshape(-1))
         _check(lib().sph2d_output_sampling_lines(self._c, i_time_step, os.fspath(dir_path).encode(), len(f) // 2,
                                                  _dp(f), _dp(l), n_points))
+
+    def wavemaker_update(self, position: float, velocity: float):
+        """Wavemaker_Subroutine.f90:140-215 before a step: place the wavemaker particles, pull back."""
+        _check(lib().sph2d_wavemaker_update(self._c, position, velocity))
 
     def wavemaker_pull_back(self, position: float, velocity: float):
         """Wavemaker_Subroutine.f90:196-215: mirror fluid particles behind the paddle."""

@@ -566,3 +566,46 @@ def test_dam_break_with_walls_gravity_damping_and_probes():
     assert abs(front - rf) < 1e-12 and np.max(np.abs(el - rel)) < 1e-12
     assert el[2] == 0.0 and el[0] > 0.45
     sim.close()
+
+
+def test_piston_wave_tank_parity():
+    """A small piston wave tank: still water of depth 0.5 in the walled box under
+    gravity, three columns of wavemaker particles driven by the Airy paddle law
+    (sph2d_wavemaker_motion, then sph2d_wavemaker_update before each step), the
+    cosine end damping and the wave gauges; 30 predictor-corrector steps against
+    the oracle driven the same way."""
+    say("test_piston_wave_tank_parity")
+    from test_oracle_physics import hydrostatic_tank
+    n, H, g = 40, 0.5, 9.81
+    st = hydrostatic_tank(n, H, g)
+    M = len(st[0])
+    om = 4 * np.pi
+    k = om * om / g
+    for _ in range(50):                                  # omega^2 = g k tanh(k d)
+        k = k - (g * k * np.tanh(k * H) - om * om) / (g * np.tanh(k * H) + g * k * H / np.cosh(k * H) ** 2)
+    kw = dict(integrator=2, walls=4, gravity=g, wavemaker_layers=3, damping=sph2d.DAMPING_COSINE,
+              damping_length=0.2, water_depth=H)
+    prm = sph2d.tgv_params(n, periodic=0, dt=1e-4 * 100 / n * 0.5, **kw)
+    sim = sph2d.Simulation(prm, capacity=n * n)
+    sim.upload_state(*st)
+    o = Oracle(nx=n, ny=n, dt=prm.dt, **kw)
+    o.set_fluid_state(*st)
+    w = sph2d.Wavemaker(wave_type=sph2d.WAVE_AIRY, soft_steps=1, standing_steps=0, dt=prm.dt, g=g, water_depth=H,
+                        wave_height=0.02, wave_number=k, wave_frequency=om, phase=0.0)
+    for it in range(1, 31):
+        xp, up, _ = sph2d.wavemaker_motion(w, it)
+        sim.wavemaker_update(xp, up)
+        o.wavemaker_update(xp, up)
+        sim.time_step(it)
+        o.time_step(it)
+    assert up != 0.0 and xp != 0.0
+    gs = sim.download_state()
+    for f, of in (("x", "g_x"), ("y", "g_y"), ("u", "g_u"), ("v", "g_v"), ("rho", "g_rho"), ("p", "g_p")):
+        r = o.f64(of)[:M]
+        e = float(np.max(np.abs(gs[f] - r))) if f in ("x", "y") else rel_err(gs[f], r)
+        say(f, e)
+        assert e < (1e-12 if f in ("x", "y") else 1e-10), (f, e)
+    front, el = sim.wave_probes([0.25, 0.5])
+    rf, rel = o.wave_probes([0.25, 0.5])
+    assert abs(front - rf) < 1e-12 and np.max(np.abs(el - rel)) < 1e-12
+    sim.close()

@@ -679,3 +679,39 @@ def test_hydrostatic_tank_stays_at_rest():
         else:
             assert vmax > 0.05, vmax                     # the pressure gradient pushes the column up
         o.close()
+
+
+def test_wavemaker_particles_layout_and_placement():
+    """Distribute_Initial_Particle.f90:254-311 and Wavemaker_Subroutine.f90:140-193:
+    wave_maker_layer columns of type-130 particles over the box height, right after
+    the fluid; an update puts column j at x_p - (j - 1) dx - dx/2 with the paddle's
+    velocity, and below the water height next to the paddle (largest fluid y within
+    3 dx, plus dx/2) the density rho0 (1 - 6 g (y - height)/c0^2)^(1/6), which grows
+    downwards, with mass rho V0; above it rho0 and mass 0."""
+    n, L = 30, 3
+    st = hydrostatic_tank(n, 0.5)
+    o = po.Oracle(nx=n, ny=n, walls=4, wavemaker_layers=L, gravity=9.81)
+    o.set_fluid_state(*st)
+    M = len(st[0])
+    W = L * n
+    t = o.i32("g_type")
+    assert np.all(t[M:M + W] == 130) and np.all(t[M + W:M + W + 544] == -3)
+    layer = o.i32("g_btype")[M:M + W]
+    xp, up = 0.013, 0.07
+    o.wavemaker_update(xp, up)
+    x, y = o.f64("g_x")[M:M + W], o.f64("g_y")[M:M + W]
+    dx = 1.0 / n
+    np.testing.assert_allclose(x, xp - (layer - 1) * dx - dx / 2, rtol=0, atol=1e-15)
+    assert np.all(o.f64("g_u")[M:M + W] == up)
+    fx, fy = o.f64("g_x")[:M], o.f64("g_y")[:M]
+    height = fy[np.abs(xp - fx) <= 3 * dx].max() + 0.5 * dx
+    rho, mass = o.f64("g_rho")[M:M + W], o.f64("g_mass")[M:M + W]
+    below = y <= height
+    assert below.sum() == L * int(round(0.5 * n)) and np.all(rho[~below] == 1000.0) and np.all(mass[~below] == 0)
+    order = np.argsort(y[below])
+    assert np.all(np.diff(rho[below][order]) <= 0)                   # denser deeper down
+    np.testing.assert_allclose(mass[below], rho[below] * dx * dx, rtol=1e-15)
+    np.testing.assert_allclose(rho[below], 1000.0 * (1 - 6 * 9.81 * (y[below] - height) / 100.0) ** (1 / 6.0),
+                               rtol=1e-7)
+    assert np.all(o.f64("g_x")[:M] >= xp)                             # the pull-back ran too
+    o.close()
