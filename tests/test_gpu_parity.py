@@ -14,6 +14,7 @@ import pytest
 
 import sph2d
 from oracle.pyoracle import Oracle
+from cases import hydrostatic_tank
 
 pytestmark = pytest.mark.gpu
 
@@ -507,7 +508,6 @@ def test_hydrostatic_tank_with_gravity_parity():
     """Gravity on the fluid and the hydrostatic term of the wall pressure: a tank at
     rest in the walled box, 21 predictor-corrector steps against the oracle."""
     say("test_hydrostatic_tank_with_gravity_parity")
-    from test_oracle_physics import hydrostatic_tank
     n = 30
     st = hydrostatic_tank(n)
     M = len(st[0])
@@ -575,7 +575,6 @@ def test_piston_wave_tank_parity():
     cosine end damping and the wave gauges; 30 predictor-corrector steps against
     the oracle driven the same way."""
     say("test_piston_wave_tank_parity")
-    from test_oracle_physics import hydrostatic_tank
     n, H, g = 40, 0.5, 9.81
     st = hydrostatic_tank(n, H, g)
     M = len(st[0])
