@@ -113,9 +113,13 @@ def batched_leg(sph2d, instances: int, steps: int, warmup: int) -> dict:
     sims = make()
     batch = sph2d.Batch(sims)
     batch.run(1, warmup)
+    # the autotuner times each launch configuration on 20 real steps (one MLS step each)
+    trial = 20
+    tune_ms, chosen = batch.autotune(1 + warmup, trial)
+    first = 1 + warmup + 4 * (trial + 1)
     hip.hipDeviceSynchronize()
     t0 = time.perf_counter()
-    batch.run(1 + warmup, steps)
+    batch.run(first, steps)
     hip.hipDeviceSynchronize()
     t_batch = time.perf_counter() - t0
     nodes, replays = batch.info()
@@ -125,11 +129,11 @@ def batched_leg(sph2d, instances: int, steps: int, warmup: int) -> dict:
         s.close()
     sims = make()
     for s in sims:
-        s.run(1, warmup)
+        s.run(1, first - 1)
     hip.hipDeviceSynchronize()
     t0 = time.perf_counter()
     for s in sims:
-        s.run(1 + warmup, steps)
+        s.run(first, steps)
         hip.hipDeviceSynchronize()
     t_seq = time.perf_counter() - t0
     same = all(abs(s.sample()[0] - k) == 0.0 for s, k in zip(sims, ke))
@@ -142,6 +146,9 @@ def batched_leg(sph2d, instances: int, steps: int, warmup: int) -> dict:
             "particle_steps_per_s": total * steps / t_batch, "ms_per_step": 1000.0 * t_batch / steps,
             "sequential_particle_steps_per_s": total * steps / t_seq,
             "speedup_vs_sequential": t_seq / t_batch, "graph_nodes": nodes, "graph_replays": replays,
+            "autotune": {"ms_per_step": {name: float(v) for name, v in zip(sph2d.Batch.CONFIGS, tune_ms)},
+                         "chosen": sph2d.Batch.CONFIGS[chosen], "trial_steps": trial},
+            "timed_steps_from": first,
             "ke_identical_to_sequential": same,
             "timing": "host wall clock with device synchronisation, launches included"}
 

@@ -269,6 +269,14 @@ int sph2d_batch_create(int32_t n, sph2d_ctx* const* ctxs, sph2d_batch** out);
 int sph2d_batch_run(sph2d_batch* batch, int32_t first_step, int32_t nsteps);
 /* Nodes of the captured step graph (0 before the first replayable step) and replays so far. */
 int sph2d_batch_info(sph2d_batch* batch, int32_t* graph_nodes, int32_t* graph_replays);
+/* Autotuner of the fused launches: times trial_steps real steps (after one untimed
+ * step that captures the graphs) of each of 4 launch configurations and keeps the
+ * fastest: instance order in the block tables (0 as given, 1 by size, largest
+ * first) x block order of the neighbour passes (0 XCD-aware, 2 linear).
+ * Advances every instance by 4 (trial_steps + 1) steps from first_step;
+ * trial_ms (4, may be NULL) receives ms per step, chosen the configuration. */
+int sph2d_batch_autotune(sph2d_batch* batch, int32_t first_step, int32_t trial_steps, double* trial_ms,
+                         int32_t* chosen);
 int sph2d_batch_destroy(sph2d_batch* batch);
 
 /* --- piston wavemaker (Wavemaker_Subroutine.f90, PistonType) ------------------- */

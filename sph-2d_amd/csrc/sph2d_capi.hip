@@ -2,6 +2,7 @@
 // (the restated time_iteration.f90), uploads/downloads, diagnostics.
 #include "sph2d_internal.h"
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1605,11 +1606,20 @@ bool fusable(const std::vector<sph2d_ctx*>& v) {
   return true;
 }
 
-int batch_build_dev(sph2d_batch* b) {
+// The instance tables of the fused launches, instances in the given order (or by
+// local capacity, largest first, when by_size); xcd: XCD-aware block order of the
+// neighbour passes. The device tables are allocated on the first call.
+int batch_build_dev(sph2d_batch* b, bool by_size = false, int xcd = 1) {
   const int n = (int)b->ctx.size();
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  if (by_size)
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int z) { return b->ctx[a]->cap_local > b->ctx[z]->cap_local; });
   std::vector<Inst> inst(n);
-  for (int i = 0; i < n; ++i) {
-    const sph2d_ctx* c = b->ctx[i];
+  for (int k = 0; k < n; ++k) {
+    const int i = k;
+    const sph2d_ctx* c = b->ctx[order[k]];
     inst[i].P = c->P;
     inst[i].P.write_debug = 0;
     inst[i].B = c->B;
@@ -1631,13 +1641,17 @@ int batch_build_dev(sph2d_batch* b) {
   }
   std::vector<unsigned char> q((size_t)n * q32_bytes());
   for (int i = 0; i < n; ++i) q32_fill(inst[i].P, q.data() + (size_t)i * q32_bytes());
-  void *di = nullptr, *ds = nullptr, *dq = nullptr;
-  HIP_OR_FAIL(hipMalloc(&di, sizeof(Inst) * n));
-  b->dev_allocs.push_back(di);
-  HIP_OR_FAIL(hipMalloc(&ds, sizeof(int) * starts.size()));
-  b->dev_allocs.push_back(ds);
-  HIP_OR_FAIL(hipMalloc(&dq, q.size()));
-  b->dev_allocs.push_back(dq);
+  if (b->dev_allocs.empty()) {
+    void *di = nullptr, *ds = nullptr, *dq = nullptr;
+    HIP_OR_FAIL(hipMalloc(&di, sizeof(Inst) * n));
+    b->dev_allocs.push_back(di);
+    HIP_OR_FAIL(hipMalloc(&ds, sizeof(int) * starts.size()));
+    b->dev_allocs.push_back(ds);
+    HIP_OR_FAIL(hipMalloc(&dq, q.size()));
+    b->dev_allocs.push_back(dq);
+  }
+  void *di = b->dev_allocs[0], *ds = b->dev_allocs[1], *dq = b->dev_allocs[2];
+  HIP_OR_FAIL(hipDeviceSynchronize());             // no launch may still read the old tables
   HIP_OR_FAIL(hipMemcpy(di, inst.data(), sizeof(Inst) * n, hipMemcpyHostToDevice));
   HIP_OR_FAIL(hipMemcpy(ds, starts.data(), sizeof(int) * starts.size(), hipMemcpyHostToDevice));
   HIP_OR_FAIL(hipMemcpy(dq, q.data(), q.size(), hipMemcpyHostToDevice));
@@ -1645,7 +1659,18 @@ int batch_build_dev(sph2d_batch* b) {
   b->dev.n = n;
   for (int cls = 0; cls < BC_COUNT; ++cls) b->dev.start[cls] = static_cast<const int*>(ds) + (size_t)cls * (n + 1);
   b->dev.q32 = dq;
+  b->dev.xcd = xcd;
   return SPH2D_OK;
+}
+
+void batch_drop_graphs(sph2d_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->exec_mls) (void)hipGraphExecDestroy(b->exec_mls);
+  if (b->graph_mls) (void)hipGraphDestroy(b->graph_mls);
+  b->exec = b->exec_mls = nullptr;
+  b->graph = b->graph_mls = nullptr;
+  b->nodes = 0;
 }
 
 int batch_capture(sph2d_batch* b, int it, bool mls) {
@@ -1772,6 +1797,36 @@ int sph2d_batch_info(sph2d_batch* b, int32_t* graph_nodes, int32_t* graph_replay
   // per instance otherwise)
   if (graph_nodes) *graph_nodes = (int32_t)b->nodes;
   if (graph_replays) *graph_replays = b->replays;
+  return SPH2D_OK;
+}
+
+int sph2d_batch_autotune(sph2d_batch* b, int32_t first, int32_t trial_steps, double* trial_ms, int32_t* chosen) {
+  if (!b) return fail(SPH2D_ERR_ARGUMENT, "batch is NULL");
+  if (!b->fused) return fail(SPH2D_ERR_UNSUPPORTED, "the contexts of this batch do not share fused launches");
+  if (first < 1 || trial_steps < 1) return fail(SPH2D_ERR_ARGUMENT, "bad step range");
+  double best = 0.0;
+  int best_cfg = 0, it = first;
+  for (int cfg = 0; cfg < 4; ++cfg) {
+    batch_drop_graphs(b);
+    int rc = batch_build_dev(b, (cfg & 1) != 0, (cfg & 2) ? 0 : 1);
+    if (!rc) rc = sph2d_batch_run(b, it, 1);        // captures the graphs of this configuration
+    ++it;
+    if (rc) return rc;
+    HIP_OR_FAIL(hipDeviceSynchronize());
+    const auto t0 = std::chrono::steady_clock::now();
+    rc = sph2d_batch_run(b, it, trial_steps);
+    if (rc) return rc;
+    HIP_OR_FAIL(hipDeviceSynchronize());
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
+                      trial_steps;
+    it += trial_steps;
+    if (trial_ms) trial_ms[cfg] = ms;
+    if (cfg == 0 || ms < best) { best = ms; best_cfg = cfg; }
+  }
+  batch_drop_graphs(b);
+  const int rc = batch_build_dev(b, (best_cfg & 1) != 0, (best_cfg & 2) ? 0 : 1);
+  if (rc) return rc;
+  if (chosen) *chosen = best_cfg;
   return SPH2D_OK;
 }
 

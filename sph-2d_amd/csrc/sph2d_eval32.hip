@@ -476,7 +476,7 @@ void q32_fill(const DevParams& P, void* out) {
 // Batched mixed passes A and C (see launch_batch_step).
 template <bool MUSCL>
 __global__ void __launch_bounds__(kBlock) kb_eval_a32(BatchArgs A, const DevParams32* Qs) {
-  const BatchWhere w = batch_where(A, xcd_block());
+  const BatchWhere w = batch_where(A, A.xcd ? xcd_block() : (int)blockIdx.x);
   const DevParams P = A.inst[w.i].P;
   const DevParams32 Q = Qs[w.i];
   Buffers B = A.inst[w.i].B;
@@ -485,7 +485,7 @@ __global__ void __launch_bounds__(kBlock) kb_eval_a32(BatchArgs A, const DevPara
 
 template <int SOLVER>
 __global__ void __launch_bounds__(kBlock) kb_eval_c32(BatchArgs A, const DevParams32* Qs) {
-  const BatchWhere w = batch_where(A, xcd_block());
+  const BatchWhere w = batch_where(A, A.xcd ? xcd_block() : (int)blockIdx.x);
   const DevParams P = A.inst[w.i].P;
   const DevParams32 Q = Qs[w.i];
   Buffers B = A.inst[w.i].B;
@@ -494,7 +494,7 @@ __global__ void __launch_bounds__(kBlock) kb_eval_c32(BatchArgs A, const DevPara
 
 void launch_batch_eval32(hipStream_t s, const BatchDev& D, int pass, int solver) {
   const int g = D.total[BC_LOCAL];
-  const BatchArgs A{D.inst, D.start[BC_LOCAL], D.n};
+  const BatchArgs A{D.inst, D.start[BC_LOCAL], D.n, D.xcd};
   const auto* Qs = static_cast<const DevParams32*>(D.q32);
   if (pass == 0) {
     if (solver == SPH2D_SOLVER_MUSCL) hipLaunchKernelGGL(kb_eval_a32<true>, dim3(g), dim3(kBlock), 0, s, A, Qs);

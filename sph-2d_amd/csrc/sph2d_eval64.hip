@@ -468,21 +468,21 @@ void launch_eval_pass(hipStream_t s, const DevParams& P, Buffers& B, int cap_loc
 // the whole launch, then the instance of the block.
 template <bool MUSCL>
 __global__ void __launch_bounds__(kBlock) kb_eval_a(BatchArgs A) {
-  const BatchWhere w = batch_where(A, xcd_block());
+  const BatchWhere w = batch_where(A, A.xcd ? xcd_block() : (int)blockIdx.x);
   const DevParams P = A.inst[w.i].P;
   Buffers B = A.inst[w.i].B;
   k_eval_a_body<MUSCL>(P, B, w.blk, w.nblk);
 }
 
 __global__ void __launch_bounds__(kBlock) kb_eval_b1(BatchArgs A) {
-  const BatchWhere w = batch_where(A, xcd_block());
+  const BatchWhere w = batch_where(A, A.xcd ? xcd_block() : (int)blockIdx.x);
   const DevParams P = A.inst[w.i].P;
   Buffers B = A.inst[w.i].B;
   k_eval_b1_body(P, B, w.blk, w.nblk);
 }
 
 __global__ void __launch_bounds__(kBlock) kb_eval_b2(BatchArgs A) {
-  const BatchWhere w = batch_where(A, xcd_block());
+  const BatchWhere w = batch_where(A, A.xcd ? xcd_block() : (int)blockIdx.x);
   const DevParams P = A.inst[w.i].P;
   Buffers B = A.inst[w.i].B;
   k_eval_b2_body(P, B, w.blk, w.nblk);
@@ -490,7 +490,7 @@ __global__ void __launch_bounds__(kBlock) kb_eval_b2(BatchArgs A) {
 
 template <int SOLVER>
 __global__ void __launch_bounds__(kBlock) kb_eval_c(BatchArgs A) {
-  const BatchWhere w = batch_where(A, xcd_block());
+  const BatchWhere w = batch_where(A, A.xcd ? xcd_block() : (int)blockIdx.x);
   const DevParams P = A.inst[w.i].P;
   const Buffers B = A.inst[w.i].B;
   eval_c<false, SOLVER>(P, B, 1, w.blk);
@@ -499,7 +499,7 @@ __global__ void __launch_bounds__(kBlock) kb_eval_c(BatchArgs A) {
 void launch_batch_eval(hipStream_t s, const BatchDev& D, int mixed, int solver) {
   const int g = D.total[BC_LOCAL];
   if (g <= 0) return;
-  const BatchArgs A{D.inst, D.start[BC_LOCAL], D.n};
+  const BatchArgs A{D.inst, D.start[BC_LOCAL], D.n, D.xcd};
   if (mixed) launch_batch_eval32(s, D, 0, solver);
   else if (solver == SPH2D_SOLVER_MUSCL) hipLaunchKernelGGL(kb_eval_a<true>, dim3(g), dim3(kBlock), 0, s, A);
   else hipLaunchKernelGGL(kb_eval_a<false>, dim3(g), dim3(kBlock), 0, s, A);

@@ -1791,7 +1791,7 @@ __global__ void kb_stage(BatchArgs A, int kind) {
 }
 
 __global__ void __launch_bounds__(kBlock) kb_mls(BatchArgs A) {
-  const BatchWhere w = batch_where(A, xcd_block());
+  const BatchWhere w = batch_where(A, A.xcd ? xcd_block() : (int)blockIdx.x);
   const DevParams P = A.inst[w.i].P;
   Buffers B = A.inst[w.i].B;
   k_mls_body(P, B, w.blk, w.nblk);
@@ -1811,7 +1811,7 @@ __global__ void kb_correct_refresh(BatchArgs A, int shift_on) {
   k_correct_refresh_body(P, B, shift_on, w.blk, w.nblk);
 }
 
-static BatchArgs bargs(const BatchDev& D, int cls) { return BatchArgs{D.inst, D.start[cls], D.n}; }
+static BatchArgs bargs(const BatchDev& D, int cls) { return BatchArgs{D.inst, D.start[cls], D.n, D.xcd}; }
 
 void launch_batch_step(hipStream_t s, const BatchDev& D, int mixed, int mls, int shift_on, int solver) {
   const int lean = mixed && !mls;

@@ -168,10 +168,12 @@ struct BatchArgs {
   const Inst* inst;
   const int* start;                       // n + 1 first blocks of the instances (this class)
   int n;
+  int xcd;                                // neighbour passes: XCD-aware block order over the launch
 };
 struct BatchDev {
   const Inst* inst = nullptr;
   int n = 0;
+  int xcd = 1;
   const int* start[BC_COUNT] = {};
   int total[BC_COUNT] = {};
   const void* q32 = nullptr;              // n FP32 parameter sets (mixed form)

@@ -125,7 +125,7 @@ EXPORTS = (
     "sph2d_format_list_directed", "sph2d_write_list_directed", "sph2d_write_f20_8", "sph2d_kalman_filter",
     "sph2d_sampling_line_points", "sph2d_write_sampling_line_positions", "sph2d_write_sampling_line_results",
     "sph2d_wavemaker_motion", "sph2d_wavemaker_pull_back", "sph2d_wavemaker_update", "sph2d_batch_create", "sph2d_batch_run",
-    "sph2d_batch_info", "sph2d_batch_destroy",
+    "sph2d_batch_info", "sph2d_batch_destroy", "sph2d_batch_autotune",
 )
 
 _lib = None
@@ -211,6 +211,7 @@ def lib():
             "sph2d_batch_run": [vp, i32, i32],
             "sph2d_batch_info": [vp, ip, ip],
             "sph2d_batch_destroy": [vp],
+            "sph2d_batch_autotune": [vp, i32, i32, dp, ip],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -786,6 +787,16 @@ class Batch:
 
     def run(self, first_step: int, nsteps: int):
         _check(lib().sph2d_batch_run(self._b, first_step, nsteps))
+
+    CONFIGS = ("given order, XCD-aware", "by size, XCD-aware", "given order, linear", "by size, linear")
+
+    def autotune(self, first_step: int, trial_steps: int = 20):
+        """Time the 4 launch configurations (each on trial_steps real steps after one
+        capture step) and keep the fastest; returns (ms per step of each, chosen)."""
+        ms = np.zeros(4)
+        ch = ctypes.c_int32()
+        _check(lib().sph2d_batch_autotune(self._b, first_step, trial_steps, _dp(ms), ctypes.byref(ch)))
+        return ms, ch.value
 
     def info(self):
         nodes, reps = ctypes.c_int32(), ctypes.c_int32()

@@ -71,3 +71,27 @@ def test_batch_rejects_mismatched_schedules():
         sph2d.Batch([a, a])
     a.close()
     b.close()
+
+
+def test_autotuned_batch_stays_bit_identical():
+    """The autotuner re-lays the fused launches four times (instance order, block
+    order) while it advances the instances; every configuration must leave each
+    instance bit-identical to running it alone."""
+    say("test_autotuned_batch_stays_bit_identical")
+    bat = make(sph2d.FP32_MIXED, sph2d.INTEGRATOR_PC)
+    ref = make(sph2d.FP32_MIXED, sph2d.INTEGRATOR_PC)
+    batch = sph2d.Batch(bat)
+    batch.run(1, 5)
+    ms, chosen = batch.autotune(6, 4)                # steps 6..25
+    say(f"autotune ms/step {ms}, chosen {sph2d.Batch.CONFIGS[chosen]}")
+    assert np.all(ms > 0) and 0 <= chosen < 4
+    batch.run(26, 6)
+    for s in ref:
+        s.run(1, 31)
+    for k, (a, b) in enumerate(zip(bat, ref)):
+        ga, gb = a.download_state(), b.download_state()
+        for f in ("x", "y", "u", "v", "rho", "p"):
+            assert np.array_equal(ga[f], gb[f]), (k, f)
+    batch.close()
+    for s in bat + ref:
+        s.close()
