@@ -95,3 +95,33 @@ def test_autotuned_batch_stays_bit_identical():
     batch.close()
     for s in bat + ref:
         s.close()
+
+
+def test_batch_of_walled_tanks_uses_branches_and_stays_identical():
+    """Contexts with fixed walls and gravity cannot share fused launches; the batch
+    replays one graph with each instance's step as a branch, bit-identical to
+    running each alone."""
+    say("test_batch_of_walled_tanks_uses_branches_and_stays_identical")
+
+    def tanks():
+        out = []
+        for n in (30, 36):
+            prm = sph2d.tgv_params(n, walls=4, periodic=0, gravity=9.81, sample_every=100)
+            sim = sph2d.Simulation(prm)
+            sim.init_tgv()
+            out.append(sim)
+        return out
+    bat, ref = tanks(), tanks()
+    batch = sph2d.Batch(bat)
+    batch.run(1, 12)
+    for s in ref:
+        s.run(1, 12)
+    nodes, replays = batch.info()
+    assert replays == 11 and nodes > 30
+    for a, b in zip(bat, ref):
+        ga, gb = a.download_state(), b.download_state()
+        for f in ("x", "y", "u", "v", "rho", "p"):
+            assert np.array_equal(ga[f], gb[f]), f
+    batch.close()
+    for s in bat + ref:
+        s.close()
