@@ -89,6 +89,41 @@ def cpu_baseline(n_side: int = 512, steps: int = 20) -> dict:
                       f"timed ({dt:.1f} s); oracle/liboracle.so single-threaded"}
 
 
+_CPU_WORKER = """
+import sys, time
+sys.path.insert(0, {root!r})
+from oracle.pyoracle import Oracle
+n, steps = {n}, {steps}
+o = Oracle(nx=n, ny=n, integrator=2, dt=1e-4 * 100.0 / n)
+o.init_tgv()
+o.time_step(1)
+t0 = time.perf_counter()
+for it in range(2, 2 + steps):
+    o.time_step(it)
+print(n * n * steps / (time.perf_counter() - t0))
+"""
+
+
+def cpu_baseline_all_cores(n_side: int = 512, steps: int = 20) -> dict:
+    """The same sample on every host core at once: one oracle process per core (the
+    oracle, like one reference rank, runs one simulation on one core), each its own
+    instance; the value is the sum of their rates. Children are started as new
+    programs, not forks of this GPU process."""
+    import subprocess
+    cores = max(1, min(16, os.cpu_count() or 1))
+    code = _CPU_WORKER.format(root=ROOT, n=n_side, steps=steps)
+    procs = [subprocess.Popen([sys.executable, "-c", code], stdout=subprocess.PIPE, text=True) for _ in range(cores)]
+    rates = []
+    for p in procs:
+        out, _ = p.communicate(timeout=600)
+        if p.returncode == 0:
+            rates.append(float(out.strip().splitlines()[-1]))
+    return {"value": sum(rates), "unit": "particle-steps/s", "cores": len(rates),
+            "kind": "port, independent instances",
+            "sample": f"{len(rates)} concurrent single-core oracle processes, each TGV {n_side}^2 "
+                      f"predictor-corrector FP64, steps 2..{1 + steps}"}
+
+
 def batched_leg(sph2d, instances: int, steps: int, warmup: int) -> dict:
     """Many independent periodic TGV instances of uneven size and Reynolds number
     (n_side = round(64 * 8^U), Re = 50 * 20^U', U, U' uniform, seed 3), FP32-mixed
@@ -325,6 +360,10 @@ def main() -> int:
     if rank == 0 and not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_side, args.cpu_steps)
+        try:
+            out["cpu_baseline"]["all_cores"] = cpu_baseline_all_cores(args.cpu_side, args.cpu_steps)
+        except Exception as e:                    # the headline line is printed regardless
+            out["cpu_baseline"]["all_cores"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
     return 0
 
