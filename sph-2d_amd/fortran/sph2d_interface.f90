@@ -12,6 +12,14 @@ module sph2d_interface
   integer(c_int), parameter :: SPH2D_INTEGRATOR_PC = 2, SPH2D_INTEGRATOR_RK = 3
   integer(c_int), parameter :: SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN = 0, SPH2D_SOLVER_LINEAR_RIEMANN = 1, &
                                SPH2D_SOLVER_HLLC = 2, SPH2D_SOLVER_MUSCL = 3
+  integer(c_int), parameter :: SPH2D_DAMPING_OFF = 0, SPH2D_DAMPING_COSINE = 1, SPH2D_DAMPING_OPENFOAM = 2, &
+                               SPH2D_DAMPING_DUALSPH = 3
+  integer(c_int), parameter :: SPH2D_WAVE_SOLITARY = 0, SPH2D_WAVE_STOKES2 = 1, SPH2D_WAVE_AIRY = 2
+
+  type, bind(C) :: sph2d_wavemaker
+    integer(c_int32_t) :: wave_type, soft_steps, standing_steps
+    real(c_double) :: dt, g, water_depth, wave_height, wave_number, wave_frequency, phase
+  end type sph2d_wavemaker
 
   type, bind(C) :: sph2d_params
     integer(c_int32_t) :: struct_size
@@ -270,6 +278,16 @@ module sph2d_interface
       character(kind=c_char), intent(in) :: path(*)
       integer(c_int32_t), value :: n
       real(c_double), intent(in) :: values(n)
+      integer(c_int) :: rc
+    end function
+
+    ! Wavemaker_Subroutine.f90:66-138 and wavemaker_plate_motion.f90: the paddle at step i_time_step
+    function sph2d_wavemaker_motion(w, i_time_step, position, velocity, acceleration) &
+        bind(C, name="sph2d_wavemaker_motion") result(rc)
+      import :: sph2d_wavemaker, c_int, c_int32_t, c_double
+      type(sph2d_wavemaker), intent(in) :: w
+      integer(c_int32_t), value :: i_time_step
+      real(c_double), intent(out) :: position, velocity, acceleration
       integer(c_int) :: rc
     end function
 
