@@ -178,6 +178,19 @@ int sph2d_distribute_period_boundary(sph2d_ctx* ctx);
 int sph2d_devide_block(sph2d_ctx* ctx, int32_t i_time_step);
 /* single_step_compute.f90:62-171: EOS, pair search, free surface, solver, rate assembly. */
 int sph2d_single_step_compute(sph2d_ctx* ctx, int32_t i_time_step);
+/* Reverse of the last sph2d_single_step_compute (no reference counterpart: the
+ * adjoint of the evaluation above, for gradients through the step). Given weights
+ * per fluid particle (gid order) of its outputs -- lam_drhodt (n), lam_acc (ld,2)
+ * for dv/dt, lam_avg (ld,2) for the XSPH velocity or NULL -- writes the gradient of
+ * sum(weights * outputs) with respect to every fluid particle's position (ld,2),
+ * velocity (ld,2) and density (n), its periodic ghosts' shares added (any output
+ * may be NULL). Held fixed: the neighbour set, the free-surface flags, the limiter
+ * branches and the masses. FP64 form, modified or plain linear Riemann solver,
+ * periodic box (no walls, no damping zone); the evaluation must be the context's
+ * last one. Temporary device memory is allocated and freed per call. */
+int sph2d_single_step_compute_vjp(sph2d_ctx* ctx, int32_t n, const double* lam_drhodt, const double* lam_acc,
+                                  const double* lam_avg, int32_t ld, double* grad_pos, double* grad_vel,
+                                  double* grad_rho);
 /* Reinitialize_particle_density.f90:66-182 (MLS order 6 or 1). */
 int sph2d_reinitialize_particle_density(sph2d_ctx* ctx, int32_t i_time_step);
 /* Shift_particle_position.f90:56-136 with the last evaluation's pair data. */
@@ -387,7 +400,8 @@ int sph2d_write_sampling(const char* dir, double nondim_time, const double* out1
  * current on sampling and MLS steps only: the other steps' evaluations read the
  * FP32 records instead):
  * f64: x y u v rho mass p c volume fra_v sum_dwdx_x sum_dwdx_y tau_xx tau_xy tau_yy
- *      grad_rho_x grad_rho_y drhodt ax ay shift_x shift_y normal_x normal_y
+ *      grad_rho_x grad_rho_y drhodt ax ay shift_x shift_y avg_x avg_y (XSPH velocity,
+ *      when epsilon_xsph is not 0) normal_x normal_y
  *      internal_x internal_y visc_x visc_y art_internal_x art_internal_y sps_x sps_y
  *      lap_x lap_y (the last 10 only after sph2d_set_debug(ctx,1))
  *      grad_p_x grad_p_y grad_u_x grad_u_y grad_v_x grad_v_y (SPH2D_SOLVER_MUSCL only)

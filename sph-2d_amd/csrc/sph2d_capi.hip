@@ -782,6 +782,56 @@ int sph2d_single_step_compute(sph2d_ctx* c, int32_t it) {
   return SPH2D_OK;
 }
 
+int sph2d_single_step_compute_vjp(sph2d_ctx* c, int32_t n, const double* lam_drhodt, const double* lam_acc,
+                                  const double* lam_avg, int32_t ld, double* grad_pos, double* grad_vel,
+                                  double* grad_rho) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (n != c->n_fluid) return fail(SPH2D_ERR_ARGUMENT, "n must equal the number of fluid particles");
+  if (ld < n) return fail(SPH2D_ERR_ARGUMENT, "ld < n");
+  if (!lam_drhodt || !lam_acc) return fail(SPH2D_ERR_ARGUMENT, "lam_drhodt and lam_acc are required");
+  if (!c->evaluated) return fail(SPH2D_ERR_STATE, "no force evaluation to reverse");
+  if (c->P.mixed) return fail(SPH2D_ERR_UNSUPPORTED, "the reverse evaluation is built for the FP64 form");
+  if (c->P.solver != SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && c->P.solver != SPH2D_SOLVER_LINEAR_RIEMANN)
+    return fail(SPH2D_ERR_UNSUPPORTED, "the reverse evaluation is built for the two linear Riemann solvers");
+  if (c->P.walls || c->P.damping || c->n_wm)
+    return fail(SPH2D_ERR_UNSUPPORTED, "the reverse evaluation is built for the periodic box");
+  if (!(c->prm.box_x > 2 * c->P.three_h) || !(c->prm.box_y > 2 * c->P.three_h))
+    return fail(SPH2D_ERR_UNSUPPORTED, "the box must be wider than two kernel supports");
+  std::vector<double> lam(5 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    lam[5 * (size_t)i + 0] = lam_drhodt[i];
+    lam[5 * (size_t)i + 1] = lam_acc[i];
+    lam[5 * (size_t)i + 2] = lam_acc[ld + i];
+    lam[5 * (size_t)i + 3] = lam_avg ? lam_avg[i] : 0.0;
+    lam[5 * (size_t)i + 4] = lam_avg ? lam_avg[ld + i] : 0.0;
+  }
+  const size_t nl = (size_t)c->cap_local, ng = (size_t)c->cap_global;
+  const size_t bytes = sizeof(double) * (5 * (size_t)n + 16 * nl + 5 * ng);
+  double* mem = nullptr;
+  HIP_OR_FAIL(hipMalloc((void**)&mem, bytes));
+  double *d_lam = mem, *w = d_lam + 5 * (size_t)n, *gs = w + 6 * nl, *ga = gs + 5 * nl, *G = ga + 5 * nl;
+  std::vector<double> out(5 * (size_t)n);
+  hipError_t e = hipMemcpyAsync(d_lam, lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    launch_eval_vjp(c->stream, c->P, c->B, c->cap_local, c->cap_fluid, c->cap_ghost, c->cap_global, d_lam, w, gs,
+                    ga, G);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out.data(), G, sizeof(double) * out.size(), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(mem);
+  if (e != hipSuccess) return fail(SPH2D_ERR_HIP, std::string("reverse evaluation: ") + hipGetErrorString(e));
+  for (int i = 0; i < n; ++i) {
+    const double* g = &out[5 * (size_t)i];
+    if (grad_pos) { grad_pos[i] = g[0]; grad_pos[ld + i] = g[1]; }
+    if (grad_vel) { grad_vel[i] = g[2]; grad_vel[ld + i] = g[3]; }
+    if (grad_rho) grad_rho[i] = g[4];
+  }
+  return SPH2D_OK;
+}
+
 int sph2d_reinitialize_particle_density(sph2d_ctx* c, int32_t it) {
   (void)it;
   int rc = need_state(c);
@@ -1337,6 +1387,8 @@ int sph2d_download_local(sph2d_ctx* c, const char* name, void* out, int32_t n) {
   if (s == "ay") return d2(B.acc, 1);
   if (s == "shift_x") return d2(B.shift, 0);
   if (s == "shift_y") return d2(B.shift, 1);
+  if (s == "avg_x") return d2(B.avg, 0);
+  if (s == "avg_y") return d2(B.avg, 1);
   if (s == "type") return i1(B.l_type);
   if (s == "name") return i1(B.name);
   if (s == "surface") return i1(B.surf);

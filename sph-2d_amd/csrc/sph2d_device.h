@@ -312,6 +312,17 @@ __device__ __forceinline__ void or_shl128(uint64_t& w0, uint64_t& w1, uint64_t& 
   w3 |= q == 2 ? b : (q == 1 ? c : 0ull);
 }
 
+// The flags of one wrapped fluid position: x-ghost to the right of the box (l),
+// to the left (r), y-ghosts above (yl) and below (yr)
+// (Distribute_Period_Boundary_Particle.f90:98-148, :153-206).
+__device__ __forceinline__ void ghost_flags(const DevParams& P, double2 p, int& l, int& r, int& yl, int& yr) {
+  const double r3h = P.three_h;
+  l = (p.x <= (0.0 + r3h) && p.x >= 0.0);
+  r = (p.x >= (P.box_x - r3h) && p.x <= P.box_x);
+  yl = (p.y <= (0.0 + r3h) && p.y >= 0.0);
+  yr = (p.y >= (P.box_y - r3h) && p.y <= P.box_y);
+}
+
 template <int MODE = kNbScan, class F>
 __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int* __restrict__ cell_start,
                                                    const double2* __restrict__ pos,

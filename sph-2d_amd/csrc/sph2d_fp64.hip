@@ -209,16 +209,6 @@ __global__ void __launch_bounds__(1024) k_scan_top2(int* a, int nb, int* t0, int
   k_scan_top2_body(a, nb, t0, t1, blockIdx.x, gridDim.x);
 }
 
-// The flags of one wrapped fluid position: x-ghost to the right of the box (l),
-// to the left (r), y-ghosts above (yl) and below (yr) (:98-148, :153-206).
-__device__ __forceinline__ void ghost_flags(const DevParams& P, double2 p, int& l, int& r, int& yl, int& yr) {
-  const double r3h = P.three_h;
-  l = (p.x <= (0.0 + r3h) && p.x >= 0.0);
-  r = (p.x >= (P.box_x - r3h) && p.x <= P.box_x);
-  yl = (p.y <= (0.0 + r3h) && p.y >= 0.0);
-  yr = (p.y >= (P.box_y - r3h) && p.y <= P.box_y);
-}
-
 // :56-90 wrap (written back only where it moves a particle), then per block of
 // kBlock gids the number of x-ghosts (gblk[b]) and of y-ghosts of fluid
 // particles (gblk[nb + b]).

@@ -113,6 +113,10 @@ int set_error(int code, const std::string& msg);
 
 // Launchers (sph2d_fp64.hip). All run on `s` and take sizes from device counters.
 void launch_init_tgv(hipStream_t s, const DevParams& P, Buffers& B, int nx, int ny);
+// Reverse of the last FP64 evaluation (sph2d_adjoint.hip); lam: 5 per fluid gid,
+// w: 6 and gs, ga: 5 per local particle, G: 5 per gid (result in the first n_fluid).
+void launch_eval_vjp(hipStream_t st, const DevParams& P, Buffers& B, int cap_local, int cap_fluid, int cap_ghost,
+                     int cap_global, const double* lam, double* w, double* gs, double* ga, double* G);
 void launch_period_boundary(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int cap_ghost);
 // predict 1: the local copy also applies the predictor-corrector's predictor (k_gather_predict);
 // 2: and keeps the step-start state in before_* for k_stage's corrector; 3: as 1 without

@@ -110,6 +110,7 @@ EXPORTS = (
     "sph2d_init_tgv", "sph2d_upload_state", "sph2d_download_state", "sph2d_compute_kernel",
     "sph2d_compute_kernel_n", "sph2d_status_equation", "sph2d_status_equation_n",
     "sph2d_distribute_period_boundary", "sph2d_devide_block", "sph2d_single_step_compute",
+    "sph2d_single_step_compute_vjp",
     "sph2d_reinitialize_particle_density", "sph2d_shift_particle_position",
     "sph2d_refresh_domain", "sph2d_time_step", "sph2d_run", "sph2d_sample",
     "sph2d_download_local", "sph2d_download_pairs", "sph2d_set_debug",
@@ -156,6 +157,7 @@ def lib():
             "sph2d_distribute_period_boundary": [vp],
             "sph2d_devide_block": [vp, i32],
             "sph2d_single_step_compute": [vp, i32],
+            "sph2d_single_step_compute_vjp": [vp, i32, dp, dp, dp, i32, dp, dp, dp],
             "sph2d_reinitialize_particle_density": [vp, i32],
             "sph2d_shift_particle_position": [vp, i32],
             "sph2d_refresh_domain": [vp, i32],
@@ -255,7 +257,7 @@ def tgv_params(n_side: int, **kw) -> Params:
 
 LOCAL_F64 = ("x", "y", "u", "v", "rho", "mass", "p", "c", "volume", "fra_v", "sum_dwdx_x",
              "sum_dwdx_y", "tau_xx", "tau_xy", "tau_yy", "grad_rho_x", "grad_rho_y", "drhodt",
-             "ax", "ay", "shift_x", "shift_y", "normal_x", "normal_y", "internal_x", "internal_y",
+             "ax", "ay", "shift_x", "shift_y", "avg_x", "avg_y", "normal_x", "normal_y", "internal_x", "internal_y",
              "visc_x", "visc_y", "art_internal_x", "art_internal_y", "sps_x", "sps_y", "lap_x",
              "lap_y", "grad_p_x", "grad_p_y", "grad_u_x", "grad_u_y", "grad_v_x", "grad_v_y")
 LOCAL_I32 = ("type", "name", "surface", "near_surface", "neighbour_count")
@@ -326,6 +328,23 @@ class Simulation:
 
     def single_step_compute(self, i_time_step: int = 1):
         _check(lib().sph2d_single_step_compute(self._c, i_time_step))
+
+    def single_step_compute_vjp(self, lam_drhodt, lam_ax, lam_ay, lam_avg_x=None, lam_avg_y=None):
+        """Reverse of the last single_step_compute: the gradient of
+        sum(lam_drhodt*drho/dt + lam_ax*ax + lam_ay*ay [+ lam_avg*XSPH velocity]) with
+        respect to every fluid particle's x, y, u, v, rho (gid order), the periodic
+        ghosts' shares included. Neighbour set, surface flags and limiter branches
+        are held fixed."""
+        n = self.n_fluid
+        ld = np.ascontiguousarray(lam_drhodt, dtype=np.float64)
+        la = np.ascontiguousarray(np.concatenate([lam_ax, lam_ay]), dtype=np.float64)
+        lg = None
+        if lam_avg_x is not None:
+            lg = np.ascontiguousarray(np.concatenate([lam_avg_x, lam_avg_y]), dtype=np.float64)
+        gp, gv, gr = np.zeros(2 * n), np.zeros(2 * n), np.zeros(n)
+        _check(lib().sph2d_single_step_compute_vjp(self._c, n, _dp(ld), _dp(la), _dp(lg) if lg is not None else None,
+                                                   n, _dp(gp), _dp(gv), _dp(gr)))
+        return {"x": gp[:n], "y": gp[n:], "u": gv[:n], "v": gv[n:], "rho": gr}
 
     def reinitialize_particle_density(self, i_time_step: int):
         _check(lib().sph2d_reinitialize_particle_density(self._c, i_time_step))
