@@ -188,6 +188,37 @@ def batched_leg(sph2d, instances: int, steps: int, warmup: int) -> dict:
             "timing": "host wall clock with device synchronisation, launches included"}
 
 
+def adjoint_leg(sph2d, n_side: int, reps: int) -> dict:
+    """Reverse of one FP64 force evaluation (sph2d_single_step_compute_vjp) beside the
+    forward evaluation it reverses (sph2d_single_step_compute), TGV lattice of n_side^2:
+    wall-clock per call between device synchronisations. The reverse call includes
+    its host-side weights upload, the temporary device allocation and the gradient
+    download; the forward call is device work only."""
+    import numpy as np
+    sim = sph2d.Simulation(sph2d.tgv_params(n_side, integrator=2))
+    sim.init_tgv()
+    sim.distribute_period_boundary()
+    sim.devide_block(1)
+    hip = sph2d._hiprt()
+    sim.single_step_compute(1)
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.single_step_compute(1)
+    hip.hipDeviceSynchronize()
+    fwd = (time.perf_counter() - t0) / reps
+    N = n_side * n_side
+    rng = np.random.default_rng(0)
+    lam = [rng.normal(0.0, 1.0, N) for _ in range(3)]
+    sim.single_step_compute_vjp(*lam)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.single_step_compute_vjp(*lam)
+    rev = (time.perf_counter() - t0) / reps
+    return {"n_side": n_side, "particles": N, "forward_evaluation_ms": 1e3 * fwd,
+            "reverse_evaluation_ms": 1e3 * rev, "reverse_over_forward": rev / fwd, "reps": reps}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +237,8 @@ def main() -> int:
     ap.add_argument("--batch-instances", type=int, default=64,
                     help="batched leg: independent TGV instances advanced by sph2d_batch (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--adjoint-side", type=int, default=1024,
+                    help="reverse-evaluation leg on an n^2 TGV lattice (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU form (spatial slabs "
@@ -357,6 +390,11 @@ def main() -> int:
             out["batched"] = batched_leg(sph2d, args.batch_instances, args.batch_steps, args.warmup)
         except Exception as e:                    # the headline line is printed regardless
             out["batched"] = {"error": repr(e)}
+    if args.adjoint_side > 0:
+        try:
+            out["adjoint"] = adjoint_leg(sph2d, args.adjoint_side, 5)
+        except Exception as e:                    # the headline line is printed regardless
+            out["adjoint"] = {"error": repr(e)}
     if rank == 0 and not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_side, args.cpu_steps)

@@ -124,6 +124,18 @@ module sph2d_interface
       integer(c_int) :: rc
     end function
 
+    ! Reverse of the last evaluation (no reference counterpart): weights of drho/dt,
+    ! dv/dt and the XSPH velocity in, d(weighted outputs)/d(state) out, gid order.
+    function sph2d_single_step_compute_vjp(ctx, n, lam_drhodt, lam_acc, lam_avg, ld, grad_pos, grad_vel, &
+        grad_rho) bind(C, name="sph2d_single_step_compute_vjp") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: n, ld
+      real(c_double), intent(in) :: lam_drhodt(ld), lam_acc(ld, 2), lam_avg(ld, 2)
+      real(c_double), intent(out) :: grad_pos(ld, 2), grad_vel(ld, 2), grad_rho(ld)
+      integer(c_int) :: rc
+    end function
+
     function sph2d_time_step(ctx, i_time_step) bind(C, name="sph2d_time_step") result(rc)
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: ctx
