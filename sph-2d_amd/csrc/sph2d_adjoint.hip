@@ -10,23 +10,26 @@
 // d(state) per fluid particle, the ghosts' shares added to the particles they
 // copy. What stays fixed, as in any discrete adjoint of this step: the
 // neighbour set (the acceptance masks of the last evaluation), the free-surface
-// and near-surface flags, the branches of the limiter (vx < 0, max, min) and the
-// particle masses. The identity renormalisation of the reference keeps the
+// and near-surface flags, the branches of the limiter (vx < 0, max, min), the
+// particle masses and the local order (the SPS term reads particle_mass at the
+// local index, Solver...SPS.f90:450-492). The identity renormalisation of the reference keeps the
 // normalised kernel gradient equal to the raw one, so no matrix inverse enters.
 //
 // Structure (all gather form, no float atomics, fixed summation order):
 //   k_adj_weights   per local particle: its output weights (zero for ghosts)
 //   k_adj_force     per local s, over neighbours o: d/d(state_s, grad rho_s, tau_s)
 //                   of the force-pass pair terms summed by s (weights of s) and by
-//                   o (weights of o)
+//                   o (weights of o); state directions by 5-direction duals, grad rho
+//                   and tau (linear in the pair terms) in closed form
 //   k_adj_tau       per local s: recompute the strain S_s, reverse tau(S_s, rho_s)
 //   k_adj_sums      per local s, over neighbours o: reverse of the pass-A sums
 //                   (S and grad rho) into d/d(state_s); result stored by gid
 //   k_adj_corners   corner ghosts into the x-ghost they copy
 //   k_adj_owner     every ghost into its fluid particle (same slot arithmetic as
 //                   the ghost emission of Distribute_Period_Boundary_Particle.f90)
-// Pair-level Jacobians come from forward-mode duals over the same pair formulas
-// as the FP64 force pass; the reverse sweep over the particle graph is explicit.
+// Pair-level Jacobians with respect to the state come from forward-mode duals over
+// the same pair formulas as the FP64 force pass; the reverse sweep over the
+// particle graph is explicit.
 #include "sph2d_internal.h"
 #include "sph2d_device.h"
 
@@ -128,9 +131,11 @@ __device__ __forceinline__ void tait_t(const DevParams& P, const T& rho, T& p, T
 
 // Weighted force-pass pair terms summed by particle S with neighbour O: the same
 // formulas as eval_c in FP64 (Solver_Normalized_*_Riemann_Delta_SPS.f90:300-583).
+// gaux receives d/d(grad rho x, y, tau xx, xy, yy) of either side: those enter the
+// pair terms linearly, through the sums S + O, so both sides share one closed form.
 template <class T, int SOLVER>
 __device__ __forceinline__ T force_pair(const DevParams& P, const Side<T>& S, const Side<T>& O, double mo,
-                                        double gmo, int nes, const PairLam& L) {
+                                        double gmo, int nes, const PairLam& L, double* gaux) {
   const T dx = S.x - O.x, dy = S.y - O.y;
   const T r2 = dx * dx + dy * dy;
   const T r = tsqrt(r2);
@@ -203,7 +208,16 @@ __device__ __forceinline__ T force_pair(const DevParams& P, const Side<T>& S, co
     J = J - L.gx * (f * vijx) - L.gy * (f * vijy);
   }
   const T psi = (O.rho - S.rho) + 0.5 * ((S.gx + O.gx) * dx + (S.gy + O.gy) * dy);
-  J = J - L.c * (2.0 * P.delta * P.h * cbar * psi * Rd / r2 * Vo);
+  const T cf = 2.0 * P.delta * P.h * cbar * Rd / r2 * Vo;
+  J = J - L.c * (cf * psi);
+  {
+    const double rv = val(rij), wx = val(dwx), wy = val(dwy), cv = val(cf);
+    gaux[0] = -L.c * cv * 0.5 * val(dx);
+    gaux[1] = -L.c * cv * 0.5 * val(dy);
+    gaux[2] = gmo * rv * (L.ax * wx + L.ay * (wx * rv));
+    gaux[3] = gmo * rv * (L.ax * wy + L.ay * (wy * rv + wx));
+    gaux[4] = gmo * rv * (L.ay * wy);
+  }
   return J;
 }
 
@@ -288,18 +302,22 @@ __global__ void __launch_bounds__(kBlock) k_adj_force(DevParams P, Buffers B, Ad
   for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) {
     if (ts != 2 && B.l_type[o] != 2) return;
     const Side<double> vo = load_side<double>(B, o);
-    Side<Dual<10>> S, O;
+    Side<Dual<5>> S, O;
     S.x = vs.x; S.y = vs.y; S.u = vs.u; S.v = vs.v; S.rho = vs.rho;
     S.gx = vs.gx; S.gy = vs.gy; S.txx = vs.txx; S.txy = vs.txy; S.tyy = vs.tyy;
     O.x = vo.x; O.y = vo.y; O.u = vo.u; O.v = vo.v; O.rho = vo.rho;
     O.gx = vo.gx; O.gy = vo.gy; O.txx = vo.txx; O.txy = vo.txy; O.tyy = vo.tyy;
-    seed_side(S, 0, 10);
+    seed_side(S, 0, 5);
+    double a1[5], a2[5];
     // terms summed by s (s is the summing particle)
-    const Dual<10> J1 = force_pair<Dual<10>, SOLVER>(P, S, O, B.l_mass[o], B.g_mass[o], nes, Ls);
+    const Dual<5> J1 = force_pair<Dual<5>, SOLVER>(P, S, O, B.l_mass[o], B.g_mass[o], nes, Ls, a1);
     // terms summed by o (s is its neighbour)
-    const Dual<10> J2 = force_pair<Dual<10>, SOLVER>(P, O, S, ms, gms, B.near_[o], pair_lam(A, o));
+    const Dual<5> J2 = force_pair<Dual<5>, SOLVER>(P, O, S, ms, gms, B.near_[o], pair_lam(A, o), a2);
 #pragma unroll
-    for (int k = 0; k < 10; ++k) acc[k] = acc[k] + (J1.d[k] + J2.d[k]);
+    for (int k = 0; k < 5; ++k) {
+      acc[k] = acc[k] + (J1.d[k] + J2.d[k]);
+      acc[5 + k] = acc[5 + k] + (a1[k] + a2[k]);
+    }
   }, B.nb_mask, B.kstride);
   double* gs = A.gs + 5 * (size_t)s;
   double* ga = A.ga + 5 * (size_t)s;
