@@ -192,8 +192,8 @@ def adjoint_leg(sph2d, n_side: int, reps: int) -> dict:
     """Reverse of one FP64 force evaluation (sph2d_single_step_compute_vjp) beside the
     forward evaluation it reverses (sph2d_single_step_compute), TGV lattice of n_side^2:
     wall-clock per call between device synchronisations. The reverse call includes
-    its host-side weights upload, the temporary device allocation and the gradient
-    download; the forward call is device work only."""
+    its host-side weights upload and the gradient download (its device buffers are
+    allocated by the untimed first call); the forward call is device work only."""
     import numpy as np
     sim = sph2d.Simulation(sph2d.tgv_params(n_side, integrator=2))
     sim.init_tgv()

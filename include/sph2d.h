@@ -187,7 +187,8 @@ int sph2d_single_step_compute(sph2d_ctx* ctx, int32_t i_time_step);
  * may be NULL). Held fixed: the neighbour set, the free-surface flags, the limiter
  * branches and the masses. FP64 form, modified or plain linear Riemann solver,
  * periodic box (no walls, no damping zone); the evaluation must be the context's
- * last one. Temporary device memory is allocated and freed per call. */
+ * last one. The first call allocates the reverse's device buffers (16 doubles per
+ * local and 10 per global particle slot), kept until sph2d_ctx_destroy. */
 int sph2d_single_step_compute_vjp(sph2d_ctx* ctx, int32_t n, const double* lam_drhodt, const double* lam_acc,
                                   const double* lam_avg, int32_t ld, double* grad_pos, double* grad_vel,
                                   double* grad_rho);

@@ -64,6 +64,8 @@ struct sph2d_ctx {
   // sampling points and wave probes (allocated on first use, in doubles)
   double* probe_buf = nullptr;
   size_t probe_cap = 0;
+  // reverse evaluation: weights, per-local adjoints and the by-gid result (first use)
+  double* adj_buf = nullptr;
   bool pass_timing = false;
   std::vector<hipEvent_t> events;
   size_t events_used = 0;
@@ -806,23 +808,19 @@ int sph2d_single_step_compute_vjp(sph2d_ctx* c, int32_t n, const double* lam_drh
     lam[5 * (size_t)i + 3] = lam_avg ? lam_avg[i] : 0.0;
     lam[5 * (size_t)i + 4] = lam_avg ? lam_avg[ld + i] : 0.0;
   }
-  const size_t nl = (size_t)c->cap_local, ng = (size_t)c->cap_global;
-  const size_t bytes = sizeof(double) * (5 * (size_t)n + 16 * nl + 5 * ng);
-  double* mem = nullptr;
-  HIP_OR_FAIL(hipMalloc((void**)&mem, bytes));
-  double *d_lam = mem, *w = d_lam + 5 * (size_t)n, *gs = w + 6 * nl, *ga = gs + 5 * nl, *G = ga + 5 * nl;
-  std::vector<double> out(5 * (size_t)n);
-  hipError_t e = hipMemcpyAsync(d_lam, lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) {
-    launch_eval_vjp(c->stream, c->P, c->B, c->cap_local, c->cap_fluid, c->cap_ghost, c->cap_global, d_lam, w, gs,
-                    ga, G);
-    e = hipGetLastError();
+  const size_t nf = (size_t)c->cap_fluid, nl = (size_t)c->cap_local, ng = (size_t)c->cap_global;
+  if (!c->adj_buf) {                                // kept until the context is destroyed
+    rc = c->alloc(&c->adj_buf, 5 * nf + 16 * nl + 5 * ng);
+    if (rc) return rc;
   }
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(out.data(), G, sizeof(double) * out.size(), hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(mem);
-  if (e != hipSuccess) return fail(SPH2D_ERR_HIP, std::string("reverse evaluation: ") + hipGetErrorString(e));
+  double *d_lam = c->adj_buf, *w = d_lam + 5 * nf, *gs = w + 6 * nl, *ga = gs + 5 * nl, *G = ga + 5 * nl;
+  std::vector<double> out(5 * (size_t)n);
+  HIP_OR_FAIL(hipMemcpyAsync(d_lam, lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice, c->stream));
+  launch_eval_vjp(c->stream, c->P, c->B, c->cap_local, c->cap_fluid, c->cap_ghost, c->cap_global, d_lam, w, gs, ga,
+                  G);
+  HIP_OR_FAIL(hipGetLastError());
+  HIP_OR_FAIL(hipMemcpyAsync(out.data(), G, sizeof(double) * out.size(), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
   for (int i = 0; i < n; ++i) {
     const double* g = &out[5 * (size_t)i];
     if (grad_pos) { grad_pos[i] = g[0]; grad_pos[ld + i] = g[1]; }
