@@ -417,10 +417,6 @@ int sph2d_download_pairs(sph2d_ctx* ctx, int64_t capacity, int32_t* gid_lo, int3
 /* 1: also write the acceleration components of the solver (for parity tests). */
 int sph2d_set_debug(sph2d_ctx* ctx, int32_t on);
 int sph2d_get_diagnostics(sph2d_ctx* ctx, sph2d_diag* out);
-/* Average duration (ms, HIP events on the context's stream) of `reps` back-to-back
- * launches of one part of the last evaluation: pass 0 = force pass (Riemann,
- * delta-SPH, SPS, shift sums), 1 = neighbour-sum pass A, 2 = the whole evaluation.
- * No reference counterpart: measurement hook for the roofline report. */
 /* In-step pass timing (measurement hook, no reference counterpart): while on,
  * every force evaluation records HIP events around pass A and the force pass;
  * sph2d_get_pass_timing synchronises and returns the average durations (ms) over
@@ -428,6 +424,10 @@ int sph2d_get_diagnostics(sph2d_ctx* ctx, sph2d_diag* out);
  * force pass end), and their number. */
 int sph2d_set_pass_timing(sph2d_ctx* ctx, int32_t on);
 int sph2d_get_pass_timing(sph2d_ctx* ctx, double* out3, int32_t* n_evaluations);
+/* Average duration (ms, HIP events on the context's stream) of `reps` back-to-back
+ * launches of one part of the last evaluation: pass 0 = force pass (Riemann,
+ * delta-SPH, SPS, shift sums), 1 = neighbour-sum pass A, 2 = the whole evaluation.
+ * No reference counterpart: measurement hook for the roofline report. */
 int sph2d_time_pass(sph2d_ctx* ctx, int32_t pass, int32_t reps, double* avg_ms);
 /* Device stream of the context (hipStream_t) for callers that time or order work. */
 int sph2d_get_stream(sph2d_ctx* ctx, void** stream);

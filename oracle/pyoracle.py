@@ -349,6 +349,12 @@ def hllc_star(v_left, p_left, rho_left, c_left, v_right, p_right, rho_right, c_r
 _ref = None
 
 
+def ref_available() -> bool:
+    """Whether oracle/_ref is built; checks the file only, so test collection never
+    maps the compiled reference into a process that does not use it."""
+    return os.path.exists(REF_PATH)
+
+
 def ref_lib():
     """The reference's own compute_kernel/status_equation/function_module, compiled
     from /root/reference by oracle/build_ref.sh; None where it is not built."""

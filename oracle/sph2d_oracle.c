@@ -63,6 +63,9 @@ struct orc_ctx {
   double *ia_x, *ia_y, *av_x, *av_y, *ai_x, *ai_y, *sps_x, *sps_y, *lap_x, *lap_y, *ext_x, *ext_y;
   double *avg_x, *avg_y, *corr, *grx, *gry;
   double *gpx, *gpy, *gux, *guy, *gvx, *gvy;    /* MUSCL: grad_pressure, grad_velocity_u/_v */
+  /* sum over the pair terms of |term| per particle for drhodt, ax, ay: the scale of
+   * each rate before cancellation (test metric only, no reference counterpart) */
+  double *mag_drho, *mag_ax, *mag_ay;
   /* shift */
   double *shx, *shy;
   /* Refresh_domain's vorticity (z), local order; global vorticity, surface flags and
@@ -236,6 +239,7 @@ static void alloc_local(orc_ctx* c, int cap) {
   A(ia_x); A(ia_y); A(av_x); A(av_y); A(ai_x); A(ai_y); A(sps_x); A(sps_y);
   A(lap_x); A(lap_y); A(ext_x); A(ext_y); A(avg_x); A(avg_y); A(corr); A(grx); A(gry);
   A(gpx); A(gpy); A(gux); A(guy); A(gvx); A(gvy);
+  A(mag_drho); A(mag_ax); A(mag_ay);
   A(shx); A(shy); A(vort);
   A(b_rho); A(b_e); A(b_x); A(b_y); A(b_u); A(b_v);
   A(t_u); A(t_v); A(t_drho); A(t_de); A(t_ax); A(t_ay);
@@ -284,7 +288,7 @@ void orc_destroy(orc_ctx* c) {
                   c->Syy, c->txx, c->txy, c->tyy, c->drhodt, c->dedt, c->ax, c->ay, c->ia_x,
                   c->ia_y, c->av_x, c->av_y, c->ai_x, c->ai_y, c->sps_x, c->sps_y, c->lap_x,
                   c->lap_y, c->ext_x, c->ext_y, c->avg_x, c->avg_y, c->corr, c->grx, c->gry,
-                  c->gpx, c->gpy, c->gux, c->guy, c->gvx, c->gvy, c->shx, c->shy, c->vort, c->b_rho, c->b_e, c->b_x, c->b_y, c->b_u, c->b_v, c->t_u,
+                  c->gpx, c->gpy, c->gux, c->guy, c->gvx, c->gvy, c->mag_drho, c->mag_ax, c->mag_ay, c->shx, c->shy, c->vort, c->b_rho, c->b_e, c->b_x, c->b_y, c->b_u, c->b_v, c->t_u,
                   c->t_v, c->t_drho, c->t_de, c->t_ax, c->t_ay, c->mls_rho,
                   c->cell_count, c->cell_list, c->near_n, c->near_name,
                   c->pi, c->pj, c->pw, c->pdwx, c->pdwy, c->pdx, c->pdy, c->pvx, c->pvy, c->prd,
@@ -1133,14 +1137,20 @@ static void solver_riemann_delta_sps(orc_ctx* c) {
     double tj = -((Jv[0] - Jvs[0]) * dj[0] + (Jv[1] - Jvs[1]) * dj[1]);
     c->drhodt[i] = c->drhodt[i] + 2 * c->rho[i] * c->vol[j] * ti;
     c->drhodt[j] = c->drhodt[j] + 2 * c->rho[j] * c->vol[i] * tj;
+    c->mag_drho[i] += fabs(2 * c->rho[i] * c->vol[j] * ti);
+    c->mag_drho[j] += fabs(2 * c->rho[j] * c->vol[i] * tj);
     double rij = 1.0 / (c->rho[i] * c->rho[j]);
     double* iax[2] = {&c->ia_x[i], &c->ia_y[i]};
     double* jax[2] = {&c->ia_x[j], &c->ia_y[j]};
+    double* imag[2] = {&c->mag_ax[i], &c->mag_ay[i]};
+    double* jmag[2] = {&c->mag_ax[j], &c->mag_ay[j]};
     for (int L = 0; L < 2; ++L) {
       double tpi = -2 * IPs * di[L] * rij;
       double tpj = -2 * JPs * dj[L] * rij;
       *iax[L] = *iax[L] + c->mass[j] * tpi;
       *jax[L] = *jax[L] - c->mass[i] * tpj;
+      *imag[L] += fabs(c->mass[j] * tpi);
+      *jmag[L] += fabs(c->mass[i] * tpj);
     }
     double hb = c->ph_[k];
     double vij[2] = {c->pvx[k], c->pvy[k]};
@@ -1157,6 +1167,8 @@ static void solver_riemann_delta_sps(orc_ctx* c) {
         double tvj = -Pi * dj[L];
         *iav[L] = *iav[L] + c->mass[j] * tvi;
         *jav[L] = *jav[L] - c->mass[i] * tvj;
+        *imag[L] += fabs(c->mass[j] * tvi);
+        *jmag[L] += fabs(c->mass[i] * tvj);
       }
     }
     double Ri = xij[0] * di[0] + xij[1] * di[1];                  /* :432-436 */
@@ -1165,12 +1177,18 @@ static void solver_riemann_delta_sps(orc_ctx* c) {
     double Rj = xij[0] * dj[0] + xij[1] * dj[1];
     c->lap_x[j] = c->lap_x[j] - c->vol[i] * 2 * P->nu * Rj / r2 * vij[0];
     c->lap_y[j] = c->lap_y[j] - c->vol[i] * 2 * P->nu * Rj / r2 * vij[1];
+    c->mag_ax[i] += fabs(c->vol[j] * 2 * P->nu * Ri / r2 * vij[0]);
+    c->mag_ay[i] += fabs(c->vol[j] * 2 * P->nu * Ri / r2 * vij[1]);
+    c->mag_ax[j] += fabs(c->vol[i] * 2 * P->nu * Rj / r2 * vij[0]);
+    c->mag_ay[j] += fabs(c->vol[i] * 2 * P->nu * Rj / r2 * vij[1]);
     double tmp = P->delta_alpha * hb * cb * vx / r2;              /* :442-446 */
     double* iai[2] = {&c->ai_x[i], &c->ai_y[i]};
     double* jai[2] = {&c->ai_x[j], &c->ai_y[j]};
     for (int L = 0; L < 2; ++L) {
       *iai[L] = *iai[L] + tmp * di[L] * P->rho0 / c->rho[i] * c->vol[j];
       *jai[L] = *jai[L] - tmp * dj[L] * P->rho0 / c->rho[j] * c->vol[i];
+      *imag[L] += fabs(tmp * di[L] * P->rho0 / c->rho[i] * c->vol[j]);
+      *jmag[L] += fabs(tmp * dj[L] * P->rho0 / c->rho[j] * c->vol[i]);
     }
     /* :450-492 SPS with raw dwdx and the GLOBAL particle_mass at LOCAL index */
     double d1 = c->pdwx[k], d2 = c->pdwy[k];
@@ -1180,10 +1198,14 @@ static void solver_riemann_delta_sps(orc_ctx* c) {
     tp = tp * rij;
     c->sps_x[i] = c->sps_x[i] + c->gmass[j] * tp;
     c->sps_x[j] = c->sps_x[j] - c->gmass[i] * tp;
+    c->mag_ax[i] += fabs(c->gmass[j] * tp);
+    c->mag_ax[j] += fabs(c->gmass[i] * tp);
     tp = tp + (c->tyy[i] + c->tyy[j]) * d2 + (c->txy[i] + c->txy[j]) * d1;
     tp = tp * rij;
     c->sps_y[i] = c->sps_y[i] + c->gmass[j] * tp;
     c->sps_y[j] = c->sps_y[j] - c->gmass[i] * tp;
+    c->mag_ay[i] += fabs(c->gmass[j] * tp);
+    c->mag_ay[j] += fabs(c->gmass[i] * tp);
     if (P->solver == ORC_SOLVER_MODIFIED_LINEAR_RIEMANN) {       /* :495-500 XSPH */
       double rs = c->rho[i] + c->rho[j];
       c->avg_x[i] = c->avg_x[i] - P->epsilon_xsph * 2 * c->mass[j] * vij[0] / rs * c->pw[k];
@@ -1218,6 +1240,8 @@ static void solver_riemann_delta_sps(orc_ctx* c) {
     double tj = (pd[0] * c->ndjx[k] + pd[1] * c->ndjy[k]) / r2;
     c->corr[i] = c->corr[i] + 2 * P->delta * hb * cb * psi * ti * c->vol[j];
     c->corr[j] = c->corr[j] - 2 * P->delta * hb * cb * psi * tj * c->vol[i];
+    c->mag_drho[i] += fabs(2 * P->delta * hb * cb * psi * ti * c->vol[j]);
+    c->mag_drho[j] += fabs(2 * P->delta * hb * cb * psi * tj * c->vol[i]);
   }
   for (int i = 0; i < n; ++i) {                                   /* :627-635 */
     if (c->type[i] != 2) continue;
@@ -1252,6 +1276,7 @@ static void end_damping_zone(orc_ctx* c) {
       f = 1 - 6 * (q * q) * ((d - y) / d);
     }
     c->ax[i] = c->ax[i] * f; c->ay[i] = c->ay[i] * f;
+    c->mag_ax[i] = c->mag_ax[i] * fabs(f); c->mag_ay[i] = c->mag_ay[i] * fabs(f);
     c->avg_x[i] = c->avg_x[i] * f; c->avg_y[i] = c->avg_y[i] * f;
     c->u[i] = c->u[i] * f; c->v[i] = c->v[i] * f;
   }
@@ -1276,6 +1301,7 @@ void orc_single_step_compute(orc_ctx* c, int32_t i_time_step) {
   memset(c->ext_x, 0, nb); memset(c->ext_y, 0, nb);
   memset(c->avg_x, 0, nb); memset(c->avg_y, 0, nb);
   memset(c->corr, 0, nb);
+  memset(c->mag_drho, 0, nb); memset(c->mag_ax, 0, nb); memset(c->mag_ay, 0, nb);
   memset(c->lap_x, 0, nb); memset(c->lap_y, 0, nb);
   memset(c->sps_x, 0, nb); memset(c->sps_y, 0, nb);
   solver_riemann_delta_sps(c);                                   /* :137 */
@@ -1283,6 +1309,7 @@ void orc_single_step_compute(orc_ctx* c, int32_t i_time_step) {
   for (int i = 0; i < n; ++i) {
     if (c->type[i] != 2) continue;
     if (c->P.gravity != 0.0) c->ext_y[i] = c->ext_y[i] - c->P.gravity;   /* :161-163 */
+    c->mag_ay[i] += fabs(c->ext_y[i]);
     c->ax[i] = c->ia_x[i] + c->ext_x[i] + c->av_x[i] + c->ai_x[i] + c->sps_x[i] + c->lap_x[i];
     c->ay[i] = c->ia_y[i] + c->ext_y[i] + c->av_y[i] + c->ai_y[i] + c->sps_y[i] + c->lap_y[i];
     c->dedt[i] = c->dedt[i] + 0.0 + 0.0;
@@ -1643,6 +1670,7 @@ static const fld F64[] = {
   FL("shift_y", shy, 0), FL("vorticity", vort, 0), FL("g_vorticity", gvort, 1), FL("g_normal_x", gnx, 1), FL("g_normal_y", gny, 1),
   FL("grad_p_x", gpx, 0), FL("grad_p_y", gpy, 0), FL("grad_u_x", gux, 0), FL("grad_u_y", guy, 0),
   FL("grad_v_x", gvx, 0), FL("grad_v_y", gvy, 0),
+  FL("mag_drhodt", mag_drho, 0), FL("mag_ax", mag_ax, 0), FL("mag_ay", mag_ay, 0),
   FL("T", T, 3), FL("T_inv", Tinv, 3),
   FL("g_x", gx, 1), FL("g_y", gy, 1), FL("g_u", gu, 1), FL("g_v", gv, 1),
   FL("g_mass", gmass, 1), FL("g_rho", grho, 1), FL("g_h", gh, 1), FL("g_c", gc, 1),

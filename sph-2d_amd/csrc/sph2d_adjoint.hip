@@ -409,7 +409,9 @@ __global__ void k_adj_corners(Buffers B, AdjBufs A, int cap_ghost) {
   const int nc = B.cnt_yx[g];
   double* dst = A.G + 5 * (size_t)(N + g);
   for (int t = 0; t < nc; ++t) {
-    const double* src = A.G + 5 * (size_t)(N + nxg + nyf + B.off_yx[g] + t);
+    const int slot = nxg + nyf + B.off_yx[g] + t;
+    if (slot >= cap_ghost) break;                     // the emission clamps at the capacity too
+    const double* src = A.G + 5 * (size_t)(N + slot);
     for (int k = 0; k < 5; ++k) dst[k] = dst[k] + src[k];
   }
 }
@@ -417,7 +419,8 @@ __global__ void k_adj_corners(Buffers B, AdjBufs A, int cap_ghost) {
 // Every x- and y-ghost into the fluid particle it copies. The slots are found by
 // the arithmetic of the ghost emission (block offsets gblk, ballot ranks within
 // the block), so every fluid particle adds its own ghosts in a fixed order.
-__global__ void __launch_bounds__(kBlock) k_adj_owner(DevParams P, Buffers B, AdjBufs A, int cap_fluid) {
+__global__ void __launch_bounds__(kBlock) k_adj_owner(DevParams P, Buffers B, AdjBufs A, int cap_fluid,
+                                                      int cap_ghost) {
   __shared__ int sx[kBlock / 64], sy[kBlock / 64];
   const int b = blockIdx.x, nb = gridDim.x;
   const int nxg = B.counters[C_NXG];
@@ -446,6 +449,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_owner(DevParams P, Buffers B, Ad
   if (yl) slots[ns++] = k++;
   if (yr) slots[ns++] = k;
   for (int t = 0; t < ns; ++t) {
+    if (slots[t] >= cap_ghost) continue;               // not emitted: the capacity clamps the emission
     const double* src = A.G + 5 * (size_t)(N + slots[t]);
     for (int q = 0; q < 5; ++q) acc[q] = acc[q] + src[q];
   }
@@ -466,7 +470,7 @@ void launch_eval_vjp(hipStream_t st, const DevParams& P, Buffers& B, int cap_loc
   hipLaunchKernelGGL(k_adj_tau, dim3(g), dim3(kBlock), 0, st, P, B, A);
   hipLaunchKernelGGL(k_adj_sums, dim3(g), dim3(kBlock), 0, st, P, B, A);
   hipLaunchKernelGGL(k_adj_corners, dim3(grid_for(cap_ghost)), dim3(kBlock), 0, st, B, A, cap_ghost);
-  hipLaunchKernelGGL(k_adj_owner, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, st, P, B, A, cap_fluid);
+  hipLaunchKernelGGL(k_adj_owner, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, st, P, B, A, cap_fluid, cap_ghost);
 }
 
 }  // namespace sph2d

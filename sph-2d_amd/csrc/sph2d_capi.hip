@@ -50,6 +50,9 @@ struct sph2d_ctx {
   bool have_state = false;
   bool debug = false;
   bool evaluated = false;
+  // The reverse evaluation needs the local order, state, masks and flags of the last
+  // sph2d_single_step_compute unchanged; every entry that rewrites any of them clears this.
+  bool vjp_ready = false;
   bool mls_pending = false;
   double* d_out = nullptr;          // 16 doubles for reductions
   std::vector<void*> allocs;
@@ -500,6 +503,7 @@ int sph2d_init_tgv(sph2d_ctx* c) {
   if (rc) return rc;
   c->have_state = true;
   c->evaluated = false;
+  c->vjp_ready = false;
   return SPH2D_OK;
 }
 
@@ -546,6 +550,7 @@ int sph2d_upload_state(sph2d_ctx* c, int32_t n, const double* pos, const double*
   if (rc) return rc;
   c->have_state = true;
   c->evaluated = false;
+  c->vjp_ready = false;
   return SPH2D_OK;
 }
 
@@ -729,6 +734,7 @@ static int need_state(sph2d_ctx* c) {
 int sph2d_distribute_period_boundary(sph2d_ctx* c) {
   int rc = need_state(c);
   if (rc) return rc;
+  c->vjp_ready = false;
   if (c->prm.walls) return SPH2D_OK;                 // fixed walls: no periodic ghosts
   launch_period_boundary(c->stream, c->P, c->B, c->cap_fluid, c->cap_ghost);
   HIP_OR_FAIL(hipGetLastError());
@@ -738,6 +744,7 @@ int sph2d_distribute_period_boundary(sph2d_ctx* c) {
 static int devide_block(sph2d_ctx* c, int32_t it, int predict) {
   int rc = need_state(c);
   if (rc) return rc;
+  c->vjp_ready = false;
   const bool ana = (it % c->prm.sample_every) == 0;
   const double t = it * c->prm.dt / (c->prm.box_x / c->prm.U);   // Nondimensionalized_real_time
   launch_fill_i32(c->stream, c->B.counters + C_NILL, 1, 0);
@@ -781,6 +788,7 @@ int sph2d_single_step_compute(sph2d_ctx* c, int32_t it) {
   if (rc) return rc;
   do_eval(c, true, it);
   HIP_OR_FAIL(hipGetLastError());
+  c->vjp_ready = true;
   return SPH2D_OK;
 }
 
@@ -792,7 +800,9 @@ int sph2d_single_step_compute_vjp(sph2d_ctx* c, int32_t n, const double* lam_drh
   if (n != c->n_fluid) return fail(SPH2D_ERR_ARGUMENT, "n must equal the number of fluid particles");
   if (ld < n) return fail(SPH2D_ERR_ARGUMENT, "ld < n");
   if (!lam_drhodt || !lam_acc) return fail(SPH2D_ERR_ARGUMENT, "lam_drhodt and lam_acc are required");
-  if (!c->evaluated) return fail(SPH2D_ERR_STATE, "no force evaluation to reverse");
+  if (!c->vjp_ready)
+    return fail(SPH2D_ERR_STATE, "the reverse evaluation needs sph2d_single_step_compute as the context's last "
+                                 "state-changing call");
   if (c->P.mixed) return fail(SPH2D_ERR_UNSUPPORTED, "the reverse evaluation is built for the FP64 form");
   if (c->P.solver != SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && c->P.solver != SPH2D_SOLVER_LINEAR_RIEMANN)
     return fail(SPH2D_ERR_UNSUPPORTED, "the reverse evaluation is built for the two linear Riemann solvers");
@@ -814,6 +824,12 @@ int sph2d_single_step_compute_vjp(sph2d_ctx* c, int32_t n, const double* lam_drh
     if (rc) return rc;
   }
   double *d_lam = c->adj_buf, *w = d_lam + 5 * nf, *gs = w + 6 * nl, *ga = gs + 5 * nl, *G = ga + 5 * nl;
+  {
+    // a clamped ghost emission leaves slots unwritten that the fold would read
+    int cnt[C_COUNT];
+    rc = read_counters(c, cnt);
+    if (rc) return rc;
+  }
   std::vector<double> out(5 * (size_t)n);
   HIP_OR_FAIL(hipMemcpyAsync(d_lam, lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice, c->stream));
   launch_eval_vjp(c->stream, c->P, c->B, c->cap_local, c->cap_fluid, c->cap_ghost, c->cap_global, d_lam, w, gs, ga,
@@ -835,6 +851,7 @@ int sph2d_reinitialize_particle_density(sph2d_ctx* c, int32_t it) {
   int rc = need_state(c);
   if (rc) return rc;
   if (!c->evaluated) return fail(SPH2D_ERR_STATE, "no force evaluation in this step yet");
+  c->vjp_ready = false;
   launch_fill_i32(c->stream, c->B.counters + C_MLSFB, 1, 0);
   launch_mls(c->stream, c->P, c->B, c->cap_local);
   launch_shift_apply_mode(c->stream, c->P, c->B, c->cap_local, 2);   // commit rho
@@ -847,6 +864,7 @@ int sph2d_shift_particle_position(sph2d_ctx* c, int32_t it) {
   int rc = need_state(c);
   if (rc) return rc;
   if (!c->evaluated) return fail(SPH2D_ERR_STATE, "no force evaluation in this step yet");
+  c->vjp_ready = false;
   launch_shift_apply_mode(c->stream, c->P, c->B, c->cap_local, 0);
   HIP_OR_FAIL(hipGetLastError());
   return SPH2D_OK;
@@ -856,6 +874,7 @@ int sph2d_refresh_domain(sph2d_ctx* c, int32_t it) {
   (void)it;
   int rc = need_state(c);
   if (rc) return rc;
+  c->vjp_ready = false;
   launch_refresh(c->stream, c->P, c->B, c->cap_local);
   if (c->P.walls) launch_wall_refresh(c->stream, c->P, c->B, c->cap_local);
   HIP_OR_FAIL(hipGetLastError());
@@ -864,6 +883,7 @@ int sph2d_refresh_domain(sph2d_ctx* c, int32_t it) {
 
 // time_iteration.f90 PC (:509-657) / RK-variant (:779-1049), then :1064-1078.
 static int enqueue_step(sph2d_ctx* c, int it) {
+  c->vjp_ready = false;
   int rc = sph2d_distribute_period_boundary(c);
   if (rc) return rc;
   const bool pc = c->prm.integrator == SPH2D_INTEGRATOR_PC;
@@ -1266,6 +1286,7 @@ int sph2d_wavemaker_update(sph2d_ctx* c, double position, double velocity) {
     return fail(SPH2D_ERR_ARGUMENT, "paddle position and velocity must be finite");
   rc = probe_space(c, 1);
   if (rc) return rc;
+  c->vjp_ready = false;
   auto* bits = reinterpret_cast<unsigned long long*>(c->probe_buf);
   launch_fill_i32(c->stream, reinterpret_cast<int*>(bits), 2, 0);
   launch_wavemaker_place(c->stream, c->P, c->B, c->cap_fluid, c->n_wm, position, velocity, bits);
@@ -1279,6 +1300,7 @@ int sph2d_wavemaker_pull_back(sph2d_ctx* c, double position, double velocity) {
   if (rc) return rc;
   if (!std::isfinite(position) || !std::isfinite(velocity))
     return fail(SPH2D_ERR_ARGUMENT, "paddle position and velocity must be finite");
+  c->vjp_ready = false;
   launch_wavemaker_pull_back(c->stream, c->P, c->B, c->cap_fluid, position, velocity);
   HIP_OR_FAIL(hipGetLastError());
   return SPH2D_OK;
@@ -1809,6 +1831,7 @@ int sph2d_batch_run(sph2d_batch* b, int32_t first, int32_t nsteps) {
     const int rc = need_state(c);
     if (rc) return rc;
   }
+  for (sph2d_ctx* c : b->ctx) c->vjp_ready = false;
   int rc = SPH2D_OK;
   for (int k = 0; k < nsteps && rc == SPH2D_OK; ++k) {
     const int it = first + k;

@@ -131,8 +131,10 @@ module sph2d_interface
       import :: c_int, c_int32_t, c_ptr, c_double
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: n, ld
-      real(c_double), intent(in) :: lam_drhodt(ld), lam_acc(ld, 2), lam_avg(ld, 2)
-      real(c_double), intent(out) :: grad_pos(ld, 2), grad_vel(ld, 2), grad_rho(ld)
+      real(c_double), intent(in) :: lam_drhodt(ld), lam_acc(ld, 2)
+      ! absent optional dummies of a bind(C) interface pass NULL, as the header allows
+      real(c_double), intent(in), optional :: lam_avg(ld, 2)
+      real(c_double), intent(out), optional :: grad_pos(ld, 2), grad_vel(ld, 2), grad_rho(ld)
       integer(c_int) :: rc
     end function
 

@@ -34,6 +34,7 @@ DAMPING_OPENFOAM = 2
 DAMPING_DUALSPH = 3
 
 ERRORS = {1: "argument", 2: "capacity", 3: "hip", 4: "state", 5: "unsupported", 6: "nonfinite"}
+ERR_ARGUMENT, ERR_CAPACITY, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED, ERR_NONFINITE = 1, 2, 3, 4, 5, 6
 
 
 class SphError(RuntimeError):
@@ -336,11 +337,20 @@ class Simulation:
         ghosts' shares included. Neighbour set, surface flags and limiter branches
         are held fixed."""
         n = self.n_fluid
-        ld = np.ascontiguousarray(lam_drhodt, dtype=np.float64)
-        la = np.ascontiguousarray(np.concatenate([lam_ax, lam_ay]), dtype=np.float64)
+
+        def weight(a, name):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.ndim != 1 or a.shape[0] != n:
+                raise ValueError(f"{name} must be 1-D with one entry per fluid particle ({n}), got {a.shape}")
+            return a
+
+        if (lam_avg_x is None) != (lam_avg_y is None):
+            raise ValueError("lam_avg_x and lam_avg_y go together")
+        ld = weight(lam_drhodt, "lam_drhodt")
+        la = np.concatenate([weight(lam_ax, "lam_ax"), weight(lam_ay, "lam_ay")])
         lg = None
         if lam_avg_x is not None:
-            lg = np.ascontiguousarray(np.concatenate([lam_avg_x, lam_avg_y]), dtype=np.float64)
+            lg = np.concatenate([weight(lam_avg_x, "lam_avg_x"), weight(lam_avg_y, "lam_avg_y")])
         gp, gv, gr = np.zeros(2 * n), np.zeros(2 * n), np.zeros(n)
         _check(lib().sph2d_single_step_compute_vjp(self._c, n, _dp(ld), _dp(la), _dp(lg) if lg is not None else None,
                                                    n, _dp(gp), _dp(gv), _dp(gr)))

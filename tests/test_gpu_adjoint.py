@@ -218,3 +218,35 @@ def test_reverse_rejects_what_it_does_not_cover():
     sim.upload_state(*state, m)
     with pytest.raises(sph2d.SphError):
         sim.single_step_compute_vjp(*lam)       # no evaluation yet
+
+
+def test_vjp_refuses_a_state_changed_since_the_evaluation():
+    """The reverse needs the local order, state and flags of the last evaluation:
+    after a whole step, a fresh ordering or a shift it must refuse (SPH2D_ERR_STATE)
+    instead of returning the gradient of another function."""
+    say("test_vjp_refuses_a_state_changed_since_the_evaluation")
+    n = 24
+    st, m = noisy_state(n, 3)
+    sim = sph2d.Simulation(sph2d.tgv_params(n, integrator=2))
+    N = n * n
+    lam = np.ones(N)
+
+    def evaluate():
+        sim.upload_state(*st, m)
+        sim.distribute_period_boundary()
+        sim.devide_block(1)
+        sim.single_step_compute(1)
+
+    evaluate()
+    sim.single_step_compute_vjp(lam, lam, lam)            # accepted right after the evaluation
+    for change in (lambda: sim.time_step(1), lambda: sim.devide_block(1),
+                   lambda: sim.shift_particle_position(1), lambda: sim.refresh_domain(1)):
+        evaluate()
+        change()
+        with pytest.raises(sph2d.SphError) as e:
+            sim.single_step_compute_vjp(lam, lam, lam)
+        assert e.value.code == sph2d.ERR_STATE
+    evaluate()
+    with pytest.raises(ValueError):
+        sim.single_step_compute_vjp(lam[:-1], lam, lam)   # a weight array of the wrong length
+    sim.close()

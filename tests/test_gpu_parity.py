@@ -1,10 +1,14 @@
 """GPU parity of the HIP step (through the C-ABI) against the oracle.
 
 Bars: integer/index work (ghost generation, local order, pair set, flags) is
-bit-exact; FP64 fields agree with the oracle within 1e-12 relative to the largest
-magnitude of the field (of its vector or tensor: components share one scale; the gather sums run in another order than the
-reference's scatter sums, and exp() differs in the last bit); the Tait EOS,
-which has no sum and no exp, is bit-exact.
+bit-exact. The rates drho/dt and dv/dt agree PER PARTICLE within 1e-12 relative,
+|gpu_i - ref_i| / max(|ref_i|, 1e-3 sum_k |term_ik|), where sum_k |term_ik| is the
+oracle's sum of the magnitudes of particle i's pair terms (a rate that cancels to
+near zero is measured against the size of what cancelled: the gather sums run in
+another order than the reference's scatter sums, and exp() differs in the last
+bit). Intermediate FP64 fields agree within 1e-12 relative to the largest
+magnitude of the field (of its vector or tensor: components share one scale).
+The Tait EOS, which has no sum and no exp, is bit-exact.
 """
 import os
 import sys
@@ -32,6 +36,26 @@ def rel_err(g, r):
     if scale == 0:
         return float(np.max(np.abs(g)))
     return float(np.max(np.abs(g - r)) / scale)
+
+
+def particle_rel_err(g, r, mag):
+    """Per-particle |g - r| / max(|r|, 1e-3 * mag), its largest value."""
+    den = np.maximum(np.abs(r), 1e-3 * mag)
+    e = np.abs(g - r)
+    q = np.where(den > 0, e / np.where(den > 0, den, 1.0), e)
+    return float(np.max(q)) if len(q) else 0.0
+
+
+def rates_err(sim, o):
+    """Per-particle errors of drhodt, ax, ay (local order, which is bit-exact)."""
+    return {f: particle_rel_err(sim.local(f), o.f64(f), o.f64("mag_" + f)) for f in ("drhodt", "ax", "ay")}
+
+
+def assert_rates(sim, o, tol=TOL):
+    errs = rates_err(sim, o)
+    say("per-particle rate errors", errs)
+    for f, e in errs.items():
+        assert e < tol, (f, e)
 
 
 ORACLE_KW = ("shift_on", "mls_on", "solver", "damping", "damping_length", "damping_amplitude", "water_depth", "walls")
@@ -166,6 +190,7 @@ def test_single_step_compute_parity(n, perturb, solver):
     say(worst)
     for f, e in worst.items():
         assert e < TOL, (f, e)
+    assert_rates(sim, o)
     # EOS has no sum: bit-exact
     assert np.array_equal(sim.local("p"), o.f64("p"))
 
@@ -253,8 +278,8 @@ def test_free_surface_flags_bit_exact():
     assert s_o.sum() > 50
     assert np.array_equal(sim.local("surface"), s_o)
     assert np.array_equal(sim.local("near_surface"), o.i32("near_surface"))
-    for f in ("drhodt", "ax", "ay", "fra_v"):
-        assert rel_err(sim.local(f), o.f64(f)) < TOL, f
+    assert rel_err(sim.local("fra_v"), o.f64("fra_v")) < TOL
+    assert_rates(sim, o)
 
 
 def test_tgv_100_to_t05_matches_reference_run():
@@ -438,8 +463,7 @@ def test_end_damping_zone_parity(damping):
     kw = dict(damping=damping, damping_length=0.3, damping_amplitude=float(np.float32(0.3)), water_depth=1.0)
     sim, o = pair(n, integrator=2, **kw)
     evaluate_both(sim, o)
-    for f in ("ax", "ay", "drhodt"):
-        assert rel_err(sim.local(f), o.f64(f)) < TOL, f
+    assert_rates(sim, o)
     sim.close()
     sim, o = pair(n, integrator=2, **kw)
     free = Oracle(nx=n, ny=n, integrator=2, dt=sim.params.dt)
@@ -472,8 +496,8 @@ def test_fixed_walls_parity():
     assert np.array_equal(sim.local("type"), t) and (t == -3).sum() == 704
     for f in ("p", "u", "v", "rho", "mass"):
         assert rel_err(sim.local(f), o.f64(f)) < TOL, f
-    for f in ("drhodt", "ax", "ay", "fra_v"):
-        assert rel_err(sim.local(f), o.f64(f)) < TOL, f
+    assert rel_err(sim.local("fra_v"), o.f64("fra_v")) < TOL
+    assert_rates(sim, o)
     sim.close()
     sim, o = pair(n, integrator=2, walls=4, periodic=0)
     for it in range(1, 22):

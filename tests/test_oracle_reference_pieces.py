@@ -10,8 +10,17 @@ import pytest
 
 from oracle import pyoracle as po
 
-ref = po.ref_lib()
-pytestmark = pytest.mark.skipif(ref is None, reason="oracle/_ref not built (reference sources absent)")
+pytestmark = pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built (reference sources absent)")
+
+
+class _Lazy:
+    """The compiled reference, loaded on first use (never at collection)."""
+
+    def __getattr__(self, name):
+        return getattr(po.ref_lib(), name)
+
+
+ref = _Lazy()
 
 D = ctypes.c_double
 I = ctypes.c_int32

@@ -37,7 +37,7 @@ def ref_limiter(di, dj, dij):
     return li.value, lj.value
 
 
-@pytest.mark.skipif(po.ref_lib() is None, reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built (reference sources absent)")
 def test_muscl_limiter_bit_exact_vs_reference():
     rng = np.random.default_rng(3)
     cases = [(0.0, 0.0, 0.0), (1.0, 1.0, 2.0), (1.0, 1.0, 1.0), (1.0, -1.0, 2.0), (-1.0, 1.0, -2.0),
