@@ -98,6 +98,10 @@ typedef struct {
                                 y acceleration, hydrostatic term in the wall pressure */
   int32_t wavemaker_layers;  /* wave_maker_layer (:214) columns of piston wavemaker particles left of
                                 x = 0 (fix_wave_maker_or_not, make_wave_or_not = 1); needs walls */
+  int32_t sps_on;            /* 1: the solvers' _SPS forms (single_step_compute.f90:137, the default);
+                                0: their Delta forms without the SPS stress, e.g.
+                                Solver_Normalized_Modified_Linear_Riemann_Delta.f90, whose XSPH term
+                                is epsilon m_j v_ij / (rho_i + rho_j) W instead of 2 epsilon (...) */
 } sph2d_params;
 
 typedef struct sph2d_ctx sph2d_ctx;
@@ -192,6 +196,48 @@ int sph2d_single_step_compute(sph2d_ctx* ctx, int32_t i_time_step);
 int sph2d_single_step_compute_vjp(sph2d_ctx* ctx, int32_t n, const double* lam_drhodt, const double* lam_acc,
                                   const double* lam_avg, int32_t ld, double* grad_pos, double* grad_vel,
                                   double* grad_rho);
+/* --- the evaluation routine by routine (single_step_compute.f90:62-171) --------
+ * sph2d_find_particle_pair, sph2d_define_freesurface and sph2d_solver, called in
+ * that order after sph2d_devide_block (or a stage update), give the same results,
+ * bit for bit, as sph2d_single_step_compute. */
+/* find_particle_pair.f90:21-163, with the EOS and volumes of single_step_compute.f90:
+ * 62-80 before it: the pair set of the current local state (1e-7 <= r <= 3 hbar, one
+ * of the two fluid), recorded per particle. The same neighbour walk forms the sums
+ * the later routines read: fra_v and sum dW of Define_freesurface_and_Normailized_
+ * derivates.f90:79-159, the SPS strain -> tau (Solver...SPS.f90:144-296) and grad rho
+ * (:508-526). */
+int sph2d_find_particle_pair(sph2d_ctx* ctx, int32_t i_time_step);
+/* The pair list of the last sph2d_find_particle_pair as the reference holds it
+ * (find_particle_pair.f90:64-123): pair_i ascending, then the candidates j > i in the
+ * stencil order self, left, right, upper left, up, upper right, lower left, down,
+ * lower right (Distribute_Backgroud_Grid.f90:102-112), each cell's particles in
+ * local order; 1-based LOCAL indices (sph2d_download_local "name" maps them to
+ * gids). capacity 0 (or NULL arrays) only counts. */
+int sph2d_download_pair_list(sph2d_ctx* ctx, int64_t capacity, int32_t* pair_i, int32_t* pair_j,
+                             int64_t* pair_number);
+/* Define_freesurface_and_Normailized_derivates.f90:21-444 after sph2d_find_particle_pair:
+ * surface flags (fra_v <= 0.65), unit normals with the geometric un-flag tests,
+ * near-surface flags; the renormalisation inverse is the identity, as in the reference. */
+int sph2d_define_freesurface(sph2d_ctx* ctx);
+/* Solver_Normalized_<variant>_Riemann_Delta[_SPS].f90 and the rate assembly of
+ * single_step_compute.f90:152-260 (gravity, end damping) after sph2d_define_freesurface.
+ * variant: SPH2D_SOLVER_*; MUSCL needs a context created with the MUSCL solver (its
+ * gradients are formed in the pair walk). sps_on 0 gives the Delta forms without SPS
+ * (see sph2d_params.sps_on; it clears this evaluation's tau); 1 needs a context
+ * created with sps_on 1. */
+int sph2d_solver(sph2d_ctx* ctx, int32_t variant, int32_t sps_on);
+/* Stage updates of time_iteration.f90 on the fluid particles, with the rates of the
+ * last evaluation by local index (the reference's re-use). scheme SPH2D_INTEGRATOR_PC:
+ * stage 1 = predictor (:553-576: before_* = state, Euler step, temp_* = the rates),
+ * stage 2 = corrector (:590-599). scheme SPH2D_INTEGRATOR_RK: stages 1, 2, 3 = the
+ * K_1..K_3 updates (:825-836, :859-870, :889-900; stage 1 also sets before_*),
+ * stage 4 = the final combination (:918-928). stage 0 (either scheme) = before_* =
+ * state only. */
+int sph2d_stage_update(sph2d_ctx* ctx, int32_t scheme, int32_t stage);
+/* drhodt(ld) and particle_acceleration(ld,2) of the last evaluation for the n fluid
+ * particles in global (reference) order (gid = reference index - 1). Either may be NULL. */
+int sph2d_download_rates(sph2d_ctx* ctx, int32_t n, double* drhodt, double* acc, int32_t ld);
+
 /* Reinitialize_particle_density.f90:66-182 (MLS order 6 or 1). */
 int sph2d_reinitialize_particle_density(sph2d_ctx* ctx, int32_t i_time_step);
 /* Shift_particle_position.f90:56-136 with the last evaluation's pair data. */

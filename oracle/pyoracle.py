@@ -41,6 +41,7 @@ class OrcParams(ctypes.Structure):
         ("damping_amplitude", ctypes.c_double), ("water_depth", ctypes.c_double),
         ("walls", ctypes.c_int32), ("gravity", ctypes.c_double),
         ("wavemaker_layers", ctypes.c_int32),
+        ("sps_on", ctypes.c_int32),
     ]
 
 

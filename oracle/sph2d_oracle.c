@@ -63,9 +63,12 @@ struct orc_ctx {
   double *ia_x, *ia_y, *av_x, *av_y, *ai_x, *ai_y, *sps_x, *sps_y, *lap_x, *lap_y, *ext_x, *ext_y;
   double *avg_x, *avg_y, *corr, *grx, *gry;
   double *gpx, *gpy, *gux, *guy, *gvx, *gvy;    /* MUSCL: grad_pressure, grad_velocity_u/_v */
-  /* sum over the pair terms of |term| per particle for drhodt, ax, ay: the scale of
-   * each rate before cancellation (test metric only, no reference counterpart) */
-  double *mag_drho, *mag_ax, *mag_ay;
+  /* first-order rounding scale of drhodt, ax, ay per particle (test metric only, no
+   * reference counterpart): the sum over the pair terms of each term evaluated with
+   * the magnitudes of the operands that cancel inside it (v_i - v*, the pressure
+   * and its dissipation, psi = (rho_j - rho_i) - grad rho . x_ji with the rounding
+   * scale of grad rho); mag_grho is that scale of grad rho */
+  double *mag_drho, *mag_ax, *mag_ay, *mag_grho;
   /* shift */
   double *shx, *shy;
   /* Refresh_domain's vorticity (z), local order; global vorticity, surface flags and
@@ -121,6 +124,7 @@ void orc_default_params(orc_params* p) {
   p->walls = 0;                                    /* periodic box (TGV default)     */
   p->gravity = 0.0;                                /* Gravity_Switch = 'off'         */
   p->wavemaker_layers = 0;                         /* make_wave_or_not = 0           */
+  p->sps_on = 1;                                   /* Solver_..._SPS (single_step_compute.f90:137) */
 }
 
 /* ------------------------------------------------------------------ */
@@ -239,7 +243,7 @@ static void alloc_local(orc_ctx* c, int cap) {
   A(ia_x); A(ia_y); A(av_x); A(av_y); A(ai_x); A(ai_y); A(sps_x); A(sps_y);
   A(lap_x); A(lap_y); A(ext_x); A(ext_y); A(avg_x); A(avg_y); A(corr); A(grx); A(gry);
   A(gpx); A(gpy); A(gux); A(guy); A(gvx); A(gvy);
-  A(mag_drho); A(mag_ax); A(mag_ay);
+  A(mag_drho); A(mag_ax); A(mag_ay); A(mag_grho);
   A(shx); A(shy); A(vort);
   A(b_rho); A(b_e); A(b_x); A(b_y); A(b_u); A(b_v);
   A(t_u); A(t_v); A(t_drho); A(t_de); A(t_ax); A(t_ay);
@@ -288,7 +292,7 @@ void orc_destroy(orc_ctx* c) {
                   c->Syy, c->txx, c->txy, c->tyy, c->drhodt, c->dedt, c->ax, c->ay, c->ia_x,
                   c->ia_y, c->av_x, c->av_y, c->ai_x, c->ai_y, c->sps_x, c->sps_y, c->lap_x,
                   c->lap_y, c->ext_x, c->ext_y, c->avg_x, c->avg_y, c->corr, c->grx, c->gry,
-                  c->gpx, c->gpy, c->gux, c->guy, c->gvx, c->gvy, c->mag_drho, c->mag_ax, c->mag_ay, c->shx, c->shy, c->vort, c->b_rho, c->b_e, c->b_x, c->b_y, c->b_u, c->b_v, c->t_u,
+                  c->gpx, c->gpy, c->gux, c->guy, c->gvx, c->gvy, c->mag_drho, c->mag_ax, c->mag_ay, c->mag_grho, c->shx, c->shy, c->vort, c->b_rho, c->b_e, c->b_x, c->b_y, c->b_u, c->b_v, c->t_u,
                   c->t_v, c->t_drho, c->t_de, c->t_ax, c->t_ay, c->mls_rho,
                   c->cell_count, c->cell_list, c->near_n, c->near_name,
                   c->pi, c->pj, c->pw, c->pdwx, c->pdwy, c->pdx, c->pdy, c->pvx, c->pvy, c->prd,
@@ -1040,6 +1044,7 @@ static void solver_riemann_delta_sps(orc_ctx* c) {
   const double c23b = (double)((2.0f / 3.0f) * 0.0066f);
   for (int i = 0; i < n; ++i) {                                   /* :256-296 */
     if (c->type[i] != 2) continue;
+    if (!P->sps_on) continue;          /* Solver_..._Delta.f90 has no SPS stress: tau stays 0 */
     double master = c->Sxx[i] * c->Sxx[i] + c->Syy[i] * c->Syy[i] + 0.0;
     double last = c->Sxy[i] * c->Sxy[i] + 0.0 + 0.0;
     double abs2 = 2 * master + last;
@@ -1137,20 +1142,28 @@ static void solver_riemann_delta_sps(orc_ctx* c) {
     double tj = -((Jv[0] - Jvs[0]) * dj[0] + (Jv[1] - Jvs[1]) * dj[1]);
     c->drhodt[i] = c->drhodt[i] + 2 * c->rho[i] * c->vol[j] * ti;
     c->drhodt[j] = c->drhodt[j] + 2 * c->rho[j] * c->vol[i] * tj;
-    c->mag_drho[i] += fabs(2 * c->rho[i] * c->vol[j] * ti);
-    c->mag_drho[j] += fabs(2 * c->rho[j] * c->vol[i] * tj);
+    {
+      /* v* is built from v_i, v_j and (P_L - P_R)/(rho c): its rounding scale */
+      const double vsc = fabs(Iv[0]) + fabs(Iv[1]) + fabs(Jv[0]) + fabs(Jv[1]) +
+                         (fabs(PL) + fabs(PR)) / (rb * sqc0);
+      const double ai = (fabs(di[0]) + fabs(di[1])) * vsc, aj = (fabs(dj[0]) + fabs(dj[1])) * vsc;
+      c->mag_drho[i] += 2 * c->rho[i] * c->vol[j] * ai;
+      c->mag_drho[j] += 2 * c->rho[j] * c->vol[i] * aj;
+    }
     double rij = 1.0 / (c->rho[i] * c->rho[j]);
     double* iax[2] = {&c->ia_x[i], &c->ia_y[i]};
     double* jax[2] = {&c->ia_x[j], &c->ia_y[j]};
     double* imag[2] = {&c->mag_ax[i], &c->mag_ay[i]};
     double* jmag[2] = {&c->mag_ax[j], &c->mag_ay[j]};
+    /* P* = mean pressure plus a dissipation ~ rho c (U_L - U_R): its rounding scale */
+    const double psc = fabs(PL) + fabs(PR) + rb * cb * (fabs(UL) + fabs(UR));
     for (int L = 0; L < 2; ++L) {
       double tpi = -2 * IPs * di[L] * rij;
       double tpj = -2 * JPs * dj[L] * rij;
       *iax[L] = *iax[L] + c->mass[j] * tpi;
       *jax[L] = *jax[L] - c->mass[i] * tpj;
-      *imag[L] += fabs(c->mass[j] * tpi);
-      *jmag[L] += fabs(c->mass[i] * tpj);
+      *imag[L] += c->mass[j] * 2 * psc * fabs(di[L]) * rij;
+      *jmag[L] += c->mass[i] * 2 * psc * fabs(dj[L]) * rij;
     }
     double hb = c->ph_[k];
     double vij[2] = {c->pvx[k], c->pvy[k]};
@@ -1207,14 +1220,17 @@ static void solver_riemann_delta_sps(orc_ctx* c) {
     c->mag_ay[i] += fabs(c->gmass[j] * tp);
     c->mag_ay[j] += fabs(c->gmass[i] * tp);
     if (P->solver == ORC_SOLVER_MODIFIED_LINEAR_RIEMANN) {       /* :495-500 XSPH */
+      /* 2 epsilon in the _SPS solver; epsilon in Solver_..._Delta.f90:139-140 */
+      const double xf = P->sps_on ? 2.0 : 1.0;
       double rs = c->rho[i] + c->rho[j];
-      c->avg_x[i] = c->avg_x[i] - P->epsilon_xsph * 2 * c->mass[j] * vij[0] / rs * c->pw[k];
-      c->avg_y[i] = c->avg_y[i] - P->epsilon_xsph * 2 * c->mass[j] * vij[1] / rs * c->pw[k];
-      c->avg_x[j] = c->avg_x[j] + P->epsilon_xsph * 2 * c->mass[i] * vij[0] / rs * c->pw[k];
-      c->avg_y[j] = c->avg_y[j] + P->epsilon_xsph * 2 * c->mass[i] * vij[1] / rs * c->pw[k];
+      c->avg_x[i] = c->avg_x[i] - P->epsilon_xsph * xf * c->mass[j] * vij[0] / rs * c->pw[k];
+      c->avg_y[i] = c->avg_y[i] - P->epsilon_xsph * xf * c->mass[j] * vij[1] / rs * c->pw[k];
+      c->avg_x[j] = c->avg_x[j] + P->epsilon_xsph * xf * c->mass[i] * vij[0] / rs * c->pw[k];
+      c->avg_y[j] = c->avg_y[j] + P->epsilon_xsph * xf * c->mass[i] * vij[1] / rs * c->pw[k];
     }
   }
   memset(c->grx, 0, sizeof(double) * n); memset(c->gry, 0, sizeof(double) * n);
+  memset(c->mag_grho, 0, sizeof(double) * n);
   for (int64_t k = 0; k < c->npair; ++k) {                       /* :508-526 */
     int i = c->pi[k], j = c->pj[k];
     double rd = -c->prd[k];
@@ -1227,6 +1243,8 @@ static void solver_riemann_delta_sps(orc_ctx* c) {
     c->gry[i] = c->gry[i] + rd * c->vol[j] * mi2;
     c->grx[j] = c->grx[j] + rd * c->vol[i] * mj1;
     c->gry[j] = c->gry[j] + rd * c->vol[i] * mj2;
+    c->mag_grho[i] += fabs(rd) * c->vol[j] * (fabs(mi1) + fabs(mi2));
+    c->mag_grho[j] += fabs(rd) * c->vol[i] * (fabs(mj1) + fabs(mj2));
   }
   for (int64_t k = 0; k < c->npair; ++k) {                       /* :552-583 */
     int i = c->pi[k], j = c->pj[k];
@@ -1240,8 +1258,14 @@ static void solver_riemann_delta_sps(orc_ctx* c) {
     double tj = (pd[0] * c->ndjx[k] + pd[1] * c->ndjy[k]) / r2;
     c->corr[i] = c->corr[i] + 2 * P->delta * hb * cb * psi * ti * c->vol[j];
     c->corr[j] = c->corr[j] - 2 * P->delta * hb * cb * psi * tj * c->vol[i];
-    c->mag_drho[i] += fabs(2 * P->delta * hb * cb * psi * ti * c->vol[j]);
-    c->mag_drho[j] += fabs(2 * P->delta * hb * cb * psi * tj * c->vol[i]);
+    {
+      /* psi cancels to second order: its scale is |rho_j - rho_i| plus the grad rho
+       * products and their own rounding scale */
+      const double psc = fabs(rd) + 0.5 * (fabs(sg[0] * pd[0]) + fabs(sg[1] * pd[1])) +
+                         0.5 * (c->mag_grho[i] + c->mag_grho[j]) * (fabs(pd[0]) + fabs(pd[1]));
+      c->mag_drho[i] += fabs(2 * P->delta * hb * cb * psc * ti * c->vol[j]);
+      c->mag_drho[j] += fabs(2 * P->delta * hb * cb * psc * tj * c->vol[i]);
+    }
   }
   for (int i = 0; i < n; ++i) {                                   /* :627-635 */
     if (c->type[i] != 2) continue;

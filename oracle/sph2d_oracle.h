@@ -54,6 +54,7 @@ typedef struct {
   /* piston wavemaker particles (fix_wave_maker_or_not, wave_maker_layer :213-214,  */
   /* PistonType): columns of type-130 particles left of x = 0, before the walls    */
   int32_t wavemaker_layers;
+  int32_t sps_on;            /* 1: _SPS solvers; 0: Delta solvers without SPS (XSPH factor 1) */
 } orc_params;
 
 enum { ORC_SOLVER_MODIFIED_LINEAR_RIEMANN = 0,   /* Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90 */

@@ -49,19 +49,21 @@ FORTRAN = os.path.join(HERE, "fortran")
 
 
 def build_fortran(verbose: bool = False) -> str:
-    """The ISO_C_BINDING interface module and the Fortran example driver, linked
-    against libsph2d.so (skipped when flang is absent)."""
+    """The ISO_C_BINDING interface module and the Fortran drivers (the whole step,
+    and the step routine by routine), linked against libsph2d.so (skipped when
+    flang is absent)."""
     if not os.path.exists(FC):
         return ""
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
-    exe = os.path.join(FORTRAN, "tgv_driver")
-    cmd = [FC, "-O2", "-J", bdir, "-o", exe,
-           os.path.join(FORTRAN, "sph2d_interface.f90"), os.path.join(FORTRAN, "tgv_driver.f90"),
-           "-L" + HERE, "-lsph2d", "-Wl,-rpath,$ORIGIN/.."]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    _run(cmd)
+    for prog in ("tgv_driver", "routine_driver"):
+        exe = os.path.join(FORTRAN, prog)
+        cmd = [FC, "-O2", "-J", bdir, "-o", exe,
+               os.path.join(FORTRAN, "sph2d_interface.f90"), os.path.join(FORTRAN, prog + ".f90"),
+               "-L" + HERE, "-lsph2d", "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        _run(cmd)
     return exe
 
 

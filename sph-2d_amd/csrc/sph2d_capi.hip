@@ -53,6 +53,9 @@ struct sph2d_ctx {
   // The reverse evaluation needs the local order, state, masks and flags of the last
   // sph2d_single_step_compute unchanged; every entry that rewrites any of them clears this.
   bool vjp_ready = false;
+  // the evaluation routine by routine: 0 none since the last state change, 1 pairs
+  // found, 2 free surface defined, 3 solved
+  int eval_part = 0;
   bool mls_pending = false;
   double* d_out = nullptr;          // 16 doubles for reductions
   std::vector<void*> allocs;
@@ -174,6 +177,8 @@ void make_devparams(const sph2d_params& p, DevParams& P) {
   P.mls_order = p.mls_order;
   P.write_debug = 0;
   P.mixed = p.precision == SPH2D_FP32_MIXED ? 1 : 0;
+  P.sps = p.sps_on;
+  P.xsph_f = p.sps_on ? 2.0 : 1.0;
 }
 
 int check_params(const sph2d_params* p) {
@@ -204,6 +209,7 @@ int check_params(const sph2d_params* p) {
     return fail(SPH2D_ERR_ARGUMENT, "mls_every and sample_every must be positive");
   if (p->damping < SPH2D_DAMPING_OFF || p->damping > SPH2D_DAMPING_DUALSPH)
     return fail(SPH2D_ERR_ARGUMENT, "damping must be one of SPH2D_DAMPING_*");
+  if (p->sps_on != 0 && p->sps_on != 1) return fail(SPH2D_ERR_ARGUMENT, "sps_on must be 0 or 1");
   if (p->damping != SPH2D_DAMPING_OFF && (!(p->damping_length > 0) || !(p->water_depth > 0)))
     return fail(SPH2D_ERR_ARGUMENT, "damping_length and water_depth must be positive");
   return SPH2D_OK;
@@ -296,6 +302,7 @@ int sph2d_default_params(sph2d_params* p) {
   p->walls = 0;                                    // DistributeFixedBoundaryParticleOrNot = 0
   p->gravity = 0.0;                                // Gravity_Switch = 'off'
   p->wavemaker_layers = 0;                         // make_wave_or_not = 0
+  p->sps_on = 1;                                   // Solver_..._SPS (single_step_compute.f90:137)
   return SPH2D_OK;
 }
 
@@ -504,6 +511,7 @@ int sph2d_init_tgv(sph2d_ctx* c) {
   c->have_state = true;
   c->evaluated = false;
   c->vjp_ready = false;
+  c->eval_part = 0;
   return SPH2D_OK;
 }
 
@@ -551,6 +559,7 @@ int sph2d_upload_state(sph2d_ctx* c, int32_t n, const double* pos, const double*
   c->have_state = true;
   c->evaluated = false;
   c->vjp_ready = false;
+  c->eval_part = 0;
   return SPH2D_OK;
 }
 
@@ -735,6 +744,7 @@ int sph2d_distribute_period_boundary(sph2d_ctx* c) {
   int rc = need_state(c);
   if (rc) return rc;
   c->vjp_ready = false;
+  c->eval_part = 0;
   if (c->prm.walls) return SPH2D_OK;                 // fixed walls: no periodic ghosts
   launch_period_boundary(c->stream, c->P, c->B, c->cap_fluid, c->cap_ghost);
   HIP_OR_FAIL(hipGetLastError());
@@ -745,6 +755,7 @@ static int devide_block(sph2d_ctx* c, int32_t it, int predict) {
   int rc = need_state(c);
   if (rc) return rc;
   c->vjp_ready = false;
+  c->eval_part = 0;
   const bool ana = (it % c->prm.sample_every) == 0;
   const double t = it * c->prm.dt / (c->prm.box_x / c->prm.U);   // Nondimensionalized_real_time
   launch_fill_i32(c->stream, c->B.counters + C_NILL, 1, 0);
@@ -789,6 +800,131 @@ int sph2d_single_step_compute(sph2d_ctx* c, int32_t it) {
   do_eval(c, true, it);
   HIP_OR_FAIL(hipGetLastError());
   c->vjp_ready = true;
+  c->eval_part = 3;
+  return SPH2D_OK;
+}
+
+// ---- the evaluation routine by routine ----------------------------------------------
+int sph2d_find_particle_pair(sph2d_ctx* c, int32_t it) {
+  (void)it;
+  int rc = need_state(c);
+  if (rc) return rc;
+  c->vjp_ready = false;
+  launch_stage(c->stream, c->P, c->B, c->cap_local, ST_EOS);     // single_step_compute.f90:62-73
+  c->P.write_debug = c->debug ? 1 : 0;
+  if (c->P.walls) launch_assign_walls(c->stream, c->P, c->B, c->cap_local);   // :80
+  launch_eval_part(c->stream, c->P, c->B, c->cap_local, 0);
+  HIP_OR_FAIL(hipGetLastError());
+  c->eval_part = 1;
+  return SPH2D_OK;
+}
+
+int sph2d_download_pair_list(sph2d_ctx* c, int64_t capacity, int32_t* pi, int32_t* pj, int64_t* np_out) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (!np_out) return fail(SPH2D_ERR_ARGUMENT, "pair_number is NULL");
+  if (c->eval_part < 1) return fail(SPH2D_ERR_STATE, "no pair search on the current state: call sph2d_find_particle_pair");
+  launch_count_pair_list(c->stream, c->P, c->B, c->cap_local);
+  int cnt[C_COUNT];
+  rc = read_counters(c, cnt);
+  if (rc) return rc;
+  const int64_t np = (int64_t)(unsigned)cnt[C_NPAIR_LO];
+  *np_out = np;
+  if (!pi || !pj || capacity <= 0) return SPH2D_OK;
+  if (capacity < np) return fail(SPH2D_ERR_CAPACITY, "pair capacity too small");
+  DeviceScratch tmp;
+  int *di = nullptr, *dj = nullptr;
+  HIP_OR_FAIL(tmp.get(&di, (size_t)np));
+  HIP_OR_FAIL(tmp.get(&dj, (size_t)np));
+  Buffers B = c->B;
+  B.pair_lo = di; B.pair_hi = dj;
+  launch_pair_list(c->stream, c->P, B, c->cap_local, np);
+  HIP_OR_FAIL(hipGetLastError());
+  HIP_OR_FAIL(hipMemcpyAsync(pi, di, np * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(pj, dj, np * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  return SPH2D_OK;
+}
+
+int sph2d_define_freesurface(sph2d_ctx* c) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (c->eval_part != 1) return fail(SPH2D_ERR_STATE, "call sph2d_find_particle_pair first");
+  launch_eval_part(c->stream, c->P, c->B, c->cap_local, 1);
+  HIP_OR_FAIL(hipGetLastError());
+  c->eval_part = 2;
+  return SPH2D_OK;
+}
+
+int sph2d_solver(sph2d_ctx* c, int32_t variant, int32_t sps_on) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (c->eval_part != 2) return fail(SPH2D_ERR_STATE, "call sph2d_define_freesurface first");
+  if (variant < SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN || variant > SPH2D_SOLVER_MUSCL)
+    return fail(SPH2D_ERR_UNSUPPORTED, "variant must be one of SPH2D_SOLVER_*");
+  if (sps_on != 0 && sps_on != 1) return fail(SPH2D_ERR_ARGUMENT, "sps_on must be 0 or 1");
+  if (variant == SPH2D_SOLVER_MUSCL && c->P.solver != SPH2D_SOLVER_MUSCL)
+    return fail(SPH2D_ERR_UNSUPPORTED, "the MUSCL gradients are formed by the pair walk of a context created "
+                                       "with the MUSCL solver");
+  if (sps_on && !c->P.sps)
+    return fail(SPH2D_ERR_UNSUPPORTED, "tau is formed by the pair walk of a context created with sps_on = 1");
+  DevParams Q = c->P;
+  Q.solver = variant;
+  Q.sps = sps_on;
+  Q.xsph_f = sps_on ? 2.0 : 1.0;
+  Q.write_debug = c->debug ? 1 : 0;
+  if (!sps_on && c->P.sps) launch_clear_tau(c->stream, Q, c->B, c->cap_local);
+  launch_eval_part(c->stream, Q, c->B, c->cap_local, 2);
+  if (Q.damping) launch_end_damping(c->stream, Q, c->B, c->cap_local);   // single_step_compute.f90:174-260
+  HIP_OR_FAIL(hipGetLastError());
+  c->evaluated = true;
+  c->eval_part = 3;
+  c->vjp_ready = variant == c->P.solver && sps_on == c->P.sps;
+  return SPH2D_OK;
+}
+
+int sph2d_stage_update(sph2d_ctx* c, int32_t scheme, int32_t stage) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  int kind = -1;
+  if (stage == 0) kind = ST_SAVE_BEFORE;
+  else if (scheme == SPH2D_INTEGRATOR_PC && stage == 1) kind = ST_PC_PREDICT;
+  else if (scheme == SPH2D_INTEGRATOR_PC && stage == 2) kind = ST_PC_CORRECT;
+  else if (scheme == SPH2D_INTEGRATOR_RK && stage >= 1 && stage <= 4)
+    kind = stage == 1 ? ST_RK_HALF0 : (stage == 2 ? ST_RK_HALF1 : (stage == 3 ? ST_RK_FULL2 : ST_RK_FINAL));
+  if (kind < 0 || (scheme != SPH2D_INTEGRATOR_PC && scheme != SPH2D_INTEGRATOR_RK))
+    return fail(SPH2D_ERR_ARGUMENT, "scheme must be SPH2D_INTEGRATOR_PC (stages 0-2) or SPH2D_INTEGRATOR_RK (0-4)");
+  if (kind != ST_SAVE_BEFORE && !c->evaluated) return fail(SPH2D_ERR_STATE, "no force evaluation yet");
+  c->vjp_ready = false;
+  c->eval_part = 0;
+  launch_stage(c->stream, c->P, c->B, c->cap_local, kind);
+  HIP_OR_FAIL(hipGetLastError());
+  return SPH2D_OK;
+}
+
+int sph2d_download_rates(sph2d_ctx* c, int32_t n, double* drhodt, double* acc, int32_t ld) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (n != c->n_fluid) return fail(SPH2D_ERR_ARGUMENT, "n must equal the number of fluid particles");
+  if (ld < n) return fail(SPH2D_ERR_ARGUMENT, "ld < n");
+  if (!c->evaluated) return fail(SPH2D_ERR_STATE, "no force evaluation yet");
+  int cnt[C_COUNT];
+  rc = read_counters(c, cnt);
+  if (rc) return rc;
+  const int nl = cnt[C_NLOCAL];
+  std::vector<int> name(nl);
+  std::vector<double> d(nl);
+  std::vector<double2> a(nl);
+  HIP_OR_FAIL(hipMemcpyAsync(name.data(), c->B.name, nl * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(d.data(), c->B.drhodt, nl * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(a.data(), c->B.acc, nl * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  for (int s = 0; s < nl; ++s) {
+    const int g = name[s];
+    if (g < 0 || g >= n) continue;                   // ghosts, wavemaker and wall particles
+    if (drhodt) drhodt[g] = d[s];
+    if (acc) { acc[g] = a[s].x; acc[(size_t)ld + g] = a[s].y; }
+  }
   return SPH2D_OK;
 }
 
@@ -804,6 +940,7 @@ int sph2d_single_step_compute_vjp(sph2d_ctx* c, int32_t n, const double* lam_drh
     return fail(SPH2D_ERR_STATE, "the reverse evaluation needs sph2d_single_step_compute as the context's last "
                                  "state-changing call");
   if (c->P.mixed) return fail(SPH2D_ERR_UNSUPPORTED, "the reverse evaluation is built for the FP64 form");
+  if (!c->P.sps) return fail(SPH2D_ERR_UNSUPPORTED, "the reverse evaluation is built for the _SPS solvers");
   if (c->P.solver != SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && c->P.solver != SPH2D_SOLVER_LINEAR_RIEMANN)
     return fail(SPH2D_ERR_UNSUPPORTED, "the reverse evaluation is built for the two linear Riemann solvers");
   if (c->P.walls || c->P.damping || c->n_wm)
@@ -852,6 +989,7 @@ int sph2d_reinitialize_particle_density(sph2d_ctx* c, int32_t it) {
   if (rc) return rc;
   if (!c->evaluated) return fail(SPH2D_ERR_STATE, "no force evaluation in this step yet");
   c->vjp_ready = false;
+  c->eval_part = 0;
   launch_fill_i32(c->stream, c->B.counters + C_MLSFB, 1, 0);
   launch_mls(c->stream, c->P, c->B, c->cap_local);
   launch_shift_apply_mode(c->stream, c->P, c->B, c->cap_local, 2);   // commit rho
@@ -865,6 +1003,7 @@ int sph2d_shift_particle_position(sph2d_ctx* c, int32_t it) {
   if (rc) return rc;
   if (!c->evaluated) return fail(SPH2D_ERR_STATE, "no force evaluation in this step yet");
   c->vjp_ready = false;
+  c->eval_part = 0;
   launch_shift_apply_mode(c->stream, c->P, c->B, c->cap_local, 0);
   HIP_OR_FAIL(hipGetLastError());
   return SPH2D_OK;
@@ -875,6 +1014,7 @@ int sph2d_refresh_domain(sph2d_ctx* c, int32_t it) {
   int rc = need_state(c);
   if (rc) return rc;
   c->vjp_ready = false;
+  c->eval_part = 0;
   launch_refresh(c->stream, c->P, c->B, c->cap_local);
   if (c->P.walls) launch_wall_refresh(c->stream, c->P, c->B, c->cap_local);
   HIP_OR_FAIL(hipGetLastError());
@@ -884,6 +1024,7 @@ int sph2d_refresh_domain(sph2d_ctx* c, int32_t it) {
 // time_iteration.f90 PC (:509-657) / RK-variant (:779-1049), then :1064-1078.
 static int enqueue_step(sph2d_ctx* c, int it) {
   c->vjp_ready = false;
+  c->eval_part = 0;
   int rc = sph2d_distribute_period_boundary(c);
   if (rc) return rc;
   const bool pc = c->prm.integrator == SPH2D_INTEGRATOR_PC;
@@ -1287,6 +1428,7 @@ int sph2d_wavemaker_update(sph2d_ctx* c, double position, double velocity) {
   rc = probe_space(c, 1);
   if (rc) return rc;
   c->vjp_ready = false;
+  c->eval_part = 0;
   auto* bits = reinterpret_cast<unsigned long long*>(c->probe_buf);
   launch_fill_i32(c->stream, reinterpret_cast<int*>(bits), 2, 0);
   launch_wavemaker_place(c->stream, c->P, c->B, c->cap_fluid, c->n_wm, position, velocity, bits);
@@ -1301,6 +1443,7 @@ int sph2d_wavemaker_pull_back(sph2d_ctx* c, double position, double velocity) {
   if (!std::isfinite(position) || !std::isfinite(velocity))
     return fail(SPH2D_ERR_ARGUMENT, "paddle position and velocity must be finite");
   c->vjp_ready = false;
+  c->eval_part = 0;
   launch_wavemaker_pull_back(c->stream, c->P, c->B, c->cap_fluid, position, velocity);
   HIP_OR_FAIL(hipGetLastError());
   return SPH2D_OK;
@@ -1831,7 +1974,7 @@ int sph2d_batch_run(sph2d_batch* b, int32_t first, int32_t nsteps) {
     const int rc = need_state(c);
     if (rc) return rc;
   }
-  for (sph2d_ctx* c : b->ctx) c->vjp_ready = false;
+  for (sph2d_ctx* c : b->ctx) { c->vjp_ready = false; c->eval_part = 0; }
   int rc = SPH2D_OK;
   for (int k = 0; k < nsteps && rc == SPH2D_OK; ++k) {
     const int it = first + k;

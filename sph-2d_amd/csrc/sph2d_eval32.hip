@@ -55,7 +55,7 @@ static DevParams32 make32(const DevParams& P) {
   Q.shift_beta_h2 = (float)(P.shift_beta * P.h2);
   Q.shift_R = (float)P.shift_R;
   Q.inv_wdx = (float)P.inv_wdx;
-  Q.epsilon2 = (float)(P.epsilon_xsph * 2);
+  Q.epsilon2 = (float)(P.epsilon_xsph * P.xsph_f);
   Q.x_over_t = (float)P.x_over_t;
   Q.inv_rho0 = (float)(1.0 / P.rho0);
   Q.eos_b = (float)P.eos_b;
@@ -162,7 +162,7 @@ __device__ __forceinline__ void k_eval_a32_body(const DevParams& P, const DevPar
     B.mg[3 * s + 2] = make_double2((double)gvx.s, (double)gvy.s);
   }
   double txx = 0.0, txy = 0.0, tyy = 0.0;
-  if (ts == 2) {                                   // SPS closure in FP64 (per particle)
+  if (ts == 2 && P.sps) {                          // SPS closure in FP64 (per particle)
     const double sxx = Sxx.s, sxy = Sxy.s, syy = Syy.s;
     const double rhos = (double)drs + P.rho0;
     const double master = sxx * sxx + syy * syy + 0.0;

@@ -99,7 +99,7 @@ __device__ __forceinline__ void k_eval_a_body(const DevParams& P, Buffers& B, in
     B.mg[3 * s + 2] = make_double2(gvx, gvy);
   }
   double txx = 0.0, txy = 0.0, tyy = 0.0;
-  if (ts == 2) {                                   // Solver...SPS.f90:256-296
+  if (ts == 2 && P.sps) {                          // Solver...SPS.f90:256-296
     const double master = Sxx * Sxx + Syy * Syy + 0.0;
     const double last = Sxy * Sxy + 0.0 + 0.0;
     const double abs2 = 2 * master + last;
@@ -341,7 +341,7 @@ __device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int
       ay = ay + ((((tiy + tvy) + tay) + tsy) + tly);
     }
     if (SOLVER == SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && P.epsilon_xsph != 0.0) {   // :495-500
-      const double f = P.epsilon_xsph * 2 * mo / (rhos + rhoo) * K.w;
+      const double f = P.epsilon_xsph * P.xsph_f * mo / (rhos + rhoo) * K.w;
       agx = agx - f * vijx;
       agy = agy - f * vijy;
     }
@@ -401,6 +401,36 @@ static void launch_c64(hipStream_t s, const DevParams& P, Buffers& B, int g) {
 static void launch_a64(hipStream_t s, const DevParams& P, Buffers& B, int g) {
   if (P.solver == SPH2D_SOLVER_MUSCL) hipLaunchKernelGGL(k_eval_a<true>, dim3(g), dim3(kBlock), 0, s, P, B);
   else hipLaunchKernelGGL(k_eval_a<false>, dim3(g), dim3(kBlock), 0, s, P, B);
+}
+
+void launch_eval_part(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int part) {
+  const int g = grid_for(cap_local);
+  if (part == 0) {
+    launch_fill_i32(s, B.counters + C_NSURF, 1, 0);
+    if (P.mixed) launch_eval32_a(s, P, B, cap_local);
+    else launch_a64(s, P, B, g);
+  } else if (part == 1) {
+    hipLaunchKernelGGL(k_eval_b1, dim3(g), dim3(kBlock), 0, s, P, B);
+    hipLaunchKernelGGL(k_eval_b2, dim3(g), dim3(kBlock), 0, s, P, B);
+  } else {
+    if (P.mixed) launch_eval32_c(s, P, B, cap_local);
+    else if (P.write_debug) launch_c64<true>(s, P, B, g);
+    else launch_c64<false>(s, P, B, g);
+  }
+}
+
+__global__ void k_clear_tau(Buffers B, int mixed) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.counters[C_NLOCAL]) return;
+  B.tau[3 * s] = 0.0; B.tau[3 * s + 1] = 0.0; B.tau[3 * s + 2] = 0.0;
+  if (mixed) {
+    const float4 r = B.m_r1[s];
+    B.m_r1[s] = make_float4(0.f, 0.f, 0.f, r.w);   // .w: particle_mass at the local index
+  }
+}
+
+void launch_clear_tau(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
+  hipLaunchKernelGGL(k_clear_tau, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, B, P.mixed);
 }
 
 void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, hipEvent_t* ev) {

@@ -1588,6 +1588,53 @@ __global__ void k_pair_write(DevParams P, Buffers B, int64_t capacity) {
       }
 }
 
+// find_particle_pair.f90:64-123 in the reference's own order: particle i's
+// candidates are the 3x3 stencil of Distribute_Backgroud_Grid.f90:102-112 (self,
+// left, right, upper left, up, upper right, lower left, down, lower right), each
+// cell's particles in local order, at most 300 per cell and 300 in all
+// (Assign_boundary_particles.f90:124-152); the pairs are the candidates j > i that
+// pass the distance test. WRITE = false counts, true writes 1-based (i, j).
+__constant__ int kStencilDx[9] = {0, -1, 1, -1, 0, 1, -1, 0, 1};
+__constant__ int kStencilDy[9] = {0, 0, 0, 1, 1, 1, -1, -1, -1};
+
+template <bool WRITE>
+__global__ void k_ref_pairs(DevParams P, Buffers B, int64_t capacity) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B.counters[C_NLOCAL];
+  if (s >= n) return;
+  const double2 ps = B.l_pos[s];
+  const int ts = B.l_type[s];
+  const int cell = B.l_cell[s];
+  const int cx = cell / P.chain_ny, cy = cell - cx * P.chain_ny;
+  int cnt = 0, cand = 0;
+  int64_t k = WRITE ? (int64_t)B.pair_off[s] : 0;
+  for (int d = 0; d < 9 && cand < 300; ++d) {
+    const int x = cx + kStencilDx[d], y = cy + kStencilDy[d];
+    if (x < 0 || x >= P.chain_nx || y < 0 || y >= P.chain_ny) continue;
+    const int key = x * P.chain_ny + y;
+    const int lo = B.cell_start[key];
+    const int hi = min(B.cell_start[key + 1], lo + 300);
+    for (int o = lo; o < hi && cand < 300; ++o, ++cand) {
+      if (o > s && is_pair(P, B, s, o, ps, ts)) {
+        if (WRITE && k < capacity) { B.pair_lo[k] = s + 1; B.pair_hi[k] = o + 1; }
+        ++k;
+        ++cnt;
+      }
+    }
+  }
+  if (!WRITE) B.pair_count[s] = cnt;
+}
+
+void launch_count_pair_list(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
+  hipLaunchKernelGGL(k_ref_pairs<false>, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B, (int64_t)0);
+  launch_fill_i32(s, B.counters + C_NPAIR_LO, 1, 0);
+  exclusive_scan(s, B.pair_count, cap_local, B.pair_off, B.scan_blocks, B.counters + C_NPAIR_LO);
+}
+
+void launch_pair_list(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int64_t capacity) {
+  hipLaunchKernelGGL(k_ref_pairs<true>, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B, capacity);
+}
+
 void launch_count_pairs(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
   hipLaunchKernelGGL(k_pair_count, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B);
   launch_fill_i32(s, B.counters + C_NPAIR_LO, 1, 0);

@@ -49,6 +49,8 @@ struct DevParams {
   int solver, write_debug;
   int mls_order;
   int mixed;                             // 1: FP32-mixed evaluation (mirrors are written)
+  int sps;                               // 1: SPS stress (the _SPS solvers); 0: tau = 0
+  double xsph_f;                         // XSPH factor: 2 in the _SPS solvers, 1 without SPS
 };
 
 // Device counters (one int array, indices below).
@@ -128,6 +130,16 @@ void launch_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, h
 // FP32-mixed passes A and C (sph2d_eval32.hip)
 void launch_eval32_a(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
 void launch_eval32_c(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+// One part of launch_eval: 0 = pass A (pair masks and neighbour sums), 1 = the free
+// surface (flags, normals, near-surface flags), 2 = the force pass.
+void launch_eval_part(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int part);
+// The pair list in the reference's order (1-based local indices): per-particle counts
+// into pair_count, scan into pair_off and the total into counters[C_NPAIR_LO], then
+// the list (at most capacity entries) into pair_lo (i) and pair_hi (j).
+void launch_count_pair_list(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+void launch_pair_list(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int64_t capacity);
+// tau = 0 for every local particle (the Delta solvers without SPS)
+void launch_clear_tau(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
 // pass 0: force pass only, 1: pass A only, 2: the whole evaluation
 void launch_eval_pass(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int pass);
 void launch_stage(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int kind);

@@ -34,6 +34,7 @@ module sph2d_interface
     integer(c_int32_t) :: walls
     real(c_double) :: gravity
     integer(c_int32_t) :: wavemaker_layers
+    integer(c_int32_t) :: sps_on
   end type sph2d_params
 
   interface
@@ -121,6 +122,80 @@ module sph2d_interface
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: i_time_step
+      integer(c_int) :: rc
+    end function
+
+    ! single_step_compute routine by routine (single_step_compute.f90:62-171): the
+    ! same call sequence as the reference's find_particle_pair (:93),
+    ! Define_freesurface_and_Normailized_derivates (:108) and Solver_* (:137)
+    function sph2d_find_particle_pair(ctx, i_time_step) bind(C, name="sph2d_find_particle_pair") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: i_time_step
+      integer(c_int) :: rc
+    end function
+
+    ! pair_i/pair_j (1-based local indices) in find_particle_pair.f90's order
+    function sph2d_download_pair_list(ctx, capacity, pair_i, pair_j, pair_number) &
+        bind(C, name="sph2d_download_pair_list") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: capacity
+      integer(c_int32_t), intent(out) :: pair_i(*), pair_j(*)
+      integer(c_int64_t), intent(out) :: pair_number
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_define_freesurface(ctx) bind(C, name="sph2d_define_freesurface") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_solver(ctx, variant, sps_on) bind(C, name="sph2d_solver") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: variant, sps_on
+      integer(c_int) :: rc
+    end function
+
+    ! Reinitialize_particle_density, Shift_particle_position, Refresh_domain (time_iteration.f90:1064-1078)
+    function sph2d_reinitialize_particle_density(ctx, i_time_step) &
+        bind(C, name="sph2d_reinitialize_particle_density") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: i_time_step
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_shift_particle_position(ctx, i_time_step) bind(C, name="sph2d_shift_particle_position") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: i_time_step
+      integer(c_int) :: rc
+    end function
+
+    function sph2d_refresh_domain(ctx, i_time_step) bind(C, name="sph2d_refresh_domain") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: i_time_step
+      integer(c_int) :: rc
+    end function
+
+    ! time_iteration.f90's stage updates: scheme 2 (predictor 1, corrector 2) or 3 (K_1..K_3, final 4)
+    function sph2d_stage_update(ctx, scheme, stage) bind(C, name="sph2d_stage_update") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: scheme, stage
+      integer(c_int) :: rc
+    end function
+
+    ! drhodt(ld) and particle_acceleration(ld,2) in the reference's global order
+    function sph2d_download_rates(ctx, n, drhodt, acc, ld) bind(C, name="sph2d_download_rates") result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: n, ld
+      real(c_double), intent(out) :: drhodt(ld), acc(ld, 2)
       integer(c_int) :: rc
     end function
 

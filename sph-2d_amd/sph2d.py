@@ -66,6 +66,7 @@ class Params(ctypes.Structure):
         ("damping_amplitude", ctypes.c_double), ("water_depth", ctypes.c_double),
         ("walls", ctypes.c_int32), ("gravity", ctypes.c_double),
         ("wavemaker_layers", ctypes.c_int32),
+        ("sps_on", ctypes.c_int32),
     ]
 
 
@@ -111,7 +112,8 @@ EXPORTS = (
     "sph2d_init_tgv", "sph2d_upload_state", "sph2d_download_state", "sph2d_compute_kernel",
     "sph2d_compute_kernel_n", "sph2d_status_equation", "sph2d_status_equation_n",
     "sph2d_distribute_period_boundary", "sph2d_devide_block", "sph2d_single_step_compute",
-    "sph2d_single_step_compute_vjp",
+    "sph2d_single_step_compute_vjp", "sph2d_find_particle_pair", "sph2d_download_pair_list",
+    "sph2d_define_freesurface", "sph2d_solver", "sph2d_stage_update", "sph2d_download_rates",
     "sph2d_reinitialize_particle_density", "sph2d_shift_particle_position",
     "sph2d_refresh_domain", "sph2d_time_step", "sph2d_run", "sph2d_sample",
     "sph2d_download_local", "sph2d_download_pairs", "sph2d_set_debug",
@@ -215,6 +217,12 @@ def lib():
             "sph2d_batch_info": [vp, ip, ip],
             "sph2d_batch_destroy": [vp],
             "sph2d_batch_autotune": [vp, i32, i32, dp, ip],
+            "sph2d_find_particle_pair": [vp, i32],
+            "sph2d_download_pair_list": [vp, i64, ip, ip, P(i64)],
+            "sph2d_define_freesurface": [vp],
+            "sph2d_solver": [vp, i32, i32],
+            "sph2d_stage_update": [vp, i32, i32],
+            "sph2d_download_rates": [vp, i32, dp, dp, i32],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -355,6 +363,36 @@ class Simulation:
         _check(lib().sph2d_single_step_compute_vjp(self._c, n, _dp(ld), _dp(la), _dp(lg) if lg is not None else None,
                                                    n, _dp(gp), _dp(gv), _dp(gr)))
         return {"x": gp[:n], "y": gp[n:], "u": gv[:n], "v": gv[n:], "rho": gr}
+
+    # --- single_step_compute routine by routine ---------------------------------------
+    def find_particle_pair(self, i_time_step: int = 1):
+        _check(lib().sph2d_find_particle_pair(self._c, i_time_step))
+
+    def pair_list(self):
+        """(pair_i, pair_j): 1-based local indices in the reference's pair order."""
+        n = ctypes.c_int64()
+        _check(lib().sph2d_download_pair_list(self._c, 0, None, None, ctypes.byref(n)))
+        pi = np.zeros(max(n.value, 1), dtype=np.int32)
+        pj = np.zeros(max(n.value, 1), dtype=np.int32)
+        _check(lib().sph2d_download_pair_list(self._c, n.value, _ip(pi), _ip(pj), ctypes.byref(n)))
+        return pi[:n.value], pj[:n.value]
+
+    def define_freesurface(self):
+        _check(lib().sph2d_define_freesurface(self._c))
+
+    def solver(self, variant: int, sps_on: int = 1):
+        _check(lib().sph2d_solver(self._c, variant, sps_on))
+
+    def stage_update(self, scheme: int, stage: int):
+        _check(lib().sph2d_stage_update(self._c, scheme, stage))
+
+    def rates(self) -> Dict[str, np.ndarray]:
+        """drhodt and particle_acceleration of the fluid particles in gid order."""
+        n = self.n_fluid
+        d = np.zeros(n)
+        a = np.zeros(2 * n)
+        _check(lib().sph2d_download_rates(self._c, n, _dp(d), _dp(a), n))
+        return {"drhodt": d, "ax": a[:n], "ay": a[n:]}
 
     def reinitialize_particle_density(self, i_time_step: int):
         _check(lib().sph2d_reinitialize_particle_density(self._c, i_time_step))
