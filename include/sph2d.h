@@ -173,6 +173,20 @@ int sph2d_reorder_particle_index(int32_t n_total, int32_t n_fluid, double box_x,
                                  int32_t n_i32, int32_t* const* i32_cols, int32_t* perm, int32_t* counts,
                                  void* stream);
 
+/* No counterpart in the reference: sph2d_reorder_particle_index for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, results and errors, but every
+ * scratch buffer (category tables, scan tiles, counters, staging column, and perm
+ * when NULL) lives in a file-scope device array of the library, one copy per GPU,
+ * held until the process ends (~24.5 MiB): no hipMalloc, hipFree or other
+ * allocation. At most 2,097,152 particles (n_total larger: SPH2D_ERR_CAPACITY) and
+ * 64 columns of each kind. Calls that overlap on one GPU share that array and race;
+ * the caller serialises them. */
+int sph2d_reorder_particle_index_noalloc(int32_t n_total, int32_t n_fluid, double box_x, double box_y,
+                                         double* x, double* y, int32_t* type, int32_t n_f64,
+                                         double* const* f64_cols, int32_t n_i32, int32_t* const* i32_cols,
+                                         int32_t* perm, int32_t* counts, void* stream);
+
 /* --- one entry per replaced routine (context form) ------------------------- */
 /* Distribute_Period_Boundary_Particle.f90:56-206: wrap into the box and append ghosts. */
 int sph2d_distribute_period_boundary(sph2d_ctx* ctx);

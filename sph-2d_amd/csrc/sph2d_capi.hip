@@ -13,12 +13,16 @@ using namespace sph2d;
 
 namespace {
 
-thread_local std::string g_err;
+// The message of the calling thread's last failure, in a fixed buffer: recording
+// an error allocates nothing (the no-allocation entry point relies on that).
+thread_local char g_err[512];
 
-int fail(int code, const std::string& msg) {
-  g_err = msg;
+int fail(int code, const char* msg) {
+  std::snprintf(g_err, sizeof(g_err), "%s", msg);
   return code;
 }
+
+int fail(int code, const std::string& msg) { return fail(code, msg.c_str()); }
 
 }  // namespace
 
@@ -264,7 +268,7 @@ struct DeviceScratch {
 
 extern "C" {
 
-const char* sph2d_last_error(void) { return g_err.c_str(); }
+const char* sph2d_last_error(void) { return g_err; }
 
 int sph2d_default_params(sph2d_params* p) {
   if (!p) return fail(SPH2D_ERR_ARGUMENT, "params is NULL");
@@ -728,6 +732,87 @@ int sph2d_reorder_particle_index(int32_t n_total, int32_t n_fluid, double box_x,
   for (int k = 0; k < n_i32 && !rc; ++k) rc = permute_i32(i32_cols[k]);
   if (rc) return rc;
   HIP_OR_FAIL(hipGetLastError());
+  HIP_OR_FAIL(hipStreamSynchronize(s));
+  for (int k = 0; k < 6; ++k) counts[k] = hc[k];
+  return SPH2D_OK;
+}
+
+// The same routine with its scratch in the library's per-device static array: no
+// hipMalloc, hipFree or other allocation, at most 2^21 particles; calls that overlap
+// on one GPU share that array and race.
+int sph2d_reorder_particle_index_noalloc(int32_t n_total, int32_t n_fluid, double box_x, double box_y, double* x,
+                                         double* y, int32_t* type, int32_t n_f64, double* const* f64_cols,
+                                         int32_t n_i32, int32_t* const* i32_cols, int32_t* perm, int32_t* counts,
+                                         void* stream) {
+  if (n_total <= 0 || n_fluid < 0 || n_fluid > n_total || n_f64 < 0 || n_i32 < 0 || !counts)
+    return fail(SPH2D_ERR_ARGUMENT, "bad sizes or NULL counts");
+  if ((n_f64 > 0 && !f64_cols) || (n_i32 > 0 && !i32_cols)) return fail(SPH2D_ERR_ARGUMENT, "NULL column list");
+  constexpr int kMaxCols = 64;
+  if (n_f64 > kMaxCols || n_i32 > kMaxCols) return fail(SPH2D_ERR_ARGUMENT, "at most 64 columns of each kind");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(SPH2D_ERR_HIP, "no HIP device: the sph2d library runs on the GPU only");
+  void* base = nullptr;
+  size_t bytes = 0;
+  int max_n = 0;
+  int rc = reorder_scratch(&base, &bytes, &max_n);
+  if (rc) return rc;
+  if (n_total > max_n)
+    return fail(SPH2D_ERR_CAPACITY, "n_total exceeds the 2097152 particles the no-allocation scratch holds");
+  const size_t nd = (size_t)n_total * sizeof(double), ni = (size_t)n_total * sizeof(int32_t);
+  if (!device_range(x, nd) || !device_range(y, nd) || !device_range(type, ni))
+    return fail(SPH2D_ERR_ARGUMENT, "x, y and type must be device arrays of n_total entries");
+  for (int k = 0; k < n_f64; ++k)
+    if (!device_range(f64_cols[k], nd)) return fail(SPH2D_ERR_ARGUMENT, "an f64 column is not a device array of n_total entries");
+  for (int k = 0; k < n_i32; ++k)
+    if (!device_range(i32_cols[k], ni)) return fail(SPH2D_ERR_ARGUMENT, "an i32 column is not a device array of n_total entries");
+  if (perm && !device_range(perm, ni)) return fail(SPH2D_ERR_ARGUMENT, "perm must be a device array of n_total entries");
+  {
+    const void* all[3 + 2 * kMaxCols + 1];
+    int na = 0;
+    all[na++] = x; all[na++] = y; all[na++] = type;
+    for (int k = 0; k < n_f64; ++k) all[na++] = f64_cols[k];
+    for (int k = 0; k < n_i32; ++k) all[na++] = i32_cols[k];
+    if (perm) all[na++] = perm;
+    for (int a = 0; a < na; ++a)
+      for (int b = a + 1; b < na; ++b)
+        if (all[a] == all[b]) return fail(SPH2D_ERR_ARGUMENT, "x, y, type, perm and the columns must be distinct arrays");
+  }
+  // carve the scratch as sph2d_reorder.hip sized it (each piece 256-byte aligned)
+  const size_t Gmax = ((size_t)max_n + kBlock - 1) / kBlock;
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  char* p = static_cast<char*>(base);
+  int* blk = reinterpret_cast<int*>(p); p += al(sizeof(int) * 5 * Gmax);
+  int* off = reinterpret_cast<int*>(p); p += al(sizeof(int) * 5 * Gmax);
+  int* sblk = reinterpret_cast<int*>(p); p += al(sizeof(int) * 1024);
+  int* misc = reinterpret_cast<int*>(p); p += al(sizeof(int) * 8);
+  double* tmp = reinterpret_cast<double*>(p); p += al(sizeof(double) * (size_t)max_n);
+  int* sperm = reinterpret_cast<int*>(p); p += al(sizeof(int) * (size_t)max_n);
+  const int G = (n_total + kBlock - 1) / kBlock;
+  if ((size_t)(p - static_cast<char*>(base)) > bytes || scan_block_count(5 * G) + 1 > 1024)
+    return fail(SPH2D_ERR_CAPACITY, "no-allocation scratch layout exceeds its array");
+  int* dperm = perm ? perm : sperm;
+  int* total = misc;
+  int* dcounts = misc + 1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  launch_fill_i32(s, misc, 8, 0);                    // the previous call's counters are stale
+  launch_reorder_partition(s, n_total, n_fluid, type, x, y, box_x, box_y, blk, off, sblk, total, dcounts, dperm);
+  HIP_OR_FAIL(hipGetLastError());
+  for (int k = -3; k < n_f64 + n_i32; ++k) {
+    if (k == -1 || k >= n_f64) {
+      int32_t* col = k == -1 ? type : i32_cols[k - n_f64];
+      int* t = reinterpret_cast<int*>(tmp);
+      launch_reorder_gather_i32(s, n_total, dperm, total, col, t);
+      launch_reorder_copy_i32(s, n_total, total, t, col);
+    } else {
+      double* col = k == -3 ? x : (k == -2 ? y : f64_cols[k]);
+      launch_reorder_gather_f64(s, n_total, dperm, total, col, tmp);
+      launch_reorder_copy_f64(s, n_total, total, tmp, col);
+    }
+  }
+  HIP_OR_FAIL(hipGetLastError());
+  int hc[6];
+  HIP_OR_FAIL(hipMemcpyAsync(hc, dcounts, sizeof(hc), hipMemcpyDeviceToHost, s));
   HIP_OR_FAIL(hipStreamSynchronize(s));
   for (int k = 0; k < 6; ++k) counts[k] = hc[k];
   return SPH2D_OK;

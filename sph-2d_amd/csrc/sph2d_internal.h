@@ -244,6 +244,13 @@ void launch_max_cell(hipStream_t s, Buffers& B, int n_cells);
 void launch_reorder_partition(hipStream_t s, int n_total, int n_fluid, int* type, const double* x,
                               const double* y, double box_x, double box_y, int* blk, int* off,
                               int* scan_blocks, int* total, int* counts, int* perm);
+// Scratch of the no-allocation Reorder_Particle_Index (sph2d_reorder.hip): the
+// device address of this device's copy of the file-scope array, looked up once per
+// device; bytes = its size, max_n = the largest n_total it serves.
+int reorder_scratch(void** ptr, size_t* bytes, int* max_n);
+// Copy of the first *total entries of src to dst (device counts, no host size).
+void launch_reorder_copy_f64(hipStream_t s, int n_total, const int* total, const double* src, double* dst);
+void launch_reorder_copy_i32(hipStream_t s, int n_total, const int* total, const int* src, int* dst);
 void launch_reorder_gather_f64(hipStream_t s, int n_total, const int* perm, const int* total, const double* src,
                                double* dst);
 void launch_reorder_gather_i32(hipStream_t s, int n_total, const int* perm, const int* total, const int* src,

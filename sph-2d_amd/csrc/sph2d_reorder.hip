@@ -11,6 +11,8 @@
 // into `perm`, then one gather per column. Integer and copy work, HBM-bound.
 #include "sph2d_internal.h"
 
+#include <mutex>
+
 namespace sph2d {
 
 constexpr int kRoCats = 5;
@@ -111,6 +113,59 @@ void launch_reorder_partition(hipStream_t s, int n_total, int n_fluid, int* type
   launch_exclusive_scan(s, blk, kRoCats * G, off, scan_blocks, total);
   hipLaunchKernelGGL(k_ro_scatter, dim3(G), dim3(kBlock), 0, s, n_total, type, off, G, perm);
   hipLaunchKernelGGL(k_ro_counts, dim3(1), dim3(64), 0, s, off, G, total, counts);
+}
+
+// ---- scratch of sph2d_reorder_particle_index_noalloc ----------------------------------
+// One file-scope array per device (the HIP loader places a copy on every GPU the
+// library runs on; it is held until the process ends) for inputs of at most
+// kNoAllocMax particles: the category tables of k_ro_count/k_ro_scatter (5 per block),
+// the scan tiles, the counters, the gather staging column and the permutation.
+constexpr int kNoAllocMax = 1 << 21;
+constexpr size_t kAlign = 256;
+constexpr size_t align_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+constexpr int kNoAllocBlocks = (kNoAllocMax + kBlock - 1) / kBlock;
+constexpr size_t kNoAllocBytes = 2 * align_up(sizeof(int) * kRoCats * kNoAllocBlocks) +
+                                 align_up(sizeof(int) * 1024) + align_up(sizeof(int) * 8) +
+                                 align_up(sizeof(double) * kNoAllocMax) + align_up(sizeof(int) * kNoAllocMax);
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+int reorder_scratch(void** ptr, size_t* bytes, int* max_n) {
+  constexpr int kMaxDevices = 64;
+  static void* table[kMaxDevices] = {};
+  static std::mutex lock;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return set_error(SPH2D_ERR_HIP, "hipGetDevice failed");
+  {
+    std::lock_guard<std::mutex> g(lock);
+    if (!table[dev]) {
+      void* p = nullptr;
+      // host code holds only a stand-in for the array: its device address comes from
+      // the runtime, per device, with that device current
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) return set_error(SPH2D_ERR_HIP, "hipGetSymbolAddress(scratch) failed");
+      table[dev] = p;
+    }
+    *ptr = table[dev];
+  }
+  *bytes = kNoAllocBytes;
+  *max_n = kNoAllocMax;
+  return SPH2D_OK;
+}
+
+template <typename T>
+__global__ void k_ro_copy(const int* total, const T* src, T* dst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < *total) dst[k] = src[k];
+}
+
+void launch_reorder_copy_f64(hipStream_t s, int n_total, const int* total, const double* src, double* dst) {
+  hipLaunchKernelGGL(k_ro_copy<double>, dim3((n_total + kBlock - 1) / kBlock), dim3(kBlock), 0, s, total, src, dst);
+}
+
+void launch_reorder_copy_i32(hipStream_t s, int n_total, const int* total, const int* src, int* dst) {
+  hipLaunchKernelGGL(k_ro_copy<int>, dim3((n_total + kBlock - 1) / kBlock), dim3(kBlock), 0, s, total, src, dst);
 }
 
 void launch_reorder_gather_f64(hipStream_t s, int n_total, const int* perm, const int* total, const double* src,

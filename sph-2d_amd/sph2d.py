@@ -118,7 +118,8 @@ EXPORTS = (
     "sph2d_refresh_domain", "sph2d_time_step", "sph2d_run", "sph2d_sample",
     "sph2d_download_local", "sph2d_download_pairs", "sph2d_set_debug",
     "sph2d_get_diagnostics", "sph2d_get_stream", "sph2d_time_pass",
-    "sph2d_reorder_particle_index", "sph2d_set_pass_timing", "sph2d_get_pass_timing",
+    "sph2d_reorder_particle_index", "sph2d_reorder_particle_index_noalloc", "sph2d_set_pass_timing",
+    "sph2d_get_pass_timing",
     "sph2d_get_scales", "sph2d_download_output_fields", "sph2d_output_for_tecplot",
     "sph2d_output_for_paraview", "sph2d_format_fortran", "sph2d_write_final_results",
     "sph2d_write_tecplot_zone", "sph2d_write_surface_zone", "sph2d_write_tecplot_binary",
@@ -217,6 +218,8 @@ def lib():
             "sph2d_batch_info": [vp, ip, ip],
             "sph2d_batch_destroy": [vp],
             "sph2d_batch_autotune": [vp, i32, i32, dp, ip],
+            "sph2d_reorder_particle_index_noalloc": [i32, i32, ctypes.c_double, ctypes.c_double, vp, vp, vp, i32,
+                                                     P(vp), i32, P(vp), vp, ip, vp],
             "sph2d_find_particle_pair": [vp, i32],
             "sph2d_download_pair_list": [vp, i64, ip, ip, P(i64)],
             "sph2d_define_freesurface": [vp],
@@ -628,27 +631,31 @@ class DeviceArray:
             pass
 
 
-def reorder_particle_index(n_fluid, box_x, box_y, x, y, type_, f64_cols=(), i32_cols=()):
+def reorder_particle_index(n_fluid, box_x, box_y, x, y, type_, f64_cols=(), i32_cols=(), no_alloc=False,
+                           with_perm=True):
     """Reorder_Particle_Index (MS/Reorder_Particle_Index.f90:54-287) on the GPU for host
-    arrays: uploads, calls sph2d_reorder_particle_index, downloads. Returns
-    (x, y, type, f64_cols, i32_cols, perm, counts) with counts = (fluid, wavemaker,
-    fixed ghost, inlet, outlet, total kept); arrays hold the kept particles only."""
+    arrays: uploads, calls sph2d_reorder_particle_index (no_alloc: its variant with
+    the library's static scratch, sph2d_reorder_particle_index_noalloc), downloads.
+    Returns (x, y, type, f64_cols, i32_cols, perm, counts) with counts = (fluid,
+    wavemaker, fixed ghost, inlet, outlet, total kept); arrays hold the kept
+    particles only (perm is None when with_perm is False: the call gets NULL)."""
     n = len(x)
     dx = DeviceArray(np.asarray(x, dtype=np.float64))
     dy = DeviceArray(np.asarray(y, dtype=np.float64))
     dt = DeviceArray(np.asarray(type_, dtype=np.int32))
     df = [DeviceArray(np.asarray(c, dtype=np.float64)) for c in f64_cols]
     di = [DeviceArray(np.asarray(c, dtype=np.int32)) for c in i32_cols]
-    dp = DeviceArray(np.full(n, -1, dtype=np.int32))
+    dp = DeviceArray(np.full(n, -1, dtype=np.int32)) if with_perm else None
     fptrs = (ctypes.c_void_p * max(len(df), 1))(*[d.ptr.value for d in df])
     iptrs = (ctypes.c_void_p * max(len(di), 1))(*[d.ptr.value for d in di])
     counts = np.zeros(6, dtype=np.int32)
-    _check(lib().sph2d_reorder_particle_index(n, n_fluid, box_x, box_y, dx.ptr, dy.ptr, dt.ptr, len(df), fptrs,
-                                              len(di), iptrs, dp.ptr, _ip(counts), None))
+    f = lib().sph2d_reorder_particle_index_noalloc if no_alloc else lib().sph2d_reorder_particle_index
+    _check(f(n, n_fluid, box_x, box_y, dx.ptr, dy.ptr, dt.ptr, len(df), fptrs, len(di), iptrs,
+             dp.ptr if dp is not None else None, _ip(counts), None))
     k = int(counts[5])
     out = (dx.get()[:k], dy.get()[:k], dt.get()[:k], [d.get()[:k] for d in df], [d.get()[:k] for d in di],
-           dp.get()[:k], counts)
-    for d in [dx, dy, dt, dp] + df + di:
+           dp.get()[:k] if dp is not None else None, counts)
+    for d in [dx, dy, dt] + ([dp] if dp is not None else []) + df + di:
         d.free()
     return out
 
