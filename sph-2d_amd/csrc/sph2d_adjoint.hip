@@ -318,7 +318,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_force(DevParams P, Buffers B, Ad
       acc[k] = acc[k] + (J1.d[k] + J2.d[k]);
       acc[5 + k] = acc[5 + k] + (a1[k] + a2[k]);
     }
-  }, B.nb_mask, B.kstride);
+  }, B.nb_list, B.nb_count, B.kstride);
   double* gs = A.gs + 5 * (size_t)s;
   double* ga = A.ga + 5 * (size_t)s;
   for (int k = 0; k < 5; ++k) { gs[k] = acc[k]; ga[k] = acc[5 + k]; }
@@ -350,7 +350,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_tau(DevParams P, Buffers B, AdjB
       Sxx = Sxx + vdx * dwx * Vo;
       Sxy = Sxy + (vdx * dwy + vdy * dwx) * Vo;
       Syy = Syy + vdy * dwy * Vo;
-    }, B.nb_mask, B.kstride);
+    }, B.nb_list, B.nb_count, B.kstride);
     Dual<4> xx(Sxx), xy(Sxy), yy(Syy), rho(B.l_rv[s].x);
     xx.d[0] = 1.0; xy.d[1] = 1.0; yy.d[2] = 1.0; rho.d[3] = 1.0;
     const Dual<4> abs2 = 2.0 * (xx * xx + yy * yy) + xy * xy;
@@ -387,7 +387,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_sums(DevParams P, Buffers B, Adj
     const Dual<5> J2 = sums_pair<Dual<5>>(P, O, S, ms, A.ga + 5 * (size_t)o);
 #pragma unroll
     for (int k = 0; k < 5; ++k) acc[k] = acc[k] + (J1.d[k] + J2.d[k]);
-  }, B.nb_mask, B.kstride);
+  }, B.nb_list, B.nb_count, B.kstride);
   const int g = B.name[s];
   const double* gs = A.gs + 5 * (size_t)s;
   for (int k = 0; k < 5; ++k) A.G[5 * (size_t)g + k] = gs[k] + acc[k];

@@ -359,7 +359,8 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   rc |= c->alloc(&B.fra_v, L); rc |= c->alloc(&B.sdw, L); rc |= c->alloc(&B.grad, L);
   rc |= c->alloc(&B.nrm, L); rc |= c->alloc(&B.shift, L); rc |= c->alloc(&B.avg, L);
   rc |= c->alloc(&B.tau, 3 * L); rc |= c->alloc(&B.surf, L); rc |= c->alloc(&B.near_, L);
-  rc |= c->alloc(&B.nb_mask, 4 * L);
+  rc |= c->alloc(&B.nb_list, (size_t)kNbListWords * L);
+  rc |= c->alloc(&B.nb_count, L);
   rc |= c->alloc(&B.drhodt, L); rc |= c->alloc(&B.acc, L);
   rc |= c->alloc(&B.b_rho, L); rc |= c->alloc(&B.t_drho, L); rc |= c->alloc(&B.k_drho, 3 * L);
   rc |= c->alloc(&B.b_pos, L); rc |= c->alloc(&B.b_vel, L);
@@ -372,6 +373,7 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   rc |= c->alloc(&c->d_out, 16);
   if (c->P.mixed) {
     rc |= c->alloc(&B.m_r0, L); rc |= c->alloc(&B.m_r1, L); rc |= c->alloc(&B.m_r2, L);
+    rc |= c->alloc(&B.m_pc, L);
   }
   B.mg = nullptr; B.m_mg = nullptr;
   if (p->solver == SPH2D_SOLVER_MUSCL) {
@@ -1641,18 +1643,12 @@ int sph2d_download_local(sph2d_ctx* c, const char* name, void* out, int32_t n) {
   if (s == "name") return i1(B.name);
   if (s == "surface") return i1(B.surf);
   if (s == "near_surface") return i1(B.near_);
-  if (s == "neighbour_count") {                 // set bits of the last pass A's recorded masks
-    std::vector<uint64_t> t((size_t)nl * 4);
-    for (int w = 0; w < 4; ++w)
-      HIP_OR_FAIL(hipMemcpyAsync(t.data() + (size_t)w * nl, B.nb_mask + (size_t)w * B.kstride, nl * sizeof(uint64_t),
-                                 hipMemcpyDeviceToHost, c->stream));
+  if (s == "neighbour_count") {                 // the last pass A's recorded neighbours
+    std::vector<int> t((size_t)nl);
+    HIP_OR_FAIL(hipMemcpyAsync(t.data(), B.nb_count, nl * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_OR_FAIL(hipStreamSynchronize(c->stream));
     int* o = static_cast<int*>(out);
-    for (int i = 0; i < nl; ++i) {
-      int k = 0;
-      for (int w = 0; w < 4; ++w) k += __builtin_popcountll(t[(size_t)w * nl + i]);
-      o[i] = k;
-    }
+    for (int i = 0; i < nl; ++i) o[i] = t[i] & (kNbOverflow - 1);
     return SPH2D_OK;
   }
   static const char* dbg_names[10] = {"internal_x", "internal_y", "visc_x", "visc_y", "art_internal_x",

@@ -28,11 +28,16 @@ __device__ __forceinline__ void tait(const DevParams& P, double rho, double& p, 
   c = P.sqrt_c0sq * (x * x * x);
 }
 
-// FP32 record r0 of the mixed form: (u, v, rho - rho0, m), m negated if not fluid.
+// FP32 record r0 of the mixed form: (u, v, rho - rho0, m), m negated if not fluid;
+// and (p, c) of the FP64 Tait law (status_equation.f90:61-63) rounded to FP32.
 __device__ __forceinline__ void write_r0(const DevParams& P, const Buffers& B, int s, double2 vel, double rho,
                                          double m, int type) {
   const float mf = (float)m;
   B.m_r0[s] = make_float4((float)vel.x, (float)vel.y, (float)(rho - P.rho0), type == 2 ? mf : -mf);
+  const double x = rho / P.rho0;
+  const double p = P.eos_b * (x * x * x * x * x * x * x - 1.0);
+  const double c = P.sqrt_c0sq * (x * x * x);
+  B.m_pc[s] = make_float2((float)p, (float)c);
 }
 
 // Offset of a position from the lower-left corner of its binned cell (the cell of
@@ -57,6 +62,12 @@ __device__ __forceinline__ float4 buf_f4(__amdgpu_buffer_rsrc_t r, int idx) {
   typedef unsigned int u4 __attribute__((ext_vector_type(4)));
   const u4 w = __builtin_amdgcn_raw_buffer_load_b128(r, idx * 16, 0, 0);
   return make_float4(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z), __uint_as_float(w.w));
+}
+
+__device__ __forceinline__ float2 buf_f2(__amdgpu_buffer_rsrc_t r, int idx) {
+  typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+  const u2 w = __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8, 0, 0);
+  return make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
 }
 
 __device__ __forceinline__ double2 buf_d2(__amdgpu_buffer_rsrc_t r, int idx) {
@@ -176,13 +187,14 @@ namespace sph2d {
 // accepted ones, in ascending candidate order, so the lanes of a wave stay
 // converged on the pair body. Lists longer than 256 run in chunks.
 //
-// Modes: kNbScan screens and tests; kNbRecord also stores the exact acceptance
-// mask of a list of at most 256 candidates (4 words per particle, word w of
-// particle s at masks[w * mstride + s]); kNbReplay loads that mask instead of
-// screening and testing, so later passes of the same evaluation (same positions,
-// same cell table) visit the same neighbours in the same order without reading
-// positions for the test. Bits at or beyond the list length are ignored, and a
-// list longer than 256 is screened and tested again.
+// Modes: kNbScan screens and tests; kNbRecord also records the exactly accepted
+// candidates of a list of at most 256 (their slots, one byte each, 4 per word,
+// word w of particle s at list[w * lstride + s], and their number in counts[s]);
+// kNbReplay reads that record instead of screening and testing, so later passes of
+// the same evaluation (same positions, same cell table) visit the same neighbours
+// in the same order without reading positions for the test. A list longer than
+// 256 candidates, or one with more than kNbListCap neighbours, is screened and
+// tested again.
 // find_particle_pair.f90:87 with r2 = dx*dx+dy*dy rounded as the reference rounds it
 // (no contraction, whatever the translation unit's flags).
 __device__ __forceinline__ bool is_pair_r2(const DevParams& P, double dx, double dy) {
@@ -323,106 +335,122 @@ __device__ __forceinline__ void ghost_flags(const DevParams& P, double2 p, int& 
   yr = (p.y >= (P.box_y - r3h) && p.y <= P.box_y);
 }
 
+// The FP32 screen of candidates c0 .. c0+255 of a particle's list: bit k of the
+// result is set when candidate c0 + k may lie within 3h (no pair within 3h fails
+// it); the particle itself is never set.
+__device__ __forceinline__ Mask256 screen_chunk(const DevParams& P, const float2* __restrict__ pos32, int s,
+                                                float2 ps32, const CellRanges& R, int n0, int n1, int n2, int n01,
+                                                int total, int l0, int l1, int l2, bool staged, int s0, int s1,
+                                                int s2, const float2* lds32, int c0) {
+uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+  if (staged && total <= 256 && n0 <= 128 && n1 <= 128 && n2 <= 128) {
+    // Column by column from LDS: a column's slots are contiguous and the loop
+    // index is the same in every lane, so a candidate costs one LDS read, the
+    // distance test and one bit insert; the self pair is cleared afterwards.
+#pragma unroll 1
+    for (int d = 0; d < 3; ++d) {
+      const int nd = d == 0 ? n0 : (d == 1 ? n1 : n2);
+      const float2* q = lds32 + (d == 0 ? s0 : (d == 1 ? s1 + n0 : s2 + n01));
+      uint64_t lo = 0, hi = 0;
+#pragma unroll 1
+      for (int j = 0; j < 4; ++j) {
+        uint32_t acc = 0;
+        const int kend = min(nd - 32 * j, 32);
+#pragma unroll 4
+        for (int k = 0; k < kend; ++k) {
+          const float2 v = q[32 * j + k];
+          const float fx = ps32.x - v.x, fy = ps32.y - v.y;
+          acc |= (uint32_t)(fx * fx + fy * fy <= P.cut2_32) << k;
+        }
+        if (j == 0) lo |= acc;
+        else if (j == 1) lo |= (uint64_t)acc << 32;
+        else if (j == 2) hi |= acc;
+        else hi |= (uint64_t)acc << 32;
+      }
+      if (d == 1) {
+        const int ks = s - R.lo[1];                 // the particle itself
+        if (ks < 64) lo &= ~(1ull << ks); else hi &= ~(1ull << (ks - 64));
+      }
+      or_shl128(m0, m1, m2, m3, lo, hi, d == 0 ? 0 : (d == 1 ? n0 : n01));
+    }
+  } else {
+#pragma unroll 1
+    for (int w = 0; w < 4; ++w) {
+      const int base = c0 + 64 * w;
+      const int cn = min(64, total - base);
+      uint64_t mw = 0;
+      if (staged) {
+#pragma unroll 2
+        for (int k = 0; k < cn; ++k) {
+          const int kk = base + k;
+          const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
+          const int li = kk < n0 ? s0 + kk : (kk < n01 ? s1 + kk : s2 + kk);
+          const float2 q = lds32[li];
+          const float fx = ps32.x - q.x, fy = ps32.y - q.y;
+          const bool ok = (fx * fx + fy * fy <= P.cut2_32) && (o != s);
+          mw |= (uint64_t)ok << k;
+        }
+      } else {
+#pragma unroll 2
+        for (int k = 0; k < cn; ++k) {
+          const int kk = base + k;
+          const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
+          const float2 q = pos32[o];
+          const float fx = ps32.x - q.x, fy = ps32.y - q.y;
+          const bool ok = (fx * fx + fy * fy <= P.cut2_32) && (o != s);
+          mw |= (uint64_t)ok << k;
+        }
+      }
+      if (w == 0) m0 = mw; else if (w == 1) m1 = mw; else if (w == 2) m2 = mw; else m3 = mw;
+    }
+  }
+  return Mask256{m0, m1, m2, m3};
+}
+
 template <int MODE = kNbScan, class F>
 __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int* __restrict__ cell_start,
                                                    const double2* __restrict__ pos,
                                                    const float2* __restrict__ pos32, int s, double2 ps,
-                                                   int cell, F&& f, uint64_t* __restrict__ masks = nullptr,
-                                                   int mstride = 0, const NbStage* st = nullptr,
+                                                   int cell, F&& f, uint32_t* __restrict__ list = nullptr,
+                                                   int* __restrict__ counts = nullptr, int lstride = 0,
+                                                   const NbStage* st = nullptr,
                                                    const float2* lds32 = nullptr, const double2* lds64 = nullptr) {
   const CellRanges R = cell_ranges(P, cell_start, cell);
   const int n0 = R.hi[0] - R.lo[0], n1 = R.hi[1] - R.lo[1], n2 = R.hi[2] - R.lo[2];
   const int n01 = n0 + n1, total = n01 + n2;
   const int l0 = R.lo[0], l1 = R.lo[1] - n0, l2 = R.lo[2] - n01;
   const bool stored = total <= 256;
-  const bool replay = MODE == kNbReplay && stored;
   const bool staged = st != nullptr && st->use;     // uniform over the block
   // LDS slot of candidate kk: kk + s0/s1/s2 by stencil column
   const int s0 = staged ? l0 - st->lo[0] + st->base[0] : 0;
   const int s1 = staged ? l1 - st->lo[1] + st->base[1] : 0;
   const int s2 = staged ? l2 - st->lo[2] + st->base[2] : 0;
-  float2 ps32 = make_float2(0.f, 0.f);
-  if (!replay) ps32 = pos32[s];
-  const __amdgpu_buffer_rsrc_t rpos = rec_rsrc(reinterpret_cast<const float4*>(pos));
-  for (int c0 = 0; c0 < total; c0 += 256) {
-    uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-    if (replay) {
-      // bit k of word w is candidate 64 w + k; keep bits below total only
-      m0 = masks[s];
-      m1 = masks[mstride + s];
-      m2 = masks[2 * mstride + s];
-      m3 = masks[3 * mstride + s];
-      if (total < 256) {
-        const int t = total & 63;
-        const uint64_t part = t ? ((1ull << t) - 1ull) : 0ull;
-        const int wl = total >> 6;                 // the word holding the last partial bits
-        if (wl == 0) { m0 &= part; m1 = 0; m2 = 0; m3 = 0; }
-        else if (wl == 1) { m1 &= part; m2 = 0; m3 = 0; }
-        else if (wl == 2) { m2 &= part; m3 = 0; }
-        else { m3 &= part; }
-      }
-    } else if (staged && total <= 256 && n0 <= 128 && n1 <= 128 && n2 <= 128) {
-      // Column by column from LDS: a column's slots are contiguous and the loop
-      // index is the same in every lane, so a candidate costs one LDS read, the
-      // distance test and one bit insert; the self pair is cleared afterwards.
+  if (MODE == kNbReplay && stored) {
+    const int cnt = counts[s];
+    if (!(cnt & kNbOverflow)) {
+      // the recorded slots in ascending order: one byte per neighbour; the loop
+      // index advances in every active lane together, so the word loads are uniform
+      uint32_t word = 0;
 #pragma unroll 1
-      for (int d = 0; d < 3; ++d) {
-        const int nd = d == 0 ? n0 : (d == 1 ? n1 : n2);
-        const float2* q = lds32 + (d == 0 ? s0 : (d == 1 ? s1 + n0 : s2 + n01));
-        uint64_t lo = 0, hi = 0;
-#pragma unroll 1
-        for (int j = 0; j < 4; ++j) {
-          uint32_t acc = 0;
-          const int kend = min(nd - 32 * j, 32);
-#pragma unroll 4
-          for (int k = 0; k < kend; ++k) {
-            const float2 v = q[32 * j + k];
-            const float fx = ps32.x - v.x, fy = ps32.y - v.y;
-            acc |= (uint32_t)(fx * fx + fy * fy <= P.cut2_32) << k;
-          }
-          if (j == 0) lo |= acc;
-          else if (j == 1) lo |= (uint64_t)acc << 32;
-          else if (j == 2) hi |= acc;
-          else hi |= (uint64_t)acc << 32;
-        }
-        if (d == 1) {
-          const int ks = s - R.lo[1];                 // the particle itself
-          if (ks < 64) lo &= ~(1ull << ks); else hi &= ~(1ull << (ks - 64));
-        }
-        or_shl128(m0, m1, m2, m3, lo, hi, d == 0 ? 0 : (d == 1 ? n0 : n01));
+      for (int t = 0; t < cnt; ++t) {
+        if ((t & 3) == 0) word = list[(t >> 2) * lstride + s];
+        const int kk = (int)((word >> ((t & 3) << 3)) & 255u);
+        const int d = kk < n0 ? 0 : (kk < n01 ? 1 : 2);
+        const int o = d == 0 ? l0 + kk : (d == 1 ? l1 + kk : l2 + kk);
+        const int li = staged ? (d == 0 ? s0 + kk : (d == 1 ? s1 + kk : s2 + kk)) : -1;
+        f(o, d, li);
       }
-    } else {
-#pragma unroll 1
-      for (int w = 0; w < 4; ++w) {
-        const int base = c0 + 64 * w;
-        const int cn = min(64, total - base);
-        uint64_t mw = 0;
-        if (staged) {
-#pragma unroll 2
-          for (int k = 0; k < cn; ++k) {
-            const int kk = base + k;
-            const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
-            const int li = kk < n0 ? s0 + kk : (kk < n01 ? s1 + kk : s2 + kk);
-            const float2 q = lds32[li];
-            const float fx = ps32.x - q.x, fy = ps32.y - q.y;
-            const bool ok = (fx * fx + fy * fy <= P.cut2_32) && (o != s);
-            mw |= (uint64_t)ok << k;
-          }
-        } else {
-#pragma unroll 2
-          for (int k = 0; k < cn; ++k) {
-            const int kk = base + k;
-            const int o = kk < n0 ? l0 + kk : (kk < n01 ? l1 + kk : l2 + kk);
-            const float2 q = pos32[o];
-            const float fx = ps32.x - q.x, fy = ps32.y - q.y;
-            const bool ok = (fx * fx + fy * fy <= P.cut2_32) && (o != s);
-            mw |= (uint64_t)ok << k;
-          }
-        }
-        if (w == 0) m0 = mw; else if (w == 1) m1 = mw; else if (w == 2) m2 = mw; else m3 = mw;
-      }
+      return;
     }
-    uint64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;        // kNbRecord: the exactly accepted bits
+  }
+  const float2 ps32 = pos32[s];
+  const __amdgpu_buffer_rsrc_t rpos = rec_rsrc(reinterpret_cast<const float4*>(pos));
+  int nrec = 0;                                      // kNbRecord: accepted so far
+  uint32_t word = 0;
+  for (int c0 = 0; c0 < total; c0 += 256) {
+    const Mask256 sm = screen_chunk(P, pos32, s, ps32, R, n0, n1, n2, n01, total, l0, l1, l2, staged, s0, s1, s2,
+                                    lds32, c0);
+    uint64_t m0 = sm.w0, m1 = sm.w1, m2 = sm.w2, m3 = sm.w3;
 #pragma unroll 1
     while ((m0 | m1 | m2 | m3) != 0) {
       const int w = m0 ? 0 : (m1 ? 1 : (m2 ? 2 : 3));
@@ -434,24 +462,28 @@ __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int
       const int d = kk < n0 ? 0 : (kk < n01 ? 1 : 2);     // stencil column of the candidate
       const int o = d == 0 ? l0 + kk : (d == 1 ? l1 + kk : l2 + kk);
       const int li = staged ? (d == 0 ? s0 + kk : (d == 1 ? s1 + kk : s2 + kk)) : -1;   // LDS slot
-      bool accept = true;                         // a replayed mask is exact
-      if (!replay) {
-        double2 po;
-        if (staged) po = lds64[li];
-        else po = buf_d2(rpos, o);                 // a buffer load: never merged with the LDS one
-        accept = is_pair_r2(P, ps.x - po.x, ps.y - po.y);
-        if (MODE == kNbRecord && accept) {
-          if (w == 0) a0 |= bit; else if (w == 1) a1 |= bit; else if (w == 2) a2 |= bit; else a3 |= bit;
+      double2 po;
+      if (staged) po = lds64[li];
+      else po = buf_d2(rpos, o);                   // a buffer load: never merged with the LDS one
+      const bool accept = is_pair_r2(P, ps.x - po.x, ps.y - po.y);
+      if (MODE == kNbRecord && accept && stored) {
+        if (nrec < kNbListCap) {
+          word |= (uint32_t)kk << ((nrec & 3) << 3);
+          if ((nrec & 3) == 3) {
+            list[(nrec >> 2) * lstride + s] = word;
+            word = 0;
+          }
         }
+        ++nrec;
       }
       if (accept) f(o, d, li);                    // one call site: the body inlines once
     }
-    if (MODE == kNbRecord && stored) {
-      masks[s] = a0;
-      masks[mstride + s] = a1;
-      masks[2 * mstride + s] = a2;
-      masks[3 * mstride + s] = a3;
-    }
+  }
+  if (MODE == kNbRecord) {
+    if (stored && (nrec & 3) && nrec < kNbListCap) list[(nrec >> 2) * lstride + s] = word;
+    // a list longer than 256 candidates, or with more neighbours than recorded, is
+    // screened again by every later pass
+    counts[s] = nrec | ((!stored || nrec > kNbListCap) ? kNbOverflow : 0);
   }
 }
 

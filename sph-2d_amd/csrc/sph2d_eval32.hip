@@ -2,15 +2,16 @@
 //
 // Same passes, order and pair terms as sph2d_eval64.hip, with the pair sums in
 // FP32 and Kahan-compensated accumulation. What stays FP64: particle positions
-// (the exact pair test 1e-7 <= r <= 3h in pass A, whose acceptance mask the
-// force pass replays), the integrator state and stage updates and the
-// per-particle SPS closure. Density enters the pair sums as delta rho =
-// rho - rho0 (FP32), so rho_o - rho_s carries no cancellation; p and c of a
-// neighbour follow from its delta rho by the Tait law written as polynomials in
-// delta rho / rho0 (no cancellation either). In the force pass the offset
-// x_s - x_o comes from FP32 offsets to each particle's cell corner plus the
-// whole-cell difference, which keeps it to ~1e-7 h; a neighbour costs 48 bytes
-// (three float4 records) there, 16 in pass A.
+// (the exact pair test 1e-7 <= r <= 3h in pass A, whose record of accepted
+// neighbours the force pass replays), the integrator state and stage updates,
+// the per-particle SPS closure and the Tait law: p and c of every particle are
+// evaluated in FP64 from its FP64 density by the kernel that writes its records
+// and stored rounded to FP32 (m_pc), so the pair sums never evaluate an EOS.
+// Density enters the pair sums as delta rho = rho - rho0 (FP32), so rho_o - rho_s
+// carries no cancellation. In the force pass the offset x_s - x_o comes from FP32
+// offsets to each particle's cell corner plus the whole-cell difference, which
+// keeps it to ~1e-7 h; a neighbour costs 56 bytes (three float4 records and the
+// (p, c) pair) there, 32 in pass A.
 #include <cstring>
 #include "sph2d_internal.h"
 #include "sph2d_device.h"
@@ -34,7 +35,7 @@ struct KSum {
 struct DevParams32 {
   float inv_h, e9, kfactor, inv_sv, rho0, inv_c0, h, fai_h2, alpha, beta, two_nu;
   float delta_alpha_h, two_delta_h, shift_beta_h2, shift_R, inv_wdx, epsilon2, x_over_t;
-  float inv_rho0, eos_b, c0, cell_d;
+  float cell_d;
 };
 
 static DevParams32 make32(const DevParams& P) {
@@ -57,23 +58,8 @@ static DevParams32 make32(const DevParams& P) {
   Q.inv_wdx = (float)P.inv_wdx;
   Q.epsilon2 = (float)(P.epsilon_xsph * P.xsph_f);
   Q.x_over_t = (float)P.x_over_t;
-  Q.inv_rho0 = (float)(1.0 / P.rho0);
-  Q.eos_b = (float)P.eos_b;
-  Q.c0 = (float)P.sqrt_c0sq;
   Q.cell_d = (float)P.chain_d;
   return Q;
-}
-
-// status_equation.f90:61-63 in terms of x = (rho - rho0)/rho0:
-// p = B((1+x)^7 - 1) = B x (7 + 21x + 35x^2 + 35x^3 + 21x^4 + 7x^5 + x^6), c = c0 (1+x)^3.
-__device__ __forceinline__ float tait_p32(const DevParams32& Q, float drho) {
-  const float x = drho * Q.inv_rho0;
-  return Q.eos_b * (x * (7.f + x * (21.f + x * (35.f + x * (35.f + x * (21.f + x * (7.f + x)))))));
-}
-
-__device__ __forceinline__ float tait_c32(const DevParams32& Q, float drho) {
-  const float x = drho * Q.inv_rho0;
-  return Q.c0 * (1.f + x * (3.f + x * (3.f + x)));
 }
 
 struct KernelTerm32 {
@@ -106,46 +92,118 @@ __device__ __forceinline__ void k_eval_a32_body(const DevParams& P, const DevPar
   const float4 r0s = B.m_r0[s];
   const float2 vs = make_float2(r0s.x, r0s.y);
   const float drs = r0s.z;
-  const float pres = MUSCL ? tait_p32(Q, drs) : 0.f;
+  const float pres = MUSCL ? B.m_pc[s].x : 0.f;   // FP64 Tait law, rounded to FP32
   const int ts = B.l_type[s];
   KSum fra, sdx, sdy, Sxx, Sxy, Syy, grx, gry;
   KSum gpx, gpy, gux, guy, gvx, gvy;
   const __amdgpu_buffer_rsrc_t br0 = rec_rsrc(B.m_r0), bpos = rec_rsrc((const float4*)B.l_pos);
-  for_each_neighbour<kNbRecord>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) __attribute__((always_inline)) {
-    const float4 r0o = buf_f4(br0, o);
-    if (ts != 2 && r0o.w < 0.f) return;          // no pair between two non-fluid particles
-    double2 po;
-    if (li >= 0) po = lds64[li];                  // staged: the exact test's copy
-    else po = buf_d2(bpos, o);
-    const float dx = (float)(ps.x - po.x), dy = (float)(ps.y - po.y);
-    const float r2 = dx * dx + dy * dy;
-    const float inv_r = __builtin_amdgcn_rsqf(r2);   // r2 >= 1e-14: no denormal scaling
-    const float r = r2 * inv_r;
-    const KernelTerm32 K = kernel_term32(Q, dx, dy, r, inv_r);
-    const float2 rvo = make_float2(r0o.z, fabsf(r0o.w) * __builtin_amdgcn_rcpf(Q.rho0 + r0o.z));
-    const float2 vo = make_float2(r0o.x, r0o.y);
-    const float Vo = rvo.y;
-    fra.add(Vo * Q.inv_sv);
-    sdx.add(K.dwx * Vo);
-    sdy.add(K.dwy * Vo);
-    const float vdx = vo.x - vs.x, vdy = vo.y - vs.y;
-    Sxx.add(vdx * K.dwx * Vo);
-    Sxy.add((vdx * K.dwy + vdy * K.dwx) * Vo);
-    Syy.add(vdy * K.dwy * Vo);
-    const float rd = (rvo.x - drs) * Vo;
-    grx.add(rd * K.dwx);
-    gry.add(rd * K.dwy);
-    if (MUSCL) {
-      const float pd = (tait_p32(Q, rvo.x) - pres) * Vo;
-      gpx.add(pd * K.dwx);
-      gpy.add(pd * K.dwy);
-      const float ud = vdx * Vo, vd = vdy * Vo;
-      gux.add(ud * K.dwx);
-      guy.add(ud * K.dwy);
-      gvx.add(vd * K.dwx);
-      gvy.add(vd * K.dwy);
+  const __amdgpu_buffer_rsrc_t bpc = rec_rsrc(reinterpret_cast<const float4*>(B.m_pc));
+  // find_particle_pair.f90:64-123 written out (the shared neighbour loop records
+  // the same slots): the FP32 screen of the 3x3 stencil, then the survivors in
+  // ascending order, each with the exact FP64 test of :87; the next survivor's
+  // loads are issued before the current survivor's test and sums, so their
+  // latency overlaps that work. The accepted slots are recorded for the later
+  // passes (one byte each, kNbOverflow when they do not fit).
+  const float2 ps32 = B.l_pos32[s];
+  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
+  const int n0 = R.hi[0] - R.lo[0], n1 = R.hi[1] - R.lo[1], n2 = R.hi[2] - R.lo[2];
+  const int n01 = n0 + n1, total = n01 + n2;
+  const int l0 = R.lo[0], l1 = R.lo[1] - n0, l2 = R.lo[2] - n01;
+  const bool staged = st.use;
+  const int so0 = staged ? l0 - st.lo[0] + st.base[0] : 0;
+  const int so1 = staged ? l1 - st.lo[1] + st.base[1] : 0;
+  const int so2 = staged ? l2 - st.lo[2] + st.base[2] : 0;
+  const bool stored = total <= 256;
+  uint32_t* const nbl = B.nb_list;
+  const int lst = B.kstride;
+  int nrec = 0;
+  uint32_t word = 0;
+  for (int c0 = 0; c0 < total; c0 += 256) {
+    const Mask256 sm = screen_chunk(P, B.l_pos32, s, ps32, R, n0, n1, n2, n01, total, l0, l1, l2, staged, so0, so1,
+                                    so2, lds32, c0);
+    uint64_t m0 = sm.w0, m1 = sm.w1, m2 = sm.w2, m3 = sm.w3;
+    int kk_n = -1;
+    if ((m0 | m1 | m2 | m3) != 0) {
+      const int w = m0 ? 0 : (m1 ? 1 : (m2 ? 2 : 3));
+      const uint64_t mm = m0 ? m0 : (m1 ? m1 : (m2 ? m2 : m3));
+      const int b = __builtin_ctzll(mm);
+      const uint64_t bit = 1ull << b;
+      if (w == 0) m0 ^= bit; else if (w == 1) m1 ^= bit; else if (w == 2) m2 ^= bit; else m3 ^= bit;
+      kk_n = c0 + w * 64 + b;
     }
-  }, B.nb_mask, B.kstride, &st, lds32, lds64);
+    float4 r0n = make_float4(0.f, 0.f, 0.f, 0.f);
+    double2 pn = make_double2(0.0, 0.0);
+    if (kk_n >= 0) {
+      const int dn = kk_n < n0 ? 0 : (kk_n < n01 ? 1 : 2);
+      const int on = dn == 0 ? l0 + kk_n : (dn == 1 ? l1 + kk_n : l2 + kk_n);
+      r0n = buf_f4(br0, on);
+      if (staged) pn = lds64[dn == 0 ? so0 + kk_n : (dn == 1 ? so1 + kk_n : so2 + kk_n)];
+      else pn = buf_d2(bpos, on);
+    }
+    while (kk_n >= 0) {
+      const int kk = kk_n;
+      const int d = kk < n0 ? 0 : (kk < n01 ? 1 : 2);
+      const int o = d == 0 ? l0 + kk : (d == 1 ? l1 + kk : l2 + kk);
+      const float4 r0o = r0n;
+      const double2 po = pn;
+      kk_n = -1;
+      if ((m0 | m1 | m2 | m3) != 0) {
+        const int w = m0 ? 0 : (m1 ? 1 : (m2 ? 2 : 3));
+        const uint64_t mm = m0 ? m0 : (m1 ? m1 : (m2 ? m2 : m3));
+        const int b = __builtin_ctzll(mm);
+        const uint64_t bit = 1ull << b;
+        if (w == 0) m0 ^= bit; else if (w == 1) m1 ^= bit; else if (w == 2) m2 ^= bit; else m3 ^= bit;
+        kk_n = c0 + w * 64 + b;
+        const int dn = kk_n < n0 ? 0 : (kk_n < n01 ? 1 : 2);
+        const int on = dn == 0 ? l0 + kk_n : (dn == 1 ? l1 + kk_n : l2 + kk_n);
+        r0n = buf_f4(br0, on);
+        if (staged) pn = lds64[dn == 0 ? so0 + kk_n : (dn == 1 ? so1 + kk_n : so2 + kk_n)];
+        else pn = buf_d2(bpos, on);
+      }
+      if (!is_pair_r2(P, ps.x - po.x, ps.y - po.y)) continue;
+      if (stored) {
+        if (nrec < kNbListCap) {
+          word |= (uint32_t)kk << ((nrec & 3) << 3);
+          if ((nrec & 3) == 3) {
+            nbl[(nrec >> 2) * lst + s] = word;
+            word = 0;
+          }
+        }
+        ++nrec;
+      }
+      if (ts != 2 && r0o.w < 0.f) continue;        // no pair between two non-fluid particles
+      const float dx = (float)(ps.x - po.x), dy = (float)(ps.y - po.y);
+      const float r2 = dx * dx + dy * dy;
+      const float inv_r = __builtin_amdgcn_rsqf(r2);   // r2 >= 1e-14: no denormal scaling
+      const float r = r2 * inv_r;
+      const KernelTerm32 K = kernel_term32(Q, dx, dy, r, inv_r);
+      const float2 rvo = make_float2(r0o.z, fabsf(r0o.w) * __builtin_amdgcn_rcpf(Q.rho0 + r0o.z));
+      const float2 vo = make_float2(r0o.x, r0o.y);
+      const float Vo = rvo.y;
+      fra.add(Vo * Q.inv_sv);
+      sdx.add(K.dwx * Vo);
+      sdy.add(K.dwy * Vo);
+      const float vdx = vo.x - vs.x, vdy = vo.y - vs.y;
+      Sxx.add(vdx * K.dwx * Vo);
+      Sxy.add((vdx * K.dwy + vdy * K.dwx) * Vo);
+      Syy.add(vdy * K.dwy * Vo);
+      const float rd = (rvo.x - drs) * Vo;
+      grx.add(rd * K.dwx);
+      gry.add(rd * K.dwy);
+      if (MUSCL) {
+        const float pd = (buf_f2(bpc, o).x - pres) * Vo;
+        gpx.add(pd * K.dwx);
+        gpy.add(pd * K.dwy);
+        const float ud = vdx * Vo, vd = vdy * Vo;
+        gux.add(ud * K.dwx);
+        guy.add(ud * K.dwy);
+        gvx.add(vd * K.dwx);
+        gvy.add(vd * K.dwy);
+      }
+    }
+  }
+  if (stored && (nrec & 3) && nrec < kNbListCap) nbl[(nrec >> 2) * lst + s] = word;
+  B.nb_count[s] = nrec | ((!stored || nrec > kNbListCap) ? kNbOverflow : 0);
   const double f = (double)fra.s;
   B.fra_v[s] = f;
   B.sdw[s] = make_double2((double)sdx.s, (double)sdy.s);
@@ -216,7 +274,7 @@ __device__ __forceinline__ void k_eval_c32_body(const DevParams& P, const DevPar
   const float4 r2s = B.m_r2[s];
   const float2 vs = make_float2(r0s.x, r0s.y);
   const float2 rvs = make_float2(r0s.z, 0.f);
-  const float2 pcs = make_float2(tait_p32(Q, r0s.z), tait_c32(Q, r0s.z));
+  const float2 pcs = B.m_pc[s];                    // FP64 Tait law, rounded to FP32
   const int ts = B.l_type[s];
   const float2 grs = make_float2(r2s.x, r2s.y);
   const int nes = B.near_[s];
@@ -247,6 +305,7 @@ __device__ __forceinline__ void k_eval_c32_body(const DevParams& P, const DevPar
   // LDS loads are different instructions, never merged into one flat load of a
   // selected address (flat loads of LDS data are slower than ds_read)
   const __amdgpu_buffer_rsrc_t br0 = rec_rsrc(B.m_r0), br1 = rec_rsrc(B.m_r1), br2 = rec_rsrc(B.m_r2);
+  const __amdgpu_buffer_rsrc_t bpc = rec_rsrc(reinterpret_cast<const float4*>(B.m_pc));
   // The neighbours recorded by pass A (for_each_neighbour<kNbReplay>, written out:
   // a closure over this kernel's locals would be kept in scratch memory).
   const CellRanges R = cell_ranges(P, B.cell_start, cell_s);
@@ -255,46 +314,78 @@ __device__ __forceinline__ void k_eval_c32_body(const DevParams& P, const DevPar
   const int so0 = staged ? l0 - st.lo[0] + st.base[0] : 0;
   const int so1 = staged ? l1 - st.lo[1] + st.base[1] : 0;
   const int so2 = staged ? l2 - st.lo[2] + st.base[2] : 0;
+  // pass A's record of the accepted neighbours (one byte per neighbour, in ascending
+  // candidate order); a list longer than 256 candidates or more neighbours than the
+  // record holds are screened and tested again, 256 candidates at a time
+  const int cnt = B.nb_count[s];
+  const bool listed = total <= 256 && !(cnt & kNbOverflow);
+  const uint32_t* const nbl = B.nb_list;
+  const int lst = B.kstride;
   for (int c0 = 0; c0 < total; c0 += 256) {
-  uint64_t m0, m1, m2, m3;
-  if (total <= 256) {
-    const uint64_t* msk = B.nb_mask;
-    const int ms = B.kstride;
-    m0 = msk[s];
-    m1 = msk[ms + s];
-    m2 = msk[2 * ms + s];
-    m3 = msk[3 * ms + s];
-    if (total < 256) {                             // keep bits below total only
-      const int t = total & 63;
-      const uint64_t part = t ? ((1ull << t) - 1ull) : 0ull;
-      const int wl = total >> 6;
-      if (wl == 0) { m0 &= part; m1 = 0; m2 = 0; m3 = 0; }
-      else if (wl == 1) { m1 &= part; m2 = 0; m3 = 0; }
-      else if (wl == 2) { m2 &= part; m3 = 0; }
-      else { m3 &= part; }
-    }
-  } else {
+  uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+  if (!listed) {
     const Mask256 mk = exact_chunk_mask(P, B.l_pos, B.l_pos32, s, B.l_pos[s], n0, n01, l0, l1, l2, total, c0);
     m0 = mk.w0; m1 = mk.w1; m2 = mk.w2; m3 = mk.w3;
   }
-  while ((m0 | m1 | m2 | m3) != 0) {
+  // The slots come one ahead of the pair they feed: each iteration takes the
+  // next slot and issues its record loads before the current pair's arithmetic,
+  // so the L2 and LDS latency of the loads overlaps the work of the pair before.
+  uint32_t word = 0, word_nx = 0;                  // the list word in use and the next one
+  int t = 0;
+  int kk_n = -1;
+  if (listed) {
+    if (cnt > 0) { word = nbl[s]; kk_n = (int)(word & 255u); }
+    if (cnt > 4) word_nx = nbl[lst + s];
+  } else if ((m0 | m1 | m2 | m3) != 0) {
     const int w = m0 ? 0 : (m1 ? 1 : (m2 ? 2 : 3));
     const uint64_t mm = m0 ? m0 : (m1 ? m1 : (m2 ? m2 : m3));
     const int b = __builtin_ctzll(mm);
     const uint64_t bit = 1ull << b;
     if (w == 0) m0 ^= bit; else if (w == 1) m1 ^= bit; else if (w == 2) m2 ^= bit; else m3 ^= bit;
-    const int kk = c0 + w * 64 + b;
-    const int d = kk < n0 ? 0 : (kk < n01 ? 1 : 2);
-    const int o = d == 0 ? l0 + kk : (d == 1 ? l1 + kk : l2 + kk);
-    const int li = d == 0 ? so0 + kk : (d == 1 ? so1 + kk : so2 + kk);
-    float4 r0o, r1o, r2o;
-    r1o = buf_f4(br1, o);
-    if (staged) {
-      r0o = rec0[li];
-      r2o = rec2[li];
-    } else {
-      r0o = buf_f4(br0, o);
-      r2o = buf_f4(br2, o);
+    kk_n = c0 + w * 64 + b;
+  }
+  float4 r0n = make_float4(0.f, 0.f, 0.f, 0.f), r1n = r0n, r2n = r0n;
+  float2 pcn = make_float2(0.f, 0.f);
+  int dn = 0, on = 0;
+  if (kk_n >= 0) {
+    dn = kk_n < n0 ? 0 : (kk_n < n01 ? 1 : 2);
+    on = dn == 0 ? l0 + kk_n : (dn == 1 ? l1 + kk_n : l2 + kk_n);
+    const int lin = dn == 0 ? so0 + kk_n : (dn == 1 ? so1 + kk_n : so2 + kk_n);
+    r1n = buf_f4(br1, on);
+    pcn = buf_f2(bpc, on);
+    if (staged) { r0n = rec0[lin]; r2n = rec2[lin]; }
+    else { r0n = buf_f4(br0, on); r2n = buf_f4(br2, on); }
+  }
+  while (kk_n >= 0) {
+    const int d = dn, o = on;
+    const float4 r0o = r0n, r1o = r1n, r2o = r2n;
+    const float2 pco = pcn;
+    ++t;
+    kk_n = -1;
+    if (listed) {
+      if (t < cnt) {
+        if ((t & 3) == 0) {
+          word = word_nx;
+          if (t + 4 < cnt) word_nx = nbl[((t >> 2) + 1) * lst + s];
+        }
+        kk_n = (int)((word >> ((t & 3) << 3)) & 255u);
+      }
+    } else if ((m0 | m1 | m2 | m3) != 0) {
+      const int w = m0 ? 0 : (m1 ? 1 : (m2 ? 2 : 3));
+      const uint64_t mm = m0 ? m0 : (m1 ? m1 : (m2 ? m2 : m3));
+      const int b = __builtin_ctzll(mm);
+      const uint64_t bit = 1ull << b;
+      if (w == 0) m0 ^= bit; else if (w == 1) m1 ^= bit; else if (w == 2) m2 ^= bit; else m3 ^= bit;
+      kk_n = c0 + w * 64 + b;
+    }
+    if (kk_n >= 0) {
+      dn = kk_n < n0 ? 0 : (kk_n < n01 ? 1 : 2);
+      on = dn == 0 ? l0 + kk_n : (dn == 1 ? l1 + kk_n : l2 + kk_n);
+      const int lin = dn == 0 ? so0 + kk_n : (dn == 1 ? so1 + kk_n : so2 + kk_n);
+      r1n = buf_f4(br1, on);
+      pcn = buf_f2(bpc, on);
+      if (staged) { r0n = rec0[lin]; r2n = rec2[lin]; }
+      else { r0n = buf_f4(br0, on); r2n = buf_f4(br2, on); }
     }
     if (ts != 2 && r0o.w < 0.f) continue;        // no pair between two non-fluid particles
     // bitwise blends, not selects: a select of captured values can become a
@@ -314,7 +405,6 @@ __device__ __forceinline__ void k_eval_c32_body(const DevParams& P, const DevPar
     const float mo_x = fabsf(r0o.w);
     const float rhoo = Q.rho0 + r0o.z;
     const float2 rvo = make_float2(r0o.z, mo_x * __builtin_amdgcn_rcpf(rhoo));
-    const float2 pco = make_float2(tait_p32(Q, r0o.z), tait_c32(Q, r0o.z));
     const float Vo = rvo.y;
     const float ex = -(dx * inv_r), ey = -(dy * inv_r);
     const float UL = vs.x * ex + vs.y * ey;

@@ -89,7 +89,7 @@ __device__ __forceinline__ void k_eval_a_body(const DevParams& P, Buffers& B, in
       gvx = gvx + vd * K.dwx;
       gvy = gvy + vd * K.dwy;
     }
-  }, B.nb_mask, B.kstride, &st, lds32, lds64);
+  }, B.nb_list, B.nb_count, B.kstride, &st, lds32, lds64);
   B.fra_v[s] = fra;
   B.sdw[s] = make_double2(sdx, sdy);
   B.grad[s] = make_double2(grx, gry);
@@ -183,7 +183,7 @@ __device__ __forceinline__ void k_eval_b2_body(const DevParams& P, Buffers& B, i
   int nr = 0;
   for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, B.l_pos[s], B.l_cell[s], [&](int o, int d, int li) __attribute__((always_inline)) {
     if (B.surf[o] == 1 && (ts == 2 || B.l_type[o] == 2)) nr = 1;
-  }, B.nb_mask, B.kstride);
+  }, B.nb_list, B.nb_count, B.kstride);
   B.near_[s] = nr;
 }
 
@@ -356,7 +356,7 @@ __device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int
       shx = shx - f * K.dwx;
       shy = shy - f * K.dwy;
     }
-  }, B.nb_mask, B.kstride);
+  }, B.nb_list, B.nb_count, B.kstride);
   if (ts == 2) drho = drho + corr;
   B.drhodt[s] = drho;
   if (DEBUG) {
@@ -473,7 +473,7 @@ __global__ void __launch_bounds__(kBlock) k_vorticity(DevParams P, Buffers B) {
     const double2 vo = B.l_vel[o];
     const double vx = vs.x - vo.x, vy = vs.y - vo.y;
     wz = wz + B.l_rv[o].y * (vx * dwy - vy * dwx);   // Cross_Product(...)(3), function_module.f90:150
-  }, B.nb_mask, B.kstride);
+  }, B.nb_list, B.nb_count, B.kstride);
   B.g_vort[g] = wz;
 }
 

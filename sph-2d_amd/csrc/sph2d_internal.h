@@ -16,6 +16,9 @@
 namespace sph2d {
 
 constexpr int kBlock = 256;
+constexpr int kNbListCap = 96;                  // recorded neighbours per particle (24 words)
+constexpr int kNbListWords = kNbListCap / 4;
+constexpr int kNbOverflow = 1 << 30;            // nb_count flag: the list is not complete
 
 // Constants of one run, passed by value to every kernel.
 struct DevParams {
@@ -83,8 +86,11 @@ struct Buffers {
   double *tau;                            // 3 per particle: xx, xy, yy
   double2 *mg;                            // MUSCL only, 3 per particle: grad p, grad u, grad v
   int *surf, *near_;
-  // accepted-neighbour masks of the last pass A: 4 words per particle, word w at [w * kstride + s]
-  uint64_t* nb_mask;
+  // accepted neighbours of the last pass A: the candidate slots (0..255) in ascending
+  // order, one byte each, 4 per word, word w of particle s at nb_list[w * kstride + s];
+  // nb_count[s] their number, with kNbOverflow set when they exceeded kNbListCap
+  uint32_t* nb_list;
+  int* nb_count;
   double *drhodt;
   double2 *acc;
   double *dbg;                            // 10 per particle (debug components)
@@ -105,6 +111,7 @@ struct Buffers {
   //          corner of the particle's binned cell, so neighbour offsets keep
   //          FP32 precision relative to h
   float4 *m_r0, *m_r1, *m_r2;
+  float2 *m_pc;                           // (p, c) from the FP64 Tait law of the particle's rho
   float2 *m_mg;                           // MUSCL only: FP32 copy of mg
 
   int kstride;                            // stride of the 3 RK stage slabs (= local capacity)
