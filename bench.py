@@ -12,10 +12,13 @@ density reinitialisation every 20 steps, particle shifting and the refresh.
 The state is resident in HBM when the timed region starts; timing uses HIP
 events on the context's stream. Prints ONE JSON line.
 
-Also reported: the force pass's roofline (algorithmic bytes per launch over its
-duration, from HIP events recorded around the pass inside real steps on the
-context's stream) and the CPU baseline (the C oracle, one core, on a bounded
-sample of the same workload).
+Also reported: the HBM roofline of the dominant neighbour pass (the frozen
+algorithmic bytes per particle of that pass times the particles of one launch,
+over its duration from HIP events recorded around the pass inside real steps on
+the context's stream), the same model for the whole step, the kinetic-energy
+error between the two forms at t = 0.5, and the CPU baseline (the C oracle on
+the host: one core on the 2048^2 workload itself, and one process per core of
+the box's share on 512^2 samples).
 """
 from __future__ import annotations
 
@@ -44,40 +47,55 @@ def hip_events():
     return hip
 
 
-def force_pass_bytes_per_particle(form: int) -> int:
-    """Algorithmic HBM bytes per subdomain particle of one force-pass launch:
-    every array the pass reads for its own particle once, every output once; the
-    neighbours' data are the same arrays (served from L2/L1), not counted again.
-    FP64 (k_eval_c) reads: pos 16, vel 16, (rho,V) 16, (p,c) 16, type 4, cell 4,
-      near 4, tau 24, grad rho 16, mass 8, particle_mass at local index 8, the
-      recorded neighbour mask 32 = 164
-    FP32-mixed (k_eval_c32) reads: the three float4 records 48, type 4, cell 4,
-      near 4, the neighbour mask 32 = 92
-    both write: drho/dt 8, acceleration 16, shift 16 = 40."""
-    return (164 if form == 64 else 92) + 40
+# Frozen algorithmic HBM bytes per fluid particle (compulsory traffic of the fused
+# three-pass design: each per-particle array read once and written once per pass,
+# neighbour data served from LDS/L2, pairs never stored; FP64 state x, y, u, v, rho,
+# m at 8 B plus type 4 B = 52 B, the mixed form's u, v, delta rho, m at 4 B = 36 B).
+PASS_BYTES = {
+    # P1 neighbour sums (S_xx, S_xy, S_yy, grad rho, fra_v, sum dW): read state, write 8 values
+    "pass_a": {64: 52 + 64, 32: 36 + 32},
+    # P3 forces: read state + S + grad rho + flag, write drho/dt and a
+    "force": {64: 93 + 24, 32: 57 + 12},
+}
+# one predictor-corrector step with one evaluation (P1 + P2 flags + P3), the stage
+# update, the key/sort/permute and the shift: 295 + 144 + 168 + 76 (FP64) and
+# 179 + 96 + 136 + 56 (mixed) bytes per particle-step
+STEP_BYTES = {64: 295 + 144 + 168 + 76, 32: 179 + 96 + 136 + 56}
 
 
-# HBM bytes per launch of the FP32-mixed force pass (k_eval_c32) at 2048^2, measured by
-# rocprofv3 --pmc FETCH_SIZE and then WRITE_SIZE on this build (summary and commands in
-# profiles/tgv2048_force_pass_hbm_traffic.md), each counter scaled by what it reads for a
-# 256 MiB device-to-device copy (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00): 392,315 KiB
-# fetched x 2.00 + 165,123 KiB written = 570.8 MB for 4,227,136 subdomain particles.
-FORCE_PASS_TRAFFIC = {"bytes_per_particle": 135.0, "n_side": 2048,
-                      "source": "profiles/tgv2048_force_pass_hbm_traffic.md"}
-# FLOPs per subdomain particle of one force-pass launch (SQ_INSTS_VALU_FLOPS_* x 64 lanes)
-FORCE_PASS_VALU = {"fp32_flop_per_particle": 7299.1, "fp64_flop_per_particle": 9006.3,
-                   "source": "profiles/tgv2048_force_pass_valu.md"}
+# HBM bytes per launch of the mixed passes at 2048^2, per subdomain particle, measured by
+# rocprofv3 --pmc FETCH_SIZE and then WRITE_SIZE on this build, each counter scaled by
+# what it reads for a 256 MiB device-to-device copy (summary and commands in the source).
+_TRAFFIC_SRC = "profiles/tgv2048_force_pass_hbm_traffic.md"
+PASS_TRAFFIC = {
+    "force": {"bytes_per_particle": 152.7, "n_side": 2048, "source": _TRAFFIC_SRC},
+    "pass_a": {"bytes_per_particle": 165.7, "n_side": 2048, "source": _TRAFFIC_SRC},
+}
+# what each mixed pass reads and writes for its own particle, per launch:
+# force pass: records 48 + (p, c) 8, type, cell, near 12, neighbour count 4 and list ~40;
+#   writes drho/dt 8, acceleration 16, shift 16
+# pass A: r0 16, FP64 position 16, FP32 position 8, type and cell 8; writes the list ~36,
+#   its count 4, fra_v 8, sum dW 16, tau 16 (r1), grad rho 16 (r2), flags 8
+PASS_READ_WRITE = {"force": 48 + 8 + 12 + 44 + 40, "pass_a": 48 + 104}
+# FLOPs per subdomain particle of one launch (SQ_INSTS_VALU_FLOPS_* x 64 lanes)
+_VALU_SRC = "profiles/tgv2048_force_pass_valu.md"
+PASS_VALU = {
+    "force": {"fp32_flop_per_particle": 6450.0, "fp64_flop_per_particle": 2.0, "source": _VALU_SRC},
+    "pass_a": {"fp32_flop_per_particle": 3522.0, "fp64_flop_per_particle": 225.0, "source": _VALU_SRC},
+}
 FP32_VECTOR_TFLOPS = 157.3
+FP32_VECTOR_TFLOPS_UNPACKED = 78.6
 
 
-def cpu_baseline(n_side: int = 512, steps: int = 20) -> dict:
+def cpu_baseline(n_side: int = 1024, steps: int = 1) -> dict:
     """The C oracle (oracle/liboracle.so, the reference's algorithm restated,
     scatter over a half pair list as in find_particle_pair.f90), one core, on a
-    bounded sample: n_side^2 TGV, predictor-corrector, `steps` steps."""
+    bounded sample of the workload itself: n_side^2 TGV, predictor-corrector,
+    `steps` steps after step 1 (which has the extra evaluation)."""
     from oracle.pyoracle import Oracle
     o = Oracle(nx=n_side, ny=n_side, integrator=2, dt=1e-4 * 100.0 / n_side)
     o.init_tgv()
-    o.time_step(1)                                # step 1 has the extra evaluation
+    o.time_step(1)
     t0 = time.perf_counter()
     for it in range(2, 2 + steps):
         o.time_step(it)
@@ -103,25 +121,78 @@ for it in range(2, 2 + steps):
 print(n * n * steps / (time.perf_counter() - t0))
 """
 
+# host cores a one-GPU job may use on these machines (its share of the node, whatever
+# lscpu reports for the whole node)
+BOX_CORE_SHARE = 16
 
-def cpu_baseline_all_cores(n_side: int = 512, steps: int = 20) -> dict:
-    """The same sample on every host core at once: one oracle process per core (the
-    oracle, like one reference rank, runs one simulation on one core), each its own
-    instance; the value is the sum of their rates. Children are started as new
-    programs, not forks of this GPU process."""
+
+def physical_cores() -> int:
+    """Physical cores of the host from lscpu (sockets x cores per socket)."""
     import subprocess
-    cores = max(1, min(16, os.cpu_count() or 1))
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        return int(kv["Socket(s)"].strip()) * int(kv["Core(s) per socket"].strip())
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline_all_cores(n_side: int, steps: int) -> dict:
+    """One oracle process per core (the oracle, like one reference rank, runs one
+    simulation on one core), each its own instance of an n_side^2 sample, all at
+    once; the value is the sum of their rates. The core count is lscpu's physical
+    cores, capped at the host share of a one-GPU job (16 cores; a 1024^2 instance
+    holds ~4.4 GB, so 16 of them fit the job's host memory where 2048^2 ones would
+    not). Children are started as new programs, not forks of this GPU process."""
+    import subprocess
+    phys = physical_cores()
+    cores = max(1, min(phys, BOX_CORE_SHARE))
     code = _CPU_WORKER.format(root=ROOT, n=n_side, steps=steps)
+    t0 = time.perf_counter()
     procs = [subprocess.Popen([sys.executable, "-c", code], stdout=subprocess.PIPE, text=True) for _ in range(cores)]
     rates = []
     for p in procs:
-        out, _ = p.communicate(timeout=600)
+        out, _ = p.communicate(timeout=900)
         if p.returncode == 0:
             rates.append(float(out.strip().splitlines()[-1]))
     return {"value": sum(rates), "unit": "particle-steps/s", "cores": len(rates),
-            "kind": "port, independent instances",
-            "sample": f"{len(rates)} concurrent single-core oracle processes, each TGV {n_side}^2 "
-                      f"predictor-corrector FP64, steps 2..{1 + steps}"}
+            "physical_cores_lscpu": phys, "core_share": BOX_CORE_SHARE,
+            "kind": "port", "per_core_mean": sum(rates) / max(1, len(rates)),
+            "wall_s": time.perf_counter() - t0,
+            "sample": f"{len(rates)} concurrent single-core oracle processes (independent instances), each "
+                      f"TGV {n_side}^2 predictor-corrector FP64, steps 2..{1 + steps} timed"}
+
+
+def ke_gate_leg(sph2d, n_side: int, t_end: float = 0.5) -> dict:
+    """The accuracy bar of the mixed form: both forms run the n_side^2 TGV
+    (predictor-corrector, dt = 1e-4 * 100 / n_side) to t_end from the same initial
+    condition; relative KE/KE0 difference and each form's error against the analytic
+    decay exp(-16 pi^2 nu t)."""
+    import math
+    dt = 1e-4 * 100.0 / n_side
+    steps = int(round(t_end / dt))
+    res = {}
+    for prec in (sph2d.FP32_MIXED, sph2d.FP64):
+        sim = sph2d.Simulation(sph2d.tgv_params(n_side, integrator=sph2d.INTEGRATOR_PC, precision=prec))
+        sim.init_tgv()
+        t0 = time.perf_counter()
+        chunk = 5000
+        for first in range(1, steps + 1, chunk):
+            sim.run(first, min(chunk, steps + 1 - first))
+            sim.diagnostics()                      # synchronises: a progress line per chunk
+            print(f"# ke gate form {prec}: step {min(first + chunk - 1, steps)} of {steps}", file=sys.stderr,
+                  flush=True)
+        s = sim.sample()
+        res[prec] = (s, time.perf_counter() - t0)
+        sim.close()
+    km, kf = res[sph2d.FP32_MIXED][0][0], res[sph2d.FP64][0][0]
+    nu = sph2d.tgv_params(n_side).nu
+    ka = math.exp(-16 * math.pi ** 2 * nu * t_end)
+    return {"n_side": n_side, "t": steps * dt, "steps": steps, "ke_over_ke0_mixed": km, "ke_over_ke0_fp64": kf,
+            "ke_rel_err_mixed_vs_fp64": abs(km - kf) / kf, "bar": 1e-4,
+            "ke_analytic": ka, "fp64_vs_analytic": (kf - ka) / ka, "mixed_vs_analytic": (km - ka) / ka,
+            "l2_u_mixed": res[sph2d.FP32_MIXED][0][4], "l2_u_fp64": res[sph2d.FP64][0][4],
+            "wall_s": {"mixed": res[sph2d.FP32_MIXED][1], "fp64": res[sph2d.FP64][1]}}
 
 
 def batched_leg(sph2d, instances: int, steps: int, warmup: int) -> dict:
@@ -226,8 +297,17 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n-side", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-side", type=int, default=512)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-all-side", type=int, default=1024,
+                    help="CPU baseline (the reported value): one oracle instance of this lattice per core")
+    ap.add_argument("--cpu-all-steps", type=int, default=1)
+    ap.add_argument("--cpu-side", type=int, default=1024, help="one-core CPU sample on this lattice")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-small-side", type=int, default=100,
+                    help="all-core CPU sample on the reference's own 100^2 case")
+    ap.add_argument("--cpu-small-steps", type=int, default=500)
+    ap.add_argument("--ke-gate-side", type=int, default=1024,
+                    help="KE error of the mixed form vs FP64 at t = 0.5 on this lattice (0 skips; 2048 is the "
+                         "4.2M-particle gate, 102,400 steps per form)")
     ap.add_argument("--forms", choices=("both", "fp64", "mixed"), default="both")
     ap.add_argument("--timing-steps", type=int, default=20,
                     help="steps after the timed run that record the per-pass HIP events")
@@ -318,11 +398,19 @@ def main() -> int:
     sps32 = N * args.steps / r32["total_s"]
     ke_err = abs(r32["sample"][0] - r64["sample"][0]) / r64["sample"][0]
     t_end = (args.warmup + args.steps) * r64["dt"]
-    bytes64 = force_pass_bytes_per_particle(64) * r64["n_local"]
-    bytes32 = force_pass_bytes_per_particle(32) * r32["n_local"]
-    ach32 = bytes32 / (r32["force_ms"] * 1e-3) / 1e9
-    ach64 = bytes64 / (r64["force_ms"] * 1e-3) / 1e9
+    # the dominant neighbour pass of the mixed form in this run, against the HBM roofline
+    # with the frozen algorithmic bytes of that pass (PASS_BYTES)
+    dom = "force" if r32["force_ms"] >= r32["passa_ms"] else "pass_a"
+    dom_ms = r32["force_ms"] if dom == "force" else r32["passa_ms"]
+    dom_kernel = "k_eval_c32 (FP32-mixed force pass)" if dom == "force" else "k_eval_a32 (FP32-mixed pass A)"
+    alg32 = PASS_BYTES[dom][32] * r32["n_local"]
+    ach32 = alg32 / (dom_ms * 1e-3) / 1e9
+    traffic = PASS_TRAFFIC.get(dom)
+    # the FP64 form's dominant pass (its force pass) on the same model
+    ach64 = PASS_BYTES["force"][64] * r64["n_local"] / (r64["force_ms"] * 1e-3) / 1e9
     steps_per_s = args.steps / r32["total_s"]
+    step_ach32 = STEP_BYTES[32] * sps32 / 1e9
+    step_ach64 = STEP_BYTES[64] * sps64 / 1e9
     out = {
         "metric": "particle-steps/s, 4.2M-particle TGV, FP32-mixed vs FP64 at stated KE error",
         "value": sps32,
@@ -353,38 +441,50 @@ def main() -> int:
             "pass_a_ms": {"fp64": r64["passa_ms"], "mixed": r32["passa_ms"]},
             "evaluation_ms": {"fp64": r64["eval_ms"], "mixed": r32["eval_ms"]},
             "fp64_force_pass_roofline": {"achieved": ach64, "unit": "GB/s", "frac": ach64 / HBM_PEAK_GBS,
-                                         "bytes_per_particle": force_pass_bytes_per_particle(64)},
+                                         "bytes_per_particle": PASS_BYTES["force"][64]},
         },
         "roofline": {
-            "kernel": "k_eval_c32 (FP32-mixed force pass)",
+            "kernel": dom_kernel,
             "bound": "hbm",
             "achieved": ach32,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": ach32 / HBM_PEAK_GBS,
-            "traffic": (FORCE_PASS_TRAFFIC["bytes_per_particle"] * r32["n_local"]
-                        if n == FORCE_PASS_TRAFFIC["n_side"] else None),
-            "traffic_source": FORCE_PASS_TRAFFIC["source"],
-            "bytes_per_particle": force_pass_bytes_per_particle(32),
+            "traffic": (traffic["bytes_per_particle"] * r32["n_local"]
+                        if traffic is not None and n == traffic["n_side"] else None),
+            "traffic_source": traffic["source"] if traffic is not None else None,
+            "bytes_per_particle": PASS_BYTES[dom][32],
+            "bytes_model": "frozen algorithmic bytes of this pass per particle",
             "units_per_launch": r32["n_local"],
-            "duration_ms": r32["force_ms"],
+            "duration_ms": dom_ms,
             "launches_timed": r32["n_eval_timed"],
+            # what the pass reads and writes for its own particle (records, flags, the
+            # neighbour record, outputs), a per-launch count beside the frozen model
+            "bytes_moved_per_particle": PASS_READ_WRITE[dom],
         },
-        # the same kernel against the vector-ALU roofline it is bound by: FLOPs per particle
-        # from the counter profile, duration measured live (the HBM object above)
-        "valu_roofline": {
-            "kernel": "k_eval_c32 (FP32-mixed force pass)",
-            "bound": "valu",
-            "achieved": FORCE_PASS_VALU["fp32_flop_per_particle"] * r32["n_local"] / (r32["force_ms"] * 1e-3) / 1e12,
-            "peak": FP32_VECTOR_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": FORCE_PASS_VALU["fp32_flop_per_particle"] * r32["n_local"] / (r32["force_ms"] * 1e-3) / 1e12
-                    / FP32_VECTOR_TFLOPS,
-            "fp64_form": FORCE_PASS_VALU["fp64_flop_per_particle"] * r64["n_local"] / (r64["force_ms"] * 1e-3)
-                         / 1e12 / FP64_VECTOR_TFLOPS,
-            "source": FORCE_PASS_VALU["source"],
+        # the whole step on the same frozen model (one evaluation per predictor-corrector step)
+        "step_roofline": {
+            "bytes_per_particle_step": {"mixed": STEP_BYTES[32], "fp64": STEP_BYTES[64]},
+            "achieved_gbs": {"mixed": step_ach32, "fp64": step_ach64},
+            "frac": {"mixed": step_ach32 / HBM_PEAK_GBS, "fp64": step_ach64 / HBM_PEAK_GBS},
         },
     }
+    if dom in PASS_VALU:
+        flop = PASS_VALU[dom]["fp32_flop_per_particle"] * r32["n_local"] / (dom_ms * 1e-3) / 1e12
+        out["valu_roofline"] = {
+            "kernel": dom_kernel, "bound": "valu", "achieved": flop, "unit": "TFLOP/s",
+            "peak": FP32_VECTOR_TFLOPS, "frac": flop / FP32_VECTOR_TFLOPS,
+            "peak_unpacked": FP32_VECTOR_TFLOPS_UNPACKED, "frac_unpacked": flop / FP32_VECTOR_TFLOPS_UNPACKED,
+            "note": "packed FP32 instructions are disabled in this build, so scalar FP32 issues at the FP64 "
+                    "vector rate (78.6 TFLOP/s); 157.3 is the published (packed) FP32 peak",
+            "fp64_achieved": PASS_VALU[dom]["fp64_flop_per_particle"] * r32["n_local"] / (dom_ms * 1e-3) / 1e12,
+            "source": PASS_VALU[dom]["source"],
+        }
+    if args.ke_gate_side > 0:
+        try:
+            out["ke_gate"] = ke_gate_leg(sph2d, args.ke_gate_side)
+        except Exception as e:                    # the headline line is printed regardless
+            out["ke_gate"] = {"error": repr(e)}
     if args.batch_instances > 0:
         try:
             out["batched"] = batched_leg(sph2d, args.batch_instances, args.batch_steps, args.warmup)
@@ -397,11 +497,16 @@ def main() -> int:
             out["adjoint"] = {"error": repr(e)}
     if rank == 0 and not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(args.cpu_side, args.cpu_steps)
+        head = cpu_baseline_all_cores(args.cpu_all_side, args.cpu_all_steps)
+        samples = {}
         try:
-            out["cpu_baseline"]["all_cores"] = cpu_baseline_all_cores(args.cpu_side, args.cpu_steps)
+            samples[f"{args.cpu_side}_one_core"] = cpu_baseline(args.cpu_side, args.cpu_steps)
+            print("# cpu baseline: small lattice on all cores ...", file=sys.stderr, flush=True)
+            samples[f"{args.cpu_small_side}_all_cores"] = cpu_baseline_all_cores(args.cpu_small_side,
+                                                                               args.cpu_small_steps)
         except Exception as e:                    # the headline line is printed regardless
-            out["cpu_baseline"]["all_cores"] = {"error": repr(e)}
+            samples["error"] = repr(e)
+        out["cpu_baseline"] = dict(head, samples=samples)
     print(json.dumps(out), flush=True)
     return 0
 
