@@ -48,8 +48,11 @@ enum { SPH2D_INTEGRATOR_PC = 2, SPH2D_INTEGRATOR_RK = 3 };
  * box_x - x <= damping_length: acceleration and XSPH velocity scaled by
  * 1 - 0.5 A (1 - cos(pi (x + L - Lx) / L)) (COSINE), by
  * (exp(((Lx - x) / L)^3.5) - 1) / (e - 1) (y / depth) (OPENFOAM) or by
- * 1 - 6 ((x + L - Lx) / L)^2 ((depth - y) / depth) (DUALSPH). */
-enum { SPH2D_DAMPING_OFF = 0, SPH2D_DAMPING_COSINE = 1, SPH2D_DAMPING_OPENFOAM = 2, SPH2D_DAMPING_DUALSPH = 3 };
+ * 1 - 6 ((x + L - Lx) / L)^2 ((depth - y) / depth) (DUALSPH). LINEAR has no reference
+ * counterpart: the Cosin_Type zone with the ramp 1 - A (x + L - Lx) / L, a linear
+ * damping zone over the last damping_length of the tank. */
+enum { SPH2D_DAMPING_OFF = 0, SPH2D_DAMPING_COSINE = 1, SPH2D_DAMPING_OPENFOAM = 2, SPH2D_DAMPING_DUALSPH = 3,
+       SPH2D_DAMPING_LINEAR = 4 };
 
 /* Arithmetic forms. */
 enum { SPH2D_FP64 = 0, SPH2D_FP32_MIXED = 1 };
@@ -381,6 +384,43 @@ int sph2d_wavemaker_pull_back(sph2d_ctx* ctx, double position, double velocity);
  * height next to the paddle, the wavemaker particles placed behind the paddle at
  * its velocity with the hydrostatic density below that height, then the pull-back. */
 int sph2d_wavemaker_update(sph2d_ctx* ctx, double position, double velocity);
+
+/* --- differentiable wave tank ------------------------------------------------------
+ * A discrete adjoint of the predictor-corrector step (FP64 form) of a walled tank with
+ * piston wavemaker particles, as config 4's gradient of a probe-pressure misfit with
+ * respect to paddle coefficients needs it. No counterpart in the reference: it
+ * differentiates the steps of time_iteration.f90:541-657 (PC), Wavemaker_Subroutine.f90
+ * :140-215 (paddle placement and pull-back), Assign_boundary_particles.f90:157-299
+ * (walls), single_step_compute.f90:62-171 with its end damping (:174-260),
+ * Shift_particle_position.f90 and the probe MLS of Sampling_subroutine.f90:21-177. */
+typedef struct {
+  int32_t struct_size;       /* sizeof(sph2d_adjoint_cfg), checked */
+  int32_t n_harmonics;       /* 1..16: stroke x_p(t) = sum_k a_k sin(2 pi k f0 t), u_p = dx_p/dt */
+  double f0;                 /* base frequency, Hz */
+  int32_t standing_steps;    /* >= 1: the paddle rests for i_time_step <= standing_steps;
+                                t = dt (i_time_step - standing_steps) (standing_time_step) */
+  int32_t sample_every;      /* the probes are sampled after every sample_every-th step of the window */
+  int32_t n_probes;          /* 1..8 */
+  double probe_x[8], probe_y[8];
+  int32_t checkpoint_every;  /* K: one snapshot every K steps; the reverse recomputes each segment */
+} sph2d_adjoint_cfg;
+/* Runs steps first..first+n_steps-1 (first >= 2: step 1's extra evaluation stays outside
+ * the window), each after sph2d_wavemaker_update with the stroke of coefficients a, keeping a
+ * snapshot every K steps; probe_p (host, may be NULL) receives the MLS pressure (with the
+ * hydrostatic correction when gravity is on) of each probe after every sample_every-th
+ * step: n_steps / sample_every rows of n_probes. Needs the FP64 form, the predictor-
+ * corrector, walls, wavemaker particles, no periodic box, MLS reinitialisation off,
+ * epsilon_xsph = 0, an _SPS linear Riemann solver and the reference's rate re-use. */
+int sph2d_forward(sph2d_ctx* ctx, const sph2d_adjoint_cfg* cfg, const double* a, int32_t first_step,
+                  int32_t n_steps, double* probe_p);
+/* The reverse of the last sph2d_forward: dl_dprobe (host, the shape of probe_p) are the
+ * weights of the probe pressures; dl_da (host, n_harmonics) receives sum of
+ * dl_dprobe * d(probe_p)/d(a). The context's state is the forward's final state again
+ * afterwards. What the reverse holds fixed: neighbour sets, free-surface and
+ * near-surface flags, limiter, clamp, damping-zone and pull-back branches, the water
+ * height's arg-max particle, which walls and wavemaker particles carry mass, and the SPS
+ * term's particle_mass read by local index. */
+int sph2d_backward(sph2d_ctx* ctx, const double* dl_dprobe, double* dl_da);
 
 /* Host-array writers (no device; Fortran layout, ld >= n). Text follows the
  * reference's edit descriptors exactly; files named "appended" get the

@@ -1281,7 +1281,9 @@ static void solver_riemann_delta_sps(orc_ctx* c) {
 /* by 1 - 0.5 A (1 - cos(PI (x + L - Lx) / L)) (Cosin_Type, :181-201), by          */
 /* (exp(((Lx - x) / L)**3.5) - 1) / (exp(1.0) - 1) (y / depth) (OpenFoam_Type,     */
 /* exp(1.0) a REAL(4) intrinsic, :203-228) or by                                  */
-/* 1 - 6 ((x + L - Lx) / L)**2 ((depth - y) / depth) (DualSPH_Type, :230-255).     */
+/* 1 - 6 ((x + L - Lx) / L)**2 ((depth - y) / depth) (DualSPH_Type, :230-255);     */
+/* damping 4 is the linear ramp 1 - A (x + L - Lx) / L, applied like Cosin_Type     */
+/* (a damping form of this library with no reference counterpart).                */
 static void end_damping_zone(orc_ctx* c) {
   const orc_params* P = &c->P;
   const double Lx = P->box_x, L = P->damping_length, d = P->water_depth;
@@ -1295,6 +1297,8 @@ static void end_damping_zone(orc_ctx* c) {
       f = 1 - coef;
     } else if (P->damping == 2) {
       f = (exp(pow((Lx - x) / L, 3.5)) - 1.0) / ((double)expf(1.0f) - 1.0) * (y / d);
+    } else if (P->damping == 4) {
+      f = 1 - P->damping_amplitude * ((x + L - Lx) / L);
     } else {
       const double q = (x + L - Lx) / L;
       f = 1 - 6 * (q * q) * ((d - y) / d);

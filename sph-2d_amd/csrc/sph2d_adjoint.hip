@@ -107,18 +107,48 @@ __device__ __forceinline__ Dual<N> texp(const Dual<N>& a) {
   for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * r.v;
   return r;
 }
+__device__ __forceinline__ double tpow(double x, double e) { return pow(x, e); }
+template <int N>
+__device__ __forceinline__ Dual<N> tpow(const Dual<N>& a, double e) {
+  Dual<N> r;
+  r.v = pow(a.v, e);
+  const double g = a.v != 0.0 ? e * r.v / a.v : 0.0;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * g;
+  return r;
+}
 
-// One side of a pair: state, grad rho and the SPS stress of the particle.
+// Kinds of particle as the evaluation sees them. FLUID (and periodic ghosts):
+// the state is (x, y, u, v, rho), p and c from the Tait law, the mass fixed.
+// TIED: wavemaker particles below the water height: the same, with mass rho V0
+// (Wavemaker_Subroutine.f90:171-193 sets both from one rho). WALL: fixed ghosts
+// whose (u, v, p) come from their neighbours (Assign_boundary_particles.f90:157-299):
+// rho from the inverse Tait law with the REAL(4) exponent, c = c0 (rho/rho0)^3,
+// mass rho V0. INERT: zero mass (walls with no support, wavemaker particles above
+// the water), which adds nothing to any sum.
+enum { KIND_FLUID = 0, KIND_TIED = 1, KIND_WALL = 2, KIND_INERT = 3 };
+
+__device__ __forceinline__ int particle_kind(const Buffers& B, int s) {
+  const int t = B.l_type[s];
+  if (t == -3) return B.l_mass[s] != 0.0 ? KIND_WALL : KIND_INERT;
+  if (t == 130) return B.l_mass[s] != 0.0 ? KIND_TIED : KIND_INERT;
+  return KIND_FLUID;
+}
+
+// One side of a pair: the primary state (x, y, u, v, q with q = rho, or p for a
+// wall), what follows from it (rho, p, c, mass), grad rho and the SPS stress.
 template <class T>
 struct Side {
-  T x, y, u, v, rho, gx, gy, txx, txy, tyy;
+  T x, y, u, v, q, rho, p, c, m, gx, gy, txx, txy, tyy;
 };
 
 // Adjoint weights of the summing particle's outputs: drho/dt (solver part),
-// the delta-SPH correction (fluid only), acceleration (fluid only), XSPH.
+// the delta-SPH correction (fluid only), acceleration (fluid only), XSPH, the
+// shift sums (Shift_particle_position.f90:78-103) and grad rho.
 struct PairLam {
-  double d, c, ax, ay, gx, gy;
+  double d, c, ax, ay, gx, gy, shx, shy;
 };
+constexpr int kLam = 10;   // per local particle: d, c, ax, ay, gx, gy, shx, shy, grad rho x, y
 
 // status_equation.f90:52-61 (Tait), as in the forward evaluation.
 template <class T>
@@ -129,13 +159,31 @@ __device__ __forceinline__ void tait_t(const DevParams& P, const T& rho, T& p, T
   c = P.sqrt_c0sq * x3;
 }
 
+// rho, p, c and the mass from the primary state, per kind (k_assign_walls and the
+// wavemaker placement compute them this way).
+template <class T>
+__device__ __forceinline__ void derive_side(const DevParams& P, Side<T>& a, int kind, double mass) {
+  if (kind == KIND_WALL) {
+    a.p = a.q;
+    a.rho = P.rho0 * tpow(1.0 + 7.0 * a.p / (P.c0sq * P.rho0), (double)0.142857142f);
+    const T x = a.rho * (1.0 / P.rho0);
+    a.c = P.sqrt_c0sq * (x * x * x);
+    a.m = a.rho * P.vol0;
+  } else {
+    a.rho = a.q;
+    tait_t(P, a.rho, a.p, a.c);
+    a.m = kind == KIND_TIED ? a.rho * P.vol0 : T(mass);
+  }
+}
+
 // Weighted force-pass pair terms summed by particle S with neighbour O: the same
-// formulas as eval_c in FP64 (Solver_Normalized_*_Riemann_Delta_SPS.f90:300-583).
-// gaux receives d/d(grad rho x, y, tau xx, xy, yy) of either side: those enter the
-// pair terms linearly, through the sums S + O, so both sides share one closed form.
+// formulas as eval_c in FP64 (Solver_Normalized_*_Riemann_Delta_SPS.f90:300-583)
+// and the shift sum (Shift_particle_position.f90:78-103). gaux receives d/d(grad rho
+// x, y, tau xx, xy, yy) of either side: those enter the pair terms linearly, through
+// the sums S + O, so both sides share one closed form.
 template <class T, int SOLVER>
-__device__ __forceinline__ T force_pair(const DevParams& P, const Side<T>& S, const Side<T>& O, double mo,
-                                        double gmo, int nes, const PairLam& L, double* gaux) {
+__device__ __forceinline__ T force_pair(const DevParams& P, const Side<T>& S, const Side<T>& O, double gmo, int nes,
+                                        const PairLam& L, int do_shift, double* gaux) {
   const T dx = S.x - O.x, dy = S.y - O.y;
   const T r2 = dx * dx + dy * dy;
   const T r = tsqrt(r2);
@@ -145,9 +193,8 @@ __device__ __forceinline__ T force_pair(const DevParams& P, const Side<T>& S, co
   const T w = (eq - P.e9) * P.kfactor;
   const T kt = eq * (-2.0 * q) * (inv_r * P.inv_h) * P.kfactor;
   const T dwx = kt * dx, dwy = kt * dy;
-  T ps, cs, po, co;
-  tait_t(P, S.rho, ps, cs);
-  tait_t(P, O.rho, po, co);
+  const T ps = S.p, cs = S.c, po = O.p, co = O.c;
+  const T mo = O.m;
   const T Vo = mo / O.rho;
   const T ex = -(dx * inv_r), ey = -(dy * inv_r);
   const T UL = S.u * ex + S.v * ey;
@@ -204,12 +251,18 @@ __device__ __forceinline__ T force_pair(const DevParams& P, const Side<T>& S, co
   ayp = ayp + gmo * tp;
   T J = L.d * drho + L.ax * axp + L.ay * ayp;
   if (SOLVER == SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && P.epsilon_xsph != 0.0) {
-    const T f = P.epsilon_xsph * 2.0 * mo / (S.rho + O.rho) * w;
+    const T f = P.epsilon_xsph * P.xsph_f * mo / (S.rho + O.rho) * w;
     J = J - L.gx * (f * vijx) - L.gy * (f * vijy);
   }
   const T psi = (O.rho - S.rho) + 0.5 * ((S.gx + O.gx) * dx + (S.gy + O.gy) * dy);
   const T cf = 2.0 * P.delta * P.h * cbar * Rd / r2 * Vo;
   J = J - L.c * (cf * psi);
+  if (do_shift && (L.shx != 0.0 || L.shy != 0.0)) {
+    const T wr = w * P.inv_wdx;
+    const T wr2 = wr * wr;
+    const T f = P.shift_beta * P.h2 * (1.0 + P.shift_R * (wr2 * wr2)) * Vo;
+    J = J - L.shx * (f * dwx) - L.shy * (f * dwy);
+  }
   {
     const double rv = val(rij), wx = val(dwx), wy = val(dwy), cv = val(cf);
     gaux[0] = -L.c * cv * 0.5 * val(dx);
@@ -224,45 +277,62 @@ __device__ __forceinline__ T force_pair(const DevParams& P, const Side<T>& S, co
 // Weighted pass-A pair terms of S with neighbour O: strain S_xx, S_xy, S_yy and
 // grad rho (Solver...SPS.f90:144-182,508-526 with the identity renormalisation).
 template <class T>
-__device__ __forceinline__ T sums_pair(const DevParams& P, const Side<T>& S, const Side<T>& O, double mo,
-                                       const double* g) {
+__device__ __forceinline__ T sums_pair(const DevParams& P, const Side<T>& S, const Side<T>& O, const double* g) {
   const T dx = S.x - O.x, dy = S.y - O.y;
   const T r = tsqrt(dx * dx + dy * dy);
   const T q = r * P.inv_h;
   const T eq = texp(-(q * q));
   const T kt = eq * (-2.0 * q) * ((1.0 / r) * P.inv_h) * P.kfactor;
   const T dwx = kt * dx, dwy = kt * dy;
-  const T Vo = mo / O.rho;
+  const T Vo = O.m / O.rho;
   const T vdx = O.u - S.u, vdy = O.v - S.v;
   const T rd = (O.rho - S.rho) * Vo;
   return g[0] * (vdx * dwx * Vo) + g[1] * ((vdx * dwy + vdy * dwx) * Vo) + g[2] * (vdy * dwy * Vo) +
          g[3] * (rd * dwx) + g[4] * (rd * dwy);
 }
 
-template <class T>
-__device__ __forceinline__ Side<T> load_side(const Buffers& B, int s) {
-  Side<T> a;
+// The primary state of local particle s as plain values.
+__device__ __forceinline__ Side<double> load_side(const DevParams& P, const Buffers& B, int s, int& kind) {
+  Side<double> a;
+  kind = particle_kind(B, s);
   const double2 p = B.l_pos[s], v = B.l_vel[s];
   const double2 g = B.grad[s];
-  a.x = p.x; a.y = p.y; a.u = v.x; a.v = v.y; a.rho = B.l_rv[s].x;
+  a.x = p.x; a.y = p.y; a.u = v.x; a.v = v.y;
+  a.q = kind == KIND_WALL ? B.l_pc[s].x : B.l_rv[s].x;
   a.gx = g.x; a.gy = g.y;
   a.txx = B.tau[3 * s]; a.txy = B.tau[3 * s + 1]; a.tyy = B.tau[3 * s + 2];
+  derive_side(P, a, kind, B.l_mass[s]);
   return a;
 }
 
+// The same side with 5 tangent directions seeded on (x, y, u, v, q).
 template <int N>
-__device__ __forceinline__ void seed_side(Side<Dual<N>>& a, int first, int count) {
-  Dual<N>* f[10] = {&a.x, &a.y, &a.u, &a.v, &a.rho, &a.gx, &a.gy, &a.txx, &a.txy, &a.tyy};
-  for (int k = 0; k < count; ++k) f[k]->d[first + k] = 1.0;
+__device__ __forceinline__ Side<Dual<N>> seeded_side(const DevParams& P, const Side<double>& a, int kind,
+                                                    double mass) {
+  Side<Dual<N>> r;
+  r.x = a.x; r.y = a.y; r.u = a.u; r.v = a.v; r.q = a.q;
+  r.x.d[0] = 1.0; r.y.d[1] = 1.0; r.u.d[2] = 1.0; r.v.d[3] = 1.0; r.q.d[4] = 1.0;
+  r.gx = a.gx; r.gy = a.gy; r.txx = a.txx; r.txy = a.txy; r.tyy = a.tyy;
+  derive_side(P, r, kind, mass);
+  return r;
+}
+
+template <int N>
+__device__ __forceinline__ Side<Dual<N>> plain_side(const Side<double>& a) {
+  Side<Dual<N>> r;
+  r.x = a.x; r.y = a.y; r.u = a.u; r.v = a.v; r.q = a.q; r.rho = a.rho; r.p = a.p; r.c = a.c; r.m = a.m;
+  r.gx = a.gx; r.gy = a.gy; r.txx = a.txx; r.txy = a.txy; r.tyy = a.tyy;
+  return r;
 }
 
 struct AdjBufs {
-  const double* lam;   // 5 per fluid gid: drho, ax, ay, avg_x, avg_y
-  double* w;           // 6 per local particle: PairLam
-  double* gs;          // 5 per local particle: d/d(x, y, u, v, rho)
+  const double* lam;   // gid mode: 5 per fluid gid (drho, ax, ay, avg_x, avg_y); slot mode: unused
+  double* w;           // kLam per local particle (PairLam, then grad rho x, y)
+  double* gs;          // 5 per local particle: d/d(x, y, u, v, q)
   double* ga;          // 5 per local particle: d/d(grad rho x, y) then d/d(tau xx, xy, yy); then
                        //   d/d(S xx, xy, yy, grad rho x, y) after k_adj_tau
-  double* G;           // 5 per gid (fluid and ghosts)
+  double* G;           // gid mode: 5 per gid (fluid and ghosts); slot mode: 5 per local particle
+  int do_shift;        // the evaluation's shift sums carry weights
 };
 
 __global__ void __launch_bounds__(kBlock) k_adj_weights(DevParams P, Buffers B, AdjBufs A) {
@@ -274,18 +344,19 @@ __global__ void __launch_bounds__(kBlock) k_adj_weights(DevParams P, Buffers B, 
   if (g >= 0 && g < B.counters[C_NFLUID])
     for (int k = 0; k < 5; ++k) l[k] = A.lam[5 * (size_t)g + k];
   const bool xsph = P.solver == SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && P.epsilon_xsph != 0.0;
-  double* w = A.w + 6 * (size_t)s;
+  double* w = A.w + kLam * (size_t)s;
   w[0] = l[0];
   w[1] = fluid ? l[0] : 0.0;
   w[2] = fluid ? l[1] : 0.0;
   w[3] = fluid ? l[2] : 0.0;
   w[4] = xsph ? l[3] : 0.0;
   w[5] = xsph ? l[4] : 0.0;
+  w[6] = 0.0; w[7] = 0.0; w[8] = 0.0; w[9] = 0.0;
 }
 
 __device__ __forceinline__ PairLam pair_lam(const AdjBufs& A, int s) {
-  const double* w = A.w + 6 * (size_t)s;
-  return PairLam{w[0], w[1], w[2], w[3], w[4], w[5]};
+  const double* w = A.w + kLam * (size_t)s;
+  return PairLam{w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7]};
 }
 
 template <int SOLVER>
@@ -294,31 +365,32 @@ __global__ void __launch_bounds__(kBlock) k_adj_force(DevParams P, Buffers B, Ad
   if (s >= B.counters[C_NLOCAL]) return;
   const int ts = B.l_type[s];
   const double2 ps = B.l_pos[s];
-  const Side<double> vs = load_side<double>(B, s);
+  int ks;
+  const Side<double> vs = load_side(P, B, s, ks);
+  const Side<Dual<5>> S = seeded_side<5>(P, vs, ks, B.l_mass[s]);
   const PairLam Ls = pair_lam(A, s);
   const int nes = B.near_[s];
-  const double ms = B.l_mass[s], gms = B.g_mass[s];
+  const double gms = B.g_mass[s];
   double acc[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) {
     if (ts != 2 && B.l_type[o] != 2) return;
-    const Side<double> vo = load_side<double>(B, o);
-    Side<Dual<5>> S, O;
-    S.x = vs.x; S.y = vs.y; S.u = vs.u; S.v = vs.v; S.rho = vs.rho;
-    S.gx = vs.gx; S.gy = vs.gy; S.txx = vs.txx; S.txy = vs.txy; S.tyy = vs.tyy;
-    O.x = vo.x; O.y = vo.y; O.u = vo.u; O.v = vo.v; O.rho = vo.rho;
-    O.gx = vo.gx; O.gy = vo.gy; O.txx = vo.txx; O.txy = vo.txy; O.tyy = vo.tyy;
-    seed_side(S, 0, 5);
+    int ko;
+    const Side<double> vo = load_side(P, B, o, ko);
+    const Side<Dual<5>> O = plain_side<5>(vo);
     double a1[5], a2[5];
     // terms summed by s (s is the summing particle)
-    const Dual<5> J1 = force_pair<Dual<5>, SOLVER>(P, S, O, B.l_mass[o], B.g_mass[o], nes, Ls, a1);
+    const Dual<5> J1 = force_pair<Dual<5>, SOLVER>(P, S, O, B.g_mass[o], nes, Ls, A.do_shift, a1);
     // terms summed by o (s is its neighbour)
-    const Dual<5> J2 = force_pair<Dual<5>, SOLVER>(P, O, S, ms, gms, B.near_[o], pair_lam(A, o), a2);
+    const Dual<5> J2 = force_pair<Dual<5>, SOLVER>(P, O, S, gms, B.near_[o], pair_lam(A, o), A.do_shift, a2);
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       acc[k] = acc[k] + (J1.d[k] + J2.d[k]);
       acc[5 + k] = acc[5 + k] + (a1[k] + a2[k]);
     }
   }, B.nb_list, B.nb_count, B.kstride);
+  const double* w = A.w + kLam * (size_t)s;
+  acc[5] = acc[5] + w[8];                         // grad rho is an output too (the shift uses it)
+  acc[6] = acc[6] + w[9];
   double* gs = A.gs + 5 * (size_t)s;
   double* ga = A.ga + 5 * (size_t)s;
   for (int k = 0; k < 5; ++k) { gs[k] = acc[k]; ga[k] = acc[5 + k]; }
@@ -367,30 +439,115 @@ __global__ void __launch_bounds__(kBlock) k_adj_tau(DevParams P, Buffers B, AdjB
   ga[0] = gS[0]; ga[1] = gS[1]; ga[2] = gS[2]; ga[3] = ggx; ga[4] = ggy;
 }
 
+// Reverse of the pass-A sums (S and grad rho) into d/d(state_s). SLOT: the result
+// per local particle in G; else per gid.
+template <bool SLOT>
 __global__ void __launch_bounds__(kBlock) k_adj_sums(DevParams P, Buffers B, AdjBufs A) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= B.counters[C_NLOCAL]) return;
   const int ts = B.l_type[s];
   const double2 ps = B.l_pos[s];
-  const Side<double> vs = load_side<double>(B, s);
-  const double ms = B.l_mass[s];
+  int ks;
+  const Side<double> vs = load_side(P, B, s, ks);
+  const Side<Dual<5>> S = seeded_side<5>(P, vs, ks, B.l_mass[s]);
   const double* gas = A.ga + 5 * (size_t)s;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) {
     if (ts != 2 && B.l_type[o] != 2) return;
-    const Side<double> vo = load_side<double>(B, o);
-    Side<Dual<5>> S, O;
-    S.x = vs.x; S.y = vs.y; S.u = vs.u; S.v = vs.v; S.rho = vs.rho;
-    O.x = vo.x; O.y = vo.y; O.u = vo.u; O.v = vo.v; O.rho = vo.rho;
-    seed_side(S, 0, 5);
-    const Dual<5> J1 = sums_pair<Dual<5>>(P, S, O, B.l_mass[o], gas);
-    const Dual<5> J2 = sums_pair<Dual<5>>(P, O, S, ms, A.ga + 5 * (size_t)o);
+    int ko;
+    const Side<double> vo = load_side(P, B, o, ko);
+    const Side<Dual<5>> O = plain_side<5>(vo);
+    const Dual<5> J1 = sums_pair<Dual<5>>(P, S, O, gas);
+    const Dual<5> J2 = sums_pair<Dual<5>>(P, O, S, A.ga + 5 * (size_t)o);
 #pragma unroll
     for (int k = 0; k < 5; ++k) acc[k] = acc[k] + (J1.d[k] + J2.d[k]);
   }, B.nb_list, B.nb_count, B.kstride);
-  const int g = B.name[s];
+  const size_t dst = SLOT ? (size_t)s : (size_t)B.name[s];
   const double* gs = A.gs + 5 * (size_t)s;
-  for (int k = 0; k < 5; ++k) A.G[5 * (size_t)g + k] = gs[k] + acc[k];
+  for (int k = 0; k < 5; ++k) A.G[5 * dst + k] = gs[k] + acc[k];
+}
+
+// Assign_boundary_particles.f90:157-299 reversed, gather form: each non-wall local
+// particle j adds, over the walls w whose Shepard support holds it, the weights of
+// (u_w, v_w, p_w) times the derivatives of the wall's interpolation with respect to
+// (x_j, y_j, u_j, v_j, rho_j): p_w = sum_k F_k (p_k - g rho_k (y_w - y_k)),
+// u_w = s_u sum_k F_k u_k, F_k = W_k / sum W, so dF_k/dx_j = dW_j/dx_j (delta_kj - F_k)
+// / sum W and p_k = Tait(rho_k). Walls with fewer than 3 supports or p_w = 0 keep
+// fixed values. A first pass stores each wall's sum W.
+__device__ __forceinline__ double wall_wdw(const DevParams& P, double r, double& dwdr) {
+  const double q = r / P.wall_h;
+  const double e = exp(-(q * q));
+  dwdr = e * (-2.0 * q) / P.wall_h * P.wall_kfactor;
+  return (e - P.e9) * P.wall_kfactor;
+}
+
+__global__ void __launch_bounds__(kBlock) k_adj_wall_sum(DevParams P, Buffers B, double* wsum) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.counters[C_NLOCAL]) return;
+  if (B.l_type[s] != -3) return;
+  const double2 xs = B.l_pos[s];
+  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
+  const double rmax = 3 * P.wall_h, rmin = (double)1.0E-7f;
+  double a = 0.0;
+  int nt = 0;
+  for (int d = 0; d < 3; ++d)
+    for (int o = R.lo[d]; o < R.hi[d]; ++o) {
+      if (B.l_type[o] == -3) continue;
+      const double2 xo = B.l_pos[o];
+      const double px = xs.x - xo.x, py = xs.y - xo.y;
+      const double r = sqrt(px * px + py * py);
+      if (r <= rmax && r >= rmin) {
+        double dw;
+        a = a + wall_wdw(P, r, dw) * 1.0;
+        ++nt;
+      }
+    }
+  wsum[s] = (nt >= 3 && B.l_pc[s].x != 0.0) ? a : 0.0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_adj_wall_push(DevParams P, Buffers B, AdjBufs A, const double* wsum) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= B.counters[C_NLOCAL]) return;
+  if (B.l_type[j] == -3) return;
+  const double2 xj = B.l_pos[j];
+  const double2 vj = B.l_vel[j];
+  const double rhoj = B.l_rv[j].x;
+  const double pj = B.l_pc[j].x;
+  const double x7 = rhoj / P.rho0, x6 = x7 * x7 * x7 * x7 * x7 * x7;
+  const double dpdrho = P.eos_b * 7.0 * x6 / P.rho0;         // d Tait / d rho
+  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[j]);
+  const double rmax = 3 * P.wall_h, rmin = (double)1.0E-7f;
+  double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int d = 0; d < 3; ++d)
+    for (int w = R.lo[d]; w < R.hi[d]; ++w) {
+      if (B.l_type[w] != -3) continue;
+      const double sw = wsum[w];
+      if (sw == 0.0) continue;
+      const double2 xw = B.l_pos[w];
+      const double px = xw.x - xj.x, py = xw.y - xj.y;
+      const double r = sqrt(px * px + py * py);
+      if (!(r <= rmax && r >= rmin)) continue;
+      double dwdr;
+      const double W = wall_wdw(P, r, dwdr);
+      const double F = 1.0 * ((1.0 / sw) * W);
+      const int bt = B.g_btype[B.name[w]];
+      const double su = (bt == -7 || bt == -8) ? -1.0 : 1.0, sv = (bt == -6 || bt == -7) ? -1.0 : 1.0;
+      const double* gw = A.G + 5 * (size_t)w;
+      const double lu = su * gw[2], lv = sv * gw[3], lp = gw[4];
+      const double2 vw = B.l_vel[w];
+      const double ub = su * vw.x, vb = sv * vw.y, pw = B.l_pc[w].x;
+      const double qj = P.g != 0.0 ? pj - P.g * rhoj * (xw.y - xj.y) : pj;
+      // dW_j / d(x_j, y_j) with r = |x_w - x_j|
+      const double wx = -dwdr * px / r, wy = -dwdr * py / r;
+      const double cw = (lp * (qj - pw) + lu * (vj.x - ub) + lv * (vj.y - vb)) / sw;
+      g[0] += cw * wx;
+      g[1] += cw * wy + (P.g != 0.0 ? lp * F * P.g * rhoj : 0.0);
+      g[2] += lu * F;
+      g[3] += lv * F;
+      g[4] += lp * F * (dpdrho - (P.g != 0.0 ? P.g * (xw.y - xj.y) : 0.0));
+    }
+  double* gj = A.G + 5 * (size_t)j;
+  for (int k = 0; k < 5; ++k) gj[k] = gj[k] + g[k];
 }
 
 __global__ void k_adj_zero(double* p, size_t n) {
@@ -456,21 +613,41 @@ __global__ void __launch_bounds__(kBlock) k_adj_owner(DevParams P, Buffers B, Ad
   for (int q = 0; q < 5; ++q) dst[q] = acc[q];
 }
 
-void launch_eval_vjp(hipStream_t st, const DevParams& P, Buffers& B, int cap_local, int cap_fluid, int cap_ghost,
-                     int cap_global, const double* lam, double* w, double* gs, double* ga, double* G) {
-  const AdjBufs A{lam, w, gs, ga, G};
-  const int g = grid_for(cap_local);
-  const size_t nG = 5 * (size_t)cap_global;
-  hipLaunchKernelGGL(k_adj_zero, dim3((unsigned)((nG + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, G, nG);
-  hipLaunchKernelGGL(k_adj_weights, dim3(g), dim3(kBlock), 0, st, P, B, A);
+static void launch_force_tau(hipStream_t st, const DevParams& P, Buffers& B, int g, const AdjBufs& A) {
   if (P.solver == SPH2D_SOLVER_LINEAR_RIEMANN)
     hipLaunchKernelGGL(k_adj_force<SPH2D_SOLVER_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, st, P, B, A);
   else
     hipLaunchKernelGGL(k_adj_force<SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, st, P, B, A);
   hipLaunchKernelGGL(k_adj_tau, dim3(g), dim3(kBlock), 0, st, P, B, A);
-  hipLaunchKernelGGL(k_adj_sums, dim3(g), dim3(kBlock), 0, st, P, B, A);
+}
+
+void launch_eval_vjp(hipStream_t st, const DevParams& P, Buffers& B, int cap_local, int cap_fluid, int cap_ghost,
+                     int cap_global, const double* lam, double* w, double* gs, double* ga, double* G) {
+  const AdjBufs A{lam, w, gs, ga, G, 0};
+  const int g = grid_for(cap_local);
+  const size_t nG = 5 * (size_t)cap_global;
+  hipLaunchKernelGGL(k_adj_zero, dim3((unsigned)((nG + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, G, nG);
+  hipLaunchKernelGGL(k_adj_weights, dim3(g), dim3(kBlock), 0, st, P, B, A);
+  launch_force_tau(st, P, B, g, A);
+  hipLaunchKernelGGL(k_adj_sums<false>, dim3(g), dim3(kBlock), 0, st, P, B, A);
   hipLaunchKernelGGL(k_adj_corners, dim3(grid_for(cap_ghost)), dim3(kBlock), 0, st, B, A, cap_ghost);
   hipLaunchKernelGGL(k_adj_owner, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, st, P, B, A, cap_fluid, cap_ghost);
+}
+
+// Slot form (the step reverse): weights w (kLam per local particle) already set by
+// the caller; G receives d/d(x, y, u, v, q) per local particle with the walls'
+// shares pushed into the particles they interpolate. wsum: one double per local
+// particle.
+void launch_eval_vjp_slots(hipStream_t st, const DevParams& P, Buffers& B, int cap_local, int do_shift, double* w,
+                           double* gs, double* ga, double* G, double* wsum) {
+  const AdjBufs A{nullptr, w, gs, ga, G, do_shift};
+  const int g = grid_for(cap_local);
+  launch_force_tau(st, P, B, g, A);
+  hipLaunchKernelGGL(k_adj_sums<true>, dim3(g), dim3(kBlock), 0, st, P, B, A);
+  if (P.walls) {
+    hipLaunchKernelGGL(k_adj_wall_sum, dim3(g), dim3(kBlock), 0, st, P, B, wsum);
+    hipLaunchKernelGGL(k_adj_wall_push, dim3(g), dim3(kBlock), 0, st, P, B, A, wsum);
+  }
 }
 
 }  // namespace sph2d

@@ -42,6 +42,9 @@ constexpr double kPiModule = 3.14159265358979324;  // information_module.f90:29
 
 }  // namespace
 
+struct AdjTape;
+void tape_free(AdjTape* t);
+
 struct sph2d_ctx {
   sph2d_params prm{};
   DevParams P{};
@@ -76,6 +79,8 @@ struct sph2d_ctx {
   size_t probe_cap = 0;
   // reverse evaluation: weights, per-local adjoints and the by-gid result (first use)
   double* adj_buf = nullptr;
+  // sph2d_forward's checkpoints and sph2d_backward's work space (first use)
+  AdjTape* tape = nullptr;
   bool pass_timing = false;
   std::vector<hipEvent_t> events;
   size_t events_used = 0;
@@ -211,7 +216,7 @@ int check_params(const sph2d_params* p) {
     return fail(SPH2D_ERR_UNSUPPORTED, "the box is periodic or has fixed ghost walls (walls > 0)");
   if (p->mls_every <= 0 || p->sample_every <= 0)
     return fail(SPH2D_ERR_ARGUMENT, "mls_every and sample_every must be positive");
-  if (p->damping < SPH2D_DAMPING_OFF || p->damping > SPH2D_DAMPING_DUALSPH)
+  if (p->damping < SPH2D_DAMPING_OFF || p->damping > SPH2D_DAMPING_LINEAR)
     return fail(SPH2D_ERR_ARGUMENT, "damping must be one of SPH2D_DAMPING_*");
   if (p->sps_on != 0 && p->sps_on != 1) return fail(SPH2D_ERR_ARGUMENT, "sps_on must be 0 or 1");
   if (p->damping != SPH2D_DAMPING_OFF && (!(p->damping_length > 0) || !(p->water_depth > 0)))
@@ -396,6 +401,7 @@ int sph2d_ctx_destroy(sph2d_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
+  tape_free(c->tape);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return SPH2D_OK;
@@ -1044,10 +1050,10 @@ int sph2d_single_step_compute_vjp(sph2d_ctx* c, int32_t n, const double* lam_drh
   }
   const size_t nf = (size_t)c->cap_fluid, nl = (size_t)c->cap_local, ng = (size_t)c->cap_global;
   if (!c->adj_buf) {                                // kept until the context is destroyed
-    rc = c->alloc(&c->adj_buf, 5 * nf + 16 * nl + 5 * ng);
+    rc = c->alloc(&c->adj_buf, 5 * nf + 20 * nl + 5 * ng);
     if (rc) return rc;
   }
-  double *d_lam = c->adj_buf, *w = d_lam + 5 * nf, *gs = w + 6 * nl, *ga = gs + 5 * nl, *G = ga + 5 * nl;
+  double *d_lam = c->adj_buf, *w = d_lam + 5 * nf, *gs = w + 10 * nl, *ga = gs + 5 * nl, *G = ga + 5 * nl;
   {
     // a clamped ghost emission leaves slots unwritten that the fold would read
     int cnt[C_COUNT];
@@ -2140,5 +2146,269 @@ int sph2d_batch_destroy(sph2d_batch* b) {
   if (b->stream) (void)hipStreamDestroy(b->stream);
   for (void* p : b->dev_allocs) (void)hipFree(p);
   delete b;
+  return SPH2D_OK;
+}
+
+// ---- the differentiable wave tank: forward with checkpoints, reverse sweep ----------
+// A snapshot is everything a predictor-corrector step reads from the steps before it:
+// the global state of every particle and the previous rates by local slot (read by the
+// predictor's local index), with the device counters.
+struct AdjTape {
+  sph2d_adjoint_cfg cfg{};
+  double a[16] = {};
+  int first = 0, nsteps = 0, nsamples = 0, nckpt = 0;
+  size_t snap_bytes = 0;
+  size_t off[10] = {};
+  char* ckpt = nullptr;            // nckpt snapshots
+  char* seg = nullptr;             // K snapshots of the segment being reversed
+  char* fin = nullptr;             // the state after the forward
+  size_t ckpt_cap = 0, seg_cap = 0, fin_cap = 0;
+  double* probe_dev = nullptr;     // probe positions (2 per probe), samples, their weights
+  size_t probe_cap = 0;
+  double* ws = nullptr;            // reverse work space
+  size_t ws_cap = 0;
+  double* pad = nullptr;           // (dL/dx_p, dL/du_p) per step
+  size_t pad_cap = 0;
+  int* iws = nullptr;              // flags per gid, arg-max, height bits
+  size_t iws_cap = 0;
+  bool ready = false;
+
+  void release() {
+    for (void* p : {(void*)ckpt, (void*)seg, (void*)fin, (void*)probe_dev, (void*)ws, (void*)pad, (void*)iws})
+      if (p) (void)hipFree(p);
+    ckpt = seg = fin = nullptr; probe_dev = ws = pad = nullptr; iws = nullptr;
+    ckpt_cap = seg_cap = fin_cap = probe_cap = ws_cap = pad_cap = iws_cap = 0;
+  }
+};
+
+void tape_free(AdjTape* t) {
+  if (!t) return;
+  t->release();
+  delete t;
+}
+
+namespace {
+
+template <class T>
+int grow(T** p, size_t* cap, size_t n) {
+  if (*cap >= n && *p) return SPH2D_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  void* q = nullptr;
+  const hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess)
+    return fail(SPH2D_ERR_CAPACITY, std::string("hipMalloc of the adjoint tape: ") + hipGetErrorString(e));
+  *p = static_cast<T*>(q);
+  *cap = n;
+  return SPH2D_OK;
+}
+
+void snap_layout(sph2d_ctx* c, AdjTape* t) {
+  const size_t G = (size_t)c->cap_global, L = (size_t)c->cap_local;
+  const size_t sz[10] = {16 * G, 16 * G, 8 * G, 8 * G, 8 * G, 8 * G, 4 * G, 8 * L, 16 * L, 4 * (size_t)C_COUNT};
+  size_t o = 0;
+  for (int k = 0; k < 10; ++k) {
+    t->off[k] = o;
+    o += (sz[k] + 255) / 256 * 256;
+  }
+  t->snap_bytes = o;
+}
+
+int snap_copy(sph2d_ctx* c, AdjTape* t, char* snap, bool save) {
+  const size_t G = (size_t)c->cap_global, L = (size_t)c->cap_local;
+  void* dev[10] = {c->B.g_pos, c->B.g_vel, c->B.g_rho, c->B.g_p, c->B.g_c, c->B.g_mass, c->B.g_type,
+                   c->B.drhodt, c->B.acc, c->B.counters};
+  const size_t sz[10] = {16 * G, 16 * G, 8 * G, 8 * G, 8 * G, 8 * G, 4 * G, 8 * L, 16 * L, 4 * (size_t)C_COUNT};
+  for (int k = 0; k < 10; ++k) {
+    char* s = snap + t->off[k];
+    HIP_OR_FAIL(hipMemcpyAsync(save ? (void*)s : dev[k], save ? dev[k] : (void*)s, sz[k], hipMemcpyDeviceToDevice,
+                               c->stream));
+  }
+  return SPH2D_OK;
+}
+
+void paddle(const AdjTape* t, double dt, int it, double& xp, double& up) {
+  xp = 0.0; up = 0.0;
+  if (it <= t->cfg.standing_steps) return;
+  const double tt = dt * (it - t->cfg.standing_steps);
+  for (int k = 1; k <= t->cfg.n_harmonics; ++k) {
+    const double om = 2.0 * kPiModule * k * t->cfg.f0;
+    xp += t->a[k - 1] * std::sin(om * tt);
+    up += t->a[k - 1] * om * std::cos(om * tt);
+  }
+}
+
+int adj_step(sph2d_ctx* c, AdjTape* t, int it) {
+  double xp, up;
+  paddle(t, c->prm.dt, it, xp, up);
+  int rc = sph2d_wavemaker_update(c, xp, up);
+  if (rc) return rc;
+  return enqueue_step(c, it);
+}
+
+}  // namespace
+
+int sph2d_forward(sph2d_ctx* c, const sph2d_adjoint_cfg* cfg, const double* a, int32_t first, int32_t nsteps,
+                  double* probe_p) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (!cfg || !a) return fail(SPH2D_ERR_ARGUMENT, "cfg and a are required");
+  if (cfg->struct_size != (int32_t)sizeof(sph2d_adjoint_cfg))
+    return fail(SPH2D_ERR_ARGUMENT, "sph2d_adjoint_cfg.struct_size mismatch");
+  if (cfg->n_harmonics < 1 || cfg->n_harmonics > 16) return fail(SPH2D_ERR_ARGUMENT, "n_harmonics must be in [1, 16]");
+  if (cfg->n_probes < 1 || cfg->n_probes > 8) return fail(SPH2D_ERR_ARGUMENT, "n_probes must be in [1, 8]");
+  if (cfg->sample_every < 1 || cfg->checkpoint_every < 1 || cfg->standing_steps < 1)
+    return fail(SPH2D_ERR_ARGUMENT, "sample_every, checkpoint_every and standing_steps must be >= 1");
+  if (first < 2 || nsteps < 1) return fail(SPH2D_ERR_ARGUMENT, "the window starts at step 2 or later, nsteps >= 1");
+  if (c->P.mixed) return fail(SPH2D_ERR_UNSUPPORTED, "the adjoint runs in the FP64 form");
+  if (c->prm.integrator != SPH2D_INTEGRATOR_PC) return fail(SPH2D_ERR_UNSUPPORTED, "the adjoint needs the predictor-corrector");
+  if (!c->prm.walls || c->prm.periodic || !c->prm.wavemaker_layers)
+    return fail(SPH2D_ERR_UNSUPPORTED, "the adjoint needs a walled tank with wavemaker particles");
+  if (c->prm.mls_on) return fail(SPH2D_ERR_UNSUPPORTED, "the adjoint needs the MLS density reinitialisation off");
+  if (c->prm.epsilon_xsph != 0.0) return fail(SPH2D_ERR_UNSUPPORTED, "the adjoint needs epsilon_xsph = 0");
+  if (!c->P.sps) return fail(SPH2D_ERR_UNSUPPORTED, "the adjoint is built for the _SPS solvers");
+  if (c->P.solver != SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && c->P.solver != SPH2D_SOLVER_LINEAR_RIEMANN)
+    return fail(SPH2D_ERR_UNSUPPORTED, "the adjoint is built for the two linear Riemann solvers");
+  if (c->prm.rate_reuse != SPH2D_RATE_REUSE_REFERENCE)
+    return fail(SPH2D_ERR_UNSUPPORTED, "the adjoint follows the reference's rate re-use by local slot");
+  const double lo_x = c->P.chain_ox, lo_y = c->P.chain_oy;
+  const double hi_x = lo_x + c->P.chain_nx * c->P.chain_d, hi_y = lo_y + c->P.chain_ny * c->P.chain_d;
+  for (int k = 0; k < cfg->n_probes; ++k)
+    if (!(cfg->probe_x[k] >= lo_x && cfg->probe_x[k] < hi_x && cfg->probe_y[k] >= lo_y && cfg->probe_y[k] < hi_y))
+      return fail(SPH2D_ERR_ARGUMENT, "probe " + std::to_string(k + 1) + " lies outside the background grid");
+  if (!c->tape) c->tape = new AdjTape();
+  AdjTape* t = c->tape;
+  t->ready = false;
+  t->cfg = *cfg;
+  for (int k = 0; k < 16; ++k) t->a[k] = k < cfg->n_harmonics ? a[k] : 0.0;
+  t->first = first;
+  t->nsteps = nsteps;
+  t->nsamples = nsteps / cfg->sample_every;
+  const int K = cfg->checkpoint_every;
+  t->nckpt = (nsteps + K - 1) / K;
+  snap_layout(c, t);
+  const size_t np = (size_t)cfg->n_probes;
+  if ((rc = grow(&t->ckpt, &t->ckpt_cap, t->snap_bytes * t->nckpt))) return rc;
+  if ((rc = grow(&t->fin, &t->fin_cap, t->snap_bytes))) return rc;
+  if ((rc = grow(&t->probe_dev, &t->probe_cap, 2 * np + 2 * np * t->nsamples + 8))) return rc;
+  std::vector<double> pos(2 * np);
+  for (size_t k = 0; k < np; ++k) { pos[2 * k] = cfg->probe_x[k]; pos[2 * k + 1] = cfg->probe_y[k]; }
+  HIP_OR_FAIL(hipMemcpyAsync(t->probe_dev, pos.data(), pos.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  double* samples = t->probe_dev + 2 * np;
+  ProbeArgs Q;
+  Q.n = cfg->n_probes;
+  Q.gravity = c->P.g;
+  Q.pos = reinterpret_cast<const double2*>(t->probe_dev);
+  for (int k = 0; k < nsteps; ++k) {
+    const int it = first + k;
+    if (k % K == 0 && (rc = snap_copy(c, t, t->ckpt + (size_t)(k / K) * t->snap_bytes, true))) return rc;
+    if ((rc = adj_step(c, t, it))) return rc;
+    if ((k + 1) % cfg->sample_every == 0) {
+      // k_probe_mls writes 4 blocks of n; the MLS pressure block lands on this sample
+      rc = probe_space(c, 4 * np);
+      if (rc) return rc;
+      Q.out = c->probe_buf;
+      launch_probe_mls(c->stream, c->P, c->B, Q);
+      const int smp = (k + 1) / cfg->sample_every - 1;
+      HIP_OR_FAIL(hipMemcpyAsync(samples + (size_t)smp * np, c->probe_buf + np, np * sizeof(double),
+                                 hipMemcpyDeviceToDevice, c->stream));
+    }
+  }
+  if ((rc = snap_copy(c, t, t->fin, true))) return rc;
+  HIP_OR_FAIL(hipGetLastError());
+  if (probe_p && t->nsamples > 0)
+    HIP_OR_FAIL(hipMemcpyAsync(probe_p, samples, np * t->nsamples * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  int cnt[C_COUNT];
+  if ((rc = read_counters(c, cnt))) return rc;
+  t->ready = true;
+  return SPH2D_OK;
+}
+
+int sph2d_backward(sph2d_ctx* c, const double* dl_dprobe, double* dl_da) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  AdjTape* t = c->tape;
+  if (!t || !t->ready) return fail(SPH2D_ERR_STATE, "sph2d_backward needs a completed sph2d_forward");
+  if (!dl_dprobe || !dl_da) return fail(SPH2D_ERR_ARGUMENT, "dl_dprobe and dl_da are required");
+  const int K = t->cfg.checkpoint_every;
+  const size_t np = (size_t)t->cfg.n_probes;
+  const size_t G = (size_t)c->cap_global, L = (size_t)c->cap_local;
+  if ((rc = grow(&t->seg, &t->seg_cap, t->snap_bytes * (size_t)K))) return rc;
+  const size_t nws = 6 * G + 3 * L + 3 * L + 11 * L + 10 * L + 5 * L + 5 * L + 5 * L + L + 3 * G;
+  if ((rc = grow(&t->ws, &t->ws_cap, nws))) return rc;
+  if ((rc = grow(&t->pad, &t->pad_cap, 2 * (size_t)t->nsteps))) return rc;
+  if ((rc = grow(&t->iws, &t->iws_cap, G + 8))) return rc;
+  double* lamG = t->ws;
+  double* lamR = lamG + 6 * G;
+  double* lamRp = lamR + 3 * L;
+  double* scr = lamRp + 3 * L;
+  double* W = scr + 11 * L;
+  double* gs = W + 10 * L;
+  double* ga = gs + 5 * L;
+  double* Gs = ga + 5 * L;
+  double* wsum = Gs + 5 * L;
+  double* c3 = wsum + L;
+  int* flags = t->iws;
+  int* argmax = t->iws + G;
+  auto* hbits = reinterpret_cast<unsigned long long*>(t->iws + G + 2);
+  double* lam_probe = t->probe_dev + 2 * np + np * t->nsamples;
+  if (t->nsamples > 0)
+    HIP_OR_FAIL(hipMemcpyAsync(lam_probe, dl_dprobe, np * t->nsamples * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+  launch_adj_zero(c->stream, lamG, 6 * G);
+  launch_adj_zero(c->stream, lamR, 3 * L);
+  std::vector<double> pos(2 * np);
+  HIP_OR_FAIL(hipMemcpyAsync(pos.data(), t->probe_dev, pos.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  for (int sg = t->nckpt - 1; sg >= 0; --sg) {
+    const int c0 = sg * K, len = std::min(K, t->nsteps - c0);
+    if ((rc = snap_copy(c, t, t->ckpt + (size_t)sg * t->snap_bytes, false))) return rc;
+    for (int j = 0; j < len; ++j) {              // the segment forward again, one snapshot per step
+      if ((rc = snap_copy(c, t, t->seg + (size_t)j * t->snap_bytes, true))) return rc;
+      if (j + 1 < len && (rc = adj_step(c, t, t->first + c0 + j))) return rc;
+    }
+    for (int j = len - 1; j >= 0; --j) {
+      const int k = c0 + j, it = t->first + k;
+      if ((rc = snap_copy(c, t, t->seg + (size_t)j * t->snap_bytes, false))) return rc;
+      double xp, up;
+      paddle(t, c->prm.dt, it, xp, up);
+      // the wavemaker's branches from the state before its update, then the step again
+      launch_adj_wm_init(c->stream, hbits, argmax);
+      launch_wavemaker_height(c->stream, c->P, c->B, c->cap_fluid, xp, hbits);
+      launch_adj_wm_flags(c->stream, c->P, c->B, c->cap_fluid, xp, flags, hbits, argmax);
+      if ((rc = adj_step(c, t, it))) return rc;
+      if ((k + 1) % t->cfg.sample_every == 0) {
+        const int smp = (k + 1) / t->cfg.sample_every - 1;
+        for (size_t q = 0; q < np; ++q)
+          launch_adj_probe(c->stream, c->P, c->B, pos[2 * q], pos[2 * q + 1], c->P.g, lam_probe + smp * np + q, lamG);
+      }
+      launch_adj_stage1(c->stream, c->P, c->B, c->cap_local, lamG, lamR, W, scr, c->prm.shift_on);
+      launch_eval_vjp_slots(c->stream, c->P, c->B, c->cap_local, c->prm.shift_on, W, gs, ga, Gs, wsum);
+      launch_adj_stage2(c->stream, c->P, c->B, c->cap_local, Gs, scr, lamR, lamG, lamRp);
+      launch_adj_wm_reverse(c->stream, c->P, c->B, c->n_fluid + c->n_wm, c->n_wm, Gs, flags, hbits, lamG, c3, argmax,
+                            t->pad + 2 * (size_t)k);
+      std::swap(lamR, lamRp);
+      HIP_OR_FAIL(hipGetLastError());
+    }
+  }
+  std::vector<double> pad(2 * (size_t)t->nsteps);
+  HIP_OR_FAIL(hipMemcpyAsync(pad.data(), t->pad, pad.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if ((rc = snap_copy(c, t, t->fin, false))) return rc;
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  for (int q = 0; q < t->cfg.n_harmonics; ++q) {
+    const double om = 2.0 * kPiModule * (q + 1) * t->cfg.f0;
+    double g = 0.0;
+    for (int k = 0; k < t->nsteps; ++k) {
+      const int it = t->first + k;
+      if (it <= t->cfg.standing_steps) continue;
+      const double tt = c->prm.dt * (it - t->cfg.standing_steps);
+      g += pad[2 * k] * std::sin(om * tt) + pad[2 * k + 1] * om * std::cos(om * tt);
+    }
+    dl_da[q] = g;
+  }
+  c->vjp_ready = false;
+  c->eval_part = 0;
   return SPH2D_OK;
 }

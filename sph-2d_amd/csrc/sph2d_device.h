@@ -487,4 +487,30 @@ __device__ __forceinline__ void for_each_neighbour(const DevParams& P, const int
   }
 }
 
+// The support of a point for the global-domain MLS (Calculation_For_PostProceeding.f90
+// :184-250, Sampling_subroutine.f90:60-95): the particles with gid < n_max (fluid,
+// no periodic ghosts) of the point's 3x3 cells in the step-start binning whose
+// refreshed position lies within [1e-7 (REAL(4)), 3h]; f(gid, position, W).
+template <class F>
+__device__ __forceinline__ void for_each_support(const DevParams& P, const Buffers& B, double ex, double ey,
+                                                 int n_max, F&& f) {
+  const int xn = (int)floor((ex - P.chain_ox) / P.chain_d + 1);
+  const int yn = (int)floor((ey - P.chain_oy) / P.chain_d + 1);
+  if (xn < 1 || xn > P.chain_nx || yn < 1 || yn > P.chain_ny) return;
+  const CellRanges R = cell_ranges(P, B.cell_start, (xn - 1) * P.chain_ny + (yn - 1));
+  const double rmin = (double)1.0e-7f, rmax = P.three_h;
+  for (int d = 0; d < 3; ++d)
+    for (int o = R.lo[d]; o < R.hi[d]; ++o) {
+      const int g = B.name[o];
+      if (g >= n_max) continue;
+      const double2 pj = B.g_pos[g];
+      const double pdx = ex - pj.x, pdy = ey - pj.y;
+      const double r = sqrt(pdx * pdx + pdy * pdy);
+      if (!(r <= rmax && r >= rmin)) continue;
+      double w, dwx, dwy;
+      kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
+      f(g, pj, w);
+    }
+}
+
 }  // namespace sph2d

@@ -814,32 +814,6 @@ void launch_mls(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
   hipLaunchKernelGGL(k_mls, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B);
 }
 
-// The support of a point for the global-domain MLS (Calculation_For_PostProceeding.f90
-// :184-250, Sampling_subroutine.f90:60-95): the particles with gid < n_max (fluid,
-// no periodic ghosts) of the point's 3x3 cells in the step-start binning whose
-// refreshed position lies within [1e-7 (REAL(4)), 3h]; f(gid, position, W).
-template <class F>
-__device__ __forceinline__ void for_each_support(const DevParams& P, const Buffers& B, double ex, double ey,
-                                                 int n_max, F&& f) {
-  const int xn = (int)floor((ex - P.chain_ox) / P.chain_d + 1);
-  const int yn = (int)floor((ey - P.chain_oy) / P.chain_d + 1);
-  if (xn < 1 || xn > P.chain_nx || yn < 1 || yn > P.chain_ny) return;
-  const CellRanges R = cell_ranges(P, B.cell_start, (xn - 1) * P.chain_ny + (yn - 1));
-  const double rmin = (double)1.0e-7f, rmax = P.three_h;
-  for (int d = 0; d < 3; ++d)
-    for (int o = R.lo[d]; o < R.hi[d]; ++o) {
-      const int g = B.name[o];
-      if (g >= n_max) continue;
-      const double2 pj = B.g_pos[g];
-      const double pdx = ex - pj.x, pdy = ey - pj.y;
-      const double r = sqrt(pdx * pdx + pdy * pdy);
-      if (!(r <= rmax && r >= rmin)) continue;
-      double w, dwx, dwy;
-      kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
-      f(g, pj, w);
-    }
-}
-
 // MLS_Interpolation_In_Gobal_Domain.f90, order 6, at (ex, ey): y = row 1 of the
 // moment inverse (mls = true) or Shepard weights (10 or fewer particles in the
 // support, or no Cholesky factor); sw = sum of W. Returns the support count.
@@ -1044,6 +1018,8 @@ __global__ void __launch_bounds__(kBlock) k_end_damping(DevParams P, Buffers B) 
     f = 1 - 0.5 * P.damp_amp * (1 - cos(pi * (x.x + L - Lx) / L));
   } else if (P.damping == 2) {                            // OpenFoam_Type (:203-228)
     f = (exp(pow((Lx - x.x) / L, 3.5)) - 1.0) / (P.e32 - 1.0) * (x.y / d);
+  } else if (P.damping == 4) {                            // linear ramp (no reference form)
+    f = 1 - P.damp_amp * ((x.x + L - Lx) / L);
   } else {                                                // DualSPH_Type (:230-255)
     const double q = (x.x + L - Lx) / L;
     f = 1 - 6 * (q * q) * ((d - x.y) / d);
@@ -1118,6 +1094,11 @@ __global__ void k_wavemaker_place(DevParams P, Buffers B, int n_wm, double xp, d
     B.g_p[i] = P.c0sq * P.rho0 * (pow(rho / P.rho0, 7.0) - 1.0) / 7.0;
     B.g_mass[i] = rho * P.vol0;
   }
+}
+
+void launch_wavemaker_height(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, double xp,
+                             unsigned long long* height_bits) {
+  hipLaunchKernelGGL(k_wavemaker_height, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, s, P, B, xp, height_bits);
 }
 
 void launch_wavemaker_place(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int n_wm, double xp,

@@ -126,6 +126,25 @@ void launch_init_tgv(hipStream_t s, const DevParams& P, Buffers& B, int nx, int 
 // w: 6 and gs, ga: 5 per local particle, G: 5 per gid (result in the first n_fluid).
 void launch_eval_vjp(hipStream_t st, const DevParams& P, Buffers& B, int cap_local, int cap_fluid, int cap_ghost,
                      int cap_global, const double* lam, double* w, double* gs, double* ga, double* G);
+// the slot form of the reverse evaluation (weights w: 10 per local particle) and the
+// reverse of whole steps (sph2d_adjstep.hip)
+void launch_eval_vjp_slots(hipStream_t st, const DevParams& P, Buffers& B, int cap_local, int do_shift, double* w,
+                           double* gs, double* ga, double* G, double* wsum);
+void launch_adj_zero(hipStream_t s, double* p, size_t n);
+void launch_adj_stage1(hipStream_t st, const DevParams& P, Buffers& B, int cap_local, const double* lamG,
+                       const double* lamR, double* W, double* scr, int shift_on);
+void launch_adj_stage2(hipStream_t st, const DevParams& P, Buffers& B, int cap_local, const double* G,
+                       const double* scr, const double* lamR, double* lamG, double* lamRp);
+void launch_adj_wm_init(hipStream_t st, unsigned long long* hbits, int* argmax);
+void launch_adj_wm_flags(hipStream_t st, const DevParams& P, Buffers& B, int cap_fluid, double xp, int* flags,
+                         const unsigned long long* hbits, int* argmax);
+void launch_adj_wm_reverse(hipStream_t st, const DevParams& P, Buffers& B, int n_total, int n_wm, const double* G,
+                           const int* flags, const unsigned long long* hbits, double* lamG, double* c3,
+                           const int* argmax, double* out2);
+void launch_adj_probe(hipStream_t st, const DevParams& P, Buffers& B, double ex, double ey, double gravity,
+                      const double* lam, double* lamG);
+void launch_wavemaker_height(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, double xp,
+                             unsigned long long* height_bits);
 void launch_period_boundary(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int cap_ghost);
 // predict 1: the local copy also applies the predictor-corrector's predictor (k_gather_predict);
 // 2: and keeps the step-start state in before_* for k_stage's corrector; 3: as 1 without

@@ -32,6 +32,7 @@ DAMPING_OFF = 0
 DAMPING_COSINE = 1
 DAMPING_OPENFOAM = 2
 DAMPING_DUALSPH = 3
+DAMPING_LINEAR = 4
 
 ERRORS = {1: "argument", 2: "capacity", 3: "hip", 4: "state", 5: "unsupported", 6: "nonfinite"}
 ERR_ARGUMENT, ERR_CAPACITY, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED, ERR_NONFINITE = 1, 2, 3, 4, 5, 6
@@ -91,6 +92,28 @@ def wavemaker_motion(w: Wavemaker, i_time_step: int):
     return x.value, u.value, a.value
 
 
+class AdjointCfg(ctypes.Structure):
+    """sph2d_adjoint_cfg: the differentiable wave tank's paddle law, probes and checkpoints."""
+    _fields_ = [("struct_size", ctypes.c_int32), ("n_harmonics", ctypes.c_int32), ("f0", ctypes.c_double),
+                ("standing_steps", ctypes.c_int32), ("sample_every", ctypes.c_int32), ("n_probes", ctypes.c_int32),
+                ("probe_x", ctypes.c_double * 8), ("probe_y", ctypes.c_double * 8),
+                ("checkpoint_every", ctypes.c_int32)]
+
+
+def adjoint_cfg(n_harmonics=8, f0=0.5, standing_steps=1, sample_every=10, probes=((2.0, 0.3),),
+                checkpoint_every=20) -> AdjointCfg:
+    c = AdjointCfg()
+    c.struct_size = ctypes.sizeof(AdjointCfg)
+    c.n_harmonics, c.f0, c.standing_steps = n_harmonics, f0, standing_steps
+    c.sample_every, c.checkpoint_every = sample_every, checkpoint_every
+    if not 1 <= len(probes) <= 8:
+        raise ValueError("1 to 8 probes")
+    c.n_probes = len(probes)
+    for k, (x, y) in enumerate(probes):
+        c.probe_x[k], c.probe_y[k] = x, y
+    return c
+
+
 class Scales(ctypes.Structure):
     """sph2d_scales: the output routines' normalisation (information_module.f90:56,150-152)."""
     _fields_ = [("domain_size_y", ctypes.c_double), ("velocity_0", ctypes.c_double),
@@ -130,7 +153,7 @@ EXPORTS = (
     "sph2d_format_list_directed", "sph2d_write_list_directed", "sph2d_write_f20_8", "sph2d_kalman_filter",
     "sph2d_sampling_line_points", "sph2d_write_sampling_line_positions", "sph2d_write_sampling_line_results",
     "sph2d_wavemaker_motion", "sph2d_wavemaker_pull_back", "sph2d_wavemaker_update", "sph2d_batch_create", "sph2d_batch_run",
-    "sph2d_batch_info", "sph2d_batch_destroy", "sph2d_batch_autotune",
+    "sph2d_batch_info", "sph2d_batch_destroy", "sph2d_batch_autotune", "sph2d_forward", "sph2d_backward",
 )
 
 _lib = None
@@ -226,6 +249,8 @@ def lib():
             "sph2d_solver": [vp, i32, i32],
             "sph2d_stage_update": [vp, i32, i32],
             "sph2d_download_rates": [vp, i32, dp, dp, i32],
+            "sph2d_forward": [vp, P(AdjointCfg), dp, i32, i32, dp],
+            "sph2d_backward": [vp, dp, dp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -366,6 +391,31 @@ class Simulation:
         _check(lib().sph2d_single_step_compute_vjp(self._c, n, _dp(ld), _dp(la), _dp(lg) if lg is not None else None,
                                                    n, _dp(gp), _dp(gv), _dp(gr)))
         return {"x": gp[:n], "y": gp[n:], "u": gv[:n], "v": gv[n:], "rho": gr}
+
+    # --- the differentiable wave tank ------------------------------------------------
+    def forward(self, cfg: AdjointCfg, a, first_step: int, n_steps: int):
+        """sph2d_forward: steps first_step..first_step+n_steps-1 driven by the stroke
+        sum_k a_k sin(2 pi k f0 t); returns the probe pressures, (n_steps //
+        sample_every, n_probes)."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.ndim != 1 or a.shape[0] != cfg.n_harmonics:
+            raise ValueError(f"a must hold n_harmonics = {cfg.n_harmonics} coefficients")
+        ns = n_steps // cfg.sample_every
+        out = np.zeros((max(ns, 1), cfg.n_probes))
+        _check(lib().sph2d_forward(self._c, ctypes.byref(cfg), _dp(a), first_step, n_steps, _dp(out)))
+        self._adj_shape = (ns, cfg.n_probes, cfg.n_harmonics)
+        return out[:ns]
+
+    def backward(self, dl_dprobe):
+        """sph2d_backward: the gradient with respect to the paddle coefficients of
+        sum(dl_dprobe * probe pressures) of the last forward."""
+        ns, npb, nh = getattr(self, "_adj_shape", (None, None, 16))
+        w = np.ascontiguousarray(dl_dprobe, dtype=np.float64)
+        if ns is not None and w.shape != (ns, npb):
+            raise ValueError(f"dl_dprobe must have shape {(ns, npb)}, got {w.shape}")
+        g = np.zeros(nh)
+        _check(lib().sph2d_backward(self._c, _dp(w.reshape(-1)) if w.size else _dp(np.zeros(1)), _dp(g)))
+        return g
 
     # --- single_step_compute routine by routine ---------------------------------------
     def find_particle_pair(self, i_time_step: int = 1):
@@ -887,3 +937,27 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def paddle_probe_function():
+    """torch.autograd.Function over the differentiable wave tank: apply(a, make_sim,
+    cfg, first_step, n_steps) takes a context from make_sim() (at the start of the
+    window), runs sph2d_forward with the coefficients a and returns the probe
+    pressures as a float64 tensor; backward runs sph2d_backward on that context with
+    the incoming gradient."""
+    import torch
+
+    class PaddleProbes(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, a, make_sim, cfg, first_step, n_steps):
+            sim = make_sim()
+            p = sim.forward(cfg, a.detach().cpu().double().numpy(), first_step, n_steps)
+            ctx.sim = sim
+            return torch.from_numpy(p.copy()).to(a.device, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, grad):
+            g = ctx.sim.backward(grad.detach().cpu().double().numpy())
+            return torch.from_numpy(g).to(grad.device), None, None, None, None
+
+    return PaddleProbes

@@ -452,7 +452,8 @@ def test_neighbour_masks_exact_at_the_support_boundary():
     assert len(bad) == 0, (bad[:10], cnt[bad[:10]], ref[bad[:10]])
 
 
-@pytest.mark.parametrize("damping", [sph2d.DAMPING_COSINE, sph2d.DAMPING_OPENFOAM, sph2d.DAMPING_DUALSPH])
+@pytest.mark.parametrize("damping", [sph2d.DAMPING_COSINE, sph2d.DAMPING_OPENFOAM, sph2d.DAMPING_DUALSPH,
+                                     sph2d.DAMPING_LINEAR])
 def test_end_damping_zone_parity(damping):
     """single_step_compute.f90:174-260 on the last 0.3 of the box: one evaluation's
     rates, then 21 predictor-corrector steps (an MLS step included) against the

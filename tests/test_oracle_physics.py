@@ -558,6 +558,20 @@ def test_end_damping_openfoam_and_dualsph_shapes():
     assert np.max(np.abs(np.polyval(fit, x[inside]) - q)) < 1e-9
 
 
+def test_end_damping_linear_ramp():
+    """The linear zone: factor 1 - A (x + L - Lx)/L, a straight line in x with slope
+    -A/L, 1 at the zone start and 1 - A at the wall, no dependence on y."""
+    L = 0.3
+    A = float(np.float32(0.3))
+    x, y, r, a0, a1 = _damped_ratio(4, L)
+    inside = (1.0 - x) <= L
+    assert np.array_equal(a0[~inside], a1[~inside])
+    fit = np.polyfit(x[inside], r[inside], 1)
+    assert abs(fit[0] + A / L) < 1e-10
+    assert abs(np.polyval(fit, 1.0 - L) - 1.0) < 1e-10 and abs(np.polyval(fit, 1.0) - (1 - A)) < 1e-10
+    assert np.max(np.abs(np.polyval(fit, x[inside]) - r[inside])) < 1e-11
+
+
 def test_wavemaker_pull_back_mirrors_behind_the_paddle():
     """Wavemaker_Subroutine.f90:196-215: particles behind the paddle land at their
     mirror image 2 xp - x, every particle ends in front of it, and exactly those
