@@ -259,7 +259,7 @@ def batched_leg(sph2d, instances: int, steps: int, warmup: int) -> dict:
             "timing": "host wall clock with device synchronisation, launches included"}
 
 
-def adjoint_leg(sph2d, n_side: int, reps: int) -> dict:
+def evaluation_vjp_leg(sph2d, n_side: int, reps: int) -> dict:
     """Reverse of one FP64 force evaluation (sph2d_single_step_compute_vjp) beside the
     forward evaluation it reverses (sph2d_single_step_compute), TGV lattice of n_side^2:
     wall-clock per call between device synchronisations. The reverse call includes
@@ -286,8 +286,82 @@ def adjoint_leg(sph2d, n_side: int, reps: int) -> dict:
     for _ in range(reps):
         sim.single_step_compute_vjp(*lam)
     rev = (time.perf_counter() - t0) / reps
+    sim.close()
     return {"n_side": n_side, "particles": N, "forward_evaluation_ms": 1e3 * fwd,
             "reverse_evaluation_ms": 1e3 * rev, "reverse_over_forward": rev / fwd, "reps": reps}
+
+
+# the wave tank of the adjoint workload: 8 m long, water 0.4 m deep, dx = 0.005 m
+TANK = dict(length=8.0, height=0.6, depth=0.4, dx=0.005, g=9.81, damping_length=2.0)
+
+
+def wave_tank(sph2d, dx: float = TANK["dx"]):
+    """(params, fluid state) of the piston wave tank: still water at hydrostatic
+    pressure (rho from the inverse Tait law with the REAL(4) exponent), c0 = 10
+    sqrt(g depth), rho0 = 1000, walls of 3 layers, 3 wavemaker columns, a linear
+    damping zone over the last 2 m, MLS reinitialisation off (the reverse holds no
+    MLS), dt = 0.083 h / c0 (the reference's c0 dt / h)."""
+    import numpy as np
+    L, Hb, d, g = TANK["length"], TANK["height"], TANK["depth"], TANK["g"]
+    nx, ny = int(round(L / dx)), int(round(Hb / dx))
+    c0 = 10.0 * np.sqrt(g * d)
+    h = 1.2 * dx
+    prm = sph2d.default_params(nx=nx, ny=ny, box_x=L, box_y=Hb, dx=dx, h=h, max_shift=0.25 * dx,
+                               rho0=1000.0, c0sq=c0 * c0, nu=1e-6, U=float(np.sqrt(g * d)),
+                               dt=0.083 * h / c0, integrator=2, walls=3, periodic=0, gravity=g,
+                               wavemaker_layers=3, damping=sph2d.DAMPING_LINEAR,
+                               damping_length=TANK["damping_length"], damping_amplitude=0.5, water_depth=d,
+                               mls_on=0)
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(int(round(d / dx))), indexing="ij")
+    x = ((ii + 0.5) * dx).reshape(-1)
+    y = ((jj + 0.5) * (Hb / ny)).reshape(-1)
+    p = 1000.0 * g * (d - y)
+    rho = 1000.0 * (1 + 7 * p / (c0 * c0 * 1000.0)) ** float(np.float32(0.142857142))
+    M = len(x)
+    return prm, (x, y, np.zeros(M), np.zeros(M), rho, rho * dx * (Hb / ny))
+
+
+def adjoint_leg(sph2d, steps: int, k: int = 20) -> dict:
+    """The adjoint workload on the full tank (~128k fluid particles): sph2d_forward over
+    `steps` steps (8-harmonic stroke, f0 = 0.5 Hz, a_k ~ N(0, 0.01 m), seed 4; MLS
+    pressure probes at x = 2, 3, 4 m, 0.1 m below the still surface, every 10 steps;
+    a checkpoint every k steps), then sph2d_backward of a misfit against a second set
+    of coefficients: wall clock of each between device synchronisations."""
+    import numpy as np
+    prm, st = wave_tank(sph2d)
+    rng = np.random.default_rng(4)
+    a, a_t = rng.normal(0.0, 0.01, 8), rng.normal(0.0, 0.01, 8)
+    cfg = sph2d.adjoint_cfg(n_harmonics=8, f0=0.5, standing_steps=1, sample_every=10,
+                            probes=[(2.0, 0.3), (3.0, 0.3), (4.0, 0.3)], checkpoint_every=k)
+    hip = sph2d._hiprt()
+
+    def fresh():
+        sim = sph2d.Simulation(prm, capacity=len(st[0]))
+        sim.upload_state(*st)
+        sim.wavemaker_update(0.0, 0.0)
+        sim.time_step(1)
+        return sim
+
+    sim = fresh()
+    target = sim.forward(cfg, a_t, 2, steps)
+    sim.close()
+    sim = fresh()
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    p = sim.forward(cfg, a, 2, steps)
+    tf = time.perf_counter() - t0
+    w = 2.0 * (p - target) / p.size
+    t0 = time.perf_counter()
+    g = sim.backward(w)
+    tb = time.perf_counter() - t0
+    n_fluid = sim.n_fluid
+    sim.close()
+    return {"workload": "piston wave tank 8 m x 0.4 m deep, dx 0.005 m, FP64, predictor-corrector",
+            "fluid_particles": n_fluid, "steps": steps, "dt": prm.dt, "checkpoint_every": k,
+            "steps_for_2s": int(round(2.0 / prm.dt)),
+            "forward_ms_per_step": 1e3 * tf / steps, "backward_ms_per_step": 1e3 * tb / steps,
+            "reverse_over_forward": tb / tf, "loss": float(np.mean((p - target) ** 2)),
+            "grad_norm": float(np.linalg.norm(g))}
 
 
 def main() -> int:
@@ -318,7 +392,9 @@ def main() -> int:
                     help="batched leg: independent TGV instances advanced by sph2d_batch (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--adjoint-side", type=int, default=1024,
-                    help="reverse-evaluation leg on an n^2 TGV lattice (0 skips it)")
+                    help="reverse of one evaluation on an n^2 TGV lattice (0 skips it)")
+    ap.add_argument("--adjoint-steps", type=int, default=4000,
+                    help="wave-tank adjoint leg: steps of sph2d_forward/sph2d_backward (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU form (spatial slabs "
@@ -490,11 +566,16 @@ def main() -> int:
             out["batched"] = batched_leg(sph2d, args.batch_instances, args.batch_steps, args.warmup)
         except Exception as e:                    # the headline line is printed regardless
             out["batched"] = {"error": repr(e)}
-    if args.adjoint_side > 0:
+    if args.adjoint_steps > 0:
         try:
-            out["adjoint"] = adjoint_leg(sph2d, args.adjoint_side, 5)
+            out["adjoint"] = adjoint_leg(sph2d, args.adjoint_steps)
         except Exception as e:                    # the headline line is printed regardless
             out["adjoint"] = {"error": repr(e)}
+    if args.adjoint_side > 0:
+        try:
+            out.setdefault("adjoint", {})["evaluation_vjp"] = evaluation_vjp_leg(sph2d, args.adjoint_side, 5)
+        except Exception as e:                    # the headline line is printed regardless
+            out.setdefault("adjoint", {})["evaluation_vjp"] = {"error": repr(e)}
     if rank == 0 and not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         head = cpu_baseline_all_cores(args.cpu_all_side, args.cpu_all_steps)

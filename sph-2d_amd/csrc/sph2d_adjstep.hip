@@ -262,15 +262,33 @@ __global__ void __launch_bounds__(kBlock) k_adj_wm_share(DevParams P, Buffers B,
   c3[3 * (size_t)g + 2] = cH;
 }
 
-// Fixed-order sum of c3 over n gids (one block), then dL/dH into the arg-max particle's
-// y and (dL/dx_p, dL/du_p) of the step into out.
-__global__ void __launch_bounds__(1024) k_adj_wm_reduce(const double* c3, int n, const int* argmax, double* lamG,
-                                                        double* out) {
-  __shared__ double sh[3][1024];
+// Fixed-order sum of c3 over n gids: block partials (k_adj_wm_part, contiguous ranges
+// summed in order by a fixed tree), then one block over the partials; dL/dH goes into
+// the arg-max particle's y and (dL/dx_p, dL/du_p) of the step into out.
+constexpr int kRedBlocks = 256;
+
+__global__ void __launch_bounds__(256) k_adj_wm_part(const double* c3, int n, double* part) {
+  __shared__ double sh[3][256];
+  const int per = (n + kRedBlocks - 1) / kRedBlocks;
+  const int lo = blockIdx.x * per, hi = min(n, lo + per);
   double a[3] = {0.0, 0.0, 0.0};
-  for (int i = threadIdx.x; i < n; i += blockDim.x)
+  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x)
     for (int k = 0; k < 3; ++k) a[k] = a[k] + c3[3 * (size_t)i + k];
   for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] = a[k];
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off)
+      for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] = sh[k][threadIdx.x] + sh[k][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) part[3 * blockIdx.x + k] = sh[k][0];
+}
+
+__global__ void __launch_bounds__(kRedBlocks) k_adj_wm_reduce(const double* part, int n, const int* argmax,
+                                                              double* lamG, double* out) {
+  __shared__ double sh[3][kRedBlocks];
+  for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] = part[3 * threadIdx.x + k];
   __syncthreads();
   for (int off = blockDim.x / 2; off > 0; off >>= 1) {
     if ((int)threadIdx.x < off)
@@ -283,6 +301,20 @@ __global__ void __launch_bounds__(1024) k_adj_wm_reduce(const double* c3, int n,
     const int am = *argmax;
     if (am >= 0 && am < n) lamG[6 * (size_t)am + 1] = lamG[6 * (size_t)am + 1] + sh[2][0];
   }
+}
+
+// A snapshot copy in one launch: up to 10 arrays, in 4-byte words.
+__global__ void __launch_bounds__(256) k_adj_copy(AdjCopy C) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (int a = 0; a < C.n; ++a) {
+    const uint32_t* src = static_cast<const uint32_t*>(C.src[a]);
+    uint32_t* dst = static_cast<uint32_t*>(C.dst[a]);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < C.words[a]; i += stride) dst[i] = src[i];
+  }
+}
+
+void launch_adj_copy(hipStream_t st, const AdjCopy& C) {
+  hipLaunchKernelGGL(k_adj_copy, dim3(1024), dim3(256), 0, st, C);
 }
 
 // 6x6 symmetric positive definite solve by Cholesky (the same factorisation as the
@@ -406,10 +438,11 @@ void launch_adj_wm_flags(hipStream_t st, const DevParams& P, Buffers& B, int cap
 
 void launch_adj_wm_reverse(hipStream_t st, const DevParams& P, Buffers& B, int n_total, int n_wm, const double* G,
                            const int* flags, const unsigned long long* hbits, double* lamG, double* c3,
-                           const int* argmax, double* out2) {
+                           double* part, const int* argmax, double* out2) {
   hipLaunchKernelGGL(k_adj_wm_share, dim3(grid_for(n_total)), dim3(kBlock), 0, st, P, B, G, flags, hbits, lamG, c3,
                      n_wm);
-  hipLaunchKernelGGL(k_adj_wm_reduce, dim3(1), dim3(1024), 0, st, c3, n_total, argmax, lamG, out2);
+  hipLaunchKernelGGL(k_adj_wm_part, dim3(kRedBlocks), dim3(256), 0, st, c3, n_total, part);
+  hipLaunchKernelGGL(k_adj_wm_reduce, dim3(1), dim3(kRedBlocks), 0, st, part, n_total, argmax, lamG, out2);
 }
 
 __global__ void k_adj_wm_init(unsigned long long* hbits, int* argmax) {

@@ -2220,11 +2220,16 @@ int snap_copy(sph2d_ctx* c, AdjTape* t, char* snap, bool save) {
   void* dev[10] = {c->B.g_pos, c->B.g_vel, c->B.g_rho, c->B.g_p, c->B.g_c, c->B.g_mass, c->B.g_type,
                    c->B.drhodt, c->B.acc, c->B.counters};
   const size_t sz[10] = {16 * G, 16 * G, 8 * G, 8 * G, 8 * G, 8 * G, 4 * G, 8 * L, 16 * L, 4 * (size_t)C_COUNT};
+  AdjCopy C;
+  C.n = 10;
   for (int k = 0; k < 10; ++k) {
     char* s = snap + t->off[k];
-    HIP_OR_FAIL(hipMemcpyAsync(save ? (void*)s : dev[k], save ? dev[k] : (void*)s, sz[k], hipMemcpyDeviceToDevice,
-                               c->stream));
+    C.src[k] = save ? dev[k] : (const void*)s;
+    C.dst[k] = save ? (void*)s : dev[k];
+    C.words[k] = sz[k] / 4;
   }
+  launch_adj_copy(c->stream, C);
+  HIP_OR_FAIL(hipGetLastError());
   return SPH2D_OK;
 }
 
@@ -2336,7 +2341,7 @@ int sph2d_backward(sph2d_ctx* c, const double* dl_dprobe, double* dl_da) {
   const size_t np = (size_t)t->cfg.n_probes;
   const size_t G = (size_t)c->cap_global, L = (size_t)c->cap_local;
   if ((rc = grow(&t->seg, &t->seg_cap, t->snap_bytes * (size_t)K))) return rc;
-  const size_t nws = 6 * G + 3 * L + 3 * L + 11 * L + 10 * L + 5 * L + 5 * L + 5 * L + L + 3 * G;
+  const size_t nws = 6 * G + 3 * L + 3 * L + 11 * L + 10 * L + 5 * L + 5 * L + 5 * L + L + 3 * G + 3 * 256;
   if ((rc = grow(&t->ws, &t->ws_cap, nws))) return rc;
   if ((rc = grow(&t->pad, &t->pad_cap, 2 * (size_t)t->nsteps))) return rc;
   if ((rc = grow(&t->iws, &t->iws_cap, G + 8))) return rc;
@@ -2350,6 +2355,7 @@ int sph2d_backward(sph2d_ctx* c, const double* dl_dprobe, double* dl_da) {
   double* Gs = ga + 5 * L;
   double* wsum = Gs + 5 * L;
   double* c3 = wsum + L;
+  double* part = c3 + 3 * G;
   int* flags = t->iws;
   int* argmax = t->iws + G;
   auto* hbits = reinterpret_cast<unsigned long long*>(t->iws + G + 2);
@@ -2387,8 +2393,8 @@ int sph2d_backward(sph2d_ctx* c, const double* dl_dprobe, double* dl_da) {
       launch_adj_stage1(c->stream, c->P, c->B, c->cap_local, lamG, lamR, W, scr, c->prm.shift_on);
       launch_eval_vjp_slots(c->stream, c->P, c->B, c->cap_local, c->prm.shift_on, W, gs, ga, Gs, wsum);
       launch_adj_stage2(c->stream, c->P, c->B, c->cap_local, Gs, scr, lamR, lamG, lamRp);
-      launch_adj_wm_reverse(c->stream, c->P, c->B, c->n_fluid + c->n_wm, c->n_wm, Gs, flags, hbits, lamG, c3, argmax,
-                            t->pad + 2 * (size_t)k);
+      launch_adj_wm_reverse(c->stream, c->P, c->B, c->n_fluid + c->n_wm, c->n_wm, Gs, flags, hbits, lamG, c3, part,
+                            argmax, t->pad + 2 * (size_t)k);
       std::swap(lamR, lamRp);
       HIP_OR_FAIL(hipGetLastError());
     }

@@ -147,20 +147,22 @@ __device__ __forceinline__ void k_eval_b1_body(const DevParams& P, Buffers& B, i
     ny = -n2 / len;
     const double2 ps = B.l_pos[s];
     const double vx = ps.x + P.h * nx, vy = ps.y + P.h * ny;
-    const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
     const double pi = 3.14159265358979324;
-    for (int d = 0; d < 3 && keep; ++d)
-      for (int o = R.lo[d]; o < R.hi[d]; ++o) {
-        if (B.l_type[o] != 2) continue;
-        const double2 po = B.l_pos[o];
-        const double pdx = po.x - ps.x, pdy = po.y - ps.y;
-        const double JI = sqrt(pdx * pdx + pdy * pdy);
-        const double Nd = nx * pdx + ny * pdy;
-        const double qx = po.x - vx, qy = po.y - vy;
-        const double JT = sqrt(qx * qx + qy * qy);
-        if (JI >= P.surf_lim && JT < P.h) { keep = false; break; }
-        if (JI < P.surf_lim && acos(Nd / JI) <= (pi / 4.0)) { keep = false; break; }
-      }
+    // The reference tests every fluid candidate of the 3x3 cells; either test can only
+    // fire within 2h (JT < h around x + h n, or JI < 1.414 h), so the accepted pair
+    // partners (within 3h) hold every candidate that matters, and "any fires" does not
+    // depend on the order.
+    for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) __attribute__((always_inline)) {
+      if (!keep || B.l_type[o] != 2) return;
+      const double2 po = B.l_pos[o];
+      const double pdx = po.x - ps.x, pdy = po.y - ps.y;
+      const double JI = sqrt(pdx * pdx + pdy * pdy);
+      const double Nd = nx * pdx + ny * pdy;
+      const double qx = po.x - vx, qy = po.y - vy;
+      const double JT = sqrt(qx * qx + qy * qy);
+      if (JI >= P.surf_lim && JT < P.h) keep = false;
+      else if (JI < P.surf_lim && acos(Nd / JI) <= (pi / 4.0)) keep = false;
+    }, B.nb_list, B.nb_count, B.kstride);
   }
   B.nrm[s] = make_double2(nx, ny);
   if (!keep) {

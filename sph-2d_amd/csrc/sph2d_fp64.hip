@@ -1121,43 +1121,14 @@ __device__ __forceinline__ double wall_w(const DevParams& P, double r) {
   return (exp(-(q * q)) - P.e9) * P.wall_kfactor;
 }
 
-__global__ void __launch_bounds__(kBlock) k_assign_walls(DevParams P, Buffers B) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= B.counters[C_NLOCAL] || B.l_type[s] != -3) return;
-  const double2 xs = B.l_pos[s];
-  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
-  const double rmax = 3 * P.wall_h, rmin = (double)1.0E-7f;
-  double a = 0.0;
-  int nt = 0;
-  for (int d = 0; d < 3; ++d)
-    for (int o = R.lo[d]; o < R.hi[d]; ++o) {
-      if (B.l_type[o] == -3) continue;
-      const double2 xo = B.l_pos[o];
-      const double px = xs.x - xo.x, py = xs.y - xo.y;
-      const double r = sqrt(px * px + py * py);
-      if (r <= rmax && r >= rmin) {
-        a = a + wall_w(P, r) * 1.0;
-        ++nt;
-      }
-    }
-  double u = 0.0, v = 0.0, pr = 0.0;
-  if (nt >= 3) {
-    const double ainv = 1.0 / a;
-    for (int d = 0; d < 3; ++d)
-      for (int o = R.lo[d]; o < R.hi[d]; ++o) {
-        if (B.l_type[o] == -3) continue;
-        const double2 xo = B.l_pos[o];
-        const double px = xs.x - xo.x, py = xs.y - xo.y;
-        const double r = sqrt(px * px + py * py);
-        if (!(r <= rmax && r >= rmin)) continue;
-        const double fai = 1.0 * (ainv * wall_w(P, r));
-        const double2 vo = B.l_vel[o];
-        if (P.g != 0.0) pr = pr + B.l_pc[o].x * fai - P.g * B.l_rv[o].x * (xs.y - xo.y) * fai;
-        else pr = pr + B.l_pc[o].x * fai;
-        u = u + vo.x * fai;
-        v = v + vo.y * fai;
-      }
-  }
+// One wavefront per local slot (a wall's slot does work, the others leave at once):
+// the lanes test the candidates of the 3x3 cells in parallel and compact the
+// accepted ones into LDS in candidate order, with what the sums need; lane 0 then
+// sums in that order, so the arithmetic is the sequential loop's. More than
+// kWallList accepted candidates fall back to lane 0 walking the cells itself.
+constexpr int kWallWaves = 4, kWallList = 128;
+
+__device__ __forceinline__ void wall_finish(const DevParams& P, Buffers& B, int s, double u, double v, double pr) {
   const int bt = B.g_btype[B.name[s]];
   if (bt == -6) v = -v;
   else if (bt == -7) { u = -u; v = -v; }
@@ -1179,8 +1150,93 @@ __global__ void __launch_bounds__(kBlock) k_assign_walls(DevParams P, Buffers B)
   if (P.mixed) write_r0(P, B, s, make_double2(u, v), rho, mass, -3);
 }
 
+__global__ void __launch_bounds__(64 * kWallWaves) k_assign_walls(DevParams P, Buffers B) {
+  __shared__ double sw[kWallWaves][kWallList], sp[kWallWaves][kWallList], sr[kWallWaves][kWallList];
+  __shared__ double sy[kWallWaves][kWallList], su[kWallWaves][kWallList], sv[kWallWaves][kWallList];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = blockIdx.x * kWallWaves + wv;
+  if (s >= B.counters[C_NLOCAL] || B.l_type[s] != -3) return;   // uniform across the wavefront
+  const double2 xs = B.l_pos[s];
+  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
+  const double rmax = 3 * P.wall_h, rmin = (double)1.0E-7f;
+  const uint64_t below = (1ull << lane) - 1ull;
+  int cnt = 0;
+  for (int d = 0; d < 3; ++d)
+    for (int base = R.lo[d]; base < R.hi[d]; base += 64) {
+      const int o = base + lane;
+      bool ok = false;
+      double w = 0.0;
+      if (o < R.hi[d] && B.l_type[o] != -3) {
+        const double2 xo = B.l_pos[o];
+        const double px = xs.x - xo.x, py = xs.y - xo.y;
+        const double r = sqrt(px * px + py * py);
+        if (r <= rmax && r >= rmin) { ok = true; w = wall_w(P, r); }
+      }
+      const uint64_t m = __ballot(ok);
+      const int k = cnt + __popcll(m & below);
+      if (ok && k < kWallList) {
+        sw[wv][k] = w;
+        sp[wv][k] = B.l_pc[o].x;
+        sr[wv][k] = B.l_rv[o].x;
+        sy[wv][k] = B.l_pos[o].y;
+        const double2 vo = B.l_vel[o];
+        su[wv][k] = vo.x;
+        sv[wv][k] = vo.y;
+      }
+      cnt += __popcll(m);
+    }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  if (lane != 0) return;
+  double u = 0.0, v = 0.0, pr = 0.0;
+  if (cnt <= kWallList) {
+    double a = 0.0;
+    for (int k = 0; k < cnt; ++k) a = a + sw[wv][k] * 1.0;
+    if (cnt >= 3) {
+      const double ainv = 1.0 / a;
+      for (int k = 0; k < cnt; ++k) {
+        const double fai = 1.0 * (ainv * sw[wv][k]);
+        if (P.g != 0.0) pr = pr + sp[wv][k] * fai - P.g * sr[wv][k] * (xs.y - sy[wv][k]) * fai;
+        else pr = pr + sp[wv][k] * fai;
+        u = u + su[wv][k] * fai;
+        v = v + sv[wv][k] * fai;
+      }
+    }
+  } else {                                               // a crowded support: the cells directly
+    double a = 0.0;
+    int nt = 0;
+    for (int d = 0; d < 3; ++d)
+      for (int o = R.lo[d]; o < R.hi[d]; ++o) {
+        if (B.l_type[o] == -3) continue;
+        const double2 xo = B.l_pos[o];
+        const double px = xs.x - xo.x, py = xs.y - xo.y;
+        const double r = sqrt(px * px + py * py);
+        if (r <= rmax && r >= rmin) { a = a + wall_w(P, r) * 1.0; ++nt; }
+      }
+    if (nt >= 3) {
+      const double ainv = 1.0 / a;
+      for (int d = 0; d < 3; ++d)
+        for (int o = R.lo[d]; o < R.hi[d]; ++o) {
+          if (B.l_type[o] == -3) continue;
+          const double2 xo = B.l_pos[o];
+          const double px = xs.x - xo.x, py = xs.y - xo.y;
+          const double r = sqrt(px * px + py * py);
+          if (!(r <= rmax && r >= rmin)) continue;
+          const double fai = 1.0 * (ainv * wall_w(P, r));
+          const double2 vo = B.l_vel[o];
+          if (P.g != 0.0) pr = pr + B.l_pc[o].x * fai - P.g * B.l_rv[o].x * (xs.y - xo.y) * fai;
+          else pr = pr + B.l_pc[o].x * fai;
+          u = u + vo.x * fai;
+          v = v + vo.y * fai;
+        }
+    }
+  }
+  wall_finish(P, B, s, u, v, pr);
+}
+
 void launch_assign_walls(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
-  hipLaunchKernelGGL(k_assign_walls, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B);
+  hipLaunchKernelGGL(k_assign_walls, dim3((cap_local + kWallWaves - 1) / kWallWaves), dim3(64 * kWallWaves), 0, s,
+                     P, B);
 }
 
 // Refresh_domain.f90:136-300 for the fixed ghosts: their assigned state goes back to

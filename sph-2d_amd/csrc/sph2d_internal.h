@@ -140,7 +140,14 @@ void launch_adj_wm_flags(hipStream_t st, const DevParams& P, Buffers& B, int cap
                          const unsigned long long* hbits, int* argmax);
 void launch_adj_wm_reverse(hipStream_t st, const DevParams& P, Buffers& B, int n_total, int n_wm, const double* G,
                            const int* flags, const unsigned long long* hbits, double* lamG, double* c3,
-                           const int* argmax, double* out2);
+                           double* part, const int* argmax, double* out2);
+struct AdjCopy {
+  const void* src[10];
+  void* dst[10];
+  size_t words[10];
+  int n;
+};
+void launch_adj_copy(hipStream_t st, const AdjCopy& C);
 void launch_adj_probe(hipStream_t st, const DevParams& P, Buffers& B, double ex, double ey, double gravity,
                       const double* lam, double* lamG);
 void launch_wavemaker_height(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, double xp,
