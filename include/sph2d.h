@@ -385,6 +385,25 @@ int sph2d_wavemaker_pull_back(sph2d_ctx* ctx, double position, double velocity);
  * its velocity with the hydrostatic density below that height, then the pull-back. */
 int sph2d_wavemaker_update(sph2d_ctx* ctx, double position, double velocity);
 
+/* --- wave-theory wavemaker and its damping (relaxation) zone ---------------------------
+ * Parallel_Taylor_Green_Vortices_2D_Release.f90:113-143: the zone's analytic fields start
+ * as the state, the initial positions of fluid and wavemaker particles are kept, fluid
+ * particles with x <= zone_length (WavemakerDampingLengh) are in the zone with the soft
+ * coefficient 1 - (exp((x/L)^3.5) - 1)/(exp(1.0 REAL(4)) - 1). zone_length <= 0 keeps
+ * only the initial positions (the wave-theory wavemaker needs them) and no zone. */
+int sph2d_relaxation_init(sph2d_ctx* ctx, double zone_length);
+/* Wavemaker_Subroutine.f90:262-396 and :617-700 (WaveTheory, Stokes_2nd or Airy) before a
+ * step: fluid below y = 0 typed 120; the wavemaker particles moved, at the wave's velocity
+ * and pressure (rho from the inverse Tait law, mass rho0 V0 below the elevation, else 0);
+ * the zone's analytic position, velocity, pressure and density. Uses w's wave_type
+ * (SPH2D_WAVE_STOKES2 or SPH2D_WAVE_AIRY), soft_steps, standing_steps, dt, g, water_depth,
+ * wave_height, wave_number, wave_frequency (period 2 pi / it) and phase. */
+int sph2d_wave_theory_update(sph2d_ctx* ctx, const sph2d_wavemaker* w, int32_t i_time_step);
+/* wave_maker_damping_zone.f90:61-95 on the subdomain state: velocity, position and pressure
+ * of zone particles blended toward the analytic fields, rho = rho0 (1 + 7 p/(c0^2 rho0))^
+ * 0.142857142. sph2d_time_step runs it after the corrector once a zone is set. */
+int sph2d_wave_maker_damping_zone(sph2d_ctx* ctx, int32_t i_time_step);
+
 /* --- differentiable wave tank ------------------------------------------------------
  * A discrete adjoint of the predictor-corrector step (FP64 form) of a walled tank with
  * piston wavemaker particles, as config 4's gradient of a probe-pressure misfit with

@@ -84,6 +84,10 @@ def lib():
         L.orc_wall_positions.restype = ctypes.c_int32
         L.orc_wavemaker_pull_back.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         L.orc_wavemaker_update.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.orc_relaxation_init.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_wave_theory_update.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32] + \
+            [ctypes.c_double] * 7 + [ctypes.c_int32]
+        L.orc_relaxation_blend.argtypes = [ctypes.c_void_p]
         L.orc_initial_ke.argtypes = [ctypes.c_void_p]
         L.orc_initial_ke.restype = ctypes.c_double
         for name in ("orc_n_fluid", "orc_n_global", "orc_n_local"):
@@ -219,6 +223,16 @@ class Oracle:
 
     def wavemaker_pull_back(self, xp: float, up: float):
         lib().orc_wavemaker_pull_back(self._c, xp, up)
+
+    def relaxation_init(self, zone_length: float):
+        lib().orc_relaxation_init(self._c, zone_length)
+
+    def wave_theory_update(self, wave_type, soft_steps, standing_steps, dt, g, depth, height, k, omega, phase, it):
+        lib().orc_wave_theory_update(self._c, wave_type, soft_steps, standing_steps, dt, g, depth, height, k, omega,
+                                     phase, it)
+
+    def relaxation_blend(self):
+        lib().orc_relaxation_blend(self._c)
 
     def wavemaker_update(self, xp: float, up: float):
         lib().orc_wavemaker_update(self._c, xp, up)

@@ -82,6 +82,12 @@ struct orc_ctx {
   double *t_u, *t_v, *t_drho, *t_de, *t_ax, *t_ay;              /* PC temp_* */
   double *k_u[3], *k_v[3], *k_drho[3], *k_de[3], *k_ax[3], *k_ay[3]; /* RK K_1..K_3 */
   double *mls_rho;
+  /* wavemaker damping (relaxation) zone and wave-theory wavemaker, global arrays:
+   * In_WaveMakerDampingZone_OrNot, position_soft_coefficient, WaveMaker_Analytical_*,
+   * Initial_particle_Position (Public_variable_module.f90:61-66) */
+  int relax_on;
+  int *rz_in;
+  double *rz_c, *an_x, *an_y, *an_u, *an_v, *an_p, *an_rho, *x0, *y0;
 };
 
 /* ------------------------------------------------------------------ */
@@ -294,6 +300,7 @@ void orc_destroy(orc_ctx* c) {
                   c->lap_y, c->ext_x, c->ext_y, c->avg_x, c->avg_y, c->corr, c->grx, c->gry,
                   c->gpx, c->gpy, c->gux, c->guy, c->gvx, c->gvy, c->mag_drho, c->mag_ax, c->mag_ay, c->mag_grho, c->shx, c->shy, c->vort, c->b_rho, c->b_e, c->b_x, c->b_y, c->b_u, c->b_v, c->t_u,
                   c->t_v, c->t_drho, c->t_de, c->t_ax, c->t_ay, c->mls_rho,
+                  c->rz_in, c->rz_c, c->an_x, c->an_y, c->an_u, c->an_v, c->an_p, c->an_rho, c->x0, c->y0,
                   c->cell_count, c->cell_list, c->near_n, c->near_name,
                   c->pi, c->pj, c->pw, c->pdwx, c->pdwy, c->pdx, c->pdy, c->pvx, c->pvy, c->prd,
                   c->pc_, c->prho_, c->ph_, c->ndix, c->ndiy, c->ndjx, c->ndjy};
@@ -1601,6 +1608,155 @@ static void remap_rates(orc_ctx* c, const int* old_name, int old_n) {
   free(d); free(ax); free(ay); free(inv); (void)NG;
 }
 
+/* ------------------------------------------------------------------ */
+/* Wave-theory wavemaker and its damping (relaxation) zone             */
+/* ------------------------------------------------------------------ */
+/* Parallel_Taylor_Green_Vortices_2D_Release.f90:113-143: the analytic fields start as */
+/* the particle state; fluid particles with x <= L are in the zone with the soft        */
+/* coefficient 1 - (exp((x/L)**3.5) - 1.0)/(exp(1.0) - 1.0) (exp(1.0) a REAL(4)          */
+/* intrinsic); Initial_particle_Position (here for fluid and wavemaker particles) is    */
+/* the current position. zone_length <= 0 keeps only the initial positions (the        */
+/* wave-theory wavemaker particles need them) and leaves the zone off.                  */
+void orc_relaxation_init(orc_ctx* c, double zone_length) {
+  const int n = c->n_fluid + c->n_wavemaker;
+  if (!c->rz_in) {
+    c->rz_in = xcalloc(c->cap_global, sizeof(int));
+#define A(f) c->f = xcalloc(c->cap_global, sizeof(double));
+    A(rz_c) A(an_x) A(an_y) A(an_u) A(an_v) A(an_p) A(an_rho) A(x0) A(y0)
+#undef A
+  }
+  for (int i = 0; i < n; ++i) {
+    c->x0[i] = c->gx[i]; c->y0[i] = c->gy[i];
+    c->an_x[i] = c->gx[i]; c->an_y[i] = c->gy[i];
+    c->an_u[i] = c->gu[i]; c->an_v[i] = c->gv[i];
+    c->an_rho[i] = c->grho[i]; c->an_p[i] = c->gp[i];
+    c->rz_in[i] = 0; c->rz_c[i] = 0.0;
+    if (i < c->n_fluid && zone_length > 0.0 && c->gx[i] <= zone_length) {
+      c->rz_in[i] = 1;
+      c->rz_c[i] = (1 - (exp(pow(c->gx[i] / zone_length, 3.5)) - 1.0) / ((double)expf(1.0f) - 1.0));
+    }
+  }
+  c->relax_on = zone_length > 0.0;
+}
+
+/* rho0 (1 + 7 p / (c0^2 rho0))**0.142857142 (REAL(4) exponent) */
+static double inv_tait(const orc_params* P, double p) {
+  return P->rho0 * pow(1 + 7 * p / (P->c0sq * P->rho0), (double)0.142857142f);
+}
+
+/* Wavemaker_Subroutine.f90:262-346 (Stokes_2nd) and :348-396 (Airy) for the wavemaker   */
+/* particles, :268-276 (fluid below y = 0 typed 120), and :617-700 (the zone's analytic */
+/* fields, Stokes_2nd and Airy) at step it, with time_soft_coefficient of :283-287     */
+/* (integer division) and current_time = dt (it - standing). wave_type 1 Stokes_2nd,    */
+/* 2 Airy; T = 2 pi / omega, wave length 2 pi / k, amplitude H / 2.                      */
+void orc_wave_theory_update(orc_ctx* c, int32_t wave_type, int32_t soft_steps, int32_t standing_steps, double dt,
+                            double g, double d, double H, double k, double om, double phase, int32_t it) {
+  const orc_params* P = &c->P;
+  const double pi = PI_MODULE;
+  const int N = c->n_fluid;
+  for (int i = 0; i < N; ++i) {
+    if (c->gtype[i] == 120) continue;
+    if (c->gy[i] < 0.0) c->gtype[i] = 120;
+  }
+  const double t = dt * (it - standing_steps);
+  double soft;
+  if (it <= soft_steps + standing_steps) soft = 0.5 * (sin((-0.5 + (it - standing_steps) / soft_steps) * pi) + 1);
+  else soft = 1.0;
+  if (!(t >= 0.0)) return;
+  const double T = 2 * pi / om, L = 2 * pi / k, A = H / 2.0;
+  const double skd = sinh(k * d), ckd = cosh(k * d);
+  const double V0 = c->dx * c->dy;
+  for (int q = 0; q < c->n_wavemaker; ++q) {                     /* :300-346, :354-396 */
+    const int i = N + q;
+    const double ang = k * c->gx[i] - om * t + phase;
+    double el, u, v;
+    const double yo = c->gy[i];
+    if (wave_type == 1) {
+      el = soft * (0.5 * H * cos(ang) + pi * H * H / (8 * L) * ckd / (skd * skd * skd) * (2 + cosh(2 * k * d)) * cos(2 * ang));
+      u = pi * H / T * cosh(k * yo) * cos(ang) / skd + 3.0 / 4.0 * (pi * pi) * H / T * (H / L) * cosh(2 * k * yo) * cos(2 * ang) / (skd * skd * skd * skd);
+      v = pi * H / T * sinh(k * yo) * sin(ang) / skd + 3.0 / 4.0 * (pi * pi) * H / T * (H / L) * sinh(2 * k * yo) * sin(2 * ang) / (skd * skd * skd * skd);
+      c->gu[i] = soft * u;
+      c->gv[i] = soft * v;
+      c->gx[i] = c->x0[i] - soft * (A * cosh(k * yo) * sin(ang) / skd + 3.0 * pi * (H * H) / (16 * L) * cosh(2 * k * yo) * sin(2 * ang) / (skd * skd * skd * skd));
+      c->gy[i] = c->y0[i] + soft * (A * sinh(k * yo) * cos(ang) / skd + 3.0 * pi * (H * H) / (16 * L) * sinh(2 * k * yo) * cos(2 * ang) / (skd * skd * skd * skd));
+    } else {
+      el = soft * 0.5 * H * cos(ang);
+      u = pi * H / T * cosh(k * yo) * cos(ang) / skd;
+      v = pi * H / T * sinh(k * yo) * sin(ang) / skd;
+      c->gu[i] = soft * u;
+      c->gv[i] = soft * v;
+      c->gx[i] = c->x0[i] - soft * A * cosh(k * yo) * sin(ang) / skd;
+      c->gy[i] = c->y0[i] + soft * A * sinh(k * yo) * cos(ang) / skd;
+    }
+    const double y = c->gy[i];
+    if (y <= el + d) {
+      double pr;
+      if (wave_type == 1)
+        pr = P->rho0 * g * (el + d - y) + soft * (P->rho0 * g * H / 2.0 * cosh(k * y) * cos(ang) / ckd + 3.0 / 4.0 * P->rho0 * g * H * (pi * H / L) * (1.0 / sinh(2 * k * d)) * (cosh(2 * k * y) / (skd * skd) - (double)(1.0f / 3.0f)) * cos(2 * ang) - 1.0 / 4.0 * P->rho0 * g * H * (pi * H / L) * cosh(2 * k * y) / sinh(2 * k * d));
+      else
+        pr = P->rho0 * g * (d + el - y) + soft * (P->rho0 * g * H / 2.0 * cosh(k * y) * cos(ang) / ckd);
+      c->gp[i] = pr;
+      c->grho[i] = inv_tait(P, pr);
+      c->gmass[i] = P->rho0 * V0;
+      c->gc[i] = sqrt(P->c0sq);
+    } else {
+      c->gp[i] = 0.0; c->grho[i] = P->rho0; c->gmass[i] = 0.0; c->gc[i] = sqrt(P->c0sq);
+    }
+  }
+  if (!c->relax_on) return;
+  for (int i = 0; i < N; ++i) {                                  /* :617-700 */
+    if (c->gtype[i] == 120 || c->rz_in[i] != 1) continue;
+    const double px = c->an_x[i], py = c->an_y[i];
+    const double ang = k * px - om * t + phase;
+    double el;
+    if (wave_type == 1) {
+      el = soft * (0.5 * H * cos(ang) + pi * H * H * (1.0 / tanh(k * d)) * (1 + 3.0 / (2 * (skd * skd * skd)) * cos(2 * ang)));
+      c->an_u[i] = pi * H / T * cosh(k * py) * cos(ang) / skd - 3.0 / 4.0 * (pi * pi) * H / T * (H / L) * cosh(2 * k * py) * cos(2 * ang) / (skd * skd * skd * skd);
+      c->an_v[i] = pi * H / T * sinh(k * py) * sin(ang) / skd - 3.0 / 4.0 * (pi * pi) * H / T * (H / L) * sinh(2 * k * py) * sin(2 * ang) / (skd * skd * skd * skd);
+      c->an_u[i] = soft * c->an_u[i];
+      c->an_v[i] = soft * c->an_v[i];
+      c->an_x[i] = c->x0[i] - soft * (A * cosh(k * py) * sin(ang) / skd + 3.0 * pi * (H * H) / (16 * L) * cosh(2 * k * py) * sin(2 * ang) / (skd * skd * skd * skd));
+      c->an_y[i] = c->y0[i] + soft * (A * sinh(k * py) * cos(ang) / skd + 3.0 * pi * (H * H) / (16 * L) * sinh(2 * k * py) * cos(2 * ang) / (skd * skd * skd * skd));
+    } else {
+      el = soft * 0.5 * H * cos(ang);
+      c->an_u[i] = pi * H / T * cosh(k * py) * cos(ang) / skd;
+      c->an_v[i] = pi * H / T * sinh(k * py) * sin(ang) / skd;
+      c->an_u[i] = soft * c->an_u[i];
+      c->an_v[i] = soft * c->an_v[i];
+      c->an_x[i] = c->x0[i] - soft * A * cosh(k * py) * sin(ang) / skd;
+      c->an_y[i] = c->y0[i] + soft * A * sinh(k * py) * cos(ang) / skd;
+    }
+    const double y = c->an_y[i];
+    if (y <= el + d) {
+      if (wave_type == 1)
+        c->an_p[i] = P->rho0 * g * (el + d - y) + soft * (P->rho0 * g * H / 2.0 * cosh(k * y) * cos(ang) / ckd + 3.0 / 4.0 * P->rho0 * g * H * (pi * H / L) * (1.0 / sinh(2 * k * d)) * (cosh(2 * k * y) / (skd * skd) - (double)(1.0f / 3.0f)) * cos(2 * ang) - 1.0 / 4.0 * P->rho0 * g * H * (pi * H / L) * cosh(2 * k * y) / sinh(2 * k * d));
+      else
+        c->an_p[i] = P->rho0 * g * (el + d - y) + soft * (P->rho0 * g * H / 2.0 * cosh(k * y) * cos(ang) / ckd);
+      c->an_rho[i] = inv_tait(P, c->an_p[i]);
+    } else {
+      c->an_p[i] = 0.0;
+      c->an_rho[i] = P->rho0;
+    }
+  }
+}
+
+/* wave_maker_damping_zone.f90:61-95 on the subdomain (after the corrector): velocity,  */
+/* position and pressure blended toward the analytic fields with the particle's soft     */
+/* coefficient, then rho from the inverse Tait law (REAL(4) exponent).                   */
+void orc_relaxation_blend(orc_ctx* c) {
+  for (int s = 0; s < c->n_local; ++s) {
+    const int g = c->name[s];
+    if (g < 0 || g >= c->n_fluid || c->rz_in[g] != 1) continue;
+    const double f = c->rz_c[g];
+    c->u[s] = f * c->an_u[g] + (1 - f) * c->u[s];
+    c->v[s] = f * c->an_v[g] + (1 - f) * c->v[s];
+    c->x[s] = f * c->an_x[g] + (1 - f) * c->x[s];
+    c->y[s] = f * c->an_y[g] + (1 - f) * c->y[s];
+    c->p[s] = f * c->an_p[g] + (1 - f) * c->p[s];
+    c->rho[s] = inv_tait(&c->P, c->p[s]);
+  }
+}
+
 void orc_time_step(orc_ctx* c, int32_t it) {
   int* old_name = NULL; int old_n = c->n_local;
   if (c->P.rate_reuse == 1 && it > 1) {
@@ -1617,6 +1773,7 @@ void orc_time_step(orc_ctx* c, int32_t it) {
     pc_save(c);
     orc_single_step_compute(c, it);
     pc_correct(c);
+    if (c->relax_on) orc_relaxation_blend(c);                    /* :660-667 */
   } else {                                                       /* :779-1049 */
     save_before(c);
     if (it == 1) orc_single_step_compute(c, it);
@@ -1627,6 +1784,7 @@ void orc_time_step(orc_ctx* c, int32_t it) {
     stage_full(c); save_k(c, 2);
     orc_single_step_compute(c, it);
     rk_final(c);
+    if (c->relax_on) orc_relaxation_blend(c);                    /* :1052-1059 */
   }
   if (c->P.mls_on && (it % c->P.mls_every) == 0) orc_reinitialize_density(c);
   if (c->P.shift_on) orc_shift(c);
@@ -1705,10 +1863,13 @@ static const fld F64[] = {
   FL("g_p", gp, 1), FL("g_e", ge, 1),
   FL("ana_u", ana_u, 4), FL("ana_v", ana_v, 4), FL("ana_p", ana_p, 4), FL("ana_vm", ana_vm, 4),
   FL("pair_w", pw, 2), FL("pair_dwdx_x", pdwx, 2), FL("pair_dwdx_y", pdwy, 2),
+  FL("position_soft_coefficient", rz_c, 1), FL("an_x", an_x, 1), FL("an_y", an_y, 1), FL("an_u", an_u, 1),
+  FL("an_v", an_v, 1), FL("an_p", an_p, 1), FL("an_rho", an_rho, 1), FL("initial_x", x0, 1), FL("initial_y", y0, 1),
 };
 static const fld I32[] = {
   FL("type", type, 0), FL("name", name, 0), FL("surface", surf, 0), FL("near_surface", near, 0),
   FL("candidate_n", cand_n, 0), FL("g_type", gtype, 1), FL("g_btype", gbtype, 1), FL("g_surface", gsurf, 1), FL("pair_i", pi, 2), FL("pair_j", pj, 2),
+  FL("in_damping_zone", rz_in, 1),
 };
 static int64_t fld_count(const orc_ctx* c, int kind) {
   switch (kind) {

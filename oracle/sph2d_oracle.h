@@ -117,6 +117,11 @@ void orc_sampling_points(orc_ctx* c, int32_t n, const double* px, const double* 
 void orc_wave_probes(orc_ctx* c, int32_t n_probes, const double* probe_x, double* out);
 void orc_wavemaker_pull_back(orc_ctx* c, double xp, double up);
 void orc_wavemaker_update(orc_ctx* c, double xp, double up);
+/* Wave-theory wavemaker (wave_type 1 Stokes_2nd, 2 Airy) and its damping zone */
+void orc_relaxation_init(orc_ctx* c, double zone_length);
+void orc_wave_theory_update(orc_ctx* c, int32_t wave_type, int32_t soft_steps, int32_t standing_steps, double dt,
+                            double g, double d, double H, double k, double om, double phase, int32_t it);
+void orc_relaxation_blend(orc_ctx* c);
 
 /* Sampling_subroutine.f90:340-461. out[0]=KE, [1]=max|v|, [2..4]=u Linf,L1,L2,
  * [5..7]=v, [8..10]=p/(rho0 U^2), [11..13]=|v|. */

@@ -81,6 +81,9 @@ struct sph2d_ctx {
   double* adj_buf = nullptr;
   // sph2d_forward's checkpoints and sph2d_backward's work space (first use)
   AdjTape* tape = nullptr;
+  // wave-theory wavemaker and its damping (relaxation) zone (sph2d_relaxation_init)
+  RelaxArgs relax{};
+  bool relax_ready = false, relax_on = false;
   bool pass_timing = false;
   std::vector<hipEvent_t> events;
   size_t events_used = 0;
@@ -1128,7 +1131,8 @@ static int enqueue_step(sph2d_ctx* c, int it) {
   // of a sampling step and XSPH; other steps skip writing them.
   const bool sampling = it % c->prm.sample_every == 0;
   const bool lean = c->P.mixed && !mls && !sampling && c->prm.epsilon_xsph == 0.0 && !c->prm.walls;
-  rc = devide_block(c, it, pc && it > 1 ? (mls ? 2 : (lean ? 3 : 1)) : 0);
+  // before_* kept for k_stage's corrector: MLS steps, and every step with a damping zone
+  rc = devide_block(c, it, pc && it > 1 ? ((mls || c->relax_on) ? 2 : (lean ? 3 : 1)) : 0);
   if (rc) return rc;
   hipStream_t s = c->stream;
   if (pc) {
@@ -1137,7 +1141,7 @@ static int enqueue_step(sph2d_ctx* c, int it) {
       launch_stage(s, c->P, c->B, c->cap_local, ST_PC_PREDICT);
     }
     do_eval(c, false, it);
-    if (!mls) {
+    if (!mls && !c->relax_on) {
       // corrector, Shift_particle_position and Refresh_domain in one pass
       launch_correct_refresh(s, c->P, c->B, c->cap_local, c->prm.shift_on);
       if (c->P.walls) launch_wall_refresh(s, c->P, c->B, c->cap_local);
@@ -1145,6 +1149,7 @@ static int enqueue_step(sph2d_ctx* c, int it) {
       return SPH2D_OK;
     }
     launch_stage(s, c->P, c->B, c->cap_local, ST_PC_CORRECT);
+    if (c->relax_on) launch_relax_blend(s, c->P, c->B, c->relax, c->cap_local);   // time_iteration.f90:660-667
   } else {
     if (it == 1) do_eval(c, false, it);
     launch_stage(s, c->P, c->B, c->cap_local, ST_RK_HALF0);
@@ -1154,6 +1159,7 @@ static int enqueue_step(sph2d_ctx* c, int it) {
     launch_stage(s, c->P, c->B, c->cap_local, ST_RK_FULL2);
     do_eval(c, false, it);
     launch_stage(s, c->P, c->B, c->cap_local, ST_RK_FINAL);
+    if (c->relax_on) launch_relax_blend(s, c->P, c->B, c->relax, c->cap_local);   // :1052-1059
   }
   if (mls) {
     launch_fill_i32(s, c->B.counters + C_MLSFB, 1, 0);
@@ -1508,6 +1514,68 @@ int sph2d_output_sampling_lines(sph2d_ctx* c, int32_t it, const char* dir, int32
                                            vel.data(), mls.data(), avg.data(), n_points, &sc);
     if (rc) return rc;
   }
+  return SPH2D_OK;
+}
+
+// ---- wave-theory wavemaker and its damping (relaxation) zone ---------------------------
+int sph2d_relaxation_init(sph2d_ctx* c, double zone_length) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (!c->prm.walls) return fail(SPH2D_ERR_UNSUPPORTED, "the wavemaker zone needs a walled tank");
+  if (!std::isfinite(zone_length)) return fail(SPH2D_ERR_ARGUMENT, "zone_length must be finite");
+  if (!c->relax_ready) {
+    const size_t G = (size_t)c->cap_global;
+    if ((rc = c->alloc(&c->relax.in_zone, G))) return rc;
+    if ((rc = c->alloc(&c->relax.coef, G))) return rc;
+    if ((rc = c->alloc(&c->relax.an_pos, G))) return rc;
+    if ((rc = c->alloc(&c->relax.an_vel, G))) return rc;
+    if ((rc = c->alloc(&c->relax.init_pos, G))) return rc;
+    if ((rc = c->alloc(&c->relax.an_p, G))) return rc;
+    if ((rc = c->alloc(&c->relax.an_rho, G))) return rc;
+    c->relax_ready = true;
+  }
+  c->vjp_ready = false;
+  launch_relax_init(c->stream, c->P, c->B, c->relax, c->n_fluid + c->n_wm, zone_length);
+  HIP_OR_FAIL(hipGetLastError());
+  c->relax_on = zone_length > 0.0;
+  return SPH2D_OK;
+}
+
+int sph2d_wave_theory_update(sph2d_ctx* c, const sph2d_wavemaker* w, int32_t it) {
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (!w) return fail(SPH2D_ERR_ARGUMENT, "w is NULL");
+  if (w->wave_type != SPH2D_WAVE_STOKES2 && w->wave_type != SPH2D_WAVE_AIRY)
+    return fail(SPH2D_ERR_UNSUPPORTED, "the wave-theory wavemaker is built for Stokes_2nd and Airy waves");
+  if (w->soft_steps < 1) return fail(SPH2D_ERR_ARGUMENT, "soft_steps must be >= 1");
+  if (!c->relax_ready) return fail(SPH2D_ERR_STATE, "sph2d_relaxation_init first (it keeps the initial positions)");
+  WaveArgs W;
+  W.wave_type = w->wave_type;
+  const double pi = kPiModule;
+  // Wavemaker_Subroutine.f90:280-287: (i_time_step - standing_time_step) / wave_maker_soft_time
+  // divides integers
+  const int rel = it - w->standing_steps;
+  W.soft = it <= w->soft_steps + w->standing_steps ? 0.5 * (std::sin((-0.5 + rel / w->soft_steps) * pi) + 1) : 1.0;
+  W.t = w->dt * rel;
+  W.g = w->g; W.d = w->water_depth; W.H = w->wave_height; W.k = w->wave_number; W.om = w->wave_frequency;
+  W.phase = w->phase;
+  W.T = 2 * pi / W.om; W.L = 2 * pi / W.k; W.A = W.H / 2.0;
+  W.skd = std::sinh(W.k * W.d); W.ckd = std::cosh(W.k * W.d);
+  c->vjp_ready = false;
+  c->eval_part = 0;
+  launch_wave_theory(c->stream, c->P, c->B, c->relax, W, c->cap_fluid, c->n_wm, c->relax_on ? 1 : 0);
+  HIP_OR_FAIL(hipGetLastError());
+  return SPH2D_OK;
+}
+
+int sph2d_wave_maker_damping_zone(sph2d_ctx* c, int32_t it) {
+  (void)it;
+  int rc = need_state(c);
+  if (rc) return rc;
+  if (!c->relax_on) return fail(SPH2D_ERR_STATE, "no wavemaker damping zone (sph2d_relaxation_init with L > 0)");
+  c->vjp_ready = false;
+  launch_relax_blend(c->stream, c->P, c->B, c->relax, c->cap_local);
+  HIP_OR_FAIL(hipGetLastError());
   return SPH2D_OK;
 }
 

@@ -1108,6 +1108,171 @@ void launch_wavemaker_place(hipStream_t s, const DevParams& P, Buffers& B, int c
     hipLaunchKernelGGL(k_wavemaker_place, dim3(grid_for(n_wm)), dim3(kBlock), 0, s, P, B, n_wm, xp, up, height_bits);
 }
 
+// ---------------------------------------------------------------------------
+// Wave-theory wavemaker and its damping (relaxation) zone
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double inv_tait(const DevParams& P, double p) {
+  return P.rho0 * pow(1 + 7 * p / (P.c0sq * P.rho0), (double)0.142857142f);
+}
+
+// Parallel_Taylor_Green_Vortices_2D_Release.f90:113-143 for gid g < n_total (fluid and
+// wavemaker particles): analytic fields = the state, initial position = the position,
+// fluid with x <= L in the zone with 1 - (exp((x/L)**3.5) - 1.0)/(exp(1.0) - 1.0)
+// (exp(1.0) REAL(4)); L <= 0 leaves the zone empty.
+__global__ void k_relax_init(DevParams P, Buffers B, RelaxArgs R, int n_total, double L) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_total) return;
+  const double2 x = B.g_pos[g];
+  R.init_pos[g] = x;
+  R.an_pos[g] = x;
+  R.an_vel[g] = B.g_vel[g];
+  R.an_rho[g] = B.g_rho[g];
+  R.an_p[g] = B.g_p[g];
+  int in = 0;
+  double c = 0.0;
+  if (g < B.counters[C_NFLUID] && L > 0.0 && x.x <= L) {
+    in = 1;
+    c = (1 - (exp(pow(x.x / L, 3.5)) - 1.0) / (P.e32 - 1.0));
+  }
+  R.in_zone[g] = in;
+  R.coef[g] = c;
+}
+
+// Wavemaker_Subroutine.f90:268-276: fluid below y = 0 typed 120.
+__global__ void k_wt_ill(Buffers B) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= B.counters[C_NFLUID] || B.g_type[g] == 120) return;
+  if (B.g_pos[g].y < 0.0) B.g_type[g] = 120;
+}
+
+// :300-346 (Stokes_2nd) and :354-396 (Airy): wavemaker particle q (gid N + q) from wave
+// theory at its current position, around its initial position.
+__global__ void k_wt_wavemaker(DevParams P, Buffers B, RelaxArgs R, WaveArgs W, int n_wm) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_wm) return;
+  const int i = B.counters[C_NFLUID] + q;
+  const double pi = 3.14159265358979324;
+  const double soft = W.soft, H = W.H, T = W.T, L = W.L, A = W.A, k = W.k, d = W.d, g = W.g;
+  const double skd = W.skd, ckd = W.ckd;
+  const double2 x0 = R.init_pos[i];
+  const double2 xc = B.g_pos[i];
+  const double ang = k * xc.x - W.om * W.t + W.phase;
+  const double yo = xc.y;
+  double el, u, v;
+  double2 xn;
+  if (W.wave_type == 1) {
+    el = soft * (0.5 * H * cos(ang) + pi * H * H / (8 * L) * ckd / (skd * skd * skd) * (2 + cosh(2 * k * d)) * cos(2 * ang));
+    u = pi * H / T * cosh(k * yo) * cos(ang) / skd + 3.0 / 4.0 * (pi * pi) * H / T * (H / L) * cosh(2 * k * yo) * cos(2 * ang) / (skd * skd * skd * skd);
+    v = pi * H / T * sinh(k * yo) * sin(ang) / skd + 3.0 / 4.0 * (pi * pi) * H / T * (H / L) * sinh(2 * k * yo) * sin(2 * ang) / (skd * skd * skd * skd);
+    xn.x = x0.x - soft * (A * cosh(k * yo) * sin(ang) / skd + 3.0 * pi * (H * H) / (16 * L) * cosh(2 * k * yo) * sin(2 * ang) / (skd * skd * skd * skd));
+    xn.y = x0.y + soft * (A * sinh(k * yo) * cos(ang) / skd + 3.0 * pi * (H * H) / (16 * L) * sinh(2 * k * yo) * cos(2 * ang) / (skd * skd * skd * skd));
+  } else {
+    el = soft * 0.5 * H * cos(ang);
+    u = pi * H / T * cosh(k * yo) * cos(ang) / skd;
+    v = pi * H / T * sinh(k * yo) * sin(ang) / skd;
+    xn.x = x0.x - soft * A * cosh(k * yo) * sin(ang) / skd;
+    xn.y = x0.y + soft * A * sinh(k * yo) * cos(ang) / skd;
+  }
+  B.g_vel[i] = make_double2(soft * u, soft * v);
+  B.g_pos[i] = xn;
+  const double y = xn.y;
+  if (y <= el + d) {
+    double pr;
+    if (W.wave_type == 1)
+      pr = P.rho0 * g * (el + d - y) + soft * (P.rho0 * g * H / 2.0 * cosh(k * y) * cos(ang) / ckd + 3.0 / 4.0 * P.rho0 * g * H * (pi * H / L) * (1.0 / sinh(2 * k * d)) * (cosh(2 * k * y) / (skd * skd) - (double)(1.0f / 3.0f)) * cos(2 * ang) - 1.0 / 4.0 * P.rho0 * g * H * (pi * H / L) * cosh(2 * k * y) / sinh(2 * k * d));
+    else
+      pr = P.rho0 * g * (d + el - y) + soft * (P.rho0 * g * H / 2.0 * cosh(k * y) * cos(ang) / ckd);
+    B.g_p[i] = pr;
+    B.g_rho[i] = inv_tait(P, pr);
+    B.g_mass[i] = P.rho0 * P.vol0;
+  } else {
+    B.g_p[i] = 0.0; B.g_rho[i] = P.rho0; B.g_mass[i] = 0.0;
+  }
+  B.g_c[i] = P.sqrt_c0sq;
+}
+
+// :617-700: the zone's analytic fields, from the previous analytic position.
+__global__ void k_wt_relax_fields(DevParams P, Buffers B, RelaxArgs R, WaveArgs W) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B.counters[C_NFLUID] || B.g_type[i] == 120 || R.in_zone[i] != 1) return;
+  const double pi = 3.14159265358979324;
+  const double soft = W.soft, H = W.H, T = W.T, L = W.L, A = W.A, k = W.k, d = W.d, g = W.g;
+  const double skd = W.skd, ckd = W.ckd;
+  const double2 x0 = R.init_pos[i];
+  const double2 pc = R.an_pos[i];
+  const double px = pc.x, py = pc.y;
+  const double ang = k * px - W.om * W.t + W.phase;
+  double el, u, v;
+  double2 xn;
+  if (W.wave_type == 1) {
+    el = soft * (0.5 * H * cos(ang) + pi * H * H * (1.0 / tanh(k * d)) * (1 + 3.0 / (2 * (skd * skd * skd)) * cos(2 * ang)));
+    u = pi * H / T * cosh(k * py) * cos(ang) / skd - 3.0 / 4.0 * (pi * pi) * H / T * (H / L) * cosh(2 * k * py) * cos(2 * ang) / (skd * skd * skd * skd);
+    v = pi * H / T * sinh(k * py) * sin(ang) / skd - 3.0 / 4.0 * (pi * pi) * H / T * (H / L) * sinh(2 * k * py) * sin(2 * ang) / (skd * skd * skd * skd);
+    xn.x = x0.x - soft * (A * cosh(k * py) * sin(ang) / skd + 3.0 * pi * (H * H) / (16 * L) * cosh(2 * k * py) * sin(2 * ang) / (skd * skd * skd * skd));
+    xn.y = x0.y + soft * (A * sinh(k * py) * cos(ang) / skd + 3.0 * pi * (H * H) / (16 * L) * sinh(2 * k * py) * cos(2 * ang) / (skd * skd * skd * skd));
+  } else {
+    el = soft * 0.5 * H * cos(ang);
+    u = pi * H / T * cosh(k * py) * cos(ang) / skd;
+    v = pi * H / T * sinh(k * py) * sin(ang) / skd;
+    xn.x = x0.x - soft * A * cosh(k * py) * sin(ang) / skd;
+    xn.y = x0.y + soft * A * sinh(k * py) * cos(ang) / skd;
+  }
+  R.an_vel[i] = make_double2(soft * u, soft * v);
+  R.an_pos[i] = xn;
+  const double y = xn.y;
+  if (y <= el + d) {
+    double pr;
+    if (W.wave_type == 1)
+      pr = P.rho0 * g * (el + d - y) + soft * (P.rho0 * g * H / 2.0 * cosh(k * y) * cos(ang) / ckd + 3.0 / 4.0 * P.rho0 * g * H * (pi * H / L) * (1.0 / sinh(2 * k * d)) * (cosh(2 * k * y) / (skd * skd) - (double)(1.0f / 3.0f)) * cos(2 * ang) - 1.0 / 4.0 * P.rho0 * g * H * (pi * H / L) * cosh(2 * k * y) / sinh(2 * k * d));
+    else
+      pr = P.rho0 * g * (el + d - y) + soft * (P.rho0 * g * H / 2.0 * cosh(k * y) * cos(ang) / ckd);
+    R.an_p[i] = pr;
+    R.an_rho[i] = inv_tait(P, pr);
+  } else {
+    R.an_p[i] = 0.0;
+    R.an_rho[i] = P.rho0;
+  }
+}
+
+// wave_maker_damping_zone.f90:61-95 on local slot s after the corrector: velocity,
+// position and pressure blended toward the analytic fields, rho from the inverse Tait law.
+__global__ void k_relax_blend(DevParams P, Buffers B, RelaxArgs R) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.counters[C_NLOCAL]) return;
+  const int g = B.name[s];
+  if (g < 0 || g >= B.counters[C_NFLUID] || R.in_zone[g] != 1) return;
+  const double f = R.coef[g];
+  const double2 av = R.an_vel[g], ax = R.an_pos[g];
+  double2 v = B.l_vel[s], x = B.l_pos[s];
+  v.x = f * av.x + (1 - f) * v.x;
+  v.y = f * av.y + (1 - f) * v.y;
+  x.x = f * ax.x + (1 - f) * x.x;
+  x.y = f * ax.y + (1 - f) * x.y;
+  B.l_vel[s] = v;
+  B.l_pos[s] = x;
+  double2 pc = B.l_pc[s];
+  pc.x = f * R.an_p[g] + (1 - f) * pc.x;
+  B.l_pc[s] = pc;
+  B.l_rv[s] = make_double2(inv_tait(P, pc.x), B.l_rv[s].y);
+}
+
+void launch_relax_init(hipStream_t s, const DevParams& P, Buffers& B, const RelaxArgs& R, int n_total,
+                       double zone_length) {
+  hipLaunchKernelGGL(k_relax_init, dim3(grid_for(n_total)), dim3(kBlock), 0, s, P, B, R, n_total, zone_length);
+}
+
+void launch_wave_theory(hipStream_t s, const DevParams& P, Buffers& B, const RelaxArgs& R, const WaveArgs& W,
+                        int cap_fluid, int n_wm, int relax_on) {
+  hipLaunchKernelGGL(k_wt_ill, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, s, B);
+  if (!(W.t >= 0.0)) return;
+  if (n_wm > 0) hipLaunchKernelGGL(k_wt_wavemaker, dim3(grid_for(n_wm)), dim3(kBlock), 0, s, P, B, R, W, n_wm);
+  if (relax_on) hipLaunchKernelGGL(k_wt_relax_fields, dim3(grid_for(cap_fluid)), dim3(kBlock), 0, s, P, B, R, W);
+}
+
+void launch_relax_blend(hipStream_t s, const DevParams& P, Buffers& B, const RelaxArgs& R, int cap_local) {
+  hipLaunchKernelGGL(k_relax_blend, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B, R);
+}
+
 // Assign_boundary_particles.f90:157-299 before an evaluation's passes: each fixed
 // ghost takes the pressure and velocity of its non-ghost candidates (the 3x3 cells)
 // within 3 hbar, hbar = 1.2 (h_i + h_j) / 2 with 1.2 a REAL(4) literal, by the

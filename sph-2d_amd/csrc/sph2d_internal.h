@@ -242,6 +242,25 @@ void launch_wavemaker_pull_back(hipStream_t s, const DevParams& P, Buffers& B, i
 void launch_wavemaker_place(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid, int n_wm, double xp,
                             double up, unsigned long long* height_bits);
 // Sampling_subroutine.f90 at sampling points (sph2d_fp64.hip)
+// The wavemaker damping (relaxation) zone and the wave-theory wavemaker: per-gid arrays
+// (In_WaveMakerDampingZone_OrNot, position_soft_coefficient, WaveMaker_Analytical_*,
+// Initial_particle_Position) and one step's wave quantities.
+struct RelaxArgs {
+  int* in_zone;
+  double* coef;
+  double2 *an_pos, *an_vel, *init_pos;
+  double *an_p, *an_rho;
+};
+struct WaveArgs {
+  int wave_type;                          // 1 Stokes_2nd, 2 Airy
+  double soft, t, g, d, H, k, om, phase, T, L, A, skd, ckd;
+};
+void launch_wave_theory(hipStream_t s, const DevParams& P, Buffers& B, const RelaxArgs& R, const WaveArgs& W,
+                        int cap_fluid, int n_wm, int relax_on);
+void launch_relax_blend(hipStream_t s, const DevParams& P, Buffers& B, const RelaxArgs& R, int cap_local);
+void launch_relax_init(hipStream_t s, const DevParams& P, Buffers& B, const RelaxArgs& R, int n_total,
+                       double zone_length);
+
 struct ProbeArgs {
   int n;
   double gravity;                         // g when Gravity_Switch is on, else 0

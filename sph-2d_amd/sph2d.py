@@ -154,6 +154,7 @@ EXPORTS = (
     "sph2d_sampling_line_points", "sph2d_write_sampling_line_positions", "sph2d_write_sampling_line_results",
     "sph2d_wavemaker_motion", "sph2d_wavemaker_pull_back", "sph2d_wavemaker_update", "sph2d_batch_create", "sph2d_batch_run",
     "sph2d_batch_info", "sph2d_batch_destroy", "sph2d_batch_autotune", "sph2d_forward", "sph2d_backward",
+    "sph2d_relaxation_init", "sph2d_wave_theory_update", "sph2d_wave_maker_damping_zone",
 )
 
 _lib = None
@@ -251,6 +252,9 @@ def lib():
             "sph2d_download_rates": [vp, i32, dp, dp, i32],
             "sph2d_forward": [vp, P(AdjointCfg), dp, i32, i32, dp],
             "sph2d_backward": [vp, dp, dp],
+            "sph2d_relaxation_init": [vp, ctypes.c_double],
+            "sph2d_wave_theory_update": [vp, P(Wavemaker), i32],
+            "sph2d_wave_maker_damping_zone": [vp, i32],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -391,6 +395,16 @@ class Simulation:
         _check(lib().sph2d_single_step_compute_vjp(self._c, n, _dp(ld), _dp(la), _dp(lg) if lg is not None else None,
                                                    n, _dp(gp), _dp(gv), _dp(gr)))
         return {"x": gp[:n], "y": gp[n:], "u": gv[:n], "v": gv[n:], "rho": gr}
+
+    # --- the wave-theory wavemaker and its damping zone ---------------------------------
+    def relaxation_init(self, zone_length: float):
+        _check(lib().sph2d_relaxation_init(self._c, zone_length))
+
+    def wave_theory_update(self, w: "Wavemaker", i_time_step: int):
+        _check(lib().sph2d_wave_theory_update(self._c, ctypes.byref(w), i_time_step))
+
+    def wave_maker_damping_zone(self, i_time_step: int):
+        _check(lib().sph2d_wave_maker_damping_zone(self._c, i_time_step))
 
     # --- the differentiable wave tank ------------------------------------------------
     def forward(self, cfg: AdjointCfg, a, first_step: int, n_steps: int):
