@@ -105,6 +105,12 @@ typedef struct {
                                 0: their Delta forms without the SPS stress, e.g.
                                 Solver_Normalized_Modified_Linear_Riemann_Delta.f90, whose XSPH term
                                 is epsilon m_j v_ij / (rho_i + rho_j) W instead of 2 epsilon (...) */
+  int32_t renormalization;   /* 0: the reference's identity in place of the dyadic tensor's inverse
+                                (Define_freesurface_and_Normailized_derivates.f90:168-188); 1: the
+                                2x2 inverse itself, T_i = sum_j (x_j - x_i) (x) grad_i W V_j, used
+                                for every normalised kernel gradient (FP64 form; singular tensors
+                                keep the identity and are counted in n_singular). No reference
+                                counterpart: the reference computes the inverse and discards it. */
 } sph2d_params;
 
 typedef struct sph2d_ctx sph2d_ctx;
@@ -116,7 +122,8 @@ typedef struct {
   int32_t n_local;           /* total_particle_number_in_subdomain */
   int32_t n_ill;             /* particles typed 120 */
   int32_t n_surface;         /* subdomain_surface_particle_number after the geometric test */
-  int32_t n_singular;        /* singular dyadic tensors (matrix_inversion_less_three check_error=0) */
+  int32_t n_singular;        /* singular dyadic tensors (matrix_inversion_less_three check_error=0) of
+                                fluid particles; counted when renormalization = 1 forms the tensor */
   int32_t n_mls_fallback;    /* MLS points that fell back to Shepard */
   int32_t max_cell_count;    /* largest background-cell population */
   int64_t n_pairs;           /* pair_number of the last force evaluation (counted on request) */
@@ -440,6 +447,15 @@ int sph2d_forward(sph2d_ctx* ctx, const sph2d_adjoint_cfg* cfg, const double* a,
  * height's arg-max particle, which walls and wavemaker particles carry mass, and the SPS
  * term's particle_mass read by local index. */
 int sph2d_backward(sph2d_ctx* ctx, const double* dl_dprobe, double* dl_da);
+
+/* --- device memory through this library's HIP runtime ------------------------------
+ * For callers that pass device arrays (sph2d_reorder_particle_index and its no-allocation
+ * form) but hold no HIP runtime binding of their own (Fortran, ctypes): hipMalloc,
+ * hipFree and a synchronous hipMemcpy (to_device 1: host to device, 0: device to host)
+ * of the runtime this library is linked against. No reference counterpart. */
+int sph2d_device_alloc(size_t bytes, void** out);
+int sph2d_device_free(void* p);
+int sph2d_device_copy(void* dst, const void* src, size_t bytes, int32_t to_device);
 
 /* Host-array writers (no device; Fortran layout, ld >= n). Text follows the
  * reference's edit descriptors exactly; files named "appended" get the

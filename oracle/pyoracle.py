@@ -41,7 +41,7 @@ class OrcParams(ctypes.Structure):
         ("damping_amplitude", ctypes.c_double), ("water_depth", ctypes.c_double),
         ("walls", ctypes.c_int32), ("gravity", ctypes.c_double),
         ("wavemaker_layers", ctypes.c_int32),
-        ("sps_on", ctypes.c_int32),
+        ("sps_on", ctypes.c_int32), ("renorm", ctypes.c_int32),
     ]
 
 
@@ -90,7 +90,7 @@ def lib():
         L.orc_relaxation_blend.argtypes = [ctypes.c_void_p]
         L.orc_initial_ke.argtypes = [ctypes.c_void_p]
         L.orc_initial_ke.restype = ctypes.c_double
-        for name in ("orc_n_fluid", "orc_n_global", "orc_n_local"):
+        for name in ("orc_n_fluid", "orc_n_global", "orc_n_local", "orc_n_singular"):
             getattr(L, name).argtypes = [ctypes.c_void_p]
             getattr(L, name).restype = ctypes.c_int32
         L.orc_pair_number.argtypes = [ctypes.c_void_p]
@@ -250,6 +250,10 @@ class Oracle:
     @property
     def n_global(self):
         return lib().orc_n_global(self._c)
+
+    @property
+    def n_singular(self):
+        return lib().orc_n_singular(self._c)
 
     @property
     def n_local(self):

@@ -883,8 +883,10 @@ static void define_freesurface(orc_ctx* c) {
     int ok = orc_matrix_inversion_less_three(2, A, Ainv);
     if (!ok) c->n_singular++;
     /* check_error==1 -> the caller replaces the inverse by the identity (:180-186);
-     * check_error==0 keeps diag=1 and the previous (identity) off-diagonals. */
+     * check_error==0 keeps diag=1 and the previous (identity) off-diagonals. With
+     * renorm (a mode of this library, no reference counterpart) the inverse is kept. */
     Ti[0] = 1.0; Ti[1] = 0.0; Ti[2] = 0.0; Ti[3] = 1.0;
+    if (c->P.renorm && ok) { Ti[0] = Ainv[0]; Ti[1] = Ainv[1]; Ti[2] = Ainv[2]; Ti[3] = Ainv[3]; }
   }
   for (int L = 0; L < ns; ++L) {                                  /* :295-376 */
     int i = slist[L];
@@ -1832,6 +1834,7 @@ double orc_initial_ke(const orc_ctx* c) { return c->ke0; }
 int32_t orc_n_fluid(const orc_ctx* c) { return c->n_fluid; }
 int32_t orc_n_global(const orc_ctx* c) { return c->n_fluid + c->n_ghost; }
 int32_t orc_n_local(const orc_ctx* c) { return c->n_local; }
+int32_t orc_n_singular(const orc_ctx* c) { return c->n_singular; }
 int64_t orc_pair_number(const orc_ctx* c) { return c->npair; }
 
 /* ------------------------------------------------------------------ */

@@ -55,6 +55,7 @@ typedef struct {
   /* PistonType): columns of type-130 particles left of x = 0, before the walls    */
   int32_t wavemaker_layers;
   int32_t sps_on;            /* 1: _SPS solvers; 0: Delta solvers without SPS (XSPH factor 1) */
+  int32_t renorm;            /* 0: identity for the dyadic inverse (the reference); 1: the inverse */
 } orc_params;
 
 enum { ORC_SOLVER_MODIFIED_LINEAR_RIEMANN = 0,   /* Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90 */
@@ -132,6 +133,7 @@ double orc_initial_ke(const orc_ctx* c);
 int32_t orc_n_fluid(const orc_ctx* c);
 int32_t orc_n_global(const orc_ctx* c);      /* fluid + periodic ghosts */
 int32_t orc_n_local(const orc_ctx* c);       /* subdomain particles */
+int32_t orc_n_singular(const orc_ctx* c);    /* singular dyadic tensors of fluid particles, last evaluation */
 int64_t orc_pair_number(const orc_ctx* c);
 
 /* Named array access: returns a pointer into the context (valid until the next call

@@ -190,6 +190,7 @@ void make_devparams(const sph2d_params& p, DevParams& P) {
   P.write_debug = 0;
   P.mixed = p.precision == SPH2D_FP32_MIXED ? 1 : 0;
   P.sps = p.sps_on;
+  P.renorm = p.renormalization;
   P.xsph_f = p.sps_on ? 2.0 : 1.0;
 }
 
@@ -222,6 +223,10 @@ int check_params(const sph2d_params* p) {
   if (p->damping < SPH2D_DAMPING_OFF || p->damping > SPH2D_DAMPING_LINEAR)
     return fail(SPH2D_ERR_ARGUMENT, "damping must be one of SPH2D_DAMPING_*");
   if (p->sps_on != 0 && p->sps_on != 1) return fail(SPH2D_ERR_ARGUMENT, "sps_on must be 0 or 1");
+  if (p->renormalization != 0 && p->renormalization != 1)
+    return fail(SPH2D_ERR_ARGUMENT, "renormalization must be 0 or 1");
+  if (p->renormalization && (p->precision != SPH2D_FP64 || p->solver == SPH2D_SOLVER_MUSCL))
+    return fail(SPH2D_ERR_UNSUPPORTED, "the 2x2 renormalisation is built for the FP64 form without MUSCL");
   if (p->damping != SPH2D_DAMPING_OFF && (!(p->damping_length > 0) || !(p->water_depth > 0)))
     return fail(SPH2D_ERR_ARGUMENT, "damping_length and water_depth must be positive");
   return SPH2D_OK;
@@ -315,6 +320,7 @@ int sph2d_default_params(sph2d_params* p) {
   p->gravity = 0.0;                                // Gravity_Switch = 'off'
   p->wavemaker_layers = 0;                         // make_wave_or_not = 0
   p->sps_on = 1;                                   // Solver_..._SPS (single_step_compute.f90:137)
+  p->renormalization = 0;                          // the identity (:168-188)
   return SPH2D_OK;
 }
 
@@ -367,6 +373,7 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   rc |= c->alloc(&B.fra_v, L); rc |= c->alloc(&B.sdw, L); rc |= c->alloc(&B.grad, L);
   rc |= c->alloc(&B.nrm, L); rc |= c->alloc(&B.shift, L); rc |= c->alloc(&B.avg, L);
   rc |= c->alloc(&B.tau, 3 * L); rc |= c->alloc(&B.surf, L); rc |= c->alloc(&B.near_, L);
+  if (p->renormalization) rc |= c->alloc(&B.tinv, 4 * L);
   rc |= c->alloc(&B.nb_list, (size_t)kNbListWords * L);
   rc |= c->alloc(&B.nb_count, L);
   rc |= c->alloc(&B.drhodt, L); rc |= c->alloc(&B.acc, L);
@@ -550,7 +557,8 @@ int sph2d_upload_state(sph2d_ctx* c, int32_t n, const double* pos, const double*
       if (t[i] != 2 && t[i] != 120)
         return fail(SPH2D_ERR_UNSUPPORTED, "only fluid (2) and ill (120) particle types are built");
     }
-    if (!std::isfinite(pos[i]) || !std::isfinite(pos[ld + i]) || !std::isfinite(rho[i]) || !(rho[i] > 0))
+    if (!std::isfinite(pos[i]) || !std::isfinite(pos[ld + i]) || !std::isfinite(vel[i]) ||
+        !std::isfinite(vel[ld + i]) || !std::isfinite(mass[i]) || !std::isfinite(rho[i]) || !(rho[i] > 0))
       return fail(SPH2D_ERR_NONFINITE, "non-finite or non-positive state at particle " + std::to_string(i));
   }
   Buffers& B = c->B;
@@ -1036,6 +1044,7 @@ int sph2d_single_step_compute_vjp(sph2d_ctx* c, int32_t n, const double* lam_drh
     return fail(SPH2D_ERR_STATE, "the reverse evaluation needs sph2d_single_step_compute as the context's last "
                                  "state-changing call");
   if (c->P.mixed) return fail(SPH2D_ERR_UNSUPPORTED, "the reverse evaluation is built for the FP64 form");
+  if (c->P.renorm) return fail(SPH2D_ERR_UNSUPPORTED, "the reverse evaluation is built for the identity renormalisation");
   if (!c->P.sps) return fail(SPH2D_ERR_UNSUPPORTED, "the reverse evaluation is built for the _SPS solvers");
   if (c->P.solver != SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && c->P.solver != SPH2D_SOLVER_LINEAR_RIEMANN)
     return fail(SPH2D_ERR_UNSUPPORTED, "the reverse evaluation is built for the two linear Riemann solvers");
@@ -1970,7 +1979,8 @@ bool fusable(const std::vector<sph2d_ctx*>& v) {
   for (const sph2d_ctx* c : v) {
     const sph2d_params& a = c->prm;
     if (a.integrator != SPH2D_INTEGRATOR_PC || a.precision != z.precision || a.solver != z.solver ||
-        a.shift_on != z.shift_on || a.epsilon_xsph != 0.0 || a.damping != SPH2D_DAMPING_OFF || a.walls)
+        a.shift_on != z.shift_on || a.epsilon_xsph != 0.0 || a.damping != SPH2D_DAMPING_OFF || a.walls ||
+        a.renormalization)
       return false;
   }
   return true;
@@ -2335,6 +2345,7 @@ int sph2d_forward(sph2d_ctx* c, const sph2d_adjoint_cfg* cfg, const double* a, i
     return fail(SPH2D_ERR_ARGUMENT, "sample_every, checkpoint_every and standing_steps must be >= 1");
   if (first < 2 || nsteps < 1) return fail(SPH2D_ERR_ARGUMENT, "the window starts at step 2 or later, nsteps >= 1");
   if (c->P.mixed) return fail(SPH2D_ERR_UNSUPPORTED, "the adjoint runs in the FP64 form");
+  if (c->P.renorm) return fail(SPH2D_ERR_UNSUPPORTED, "the adjoint is built for the identity renormalisation");
   if (c->prm.integrator != SPH2D_INTEGRATOR_PC) return fail(SPH2D_ERR_UNSUPPORTED, "the adjoint needs the predictor-corrector");
   if (!c->prm.walls || c->prm.periodic || !c->prm.wavemaker_layers)
     return fail(SPH2D_ERR_UNSUPPORTED, "the adjoint needs a walled tank with wavemaker particles");
@@ -2484,5 +2495,27 @@ int sph2d_backward(sph2d_ctx* c, const double* dl_dprobe, double* dl_da) {
   }
   c->vjp_ready = false;
   c->eval_part = 0;
+  return SPH2D_OK;
+}
+
+// ---- device memory through this library's HIP runtime ------------------------------
+int sph2d_device_alloc(size_t bytes, void** out) {
+  if (!out) return fail(SPH2D_ERR_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  const hipError_t e = hipMalloc(out, std::max<size_t>(bytes, 1));
+  if (e != hipSuccess) return fail(SPH2D_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return SPH2D_OK;
+}
+
+int sph2d_device_free(void* p) {
+  const hipError_t e = hipFree(p);
+  if (e != hipSuccess) return fail(SPH2D_ERR_HIP, std::string("hipFree: ") + hipGetErrorString(e));
+  return SPH2D_OK;
+}
+
+int sph2d_device_copy(void* dst, const void* src, size_t bytes, int32_t to_device) {
+  if (bytes && (!dst || !src)) return fail(SPH2D_ERR_ARGUMENT, "NULL pointer");
+  const hipError_t e = hipMemcpy(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail(SPH2D_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
   return SPH2D_OK;
 }

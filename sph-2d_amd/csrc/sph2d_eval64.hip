@@ -44,7 +44,12 @@ __device__ __forceinline__ KernelTerm kernel_term(const DevParams& P, double dx,
 // MUSCL also sums grad p, grad u, grad v the same way
 // (Solver_Normalized_MUSCL_Linear_Riemann_Delta_SPS.f90:145-185).
 // ---------------------------------------------------------------------------
-template <bool MUSCL>
+// RENORM (a mode of this library): the dyadic tensor T = sum (x_o - x_s) (x) grad W V_o is
+// formed and inverted (matrix_inversion_less_three; singular: the identity, counted),
+// and the strain and grad rho use T^-1 grad W. T^-1 is not known inside the walk, so the
+// walk sums the raw gradients' outer products and T^-1 is applied after it (the same
+// sums in exact arithmetic, rounded differently).
+template <bool MUSCL, bool RENORM = false>
 __device__ __forceinline__ void k_eval_a_body(const DevParams& P, Buffers& B, int blk_, int nblk_) {
   __shared__ NbStage st;
   __shared__ float2 lds32[kStageCap];
@@ -60,6 +65,7 @@ __device__ __forceinline__ void k_eval_a_body(const DevParams& P, Buffers& B, in
   const int ts = B.l_type[s];
   double fra = 0.0, sdx = 0.0, sdy = 0.0, Sxx = 0.0, Sxy = 0.0, Syy = 0.0, grx = 0.0, gry = 0.0;
   double gpx = 0.0, gpy = 0.0, gux = 0.0, guy = 0.0, gvx = 0.0, gvy = 0.0;
+  double t00 = 0.0, t01 = 0.0, t10 = 0.0, t11 = 0.0, gxx = 0.0, gxy = 0.0, gyx = 0.0, gyy = 0.0;
   for_each_neighbour<kNbRecord>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) __attribute__((always_inline)) {
     if (ts != 2 && B.l_type[o] != 2) return;     // no pair between two non-fluid particles
     const double2 po = B.l_pos[o];
@@ -72,10 +78,21 @@ __device__ __forceinline__ void k_eval_a_body(const DevParams& P, Buffers& B, in
     fra = fra + Vo * P.inv_sv;
     sdx = sdx + K.dwx * Vo;
     sdy = sdy + K.dwy * Vo;
-    const double vdx = vo.x - vs.x, vdy = vo.y - vs.y;   // -(v_i - v_j); identity renormalisation
-    Sxx = Sxx + vdx * K.dwx * Vo;
-    Sxy = Sxy + (vdx * K.dwy + vdy * K.dwx) * Vo;
-    Syy = Syy + vdy * K.dwy * Vo;
+    const double vdx = vo.x - vs.x, vdy = vo.y - vs.y;   // -(v_i - v_j)
+    if (RENORM) {                                  // Define_freesurface...f90:141-159 (x_j - x_i = -dx)
+      t00 = t00 + (-dx) * K.dwx * Vo;
+      t01 = t01 + (-dx) * K.dwy * Vo;
+      t10 = t10 + (-dy) * K.dwx * Vo;
+      t11 = t11 + (-dy) * K.dwy * Vo;
+      gxx = gxx + vdx * K.dwx * Vo;
+      gxy = gxy + vdx * K.dwy * Vo;
+      gyx = gyx + vdy * K.dwx * Vo;
+      gyy = gyy + vdy * K.dwy * Vo;
+    } else {                                       // the identity renormalisation
+      Sxx = Sxx + vdx * K.dwx * Vo;
+      Sxy = Sxy + (vdx * K.dwy + vdy * K.dwx) * Vo;
+      Syy = Syy + vdy * K.dwy * Vo;
+    }
     const double rd = (rvo.x - rhos) * Vo;
     grx = grx + rd * K.dwx;
     gry = gry + rd * K.dwy;
@@ -92,6 +109,28 @@ __device__ __forceinline__ void k_eval_a_body(const DevParams& P, Buffers& B, in
   }, B.nb_list, B.nb_count, B.kstride, &st, lds32, lds64);
   B.fra_v[s] = fra;
   B.sdw[s] = make_double2(sdx, sdy);
+  if (RENORM) {
+    // matrix_inversion_less_three (function_module.f90:157-249) for fluid particles;
+    // a singular tensor keeps the identity. i00 i01 / i10 i11 = T^-1 (rows, columns).
+    double i00 = 1.0, i01 = 0.0, i10 = 0.0, i11 = 1.0;
+    if (ts == 2) {
+      const double det = (t00 * t11 - t01 * t10);
+      if (det != 0) {
+        i00 = t11 / det; i01 = -t01 / det; i10 = -t10 / det; i11 = t00 / det;
+      } else {
+        atomicAdd(&B.counters[C_NSING], 1);
+      }
+    }
+    double* ti = B.tinv + 4 * (size_t)s;
+    ti[0] = i00; ti[1] = i10; ti[2] = i01; ti[3] = i11;   // column-major, as T_inv(i, :, :)
+    // strain and grad rho with T^-1 grad W (Solver...SPS.f90:144-182, :508-526)
+    Sxx = i00 * gxx + i01 * gxy;
+    Syy = i10 * gyx + i11 * gyy;
+    Sxy = (i10 * gxx + i11 * gxy) + (i00 * gyx + i01 * gyy);
+    const double rx = grx, ry = gry;
+    grx = i00 * rx + i01 * ry;
+    gry = i10 * rx + i11 * ry;
+  }
   B.grad[s] = make_double2(grx, gry);
   if (MUSCL) {
     B.mg[3 * s + 0] = make_double2(gpx, gpy);
@@ -120,9 +159,9 @@ __device__ __forceinline__ void k_eval_a_body(const DevParams& P, Buffers& B, in
   if (sf) atomicAdd(&B.counters[C_NSURF], 1);
 }
 
-template <bool MUSCL>
+template <bool MUSCL, bool RENORM>
 __global__ void __launch_bounds__(kBlock) k_eval_a(DevParams P, Buffers B) {
-  k_eval_a_body<MUSCL>(P, B, xcd_block(), gridDim.x);
+  k_eval_a_body<MUSCL, RENORM>(P, B, xcd_block(), gridDim.x);
 }
 
 // Surface normal and geometric test (Define_freesurface...f90:295-376), surface particles only.
@@ -132,8 +171,13 @@ __device__ __forceinline__ void k_eval_b1_body(const DevParams& P, Buffers& B, i
   if (s >= n) return;
   if (B.surf[s] != 1) { B.nrm[s] = make_double2(0.0, 0.0); return; }
   const double2 sd = B.sdw[s];
-  const double n1 = 1.0 * sd.x + 0.0 * sd.y;     // identity inverse: matmul(I, sum_dwdx)
-  const double n2 = 0.0 * sd.x + 1.0 * sd.y;
+  double n1 = 1.0 * sd.x + 0.0 * sd.y;           // identity inverse: matmul(I, sum_dwdx)
+  double n2 = 0.0 * sd.x + 1.0 * sd.y;
+  if (P.renorm) {                                // matmul(T^-1, sum_dwdx)
+    const double* ti = B.tinv + 4 * (size_t)s;
+    n1 = ti[0] * sd.x + ti[2] * sd.y;
+    n2 = ti[1] * sd.x + ti[3] * sd.y;
+  }
   double len = 0.0;
   len = len + n1 * n1;
   len = len + n2 * n2;
@@ -224,6 +268,13 @@ __device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int
   double drho = 0.0, corr = 0.0, ax = 0.0, ay = 0.0, shx = 0.0, shy = 0.0, agx = 0.0, agy = 0.0;
   double iax = 0.0, iay = 0.0, avx = 0.0, avy = 0.0, aix = 0.0, aiy = 0.0;
   double spx = 0.0, spy = 0.0, lpx = 0.0, lpy = 0.0;
+  // the 2x2 renormalisation (a mode of this library): every normalised gradient below is
+  // T^-1 grad W of the summing particle; the identity otherwise
+  double i00 = 1.0, i01 = 0.0, i10 = 0.0, i11 = 1.0;
+  if (P.renorm) {
+    const double* ti = B.tinv + 4 * (size_t)s;
+    i00 = ti[0]; i10 = ti[1]; i01 = ti[2]; i11 = ti[3];
+  }
   for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) __attribute__((always_inline)) {
     if (ts != 2 && B.l_type[o] != 2) return;
     const double2 po = B.l_pos[o];
@@ -233,6 +284,11 @@ __device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int
     const double inv_r = 1.0 / r;
     const double inv_r2 = inv_r * inv_r;
     const KernelTerm K = kernel_term(P, dx, dy, r, inv_r);
+    double ndx = K.dwx, ndy = K.dwy;                  // Normalized_dwdx of the summing particle
+    if (P.renorm) {
+      ndx = i00 * K.dwx + i01 * K.dwy;
+      ndy = i10 * K.dwx + i11 * K.dwy;
+    }
     const double2 vo = B.l_vel[o];
     const double2 rvo = B.l_rv[o];
     const double2 pco = B.l_pc[o];
@@ -305,12 +361,12 @@ __device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int
       Ps = Pbar + 0.5 * bR * rbar * dU;
     }
     if (nes) { vsx = vbx; vsy = vby; Ps = Pbar; }  // near the free surface: averages
-    const double ti = (vown.x - vsx) * K.dwx + (vown.y - vsy) * K.dwy;
+    const double ti = (vown.x - vsx) * ndx + (vown.y - vsy) * ndy;
     drho = drho + 2 * rhos * Vo * ti;                        // :360-369
     const double rij = inv_rhos / rhoo;                      // 1/(rho_i rho_j)
     const double mo = B.l_mass[o];
     const double fp = mo * (-2 * Ps * rij);                  // :374-388
-    double tix = fp * K.dwx, tiy = fp * K.dwy;
+    double tix = fp * ndx, tiy = fp * ndy;
     const double vijx = vs.x - vo.x, vijy = vs.y - vo.y;
     const double vx = vijx * dx + vijy * dy;
     double tvx = 0.0, tvy = 0.0;
@@ -318,14 +374,14 @@ __device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int
       const double fh = P.fai * P.h;
       const double phi = P.h * vx / (r2 + fh * fh);
       const double Pi = (-(P.alpha * cbar * phi) + P.beta * (phi * phi)) * inv_rbar;
-      tvx = mo * (-Pi * K.dwx);
-      tvy = mo * (-Pi * K.dwy);
+      tvx = mo * (-Pi * ndx);
+      tvy = mo * (-Pi * ndy);
     }
-    const double Rd = dx * K.dwx + dy * K.dwy;
+    const double Rd = dx * ndx + dy * ndy;
     const double lf = Vo * 2 * P.nu * Rd * inv_r2;           // :432-436
     const double tlx = lf * vijx, tly = lf * vijy;
     const double tmp = P.delta_alpha * P.h * cbar * vx * inv_r2 * P.rho0 * inv_rhos * Vo;   // :442-446
-    const double tax = tmp * K.dwx, tay = tmp * K.dwy;
+    const double tax = tmp * ndx, tay = tmp * ndy;
     // SPS (:450-492): raw dwdx, particle_mass at the LOCAL index, temp_p carried from x to y
     const double gmo = B.g_mass[o];
     const double txo = B.tau[3 * o], tyo = B.tau[3 * o + 1], tzo = B.tau[3 * o + 2];
@@ -355,8 +411,8 @@ __device__ __forceinline__ void eval_c(const DevParams& P, const Buffers& B, int
     if (do_shift) {                                           // Shift_particle_position.f90:78-103
       const double wr = K.w * P.inv_wdx;
       const double f = P.shift_beta * P.h2 * (1 + P.shift_R * (wr * wr * wr * wr)) * Vo;
-      shx = shx - f * K.dwx;
-      shy = shy - f * K.dwy;
+      shx = shx - f * ndx;
+      shy = shy - f * ndy;
     }
   }, B.nb_list, B.nb_count, B.kstride);
   if (ts == 2) drho = drho + corr;
@@ -401,8 +457,9 @@ static void launch_c64(hipStream_t s, const DevParams& P, Buffers& B, int g) {
 }
 
 static void launch_a64(hipStream_t s, const DevParams& P, Buffers& B, int g) {
-  if (P.solver == SPH2D_SOLVER_MUSCL) hipLaunchKernelGGL(k_eval_a<true>, dim3(g), dim3(kBlock), 0, s, P, B);
-  else hipLaunchKernelGGL(k_eval_a<false>, dim3(g), dim3(kBlock), 0, s, P, B);
+  if (P.solver == SPH2D_SOLVER_MUSCL) hipLaunchKernelGGL((k_eval_a<true, false>), dim3(g), dim3(kBlock), 0, s, P, B);
+  else if (P.renorm) hipLaunchKernelGGL((k_eval_a<false, true>), dim3(g), dim3(kBlock), 0, s, P, B);
+  else hipLaunchKernelGGL((k_eval_a<false, false>), dim3(g), dim3(kBlock), 0, s, P, B);
 }
 
 void launch_eval_part(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int part) {
@@ -465,6 +522,11 @@ __global__ void __launch_bounds__(kBlock) k_vorticity(DevParams P, Buffers B) {
   const double2 ps = B.l_pos[s];
   const double2 vs = B.l_vel[s];
   double wz = 0.0;
+  double i00 = 1.0, i01 = 0.0, i10 = 0.0, i11 = 1.0;
+  if (P.renorm) {
+    const double* ti = B.tinv + 4 * (size_t)s;
+    i00 = ti[0]; i10 = ti[1]; i01 = ti[2]; i11 = ti[3];
+  }
   for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) __attribute__((always_inline)) {
     if (ts != 2 && B.l_type[o] != 2) return;       // find_particle_pair.f90:76
     const double2 po = B.l_pos[o];
@@ -472,6 +534,11 @@ __global__ void __launch_bounds__(kBlock) k_vorticity(DevParams P, Buffers B) {
     const double r = sqrt(pdx * pdx + pdy * pdy);
     double w, dwx, dwy;
     kernel_w_dw(P, r, pdx, pdy, w, dwx, dwy);
+    if (P.renorm) {                                 // Normalized_dwdx of the summing particle
+      const double rx = dwx, ry = dwy;
+      dwx = i00 * rx + i01 * ry;
+      dwy = i10 * rx + i11 * ry;
+    }
     const double2 vo = B.l_vel[o];
     const double vx = vs.x - vo.x, vy = vs.y - vo.y;
     wz = wz + B.l_rv[o].y * (vx * dwy - vy * dwx);   // Cross_Product(...)(3), function_module.f90:150

@@ -394,11 +394,12 @@ __device__ __forceinline__ void k_cell_key_body(const DevParams& P, Buffers& B, 
       const int yn = (int)floor((p.y - P.chain_oy) / P.chain_d + 1);
       if (xn < 1 || xn > P.chain_nx || yn < 1 || yn > P.chain_ny) {
         B.g_type[g] = 120;
-        atomicAdd(&B.counters[C_NILL], 1);
+        t = 120;
       } else {
         key = (xn - 1) * P.chain_ny + (yn - 1);
       }
     }
+    if (t == 120 && g < N) atomicAdd(&B.counters[C_NILL], 1);   // every fluid particle typed 120
     B.g_key[g] = key;
     B.l_of_g[g] = -1;                              // set again by k_cell_rank for kept particles
   }
@@ -1531,6 +1532,16 @@ void launch_shift_apply_mode(hipStream_t s, const DevParams& P, Buffers& B, int 
   hipLaunchKernelGGL(k_shift_apply, dim3(grid_for(cap_local)), dim3(kBlock), 0, s, P, B, mode);
 }
 
+// Refresh_domain.f90:136-300 on one rank rebuilds the global arrays from the subdomain:
+// a fluid particle left out of it (typed 120 by this step's binning) becomes a zero
+// entry of type 0, which the next binning skips.
+__device__ __forceinline__ void refresh_dropped(Buffers& B, int g) {
+  B.g_pos[g] = make_double2(0.0, 0.0);
+  B.g_vel[g] = make_double2(0.0, 0.0);
+  B.g_rho[g] = 0.0; B.g_p[g] = 0.0; B.g_c[g] = 0.0; B.g_mass[g] = 0.0; B.g_e[g] = 0.0;
+  B.g_type[g] = 0;
+}
+
 // Refresh_domain.f90:136-300 on one rank.
 // Gather form: one thread per fluid gid reads its local slot (l_of_g), so the
 // global arrays are written in order. Ghost gids are not written back: the
@@ -1540,7 +1551,7 @@ __global__ void k_refresh(Buffers B) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= B.counters[C_NFLUID]) return;
   const int s = B.l_of_g[g];
-  if (s < 0) return;
+  if (s < 0) { refresh_dropped(B, g); return; }
   B.g_pos[g] = B.l_pos[s];
   B.g_vel[g] = B.l_vel[s];
   const double2 rv = B.l_rv[s];
@@ -1584,7 +1595,7 @@ __device__ __forceinline__ void k_shift_refresh_body(const DevParams& P, Buffers
   const int g = blk_ * blockDim.x + threadIdx.x;
   if (g >= B.counters[C_NFLUID]) return;
   const int s = B.l_of_g[g];
-  if (s < 0) return;
+  if (s < 0) { refresh_dropped(B, g); return; }
   double2 x = B.l_pos[s];
   const double2 rv = B.l_rv[s];
   double2 pc = B.l_pc[s];
@@ -1616,7 +1627,7 @@ __device__ __forceinline__ void k_correct_refresh_body(const DevParams& P, Buffe
   const int g = blk_ * blockDim.x + threadIdx.x;
   if (g >= B.counters[C_NFLUID]) return;
   const int s = B.l_of_g[g];
-  if (s < 0) return;
+  if (s < 0) { refresh_dropped(B, g); return; }
   const int t = B.l_type[s];
   // The step-start state is still this gid's global state (before_* of the gather).
   const double2 bx = B.g_pos[g], bv = B.g_vel[g];

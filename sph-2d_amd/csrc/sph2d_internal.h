@@ -53,6 +53,7 @@ struct DevParams {
   int mls_order;
   int mixed;                             // 1: FP32-mixed evaluation (mirrors are written)
   int sps;                               // 1: SPS stress (the _SPS solvers); 0: tau = 0
+  int renorm;                            // 1: the dyadic tensor's 2x2 inverse normalises grad W
   double xsph_f;                         // XSPH factor: 2 in the _SPS solvers, 1 without SPS
 };
 
@@ -84,6 +85,7 @@ struct Buffers {
   double *fra_v;
   double2 *sdw, *grad, *nrm, *shift, *avg;
   double *tau;                            // 3 per particle: xx, xy, yy
+  double *tinv;                           // 4 per particle (renormalisation on): T^-1 column-major
   double2 *mg;                            // MUSCL only, 3 per particle: grad p, grad u, grad v
   int *surf, *near_;
   // accepted neighbours of the last pass A: the candidate slots (0..255) in ascending

@@ -35,6 +35,7 @@ module sph2d_interface
     real(c_double) :: gravity
     integer(c_int32_t) :: wavemaker_layers
     integer(c_int32_t) :: sps_on
+    integer(c_int32_t) :: renormalization
   end type sph2d_params
 
   interface

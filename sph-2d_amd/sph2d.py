@@ -67,7 +67,7 @@ class Params(ctypes.Structure):
         ("damping_amplitude", ctypes.c_double), ("water_depth", ctypes.c_double),
         ("walls", ctypes.c_int32), ("gravity", ctypes.c_double),
         ("wavemaker_layers", ctypes.c_int32),
-        ("sps_on", ctypes.c_int32),
+        ("sps_on", ctypes.c_int32), ("renormalization", ctypes.c_int32),
     ]
 
 
@@ -155,6 +155,7 @@ EXPORTS = (
     "sph2d_wavemaker_motion", "sph2d_wavemaker_pull_back", "sph2d_wavemaker_update", "sph2d_batch_create", "sph2d_batch_run",
     "sph2d_batch_info", "sph2d_batch_destroy", "sph2d_batch_autotune", "sph2d_forward", "sph2d_backward",
     "sph2d_relaxation_init", "sph2d_wave_theory_update", "sph2d_wave_maker_damping_zone",
+    "sph2d_device_alloc", "sph2d_device_free", "sph2d_device_copy",
 )
 
 _lib = None
@@ -255,6 +256,9 @@ def lib():
             "sph2d_relaxation_init": [vp, ctypes.c_double],
             "sph2d_wave_theory_update": [vp, P(Wavemaker), i32],
             "sph2d_wave_maker_damping_zone": [vp, i32],
+            "sph2d_device_alloc": [ctypes.c_size_t, P(vp)],
+            "sph2d_device_free": [vp],
+            "sph2d_device_copy": [vp, vp, ctypes.c_size_t, i32],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -655,6 +659,8 @@ _hip = None
 
 
 def _hiprt():
+    """The HIP runtime by name, for device synchronisation in processes that load no
+    other HIP runtime (the bench); device memory goes through DeviceArray."""
     global _hip
     if _hip is None:
         h = ctypes.CDLL("libamdhip64.so")
@@ -666,26 +672,26 @@ def _hiprt():
 
 
 class DeviceArray:
-    """A device copy of a 1-D numpy array (hipMalloc); .get() copies it back."""
+    """A device copy of a 1-D numpy array, allocated and copied through this library's
+    HIP runtime (sph2d_device_alloc/_copy/_free: another HIP runtime loaded into the
+    process, e.g. a framework's bundled one, is a different instance); .get() copies
+    it back."""
 
     def __init__(self, a: np.ndarray):
         self.host = np.ascontiguousarray(a)
         self.ptr = ctypes.c_void_p()
-        h = _hiprt()
-        if h.hipMalloc(ctypes.byref(self.ptr), max(self.host.nbytes, 1)) != 0:
-            raise SphError(3, "hipMalloc failed")
-        if h.hipMemcpy(self.ptr, self.host.ctypes.data, self.host.nbytes, 1) != 0:   # host to device
-            raise SphError(3, "hipMemcpy failed")
+        L = lib()
+        _check(L.sph2d_device_alloc(max(self.host.nbytes, 1), ctypes.byref(self.ptr)))
+        _check(L.sph2d_device_copy(self.ptr, self.host.ctypes.data, self.host.nbytes, 1))
 
     def get(self) -> np.ndarray:
         out = np.empty_like(self.host)
-        if _hiprt().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) != 0:        # device to host
-            raise SphError(3, "hipMemcpy failed")
+        _check(lib().sph2d_device_copy(out.ctypes.data, self.ptr, out.nbytes, 0))
         return out
 
     def free(self):
         if self.ptr:
-            _hiprt().hipFree(self.ptr)
+            lib().sph2d_device_free(self.ptr)
             self.ptr = ctypes.c_void_p()
 
     def __del__(self):
