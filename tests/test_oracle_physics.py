@@ -205,6 +205,31 @@ def test_rotation_equivariance():
         assert np.max(np.abs(fb(f) - sgn * fa(g))) < 1e-10 * (np.max(np.abs(fa(g))) + 1e-300), f
 
 
+def test_sps_carry_scales_as_inverse_square_density():
+    """The weakly compressible equations are invariant under rho0 -> k rho0 (density,
+    mass and pressure scale together; B = c0^2 rho0 / 7): every rate is unchanged,
+    except the carried temp_p of Solver...SPS.f90:458-485, which enters sps_y with one
+    more factor 1/(rho_i rho_j) and so scales as 1/k^2. With sps_y(k) = Y + C / k^2,
+    sps_y(1) - sps_y(10) = C (1 - 1e-2) and sps_y(1) - sps_y(1000) = C (1 - 1e-6). At
+    rho0 = 1 the carry is a few permille of the acceleration: the Taylor-Green case
+    run at rho0 = 1 and at rho0 = 1000 differ by it."""
+    n = 40
+    r = {k: evaluate(tgv(n, rho0=k)) for k in (1.0, 10.0, 1000.0)}
+    ay = r[1000.0].f64("ay")
+    for f in ("drhodt", "ax", "internal_y", "visc_y", "lap_y", "sps_x"):
+        for k in (1.0, 10.0):
+            s = 1000.0 / k if f == "drhodt" else 1.0
+            a, b = r[k].f64(f) * s, r[1000.0].f64(f)
+            assert np.max(np.abs(a - b)) < 1e-10 * np.max(np.abs(b)), (f, k)
+    d10 = r[1.0].f64("sps_y") - r[10.0].f64("sps_y")
+    d1000 = r[1.0].f64("sps_y") - r[1000.0].f64("sps_y")
+    assert np.max(np.abs(d1000)) > 1e-3 * np.max(np.abs(ay))
+    big = np.abs(d1000) > 1e-2 * np.max(np.abs(d1000))
+    np.testing.assert_allclose(d10[big] / d1000[big], (1 - 1e-2) / (1 - 1e-6), rtol=1e-8)
+    for o in r.values():
+        o.close()
+
+
 @pytest.mark.parametrize("solver", ALL_SOLVERS)
 def test_pairwise_antisymmetric_terms_conserve_momentum(solver):
     """Internal (Riemann pressure), artificial viscosity and delta-SPH internal terms
