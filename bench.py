@@ -195,15 +195,13 @@ def ke_gate_leg(sph2d, n_side: int, t_end: float = 0.5) -> dict:
             "wall_s": {"mixed": res[sph2d.FP32_MIXED][1], "fp64": res[sph2d.FP64][1]}}
 
 
-def batched_leg(sph2d, instances: int, steps: int, warmup: int) -> dict:
+def batched_leg(sph2d, instances: int, steps: int, warmup: int, trial: int = 20) -> dict:
     """Many independent periodic TGV instances of uneven size and Reynolds number
     (n_side = round(64 * 8^U), Re = 50 * 20^U', U, U' uniform, seed 3), FP32-mixed
     predictor-corrector: one sph2d_batch run (each replayable step one hipGraph
     launch) against the same instances run one after another, each alone."""
     import numpy as np
-    rng = np.random.default_rng(3)
-    sides = [int(round(64 * 8 ** u)) for u in rng.uniform(0, 1, instances)]
-    res = [50 * 20 ** u for u in rng.uniform(0, 1, instances)]
+    sides, res = sph2d.uneven_tgv_instances(instances, seed=3)
     total = sum(n * n for n in sides)
 
     def make():
@@ -219,10 +217,14 @@ def batched_leg(sph2d, instances: int, steps: int, warmup: int) -> dict:
     sims = make()
     batch = sph2d.Batch(sims)
     batch.run(1, warmup)
-    # the autotuner times each launch configuration on 20 real steps (one MLS step each)
-    trial = 20
-    tune_ms, chosen = batch.autotune(1 + warmup, trial)
-    first = 1 + warmup + 4 * (trial + 1)
+    # the autotuner times each launch configuration on `trial` real steps (20: one MLS
+    # step each); trial 0 keeps configuration 0
+    if trial > 0:
+        tune_ms, chosen = batch.autotune(1 + warmup, trial)
+    else:
+        tune_ms, chosen = np.zeros(len(sph2d.Batch.CONFIGS)), 0
+    tried = int(np.count_nonzero(tune_ms))
+    first = 1 + warmup + tried * (trial + 1)
     hip.hipDeviceSynchronize()
     t0 = time.perf_counter()
     batch.run(first, steps)
@@ -253,7 +255,9 @@ def batched_leg(sph2d, instances: int, steps: int, warmup: int) -> dict:
             "sequential_particle_steps_per_s": total * steps / t_seq,
             "speedup_vs_sequential": t_seq / t_batch, "graph_nodes": nodes, "graph_replays": replays,
             "autotune": {"ms_per_step": {name: float(v) for name, v in zip(sph2d.Batch.CONFIGS, tune_ms)},
-                         "chosen": sph2d.Batch.CONFIGS[chosen], "trial_steps": trial},
+                         "chosen": sph2d.Batch.CONFIGS[chosen], "trial_steps": trial,
+                         "default_over_chosen": float(tune_ms[0] / tune_ms[chosen]) if tried else None,
+                         "slowest_over_chosen": float(tune_ms.max() / tune_ms[chosen]) if tried else None},
             "timed_steps_from": first,
             "ke_identical_to_sequential": same,
             "timing": "host wall clock with device synchronisation, launches included"}
@@ -382,7 +386,8 @@ def main() -> int:
     ap.add_argument("--ke-gate-side", type=int, default=1024,
                     help="KE error of the mixed form vs FP64 at t = 0.5 on this lattice (0 skips; 2048 is the "
                          "4.2M-particle gate, 102,400 steps per form)")
-    ap.add_argument("--forms", choices=("both", "fp64", "mixed"), default="both")
+    ap.add_argument("--forms", choices=("both", "fp64", "mixed", "batched"), default="both",
+                    help="both: the full line; fp64/mixed: that form's timing only; batched: the batched leg only")
     ap.add_argument("--timing-steps", type=int, default=20,
                     help="steps after the timed run that record the per-pass HIP events")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -391,6 +396,8 @@ def main() -> int:
     ap.add_argument("--batch-instances", type=int, default=64,
                     help="batched leg: independent TGV instances advanced by sph2d_batch (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--batch-trial-steps", type=int, default=20,
+                    help="autotuner steps per launch configuration (0 keeps configuration 0)")
     ap.add_argument("--adjoint-side", type=int, default=1024,
                     help="reverse of one evaluation on an n^2 TGV lattice (0 skips it)")
     ap.add_argument("--adjoint-steps", type=int, default=4000,
@@ -461,6 +468,12 @@ def main() -> int:
         hip.hipFree(b)
         print(f"# copy calibration: {nbytes} bytes device to device", file=sys.stderr, flush=True)
 
+    if args.forms == "batched":
+        print(json.dumps({"partial_run": "batched",
+                          "batched": batched_leg(sph2d, args.batch_instances, args.batch_steps, args.warmup,
+                                                 args.batch_trial_steps)}),
+              flush=True)
+        return 0
     r64 = run_form(sph2d.FP64) if args.forms in ("both", "fp64") else None
     r32 = run_form(sph2d.FP32_MIXED) if args.forms in ("both", "mixed") else None
     if r64 is None or r32 is None:
@@ -563,7 +576,8 @@ def main() -> int:
             out["ke_gate"] = {"error": repr(e)}
     if args.batch_instances > 0:
         try:
-            out["batched"] = batched_leg(sph2d, args.batch_instances, args.batch_steps, args.warmup)
+            out["batched"] = batched_leg(sph2d, args.batch_instances, args.batch_steps, args.warmup,
+                                         args.batch_trial_steps)
         except Exception as e:                    # the headline line is printed regardless
             out["batched"] = {"error": repr(e)}
     if args.adjoint_steps > 0:

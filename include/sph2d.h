@@ -353,14 +353,25 @@ int sph2d_batch_create(int32_t n, sph2d_ctx* const* ctxs, sph2d_batch** out);
 int sph2d_batch_run(sph2d_batch* batch, int32_t first_step, int32_t nsteps);
 /* Nodes of the captured step graph (0 before the first replayable step) and replays so far. */
 int sph2d_batch_info(sph2d_batch* batch, int32_t* graph_nodes, int32_t* graph_replays);
+/* Configurations of the fused launches: config = order + 2 * block_order + 4 * shape,
+ * with order 0 (instances as given) or 1 (by local capacity, largest first) in the
+ * block tables, block_order 0 (XCD-aware) or 1 (linear) over the neighbour passes,
+ * and shape 0..3 the tile of the FP32-mixed neighbour passes: particles per block
+ * and LDS candidate-stage slots of passes A and C, {256, 1024, 1088},
+ * {128, 640, 704}, {64, 384, 416}, {512, 1792, 1856}. FP64 batches use shape 0. */
+#define SPH2D_BATCH_CONFIGS 16
 /* Autotuner of the fused launches: times trial_steps real steps (after one untimed
- * step that captures the graphs) of each of 4 launch configurations and keeps the
- * fastest: instance order in the block tables (0 as given, 1 by size, largest
- * first) x block order of the neighbour passes (0 XCD-aware, 2 linear).
- * Advances every instance by 4 (trial_steps + 1) steps from first_step;
- * trial_ms (4, may be NULL) receives ms per step, chosen the configuration. */
+ * step that captures the graphs) of each configuration (16 for FP32-mixed
+ * instances, the 4 of shape 0 for FP64) and keeps the fastest. Advances every
+ * instance by that many times (trial_steps + 1) steps from first_step; trial_ms
+ * (SPH2D_BATCH_CONFIGS, may be NULL) receives ms per step (0 for configurations not
+ * tried), chosen the configuration. Every configuration computes the same numbers. */
 int sph2d_batch_autotune(sph2d_batch* batch, int32_t first_step, int32_t trial_steps, double* trial_ms,
                          int32_t* chosen);
+/* Sets the configuration of the fused launches (0 at creation); the next replayable
+ * step captures new graphs. */
+int sph2d_batch_configure(sph2d_batch* batch, int32_t config);
+int sph2d_batch_config(sph2d_batch* batch, int32_t* config);
 int sph2d_batch_destroy(sph2d_batch* batch);
 
 /* --- piston wavemaker (Wavemaker_Subroutine.f90, PistonType) ------------------- */

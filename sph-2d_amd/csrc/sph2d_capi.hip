@@ -1944,6 +1944,7 @@ struct sph2d_batch {
   bool on_batch = false;                       // the latest work of the batch is on `stream`
   // one launch per kernel over all instances (when their forms allow it)
   bool fused = false;
+  int config = 0;                              // the fused launches' configuration (autotuner)
   BatchDev dev;
   std::vector<void*> dev_allocs;
 };
@@ -1988,8 +1989,9 @@ bool fusable(const std::vector<sph2d_ctx*>& v) {
 
 // The instance tables of the fused launches, instances in the given order (or by
 // local capacity, largest first, when by_size); xcd: XCD-aware block order of the
-// neighbour passes. The device tables are allocated on the first call.
-int batch_build_dev(sph2d_batch* b, bool by_size = false, int xcd = 1) {
+// neighbour passes; shape: kNbShape entry of the mixed neighbour passes. The device
+// tables are allocated on the first call.
+int batch_build_dev(sph2d_batch* b, bool by_size = false, int xcd = 1, int shape = 0) {
   const int n = (int)b->ctx.size();
   std::vector<int> order(n);
   for (int i = 0; i < n; ++i) order[i] = i;
@@ -2014,7 +2016,7 @@ int batch_build_dev(sph2d_batch* b, bool by_size = false, int xcd = 1) {
     int acc = 0;
     for (int i = 0; i < n; ++i) {
       starts[(size_t)cls * (n + 1) + i] = acc;
-      acc += batch_blocks(cls, inst[i]);
+      acc += batch_blocks(cls, inst[i], shape);
     }
     starts[(size_t)cls * (n + 1) + n] = acc;
     b->dev.total[cls] = acc;
@@ -2040,6 +2042,8 @@ int batch_build_dev(sph2d_batch* b, bool by_size = false, int xcd = 1) {
   for (int cls = 0; cls < BC_COUNT; ++cls) b->dev.start[cls] = static_cast<const int*>(ds) + (size_t)cls * (n + 1);
   b->dev.q32 = dq;
   b->dev.xcd = xcd;
+  b->dev.shape = shape;
+  b->config = (by_size ? 1 : 0) + (xcd ? 0 : 2) + 4 * shape;
   return SPH2D_OK;
 }
 
@@ -2051,6 +2055,13 @@ void batch_drop_graphs(sph2d_batch* b) {
   b->exec = b->exec_mls = nullptr;
   b->graph = b->graph_mls = nullptr;
   b->nodes = 0;
+}
+
+// configuration index: bit 0 instance order by size, bit 1 linear block order,
+// bits 2.. the tile shape of the mixed neighbour passes
+int batch_apply_config(sph2d_batch* b, int cfg) {
+  batch_drop_graphs(b);
+  return batch_build_dev(b, (cfg & 1) != 0, (cfg & 2) ? 0 : 1, cfg >> 2);
 }
 
 int batch_capture(sph2d_batch* b, int it, bool mls) {
@@ -2185,11 +2196,14 @@ int sph2d_batch_autotune(sph2d_batch* b, int32_t first, int32_t trial_steps, dou
   if (!b) return fail(SPH2D_ERR_ARGUMENT, "batch is NULL");
   if (!b->fused) return fail(SPH2D_ERR_UNSUPPORTED, "the contexts of this batch do not share fused launches");
   if (first < 1 || trial_steps < 1) return fail(SPH2D_ERR_ARGUMENT, "bad step range");
+  const bool mixed = b->ctx[0]->prm.precision == SPH2D_FP32_MIXED;
+  const int n_cfg = mixed ? SPH2D_BATCH_CONFIGS : 4;   // tile shapes apply to the mixed passes
+  if (trial_ms)
+    for (int c = 0; c < SPH2D_BATCH_CONFIGS; ++c) trial_ms[c] = 0.0;
   double best = 0.0;
   int best_cfg = 0, it = first;
-  for (int cfg = 0; cfg < 4; ++cfg) {
-    batch_drop_graphs(b);
-    int rc = batch_build_dev(b, (cfg & 1) != 0, (cfg & 2) ? 0 : 1);
+  for (int cfg = 0; cfg < n_cfg; ++cfg) {
+    int rc = batch_apply_config(b, cfg);
     if (!rc) rc = sph2d_batch_run(b, it, 1);        // captures the graphs of this configuration
     ++it;
     if (rc) return rc;
@@ -2204,10 +2218,24 @@ int sph2d_batch_autotune(sph2d_batch* b, int32_t first, int32_t trial_steps, dou
     if (trial_ms) trial_ms[cfg] = ms;
     if (cfg == 0 || ms < best) { best = ms; best_cfg = cfg; }
   }
-  batch_drop_graphs(b);
-  const int rc = batch_build_dev(b, (best_cfg & 1) != 0, (best_cfg & 2) ? 0 : 1);
+  const int rc = batch_apply_config(b, best_cfg);
   if (rc) return rc;
   if (chosen) *chosen = best_cfg;
+  return SPH2D_OK;
+}
+
+int sph2d_batch_configure(sph2d_batch* b, int32_t config) {
+  if (!b) return fail(SPH2D_ERR_ARGUMENT, "batch is NULL");
+  if (!b->fused) return fail(SPH2D_ERR_UNSUPPORTED, "the contexts of this batch do not share fused launches");
+  const bool mixed = b->ctx[0]->prm.precision == SPH2D_FP32_MIXED;
+  if (config < 0 || config >= (mixed ? SPH2D_BATCH_CONFIGS : 4))
+    return fail(SPH2D_ERR_ARGUMENT, "config out of range (tile shapes apply to the FP32-mixed form only)");
+  return batch_apply_config(b, config);
+}
+
+int sph2d_batch_config(sph2d_batch* b, int32_t* config) {
+  if (!b || !config) return fail(SPH2D_ERR_ARGUMENT, "batch and config must be non-NULL");
+  *config = b->config;
   return SPH2D_OK;
 }
 

@@ -210,7 +210,8 @@ enum { kNbScan = 0, kNbRecord = 1, kNbReplay = 2 };
 // stencil columns over rows first-1 .. last+1), which the block copies to LDS
 // once -- FP32 and FP64 positions -- and every thread's own ranges are
 // sub-ranges of them. A block that straddles two columns, or whose union
-// exceeds kStageCap, reads global memory as before.
+// exceeds the stage capacity (kStageCap slots unless the caller sizes its own
+// stage), reads global memory as before.
 constexpr int kStageCap = 1024;
 
 struct NbStage {
@@ -267,8 +268,8 @@ __device__ __forceinline__ int stage_source(const NbStage& st, int i) {
 __device__ __forceinline__ void stage_candidates(const DevParams& P, const int* __restrict__ cell_start,
                                                  const int* __restrict__ l_cell, const float2* __restrict__ pos32,
                                                  const double2* __restrict__ pos, int n, NbStage& st,
-                                                 float2* lds32, double2* lds64, int blk) {
-  plan_stage(P, cell_start, l_cell, n, kStageCap, st, blk);
+                                                 float2* lds32, double2* lds64, int blk, int cap = kStageCap) {
+  plan_stage(P, cell_start, l_cell, n, cap, st, blk);
   const int total = st.total;
   for (int i = threadIdx.x; i < total; i += blockDim.x) {
     const int o = stage_source(st, i);

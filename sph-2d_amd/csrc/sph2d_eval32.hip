@@ -79,14 +79,15 @@ __device__ __forceinline__ KernelTerm32 kernel_term32(const DevParams32& Q, floa
 }
 
 // pass A (mixed): see k_eval_a in sph2d_eval64.hip for the reference lines.
-template <bool MUSCL>
+// CAP: candidate slots of the block's LDS stage (24 B each).
+template <bool MUSCL, int CAP = kStageCap>
 __device__ __forceinline__ void k_eval_a32_body(const DevParams& P, const DevParams32& Q, Buffers& B, int blk_, int nblk_) {
   __shared__ NbStage st;
-  __shared__ float2 lds32[kStageCap];
-  __shared__ double2 lds64[kStageCap];
+  __shared__ float2 lds32[CAP];
+  __shared__ double2 lds64[CAP];
   const int s = blk_ * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
-  stage_candidates(P, B.cell_start, B.l_cell, B.l_pos32, B.l_pos, n, st, lds32, lds64, blk_);
+  stage_candidates(P, B.cell_start, B.l_cell, B.l_pos32, B.l_pos, n, st, lds32, lds64, blk_, CAP);
   if (s >= n) return;
   const double2 ps = B.l_pos[s];
   const float4 r0s = B.m_r0[s];
@@ -253,14 +254,15 @@ __global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q,
 // whose union does not fit read every record from global memory.
 constexpr int kRecCap = 1088;
 
-template <int SOLVER>
+// CAP: candidate slots of the block's LDS record stage (32 B each).
+template <int SOLVER, int CAP = kRecCap>
 __device__ __forceinline__ void k_eval_c32_body(const DevParams& P, const DevParams32& Q, Buffers& B, int do_shift, int blk_, int nblk_) {
   __shared__ NbStage st;
-  __shared__ float4 rec0[kRecCap], rec2[kRecCap];
+  __shared__ float4 rec0[CAP], rec2[CAP];
   const int blk = blk_;
   const int s = blk * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
-  plan_stage(P, B.cell_start, B.l_cell, n, kRecCap, st, blk);
+  plan_stage(P, B.cell_start, B.l_cell, n, CAP, st, blk);
   for (int i = threadIdx.x; i < st.total; i += blockDim.x) {
     const int o = stage_source(st, i);
     rec0[i] = B.m_r0[o];
@@ -563,46 +565,59 @@ void q32_fill(const DevParams& P, void* out) {
   std::memcpy(out, &Q, sizeof Q);
 }
 
-// Batched mixed passes A and C (see launch_batch_step).
-template <bool MUSCL>
-__global__ void __launch_bounds__(kBlock) kb_eval_a32(BatchArgs A, const DevParams32* Qs) {
+// Batched mixed passes A and C (see launch_batch_step), in the tile shape SHAPE of
+// kNbShape: BLK consecutive particles per block and the LDS stage capacities.
+template <bool MUSCL, int SHAPE>
+__global__ void __launch_bounds__(kNbShape[SHAPE].block) kb_eval_a32(BatchArgs A, const DevParams32* Qs) {
   const BatchWhere w = batch_where(A, A.xcd ? xcd_block() : (int)blockIdx.x);
   const DevParams P = A.inst[w.i].P;
   const DevParams32 Q = Qs[w.i];
   Buffers B = A.inst[w.i].B;
-  k_eval_a32_body<MUSCL>(P, Q, B, w.blk, w.nblk);
+  k_eval_a32_body<MUSCL, kNbShape[SHAPE].cap_a>(P, Q, B, w.blk, w.nblk);
 }
 
-template <int SOLVER>
-__global__ void __launch_bounds__(kBlock) kb_eval_c32(BatchArgs A, const DevParams32* Qs) {
+template <int SOLVER, int SHAPE>
+__global__ void __launch_bounds__(kNbShape[SHAPE].block) kb_eval_c32(BatchArgs A, const DevParams32* Qs) {
   const BatchWhere w = batch_where(A, A.xcd ? xcd_block() : (int)blockIdx.x);
   const DevParams P = A.inst[w.i].P;
   const DevParams32 Q = Qs[w.i];
   Buffers B = A.inst[w.i].B;
-  k_eval_c32_body<SOLVER>(P, Q, B, 1, w.blk, w.nblk);
+  k_eval_c32_body<SOLVER, kNbShape[SHAPE].cap_c>(P, Q, B, 1, w.blk, w.nblk);
 }
 
-void launch_batch_eval32(hipStream_t s, const BatchDev& D, int pass, int solver) {
-  const int g = D.total[BC_LOCAL];
-  const BatchArgs A{D.inst, D.start[BC_LOCAL], D.n, D.xcd};
+template <int SHAPE>
+static void launch_batch_eval32_shape(hipStream_t s, const BatchDev& D, int pass, int solver) {
+  const int g = D.total[BC_NBR];
+  const dim3 t(kNbShape[SHAPE].block);
+  const BatchArgs A{D.inst, D.start[BC_NBR], D.n, D.xcd};
   const auto* Qs = static_cast<const DevParams32*>(D.q32);
   if (pass == 0) {
-    if (solver == SPH2D_SOLVER_MUSCL) hipLaunchKernelGGL(kb_eval_a32<true>, dim3(g), dim3(kBlock), 0, s, A, Qs);
-    else hipLaunchKernelGGL(kb_eval_a32<false>, dim3(g), dim3(kBlock), 0, s, A, Qs);
+    if (solver == SPH2D_SOLVER_MUSCL) hipLaunchKernelGGL((kb_eval_a32<true, SHAPE>), dim3(g), t, 0, s, A, Qs);
+    else hipLaunchKernelGGL((kb_eval_a32<false, SHAPE>), dim3(g), t, 0, s, A, Qs);
     return;
   }
   switch (solver) {
     case SPH2D_SOLVER_LINEAR_RIEMANN:
-      hipLaunchKernelGGL(kb_eval_c32<SPH2D_SOLVER_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, s, A, Qs);
+      hipLaunchKernelGGL((kb_eval_c32<SPH2D_SOLVER_LINEAR_RIEMANN, SHAPE>), dim3(g), t, 0, s, A, Qs);
       break;
     case SPH2D_SOLVER_HLLC:
-      hipLaunchKernelGGL(kb_eval_c32<SPH2D_SOLVER_HLLC>, dim3(g), dim3(kBlock), 0, s, A, Qs);
+      hipLaunchKernelGGL((kb_eval_c32<SPH2D_SOLVER_HLLC, SHAPE>), dim3(g), t, 0, s, A, Qs);
       break;
     case SPH2D_SOLVER_MUSCL:
-      hipLaunchKernelGGL(kb_eval_c32<SPH2D_SOLVER_MUSCL>, dim3(g), dim3(kBlock), 0, s, A, Qs);
+      hipLaunchKernelGGL((kb_eval_c32<SPH2D_SOLVER_MUSCL, SHAPE>), dim3(g), t, 0, s, A, Qs);
       break;
     default:
-      hipLaunchKernelGGL(kb_eval_c32<SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, s, A, Qs);
+      hipLaunchKernelGGL((kb_eval_c32<SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN, SHAPE>), dim3(g), t, 0, s, A, Qs);
+  }
+}
+
+void launch_batch_eval32(hipStream_t s, const BatchDev& D, int pass, int solver) {
+  if (D.total[BC_NBR] <= 0) return;
+  switch (D.shape) {
+    case 1: launch_batch_eval32_shape<1>(s, D, pass, solver); break;
+    case 2: launch_batch_eval32_shape<2>(s, D, pass, solver); break;
+    case 3: launch_batch_eval32_shape<3>(s, D, pass, solver); break;
+    default: launch_batch_eval32_shape<0>(s, D, pass, solver);
   }
 }
 

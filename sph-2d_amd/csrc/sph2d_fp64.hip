@@ -1915,7 +1915,7 @@ void launch_eos_n(hipStream_t s, int n, double rho0, double c0sq, const double* 
 // Batched forms of the step's kernels: block b of a launch belongs to the instance
 // found by batch_where; each runs the same body with that instance's parameters.
 // ---------------------------------------------------------------------------
-int batch_blocks(int cls, const Inst& I) {
+int batch_blocks(int cls, const Inst& I, int shape) {
   switch (cls) {
     case BC_FLUID: return grid_for(I.cap_fluid);
     case BC_TWO: return 2;
@@ -1927,6 +1927,7 @@ int batch_blocks(int cls, const Inst& I) {
     case BC_SCAN_C: return scan_block_count(I.n_cells);
     case BC_RANK: return (I.n_cells + kBlock / 64 - 1) / (kBlock / 64);
     case BC_LOCAL: return grid_for(I.cap_local);
+    case BC_NBR: return (I.cap_local + kNbShape[shape].block - 1) / kNbShape[shape].block;
     default: return 0;
   }
 }

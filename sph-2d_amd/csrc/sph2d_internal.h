@@ -213,8 +213,18 @@ enum BatchClass {
   BC_SCAN_C,     // scan tiles over n_cells
   BC_RANK,       // cells / (kBlock / 64): rank in cell
   BC_LOCAL,      // grid_for(cap_local): evaluation passes, corrector
+  BC_NBR,        // cap_local / kNbShape[shape].block: the FP32-mixed neighbour passes
   BC_COUNT
 };
+// Tile shapes of the batched FP32-mixed neighbour passes, chosen by the batch
+// autotuner: particles per block (consecutive in local order, so a block covers a
+// run of cells of one cell column) and the LDS stage capacities of pass A (24 B per
+// candidate slot) and pass C (32 B). Shape 0 is the single-instance launch's.
+struct NbShape {
+  int block, cap_a, cap_c;
+};
+constexpr int kNbShapes = 4;
+constexpr NbShape kNbShape[kNbShapes] = {{256, 1024, 1088}, {128, 640, 704}, {64, 384, 416}, {512, 1792, 1856}};
 struct BatchArgs {
   const Inst* inst;
   const int* start;                       // n + 1 first blocks of the instances (this class)
@@ -225,11 +235,12 @@ struct BatchDev {
   const Inst* inst = nullptr;
   int n = 0;
   int xcd = 1;
+  int shape = 0;                          // kNbShape entry of the mixed neighbour passes
   const int* start[BC_COUNT] = {};
   int total[BC_COUNT] = {};
   const void* q32 = nullptr;              // n FP32 parameter sets (mixed form)
 };
-int batch_blocks(int cls, const Inst& I);
+int batch_blocks(int cls, const Inst& I, int shape);
 size_t q32_bytes();
 void q32_fill(const DevParams& P, void* out);
 // One predictor-corrector step after step 1 that is not a sampling step, for every

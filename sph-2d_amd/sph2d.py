@@ -153,7 +153,8 @@ EXPORTS = (
     "sph2d_format_list_directed", "sph2d_write_list_directed", "sph2d_write_f20_8", "sph2d_kalman_filter",
     "sph2d_sampling_line_points", "sph2d_write_sampling_line_positions", "sph2d_write_sampling_line_results",
     "sph2d_wavemaker_motion", "sph2d_wavemaker_pull_back", "sph2d_wavemaker_update", "sph2d_batch_create", "sph2d_batch_run",
-    "sph2d_batch_info", "sph2d_batch_destroy", "sph2d_batch_autotune", "sph2d_forward", "sph2d_backward",
+    "sph2d_batch_info", "sph2d_batch_destroy", "sph2d_batch_autotune", "sph2d_batch_configure", "sph2d_batch_config",
+    "sph2d_forward", "sph2d_backward",
     "sph2d_relaxation_init", "sph2d_wave_theory_update", "sph2d_wave_maker_damping_zone",
     "sph2d_device_alloc", "sph2d_device_free", "sph2d_device_copy",
 )
@@ -243,6 +244,8 @@ def lib():
             "sph2d_batch_info": [vp, ip, ip],
             "sph2d_batch_destroy": [vp],
             "sph2d_batch_autotune": [vp, i32, i32, dp, ip],
+            "sph2d_batch_configure": [vp, i32],
+            "sph2d_batch_config": [vp, ip],
             "sph2d_reorder_particle_index_noalloc": [i32, i32, ctypes.c_double, ctypes.c_double, vp, vp, vp, i32,
                                                      P(vp), i32, P(vp), vp, ip, vp],
             "sph2d_find_particle_pair": [vp, i32],
@@ -298,6 +301,17 @@ def tgv_params(n_side: int, **kw) -> Params:
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+def uneven_tgv_instances(count: int = 64, seed: int = 3):
+    """Lattice sides and Reynolds numbers of a set of independent periodic Taylor-Green
+    instances of uneven size: n_side = round(64 * 8^U) (log-uniform over 64..512) and
+    Re = 50 * 20^U' (log-uniform over 50..1000), U, U' uniform from numpy's default
+    generator with the given seed (sides drawn first)."""
+    rng = np.random.default_rng(seed)
+    sides = [int(round(64 * 8 ** u)) for u in rng.uniform(0, 1, count)]
+    res = [50 * 20 ** u for u in rng.uniform(0, 1, count)]
+    return sides, res
 
 
 LOCAL_F64 = ("x", "y", "u", "v", "rho", "mass", "p", "c", "volume", "fra_v", "sum_dwdx_x",
@@ -932,15 +946,28 @@ class Batch:
     def run(self, first_step: int, nsteps: int):
         _check(lib().sph2d_batch_run(self._b, first_step, nsteps))
 
-    CONFIGS = ("given order, XCD-aware", "by size, XCD-aware", "given order, linear", "by size, linear")
+    SHAPES = ((256, 1024, 1088), (128, 640, 704), (64, 384, 416), (512, 1792, 1856))
+    CONFIGS = tuple(f"{order}, {blocks}, tile {sh[0]}/{sh[1]}/{sh[2]}" for sh in SHAPES
+                    for blocks in ("XCD-aware", "linear") for order in ("given order", "by size"))
 
     def autotune(self, first_step: int, trial_steps: int = 20):
-        """Time the 4 launch configurations (each on trial_steps real steps after one
-        capture step) and keep the fastest; returns (ms per step of each, chosen)."""
-        ms = np.zeros(4)
+        """Time the launch configurations (16 for FP32-mixed instances: instance
+        order x block order x tile shape of the neighbour passes; 4 for FP64), each on
+        trial_steps real steps after one capture step, and keep the fastest; returns
+        (ms per step of each, 0 where not tried; chosen)."""
+        ms = np.zeros(len(self.CONFIGS))
         ch = ctypes.c_int32()
         _check(lib().sph2d_batch_autotune(self._b, first_step, trial_steps, _dp(ms), ctypes.byref(ch)))
         return ms, ch.value
+
+    def configure(self, config: int):
+        _check(lib().sph2d_batch_configure(self._b, int(config)))
+
+    @property
+    def config(self) -> int:
+        c = ctypes.c_int32()
+        _check(lib().sph2d_batch_config(self._b, ctypes.byref(c)))
+        return c.value
 
     def info(self):
         nodes, reps = ctypes.c_int32(), ctypes.c_int32()
