@@ -19,6 +19,8 @@ ORC_SOLVER_MODIFIED_LINEAR_RIEMANN = 0
 ORC_SOLVER_LINEAR_RIEMANN = 1
 ORC_SOLVER_HLLC = 2
 ORC_SOLVER_MUSCL = 3
+ORC_SOLVER_MS_MUSCL_HLLC = 4
+ORC_SOLVER_MS_HLLC = 5
 
 
 class OrcParams(ctypes.Structure):
@@ -42,6 +44,10 @@ class OrcParams(ctypes.Structure):
         ("walls", ctypes.c_int32), ("gravity", ctypes.c_double),
         ("wavemaker_layers", ctypes.c_int32),
         ("sps_on", ctypes.c_int32), ("renorm", ctypes.c_int32),
+        ("ms", ctypes.c_int32), ("background_p", ctypes.c_double), ("release_time", ctypes.c_double),
+        ("body_cx", ctypes.c_double), ("body_cy", ctypes.c_double), ("body_len", ctypes.c_double),
+        ("body_speed", ctypes.c_double), ("body_ramp", ctypes.c_double),
+        ("sample_every", ctypes.c_int32), ("max_shift_dx", ctypes.c_double),
     ]
 
 
@@ -117,6 +123,11 @@ def lib():
                                                  P(ctypes.c_int32), ctypes.c_int32, P(VP), ctypes.c_int32,
                                                  P(VP), P(ctypes.c_int32), P(ctypes.c_int32)]
         L.orc_hllc_star.argtypes = [P(D), D, D, D, P(D), D, D, D, P(D), D, P(D), P(D)]
+        L.orc_init_moving_square.argtypes = [VP]
+        L.orc_ms_time_step.argtypes = [VP, ctypes.c_int32]
+        L.orc_ms_body_motion.argtypes = [VP, D]
+        L.orc_ms_single_step_compute.argtypes = [VP, ctypes.c_int32, D]
+        L.orc_ms_body_force.argtypes = [VP, P(D)]
         _lib = L
     return _lib
 
@@ -154,6 +165,25 @@ class Oracle:
     # --- steps -----------------------------------------------------------
     def init_tgv(self):
         lib().orc_init_tgv(self._c)
+
+    # --- the moving-square program (ms = 1) ---------------------------------
+    def init_moving_square(self):
+        lib().orc_init_moving_square(self._c)
+
+    def ms_time_step(self, it: int):
+        lib().orc_ms_time_step(self._c, it)
+
+    def ms_body_motion(self, t: float):
+        lib().orc_ms_body_motion(self._c, t)
+
+    def ms_single_step_compute(self, it: int, t: float):
+        lib().orc_ms_single_step_compute(self._c, it, t)
+
+    def ms_body_force(self) -> np.ndarray:
+        """(pressure x, y, viscous x, y, total x, y) of the last evaluation that computed it."""
+        out = np.zeros(6)
+        lib().orc_ms_body_force(self._c, _dp(out))
+        return out
 
     def set_fluid_state(self, x, y, u, v, rho, mass, type_=None):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, u, v, rho, mass)]

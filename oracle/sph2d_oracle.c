@@ -88,11 +88,21 @@ struct orc_ctx {
   int relax_on;
   int *rz_in;
   double *rz_c, *an_x, *an_y, *an_u, *an_v, *an_p, *an_rho, *x0, *y0;
+  /* the moving-square program: particle_initial_type (global), Block_Center_X/_Y, the
+   * body's Custom_velocity(1) and Custom_accleration(1), Real_time, the time of the
+   * evaluation in progress, the body force (pressure, viscous, total; x, y) and its
+   * per-particle parts (subdomain_Near_Body_Or_Not, Body_Force_*_Acceleration) */
+  int *ginit;
+  double body_x, body_y, cust_u, cust_a, real_time, t_eval;
+  double bforce[6];
+  int *near_body;
+  double *bp_x, *bp_y, *bv_x, *bv_y;
 };
 
 /* ------------------------------------------------------------------ */
 /* small helpers                                                       */
 /* ------------------------------------------------------------------ */
+static void ms_reinitialize_density(orc_ctx* c);
 static void* xcalloc(size_t n, size_t s) {
   void* p = calloc(n ? n : 1, s);
   if (!p) { fprintf(stderr, "orc: out of memory\n"); abort(); }
@@ -131,6 +141,13 @@ void orc_default_params(orc_params* p) {
   p->gravity = 0.0;                                /* Gravity_Switch = 'off'         */
   p->wavemaker_layers = 0;                         /* make_wave_or_not = 0           */
   p->sps_on = 1;                                   /* Solver_..._SPS (single_step_compute.f90:137) */
+  p->ms = 0;
+  p->background_p = 0.0;                           /* MS Information_Module.f90:486-487 */
+  p->release_time = 10.0;                          /* MS :715 */
+  p->body_cx = 2.5; p->body_cy = 2.5; p->body_len = 1.0;
+  p->body_speed = 1.0; p->body_ramp = 0.5;
+  p->sample_every = 100;
+  p->max_shift_dx = 0.25;                          /* information_module.f90:353 */
 }
 
 /* ------------------------------------------------------------------ */
@@ -162,6 +179,11 @@ void orc_status_equation(const orc_ctx* c, double rho, double e, int32_t type,
   if (abs(type) == 1) {            /* air branch: c only, p untouched (:46-50) */
     float gamma = 1.4f;
     *cs = sqrt(((double)gamma - 1.0) * rho);
+    return;
+  }
+  if (c->P.ms) {                   /* MS Status_Equation.f90:62-64, EquationOfState = 1 */
+    *p = c->P.c0sq * (rho - c->P.rho0) + c->P.background_p;
+    *cs = sqrt(c->P.c0sq) * ipow3(rho / c->P.rho0);
     return;
   }
   double b = c->P.c0sq * c->P.rho0 / 7.0;
@@ -255,6 +277,7 @@ static void alloc_local(orc_ctx* c, int cap) {
   A(t_u); A(t_v); A(t_drho); A(t_de); A(t_ax); A(t_ay);
   for (int k = 0; k < 3; ++k) { A(k_u[k]); A(k_v[k]); A(k_drho[k]); A(k_de[k]); A(k_ax[k]); A(k_ay[k]); }
   A(mls_rho);
+  I(near_body); A(bp_x); A(bp_y); A(bv_x); A(bv_y);
 #undef A
 #undef I
 }
@@ -267,12 +290,13 @@ orc_ctx* orc_create(const orc_params* p) {
   c->dy = p->box_y / p->ny;
   c->h = p->h_factor * c->dx;
   c->chain_d = (3 + 1) * c->h;
+  if (p->ms) c->chain_d = 4 * c->dx;                             /* MS Information_Module.f90:189 */
   c->chain_ox = -(5 * c->chain_d);
   c->chain_oy = -(5 * c->chain_d);
   c->chain_nx = (int)(p->box_x / c->chain_d + 2 * 5);
   c->chain_ny = (int)(p->box_y / c->chain_d + 2 * 5);
   c->e0 = p->c0sq / (7.0 - 1.0);
-  c->max_shift = 0.25 * c->dx;
+  c->max_shift = p->max_shift_dx * c->dx;
   int n = p->nx * p->ny;
   c->cap_global = (int)(n * 2.5) + 64;      /* n_total = 2.5 * interior number (:124-125) */
   c->n_fluid = 0;
@@ -283,6 +307,7 @@ orc_ctx* orc_create(const orc_params* p) {
   c->gtype = xcalloc(c->cap_global, sizeof(int));
   c->gbtype = xcalloc(c->cap_global, sizeof(int));
   c->gsurf = xcalloc(c->cap_global, sizeof(int));
+  c->ginit = xcalloc(c->cap_global, sizeof(int));
   alloc_local(c, c->cap_global);
   return c;
 }
@@ -303,7 +328,8 @@ void orc_destroy(orc_ctx* c) {
                   c->rz_in, c->rz_c, c->an_x, c->an_y, c->an_u, c->an_v, c->an_p, c->an_rho, c->x0, c->y0,
                   c->cell_count, c->cell_list, c->near_n, c->near_name,
                   c->pi, c->pj, c->pw, c->pdwx, c->pdwy, c->pdx, c->pdy, c->pvx, c->pvy, c->prd,
-                  c->pc_, c->prho_, c->ph_, c->ndix, c->ndiy, c->ndjx, c->ndjy};
+                  c->pc_, c->prho_, c->ph_, c->ndix, c->ndiy, c->ndjx, c->ndjy,
+                  c->ginit, c->near_body, c->bp_x, c->bp_y, c->bv_x, c->bv_y};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
   for (int k = 0; k < 3; ++k) {
     free(c->k_u[k]); free(c->k_v[k]); free(c->k_drho[k]); free(c->k_de[k]);
@@ -748,6 +774,98 @@ static void assign_walls(orc_ctx* c) {
   free(tn); free(wk);
 }
 
+/* MS Assign_boundary_particles.f90:611-760 (DistributeFixedBoundaryParticleOrNot = 1,    */
+/* ExternalBoundaryCondition = 0, MLS_order_Boundary = 1): every fixed ghost i (walls     */
+/* and body) starts from v = 0, p = 0, m = 0; its non-ghost candidates within 3 hbar,    */
+/* hbar = 1.2 (REAL(4)) smooth_length, no lower bound, weigh W m_j / rho_j; with 3 or    */
+/* more of them the order-1 MLS (Fai_k = (1/sum W) W_k) gives                            */
+/*   P = sum p_j Fai + rho_j Fai g_ext . (x_i - x_j), v = sum v_j Fai, m = sum m_j Fai    */
+/* (g_ext = gravity, plus the body's acceleration for body particles). Walls, while     */
+/* 0 < t <= Shock_wave_release_time, get the characteristic correction of their normal  */
+/* velocity and pressure with reference state (u, p) = (0, 0) (boundary type -6: y,     */
+/* -7: x; -8 keeps p = 0: the reference sets p only in those two branches), otherwise    */
+/* v, p = the MLS values; the mirror then negates both components (no-slip). Body       */
+/* particles take the MLS pressure and the body's velocity. Finally rho = (p - p_b) /    */
+/* c0^2 + rho0 and c = c0 (rho/rho0)^3, or the rest state when m is 0.                   */
+static void ms_assign_walls(orc_ctx* c) {
+  const orc_params* P = &c->P;
+  const double t = c->t_eval;
+  const double hb = (double)1.2f * c->h;
+  const double sqc0 = sqrt(P->c0sq);
+  int* tn = xcalloc(300, sizeof(int));
+  double* wk = xcalloc(300, sizeof(double));
+  for (int i = 0; i < c->n_local; ++i) {
+    if (c->type[i] != -3) continue;
+    const int g = c->name[i];
+    const int body = c->ginit[g] == -4;
+    double iu = 0.0, iv = 0.0, ip = 0.0, m = 0.0, pr = 0.0;
+    int nt = 0;
+    for (int L = 0; L < c->cand_n[i]; ++L) {
+      int j = c->cand[(size_t)i * 300 + L];
+      if (c->type[j] == -3) continue;
+      double pd[2] = {c->x[i] - c->x[j], c->y[i] - c->y[j]};
+      double d = sqrt(pd[0] * pd[0] + pd[1] * pd[1]);
+      if (d <= 3.0 * hb) {
+        double w, dw[2];
+        orc_compute_kernel(2, d, pd, hb, &w, dw);
+        tn[nt] = j; wk[nt] = w * c->mass[j] / c->rho[j]; ++nt;
+      }
+    }
+    const double gx = body ? 0.0 + c->cust_a : 0.0;                /* Gravity_grad + Custom_accleration */
+    const double gy = body ? -P->gravity + 0.0 : -P->gravity;
+    if (nt >= 3) {
+      double a = 0.0;
+      for (int k = 0; k < nt; ++k) a = a + wk[k] * 1.0;
+      double ainv = 1.0 / a;
+      for (int k = 0; k < nt; ++k) {
+        double fai = 1.0 * (ainv * wk[k]);
+        int j = tn[k];
+        double dot = gx * (c->x[i] - c->x[j]) + gy * (c->y[i] - c->y[j]);
+        ip = ip + c->p[j] * fai + c->rho[j] * fai * dot;
+        iu = iu + c->u[j] * fai;
+        iv = iv + c->v[j] * fai;
+        m = m + c->mass[j] * fai;
+      }
+    }
+    double u, v;
+    if (!body) {
+      if (0.0 < t && t <= P->release_time) {
+        const double irho = (ip - P->background_p) / P->c0sq + P->rho0;
+        const double tc = sqc0 * ipow3(irho / P->rho0);
+        u = iu; v = iv;
+        const int bt = c->gbtype[g];
+        if (bt == -6) {
+          double J2 = irho * tc * (iv - 0.0) + (ip - 0.0);
+          double J3 = -(irho * tc * (iv - 0.0)) + (ip - 0.0);
+          v = 0.0 + (J2 - J3) / (2 * irho * tc);
+          pr = 0.0 + 0.5 * (J2 + J3);
+        } else if (bt == -7) {
+          double J2 = irho * tc * (iu - 0.0) + (ip - 0.0);
+          double J3 = -(irho * tc * (iu - 0.0)) + (ip - 0.0);
+          u = 0.0 + (J2 - J3) / (2 * irho * tc);
+          pr = 0.0 + 0.5 * (J2 + J3);
+        }
+      } else {
+        u = iu; v = iv; pr = ip;
+      }
+      u = -u; v = -v;                                           /* -6, -7, -8 no-slip mirrors */
+    } else {
+      pr = ip;
+      u = c->cust_u; v = 0.0;                                   /* Custom_velocity */
+    }
+    if (m != 0.0) {
+      c->e[i] = c->e0;
+      c->rho[i] = (pr - P->background_p) / P->c0sq + P->rho0;
+      c->c[i] = sqc0 * ipow3(c->rho[i] / P->rho0);
+    } else {
+      u = 0.0; v = 0.0; pr = 0.0;
+      c->e[i] = c->e0; c->rho[i] = P->rho0; c->c[i] = sqc0;
+    }
+    c->u[i] = u; c->v[i] = v; c->p[i] = pr; c->mass[i] = m;
+  }
+  free(tn); free(wk);
+}
+
 static void assign_boundary_particles(orc_ctx* c) {
   int n = c->n_local;
   double o_sub = c->chain_ox + (c->sub_start - 1) * c->chain_d;
@@ -781,7 +899,8 @@ static void assign_boundary_particles(orc_ctx* c) {
     }
     c->cand_n[i] = cnt < 300 ? cnt : 300;
   }
-  if (c->P.walls > 0) assign_walls(c);                          /* :157-299 */
+  if (c->P.walls > 0 && c->P.ms) ms_assign_walls(c);             /* MS :611-760 */
+  else if (c->P.walls > 0) assign_walls(c);                     /* :157-299 */
   for (int k = 0; k < n; ++k) c->vol[k] = c->mass[k] / c->rho[k];
 }
 
@@ -804,7 +923,8 @@ static void find_particle_pair(orc_ctx* c) {
   int n = c->n_local;
   c->npair = 0;
   grow_pairs(c, (int64_t)n * 40 + 16);
-  const double rmin = (double)1.0E-7f;
+  /* MS Find_particle_pair.f90:73-75: 1.0E-10 (REAL(4)) */
+  const double rmin = c->P.ms ? (double)1.0E-10f : (double)1.0E-7f;
   for (int i = 0; i < n - 1; ++i) {
     for (int L = 0; L < c->cand_n[i]; ++L) {
       int j = c->cand[(size_t)i * 300 + L];
@@ -852,6 +972,7 @@ static void define_freesurface(orc_ctx* c) {
   int ns = 0;
   memset(c->surf, 0, sizeof(int) * n);
   for (int i = 0; i < n; ++i) {                                   /* :119-131 */
+    if (c->P.ms) break;                  /* MS :42-51: Has_FreeSurface_Or_Not = 0 */
     if (c->type[i] != 2) continue;
     if (c->fra_v[i] <= (double)0.65f) { slist[ns++] = i; c->surf[i] = 1; }
   }
@@ -1410,6 +1531,7 @@ static void mls6_fai(const orc_ctx* c, double xi, double yi, int ne, const int* 
 }
 
 void orc_reinitialize_density(orc_ctx* c) {
+  if (c->P.ms) { ms_reinitialize_density(c); return; }
   int n = c->n_local;
   memcpy(c->mls_rho, c->rho, sizeof(double) * n);
   memset(c->rho, 0, sizeof(double) * n);
@@ -1477,8 +1599,12 @@ void orc_shift(orc_ctx* c) {
     }
     c->x[i] = c->x[i] + c->shx[i];
     c->y[i] = c->y[i] + c->shy[i];
-    c->rho[i] = c->rho[i] + c->grx[i] * c->shx[i];
-    c->rho[i] = c->rho[i] + c->gry[i] * c->shy[i];
+    if (P->ms) {                          /* MS Shift_particle_position.f90:117: DOT_PRODUCT */
+      c->rho[i] = c->rho[i] + (c->grx[i] * c->shx[i] + c->gry[i] * c->shy[i]);
+    } else {
+      c->rho[i] = c->rho[i] + c->grx[i] * c->shx[i];
+      c->rho[i] = c->rho[i] + c->gry[i] * c->shy[i];
+    }
     c->p[i] = (P->c0sq * P->rho0 / 7.0) * (ipow7(c->rho[i] / P->rho0) - 1.0);
   }
 }
@@ -1868,11 +1994,13 @@ static const fld F64[] = {
   FL("pair_w", pw, 2), FL("pair_dwdx_x", pdwx, 2), FL("pair_dwdx_y", pdwy, 2),
   FL("position_soft_coefficient", rz_c, 1), FL("an_x", an_x, 1), FL("an_y", an_y, 1), FL("an_u", an_u, 1),
   FL("an_v", an_v, 1), FL("an_p", an_p, 1), FL("an_rho", an_rho, 1), FL("initial_x", x0, 1), FL("initial_y", y0, 1),
+  FL("body_pressure_x", bp_x, 0), FL("body_pressure_y", bp_y, 0), FL("body_viscous_x", bv_x, 0),
+  FL("body_viscous_y", bv_y, 0),
 };
 static const fld I32[] = {
   FL("type", type, 0), FL("name", name, 0), FL("surface", surf, 0), FL("near_surface", near, 0),
   FL("candidate_n", cand_n, 0), FL("g_type", gtype, 1), FL("g_btype", gbtype, 1), FL("g_surface", gsurf, 1), FL("pair_i", pi, 2), FL("pair_j", pj, 2),
-  FL("in_damping_zone", rz_in, 1),
+  FL("in_damping_zone", rz_in, 1), FL("g_initial_type", ginit, 1), FL("near_body", near_body, 0),
 };
 static int64_t fld_count(const orc_ctx* c, int kind) {
   switch (kind) {
@@ -2158,4 +2286,609 @@ void orc_wavemaker_pull_back(orc_ctx* c, double xp, double up) {
     if (c->gx[i] < xp) c->gx[i] = 2 * xp - c->gx[i];
     if ((c->gx[i] - xp) < c->dx) c->gu[i] = up;
   }
+}
+
+/* ================================================================== */
+/* The moving-square program (Moving_Square_Or_Cylinder_Inside_a_     */
+/* Rectangular_Box/, "MS"), ms = 1. The local order stays the rule of  */
+/* orc_devide_block (4 dx cells column by column, gid order within a   */
+/* cell): MS's two-dimensional rank decomposition has no one-rank form */
+/* to restate, and only the HLLC pair orientation and the stale rates  */
+/* of the predictor depend on it.                                      */
+/* ================================================================== */
+
+/* One straight run of layer points (MS/Initial_Particle_From_First_Step.f90:178-620):
+ * the first layer's count k comes from walking from (x0, y0) in steps of gdx along
+ * (sx, sy) until within gdx/2 of (x1, y1); the run places k - 1 points at the spacing
+ * |(x1, y1) - (x0, y0)| / (k - 1); k then changes by dk (+2 outside walls, -2 body). */
+static int ms_run(double x0, double y0, double x1, double y1, double sx, double sy, double gdx, int* k,
+                  int first, int dk, double* x, double* y, int at) {
+  if (first) {
+    int j = 0;
+    for (;;) {
+      ++j;
+      double px = x0 + sx * ((j - 1) * gdx), py = y0 + sy * ((j - 1) * gdx);
+      if (sqrt((x1 - px) * (x1 - px) + (y1 - py) * (y1 - py)) <= gdx / 2.0) break;
+    }
+    *k = j;
+  }
+  double dist = sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0)) / (*k - 1);
+  int n = 0;
+  for (int j = 1; j <= *k - 1; ++j) {
+    if (x) { x[at + n] = x0 + sx * ((j - 1) * dist); y[at + n] = y0 + sy * ((j - 1) * dist); }
+    ++n;
+  }
+  *k = *k + dk;
+  return n;
+}
+
+/* Fixed ghosts of the tank (:178-405) then of the square (:419-620); bt: boundary
+ * type, it: initial type (-3 wall, -4 body). Returns the count (x == NULL: count only). */
+static int ms_ghost_layout(const orc_params* P, double gdx, double gdy, double* x, double* y, int* bt, int* it) {
+  const double bx = P->box_x, by = P->box_y;
+  const int L = P->walls;
+  int n = 0, k = 0, n0;
+  n0 = n;
+  for (int i = 1; i <= L; ++i)                                       /* bottom :178-219 */
+    n += ms_run(-(i - 1) * gdy - gdy / 2.0, -(i - 1) * gdy - gdy / 2.0, bx + (i - 1) * gdy + gdy / 2.0,
+                -(i - 1) * gdy - gdy / 2.0, 1, 0, gdx, &k, i == 1, 2, x, y, n);
+  if (x) for (int q = n0; q < n; ++q) { bt[q] = (0.0 <= x[q] && x[q] <= bx) ? -6 : -7; it[q] = -3; }
+  n0 = n;
+  for (int i = 1; i <= L; ++i)                                       /* right :243-279 */
+    n += ms_run(bx + (i - 1) * gdy + gdy / 2.0, -(i - 1) * gdy - gdy / 2.0, bx + (i - 1) * gdy + gdy / 2.0,
+                by + (i - 1) * gdy + gdy / 2.0, 0, 1, gdx, &k, i == 1, 2, x, y, n);
+  if (x) for (int q = n0; q < n; ++q) { bt[q] = (0.0 <= y[q] && y[q] <= by) ? -8 : -7; it[q] = -3; }
+  n0 = n;
+  for (int i = 1; i <= L; ++i)                                       /* top :300-351 */
+    n += ms_run(bx + (i - 1) * gdy + gdy / 2.0, by + (i - 1) * gdy + gdy / 2.0, -(i - 1) * gdy - gdy / 2.0,
+                by + (i - 1) * gdy + gdy / 2.0, -1, 0, gdx, &k, i == 1, 2, x, y, n);
+  if (x) for (int q = n0; q < n; ++q) { bt[q] = (0.0 <= x[q] && x[q] <= bx) ? -6 : -7; it[q] = -3; }
+  n0 = n;
+  for (int i = 1; i <= L; ++i)                                       /* left :355-405 */
+    n += ms_run(-(i - 1) * gdy - gdy / 2.0, by + (i - 1) * gdy + gdy / 2.0, -(i - 1) * gdy - gdy / 2.0,
+                -(i - 1) * gdy - gdy / 2.0, 0, -1, gdx, &k, i == 1, 2, x, y, n);
+  if (x) for (int q = n0; q < n; ++q) { bt[q] = (0.0 <= y[q] && y[q] <= by) ? -8 : -7; it[q] = -3; }
+  const double cx = P->body_cx, cy = P->body_cy, hl = 0.5 * P->body_len;
+  /* the square: per side a run whose first point is a corner (-7), the rest -6 or -8 */
+  const double sgn[4][2] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+  for (int side = 0; side < 4; ++side) {
+    n0 = n;
+    int first_of_layer[16];
+    int nl = 0;
+    for (int i = 1; i <= L; ++i) {
+      const double a = (i - 1) * gdy + gdy / 2.0;
+      double x0, y0, x1, y1;
+      if (side == 0) { x0 = (cx - hl) + a; y0 = (cy - hl) + a; x1 = (cx + hl) - a; y1 = (cy - hl) + a; }      /* :419-467 */
+      else if (side == 1) { x0 = (cx + hl) - a; y0 = (cy - hl) + a; x1 = (cx + hl) - a; y1 = (cy + hl) - a; } /* :468-516 */
+      else if (side == 2) { x0 = (cx + hl) - a; y0 = (cy + hl) - a; x1 = (cx - hl) + a; y1 = (cy + hl) - a; } /* :517-565 */
+      else { x0 = (cx - hl) + a; y0 = (cy + hl) - a; x1 = (cx - hl) + a; y1 = (cy - hl) + a; }                /* :566-614 */
+      if (nl < 16) first_of_layer[nl++] = n;
+      n += ms_run(x0, y0, x1, y1, sgn[side][0], sgn[side][1], gdx, &k, i == 1, -2, x, y, n);
+    }
+    if (x) {
+      for (int q = n0; q < n; ++q) { bt[q] = (side == 0 || side == 2) ? -6 : -8; it[q] = -4; }
+      for (int l = 0; l < nl; ++l) if (first_of_layer[l] < n) bt[first_of_layer[l]] = -7;
+    }
+  }
+  return n;
+}
+
+void orc_init_moving_square(orc_ctx* c) {
+  const orc_params* P = &c->P;
+  const double cx = P->body_cx, cy = P->body_cy, hl = 0.5 * P->body_len;
+  int k = 0;
+  for (int i = 1; i <= P->ny; ++i)                    /* SPH_Subroutine_Module.f90:889-918 */
+    for (int j = 1; j <= P->nx; ++j) {
+      double x = (j - 1) * c->dx + c->dx / 2.0;
+      double y = (i - 1) * c->dy + c->dy / 2.0;
+      if (x >= (cx - hl) && x <= (cx + hl) && y >= (cy - hl) && y <= (cy + hl)) continue;
+      if (k >= c->cap_global) { fprintf(stderr, "orc: capacity\n"); abort(); }
+      c->gx[k] = x; c->gy[k] = y;
+      c->gu[k] = 0.0; c->gv[k] = 0.0;
+      c->gp[k] = 0.0 + P->background_p;
+      c->grho[k] = (c->gp[k] - P->background_p) / P->c0sq + P->rho0;
+      ++k;
+    }
+  c->n_fluid = k;
+  const double vol0 = c->dx * c->dy;                  /* Initial_Particle_From_First_Step.f90:81-88 */
+  const double m0 = P->rho0 * vol0;
+  for (int i = 0; i < k; ++i) {
+    c->gmass[i] = m0; c->gtype[i] = 2; c->ginit[i] = 2; c->gh[i] = c->h;
+    c->gc[i] = sqrt(P->c0sq); c->ge[i] = c->e0;
+  }
+  c->ke0 = 0.0;
+  c->n_wavemaker = 0;
+  int ng = ms_ghost_layout(P, c->dx, c->dy, NULL, NULL, NULL, NULL);
+  if (k + ng > c->cap_global) { fprintf(stderr, "orc: ghost capacity\n"); abort(); }
+  ms_ghost_layout(P, c->dx, c->dy, c->gx + k, c->gy + k, c->gbtype + k, c->ginit + k);
+  for (int q = 0; q < ng; ++q) {                      /* :621-632 */
+    int m = k + q;
+    c->gu[m] = 0.0; c->gv[m] = 0.0; c->grho[m] = P->rho0; c->gmass[m] = m0;
+    c->gtype[m] = -3; c->gh[m] = c->h; c->gp[m] = 0.0 + P->background_p; c->gc[m] = sqrt(P->c0sq);
+    c->ge[m] = c->e0;
+  }
+  c->n_ghost = ng;
+  c->body_x = cx; c->body_y = cy;
+  c->cust_u = 0.0; c->cust_a = 0.0;
+  c->real_time = 0.0;
+  memset(c->bforce, 0, sizeof(c->bforce));
+}
+
+const int32_t* orc_ms_initial_type(const orc_ctx* c) { return c->ginit; }
+
+/* Body_Motion_Solver.f90:62-273 (Free_motion_or_Custom_motion = 1) with the spline of
+ * Input_Data/Motion_Body.dat replaced by the prescribed ramp u(t) = U min(t/T, 1):
+ * Custom_position(1) = x0 + U t^2 / (2T) (t < T), x0 + U T / 2 + U (t - T) after;
+ * Custom_velocity(1) = u(t), Custom_accleration(1) = U / T (t < T) or 0; the body
+ * particles move by the change of the centre (:245-271). */
+void orc_ms_body_motion(orc_ctx* c, double t) {
+  const orc_params* P = &c->P;
+  const double U = P->body_speed, T = P->body_ramp;
+  double xc, u, a;
+  if (t < T) { xc = P->body_cx + 0.5 * U * t * t / T; u = U * t / T; a = U / T; }
+  else { xc = P->body_cx + 0.5 * U * T + U * (t - T); u = U; a = 0.0; }
+  c->cust_u = u; c->cust_a = a;
+  const double ddx = xc - c->body_x, ddy = P->body_cy - c->body_y;
+  c->body_x = xc; c->body_y = P->body_cy;
+  const int NG = c->n_fluid + c->n_ghost;
+  for (int i = 0; i < NG; ++i)
+    if (c->ginit[i] == -4) { c->gx[i] = c->gx[i] + ddx; c->gy[i] = c->gy[i] + ddy; }
+}
+
+/* The MS solvers. First grad rho and grad v (and grad p for MUSCL-HLLC) with the       */
+/* normalised kernel gradients, the SPS stress of the fluid (sps_on, the block of MS/   */
+/* Solver_Normalized_HLLC_Riemann_Delta_SPS.f90:95-235), the XSPH velocity epsilon      */
+/* m_j (v_j - v_i) / (rho_i + rho_j) W of every particle; then per pair the star state: */
+/*  - MUSCL-HLLC (MS/Solver_Normalized_MUSCL_HLLC_Riemann_Delta.f90:148-412): p, u, v    */
+/*    reconstructed with the limited slopes, velocities rotated into the pair frame,     */
+/*    U* = (U_L z_L + U_R z_R + P_L - P_R) / (z_L + z_R), P* acoustic, rho* from the      */
+/*    linear EOS, HLLC fan sampled at the rotated mean velocity, P* = c0^2 (rho_HLLC -    */
+/*    rho0) + p_b;                                                                        */
+/*  - HLLC (MS/Solver_Normalized_HLLC_Riemann_Delta_SPS.f90:240-330): no reconstruction,  */
+/*    U* = (U_R z_L + U_L z_R + P_L - P_R) / (z_L + z_R) (the reference's own form);      */
+/*  - modified linear (MS/Solver_Normalized_Modified_Linear_Riemann_Delta.f90:118-170);  */
+/* the continuity with (v_i - v* + XSPH_i - XSPH_j), momentum, Monaghan viscosity,      */
+/* Morris laplacian, delta-alpha term, SPS (raw dwdx, particle_mass at the local index, */
+/* temp_p carried from x into y), and the delta-SPH density correction.                  */
+static void ms_solver(orc_ctx* c) {
+  const orc_params* P = &c->P;
+  const int n = c->n_local;
+  const int sv = P->solver;
+  size_t nb = sizeof(double) * n;
+  for (int64_t k = 0; k < c->npair; ++k) {
+    int i = c->pi[k], j = c->pj[k];
+    double* Ti = &c->Tinv[i * 4];
+    double* Tj = &c->Tinv[j * 4];
+    double d1 = c->pdwx[k], d2 = c->pdwy[k];
+    c->ndix[k] = Ti[0] * d1 + Ti[2] * d2; c->ndiy[k] = Ti[1] * d1 + Ti[3] * d2;
+    c->ndjx[k] = Tj[0] * d1 + Tj[2] * d2; c->ndjy[k] = Tj[1] * d1 + Tj[3] * d2;
+  }
+  memset(c->grx, 0, nb); memset(c->gry, 0, nb);
+  memset(c->gux, 0, nb); memset(c->guy, 0, nb); memset(c->gvx, 0, nb); memset(c->gvy, 0, nb);
+  memset(c->gpx, 0, nb); memset(c->gpy, 0, nb); memset(c->mag_grho, 0, nb);
+  for (int64_t k = 0; k < c->npair; ++k) {
+    int i = c->pi[k], j = c->pj[k];
+    double rd = -c->prd[k], ud = -c->pvx[k], vd = -c->pvy[k];
+    double Vi = c->vol[i], Vj = c->vol[j];
+    double a1 = c->ndix[k], a2 = c->ndiy[k], b1 = c->ndjx[k], b2 = c->ndjy[k];
+    c->grx[i] = c->grx[i] + rd * Vj * a1; c->gry[i] = c->gry[i] + rd * Vj * a2;
+    c->grx[j] = c->grx[j] + rd * Vi * b1; c->gry[j] = c->gry[j] + rd * Vi * b2;
+    c->mag_grho[i] += fabs(rd) * Vj * (fabs(a1) + fabs(a2));
+    c->mag_grho[j] += fabs(rd) * Vi * (fabs(b1) + fabs(b2));
+    c->gux[i] = c->gux[i] + ud * Vj * a1; c->guy[i] = c->guy[i] + ud * Vj * a2;
+    c->gux[j] = c->gux[j] + ud * Vi * b1; c->guy[j] = c->guy[j] + ud * Vi * b2;
+    c->gvx[i] = c->gvx[i] + vd * Vj * a1; c->gvy[i] = c->gvy[i] + vd * Vj * a2;
+    c->gvx[j] = c->gvx[j] + vd * Vi * b1; c->gvy[j] = c->gvy[j] + vd * Vi * b2;
+    if (sv == ORC_SOLVER_MS_MUSCL_HLLC) {
+      double pd = c->p[j] - c->p[i];
+      c->gpx[i] = c->gpx[i] + pd * Vj * a1; c->gpy[i] = c->gpy[i] + pd * Vj * a2;
+      c->gpx[j] = c->gpx[j] + pd * Vi * b1; c->gpy[j] = c->gpy[j] + pd * Vi * b2;
+    }
+  }
+  memset(c->Sxx, 0, nb); memset(c->Sxy, 0, nb); memset(c->Syy, 0, nb);
+  memset(c->txx, 0, nb); memset(c->txy, 0, nb); memset(c->tyy, 0, nb);
+  if (P->sps_on) {
+    for (int64_t k = 0; k < c->npair; ++k) {
+      int i = c->pi[k], j = c->pj[k];
+      double vd1 = -c->pvx[k], vd2 = -c->pvy[k];
+      c->Sxx[i] = c->Sxx[i] + vd1 * c->ndix[k] * c->vol[j];
+      c->Sxy[i] = c->Sxy[i] + (vd1 * c->ndiy[k] + vd2 * c->ndix[k]) * c->vol[j];
+      c->Syy[i] = c->Syy[i] + vd2 * c->ndiy[k] * c->vol[j];
+      c->Sxx[j] = c->Sxx[j] + vd1 * c->ndjx[k] * c->vol[i];
+      c->Sxy[j] = c->Sxy[j] + (vd1 * c->ndjy[k] + vd2 * c->ndjx[k]) * c->vol[i];
+      c->Syy[j] = c->Syy[j] + vd2 * c->ndjy[k] * c->vol[i];
+    }
+    const double dL = (double)(sqrtf(2.0f) / 2.0f) * c->dx;
+    const double c012 = (double)0.12f, c23 = (double)(2.0f / 3.0f), c23b = (double)((2.0f / 3.0f) * 0.0066f);
+    for (int i = 0; i < n; ++i) {
+      if (c->type[i] != 2) continue;
+      double master = c->Sxx[i] * c->Sxx[i] + c->Syy[i] * c->Syy[i] + 0.0;
+      double last = c->Sxy[i] * c->Sxy[i] + 0.0 + 0.0;
+      double abs2 = 2 * master + last;
+      double t0 = c012 * dL;
+      double nut = (t0 * t0) * sqrt(abs2);
+      double kk = c23 * nut * (c->Sxx[i] + c->Syy[i]);
+      double blin = c23b * (dL * dL) * abs2;
+      c->txx[i] = (2 * nut * c->Sxx[i] - kk - blin) * c->rho[i];
+      c->txy[i] = (nut * c->Sxy[i]) * c->rho[i];
+      c->tyy[i] = (2 * nut * c->Syy[i] - kk - blin) * c->rho[i];
+    }
+  }
+  memset(c->avg_x, 0, nb); memset(c->avg_y, 0, nb);
+  for (int64_t k = 0; k < c->npair; ++k) {
+    int i = c->pi[k], j = c->pj[k];
+    double rs = c->rho[i] + c->rho[j], w = c->pw[k];
+    c->avg_x[i] = c->avg_x[i] - P->epsilon_xsph * c->mass[j] * c->pvx[k] / rs * w;
+    c->avg_y[i] = c->avg_y[i] - P->epsilon_xsph * c->mass[j] * c->pvy[k] / rs * w;
+    c->avg_x[j] = c->avg_x[j] + P->epsilon_xsph * c->mass[i] * c->pvx[k] / rs * w;
+    c->avg_y[j] = c->avg_y[j] + P->epsilon_xsph * c->mass[i] * c->pvy[k] / rs * w;
+  }
+  const double sqc0 = sqrt(P->c0sq);
+  for (int64_t k = 0; k < c->npair; ++k) {
+    int i = c->pi[k], j = c->pj[k];
+    double PL = c->p[i], PR = c->p[j];
+    double Iv[2] = {c->u[i], c->v[i]}, Jv[2] = {c->u[j], c->v[j]};
+    double xij[2] = {c->pdx[k], c->pdy[k]};                      /* x_i - x_j */
+    double cb = c->pc_[k], rb = c->prho_[k];
+    double vs[2], Ps, Pbar, vb[2];
+    if (sv == ORC_SOLVER_MS_MUSCL_HLLC || sv == ORC_SOLVER_MS_HLLC) {
+      double e[2];
+      if (sv == ORC_SOLVER_MS_MUSCL_HLLC) {
+        double xr[2] = {-xij[0], -xij[1]};                        /* x_j - x_i */
+        double dist = sqrt(xr[0] * xr[0] + xr[1] * xr[1]);
+        e[0] = xr[0] / dist; e[1] = xr[1] / dist;
+        double li, lj;
+        orc_muscl_limiter(0.5 * (c->gpx[i] * xr[0] + c->gpy[i] * xr[1]),
+                          -(0.5 * (c->gpx[j] * xr[0] + c->gpy[j] * xr[1])), c->p[j] - c->p[i], &li, &lj);
+        PL = c->p[i] + li; PR = c->p[j] - lj;
+        orc_muscl_limiter(0.5 * (c->gux[i] * xr[0] + c->guy[i] * xr[1]),
+                          -(0.5 * (c->gux[j] * xr[0] + c->guy[j] * xr[1])), -c->pvx[k], &li, &lj);
+        Iv[0] = c->u[i] + li; Jv[0] = c->u[j] - lj;
+        orc_muscl_limiter(0.5 * (c->gvx[i] * xr[0] + c->gvy[i] * xr[1]),
+                          -(0.5 * (c->gvx[j] * xr[0] + c->gvy[j] * xr[1])), -c->pvy[k], &li, &lj);
+        Iv[1] = c->v[i] + li; Jv[1] = c->v[j] - lj;
+      } else {
+        double dist = sqrt(xij[0] * xij[0] + xij[1] * xij[1]);
+        e[0] = -xij[0] / dist; e[1] = -xij[1] / dist;
+      }
+      Pbar = (PL + PR) / 2.0;
+      vb[0] = (Iv[0] + Jv[0]) / 2.0; vb[1] = (Iv[1] + Jv[1]) / 2.0;
+      /* Transfer_Matrix = [e1 e2; -e2 e1] */
+      double L1 = e[0] * Iv[0] + e[1] * Iv[1], L2 = -e[1] * Iv[0] + e[0] * Iv[1];
+      double R1 = e[0] * Jv[0] + e[1] * Jv[1], R2 = -e[1] * Jv[0] + e[0] * Jv[1];
+      double B1 = e[0] * vb[0] + e[1] * vb[1];
+      double rL = c->rho[i], rR = c->rho[j], cL = c->c[i], cR = c->c[j];
+      double Us = (sv == ORC_SOLVER_MS_MUSCL_HLLC)
+                      ? (L1 * cL * rL + R1 * cR * rR + PL - PR) / (rL * cL + rR * cR)
+                      : (R1 * cL * rL + L1 * cR * rR + PL - PR) / (rL * cL + rR * cR);
+      double Pst = (PR * cL * rL + PL * cR * rR - cL * rL * cR * rR * (R1 - L1)) / (rL * cL + rR * cR);
+      double rst = (Pst - P->background_p) / P->c0sq + P->rho0;
+      double QL = 1.0, QR = 1.0;
+      if (!(Pst <= PL)) QL = (rL > rst) ? sqrt((rst / rL) * (Pst - PL) / (rL - rst)) / cL : 1.0;
+      if (!(Pst <= PR)) QR = (rR > rst) ? sqrt((rst / rR) * (Pst - PR) / (rR - rst)) / cR : 1.0;
+      double SL = L1 - cL * QL, SR = R1 + cR * QR, xot = B1;
+      double rH, H1, H2;
+      if (SL > xot) { rH = rL; H1 = L1; H2 = L2; }
+      else if (Us >= xot && xot >= SL) {
+        double q = rL * ((SL - L1) / (SL - Us));
+        rH = q; H1 = (Us * q) / rH; H2 = (L2 * q) / rH;
+      } else if (SR >= xot && xot >= Us) {
+        double q = rR * ((SR - R1) / (SR - Us));
+        rH = q; H1 = (Us * q) / rH; H2 = (R2 * q) / rH;
+      } else { rH = rR; H1 = R1; H2 = R2; }
+      vs[0] = e[0] * H1 + (-e[1]) * H2;                          /* TRANSPOSE(Transfer_Matrix) */
+      vs[1] = e[1] * H1 + e[0] * H2;
+      Ps = P->c0sq * (rH - P->rho0) + P->background_p;
+      if (sv == ORC_SOLVER_MS_MUSCL_HLLC) { Iv[0] = c->u[i]; Iv[1] = c->v[i]; Jv[0] = c->u[j]; Jv[1] = c->v[j]; }
+    } else {                                                      /* modified linear */
+      Pbar = (PL + PR) / 2.0;
+      vb[0] = (Iv[0] + Jv[0]) / 2.0; vb[1] = (Iv[1] + Jv[1]) / 2.0;
+      double dist = sqrt(xij[0] * xij[0] + xij[1] * xij[1]);
+      double e[2] = {-xij[0] / dist, -xij[1] / dist};
+      double UL = Iv[0] * e[0] + Iv[1] * e[1], UR = Jv[0] * e[0] + Jv[1] * e[1];
+      double Ub = (UL + UR) / 2.0;
+      double Us = Ub + 0.5 * (PL - PR) / (rb * sqc0);
+      vs[0] = (Us - Ub) * e[0] + vb[0];
+      vs[1] = (Us - Ub) * e[1] + vb[1];
+      double d = UL - UR;
+      double bR = 3.0 * (d > 0.0 ? d : 0.0);
+      bR = bR < cb ? bR : cb;
+      Ps = Pbar + 0.5 * bR * rb * (UL - UR);
+    }
+    double di[2] = {c->ndix[k], c->ndiy[k]}, dj[2] = {c->ndjx[k], c->ndjy[k]};
+    double Ivs[2], Jvs[2], IPs, JPs;
+    if (c->near[i] == 1) { Ivs[0] = vb[0]; Ivs[1] = vb[1]; IPs = Pbar; }
+    else { Ivs[0] = vs[0]; Ivs[1] = vs[1]; IPs = Ps; }
+    if (c->near[j] == 1) { Jvs[0] = vb[0]; Jvs[1] = vb[1]; JPs = Pbar; }
+    else { Jvs[0] = vs[0]; Jvs[1] = vs[1]; JPs = Ps; }
+    double qi[2] = {Iv[0] - Ivs[0] + c->avg_x[i] - c->avg_x[j], Iv[1] - Ivs[1] + c->avg_y[i] - c->avg_y[j]};
+    double qj[2] = {Jv[0] - Jvs[0] + c->avg_x[j] - c->avg_x[i], Jv[1] - Jvs[1] + c->avg_y[j] - c->avg_y[i]};
+    double ti = qi[0] * di[0] + qi[1] * di[1];
+    double tj = -(qj[0] * dj[0] + qj[1] * dj[1]);
+    c->drhodt[i] = c->drhodt[i] + 2 * c->rho[i] * c->vol[j] * ti;
+    c->drhodt[j] = c->drhodt[j] + 2 * c->rho[j] * c->vol[i] * tj;
+    {
+      const double vsc = fabs(Iv[0]) + fabs(Iv[1]) + fabs(Jv[0]) + fabs(Jv[1]) + fabs(vs[0]) + fabs(vs[1]) +
+                         fabs(c->avg_x[i]) + fabs(c->avg_y[i]) + fabs(c->avg_x[j]) + fabs(c->avg_y[j]);
+      c->mag_drho[i] += 2 * c->rho[i] * c->vol[j] * (fabs(di[0]) + fabs(di[1])) * vsc;
+      c->mag_drho[j] += 2 * c->rho[j] * c->vol[i] * (fabs(dj[0]) + fabs(dj[1])) * vsc;
+    }
+    double rij = 1.0 / (c->rho[i] * c->rho[j]);
+    double* iax[2] = {&c->ia_x[i], &c->ia_y[i]};
+    double* jax[2] = {&c->ia_x[j], &c->ia_y[j]};
+    double* imag[2] = {&c->mag_ax[i], &c->mag_ay[i]};
+    double* jmag[2] = {&c->mag_ax[j], &c->mag_ay[j]};
+    const double psc = fabs(PL) + fabs(PR) + fabs(Ps) +
+                       rb * cb * (fabs(Iv[0]) + fabs(Iv[1]) + fabs(Jv[0]) + fabs(Jv[1]));
+    for (int L = 0; L < 2; ++L) {
+      double tpi = -2 * IPs * di[L] * rij;
+      double tpj = -2 * JPs * dj[L] * rij;
+      *iax[L] = *iax[L] + c->mass[j] * tpi;
+      *jax[L] = *jax[L] - c->mass[i] * tpj;
+      *imag[L] += c->mass[j] * 2 * psc * fabs(di[L]) * rij;
+      *jmag[L] += c->mass[i] * 2 * psc * fabs(dj[L]) * rij;
+    }
+    double hb = c->ph_[k];
+    double vij[2] = {c->pvx[k], c->pvy[k]};
+    double r2 = xij[0] * xij[0] + xij[1] * xij[1];
+    double vx = vij[0] * xij[0] + vij[1] * xij[1];
+    if (vx < 0.0) {
+      double fhb = P->fai * hb;
+      double phi = hb * vx / (r2 + fhb * fhb);
+      double Pi = (-(P->alpha * cb * phi) + P->beta * (phi * phi)) / rb;
+      double* iav[2] = {&c->av_x[i], &c->av_y[i]};
+      double* jav[2] = {&c->av_x[j], &c->av_y[j]};
+      for (int L = 0; L < 2; ++L) {
+        double tvi = -Pi * di[L];
+        double tvj = -Pi * dj[L];
+        *iav[L] = *iav[L] + c->mass[j] * tvi;
+        *jav[L] = *jav[L] - c->mass[i] * tvj;
+        *imag[L] += fabs(c->mass[j] * tvi);
+        *jmag[L] += fabs(c->mass[i] * tvj);
+      }
+    }
+    double Ri = xij[0] * di[0] + xij[1] * di[1];
+    double Rj = xij[0] * dj[0] + xij[1] * dj[1];
+    c->lap_x[i] = c->lap_x[i] + c->vol[j] * 2 * P->nu * Ri / r2 * vij[0];
+    c->lap_y[i] = c->lap_y[i] + c->vol[j] * 2 * P->nu * Ri / r2 * vij[1];
+    c->lap_x[j] = c->lap_x[j] - c->vol[i] * 2 * P->nu * Rj / r2 * vij[0];
+    c->lap_y[j] = c->lap_y[j] - c->vol[i] * 2 * P->nu * Rj / r2 * vij[1];
+    c->mag_ax[i] += fabs(c->vol[j] * 2 * P->nu * Ri / r2 * vij[0]);
+    c->mag_ay[i] += fabs(c->vol[j] * 2 * P->nu * Ri / r2 * vij[1]);
+    c->mag_ax[j] += fabs(c->vol[i] * 2 * P->nu * Rj / r2 * vij[0]);
+    c->mag_ay[j] += fabs(c->vol[i] * 2 * P->nu * Rj / r2 * vij[1]);
+    double tmp = P->delta_alpha * hb * cb * vx / r2;
+    double* iai[2] = {&c->ai_x[i], &c->ai_y[i]};
+    double* jai[2] = {&c->ai_x[j], &c->ai_y[j]};
+    for (int L = 0; L < 2; ++L) {
+      *iai[L] = *iai[L] + tmp * di[L] * P->rho0 / c->rho[i] * c->vol[j];
+      *jai[L] = *jai[L] - tmp * dj[L] * P->rho0 / c->rho[j] * c->vol[i];
+      *imag[L] += fabs(tmp * di[L] * P->rho0 / c->rho[i] * c->vol[j]);
+      *jmag[L] += fabs(tmp * dj[L] * P->rho0 / c->rho[j] * c->vol[i]);
+    }
+    if (P->sps_on) {                                  /* HLLC_Riemann_Delta_SPS.f90:260-282 */
+      double d1 = c->pdwx[k], d2 = c->pdwy[k];
+      double tp = 0.0;
+      tp = tp + (c->txx[i] + c->txx[j]) * d1;
+      tp = tp + (c->txy[i] + c->txy[j]) * d2;
+      c->sps_x[i] = c->sps_x[i] + c->gmass[j] * tp * rij;
+      c->sps_x[j] = c->sps_x[j] - c->gmass[i] * tp * rij;
+      c->mag_ax[i] += fabs(c->gmass[j] * tp * rij);
+      c->mag_ax[j] += fabs(c->gmass[i] * tp * rij);
+      tp = tp + (c->tyy[i] + c->tyy[j]) * d2 + (c->txy[i] + c->txy[j]) * d1;
+      c->sps_y[i] = c->sps_y[i] + c->gmass[j] * tp * rij;
+      c->sps_y[j] = c->sps_y[j] - c->gmass[i] * tp * rij;
+      c->mag_ay[i] += fabs(c->gmass[j] * tp * rij);
+      c->mag_ay[j] += fabs(c->gmass[i] * tp * rij);
+    }
+  }
+  for (int64_t k = 0; k < c->npair; ++k) {                       /* delta-SPH density correction */
+    int i = c->pi[k], j = c->pj[k];
+    double rd = -c->prd[k];
+    double hb = c->ph_[k], cb = c->pc_[k];
+    double pd[2] = {-c->pdx[k], -c->pdy[k]};
+    double r2 = pd[0] * pd[0] + pd[1] * pd[1];
+    double sg[2] = {c->grx[i] + c->grx[j], c->gry[i] + c->gry[j]};
+    double psi = rd - 0.5 * (sg[0] * pd[0] + sg[1] * pd[1]);
+    double ti = (pd[0] * c->ndix[k] + pd[1] * c->ndiy[k]) / r2;
+    double tj = (pd[0] * c->ndjx[k] + pd[1] * c->ndjy[k]) / r2;
+    c->corr[i] = c->corr[i] + 2 * P->delta * hb * cb * psi * ti * c->vol[j];
+    c->corr[j] = c->corr[j] - 2 * P->delta * hb * cb * psi * tj * c->vol[i];
+    const double psc = fabs(rd) + 0.5 * (fabs(sg[0] * pd[0]) + fabs(sg[1] * pd[1])) +
+                       0.5 * (c->mag_grho[i] + c->mag_grho[j]) * (fabs(pd[0]) + fabs(pd[1]));
+    c->mag_drho[i] += fabs(2 * P->delta * hb * cb * psc * ti * c->vol[j]);
+    c->mag_drho[j] += fabs(2 * P->delta * hb * cb * psc * tj * c->vol[i]);
+  }
+}
+
+/* Calculate_Body_Force.f90:51-142 (laplacianTerm = 1) and its sums in                  */
+/* Single_step_compute.f90:219-249: fluid particles paired with a body particle j get  */
+/* -V_j (p_i + p_j) grad W_i and rho_i V_j 2 nu (x_ij . grad W_i) / r^2 v_ij; the force */
+/* is the sum of V_i times those (and V_i rho_i g for the total) over those particles.  */
+static void ms_body_force(orc_ctx* c) {
+  const orc_params* P = &c->P;
+  const int n = c->n_local;
+  memset(c->near_body, 0, sizeof(int) * n);
+  size_t nb = sizeof(double) * n;
+  memset(c->bp_x, 0, nb); memset(c->bp_y, 0, nb); memset(c->bv_x, 0, nb); memset(c->bv_y, 0, nb);
+  for (int64_t k = 0; k < c->npair; ++k) {
+    int i = c->pi[k], j = c->pj[k];
+    const int bi = c->ginit[c->name[i]] == -4, bj = c->ginit[c->name[j]] == -4;
+    if (!bi && !bj) continue;
+    double vij[2] = {c->pvx[k], c->pvy[k]}, xij[2] = {c->pdx[k], c->pdy[k]};
+    double r2 = xij[0] * xij[0] + xij[1] * xij[1];
+    if (bj) {
+      c->near_body[i] = 1;
+      c->bp_x[i] = c->bp_x[i] - c->vol[j] * (c->p[i] + c->p[j]) * c->ndix[k];
+      c->bp_y[i] = c->bp_y[i] - c->vol[j] * (c->p[i] + c->p[j]) * c->ndiy[k];
+      double R = xij[0] * c->ndix[k] + xij[1] * c->ndiy[k];
+      c->bv_x[i] = c->bv_x[i] + c->rho[i] * c->vol[j] * 2 * P->nu * R / r2 * vij[0];
+      c->bv_y[i] = c->bv_y[i] + c->rho[i] * c->vol[j] * 2 * P->nu * R / r2 * vij[1];
+    }
+    if (bi) {
+      c->near_body[j] = 1;
+      c->bp_x[j] = c->bp_x[j] + c->vol[i] * (c->p[i] + c->p[j]) * c->ndjx[k];
+      c->bp_y[j] = c->bp_y[j] + c->vol[i] * (c->p[i] + c->p[j]) * c->ndjy[k];
+      double R = xij[0] * c->ndjx[k] + xij[1] * c->ndjy[k];
+      c->bv_x[j] = c->bv_x[j] - c->rho[j] * c->vol[i] * 2 * P->nu * R / r2 * vij[0];
+      c->bv_y[j] = c->bv_y[j] - c->rho[j] * c->vol[i] * 2 * P->nu * R / r2 * vij[1];
+    }
+  }
+  double f[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < n; ++i) {
+    if (c->near_body[i] != 1) continue;
+    double ex = 0.0 + c->rho[i] * 0.0, ey = 0.0 + c->rho[i] * (-P->gravity);
+    f[0] = f[0] + c->vol[i] * c->bp_x[i];
+    f[1] = f[1] + c->vol[i] * c->bp_y[i];
+    f[2] = f[2] + c->vol[i] * c->bv_x[i];
+    f[3] = f[3] + c->vol[i] * c->bv_y[i];
+    f[4] = f[4] + c->vol[i] * (c->bp_x[i] + c->bv_x[i] + ex);
+    f[5] = f[5] + c->vol[i] * (c->bp_y[i] + c->bv_y[i] + ey);
+  }
+  memcpy(c->bforce, f, sizeof(f));
+}
+
+void orc_ms_body_force(const orc_ctx* c, double* out) { memcpy(out, c->bforce, sizeof(c->bforce)); }
+
+void orc_ms_single_step_compute(orc_ctx* c, int32_t i_time_step, double t) {
+  const int n = c->n_local;
+  c->t_eval = t;
+  for (int i = 0; i < n; ++i) {                                   /* :35-45 EOS */
+    if (c->type[i] == 120) continue;
+    orc_status_equation(c, c->rho[i], c->e[i], c->type[i], &c->p[i], &c->c[i]);
+  }
+  memset(c->vol, 0, sizeof(double) * n);
+  assign_boundary_particles(c);                                  /* :49 */
+  find_particle_pair(c);                                         /* :62 */
+  define_freesurface(c);                                         /* :74 */
+  size_t nb = sizeof(double) * n;                                /* :80-93 */
+  memset(c->drhodt, 0, nb); memset(c->dedt, 0, nb);
+  memset(c->ia_x, 0, nb); memset(c->ia_y, 0, nb);
+  memset(c->ai_x, 0, nb); memset(c->ai_y, 0, nb);
+  memset(c->av_x, 0, nb); memset(c->av_y, 0, nb);
+  memset(c->ext_x, 0, nb); memset(c->ext_y, 0, nb);
+  memset(c->corr, 0, nb);
+  memset(c->mag_drho, 0, nb); memset(c->mag_ax, 0, nb); memset(c->mag_ay, 0, nb);
+  memset(c->lap_x, 0, nb); memset(c->lap_y, 0, nb);
+  memset(c->sps_x, 0, nb); memset(c->sps_y, 0, nb);
+  ms_solver(c);                                                  /* :111-140 */
+  if (c->P.sample_every > 0 && (i_time_step % c->P.sample_every) == 0) ms_body_force(c);   /* :219-249 */
+  memset(c->ax, 0, nb); memset(c->ay, 0, nb);                    /* :253-283 */
+  for (int i = 0; i < n; ++i) {
+    if (c->type[i] != 2) continue;
+    c->ext_y[i] = c->ext_y[i] + (-c->P.gravity);
+    c->mag_ay[i] += fabs(c->ext_y[i]);
+    c->ax[i] = c->ia_x[i] + c->ext_x[i] + c->av_x[i] + c->ai_x[i] + c->sps_x[i] + c->lap_x[i];
+    c->ay[i] = c->ia_y[i] + c->ext_y[i] + c->av_y[i] + c->ai_y[i] + c->sps_y[i] + c->lap_y[i];
+    c->dedt[i] = c->dedt[i] + 0.0 + 0.0;
+    c->drhodt[i] = c->drhodt[i] + c->corr[i];
+  }
+}
+
+/* MS Reinitialize_particle_density.f90:29-73: order 1 from the pairs (same-type pairs, */
+/* sum_w of the free-surface routine; every other density 0), or order 3 per fluid      */
+/* particle over its candidates within 3h (no lower bound: the particle itself counts), */
+/* W weighted by V_j, MLS_Interpolation_In_Subdomain_Domain (SPH_Subroutine_Module.f90  */
+/* :2565-2792: basis (1, dx/h, dy/h), alpha rows, the 3x3 inverse of                    */
+/* matrix_inversion_less_three, Shepard when singular); NaN keeps the old density;      */
+/* other particles keep theirs.                                                          */
+static void ms_reinitialize_density(orc_ctx* c) {
+  const orc_params* P = &c->P;
+  const int n = c->n_local;
+  memcpy(c->mls_rho, c->rho, sizeof(double) * n);
+  memset(c->rho, 0, sizeof(double) * n);
+  if (P->mls_order == 1) {
+    for (int64_t k = 0; k < c->npair; ++k) {
+      int i = c->pi[k], j = c->pj[k];
+      if (c->type[i] == c->type[j]) {
+        if (c->type[i] == 2) c->rho[i] = c->rho[i] + c->mass[j] * c->pw[k] / c->sum_w[i];
+        if (c->type[j] == 2) c->rho[j] = c->rho[j] + c->mass[i] * c->pw[k] / c->sum_w[j];
+      }
+    }
+    return;
+  }
+  int nm[300]; double wk[300], p2[300], p3[300];
+  const double al = P->mls_alpha;
+  for (int i = 0; i < n; ++i) {
+    if (c->type[i] != 2) { c->rho[i] = c->mls_rho[i]; continue; }
+    int ne = 0;
+    for (int L = 0; L < c->cand_n[i]; ++L) {
+      int j = c->cand[(size_t)i * 300 + L];
+      double pd[2] = {c->x[i] - c->x[j], c->y[i] - c->y[j]};
+      double r = sqrt(pd[0] * pd[0] + pd[1] * pd[1]);
+      double hb = (c->hh[i] + c->hh[j]) / 2.0;
+      if (r <= 3.0 * hb) {
+        double w, dw[2];
+        orc_compute_kernel(2, r, pd, hb, &w, dw);
+        nm[ne] = j; wk[ne] = w * c->vol[j];
+        p2[ne] = (c->x[j] - c->x[i]) / c->h; p3[ne] = (c->y[j] - c->y[i]) / c->h;
+        ++ne;
+      }
+    }
+    /* A = P W P^T + alpha^3 I (the alpha columns come last), column-major */
+    double A[9] = {0};
+    for (int k = 0; k < ne; ++k) {
+      double pk[3] = {1.0, p2[k], p3[k]};
+      for (int m = 0; m < 3; ++m)
+        for (int l = 0; l < 3; ++l) A[m + 3 * l] = A[m + 3 * l] + pk[m] * wk[k] * pk[l];
+    }
+    for (int l = 0; l < 3; ++l) A[l + 3 * l] = A[l + 3 * l] + (al * al) * al;
+    double Ai[9];
+    int ok = orc_matrix_inversion_less_three(3, A, Ai);
+    double rho = 0.0;
+    if (ok) {
+      for (int k = 0; k < ne; ++k) {
+        double b[3] = {1.0 * wk[k], p2[k] * wk[k], p3[k] * wk[k]};
+        double fai = Ai[0] * b[0] + Ai[3] * b[1] + Ai[6] * b[2];
+        rho = rho + c->mls_rho[nm[k]] * fai;
+      }
+    } else {
+      double sw = 0.0;
+      for (int k = 0; k < ne; ++k) sw = sw + wk[k];
+      for (int k = 0; k < ne; ++k) rho = rho + c->mls_rho[nm[k]] * (wk[k] / sw);
+    }
+    c->rho[i] = isnan(rho) ? c->mls_rho[i] : rho;
+  }
+}
+
+void orc_ms_time_step(orc_ctx* c, int32_t it) {
+  const orc_params* P = &c->P;
+  const double dt = P->dt;
+  c->real_time = c->real_time + dt;                              /* :58-62 */
+  orc_ms_body_motion(c, c->real_time);                           /* :113-115 */
+  for (int i = 0; i < c->n_fluid; ++i) {                         /* Reorder_Particle_Index.f90:80-95 */
+    if (c->gtype[i] != 2) continue;
+    double x = c->gx[i], y = c->gy[i];
+    if (x < (double)-1.0E-10f || x > P->box_x || y < (double)-1.0E-10f || y > P->box_y) c->gtype[i] = 120;
+  }
+  orc_devide_block(c, it);
+  const int n = c->n_local;
+  save_before(c);                                                /* :313-317 */
+  if (it == 1) orc_ms_single_step_compute(c, it, c->real_time);
+  size_t nb = sizeof(double) * n;                                /* :321-324 */
+  memcpy(c->t_drho, c->drhodt, nb); memcpy(c->t_de, c->dedt, nb);
+  memcpy(c->t_ax, c->ax, nb); memcpy(c->t_ay, c->ay, nb);
+  memcpy(c->t_u, c->u, nb); memcpy(c->t_v, c->v, nb);
+  for (int i = 0; i < n; ++i) {                                  /* :326-333 */
+    if (c->type[i] != 2) continue;
+    c->rho[i] = c->b_rho[i] + c->drhodt[i] * dt;
+    c->e[i] = c->b_e[i] + c->dedt[i] * dt;
+    c->x[i] = c->b_x[i] + dt * (c->u[i] + c->avg_x[i]) + 0.5 * c->ax[i] * (dt * dt);
+    c->y[i] = c->b_y[i] + dt * (c->v[i] + c->avg_y[i]) + 0.5 * c->ay[i] * (dt * dt);
+    c->u[i] = c->b_u[i] + dt * c->ax[i];
+    c->v[i] = c->b_v[i] + dt * c->ay[i];
+  }
+  orc_ms_single_step_compute(c, it, c->real_time);               /* :336-337 */
+  for (int i = 0; i < n; ++i) {                                  /* :339-346 */
+    if (c->type[i] != 2) continue;
+    c->rho[i] = c->b_rho[i] + 0.5 * (c->drhodt[i] + c->t_drho[i]) * dt;
+    c->e[i] = c->b_e[i] + 0.5 * (c->dedt[i] + c->t_de[i]) * dt;
+    c->x[i] = c->b_x[i] + 0.5 * dt * (c->u[i] + c->t_u[i]) + c->avg_x[i] * dt + 0.5 * c->ax[i] * (dt * dt);
+    c->y[i] = c->b_y[i] + 0.5 * dt * (c->v[i] + c->t_v[i]) + c->avg_y[i] * dt + 0.5 * c->ay[i] * (dt * dt);
+    c->u[i] = c->b_u[i] + 0.5 * dt * (c->ax[i] + c->t_ax[i]);
+    c->v[i] = c->b_v[i] + 0.5 * dt * (c->ay[i] + c->t_ay[i]);
+  }
+  if (P->mls_on && (it % P->mls_every) == 0) ms_reinitialize_density(c);   /* :508-510 */
+  if (P->shift_on) orc_shift(c);                                 /* :513-515 */
+  orc_refresh(c);                                                /* :518 */
 }

@@ -56,12 +56,34 @@ typedef struct {
   int32_t wavemaker_layers;
   int32_t sps_on;            /* 1: _SPS solvers; 0: Delta solvers without SPS (XSPH factor 1) */
   int32_t renorm;            /* 0: identity for the dyadic inverse (the reference); 1: the inverse */
+  /* ---- the moving-square program (Moving_Square_Or_Cylinder_Inside_a_Rectangular_Box/, */
+  /* "MS"): ms = 1 selects its semantics wherever they differ from the TGV program --    */
+  /* linear EOS (Status_Equation.f90:56-90, EquationOfState = 1), pair bound 1.0E-10     */
+  /* (Find_particle_pair.f90:73-75), 4 dx cells (Information_Module.f90:189), no free    */
+  /* surface (Has_FreeSurface_Or_Not = 0), the fixed-ghost MLS with mass and the         */
+  /* characteristic correction (Assign_boundary_particles.f90:611-760), XSPH velocity    */
+  /* in the continuity equation of every MS solver, the MS predictor-corrector           */
+  /* (SPH_Time_Iteration.f90:313-387), the MS density reinitialisation and the moving    */
+  /* body with its force. Layout by orc_init_moving_square. */
+  int32_t ms;
+  double background_p;       /* Background_Pressure (MS Information_Module.f90:487) */
+  double release_time;       /* Shock_wave_release_time (:715): characteristic wall correction for 0 < t <= it */
+  double body_cx, body_cy, body_len;   /* Block_Center_X/_Y, Block_Length (:152-154) */
+  double body_speed, body_ramp;        /* prescribed motion u(t) = body_speed min(t / body_ramp, 1) in x */
+  int32_t sample_every;      /* Sampling_time_step (:748): the body force is computed on these steps */
+  double max_shift_dx;       /* max_shift / interior_dx: 0.25 (TGV information_module.f90:353), 0.15 (MS :679) */
 } orc_params;
 
 enum { ORC_SOLVER_MODIFIED_LINEAR_RIEMANN = 0,   /* Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90 */
        ORC_SOLVER_LINEAR_RIEMANN = 1,            /* Solver_Normalized_Linear_Riemann_Delta_SPS.f90 */
        ORC_SOLVER_HLLC = 2,                      /* Solver_Normalized_HLLC_Linear_Riemann_Delta_SPS.f90 */
-       ORC_SOLVER_MUSCL = 3 };                   /* Solver_Normalized_MUSCL_Linear_Riemann_Delta_SPS.f90 */
+       ORC_SOLVER_MUSCL = 3,                     /* Solver_Normalized_MUSCL_Linear_Riemann_Delta_SPS.f90 */
+       /* MS solvers (ms = 1; MS ORC_SOLVER_MODIFIED_LINEAR_RIEMANN with sps_on = 0 is
+        * MS/Solver_Normalized_Modified_Linear_Riemann_Delta.f90, the MS default) */
+       ORC_SOLVER_MS_MUSCL_HLLC = 4,             /* MS/Solver_Normalized_MUSCL_HLLC_Riemann_Delta.f90 (+ the SPS
+                                                    block of MS/Solver_Normalized_HLLC_Riemann_Delta_SPS.f90) */
+       ORC_SOLVER_MS_HLLC = 5 };                 /* MS/Solver_Normalized_HLLC_Riemann_Delta_SPS.f90 (sps_on = 0:
+                                                    MS/Solver_Normalized_HLLC_Riemann_Delta.f90) */
 
 typedef struct orc_ctx orc_ctx;
 
@@ -123,6 +145,28 @@ void orc_relaxation_init(orc_ctx* c, double zone_length);
 void orc_wave_theory_update(orc_ctx* c, int32_t wave_type, int32_t soft_steps, int32_t standing_steps, double dt,
                             double g, double d, double H, double k, double om, double phase, int32_t it);
 void orc_relaxation_blend(orc_ctx* c);
+
+/* The moving-square program (ms = 1). Layout of MS/Initial_Particle_From_First_Step.f90
+ * :60-620 with MS/SPH_Subroutine_Module.f90:871-925: fluid on the nx x ny lattice of
+ * [0,box_x]x[0,box_y] (rows in y) without the points of the square [body_cx -/+ L/2] x
+ * [body_cy -/+ L/2], rho = rho0, p = Background_Pressure, v = 0, m = rho0 dx dy; then
+ * `walls` layers of fixed ghosts (type -3) around the tank (bottom, right, top, left) and
+ * `walls` layers inside the square's edges (type -3, initial type -4 = body). */
+void orc_init_moving_square(orc_ctx* c);
+/* one MS time step (SPH_Time_Iteration.f90:58-540 with Iteration_Method = 2) */
+void orc_ms_time_step(orc_ctx* c, int32_t i_time_step);
+/* Body_Motion_Solver.f90:62-273 with the prescribed ramp in place of the spline file:
+ * t = present time; moves the body particles (global arrays) and sets the body velocity
+ * and acceleration used by the wall assignment. */
+void orc_ms_body_motion(orc_ctx* c, double t);
+/* MS single_step_compute (Single_step_compute.f90:35-283) at step i_time_step, time t. */
+void orc_ms_single_step_compute(orc_ctx* c, int32_t i_time_step, double t);
+/* Body force of the last MS evaluation that computed it (Calculate_Body_Force.f90:51-142,
+ * Single_step_compute.f90:219-249): out[6] = pressure x, y, viscous x, y, total x, y
+ * (force on the water; Body_Force.dat divides by force_0 = 0.5 rho0 U^2 L). */
+void orc_ms_body_force(const orc_ctx* c, double* out);
+/* initial types of the global particles (particle_initial_type: 2, -3, -4 body) */
+const int32_t* orc_ms_initial_type(const orc_ctx* c);
 
 /* Sampling_subroutine.f90:340-461. out[0]=KE, [1]=max|v|, [2..4]=u Linf,L1,L2,
  * [5..7]=v, [8..10]=p/(rho0 U^2), [11..13]=|v|. */
