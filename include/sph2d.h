@@ -38,7 +38,14 @@ enum {
   SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN = 0, /* Solver_Normalized_Modified_Linear_Riemann_Delta_SPS.f90 (default, single_step_compute.f90:137) */
   SPH2D_SOLVER_LINEAR_RIEMANN = 1,          /* Solver_Normalized_Linear_Riemann_Delta_SPS.f90 */
   SPH2D_SOLVER_HLLC = 2,                    /* Solver_Normalized_HLLC_Linear_Riemann_Delta_SPS.f90 (single_step_compute.f90:143) */
-  SPH2D_SOLVER_MUSCL = 3                    /* Solver_Normalized_MUSCL_Linear_Riemann_Delta_SPS.f90 (single_step_compute.f90:146) */
+  SPH2D_SOLVER_MUSCL = 3,                   /* Solver_Normalized_MUSCL_Linear_Riemann_Delta_SPS.f90 (single_step_compute.f90:146) */
+  /* the moving-square program (ms = 1; Moving_Square_Or_Cylinder_Inside_a_Rectangular_Box/):
+   * SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN with sps_on = 0 is its default
+   * Solver_Normalized_Modified_Linear_Riemann_Delta.f90 (Single_step_compute.f90:131) */
+  SPH2D_SOLVER_MS_MUSCL_HLLC = 4,           /* Solver_Normalized_MUSCL_HLLC_Riemann_Delta.f90; sps_on = 1 adds the SPS
+                                               stress of Solver_Normalized_HLLC_Riemann_Delta_SPS.f90 */
+  SPH2D_SOLVER_MS_HLLC = 5                  /* Solver_Normalized_HLLC_Riemann_Delta_SPS.f90 (sps_on = 0:
+                                               Solver_Normalized_HLLC_Riemann_Delta.f90) */
 };
 
 /* Integrators of time_iteration.f90 (leapforg_or_eular, information_module.f90:303). */
@@ -111,9 +118,51 @@ typedef struct {
                                 for every normalised kernel gradient (FP64 form; singular tensors
                                 keep the identity and are counted in n_singular). No reference
                                 counterpart: the reference computes the inverse and discards it. */
+  /* The moving-square program (Moving_Square_Or_Cylinder_Inside_a_Rectangular_Box/), ms = 1,
+   * with fixed walls (walls layers around [0,box_x]x[0,box_y], the same number inside the
+   * square body; sph2d_init_moving_square), the predictor-corrector and MLS order 3 or off:
+   * linear EOS p = c0sq (rho - rho0) + background_p (Status_Equation.f90:56-90), 4 dx cells
+   * (Information_Module.f90:189), pairs down to 1.0E-10 (Find_particle_pair.f90:73-75), no
+   * free surface, the fixed-ghost MLS with mass and the characteristic correction
+   * (Assign_boundary_particles.f90:611-760), XSPH inside the continuity equation, the MS
+   * predictor-corrector (SPH_Time_Iteration.f90:313-347), the MS density reinitialisation,
+   * the body moved at u(t) = body_speed min(t / body_ramp, 1) in x (Body_Motion_Solver.f90
+   * with this closed form instead of the Motion_Body.dat spline) and its force
+   * (Calculate_Body_Force.f90) on sampling steps. */
+  int32_t ms;
+  double background_p;       /* Background_Pressure (Information_Module.f90:487) */
+  double release_time;       /* Shock_wave_release_time (:715) */
+  double body_cx, body_cy;   /* Block_Center_X/_Y at t = 0 (:152-153) */
+  double body_len;           /* Block_Length (:154) */
+  double body_speed;         /* U of the prescribed motion */
+  double body_ramp;          /* its ramp time */
 } sph2d_params;
 
 typedef struct sph2d_ctx sph2d_ctx;
+
+/* --- the moving-square program (ms = 1) ------------------------------------------
+ * The tank and body layout (no GPU needed): fluid on the nx x ny lattice of the box
+ * without the points of the square (rows in y; MS/SPH_Subroutine_Module.f90:871-925),
+ * then `walls` layers of fixed ghosts around the tank (bottom, right, top, left) and
+ * `walls` layers inside the square's edges (MS/Initial_Particle_From_First_Step.f90
+ * :178-620). Fills at most cap entries of x, y (positions), btype (-6, -7, -8; 0 for the
+ * fluid) and init (particle_initial_type: 2 fluid, -3 wall, -4 body); *n_fluid and
+ * *n_total receive the counts. Any output pointer may be NULL (counts only). */
+int sph2d_moving_square_layout(const sph2d_params* p, int32_t cap, double* x, double* y, int32_t* btype,
+                               int32_t* init, int32_t* n_fluid, int32_t* n_total);
+/* The layout above on the GPU: fluid at rest, rho = rho0, p = background_p, m = rho0 dx dy;
+ * ghosts at rest with rho0, m = rho0 dx dy, p = background_p; time 0. sph2d_time_step then
+ * runs the MS step (SPH_Time_Iteration.f90:58-540, Iteration_Method = 2). */
+int sph2d_init_moving_square(sph2d_ctx* ctx);
+/* Body_Motion_Solver.f90:62-273 at physical time t (the prescribed ramp): moves the body
+ * particles to the centre x(t) and sets the body velocity and acceleration used by the
+ * next evaluations. sph2d_time_step calls it itself at t = i_time_step * dt (summed). */
+int sph2d_body_motion(sph2d_ctx* ctx, double t);
+/* The body force of the last evaluation of a sampling step (Single_step_compute.f90
+ * :219-249, Calculate_Body_Force.f90:51-142): out6 = pressure x, y, viscous x, y,
+ * total x, y, the force on the water (Body_Force.dat divides by 0.5 rho0 U^2 L);
+ * *step receives that step (0: none yet). */
+int sph2d_body_force(sph2d_ctx* ctx, double* out6, int32_t* step);
 
 /* Counters of the last step (sph2d_get_diagnostics). */
 typedef struct {

@@ -2356,12 +2356,14 @@ static int ms_ghost_layout(const orc_params* P, double gdx, double gdy, double* 
     int first_of_layer[16];
     int nl = 0;
     for (int i = 1; i <= L; ++i) {
-      const double a = (i - 1) * gdy + gdy / 2.0;
+      /* (Block_Center -/+ 0.5 Block_Length) +/- (i-1) dy +/- dy/2, left to right */
+      const double lo_x = (cx - hl) + (i - 1) * gdy + gdy / 2.0, hi_x = (cx + hl) - (i - 1) * gdy - gdy / 2.0;
+      const double lo_y = (cy - hl) + (i - 1) * gdy + gdy / 2.0, hi_y = (cy + hl) - (i - 1) * gdy - gdy / 2.0;
       double x0, y0, x1, y1;
-      if (side == 0) { x0 = (cx - hl) + a; y0 = (cy - hl) + a; x1 = (cx + hl) - a; y1 = (cy - hl) + a; }      /* :419-467 */
-      else if (side == 1) { x0 = (cx + hl) - a; y0 = (cy - hl) + a; x1 = (cx + hl) - a; y1 = (cy + hl) - a; } /* :468-516 */
-      else if (side == 2) { x0 = (cx + hl) - a; y0 = (cy + hl) - a; x1 = (cx - hl) + a; y1 = (cy + hl) - a; } /* :517-565 */
-      else { x0 = (cx - hl) + a; y0 = (cy + hl) - a; x1 = (cx - hl) + a; y1 = (cy - hl) + a; }                /* :566-614 */
+      if (side == 0) { x0 = lo_x; y0 = lo_y; x1 = hi_x; y1 = lo_y; }          /* :419-467 */
+      else if (side == 1) { x0 = hi_x; y0 = lo_y; x1 = hi_x; y1 = hi_y; }     /* :468-516 */
+      else if (side == 2) { x0 = hi_x; y0 = hi_y; x1 = lo_x; y1 = hi_y; }     /* :517-565 */
+      else { x0 = lo_x; y0 = hi_y; x1 = lo_x; y1 = lo_y; }                    /* :566-614 */
       if (nl < 16) first_of_layer[nl++] = n;
       n += ms_run(x0, y0, x1, y1, sgn[side][0], sgn[side][1], gdx, &k, i == 1, -2, x, y, n);
     }

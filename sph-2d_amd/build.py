@@ -33,6 +33,7 @@ UNITS = [
     ("sph2d_output.hip", ["-ffp-contract=off"]),
     ("sph2d_adjoint.hip", ["-ffp-contract=off"]),
     ("sph2d_adjstep.hip", ["-ffp-contract=off"]),
+    ("sph2d_ms.hip", ["-ffp-contract=off"]),
 ]
 
 HEADERS = ["sph2d_internal.h", "sph2d_device.h", "sph2d_riemann.h"]

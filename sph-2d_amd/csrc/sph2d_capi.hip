@@ -85,6 +85,11 @@ struct sph2d_ctx {
   RelaxArgs relax{};
   bool relax_ready = false, relax_on = false;
   bool pass_timing = false;
+  // the moving-square program: Real_time, the body centre, velocity and acceleration,
+  // the step of the last body force and its device totals
+  double ms_time = 0.0, body_x = 0.0, body_y = 0.0, cust_u = 0.0, cust_a = 0.0;
+  int bforce_step = 0;
+  double* d_bforce = nullptr;
   std::vector<hipEvent_t> events;
   size_t events_used = 0;
 
@@ -125,10 +130,10 @@ void make_devparams(const sph2d_params& p, DevParams& P) {
   const float c4 = 1.0f - 10.0f * expf(-9.0f);
   P.kfactor = 1.0 / ((double)3.14159265358f * (p.h * p.h) * (double)c4);
   P.e9 = (double)expf(-9.0f);
-  P.r_min = (double)1.0E-7f;
+  P.r_min = p.ms ? (double)1.0E-10f : (double)1.0E-7f;   // MS Find_particle_pair.f90:73-75
   P.support_volume = kPiModule * ((3 * p.h) * (3 * p.h));
   P.fra_limit = (double)0.65f;
-  P.chain_d = (3 + 1) * p.h;
+  P.chain_d = p.ms ? 4 * p.dx : (3 + 1) * p.h;      // MS Information_Module.f90:189: 4 interior_dx
   P.chain_ox = -(5 * P.chain_d);
   P.chain_oy = -(5 * P.chain_d);
   P.chain_nx = (int)(p.box_x / P.chain_d + 2 * 5);
@@ -192,6 +197,14 @@ void make_devparams(const sph2d_params& p, DevParams& P) {
   P.sps = p.sps_on;
   P.renorm = p.renormalization;
   P.xsph_f = p.sps_on ? 2.0 : 1.0;
+  P.rebin_m = 0.5 * (P.chain_d - P.three_h);
+  P.ms = p.ms;
+  P.pb = p.background_p;
+  P.release_time = p.release_time;
+  P.t_eval = 0.0;
+  P.cust_u = 0.0;
+  P.cust_a = 0.0;
+  P.body_force = 0;
 }
 
 int check_params(const sph2d_params* p) {
@@ -202,13 +215,29 @@ int check_params(const sph2d_params* p) {
     return fail(SPH2D_ERR_ARGUMENT, "h, dx, box and dt must be positive");
   if (p->integrator != SPH2D_INTEGRATOR_PC && p->integrator != SPH2D_INTEGRATOR_RK)
     return fail(SPH2D_ERR_UNSUPPORTED, "integrator must be 2 (predictor-corrector) or 3 (RK-variant)");
-  if (p->solver < SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN || p->solver > SPH2D_SOLVER_MUSCL)
+  if (p->solver < SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN || p->solver > SPH2D_SOLVER_MS_HLLC)
     return fail(SPH2D_ERR_UNSUPPORTED, "solver must be one of SPH2D_SOLVER_*");
+  if (p->ms != 0 && p->ms != 1) return fail(SPH2D_ERR_ARGUMENT, "ms must be 0 or 1");
+  if (!p->ms && p->solver > SPH2D_SOLVER_MUSCL)
+    return fail(SPH2D_ERR_UNSUPPORTED, "the MS solvers belong to the moving-square program (ms = 1)");
+  if (p->ms) {
+    if (p->solver != SPH2D_SOLVER_MS_MUSCL_HLLC && p->solver != SPH2D_SOLVER_MS_HLLC &&
+        p->solver != SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN)
+      return fail(SPH2D_ERR_UNSUPPORTED, "ms = 1 takes SPH2D_SOLVER_MS_MUSCL_HLLC, _MS_HLLC or _MODIFIED_LINEAR_RIEMANN");
+    if (p->integrator != SPH2D_INTEGRATOR_PC || !p->walls || p->periodic || p->wavemaker_layers || p->renormalization ||
+        p->damping != SPH2D_DAMPING_OFF)
+      return fail(SPH2D_ERR_UNSUPPORTED, "ms = 1 runs the predictor-corrector in a walled tank without wavemaker, "
+                                         "damping or renormalisation");
+    if (p->mls_on && p->mls_order != 3) return fail(SPH2D_ERR_UNSUPPORTED, "ms = 1 reinitialises with MLS order 3");
+    if (!(p->body_len > 0) || !(p->body_ramp > 0) || !std::isfinite(p->body_speed))
+      return fail(SPH2D_ERR_ARGUMENT, "body_len and body_ramp must be positive");
+    if (4 * p->dx < 3 * p->h) return fail(SPH2D_ERR_ARGUMENT, "ms = 1 needs 3 h <= 4 dx (the 4 dx cells)");
+  }
   if (p->precision != SPH2D_FP64 && p->precision != SPH2D_FP32_MIXED)
     return fail(SPH2D_ERR_UNSUPPORTED, "precision must be SPH2D_FP64 or SPH2D_FP32_MIXED");
   if (p->rate_reuse != SPH2D_RATE_REUSE_REFERENCE)
     return fail(SPH2D_ERR_UNSUPPORTED, "only the reference's rate re-use is built");
-  if (p->mls_on && p->mls_order != 6)
+  if (!p->ms && p->mls_on && p->mls_order != 6)
     return fail(SPH2D_ERR_UNSUPPORTED, "only MLS order 6 (the reference default) is built");
   if (p->walls < 0 || p->walls > 64) return fail(SPH2D_ERR_ARGUMENT, "walls must be in [0, 64]");
   if (!std::isfinite(p->gravity) || p->gravity < 0) return fail(SPH2D_ERR_ARGUMENT, "gravity must be >= 0");
@@ -321,6 +350,11 @@ int sph2d_default_params(sph2d_params* p) {
   p->wavemaker_layers = 0;                         // make_wave_or_not = 0
   p->sps_on = 1;                                   // Solver_..._SPS (single_step_compute.f90:137)
   p->renormalization = 0;                          // the identity (:168-188)
+  p->ms = 0;                                       // the Taylor-Green program
+  p->background_p = 0.0;                           // MS Information_Module.f90:486-487
+  p->release_time = 10.0;                          // :715
+  p->body_cx = 2.5; p->body_cy = 2.5; p->body_len = 1.0;
+  p->body_speed = 1.0; p->body_ramp = 0.5;
   return SPH2D_OK;
 }
 
@@ -344,7 +378,9 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   c->cap_fluid = n_fluid_capacity;
   c->cap_ghost = (int)std::max(16384L, 3 * ghost_bound);
   if (p->walls) {
-    const int nw = wall_layout(p->box_x, p->box_y, p->dx, p->box_y / p->ny, p->walls, nullptr, nullptr, nullptr);
+    const int nw = p->ms ? ms_ghost_layout(p->box_x, p->box_y, p->dx, p->box_y / p->ny, p->walls, p->body_cx,
+                                           p->body_cy, p->body_len, nullptr, nullptr, nullptr, nullptr)
+                         : wall_layout(p->box_x, p->box_y, p->dx, p->box_y / p->ny, p->walls, nullptr, nullptr, nullptr);
     c->cap_ghost = std::max(c->cap_ghost, wavemaker_count(*p) + nw + 64);
   }
   c->cap_global = c->cap_fluid + c->cap_ghost;
@@ -397,6 +433,14 @@ int sph2d_ctx_create(const sph2d_params* p, int32_t n_fluid_capacity, sph2d_ctx*
   }
   B.dbg = nullptr; B.pair_lo = nullptr; B.pair_hi = nullptr;
   B.kstride = (int)L;
+  B.g_init = nullptr; B.bp = nullptr; B.bv = nullptr; B.near_body = nullptr; B.bpart = nullptr;
+  if (p->ms) {
+    if (!B.mg) rc |= c->alloc(&B.mg, 3 * L);
+    rc |= c->alloc(&B.g_init, G);
+    rc |= c->alloc(&B.bp, L); rc |= c->alloc(&B.bv, L); rc |= c->alloc(&B.near_body, L);
+    rc |= c->alloc(&B.bpart, 6 * (size_t)grid_n((int)L) + 6);
+    rc |= c->alloc(&c->d_bforce, 6);
+  }
   if (rc) {
     std::string msg = g_err;
     sph2d_ctx_destroy(c);
@@ -537,6 +581,132 @@ int sph2d_init_tgv(sph2d_ctx* c) {
   return SPH2D_OK;
 }
 
+// ---- the moving-square program --------------------------------------------------------
+int sph2d_moving_square_layout(const sph2d_params* p, int32_t cap, double* x, double* y, int32_t* btype,
+                               int32_t* init, int32_t* n_fluid, int32_t* n_total) {
+  if (!p) return fail(SPH2D_ERR_ARGUMENT, "params is NULL");
+  if (!(p->dx > 0) || p->nx <= 0 || p->ny <= 0 || p->walls < 0 || p->walls > 64)
+    return fail(SPH2D_ERR_ARGUMENT, "bad lattice or wall layers");
+  const double dx = p->box_x / p->nx, dy = p->box_y / p->ny;
+  const double hl = 0.5 * p->body_len, cx = p->body_cx, cy = p->body_cy;
+  int k = 0;
+  for (int i = 1; i <= p->ny; ++i)                       // SPH_Subroutine_Module.f90:889-918
+    for (int j = 1; j <= p->nx; ++j) {
+      const double px = (j - 1) * dx + dx / 2.0, py = (i - 1) * dy + dy / 2.0;
+      if (px >= (cx - hl) && px <= (cx + hl) && py >= (cy - hl) && py <= (cy + hl)) continue;
+      if (k < cap) {
+        if (x) x[k] = px;
+        if (y) y[k] = py;
+        if (btype) btype[k] = 0;
+        if (init) init[k] = 2;
+      }
+      ++k;
+    }
+  const int ng = ms_ghost_layout(p->box_x, p->box_y, dx, dy, p->walls, cx, cy, p->body_len, nullptr, nullptr,
+                                 nullptr, nullptr);
+  if (n_fluid) *n_fluid = k;
+  if (n_total) *n_total = k + ng;
+  if (x || y || btype || init) {
+    if (k + ng > cap) return fail(SPH2D_ERR_CAPACITY, "cap is smaller than the layout");
+    std::vector<double> gx(ng), gy(ng);
+    std::vector<int> gb(ng), gi(ng);
+    ms_ghost_layout(p->box_x, p->box_y, dx, dy, p->walls, cx, cy, p->body_len, gx.data(), gy.data(), gb.data(),
+                    gi.data());
+    for (int q = 0; q < ng; ++q) {
+      if (x) x[k + q] = gx[q];
+      if (y) y[k + q] = gy[q];
+      if (btype) btype[k + q] = gb[q];
+      if (init) init[k + q] = gi[q];
+    }
+  }
+  return SPH2D_OK;
+}
+
+int sph2d_init_moving_square(sph2d_ctx* c) {
+  if (!c) return fail(SPH2D_ERR_ARGUMENT, "ctx is NULL");
+  const sph2d_params& p = c->prm;
+  if (!p.ms) return fail(SPH2D_ERR_STATE, "the context was not created with ms = 1");
+  int nf = 0, nt = 0;
+  int rc = sph2d_moving_square_layout(&p, 0, nullptr, nullptr, nullptr, nullptr, &nf, &nt);
+  if (rc) return rc;
+  if (nf > c->cap_fluid) return fail(SPH2D_ERR_CAPACITY, "the tank's fluid exceeds the context capacity");
+  if (nt - nf > c->cap_ghost) return fail(SPH2D_ERR_CAPACITY, "the fixed ghosts exceed the ghost capacity");
+  std::vector<double> x(nt), y(nt);
+  std::vector<int> bt(nt), init(nt);
+  rc = sph2d_moving_square_layout(&p, nt, x.data(), y.data(), bt.data(), init.data(), &nf, &nt);
+  if (rc) return rc;
+  rc = set_nfluid(c, nf);
+  if (rc) return rc;
+  const double dy = p.box_y / p.ny, m0 = p.rho0 * (p.dx * dy);
+  const double rho = (0.0 + p.background_p - p.background_p) / p.c0sq + p.rho0;   // :900-903
+  std::vector<double2> pos(nt), vel(nt, make_double2(0.0, 0.0));
+  for (int q = 0; q < nt; ++q) pos[q] = make_double2(x[q], y[q]);
+  std::vector<double> vr(nt, rho), vm(nt, m0), vp(nt, 0.0 + p.background_p), vc(nt, std::sqrt(p.c0sq)),
+      ve(nt, p.c0sq / (7.0 - 1.0));
+  for (int q = nf; q < nt; ++q) vr[q] = p.rho0;
+  std::vector<int> t(nt, 2);
+  for (int q = nf; q < nt; ++q) t[q] = -3;
+  Buffers& B = c->B;
+  hipStream_t st = c->stream;
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_pos, pos.data(), nt * sizeof(double2), hipMemcpyHostToDevice, st));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_vel, vel.data(), nt * sizeof(double2), hipMemcpyHostToDevice, st));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_rho, vr.data(), nt * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_mass, vm.data(), nt * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_p, vp.data(), nt * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_c, vc.data(), nt * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_e, ve.data(), nt * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_type, t.data(), nt * sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_btype, bt.data(), nt * sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_OR_FAIL(hipMemcpyAsync(B.g_init, init.data(), nt * sizeof(int), hipMemcpyHostToDevice, st));
+  launch_fill_i32(st, B.counters + C_NGHOST, 1, nt - nf);
+  HIP_OR_FAIL(hipGetLastError());
+  HIP_OR_FAIL(hipStreamSynchronize(st));
+  c->n_wm = 0;
+  c->ke0 = 0.0;
+  c->ms_time = 0.0;
+  c->body_x = p.body_cx; c->body_y = p.body_cy;
+  c->cust_u = 0.0; c->cust_a = 0.0;
+  c->bforce_step = 0;
+  c->have_state = true;
+  c->evaluated = false;
+  c->vjp_ready = false;
+  c->eval_part = 0;
+  return SPH2D_OK;
+}
+
+int sph2d_body_motion(sph2d_ctx* c, double t) {
+  if (!c) return fail(SPH2D_ERR_ARGUMENT, "ctx is NULL");
+  if (!c->prm.ms) return fail(SPH2D_ERR_STATE, "the context was not created with ms = 1");
+  if (!c->have_state) return fail(SPH2D_ERR_STATE, "no particle state: call sph2d_init_moving_square");
+  const double U = c->prm.body_speed, T = c->prm.body_ramp;
+  double xc, u, a;
+  if (t < T) { xc = c->prm.body_cx + 0.5 * U * t * t / T; u = U * t / T; a = U / T; }
+  else { xc = c->prm.body_cx + 0.5 * U * T + U * (t - T); u = U; a = 0.0; }
+  c->ms_time = t;                                 // Real_time of the evaluations that follow
+  c->cust_u = u; c->cust_a = a;
+  const double ddx = xc - c->body_x, ddy = c->prm.body_cy - c->body_y;
+  c->body_x = xc; c->body_y = c->prm.body_cy;
+  c->vjp_ready = false;
+  c->eval_part = 0;
+  launch_ms_body_move(c->stream, c->P, c->B, c->cap_global, ddx, ddy);
+  HIP_OR_FAIL(hipGetLastError());
+  return SPH2D_OK;
+}
+
+int sph2d_body_force(sph2d_ctx* c, double* out6, int32_t* step) {
+  if (!c) return fail(SPH2D_ERR_ARGUMENT, "ctx is NULL");
+  if (!c->prm.ms) return fail(SPH2D_ERR_STATE, "the context was not created with ms = 1");
+  if (step) *step = c->bforce_step;
+  if (!out6) return SPH2D_OK;
+  if (c->bforce_step == 0) {
+    for (int k = 0; k < 6; ++k) out6[k] = 0.0;
+    return SPH2D_OK;
+  }
+  HIP_OR_FAIL(hipMemcpyAsync(out6, c->d_bforce, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  return SPH2D_OK;
+}
+
 int sph2d_upload_state(sph2d_ctx* c, int32_t n, const double* pos, const double* vel,
                        const double* rho, const double* mass, const double* energy,
                        const double* h, const int32_t* type, int32_t ld) {
@@ -562,10 +732,22 @@ int sph2d_upload_state(sph2d_ctx* c, int32_t n, const double* pos, const double*
       return fail(SPH2D_ERR_NONFINITE, "non-finite or non-positive state at particle " + std::to_string(i));
   }
   Buffers& B = c->B;
+  if (c->prm.ms && (!c->have_state || n != c->n_fluid))
+    return fail(SPH2D_ERR_STATE, "the moving-square tank takes the fluid state of its layout: call "
+                                 "sph2d_init_moving_square first and pass n = its fluid count");
   HIP_OR_FAIL(hipMemcpyAsync(B.g_pos, p2.data(), n * sizeof(double2), hipMemcpyHostToDevice, c->stream));
   HIP_OR_FAIL(hipMemcpyAsync(B.g_vel, v2.data(), n * sizeof(double2), hipMemcpyHostToDevice, c->stream));
   HIP_OR_FAIL(hipMemcpyAsync(B.g_rho, rho, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_OR_FAIL(hipMemcpyAsync(B.g_mass, mass, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (c->prm.ms) {                 // the fluid only: the tank's and body's ghosts stay
+    HIP_OR_FAIL(hipMemcpyAsync(B.g_e, e.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(B.g_type, t.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+    c->evaluated = false;
+    c->vjp_ready = false;
+    c->eval_part = 0;
+    return SPH2D_OK;
+  }
   HIP_OR_FAIL(hipMemcpyAsync(B.g_e, e.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_OR_FAIL(hipMemcpyAsync(B.g_p, pr.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_OR_FAIL(hipMemcpyAsync(B.g_c, cs.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -860,7 +1042,7 @@ static int devide_block(sph2d_ctx* c, int32_t it, int predict) {
   if (rc) return rc;
   c->vjp_ready = false;
   c->eval_part = 0;
-  const bool ana = (it % c->prm.sample_every) == 0;
+  const bool ana = !c->prm.ms && (it % c->prm.sample_every) == 0;   // the TGV analytic fields
   const double t = it * c->prm.dt / (c->prm.box_x / c->prm.U);   // Nondimensionalized_real_time
   launch_fill_i32(c->stream, c->B.counters + C_NILL, 1, 0);
   launch_devide_block(c->stream, c->P, c->B, c->cap_global, c->n_cells, ana, t, predict);
@@ -871,7 +1053,27 @@ static int devide_block(sph2d_ctx* c, int32_t it, int predict) {
 
 int sph2d_devide_block(sph2d_ctx* c, int32_t it) { return devide_block(c, it, 0); }
 
+// One evaluation of the moving-square program (MS Single_step_compute.f90:35-283): EOS
+// for every particle, the fixed ghosts, pass A and the force pass, and on sampling
+// steps the body force.
+static void do_eval_ms(sph2d_ctx* c, int it) {
+  hipStream_t s = c->stream;
+  launch_stage(s, c->P, c->B, c->cap_local, ST_EOS);
+  c->P.t_eval = c->ms_time;
+  c->P.cust_u = c->cust_u;
+  c->P.cust_a = c->cust_a;
+  c->P.body_force = (it > 0 && it % c->prm.sample_every == 0) ? 1 : 0;
+  launch_ms_assign_walls(s, c->P, c->B, c->cap_local);
+  launch_ms_eval(s, c->P, c->B, c->cap_local);
+  if (c->P.body_force) {
+    launch_ms_body_sum(s, c->P, c->B, c->cap_local, c->d_bforce);
+    c->bforce_step = it;
+  }
+  c->evaluated = true;
+}
+
 static void do_eval(sph2d_ctx* c, bool with_eos, int it) {
+  if (c->prm.ms) { do_eval_ms(c, it); return; }
   if (with_eos) launch_stage(c->stream, c->P, c->B, c->cap_local, ST_EOS);
   c->P.write_debug = c->debug ? 1 : 0;
   hipEvent_t* ev = nullptr;
@@ -1126,8 +1328,38 @@ int sph2d_refresh_domain(sph2d_ctx* c, int32_t it) {
   return SPH2D_OK;
 }
 
+// MS SPH_Time_Iteration.f90:58-540 with Iteration_Method = 2: Real_time += dt, the body
+// motion, ill fluid, the local order, the predictor-corrector (an evaluation first on
+// step 1), the density MLS every mls_every steps, shifting and the refresh.
+static int enqueue_ms_step(sph2d_ctx* c, int it) {
+  c->vjp_ready = false;
+  c->eval_part = 0;
+  hipStream_t s = c->stream;
+  c->ms_time = c->ms_time + c->prm.dt;
+  int rc = sph2d_body_motion(c, c->ms_time);
+  if (rc) return rc;
+  launch_ms_ill(s, c->P, c->B, c->cap_fluid);
+  rc = devide_block(c, it, 0);
+  if (rc) return rc;
+  if (it == 1) do_eval_ms(c, it);
+  launch_ms_stage(s, c->P, c->B, c->cap_local, MS_PREDICT);
+  do_eval_ms(c, it);
+  launch_ms_stage(s, c->P, c->B, c->cap_local, MS_CORRECT);
+  const bool mls = c->prm.mls_on && (it % c->prm.mls_every) == 0;
+  if (mls) {
+    launch_fill_i32(s, c->B.counters + C_MLSFB, 1, 0);
+    launch_ms_mls(s, c->P, c->B, c->cap_local);
+  }
+  const int mode = mls ? (c->prm.shift_on ? 1 : 2) : 0;
+  launch_shift_refresh(s, c->P, c->B, c->cap_local, mode, c->prm.shift_on);
+  launch_wall_refresh(s, c->P, c->B, c->cap_local);
+  HIP_OR_FAIL(hipGetLastError());
+  return SPH2D_OK;
+}
+
 // time_iteration.f90 PC (:509-657) / RK-variant (:779-1049), then :1064-1078.
 static int enqueue_step(sph2d_ctx* c, int it) {
+  if (c->prm.ms) return enqueue_ms_step(c, it);
   c->vjp_ready = false;
   c->eval_part = 0;
   int rc = sph2d_distribute_period_boundary(c);

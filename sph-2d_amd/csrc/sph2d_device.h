@@ -22,9 +22,12 @@ __device__ __forceinline__ void kernel_w_dw(const DevParams& P, double r, double
 
 // status_equation.f90:61-63: p = B*((rho/rho0)**7-1), c = sqrt(c0^2)*(rho/rho0)**3,
 // with flang's left-to-right expansion of the integer powers.
+// The moving-square program's law instead (Moving_Square_Or_Cylinder_Inside_a_Rectangular_Box/
+// Status_Equation.f90:62-64, EquationOfState = 1): p = c0^2 (rho - rho0) + p_b.
 __device__ __forceinline__ void tait(const DevParams& P, double rho, double& p, double& c) {
   const double x = rho / P.rho0;
-  p = P.eos_b * (x * x * x * x * x * x * x - 1.0);
+  if (P.ms) p = P.c0sq * (rho - P.rho0) + P.pb;
+  else p = P.eos_b * (x * x * x * x * x * x * x - 1.0);
   c = P.sqrt_c0sq * (x * x * x);
 }
 
@@ -34,9 +37,8 @@ __device__ __forceinline__ void write_r0(const DevParams& P, const Buffers& B, i
                                          double m, int type) {
   const float mf = (float)m;
   B.m_r0[s] = make_float4((float)vel.x, (float)vel.y, (float)(rho - P.rho0), type == 2 ? mf : -mf);
-  const double x = rho / P.rho0;
-  const double p = P.eos_b * (x * x * x * x * x * x * x - 1.0);
-  const double c = P.sqrt_c0sq * (x * x * x);
+  double p, c;
+  tait(P, rho, p, c);
   B.m_pc[s] = make_float2((float)p, (float)c);
 }
 

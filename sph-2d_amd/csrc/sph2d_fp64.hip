@@ -600,7 +600,7 @@ __device__ __forceinline__ void pc_correct(double dt, double drho, double2 a, do
 __device__ __forceinline__ void rebin_check(const DevParams& P, Buffers& B, int cell, double2 x) {
   const int cx = cell / P.chain_ny, cy = cell - cx * P.chain_ny;
   const double x0 = P.chain_ox + cx * P.chain_d, y0 = P.chain_oy + cy * P.chain_d;
-  const double m = 0.5 * P.h;
+  const double m = P.rebin_m;
   if (x.x < x0 - m || x.x > x0 + P.chain_d + m || x.y < y0 - m || x.y > y0 + P.chain_d + m)
     atomicOr(&B.counters[C_REBIN], 1);
 }
@@ -1517,8 +1517,12 @@ __global__ void k_shift_apply(DevParams P, Buffers B, int mode) {
   x.y = x.y + sh.y;
   B.l_pos[s] = x;
   const double2 gr = grad_rho(P, B, s);
-  rho = rho + gr.x * sh.x;
-  rho = rho + gr.y * sh.y;
+  if (P.ms) {
+    rho = rho + (gr.x * sh.x + gr.y * sh.y);
+  } else {
+    rho = rho + gr.x * sh.x;
+    rho = rho + gr.y * sh.y;
+  }
   B.l_rv[s] = make_double2(rho, B.l_rv[s].y);
   B.l_pc[s] = make_double2(tait_p(P, rho), B.l_pc[s].y);
 }
@@ -1586,8 +1590,12 @@ __device__ __forceinline__ void shift_one(const DevParams& P, const Buffers& B, 
   x.x = x.x + sh.x;
   x.y = x.y + sh.y;
   const double2 gr = grad_rho(P, B, s);
-  rho = rho + gr.x * sh.x;
-  rho = rho + gr.y * sh.y;
+  if (P.ms) {                      // MS Shift_particle_position.f90:117 (DOT_PRODUCT)
+    rho = rho + (gr.x * sh.x + gr.y * sh.y);
+  } else {
+    rho = rho + gr.x * sh.x;
+    rho = rho + gr.y * sh.y;
+  }
   p = tait_p(P, rho);
 }
 

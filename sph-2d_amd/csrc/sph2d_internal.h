@@ -55,6 +55,16 @@ struct DevParams {
   int sps;                               // 1: SPS stress (the _SPS solvers); 0: tau = 0
   int renorm;                            // 1: the dyadic tensor's 2x2 inverse normalises grad W
   double xsph_f;                         // XSPH factor: 2 in the _SPS solvers, 1 without SPS
+  double rebin_m;                        // how far a particle may leave its binned cell (3x3 stencil covers 3h)
+  // the moving-square program (Moving_Square_Or_Cylinder_Inside_a_Rectangular_Box/): linear
+  // EOS, no free surface, the fixed-ghost MLS with mass and the characteristic correction,
+  // XSPH in the continuity equation, the moving body and its force
+  int ms;
+  double pb;                             // Background_Pressure
+  double release_time;                   // Shock_wave_release_time
+  double t_eval;                         // physical time of the evaluation in progress
+  double cust_u, cust_a;                 // the body's velocity and acceleration (x)
+  int body_force;                        // 1: this evaluation forms the body force
 };
 
 // Device counters (one int array, indices below).
@@ -117,6 +127,12 @@ struct Buffers {
   float2 *m_mg;                           // MUSCL only: FP32 copy of mg
 
   int kstride;                            // stride of the 3 RK stage slabs (= local capacity)
+  // the moving-square program: initial type per gid (-4 body), the body-force parts per
+  // local particle (pressure, viscous) and its near-body flag, per-block partial sums
+  int *g_init;
+  double2 *bp, *bv;
+  int *near_body;
+  double *bpart;
 };
 
 // Records the message sph2d_last_error returns and passes the code through.
@@ -190,6 +206,23 @@ struct GridArgs {
 void launch_grid_mls(hipStream_t s, const DevParams& P, Buffers& B, const GridArgs& G);
 void launch_end_damping(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
 void launch_assign_walls(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+// The moving-square program (sph2d_ms.hip). The fixed ghosts of the tank and the body
+// (Assign_boundary_particles.f90:611-760); one MS evaluation after the ghosts are
+// assigned (pass A: gradients, strain and tau, XSPH; force pass with the body-force
+// parts); the body force totals into out6 (device, 6 doubles); the MS predictor and
+// corrector (SPH_Time_Iteration.f90:313-347); the order-3 density MLS into mls_rho;
+// the body's move by (ddx, ddy); fluid outside the tank typed 120.
+void launch_ms_assign_walls(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+void launch_ms_eval(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+void launch_ms_body_sum(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, double* out6);
+void launch_ms_stage(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, int kind);
+void launch_ms_mls(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
+void launch_ms_body_move(hipStream_t s, const DevParams& P, Buffers& B, int n_total, double ddx, double ddy);
+void launch_ms_ill(hipStream_t s, const DevParams& P, Buffers& B, int cap_fluid);
+// host layout of the moving-square tank (x, y, btype, init may be null: count only)
+int ms_ghost_layout(double box_x, double box_y, double gdx, double gdy, int layers, double cx, double cy,
+                    double len, double* x, double* y, int* btype, int* init);
+enum MsStage { MS_PREDICT = 0, MS_CORRECT = 1 };
 void launch_wall_refresh(hipStream_t s, const DevParams& P, Buffers& B, int cap_local);
 // Distribute_Initial_Particle.f90:317-495 and Distribute_Fixed_Boundary_Particle.f90:52-104:
 // positions and boundary types of the fixed ghost layers (x, y, type may be null: count only)

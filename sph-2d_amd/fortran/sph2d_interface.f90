@@ -36,6 +36,8 @@ module sph2d_interface
     integer(c_int32_t) :: wavemaker_layers
     integer(c_int32_t) :: sps_on
     integer(c_int32_t) :: renormalization
+    integer(c_int32_t) :: ms
+    real(c_double) :: background_p, release_time, body_cx, body_cy, body_len, body_speed, body_ramp
   end type sph2d_params
 
   interface
@@ -62,6 +64,25 @@ module sph2d_interface
     function sph2d_init_tgv(ctx) bind(C, name="sph2d_init_tgv") result(rc)
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    ! the moving-square program (Moving_Square_Or_Cylinder_Inside_a_Rectangular_Box/)
+    function sph2d_init_moving_square(ctx) bind(C, name="sph2d_init_moving_square") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function sph2d_body_motion(ctx, t) bind(C, name="sph2d_body_motion") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: t
+      integer(c_int) :: rc
+    end function
+    function sph2d_body_force(ctx, out6, step) bind(C, name="sph2d_body_force") result(rc)
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double) :: out6(6)
+      integer(c_int32_t) :: step
       integer(c_int) :: rc
     end function
 

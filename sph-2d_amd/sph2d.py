@@ -24,6 +24,8 @@ SOLVER_MODIFIED_LINEAR_RIEMANN = 0
 SOLVER_LINEAR_RIEMANN = 1
 SOLVER_HLLC = 2
 SOLVER_MUSCL = 3
+SOLVER_MS_MUSCL_HLLC = 4
+SOLVER_MS_HLLC = 5
 INTEGRATOR_PC = 2
 INTEGRATOR_RK = 3
 FP64 = 0
@@ -68,6 +70,9 @@ class Params(ctypes.Structure):
         ("walls", ctypes.c_int32), ("gravity", ctypes.c_double),
         ("wavemaker_layers", ctypes.c_int32),
         ("sps_on", ctypes.c_int32), ("renormalization", ctypes.c_int32),
+        ("ms", ctypes.c_int32), ("background_p", ctypes.c_double), ("release_time", ctypes.c_double),
+        ("body_cx", ctypes.c_double), ("body_cy", ctypes.c_double), ("body_len", ctypes.c_double),
+        ("body_speed", ctypes.c_double), ("body_ramp", ctypes.c_double),
     ]
 
 
@@ -157,6 +162,7 @@ EXPORTS = (
     "sph2d_forward", "sph2d_backward",
     "sph2d_relaxation_init", "sph2d_wave_theory_update", "sph2d_wave_maker_damping_zone",
     "sph2d_device_alloc", "sph2d_device_free", "sph2d_device_copy",
+    "sph2d_moving_square_layout", "sph2d_init_moving_square", "sph2d_body_motion", "sph2d_body_force",
 )
 
 _lib = None
@@ -262,6 +268,10 @@ def lib():
             "sph2d_device_alloc": [ctypes.c_size_t, P(vp)],
             "sph2d_device_free": [vp],
             "sph2d_device_copy": [vp, vp, ctypes.c_size_t, i32],
+            "sph2d_moving_square_layout": [P(Params), i32, dp, dp, ip, ip, ip, ip],
+            "sph2d_init_moving_square": [vp],
+            "sph2d_body_motion": [vp, ctypes.c_double],
+            "sph2d_body_force": [vp, dp, ip],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -301,6 +311,43 @@ def tgv_params(n_side: int, **kw) -> Params:
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+def moving_square_params(n_per_d: int = 100, re: float = 100.0, tank=(10.0, 5.0), layers: int = 3, **kw) -> Params:
+    """The moving square (the reference's Moving_Square_Or_Cylinder_Inside_a_Rectangular_Box
+    program, ms = 1): a tank of tank[0] x tank[1] body lengths D = 1 with `layers` layers of
+    fixed ghosts, the square of side D centred at (2.5, tank[1]/2), dx = D / n_per_d,
+    h = 1.15 dx (Information_Module.f90:616), c0 = 10 U (c0sq = 100), rho0 = 1, nu = U D /
+    Re, the body's speed ramped linearly to U = 1 over t = 0.5; MUSCL-HLLC + delta-SPH +
+    SPS with XSPH epsilon 0.1, shifting (max 0.15 dx), MLS order 3 every 40 steps, the
+    predictor-corrector at dt = 0.2 * 0.25 h / (1.1 c0) (the CFL form of
+    Initialize_SPH_Calculation.f90), body force every 100 steps."""
+    dx = 1.0 / n_per_d
+    h = 1.15 * dx
+    p = default_params(nx=int(round(tank[0] * n_per_d)), ny=int(round(tank[1] * n_per_d)), box_x=tank[0],
+                       box_y=tank[1], dx=dx, h=h, rho0=1.0, c0sq=100.0, nu=1.0 / re, U=1.0,
+                       dt=0.2 * 0.25 * h / (1.1 * 10.0), integrator=INTEGRATOR_PC, solver=SOLVER_MS_MUSCL_HLLC,
+                       sps_on=1, shift_on=1, max_shift=0.15 * dx, mls_on=1, mls_every=40, mls_order=3,
+                       mls_alpha=1e-5, alpha=0.03, beta=2.0, fai=0.1, delta=0.1, delta_alpha=0.02,
+                       epsilon_xsph=0.1, sample_every=100, periodic=0, walls=layers, ms=1, background_p=0.0,
+                       release_time=10.0, body_cx=2.5, body_cy=tank[1] / 2, body_len=1.0, body_speed=1.0,
+                       body_ramp=0.5, water_depth=tank[1])
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def moving_square_layout(params: Params):
+    """(x, y, btype, init, n_fluid) of the moving-square tank (sph2d_moving_square_layout)."""
+    nf, nt = ctypes.c_int32(), ctypes.c_int32()
+    _check(lib().sph2d_moving_square_layout(ctypes.byref(params), 0, None, None, None, None, ctypes.byref(nf),
+                                            ctypes.byref(nt)))
+    n = nt.value
+    x, y = np.zeros(n), np.zeros(n)
+    bt, it = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.int32)
+    _check(lib().sph2d_moving_square_layout(ctypes.byref(params), n, _dp(x), _dp(y), _ip(bt), _ip(it),
+                                            ctypes.byref(nf), ctypes.byref(nt)))
+    return x, y, bt, it, nf.value
 
 
 def uneven_tgv_instances(count: int = 64, seed: int = 3):
@@ -354,6 +401,23 @@ class Simulation:
     def init_tgv(self):
         _check(lib().sph2d_init_tgv(self._c))
         self.n_fluid = self.params.nx * self.params.ny
+
+    # --- the moving-square program (ms = 1) --------------------------------------------
+    def init_moving_square(self):
+        _check(lib().sph2d_init_moving_square(self._c))
+        self.n_fluid = moving_square_layout(self.params)[4]
+
+    def body_motion(self, t: float):
+        """Body_Motion_Solver at physical time t (time_step calls it itself)."""
+        _check(lib().sph2d_body_motion(self._c, float(t)))
+
+    def body_force(self):
+        """(pressure x, y, viscous x, y, total x, y) on the water from the last sampling step's
+        evaluation, and that step (0: none yet)."""
+        out = np.zeros(6)
+        st = ctypes.c_int32()
+        _check(lib().sph2d_body_force(self._c, _dp(out), ctypes.byref(st)))
+        return out, st.value
 
     def upload_state(self, x, y, u, v, rho, mass, energy=None, type_=None):
         n = len(x)

@@ -186,3 +186,25 @@ def test_solitary_paddle_follows_goring():
     assert abs(us.max() - H * c / (d + H)) < 1e-3 * H * c / (d + H)
     fd = np.gradient(xs, np.array(ts))
     assert np.max(np.abs(fd - us)) < 0.02 * us.max()
+
+
+def test_moving_square_layout_matches_the_oracle():
+    """sph2d_moving_square_layout (a host routine, no GPU) and the oracle's restatement of
+    MS/Initial_Particle_From_First_Step.f90 give the same particles in the same order:
+    positions, boundary types and initial types bit for bit."""
+    if not os.path.exists(sph2d.LIB_PATH):
+        pytest.skip("libsph2d.so not built")
+    from oracle import pyoracle as po
+    for (nx, ny, bx, by, cx, cy, L, layers) in ((100, 48, 2.0, 0.96, 0.6, 0.48, 0.4, 3),
+                                                (200, 100, 10.0, 5.0, 2.5, 2.5, 1.0, 4)):
+        p = sph2d.moving_square_params(n_per_d=int(round(nx / bx)), tank=(bx, by), layers=layers,
+                                       nx=nx, ny=ny, body_cx=cx, body_cy=cy, body_len=L)
+        x, y, bt, it, nf = sph2d.moving_square_layout(p)
+        o = po.Oracle(ms=1, nx=nx, ny=ny, box_x=bx, box_y=by, body_cx=cx, body_cy=cy, body_len=L, walls=layers,
+                      h_factor=1.15)
+        o.init_moving_square()
+        n = o.n_global
+        assert nf == o.n_fluid and len(x) == n
+        assert np.array_equal(x, o.f64("g_x")[:n]) and np.array_equal(y, o.f64("g_y")[:n])
+        assert np.array_equal(it, o.i32("g_initial_type")[:n])
+        assert np.array_equal(bt[nf:], o.i32("g_btype")[nf:n])
