@@ -113,8 +113,10 @@ def test_ms_single_evaluation_parity(solver, sps_on):
         e = particle_rel_err(sim.local(k)[f], o.f64(k)[f], o.f64("mag_" + k)[f])
         say(k, e)
         assert e < 1e-12, k
+    o.shift_sums()                 # Shift_particle_position.f90:78-103 over this evaluation's pairs
     for k in ("shift_x", "shift_y"):
         e = rel_err(sim.local(k)[f], o.f64(k)[f])
+        say(k, e)
         assert e < 1e-12, k
     bf, step = sim.body_force()
     ob = o.ms_body_force()
