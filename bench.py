@@ -368,6 +368,67 @@ def adjoint_leg(sph2d, steps: int, k: int = 20) -> dict:
             "grad_norm": float(np.linalg.norm(g))}
 
 
+def moving_square_leg(sph2d, steps: int, res, every: int = 100) -> dict:
+    """The moving square (the reference's Moving_Square_Or_Cylinder_Inside_a_Rectangular_Box
+    program): 10 D x 5 D tank, dx = D/100 (490,000 fluid particles, 3 ghost layers), the
+    square ramped to U = 1 over t = 0.5, MUSCL-HLLC + delta-SPH + SPS + shifting, the FP64
+    and FP32-mixed forms for each Re. Reports each form's particle-steps/s (device time of
+    the step loop, body-force downloads excluded) and the force on the water every `every`
+    steps (in units of 0.5 rho0 U^2 D), and the mixed form's drag and lift against FP64."""
+    import numpy as np
+    hip = hip_events()
+    out = {"workload": f"moving square, 10D x 5D tank, dx = D/100, {steps} steps per form, MUSCL-HLLC + "
+                       f"delta-SPH + SPS + shifting, MLS order 3 every 40 steps", "steps": steps, "cases": {}}
+    for re_ in res:
+        case = {}
+        for prec, name in ((sph2d.FP64, "fp64"), (sph2d.FP32_MIXED, "mixed")):
+            p = sph2d.moving_square_params(n_per_d=100, re=float(re_), precision=prec, sample_every=every)
+            sim = sph2d.Simulation(p, capacity=p.nx * p.ny)
+            sim.init_moving_square()
+            stream = ctypes.c_void_p(sim.stream())
+            e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+            hip.hipEventCreate(ctypes.byref(e0))
+            hip.hipEventCreate(ctypes.byref(e1))
+            hist, dev_ms = [], 0.0
+            it = 1
+            while it <= steps:
+                k = min(every - (it - 1) % every, steps - it + 1)
+                hip.hipEventRecord(e0, stream)
+                sim.run(it, k)
+                hip.hipEventRecord(e1, stream)
+                hip.hipEventSynchronize(e1)
+                ms = ctypes.c_float()
+                hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+                dev_ms += ms.value
+                it += k
+                if (it - 1) % every == 0:
+                    bf, st = sim.body_force()
+                    hist.append([st * p.dt] + [float(v) / 0.5 for v in bf])
+            hip.hipEventDestroy(e0)
+            hip.hipEventDestroy(e1)
+            d = sim.diagnostics()
+            g = sim.download_state()
+            case[name] = {"particle_steps_per_s": sim.n_fluid * steps / (dev_ms * 1e-3),
+                          "ms_per_step": dev_ms / steps, "n_local": d.n_local,
+                          "history": hist, "finite": bool(np.all(np.isfinite(g["u"])) and np.all(np.isfinite(g["v"]))),
+                          "error_flags": d.error_flags, "rebin_needed": d.rebin_needed}
+            sim.close()
+            print(f"# moving square Re {re_} {name}: {dev_ms / steps:.3f} ms/step", file=sys.stderr, flush=True)
+        h64 = np.array(case["fp64"]["history"])
+        h32 = np.array(case["mixed"]["history"])
+        drag64, drag32 = h64[:, 5], h32[:, 5]
+        lift64, lift32 = h64[:, 6], h32[:, 6]
+        scale = np.max(np.abs(drag64))
+        case["t_end"] = float(h64[-1, 0])
+        case["total_drag_coefficient_end"] = {"fp64": float(drag64[-1]), "mixed": float(drag32[-1])}
+        case["total_lift_coefficient_end"] = {"fp64": float(lift64[-1]), "mixed": float(lift32[-1])}
+        case["mixed_vs_fp64_drag_max_rel"] = float(np.max(np.abs(drag32 - drag64)) / scale)
+        case["mixed_vs_fp64_lift_max_rel_to_drag"] = float(np.max(np.abs(lift32 - lift64)) / scale)
+        case["mixed_over_fp64_speedup"] = case["mixed"]["particle_steps_per_s"] / case["fp64"]["particle_steps_per_s"]
+        out["cases"][str(re_)] = case
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -402,6 +463,9 @@ def main() -> int:
                     help="reverse of one evaluation on an n^2 TGV lattice (0 skips it)")
     ap.add_argument("--adjoint-steps", type=int, default=4000,
                     help="wave-tank adjoint leg: steps of sph2d_forward/sph2d_backward (0 skips it)")
+    ap.add_argument("--ms-steps", type=int, default=5000,
+                    help="moving-square leg: steps per form and Reynolds number (0 skips it)")
+    ap.add_argument("--ms-re", type=str, default="50,100,150", help="moving-square leg: Reynolds numbers")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU form (spatial slabs "
@@ -585,6 +649,11 @@ def main() -> int:
             out["adjoint"] = adjoint_leg(sph2d, args.adjoint_steps)
         except Exception as e:                    # the headline line is printed regardless
             out["adjoint"] = {"error": repr(e)}
+    if args.ms_steps > 0:
+        try:
+            out["moving_square"] = moving_square_leg(sph2d, args.ms_steps, [float(v) for v in args.ms_re.split(",")])
+        except Exception as e:                    # the headline line is printed regardless
+            out["moving_square"] = {"error": repr(e)}
     if args.adjoint_side > 0:
         try:
             out.setdefault("adjoint", {})["evaluation_vjp"] = evaluation_vjp_leg(sph2d, args.adjoint_side, 5)
