@@ -67,6 +67,16 @@ module sph2d_interface
       integer(c_int) :: rc
     end function
     ! the moving-square program (Moving_Square_Or_Cylinder_Inside_a_Rectangular_Box/)
+    ! x, y, btype, init may be c_null_ptr (counts only) or c_loc of arrays of cap entries
+    function sph2d_moving_square_layout(p, cap, x, y, btype, init, n_fluid, n_total) &
+        bind(C, name="sph2d_moving_square_layout") result(rc)
+      import :: c_int, c_int32_t, c_ptr, sph2d_params
+      type(sph2d_params), intent(in) :: p
+      integer(c_int32_t), value :: cap
+      type(c_ptr), value :: x, y, btype, init
+      integer(c_int32_t), intent(out) :: n_fluid, n_total
+      integer(c_int) :: rc
+    end function
     function sph2d_init_moving_square(ctx) bind(C, name="sph2d_init_moving_square") result(rc)
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
