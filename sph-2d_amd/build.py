@@ -2,7 +2,8 @@
 
 Translation units: the FP64 step and the FP64 force evaluation (no FMA
 contraction, for parity with the reference build), the FP32-mixed force
-evaluation (contraction allowed), the C-ABI/host driver and the host-side
+evaluation (contraction within each expression only, so every kernel that
+inlines the same pair terms rounds them alike), the C-ABI/host driver and the host-side
 writers of the reference's file formats. Packed FP32
 instructions are disabled on every HIP line.
 """
@@ -27,7 +28,7 @@ COMMON = [
 UNITS = [
     ("sph2d_fp64.hip", ["-ffp-contract=off"]),
     ("sph2d_eval64.hip", ["-ffp-contract=off"]),
-    ("sph2d_eval32.hip", ["-ffp-contract=fast"]),
+    ("sph2d_eval32.hip", ["-ffp-contract=on"]),
     ("sph2d_reorder.hip", ["-ffp-contract=off"]),
     ("sph2d_capi.hip", ["-ffp-contract=off"]),
     ("sph2d_output.hip", ["-ffp-contract=off"]),

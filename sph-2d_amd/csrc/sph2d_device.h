@@ -44,9 +44,13 @@ __device__ __forceinline__ void write_r0(const DevParams& P, const Buffers& B, i
 
 // Offset of a position from the lower-left corner of its binned cell (the cell of
 // Devide_block's binning: x_cell = chain_origin_x + cx * chain_dx).
+// FP32 offset of a position to the centre of its cell in x (|offset| <= d/2) and to
+// the centre of its pair of cell rows (rows 2k and 2k+1) in y (|offset| <= d): the
+// mixed force pass recovers a pair's y offset modulo two rows (see c32_pair).
 __device__ __forceinline__ float2 cell_offset(const DevParams& P, double2 x, int cell) {
   const int cx = cell / P.chain_ny, cy = cell - cx * P.chain_ny;
-  return make_float2((float)(x.x - (P.chain_ox + cx * P.chain_d)), (float)(x.y - (P.chain_oy + cy * P.chain_d)));
+  return make_float2((float)(x.x - (P.chain_ox + (cx + 0.5) * P.chain_d)),
+                     (float)(x.y - (P.chain_oy + ((cy & ~1) + 1) * P.chain_d)));
 }
 
 __device__ __forceinline__ double tait_p(const DevParams& P, double rho) {

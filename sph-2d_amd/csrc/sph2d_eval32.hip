@@ -9,9 +9,9 @@
 // and stored rounded to FP32 (m_pc), so the pair sums never evaluate an EOS.
 // Density enters the pair sums as delta rho = rho - rho0 (FP32), so rho_o - rho_s
 // carries no cancellation. In the force pass the offset x_s - x_o comes from FP32
-// offsets to each particle's cell corner plus the whole-cell difference, which
-// keeps it to ~1e-7 h; a neighbour costs 56 bytes (three float4 records and the
-// (p, c) pair) there, 32 in pass A.
+// offsets to each particle's cell centre (in y: the centre of its pair of cell
+// rows) plus the whole-cell difference, which keeps it to ~1e-7 h; a neighbour
+// costs 56 bytes (three float4 records and the (p, c) pair) there, 32 in pass A.
 #include <cstring>
 #include "sph2d_internal.h"
 #include "sph2d_device.h"
@@ -35,7 +35,7 @@ struct KSum {
 struct DevParams32 {
   float inv_h, e9, kfactor, inv_sv, rho0, inv_c0, h, fai_h2, alpha, beta, two_nu;
   float delta_alpha_h, two_delta_h, shift_beta_h2, shift_R, inv_wdx, epsilon2, x_over_t;
-  float cell_d;
+  float cell_d, two_cd, inv_two_cd;
 };
 
 static DevParams32 make32(const DevParams& P) {
@@ -59,11 +59,13 @@ static DevParams32 make32(const DevParams& P) {
   Q.epsilon2 = (float)(P.epsilon_xsph * P.xsph_f);
   Q.x_over_t = (float)P.x_over_t;
   Q.cell_d = (float)P.chain_d;
+  Q.two_cd = (float)(2 * P.chain_d);
+  Q.inv_two_cd = (float)(1.0 / (2 * P.chain_d));
   return Q;
 }
 
 struct KernelTerm32 {
-  float w, dwx, dwy;
+  float w, dwx, dwy, t;   // grad W = t (dx, dy)
 };
 
 __device__ __forceinline__ KernelTerm32 kernel_term32(const DevParams32& Q, float dx, float dy, float r,
@@ -73,9 +75,117 @@ __device__ __forceinline__ KernelTerm32 kernel_term32(const DevParams32& Q, floa
   const float eq = __expf(-(q * q));
   k.w = (eq - Q.e9) * Q.kfactor;
   const float t = eq * (-2.0f * q) * (inv_r * Q.inv_h) * Q.kfactor;
+  k.t = t;
   k.dwx = t * dx;
   k.dwy = t * dy;
   return k;
+}
+
+// The next set bit of a 256-bit candidate mask (words m0..m3 of chunk c0), walked
+// one word at a time: cur is the rest of word wi. Returns the candidate number, or
+// -1 when the mask is spent; ascending order.
+__device__ __forceinline__ int next_slot(uint64_t& cur, int& wi, uint64_t m1, uint64_t m2, uint64_t m3, int c0) {
+  while (cur == 0) {
+    if (wi >= 3) return -1;
+    ++wi;
+    cur = wi == 1 ? m1 : (wi == 2 ? m2 : m3);
+  }
+  const int b = __builtin_ctzll(cur);
+  cur &= cur - 1;
+  return c0 + 64 * wi + b;
+}
+
+// Pass A: the particle's own values, one survivor of the screen as loaded, and
+// one survivor's terms.
+struct A32Self {
+  double2 ps;
+  float2 vs;
+  float drs, pres;
+  bool fluid;
+};
+
+struct A32Rec {
+  float4 r0;
+  double2 pos;
+  float p;
+  int kk;
+};
+
+struct A32Terms {
+  float fra, sdx, sdy, sxx, sxy, syy, grx, gry, gpx, gpy, gux, guy, gvx, gvy;
+};
+
+__device__ __forceinline__ A32Terms operator+(const A32Terms& a, const A32Terms& b) {
+  return A32Terms{a.fra + b.fra, a.sdx + b.sdx, a.sdy + b.sdy, a.sxx + b.sxx, a.sxy + b.sxy,
+                  a.syy + b.syy, a.grx + b.grx, a.gry + b.gry, a.gpx + b.gpx, a.gpy + b.gpy,
+                  a.gux + b.gux, a.guy + b.guy, a.gvx + b.gvx, a.gvy + b.gvy};
+}
+
+// Pass A's sums, Kahan-compensated over the terms of one survivor or the FP32 sum
+// of two (as the force pass's sums).
+template <bool MUSCL>
+struct A32Sums {
+  KSum fra, sdx, sdy, Sxx, Sxy, Syy, grx, gry, gpx, gpy, gux, guy, gvx, gvy;
+  __device__ __forceinline__ void add(const A32Terms& t) {
+    fra.add(t.fra);
+    sdx.add(t.sdx);
+    sdy.add(t.sdy);
+    Sxx.add(t.sxx);
+    Sxy.add(t.sxy);
+    Syy.add(t.syy);
+    grx.add(t.grx);
+    gry.add(t.gry);
+    if (MUSCL) {
+      gpx.add(t.gpx);
+      gpy.add(t.gpy);
+      gux.add(t.gux);
+      guy.add(t.guy);
+      gvx.add(t.gvx);
+      gvy.add(t.gvy);
+    }
+  }
+};
+
+// One survivor of the FP32 screen: the exact FP64 pair test of find_particle_pair.f90
+// :87 (acc), then the pass A terms of k_eval_a (sph2d_eval64.hip) in FP32. A survivor
+// that fails the test, or a pair of two non-fluid particles, gets the volume zero,
+// so every term is zero without a branch (r^2 is kept above 1e-30 so that a
+// coincident survivor's terms stay finite).
+template <bool MUSCL>
+__device__ __forceinline__ A32Terms a32_terms(const DevParams& P, const DevParams32& Q, const A32Self& S,
+                                              const A32Rec& N, bool& acc) {
+  const double ddx = S.ps.x - N.pos.x, ddy = S.ps.y - N.pos.y;
+  acc = is_pair_r2(P, ddx, ddy);
+  const bool pair = acc && (S.fluid || N.r0.w >= 0.f);
+  const float dx = (float)ddx, dy = (float)ddy;
+  const float r2 = fmaxf(dx * dx + dy * dy, 1e-30f);
+  const float inv_r = __builtin_amdgcn_rsqf(r2);
+  const float r = r2 * inv_r;
+  const KernelTerm32 K = kernel_term32(Q, dx, dy, r, inv_r);
+  const float Vo = pair ? fabsf(N.r0.w) * __builtin_amdgcn_rcpf(Q.rho0 + N.r0.z) : 0.f;
+  A32Terms T;
+  T.fra = Vo * Q.inv_sv;
+  T.sdx = K.dwx * Vo;
+  T.sdy = K.dwy * Vo;
+  const float vdx = N.r0.x - S.vs.x, vdy = N.r0.y - S.vs.y;
+  T.sxx = vdx * K.dwx * Vo;
+  T.sxy = (vdx * K.dwy + vdy * K.dwx) * Vo;
+  T.syy = vdy * K.dwy * Vo;
+  const float rd = (N.r0.z - S.drs) * Vo;
+  T.grx = rd * K.dwx;
+  T.gry = rd * K.dwy;
+  T.gpx = T.gpy = T.gux = T.guy = T.gvx = T.gvy = 0.f;
+  if (MUSCL) {
+    const float pd = (N.p - S.pres) * Vo;
+    T.gpx = pd * K.dwx;
+    T.gpy = pd * K.dwy;
+    const float ud = vdx * Vo, vd = vdy * Vo;
+    T.gux = ud * K.dwx;
+    T.guy = ud * K.dwy;
+    T.gvx = vd * K.dwx;
+    T.gvy = vd * K.dwy;
+  }
+  return T;
 }
 
 // pass A (mixed): see k_eval_a in sph2d_eval64.hip for the reference lines.
@@ -95,16 +205,17 @@ __device__ __forceinline__ void k_eval_a32_body(const DevParams& P, const DevPar
   const float drs = r0s.z;
   const float pres = MUSCL ? B.m_pc[s].x : 0.f;   // FP64 Tait law, rounded to FP32
   const int ts = B.l_type[s];
-  KSum fra, sdx, sdy, Sxx, Sxy, Syy, grx, gry;
-  KSum gpx, gpy, gux, guy, gvx, gvy;
+  A32Sums<MUSCL> A;
   const __amdgpu_buffer_rsrc_t br0 = rec_rsrc(B.m_r0), bpos = rec_rsrc((const float4*)B.l_pos);
   const __amdgpu_buffer_rsrc_t bpc = rec_rsrc(reinterpret_cast<const float4*>(B.m_pc));
   // find_particle_pair.f90:64-123 written out (the shared neighbour loop records
   // the same slots): the FP32 screen of the 3x3 stencil, then the survivors in
-  // ascending order, each with the exact FP64 test of :87; the next survivor's
-  // loads are issued before the current survivor's test and sums, so their
-  // latency overlaps that work. The accepted slots are recorded for the later
-  // passes (one byte each, kNbOverflow when they do not fit).
+  // ascending order, each with the exact FP64 test of :87. Two survivors' records
+  // are in flight in turn (the next one's loads are issued before the current one's
+  // test and sums, so their latency overlaps that work, and no loaded record is
+  // copied between registers); the sums take the two survivors' terms together.
+  // The accepted slots are recorded for the later passes (one byte each,
+  // kNbOverflow when they do not fit).
   const float2 ps32 = B.l_pos32[s];
   const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
   const int n0 = R.hi[0] - R.lo[0], n1 = R.hi[1] - R.lo[1], n2 = R.hi[2] - R.lo[2];
@@ -119,110 +230,84 @@ __device__ __forceinline__ void k_eval_a32_body(const DevParams& P, const DevPar
   const int lst = B.kstride;
   int nrec = 0;
   uint32_t word = 0;
+  A32Self S;
+  S.ps = ps;
+  S.vs = vs;
+  S.drs = drs;
+  S.pres = pres;
+  S.fluid = ts == 2;
+#define SPH2D_A32_FETCH(R, KK)                                                              \
+  do {                                                                                     \
+    const int kk_ = (KK);                                                                  \
+    const bool a_ = kk_ < n0, b_ = kk_ < n01;                                              \
+    R.kk = kk_;                                                                            \
+    const int o_ = kk_ + (a_ ? l0 : (b_ ? l1 : l2));                                       \
+    R.r0 = buf_f4(br0, o_);                                                                \
+    if (MUSCL) R.p = buf_f2(bpc, o_).x;                                                    \
+    if (staged) R.pos = lds64[kk_ + (a_ ? so0 : (b_ ? so1 : so2))];                        \
+    else R.pos = buf_d2(bpos, o_);                                                         \
+  } while (0)
+  // the accepted slot kk joins the record (ascending order)
+#define SPH2D_A32_RECORD(KK)                                                                \
+  do {                                                                                     \
+    if (stored) {                                                                          \
+      if (nrec < kNbListCap) {                                                             \
+        word |= (uint32_t)(KK) << ((nrec & 3) << 3);                                       \
+        if ((nrec & 3) == 3) {                                                             \
+          nbl[(nrec >> 2) * lst + s] = word;                                               \
+          word = 0;                                                                        \
+        }                                                                                  \
+      }                                                                                    \
+      ++nrec;                                                                              \
+    }                                                                                      \
+  } while (0)
   for (int c0 = 0; c0 < total; c0 += 256) {
     const Mask256 sm = screen_chunk(P, B.l_pos32, s, ps32, R, n0, n1, n2, n01, total, l0, l1, l2, staged, so0, so1,
                                     so2, lds32, c0);
-    uint64_t m0 = sm.w0, m1 = sm.w1, m2 = sm.w2, m3 = sm.w3;
-    int kk_n = -1;
-    if ((m0 | m1 | m2 | m3) != 0) {
-      const int w = m0 ? 0 : (m1 ? 1 : (m2 ? 2 : 3));
-      const uint64_t mm = m0 ? m0 : (m1 ? m1 : (m2 ? m2 : m3));
-      const int b = __builtin_ctzll(mm);
-      const uint64_t bit = 1ull << b;
-      if (w == 0) m0 ^= bit; else if (w == 1) m1 ^= bit; else if (w == 2) m2 ^= bit; else m3 ^= bit;
-      kk_n = c0 + w * 64 + b;
-    }
-    float4 r0n = make_float4(0.f, 0.f, 0.f, 0.f);
-    double2 pn = make_double2(0.0, 0.0);
-    if (kk_n >= 0) {
-      const int dn = kk_n < n0 ? 0 : (kk_n < n01 ? 1 : 2);
-      const int on = dn == 0 ? l0 + kk_n : (dn == 1 ? l1 + kk_n : l2 + kk_n);
-      r0n = buf_f4(br0, on);
-      if (staged) pn = lds64[dn == 0 ? so0 + kk_n : (dn == 1 ? so1 + kk_n : so2 + kk_n)];
-      else pn = buf_d2(bpos, on);
-    }
-    while (kk_n >= 0) {
-      const int kk = kk_n;
-      const int d = kk < n0 ? 0 : (kk < n01 ? 1 : 2);
-      const int o = d == 0 ? l0 + kk : (d == 1 ? l1 + kk : l2 + kk);
-      const float4 r0o = r0n;
-      const double2 po = pn;
-      kk_n = -1;
-      if ((m0 | m1 | m2 | m3) != 0) {
-        const int w = m0 ? 0 : (m1 ? 1 : (m2 ? 2 : 3));
-        const uint64_t mm = m0 ? m0 : (m1 ? m1 : (m2 ? m2 : m3));
-        const int b = __builtin_ctzll(mm);
-        const uint64_t bit = 1ull << b;
-        if (w == 0) m0 ^= bit; else if (w == 1) m1 ^= bit; else if (w == 2) m2 ^= bit; else m3 ^= bit;
-        kk_n = c0 + w * 64 + b;
-        const int dn = kk_n < n0 ? 0 : (kk_n < n01 ? 1 : 2);
-        const int on = dn == 0 ? l0 + kk_n : (dn == 1 ? l1 + kk_n : l2 + kk_n);
-        r0n = buf_f4(br0, on);
-        if (staged) pn = lds64[dn == 0 ? so0 + kk_n : (dn == 1 ? so1 + kk_n : so2 + kk_n)];
-        else pn = buf_d2(bpos, on);
+    uint64_t cur = sm.w0;
+    int wi = 0;
+    A32Rec ra, rb;
+    int ka = next_slot(cur, wi, sm.w1, sm.w2, sm.w3, c0);
+    if (ka >= 0) SPH2D_A32_FETCH(ra, ka);
+    while (ka >= 0) {
+      const int kb = next_slot(cur, wi, sm.w1, sm.w2, sm.w3, c0);
+      if (kb >= 0) SPH2D_A32_FETCH(rb, kb);
+      bool acc;
+      const A32Terms ta = a32_terms<MUSCL>(P, Q, S, ra, acc);
+      if (acc) SPH2D_A32_RECORD(ra.kk);
+      if (kb < 0) {
+        A.add(ta);
+        break;
       }
-      if (!is_pair_r2(P, ps.x - po.x, ps.y - po.y)) continue;
-      if (stored) {
-        if (nrec < kNbListCap) {
-          word |= (uint32_t)kk << ((nrec & 3) << 3);
-          if ((nrec & 3) == 3) {
-            nbl[(nrec >> 2) * lst + s] = word;
-            word = 0;
-          }
-        }
-        ++nrec;
-      }
-      if (ts != 2 && r0o.w < 0.f) continue;        // no pair between two non-fluid particles
-      const float dx = (float)(ps.x - po.x), dy = (float)(ps.y - po.y);
-      const float r2 = dx * dx + dy * dy;
-      const float inv_r = __builtin_amdgcn_rsqf(r2);   // r2 >= 1e-14: no denormal scaling
-      const float r = r2 * inv_r;
-      const KernelTerm32 K = kernel_term32(Q, dx, dy, r, inv_r);
-      const float2 rvo = make_float2(r0o.z, fabsf(r0o.w) * __builtin_amdgcn_rcpf(Q.rho0 + r0o.z));
-      const float2 vo = make_float2(r0o.x, r0o.y);
-      const float Vo = rvo.y;
-      fra.add(Vo * Q.inv_sv);
-      sdx.add(K.dwx * Vo);
-      sdy.add(K.dwy * Vo);
-      const float vdx = vo.x - vs.x, vdy = vo.y - vs.y;
-      Sxx.add(vdx * K.dwx * Vo);
-      Sxy.add((vdx * K.dwy + vdy * K.dwx) * Vo);
-      Syy.add(vdy * K.dwy * Vo);
-      const float rd = (rvo.x - drs) * Vo;
-      grx.add(rd * K.dwx);
-      gry.add(rd * K.dwy);
-      if (MUSCL) {
-        const float pd = (buf_f2(bpc, o).x - pres) * Vo;
-        gpx.add(pd * K.dwx);
-        gpy.add(pd * K.dwy);
-        const float ud = vdx * Vo, vd = vdy * Vo;
-        gux.add(ud * K.dwx);
-        guy.add(ud * K.dwy);
-        gvx.add(vd * K.dwx);
-        gvy.add(vd * K.dwy);
-      }
+      ka = next_slot(cur, wi, sm.w1, sm.w2, sm.w3, c0);
+      if (ka >= 0) SPH2D_A32_FETCH(ra, ka);
+      const A32Terms tb = a32_terms<MUSCL>(P, Q, S, rb, acc);
+      if (acc) SPH2D_A32_RECORD(rb.kk);
+      A.add(ta + tb);
     }
   }
+#undef SPH2D_A32_FETCH
+#undef SPH2D_A32_RECORD
   if (stored && (nrec & 3) && nrec < kNbListCap) nbl[(nrec >> 2) * lst + s] = word;
   B.nb_count[s] = nrec | ((!stored || nrec > kNbListCap) ? kNbOverflow : 0);
-  const double f = (double)fra.s;
+  const double f = (double)A.fra.s;
   B.fra_v[s] = f;
-  B.sdw[s] = make_double2((double)sdx.s, (double)sdy.s);
+  B.sdw[s] = make_double2((double)A.sdx.s, (double)A.sdy.s);
   {
     const float4 r2s = B.m_r2[s];                  // keep the position offsets
-    B.m_r2[s] = make_float4(grx.s, gry.s, r2s.z, r2s.w);
+    B.m_r2[s] = make_float4(A.grx.s, A.gry.s, r2s.z, r2s.w);
   }
   if (MUSCL) {
-    B.m_mg[3 * s + 0] = make_float2(gpx.s, gpy.s);
-    B.m_mg[3 * s + 1] = make_float2(gux.s, guy.s);
-    B.m_mg[3 * s + 2] = make_float2(gvx.s, gvy.s);
-    B.mg[3 * s + 0] = make_double2((double)gpx.s, (double)gpy.s);
-    B.mg[3 * s + 1] = make_double2((double)gux.s, (double)guy.s);
-    B.mg[3 * s + 2] = make_double2((double)gvx.s, (double)gvy.s);
+    B.m_mg[3 * s + 0] = make_float2(A.gpx.s, A.gpy.s);
+    B.m_mg[3 * s + 1] = make_float2(A.gux.s, A.guy.s);
+    B.m_mg[3 * s + 2] = make_float2(A.gvx.s, A.gvy.s);
+    B.mg[3 * s + 0] = make_double2((double)A.gpx.s, (double)A.gpy.s);
+    B.mg[3 * s + 1] = make_double2((double)A.gux.s, (double)A.guy.s);
+    B.mg[3 * s + 2] = make_double2((double)A.gvx.s, (double)A.gvy.s);
   }
   double txx = 0.0, txy = 0.0, tyy = 0.0;
   if (ts == 2 && P.sps) {                          // SPS closure in FP64 (per particle)
-    const double sxx = Sxx.s, sxy = Sxy.s, syy = Syy.s;
+    const double sxx = A.Sxx.s, sxy = A.Sxy.s, syy = A.Syy.s;
     const double rhos = (double)drs + P.rho0;
     const double master = sxx * sxx + syy * syy + 0.0;
     const double last = sxy * sxy + 0.0 + 0.0;
@@ -242,9 +327,210 @@ __device__ __forceinline__ void k_eval_a32_body(const DevParams& P, const DevPar
   if (sf) atomicAdd(&B.counters[C_NSURF], 1);
 }
 
-template <bool MUSCL>
-__global__ void __launch_bounds__(kBlock) k_eval_a32(DevParams P, DevParams32 Q, Buffers B) {
-  k_eval_a32_body<MUSCL>(P, Q, B, xcd_block(), gridDim.x);
+// Pass A waits on its loads more than on its arithmetic: without the MUSCL
+// gradients it keeps 5 waves per SIMD (at most 96 registers, no spills), which hide
+// that latency better than the 4 that 104 registers would leave; the MUSCL form
+// needs its registers.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+k_eval_a32(DevParams P, DevParams32 Q, Buffers B) {
+  k_eval_a32_body<false>(P, Q, B, xcd_block(), gridDim.x);
+}
+
+__global__ void __launch_bounds__(kBlock) k_eval_a32_muscl(DevParams P, DevParams32 Q, Buffers B) {
+  k_eval_a32_body<true>(P, Q, B, xcd_block(), gridDim.x);
+}
+
+// The particle's own values in the force pass.
+struct C32Self {
+  float2 vs, pcs, grs, gps, gus, gvs;
+  float drs, rhos, inv_rhos, offx, offy;
+  float4 taus;
+  int s, ts, nes;
+};
+
+// One neighbour's contributions to the force pass's sums.
+struct C32Terms {
+  float drho, corr, ax, ay, shx, shy, agx, agy;
+};
+
+__device__ __forceinline__ C32Terms operator+(const C32Terms& a, const C32Terms& b) {
+  return C32Terms{a.drho + b.drho, a.corr + b.corr, a.ax + b.ax, a.ay + b.ay,
+                  a.shx + b.shx, a.shy + b.shy, a.agx + b.agx, a.agy + b.agy};
+}
+
+// The force pass's sums, Kahan-compensated. add() takes the contributions of one
+// neighbour or the FP32 sum of two (the listed loop advances two neighbours at a
+// time): the compensation then runs over neighbour pairs, and the one extra rounding
+// of a two-term sum is of the order of each term's own FP32 rounding.
+struct C32Sums {
+  KSum drho, corr, ax, ay, shx, shy, agx, agy;
+  __device__ __forceinline__ void add(const C32Terms& t, bool xsph, int do_shift) {
+    drho.add(t.drho);
+    corr.add(t.corr);
+    ax.add(t.ax);
+    ay.add(t.ay);
+    if (xsph) { agx.add(t.agx); agy.add(t.agy); }
+    if (do_shift) { shx.add(t.shx); shy.add(t.shy); }
+  }
+};
+
+// A neighbour's records as the force pass loads them: r0 (u, v, delta rho, +-m),
+// r1 (tau, m), r2 (grad rho, FP32 offsets), (p, c); its stencil column and slot.
+struct C32Rec {
+  float4 r0, r1, r2;
+  float2 pc;
+  int d, o;
+};
+
+// One recorded neighbour o (stencil column d) of the force pass: the pair terms of
+// k_eval_c (sph2d_eval64.hip) in FP32. The offset x_s - x_o is the difference of the
+// two particles' FP32 offsets to their cells' centres plus the whole-column
+// difference (1 - d) cells in x; in y the offsets are taken to the centre of the pair
+// of cell rows (rows 2k, 2k+1) that holds the particle, so the row difference aliases
+// modulo two rows (8h) and the pair's |dy| <= 3h < 4h picks the one value, exactly.
+//
+// The modified linear Riemann solver's terms are written in the kernel gradient's
+// scalar factor: with grad W = t (dx, dy), e = -(dx, dy) / r and v_ij.(dx, dy) = vx,
+// U_L - U_R = -vx / r, (v_s - v*).grad W = t (vx / 2 + (U* - U_bar) r) (and t vx / 2
+// near the surface, where v* = v_bar), (dx, dy).grad W / r^2 = t: the same terms as
+// the reference's vector form, fewer operations, FP32 rounding in another order.
+template <int SOLVER>
+__device__ __forceinline__ C32Terms c32_terms(const DevParams32& Q, const Buffers& B, const C32Self& S,
+                                              const C32Rec& N, int do_shift) {
+  C32Terms T{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const float4 r0o = N.r0, r1o = N.r1, r2o = N.r2;
+  const float2 pco = N.pc;
+  const int o = N.o, d = N.d;
+  // no pair between two non-fluid particles: such a pair's masses count as zero, so
+  // every term below vanishes (no branch)
+  const bool none = S.ts != 2 && r0o.w < 0.f;
+  const float dx = (S.offx - r2o.z) + (float)(1 - d) * Q.cell_d;
+  const float dyr = S.offy - r2o.w;
+  const float dy = fmaf(-Q.two_cd, __builtin_rintf(dyr * Q.inv_two_cd), dyr);
+  const float r2 = dx * dx + dy * dy;
+  const float inv_r = __builtin_amdgcn_rsqf(r2);   // r2 >= 1e-14: no denormal scaling
+  const float r = r2 * inv_r;
+  const float inv_r2 = inv_r * inv_r;
+  const float2 vs = S.vs, pcs = S.pcs;
+  const float2 vo = make_float2(r0o.x, r0o.y);
+  const float mo = none ? 0.f : fabsf(r0o.w);
+  const float m1 = none ? 0.f : r1o.w;
+  const float rhoo = Q.rho0 + r0o.z;
+  const float inv_rhoo = __builtin_amdgcn_rcpf(rhoo);
+  const float Vo = mo * inv_rhoo;
+  const float vijx = vs.x - vo.x, vijy = vs.y - vo.y;
+  const float vx = vijx * dx + vijy * dy;
+  const float Pbar = (pcs.x + pco.x) * 0.5f;
+  const float cbar = (pcs.y + pco.y) * 0.5f;
+  const float rbar = Q.rho0 + (S.drs + r0o.z) * 0.5f;
+  const float inv_rbar = __builtin_amdgcn_rcpf(rbar);    // v_rcp_f32, 1 ulp
+  const float rij = S.inv_rhos * inv_rhoo;
+  const KernelTerm32 K = kernel_term32(Q, dx, dy, r, inv_r);
+  // artificial viscosity of approaching pairs (vx < 0); phi = 0 otherwise, which
+  // adds zero: no branch
+  const float phi = Q.h * (vx < 0.0f ? vx : 0.0f) * __builtin_amdgcn_rcpf(r2 + Q.fai_h2);
+  const float Pi = (-(Q.alpha * cbar * phi) + Q.beta * (phi * phi)) * inv_rbar;
+  const float dtmp = Q.delta_alpha_h * cbar * vx * inv_r2 * Q.rho0 * S.inv_rhos * Vo;
+  float tx, ty;
+  if (SOLVER == SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN) {
+    const float t = K.t;
+    const float dU = -vx * inv_r;
+    // near the surface v* = v_bar and P* = P_bar: U* - U_bar and the dissipation
+    // are zero there (selects, no branch)
+    const float dUs = S.nes ? 0.f : 0.5f * (pcs.x - pco.x) * inv_rbar * Q.inv_c0;   // U* - U_bar
+    float bR = 3.0f * (dU > 0.0f ? dU : 0.0f);
+    bR = bR < cbar ? bR : cbar;
+    bR = S.nes ? 0.f : bR;
+    const float Ps = Pbar + 0.5f * bR * rbar * dU;
+    const float ti = t * (0.5f * vx + dUs * r);
+    T.drho = 2.0f * S.rhos * Vo * ti;
+    const float a = t * (mo * (-2.0f * Ps * rij) - mo * Pi + dtmp);
+    tx = a * dx;
+    ty = a * dy;
+    if (Q.epsilon2 != 0.0f) {
+      const float f = Q.epsilon2 * mo * __builtin_amdgcn_rcpf(S.rhos + rhoo) * K.w;
+      T.agx = -f * vijx;
+      T.agy = -f * vijy;
+    }
+  } else {
+    const float ex = -(dx * inv_r), ey = -(dy * inv_r);
+    const float UL = vs.x * ex + vs.y * ey;
+    const float UR = vo.x * ex + vo.y * ey;
+    const float vbx = (vs.x + vo.x) * 0.5f, vby = (vs.y + vo.y) * 0.5f;
+    float vsx, vsy, Ps;
+    float2 vown = vs;
+    if (SOLVER == SPH2D_SOLVER_HLLC || SOLVER == SPH2D_SOLVER_MUSCL) {
+      const bool left = S.s < o;                          // pair_i = lower local index
+      const float epx = left ? ex : -ex, epy = left ? ey : -ey;
+      const float2 vi = left ? vs : vo, vj = left ? vo : vs;
+      const float2 pci = left ? pcs : pco, pcj = left ? pco : pcs;
+      if (SOLVER == SPH2D_SOLVER_HLLC) {
+        const float rhoi = left ? S.rhos : rhoo, rhoj = left ? rhoo : S.rhos;
+        hllc_star<float>(vi.x, vi.y, pci.x, rhoi, pci.y, vj.x, vj.y, pcj.x, rhoj, pcj.y, epx, epy, Q.x_over_t,
+                         vsx, vsy, Ps);
+      } else {
+        const float2* mgo = B.m_mg + 3 * o;
+        const float2 gpo = mgo[0], guo = mgo[1], gvo = mgo[2];
+        const float2 gpi = left ? S.gps : gpo, gpj = left ? gpo : S.gps;
+        const float2 gui = left ? S.gus : guo, guj = left ? guo : S.gus;
+        const float2 gvi = left ? S.gvs : gvo, gvj = left ? gvo : S.gvs;
+        const float xrx = left ? -dx : dx, xry = left ? -dy : dy;
+        float li, lj;
+        muscl_limiter<float>(0.5f * (gpi.x * xrx + gpi.y * xry), -(0.5f * (gpj.x * xrx + gpj.y * xry)),
+                             pcj.x - pci.x, li, lj);
+        const float PL = pci.x + li, PR = pcj.x - lj;
+        muscl_limiter<float>(0.5f * (gui.x * xrx + gui.y * xry), -(0.5f * (guj.x * xrx + guj.y * xry)),
+                             vj.x - vi.x, li, lj);
+        const float Iu = vi.x + li, Ju = vj.x - lj;
+        muscl_limiter<float>(0.5f * (gvi.x * xrx + gvi.y * xry), -(0.5f * (gvj.x * xrx + gvj.y * xry)),
+                             vj.y - vi.y, li, lj);
+        const float Iv = vi.y + li, Jv = vj.y - lj;
+        const float ULm = Iu * epx + Iv * epy;
+        const float URm = Ju * epx + Jv * epy;
+        const float dUs = 0.5f * (PL - PR) * inv_rbar * Q.inv_c0;   // U* - U_bar
+        vsx = dUs * epx + vbx;
+        vsy = dUs * epy + vby;
+        const float dUp = ULm - URm;
+        float bR = 3.0f * (dUp > 0.0f ? dUp : 0.0f);
+        bR = bR < cbar ? bR : cbar;
+        Ps = Pbar + 0.5f * bR * rbar * dUp;
+        vown = left ? make_float2(Iu, Iv) : make_float2(Ju, Jv);
+      }
+    } else {                                           // SPH2D_SOLVER_LINEAR_RIEMANN
+      const float dU = UL - UR;
+      const float zs = S.rhos * pcs.y, zo = rhoo * pco.y;
+      const float inv_z = __builtin_amdgcn_rcpf(zs + zo);
+      const float Us = (zs * UR + zo * UL + (pcs.x - pco.x)) * inv_z;
+      vsx = Us * ex;
+      vsy = Us * ey;
+      Ps = (zs * pco.x + zo * pcs.x + zs * zo * dU) * inv_z;
+    }
+    if (S.nes) { vsx = vbx; vsy = vby; Ps = Pbar; }
+    const float ti = (vown.x - vsx) * K.dwx + (vown.y - vsy) * K.dwy;
+    T.drho = 2.0f * S.rhos * Vo * ti;
+    const float fp = mo * (-2.0f * Ps * rij);
+    tx = fp * K.dwx + mo * (-Pi * K.dwx) + dtmp * K.dwx;
+    ty = fp * K.dwy + mo * (-Pi * K.dwy) + dtmp * K.dwy;
+  }
+  const float4 taus = S.taus, tauo = r1o;
+  float tp = (taus.x + tauo.x) * K.dwx + (taus.y + tauo.y) * K.dwy;
+  tp = tp * rij;
+  tx += m1 * tp;
+  tp = tp + (taus.z + tauo.z) * K.dwy + (taus.y + tauo.y) * K.dwx;
+  tp = tp * rij;
+  ty += m1 * tp;
+  const float lf = Vo * Q.two_nu * K.t;                 // (dx, dy).grad W / r^2 = t
+  T.ax = tx + lf * vijx;
+  T.ay = ty + lf * vijy;
+  const float psi = (r0o.z - S.drs) + 0.5f * ((S.grs.x + r2o.x) * dx + (S.grs.y + r2o.y) * dy);
+  T.corr = -(Q.two_delta_h * cbar * psi * K.t * Vo);
+  if (do_shift) {
+    const float wr = K.w * Q.inv_wdx;
+    const float f = Q.shift_beta_h2 * (1.0f + Q.shift_R * (wr * wr * wr * wr)) * Vo;
+    T.shx = -f * K.dwx;
+    T.shy = -f * K.dwy;
+  }
+  return T;
 }
 
 // pass C (mixed): see k_eval_c in sph2d_eval64.hip for the reference lines.
@@ -271,38 +557,25 @@ __device__ __forceinline__ void k_eval_c32_body(const DevParams& P, const DevPar
   __syncthreads();
   const bool staged = st.use;                      // uniform over the block
   if (s >= n) return;
+  C32Self S;
   const float4 r0s = B.m_r0[s];
-  const float4 taus = B.m_r1[s];
   const float4 r2s = B.m_r2[s];
-  const float2 vs = make_float2(r0s.x, r0s.y);
-  const float2 rvs = make_float2(r0s.z, 0.f);
-  const float2 pcs = B.m_pc[s];                    // FP64 Tait law, rounded to FP32
-  const int ts = B.l_type[s];
-  const float2 grs = make_float2(r2s.x, r2s.y);
-  const int nes = B.near_[s];
-  const float rhos = Q.rho0 + rvs.x;
-  const float inv_rhos = 1.0f / rhos;
-  // Cell rows of the stencil columns: a neighbour's row (hence its cell corner
-  // relative to ours) follows from where its local index falls in its column.
-  const int cell_s = B.l_cell[s];
-  const int ny = P.chain_ny;
-  const int cxs = cell_s / ny, cys = cell_s - cxs * ny;
-  const int row0 = cys > 0 ? cys - 1 : 0;
-  const int row1 = cys + 1 < ny ? cys + 1 : ny - 1;
-  // first local index of rows row0+1 and row0+2 in stencil columns 0, 1, 2 (none: INT_MAX)
-  const bool c0ok = cxs >= 1, c2ok = cxs + 1 < P.chain_nx;
-  const bool r1ok = row0 + 1 <= row1, r2ok = row0 + 2 <= row1;
-  const int* const cs = B.cell_start;
-  const int rb10 = c0ok && r1ok ? cs[(cxs - 1) * ny + row0 + 1] : 0x7fffffff;
-  const int rb11 = r1ok ? cs[cxs * ny + row0 + 1] : 0x7fffffff;
-  const int rb12 = c2ok && r1ok ? cs[(cxs + 1) * ny + row0 + 1] : 0x7fffffff;
-  const int rb20 = c0ok && r2ok ? cs[(cxs - 1) * ny + row0 + 2] : 0x7fffffff;
-  const int rb21 = r2ok ? cs[cxs * ny + row0 + 2] : 0x7fffffff;
-  const int rb22 = c2ok && r2ok ? cs[(cxs + 1) * ny + row0 + 2] : 0x7fffffff;
+  S.s = s;
+  S.vs = make_float2(r0s.x, r0s.y);
+  S.drs = r0s.z;
+  S.taus = B.m_r1[s];
+  S.pcs = B.m_pc[s];                               // FP64 Tait law, rounded to FP32
+  S.ts = B.l_type[s];
+  S.grs = make_float2(r2s.x, r2s.y);
+  S.offx = r2s.z;
+  S.offy = r2s.w;
+  S.nes = B.near_[s];
+  S.rhos = Q.rho0 + S.drs;
+  S.inv_rhos = 1.0f / S.rhos;
   constexpr bool kMuscl = SOLVER == SPH2D_SOLVER_MUSCL;
-  float2 gps = make_float2(0.f, 0.f), gus = gps, gvs = gps;
-  if (kMuscl) { gps = B.m_mg[3 * s]; gus = B.m_mg[3 * s + 1]; gvs = B.m_mg[3 * s + 2]; }
-  KSum drho, corr, ax, ay, shx, shy, agx, agy;
+  S.gps = make_float2(0.f, 0.f); S.gus = S.gps; S.gvs = S.gps;
+  if (kMuscl) { S.gps = B.m_mg[3 * s]; S.gus = B.m_mg[3 * s + 1]; S.gvs = B.m_mg[3 * s + 2]; }
+  C32Sums A;
   // Buffer-resource views of the records for blocks without an LDS copy: buffer and
   // LDS loads are different instructions, never merged into one flat load of a
   // selected address (flat loads of LDS data are slower than ds_read)
@@ -310,7 +583,7 @@ __device__ __forceinline__ void k_eval_c32_body(const DevParams& P, const DevPar
   const __amdgpu_buffer_rsrc_t bpc = rec_rsrc(reinterpret_cast<const float4*>(B.m_pc));
   // The neighbours recorded by pass A (for_each_neighbour<kNbReplay>, written out:
   // a closure over this kernel's locals would be kept in scratch memory).
-  const CellRanges R = cell_ranges(P, B.cell_start, cell_s);
+  const CellRanges R = cell_ranges(P, B.cell_start, B.l_cell[s]);
   const int n0 = R.hi[0] - R.lo[0], n01 = n0 + R.hi[1] - R.lo[1], total = n01 + R.hi[2] - R.lo[2];
   const int l0 = R.lo[0], l1 = R.lo[1] - n0, l2 = R.lo[2] - n01;
   const int so0 = staged ? l0 - st.lo[0] + st.base[0] : 0;
@@ -318,214 +591,75 @@ __device__ __forceinline__ void k_eval_c32_body(const DevParams& P, const DevPar
   const int so2 = staged ? l2 - st.lo[2] + st.base[2] : 0;
   // pass A's record of the accepted neighbours (one byte per neighbour, in ascending
   // candidate order); a list longer than 256 candidates or more neighbours than the
-  // record holds are screened and tested again, 256 candidates at a time
+  // record holds are screened and tested again, 256 candidates at a time.
+  // The slots come one ahead of the pair they feed: each iteration takes the next
+  // slot and issues its record loads before the current pair's arithmetic, so the
+  // L2 and LDS latency of the loads overlaps the work of the pair before.
   const int cnt = B.nb_count[s];
   const bool listed = total <= 256 && !(cnt & kNbOverflow);
-  const uint32_t* const nbl = B.nb_list;
-  const int lst = B.kstride;
-  for (int c0 = 0; c0 < total; c0 += 256) {
-  uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-  if (!listed) {
-    const Mask256 mk = exact_chunk_mask(P, B.l_pos, B.l_pos32, s, B.l_pos[s], n0, n01, l0, l1, l2, total, c0);
-    m0 = mk.w0; m1 = mk.w1; m2 = mk.w2; m3 = mk.w3;
-  }
-  // The slots come one ahead of the pair they feed: each iteration takes the
-  // next slot and issues its record loads before the current pair's arithmetic,
-  // so the L2 and LDS latency of the loads overlaps the work of the pair before.
-  uint32_t word = 0, word_nx = 0;                  // the list word in use and the next one
-  int t = 0;
-  int kk_n = -1;
+  const bool xsph = SOLVER == SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && Q.epsilon2 != 0.0f;
+#define SPH2D_C32_FETCH(R, KK)                                                             \
+  do {                                                                                    \
+    const int kk_ = (KK);                                                                 \
+    const bool a_ = kk_ < n0, b_ = kk_ < n01;                                             \
+    R.d = a_ ? 0 : (b_ ? 1 : 2);                                                          \
+    R.o = kk_ + (a_ ? l0 : (b_ ? l1 : l2));                                               \
+    const int lin_ = kk_ + (a_ ? so0 : (b_ ? so1 : so2));                                 \
+    R.r1 = buf_f4(br1, R.o);                                                              \
+    R.pc = buf_f2(bpc, R.o);                                                              \
+    if (staged) { R.r0 = rec0[lin_]; R.r2 = rec2[lin_]; }                                 \
+    else { R.r0 = buf_f4(br0, R.o); R.r2 = buf_f4(br2, R.o); }                            \
+  } while (0)
   if (listed) {
-    if (cnt > 0) { word = nbl[s]; kk_n = (int)(word & 255u); }
+    // two record sets in turn (the loop advances two neighbours), so the loads in
+    // flight never need copying between registers
+    const uint32_t* const nbl = B.nb_list;
+    const int lst = B.kstride;
+    uint32_t word = 0, word_nx = 0;                // the list word in use and the next one
+    if (cnt > 0) word = nbl[s];
     if (cnt > 4) word_nx = nbl[lst + s];
-  } else if ((m0 | m1 | m2 | m3) != 0) {
-    const int w = m0 ? 0 : (m1 ? 1 : (m2 ? 2 : 3));
-    const uint64_t mm = m0 ? m0 : (m1 ? m1 : (m2 ? m2 : m3));
-    const int b = __builtin_ctzll(mm);
-    const uint64_t bit = 1ull << b;
-    if (w == 0) m0 ^= bit; else if (w == 1) m1 ^= bit; else if (w == 2) m2 ^= bit; else m3 ^= bit;
-    kk_n = c0 + w * 64 + b;
-  }
-  float4 r0n = make_float4(0.f, 0.f, 0.f, 0.f), r1n = r0n, r2n = r0n;
-  float2 pcn = make_float2(0.f, 0.f);
-  int dn = 0, on = 0;
-  if (kk_n >= 0) {
-    dn = kk_n < n0 ? 0 : (kk_n < n01 ? 1 : 2);
-    on = dn == 0 ? l0 + kk_n : (dn == 1 ? l1 + kk_n : l2 + kk_n);
-    const int lin = dn == 0 ? so0 + kk_n : (dn == 1 ? so1 + kk_n : so2 + kk_n);
-    r1n = buf_f4(br1, on);
-    pcn = buf_f2(bpc, on);
-    if (staged) { r0n = rec0[lin]; r2n = rec2[lin]; }
-    else { r0n = buf_f4(br0, on); r2n = buf_f4(br2, on); }
-  }
-  while (kk_n >= 0) {
-    const int d = dn, o = on;
-    const float4 r0o = r0n, r1o = r1n, r2o = r2n;
-    const float2 pco = pcn;
-    ++t;
-    kk_n = -1;
-    if (listed) {
-      if (t < cnt) {
-        if ((t & 3) == 0) {
+    C32Rec ra, rb;
+    if (cnt > 0) SPH2D_C32_FETCH(ra, (int)(word & 255u));
+    for (int t = 0; t < cnt; t += 2) {
+      if (t + 1 < cnt) SPH2D_C32_FETCH(rb, (int)((word >> (((t + 1) & 3) << 3)) & 255u));
+      const C32Terms ta = c32_terms<SOLVER>(Q, B, S, ra, do_shift);
+      if (t + 1 >= cnt) {
+        A.add(ta, xsph, do_shift);
+        break;
+      }
+      if (t + 2 < cnt) {
+        if (((t + 2) & 3) == 0) {
           word = word_nx;
-          if (t + 4 < cnt) word_nx = nbl[((t >> 2) + 1) * lst + s];
+          if (t + 6 < cnt) word_nx = nbl[(((t + 2) >> 2) + 1) * lst + s];
         }
-        kk_n = (int)((word >> ((t & 3) << 3)) & 255u);
+        SPH2D_C32_FETCH(ra, (int)((word >> (((t + 2) & 3) << 3)) & 255u));
       }
-    } else if ((m0 | m1 | m2 | m3) != 0) {
-      const int w = m0 ? 0 : (m1 ? 1 : (m2 ? 2 : 3));
-      const uint64_t mm = m0 ? m0 : (m1 ? m1 : (m2 ? m2 : m3));
-      const int b = __builtin_ctzll(mm);
-      const uint64_t bit = 1ull << b;
-      if (w == 0) m0 ^= bit; else if (w == 1) m1 ^= bit; else if (w == 2) m2 ^= bit; else m3 ^= bit;
-      kk_n = c0 + w * 64 + b;
+      A.add(ta + c32_terms<SOLVER>(Q, B, S, rb, do_shift), xsph, do_shift);
     }
-    if (kk_n >= 0) {
-      dn = kk_n < n0 ? 0 : (kk_n < n01 ? 1 : 2);
-      on = dn == 0 ? l0 + kk_n : (dn == 1 ? l1 + kk_n : l2 + kk_n);
-      const int lin = dn == 0 ? so0 + kk_n : (dn == 1 ? so1 + kk_n : so2 + kk_n);
-      r1n = buf_f4(br1, on);
-      pcn = buf_f2(bpc, on);
-      if (staged) { r0n = rec0[lin]; r2n = rec2[lin]; }
-      else { r0n = buf_f4(br0, on); r2n = buf_f4(br2, on); }
-    }
-    if (ts != 2 && r0o.w < 0.f) continue;        // no pair between two non-fluid particles
-    // bitwise blends, not selects: a select of captured values can become a
-    // select of their addresses, which keeps the closure in scratch memory
-    const int e0 = -(d == 0), e1 = -(d == 1), e2 = -(d == 2);
-    const int b1 = (rb10 & e0) | (rb11 & e1) | (rb12 & e2);
-    const int b2 = (rb20 & e0) | (rb21 & e1) | (rb22 & e2);
-    const int dcy = cys - (row0 + (o >= b1) + (o >= b2));   // cy_s - cy_o
-    const float dx = (r2s.z - r2o.z) + (float)(1 - d) * Q.cell_d;
-    const float dy = (r2s.w - r2o.w) + (float)dcy * Q.cell_d;
-    const float r2 = dx * dx + dy * dy;
-    const float inv_r = __builtin_amdgcn_rsqf(r2);   // r2 >= 1e-14: no denormal scaling
-    const float r = r2 * inv_r;
-    const float inv_r2 = inv_r * inv_r;
-    const KernelTerm32 K = kernel_term32(Q, dx, dy, r, inv_r);
-    const float2 vo = make_float2(r0o.x, r0o.y);
-    const float mo_x = fabsf(r0o.w);
-    const float rhoo = Q.rho0 + r0o.z;
-    const float2 rvo = make_float2(r0o.z, mo_x * __builtin_amdgcn_rcpf(rhoo));
-    const float Vo = rvo.y;
-    const float ex = -(dx * inv_r), ey = -(dy * inv_r);
-    const float UL = vs.x * ex + vs.y * ey;
-    const float UR = vo.x * ex + vo.y * ey;
-    const float dU = UL - UR;
-    const float Pbar = (pcs.x + pco.x) * 0.5f;
-    const float vbx = (vs.x + vo.x) * 0.5f, vby = (vs.y + vo.y) * 0.5f;
-    const float cbar = (pcs.y + pco.y) * 0.5f;
-    const float rbar = Q.rho0 + (rvs.x + rvo.x) * 0.5f;
-    const float inv_rbar = __builtin_amdgcn_rcpf(rbar);    // v_rcp_f32, 1 ulp
-    float vsx, vsy, Ps;
-    float2 vown = vs;
-    if (SOLVER == SPH2D_SOLVER_HLLC || SOLVER == SPH2D_SOLVER_MUSCL) {
-      const bool left = s < o;                          // pair_i = lower local index
-      const float epx = left ? ex : -ex, epy = left ? ey : -ey;
-      const float2 vi = left ? vs : vo, vj = left ? vo : vs;
-      const float2 pci = left ? pcs : pco, pcj = left ? pco : pcs;
-      if (SOLVER == SPH2D_SOLVER_HLLC) {
-        const float rhoi = left ? rhos : rhoo, rhoj = left ? rhoo : rhos;
-        hllc_star<float>(vi.x, vi.y, pci.x, rhoi, pci.y, vj.x, vj.y, pcj.x, rhoj, pcj.y, epx, epy, Q.x_over_t,
-                         vsx, vsy, Ps);
-      } else {
-        const float2* mgo = B.m_mg + 3 * o;
-        const float2 gpo = mgo[0], guo = mgo[1], gvo = mgo[2];
-        const float2 gpi = left ? gps : gpo, gpj = left ? gpo : gps;
-        const float2 gui = left ? gus : guo, guj = left ? guo : gus;
-        const float2 gvi = left ? gvs : gvo, gvj = left ? gvo : gvs;
-        const float xrx = left ? -dx : dx, xry = left ? -dy : dy;
-        float li, lj;
-        muscl_limiter<float>(0.5f * (gpi.x * xrx + gpi.y * xry), -(0.5f * (gpj.x * xrx + gpj.y * xry)),
-                             pcj.x - pci.x, li, lj);
-        const float PL = pci.x + li, PR = pcj.x - lj;
-        muscl_limiter<float>(0.5f * (gui.x * xrx + gui.y * xry), -(0.5f * (guj.x * xrx + guj.y * xry)),
-                             vj.x - vi.x, li, lj);
-        const float Iu = vi.x + li, Ju = vj.x - lj;
-        muscl_limiter<float>(0.5f * (gvi.x * xrx + gvi.y * xry), -(0.5f * (gvj.x * xrx + gvj.y * xry)),
-                             vj.y - vi.y, li, lj);
-        const float Iv = vi.y + li, Jv = vj.y - lj;
-        const float UL = Iu * epx + Iv * epy;
-        const float UR = Ju * epx + Jv * epy;
-        const float dUs = 0.5f * (PL - PR) * inv_rbar * Q.inv_c0;   // U* - U_bar
-        vsx = dUs * epx + vbx;
-        vsy = dUs * epy + vby;
-        const float dUp = UL - UR;
-        float bR = 3.0f * (dUp > 0.0f ? dUp : 0.0f);
-        bR = bR < cbar ? bR : cbar;
-        Ps = Pbar + 0.5f * bR * rbar * dUp;
-        vown = left ? make_float2(Iu, Iv) : make_float2(Ju, Jv);
+  } else {
+    C32Rec rn;
+    for (int c0 = 0; c0 < total; c0 += 256) {
+      const Mask256 mk = exact_chunk_mask(P, B.l_pos, B.l_pos32, s, B.l_pos[s], n0, n01, l0, l1, l2, total, c0);
+      uint64_t cur = mk.w0;
+      int wi = 0;
+      int kk_n = next_slot(cur, wi, mk.w1, mk.w2, mk.w3, c0);
+      if (kk_n >= 0) SPH2D_C32_FETCH(rn, kk_n);
+      while (kk_n >= 0) {
+        const C32Rec rc = rn;
+        kk_n = next_slot(cur, wi, mk.w1, mk.w2, mk.w3, c0);
+        if (kk_n >= 0) SPH2D_C32_FETCH(rn, kk_n);
+        A.add(c32_terms<SOLVER>(Q, B, S, rc, do_shift), xsph, do_shift);
       }
-    } else if (SOLVER == SPH2D_SOLVER_LINEAR_RIEMANN) {
-      const float zs = rhos * pcs.y, zo = rhoo * pco.y;
-      const float inv_z = __builtin_amdgcn_rcpf(zs + zo);
-      const float Us = (zs * UR + zo * UL + (pcs.x - pco.x)) * inv_z;
-      vsx = Us * ex;
-      vsy = Us * ey;
-      Ps = (zs * pco.x + zo * pcs.x + zs * zo * dU) * inv_z;
-    } else {
-      const float Ub = (UL + UR) * 0.5f;
-      const float dUs = 0.5f * (pcs.x - pco.x) * inv_rbar * Q.inv_c0;   // U* - U_bar
-      vsx = dUs * ex + vbx;
-      vsy = dUs * ey + vby;
-      float bR = 3.0f * (dU > 0.0f ? dU : 0.0f);
-      bR = bR < cbar ? bR : cbar;
-      Ps = Pbar + 0.5f * bR * rbar * dU;
-      (void)Ub;
-    }
-    if (nes) { vsx = vbx; vsy = vby; Ps = Pbar; }
-    const float ti = (vown.x - vsx) * K.dwx + (vown.y - vsy) * K.dwy;
-    drho.add(2.0f * rhos * Vo * ti);
-    const float rij = inv_rhos * __builtin_amdgcn_rcpf(rhoo);
-    const float2 mo = make_float2(mo_x, r1o.w);
-    const float fp = mo.x * (-2.0f * Ps * rij);
-    float tx = fp * K.dwx, ty = fp * K.dwy;
-    const float vijx = vs.x - vo.x, vijy = vs.y - vo.y;
-    const float vx = vijx * dx + vijy * dy;
-    if (vx < 0.0f) {
-      const float phi = Q.h * vx * __builtin_amdgcn_rcpf(r2 + Q.fai_h2);
-      const float Pi = (-(Q.alpha * cbar * phi) + Q.beta * (phi * phi)) * inv_rbar;
-      tx += mo.x * (-Pi * K.dwx);
-      ty += mo.x * (-Pi * K.dwy);
-    }
-    const float Rd = dx * K.dwx + dy * K.dwy;
-    const float tmp = Q.delta_alpha_h * cbar * vx * inv_r2 * Q.rho0 * inv_rhos * Vo;
-    tx += tmp * K.dwx;
-    ty += tmp * K.dwy;
-    const float4 tauo = r1o;
-    float tp = (taus.x + tauo.x) * K.dwx + (taus.y + tauo.y) * K.dwy;
-    tp = tp * rij;
-    tx += mo.y * tp;
-    tp = tp + (taus.z + tauo.z) * K.dwy + (taus.y + tauo.y) * K.dwx;
-    tp = tp * rij;
-    ty += mo.y * tp;
-    const float lf = Vo * Q.two_nu * Rd * inv_r2;
-    tx += lf * vijx;
-    ty += lf * vijy;
-    ax.add(tx);
-    ay.add(ty);
-    if (SOLVER == SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && Q.epsilon2 != 0.0f) {
-      const float f = Q.epsilon2 * mo.x / (rhos + rhoo) * K.w;
-      agx.add(-f * vijx);
-      agy.add(-f * vijy);
-    }
-    const float2 gro = make_float2(r2o.x, r2o.y);
-    const float psi = (rvo.x - rvs.x) + 0.5f * ((grs.x + gro.x) * dx + (grs.y + gro.y) * dy);
-    corr.add(-(Q.two_delta_h * cbar * psi * Rd * inv_r2 * Vo));
-    if (do_shift) {
-      const float wr = K.w * Q.inv_wdx;
-      const float f = Q.shift_beta_h2 * (1.0f + Q.shift_R * (wr * wr * wr * wr)) * Vo;
-      shx.add(-f * K.dwx);
-      shy.add(-f * K.dwy);
     }
   }
-  }
-  double dr = (double)drho.s;
-  if (ts == 2) dr = dr + (double)corr.s;
+#undef SPH2D_C32_FETCH
+  const int ts = S.ts;
+  double dr = (double)A.drho.s;
+  if (ts == 2) dr = dr + (double)A.corr.s;
   B.drhodt[s] = dr;
-  B.acc[s] = ts == 2 ? make_double2((double)ax.s, (double)ay.s - P.g) : make_double2(0.0, 0.0);
-  if (P.epsilon_xsph != 0.0) B.avg[s] = make_double2((double)agx.s, (double)agy.s);
-  if (do_shift) B.shift[s] = make_double2((double)shx.s, (double)shy.s);
+  B.acc[s] = ts == 2 ? make_double2((double)A.ax.s, (double)A.ay.s - P.g) : make_double2(0.0, 0.0);
+  if (P.epsilon_xsph != 0.0) B.avg[s] = make_double2((double)A.agx.s, (double)A.agy.s);
+  if (do_shift) B.shift[s] = make_double2((double)A.shx.s, (double)A.shy.s);
 }
 
 template <int SOLVER>
@@ -536,9 +670,9 @@ __global__ void __launch_bounds__(kBlock) k_eval_c32(DevParams P, DevParams32 Q,
 void launch_eval32_a(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
   const int g = grid_for(cap_local);
   if (P.solver == SPH2D_SOLVER_MUSCL)
-    hipLaunchKernelGGL(k_eval_a32<true>, dim3(g), dim3(kBlock), 0, s, P, make32(P), B);
+    hipLaunchKernelGGL(k_eval_a32_muscl, dim3(g), dim3(kBlock), 0, s, P, make32(P), B);
   else
-    hipLaunchKernelGGL(k_eval_a32<false>, dim3(g), dim3(kBlock), 0, s, P, make32(P), B);
+    hipLaunchKernelGGL(k_eval_a32, dim3(g), dim3(kBlock), 0, s, P, make32(P), B);
 }
 
 void launch_eval32_c(hipStream_t s, const DevParams& P, Buffers& B, int cap_local) {
@@ -567,13 +701,23 @@ void q32_fill(const DevParams& P, void* out) {
 
 // Batched mixed passes A and C (see launch_batch_step), in the tile shape SHAPE of
 // kNbShape: BLK consecutive particles per block and the LDS stage capacities.
-template <bool MUSCL, int SHAPE>
-__global__ void __launch_bounds__(kNbShape[SHAPE].block) kb_eval_a32(BatchArgs A, const DevParams32* Qs) {
+template <int SHAPE>
+__global__ void __launch_bounds__(kNbShape[SHAPE].block) __attribute__((amdgpu_waves_per_eu(5)))
+kb_eval_a32(BatchArgs A, const DevParams32* Qs) {
   const BatchWhere w = batch_where(A, A.xcd ? xcd_block() : (int)blockIdx.x);
   const DevParams P = A.inst[w.i].P;
   const DevParams32 Q = Qs[w.i];
   Buffers B = A.inst[w.i].B;
-  k_eval_a32_body<MUSCL, kNbShape[SHAPE].cap_a>(P, Q, B, w.blk, w.nblk);
+  k_eval_a32_body<false, kNbShape[SHAPE].cap_a>(P, Q, B, w.blk, w.nblk);
+}
+
+template <int SHAPE>
+__global__ void __launch_bounds__(kNbShape[SHAPE].block) kb_eval_a32_muscl(BatchArgs A, const DevParams32* Qs) {
+  const BatchWhere w = batch_where(A, A.xcd ? xcd_block() : (int)blockIdx.x);
+  const DevParams P = A.inst[w.i].P;
+  const DevParams32 Q = Qs[w.i];
+  Buffers B = A.inst[w.i].B;
+  k_eval_a32_body<true, kNbShape[SHAPE].cap_a>(P, Q, B, w.blk, w.nblk);
 }
 
 template <int SOLVER, int SHAPE>
@@ -592,8 +736,8 @@ static void launch_batch_eval32_shape(hipStream_t s, const BatchDev& D, int pass
   const BatchArgs A{D.inst, D.start[BC_NBR], D.n, D.xcd};
   const auto* Qs = static_cast<const DevParams32*>(D.q32);
   if (pass == 0) {
-    if (solver == SPH2D_SOLVER_MUSCL) hipLaunchKernelGGL((kb_eval_a32<true, SHAPE>), dim3(g), t, 0, s, A, Qs);
-    else hipLaunchKernelGGL((kb_eval_a32<false, SHAPE>), dim3(g), t, 0, s, A, Qs);
+    if (solver == SPH2D_SOLVER_MUSCL) hipLaunchKernelGGL((kb_eval_a32_muscl<SHAPE>), dim3(g), t, 0, s, A, Qs);
+    else hipLaunchKernelGGL((kb_eval_a32<SHAPE>), dim3(g), t, 0, s, A, Qs);
     return;
   }
   switch (solver) {

@@ -757,8 +757,13 @@ __global__ void __launch_bounds__(kBlock) k_gather_predict(DevParams P, Buffers 
 // neighbours fall back to the Shepard average, as the reference does (:284-295).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void k_mls_body(const DevParams& P, Buffers& B, int blk_, int nblk_) {
+  // the block's candidate positions in LDS (as the evaluation's pass A stages them)
+  __shared__ NbStage st;
+  __shared__ float2 lds32[kStageCap];
+  __shared__ double2 lds64[kStageCap];
   const int s = blk_ * blockDim.x + threadIdx.x;
   const int n = B.counters[C_NLOCAL];
+  stage_candidates(P, B.cell_start, B.l_cell, B.l_pos32, B.l_pos, n, st, lds32, lds64, blk_);
   if (s >= n) return;
   const int ts = B.l_type[s];
   if (ts != 2) { B.mls_rho[s] = 0.0; return; }     // :69 rho=0 for every non-fluid particle
@@ -771,7 +776,7 @@ __device__ __forceinline__ void k_mls_body(const DevParams& P, Buffers& B, int b
   double sw = 0.0, srw = 0.0;
   int ne = 0;
   for_each_neighbour(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) __attribute__((always_inline)) {
-    const double2 po = B.l_pos[o];
+    const double2 po = li >= 0 ? lds64[li] : B.l_pos[o];
     const double pdx = ps.x - po.x, pdy = ps.y - po.y;
     const double r = sqrt(pdx * pdx + pdy * pdy);
     double w, dwx, dwy;
@@ -790,7 +795,7 @@ __device__ __forceinline__ void k_mls_body(const DevParams& P, Buffers& B, int b
     sw = sw + w;
     srw = srw + rho_o * w;
     ++ne;
-  });
+  }, nullptr, nullptr, 0, &st, lds32, lds64);
   const double al3 = (P.mls_alpha * P.mls_alpha) * P.mls_alpha;
   double rho = 0.0;
   bool ok = ne > 8;

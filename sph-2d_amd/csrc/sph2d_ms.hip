@@ -535,17 +535,21 @@ __global__ void __launch_bounds__(kBlock) k_ms_body_part(DevParams P, Buffers B)
     for (int k = 0; k < 6; ++k) B.bpart[6 * blockIdx.x + k] = lds[k][0];
 }
 
-__global__ void k_ms_body_final(Buffers B, int nb, double* out) {
-  if (threadIdx.x >= 6) return;
+// One wavefront per component: lane l sums the partials l, l + 64, ... in order,
+// then the 64 lane sums fold pairwise (xor butterfly): a fixed order, so the
+// result does not depend on the launch.
+__global__ void __launch_bounds__(384) k_ms_body_final(Buffers B, int nb, double* out) {
+  const int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double a = 0.0;
-  for (int b = 0; b < nb; ++b) a = a + B.bpart[6 * b + threadIdx.x];
-  out[threadIdx.x] = a;
+  for (int b = lane; b < nb; b += 64) a = a + B.bpart[6 * b + k];
+  for (int off = 32; off > 0; off >>= 1) a = a + __shfl_xor(a, off);
+  if (lane == 0) out[k] = a;
 }
 
 void launch_ms_body_sum(hipStream_t s, const DevParams& P, Buffers& B, int cap_local, double* out6) {
   const int g = ms_grid(cap_local);
   hipLaunchKernelGGL(k_ms_body_part, dim3(g), dim3(kBlock), 0, s, P, B);
-  hipLaunchKernelGGL(k_ms_body_final, dim3(1), dim3(64), 0, s, B, g, out6);
+  hipLaunchKernelGGL(k_ms_body_final, dim3(1), dim3(384), 0, s, B, g, out6);
 }
 
 // ---------------------------------------------------------------------------
