@@ -447,9 +447,9 @@ def main() -> int:
     ap.add_argument("--ke-gate-side", type=int, default=1024,
                     help="KE error of the mixed form vs FP64 at t = 0.5 on this lattice (0 skips; 2048 is the "
                          "4.2M-particle gate, 102,400 steps per form)")
-    ap.add_argument("--forms", choices=("both", "fp64", "mixed", "batched", "ms"), default="both",
+    ap.add_argument("--forms", choices=("both", "fp64", "mixed", "batched", "ms", "adjoint"), default="both",
                     help="both: the full line; fp64/mixed: that form's timing only; batched: the batched "
-                         "leg only; ms: the moving-square leg only")
+                         "leg only; ms: the moving-square leg only; adjoint: the wave-tank adjoint leg only")
     ap.add_argument("--timing-steps", type=int, default=20,
                     help="steps after the timed run that record the per-pass HIP events")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -533,6 +533,9 @@ def main() -> int:
         hip.hipFree(b)
         print(f"# copy calibration: {nbytes} bytes device to device", file=sys.stderr, flush=True)
 
+    if args.forms == "adjoint":
+        print(json.dumps({"partial_run": "adjoint", "adjoint": adjoint_leg(sph2d, args.adjoint_steps)}), flush=True)
+        return 0
     if args.forms == "ms":
         print(json.dumps({"partial_run": "ms",
                           "moving_square": moving_square_leg(sph2d, args.ms_steps,

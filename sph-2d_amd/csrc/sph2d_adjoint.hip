@@ -325,6 +325,316 @@ __device__ __forceinline__ Side<Dual<N>> plain_side(const Side<double>& a) {
   return r;
 }
 
+// Reverse mode of force_pair<double, SOLVER> (the weighted force-pass pair terms of
+// A summing over neighbour B): the same forward formulas, then their adjoints in
+// reverse order. bA, bB receive d(J)/d(x, y, u, v, rho, p, c, m) of either side
+// (what a side's primary state q moves is chained by side_q_bar), gaux the closed
+// form of force_pair. The branches of the forward (the limiter's cases, vx < 0, the
+// near-surface flag) are taken as the forward takes them. One primal sweep and one
+// adjoint sweep in FP64 registers: a fraction of the work and registers of 5-direction
+// duals over the same formulas.
+enum { RB_X = 0, RB_Y, RB_U, RB_V, RB_RHO, RB_P, RB_C, RB_M, RB_N };
+
+template <int SOLVER>
+__device__ __forceinline__ void force_pair_rev(const DevParams& P, const Side<double>& A, const Side<double>& B,
+                                               double gmo, int nes, const PairLam& L, int do_shift, double* bA,
+                                               double* bB, double* gaux) {
+#pragma unroll
+  for (int k = 0; k < RB_N; ++k) { bA[k] = 0.0; bB[k] = 0.0; }
+  // forward
+  const double dx = A.x - B.x, dy = A.y - B.y;
+  const double r2 = dx * dx + dy * dy;
+  const double r = sqrt(r2);
+  const double inv_r = 1.0 / r;
+  const double q = r * P.inv_h;
+  const double eq = exp(-(q * q));
+  const double w = (eq - P.e9) * P.kfactor;
+  const double kq = (inv_r * P.inv_h) * P.kfactor;          // kt = eq (-2 q) kq
+  const double kt = eq * (-2.0 * q) * kq;
+  const double dwx = kt * dx, dwy = kt * dy;
+  const double mo = B.m;
+  const double Vo = mo / B.rho;
+  const double ex = -(dx * inv_r), ey = -(dy * inv_r);
+  const double UL = A.u * ex + A.v * ey;
+  const double UR = B.u * ex + B.v * ey;
+  const double dU = UL - UR;
+  const double Pbar = (A.p + B.p) * 0.5;
+  const double vbx = (A.u + B.u) * 0.5, vby = (A.v + B.v) * 0.5;
+  const double cbar = (A.c + B.c) * 0.5;
+  const double rbar = (A.rho + B.rho) * 0.5;
+  double vsx, vsy, Ps;
+  // modified linear: D = U* - U_bar; linear: zs, zo, 1 / (zs + zo), U*
+  double D = 0.0, bR = 0.0, zs = 0.0, zo = 0.0, iz = 0.0, Us = 0.0, NP = 0.0, NU = 0.0;
+  int bcase = 0;                                  // 0: bR = 0, 1: bR = 3 dU, 2: bR = c_bar
+  if (SOLVER == SPH2D_SOLVER_LINEAR_RIEMANN) {
+    zs = A.rho * A.c;
+    zo = B.rho * B.c;
+    iz = 1.0 / (zs + zo);
+    NU = zs * UR + zo * UL + A.p - B.p;
+    NP = zs * B.p + zo * A.p + zs * zo * dU;
+    Us = NU * iz;
+    vsx = Us * ex;
+    vsy = Us * ey;
+    Ps = NP * iz;
+  } else {
+    D = 0.5 * (A.p - B.p) / rbar * P.inv_c0;
+    vsx = D * ex + vbx;
+    vsy = D * ey + vby;
+    if (dU > 0.0) { bR = 3.0 * dU; bcase = 1; }
+    if (bR >= cbar) { bR = cbar; bcase = 2; }
+    Ps = Pbar + 0.5 * bR * rbar * dU;
+  }
+  if (nes) { vsx = vbx; vsy = vby; Ps = Pbar; }
+  const double ti = (A.u - vsx) * dwx + (A.v - vsy) * dwy;
+  const double rij = 1.0 / (A.rho * B.rho);
+  const double fp = mo * (-2.0 * Ps * rij);
+  const double vijx = A.u - B.u, vijy = A.v - B.v;
+  const double vx = vijx * dx + vijy * dy;
+  const bool visc = vx < 0.0;
+  const double fh2 = (P.fai * P.h) * (P.fai * P.h);
+  double phi = 0.0, Pi = 0.0;
+  if (visc) {
+    phi = P.h * vx / (r2 + fh2);
+    Pi = (-(P.alpha * cbar * phi) + P.beta * (phi * phi)) / rbar;
+  }
+  const double Rd = dx * dwx + dy * dwy;
+  const double lf = Vo * 2.0 * P.nu * Rd / r2;
+  const double k3 = P.delta_alpha * P.h * P.rho0;
+  const double tmp = k3 * cbar * vx * Vo / (r2 * A.rho);
+  const double sxx = A.txx + B.txx, sxy = A.txy + B.txy, syy = A.tyy + B.tyy;
+  const double tp1 = sxx * dwx + sxy * dwy;
+  const double tp1r = tp1 * rij;
+  const double tp2p = tp1r + syy * dwy + sxy * dwx;
+  const double xsphf = (SOLVER == SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && P.epsilon_xsph != 0.0)
+                           ? P.epsilon_xsph * P.xsph_f : 0.0;
+  const double psi = (B.rho - A.rho) + 0.5 * ((A.gx + B.gx) * dx + (A.gy + B.gy) * dy);
+  const double k2 = 2.0 * P.delta * P.h;
+  const double cf = k2 * cbar * Rd / r2 * Vo;
+  // d/d(grad rho x, y, tau xx, xy, yy): linear, through the sums A + B (as force_pair)
+  gaux[0] = -L.c * cf * 0.5 * dx;
+  gaux[1] = -L.c * cf * 0.5 * dy;
+  gaux[2] = gmo * rij * (L.ax * dwx + L.ay * (dwx * rij));
+  gaux[3] = gmo * rij * (L.ax * dwy + L.ay * (dwy * rij + dwx));
+  gaux[4] = gmo * rij * (L.ay * dwy);
+  // reverse: adjoints of the intermediates
+  double b_dx = 0.0, b_dy = 0.0, b_r2 = 0.0, b_w = 0.0, b_dwx = 0.0, b_dwy = 0.0, b_Vo = 0.0, b_cbar = 0.0;
+  double b_rbar = 0.0, b_Rd = 0.0, b_vx = 0.0, b_vijx = 0.0, b_vijy = 0.0, b_rij = 0.0;
+  if (do_shift && (L.shx != 0.0 || L.shy != 0.0)) {
+    const double wr = w * P.inv_wdx;
+    const double wr2 = wr * wr;
+    const double sb = P.shift_beta * P.h2;
+    const double f = sb * (1.0 + P.shift_R * (wr2 * wr2)) * Vo;
+    const double b_f = -(L.shx * dwx + L.shy * dwy);
+    b_dwx += -L.shx * f;
+    b_dwy += -L.shy * f;
+    b_Vo += b_f * sb * (1.0 + P.shift_R * (wr2 * wr2));
+    const double b_wr2 = b_f * sb * Vo * P.shift_R * 2.0 * wr2;
+    b_w += b_wr2 * 2.0 * wr * P.inv_wdx;
+  }
+  {                                               // delta-SPH: J -= L.c cf psi
+    const double b_cf = -L.c * psi, b_psi = -L.c * cf;
+    bB[RB_RHO] += b_psi;
+    bA[RB_RHO] -= b_psi;
+    b_dx += b_psi * 0.5 * (A.gx + B.gx);
+    b_dy += b_psi * 0.5 * (A.gy + B.gy);
+    b_cbar += b_cf * k2 * Rd / r2 * Vo;
+    b_Rd += b_cf * k2 * cbar / r2 * Vo;
+    b_r2 -= b_cf * cf / r2;
+    b_Vo += b_cf * k2 * cbar * Rd / r2;
+  }
+  if (xsphf != 0.0) {                             // XSPH: J -= (L.gx vijx + L.gy vijy) f
+    const double srho = A.rho + B.rho;
+    const double f = xsphf * mo / srho * w;
+    const double b_f = -(L.gx * vijx + L.gy * vijy);
+    b_vijx += -L.gx * f;
+    b_vijy += -L.gy * f;
+    bB[RB_M] += b_f * xsphf / srho * w;
+    const double b_s = -b_f * f / srho;
+    bA[RB_RHO] += b_s;
+    bB[RB_RHO] += b_s;
+    b_w += b_f * xsphf * mo / srho;
+  }
+  const double b_drho = L.d, b_ax = L.ax, b_ay = L.ay;
+  {                                               // SPS stress
+    const double b_tp2 = b_ay * gmo;
+    b_rij += b_tp2 * tp2p;
+    const double b_tp2p = b_tp2 * rij;
+    const double b_tp1r = b_tp2p + b_ax * gmo;
+    b_dwy += b_tp2p * syy;
+    b_dwx += b_tp2p * sxy;
+    const double b_tp1 = b_tp1r * rij;
+    b_rij += b_tp1r * tp1;
+    b_dwx += b_tp1 * sxx;
+    b_dwy += b_tp1 * sxy;
+  }
+  {                                               // delta-alpha term
+    const double b_tmp = b_ax * dwx + b_ay * dwy;
+    b_dwx += b_ax * tmp;
+    b_dwy += b_ay * tmp;
+    const double den = r2 * A.rho;
+    b_cbar += b_tmp * k3 * vx * Vo / den;
+    b_vx += b_tmp * k3 * cbar * Vo / den;
+    b_Vo += b_tmp * k3 * cbar * vx / den;
+    b_r2 -= b_tmp * tmp / r2;
+    bA[RB_RHO] -= b_tmp * tmp / A.rho;
+  }
+  {                                               // laminar viscosity
+    const double b_lf = b_ax * vijx + b_ay * vijy;
+    b_vijx += b_ax * lf;
+    b_vijy += b_ay * lf;
+    b_Vo += b_lf * 2.0 * P.nu * Rd / r2;
+    b_Rd += b_lf * Vo * 2.0 * P.nu / r2;
+    b_r2 -= b_lf * lf / r2;
+  }
+  b_dx += b_Rd * dwx;
+  b_dwx += b_Rd * dx;
+  b_dy += b_Rd * dwy;
+  b_dwy += b_Rd * dy;
+  if (visc) {                                     // artificial viscosity
+    const double g = b_ax * dwx + b_ay * dwy;
+    bB[RB_M] += -Pi * g;
+    const double b_Pi = -mo * g;
+    b_dwx += -mo * Pi * b_ax;
+    b_dwy += -mo * Pi * b_ay;
+    b_cbar += b_Pi * (-P.alpha * phi) / rbar;
+    const double b_phi = b_Pi * (-P.alpha * cbar + 2.0 * P.beta * phi) / rbar;
+    b_rbar -= b_Pi * Pi / rbar;
+    b_vx += b_phi * P.h / (r2 + fh2);
+    b_r2 -= b_phi * phi / (r2 + fh2);
+  }
+  b_vijx += b_vx * dx;
+  b_vijy += b_vx * dy;
+  b_dx += b_vx * vijx;
+  b_dy += b_vx * vijy;
+  bA[RB_U] += b_vijx;
+  bB[RB_U] -= b_vijx;
+  bA[RB_V] += b_vijy;
+  bB[RB_V] -= b_vijy;
+  double b_Ps;
+  {                                               // pressure term
+    const double b_fp = b_ax * dwx + b_ay * dwy;
+    b_dwx += b_ax * fp;
+    b_dwy += b_ay * fp;
+    bB[RB_M] += b_fp * (-2.0 * Ps * rij);
+    b_Ps = b_fp * mo * (-2.0 * rij);
+    b_rij += b_fp * mo * (-2.0 * Ps);
+  }
+  bA[RB_RHO] -= b_rij * rij / A.rho;
+  bB[RB_RHO] -= b_rij * rij / B.rho;
+  double b_vsx, b_vsy;
+  {                                               // continuity
+    bA[RB_RHO] += b_drho * 2.0 * Vo * ti;
+    b_Vo += b_drho * 2.0 * A.rho * ti;
+    const double b_ti = b_drho * 2.0 * A.rho * Vo;
+    bA[RB_U] += b_ti * dwx;
+    bA[RB_V] += b_ti * dwy;
+    b_vsx = -b_ti * dwx;
+    b_vsy = -b_ti * dwy;
+    b_dwx += b_ti * (A.u - vsx);
+    b_dwy += b_ti * (A.v - vsy);
+  }
+  double b_Pbar = 0.0, b_vbx = 0.0, b_vby = 0.0, b_dU = 0.0, b_ex = 0.0, b_ey = 0.0, b_UL = 0.0, b_UR = 0.0;
+  if (nes) {
+    b_vbx += b_vsx;
+    b_vby += b_vsy;
+    b_Pbar += b_Ps;
+  } else if (SOLVER == SPH2D_SOLVER_LINEAR_RIEMANN) {
+    const double b_Us = b_vsx * ex + b_vsy * ey;
+    b_ex += b_vsx * Us;
+    b_ey += b_vsy * Us;
+    const double b_NP = b_Ps * iz, b_NU = b_Us * iz;
+    const double b_iz = b_Ps * NP + b_Us * NU;
+    double b_zs = -b_iz * iz * iz, b_zo = -b_iz * iz * iz;
+    b_zs += b_NP * (B.p + zo * dU);
+    bB[RB_P] += b_NP * zs;
+    b_zo += b_NP * (A.p + zs * dU);
+    bA[RB_P] += b_NP * zo;
+    b_dU += b_NP * zs * zo;
+    b_zs += b_NU * UR;
+    b_UR += b_NU * zs;
+    b_zo += b_NU * UL;
+    b_UL += b_NU * zo;
+    bA[RB_P] += b_NU;
+    bB[RB_P] -= b_NU;
+    bA[RB_RHO] += b_zs * A.c;
+    bA[RB_C] += b_zs * A.rho;
+    bB[RB_RHO] += b_zo * B.c;
+    bB[RB_C] += b_zo * B.rho;
+  } else {
+    b_Pbar += b_Ps;
+    const double b_bR = b_Ps * 0.5 * rbar * dU;
+    b_rbar += b_Ps * 0.5 * bR * dU;
+    b_dU += b_Ps * 0.5 * bR * rbar;
+    if (bcase == 1) b_dU += 3.0 * b_bR;
+    else if (bcase == 2) b_cbar += b_bR;
+    const double b_D = b_vsx * ex + b_vsy * ey;
+    b_ex += b_vsx * D;
+    b_ey += b_vsy * D;
+    b_vbx += b_vsx;
+    b_vby += b_vsy;
+    const double k = 0.5 / rbar * P.inv_c0;
+    bA[RB_P] += b_D * k;
+    bB[RB_P] -= b_D * k;
+    b_rbar -= b_D * D / rbar;
+  }
+  bA[RB_P] += 0.5 * b_Pbar;
+  bB[RB_P] += 0.5 * b_Pbar;
+  bA[RB_U] += 0.5 * b_vbx;
+  bB[RB_U] += 0.5 * b_vbx;
+  bA[RB_V] += 0.5 * b_vby;
+  bB[RB_V] += 0.5 * b_vby;
+  bA[RB_C] += 0.5 * b_cbar;
+  bB[RB_C] += 0.5 * b_cbar;
+  bA[RB_RHO] += 0.5 * b_rbar;
+  bB[RB_RHO] += 0.5 * b_rbar;
+  b_UL += b_dU;
+  b_UR -= b_dU;
+  bA[RB_U] += b_UL * ex;
+  bA[RB_V] += b_UL * ey;
+  b_ex += b_UL * A.u + b_UR * B.u;
+  b_ey += b_UL * A.v + b_UR * B.v;
+  bB[RB_U] += b_UR * ex;
+  bB[RB_V] += b_UR * ey;
+  double b_inv_r = -(b_ex * dx + b_ey * dy);
+  b_dx += -b_ex * inv_r;
+  b_dy += -b_ey * inv_r;
+  bB[RB_M] += b_Vo / B.rho;
+  bB[RB_RHO] -= b_Vo * Vo / B.rho;
+  // the kernel: dwx = kt dx, dwy = kt dy, kt = eq (-2 q) kq, kq = inv_r inv_h kfactor
+  const double b_kt = b_dwx * dx + b_dwy * dy;
+  b_dx += b_dwx * kt;
+  b_dy += b_dwy * kt;
+  double b_eq = b_kt * (-2.0 * q) * kq + b_w * P.kfactor;
+  double b_q = b_kt * eq * (-2.0) * kq;
+  b_inv_r += b_kt * eq * (-2.0 * q) * P.inv_h * P.kfactor;
+  b_q += b_eq * eq * (-2.0 * q);
+  double b_r = b_q * P.inv_h - b_inv_r * inv_r * inv_r;
+  b_r2 += b_r * 0.5 / r;
+  b_dx += b_r2 * 2.0 * dx;
+  b_dy += b_r2 * 2.0 * dy;
+  bA[RB_X] += b_dx;
+  bB[RB_X] -= b_dx;
+  bA[RB_Y] += b_dy;
+  bB[RB_Y] -= b_dy;
+}
+
+// d(J)/dq of a side from d(J)/d(rho, p, c, m), through derive_side.
+__device__ __forceinline__ double side_q_bar(const DevParams& P, const Side<double>& a, int kind, const double* b) {
+  if (kind == KIND_WALL) {
+    const double base = 1.0 + 7.0 * a.p / (P.c0sq * P.rho0);
+    const double e = (double)0.142857142f;
+    const double drho_dp = P.rho0 * (base != 0.0 ? e * pow(base, e) / base : 0.0) * (7.0 / (P.c0sq * P.rho0));
+    const double x = a.rho * (1.0 / P.rho0);
+    const double dc_drho = P.sqrt_c0sq * 3.0 * x * x * (1.0 / P.rho0);
+    return b[RB_P] + (b[RB_RHO] + b[RB_C] * dc_drho + b[RB_M] * P.vol0) * drho_dp;
+  }
+  const double x = a.rho * (1.0 / P.rho0);
+  const double x2 = x * x, x3 = x2 * x;
+  const double dp = P.eos_b * 7.0 * x3 * x3 * (1.0 / P.rho0);
+  const double dc = P.sqrt_c0sq * 3.0 * x2 * (1.0 / P.rho0);
+  return b[RB_RHO] + b[RB_P] * dp + b[RB_C] * dc + (kind == KIND_TIED ? b[RB_M] * P.vol0 : 0.0);
+}
+
 struct AdjBufs {
   const double* lam;   // gid mode: 5 per fluid gid (drho, ax, ay, avg_x, avg_y); slot mode: unused
   double* w;           // kLam per local particle (PairLam, then grad rho x, y)
@@ -367,27 +677,31 @@ __global__ void __launch_bounds__(kBlock) k_adj_force(DevParams P, Buffers B, Ad
   const double2 ps = B.l_pos[s];
   int ks;
   const Side<double> vs = load_side(P, B, s, ks);
-  const Side<Dual<5>> S = seeded_side<5>(P, vs, ks, B.l_mass[s]);
   const PairLam Ls = pair_lam(A, s);
   const int nes = B.near_[s];
   const double gms = B.g_mass[s];
+  // d/d(x, y, u, v, rho, p, c, m) of s, then d/d(grad rho, tau) of s
+  double bs[RB_N] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double acc[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) {
     if (ts != 2 && B.l_type[o] != 2) return;
     int ko;
     const Side<double> vo = load_side(P, B, o, ko);
-    const Side<Dual<5>> O = plain_side<5>(vo);
-    double a1[5], a2[5];
+    double b1s[RB_N], b1o[RB_N], b2o[RB_N], b2s[RB_N], a1[5], a2[5];
     // terms summed by s (s is the summing particle)
-    const Dual<5> J1 = force_pair<Dual<5>, SOLVER>(P, S, O, B.g_mass[o], nes, Ls, A.do_shift, a1);
+    force_pair_rev<SOLVER>(P, vs, vo, B.g_mass[o], nes, Ls, A.do_shift, b1s, b1o, a1);
     // terms summed by o (s is its neighbour)
-    const Dual<5> J2 = force_pair<Dual<5>, SOLVER>(P, O, S, gms, B.near_[o], pair_lam(A, o), A.do_shift, a2);
+    force_pair_rev<SOLVER>(P, vo, vs, gms, B.near_[o], pair_lam(A, o), A.do_shift, b2o, b2s, a2);
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      acc[k] = acc[k] + (J1.d[k] + J2.d[k]);
-      acc[5 + k] = acc[5 + k] + (a1[k] + a2[k]);
-    }
+    for (int k = 0; k < RB_N; ++k) bs[k] = bs[k] + (b1s[k] + b2s[k]);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) acc[5 + k] = acc[5 + k] + (a1[k] + a2[k]);
   }, B.nb_list, B.nb_count, B.kstride);
+  acc[0] = bs[RB_X];
+  acc[1] = bs[RB_Y];
+  acc[2] = bs[RB_U];
+  acc[3] = bs[RB_V];
+  acc[4] = side_q_bar(P, vs, ks, bs);
   const double* w = A.w + kLam * (size_t)s;
   acc[5] = acc[5] + w[8];                         // grad rho is an output too (the shift uses it)
   acc[6] = acc[6] + w[9];

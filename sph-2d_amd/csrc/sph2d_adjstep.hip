@@ -304,12 +304,24 @@ __global__ void __launch_bounds__(kRedBlocks) k_adj_wm_reduce(const double* part
 }
 
 // A snapshot copy in one launch: up to 10 arrays, in 4-byte words.
+// Up to 10 copies of whole 4-byte words; 16 bytes a lane where both ends are 16-byte
+// aligned (every allocation and image slot is), the last words one at a time.
 __global__ void __launch_bounds__(256) k_adj_copy(AdjCopy C) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (int a = 0; a < C.n; ++a) {
     const uint32_t* src = static_cast<const uint32_t*>(C.src[a]);
     uint32_t* dst = static_cast<uint32_t*>(C.dst[a]);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < C.words[a]; i += stride) dst[i] = src[i];
+    const size_t w = C.words[a];
+    size_t done = 0;
+    if ((((uintptr_t)src | (uintptr_t)dst) & 15) == 0) {
+      const size_t n4 = w / 4;
+      const uint4* s4 = reinterpret_cast<const uint4*>(src);
+      uint4* d4 = reinterpret_cast<uint4*>(dst);
+      for (size_t i = t0; i < n4; i += stride) d4[i] = s4[i];
+      done = 4 * n4;
+    }
+    for (size_t i = done + t0; i < w; i += stride) dst[i] = src[i];
   }
 }
 

@@ -66,6 +66,7 @@ struct sph2d_ctx {
   bool mls_pending = false;
   double* d_out = nullptr;          // 16 doubles for reductions
   std::vector<void*> allocs;
+  std::vector<size_t> alloc_bytes;  // bytes of each entry of allocs
   // in-step pass timing: 4 events per evaluation (pass A start/end, force pass start/end)
   int ana_step = 0, vort_step = 0;   // last steps whose analytic fields / vorticity are held
   // post-processing grid (allocated on first use)
@@ -103,6 +104,7 @@ struct sph2d_ctx {
     e = hipMemsetAsync(q, 0, std::max<size_t>(n, 1) * sizeof(T), stream);
     if (e != hipSuccess) return fail(SPH2D_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
     allocs.push_back(q);
+    alloc_bytes.push_back(std::max<size_t>(n, 1) * sizeof(T));
     *p = static_cast<T*>(q);
     return SPH2D_OK;
   }
@@ -2507,15 +2509,20 @@ struct AdjTape {
   size_t ws_cap = 0;
   double* pad = nullptr;           // (dL/dx_p, dL/du_p) per step
   size_t pad_cap = 0;
-  int* iws = nullptr;              // flags per gid, arg-max, height bits
+  int* iws = nullptr;              // flags per gid, arg-max, height bits (K sets with images)
   size_t iws_cap = 0;
+  // The segment's steps as whole device images (every allocation of the context after
+  // the step), so the reverse restores a step instead of computing it again
+  char* img = nullptr;
+  size_t img_cap = 0;
   bool ready = false;
 
   void release() {
-    for (void* p : {(void*)ckpt, (void*)seg, (void*)fin, (void*)probe_dev, (void*)ws, (void*)pad, (void*)iws})
+    for (void* p : {(void*)ckpt, (void*)seg, (void*)fin, (void*)probe_dev, (void*)ws, (void*)pad, (void*)iws,
+                    (void*)img})
       if (p) (void)hipFree(p);
-    ckpt = seg = fin = nullptr; probe_dev = ws = pad = nullptr; iws = nullptr;
-    ckpt_cap = seg_cap = fin_cap = probe_cap = ws_cap = pad_cap = iws_cap = 0;
+    ckpt = seg = fin = nullptr; probe_dev = ws = pad = nullptr; iws = nullptr; img = nullptr;
+    ckpt_cap = seg_cap = fin_cap = probe_cap = ws_cap = pad_cap = iws_cap = img_cap = 0;
   }
 };
 
@@ -2567,6 +2574,56 @@ int snap_copy(sph2d_ctx* c, AdjTape* t, char* snap, bool save) {
     C.words[k] = sz[k] / 4;
   }
   launch_adj_copy(c->stream, C);
+  HIP_OR_FAIL(hipGetLastError());
+  return SPH2D_OK;
+}
+
+// Allocations a step image leaves out: what the predictor-corrector step of the
+// walled tank neither reads nor writes (the RK stage slabs, the MLS density, the
+// diagnostic pair census, the analytic and vorticity fields and reductions of
+// sampling, the post-processing grid).
+bool image_skip(const sph2d_ctx* c, const void* q) {
+  const Buffers& B = c->B;
+  for (const void* x : {(const void*)B.k_drho, (const void*)B.k_vel, (const void*)B.k_acc, (const void*)B.mls_rho,
+                        (const void*)B.pair_count, (const void*)B.pair_off, (const void*)B.ana,
+                        (const void*)B.g_vort, (const void*)B.red, (const void*)c->grid_out})
+    if (q == x) return true;
+  return false;
+}
+
+// Bytes of a whole device image of the context: its allocations but the skipped ones,
+// each from a 256-byte boundary.
+size_t image_bytes(const sph2d_ctx* c) {
+  size_t b = 0;
+  for (size_t k = 0; k < c->allocs.size(); ++k)
+    if (!image_skip(c, c->allocs[k])) b += (c->alloc_bytes[k] + 255) & ~(size_t)255;
+  return b;
+}
+
+// Saves (save) or restores the allocations of the context's image into / from img.
+int image_copy(sph2d_ctx* c, char* img, bool save) {
+  AdjCopy C;
+  C.n = 0;
+  size_t off = 0;
+  for (size_t k = 0; k < c->allocs.size(); ++k) {
+    if (image_skip(c, c->allocs[k])) continue;
+    const size_t n = c->alloc_bytes[k];
+    char* q = img + off;
+    off += (n + 255) & ~(size_t)255;
+    if (n % 4) {                                  // not a whole number of words: a plain copy
+      HIP_OR_FAIL(hipMemcpyAsync(save ? (void*)q : c->allocs[k], save ? c->allocs[k] : (const void*)q, n,
+                                 hipMemcpyDeviceToDevice, c->stream));
+      continue;
+    }
+    C.src[C.n] = save ? c->allocs[k] : (const void*)q;
+    C.dst[C.n] = save ? (void*)q : c->allocs[k];
+    C.words[C.n] = n / 4;
+    if (++C.n == 10) {
+      launch_adj_copy(c->stream, C);
+      C.n = 0;
+    }
+  }
+  if (C.n) launch_adj_copy(c->stream, C);
   HIP_OR_FAIL(hipGetLastError());
   return SPH2D_OK;
 }
@@ -2679,11 +2736,32 @@ int sph2d_backward(sph2d_ctx* c, const double* dl_dprobe, double* dl_da) {
   const int K = t->cfg.checkpoint_every;
   const size_t np = (size_t)t->cfg.n_probes;
   const size_t G = (size_t)c->cap_global, L = (size_t)c->cap_local;
-  if ((rc = grow(&t->seg, &t->seg_cap, t->snap_bytes * (size_t)K))) return rc;
+  // Each step of a segment runs once more: its whole device image is kept (K images)
+  // for the reverse, with the wavemaker flags of its start state. Where K images do
+  // not fit, the segment keeps step-start snapshots and the reverse runs each step
+  // again from its snapshot.
+  const size_t ib = image_bytes(c);
+  // flags per gid, the arg-max, then the height bits from an 8-byte boundary
+  const size_t hoff = (G + 2 + 1) & ~(size_t)1, nflag = hoff + 4;
+  bool images = t->img_cap >= ib * (size_t)K && t->img;
+  if (!images) {
+    if (t->img) (void)hipFree(t->img);
+    t->img = nullptr;
+    t->img_cap = 0;
+    void* q = nullptr;
+    if (hipMalloc(&q, ib * (size_t)K) == hipSuccess) {
+      t->img = static_cast<char*>(q);
+      t->img_cap = ib * (size_t)K;
+      images = true;
+    } else {
+      (void)hipGetLastError();                    // the failed allocation leaves no error behind
+    }
+  }
+  if (!images && (rc = grow(&t->seg, &t->seg_cap, t->snap_bytes * (size_t)K))) return rc;
   const size_t nws = 6 * G + 3 * L + 3 * L + 11 * L + 10 * L + 5 * L + 5 * L + 5 * L + L + 3 * G + 3 * 256;
   if ((rc = grow(&t->ws, &t->ws_cap, nws))) return rc;
   if ((rc = grow(&t->pad, &t->pad_cap, 2 * (size_t)t->nsteps))) return rc;
-  if ((rc = grow(&t->iws, &t->iws_cap, G + 8))) return rc;
+  if ((rc = grow(&t->iws, &t->iws_cap, nflag * (images ? (size_t)K : 1)))) return rc;
   double* lamG = t->ws;
   double* lamR = lamG + 6 * G;
   double* lamRp = lamR + 3 * L;
@@ -2697,7 +2775,7 @@ int sph2d_backward(sph2d_ctx* c, const double* dl_dprobe, double* dl_da) {
   double* part = c3 + 3 * G;
   int* flags = t->iws;
   int* argmax = t->iws + G;
-  auto* hbits = reinterpret_cast<unsigned long long*>(t->iws + G + 2);
+  auto* hbits = reinterpret_cast<unsigned long long*>(t->iws + hoff);
   double* lam_probe = t->probe_dev + 2 * np + np * t->nsamples;
   if (t->nsamples > 0)
     HIP_OR_FAIL(hipMemcpyAsync(lam_probe, dl_dprobe, np * t->nsamples * sizeof(double), hipMemcpyHostToDevice,
@@ -2710,20 +2788,44 @@ int sph2d_backward(sph2d_ctx* c, const double* dl_dprobe, double* dl_da) {
   for (int sg = t->nckpt - 1; sg >= 0; --sg) {
     const int c0 = sg * K, len = std::min(K, t->nsteps - c0);
     if ((rc = snap_copy(c, t, t->ckpt + (size_t)sg * t->snap_bytes, false))) return rc;
-    for (int j = 0; j < len; ++j) {              // the segment forward again, one snapshot per step
-      if ((rc = snap_copy(c, t, t->seg + (size_t)j * t->snap_bytes, true))) return rc;
-      if (j + 1 < len && (rc = adj_step(c, t, t->first + c0 + j))) return rc;
+    if (images) {
+      // the segment forward again: the wavemaker's branches from each step's start
+      // state, the step, its image
+      for (int j = 0; j < len; ++j) {
+        const int it = t->first + c0 + j;
+        int* fl = t->iws + (size_t)j * nflag;
+        double xp, up;
+        paddle(t, c->prm.dt, it, xp, up);
+        auto* hb = reinterpret_cast<unsigned long long*>(fl + hoff);
+        launch_adj_wm_init(c->stream, hb, fl + G);
+        launch_wavemaker_height(c->stream, c->P, c->B, c->cap_fluid, xp, hb);
+        launch_adj_wm_flags(c->stream, c->P, c->B, c->cap_fluid, xp, fl, hb, fl + G);
+        if ((rc = adj_step(c, t, it))) return rc;
+        if ((rc = image_copy(c, t->img + (size_t)j * ib, true))) return rc;
+      }
+    } else {
+      for (int j = 0; j < len; ++j) {            // the segment forward again, one snapshot per step
+        if ((rc = snap_copy(c, t, t->seg + (size_t)j * t->snap_bytes, true))) return rc;
+        if (j + 1 < len && (rc = adj_step(c, t, t->first + c0 + j))) return rc;
+      }
     }
     for (int j = len - 1; j >= 0; --j) {
       const int k = c0 + j, it = t->first + k;
-      if ((rc = snap_copy(c, t, t->seg + (size_t)j * t->snap_bytes, false))) return rc;
-      double xp, up;
-      paddle(t, c->prm.dt, it, xp, up);
-      // the wavemaker's branches from the state before its update, then the step again
-      launch_adj_wm_init(c->stream, hbits, argmax);
-      launch_wavemaker_height(c->stream, c->P, c->B, c->cap_fluid, xp, hbits);
-      launch_adj_wm_flags(c->stream, c->P, c->B, c->cap_fluid, xp, flags, hbits, argmax);
-      if ((rc = adj_step(c, t, it))) return rc;
+      if (images) {
+        flags = t->iws + (size_t)j * nflag;
+        argmax = flags + G;
+        hbits = reinterpret_cast<unsigned long long*>(flags + hoff);
+        if ((rc = image_copy(c, t->img + (size_t)j * ib, false))) return rc;
+      } else {
+        if ((rc = snap_copy(c, t, t->seg + (size_t)j * t->snap_bytes, false))) return rc;
+        double xp, up;
+        paddle(t, c->prm.dt, it, xp, up);
+        // the wavemaker's branches from the state before its update, then the step again
+        launch_adj_wm_init(c->stream, hbits, argmax);
+        launch_wavemaker_height(c->stream, c->P, c->B, c->cap_fluid, xp, hbits);
+        launch_adj_wm_flags(c->stream, c->P, c->B, c->cap_fluid, xp, flags, hbits, argmax);
+        if ((rc = adj_step(c, t, it))) return rc;
+      }
       if ((k + 1) % t->cfg.sample_every == 0) {
         const int smp = (k + 1) / t->cfg.sample_every - 1;
         for (size_t q = 0; q < np; ++q)
