@@ -19,17 +19,20 @@
 //   k_adj_weights   per local particle: its output weights (zero for ghosts)
 //   k_adj_force     per local s, over neighbours o: d/d(state_s, grad rho_s, tau_s)
 //                   of the force-pass pair terms summed by s (weights of s) and by
-//                   o (weights of o); state directions by 5-direction duals, grad rho
-//                   and tau (linear in the pair terms) in closed form
+//                   o (weights of o); the state directions by reverse mode over the
+//                   pair formulas, grad rho and tau (linear in the pair terms) in
+//                   closed form
 //   k_adj_tau       per local s: recompute the strain S_s, reverse tau(S_s, rho_s)
 //   k_adj_sums      per local s, over neighbours o: reverse of the pass-A sums
 //                   (S and grad rho) into d/d(state_s); result stored by gid
 //   k_adj_corners   corner ghosts into the x-ghost they copy
 //   k_adj_owner     every ghost into its fluid particle (same slot arithmetic as
 //                   the ghost emission of Distribute_Period_Boundary_Particle.f90)
-// Pair-level Jacobians with respect to the state come from forward-mode duals over
-// the same pair formulas as the FP64 force pass; the reverse sweep over the
-// particle graph is explicit.
+// Pair-level vector-Jacobian products with respect to the state come from a
+// hand-written reverse mode of the FP64 force pass's pair formulas (one primal and
+// one adjoint sweep per pair and side); the per-particle closures (the SPS stress of
+// k_adj_tau, the walls) use forward-mode duals; the reverse sweep over the particle
+// graph is explicit.
 #include "sph2d_internal.h"
 #include "sph2d_device.h"
 
@@ -176,121 +179,6 @@ __device__ __forceinline__ void derive_side(const DevParams& P, Side<T>& a, int 
   }
 }
 
-// Weighted force-pass pair terms summed by particle S with neighbour O: the same
-// formulas as eval_c in FP64 (Solver_Normalized_*_Riemann_Delta_SPS.f90:300-583)
-// and the shift sum (Shift_particle_position.f90:78-103). gaux receives d/d(grad rho
-// x, y, tau xx, xy, yy) of either side: those enter the pair terms linearly, through
-// the sums S + O, so both sides share one closed form.
-template <class T, int SOLVER>
-__device__ __forceinline__ T force_pair(const DevParams& P, const Side<T>& S, const Side<T>& O, double gmo, int nes,
-                                        const PairLam& L, int do_shift, double* gaux) {
-  const T dx = S.x - O.x, dy = S.y - O.y;
-  const T r2 = dx * dx + dy * dy;
-  const T r = tsqrt(r2);
-  const T inv_r = 1.0 / r;
-  const T q = r * P.inv_h;
-  const T eq = texp(-(q * q));
-  const T w = (eq - P.e9) * P.kfactor;
-  const T kt = eq * (-2.0 * q) * (inv_r * P.inv_h) * P.kfactor;
-  const T dwx = kt * dx, dwy = kt * dy;
-  const T ps = S.p, cs = S.c, po = O.p, co = O.c;
-  const T mo = O.m;
-  const T Vo = mo / O.rho;
-  const T ex = -(dx * inv_r), ey = -(dy * inv_r);
-  const T UL = S.u * ex + S.v * ey;
-  const T UR = O.u * ex + O.v * ey;
-  const T dU = UL - UR;
-  const T Pbar = (ps + po) * 0.5;
-  const T vbx = (S.u + O.u) * 0.5, vby = (S.v + O.v) * 0.5;
-  const T cbar = (cs + co) * 0.5;
-  const T rbar = (S.rho + O.rho) * 0.5;
-  T vsx, vsy, Ps;
-  if (SOLVER == SPH2D_SOLVER_LINEAR_RIEMANN) {
-    const T zs = S.rho * cs, zo = O.rho * co;
-    const T inv_z = 1.0 / (zs + zo);
-    const T Us = (zs * UR + zo * UL + ps - po) * inv_z;
-    vsx = Us * ex;
-    vsy = Us * ey;
-    Ps = (zs * po + zo * ps + zs * zo * dU) * inv_z;
-  } else {
-    const T Ub = (UL + UR) * 0.5;
-    const T Us = Ub + 0.5 * (ps - po) / rbar * P.inv_c0;
-    vsx = (Us - Ub) * ex + vbx;
-    vsy = (Us - Ub) * ey + vby;
-    T bR = val(dU) > 0.0 ? 3.0 * dU : T(0.0);
-    bR = val(bR) < val(cbar) ? bR : cbar;
-    Ps = Pbar + 0.5 * bR * rbar * dU;
-  }
-  if (nes) { vsx = vbx; vsy = vby; Ps = Pbar; }
-  const T ti = (S.u - vsx) * dwx + (S.v - vsy) * dwy;
-  const T drho = 2.0 * S.rho * Vo * ti;
-  const T rij = 1.0 / (S.rho * O.rho);
-  const T fp = mo * (-2.0 * Ps * rij);
-  T axp = fp * dwx, ayp = fp * dwy;
-  const T vijx = S.u - O.u, vijy = S.v - O.v;
-  const T vx = vijx * dx + vijy * dy;
-  if (val(vx) < 0.0) {
-    const double fh = P.fai * P.h;
-    const T phi = P.h * vx / (r2 + fh * fh);
-    const T Pi = (-(P.alpha * cbar * phi) + P.beta * (phi * phi)) / rbar;
-    axp = axp + mo * (-Pi * dwx);
-    ayp = ayp + mo * (-Pi * dwy);
-  }
-  const T Rd = dx * dwx + dy * dwy;
-  const T lf = Vo * 2.0 * P.nu * Rd / r2;
-  axp = axp + lf * vijx;
-  ayp = ayp + lf * vijy;
-  const T tmp = P.delta_alpha * P.h * cbar * vx / r2 * P.rho0 / S.rho * Vo;
-  axp = axp + tmp * dwx;
-  ayp = ayp + tmp * dwy;
-  T tp = (S.txx + O.txx) * dwx + (S.txy + O.txy) * dwy;
-  tp = tp * rij;
-  axp = axp + gmo * tp;
-  tp = tp + (S.tyy + O.tyy) * dwy + (S.txy + O.txy) * dwx;
-  tp = tp * rij;
-  ayp = ayp + gmo * tp;
-  T J = L.d * drho + L.ax * axp + L.ay * ayp;
-  if (SOLVER == SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN && P.epsilon_xsph != 0.0) {
-    const T f = P.epsilon_xsph * P.xsph_f * mo / (S.rho + O.rho) * w;
-    J = J - L.gx * (f * vijx) - L.gy * (f * vijy);
-  }
-  const T psi = (O.rho - S.rho) + 0.5 * ((S.gx + O.gx) * dx + (S.gy + O.gy) * dy);
-  const T cf = 2.0 * P.delta * P.h * cbar * Rd / r2 * Vo;
-  J = J - L.c * (cf * psi);
-  if (do_shift && (L.shx != 0.0 || L.shy != 0.0)) {
-    const T wr = w * P.inv_wdx;
-    const T wr2 = wr * wr;
-    const T f = P.shift_beta * P.h2 * (1.0 + P.shift_R * (wr2 * wr2)) * Vo;
-    J = J - L.shx * (f * dwx) - L.shy * (f * dwy);
-  }
-  {
-    const double rv = val(rij), wx = val(dwx), wy = val(dwy), cv = val(cf);
-    gaux[0] = -L.c * cv * 0.5 * val(dx);
-    gaux[1] = -L.c * cv * 0.5 * val(dy);
-    gaux[2] = gmo * rv * (L.ax * wx + L.ay * (wx * rv));
-    gaux[3] = gmo * rv * (L.ax * wy + L.ay * (wy * rv + wx));
-    gaux[4] = gmo * rv * (L.ay * wy);
-  }
-  return J;
-}
-
-// Weighted pass-A pair terms of S with neighbour O: strain S_xx, S_xy, S_yy and
-// grad rho (Solver...SPS.f90:144-182,508-526 with the identity renormalisation).
-template <class T>
-__device__ __forceinline__ T sums_pair(const DevParams& P, const Side<T>& S, const Side<T>& O, const double* g) {
-  const T dx = S.x - O.x, dy = S.y - O.y;
-  const T r = tsqrt(dx * dx + dy * dy);
-  const T q = r * P.inv_h;
-  const T eq = texp(-(q * q));
-  const T kt = eq * (-2.0 * q) * ((1.0 / r) * P.inv_h) * P.kfactor;
-  const T dwx = kt * dx, dwy = kt * dy;
-  const T Vo = O.m / O.rho;
-  const T vdx = O.u - S.u, vdy = O.v - S.v;
-  const T rd = (O.rho - S.rho) * Vo;
-  return g[0] * (vdx * dwx * Vo) + g[1] * ((vdx * dwy + vdy * dwx) * Vo) + g[2] * (vdy * dwy * Vo) +
-         g[3] * (rd * dwx) + g[4] * (rd * dwy);
-}
-
 // The primary state of local particle s as plain values.
 __device__ __forceinline__ Side<double> load_side(const DevParams& P, const Buffers& B, int s, int& kind) {
   Side<double> a;
@@ -305,31 +193,12 @@ __device__ __forceinline__ Side<double> load_side(const DevParams& P, const Buff
   return a;
 }
 
-// The same side with 5 tangent directions seeded on (x, y, u, v, q).
-template <int N>
-__device__ __forceinline__ Side<Dual<N>> seeded_side(const DevParams& P, const Side<double>& a, int kind,
-                                                    double mass) {
-  Side<Dual<N>> r;
-  r.x = a.x; r.y = a.y; r.u = a.u; r.v = a.v; r.q = a.q;
-  r.x.d[0] = 1.0; r.y.d[1] = 1.0; r.u.d[2] = 1.0; r.v.d[3] = 1.0; r.q.d[4] = 1.0;
-  r.gx = a.gx; r.gy = a.gy; r.txx = a.txx; r.txy = a.txy; r.tyy = a.tyy;
-  derive_side(P, r, kind, mass);
-  return r;
-}
-
-template <int N>
-__device__ __forceinline__ Side<Dual<N>> plain_side(const Side<double>& a) {
-  Side<Dual<N>> r;
-  r.x = a.x; r.y = a.y; r.u = a.u; r.v = a.v; r.q = a.q; r.rho = a.rho; r.p = a.p; r.c = a.c; r.m = a.m;
-  r.gx = a.gx; r.gy = a.gy; r.txx = a.txx; r.txy = a.txy; r.tyy = a.tyy;
-  return r;
-}
-
-// Reverse mode of force_pair<double, SOLVER> (the weighted force-pass pair terms of
-// A summing over neighbour B): the same forward formulas, then their adjoints in
-// reverse order. bA, bB receive d(J)/d(x, y, u, v, rho, p, c, m) of either side
-// (what a side's primary state q moves is chained by side_q_bar), gaux the closed
-// form of force_pair. The branches of the forward (the limiter's cases, vx < 0, the
+// Reverse mode of the weighted force-pass pair terms of A summing over neighbour B
+// (Solver_Normalized_*_Riemann_Delta_SPS.f90:300-583 as the FP64 force pass computes
+// them, with the shift sum of Shift_particle_position.f90:78-103): the forward
+// formulas, then their adjoints in reverse order. bA, bB receive d(J)/d(x, y, u, v, rho, p, c, m) of either side
+// (what a side's primary state q moves is chained by side_q_bar), gaux d(J)/d(grad
+// rho x, y, tau xx, xy, yy) of either side (linear, through the sums A + B). The branches of the forward (the limiter's cases, vx < 0, the
 // near-surface flag) are taken as the forward takes them. One primal sweep and one
 // adjoint sweep in FP64 registers: a fraction of the work and registers of 5-direction
 // duals over the same formulas.
@@ -410,7 +279,7 @@ __device__ __forceinline__ void force_pair_rev(const DevParams& P, const Side<do
   const double psi = (B.rho - A.rho) + 0.5 * ((A.gx + B.gx) * dx + (A.gy + B.gy) * dy);
   const double k2 = 2.0 * P.delta * P.h;
   const double cf = k2 * cbar * Rd / r2 * Vo;
-  // d/d(grad rho x, y, tau xx, xy, yy): linear, through the sums A + B (as force_pair)
+  // d/d(grad rho x, y, tau xx, xy, yy): linear, through the sums A + B
   gaux[0] = -L.c * cf * 0.5 * dx;
   gaux[1] = -L.c * cf * 0.5 * dy;
   gaux[2] = gmo * rij * (L.ax * dwx + L.ay * (dwx * rij));
@@ -635,6 +504,53 @@ __device__ __forceinline__ double side_q_bar(const DevParams& P, const Side<doub
   return b[RB_RHO] + b[RB_P] * dp + b[RB_C] * dc + (kind == KIND_TIED ? b[RB_M] * P.vol0 : 0.0);
 }
 
+
+// Reverse mode of the weighted pass-A terms of A with neighbour B: strain S_xx, S_xy,
+// S_yy and grad rho (Solver...SPS.f90:144-182,508-526 with the identity
+// renormalisation), weights g; bA, bB receive d(J)/d(x, y, u, v, rho, p, c, m) of
+// either side.
+__device__ __forceinline__ void sums_pair_rev(const DevParams& P, const Side<double>& A, const Side<double>& B,
+                                              const double* g, double* bA, double* bB) {
+#pragma unroll
+  for (int k = 0; k < RB_N; ++k) { bA[k] = 0.0; bB[k] = 0.0; }
+  const double dx = A.x - B.x, dy = A.y - B.y;
+  const double r = sqrt(dx * dx + dy * dy);
+  const double q = r * P.inv_h;
+  const double eq = exp(-(q * q));
+  const double kq = ((1.0 / r) * P.inv_h) * P.kfactor;
+  const double kt = eq * (-2.0 * q) * kq;
+  const double dwx = kt * dx, dwy = kt * dy;
+  const double Vo = B.m / B.rho;
+  const double vdx = B.u - A.u, vdy = B.v - A.v;
+  const double drh = B.rho - A.rho;
+  const double rd = drh * Vo;
+  const double b_dwx = g[0] * vdx * Vo + g[1] * vdy * Vo + g[3] * rd;
+  const double b_dwy = g[1] * vdx * Vo + g[2] * vdy * Vo + g[4] * rd;
+  const double b_vdx = g[0] * dwx * Vo + g[1] * dwy * Vo;
+  const double b_vdy = g[1] * dwx * Vo + g[2] * dwy * Vo;
+  const double b_rd = g[3] * dwx + g[4] * dwy;
+  const double b_Vo = g[0] * vdx * dwx + g[1] * (vdx * dwy + vdy * dwx) + g[2] * vdy * dwy + b_rd * drh;
+  bB[RB_RHO] += b_rd * Vo - b_Vo * Vo / B.rho;
+  bA[RB_RHO] -= b_rd * Vo;
+  bB[RB_U] += b_vdx;
+  bA[RB_U] -= b_vdx;
+  bB[RB_V] += b_vdy;
+  bA[RB_V] -= b_vdy;
+  bB[RB_M] += b_Vo / B.rho;
+  const double b_kt = b_dwx * dx + b_dwy * dy;
+  double b_dx = b_dwx * kt, b_dy = b_dwy * kt;
+  const double b_eq = b_kt * (-2.0 * q) * kq;
+  double b_q = b_kt * eq * (-2.0) * kq + b_eq * eq * (-2.0 * q);
+  const double b_invr = b_kt * eq * (-2.0 * q) * P.inv_h * P.kfactor;
+  const double b_r = b_q * P.inv_h - b_invr / (r * r);
+  b_dx += b_r * dx / r;
+  b_dy += b_r * dy / r;
+  bA[RB_X] += b_dx;
+  bB[RB_X] -= b_dx;
+  bA[RB_Y] += b_dy;
+  bB[RB_Y] -= b_dy;
+}
+
 struct AdjBufs {
   const double* lam;   // gid mode: 5 per fluid gid (drho, ax, ay, avg_x, avg_y); slot mode: unused
   double* w;           // kLam per local particle (PairLam, then grad rho x, y)
@@ -763,19 +679,19 @@ __global__ void __launch_bounds__(kBlock) k_adj_sums(DevParams P, Buffers B, Adj
   const double2 ps = B.l_pos[s];
   int ks;
   const Side<double> vs = load_side(P, B, s, ks);
-  const Side<Dual<5>> S = seeded_side<5>(P, vs, ks, B.l_mass[s]);
   const double* gas = A.ga + 5 * (size_t)s;
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double bs[RB_N] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for_each_neighbour<kNbReplay>(P, B.cell_start, B.l_pos, B.l_pos32, s, ps, B.l_cell[s], [&](int o, int d, int li) {
     if (ts != 2 && B.l_type[o] != 2) return;
     int ko;
     const Side<double> vo = load_side(P, B, o, ko);
-    const Side<Dual<5>> O = plain_side<5>(vo);
-    const Dual<5> J1 = sums_pair<Dual<5>>(P, S, O, gas);
-    const Dual<5> J2 = sums_pair<Dual<5>>(P, O, S, A.ga + 5 * (size_t)o);
+    double b1s[RB_N], b1o[RB_N], b2o[RB_N], b2s[RB_N];
+    sums_pair_rev(P, vs, vo, gas, b1s, b1o);                       // s sums over o
+    sums_pair_rev(P, vo, vs, A.ga + 5 * (size_t)o, b2o, b2s);       // o sums over s
 #pragma unroll
-    for (int k = 0; k < 5; ++k) acc[k] = acc[k] + (J1.d[k] + J2.d[k]);
+    for (int k = 0; k < RB_N; ++k) bs[k] = bs[k] + (b1s[k] + b2s[k]);
   }, B.nb_list, B.nb_count, B.kstride);
+  const double acc[5] = {bs[RB_X], bs[RB_Y], bs[RB_U], bs[RB_V], side_q_bar(P, vs, ks, bs)};
   const size_t dst = SLOT ? (size_t)s : (size_t)B.name[s];
   const double* gs = A.gs + 5 * (size_t)s;
   for (int k = 0; k < 5; ++k) A.G[5 * dst + k] = gs[k] + acc[k];
