@@ -585,17 +585,19 @@ __device__ __forceinline__ PairLam pair_lam(const AdjBufs& A, int s) {
   return PairLam{w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7]};
 }
 
-template <int SOLVER>
-__global__ void __launch_bounds__(kBlock) k_adj_force(DevParams P, Buffers B, AdjBufs A) {
+// The two halves of the force pass's reverse for local s: SIDE 0, the terms s sums
+// over its neighbours (weights of s; writes gs, ga); SIDE 1, the terms each neighbour
+// o sums over s (weights of o; adds to gs, ga). Apart, each half fits 256 registers
+// and runs 2 waves per SIMD where the two together would take one.
+template <int SOLVER, int SIDE>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
+k_adj_force(DevParams P, Buffers B, AdjBufs A) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= B.counters[C_NLOCAL]) return;
   const int ts = B.l_type[s];
   const double2 ps = B.l_pos[s];
   int ks;
   const Side<double> vs = load_side(P, B, s, ks);
-  const PairLam Ls = pair_lam(A, s);
-  const int nes = B.near_[s];
-  const double gms = B.g_mass[s];
   // d/d(x, y, u, v, rho, p, c, m) of s, then d/d(grad rho, tau) of s
   double bs[RB_N] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double acc[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -603,27 +605,34 @@ __global__ void __launch_bounds__(kBlock) k_adj_force(DevParams P, Buffers B, Ad
     if (ts != 2 && B.l_type[o] != 2) return;
     int ko;
     const Side<double> vo = load_side(P, B, o, ko);
-    double b1s[RB_N], b1o[RB_N], b2o[RB_N], b2s[RB_N], a1[5], a2[5];
-    // terms summed by s (s is the summing particle)
-    force_pair_rev<SOLVER>(P, vs, vo, B.g_mass[o], nes, Ls, A.do_shift, b1s, b1o, a1);
-    // terms summed by o (s is its neighbour)
-    force_pair_rev<SOLVER>(P, vo, vs, gms, B.near_[o], pair_lam(A, o), A.do_shift, b2o, b2s, a2);
+    double bx[RB_N], by[RB_N], a[5];
+    if (SIDE == 0) {
+      force_pair_rev<SOLVER>(P, vs, vo, B.g_mass[o], B.near_[s], pair_lam(A, s), A.do_shift, bx, by, a);
 #pragma unroll
-    for (int k = 0; k < RB_N; ++k) bs[k] = bs[k] + (b1s[k] + b2s[k]);
+      for (int k = 0; k < RB_N; ++k) bs[k] = bs[k] + bx[k];
+    } else {
+      force_pair_rev<SOLVER>(P, vo, vs, B.g_mass[s], B.near_[o], pair_lam(A, o), A.do_shift, by, bx, a);
 #pragma unroll
-    for (int k = 0; k < 5; ++k) acc[5 + k] = acc[5 + k] + (a1[k] + a2[k]);
+      for (int k = 0; k < RB_N; ++k) bs[k] = bs[k] + bx[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) acc[5 + k] = acc[5 + k] + a[k];
   }, B.nb_list, B.nb_count, B.kstride);
   acc[0] = bs[RB_X];
   acc[1] = bs[RB_Y];
   acc[2] = bs[RB_U];
   acc[3] = bs[RB_V];
   acc[4] = side_q_bar(P, vs, ks, bs);
-  const double* w = A.w + kLam * (size_t)s;
-  acc[5] = acc[5] + w[8];                         // grad rho is an output too (the shift uses it)
-  acc[6] = acc[6] + w[9];
   double* gs = A.gs + 5 * (size_t)s;
   double* ga = A.ga + 5 * (size_t)s;
-  for (int k = 0; k < 5; ++k) { gs[k] = acc[k]; ga[k] = acc[5 + k]; }
+  if (SIDE == 0) {
+    const double* w = A.w + kLam * (size_t)s;
+    acc[5] = acc[5] + w[8];                       // grad rho is an output too (the shift uses it)
+    acc[6] = acc[6] + w[9];
+    for (int k = 0; k < 5; ++k) { gs[k] = acc[k]; ga[k] = acc[5 + k]; }
+  } else {
+    for (int k = 0; k < 5; ++k) { gs[k] = gs[k] + acc[k]; ga[k] = ga[k] + acc[5 + k]; }
+  }
 }
 
 // tau = rho (2 nut S - k - b) etc. (Solver...SPS.f90:256-296) reversed: the tau
@@ -844,10 +853,13 @@ __global__ void __launch_bounds__(kBlock) k_adj_owner(DevParams P, Buffers B, Ad
 }
 
 static void launch_force_tau(hipStream_t st, const DevParams& P, Buffers& B, int g, const AdjBufs& A) {
-  if (P.solver == SPH2D_SOLVER_LINEAR_RIEMANN)
-    hipLaunchKernelGGL(k_adj_force<SPH2D_SOLVER_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, st, P, B, A);
-  else
-    hipLaunchKernelGGL(k_adj_force<SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN>, dim3(g), dim3(kBlock), 0, st, P, B, A);
+  if (P.solver == SPH2D_SOLVER_LINEAR_RIEMANN) {
+    hipLaunchKernelGGL((k_adj_force<SPH2D_SOLVER_LINEAR_RIEMANN, 0>), dim3(g), dim3(kBlock), 0, st, P, B, A);
+    hipLaunchKernelGGL((k_adj_force<SPH2D_SOLVER_LINEAR_RIEMANN, 1>), dim3(g), dim3(kBlock), 0, st, P, B, A);
+  } else {
+    hipLaunchKernelGGL((k_adj_force<SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN, 0>), dim3(g), dim3(kBlock), 0, st, P, B, A);
+    hipLaunchKernelGGL((k_adj_force<SPH2D_SOLVER_MODIFIED_LINEAR_RIEMANN, 1>), dim3(g), dim3(kBlock), 0, st, P, B, A);
+  }
   hipLaunchKernelGGL(k_adj_tau, dim3(g), dim3(kBlock), 0, st, P, B, A);
 }
 

@@ -304,8 +304,8 @@ __global__ void __launch_bounds__(kRedBlocks) k_adj_wm_reduce(const double* part
 }
 
 // A snapshot copy in one launch: up to 10 arrays, in 4-byte words.
-// Up to 10 copies of whole 4-byte words; 16 bytes a lane where both ends are 16-byte
-// aligned (every allocation and image slot is), the last words one at a time.
+// Up to kAdjCopies copies of whole 4-byte words; 16 bytes a lane where both ends are
+// 16-byte aligned (every allocation and image slot is), the last words one at a time.
 __global__ void __launch_bounds__(256) k_adj_copy(AdjCopy C) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -2618,7 +2618,7 @@ int image_copy(sph2d_ctx* c, char* img, bool save) {
     C.src[C.n] = save ? c->allocs[k] : (const void*)q;
     C.dst[C.n] = save ? (void*)q : c->allocs[k];
     C.words[C.n] = n / 4;
-    if (++C.n == 10) {
+    if (++C.n == kAdjCopies) {
       launch_adj_copy(c->stream, C);
       C.n = 0;
     }

@@ -159,10 +159,11 @@ void launch_adj_wm_flags(hipStream_t st, const DevParams& P, Buffers& B, int cap
 void launch_adj_wm_reverse(hipStream_t st, const DevParams& P, Buffers& B, int n_total, int n_wm, const double* G,
                            const int* flags, const unsigned long long* hbits, double* lamG, double* c3,
                            double* part, const int* argmax, double* out2);
+constexpr int kAdjCopies = 64;                   // copies of one k_adj_copy launch
 struct AdjCopy {
-  const void* src[10];
-  void* dst[10];
-  size_t words[10];
+  const void* src[kAdjCopies];
+  void* dst[kAdjCopies];
+  size_t words[kAdjCopies];
   int n;
 };
 void launch_adj_copy(hipStream_t st, const AdjCopy& C);
