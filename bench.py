@@ -68,8 +68,8 @@ STEP_BYTES = {64: 295 + 144 + 168 + 76, 32: 179 + 96 + 136 + 56}
 # what it reads for a 256 MiB device-to-device copy (summary and commands in the source).
 _TRAFFIC_SRC = "profiles/tgv2048_force_pass_hbm_traffic.md"
 PASS_TRAFFIC = {
-    "force": {"bytes_per_particle": 152.7, "n_side": 2048, "source": _TRAFFIC_SRC},
-    "pass_a": {"bytes_per_particle": 165.7, "n_side": 2048, "source": _TRAFFIC_SRC},
+    "force": {"bytes_per_particle": 151.8, "n_side": 2048, "source": _TRAFFIC_SRC},
+    "pass_a": {"bytes_per_particle": 163.9, "n_side": 2048, "source": _TRAFFIC_SRC},
 }
 # what each mixed pass reads and writes for its own particle, per launch:
 # force pass: records 48 + (p, c) 8, type, cell, near 12, neighbour count 4 and list ~40;
@@ -80,8 +80,8 @@ PASS_READ_WRITE = {"force": 48 + 8 + 12 + 44 + 40, "pass_a": 48 + 104}
 # FLOPs per subdomain particle of one launch (SQ_INSTS_VALU_FLOPS_* x 64 lanes)
 _VALU_SRC = "profiles/tgv2048_force_pass_valu.md"
 PASS_VALU = {
-    "force": {"fp32_flop_per_particle": 6450.0, "fp64_flop_per_particle": 2.0, "source": _VALU_SRC},
-    "pass_a": {"fp32_flop_per_particle": 3522.0, "fp64_flop_per_particle": 225.0, "source": _VALU_SRC},
+    "force": {"fp32_flop_per_particle": 5293.0, "fp64_flop_per_particle": 2.0, "source": _VALU_SRC},
+    "pass_a": {"fp32_flop_per_particle": 3094.0, "fp64_flop_per_particle": 225.0, "source": _VALU_SRC},
 }
 FP32_VECTOR_TFLOPS = 157.3
 FP32_VECTOR_TFLOPS_UNPACKED = 78.6
