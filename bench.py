@@ -290,9 +290,21 @@ def evaluation_vjp_leg(sph2d, n_side: int, reps: int) -> dict:
     for _ in range(reps):
         sim.single_step_compute_vjp(*lam)
     rev = (time.perf_counter() - t0) / reps
+    # the same reverse on device arrays (sph2d_single_step_compute_vjp_device): the
+    # kernels only, no host packing or transfers
+    d_ld, d_la = sph2d.DeviceArray(lam[0]), sph2d.DeviceArray(np.concatenate([lam[1], lam[2]]))
+    gp, gv, gr = (sph2d.DeviceArray(np.zeros(k * N)) for k in (2, 2, 1))
+    sim.single_step_compute_vjp_device(d_ld, d_la, gp, gv, gr)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.single_step_compute_vjp_device(d_ld, d_la, gp, gv, gr)
+    rev_d = (time.perf_counter() - t0) / reps
+    for a in (d_ld, d_la, gp, gv, gr):
+        a.free()
     sim.close()
     return {"n_side": n_side, "particles": N, "forward_evaluation_ms": 1e3 * fwd,
-            "reverse_evaluation_ms": 1e3 * rev, "reverse_over_forward": rev / fwd, "reps": reps}
+            "reverse_evaluation_ms": 1e3 * rev, "reverse_over_forward": rev / fwd,
+            "reverse_evaluation_device_ms": 1e3 * rev_d, "reverse_device_over_forward": rev_d / fwd, "reps": reps}
 
 
 # the wave tank of the adjoint workload: 8 m long, water 0.4 m deep, dx = 0.005 m

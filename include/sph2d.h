@@ -269,6 +269,14 @@ int sph2d_single_step_compute(sph2d_ctx* ctx, int32_t i_time_step);
 int sph2d_single_step_compute_vjp(sph2d_ctx* ctx, int32_t n, const double* lam_drhodt, const double* lam_acc,
                                   const double* lam_avg, int32_t ld, double* grad_pos, double* grad_vel,
                                   double* grad_rho);
+/* The same reverse with device arrays (no reference counterpart: for callers whose
+ * weights and gradients live in device memory, e.g. a training loop): every pointer
+ * is device memory of the context's GPU, in the same (ld,2) layouts; the weights are
+ * packed and the gradients unpacked on the device. Enqueued on the context's stream
+ * (sph2d_get_stream); the gradients are complete when that stream reaches them. */
+int sph2d_single_step_compute_vjp_device(sph2d_ctx* ctx, int32_t n, const double* lam_drhodt,
+                                         const double* lam_acc, const double* lam_avg, int32_t ld,
+                                         double* grad_pos, double* grad_vel, double* grad_rho);
 /* --- the evaluation routine by routine (single_step_compute.f90:62-171) --------
  * sph2d_find_particle_pair, sph2d_define_freesurface and sph2d_solver, called in
  * that order after sph2d_devide_block (or a stage update), give the same results,

@@ -852,6 +852,37 @@ __global__ void __launch_bounds__(kBlock) k_adj_owner(DevParams P, Buffers B, Ad
   for (int q = 0; q < 5; ++q) dst[q] = acc[q];
 }
 
+// Weights of the device-array reverse: lam per fluid gid (drho/dt, ax, ay, avg x, y)
+// from the caller's (ld, 2) columns; and the gradients back into them.
+__global__ void k_vjp_pack(int n, int ld, const double* ldr, const double* lacc, const double* lavg, double* lam) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double* l = lam + 5 * (size_t)i;
+  l[0] = ldr[i];
+  l[1] = lacc[i];
+  l[2] = lacc[(size_t)ld + i];
+  l[3] = lavg ? lavg[i] : 0.0;
+  l[4] = lavg ? lavg[(size_t)ld + i] : 0.0;
+}
+
+__global__ void k_vjp_unpack(int n, int ld, const double* G, double* gpos, double* gvel, double* grho) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* g = G + 5 * (size_t)i;
+  if (gpos) { gpos[i] = g[0]; gpos[(size_t)ld + i] = g[1]; }
+  if (gvel) { gvel[i] = g[2]; gvel[(size_t)ld + i] = g[3]; }
+  if (grho) grho[i] = g[4];
+}
+
+void launch_vjp_pack(hipStream_t st, int n, int ld, const double* ldr, const double* lacc, const double* lavg,
+                     double* lam) {
+  hipLaunchKernelGGL(k_vjp_pack, dim3(grid_for(n)), dim3(kBlock), 0, st, n, ld, ldr, lacc, lavg, lam);
+}
+
+void launch_vjp_unpack(hipStream_t st, int n, int ld, const double* G, double* gpos, double* gvel, double* grho) {
+  hipLaunchKernelGGL(k_vjp_unpack, dim3(grid_for(n)), dim3(kBlock), 0, st, n, ld, G, gpos, gvel, grho);
+}
+
 static void launch_force_tau(hipStream_t st, const DevParams& P, Buffers& B, int g, const AdjBufs& A) {
   if (P.solver == SPH2D_SOLVER_LINEAR_RIEMANN) {
     hipLaunchKernelGGL((k_adj_force<SPH2D_SOLVER_LINEAR_RIEMANN, 0>), dim3(g), dim3(kBlock), 0, st, P, B, A);

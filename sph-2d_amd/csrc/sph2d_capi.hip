@@ -1236,9 +1236,8 @@ int sph2d_download_rates(sph2d_ctx* c, int32_t n, double* drhodt, double* acc, i
   return SPH2D_OK;
 }
 
-int sph2d_single_step_compute_vjp(sph2d_ctx* c, int32_t n, const double* lam_drhodt, const double* lam_acc,
-                                  const double* lam_avg, int32_t ld, double* grad_pos, double* grad_vel,
-                                  double* grad_rho) {
+// What both forms of the reverse evaluation check; then the work space.
+static int vjp_prepare(sph2d_ctx* c, int32_t n, int32_t ld, const void* lam_drhodt, const void* lam_acc) {
   int rc = need_state(c);
   if (rc) return rc;
   if (n != c->n_fluid) return fail(SPH2D_ERR_ARGUMENT, "n must equal the number of fluid particles");
@@ -1256,6 +1255,36 @@ int sph2d_single_step_compute_vjp(sph2d_ctx* c, int32_t n, const double* lam_drh
     return fail(SPH2D_ERR_UNSUPPORTED, "the reverse evaluation is built for the periodic box");
   if (!(c->prm.box_x > 2 * c->P.three_h) || !(c->prm.box_y > 2 * c->P.three_h))
     return fail(SPH2D_ERR_UNSUPPORTED, "the box must be wider than two kernel supports");
+  const size_t nf = (size_t)c->cap_fluid, nl = (size_t)c->cap_local, ng = (size_t)c->cap_global;
+  if (!c->adj_buf) {                                // kept until the context is destroyed
+    rc = c->alloc(&c->adj_buf, 5 * nf + 20 * nl + 5 * ng);
+    if (rc) return rc;
+  }
+  // a clamped ghost emission leaves slots unwritten that the fold would read
+  int cnt[C_COUNT];
+  return read_counters(c, cnt);
+}
+
+int sph2d_single_step_compute_vjp_device(sph2d_ctx* c, int32_t n, const double* lam_drhodt, const double* lam_acc,
+                                         const double* lam_avg, int32_t ld, double* grad_pos, double* grad_vel,
+                                         double* grad_rho) {
+  int rc = vjp_prepare(c, n, ld, lam_drhodt, lam_acc);
+  if (rc) return rc;
+  const size_t nf = (size_t)c->cap_fluid, nl = (size_t)c->cap_local;
+  double *d_lam = c->adj_buf, *w = d_lam + 5 * nf, *gs = w + 10 * nl, *ga = gs + 5 * nl, *G = ga + 5 * nl;
+  launch_vjp_pack(c->stream, n, ld, lam_drhodt, lam_acc, lam_avg, d_lam);
+  launch_eval_vjp(c->stream, c->P, c->B, c->cap_local, c->cap_fluid, c->cap_ghost, c->cap_global, d_lam, w, gs, ga,
+                  G);
+  launch_vjp_unpack(c->stream, n, ld, G, grad_pos, grad_vel, grad_rho);
+  HIP_OR_FAIL(hipGetLastError());
+  return SPH2D_OK;
+}
+
+int sph2d_single_step_compute_vjp(sph2d_ctx* c, int32_t n, const double* lam_drhodt, const double* lam_acc,
+                                  const double* lam_avg, int32_t ld, double* grad_pos, double* grad_vel,
+                                  double* grad_rho) {
+  int rc = vjp_prepare(c, n, ld, lam_drhodt, lam_acc);
+  if (rc) return rc;
   std::vector<double> lam(5 * (size_t)n);
   for (int i = 0; i < n; ++i) {
     lam[5 * (size_t)i + 0] = lam_drhodt[i];
@@ -1264,18 +1293,8 @@ int sph2d_single_step_compute_vjp(sph2d_ctx* c, int32_t n, const double* lam_drh
     lam[5 * (size_t)i + 3] = lam_avg ? lam_avg[i] : 0.0;
     lam[5 * (size_t)i + 4] = lam_avg ? lam_avg[ld + i] : 0.0;
   }
-  const size_t nf = (size_t)c->cap_fluid, nl = (size_t)c->cap_local, ng = (size_t)c->cap_global;
-  if (!c->adj_buf) {                                // kept until the context is destroyed
-    rc = c->alloc(&c->adj_buf, 5 * nf + 20 * nl + 5 * ng);
-    if (rc) return rc;
-  }
+  const size_t nf = (size_t)c->cap_fluid, nl = (size_t)c->cap_local;
   double *d_lam = c->adj_buf, *w = d_lam + 5 * nf, *gs = w + 10 * nl, *ga = gs + 5 * nl, *G = ga + 5 * nl;
-  {
-    // a clamped ghost emission leaves slots unwritten that the fold would read
-    int cnt[C_COUNT];
-    rc = read_counters(c, cnt);
-    if (rc) return rc;
-  }
   std::vector<double> out(5 * (size_t)n);
   HIP_OR_FAIL(hipMemcpyAsync(d_lam, lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice, c->stream));
   launch_eval_vjp(c->stream, c->P, c->B, c->cap_local, c->cap_fluid, c->cap_ghost, c->cap_global, d_lam, w, gs, ga,

@@ -142,6 +142,9 @@ int set_error(int code, const std::string& msg);
 void launch_init_tgv(hipStream_t s, const DevParams& P, Buffers& B, int nx, int ny);
 // Reverse of the last FP64 evaluation (sph2d_adjoint.hip); lam: 5 per fluid gid,
 // w: 6 and gs, ga: 5 per local particle, G: 5 per gid (result in the first n_fluid).
+void launch_vjp_pack(hipStream_t st, int n, int ld, const double* ldr, const double* lacc, const double* lavg,
+                     double* lam);
+void launch_vjp_unpack(hipStream_t st, int n, int ld, const double* G, double* gpos, double* gvel, double* grho);
 void launch_eval_vjp(hipStream_t st, const DevParams& P, Buffers& B, int cap_local, int cap_fluid, int cap_ghost,
                      int cap_global, const double* lam, double* w, double* gs, double* ga, double* G);
 // the slot form of the reverse evaluation (weights w: 10 per local particle) and the

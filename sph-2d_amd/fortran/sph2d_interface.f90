@@ -244,6 +244,15 @@ module sph2d_interface
       real(c_double), intent(out), optional :: grad_pos(ld, 2), grad_vel(ld, 2), grad_rho(ld)
       integer(c_int) :: rc
     end function
+    ! the same reverse on device arrays (c_ptr of device memory, (ld,2) layouts; NULL
+    ! for lam_avg or an unwanted gradient)
+    function sph2d_single_step_compute_vjp_device(ctx, n, lam_drhodt, lam_acc, lam_avg, ld, grad_pos, &
+        grad_vel, grad_rho) bind(C, name="sph2d_single_step_compute_vjp_device") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx, lam_drhodt, lam_acc, lam_avg, grad_pos, grad_vel, grad_rho
+      integer(c_int32_t), value :: n, ld
+      integer(c_int) :: rc
+    end function
 
     function sph2d_time_step(ctx, i_time_step) bind(C, name="sph2d_time_step") result(rc)
       import :: c_int, c_int32_t, c_ptr

@@ -140,7 +140,8 @@ EXPORTS = (
     "sph2d_init_tgv", "sph2d_upload_state", "sph2d_download_state", "sph2d_compute_kernel",
     "sph2d_compute_kernel_n", "sph2d_status_equation", "sph2d_status_equation_n",
     "sph2d_distribute_period_boundary", "sph2d_devide_block", "sph2d_single_step_compute",
-    "sph2d_single_step_compute_vjp", "sph2d_find_particle_pair", "sph2d_download_pair_list",
+    "sph2d_single_step_compute_vjp", "sph2d_single_step_compute_vjp_device", "sph2d_find_particle_pair",
+    "sph2d_download_pair_list",
     "sph2d_define_freesurface", "sph2d_solver", "sph2d_stage_update", "sph2d_download_rates",
     "sph2d_reinitialize_particle_density", "sph2d_shift_particle_position",
     "sph2d_refresh_domain", "sph2d_time_step", "sph2d_run", "sph2d_sample",
@@ -194,6 +195,7 @@ def lib():
             "sph2d_devide_block": [vp, i32],
             "sph2d_single_step_compute": [vp, i32],
             "sph2d_single_step_compute_vjp": [vp, i32, dp, dp, dp, i32, dp, dp, dp],
+            "sph2d_single_step_compute_vjp_device": [vp, i32, vp, vp, vp, i32, vp, vp, vp],
             "sph2d_reinitialize_particle_density": [vp, i32],
             "sph2d_shift_particle_position": [vp, i32],
             "sph2d_refresh_domain": [vp, i32],
@@ -477,6 +479,22 @@ class Simulation:
         _check(lib().sph2d_single_step_compute_vjp(self._c, n, _dp(ld), _dp(la), _dp(lg) if lg is not None else None,
                                                    n, _dp(gp), _dp(gv), _dp(gr)))
         return {"x": gp[:n], "y": gp[n:], "u": gv[:n], "v": gv[n:], "rho": gr}
+
+    def single_step_compute_vjp_device(self, lam_drhodt: "DeviceArray", lam_acc: "DeviceArray",
+                                       grad_pos: "DeviceArray", grad_vel: "DeviceArray", grad_rho: "DeviceArray",
+                                       lam_avg: "DeviceArray" = None):
+        """sph2d_single_step_compute_vjp_device: the same reverse on device arrays
+        (lam_drhodt n, lam_acc (n,2) columns, lam_avg (n,2) or None; gradients of the
+        same shapes), enqueued on the context's stream and synchronised here."""
+        n = self.n_fluid
+        for a, m, name in ((lam_drhodt, 1, "lam_drhodt"), (lam_acc, 2, "lam_acc"), (grad_pos, 2, "grad_pos"),
+                           (grad_vel, 2, "grad_vel"), (grad_rho, 1, "grad_rho"), (lam_avg, 2, "lam_avg")):
+            if a is not None and a.host.size != m * n:
+                raise ValueError(f"{name} must hold {m} x {n} values, got {a.host.size}")
+        _check(lib().sph2d_single_step_compute_vjp_device(
+            self._c, n, lam_drhodt.ptr, lam_acc.ptr, lam_avg.ptr if lam_avg is not None else None, n,
+            grad_pos.ptr, grad_vel.ptr, grad_rho.ptr))
+        self.diagnostics()                         # synchronises the context's stream
 
     # --- the wave-theory wavemaker and its damping zone ---------------------------------
     def relaxation_init(self, zone_length: float):

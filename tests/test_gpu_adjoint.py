@@ -250,3 +250,34 @@ def test_vjp_refuses_a_state_changed_since_the_evaluation():
     with pytest.raises(ValueError):
         sim.single_step_compute_vjp(lam[:-1], lam, lam)   # a weight array of the wrong length
     sim.close()
+
+
+def test_device_array_reverse_equals_the_host_array_one():
+    """sph2d_single_step_compute_vjp_device packs the weights and unpacks the
+    gradients on the device: with the same weights it returns the host-array
+    entry's gradients bit for bit, with and without XSPH weights."""
+    say("test_device_array_reverse_equals_the_host_array_one")
+    n = 32
+    st, m = noisy_state(n, 11)
+    sim = sph2d.Simulation(sph2d.tgv_params(n, integrator=2))
+    N = n * n
+    rng = np.random.default_rng(13)
+    lam = [rng.normal(0, 1.0, N) for _ in range(5)]
+    sim.upload_state(*st, m)
+    sim.distribute_period_boundary()
+    sim.devide_block(1)
+    sim.single_step_compute(1)
+    for with_avg in (False, True):
+        host = sim.single_step_compute_vjp(*lam) if with_avg else sim.single_step_compute_vjp(*lam[:3])
+        d_ld = sph2d.DeviceArray(lam[0])
+        d_la = sph2d.DeviceArray(np.concatenate([lam[1], lam[2]]))
+        d_lg = sph2d.DeviceArray(np.concatenate([lam[3], lam[4]])) if with_avg else None
+        gp, gv, gr = (sph2d.DeviceArray(np.full(k * N, np.nan)) for k in (2, 2, 1))
+        sim.single_step_compute_vjp_device(d_ld, d_la, gp, gv, gr, lam_avg=d_lg)
+        p, v, r = gp.get(), gv.get(), gr.get()
+        for k, dev in (("x", p[:N]), ("y", p[N:]), ("u", v[:N]), ("v", v[N:]), ("rho", r)):
+            assert np.array_equal(dev, host[k]), (with_avg, k)
+        say(f"xsph weights {with_avg}: identical")
+    with pytest.raises(ValueError):
+        sim.single_step_compute_vjp_device(sph2d.DeviceArray(lam[0][:-1]), d_la, gp, gv, gr)
+    sim.close()
