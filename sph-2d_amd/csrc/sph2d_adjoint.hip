@@ -749,6 +749,10 @@ __global__ void __launch_bounds__(kBlock) k_adj_wall_push(DevParams P, Buffers B
   if (j >= B.counters[C_NLOCAL]) return;
   if (B.l_type[j] == -3) return;
   const double2 xj = B.l_pos[j];
+  // the wall particles lie outside [0, box_x] x [0, box_y] (wall_layout): a particle
+  // farther than their support from every side of the box has none within it
+  const double rsup = 3 * P.wall_h;
+  if (xj.x > rsup && P.box_x - xj.x > rsup && xj.y > rsup && P.box_y - xj.y > rsup) return;
   const double2 vj = B.l_vel[j];
   const double rhoj = B.l_rv[j].x;
   const double pj = B.l_pc[j].x;
